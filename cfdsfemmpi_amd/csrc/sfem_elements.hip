@@ -1,0 +1,585 @@
+// sfem_elements.hip -- element geometry and the stabilised Navier-Stokes element
+// kernels (residual Fe and tangent Ke) with scatter-add into the device CSR.
+//
+// One "team" of lanes per element: a team of TEAM lanes (64 for Hexa8 = one
+// wavefront, 16 for the 2D / Tetra4 elements = a quarter wavefront) owns one
+// element.  Lane (i, j) of the team owns the ndof x ndof block of node pair
+// (i, j) of Ke and keeps it in registers across all Gauss points; the
+// per-Gauss-point data (shape gradients, interpolated state, stabilisation
+// parameters) is computed cooperatively by lane (gp, k) and staged in LDS.
+//
+// The arithmetic restates calcStiffnessAndResidual of the reference
+// (src/base/LagrangeElem3DNavierStokesHexa8Node.cpp:764-1074,
+// ...Tetra4Node.cpp:769-1078, ...Quad4Node.cpp:683-970, ...Tria3Node.cpp:683-968),
+// regrouped so that every (i, j, gp) term is a handful of FMAs on per-node and
+// per-Gauss-point factors:
+//   K_uu(a,b) += (w Ni Nj) A_ab + d_ab [c w Ni Nj + mu af w dNi.dNj + w Ni u_j]
+//   K_up(a)   -= af (w dNi_a) Nj
+//   K_pu(b)   += s af w Ni dNj_b + Nj g_i(b) + s tauP (w dNi_b) d_j     (PSPG)
+//   K_pp      += s tauP af w dNi.dNj
+//   2D SUPG:    K_u(a). += Da_i Dj(a, .)
+// with A = af rho gradPrev, u_j = af rho (uPrev.dNj), d_j = u_j + rho td8 Nj,
+// g_i(b) = s tauP sum_a (w dNi_a) A_ab, s = -1 for Hexa8 and +1 otherwise
+// (Hexa8Node.cpp:984-986,1027-1030 vs Tetra4Node.cpp:990-992,1033-1036).
+//
+// Scatter: elements are processed one colour class per launch; no two
+// elements of a class share a node, so every CSR entry and RHS entry has at
+// most one writer per launch and plain read-modify-write stores are exact and
+// deterministic (summation order = colour order).
+#include <hip/hip_runtime.h>
+
+#include "sfem_internal.hpp"
+
+namespace sfem {
+
+template <int ET> struct Traits;
+template <> struct Traits<TRIA3> {
+  static constexpr int NDIM = 2, NPE = 3, NDOF = 3, NGP = 1, TEAM = 16;
+  static constexpr double SGN = 1.0, CI = 4.0;
+  static constexpr bool SUPG = true;
+};
+template <> struct Traits<QUAD4> {
+  static constexpr int NDIM = 2, NPE = 4, NDOF = 3, NGP = 4, TEAM = 16;
+  static constexpr double SGN = 1.0, CI = 4.0;
+  static constexpr bool SUPG = true;
+};
+template <> struct Traits<TETRA4> {
+  static constexpr int NDIM = 3, NPE = 4, NDOF = 4, NGP = 1, TEAM = 16;
+  static constexpr double SGN = 1.0, CI = 10.0;
+  static constexpr bool SUPG = false;
+};
+template <> struct Traits<HEXA8> {
+  static constexpr int NDIM = 3, NPE = 8, NDOF = 4, NGP = 8, TEAM = 64;
+  static constexpr double SGN = -1.0, CI = 10.0;
+  static constexpr bool SUPG = false;
+};
+
+// PI of the reference (headersBasic.h:28) -- the nearest double to pi.
+constexpr double kPi = 3.14159265358979323846;
+// Two-point Gauss abscissa as the reference writes it (UtilitiesQuadrature.cpp:25-26).
+constexpr double kG2 = 0.577350269189626;
+
+// Gauss point gp of the element's rule: x-fastest ordering (UtilitiesQuadrature.cpp:245-253,
+// 285-297); Tria3 uses the quad 1-point rule at (0,0) with weight 4 (Tria3Node.cpp:56).
+template <int ET>
+__host__ __device__ inline void gauss_point(int gp, double* xi, double& wgt) {
+  if constexpr (ET == HEXA8) {
+    xi[0] = (gp & 1) ? kG2 : -kG2;
+    xi[1] = (gp & 2) ? kG2 : -kG2;
+    xi[2] = (gp & 4) ? kG2 : -kG2;
+    wgt = 1.0;
+  } else if constexpr (ET == QUAD4) {
+    xi[0] = (gp & 1) ? kG2 : -kG2;
+    xi[1] = (gp & 2) ? kG2 : -kG2;
+    xi[2] = 0.0;
+    wgt = 1.0;
+  } else if constexpr (ET == TETRA4) {
+    xi[0] = 0.25; xi[1] = 0.25; xi[2] = 0.25;
+    wgt = 1.0 / 6.0;
+  } else {
+    xi[0] = 0.0; xi[1] = 0.0; xi[2] = 0.0;
+    wgt = 4.0;
+  }
+}
+
+// Degree-1 shape functions and parametric derivatives (BasisFunctionsLagrange.cpp:517-546,
+// 687-732, 873-911, 1019-1080).  For Hexa8 the xi2/xi3 derivatives of nodes 2,3 and 6,7
+// are taken exactly as the reference tabulates them (:1056-1074), which pairs them
+// with the opposite node of each x-edge.
+template <int ET>
+__host__ __device__ inline void shape(const double* xi, double* N, double* d1, double* d2, double* d3) {
+  if constexpr (ET == HEXA8) {
+    const double v11 = 1.0 - xi[0], v12 = 1.0 + xi[0];
+    const double v21 = 1.0 - xi[1], v22 = 1.0 + xi[1];
+    const double v31 = 1.0 - xi[2], v32 = 1.0 + xi[2];
+    N[0] = 0.125 * v11 * v21 * v31; N[1] = 0.125 * v12 * v21 * v31;
+    N[2] = 0.125 * v12 * v22 * v31; N[3] = 0.125 * v11 * v22 * v31;
+    N[4] = 0.125 * v11 * v21 * v32; N[5] = 0.125 * v12 * v21 * v32;
+    N[6] = 0.125 * v12 * v22 * v32; N[7] = 0.125 * v11 * v22 * v32;
+    d1[0] = -0.125 * v21 * v31; d1[1] = 0.125 * v21 * v31;
+    d1[2] = 0.125 * v22 * v31;  d1[3] = -0.125 * v22 * v31;
+    d1[4] = -0.125 * v21 * v32; d1[5] = 0.125 * v21 * v32;
+    d1[6] = 0.125 * v22 * v32;  d1[7] = -0.125 * v22 * v32;
+    d2[0] = -0.125 * v11 * v31; d2[1] = -0.125 * v12 * v31;
+    d2[2] = 0.125 * v11 * v31;  d2[3] = 0.125 * v12 * v31;
+    d2[4] = -0.125 * v11 * v32; d2[5] = -0.125 * v12 * v32;
+    d2[6] = 0.125 * v11 * v32;  d2[7] = 0.125 * v12 * v32;
+    d3[0] = -0.125 * v11 * v21; d3[1] = -0.125 * v12 * v21;
+    d3[2] = -0.125 * v11 * v22; d3[3] = -0.125 * v12 * v22;
+    d3[4] = 0.125 * v11 * v21;  d3[5] = 0.125 * v12 * v21;
+    d3[6] = 0.125 * v11 * v22;  d3[7] = 0.125 * v12 * v22;
+  } else if constexpr (ET == QUAD4) {
+    const double b1 = 0.5 * (1.0 - xi[1]), b2 = 0.5 * (1.0 + xi[1]);
+    const double a1 = 0.5 * (1.0 - xi[0]), a2 = 0.5 * (1.0 + xi[0]);
+    N[0] = b1 * a1; N[1] = b1 * a2; N[2] = b2 * a2; N[3] = b2 * a1;
+    d1[0] = b1 * -0.5; d1[1] = b1 * 0.5; d1[2] = b2 * 0.5; d1[3] = b2 * -0.5;
+    d2[0] = -0.5 * a1; d2[1] = -0.5 * a2; d2[2] = 0.5 * a2; d2[3] = 0.5 * a1;
+  } else if constexpr (ET == TETRA4) {
+    N[0] = 1.0 - xi[0] - xi[1] - xi[2]; N[1] = xi[0]; N[2] = xi[1]; N[3] = xi[2];
+    d1[0] = -1.0; d1[1] = 1.0; d1[2] = 0.0; d1[3] = 0.0;
+    d2[0] = -1.0; d2[1] = 0.0; d2[2] = 1.0; d2[3] = 0.0;
+    d3[0] = -1.0; d3[1] = 0.0; d3[2] = 0.0; d3[3] = 1.0;
+  } else {
+    N[0] = 1.0 - xi[0] - xi[1]; N[1] = xi[0]; N[2] = xi[1];
+    d1[0] = -1.0; d1[1] = 1.0; d1[2] = 0.0;
+    d2[0] = -1.0; d2[1] = 0.0; d2[2] = 1.0;
+  }
+}
+
+// Geometry at one Gauss point (computeBasisFunctions2D/3D, BasisFunctionsLagrange.cpp:18-186):
+// B(r,c) = sum_n x_c(n) dN_n/dxi_r, Jac = det B, dN/dx_c = sum_r dN/dxi_r Binv(c,r).
+// X is [NPE][3].  Writes the physical gradients of node k only when k >= 0.
+template <int ET>
+__host__ __device__ inline void gp_geometry(const double (*X)[3], int gp, int k, double& Nk,
+                                            double* dNk, double& dvol) {
+  using C = Traits<ET>;
+  double xi[3], wgt;
+  gauss_point<ET>(gp, xi, wgt);
+  double N[C::NPE], d1[C::NPE], d2[C::NPE], d3[C::NPE];
+  shape<ET>(xi, N, d1, d2, d3);
+  double Jac;
+  if constexpr (C::NDIM == 2) {
+    double B00 = 0, B10 = 0, B01 = 0, B11 = 0;
+    for (int n = 0; n < C::NPE; ++n) {
+      B00 += X[n][0] * d1[n]; B10 += X[n][0] * d2[n];
+      B01 += X[n][1] * d1[n]; B11 += X[n][1] * d2[n];
+    }
+    Jac = B00 * B11 - B01 * B10;
+    if (k >= 0) {
+      const double di = 1.0 / Jac;
+      const double i00 = B11 * di, i01 = -B01 * di, i10 = -B10 * di, i11 = B00 * di;
+      Nk = N[k];
+      dNk[0] = d1[k] * i00 + d2[k] * i01;
+      dNk[1] = d1[k] * i10 + d2[k] * i11;
+      dNk[2] = 0.0;
+    }
+  } else {
+    double B[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int n = 0; n < C::NPE; ++n)
+      for (int c = 0; c < 3; ++c) {
+        B[0][c] += X[n][c] * d1[n];
+        B[1][c] += X[n][c] * d2[n];
+        B[2][c] += X[n][c] * d3[n];
+      }
+    const double c00 = B[1][1] * B[2][2] - B[1][2] * B[2][1];
+    const double c01 = B[1][2] * B[2][0] - B[1][0] * B[2][2];
+    const double c02 = B[1][0] * B[2][1] - B[1][1] * B[2][0];
+    Jac = B[0][0] * c00 + B[0][1] * c01 + B[0][2] * c02;
+    if (k >= 0) {
+      const double id = 1.0 / Jac;
+      // rows c of Binv: Binv(c, r)
+      const double i00 = c00 * id, i01 = (B[0][2] * B[2][1] - B[0][1] * B[2][2]) * id,
+                   i02 = (B[0][1] * B[1][2] - B[0][2] * B[1][1]) * id;
+      const double i10 = c01 * id, i11 = (B[0][0] * B[2][2] - B[0][2] * B[2][0]) * id,
+                   i12 = (B[0][2] * B[1][0] - B[0][0] * B[1][2]) * id;
+      const double i20 = c02 * id, i21 = (B[0][1] * B[2][0] - B[0][0] * B[2][1]) * id,
+                   i22 = (B[0][0] * B[1][1] - B[0][1] * B[1][0]) * id;
+      Nk = N[k];
+      dNk[0] = d1[k] * i00 + d2[k] * i01 + d3[k] * i02;
+      dNk[1] = d1[k] * i10 + d2[k] * i11 + d3[k] * i12;
+      dNk[2] = d1[k] * i20 + d2[k] * i21 + d3[k] * i22;
+    }
+  }
+  dvol = wgt * Jac;
+}
+
+// ---------------------------------------------------------------------------
+// prepareElemData: volume, charlen (2D sqrt(4|A|/pi), 3D (6|V|/pi)^(1/3)),
+// negative-Jacobian detection (Hexa8Node.cpp:31-106, Quad4Node.cpp:36-113).
+// ---------------------------------------------------------------------------
+template <int ET>
+__global__ __launch_bounds__(256) void k_prepare(int64_t nElem, const double* __restrict__ coords,
+                                                 const int32_t* __restrict__ conn,
+                                                 double* __restrict__ charlen,
+                                                 double* __restrict__ vol, int64_t* bad) {
+  using C = Traits<ET>;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nElem) return;
+  double X[C::NPE][3];
+  for (int n = 0; n < C::NPE; ++n) {
+    const int64_t node = conn[e * C::NPE + n];
+    for (int c = 0; c < 3; ++c) X[n][c] = (c < C::NDIM) ? coords[node * C::NDIM + c] : 0.0;
+  }
+  double v = 0.0;
+  bool neg = false;
+  for (int gp = 0; gp < C::NGP; ++gp) {
+    double Nk, dNk[3], dvol;
+    gp_geometry<ET>(X, gp, -1, Nk, dNk, dvol);
+    if (dvol < 0.0) neg = true;  // weights are positive, so sign(dvol) = sign(Jac)
+    v += dvol;
+  }
+  if (neg) atomicMin((unsigned long long*)bad, (unsigned long long)e);
+  vol[e] = v;
+  charlen[e] = (C::NDIM == 2) ? sqrt(4.0 * fabs(v) / kPi) : pow(6.0 * fabs(v) / kPi, 1.0 / 3.0);
+}
+
+// Interpolated Gauss-point quantities, in the order of the reference's loop
+// (Hexa8Node.cpp:828-889): uPrev, gradPrev, u, grad, p, grad p, uDot.
+// src: 0 solnPrev, 1 solnCur, 2 solnDotCur; der: 0 = N, 1+c = d/dx_c.
+template <int NDIM>
+__device__ inline void qdesc(int t, int& src, int& comp, int& der) {
+  constexpr int D = NDIM, DD = NDIM * NDIM;
+  if (t < D) { src = 0; comp = t; der = 0; return; }
+  t -= D;
+  if (t < DD) { src = 0; comp = t / D; der = 1 + t % D; return; }
+  t -= DD;
+  if (t < D) { src = 1; comp = t; der = 0; return; }
+  t -= D;
+  if (t < DD) { src = 1; comp = t / D; der = 1 + t % D; return; }
+  t -= DD;
+  if (t == 0) { src = 1; comp = D; der = 0; return; }
+  t -= 1;
+  if (t < D) { src = 1; comp = D; der = 1 + t; return; }
+  t -= D;
+  src = 2; comp = t; der = 0;
+}
+
+template <int ET>
+struct TeamLds {
+  using C = Traits<ET>;
+  static constexpr int NQ = 2 * C::NDIM + 2 * C::NDIM * C::NDIM + 1 + 2 * C::NDIM;
+  double X[C::NPE][3];
+  double S[3][C::NPE][4];
+  double app[C::NPE][4];
+  double phi[C::NGP][C::NPE][4];
+  double W[C::NGP];
+  double Q[C::NGP][NQ];
+  double U[C::NGP][10];              // A (NDIM x NDIM, row-major in 3x3 slots), s*tauP
+  double P[C::NGP][C::NPE][6];       // g_k(0..2), u_k, d_k, Da_k
+  double Fk[C::NGP][C::NPE][4];
+  double lift[C::NPE][C::NPE][4];
+  int32_t info[C::NPE][4];           // rowStart, rowlen, mask, firstRow
+};
+
+// Ke accumulation of lane (i, j) in scalar type T (double, or float for the mixed path).
+template <int ET, typename T>
+__device__ inline void accumulate_block(const TeamLds<ET>& L, int i, int j, const ElemParams& p,
+                                        T (&K)[Traits<ET>::NDOF][Traits<ET>::NDOF]) {
+  using C = Traits<ET>;
+  constexpr int D = C::NDIM;
+  const T af = (T)p.af, cst = (T)(p.rho * p.acceFact), muaf = (T)(p.mu * p.af);
+  const T saf = (T)(C::SGN * p.af);
+  for (int a = 0; a < C::NDOF; ++a)
+    for (int b = 0; b < C::NDOF; ++b) K[a][b] = (T)0;
+#pragma unroll
+  for (int gp = 0; gp < C::NGP; ++gp) {
+    const T w = (T)L.W[gp];
+    const T Ni = (T)L.phi[gp][i][0], Nj = (T)L.phi[gp][j][0];
+    T wdNi[D], dNj[D];
+    for (int c = 0; c < D; ++c) {
+      wdNi[c] = w * (T)L.phi[gp][i][1 + c];
+      dNj[c] = (T)L.phi[gp][j][1 + c];
+    }
+    const T wNi = w * Ni;
+    const T s1 = wNi * Nj;
+    T dd = wdNi[0] * dNj[0];
+    for (int c = 1; c < D; ++c) dd += wdNi[c] * dNj[c];
+    const T uj = (T)L.P[gp][j][3], dj = (T)L.P[gp][j][4];
+    const T s2 = cst * s1 + muaf * dd + wNi * uj;
+    const T stp = (T)L.U[gp][9];
+    for (int a = 0; a < D; ++a) {
+      for (int b = 0; b < D; ++b) K[a][b] += s1 * (T)L.U[gp][3 * a + b];
+      K[a][a] += s2;
+      K[a][D] -= af * wdNi[a] * Nj;
+    }
+    for (int b = 0; b < D; ++b)
+      K[D][b] += saf * wNi * dNj[b] + Nj * (T)L.P[gp][i][b] + stp * wdNi[b] * dj;
+    K[D][D] += stp * af * dd;
+    if constexpr (C::SUPG) {
+      const T Da = (T)L.P[gp][i][5];
+      for (int a = 0; a < D; ++a) {
+        for (int b = 0; b < D; ++b) K[a][b] += Da * (T)L.U[gp][3 * a + b] * Nj;
+        K[a][a] += Da * dj;
+        K[a][D] += Da * af * dNj[a];
+      }
+    }
+  }
+}
+
+template <int ET, int MODE>
+__global__ __launch_bounds__(256) void k_assemble(AsmView v, ElemParams p,
+                                                  const int32_t* __restrict__ elems, int64_t n,
+                                                  int lift, int64_t probe_e) {
+  using C = Traits<ET>;
+  constexpr int TEAMS = 256 / C::TEAM;
+  constexpr int D = C::NDIM, NPE = C::NPE, NGP = C::NGP, NDOF = C::NDOF;
+  constexpr int NQ = TeamLds<ET>::NQ;
+  __shared__ TeamLds<ET> lds[TEAMS];
+  const int team = threadIdx.x / C::TEAM;
+  const int l = threadIdx.x % C::TEAM;
+  TeamLds<ET>& L = lds[team];
+  const int64_t slot = (int64_t)blockIdx.x * TEAMS + team;
+  const bool probe = probe_e >= 0;
+  const bool active = probe ? (slot == 0) : (slot < n);
+  const int64_t e = active ? (probe ? probe_e : (int64_t)elems[slot]) : 0;
+  const bool doLift = (lift != 0) && !probe;
+
+  // P1: nodal data of the element
+  if (active && l < NPE) {
+    const int64_t node = v.conn[e * NPE + l];
+    for (int c = 0; c < 3; ++c) L.X[l][c] = (c < D) ? v.coords[node * D + c] : 0.0;
+    const int64_t o = node * NDOF;
+    for (int c = 0; c < NDOF; ++c) {
+      L.S[0][l][c] = v.solnPrev[o + c];
+      L.S[1][l][c] = v.solnCur[o + c];
+      L.S[2][l][c] = v.solnDotCur[o + c];
+      L.app[l][c] = doLift ? v.solnApplied[o + c] : 0.0;
+    }
+    const int4 inf = reinterpret_cast<const int4*>(v.nodeInfo)[node];
+    L.info[l][0] = inf.x; L.info[l][1] = inf.y; L.info[l][2] = inf.z; L.info[l][3] = inf.w;
+  }
+  __syncthreads();
+
+  // P2: shape gradients of node k at Gauss point gp
+  if (active && l < NGP * NPE) {
+    const int gp = l / NPE, k = l % NPE;
+    double Nk, dNk[3], dvol;
+    gp_geometry<ET>(L.X, gp, k, Nk, dNk, dvol);
+    L.phi[gp][k][0] = Nk;
+    L.phi[gp][k][1] = dNk[0];
+    L.phi[gp][k][2] = dNk[1];
+    L.phi[gp][k][3] = dNk[2];
+    if (k == 0) L.W[gp] = dvol;
+  }
+  __syncthreads();
+
+  // P3: interpolated state at each Gauss point, summed over nodes in node order
+  if (active && l < NGP * NPE) {
+    const int gp = l / NPE, k = l % NPE;
+    for (int t = k; t < NQ; t += NPE) {
+      int src, comp, der;
+      qdesc<D>(t, src, comp, der);
+      double s = 0.0;
+      for (int nn = 0; nn < NPE; ++nn) s += L.S[src][nn][comp] * L.phi[gp][nn][der];
+      L.Q[gp][t] = s;
+    }
+  }
+  __syncthreads();
+
+  // P4: per-Gauss-point derived data, residual contributions of node k, and
+  // the per-(node, gp) factors of the tangent
+  if (active && l < NGP * NPE) {
+    const int gp = l / NPE, k = l % NPE;
+    const double* Q = L.Q[gp];
+    const double* vp = Q;
+    const double* gpv = Q + D;                  // gradPrev row-major
+    const double* vel = Q + D + D * D;
+    const double* gr = Q + 2 * D + D * D;       // grad row-major
+    const double pres = Q[2 * D + 2 * D * D];
+    const double* dp = Q + 2 * D + 2 * D * D + 1;
+    const double* vd = Q + 3 * D + 2 * D * D + 1;
+    const double rho = p.rho, mu = p.mu, af = p.af;
+
+    double res2[3], rStab[3];
+    for (int a = 0; a < D; ++a) {
+      double s1 = gpv[a * D] * vel[0], s2 = gr[a * D] * vp[0], s3 = gpv[a * D] * vp[0];
+      for (int c = 1; c < D; ++c) {
+        s1 += gpv[a * D + c] * vel[c];
+        s2 += gr[a * D + c] * vp[c];
+        s3 += gpv[a * D + c] * vp[c];
+      }
+      res2[a] = rho * (vd[a] + (s1 + s2 - s3));
+      rStab[a] = res2[a] + dp[a];
+    }
+    // evaluateStabParams_algo3 (stabilisationRoutines.h:128-143), G = 4/h^2 I_ndim
+    const double h = v.charlen[e];
+    const double fact = 4.0 / h / h;
+    double uGu = 0.0;
+    for (int a = 0; a < D; ++a) uGu += vp[a] * (fact * vp[a]);
+    const double G2 = D * (fact * fact), trG = D * fact;
+    double tau0 = 1.0 / sqrt(uGu + C::CI * (mu * mu / rho / rho) * G2);
+    (void)trG;
+    tau0 = tau0 / rho;
+    const double tauS = tau0 * p.stab[0];
+    const double tauP = tau0 * p.stab[1];
+    double trace = gr[0];
+    for (int a = 1; a < D; ++a) trace += gr[a * D + a];
+
+    const double w = L.W[gp];
+    const double Nk = L.phi[gp][k][0];
+    double b[3];
+    for (int c = 0; c < D; ++c) b[c] = L.phi[gp][k][1 + c] * w;
+    const double b4 = Nk * w;
+    double Da = vp[0] * b[0];
+    for (int c = 1; c < D; ++c) Da += vp[c] * b[c];
+    Da = rho * Da * tauS;
+    // residual of node k at gp (Hexa8Node.cpp:1049-1060, Quad4Node.cpp:936-950)
+    for (int a = 0; a < D; ++a) {
+      double s = b4 * res2[a];
+      for (int c = 0; c < D; ++c) s += b[c] * (mu * gr[a * D + c] - (a == c ? pres : 0.0));
+      double f = -s;
+      if constexpr (C::SUPG) f -= Da * rStab[a];
+      L.Fk[gp][k][a] = f;
+    }
+    double br = b[0] * rStab[0];
+    for (int a = 1; a < D; ++a) br += b[a] * rStab[a];
+    L.Fk[gp][k][D] = C::SGN * (-(b4 * trace) - tauP * br);
+    // tangent factors
+    const double stp = C::SGN * tauP;
+    for (int c = 0; c < D; ++c) {
+      double g = 0.0;
+      for (int a = 0; a < D; ++a) g += b[a] * (af * rho * gpv[a * D + c]);
+      L.P[gp][k][c] = stp * g;
+    }
+    double ud = vp[0] * L.phi[gp][k][1];
+    for (int c = 1; c < D; ++c) ud += vp[c] * L.phi[gp][k][1 + c];
+    const double uk = af * rho * ud;
+    L.P[gp][k][3] = uk;
+    L.P[gp][k][4] = uk + rho * p.acceFact * Nk;
+    L.P[gp][k][5] = Da;
+    if (k == 0) {
+      for (int a = 0; a < D; ++a)
+        for (int c = 0; c < D; ++c) L.U[gp][3 * a + c] = af * rho * gpv[a * D + c];
+      L.U[gp][9] = stp;
+    }
+  }
+  __syncthreads();
+
+  // P5: tangent block of node pair (i, j), lift partial sums, CSR scatter
+  if constexpr (MODE != 1) {
+    if (active && l < NPE * NPE) {
+      const int i = l / NPE, j = l % NPE;
+      const int mi = L.info[i][2], mj = L.info[j][2];
+      if constexpr (MODE == 0) {
+        double K[NDOF][NDOF];
+        accumulate_block<ET, double>(L, i, j, p, K);
+        if (probe) {
+          for (int a = 0; a < NDOF; ++a)
+            for (int bq = 0; bq < NDOF; ++bq)
+              v.probeK[(i * NDOF + a) * (NPE * NDOF) + j * NDOF + bq] = K[a][bq];
+        } else {
+          if (doLift) {
+            for (int a = 0; a < NDOF; ++a) {
+              double s = 0.0;
+              for (int bq = 0; bq < NDOF; ++bq) {
+                const double dlt = L.app[j][bq];
+                if (!((mj >> bq) & 1) && fabs(dlt) > 1.0e-10) s += K[a][bq] * dlt;
+              }
+              L.lift[i][j][a] = s;
+            }
+          }
+          if (mi != 0 && mj != 0) {
+            const int32_t co = v.colofs[(e * NPE + i) * NPE + j];
+            const int32_t rs = L.info[i][0], rl = L.info[i][1];
+            int ra = 0;
+            for (int a = 0; a < NDOF; ++a) {
+              if (!((mi >> a) & 1)) continue;
+              double* row = v.vals + (int64_t)rs + (int64_t)ra * rl + co;
+              int rb = 0;
+              for (int bq = 0; bq < NDOF; ++bq) {
+                if (!((mj >> bq) & 1)) continue;
+                row[rb] += K[a][bq];
+                ++rb;
+              }
+              ++ra;
+            }
+          }
+        }
+      } else {
+        float K[NDOF][NDOF];
+        accumulate_block<ET, float>(L, i, j, p, K);
+        if (probe) {
+          for (int a = 0; a < NDOF; ++a)
+            for (int bq = 0; bq < NDOF; ++bq)
+              v.probeK[(i * NDOF + a) * (NPE * NDOF) + j * NDOF + bq] = (double)K[a][bq];
+        } else if (mi != 0 && mj != 0) {
+          const int32_t co = v.colofs[(e * NPE + i) * NPE + j];
+          const int32_t rs = L.info[i][0], rl = L.info[i][1];
+          int ra = 0;
+          for (int a = 0; a < NDOF; ++a) {
+            if (!((mi >> a) & 1)) continue;
+            float* row = v.vals32 + (int64_t)rs + (int64_t)ra * rl + co;
+            int rb = 0;
+            for (int bq = 0; bq < NDOF; ++bq) {
+              if (!((mj >> bq) & 1)) continue;
+              row[rb] += K[a][bq];
+              ++rb;
+            }
+            ++ra;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // P6: residual of node i summed over Gauss points, lift, RHS scatter
+  if (active && l < NPE) {
+    const int i = l;
+    const int mi = L.info[i][2];
+    double F[NDOF];
+    for (int a = 0; a < NDOF; ++a) {
+      double s = 0.0;
+      for (int gp = 0; gp < NGP; ++gp) s += L.Fk[gp][i][a];
+      F[a] = s;
+    }
+    if (probe) {
+      for (int a = 0; a < NDOF; ++a) v.probeF[i * NDOF + a] = F[a];
+    } else {
+      if (MODE == 0 && doLift) {
+        for (int a = 0; a < NDOF; ++a)
+          for (int j = 0; j < NPE; ++j) F[a] -= L.lift[i][j][a];
+      }
+      if (mi != 0) {
+        const int32_t r0 = L.info[i][3];
+        int ra = 0;
+        for (int a = 0; a < NDOF; ++a) {
+          if (!((mi >> a) & 1)) continue;
+          v.rhs[r0 + ra] += F[a];
+          ++ra;
+        }
+      }
+    }
+  }
+}
+
+template <int ET>
+static hipError_t prepare_t(hipStream_t s, int64_t nElem, const double* coords, const int32_t* conn,
+                            double* charlen, double* vol, int64_t* bad) {
+  if (nElem == 0) return hipSuccess;
+  const int64_t grid = (nElem + 255) / 256;
+  hipLaunchKernelGGL((k_prepare<ET>), dim3((unsigned)grid), dim3(256), 0, s, nElem, coords, conn,
+                     charlen, vol, bad);
+  return hipGetLastError();
+}
+
+hipError_t launch_prepare(int et, hipStream_t s, int64_t nElem, const double* coords,
+                          const int32_t* conn, double* charlen, double* vol, int64_t* bad) {
+  switch (et) {
+    case TRIA3: return prepare_t<TRIA3>(s, nElem, coords, conn, charlen, vol, bad);
+    case QUAD4: return prepare_t<QUAD4>(s, nElem, coords, conn, charlen, vol, bad);
+    case TETRA4: return prepare_t<TETRA4>(s, nElem, coords, conn, charlen, vol, bad);
+    default: return prepare_t<HEXA8>(s, nElem, coords, conn, charlen, vol, bad);
+  }
+}
+
+template <int ET>
+static hipError_t assemble_t(hipStream_t s, const AsmView& v, const ElemParams& p,
+                             const int32_t* elems, int64_t n, int mode, int lift, int64_t probe_e) {
+  constexpr int TEAMS = 256 / Traits<ET>::TEAM;
+  const int64_t cnt = probe_e >= 0 ? 1 : n;
+  if (cnt == 0) return hipSuccess;
+  const int64_t grid = (cnt + TEAMS - 1) / TEAMS;
+  if (mode == 0)
+    hipLaunchKernelGGL((k_assemble<ET, 0>), dim3((unsigned)grid), dim3(256), 0, s, v, p, elems, n,
+                       lift, probe_e);
+  else if (mode == 1)
+    hipLaunchKernelGGL((k_assemble<ET, 1>), dim3((unsigned)grid), dim3(256), 0, s, v, p, elems, n,
+                       lift, probe_e);
+  else
+    hipLaunchKernelGGL((k_assemble<ET, 2>), dim3((unsigned)grid), dim3(256), 0, s, v, p, elems, n,
+                       lift, probe_e);
+  return hipGetLastError();
+}
+
+hipError_t launch_assemble(int et, hipStream_t s, const AsmView& v, const ElemParams& p,
+                           const int32_t* elems, int64_t n, int mode, int lift, int64_t probe_e) {
+  switch (et) {
+    case TRIA3: return assemble_t<TRIA3>(s, v, p, elems, n, mode, lift, probe_e);
+    case QUAD4: return assemble_t<QUAD4>(s, v, p, elems, n, mode, lift, probe_e);
+    case TETRA4: return assemble_t<TETRA4>(s, v, p, elems, n, mode, lift, probe_e);
+    default: return assemble_t<HEXA8>(s, v, p, elems, n, mode, lift, probe_e);
+  }
+}
+
+}  // namespace sfem

@@ -1,0 +1,1015 @@
+// sfem_linalg.hip -- Krylov solvers and preconditioners on the device CSR.
+//
+// Replaces the PETSc KSP/PC calls behind SolverPetsc::solve
+// (src/base/SolverPetsc.cpp:224-277): BiCGStab (-ksp_type bcgs) and
+// restarted GMRES (-ksp_type gmres), right-preconditioned so that the stopping
+// test ||b - A x||_2 <= rtol ||b||_2 is on the true residual, with a zero
+// initial guess (VecZeroEntries at :249).  Preconditioners: point Jacobi,
+// nodal block Jacobi (the free DOFs of one node form a block of size <= 4),
+// and ILU(0) on the CSR pattern, factorised and applied by level scheduling
+// over nodes (all free rows of a node are handled by one wavefront).
+//
+// Reductions are two-stage (per-block partials, then one block summing the
+// partials in index order), so every dot product is deterministic.  Krylov
+// scalars (alpha, omega, beta, ...) stay on the device; the host reads back
+// one small block of scalars per iteration for the convergence test.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "sfem_internal.hpp"
+#include "sfem_linalg.hpp"
+
+namespace sfem {
+
+constexpr int kRedBlocks = 1024;  // partials per reduction
+constexpr int kTPB = 256;
+
+// scalar slots
+enum {
+  S_RHO0 = 0, S_RHO1, S_ALPHA, S_OMEGA, S_BETA, S_RV, S_SS, S_TT, S_TS, S_RR, S_BB, S_TOL2,
+  S_FLAG, S_MISC0, S_MISC1, S_NSLOTS = 32
+};
+
+// ---------------------------------------------------------------------------
+// reductions
+// ---------------------------------------------------------------------------
+__device__ inline double warp_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+template <int ND>
+__device__ inline void block_sum_store(double (&v)[ND], double* out /*ND per block*/) {
+  __shared__ double sh[ND][kTPB / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    double s = warp_sum(v[d]);
+    if (lane == 0) sh[d][wid] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < ND) {
+    double s = 0.0;
+    for (int w = 0; w < kTPB / 64; ++w) s += sh[threadIdx.x][w];
+    out[(size_t)blockIdx.x * ND + threadIdx.x] = s;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kTPB) void k_dot(int64_t n, const T* __restrict__ a,
+                                              const T* __restrict__ b, double* part) {
+  double v[1] = {0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kTPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kTPB)
+    v[0] += (double)a[i] * (double)b[i];
+  block_sum_store<1>(v, part);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kTPB) void k_dot2(int64_t n, const T* __restrict__ a1,
+                                               const T* __restrict__ b1, const T* __restrict__ a2,
+                                               const T* __restrict__ b2, double* part) {
+  double v[2] = {0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kTPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kTPB) {
+    v[0] += (double)a1[i] * (double)b1[i];
+    v[1] += (double)a2[i] * (double)b2[i];
+  }
+  block_sum_store<2>(v, part);
+}
+
+// Finalisation ops: combine summed partials into Krylov scalars.
+enum { FIN_STORE = 0, FIN_RHO1, FIN_ALPHA, FIN_SS, FIN_OMEGA, FIN_RR };
+
+template <int ND>
+__global__ __launch_bounds__(kTPB) void k_finalize(const double* __restrict__ part, int nb,
+                                                   double* scal, int slot, int op) {
+  __shared__ double sh[ND][kTPB];
+  for (int d = 0; d < ND; ++d) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nb; i += kTPB) s += part[(size_t)i * ND + d];
+    sh[d][threadIdx.x] = s;
+  }
+  __syncthreads();
+  for (int w = kTPB / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int d = 0; d < ND; ++d) sh[d][threadIdx.x] += sh[d][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  double r[ND];
+  for (int d = 0; d < ND; ++d) r[d] = sh[d][0];
+  switch (op) {
+    case FIN_STORE:
+      for (int d = 0; d < ND; ++d) scal[slot + d] = r[d];
+      break;
+    case FIN_RHO1: {  // rho1 = (rh, r); beta = (rho1/rho0)(alpha/omega)
+      scal[S_RHO1] = r[0];
+      if (r[0] == 0.0) scal[S_FLAG] = 2.0;  // breakdown
+      const double rho0 = scal[S_RHO0], om = scal[S_OMEGA];
+      scal[S_BETA] = (rho0 != 0.0 && om != 0.0) ? (r[0] / rho0) * (scal[S_ALPHA] / om) : 0.0;
+      break;
+    }
+    case FIN_ALPHA:  // alpha = rho1 / (rh, v)
+      scal[S_RV] = r[0];
+      if (r[0] == 0.0) { scal[S_FLAG] = 2.0; scal[S_ALPHA] = 0.0; }
+      else scal[S_ALPHA] = scal[S_RHO1] / r[0];
+      break;
+    case FIN_SS:
+      scal[S_SS] = r[0];
+      break;
+    case FIN_OMEGA: {  // omega = (t, s)/(t, t); 0 when s already meets the tolerance
+      scal[S_TT] = r[0];
+      scal[S_TS] = r[1];
+      const bool half = scal[S_SS] <= scal[S_TOL2];
+      scal[S_OMEGA] = (half || r[0] == 0.0) ? 0.0 : r[1] / r[0];
+      if (half) scal[S_FLAG] = 1.0;
+      break;
+    }
+    case FIN_RR:
+      scal[S_RR] = r[0];
+      scal[S_RHO0] = scal[S_RHO1];
+      break;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// vector kernels
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kTPB) void k_copy(int64_t n, const T* __restrict__ a, T* __restrict__ b) {
+  for (int64_t i = (int64_t)blockIdx.x * kTPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kTPB)
+    b[i] = a[i];
+}
+
+template <typename TA, typename TB>
+__global__ __launch_bounds__(kTPB) void k_convert(int64_t n, const TA* __restrict__ a,
+                                                  TB* __restrict__ b, double scale) {
+  for (int64_t i = (int64_t)blockIdx.x * kTPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kTPB)
+    b[i] = (TB)((double)a[i] * scale);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kTPB) void k_fill(int64_t n, T* __restrict__ a, T v) {
+  for (int64_t i = (int64_t)blockIdx.x * kTPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kTPB)
+    a[i] = v;
+}
+
+// p = r + beta (p - omega v)   (first iteration: p = r)
+template <typename T>
+__global__ __launch_bounds__(kTPB) void k_bicg_p(int64_t n, const T* __restrict__ r, T* __restrict__ p,
+                                                 const T* __restrict__ v, const double* scal, int first) {
+  const T beta = first ? (T)0 : (T)scal[S_BETA];
+  const T om = first ? (T)0 : (T)scal[S_OMEGA];
+  for (int64_t i = (int64_t)blockIdx.x * kTPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kTPB)
+    p[i] = first ? r[i] : r[i] + beta * (p[i] - om * v[i]);
+}
+
+// s = r - alpha v ; partial (s, s)
+template <typename T>
+__global__ __launch_bounds__(kTPB) void k_bicg_s(int64_t n, const T* __restrict__ r,
+                                                 const T* __restrict__ v, T* __restrict__ s,
+                                                 const double* scal, double* part) {
+  const T alpha = (T)scal[S_ALPHA];
+  double acc[1] = {0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kTPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kTPB) {
+    const T si = r[i] - alpha * v[i];
+    s[i] = si;
+    acc[0] += (double)si * (double)si;
+  }
+  block_sum_store<1>(acc, part);
+}
+
+// x += alpha ph + omega sh ; r = s - omega t ; partial (r, r)
+template <typename T>
+__global__ __launch_bounds__(kTPB) void k_bicg_xr(int64_t n, T* __restrict__ x, const T* __restrict__ ph,
+                                                  const T* __restrict__ sh, T* __restrict__ r,
+                                                  const T* __restrict__ s, const T* __restrict__ t,
+                                                  const double* scal, double* part) {
+  const T alpha = (T)scal[S_ALPHA], om = (T)scal[S_OMEGA];
+  double acc[1] = {0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kTPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kTPB) {
+    x[i] += alpha * ph[i] + om * sh[i];
+    const T ri = s[i] - om * t[i];
+    r[i] = ri;
+    acc[0] += (double)ri * (double)ri;
+  }
+  block_sum_store<1>(acc, part);
+}
+
+// y = a*x + b*y with host scalars (GMRES)
+template <typename T>
+__global__ __launch_bounds__(kTPB) void k_axpby(int64_t n, double a, const T* __restrict__ x, double b,
+                                                T* __restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * kTPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kTPB)
+    y[i] = (T)(a * (double)x[i] + b * (double)y[i]);
+}
+
+// ---------------------------------------------------------------------------
+// SpMV: SUB lanes per row (SUB a power of two <= 64)
+// ---------------------------------------------------------------------------
+template <typename VT, typename XT, int SUB>
+__global__ __launch_bounds__(kTPB) void k_spmv(int64_t nrows, const int32_t* __restrict__ rowptr,
+                                               const int32_t* __restrict__ colidx,
+                                               const VT* __restrict__ vals, const XT* __restrict__ x,
+                                               XT* __restrict__ y) {
+  const int64_t gt = (int64_t)blockIdx.x * kTPB + threadIdx.x;
+  const int64_t row = gt / SUB;
+  const int lane = (int)(gt % SUB);
+  using AT = typename std::conditional<sizeof(VT) == 8, double, float>::type;
+  AT s = 0;
+  if (row < nrows) {
+    const int32_t b = rowptr[row], e = rowptr[row + 1];
+    for (int32_t k = b + lane; k < e; k += SUB) s += (AT)vals[k] * (AT)x[colidx[k]];
+  }
+  for (int o = SUB / 2; o > 0; o >>= 1) s += __shfl_down(s, o, SUB);
+  if (row < nrows && lane == 0) y[row] = (XT)s;
+}
+
+// ---------------------------------------------------------------------------
+// Jacobi and nodal block Jacobi
+// ---------------------------------------------------------------------------
+__device__ inline double csr_lookup(const int32_t* rowptr, const int32_t* colidx, const double* vals,
+                                    const float* vals32, int64_t r, int32_t c, bool& found) {
+  int32_t lo = rowptr[r], hi = rowptr[r + 1];
+  while (lo < hi) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (colidx[mid] < c) lo = mid + 1; else hi = mid;
+  }
+  found = (lo < rowptr[r + 1] && colidx[lo] == c);
+  if (!found) return 0.0;
+  return vals ? vals[lo] : (double)vals32[lo];
+}
+
+// Inverse of each nodal diagonal block (Gauss-Jordan with partial pivoting, m <= 4).
+template <typename PT>
+__global__ __launch_bounds__(kTPB) void k_bjac_setup(int64_t nb, const int32_t* __restrict__ blkStart,
+                                                     const int32_t* __restrict__ rowptr,
+                                                     const int32_t* __restrict__ colidx,
+                                                     const double* __restrict__ vals,
+                                                     const float* __restrict__ vals32,
+                                                     PT* __restrict__ inv, double* scal) {
+  const int64_t k = (int64_t)blockIdx.x * kTPB + threadIdx.x;
+  if (k >= nb) return;
+  const int32_t r0 = blkStart[k];
+  const int m = blkStart[k + 1] - r0;
+  double a[4][8];
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < m; ++j) {
+      bool f;
+      a[i][j] = csr_lookup(rowptr, colidx, vals, vals32, r0 + i, r0 + j, f);
+      a[i][m + j] = (i == j) ? 1.0 : 0.0;
+    }
+  bool ok = true;
+  for (int c = 0; c < m; ++c) {
+    int p = c;
+    for (int r = c + 1; r < m; ++r)
+      if (fabs(a[r][c]) > fabs(a[p][c])) p = r;
+    if (a[p][c] == 0.0) { ok = false; break; }
+    if (p != c)
+      for (int j = 0; j < 2 * m; ++j) { double t = a[p][j]; a[p][j] = a[c][j]; a[c][j] = t; }
+    const double piv = 1.0 / a[c][c];
+    for (int j = 0; j < 2 * m; ++j) a[c][j] *= piv;
+    for (int r = 0; r < m; ++r) {
+      if (r == c) continue;
+      const double f = a[r][c];
+      if (f == 0.0) continue;
+      for (int j = 0; j < 2 * m; ++j) a[r][j] -= f * a[c][j];
+    }
+  }
+  if (!ok) { scal[S_MISC1] = 1.0; for (int i = 0; i < m; ++i) a[i][m + i] = 1.0; }
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) inv[k * 16 + i * 4 + j] = (PT)((i < m && j < m) ? a[i][m + j] : 0.0);
+}
+
+template <typename T, typename PT>
+__global__ __launch_bounds__(kTPB) void k_bjac_apply(int64_t nb, const int32_t* __restrict__ blkStart,
+                                                     const PT* __restrict__ inv, const T* __restrict__ r,
+                                                     T* __restrict__ z) {
+  const int64_t k = (int64_t)blockIdx.x * kTPB + threadIdx.x;
+  if (k >= nb) return;
+  const int32_t r0 = blkStart[k];
+  const int m = blkStart[k + 1] - r0;
+  T rv[4];
+  for (int j = 0; j < 4; ++j) rv[j] = (j < m) ? r[r0 + j] : (T)0;
+  const PT* B = inv + k * 16;
+  for (int i = 0; i < m; ++i) {
+    T s = 0;
+    for (int j = 0; j < 4; ++j) s += (T)B[i * 4 + j] * rv[j];
+    z[r0 + i] = s;
+  }
+}
+
+template <typename PT>
+__global__ __launch_bounds__(kTPB) void k_jac_setup(int64_t n, const int32_t* __restrict__ diagpos,
+                                                    const double* __restrict__ vals,
+                                                    const float* __restrict__ vals32,
+                                                    PT* __restrict__ dinv, double* scal) {
+  const int64_t i = (int64_t)blockIdx.x * kTPB + threadIdx.x;
+  if (i >= n) return;
+  const double d = vals ? vals[diagpos[i]] : (double)vals32[diagpos[i]];
+  if (d == 0.0) scal[S_MISC1] = 1.0;
+  dinv[i] = (PT)(d == 0.0 ? 1.0 : 1.0 / d);
+}
+
+template <typename T, typename PT>
+__global__ __launch_bounds__(kTPB) void k_jac_apply(int64_t n, const PT* __restrict__ dinv,
+                                                    const T* __restrict__ r, T* __restrict__ z) {
+  for (int64_t i = (int64_t)blockIdx.x * kTPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kTPB)
+    z[i] = (T)dinv[i] * r[i];
+}
+
+// ---------------------------------------------------------------------------
+// ILU(0), level-scheduled over nodal blocks; one 64-lane workgroup per block.
+// All rows of a block share one column list (the pattern is the union of full
+// node blocks), so the block's rows are staged in LDS against that list.
+// ---------------------------------------------------------------------------
+constexpr int kIluMaxRow = 256;  // longest row handled in LDS (checked on the host)
+
+__device__ inline int lds_find(const int32_t* cols, int len, int32_t c) {
+  int lo = 0, hi = len;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (cols[mid] < c) lo = mid + 1; else hi = mid;
+  }
+  return (lo < len && cols[lo] == c) ? lo : -1;
+}
+
+template <typename LT>
+__global__ __launch_bounds__(64) void k_ilu_factor_level(const int32_t* __restrict__ list, int count,
+                                                         const int32_t* __restrict__ blkStart,
+                                                         const int32_t* __restrict__ rowptr,
+                                                         const int32_t* __restrict__ colidx,
+                                                         const int32_t* __restrict__ diagpos,
+                                                         LT* __restrict__ lu, double* scal) {
+  __shared__ int32_t cols[kIluMaxRow];
+  __shared__ double rows[4][kIluMaxRow];
+  __shared__ double piv;
+  const int w = blockIdx.x;
+  if (w >= count) return;
+  const int lane = threadIdx.x;
+  const int32_t blk = list[w];
+  const int32_t r0 = blkStart[blk];
+  const int m = blkStart[blk + 1] - r0;
+  const int32_t base0 = rowptr[r0];
+  const int len = rowptr[r0 + 1] - base0;
+  for (int q = lane; q < len; q += 64) cols[q] = colidx[base0 + q];
+  for (int a = 0; a < m; ++a)
+    for (int q = lane; q < len; q += 64) rows[a][q] = (double)lu[rowptr[r0 + a] + q];
+  __syncthreads();
+  for (int a = 0; a < m; ++a) {
+    const int32_t r = r0 + a;
+    // eliminate with every lower column c < r in ascending order
+    for (int k = 0; k < len && cols[k] < r; ++k) {
+      const int32_t c = cols[k];
+      const bool internal = (c >= r0);
+      if (lane == 0) {
+        const double d = internal ? rows[c - r0][lds_find(cols, len, c)] : (double)lu[diagpos[c]];
+        if (d == 0.0) scal[S_MISC1] = 2.0;
+        piv = rows[a][k] / (d == 0.0 ? 1.0 : d);
+        rows[a][k] = piv;
+      }
+      __syncthreads();
+      const double lrk = piv;
+      if (internal) {
+        const int dc = lds_find(cols, len, c);
+        for (int q = dc + 1 + lane; q < len; q += 64) rows[a][q] -= lrk * rows[c - r0][q];
+      } else {
+        const int32_t cb = diagpos[c] + 1, ce = rowptr[c + 1];
+        for (int32_t q = cb + lane; q < ce; q += 64) {
+          const int pos = lds_find(cols, len, colidx[q]);
+          if (pos >= 0) rows[a][pos] -= lrk * (double)lu[q];
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int a = 0; a < m; ++a)
+    for (int q = lane; q < len; q += 64) lu[rowptr[r0 + a] + q] = (LT)rows[a][q];
+}
+
+// L y = b (unit lower) for the blocks of one level
+template <typename T, typename LT>
+__global__ __launch_bounds__(64) void k_ilu_lsolve_level(const int32_t* __restrict__ list, int count,
+                                                         const int32_t* __restrict__ blkStart,
+                                                         const int32_t* __restrict__ rowptr,
+                                                         const int32_t* __restrict__ colidx,
+                                                         const LT* __restrict__ lu,
+                                                         const T* __restrict__ b, T* __restrict__ z) {
+  __shared__ double zl[4];
+  const int w = blockIdx.x;
+  if (w >= count) return;
+  const int lane = threadIdx.x;
+  const int32_t blk = list[w];
+  const int32_t r0 = blkStart[blk];
+  const int m = blkStart[blk + 1] - r0;
+  for (int a = 0; a < m; ++a) {
+    const int32_t r = r0 + a;
+    const int32_t rb = rowptr[r], re = rowptr[r + 1];
+    double s = 0.0;
+    for (int32_t q = rb + lane; q < re; q += 64) {
+      const int32_t c = colidx[q];
+      if (c < r0) s += (double)lu[q] * (double)z[c];
+      else if (c < r) s += (double)lu[q] * zl[c - r0];
+    }
+    s = warp_sum(s);
+    if (lane == 0) zl[a] = (double)b[r] - s;
+    __syncthreads();
+  }
+  if (lane < m) z[r0 + lane] = (T)zl[lane];
+}
+
+// U z = y for the blocks of one level (levels ordered from the last node down)
+template <typename T, typename LT>
+__global__ __launch_bounds__(64) void k_ilu_usolve_level(const int32_t* __restrict__ list, int count,
+                                                         const int32_t* __restrict__ blkStart,
+                                                         const int32_t* __restrict__ rowptr,
+                                                         const int32_t* __restrict__ colidx,
+                                                         const int32_t* __restrict__ diagpos,
+                                                         const LT* __restrict__ lu,
+                                                         T* __restrict__ z) {
+  __shared__ double zl[4];
+  const int w = blockIdx.x;
+  if (w >= count) return;
+  const int lane = threadIdx.x;
+  const int32_t blk = list[w];
+  const int32_t r0 = blkStart[blk];
+  const int m = blkStart[blk + 1] - r0;
+  const int32_t r1 = r0 + m;
+  for (int a = m - 1; a >= 0; --a) {
+    const int32_t r = r0 + a;
+    const int32_t dp = diagpos[r], re = rowptr[r + 1];
+    double s = 0.0;
+    for (int32_t q = dp + 1 + lane; q < re; q += 64) {
+      const int32_t c = colidx[q];
+      if (c >= r1) s += (double)lu[q] * (double)z[c];
+      else s += (double)lu[q] * zl[c - r0];
+    }
+    s = warp_sum(s);
+    if (lane == 0) zl[a] = ((double)z[r] - s) / (double)lu[dp];
+    __syncthreads();
+  }
+  if (lane < m) z[r0 + lane] = (T)zl[lane];
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+#define LA_CHECK(x)                                  \
+  do {                                               \
+    hipError_t _e = (x);                             \
+    if (_e != hipSuccess) return set_hip_error(_e, #x); \
+  } while (0)
+
+static int grid_for(int64_t n) {
+  int64_t g = (n + kTPB - 1) / kTPB;
+  if (g > kRedBlocks) g = kRedBlocks;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+template <typename T>
+static hipError_t alloc(T** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  return hipMalloc((void**)p, n * sizeof(T));
+}
+
+LinAlg::~LinAlg() {
+  void* ptrs[] = {d_blkStart, d_diagpos, d_part, d_scal, r, rh, p, v, s, t, ph, sh, x, w, z,
+                  r32, rh32, p32, v32, s32, t32, ph32, sh32, x32, dinv, dinv32, binv, binv32,
+                  lu, lu32, d_levL, d_levU, gm_V, gm_V32};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  if (h_scal) (void)hipHostFree(h_scal);
+}
+
+int la_create(sfem_ctx* ctx) {
+  LinAlg* L = new LinAlg;
+  ctx->la = L;
+  const int64_t n = ctx->nfree;
+  L->n = n;
+  // nodal blocks: free rows of each node with at least one free dof
+  std::vector<int32_t> bs;
+  bs.reserve(ctx->nNode + 1);
+  const int ndof = ctx->dims.ndof;
+  int32_t acc = 0;
+  bs.push_back(0);
+  for (int64_t i = 0; i < ctx->nNode; ++i) {
+    int c = 0;
+    for (int d = 0; d < ndof; ++d) c += ctx->nodeDof[i * ndof + d] >= 0;
+    if (c) { acc += c; bs.push_back(acc); }
+  }
+  L->nblk = (int64_t)bs.size() - 1;
+  L->h_blkStart = bs;
+  // diagonal positions
+  std::vector<int32_t> dp(n);
+  int maxrow = 0;
+  for (int64_t r = 0; r < n; ++r) {
+    const int64_t b = ctx->rowptr[r], e = ctx->rowptr[r + 1];
+    maxrow = std::max(maxrow, (int)(e - b));
+    const int32_t* f = std::lower_bound(&ctx->colidx[0] + b, &ctx->colidx[0] + e, (int32_t)r);
+    dp[r] = (f != &ctx->colidx[0] + e && *f == r) ? (int32_t)(f - &ctx->colidx[0]) : -1;
+  }
+  L->maxrow = maxrow;
+  L->h_diagpos = dp;
+  LA_CHECK(alloc(&L->d_blkStart, bs.size()));
+  LA_CHECK(hipMemcpy(L->d_blkStart, bs.data(), bs.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  LA_CHECK(alloc(&L->d_diagpos, n));
+  LA_CHECK(hipMemcpy(L->d_diagpos, dp.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+  LA_CHECK(alloc(&L->d_part, (size_t)kRedBlocks * 4));
+  LA_CHECK(alloc(&L->d_scal, S_NSLOTS));
+  LA_CHECK(hipHostMalloc((void**)&L->h_scal, S_NSLOTS * sizeof(double), hipHostMallocDefault));
+  // SpMV lanes per row from the mean row length
+  const double avg = n ? (double)ctx->nnz / (double)n : 1.0;
+  int sub = 4;
+  while (sub < 64 && sub * 2 <= avg) sub *= 2;
+  L->spmv_sub = sub;
+  return SFEM_OK;
+}
+
+void la_destroy(sfem_ctx* ctx) {
+  delete ctx->la;
+  ctx->la = nullptr;
+}
+
+// --- level sets over nodal blocks (host) ---
+static void build_levels(sfem_ctx* ctx, LinAlg* L) {
+  const int64_t nb = L->nblk;
+  std::vector<int32_t> rowBlk(L->n);
+  for (int64_t k = 0; k < nb; ++k)
+    for (int32_t r = L->h_blkStart[k]; r < L->h_blkStart[k + 1]; ++r) rowBlk[r] = (int32_t)k;
+  std::vector<int32_t> lvL(nb, 0), lvU(nb, 0);
+  for (int64_t k = 0; k < nb; ++k) {
+    const int64_t r = L->h_blkStart[k];
+    int lv = 0;
+    for (int64_t q = ctx->rowptr[r]; q < ctx->rowptr[r + 1]; ++q) {
+      const int32_t bk = rowBlk[ctx->colidx[q]];
+      if (bk < k) lv = std::max(lv, lvL[bk] + 1);
+    }
+    lvL[k] = lv;
+  }
+  for (int64_t k = nb - 1; k >= 0; --k) {
+    const int64_t r = L->h_blkStart[k];
+    int lv = 0;
+    for (int64_t q = ctx->rowptr[r]; q < ctx->rowptr[r + 1]; ++q) {
+      const int32_t bk = rowBlk[ctx->colidx[q]];
+      if (bk > k) lv = std::max(lv, lvU[bk] + 1);
+    }
+    lvU[k] = lv;
+  }
+  auto bucket = [&](const std::vector<int32_t>& lv, std::vector<int32_t>& ptr, std::vector<int32_t>& list) {
+    int nl = 0;
+    for (int32_t x : lv) nl = std::max(nl, x + 1);
+    ptr.assign(nl + 1, 0);
+    for (int32_t x : lv) ptr[x + 1]++;
+    for (int i = 0; i < nl; ++i) ptr[i + 1] += ptr[i];
+    list.resize(lv.size());
+    std::vector<int32_t> pos(ptr.begin(), ptr.end() - 1);
+    for (int64_t k = 0; k < (int64_t)lv.size(); ++k) list[pos[lv[k]]++] = (int32_t)k;
+  };
+  bucket(lvL, L->levLPtr, L->h_levL);
+  bucket(lvU, L->levUPtr, L->h_levU);
+}
+
+template <typename T>
+static int ensure_vec(T** p, int64_t n) {
+  if (*p) return SFEM_OK;
+  hipError_t e = alloc(p, (size_t)n);
+  if (e != hipSuccess) return set_hip_error(e, "hipMalloc krylov vector");
+  return SFEM_OK;
+}
+
+// Build the preconditioner from the current matrix (FP64 vals or FP32 vals32).
+int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32) {
+  LinAlg* L = ctx->la;
+  hipStream_t st = ctx->stream;
+  const int64_t n = L->n;
+  L->pc_type = pc_type;
+  L->pc_fp32 = fp32;
+  const double* vals = fp32 ? nullptr : ctx->d_vals;
+  const float* vals32 = fp32 ? ctx->d_vals32 : nullptr;
+  LA_CHECK(hipMemsetAsync(L->d_scal + S_MISC1, 0, sizeof(double), st));
+  if (pc_type == SFEM_PC_JACOBI) {
+    for (int64_t r = 0; r < n; ++r)
+      if (L->h_diagpos[r] < 0) return set_error(SFEM_ERR_ARG, "Jacobi: missing diagonal entry");
+    if (fp32) {
+      int rc = ensure_vec(&L->dinv32, n); if (rc) return rc;
+      hipLaunchKernelGGL((k_jac_setup<float>), dim3((unsigned)((n + kTPB - 1) / kTPB)), dim3(kTPB), 0, st,
+                         n, L->d_diagpos, vals, vals32, L->dinv32, L->d_scal);
+    } else {
+      int rc = ensure_vec(&L->dinv, n); if (rc) return rc;
+      hipLaunchKernelGGL((k_jac_setup<double>), dim3((unsigned)((n + kTPB - 1) / kTPB)), dim3(kTPB), 0, st,
+                         n, L->d_diagpos, vals, vals32, L->dinv, L->d_scal);
+    }
+    LA_CHECK(hipGetLastError());
+  } else if (pc_type == SFEM_PC_BJACOBI4) {
+    const int64_t nb = L->nblk;
+    const unsigned g = (unsigned)((nb + kTPB - 1) / kTPB);
+    if (fp32) {
+      int rc = ensure_vec(&L->binv32, nb * 16); if (rc) return rc;
+      if (nb) hipLaunchKernelGGL((k_bjac_setup<float>), dim3(g), dim3(kTPB), 0, st, nb, L->d_blkStart,
+                                 ctx->d_rowptr32, ctx->d_colidx, vals, vals32, L->binv32, L->d_scal);
+    } else {
+      int rc = ensure_vec(&L->binv, nb * 16); if (rc) return rc;
+      if (nb) hipLaunchKernelGGL((k_bjac_setup<double>), dim3(g), dim3(kTPB), 0, st, nb, L->d_blkStart,
+                                 ctx->d_rowptr32, ctx->d_colidx, vals, vals32, L->binv, L->d_scal);
+    }
+    LA_CHECK(hipGetLastError());
+  } else if (pc_type == SFEM_PC_ILU0) {
+    if (L->maxrow > kIluMaxRow) return set_error(SFEM_ERR_UNSUPPORTED, "ILU(0): row longer than 256 entries");
+    for (int64_t r = 0; r < n; ++r)
+      if (L->h_diagpos[r] < 0) return set_error(SFEM_ERR_ARG, "ILU(0): missing diagonal entry");
+    if (L->h_levL.empty() && L->nblk > 0) {
+      build_levels(ctx, L);
+      LA_CHECK(alloc(&L->d_levL, L->h_levL.size()));
+      LA_CHECK(hipMemcpy(L->d_levL, L->h_levL.data(), L->h_levL.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+      LA_CHECK(alloc(&L->d_levU, L->h_levU.size()));
+      LA_CHECK(hipMemcpy(L->d_levU, L->h_levU.data(), L->h_levU.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
+    const int64_t nnz = ctx->nnz;
+    if (fp32) {
+      int rc = ensure_vec(&L->lu32, nnz); if (rc) return rc;
+      LA_CHECK(hipMemcpyAsync(L->lu32, ctx->d_vals32, nnz * sizeof(float), hipMemcpyDeviceToDevice, st));
+    } else {
+      int rc = ensure_vec(&L->lu, nnz); if (rc) return rc;
+      LA_CHECK(hipMemcpyAsync(L->lu, ctx->d_vals, nnz * sizeof(double), hipMemcpyDeviceToDevice, st));
+    }
+    const int nl = (int)L->levLPtr.size() - 1;
+    for (int lv = 0; lv < nl; ++lv) {
+      const int cnt = L->levLPtr[lv + 1] - L->levLPtr[lv];
+      if (fp32)
+        hipLaunchKernelGGL((k_ilu_factor_level<float>), dim3(cnt), dim3(64), 0, st, L->d_levL + L->levLPtr[lv],
+                           cnt, L->d_blkStart, ctx->d_rowptr32, ctx->d_colidx, L->d_diagpos, L->lu32, L->d_scal);
+      else
+        hipLaunchKernelGGL((k_ilu_factor_level<double>), dim3(cnt), dim3(64), 0, st, L->d_levL + L->levLPtr[lv],
+                           cnt, L->d_blkStart, ctx->d_rowptr32, ctx->d_colidx, L->d_diagpos, L->lu, L->d_scal);
+    }
+    LA_CHECK(hipGetLastError());
+  }
+  return SFEM_OK;
+}
+
+template <typename T>
+static int pc_apply_impl(sfem_ctx* ctx, const T* r, T* z);
+
+template <typename T>
+static int pc_apply(sfem_ctx* ctx, const T* r, T* z) {
+  ctx->kt.begin(SFEM_KT_PC, ctx->stream);
+  const int rc = pc_apply_impl<T>(ctx, r, z);
+  ctx->kt.end(SFEM_KT_PC, ctx->stream);
+  return rc;
+}
+
+template <typename T>
+static int pc_apply_impl(sfem_ctx* ctx, const T* r, T* z) {
+  LinAlg* L = ctx->la;
+  hipStream_t st = ctx->stream;
+  const int64_t n = L->n;
+  const int g = grid_for(n);
+  switch (L->pc_type) {
+    case SFEM_PC_JACOBI:
+      if (L->pc_fp32) hipLaunchKernelGGL((k_jac_apply<T, float>), dim3(g), dim3(kTPB), 0, st, n, L->dinv32, r, z);
+      else hipLaunchKernelGGL((k_jac_apply<T, double>), dim3(g), dim3(kTPB), 0, st, n, L->dinv, r, z);
+      break;
+    case SFEM_PC_BJACOBI4: {
+      const unsigned gb = (unsigned)((L->nblk + kTPB - 1) / kTPB);
+      if (L->nblk == 0) break;
+      if (L->pc_fp32) hipLaunchKernelGGL((k_bjac_apply<T, float>), dim3(gb), dim3(kTPB), 0, st, L->nblk, L->d_blkStart, L->binv32, r, z);
+      else hipLaunchKernelGGL((k_bjac_apply<T, double>), dim3(gb), dim3(kTPB), 0, st, L->nblk, L->d_blkStart, L->binv, r, z);
+      break;
+    }
+    case SFEM_PC_ILU0: {
+      const int nl = (int)L->levLPtr.size() - 1;
+      for (int lv = 0; lv < nl; ++lv) {
+        const int cnt = L->levLPtr[lv + 1] - L->levLPtr[lv];
+        if (L->pc_fp32)
+          hipLaunchKernelGGL((k_ilu_lsolve_level<T, float>), dim3(cnt), dim3(64), 0, st, L->d_levL + L->levLPtr[lv], cnt,
+                             L->d_blkStart, ctx->d_rowptr32, ctx->d_colidx, L->lu32, r, z);
+        else
+          hipLaunchKernelGGL((k_ilu_lsolve_level<T, double>), dim3(cnt), dim3(64), 0, st, L->d_levL + L->levLPtr[lv], cnt,
+                             L->d_blkStart, ctx->d_rowptr32, ctx->d_colidx, L->lu, r, z);
+      }
+      const int nu = (int)L->levUPtr.size() - 1;
+      for (int lv = 0; lv < nu; ++lv) {
+        const int cnt = L->levUPtr[lv + 1] - L->levUPtr[lv];
+        if (L->pc_fp32)
+          hipLaunchKernelGGL((k_ilu_usolve_level<T, float>), dim3(cnt), dim3(64), 0, st, L->d_levU + L->levUPtr[lv], cnt,
+                             L->d_blkStart, ctx->d_rowptr32, ctx->d_colidx, L->d_diagpos, L->lu32, z);
+        else
+          hipLaunchKernelGGL((k_ilu_usolve_level<T, double>), dim3(cnt), dim3(64), 0, st, L->d_levU + L->levUPtr[lv], cnt,
+                             L->d_blkStart, ctx->d_rowptr32, ctx->d_colidx, L->d_diagpos, L->lu, z);
+      }
+      break;
+    }
+    default:
+      hipLaunchKernelGGL((k_copy<T>), dim3(g), dim3(kTPB), 0, st, n, r, z);
+  }
+  return hipGetLastError() == hipSuccess ? SFEM_OK : set_error(SFEM_ERR_HIP, "preconditioner launch");
+}
+
+template <typename T>
+static void spmv(sfem_ctx* ctx, const T* x, T* y) {
+  LinAlg* L = ctx->la;
+  const int64_t n = L->n;
+  const int sub = L->spmv_sub;
+  const int64_t threads = n * sub;
+  const unsigned g = (unsigned)((threads + kTPB - 1) / kTPB);
+  if (n == 0) return;
+  ctx->kt.begin(SFEM_KT_SPMV, ctx->stream);
+#define SPMV_CASE(S)                                                                          \
+  case S:                                                                                     \
+    if (sizeof(T) == 8)                                                                       \
+      hipLaunchKernelGGL((k_spmv<double, T, S>), dim3(g), dim3(kTPB), 0, ctx->stream, n,      \
+                         ctx->d_rowptr32, ctx->d_colidx, ctx->d_vals, x, y);                  \
+    else                                                                                      \
+      hipLaunchKernelGGL((k_spmv<float, T, S>), dim3(g), dim3(kTPB), 0, ctx->stream, n,       \
+                         ctx->d_rowptr32, ctx->d_colidx, ctx->d_vals32, x, y);                \
+    break;
+  switch (sub) {
+    SPMV_CASE(4)
+    SPMV_CASE(8)
+    SPMV_CASE(16)
+    SPMV_CASE(32)
+    default:
+      if (sizeof(T) == 8)
+        hipLaunchKernelGGL((k_spmv<double, T, 64>), dim3(g), dim3(kTPB), 0, ctx->stream, n,
+                           ctx->d_rowptr32, ctx->d_colidx, ctx->d_vals, x, y);
+      else
+        hipLaunchKernelGGL((k_spmv<float, T, 64>), dim3(g), dim3(kTPB), 0, ctx->stream, n,
+                           ctx->d_rowptr32, ctx->d_colidx, ctx->d_vals32, x, y);
+  }
+#undef SPMV_CASE
+  ctx->kt.end(SFEM_KT_SPMV, ctx->stream);
+}
+
+int la_spmv64(sfem_ctx* ctx, const double* x, double* y) {
+  spmv<double>(ctx, x, y);
+  return hipGetLastError() == hipSuccess ? SFEM_OK : set_error(SFEM_ERR_HIP, "spmv launch");
+}
+int la_spmv32(sfem_ctx* ctx, const float* x, float* y) {
+  spmv<float>(ctx, x, y);
+  return hipGetLastError() == hipSuccess ? SFEM_OK : set_error(SFEM_ERR_HIP, "spmv launch");
+}
+
+// Sum of squares of a device vector into host (used for the Newton residual norm).
+int la_norm2(sfem_ctx* ctx, const double* a, double* out) {
+  LinAlg* L = ctx->la;
+  const int g = grid_for(L->n);
+  hipLaunchKernelGGL((k_dot<double>), dim3(g), dim3(kTPB), 0, ctx->stream, L->n, a, a, L->d_part);
+  hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(kTPB), 0, ctx->stream, L->d_part, g, L->d_scal,
+                     (int)S_MISC0, (int)FIN_STORE);
+  LA_CHECK(hipGetLastError());
+  LA_CHECK(hipMemcpyAsync(L->h_scal + S_MISC0, L->d_scal + S_MISC0, sizeof(double), hipMemcpyDeviceToHost,
+                          ctx->stream));
+  LA_CHECK(hipStreamSynchronize(ctx->stream));
+  *out = std::sqrt(L->h_scal[S_MISC0]);
+  return SFEM_OK;
+}
+
+template <typename T>
+static int bicgstab(sfem_ctx* ctx, const T* b, T* x, double rtol, int maxit, int* iters,
+                    double* relres, T* r, T* rh, T* p, T* v, T* s, T* t, T* ph, T* sh) {
+  LinAlg* L = ctx->la;
+  hipStream_t st = ctx->stream;
+  const int64_t n = L->n;
+  const int g = grid_for(n);
+  double* sc = L->d_scal;
+  double* hs = L->h_scal;
+  *iters = 0;
+  // x = 0, r = rh = b, ||b||
+  hipLaunchKernelGGL((k_fill<T>), dim3(g), dim3(kTPB), 0, st, n, x, (T)0);
+  hipLaunchKernelGGL((k_copy<T>), dim3(g), dim3(kTPB), 0, st, n, b, r);
+  hipLaunchKernelGGL((k_copy<T>), dim3(g), dim3(kTPB), 0, st, n, b, rh);
+  hipLaunchKernelGGL((k_fill<T>), dim3(g), dim3(kTPB), 0, st, n, v, (T)0);
+  hipLaunchKernelGGL((k_fill<T>), dim3(g), dim3(kTPB), 0, st, n, p, (T)0);
+  hipLaunchKernelGGL((k_dot<T>), dim3(g), dim3(kTPB), 0, st, n, b, b, L->d_part);
+  hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(kTPB), 0, st, L->d_part, g, sc, (int)S_BB, (int)FIN_STORE);
+  LA_CHECK(hipGetLastError());
+  LA_CHECK(hipMemcpyAsync(hs + S_BB, sc + S_BB, sizeof(double), hipMemcpyDeviceToHost, st));
+  LA_CHECK(hipStreamSynchronize(st));
+  const double bnorm = std::sqrt(hs[S_BB]);
+  if (!(bnorm > 0.0)) {
+    *relres = 0.0;
+    return std::isfinite(bnorm) ? SFEM_OK : set_error(SFEM_ERR_DIVERGED, "BiCGStab: non-finite rhs");
+  }
+  double init[S_NSLOTS];
+  std::memset(init, 0, sizeof(init));
+  init[S_RHO0] = 1.0; init[S_ALPHA] = 1.0; init[S_OMEGA] = 1.0;
+  init[S_BB] = hs[S_BB];
+  init[S_TOL2] = (rtol * bnorm) * (rtol * bnorm);
+  std::memcpy(hs, init, sizeof(init));
+  LA_CHECK(hipMemcpyAsync(sc, hs, S_NSLOTS * sizeof(double), hipMemcpyHostToDevice, st));
+  double rnorm = bnorm;
+  for (int it = 1; it <= maxit; ++it) {
+    hipLaunchKernelGGL((k_dot<T>), dim3(g), dim3(kTPB), 0, st, n, rh, r, L->d_part);
+    hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(kTPB), 0, st, L->d_part, g, sc, 0, (int)FIN_RHO1);
+    hipLaunchKernelGGL((k_bicg_p<T>), dim3(g), dim3(kTPB), 0, st, n, r, p, v, sc, it == 1 ? 1 : 0);
+    int rc = pc_apply<T>(ctx, p, ph); if (rc) return rc;
+    spmv<T>(ctx, ph, v);
+    hipLaunchKernelGGL((k_dot<T>), dim3(g), dim3(kTPB), 0, st, n, rh, v, L->d_part);
+    hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(kTPB), 0, st, L->d_part, g, sc, 0, (int)FIN_ALPHA);
+    hipLaunchKernelGGL((k_bicg_s<T>), dim3(g), dim3(kTPB), 0, st, n, r, v, s, sc, L->d_part);
+    hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(kTPB), 0, st, L->d_part, g, sc, 0, (int)FIN_SS);
+    rc = pc_apply<T>(ctx, s, sh); if (rc) return rc;
+    spmv<T>(ctx, sh, t);
+    hipLaunchKernelGGL((k_dot2<T>), dim3(g), dim3(kTPB), 0, st, n, t, t, t, s, L->d_part);
+    hipLaunchKernelGGL((k_finalize<2>), dim3(1), dim3(kTPB), 0, st, L->d_part, g, sc, 0, (int)FIN_OMEGA);
+    hipLaunchKernelGGL((k_bicg_xr<T>), dim3(g), dim3(kTPB), 0, st, n, x, ph, sh, r, s, t, sc, L->d_part);
+    hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(kTPB), 0, st, L->d_part, g, sc, 0, (int)FIN_RR);
+    LA_CHECK(hipGetLastError());
+    LA_CHECK(hipMemcpyAsync(hs, sc, S_NSLOTS * sizeof(double), hipMemcpyDeviceToHost, st));
+    LA_CHECK(hipStreamSynchronize(st));
+    *iters = it;
+    rnorm = std::sqrt(hs[S_RR]);
+    if (!std::isfinite(rnorm)) {
+      *relres = rnorm;
+      return set_error(SFEM_ERR_DIVERGED, "BiCGStab: non-finite residual");
+    }
+    if (hs[S_FLAG] == 1.0 || rnorm <= rtol * bnorm) {
+      *relres = rnorm / bnorm;
+      return SFEM_OK;
+    }
+    if (hs[S_FLAG] == 2.0) {
+      *relres = rnorm / bnorm;
+      return set_error(SFEM_ERR_DIVERGED, "BiCGStab: breakdown (rho or (rh, v) = 0)");
+    }
+  }
+  *relres = rnorm / bnorm;
+  return set_error(SFEM_ERR_MAXIT, "BiCGStab: maximum iterations reached");
+}
+
+// Restarted GMRES(m), right preconditioned, modified Gram-Schmidt; the small
+// Hessenberg least-squares problem is solved on the host with Givens rotations.
+template <typename T>
+static int gmres(sfem_ctx* ctx, const T* b, T* x, int m, double rtol, int maxit, int* iters,
+                 double* relres, T* V, T* w, T* z) {
+  LinAlg* L = ctx->la;
+  hipStream_t st = ctx->stream;
+  const int64_t n = L->n;
+  const int g = grid_for(n);
+  double* sc = L->d_scal;
+  double* hs = L->h_scal;
+  auto dotv = [&](const T* a, const T* c, double* out) -> int {
+    hipLaunchKernelGGL((k_dot<T>), dim3(g), dim3(kTPB), 0, st, n, a, c, L->d_part);
+    hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(kTPB), 0, st, L->d_part, g, sc, (int)S_MISC0, (int)FIN_STORE);
+    LA_CHECK(hipGetLastError());
+    LA_CHECK(hipMemcpyAsync(hs + S_MISC0, sc + S_MISC0, sizeof(double), hipMemcpyDeviceToHost, st));
+    LA_CHECK(hipStreamSynchronize(st));
+    *out = hs[S_MISC0];
+    return SFEM_OK;
+  };
+  hipLaunchKernelGGL((k_fill<T>), dim3(g), dim3(kTPB), 0, st, n, x, (T)0);
+  double bb;
+  int rc = dotv(b, b, &bb); if (rc) return rc;
+  const double bnorm = std::sqrt(bb);
+  *iters = 0;
+  if (!(bnorm > 0.0)) { *relres = 0.0; return SFEM_OK; }
+  std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), gv(m + 1), y(m);
+  int it = 0;
+  double rnorm = bnorm;
+  T* r = z;  // reuse z as residual storage at restart
+  while (it < maxit) {
+    spmv<T>(ctx, x, w);
+    hipLaunchKernelGGL((k_copy<T>), dim3(g), dim3(kTPB), 0, st, n, b, r);
+    hipLaunchKernelGGL((k_axpby<T>), dim3(g), dim3(kTPB), 0, st, n, -1.0, w, 1.0, r);
+    double beta2;
+    rc = dotv(r, r, &beta2); if (rc) return rc;
+    const double beta = std::sqrt(beta2);
+    rnorm = beta;
+    if (!std::isfinite(beta)) return set_error(SFEM_ERR_DIVERGED, "GMRES: non-finite residual");
+    if (beta <= rtol * bnorm) break;
+    hipLaunchKernelGGL((k_convert<T, T>), dim3(g), dim3(kTPB), 0, st, n, r, V, 1.0 / beta);
+    std::fill(gv.begin(), gv.end(), 0.0);
+    gv[0] = beta;
+    int k = 0;
+    for (; k < m && it < maxit; ++k) {
+      ++it;
+      T* vk = V + (size_t)k * n;
+      rc = pc_apply<T>(ctx, vk, z); if (rc) return rc;
+      spmv<T>(ctx, z, w);
+      for (int j = 0; j <= k; ++j) {
+        double h;
+        rc = dotv(w, V + (size_t)j * n, &h); if (rc) return rc;
+        H[(size_t)j * m + k] = h;
+        hipLaunchKernelGGL((k_axpby<T>), dim3(g), dim3(kTPB), 0, st, n, -h, V + (size_t)j * n, 1.0, w);
+      }
+      double hn2;
+      rc = dotv(w, w, &hn2); if (rc) return rc;
+      const double hn = std::sqrt(hn2);
+      H[(size_t)(k + 1) * m + k] = hn;
+      if (hn != 0.0)
+        hipLaunchKernelGGL((k_convert<T, T>), dim3(g), dim3(kTPB), 0, st, n, w, V + (size_t)(k + 1) * n, 1.0 / hn);
+      for (int j = 0; j < k; ++j) {
+        const double t1 = H[(size_t)j * m + k], t2 = H[(size_t)(j + 1) * m + k];
+        H[(size_t)j * m + k] = cs[j] * t1 + sn[j] * t2;
+        H[(size_t)(j + 1) * m + k] = -sn[j] * t1 + cs[j] * t2;
+      }
+      const double a = H[(size_t)k * m + k], bq = H[(size_t)(k + 1) * m + k];
+      const double d = std::sqrt(a * a + bq * bq);
+      cs[k] = d == 0.0 ? 1.0 : a / d;
+      sn[k] = d == 0.0 ? 0.0 : bq / d;
+      H[(size_t)k * m + k] = d;
+      H[(size_t)(k + 1) * m + k] = 0.0;
+      gv[k + 1] = -sn[k] * gv[k];
+      gv[k] = cs[k] * gv[k];
+      rnorm = std::fabs(gv[k + 1]);
+      if (rnorm <= rtol * bnorm || hn == 0.0) { ++k; break; }
+    }
+    for (int j = k - 1; j >= 0; --j) {
+      double s = gv[j];
+      for (int q = j + 1; q < k; ++q) s -= H[(size_t)j * m + q] * y[q];
+      y[j] = s / H[(size_t)j * m + j];
+    }
+    hipLaunchKernelGGL((k_fill<T>), dim3(g), dim3(kTPB), 0, st, n, w, (T)0);
+    for (int j = 0; j < k; ++j)
+      hipLaunchKernelGGL((k_axpby<T>), dim3(g), dim3(kTPB), 0, st, n, y[j], V + (size_t)j * n, 1.0, w);
+    rc = pc_apply<T>(ctx, w, z); if (rc) return rc;
+    hipLaunchKernelGGL((k_axpby<T>), dim3(g), dim3(kTPB), 0, st, n, 1.0, z, 1.0, x);
+    LA_CHECK(hipGetLastError());
+    if (rnorm <= rtol * bnorm) break;
+  }
+  // true residual
+  spmv<T>(ctx, x, w);
+  hipLaunchKernelGGL((k_copy<T>), dim3(g), dim3(kTPB), 0, st, n, b, z);
+  hipLaunchKernelGGL((k_axpby<T>), dim3(g), dim3(kTPB), 0, st, n, -1.0, w, 1.0, z);
+  double rr;
+  rc = dotv(z, z, &rr); if (rc) return rc;
+  rnorm = std::sqrt(rr);
+  *iters = it;
+  *relres = rnorm / bnorm;
+  if (!std::isfinite(rnorm)) return set_error(SFEM_ERR_DIVERGED, "GMRES: non-finite residual");
+  // the true residual may sit a few ulps above the recurrence's estimate
+  if (rnorm > rtol * bnorm * 1.001) return set_error(SFEM_ERR_MAXIT, "GMRES: maximum iterations reached");
+  return SFEM_OK;
+}
+
+// Solve A du = rhs.  FP64 mode: everything in double.  Mixed mode: FP32
+// matrix and Krylov vectors; the FP64 rhs is scaled by 1/||rhs|| into FP32 and
+// the FP32 solution scaled back into the FP64 increment.
+int la_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int* iters, double* relres) {
+  LinAlg* L = ctx->la;
+  const int64_t n = L->n;
+  const bool fp32 = ctx->precision == SFEM_PREC_MIXED;
+  int rc;
+  // the tangent is constant within a time step: rebuild the preconditioner only when it changed
+  if (L->pc_K_version != ctx->la_K_version || L->pc_type != o->pc_type || L->pc_fp32 != fp32) {
+    if ((rc = la_setup_pc(ctx, o->pc_type, fp32))) return rc;
+    L->pc_K_version = ctx->la_K_version;
+  }
+  const int m = std::max(1, o->restart);
+  if (!fp32) {
+    if (o->ksp_type == SFEM_KSP_GMRES) {
+      if ((rc = ensure_vec(&L->w, n)) || (rc = ensure_vec(&L->z, n))) return rc;
+      if (!L->gm_V || L->gm_m < m) {
+        if (L->gm_V) (void)hipFree(L->gm_V);
+        L->gm_V = nullptr;
+        if ((rc = ensure_vec(&L->gm_V, (int64_t)(m + 1) * n))) return rc;
+        L->gm_m = m;
+      }
+      return gmres<double>(ctx, ctx->d_rhs, ctx->d_du, m, o->rtol, o->max_it, iters, relres, L->gm_V,
+                           L->w, L->z);
+    }
+    double** vs[] = {&L->r, &L->rh, &L->p, &L->v, &L->s, &L->t, &L->ph, &L->sh};
+    for (double** q : vs)
+      if ((rc = ensure_vec(q, n))) return rc;
+    return bicgstab<double>(ctx, ctx->d_rhs, ctx->d_du, o->rtol, o->max_it, iters, relres, L->r, L->rh,
+                            L->p, L->v, L->s, L->t, L->ph, L->sh);
+  }
+  // mixed precision
+  float** vs[] = {&L->r32, &L->rh32, &L->p32, &L->v32, &L->s32, &L->t32, &L->ph32, &L->sh32, &L->x32};
+  for (float** q : vs)
+    if ((rc = ensure_vec(q, n))) return rc;
+  double bn;
+  if ((rc = la_norm2(ctx, ctx->d_rhs, &bn))) return rc;
+  const int g = grid_for(n);
+  if (!(bn > 0.0)) {
+    hipLaunchKernelGGL((k_fill<double>), dim3(g), dim3(kTPB), 0, ctx->stream, n, ctx->d_du, 0.0);
+    *iters = 0; *relres = 0.0;
+    return SFEM_OK;
+  }
+  // b32 = rhs / ||rhs|| into the FP32 "s" slot's sibling (rh32 is overwritten by the solver, use x32 as b)
+  float* b32 = L->x32;
+  hipLaunchKernelGGL((k_convert<double, float>), dim3(g), dim3(kTPB), 0, ctx->stream, n, ctx->d_rhs, b32, 1.0 / bn);
+  if (!L->du32) { if ((rc = ensure_vec(&L->du32, n))) return rc; }
+  if (o->ksp_type == SFEM_KSP_GMRES) {
+    if ((rc = ensure_vec(&L->w32, n)) || (rc = ensure_vec(&L->z32, n))) return rc;
+    if (!L->gm_V32 || L->gm_m32 < m) {
+      if (L->gm_V32) (void)hipFree(L->gm_V32);
+      L->gm_V32 = nullptr;
+      if ((rc = ensure_vec(&L->gm_V32, (int64_t)(m + 1) * n))) return rc;
+      L->gm_m32 = m;
+    }
+    rc = gmres<float>(ctx, b32, L->du32, m, o->rtol, o->max_it, iters, relres, L->gm_V32, L->w32, L->z32);
+  } else {
+    rc = bicgstab<float>(ctx, b32, L->du32, o->rtol, o->max_it, iters, relres, L->r32, L->rh32, L->p32,
+                         L->v32, L->s32, L->t32, L->ph32, L->sh32);
+  }
+  hipLaunchKernelGGL((k_convert<float, double>), dim3(g), dim3(kTPB), 0, ctx->stream, n, L->du32, ctx->d_du, bn);
+  if (hipGetLastError() != hipSuccess) return set_error(SFEM_ERR_HIP, "convert launch");
+  return rc;
+}
+
+}  // namespace sfem

@@ -1,0 +1,59 @@
+// sfem_linalg.hpp -- Krylov workspace and entry points (implemented in sfem_linalg.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <type_traits>
+#include <vector>
+
+#include "sfem_internal.hpp"
+
+namespace sfem {
+
+// Error reporting shared by all translation units (sfem_api.cpp).
+int set_error(int code, const char* msg);
+int set_hip_error(hipError_t e, const char* what);
+
+struct LinAlg {
+  int64_t n = 0;             // free DOFs
+  int64_t nblk = 0;          // nodal blocks (nodes with >= 1 free DOF)
+  int maxrow = 0;
+  int spmv_sub = 4;          // lanes per CSR row
+  int pc_type = SFEM_PC_NONE;
+  bool pc_fp32 = false;
+  int64_t pc_K_version = -1;  // tangent version the preconditioner was built from
+  std::vector<int32_t> h_blkStart, h_diagpos;
+  std::vector<int32_t> levLPtr, levUPtr, h_levL, h_levU;
+  int32_t *d_blkStart = nullptr, *d_diagpos = nullptr, *d_levL = nullptr, *d_levU = nullptr;
+  double* d_part = nullptr;  // reduction partials
+  double* d_scal = nullptr;  // Krylov scalars
+  double* h_scal = nullptr;  // pinned mirror
+  // FP64 Krylov vectors
+  double *r = nullptr, *rh = nullptr, *p = nullptr, *v = nullptr, *s = nullptr, *t = nullptr,
+         *ph = nullptr, *sh = nullptr, *x = nullptr, *w = nullptr, *z = nullptr, *gm_V = nullptr;
+  int gm_m = 0;
+  // FP32 Krylov vectors
+  float *r32 = nullptr, *rh32 = nullptr, *p32 = nullptr, *v32 = nullptr, *s32 = nullptr,
+        *t32 = nullptr, *ph32 = nullptr, *sh32 = nullptr, *x32 = nullptr, *du32 = nullptr,
+        *w32 = nullptr, *z32 = nullptr, *gm_V32 = nullptr;
+  int gm_m32 = 0;
+  // preconditioners
+  double* dinv = nullptr;
+  float* dinv32 = nullptr;
+  double* binv = nullptr;
+  float* binv32 = nullptr;
+  double* lu = nullptr;
+  float* lu32 = nullptr;
+  ~LinAlg();
+};
+
+int la_create(sfem_ctx* ctx);
+void la_destroy(sfem_ctx* ctx);
+int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32);
+int la_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int* iters, double* relres);
+int la_norm2(sfem_ctx* ctx, const double* a, double* out);
+int la_spmv64(sfem_ctx* ctx, const double* x, double* y);
+int la_spmv32(sfem_ctx* ctx, const float* x, float* y);
+
+}  // namespace sfem

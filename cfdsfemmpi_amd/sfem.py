@@ -1,0 +1,355 @@
+"""ctypes binding of libsfem_hip.so and the host driver mirroring ``StabFEM``.
+
+``StabFEM`` keeps the reference driver's method names and call order
+(src/base/StabFEM.h:101-153, src/cfdsfemmpi.cpp:22-89):
+``readInputData`` -> ``readControlParameters`` -> ``prepareInputData`` ->
+``setSolver`` -> ``solveFullyImplicit`` -> ``checkResult`` / ``writeResult``.
+Every numerical step runs in the HIP library through the C ABI of
+``include/sfem.h``; there is no CPU fallback -- if the library cannot be
+loaded or no GPU is present, construction raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+from . import deck as _deck
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsfem_hip.so")
+
+SFEM_OK = 0
+ERRORS = {-1: "SFEM_ERR_ARG", -2: "SFEM_ERR_STATE", -3: "SFEM_ERR_NEGJAC", -4: "SFEM_ERR_HIP",
+          -5: "SFEM_ERR_DIVERGED", -6: "SFEM_ERR_MAXIT", -7: "SFEM_ERR_NOMEM", -8: "SFEM_ERR_NOTCONV",
+          -9: "SFEM_ERR_UNSUPPORTED"}
+PREC_FP64, PREC_MIXED = 0, 1
+KT_ASSEMBLY, KT_SPMV, KT_PC = 0, 1, 2
+KSP = {"bcgs": 0, "gmres": 1}
+PC = {"none": 0, "jacobi": 1, "bjacobi4": 2, "ilu": 3}
+
+# (function name, restype, argtypes) of every entry point declared in include/sfem.h
+_i64p = C.POINTER(C.c_int64)
+_i32p = C.POINTER(C.c_int32)
+_f64p = C.POINTER(C.c_double)
+_vp = C.c_void_p
+
+
+class SolverOpts(C.Structure):
+    _fields_ = [("ksp_type", C.c_int), ("pc_type", C.c_int), ("rtol", C.c_double),
+                ("max_it", C.c_int), ("restart", C.c_int)]
+
+
+class StepStats(C.Structure):
+    _fields_ = [("newton_iters", C.c_int), ("krylov_iters", C.c_int), ("rhs_norm0", C.c_double),
+                ("rhs_norm", C.c_double), ("t_assembly_ms", C.c_double), ("t_solver_ms", C.c_double)]
+
+
+ABI = [
+    ("sfem_version", C.c_char_p, []),
+    ("sfem_last_error", C.c_char_p, []),
+    ("sfem_create", _vp, [C.c_int, C.c_int, C.c_int64, C.c_int64, _f64p, _i32p, C.c_int]),
+    ("sfem_destroy", None, [_vp]),
+    ("sfem_set_dirichlet", C.c_int, [_vp, C.c_int64, _i32p, _i32p, _f64p]),
+    ("sfem_prepare_elements", C.c_int, [_vp]),
+    ("sfem_build_pattern", C.c_int, [_vp, _i64p, _i64p]),
+    ("sfem_get_pattern", C.c_int, [_vp, _i64p, _i32p, _i32p]),
+    ("sfem_set_params", C.c_int, [_vp, _f64p, C.c_int, C.c_double, C.c_double]),
+    ("sfem_set_precision", C.c_int, [_vp, C.c_int]),
+    ("sfem_time_update", C.c_int, [_vp, C.c_double]),
+    ("sfem_assemble", C.c_int, [_vp, C.c_int, C.c_int, _f64p]),
+    ("sfem_element_KF", C.c_int, [_vp, C.c_int64, _f64p, _f64p]),
+    ("sfem_get_system", C.c_int, [_vp, _f64p, _f64p]),
+    ("sfem_solve", C.c_int, [_vp, C.POINTER(SolverOpts), C.POINTER(C.c_int), _f64p]),
+    ("sfem_update_solution", C.c_int, [_vp]),
+    ("sfem_step", C.c_int, [_vp, C.c_double, C.POINTER(SolverOpts), C.c_double, C.c_int,
+                            C.POINTER(StepStats)]),
+    ("sfem_get_solution", C.c_int, [_vp, _f64p]),
+    ("sfem_set_solution", C.c_int, [_vp, _f64p]),
+    ("sfem_get_increment", C.c_int, [_vp, _f64p]),
+    ("sfem_set_state", C.c_int, [_vp, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p]),
+    ("sfem_reset_state", C.c_int, [_vp]),
+    ("sfem_enable_kernel_timing", C.c_int, [_vp, C.c_int]),
+    ("sfem_kernel_timing", C.c_int, [_vp, C.c_int, _i64p, _f64p]),
+    ("sfem_info", C.c_int, [_vp, _i64p, _i64p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    ("sfem_last_timings", C.c_int, [_vp, _f64p, _f64p]),
+]
+
+_LIB = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libsfem_hip.so (raises if it is missing -- build it first)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: run `python -m cfdsfemmpi_amd.build` first")
+    L = C.CDLL(path)
+    for name, res, args in ABI:
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _LIB = L
+    return L
+
+
+class SfemError(RuntimeError):
+    def __init__(self, code: int, where: str):
+        lib = load_library()
+        msg = lib.sfem_last_error().decode()
+        super().__init__(f"{where}: {ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def _p(a: Optional[np.ndarray], ct):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def _check(code: int, where: str):
+    if code != SFEM_OK:
+        raise SfemError(code, where)
+
+
+class Context:
+    """Thin owner of one ``sfem_ctx``."""
+
+    def __init__(self, mesh: _deck.Mesh, device: int = -1):
+        self.lib = load_library()
+        self.mesh = mesh
+        self.ndof = mesh.ndim + 1
+        self._coords = np.ascontiguousarray(mesh.coords, np.float64)
+        self._conn = np.ascontiguousarray(mesh.conn, np.int32)
+        h = self.lib.sfem_create(mesh.ndim, mesh.npElem, mesh.nNode, mesh.nElem,
+                                 _p(self._coords, C.c_double), _p(self._conn, C.c_int32), device)
+        if not h:
+            raise RuntimeError("sfem_create failed: " + self.lib.sfem_last_error().decode())
+        self.h = C.c_void_p(h)
+        self.nfree = 0
+        self.nnz = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.sfem_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- setup -------------------------------------------------------------
+    def set_dirichlet(self, node, dof, value):
+        node = np.ascontiguousarray(node, np.int32)
+        dof = np.ascontiguousarray(dof, np.int32)
+        value = np.ascontiguousarray(value, np.float64)
+        _check(self.lib.sfem_set_dirichlet(self.h, len(node), _p(node, C.c_int32), _p(dof, C.c_int32),
+                                           _p(value, C.c_double)), "sfem_set_dirichlet")
+
+    def prepare_elements(self):
+        _check(self.lib.sfem_prepare_elements(self.h), "sfem_prepare_elements")
+
+    def build_pattern(self):
+        nf = C.c_int64(0)
+        nz = C.c_int64(0)
+        _check(self.lib.sfem_build_pattern(self.h, C.byref(nf), C.byref(nz)), "sfem_build_pattern")
+        self.nfree, self.nnz = nf.value, nz.value
+        return self.nfree, self.nnz
+
+    def get_pattern(self):
+        rp = np.zeros(self.nfree + 1, np.int64)
+        ci = np.zeros(max(self.nnz, 1), np.int32)
+        af = np.zeros(max(self.nfree, 1), np.int32)
+        _check(self.lib.sfem_get_pattern(self.h, _p(rp, C.c_int64), _p(ci, C.c_int32), _p(af, C.c_int32)),
+               "sfem_get_pattern")
+        return rp, ci[:self.nnz], af[:self.nfree]
+
+    def set_params(self, elemData, tis: int, rhoInf: float, dt: float):
+        ed = np.ascontiguousarray(elemData, np.float64)
+        assert ed.size == 11
+        _check(self.lib.sfem_set_params(self.h, _p(ed, C.c_double), int(tis), float(rhoInf), float(dt)),
+               "sfem_set_params")
+
+    def set_precision(self, prec: int):
+        _check(self.lib.sfem_set_precision(self.h, int(prec)), "sfem_set_precision")
+
+    # -- hot path ----------------------------------------------------------
+    def time_update(self, timeFact: float):
+        _check(self.lib.sfem_time_update(self.h, float(timeFact)), "sfem_time_update")
+
+    def assemble(self, it: int, want_K: bool = True) -> float:
+        nrm = C.c_double(0.0)
+        _check(self.lib.sfem_assemble(self.h, int(it), 1 if want_K else 0, C.byref(nrm)), "sfem_assemble")
+        return nrm.value
+
+    def element_KF(self, e: int):
+        ns = self.mesh.npElem * self.ndof
+        K = np.zeros(ns * ns)
+        F = np.zeros(ns)
+        _check(self.lib.sfem_element_KF(self.h, int(e), _p(K, C.c_double), _p(F, C.c_double)),
+               "sfem_element_KF")
+        return K.reshape(ns, ns), F
+
+    def get_system(self):
+        v = np.zeros(max(self.nnz, 1))
+        r = np.zeros(max(self.nfree, 1))
+        _check(self.lib.sfem_get_system(self.h, _p(v, C.c_double), _p(r, C.c_double)), "sfem_get_system")
+        return v[:self.nnz], r[:self.nfree]
+
+    def solve(self, opts: SolverOpts):
+        it = C.c_int(0)
+        rr = C.c_double(0.0)
+        rc = self.lib.sfem_solve(self.h, C.byref(opts), C.byref(it), C.byref(rr))
+        _check(rc, "sfem_solve")
+        return it.value, rr.value
+
+    def update_solution(self):
+        _check(self.lib.sfem_update_solution(self.h), "sfem_update_solution")
+
+    def step(self, timeFact: float, opts: SolverOpts, newton_tol: float = 1e-8, max_iter: int = 10,
+             allow_notconv: bool = True) -> StepStats:
+        st = StepStats()
+        rc = self.lib.sfem_step(self.h, float(timeFact), C.byref(opts), float(newton_tol), int(max_iter),
+                                C.byref(st))
+        if rc == -8 and allow_notconv:  # the reference moves on after 10 iterations
+            return st
+        _check(rc, "sfem_step")
+        return st
+
+    def get_solution(self) -> np.ndarray:
+        s = np.zeros(self.mesh.nNode * self.ndof)
+        _check(self.lib.sfem_get_solution(self.h, _p(s, C.c_double)), "sfem_get_solution")
+        return s
+
+    def set_solution(self, s: np.ndarray):
+        s = np.ascontiguousarray(s, np.float64)
+        _check(self.lib.sfem_set_solution(self.h, _p(s, C.c_double)), "sfem_set_solution")
+
+    def get_increment(self) -> np.ndarray:
+        d = np.zeros(max(self.nfree, 1))
+        _check(self.lib.sfem_get_increment(self.h, _p(d, C.c_double)), "sfem_get_increment")
+        return d[:self.nfree]
+
+    def set_state(self, soln=None, solnPrev=None, solnCur=None, solnDotCur=None, solnDot=None,
+                  solnDotPrev=None):
+        arrs = [None if a is None else np.ascontiguousarray(a, np.float64)
+                for a in (soln, solnPrev, solnCur, solnDotCur, solnDot, solnDotPrev)]
+        _check(self.lib.sfem_set_state(self.h, *[_p(a, C.c_double) for a in arrs]), "sfem_set_state")
+
+    def reset_state(self):
+        _check(self.lib.sfem_reset_state(self.h), "sfem_reset_state")
+
+    def enable_kernel_timing(self, on: bool = True):
+        _check(self.lib.sfem_enable_kernel_timing(self.h, 1 if on else 0), "sfem_enable_kernel_timing")
+
+    def kernel_timing(self, kind: int):
+        n, ms = C.c_int64(0), C.c_double(0.0)
+        _check(self.lib.sfem_kernel_timing(self.h, int(kind), C.byref(n), C.byref(ms)), "sfem_kernel_timing")
+        return n.value, ms.value
+
+    def info(self):
+        nf, nz = C.c_int64(0), C.c_int64(0)
+        nc, nd = C.c_int(0), C.c_int(0)
+        _check(self.lib.sfem_info(self.h, C.byref(nf), C.byref(nz), C.byref(nc), C.byref(nd)), "sfem_info")
+        return dict(nfree=nf.value, nnz=nz.value, ncolors=nc.value, ndof=nd.value)
+
+    def last_timings(self):
+        a, s = C.c_double(0.0), C.c_double(0.0)
+        _check(self.lib.sfem_last_timings(self.h, C.byref(a), C.byref(s)), "sfem_last_timings")
+        return a.value, s.value
+
+
+def solver_opts(ksp="bcgs", pc="ilu", rtol=1e-8, max_it=1000, restart=30) -> SolverOpts:
+    if pc == "lu":
+        # the reference's test decks ask for MUMPS LU; a tight ILU(0)-BiCGStab solve
+        # reproduces the direct answer to well below the decks' tolerances
+        ksp, pc, rtol, max_it = "bcgs", "ilu", 1e-13, max(max_it, 5000)
+    return SolverOpts(KSP[ksp], PC[pc], float(rtol), int(max_it), int(restart))
+
+
+class StabFEM:
+    """Host driver with the reference's orchestration (one GPU, one rank)."""
+
+    def __init__(self, device: int = -1):
+        self.device = device
+        self.mesh: Optional[_deck.Mesh] = None
+        self.ctl: Optional[_deck.Control] = None
+        self.ctx: Optional[Context] = None
+        self.opts = solver_opts()
+        self.newton_tol = 1.0e-8   # hard-coded absolute tolerance (StabFEM2.cpp:749)
+        self.max_newton = 10       # hard-coded iteration cap (StabFEM2.cpp:654)
+        self.precision = PREC_FP64
+        self.history = []          # (tstep, iter, rhs_norm, krylov_its)
+
+    # StabFEM::readInputData (StabFEM.cpp:80-264)
+    def readInputData(self, fname: str):
+        self.mesh = _deck.read_mesh(fname)
+
+    # StabFEM::readControlParameters (StabFEM.cpp:268-322)
+    def readControlParameters(self, fname: str):
+        self.ctl = _deck.read_control(fname)
+
+    def setMesh(self, mesh: _deck.Mesh):
+        self.mesh = mesh
+
+    def setControl(self, ctl: _deck.Control):
+        self.ctl = ctl
+
+    # StabFEM::prepareInputData (StabFEM.cpp:328-386)
+    def prepareInputData(self):
+        m = self.mesh
+        self.ctx = Context(m, self.device)
+        self.ctx.set_dirichlet(m.dbc_node, m.dbc_dof, m.dbc_value)
+
+    # StabFEM::setSolver -> prepareMatrixPattern (StabFEM2.cpp:10-319); the options
+    # file stands in for PetscInitialize(optfile) (src/cfdsfemmpi.cpp:40)
+    def setSolver(self, options: Optional[object] = None):
+        if isinstance(options, str):
+            so = _deck.read_solver_options(options)
+            self.opts = solver_opts(so.ksp_type, so.pc_type, so.rtol, so.max_it, so.restart)
+        elif isinstance(options, SolverOpts):
+            self.opts = options
+        self.ctx.prepare_elements()
+        self.ctx.build_pattern()
+        c = self.ctl
+        self.ctx.set_params(c.elemData, c.tis, c.rhoInf, c.dt)
+        self.ctx.set_precision(self.precision)
+
+    # StabFEM::solveFullyImplicit (StabFEM2.cpp:593-832)
+    def solveFullyImplicit(self, steps: Optional[int] = None):
+        c = self.ctl
+        ctx = self.ctx
+        stepsMax = c.stepsMax if steps is None else steps
+        timeNow = c.dt
+        for tstep in range(stepsMax):
+            timeFact = timeNow if timeNow <= 1.0 else 1.0
+            ctx.time_update(timeFact)
+            for it in range(self.max_newton):
+                nrm = ctx.assemble(it, want_K=(it == 0))
+                if nrm < self.newton_tol:
+                    self.history.append((tstep, it, nrm, 0))
+                    break
+                its, _ = ctx.solve(self.opts)
+                self.history.append((tstep, it, nrm, its))
+                ctx.update_solution()
+            timeNow = timeNow + c.dt
+            if timeNow > c.timeFinal:
+                break
+        return ctx.get_solution()
+
+    def getSolution(self) -> np.ndarray:
+        return self.ctx.get_solution()
+
+    # StabFEM::checkResult (StabFEM.cpp:501-554)
+    def checkResult(self, fname: str) -> int:
+        ref = _deck.read_result(fname)
+        return _deck.check_result(self.getSolution(), ref, self.mesh.ndim)
+
+    # StabFEM::writeResult (StabFEM.cpp:457-494)
+    def writeResult(self, fname: str):
+        _deck.write_result(fname, self.getSolution(), self.mesh.ndim + 1)

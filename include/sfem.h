@@ -1,0 +1,178 @@
+/*
+ * sfem.h -- C ABI of the MI355X-native stabilised-FEM hot path (libsfem_hip.so).
+ *
+ * The reference (chennachaos/cfdsfemmpi) has two seams on this path, both
+ * called from StabFEM: the element seam (ElementBase::prepareElemData /
+ * calcStiffnessAndResidual, src/base/ElementBase.h:81-88) and the solver seam
+ * (SolverPetsc, src/base/SolverPetsc.h:15-65).  This ABI replaces both behind
+ * one opaque context that owns all device data; each entry point below cites
+ * the reference interface it stands in for.
+ *
+ * Conventions: every int-returning call returns SFEM_OK (0) or a negative
+ * SFEM_ERR_* code and never throws; buffers are caller-owned host memory;
+ * node and element ids are 0-based; solution vectors are node-major AoS
+ * (node*ndof + dof), as SolutionData holds them.  One context drives one
+ * device stream and is not thread-safe.  sfem_last_error() gives the message
+ * of the most recent failure on the calling thread.
+ */
+#ifndef SFEM_H
+#define SFEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sfem_ctx sfem_ctx;
+
+enum {
+  SFEM_OK = 0,
+  SFEM_ERR_ARG = -1,        /* bad argument or size */
+  SFEM_ERR_STATE = -2,      /* call out of order (SolverPetsc status machine, SolverPetsc.h:13) */
+  SFEM_ERR_NEGJAC = -3,     /* negative Jacobian (the reference exit()s, Hexa8Node.cpp:79-84) */
+  SFEM_ERR_HIP = -4,        /* HIP runtime error */
+  SFEM_ERR_DIVERGED = -5,   /* Krylov breakdown or non-finite residual (SolverPetsc.cpp:264-270) */
+  SFEM_ERR_MAXIT = -6,      /* Krylov hit max iterations before rtol */
+  SFEM_ERR_NOMEM = -7,      /* device or host allocation failed */
+  SFEM_ERR_NOTCONV = -8,    /* Newton loop hit its iteration cap (the reference moves on silently) */
+  SFEM_ERR_UNSUPPORTED = -9 /* element type / option not available */
+};
+
+/* Precision of the tangent and Krylov solve (residual and update are always FP64). */
+enum { SFEM_PREC_FP64 = 0, SFEM_PREC_MIXED = 1 };
+
+/* Krylov methods and preconditioners (vocabulary of the PETSc options files). */
+enum { SFEM_KSP_BCGS = 0, SFEM_KSP_GMRES = 1 };
+enum { SFEM_PC_NONE = 0, SFEM_PC_JACOBI = 1, SFEM_PC_BJACOBI4 = 2, SFEM_PC_ILU0 = 3 };
+
+typedef struct sfem_solver_opts {
+  int ksp_type;      /* SFEM_KSP_* (-ksp_type) */
+  int pc_type;       /* SFEM_PC_*  (-pc_type / -sub_pc_type) */
+  double rtol;       /* -ksp_rtol: stop when ||b - A x||_2 <= rtol ||b||_2 */
+  int max_it;        /* -ksp_max_it */
+  int restart;       /* -ksp_gmres_restart */
+} sfem_solver_opts;
+
+typedef struct sfem_step_stats {
+  int newton_iters;      /* Newton iterations taken (assemblies) */
+  int krylov_iters;      /* total Krylov iterations in the step */
+  double rhs_norm0;      /* ||rhs_free||_2 at iteration 0 */
+  double rhs_norm;       /* last ||rhs_free||_2 (the convergence measure) */
+  double t_assembly_ms;  /* device time of the element loops */
+  double t_solver_ms;    /* device time of the Krylov solves */
+} sfem_step_stats;
+
+/* Library version string. */
+const char* sfem_version(void);
+/* Message of the most recent error on this thread ("" if none). */
+const char* sfem_last_error(void);
+
+/* Create a context from a mesh.  Replaces StabFEM::readInputData's in-memory
+ * result + prepareInputData (StabFEM.cpp:80-264, 328-386): coords is
+ * nNode*ndim, conn nElem*npElem (0-based).  Supported (ndim, npElem):
+ * (2,3) Tria3, (2,4) Quad4, (3,4) Tetra4, (3,8) Hexa8.  device < 0 picks
+ * the current device.  Returns NULL on failure (see sfem_last_error). */
+sfem_ctx* sfem_create(int ndim, int npElem, int64_t nNode, int64_t nElem, const double* coords,
+                      const int32_t* conn, int device);
+void sfem_destroy(sfem_ctx* ctx);
+
+/* Dirichlet list in file order (StabFEM.cpp:209-225 DirichletBCs): node,
+ * dof, value; duplicates are honoured as the reference does (the value of
+ * the last listing, the iteration-0 update applied once per listing). */
+int sfem_set_dirichlet(sfem_ctx* ctx, int64_t nDBC, const int32_t* node, const int32_t* dof,
+                       const double* value);
+
+/* prepareElemData for all elements (Hexa8Node.cpp:31-106 and siblings):
+ * volumes, characteristic lengths, negative-Jacobian check on the device. */
+int sfem_prepare_elements(sfem_ctx* ctx);
+
+/* DOF numbering + CSR pattern + element-to-nonzero map + element colouring
+ * (StabFEM2.cpp:34-319 prepareMatrixPattern, SolverPetsc::initialise
+ * SolverPetsc.cpp:30-116).  Returns nfree and nnz. */
+int sfem_build_pattern(sfem_ctx* ctx, int64_t* nfree, int64_t* nnz);
+
+/* Copy out the pattern for bit-exact checks: rowptr[nfree+1], colidx[nnz],
+ * assyForSoln[nfree] (StabFEM2.cpp:172-183).  Any pointer may be NULL. */
+int sfem_get_pattern(sfem_ctx* ctx, int64_t* rowptr, int32_t* colidx, int32_t* assyForSoln);
+
+/* Material and time-integration parameters (StabFEM::readControlParameters,
+ * StabFEM.cpp:268-322): elemData[11] (0 rho, 1 mu, 8-10 SUPG/PSPG/LSIC
+ * multipliers), tis / rhoInf / dt for SetTimeParametersFluid
+ * (UtilitiesGeneral.cpp:5-209). */
+int sfem_set_params(sfem_ctx* ctx, const double* elemData, int tis, double rhoInf, double dt);
+
+/* SFEM_PREC_FP64 (default) or SFEM_PREC_MIXED. */
+int sfem_set_precision(sfem_ctx* ctx, int precision);
+
+/* Start a time step (StabFEM2.cpp:632-648): td[] from SetTimeParametersFluid,
+ * SolutionData::timeUpdate (SolutionData.cpp:64-78), and
+ * assignBoundaryConditions with the given ramp factor (StabFEM.cpp:391-410). */
+int sfem_time_update(sfem_ctx* ctx, double timeFact);
+
+/* One element loop of the Newton iteration `iter` (StabFEM2.cpp:656-740):
+ * updateIterStep, zeroMtx, calcStiffnessAndResidual for every element, the
+ * iteration-0 BC lift, scatter-add into the device CSR and RHS
+ * (assembleMatrixAndVectorSerial, SolverPetsc.cpp:304-314), the iteration-0
+ * soln[dbc] += solnApplied, and ||rhs_free||_2 into *rhs_norm.
+ * want_K = 0 assembles the residual only (valid for iter > 0). */
+int sfem_assemble(sfem_ctx* ctx, int iter, int want_K, double* rhs_norm);
+
+/* Element parity probe: Ke (row-major nsize x nsize) and Fe of element e at
+ * the current state, exactly as the assembly kernel computes them before the
+ * BC lift (calcStiffnessAndResidual, ElementBase.h:87-88). */
+int sfem_element_KF(sfem_ctx* ctx, int64_t e, double* Ke, double* Fe);
+
+/* Copy out the assembled CSR values[nnz] and rhs[nfree] (FP64 mode; in mixed
+ * mode the FP32 values widened to double).  Either pointer may be NULL. */
+int sfem_get_system(sfem_ctx* ctx, double* vals, double* rhs);
+
+/* Krylov solve of the assembled system into the device increment
+ * (SolverPetsc::factoriseAndSolve / solve, SolverPetsc.cpp:224-298; zero
+ * initial guess as VecZeroEntries at :249). */
+int sfem_solve(sfem_ctx* ctx, const sfem_solver_opts* opts, int* iters, double* relres);
+
+/* soln[assyForSoln[i]] += du[i] (StabFEM2.cpp:785-788). */
+int sfem_update_solution(sfem_ctx* ctx);
+
+/* One full time step: sfem_time_update + the Newton loop of
+ * StabFEM2.cpp:654-799 (at most max_iter iterations, stop when
+ * ||rhs_free||_2 < newton_tol before solving).  Returns SFEM_ERR_NOTCONV if
+ * the cap is hit (stats are still filled). */
+int sfem_step(sfem_ctx* ctx, double timeFact, const sfem_solver_opts* opts, double newton_tol,
+              int max_iter, sfem_step_stats* stats);
+
+/* Solution vectors (node-major, nNode*ndof): soln, and the solver increment du[nfree]. */
+int sfem_get_solution(sfem_ctx* ctx, double* soln);
+int sfem_set_solution(sfem_ctx* ctx, const double* soln);
+int sfem_get_increment(sfem_ctx* ctx, double* du);
+
+/* Set the full SolutionData state (each nNode*ndof; NULL leaves a vector
+ * unchanged) -- used to probe the element kernels at arbitrary states. */
+int sfem_set_state(sfem_ctx* ctx, const double* soln, const double* solnPrev,
+                   const double* solnCur, const double* solnDotCur, const double* solnDot,
+                   const double* solnDotPrev);
+
+/* Zero every SolutionData vector (soln, solnPrev..solnPrev4, solnDot, solnDotPrev,
+ * solnCur, solnDotCur, solnApplied) -- SolutionData::initialise (SolutionData.cpp:16-38). */
+int sfem_reset_state(sfem_ctx* ctx);
+
+/* Per-kernel-class device timing with HIP events recorded on the context's
+ * stream around each launch group of that class.  Enabling (or re-enabling)
+ * clears the counters.  No counterpart in the reference (its timers are the
+ * MPI_Wtime phase totals of StabFEM.cpp:711-727). */
+enum { SFEM_KT_ASSEMBLY = 0, SFEM_KT_SPMV = 1, SFEM_KT_PC = 2, SFEM_KT_NKINDS = 3 };
+int sfem_enable_kernel_timing(sfem_ctx* ctx, int enable);
+int sfem_kernel_timing(sfem_ctx* ctx, int kind, int64_t* launches, double* total_ms);
+
+/* Sizes and counts of the context. */
+int sfem_info(sfem_ctx* ctx, int64_t* nfree, int64_t* nnz, int* ncolors, int* ndof);
+
+/* Device time (ms) of the last assembly's element kernels and of the last
+ * sfem_solve, measured with HIP events on the context's stream. */
+int sfem_last_timings(sfem_ctx* ctx, double* assembly_ms, double* solve_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,251 @@
+"""GPU parity of the HIP path against the oracle, called through the C ABI.
+
+Bars (written in each test): CSR pattern and assyForSoln bit-exact; element
+Ke/Fe and assembled CSR values / RHS within 1e-12 relative (max-norm of the
+difference over the max-norm of the oracle's result); converged fields within
+1e-8 relative to the oracle at the same Newton tolerance; the reference's own
+rule (1e-4 per node) on its two 2D fixtures.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from cfdsfemmpi_amd import deck, meshgen
+from cfdsfemmpi_amd import sfem
+
+pytestmark = pytest.mark.gpu
+
+ETYPES = {"tria3": oracle.TRIA3, "quad4": oracle.QUAD4, "tetra4": oracle.TETRA4, "hexa8": oracle.HEXA8}
+ELEM_DATA = np.array([1.0, 0.01, 0, 0, 0, 0, 0, 0, 1.0, 1.0, 0.0])
+
+
+def rel_err(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    d = np.max(np.abs(a - b)) if a.size else 0.0
+    s = np.max(np.abs(b)) if b.size else 0.0
+    return d / s if s > 0 else d
+
+
+def small_mesh(name, graded=True):
+    if name == "hexa8":
+        return meshgen.ldc3d_hex(3, graded=graded)
+    if name == "tetra4":
+        return meshgen.ldc3d_tet(2, graded=graded)
+    if name == "quad4":
+        return meshgen.ldc2d_quad(4, graded=graded)
+    return meshgen.ldc2d_tria(4, jitter=0.2, seed=3)
+
+
+def make_ctx(mesh, tis=2, dt=0.1, elemData=ELEM_DATA):
+    ctx = sfem.Context(mesh)
+    ctx.set_dirichlet(mesh.dbc_node, mesh.dbc_dof, mesh.dbc_value)
+    ctx.prepare_elements()
+    ctx.build_pattern()
+    ctx.set_params(elemData, tis, 0.0, dt)
+    return ctx
+
+
+@pytest.mark.parametrize("name", list(ETYPES))
+@pytest.mark.parametrize("tis", [0, 2])
+def test_element_KF_parity(name, tis):
+    mesh = small_mesh(name)
+    print(f"[case] element_KF {name} tis={tis} nElem={mesh.nElem}", flush=True)
+    et = ETYPES[name]
+    ctx = make_ctx(mesh, tis=tis)
+    n = mesh.nNode * (mesh.ndim + 1)
+    rng = np.random.default_rng(11)
+    prev, cur, dcur = (rng.uniform(-1, 1, n) for _ in range(3))
+    ctx.set_state(solnPrev=prev, solnCur=cur, solnDotCur=dcur)
+    prep = oracle.prepare(et, mesh.coords, mesh.conn)
+    td = oracle.time_params(tis, 0.0, 0.1)
+    for e in range(mesh.nElem):
+        Kg, Fg = ctx.element_KF(e)
+        Ko, Fo = oracle.element_KF(et, e, mesh.conn, prep, ELEM_DATA, td, prev, cur, dcur)
+        assert rel_err(Kg, Ko) <= 1e-12, (name, e, rel_err(Kg, Ko))
+        assert rel_err(Fg, Fo) <= 1e-12, (name, e, rel_err(Fg, Fo))
+    ctx.close()
+
+
+@pytest.mark.parametrize("name", list(ETYPES))
+def test_pattern_bit_exact(name):
+    mesh = small_mesh(name)
+    print(f"[case] pattern {name}", flush=True)
+    ctx = make_ctx(mesh)
+    rp, ci, af = ctx.get_pattern()
+    nodeDof, assy, nfree = oracle.dof_numbering(mesh.nNode, mesh.ndim + 1, mesh.dbc_node, mesh.dbc_dof)
+    orp, oci = oracle.pattern(mesh.conn, mesh.ndim + 1, nodeDof, nfree)
+    assert ctx.nfree == nfree
+    assert np.array_equal(rp, orp)
+    assert np.array_equal(ci, oci)
+    assert np.array_equal(af, assy)
+    ctx.close()
+
+
+def test_pattern_reference_deck(golden):
+    mesh = deck.read_mesh(os.path.join(golden, "ldc2dquad10", "LDC2Dquad10.dat"))
+    ctx = make_ctx(mesh)
+    rp, ci, af = ctx.get_pattern()
+    nodeDof, assy, nfree = oracle.dof_numbering(mesh.nNode, 3, mesh.dbc_node, mesh.dbc_dof)
+    orp, oci = oracle.pattern(mesh.conn, 3, nodeDof, nfree)
+    assert (ctx.nfree, ctx.nnz) == (282, 6354)
+    assert np.array_equal(rp, orp) and np.array_equal(ci, oci) and np.array_equal(af, assy)
+    ctx.close()
+
+
+def _oracle_fem(mesh, tis, dt):
+    ctl = deck.Control(ELEM_DATA.copy(), tis, 0.0, dt, 1e9, 100, 1, 1e-6)
+    f = oracle.OracleFEM(mesh, ctl)
+    f.prepare()
+    return f
+
+
+@pytest.mark.parametrize("name", list(ETYPES))
+def test_assembled_system_parity(name):
+    """iteration-0 assembly (updateIterStep, element loop, BC lift, scatter, BC add)."""
+    mesh = small_mesh(name)
+    print(f"[case] assembly {name}", flush=True)
+    tis, dt, tf = 2, 0.1, 0.3
+    n = mesh.nNode * (mesh.ndim + 1)
+    rng = np.random.default_rng(5)
+    S0, P0, D0 = (rng.uniform(-1, 1, n) for _ in range(3))
+    f = _oracle_fem(mesh, tis, dt)
+    f.soln[:] = S0; f.solnPrev[:] = P0; f.solnDot[:] = D0
+    f.td = oracle.time_params(tis, 0.0, dt)
+    f.time_update()
+    f.assign_boundary_conditions(tf)
+    f.update_iter_step()
+    f.assemble(0)
+    for k in range(mesh.nDBC):
+        j = int(mesh.dbc_node[k]) * (mesh.ndim + 1) + int(mesh.dbc_dof[k])
+        f.soln[j] += f.solnApplied[j]
+    ctx = make_ctx(mesh, tis=tis, dt=dt)
+    ctx.set_state(soln=S0, solnPrev=P0, solnDot=D0)
+    ctx.time_update(tf)
+    nrm = ctx.assemble(0)
+    vals, rhs = ctx.get_system()
+    assert rel_err(vals, f.vals) <= 1e-12
+    assert rel_err(rhs, f.rhs) <= 1e-12
+    assert abs(nrm - np.linalg.norm(f.rhs)) <= 1e-12 * np.linalg.norm(f.rhs)
+    assert rel_err(ctx.get_solution(), f.soln) <= 1e-15
+    # residual-only pass at iteration 1 equals the full pass' residual
+    nrm1 = ctx.assemble(1, want_K=False)
+    _, rhs1 = ctx.get_system()
+    f.update_iter_step()
+    f.assemble(1)
+    assert rel_err(rhs1, f.rhs) <= 1e-12
+    ctx.close()
+
+
+def test_ldc2dquad10_reference(golden):
+    """src/test-ldc-stru-nelem10.cpp: run the deck, check at 1e-4 per node (StabFEM.cpp:522-539)."""
+    d = os.path.join(golden, "ldc2dquad10")
+    fem = sfem.StabFEM()
+    fem.readInputData(os.path.join(d, "LDC2Dquad10"))
+    fem.readControlParameters(os.path.join(d, "control-parameters-ldc-Re100.dat"))
+    fem.prepareInputData()
+    fem.setSolver(os.path.join(d, "petsc_options.dat"))
+    s = fem.solveFullyImplicit()
+    assert fem.checkResult(os.path.join(d, "test-ldc-stru-nelem10-ref.dat")) == 0
+    ref = deck.read_result(os.path.join(d, "test-ldc-stru-nelem10-ref.dat"))
+    assert np.max(np.linalg.norm(ref - s.reshape(-1, 3), axis=1)) < 2e-8
+    # against the oracle (direct solve) at the same Newton tolerance
+    o = oracle.OracleFEM(fem.mesh, fem.ctl)
+    o.prepare()
+    so = o.solve_fully_implicit()
+    assert rel_err(s, so) <= 1e-8
+
+
+def test_ldc2dquad_mesh2_reference(golden):
+    """src/test-ldc-stru-mesh2.cpp (16,384 Quad4, Re=1000)."""
+    d = os.path.join(golden, "ldc2dquad-mesh2")
+    fem = sfem.StabFEM()
+    fem.readInputData(os.path.join(d, "LDC2Dquad-mesh2.dat.gz"))
+    fem.readControlParameters(os.path.join(d, "control-parameters-Re1000.dat"))
+    fem.prepareInputData()
+    fem.setSolver(os.path.join(d, "petsc_options.dat"))
+    s = fem.solveFullyImplicit()
+    print("[mesh2] newton/krylov history", fem.history, flush=True)
+    ref = deck.read_result(os.path.join(d, "test-ldc-stru-mesh2-ref.dat.gz"))
+    err = np.max(np.linalg.norm(ref - s.reshape(-1, 3), axis=1))
+    print(f"[mesh2] max node error vs reference {err:.3e}", flush=True)
+    assert fem.checkResult(os.path.join(d, "test-ldc-stru-mesh2-ref.dat.gz")) == 0
+    assert err < 2e-8
+
+
+def _cavity3d(n, steps, prec=sfem.PREC_FP64, opts=None, newton_tol=1e-10):
+    mesh = meshgen.ldc3d_hex(n, graded=True)
+    ctl = deck.Control(np.array([1.0, 1e-3, 0, 0, 0, 0, 0, 0, 1, 1, 0]), 2, 0.0, 0.1, 1e9, steps, 1, 1e-6)
+    fem = sfem.StabFEM()
+    fem.setMesh(mesh)
+    fem.setControl(ctl)
+    fem.precision = prec
+    fem.newton_tol = newton_tol
+    fem.prepareInputData()
+    fem.setSolver(opts or sfem.solver_opts("bcgs", "ilu", 1e-12, 2000))
+    return fem, mesh, ctl
+
+
+def test_cavity3d_vs_oracle():
+    """Graded 6^3 Hexa8 cavity, generalised-alpha, 3 steps: fields within 1e-8 of the oracle."""
+    fem, mesh, ctl = _cavity3d(6, 3)
+    s = fem.solveFullyImplicit()
+    o = oracle.OracleFEM(mesh, ctl)
+    o.prepare()
+    so = o.solve_fully_implicit(newton_tol=1e-10)
+    assert rel_err(s, so) <= 1e-8
+
+
+@pytest.mark.parametrize("ksp,pc", [("bcgs", "ilu"), ("bcgs", "bjacobi4"), ("bcgs", "jacobi"),
+                                    ("gmres", "ilu"), ("gmres", "bjacobi4")])
+def test_krylov_against_direct(ksp, pc):
+    """Krylov answer vs a direct solve of the same assembled system."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    mesh = meshgen.ldc3d_hex(5, graded=True)
+    ctx = make_ctx(mesh, tis=2, dt=0.1, elemData=np.array([1.0, 1e-2, 0, 0, 0, 0, 0, 0, 1, 1, 0]))
+    ctx.time_update(1.0)
+    ctx.assemble(0)
+    vals, rhs = ctx.get_system()
+    rp, ci, _ = ctx.get_pattern()
+    A = sp.csr_matrix((vals, ci, rp), shape=(ctx.nfree, ctx.nfree))
+    x_ref = spla.spsolve(A.tocsc(), rhs)
+    its, rr = ctx.solve(sfem.solver_opts(ksp, pc, 1e-10, 5000, 40))
+    du = ctx.get_increment()
+    print(f"[krylov] {ksp}+{pc}: its={its} relres={rr:.2e}", flush=True)
+    true_res = np.linalg.norm(rhs - A @ du) / np.linalg.norm(rhs)
+    assert true_res <= 1.01e-10
+    assert np.linalg.norm(du - x_ref) <= 1e-6 * np.linalg.norm(x_ref)
+    ctx.close()
+
+
+def test_mixed_precision_close_to_fp64():
+    """FP32 tangent + FP32 Krylov inside an FP64 residual loop stays within 1e-6 of FP64."""
+    f64, mesh, ctl = _cavity3d(6, 3, newton_tol=1e-10)
+    s64 = f64.solveFullyImplicit()
+    fmx, _, _ = _cavity3d(6, 3, prec=sfem.PREC_MIXED, opts=sfem.solver_opts("bcgs", "bjacobi4", 1e-4, 2000),
+                          newton_tol=1e-10)
+    smx = fmx.solveFullyImplicit()
+    nd = 4
+    u64 = s64.reshape(-1, nd)[:, :3]
+    umx = smx.reshape(-1, nd)[:, :3]
+    p64 = s64.reshape(-1, nd)[:, 3]
+    pmx = smx.reshape(-1, nd)[:, 3]
+    eu = np.linalg.norm(umx - u64) / np.linalg.norm(u64)
+    ep = np.linalg.norm(pmx - p64) / np.linalg.norm(p64)
+    print(f"[mixed] rel L2 u {eu:.2e} p {ep:.2e}; history {fmx.history}", flush=True)
+    assert eu <= 1e-6 and ep <= 1e-6
+
+
+def test_negative_jacobian_reported():
+    mesh = meshgen.ldc2d_quad(3)
+    mesh.conn[4] = mesh.conn[4][[0, 3, 2, 1]]  # clockwise element
+    ctx = sfem.Context(mesh)
+    ctx.set_dirichlet(mesh.dbc_node, mesh.dbc_dof, mesh.dbc_value)
+    with pytest.raises(sfem.SfemError) as ei:
+        ctx.prepare_elements()
+    assert ei.value.code == -3 and "element 4" in str(ei.value)
+    ctx.close()
