@@ -199,6 +199,22 @@ __global__ __launch_bounds__(kTPB) void k_bicg_xr(int64_t n, T* __restrict__ x, 
   block_sum_store<1>(acc, part);
 }
 
+// dst[i] = src[map[i]]
+template <typename T>
+__global__ __launch_bounds__(kTPB) void k_gather(int64_t n, const int32_t* __restrict__ map,
+                                                 const T* __restrict__ src, T* __restrict__ dst) {
+  for (int64_t i = (int64_t)blockIdx.x * kTPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kTPB)
+    dst[i] = src[map[i]];
+}
+
+// dst[map[i]] = src[i]
+template <typename T>
+__global__ __launch_bounds__(kTPB) void k_scatter(int64_t n, const int32_t* __restrict__ map,
+                                                  const T* __restrict__ src, T* __restrict__ dst) {
+  for (int64_t i = (int64_t)blockIdx.x * kTPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kTPB)
+    dst[map[i]] = src[i];
+}
+
 // y = a*x + b*y with host scalars (GMRES)
 template <typename T>
 __global__ __launch_bounds__(kTPB) void k_axpby(int64_t n, double a, const T* __restrict__ x, double b,
@@ -478,9 +494,10 @@ static hipError_t alloc(T** p, size_t n) {
 }
 
 LinAlg::~LinAlg() {
-  void* ptrs[] = {d_blkStart, d_diagpos, d_part, d_scal, r, rh, p, v, s, t, ph, sh, x, w, z,
-                  r32, rh32, p32, v32, s32, t32, ph32, sh32, x32, dinv, dinv32, binv, binv32,
-                  lu, lu32, d_levL, d_levU, gm_V, gm_V32};
+  void* ptrs[] = {d_blkStart, d_diagpos, d_part, d_scal, r, rh, p, v, s, t, ph, sh, x, w, z, pr, pz,
+                  r32, rh32, p32, v32, s32, t32, ph32, sh32, x32, du32, w32, z32, pr32, pz32,
+                  dinv, dinv32, binv, binv32, lu, lu32, gm_V, gm_V32};
+  delete ilu;
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (h_scal) (void)hipHostFree(h_scal);
@@ -535,43 +552,215 @@ void la_destroy(sfem_ctx* ctx) {
   ctx->la = nullptr;
 }
 
-// --- level sets over nodal blocks (host) ---
-static void build_levels(sfem_ctx* ctx, LinAlg* L) {
-  const int64_t nb = L->nblk;
-  std::vector<int32_t> rowBlk(L->n);
+// --- ILU(0) plan: node ordering, (permuted) pattern, level sets (host) ---
+static void bucket_levels(const std::vector<int32_t>& lv, std::vector<int32_t>& ptr, std::vector<int32_t>& list) {
+  int nl = 0;
+  for (int32_t x : lv) nl = std::max(nl, x + 1);
+  ptr.assign(nl + 1, 0);
+  for (int32_t x : lv) ptr[x + 1]++;
+  for (int i = 0; i < nl; ++i) ptr[i + 1] += ptr[i];
+  list.resize(lv.size());
+  std::vector<int32_t> pos(ptr.begin(), ptr.end() - 1);
+  for (int64_t k = 0; k < (int64_t)lv.size(); ++k) list[pos[lv[k]]++] = (int32_t)k;
+}
+
+// Level of a block = 1 + max level of the blocks it depends on (lower blocks for the
+// forward solve and the factorisation, upper blocks for the backward solve).
+static void build_levels(const std::vector<int64_t>& rowptr, const std::vector<int32_t>& colidx,
+                         const std::vector<int32_t>& bs, IluPlan& P) {
+  const int64_t nb = (int64_t)bs.size() - 1;
+  std::vector<int32_t> rowBlk(bs.back());
   for (int64_t k = 0; k < nb; ++k)
-    for (int32_t r = L->h_blkStart[k]; r < L->h_blkStart[k + 1]; ++r) rowBlk[r] = (int32_t)k;
+    for (int32_t r = bs[k]; r < bs[k + 1]; ++r) rowBlk[r] = (int32_t)k;
   std::vector<int32_t> lvL(nb, 0), lvU(nb, 0);
   for (int64_t k = 0; k < nb; ++k) {
-    const int64_t r = L->h_blkStart[k];
     int lv = 0;
-    for (int64_t q = ctx->rowptr[r]; q < ctx->rowptr[r + 1]; ++q) {
-      const int32_t bk = rowBlk[ctx->colidx[q]];
+    for (int64_t q = rowptr[bs[k]]; q < rowptr[bs[k] + 1]; ++q) {
+      const int32_t bk = rowBlk[colidx[q]];
       if (bk < k) lv = std::max(lv, lvL[bk] + 1);
     }
     lvL[k] = lv;
   }
   for (int64_t k = nb - 1; k >= 0; --k) {
-    const int64_t r = L->h_blkStart[k];
     int lv = 0;
-    for (int64_t q = ctx->rowptr[r]; q < ctx->rowptr[r + 1]; ++q) {
-      const int32_t bk = rowBlk[ctx->colidx[q]];
+    for (int64_t q = rowptr[bs[k]]; q < rowptr[bs[k] + 1]; ++q) {
+      const int32_t bk = rowBlk[colidx[q]];
       if (bk > k) lv = std::max(lv, lvU[bk] + 1);
     }
     lvU[k] = lv;
   }
-  auto bucket = [&](const std::vector<int32_t>& lv, std::vector<int32_t>& ptr, std::vector<int32_t>& list) {
-    int nl = 0;
-    for (int32_t x : lv) nl = std::max(nl, x + 1);
-    ptr.assign(nl + 1, 0);
-    for (int32_t x : lv) ptr[x + 1]++;
-    for (int i = 0; i < nl; ++i) ptr[i + 1] += ptr[i];
-    list.resize(lv.size());
-    std::vector<int32_t> pos(ptr.begin(), ptr.end() - 1);
-    for (int64_t k = 0; k < (int64_t)lv.size(); ++k) list[pos[lv[k]]++] = (int32_t)k;
+  bucket_levels(lvL, P.levLPtr, P.h_levL);
+  bucket_levels(lvU, P.levUPtr, P.h_levU);
+}
+
+// Reverse Cuthill-McKee over the block (node) graph, starting each component from a
+// pseudo-peripheral node (George & Liu, Computer Solution of Large Sparse Positive
+// Definite Systems, 1981, sec. 4.3).  Returns the old block ids in the new order.
+static std::vector<int32_t> rcm_order(const std::vector<int64_t>& adjPtr, const std::vector<int32_t>& adj) {
+  const int64_t nb = (int64_t)adjPtr.size() - 1;
+  std::vector<int32_t> deg(nb), order;
+  order.reserve(nb);
+  for (int64_t k = 0; k < nb; ++k) deg[k] = (int32_t)(adjPtr[k + 1] - adjPtr[k]);
+  std::vector<char> seen(nb, 0);
+  std::vector<int32_t> mark(nb, -1), lvl(nb, 0), queue;
+  queue.reserve(nb);
+  int stamp = 0;
+  auto bfs_last_level = [&](int32_t s, int& ecc, std::vector<int32_t>& last) {
+    ++stamp;
+    queue.clear();
+    queue.push_back(s);
+    mark[s] = stamp;
+    lvl[s] = 0;
+    ecc = 0;
+    for (size_t h = 0; h < queue.size(); ++h) {
+      const int32_t u = queue[h];
+      for (int64_t q = adjPtr[u]; q < adjPtr[u + 1]; ++q) {
+        const int32_t w = adj[q];
+        if (mark[w] == stamp || seen[w]) continue;
+        mark[w] = stamp;
+        lvl[w] = lvl[u] + 1;
+        ecc = std::max(ecc, lvl[w]);
+        queue.push_back(w);
+      }
+    }
+    last.clear();
+    for (int32_t u : queue)
+      if (lvl[u] == ecc) last.push_back(u);
   };
-  bucket(lvL, L->levLPtr, L->h_levL);
-  bucket(lvU, L->levUPtr, L->h_levU);
+  std::vector<int32_t> last, nbrs;
+  for (int64_t s0 = 0; s0 < nb; ++s0) {
+    if (seen[s0]) continue;
+    // minimum-degree node of this component as the first candidate
+    int ecc = 0;
+    bfs_last_level((int32_t)s0, ecc, last);
+    int32_t s = (int32_t)s0;
+    for (int32_t u : queue)
+      if (deg[u] < deg[s]) s = u;
+    bfs_last_level(s, ecc, last);
+    for (int it = 0; it < 8; ++it) {
+      int32_t t = last[0];
+      for (int32_t u : last)
+        if (deg[u] < deg[t]) t = u;
+      int ecc2 = 0;
+      std::vector<int32_t> last2;
+      bfs_last_level(t, ecc2, last2);
+      if (ecc2 <= ecc) break;
+      s = t;
+      ecc = ecc2;
+      last.swap(last2);
+    }
+    // Cuthill-McKee from s: neighbours in increasing degree
+    const size_t first = order.size();
+    order.push_back(s);
+    seen[s] = 1;
+    for (size_t h = first; h < order.size(); ++h) {
+      const int32_t u = order[h];
+      nbrs.clear();
+      for (int64_t q = adjPtr[u]; q < adjPtr[u + 1]; ++q)
+        if (!seen[adj[q]]) nbrs.push_back(adj[q]);
+      std::sort(nbrs.begin(), nbrs.end(),
+                [&](int32_t a, int32_t b) { return deg[a] != deg[b] ? deg[a] < deg[b] : a < b; });
+      for (int32_t w : nbrs) {
+        seen[w] = 1;
+        order.push_back(w);
+      }
+    }
+  }
+  std::reverse(order.begin(), order.end());
+  return order;
+}
+
+template <typename T>
+static int upload(T** d, const std::vector<T>& h) {
+  hipError_t e = alloc(d, h.size());
+  if (e == hipSuccess && !h.empty()) e = hipMemcpy(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
+  return e == hipSuccess ? SFEM_OK : set_hip_error(e, "upload");
+}
+
+IluPlan::~IluPlan() {
+  void* ptrs[] = {d_levL, d_levU, d_rowptr, d_colidx, d_diagpos, d_blkStart, d_pmap, d_perm};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+}
+
+static int build_ilu_plan(sfem_ctx* ctx, LinAlg* L, int reorder) {
+  delete L->ilu;
+  L->ilu = new IluPlan;
+  IluPlan& P = *L->ilu;
+  P.reorder = reorder;
+  P.identity = (reorder == 0);
+  const std::vector<int32_t>& bs = L->h_blkStart;
+  const int64_t nb = L->nblk;
+  if (P.identity) {
+    build_levels(ctx->rowptr, ctx->colidx, bs, P);
+    P.rowptr = ctx->d_rowptr32;
+    P.colidx = ctx->d_colidx;
+    P.diagpos = L->d_diagpos;
+    P.blkStart = L->d_blkStart;
+  } else {
+    // block graph from the first row of each block
+    std::vector<int32_t> rowBlk(L->n);
+    for (int64_t k = 0; k < nb; ++k)
+      for (int32_t r = bs[k]; r < bs[k + 1]; ++r) rowBlk[r] = (int32_t)k;
+    std::vector<int64_t> adjPtr(nb + 1, 0);
+    std::vector<int32_t> adj;
+    for (int64_t k = 0; k < nb; ++k) {
+      int32_t prev = -1;
+      for (int64_t q = ctx->rowptr[bs[k]]; q < ctx->rowptr[bs[k] + 1]; ++q) {
+        const int32_t b = rowBlk[ctx->colidx[q]];
+        if (b != prev && b != k) adj.push_back(b);
+        prev = b;
+      }
+      adjPtr[k + 1] = (int64_t)adj.size();
+    }
+    const std::vector<int32_t> order = rcm_order(adjPtr, adj);
+    std::vector<int32_t> nbs(nb + 1, 0), perm(L->n), iperm(L->n);
+    for (int64_t p = 0; p < nb; ++p) {
+      const int32_t k = order[p];
+      nbs[p + 1] = nbs[p] + (bs[k + 1] - bs[k]);
+      for (int32_t a = 0; a < bs[k + 1] - bs[k]; ++a) perm[nbs[p] + a] = bs[k] + a;
+    }
+    for (int64_t i = 0; i < L->n; ++i) iperm[perm[i]] = (int32_t)i;
+    // permuted rows: every row of a block shares its column list, so sort once per block
+    std::vector<int64_t> rp(L->n + 1, 0);
+    for (int64_t p = 0; p < nb; ++p) {
+      const int32_t k = order[p];
+      const int64_t len = ctx->rowptr[bs[k] + 1] - ctx->rowptr[bs[k]];
+      for (int32_t r = nbs[p]; r < nbs[p + 1]; ++r) rp[r + 1] = rp[r] + len;
+    }
+    std::vector<int32_t> ci(ctx->nnz), pmap(ctx->nnz), dp(L->n, -1);
+    std::vector<std::pair<int32_t, int32_t>> tmp;
+    for (int64_t p = 0; p < nb; ++p) {
+      const int32_t k = order[p];
+      const int64_t b0 = ctx->rowptr[bs[k]];
+      const int64_t len = ctx->rowptr[bs[k] + 1] - b0;
+      tmp.resize(len);
+      for (int64_t q = 0; q < len; ++q) tmp[q] = {iperm[ctx->colidx[b0 + q]], (int32_t)q};
+      std::sort(tmp.begin(), tmp.end());
+      for (int32_t a = 0; a < bs[k + 1] - bs[k]; ++a) {
+        const int32_t rn = nbs[p] + a;
+        const int64_t ob = ctx->rowptr[bs[k] + a];
+        for (int64_t q = 0; q < len; ++q) {
+          ci[rp[rn] + q] = tmp[q].first;
+          pmap[rp[rn] + q] = (int32_t)(ob + tmp[q].second);
+          if (tmp[q].first == rn) dp[rn] = (int32_t)(rp[rn] + q);
+        }
+      }
+    }
+    build_levels(rp, ci, nbs, P);
+    std::vector<int32_t> rp32(rp.begin(), rp.end());
+    int rc;
+    if ((rc = upload(&P.d_rowptr, rp32)) || (rc = upload(&P.d_colidx, ci)) || (rc = upload(&P.d_diagpos, dp)) ||
+        (rc = upload(&P.d_blkStart, nbs)) || (rc = upload(&P.d_pmap, pmap)) || (rc = upload(&P.d_perm, perm)))
+      return rc;
+    P.rowptr = P.d_rowptr;
+    P.colidx = P.d_colidx;
+    P.diagpos = P.d_diagpos;
+    P.blkStart = P.d_blkStart;
+  }
+  int rc;
+  if ((rc = upload(&P.d_levL, P.h_levL)) || (rc = upload(&P.d_levU, P.h_levU))) return rc;
+  return SFEM_OK;
 }
 
 template <typename T>
@@ -583,7 +772,7 @@ static int ensure_vec(T** p, int64_t n) {
 }
 
 // Build the preconditioner from the current matrix (FP64 vals or FP32 vals32).
-int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32) {
+int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder) {
   LinAlg* L = ctx->la;
   hipStream_t st = ctx->stream;
   const int64_t n = L->n;
@@ -622,30 +811,35 @@ int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32) {
     if (L->maxrow > kIluMaxRow) return set_error(SFEM_ERR_UNSUPPORTED, "ILU(0): row longer than 256 entries");
     for (int64_t r = 0; r < n; ++r)
       if (L->h_diagpos[r] < 0) return set_error(SFEM_ERR_ARG, "ILU(0): missing diagonal entry");
-    if (L->h_levL.empty() && L->nblk > 0) {
-      build_levels(ctx, L);
-      LA_CHECK(alloc(&L->d_levL, L->h_levL.size()));
-      LA_CHECK(hipMemcpy(L->d_levL, L->h_levL.data(), L->h_levL.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-      LA_CHECK(alloc(&L->d_levU, L->h_levU.size()));
-      LA_CHECK(hipMemcpy(L->d_levU, L->h_levU.data(), L->h_levU.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (!L->ilu || L->ilu->reorder != reorder) {
+      int rc = build_ilu_plan(ctx, L, reorder);
+      if (rc) return rc;
     }
+    const IluPlan& P = *L->ilu;
     const int64_t nnz = ctx->nnz;
+    const unsigned gz = (unsigned)std::min<int64_t>((nnz + kTPB - 1) / kTPB, 1 << 20);
     if (fp32) {
       int rc = ensure_vec(&L->lu32, nnz); if (rc) return rc;
-      LA_CHECK(hipMemcpyAsync(L->lu32, ctx->d_vals32, nnz * sizeof(float), hipMemcpyDeviceToDevice, st));
+      if (P.identity)
+        LA_CHECK(hipMemcpyAsync(L->lu32, ctx->d_vals32, nnz * sizeof(float), hipMemcpyDeviceToDevice, st));
+      else
+        hipLaunchKernelGGL((k_gather<float>), dim3(gz), dim3(kTPB), 0, st, nnz, P.d_pmap, ctx->d_vals32, L->lu32);
     } else {
       int rc = ensure_vec(&L->lu, nnz); if (rc) return rc;
-      LA_CHECK(hipMemcpyAsync(L->lu, ctx->d_vals, nnz * sizeof(double), hipMemcpyDeviceToDevice, st));
-    }
-    const int nl = (int)L->levLPtr.size() - 1;
-    for (int lv = 0; lv < nl; ++lv) {
-      const int cnt = L->levLPtr[lv + 1] - L->levLPtr[lv];
-      if (fp32)
-        hipLaunchKernelGGL((k_ilu_factor_level<float>), dim3(cnt), dim3(64), 0, st, L->d_levL + L->levLPtr[lv],
-                           cnt, L->d_blkStart, ctx->d_rowptr32, ctx->d_colidx, L->d_diagpos, L->lu32, L->d_scal);
+      if (P.identity)
+        LA_CHECK(hipMemcpyAsync(L->lu, ctx->d_vals, nnz * sizeof(double), hipMemcpyDeviceToDevice, st));
       else
-        hipLaunchKernelGGL((k_ilu_factor_level<double>), dim3(cnt), dim3(64), 0, st, L->d_levL + L->levLPtr[lv],
-                           cnt, L->d_blkStart, ctx->d_rowptr32, ctx->d_colidx, L->d_diagpos, L->lu, L->d_scal);
+        hipLaunchKernelGGL((k_gather<double>), dim3(gz), dim3(kTPB), 0, st, nnz, P.d_pmap, ctx->d_vals, L->lu);
+    }
+    const int nl = (int)P.levLPtr.size() - 1;
+    for (int lv = 0; lv < nl; ++lv) {
+      const int cnt = P.levLPtr[lv + 1] - P.levLPtr[lv];
+      if (fp32)
+        hipLaunchKernelGGL((k_ilu_factor_level<float>), dim3(cnt), dim3(64), 0, st, P.d_levL + P.levLPtr[lv],
+                           cnt, P.blkStart, P.rowptr, P.colidx, P.diagpos, L->lu32, L->d_scal);
+      else
+        hipLaunchKernelGGL((k_ilu_factor_level<double>), dim3(cnt), dim3(64), 0, st, P.d_levL + P.levLPtr[lv],
+                           cnt, P.blkStart, P.rowptr, P.colidx, P.diagpos, L->lu, L->d_scal);
     }
     LA_CHECK(hipGetLastError());
   }
@@ -661,6 +855,23 @@ static int pc_apply(sfem_ctx* ctx, const T* r, T* z) {
   const int rc = pc_apply_impl<T>(ctx, r, z);
   ctx->kt.end(SFEM_KT_PC, ctx->stream);
   return rc;
+}
+
+template <typename T, typename LT>
+static void ilu_solve(sfem_ctx* ctx, const IluPlan& P, const LT* lu, const T* r, T* z) {
+  hipStream_t st = ctx->stream;
+  const int nl = (int)P.levLPtr.size() - 1;
+  for (int lv = 0; lv < nl; ++lv) {
+    const int cnt = P.levLPtr[lv + 1] - P.levLPtr[lv];
+    hipLaunchKernelGGL((k_ilu_lsolve_level<T, LT>), dim3(cnt), dim3(64), 0, st, P.d_levL + P.levLPtr[lv], cnt,
+                       P.blkStart, P.rowptr, P.colidx, lu, r, z);
+  }
+  const int nu = (int)P.levUPtr.size() - 1;
+  for (int lv = 0; lv < nu; ++lv) {
+    const int cnt = P.levUPtr[lv + 1] - P.levUPtr[lv];
+    hipLaunchKernelGGL((k_ilu_usolve_level<T, LT>), dim3(cnt), dim3(64), 0, st, P.d_levU + P.levUPtr[lv], cnt,
+                       P.blkStart, P.rowptr, P.colidx, P.diagpos, lu, z);
+  }
 }
 
 template <typename T>
@@ -682,26 +893,26 @@ static int pc_apply_impl(sfem_ctx* ctx, const T* r, T* z) {
       break;
     }
     case SFEM_PC_ILU0: {
-      const int nl = (int)L->levLPtr.size() - 1;
-      for (int lv = 0; lv < nl; ++lv) {
-        const int cnt = L->levLPtr[lv + 1] - L->levLPtr[lv];
-        if (L->pc_fp32)
-          hipLaunchKernelGGL((k_ilu_lsolve_level<T, float>), dim3(cnt), dim3(64), 0, st, L->d_levL + L->levLPtr[lv], cnt,
-                             L->d_blkStart, ctx->d_rowptr32, ctx->d_colidx, L->lu32, r, z);
-        else
-          hipLaunchKernelGGL((k_ilu_lsolve_level<T, double>), dim3(cnt), dim3(64), 0, st, L->d_levL + L->levLPtr[lv], cnt,
-                             L->d_blkStart, ctx->d_rowptr32, ctx->d_colidx, L->lu, r, z);
+      const IluPlan& P = *L->ilu;
+      const T* rin = r;
+      T* zout = z;
+      T *pr = nullptr, *pz = nullptr;
+      if (!P.identity) {
+        int rc;
+        if (sizeof(T) == 8) {
+          if ((rc = ensure_vec(&L->pr, n)) || (rc = ensure_vec(&L->pz, n))) return rc;
+          pr = (T*)L->pr; pz = (T*)L->pz;
+        } else {
+          if ((rc = ensure_vec(&L->pr32, n)) || (rc = ensure_vec(&L->pz32, n))) return rc;
+          pr = (T*)L->pr32; pz = (T*)L->pz32;
+        }
+        hipLaunchKernelGGL((k_gather<T>), dim3(g), dim3(kTPB), 0, st, n, P.d_perm, r, pr);
+        rin = pr;
+        zout = pz;
       }
-      const int nu = (int)L->levUPtr.size() - 1;
-      for (int lv = 0; lv < nu; ++lv) {
-        const int cnt = L->levUPtr[lv + 1] - L->levUPtr[lv];
-        if (L->pc_fp32)
-          hipLaunchKernelGGL((k_ilu_usolve_level<T, float>), dim3(cnt), dim3(64), 0, st, L->d_levU + L->levUPtr[lv], cnt,
-                             L->d_blkStart, ctx->d_rowptr32, ctx->d_colidx, L->d_diagpos, L->lu32, z);
-        else
-          hipLaunchKernelGGL((k_ilu_usolve_level<T, double>), dim3(cnt), dim3(64), 0, st, L->d_levU + L->levUPtr[lv], cnt,
-                             L->d_blkStart, ctx->d_rowptr32, ctx->d_colidx, L->d_diagpos, L->lu, z);
-      }
+      if (L->pc_fp32) ilu_solve<T, float>(ctx, P, L->lu32, rin, zout);
+      else ilu_solve<T, double>(ctx, P, L->lu, rin, zout);
+      if (!P.identity) hipLaunchKernelGGL((k_scatter<T>), dim3(g), dim3(kTPB), 0, st, n, P.d_perm, pz, z);
       break;
     }
     default:
@@ -955,8 +1166,11 @@ int la_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int* iters, double* relre
   const bool fp32 = ctx->precision == SFEM_PREC_MIXED;
   int rc;
   // the tangent is constant within a time step: rebuild the preconditioner only when it changed
-  if (L->pc_K_version != ctx->la_K_version || L->pc_type != o->pc_type || L->pc_fp32 != fp32) {
-    if ((rc = la_setup_pc(ctx, o->pc_type, fp32))) return rc;
+  const int reorder = o->pc_type == SFEM_PC_ILU0 ? o->reorder : 0;
+  if (L->pc_K_version != ctx->la_K_version || L->pc_type != o->pc_type || L->pc_fp32 != fp32 ||
+      L->pc_reorder != reorder) {
+    if ((rc = la_setup_pc(ctx, o->pc_type, fp32, reorder))) return rc;
+    L->pc_reorder = reorder;
     L->pc_K_version = ctx->la_K_version;
   }
   const int m = std::max(1, o->restart);

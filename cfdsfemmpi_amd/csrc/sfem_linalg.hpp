@@ -15,6 +15,21 @@ namespace sfem {
 int set_error(int code, const char* msg);
 int set_hip_error(hipError_t e, const char* what);
 
+// ILU(0) plan for one node ordering: the (possibly permuted) pattern the factor lives
+// on and its level sets.  With the identity ordering the pattern pointers alias the
+// context's CSR.
+struct IluPlan {
+  int reorder = -1;
+  bool identity = true;
+  std::vector<int32_t> levLPtr, levUPtr, h_levL, h_levU;
+  int32_t *d_levL = nullptr, *d_levU = nullptr;
+  const int32_t *rowptr = nullptr, *colidx = nullptr, *diagpos = nullptr, *blkStart = nullptr;
+  int32_t *d_rowptr = nullptr, *d_colidx = nullptr, *d_diagpos = nullptr, *d_blkStart = nullptr;
+  int32_t* d_pmap = nullptr;   // permuted nonzero -> original nonzero
+  int32_t* d_perm = nullptr;   // permuted row -> original row
+  ~IluPlan();
+};
+
 struct LinAlg {
   int64_t n = 0;             // free DOFs
   int64_t nblk = 0;          // nodal blocks (nodes with >= 1 free DOF)
@@ -24,19 +39,21 @@ struct LinAlg {
   bool pc_fp32 = false;
   int64_t pc_K_version = -1;  // tangent version the preconditioner was built from
   std::vector<int32_t> h_blkStart, h_diagpos;
-  std::vector<int32_t> levLPtr, levUPtr, h_levL, h_levU;
-  int32_t *d_blkStart = nullptr, *d_diagpos = nullptr, *d_levL = nullptr, *d_levU = nullptr;
+  int pc_reorder = 0;
+  IluPlan* ilu = nullptr;
+  int32_t *d_blkStart = nullptr, *d_diagpos = nullptr;
   double* d_part = nullptr;  // reduction partials
   double* d_scal = nullptr;  // Krylov scalars
   double* h_scal = nullptr;  // pinned mirror
   // FP64 Krylov vectors
   double *r = nullptr, *rh = nullptr, *p = nullptr, *v = nullptr, *s = nullptr, *t = nullptr,
-         *ph = nullptr, *sh = nullptr, *x = nullptr, *w = nullptr, *z = nullptr, *gm_V = nullptr;
+         *ph = nullptr, *sh = nullptr, *x = nullptr, *w = nullptr, *z = nullptr, *gm_V = nullptr,
+         *pr = nullptr, *pz = nullptr;
   int gm_m = 0;
   // FP32 Krylov vectors
   float *r32 = nullptr, *rh32 = nullptr, *p32 = nullptr, *v32 = nullptr, *s32 = nullptr,
         *t32 = nullptr, *ph32 = nullptr, *sh32 = nullptr, *x32 = nullptr, *du32 = nullptr,
-        *w32 = nullptr, *z32 = nullptr, *gm_V32 = nullptr;
+        *w32 = nullptr, *z32 = nullptr, *gm_V32 = nullptr, *pr32 = nullptr, *pz32 = nullptr;
   int gm_m32 = 0;
   // preconditioners
   double* dinv = nullptr;
@@ -50,7 +67,7 @@ struct LinAlg {
 
 int la_create(sfem_ctx* ctx);
 void la_destroy(sfem_ctx* ctx);
-int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32);
+int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder);
 int la_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int* iters, double* relres);
 int la_norm2(sfem_ctx* ctx, const double* a, double* out);
 int la_spmv64(sfem_ctx* ctx, const double* x, double* y);

@@ -163,15 +163,16 @@ def read_control(path: str) -> Control:
 
 
 def write_control(path: str, c: Control) -> None:
-    ed = c.elemData
+    ed = [repr(float(x)) for x in c.elemData]
+    r = lambda x: repr(float(x))
     with open(path, "w", encoding="ascii") as f:
-        f.write(f"density           {ed[0]!r}\nviscosity         {ed[1]!r}\n"
-                f"bodyforce         {ed[2]!r} {ed[3]!r} {ed[4]!r}\n"
-                f"stabilisation     {ed[8]!r} {ed[9]!r} {ed[10]!r}\n"
-                f"tis               {c.tis}\nrhoInf            {c.rhoInf!r}\n"
-                f"dt                {c.dt!r}\nFinalTime         {c.timeFinal!r}\n"
+        f.write(f"density           {ed[0]}\nviscosity         {ed[1]}\n"
+                f"bodyforce         {ed[2]} {ed[3]} {ed[4]}\n"
+                f"stabilisation     {ed[8]} {ed[9]} {ed[10]}\n"
+                f"tis               {c.tis}\nrhoInf            {r(c.rhoInf)}\n"
+                f"dt                {r(c.dt)}\nFinalTime         {r(c.timeFinal)}\n"
                 f"MaximumSteps      {c.stepsMax}\nOutputFrequency   {c.outputFreq}\n"
-                f"tolerance         {c.conv_tol!r}\n")
+                f"tolerance         {r(c.conv_tol)}\n")
 
 
 def read_solver_options(path: str) -> SolverOptions:

@@ -39,7 +39,7 @@ _vp = C.c_void_p
 
 class SolverOpts(C.Structure):
     _fields_ = [("ksp_type", C.c_int), ("pc_type", C.c_int), ("rtol", C.c_double),
-                ("max_it", C.c_int), ("restart", C.c_int)]
+                ("max_it", C.c_int), ("restart", C.c_int), ("reorder", C.c_int)]
 
 
 class StepStats(C.Structure):
@@ -264,12 +264,12 @@ class Context:
         return a.value, s.value
 
 
-def solver_opts(ksp="bcgs", pc="ilu", rtol=1e-8, max_it=1000, restart=30) -> SolverOpts:
+def solver_opts(ksp="bcgs", pc="ilu", rtol=1e-8, max_it=1000, restart=30, reorder=0) -> SolverOpts:
     if pc == "lu":
-        # the reference's test decks ask for MUMPS LU; a tight ILU(0)-BiCGStab solve
-        # reproduces the direct answer to well below the decks' tolerances
-        ksp, pc, rtol, max_it = "bcgs", "ilu", 1e-13, max(max_it, 5000)
-    return SolverOpts(KSP[ksp], PC[pc], float(rtol), int(max_it), int(restart))
+        # the reference's test decks ask for MUMPS LU; a tight RCM-ordered ILU(0)-BiCGStab
+        # solve reproduces the direct answer to well below the decks' tolerances
+        ksp, pc, rtol, max_it, reorder = "bcgs", "ilu", 1e-13, max(max_it, 5000), 1
+    return SolverOpts(KSP[ksp], PC[pc], float(rtol), int(max_it), int(restart), int(reorder))
 
 
 class StabFEM:

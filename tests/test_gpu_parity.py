@@ -199,9 +199,10 @@ def test_cavity3d_vs_oracle():
     assert rel_err(s, so) <= 1e-8
 
 
-@pytest.mark.parametrize("ksp,pc", [("bcgs", "ilu"), ("bcgs", "bjacobi4"), ("bcgs", "jacobi"),
-                                    ("gmres", "ilu"), ("gmres", "bjacobi4")])
-def test_krylov_against_direct(ksp, pc):
+@pytest.mark.parametrize("ksp,pc,reorder", [("bcgs", "ilu", 0), ("bcgs", "ilu", 1), ("bcgs", "bjacobi4", 0),
+                                            ("bcgs", "jacobi", 0), ("gmres", "ilu", 0), ("gmres", "ilu", 1),
+                                            ("gmres", "bjacobi4", 0)])
+def test_krylov_against_direct(ksp, pc, reorder):
     """Krylov answer vs a direct solve of the same assembled system."""
     import scipy.sparse as sp
     import scipy.sparse.linalg as spla
@@ -213,9 +214,9 @@ def test_krylov_against_direct(ksp, pc):
     rp, ci, _ = ctx.get_pattern()
     A = sp.csr_matrix((vals, ci, rp), shape=(ctx.nfree, ctx.nfree))
     x_ref = spla.spsolve(A.tocsc(), rhs)
-    its, rr = ctx.solve(sfem.solver_opts(ksp, pc, 1e-10, 5000, 40))
+    its, rr = ctx.solve(sfem.solver_opts(ksp, pc, 1e-10, 5000, 40, reorder))
     du = ctx.get_increment()
-    print(f"[krylov] {ksp}+{pc}: its={its} relres={rr:.2e}", flush=True)
+    print(f"[krylov] {ksp}+{pc} reorder={reorder}: its={its} relres={rr:.2e}", flush=True)
     true_res = np.linalg.norm(rhs - A @ du) / np.linalg.norm(rhs)
     assert true_res <= 1.01e-10
     assert np.linalg.norm(du - x_ref) <= 1e-6 * np.linalg.norm(x_ref)
@@ -226,7 +227,7 @@ def test_mixed_precision_close_to_fp64():
     """FP32 tangent + FP32 Krylov inside an FP64 residual loop stays within 1e-6 of FP64."""
     f64, mesh, ctl = _cavity3d(6, 3, newton_tol=1e-10)
     s64 = f64.solveFullyImplicit()
-    fmx, _, _ = _cavity3d(6, 3, prec=sfem.PREC_MIXED, opts=sfem.solver_opts("bcgs", "bjacobi4", 1e-4, 2000),
+    fmx, _, _ = _cavity3d(6, 3, prec=sfem.PREC_MIXED, opts=sfem.solver_opts("bcgs", "ilu", 1e-4, 2000),
                           newton_tol=1e-10)
     smx = fmx.solveFullyImplicit()
     nd = 4
@@ -238,6 +239,30 @@ def test_mixed_precision_close_to_fp64():
     ep = np.linalg.norm(pmx - p64) / np.linalg.norm(p64)
     print(f"[mixed] rel L2 u {eu:.2e} p {ep:.2e}; history {fmx.history}", flush=True)
     assert eu <= 1e-6 and ep <= 1e-6
+
+
+def test_rcm_ilu_unstructured_tria():
+    """RCM-ordered FP64 and FP32 ILU(0) on a jittered Tria3 cavity reach the direct answer."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    mesh = meshgen.ldc2d_tria(24, jitter=0.2, seed=7)
+    perm = np.random.default_rng(0).permutation(mesh.nNode)   # scramble the node numbering
+    inv = np.argsort(perm)
+    mesh.coords = mesh.coords[perm]
+    mesh.conn = inv[mesh.conn].astype(np.int32)
+    mesh.dbc_node = inv[mesh.dbc_node].astype(np.int32)
+    ctx = make_ctx(mesh, tis=0, dt=0.1, elemData=np.array([1.0, 1e-3, 0, 0, 0, 0, 0, 0, 1, 1, 0]))
+    ctx.time_update(0.5)
+    ctx.assemble(0)
+    vals, rhs = ctx.get_system()
+    rp, ci, _ = ctx.get_pattern()
+    A = sp.csr_matrix((vals, ci, rp), shape=(ctx.nfree, ctx.nfree))
+    x_ref = spla.spsolve(A.tocsc(), rhs)
+    its, rr = ctx.solve(sfem.solver_opts("bcgs", "ilu", 1e-11, 5000, 30, 1))
+    du = ctx.get_increment()
+    print(f"[rcm-ilu] its={its} relres={rr:.2e}", flush=True)
+    assert np.linalg.norm(du - x_ref) <= 1e-7 * np.linalg.norm(x_ref)
+    ctx.close()
 
 
 def test_negative_jacobian_reported():

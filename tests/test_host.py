@@ -1,0 +1,120 @@
+"""Host-side logic on the CPU: the C-ABI library loads and exports every entry
+point of include/sfem.h; deck readers/writers; mesh generators against the
+reference deck; solver-option mapping."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from cfdsfemmpi_amd import deck, meshgen, sfem
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "sfem.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(sfem_[A-Za-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_header_symbol():
+    lib = ctypes.CDLL(sfem.LIB_PATH)
+    names = header_functions()
+    assert len(names) >= 25
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_binding_table_matches_header():
+    assert sorted(n for n, _, _ in sfem.ABI) == header_functions()
+
+
+def test_version_and_error_strings():
+    lib = sfem.load_library()
+    assert lib.sfem_version().decode().startswith("sfem-mi355x")
+    assert isinstance(lib.sfem_last_error(), bytes)
+
+
+def test_create_rejects_bad_mesh_without_touching_device():
+    lib = sfem.load_library()
+    coords = np.zeros((4, 2))
+    conn = np.array([[0, 1, 2, 7]], np.int32)   # node 7 does not exist
+    h = lib.sfem_create(2, 4, 4, 1, coords.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                        conn.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), -1)
+    assert not h
+    assert "connectivity" in lib.sfem_last_error().decode()
+    h = lib.sfem_create(2, 5, 4, 1, coords.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                        conn.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), -1)
+    assert not h and "unsupported" in lib.sfem_last_error().decode()
+
+
+def test_mesh_roundtrip(tmp_path):
+    m = meshgen.ldc3d_hex(3)
+    p = str(tmp_path / "m.dat")
+    deck.write_mesh(p, m)
+    r = deck.read_mesh(p)
+    assert np.array_equal(r.coords, m.coords) and np.array_equal(r.conn, m.conn)
+    assert np.array_equal(r.dbc_node, m.dbc_node) and np.array_equal(r.dbc_dof, m.dbc_dof)
+    assert np.array_equal(r.dbc_value, m.dbc_value)
+
+
+def test_control_roundtrip(tmp_path, golden):
+    c = deck.read_control(os.path.join(golden, "ldc2dquad10", "control-parameters-ldc-Re100.dat"))
+    assert c.elemData[0] == 1.0 and c.elemData[1] == 0.01 and list(c.elemData[8:11]) == [1.0, 1.0, 0.0]
+    assert (c.tis, c.rhoInf, c.dt, c.timeFinal, c.stepsMax) == (0, 0.0, 0.1, 1.0, 10)
+    p = str(tmp_path / "c.dat")
+    deck.write_control(p, c)
+    c2 = deck.read_control(p)
+    assert np.array_equal(c.elemData, c2.elemData) and c.tis == c2.tis and c.dt == c2.dt
+
+
+def test_generator_matches_reference_deck(golden):
+    """The 10x10 generator with the deck's pressure pin reproduces LDC2Dquad10.dat."""
+    ref = deck.read_mesh(os.path.join(golden, "ldc2dquad10", "LDC2Dquad10.dat"))
+    g = meshgen.ldc2d_quad(10, p_pin=(0.5, 0.0))
+    assert np.max(np.abs(g.coords - ref.coords)) < 1e-6    # the deck prints 6 decimals
+    assert np.array_equal(g.conn, ref.conn)
+    key = lambda m: sorted(zip(m.dbc_node.tolist(), m.dbc_dof.tolist(), np.abs(m.dbc_value).tolist()))
+    assert key(g) == key(ref)
+
+
+def test_solver_options_mapping(golden):
+    so = deck.read_solver_options(os.path.join(golden, "ldc2dquad10", "petsc_options.dat"))
+    assert (so.ksp_type, so.pc_type) == ("preonly", "lu")
+    o = sfem.solver_opts(so.ksp_type, so.pc_type, so.rtol, so.max_it)
+    assert o.ksp_type == sfem.KSP["bcgs"] and o.pc_type == sfem.PC["ilu"] and o.rtol <= 1e-12
+
+
+def test_duplicate_dirichlet_entries_in_mesh2(golden):
+    """LDC2Dquad-mesh2 lists 8 DOFs twice (all with value 0); the reader keeps them."""
+    m = deck.read_mesh(os.path.join(golden, "ldc2dquad-mesh2", "LDC2Dquad-mesh2.dat.gz"))
+    assert m.nDBC == 1033
+    keys = m.dbc_node.astype(np.int64) * 3 + m.dbc_dof
+    assert len(np.unique(keys)) == 1025
+
+
+def test_check_result_rule():
+    ref = np.zeros((3, 3))
+    s = np.zeros(9)
+    assert deck.check_result(s, ref, 2) == 0
+    s[4] = 2e-4
+    assert deck.check_result(s, ref, 2) == -2
+    assert deck.check_result(np.zeros(6), ref, 2) == -1
+
+
+def test_generators_geometry():
+    h = meshgen.ldc3d_hex(4)
+    assert h.nNode == 125 and h.nElem == 64
+    x = h.coords[:5, 0]
+    assert np.allclose(x, 0.5 * (1 - np.cos(np.pi * np.arange(5) / 4)))
+    t = meshgen.ldc3d_tet(2)
+    xx = t.coords[t.conn]
+    vol = np.einsum("ij,ij->i", np.cross(xx[:, 1] - xx[:, 0], xx[:, 2] - xx[:, 0]), xx[:, 3] - xx[:, 0]) / 6
+    assert np.all(vol > 0) and np.isclose(vol.sum(), 1.0)
+    tr = meshgen.ldc2d_tria(5, jitter=0.2, seed=4)
+    xx = tr.coords[tr.conn]
+    area = 0.5 * ((xx[:, 1, 0] - xx[:, 0, 0]) * (xx[:, 2, 1] - xx[:, 0, 1]) -
+                  (xx[:, 2, 0] - xx[:, 0, 0]) * (xx[:, 1, 1] - xx[:, 0, 1]))
+    assert np.all(area > 0) and np.isclose(area.sum(), 1.0)
