@@ -31,7 +31,7 @@ constexpr int kTPB = 256;
 // scalar slots
 enum {
   S_RHO0 = 0, S_RHO1, S_ALPHA, S_OMEGA, S_BETA, S_RV, S_SS, S_TT, S_TS, S_RR, S_BB, S_TOL2,
-  S_FLAG, S_MISC0, S_MISC1, S_NSLOTS = 32
+  S_FLAG, S_MISC0, S_MISC1, S_RHRH, S_VV, S_NSLOTS = 32
 };
 
 // ---------------------------------------------------------------------------
@@ -107,15 +107,25 @@ __global__ __launch_bounds__(kTPB) void k_finalize(const double* __restrict__ pa
       break;
     case FIN_RHO1: {  // rho1 = (rh, r); beta = (rho1/rho0)(alpha/omega)
       scal[S_RHO1] = r[0];
-      if (r[0] == 0.0) scal[S_FLAG] = 2.0;  // breakdown
       const double rho0 = scal[S_RHO0], om = scal[S_OMEGA];
-      scal[S_BETA] = (rho0 != 0.0 && om != 0.0) ? (r[0] / rho0) * (scal[S_ALPHA] / om) : 0.0;
+      // near breakdown: rh (almost) orthogonal to r -> restart the shadow residual
+      if (!(fabs(r[0]) > 1e-12 * sqrt(scal[S_RHRH] * scal[S_RR])) || rho0 == 0.0 || om == 0.0) {
+        scal[S_FLAG] = 3.0;
+        scal[S_BETA] = 0.0;
+      } else {
+        scal[S_BETA] = (r[0] / rho0) * (scal[S_ALPHA] / om);
+      }
       break;
     }
-    case FIN_ALPHA:  // alpha = rho1 / (rh, v)
+    case FIN_ALPHA:  // alpha = rho1 / (rh, v); r[1] = (v, v)
       scal[S_RV] = r[0];
-      if (r[0] == 0.0) { scal[S_FLAG] = 2.0; scal[S_ALPHA] = 0.0; }
-      else scal[S_ALPHA] = scal[S_RHO1] / r[0];
+      scal[S_VV] = ND > 1 ? r[ND - 1] : 0.0;
+      if (scal[S_FLAG] == 3.0 || !(fabs(r[0]) > 1e-12 * sqrt(scal[S_RHRH] * scal[S_VV]))) {
+        scal[S_FLAG] = 3.0;   // step with alpha = 0: a minimal-residual step on r
+        scal[S_ALPHA] = 0.0;
+      } else {
+        scal[S_ALPHA] = scal[S_RHO1] / r[0];
+      }
       break;
     case FIN_SS:
       scal[S_SS] = r[0];
@@ -670,6 +680,36 @@ static std::vector<int32_t> rcm_order(const std::vector<int64_t>& adjPtr, const 
   return order;
 }
 
+// Greedy distance-1 colouring of the block graph in block order; returns the old block
+// ids sorted by (colour, id).  Blocks of one colour are pairwise non-adjacent, so the
+// ILU(0) factorisation and both triangular solves advance one colour per level.  On a
+// structured hex grid this is the 2x2x2 parity colouring (8 colours).
+static std::vector<int32_t> multicolor_order(const std::vector<int64_t>& adjPtr, const std::vector<int32_t>& adj,
+                                             int* ncolors) {
+  const int64_t nb = (int64_t)adjPtr.size() - 1;
+  std::vector<int32_t> color(nb, -1), mark;
+  int nc = 0;
+  for (int64_t k = 0; k < nb; ++k) {
+    for (int64_t q = adjPtr[k]; q < adjPtr[k + 1]; ++q) {
+      const int32_t c = color[adj[q]];
+      if (c >= 0) {
+        if ((int)mark.size() <= c) mark.resize(c + 1, -1);
+        mark[c] = (int32_t)k;
+      }
+    }
+    int c = 0;
+    while (c < (int)mark.size() && mark[c] == (int32_t)k) ++c;
+    color[k] = c;
+    nc = std::max(nc, c + 1);
+  }
+  std::vector<int32_t> ptr(nc + 1, 0), order(nb);
+  for (int64_t k = 0; k < nb; ++k) ptr[color[k] + 1]++;
+  for (int c = 0; c < nc; ++c) ptr[c + 1] += ptr[c];
+  for (int64_t k = 0; k < nb; ++k) order[ptr[color[k]]++] = (int32_t)k;
+  *ncolors = nc;
+  return order;
+}
+
 template <typename T>
 static int upload(T** d, const std::vector<T>& h) {
   hipError_t e = alloc(d, h.size());
@@ -713,7 +753,8 @@ static int build_ilu_plan(sfem_ctx* ctx, LinAlg* L, int reorder) {
       }
       adjPtr[k + 1] = (int64_t)adj.size();
     }
-    const std::vector<int32_t> order = rcm_order(adjPtr, adj);
+    int ncol = 0;
+    const std::vector<int32_t> order = reorder == 2 ? multicolor_order(adjPtr, adj, &ncol) : rcm_order(adjPtr, adj);
     std::vector<int32_t> nbs(nb + 1, 0), perm(L->n), iperm(L->n);
     for (int64_t p = 0; p < nb; ++p) {
       const int32_t k = order[p];
@@ -1010,18 +1051,23 @@ static int bicgstab(sfem_ctx* ctx, const T* b, T* x, double rtol, int maxit, int
   std::memset(init, 0, sizeof(init));
   init[S_RHO0] = 1.0; init[S_ALPHA] = 1.0; init[S_OMEGA] = 1.0;
   init[S_BB] = hs[S_BB];
+  init[S_RR] = hs[S_BB];
+  init[S_RHRH] = hs[S_BB];
   init[S_TOL2] = (rtol * bnorm) * (rtol * bnorm);
   std::memcpy(hs, init, sizeof(init));
   LA_CHECK(hipMemcpyAsync(sc, hs, S_NSLOTS * sizeof(double), hipMemcpyHostToDevice, st));
   double rnorm = bnorm;
+  bool first = true;
+  int restarts = 0;
   for (int it = 1; it <= maxit; ++it) {
     hipLaunchKernelGGL((k_dot<T>), dim3(g), dim3(kTPB), 0, st, n, rh, r, L->d_part);
     hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(kTPB), 0, st, L->d_part, g, sc, 0, (int)FIN_RHO1);
-    hipLaunchKernelGGL((k_bicg_p<T>), dim3(g), dim3(kTPB), 0, st, n, r, p, v, sc, it == 1 ? 1 : 0);
+    hipLaunchKernelGGL((k_bicg_p<T>), dim3(g), dim3(kTPB), 0, st, n, r, p, v, sc, first ? 1 : 0);
+    first = false;
     int rc = pc_apply<T>(ctx, p, ph); if (rc) return rc;
     spmv<T>(ctx, ph, v);
-    hipLaunchKernelGGL((k_dot<T>), dim3(g), dim3(kTPB), 0, st, n, rh, v, L->d_part);
-    hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(kTPB), 0, st, L->d_part, g, sc, 0, (int)FIN_ALPHA);
+    hipLaunchKernelGGL((k_dot2<T>), dim3(g), dim3(kTPB), 0, st, n, rh, v, v, v, L->d_part);
+    hipLaunchKernelGGL((k_finalize<2>), dim3(1), dim3(kTPB), 0, st, L->d_part, g, sc, 0, (int)FIN_ALPHA);
     hipLaunchKernelGGL((k_bicg_s<T>), dim3(g), dim3(kTPB), 0, st, n, r, v, s, sc, L->d_part);
     hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(kTPB), 0, st, L->d_part, g, sc, 0, (int)FIN_SS);
     rc = pc_apply<T>(ctx, s, sh); if (rc) return rc;
@@ -1043,9 +1089,18 @@ static int bicgstab(sfem_ctx* ctx, const T* b, T* x, double rtol, int maxit, int
       *relres = rnorm / bnorm;
       return SFEM_OK;
     }
-    if (hs[S_FLAG] == 2.0) {
-      *relres = rnorm / bnorm;
-      return set_error(SFEM_ERR_DIVERGED, "BiCGStab: breakdown (rho or (rh, v) = 0)");
+    if (hs[S_FLAG] == 3.0) {
+      // near breakdown: restart with the current residual as shadow residual
+      if (++restarts > 100) {
+        *relres = rnorm / bnorm;
+        return set_error(SFEM_ERR_DIVERGED, "BiCGStab: repeated breakdown");
+      }
+      hipLaunchKernelGGL((k_copy<T>), dim3(g), dim3(kTPB), 0, st, n, r, rh);
+      hs[S_FLAG] = 0.0;
+      hs[S_RHO0] = 1.0; hs[S_ALPHA] = 1.0; hs[S_OMEGA] = 1.0;
+      hs[S_RHRH] = hs[S_RR];
+      LA_CHECK(hipMemcpyAsync(sc, hs, S_NSLOTS * sizeof(double), hipMemcpyHostToDevice, st));
+      first = true;
     }
   }
   *relres = rnorm / bnorm;

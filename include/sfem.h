@@ -52,8 +52,9 @@ typedef struct sfem_solver_opts {
   double rtol;       /* -ksp_rtol: stop when ||b - A x||_2 <= rtol ||b||_2 */
   int max_it;        /* -ksp_max_it */
   int restart;       /* -ksp_gmres_restart */
-  int reorder;       /* ILU(0) only: 0 the matrix's own node order, 1 reverse Cuthill-McKee over
-                        nodes (PETSc's -pc_factor_mat_ordering_type rcm) */
+  int reorder;       /* ILU(0) node ordering: 0 the matrix's own order, 1 reverse Cuthill-McKee
+                        (PETSc's -pc_factor_mat_ordering_type rcm), 2 greedy multicolour
+                        (one fully parallel level per colour) */
 } sfem_solver_opts;
 
 typedef struct sfem_step_stats {

@@ -70,6 +70,8 @@ def lib():
     L.orc_pc_create.restype = C.c_void_p
     L.orc_pc_apply.argtypes = [C.c_void_p, f64p, f64p]
     L.orc_pc_apply.restype = None
+    L.orc_pc_lu.argtypes = [C.c_void_p, f64p]
+    L.orc_pc_lu.restype = C.c_int64
     L.orc_pc_destroy.argtypes = [C.c_void_p]
     L.orc_pc_destroy.restype = None
     L.orc_bicgstab.argtypes = [C.c_int64, i64p, i32p, f64p, C.c_void_p, f64p, f64p, C.c_double,
@@ -167,6 +169,13 @@ class OraclePC:
         self.h = lib().orc_pc_create(PC_TYPES[pc_type], len(rowptr) - 1, rowptr, colidx, vals, nblk,
                                      bp, C.byref(st))
         self.status = st.value
+
+    def lu(self, nnz):
+        """ILU(0) factor values on the CSR pattern (L strictly lower, U upper)."""
+        out = np.zeros(nnz)
+        if lib().orc_pc_lu(self.h, out) != nnz:
+            raise ValueError("not an ILU(0) preconditioner")
+        return out
 
     def apply(self, r):
         z = np.zeros_like(r)

@@ -811,6 +811,12 @@ void orc_pc_apply(const orc_pc* pc, const double* r, double* z) {
 
 void orc_pc_destroy(orc_pc* pc) { delete pc; }
 
+int64_t orc_pc_lu(const orc_pc* pc, double* out) {
+  if (pc->type != 3) return -1;
+  if (out) std::copy(pc->lu.begin(), pc->lu.end(), out);
+  return (int64_t)pc->lu.size();
+}
+
 namespace {
 double dot(int64_t n, const double* a, const double* b) {
   double s = 0.0;

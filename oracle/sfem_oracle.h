@@ -105,6 +105,9 @@ orc_pc* orc_pc_create(int type, int64_t n, const int64_t* rowptr, const int32_t*
                       const double* vals, int64_t nblk, const int64_t* blk_ptr, int* status);
 void orc_pc_apply(const orc_pc* pc, const double* r, double* z);
 void orc_pc_destroy(orc_pc* pc);
+/* Copy out the ILU(0) factor values (L strictly below, U on and above the
+ * diagonal, on the CSR pattern).  Returns the count, or -1 if not ILU. */
+int64_t orc_pc_lu(const orc_pc* pc, double* out);
 
 /* Right-preconditioned BiCGStab (van der Vorst 1992; Saad Alg. 9.21 with right
  * preconditioning), x0 = 0, stop on ||b - A x||_2 <= rtol ||b||_2.

@@ -199,7 +199,8 @@ def test_cavity3d_vs_oracle():
     assert rel_err(s, so) <= 1e-8
 
 
-@pytest.mark.parametrize("ksp,pc,reorder", [("bcgs", "ilu", 0), ("bcgs", "ilu", 1), ("bcgs", "bjacobi4", 0),
+@pytest.mark.parametrize("ksp,pc,reorder", [("bcgs", "ilu", 0), ("bcgs", "ilu", 1), ("bcgs", "ilu", 2),
+                                            ("gmres", "ilu", 2), ("bcgs", "bjacobi4", 0),
                                             ("bcgs", "jacobi", 0), ("gmres", "ilu", 0), ("gmres", "ilu", 1),
                                             ("gmres", "bjacobi4", 0)])
 def test_krylov_against_direct(ksp, pc, reorder):
@@ -227,7 +228,7 @@ def test_mixed_precision_close_to_fp64():
     """FP32 tangent + FP32 Krylov inside an FP64 residual loop stays within 1e-6 of FP64."""
     f64, mesh, ctl = _cavity3d(6, 3, newton_tol=1e-10)
     s64 = f64.solveFullyImplicit()
-    fmx, _, _ = _cavity3d(6, 3, prec=sfem.PREC_MIXED, opts=sfem.solver_opts("bcgs", "ilu", 1e-4, 2000),
+    fmx, _, _ = _cavity3d(6, 3, prec=sfem.PREC_MIXED, opts=sfem.solver_opts("bcgs", "ilu", 1e-4, 2000, reorder=2),
                           newton_tol=1e-10)
     smx = fmx.solveFullyImplicit()
     nd = 4
