@@ -40,24 +40,6 @@ int set_hip_error(hipError_t e, const char* what) {
     if (_e != hipSuccess) return set_hip_error(_e, #x); \
   } while (0)
 
-// Run f(lo, hi) over [0, n) on the host's cores.
-template <typename F>
-static void parallel_for(int64_t n, F f) {
-  unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-  if (n < 4096 || nt == 1) {
-    f((int64_t)0, n);
-    return;
-  }
-  std::vector<std::thread> th;
-  const int64_t chunk = (n + nt - 1) / nt;
-  for (unsigned t = 0; t < nt; ++t) {
-    const int64_t lo = t * chunk, hi = std::min(n, lo + chunk);
-    if (lo >= hi) break;
-    th.emplace_back([=] { f(lo, hi); });
-  }
-  for (auto& x : th) x.join();
-}
-
 // Time-integration coefficients of SetTimeParametersFluid (UtilitiesGeneral.cpp:5-209).
 static int time_params(int tis, double rho, double dt, double* td) {
   for (int i = 0; i < 16; ++i) td[i] = 0.0;
@@ -621,6 +603,13 @@ int sfem_solve(sfem_ctx* c, const sfem_solver_opts* o, int* iters, double* relre
   if (iters) *iters = it;
   if (relres) *relres = rr;
   return rc;
+}
+
+int sfem_setup_solver(sfem_ctx* c, const sfem_solver_opts* o) {
+  if (!c || !o) return set_error(SFEM_ERR_ARG, "null argument");
+  if (!(c->status & ST_PATTERN)) return set_error(SFEM_ERR_STATE, "pattern not built");
+  HIP_OK(hipSetDevice(c->device));
+  return la_prepare(c, o);
 }
 
 int sfem_update_solution(sfem_ctx* c) {

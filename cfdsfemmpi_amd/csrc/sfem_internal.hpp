@@ -5,12 +5,32 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/sfem.h"
 
 namespace sfem {
+
+// Run f(lo, hi) over [0, n) on the host's cores.
+template <typename F>
+inline void parallel_for(int64_t n, F f) {
+  unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (n < 4096 || nt == 1) {
+    f((int64_t)0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  const int64_t chunk = (n + nt - 1) / nt;
+  for (unsigned t = 0; t < nt; ++t) {
+    const int64_t lo = t * chunk, hi = std::min(n, lo + chunk);
+    if (lo >= hi) break;
+    th.emplace_back([=] { f(lo, hi); });
+  }
+  for (auto& x : th) x.join();
+}
 
 // Element families, keyed as in the reference's element factory (StabFEM.cpp:351-366).
 enum ElemType { TRIA3 = 0, QUAD4 = 1, TETRA4 = 2, HEXA8 = 3 };

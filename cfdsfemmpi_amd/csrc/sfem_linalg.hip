@@ -217,41 +217,12 @@ __global__ __launch_bounds__(kTPB) void k_gather(int64_t n, const int32_t* __res
     dst[i] = src[map[i]];
 }
 
-// dst[map[i]] = src[i]
-template <typename T>
-__global__ __launch_bounds__(kTPB) void k_scatter(int64_t n, const int32_t* __restrict__ map,
-                                                  const T* __restrict__ src, T* __restrict__ dst) {
-  for (int64_t i = (int64_t)blockIdx.x * kTPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kTPB)
-    dst[map[i]] = src[i];
-}
-
 // y = a*x + b*y with host scalars (GMRES)
 template <typename T>
 __global__ __launch_bounds__(kTPB) void k_axpby(int64_t n, double a, const T* __restrict__ x, double b,
                                                 T* __restrict__ y) {
   for (int64_t i = (int64_t)blockIdx.x * kTPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kTPB)
     y[i] = (T)(a * (double)x[i] + b * (double)y[i]);
-}
-
-// ---------------------------------------------------------------------------
-// SpMV: SUB lanes per row (SUB a power of two <= 64)
-// ---------------------------------------------------------------------------
-template <typename VT, typename XT, int SUB>
-__global__ __launch_bounds__(kTPB) void k_spmv(int64_t nrows, const int32_t* __restrict__ rowptr,
-                                               const int32_t* __restrict__ colidx,
-                                               const VT* __restrict__ vals, const XT* __restrict__ x,
-                                               XT* __restrict__ y) {
-  const int64_t gt = (int64_t)blockIdx.x * kTPB + threadIdx.x;
-  const int64_t row = gt / SUB;
-  const int lane = (int)(gt % SUB);
-  using AT = typename std::conditional<sizeof(VT) == 8, double, float>::type;
-  AT s = 0;
-  if (row < nrows) {
-    const int32_t b = rowptr[row], e = rowptr[row + 1];
-    for (int32_t k = b + lane; k < e; k += SUB) s += (AT)vals[k] * (AT)x[colidx[k]];
-  }
-  for (int o = SUB / 2; o > 0; o >>= 1) s += __shfl_down(s, o, SUB);
-  if (row < nrows && lane == 0) y[row] = (XT)s;
 }
 
 // ---------------------------------------------------------------------------
@@ -363,121 +334,241 @@ __device__ inline int lds_find(const int32_t* cols, int len, int32_t c) {
   return (lo < len && cols[lo] == c) ? lo : -1;
 }
 
+template <int G, typename AT>
+__device__ inline AT group_sum(AT v) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
+  return v;
+}
+
+template <typename AT>
+__device__ inline AT pick4(const AT (&v)[4], int i) {
+  return i == 0 ? v[0] : i == 1 ? v[1] : i == 2 ? v[2] : v[3];
+}
+
+// ILU(0) factorisation of the blocks of one level (one 64-lane workgroup per block).
+// IKJ order per row: external lower columns (earlier blocks, already final) ascending,
+// then the block's own lower columns.  The m rows of the block share the column list, so
+// the position of every upper entry of an external row is looked up once for all rows.
 template <typename LT>
-__global__ __launch_bounds__(64) void k_ilu_factor_level(const int32_t* __restrict__ list, int count,
-                                                         const int32_t* __restrict__ blkStart,
-                                                         const int32_t* __restrict__ rowptr,
-                                                         const int32_t* __restrict__ colidx,
-                                                         const int32_t* __restrict__ diagpos,
-                                                         LT* __restrict__ lu, double* scal) {
+__global__ __launch_bounds__(64) void k_ilu_factor_blk(const int32_t* __restrict__ list, int first, int count,
+                                                       const int32_t* __restrict__ blkStart,
+                                                       const int32_t* __restrict__ rowptr,
+                                                       const int32_t* __restrict__ colidx,
+                                                       const int32_t* __restrict__ diagpos,
+                                                       LT* __restrict__ lu, double* scal) {
   __shared__ int32_t cols[kIluMaxRow];
   __shared__ double rows[4][kIluMaxRow];
-  __shared__ double piv;
+  __shared__ double lval[4];
   const int w = blockIdx.x;
   if (w >= count) return;
   const int lane = threadIdx.x;
-  const int32_t blk = list[w];
+  const int32_t blk = list ? list[first + w] : first + w;
   const int32_t r0 = blkStart[blk];
   const int m = blkStart[blk + 1] - r0;
-  const int32_t base0 = rowptr[r0];
-  const int len = rowptr[r0 + 1] - base0;
-  for (int q = lane; q < len; q += 64) cols[q] = colidx[base0 + q];
+  const int32_t base = rowptr[r0];
+  const int rl = rowptr[r0 + 1] - base;
+  const int dq0 = diagpos[r0] - base;
+  for (int q = lane; q < rl; q += 64) cols[q] = colidx[base + q];
   for (int a = 0; a < m; ++a)
-    for (int q = lane; q < len; q += 64) rows[a][q] = (double)lu[rowptr[r0 + a] + q];
+    for (int q = lane; q < rl; q += 64) rows[a][q] = (double)lu[base + a * rl + q];
   __syncthreads();
-  for (int a = 0; a < m; ++a) {
-    const int32_t r = r0 + a;
-    // eliminate with every lower column c < r in ascending order
-    for (int k = 0; k < len && cols[k] < r; ++k) {
-      const int32_t c = cols[k];
-      const bool internal = (c >= r0);
+  for (int q = 0; q < dq0; ++q) {
+    const int32_t c = cols[q];
+    if (lane < m) {
+      const double d = (double)lu[diagpos[c]];
+      if (d == 0.0) scal[S_MISC1] = 2.0;
+      const double l = rows[lane][q] / (d == 0.0 ? 1.0 : d);
+      rows[lane][q] = l;
+      lval[lane] = l;
+    }
+    __syncthreads();
+    const int32_t cb = diagpos[c] + 1, ce = rowptr[c + 1];
+    for (int32_t j = cb + lane; j < ce; j += 64) {
+      const int pos = lds_find(cols, rl, colidx[j]);
+      if (pos >= 0) {
+        const double u = (double)lu[j];
+        for (int a = 0; a < m; ++a) rows[a][pos] -= lval[a] * u;
+      }
+    }
+    __syncthreads();
+  }
+  for (int a = 1; a < m; ++a) {
+    for (int ap = 0; ap < a; ++ap) {
       if (lane == 0) {
-        const double d = internal ? rows[c - r0][lds_find(cols, len, c)] : (double)lu[diagpos[c]];
+        const double d = rows[ap][dq0 + ap];
         if (d == 0.0) scal[S_MISC1] = 2.0;
-        piv = rows[a][k] / (d == 0.0 ? 1.0 : d);
-        rows[a][k] = piv;
+        const double l = rows[a][dq0 + ap] / (d == 0.0 ? 1.0 : d);
+        rows[a][dq0 + ap] = l;
+        lval[0] = l;
       }
       __syncthreads();
-      const double lrk = piv;
-      if (internal) {
-        const int dc = lds_find(cols, len, c);
-        for (int q = dc + 1 + lane; q < len; q += 64) rows[a][q] -= lrk * rows[c - r0][q];
-      } else {
-        const int32_t cb = diagpos[c] + 1, ce = rowptr[c + 1];
-        for (int32_t q = cb + lane; q < ce; q += 64) {
-          const int pos = lds_find(cols, len, colidx[q]);
-          if (pos >= 0) rows[a][pos] -= lrk * (double)lu[q];
-        }
-      }
+      const double l = lval[0];
+      for (int q = dq0 + ap + 1 + lane; q < rl; q += 64) rows[a][q] -= l * rows[ap][q];
       __syncthreads();
     }
   }
   for (int a = 0; a < m; ++a)
-    for (int q = lane; q < len; q += 64) lu[rowptr[r0 + a] + q] = (LT)rows[a][q];
+    for (int q = lane; q < rl; q += 64) lu[base + a * rl + q] = (LT)rows[a][q];
 }
 
-// L y = b (unit lower) for the blocks of one level
-template <typename T, typename LT>
-__global__ __launch_bounds__(64) void k_ilu_lsolve_level(const int32_t* __restrict__ list, int count,
-                                                         const int32_t* __restrict__ blkStart,
-                                                         const int32_t* __restrict__ rowptr,
-                                                         const int32_t* __restrict__ colidx,
-                                                         const LT* __restrict__ lu,
-                                                         const T* __restrict__ b, T* __restrict__ z) {
-  __shared__ double zl[4];
-  const int w = blockIdx.x;
-  if (w >= count) return;
-  const int lane = threadIdx.x;
-  const int32_t blk = list[w];
-  const int32_t r0 = blkStart[blk];
-  const int m = blkStart[blk + 1] - r0;
-  for (int a = 0; a < m; ++a) {
-    const int32_t r = r0 + a;
-    const int32_t rb = rowptr[r], re = rowptr[r + 1];
-    double s = 0.0;
-    for (int32_t q = rb + lane; q < re; q += 64) {
-      const int32_t c = colidx[q];
-      if (c < r0) s += (double)lu[q] * (double)z[c];
-      else if (c < r) s += (double)lu[q] * zl[c - r0];
-    }
-    s = warp_sum(s);
-    if (lane == 0) zl[a] = (double)b[r] - s;
-    __syncthreads();
+// Node block descriptor: first row, first value of that row, row length, and
+// (diagonal-block column offset) | (row count << 16).
+struct __align__(16) BlkDesc {
+  int32_t r0, base, rl, dqm;
+};
+
+// acc[a] += sum_{q in [q0, q1)} V[a*rl + q] * x[col[q]] for a < m, lanes of a group of G
+// stepping by G; U columns per lane are loaded before any is used, so a lane keeps
+// U index loads, U gathers and U*m value loads in flight.
+template <int G, int U, typename VT, typename XT, typename AT>
+__device__ inline void group_dot(const int32_t* __restrict__ col, const VT* __restrict__ V, const XT* __restrict__ x,
+                                 int32_t rl, int m, int32_t q0, int32_t q1, int lane, AT (&acc)[4]) {
+  int32_t q = q0 + lane;
+  for (; q + (U - 1) * G < q1; q += U * G) {
+    int32_t c[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) c[u] = col[q + u * G];
+    AT v[4][U];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int u = 0; u < U; ++u) v[a][u] = (a < m) ? (AT)V[(int64_t)a * rl + q + u * G] : (AT)0;
+    AT xv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) xv[u] = (AT)x[c[u]];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int a = 0; a < 4; ++a) acc[a] += v[a][u] * xv[u];
   }
-  if (lane < m) z[r0 + lane] = (T)zl[lane];
+  for (; q < q1; q += G) {
+    const AT xv = (AT)x[col[q]];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+      if (a < m) acc[a] += (AT)V[(int64_t)a * rl + q] * xv;
+  }
 }
 
-// U z = y for the blocks of one level (levels ordered from the last node down)
-template <typename T, typename LT>
-__global__ __launch_bounds__(64) void k_ilu_usolve_level(const int32_t* __restrict__ list, int count,
-                                                         const int32_t* __restrict__ blkStart,
-                                                         const int32_t* __restrict__ rowptr,
-                                                         const int32_t* __restrict__ colidx,
-                                                         const int32_t* __restrict__ diagpos,
-                                                         const LT* __restrict__ lu,
-                                                         T* __restrict__ z) {
-  __shared__ double zl[4];
-  const int w = blockIdx.x;
-  if (w >= count) return;
-  const int lane = threadIdx.x;
-  const int32_t blk = list[w];
-  const int32_t r0 = blkStart[blk];
-  const int m = blkStart[blk + 1] - r0;
-  const int32_t r1 = r0 + m;
-  for (int a = m - 1; a >= 0; --a) {
-    const int32_t r = r0 + a;
-    const int32_t dp = diagpos[r], re = rowptr[r + 1];
-    double s = 0.0;
-    for (int32_t q = dp + 1 + lane; q < re; q += 64) {
-      const int32_t c = colidx[q];
-      if (c >= r1) s += (double)lu[q] * (double)z[c];
-      else s += (double)lu[q] * zl[c - r0];
-    }
-    s = warp_sum(s);
-    if (lane == 0) zl[a] = ((double)z[r] - s) / (double)lu[dp];
-    __syncthreads();
+// Node-grouped CSR SpMV.  The free rows of one node (a block of m <= 4 rows) share one
+// column list, so G lanes walk that list once, load each column index and x entry once,
+// and accumulate the m rows together; values of row a sit at base + a*rl + q.
+template <typename VT, typename XT, int G>
+__global__ __launch_bounds__(256) void k_spmv_blk(int64_t nb, const BlkDesc* __restrict__ desc,
+                                                  const int32_t* __restrict__ colidx,
+                                                  const VT* __restrict__ vals, const XT* __restrict__ x,
+                                                  XT* __restrict__ y) {
+  using AT = typename std::conditional<sizeof(VT) == 8, double, float>::type;
+  const int64_t gt = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t b = gt / G;
+  const int lane = (int)(gt % G);
+  AT acc[4] = {0, 0, 0, 0};
+  BlkDesc d = {0, 0, 0, 0};
+  if (b < nb) {
+    d = desc[b];
+    group_dot<G, 4>(colidx + d.base, vals + d.base, x, d.rl, d.dqm >> 16, 0, d.rl, lane, acc);
   }
-  if (lane < m) z[r0 + lane] = (T)zl[lane];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) acc[a] = group_sum<G>(acc[a]);
+  if (b < nb && lane < (d.dqm >> 16)) y[d.r0 + lane] = (XT)pick4(acc, lane);
+}
+
+// Forward solve L y = r over one level, G >= 16 lanes per block: the external lower part
+// as a grouped sparse dot product, then the unit-lower m x m diagonal block, whose
+// entries and right-hand sides are loaded by separate lanes and shared by shuffles.
+// Level blocks are either the contiguous range [first, first + count) or list[first + g].
+template <typename T, typename LT, int G>
+__global__ __launch_bounds__(256) void k_ilu_lsolve_blk(const int32_t* __restrict__ list, int first, int count,
+                                                        const BlkDesc* __restrict__ desc,
+                                                        const int32_t* __restrict__ colidx,
+                                                        const LT* __restrict__ lu,
+                                                        const int32_t* __restrict__ perm,
+                                                        const T* __restrict__ r, T* __restrict__ y) {
+  using AT = typename std::conditional<sizeof(LT) == 8, double, float>::type;
+  const int64_t gt = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t g = gt / G;
+  const int lane = (int)(gt % G);
+  AT acc[4] = {0, 0, 0, 0};
+  BlkDesc d = {0, 0, 0, 0};
+  AT dv = 0, rv = 0;
+  const bool act = g < count;
+  if (act) {
+    d = desc[list ? list[first + g] : first + (int32_t)g];
+    const int m = d.dqm >> 16, dq0 = d.dqm & 0xffff;
+    // diagonal block entry (lane / 4, lane % 4) and right-hand side of row `lane`
+    if (lane < 16 && (lane >> 2) < m && (lane & 3) < m)
+      dv = (AT)lu[d.base + (int64_t)(lane >> 2) * d.rl + dq0 + (lane & 3)];
+    if (lane < m) rv = (AT)r[perm ? perm[d.r0 + lane] : d.r0 + lane];
+    group_dot<G, 2>(colidx + d.base, lu + d.base, y, d.rl, m, 0, dq0, lane, acc);
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a) acc[a] = group_sum<G>(acc[a]);
+  AT D[16], R[4];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) D[t] = __shfl(dv, t, G);
+#pragma unroll
+  for (int a = 0; a < 4; ++a) R[a] = __shfl(rv, a, G);
+  const int m = d.dqm >> 16;
+  AT v[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    AT s = R[a] - acc[a];
+#pragma unroll
+    for (int ap = 0; ap < a; ++ap) s -= D[a * 4 + ap] * v[ap];
+    v[a] = s;
+  }
+  if (act && lane < m) y[d.r0 + lane] = (T)pick4(v, lane);
+}
+
+// Backward solve U z = y over one level (levels in reverse order), in place on y
+// (permuted order); the result also goes to zout through perm when perm is non-null.
+template <typename T, typename LT, int G>
+__global__ __launch_bounds__(256) void k_ilu_usolve_blk(const int32_t* __restrict__ list, int first, int count,
+                                                        const BlkDesc* __restrict__ desc,
+                                                        const int32_t* __restrict__ colidx,
+                                                        const LT* __restrict__ lu,
+                                                        const int32_t* __restrict__ perm,
+                                                        T* __restrict__ y, T* __restrict__ zout) {
+  using AT = typename std::conditional<sizeof(LT) == 8, double, float>::type;
+  const int64_t gt = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t g = gt / G;
+  const int lane = (int)(gt % G);
+  AT acc[4] = {0, 0, 0, 0};
+  BlkDesc d = {0, 0, 0, 0};
+  AT dv = 0, rv = 0;
+  const bool act = g < count;
+  if (act) {
+    d = desc[list ? list[first + g] : first + (int32_t)g];
+    const int m = d.dqm >> 16, dq0 = d.dqm & 0xffff;
+    if (lane < 16 && (lane >> 2) < m && (lane & 3) < m)
+      dv = (AT)lu[d.base + (int64_t)(lane >> 2) * d.rl + dq0 + (lane & 3)];
+    if (lane < m) rv = (AT)y[d.r0 + lane];
+    group_dot<G, 2>(colidx + d.base, lu + d.base, y, d.rl, m, dq0 + m, d.rl, lane, acc);
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a) acc[a] = group_sum<G>(acc[a]);
+  AT D[16], R[4];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) D[t] = __shfl(dv, t, G);
+#pragma unroll
+  for (int a = 0; a < 4; ++a) R[a] = __shfl(rv, a, G);
+  const int m = d.dqm >> 16;
+  AT v[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int a = 3; a >= 0; --a) {
+    if (a < m) {
+      AT s = R[a] - acc[a];
+#pragma unroll
+      for (int ap = a + 1; ap < 4; ++ap)
+        if (ap < m) s -= D[a * 4 + ap] * v[ap];
+      v[a] = s / D[a * 4 + a];
+    }
+  }
+  if (act && lane < m) {
+    const AT z = pick4(v, lane);
+    y[d.r0 + lane] = (T)z;
+    if (perm) zout[perm[d.r0 + lane]] = (T)z;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -506,12 +597,15 @@ static hipError_t alloc(T** p, size_t n) {
 LinAlg::~LinAlg() {
   void* ptrs[] = {d_blkStart, d_diagpos, d_part, d_scal, r, rh, p, v, s, t, ph, sh, x, w, z, pr, pz,
                   r32, rh32, p32, v32, s32, t32, ph32, sh32, x32, du32, w32, z32, pr32, pz32,
-                  dinv, dinv32, binv, binv32, lu, lu32, gm_V, gm_V32};
+                  dinv, dinv32, binv, binv32, lu, lu32, gm_V, gm_V32, d_desc};
   delete ilu;
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (h_scal) (void)hipHostFree(h_scal);
 }
+
+static std::vector<BlkDesc> make_desc(const std::vector<int32_t>& bs, const std::vector<int64_t>& rowptr,
+                                      const std::vector<int32_t>& diagpos);
 
 int la_create(sfem_ctx* ctx) {
   LinAlg* L = new LinAlg;
@@ -546,6 +640,11 @@ int la_create(sfem_ctx* ctx) {
   LA_CHECK(hipMemcpy(L->d_blkStart, bs.data(), bs.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   LA_CHECK(alloc(&L->d_diagpos, n));
   LA_CHECK(hipMemcpy(L->d_diagpos, dp.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+  {
+    std::vector<BlkDesc> dd = make_desc(bs, ctx->rowptr, dp);
+    LA_CHECK(alloc(&L->d_desc, dd.size()));
+    if (!dd.empty()) LA_CHECK(hipMemcpy(L->d_desc, dd.data(), dd.size() * sizeof(BlkDesc), hipMemcpyHostToDevice));
+  }
   LA_CHECK(alloc(&L->d_part, (size_t)kRedBlocks * 4));
   LA_CHECK(alloc(&L->d_scal, S_NSLOTS));
   LA_CHECK(hipHostMalloc((void**)&L->h_scal, S_NSLOTS * sizeof(double), hipHostMallocDefault));
@@ -685,7 +784,7 @@ static std::vector<int32_t> rcm_order(const std::vector<int64_t>& adjPtr, const 
 // ILU(0) factorisation and both triangular solves advance one colour per level.  On a
 // structured hex grid this is the 2x2x2 parity colouring (8 colours).
 static std::vector<int32_t> multicolor_order(const std::vector<int64_t>& adjPtr, const std::vector<int32_t>& adj,
-                                             int* ncolors) {
+                                             std::vector<int32_t>* colorPtr) {
   const int64_t nb = (int64_t)adjPtr.size() - 1;
   std::vector<int32_t> color(nb, -1), mark;
   int nc = 0;
@@ -705,8 +804,8 @@ static std::vector<int32_t> multicolor_order(const std::vector<int64_t>& adjPtr,
   std::vector<int32_t> ptr(nc + 1, 0), order(nb);
   for (int64_t k = 0; k < nb; ++k) ptr[color[k] + 1]++;
   for (int c = 0; c < nc; ++c) ptr[c + 1] += ptr[c];
+  *colorPtr = ptr;  // colour c occupies [ptr[c], ptr[c+1]) of the new order
   for (int64_t k = 0; k < nb; ++k) order[ptr[color[k]]++] = (int32_t)k;
-  *ncolors = nc;
   return order;
 }
 
@@ -717,8 +816,23 @@ static int upload(T** d, const std::vector<T>& h) {
   return e == hipSuccess ? SFEM_OK : set_hip_error(e, "upload");
 }
 
+// Descriptors of the node blocks of a (possibly permuted) pattern.
+static std::vector<BlkDesc> make_desc(const std::vector<int32_t>& bs, const std::vector<int64_t>& rowptr,
+                                      const std::vector<int32_t>& diagpos) {
+  const int64_t nb = (int64_t)bs.size() - 1;
+  std::vector<BlkDesc> d(nb);
+  for (int64_t k = 0; k < nb; ++k) {
+    const int32_t r0 = bs[k];
+    const int32_t base = (int32_t)rowptr[r0];
+    const int32_t rl = (int32_t)(rowptr[r0 + 1] - rowptr[r0]);
+    const int32_t dq0 = diagpos[r0] - base;
+    d[k] = BlkDesc{r0, base, rl, dq0 | ((bs[k + 1] - r0) << 16)};
+  }
+  return d;
+}
+
 IluPlan::~IluPlan() {
-  void* ptrs[] = {d_levL, d_levU, d_rowptr, d_colidx, d_diagpos, d_blkStart, d_pmap, d_perm};
+  void* ptrs[] = {d_levL, d_levU, d_rowptr, d_colidx, d_diagpos, d_blkStart, d_pmap, d_perm, d_desc};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
 }
@@ -733,6 +847,7 @@ static int build_ilu_plan(sfem_ctx* ctx, LinAlg* L, int reorder) {
   const int64_t nb = L->nblk;
   if (P.identity) {
     build_levels(ctx->rowptr, ctx->colidx, bs, P);
+    P.d_desc = nullptr;  // the context's descriptors are used
     P.rowptr = ctx->d_rowptr32;
     P.colidx = ctx->d_colidx;
     P.diagpos = L->d_diagpos;
@@ -753,8 +868,8 @@ static int build_ilu_plan(sfem_ctx* ctx, LinAlg* L, int reorder) {
       }
       adjPtr[k + 1] = (int64_t)adj.size();
     }
-    int ncol = 0;
-    const std::vector<int32_t> order = reorder == 2 ? multicolor_order(adjPtr, adj, &ncol) : rcm_order(adjPtr, adj);
+    std::vector<int32_t> colorPtr;
+    const std::vector<int32_t> order = reorder == 2 ? multicolor_order(adjPtr, adj, &colorPtr) : rcm_order(adjPtr, adj);
     std::vector<int32_t> nbs(nb + 1, 0), perm(L->n), iperm(L->n);
     for (int64_t p = 0; p < nb; ++p) {
       const int32_t k = order[p];
@@ -770,25 +885,43 @@ static int build_ilu_plan(sfem_ctx* ctx, LinAlg* L, int reorder) {
       for (int32_t r = nbs[p]; r < nbs[p + 1]; ++r) rp[r + 1] = rp[r] + len;
     }
     std::vector<int32_t> ci(ctx->nnz), pmap(ctx->nnz), dp(L->n, -1);
-    std::vector<std::pair<int32_t, int32_t>> tmp;
-    for (int64_t p = 0; p < nb; ++p) {
-      const int32_t k = order[p];
-      const int64_t b0 = ctx->rowptr[bs[k]];
-      const int64_t len = ctx->rowptr[bs[k] + 1] - b0;
-      tmp.resize(len);
-      for (int64_t q = 0; q < len; ++q) tmp[q] = {iperm[ctx->colidx[b0 + q]], (int32_t)q};
-      std::sort(tmp.begin(), tmp.end());
-      for (int32_t a = 0; a < bs[k + 1] - bs[k]; ++a) {
-        const int32_t rn = nbs[p] + a;
-        const int64_t ob = ctx->rowptr[bs[k] + a];
-        for (int64_t q = 0; q < len; ++q) {
-          ci[rp[rn] + q] = tmp[q].first;
-          pmap[rp[rn] + q] = (int32_t)(ob + tmp[q].second);
-          if (tmp[q].first == rn) dp[rn] = (int32_t)(rp[rn] + q);
+    parallel_for(nb, [&](int64_t lo, int64_t hi) {
+      std::vector<std::pair<int32_t, int32_t>> tmp;
+      for (int64_t p = lo; p < hi; ++p) {
+        const int32_t k = order[p];
+        const int64_t b0 = ctx->rowptr[bs[k]];
+        const int64_t len = ctx->rowptr[bs[k] + 1] - b0;
+        tmp.resize(len);
+        for (int64_t q = 0; q < len; ++q) tmp[q] = {iperm[ctx->colidx[b0 + q]], (int32_t)q};
+        std::sort(tmp.begin(), tmp.end());
+        for (int32_t a = 0; a < bs[k + 1] - bs[k]; ++a) {
+          const int32_t rn = nbs[p] + a;
+          const int64_t ob = ctx->rowptr[bs[k] + a];
+          for (int64_t q = 0; q < len; ++q) {
+            ci[rp[rn] + q] = tmp[q].first;
+            pmap[rp[rn] + q] = (int32_t)(ob + tmp[q].second);
+            if (tmp[q].first == rn) dp[rn] = (int32_t)(rp[rn] + q);
+          }
         }
       }
+    });
+    if (reorder == 2) {
+      // one level per colour: blocks of a colour are independent and contiguous in the
+      // new order; the backward solve walks the colours in reverse
+      P.contiguous = true;
+      P.levLPtr = colorPtr;
+      P.levUPtr = colorPtr;
+      P.h_levL.clear();
+      P.h_levU.clear();
+    } else {
+      build_levels(rp, ci, nbs, P);
     }
-    build_levels(rp, ci, nbs, P);
+    {
+      std::vector<BlkDesc> dd = make_desc(nbs, rp, dp);
+      hipError_t e = alloc(&P.d_desc, dd.size());
+      if (e == hipSuccess) e = hipMemcpy(P.d_desc, dd.data(), dd.size() * sizeof(BlkDesc), hipMemcpyHostToDevice);
+      if (e != hipSuccess) return set_hip_error(e, "ILU descriptors");
+    }
     std::vector<int32_t> rp32(rp.begin(), rp.end());
     int rc;
     if ((rc = upload(&P.d_rowptr, rp32)) || (rc = upload(&P.d_colidx, ci)) || (rc = upload(&P.d_diagpos, dp)) ||
@@ -810,6 +943,23 @@ static int ensure_vec(T** p, int64_t n) {
   hipError_t e = alloc(p, (size_t)n);
   if (e != hipSuccess) return set_hip_error(e, "hipMalloc krylov vector");
   return SFEM_OK;
+}
+
+// Blocks of level lv of the forward (upper=false) or backward (upper=true) sweep: a list
+// slice, or with a contiguous plan the block range of colour lv (reversed for backward).
+static void level_range(const IluPlan& P, bool upper, int lv, const int32_t** list, int* first, int* count) {
+  if (P.contiguous) {
+    const int nl = (int)P.levLPtr.size() - 1;
+    const int c = upper ? nl - 1 - lv : lv;
+    *list = nullptr;
+    *first = P.levLPtr[c];
+    *count = P.levLPtr[c + 1] - P.levLPtr[c];
+  } else {
+    const std::vector<int32_t>& ptr = upper ? P.levUPtr : P.levLPtr;
+    *list = upper ? P.d_levU : P.d_levL;
+    *first = ptr[lv];
+    *count = ptr[lv + 1] - ptr[lv];
+  }
 }
 
 // Build the preconditioner from the current matrix (FP64 vals or FP32 vals32).
@@ -874,13 +1024,15 @@ int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder) {
     }
     const int nl = (int)P.levLPtr.size() - 1;
     for (int lv = 0; lv < nl; ++lv) {
-      const int cnt = P.levLPtr[lv + 1] - P.levLPtr[lv];
+      const int32_t* list; int first, cnt;
+      level_range(P, false, lv, &list, &first, &cnt);
+      if (!cnt) continue;
       if (fp32)
-        hipLaunchKernelGGL((k_ilu_factor_level<float>), dim3(cnt), dim3(64), 0, st, P.d_levL + P.levLPtr[lv],
-                           cnt, P.blkStart, P.rowptr, P.colidx, P.diagpos, L->lu32, L->d_scal);
+        hipLaunchKernelGGL((k_ilu_factor_blk<float>), dim3(cnt), dim3(64), 0, st, list, first, cnt,
+                           P.blkStart, P.rowptr, P.colidx, P.diagpos, L->lu32, L->d_scal);
       else
-        hipLaunchKernelGGL((k_ilu_factor_level<double>), dim3(cnt), dim3(64), 0, st, P.d_levL + P.levLPtr[lv],
-                           cnt, P.blkStart, P.rowptr, P.colidx, P.diagpos, L->lu, L->d_scal);
+        hipLaunchKernelGGL((k_ilu_factor_blk<double>), dim3(cnt), dim3(64), 0, st, list, first, cnt,
+                           P.blkStart, P.rowptr, P.colidx, P.diagpos, L->lu, L->d_scal);
     }
     LA_CHECK(hipGetLastError());
   }
@@ -898,21 +1050,39 @@ static int pc_apply(sfem_ctx* ctx, const T* r, T* z) {
   return rc;
 }
 
-template <typename T, typename LT>
-static void ilu_solve(sfem_ctx* ctx, const IluPlan& P, const LT* lu, const T* r, T* z) {
+template <typename T, typename LT, int G>
+static void ilu_solve_g(sfem_ctx* ctx, const IluPlan& P, const LT* lu, const int32_t* perm, const T* r, T* y,
+                        T* zout) {
   hipStream_t st = ctx->stream;
+  const BlkDesc* desc = P.identity ? ctx->la->d_desc : P.d_desc;
   const int nl = (int)P.levLPtr.size() - 1;
   for (int lv = 0; lv < nl; ++lv) {
-    const int cnt = P.levLPtr[lv + 1] - P.levLPtr[lv];
-    hipLaunchKernelGGL((k_ilu_lsolve_level<T, LT>), dim3(cnt), dim3(64), 0, st, P.d_levL + P.levLPtr[lv], cnt,
-                       P.blkStart, P.rowptr, P.colidx, lu, r, z);
+    const int32_t* list; int first, cnt;
+    level_range(P, false, lv, &list, &first, &cnt);
+    if (!cnt) continue;
+    const unsigned gr = (unsigned)(((int64_t)cnt * G + 255) / 256);
+    hipLaunchKernelGGL((k_ilu_lsolve_blk<T, LT, G>), dim3(gr), dim3(256), 0, st, list, first, cnt, desc,
+                       P.colidx, lu, perm, r, y);
   }
   const int nu = (int)P.levUPtr.size() - 1;
   for (int lv = 0; lv < nu; ++lv) {
-    const int cnt = P.levUPtr[lv + 1] - P.levUPtr[lv];
-    hipLaunchKernelGGL((k_ilu_usolve_level<T, LT>), dim3(cnt), dim3(64), 0, st, P.d_levU + P.levUPtr[lv], cnt,
-                       P.blkStart, P.rowptr, P.colidx, P.diagpos, lu, z);
+    const int32_t* list; int first, cnt;
+    level_range(P, true, lv, &list, &first, &cnt);
+    if (!cnt) continue;
+    const unsigned gr = (unsigned)(((int64_t)cnt * G + 255) / 256);
+    hipLaunchKernelGGL((k_ilu_usolve_blk<T, LT, G>), dim3(gr), dim3(256), 0, st, list, first, cnt, desc,
+                       P.colidx, lu, perm, y, zout);
   }
+}
+
+// Apply the ILU(0) preconditioner: z = U^{-1} L^{-1} r.  With a permuted plan the
+// forward solve reads r through the permutation and the backward solve writes z back
+// through it; y is the permuted work vector.  With the identity plan y is z itself.
+template <typename T, typename LT>
+static void ilu_solve(sfem_ctx* ctx, const IluPlan& P, const LT* lu, const T* r, T* y, T* z) {
+  const int32_t* perm = P.identity ? nullptr : P.d_perm;
+  T* yy = P.identity ? z : y;
+  ilu_solve_g<T, LT, 16>(ctx, P, lu, perm, r, yy, z);
 }
 
 template <typename T>
@@ -935,25 +1105,19 @@ static int pc_apply_impl(sfem_ctx* ctx, const T* r, T* z) {
     }
     case SFEM_PC_ILU0: {
       const IluPlan& P = *L->ilu;
-      const T* rin = r;
-      T* zout = z;
-      T *pr = nullptr, *pz = nullptr;
+      T* y = nullptr;
       if (!P.identity) {
         int rc;
         if (sizeof(T) == 8) {
-          if ((rc = ensure_vec(&L->pr, n)) || (rc = ensure_vec(&L->pz, n))) return rc;
-          pr = (T*)L->pr; pz = (T*)L->pz;
+          if ((rc = ensure_vec(&L->pz, n))) return rc;
+          y = (T*)L->pz;
         } else {
-          if ((rc = ensure_vec(&L->pr32, n)) || (rc = ensure_vec(&L->pz32, n))) return rc;
-          pr = (T*)L->pr32; pz = (T*)L->pz32;
+          if ((rc = ensure_vec(&L->pz32, n))) return rc;
+          y = (T*)L->pz32;
         }
-        hipLaunchKernelGGL((k_gather<T>), dim3(g), dim3(kTPB), 0, st, n, P.d_perm, r, pr);
-        rin = pr;
-        zout = pz;
       }
-      if (L->pc_fp32) ilu_solve<T, float>(ctx, P, L->lu32, rin, zout);
-      else ilu_solve<T, double>(ctx, P, L->lu, rin, zout);
-      if (!P.identity) hipLaunchKernelGGL((k_scatter<T>), dim3(g), dim3(kTPB), 0, st, n, P.d_perm, pz, z);
+      if (L->pc_fp32) ilu_solve<T, float>(ctx, P, L->lu32, r, y, z);
+      else ilu_solve<T, double>(ctx, P, L->lu, r, y, z);
       break;
     }
     default:
@@ -965,35 +1129,27 @@ static int pc_apply_impl(sfem_ctx* ctx, const T* r, T* z) {
 template <typename T>
 static void spmv(sfem_ctx* ctx, const T* x, T* y) {
   LinAlg* L = ctx->la;
-  const int64_t n = L->n;
-  const int sub = L->spmv_sub;
-  const int64_t threads = n * sub;
-  const unsigned g = (unsigned)((threads + kTPB - 1) / kTPB);
-  if (n == 0) return;
+  const int64_t nb = L->nblk;
+  if (nb == 0) return;
   ctx->kt.begin(SFEM_KT_SPMV, ctx->stream);
-#define SPMV_CASE(S)                                                                          \
-  case S:                                                                                     \
-    if (sizeof(T) == 8)                                                                       \
-      hipLaunchKernelGGL((k_spmv<double, T, S>), dim3(g), dim3(kTPB), 0, ctx->stream, n,      \
-                         ctx->d_rowptr32, ctx->d_colidx, ctx->d_vals, x, y);                  \
-    else                                                                                      \
-      hipLaunchKernelGGL((k_spmv<float, T, S>), dim3(g), dim3(kTPB), 0, ctx->stream, n,       \
-                         ctx->d_rowptr32, ctx->d_colidx, ctx->d_vals32, x, y);                \
-    break;
-  switch (sub) {
-    SPMV_CASE(4)
-    SPMV_CASE(8)
-    SPMV_CASE(16)
-    SPMV_CASE(32)
-    default:
-      if (sizeof(T) == 8)
-        hipLaunchKernelGGL((k_spmv<double, T, 64>), dim3(g), dim3(kTPB), 0, ctx->stream, n,
-                           ctx->d_rowptr32, ctx->d_colidx, ctx->d_vals, x, y);
-      else
-        hipLaunchKernelGGL((k_spmv<float, T, 64>), dim3(g), dim3(kTPB), 0, ctx->stream, n,
-                           ctx->d_rowptr32, ctx->d_colidx, ctx->d_vals32, x, y);
+  const bool wide = L->maxrow > 64;
+  const int G = wide ? 16 : 8;
+  const unsigned g = (unsigned)((nb * G + 255) / 256);
+  if (sizeof(T) == 8) {
+    if (wide)
+      hipLaunchKernelGGL((k_spmv_blk<double, T, 16>), dim3(g), dim3(256), 0, ctx->stream, nb, L->d_desc,
+                         ctx->d_colidx, ctx->d_vals, x, y);
+    else
+      hipLaunchKernelGGL((k_spmv_blk<double, T, 8>), dim3(g), dim3(256), 0, ctx->stream, nb, L->d_desc,
+                         ctx->d_colidx, ctx->d_vals, x, y);
+  } else {
+    if (wide)
+      hipLaunchKernelGGL((k_spmv_blk<float, T, 16>), dim3(g), dim3(256), 0, ctx->stream, nb, L->d_desc,
+                         ctx->d_colidx, ctx->d_vals32, x, y);
+    else
+      hipLaunchKernelGGL((k_spmv_blk<float, T, 8>), dim3(g), dim3(256), 0, ctx->stream, nb, L->d_desc,
+                         ctx->d_colidx, ctx->d_vals32, x, y);
   }
-#undef SPMV_CASE
   ctx->kt.end(SFEM_KT_SPMV, ctx->stream);
 }
 
@@ -1210,6 +1366,16 @@ static int gmres(sfem_ctx* ctx, const T* b, T* x, int m, double rtol, int maxit,
   // the true residual may sit a few ulps above the recurrence's estimate
   if (rnorm > rtol * bnorm * 1.001) return set_error(SFEM_ERR_MAXIT, "GMRES: maximum iterations reached");
   return SFEM_OK;
+}
+
+// Symbolic set-up of the preconditioner for the given options (ILU(0): ordering,
+// permuted pattern and level sets), so that the first solve does not pay for it.
+int la_prepare(sfem_ctx* ctx, const sfem_solver_opts* o) {
+  LinAlg* L = ctx->la;
+  if (o->pc_type != SFEM_PC_ILU0) return SFEM_OK;
+  if (L->maxrow > kIluMaxRow) return set_error(SFEM_ERR_UNSUPPORTED, "ILU(0): row longer than 256 entries");
+  if (L->ilu && L->ilu->reorder == o->reorder) return SFEM_OK;
+  return build_ilu_plan(ctx, L, o->reorder);
 }
 
 // Solve A du = rhs.  FP64 mode: everything in double.  Mixed mode: FP32

@@ -18,9 +18,13 @@ int set_hip_error(hipError_t e, const char* what);
 // ILU(0) plan for one node ordering: the (possibly permuted) pattern the factor lives
 // on and its level sets.  With the identity ordering the pattern pointers alias the
 // context's CSR.
+struct BlkDesc;
+
 struct IluPlan {
   int reorder = -1;
   bool identity = true;
+  bool contiguous = false;   // level l is the block range [levPtr[l], levPtr[l+1])
+  BlkDesc* d_desc = nullptr;
   std::vector<int32_t> levLPtr, levUPtr, h_levL, h_levU;
   int32_t *d_levL = nullptr, *d_levU = nullptr;
   const int32_t *rowptr = nullptr, *colidx = nullptr, *diagpos = nullptr, *blkStart = nullptr;
@@ -42,6 +46,7 @@ struct LinAlg {
   int pc_reorder = 0;
   IluPlan* ilu = nullptr;
   int32_t *d_blkStart = nullptr, *d_diagpos = nullptr;
+  BlkDesc* d_desc = nullptr;   // per node block, original order
   double* d_part = nullptr;  // reduction partials
   double* d_scal = nullptr;  // Krylov scalars
   double* h_scal = nullptr;  // pinned mirror
@@ -69,6 +74,7 @@ int la_create(sfem_ctx* ctx);
 void la_destroy(sfem_ctx* ctx);
 int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder);
 int la_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int* iters, double* relres);
+int la_prepare(sfem_ctx* ctx, const sfem_solver_opts* o);
 int la_norm2(sfem_ctx* ctx, const double* a, double* out);
 int la_spmv64(sfem_ctx* ctx, const double* x, double* y);
 int la_spmv32(sfem_ctx* ctx, const float* x, float* y);

@@ -63,6 +63,7 @@ ABI = [
     ("sfem_element_KF", C.c_int, [_vp, C.c_int64, _f64p, _f64p]),
     ("sfem_get_system", C.c_int, [_vp, _f64p, _f64p]),
     ("sfem_solve", C.c_int, [_vp, C.POINTER(SolverOpts), C.POINTER(C.c_int), _f64p]),
+    ("sfem_setup_solver", C.c_int, [_vp, C.POINTER(SolverOpts)]),
     ("sfem_update_solution", C.c_int, [_vp]),
     ("sfem_step", C.c_int, [_vp, C.c_double, C.POINTER(SolverOpts), C.c_double, C.c_int,
                             C.POINTER(StepStats)]),
@@ -208,6 +209,9 @@ class Context:
         _check(rc, "sfem_solve")
         return it.value, rr.value
 
+    def setup_solver(self, opts: SolverOpts):
+        _check(self.lib.sfem_setup_solver(self.h, C.byref(opts)), "sfem_setup_solver")
+
     def update_solution(self):
         _check(self.lib.sfem_update_solution(self.h), "sfem_update_solution")
 
@@ -319,6 +323,7 @@ class StabFEM:
         c = self.ctl
         self.ctx.set_params(c.elemData, c.tis, c.rhoInf, c.dt)
         self.ctx.set_precision(self.precision)
+        self.ctx.setup_solver(self.opts)
 
     # StabFEM::solveFullyImplicit (StabFEM2.cpp:593-832)
     def solveFullyImplicit(self, steps: Optional[int] = None):

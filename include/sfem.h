@@ -135,6 +135,12 @@ int sfem_get_system(sfem_ctx* ctx, double* vals, double* rhs);
  * initial guess as VecZeroEntries at :249). */
 int sfem_solve(sfem_ctx* ctx, const sfem_solver_opts* opts, int* iters, double* relres);
 
+/* Symbolic set-up of the solver for these options ahead of the time loop (the
+ * ILU(0) ordering, its permuted pattern and level sets) -- the part of
+ * PCSetUp that depends only on the pattern (SolverPetsc::initialise,
+ * SolverPetsc.cpp:95-111).  Optional: sfem_solve does it on first use. */
+int sfem_setup_solver(sfem_ctx* ctx, const sfem_solver_opts* opts);
+
 /* soln[assyForSoln[i]] += du[i] (StabFEM2.cpp:785-788). */
 int sfem_update_solution(sfem_ctx* ctx);
 
