@@ -49,7 +49,7 @@ def make_case(n):
     return mesh, ctl
 
 
-def run_path(mesh, ctl, prec, opts, newton_tol, steps, warmup, label):
+def run_path(mesh, ctl, prec, opts, newton_tol, steps, warmup, label, predictor=0):
     from cfdsfemmpi_amd import sfem
     t0 = time.time()
     fem = sfem.StabFEM(device=0)
@@ -59,6 +59,7 @@ def run_path(mesh, ctl, prec, opts, newton_tol, steps, warmup, label):
     fem.prepareInputData()
     fem.setSolver(opts)
     ctx = fem.ctx
+    ctx.set_predictor(predictor)
     info = ctx.info()
     log(f"{label}: setup {time.time() - t0:.1f} s, nfree={info['nfree']} nnz={info['nnz']} colours={info['ncolors']}")
 
@@ -84,12 +85,14 @@ def run_path(mesh, ctl, prec, opts, newton_tol, steps, warmup, label):
     hist = advance(steps, True)
     s = ctx.get_solution()
     t2 = time.perf_counter()
+    tune = ctx.tuning()
+    log(f"{label}: autotuned kernel shapes {tune}")
     kt = {name: ctx.kernel_timing(k) for name, k in
           (("assembly", sfem.KT_ASSEMBLY), ("spmv", sfem.KT_SPMV), ("pc", sfem.KT_PC))}
     log(f"{label}: {steps} steps in {t2 - t1:.3f} s; device ms by class: " +
         ", ".join(f"{k} {v[1]:.1f} ({v[0]} launches)" for k, v in kt.items()))
     res = dict(label=label, seconds=t2 - t1, steps_per_s=steps / (t2 - t1), hist=hist, kt=kt, info=info,
-               soln=s)
+               soln=s, tune=tune)
     ctx.close()
     return res
 
@@ -148,6 +151,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None)
     ap.add_argument("--reorder", type=int, default=2, help="ILU(0) node ordering: 0 natural, 1 RCM, 2 multicolour")
+    ap.add_argument("--bf16-factors", type=int, default=0, help="mixed path: ILU(0) factors stored in bfloat16")
+    ap.add_argument("--predictor", choices=["previous", "extrapolate"], default="previous",
+                    help="Newton initial iterate of each step (both paths)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the multi-GPU layout is described "
@@ -159,13 +165,14 @@ def main():
     mesh, ctl = make_case(n)
     log(f"workload {n}^3 hexes: {mesh.nElem} elements, {mesh.nNode} nodes")
     o64 = sfem.solver_opts("bcgs", "ilu", 1e-8, 5000, reorder=a.reorder)
-    omx = sfem.solver_opts("bcgs", "ilu", a.inner_rtol, 5000, reorder=a.reorder)
+    omx = sfem.solver_opts("bcgs", "ilu", a.inner_rtol, 5000, reorder=a.reorder, factor_bf16=a.bf16_factors)
+    pred = sfem.PRED_EXTRAPOLATE if a.predictor == "extrapolate" else sfem.PRED_PREVIOUS
     r64 = rmx = None
     if a.only != "mixed":
-        r64 = run_path(mesh, ctl, sfem.PREC_FP64, o64, a.newton_tol, a.steps, a.warmup, "fp64")
+        r64 = run_path(mesh, ctl, sfem.PREC_FP64, o64, a.newton_tol, a.steps, a.warmup, "fp64", pred)
         log(f"fp64: {r64['steps_per_s']:.3f} steps/s")
     if a.only != "fp64":
-        rmx = run_path(mesh, ctl, sfem.PREC_MIXED, omx, a.newton_tol, a.steps, a.warmup, "mixed")
+        rmx = run_path(mesh, ctl, sfem.PREC_MIXED, omx, a.newton_tol, a.steps, a.warmup, "mixed", pred)
         log(f"mixed: {rmx['steps_per_s']:.3f} steps/s")
     main_res = rmx or r64
     err_u = err_p = None
@@ -222,13 +229,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 tangent+Krylov / fp64 residual" if fp32 else "f64",
+        "dtype": ("fp32 tangent+Krylov" + (" (bf16 ILU factors)" if a.bf16_factors else "") + " / fp64 residual")
+                 if fp32 else "f64",
         "data": "synthetic (generated mesh, zero initial state)",
         "config": {"workload": f"3D lid-driven cavity, {n}^3 graded Hexa8, Re=1000, gen-alpha rho_inf=0, "
                                f"dt=0.1, {a.steps} steps", "n_elements": mesh.nElem, "nfree": nf, "nnz": nnz,
                    "solver": "BiCGStab + ILU(0)", "ilu_ordering": ["natural", "rcm", "multicolour"][a.reorder],
                    "newton_tol": a.newton_tol,
-                   "inner_rtol_mixed": a.inner_rtol, "parallelism": "single GPU"},
+                   "inner_rtol_mixed": a.inner_rtol, "ilu_factors_mixed": "bf16" if a.bf16_factors else "fp32",
+                   "newton_initial_iterate": a.predictor, "parallelism": "single GPU"},
         "fp64_steps_per_s": r64["steps_per_s"] if r64 else None,
         "mixed_steps_per_s": rmx["steps_per_s"] if rmx else None,
         "speedup_mixed_vs_fp64": (rmx["steps_per_s"] / r64["steps_per_s"]) if (r64 and rmx) else None,
@@ -238,6 +247,7 @@ def main():
         "newton_iters": {r["label"]: [h[0] for h in r["hist"]] for r in (r64, rmx) if r},
         "kernel_ms": {r["label"]: {k: {"launches": v[0], "total_ms": v[1]} for k, v in r["kt"].items()}
                       for r in (r64, rmx) if r},
+        "autotuned": {r["label"]: r["tune"] for r in (r64, rmx) if r},
         "roofline": roof,
         "roofline_ilu_apply": roof_pc,
         "roofline_assembly": roof_asm,

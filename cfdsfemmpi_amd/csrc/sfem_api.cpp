@@ -481,6 +481,13 @@ int sfem_set_precision(sfem_ctx* c, int precision) {
   return SFEM_OK;
 }
 
+int sfem_set_predictor(sfem_ctx* c, int predictor) {
+  if (!c) return set_error(SFEM_ERR_ARG, "null context");
+  if (predictor != SFEM_PRED_PREVIOUS && predictor != SFEM_PRED_EXTRAPOLATE) return set_error(SFEM_ERR_ARG, "bad predictor");
+  c->predictor = predictor;
+  return SFEM_OK;
+}
+
 int sfem_time_update(sfem_ctx* c, double timeFact) {
   if (!c) return set_error(SFEM_ERR_ARG, "null context");
   if (!(c->status & ST_PARAMS)) return set_error(SFEM_ERR_STATE, "parameters not set");
@@ -489,6 +496,8 @@ int sfem_time_update(sfem_ctx* c, double timeFact) {
   const int64_t n = nstate(c);
   HIP_OK(launch_time_update(c->stream, n, c->d_soln, c->d_prev, c->d_prev2, c->d_prev3, c->d_prev4, c->d_dot,
                             c->d_dotPrev));
+  if (c->predictor == SFEM_PRED_EXTRAPOLATE)
+    HIP_OK(launch_extrapolate(c->stream, n, c->d_prev, c->d_prev2, c->d_soln));
   HIP_OK(launch_assign_bc(c->stream, (int64_t)c->dbc_ind.size(), c->d_dbc_ind, c->d_dbc_val, timeFact, c->d_soln,
                           c->d_applied));
   return SFEM_OK;
@@ -720,6 +729,11 @@ int sfem_kernel_timing(sfem_ctx* c, int kind, int64_t* launches, double* total_m
   if (launches) *launches = c->kt.count[kind];
   if (total_ms) *total_ms = c->kt.ms[kind];
   return SFEM_OK;
+}
+
+int sfem_tuning(sfem_ctx* c, int* out) {
+  if (!c || !out) return set_error(SFEM_ERR_ARG, "null argument");
+  return la_tuning(c, out);
 }
 
 int sfem_info(sfem_ctx* c, int64_t* nfree, int64_t* nnz, int* ncolors, int* ndof) {

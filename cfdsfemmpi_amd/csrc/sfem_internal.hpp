@@ -143,6 +143,7 @@ struct sfem_ctx {
   sfem::ElemDims dims{};
   int64_t nNode = 0, nElem = 0;
   int precision = SFEM_PREC_FP64;
+  int predictor = SFEM_PRED_PREVIOUS;
   int status = 0;  // bit flags: 1 prepared, 2 pattern, 4 params, 8 assembled
 
   // host mesh

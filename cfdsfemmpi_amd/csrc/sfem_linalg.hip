@@ -28,6 +28,26 @@ namespace sfem {
 constexpr int kRedBlocks = 1024;  // partials per reduction
 constexpr int kTPB = 256;
 
+// bfloat16 storage for the mixed path's ILU(0) factors: the top 16 bits of an IEEE
+// float, rounded to nearest even.  Loads widen it back to float exactly.
+struct bf16 {
+  uint16_t bits;
+};
+
+template <typename AT>
+__device__ inline AT widen(double v) { return (AT)v; }
+template <typename AT>
+__device__ inline AT widen(float v) { return (AT)v; }
+template <typename AT>
+__device__ inline AT widen(bf16 v) { return (AT)__uint_as_float(((uint32_t)v.bits) << 16); }
+
+__global__ __launch_bounds__(256) void k_to_bf16(int64_t n, const float* __restrict__ a, bf16* __restrict__ b) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const uint32_t u = __float_as_uint(a[i]);
+    b[i].bits = (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+  }
+}
+
 // scalar slots
 enum {
   S_RHO0 = 0, S_RHO1, S_ALPHA, S_OMEGA, S_BETA, S_RV, S_SS, S_TT, S_TS, S_RR, S_BB, S_TOL2,
@@ -433,7 +453,7 @@ __device__ inline void group_dot(const int32_t* __restrict__ col, const VT* __re
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
-      for (int u = 0; u < U; ++u) v[a][u] = (a < m) ? (AT)V[(int64_t)a * rl + q + u * G] : (AT)0;
+      for (int u = 0; u < U; ++u) v[a][u] = (a < m) ? widen<AT>(V[(int64_t)a * rl + q + u * G]) : (AT)0;
     AT xv[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) xv[u] = (AT)x[c[u]];
@@ -446,14 +466,14 @@ __device__ inline void group_dot(const int32_t* __restrict__ col, const VT* __re
     const AT xv = (AT)x[col[q]];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
-      if (a < m) acc[a] += (AT)V[(int64_t)a * rl + q] * xv;
+      if (a < m) acc[a] += widen<AT>(V[(int64_t)a * rl + q]) * xv;
   }
 }
 
 // Node-grouped CSR SpMV.  The free rows of one node (a block of m <= 4 rows) share one
 // column list, so G lanes walk that list once, load each column index and x entry once,
 // and accumulate the m rows together; values of row a sit at base + a*rl + q.
-template <typename VT, typename XT, int G>
+template <typename VT, typename XT, int G, int U>
 __global__ __launch_bounds__(256) void k_spmv_blk(int64_t nb, const BlkDesc* __restrict__ desc,
                                                   const int32_t* __restrict__ colidx,
                                                   const VT* __restrict__ vals, const XT* __restrict__ x,
@@ -466,7 +486,7 @@ __global__ __launch_bounds__(256) void k_spmv_blk(int64_t nb, const BlkDesc* __r
   BlkDesc d = {0, 0, 0, 0};
   if (b < nb) {
     d = desc[b];
-    group_dot<G, 4>(colidx + d.base, vals + d.base, x, d.rl, d.dqm >> 16, 0, d.rl, lane, acc);
+    group_dot<G, U>(colidx + d.base, vals + d.base, x, d.rl, d.dqm >> 16, 0, d.rl, lane, acc);
   }
 #pragma unroll
   for (int a = 0; a < 4; ++a) acc[a] = group_sum<G>(acc[a]);
@@ -477,14 +497,17 @@ __global__ __launch_bounds__(256) void k_spmv_blk(int64_t nb, const BlkDesc* __r
 // as a grouped sparse dot product, then the unit-lower m x m diagonal block, whose
 // entries and right-hand sides are loaded by separate lanes and shared by shuffles.
 // Level blocks are either the contiguous range [first, first + count) or list[first + g].
-template <typename T, typename LT, int G>
+template <typename LT> struct AccOf { using type = float; };
+template <> struct AccOf<double> { using type = double; };
+
+template <typename T, typename LT, int G, int U>
 __global__ __launch_bounds__(256) void k_ilu_lsolve_blk(const int32_t* __restrict__ list, int first, int count,
                                                         const BlkDesc* __restrict__ desc,
                                                         const int32_t* __restrict__ colidx,
                                                         const LT* __restrict__ lu,
                                                         const int32_t* __restrict__ perm,
                                                         const T* __restrict__ r, T* __restrict__ y) {
-  using AT = typename std::conditional<sizeof(LT) == 8, double, float>::type;
+  using AT = typename AccOf<LT>::type;
   const int64_t gt = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int64_t g = gt / G;
   const int lane = (int)(gt % G);
@@ -497,9 +520,9 @@ __global__ __launch_bounds__(256) void k_ilu_lsolve_blk(const int32_t* __restric
     const int m = d.dqm >> 16, dq0 = d.dqm & 0xffff;
     // diagonal block entry (lane / 4, lane % 4) and right-hand side of row `lane`
     if (lane < 16 && (lane >> 2) < m && (lane & 3) < m)
-      dv = (AT)lu[d.base + (int64_t)(lane >> 2) * d.rl + dq0 + (lane & 3)];
+      dv = widen<AT>(lu[d.base + (int64_t)(lane >> 2) * d.rl + dq0 + (lane & 3)]);
     if (lane < m) rv = (AT)r[perm ? perm[d.r0 + lane] : d.r0 + lane];
-    group_dot<G, 2>(colidx + d.base, lu + d.base, y, d.rl, m, 0, dq0, lane, acc);
+    group_dot<G, U>(colidx + d.base, lu + d.base, y, d.rl, m, 0, dq0, lane, acc);
   }
 #pragma unroll
   for (int a = 0; a < 4; ++a) acc[a] = group_sum<G>(acc[a]);
@@ -522,14 +545,14 @@ __global__ __launch_bounds__(256) void k_ilu_lsolve_blk(const int32_t* __restric
 
 // Backward solve U z = y over one level (levels in reverse order), in place on y
 // (permuted order); the result also goes to zout through perm when perm is non-null.
-template <typename T, typename LT, int G>
+template <typename T, typename LT, int G, int U>
 __global__ __launch_bounds__(256) void k_ilu_usolve_blk(const int32_t* __restrict__ list, int first, int count,
                                                         const BlkDesc* __restrict__ desc,
                                                         const int32_t* __restrict__ colidx,
                                                         const LT* __restrict__ lu,
                                                         const int32_t* __restrict__ perm,
                                                         T* __restrict__ y, T* __restrict__ zout) {
-  using AT = typename std::conditional<sizeof(LT) == 8, double, float>::type;
+  using AT = typename AccOf<LT>::type;
   const int64_t gt = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int64_t g = gt / G;
   const int lane = (int)(gt % G);
@@ -541,9 +564,9 @@ __global__ __launch_bounds__(256) void k_ilu_usolve_blk(const int32_t* __restric
     d = desc[list ? list[first + g] : first + (int32_t)g];
     const int m = d.dqm >> 16, dq0 = d.dqm & 0xffff;
     if (lane < 16 && (lane >> 2) < m && (lane & 3) < m)
-      dv = (AT)lu[d.base + (int64_t)(lane >> 2) * d.rl + dq0 + (lane & 3)];
+      dv = widen<AT>(lu[d.base + (int64_t)(lane >> 2) * d.rl + dq0 + (lane & 3)]);
     if (lane < m) rv = (AT)y[d.r0 + lane];
-    group_dot<G, 2>(colidx + d.base, lu + d.base, y, d.rl, m, dq0 + m, d.rl, lane, acc);
+    group_dot<G, U>(colidx + d.base, lu + d.base, y, d.rl, m, dq0 + m, d.rl, lane, acc);
   }
 #pragma unroll
   for (int a = 0; a < 4; ++a) acc[a] = group_sum<G>(acc[a]);
@@ -597,7 +620,7 @@ static hipError_t alloc(T** p, size_t n) {
 LinAlg::~LinAlg() {
   void* ptrs[] = {d_blkStart, d_diagpos, d_part, d_scal, r, rh, p, v, s, t, ph, sh, x, w, z, pr, pz,
                   r32, rh32, p32, v32, s32, t32, ph32, sh32, x32, du32, w32, z32, pr32, pz32,
-                  dinv, dinv32, binv, binv32, lu, lu32, gm_V, gm_V32, d_desc};
+                  dinv, dinv32, binv, binv32, lu, lu32, gm_V, gm_V32, d_desc, lu16};
   delete ilu;
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
@@ -648,11 +671,6 @@ int la_create(sfem_ctx* ctx) {
   LA_CHECK(alloc(&L->d_part, (size_t)kRedBlocks * 4));
   LA_CHECK(alloc(&L->d_scal, S_NSLOTS));
   LA_CHECK(hipHostMalloc((void**)&L->h_scal, S_NSLOTS * sizeof(double), hipHostMallocDefault));
-  // SpMV lanes per row from the mean row length
-  const double avg = n ? (double)ctx->nnz / (double)n : 1.0;
-  int sub = 4;
-  while (sub < 64 && sub * 2 <= avg) sub *= 2;
-  L->spmv_sub = sub;
   return SFEM_OK;
 }
 
@@ -963,7 +981,7 @@ static void level_range(const IluPlan& P, bool upper, int lv, const int32_t** li
 }
 
 // Build the preconditioner from the current matrix (FP64 vals or FP32 vals32).
-int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder) {
+int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f) {
   LinAlg* L = ctx->la;
   hipStream_t st = ctx->stream;
   const int64_t n = L->n;
@@ -1034,6 +1052,11 @@ int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder) {
         hipLaunchKernelGGL((k_ilu_factor_blk<double>), dim3(cnt), dim3(64), 0, st, list, first, cnt,
                            P.blkStart, P.rowptr, P.colidx, P.diagpos, L->lu, L->d_scal);
     }
+    L->pc_bf16 = fp32 && bf16f;
+    if (L->pc_bf16) {
+      int rc = ensure_vec(&L->lu16, nnz); if (rc) return rc;
+      hipLaunchKernelGGL(k_to_bf16, dim3(gz), dim3(256), 0, st, nnz, L->lu32, L->lu16);
+    }
     LA_CHECK(hipGetLastError());
   }
   return SFEM_OK;
@@ -1050,7 +1073,7 @@ static int pc_apply(sfem_ctx* ctx, const T* r, T* z) {
   return rc;
 }
 
-template <typename T, typename LT, int G>
+template <typename T, typename LT, int G, int U>
 static void ilu_solve_g(sfem_ctx* ctx, const IluPlan& P, const LT* lu, const int32_t* perm, const T* r, T* y,
                         T* zout) {
   hipStream_t st = ctx->stream;
@@ -1061,7 +1084,7 @@ static void ilu_solve_g(sfem_ctx* ctx, const IluPlan& P, const LT* lu, const int
     level_range(P, false, lv, &list, &first, &cnt);
     if (!cnt) continue;
     const unsigned gr = (unsigned)(((int64_t)cnt * G + 255) / 256);
-    hipLaunchKernelGGL((k_ilu_lsolve_blk<T, LT, G>), dim3(gr), dim3(256), 0, st, list, first, cnt, desc,
+    hipLaunchKernelGGL((k_ilu_lsolve_blk<T, LT, G, U>), dim3(gr), dim3(256), 0, st, list, first, cnt, desc,
                        P.colidx, lu, perm, r, y);
   }
   const int nu = (int)P.levUPtr.size() - 1;
@@ -1070,7 +1093,7 @@ static void ilu_solve_g(sfem_ctx* ctx, const IluPlan& P, const LT* lu, const int
     level_range(P, true, lv, &list, &first, &cnt);
     if (!cnt) continue;
     const unsigned gr = (unsigned)(((int64_t)cnt * G + 255) / 256);
-    hipLaunchKernelGGL((k_ilu_usolve_blk<T, LT, G>), dim3(gr), dim3(256), 0, st, list, first, cnt, desc,
+    hipLaunchKernelGGL((k_ilu_usolve_blk<T, LT, G, U>), dim3(gr), dim3(256), 0, st, list, first, cnt, desc,
                        P.colidx, lu, perm, y, zout);
   }
 }
@@ -1082,7 +1105,15 @@ template <typename T, typename LT>
 static void ilu_solve(sfem_ctx* ctx, const IluPlan& P, const LT* lu, const T* r, T* y, T* z) {
   const int32_t* perm = P.identity ? nullptr : P.d_perm;
   T* yy = P.identity ? z : y;
-  ilu_solve_g<T, LT, 16>(ctx, P, lu, perm, r, yy, z);
+  const int v = ctx->la->tune_ilu[sizeof(T) == 8 ? 0 : 1];
+  switch (v) {
+    case 0: ilu_solve_g<T, LT, 16, 1>(ctx, P, lu, perm, r, yy, z); break;
+    case 1: ilu_solve_g<T, LT, 16, 2>(ctx, P, lu, perm, r, yy, z); break;
+    case 2: ilu_solve_g<T, LT, 16, 4>(ctx, P, lu, perm, r, yy, z); break;
+    case 3: ilu_solve_g<T, LT, 32, 1>(ctx, P, lu, perm, r, yy, z); break;
+    case 4: ilu_solve_g<T, LT, 32, 2>(ctx, P, lu, perm, r, yy, z); break;
+    default: ilu_solve_g<T, LT, 32, 4>(ctx, P, lu, perm, r, yy, z); break;
+  }
 }
 
 template <typename T>
@@ -1116,7 +1147,8 @@ static int pc_apply_impl(sfem_ctx* ctx, const T* r, T* z) {
           y = (T*)L->pz32;
         }
       }
-      if (L->pc_fp32) ilu_solve<T, float>(ctx, P, L->lu32, r, y, z);
+      if (L->pc_bf16) ilu_solve<T, bf16>(ctx, P, L->lu16, r, y, z);
+      else if (L->pc_fp32) ilu_solve<T, float>(ctx, P, L->lu32, r, y, z);
       else ilu_solve<T, double>(ctx, P, L->lu, r, y, z);
       break;
     }
@@ -1126,31 +1158,95 @@ static int pc_apply_impl(sfem_ctx* ctx, const T* r, T* z) {
   return hipGetLastError() == hipSuccess ? SFEM_OK : set_error(SFEM_ERR_HIP, "preconditioner launch");
 }
 
+// (lanes per node, columns in flight per lane) of the tunable kernels
+static const int kVarG[6] = {16, 16, 16, 32, 32, 32};
+static const int kVarU[6] = {1, 2, 4, 1, 2, 4};
+static const int kSpmvG[6] = {8, 8, 16, 16, 32, 32};
+static const int kSpmvU[6] = {2, 4, 2, 4, 2, 4};
+
+template <typename VT, typename T>
+static void spmv_launch(hipStream_t st, int v, int64_t nb, const BlkDesc* desc, const int32_t* colidx, const VT* vals,
+                        const T* x, T* y) {
+  const int G = kSpmvG[v];
+  const unsigned g = (unsigned)((nb * G + 255) / 256);
+  switch (v) {
+    case 0: hipLaunchKernelGGL((k_spmv_blk<VT, T, 8, 2>), dim3(g), dim3(256), 0, st, nb, desc, colidx, vals, x, y); break;
+    case 1: hipLaunchKernelGGL((k_spmv_blk<VT, T, 8, 4>), dim3(g), dim3(256), 0, st, nb, desc, colidx, vals, x, y); break;
+    case 2: hipLaunchKernelGGL((k_spmv_blk<VT, T, 16, 2>), dim3(g), dim3(256), 0, st, nb, desc, colidx, vals, x, y); break;
+    case 3: hipLaunchKernelGGL((k_spmv_blk<VT, T, 16, 4>), dim3(g), dim3(256), 0, st, nb, desc, colidx, vals, x, y); break;
+    case 4: hipLaunchKernelGGL((k_spmv_blk<VT, T, 32, 2>), dim3(g), dim3(256), 0, st, nb, desc, colidx, vals, x, y); break;
+    default: hipLaunchKernelGGL((k_spmv_blk<VT, T, 32, 4>), dim3(g), dim3(256), 0, st, nb, desc, colidx, vals, x, y); break;
+  }
+}
+
 template <typename T>
 static void spmv(sfem_ctx* ctx, const T* x, T* y) {
   LinAlg* L = ctx->la;
   const int64_t nb = L->nblk;
   if (nb == 0) return;
   ctx->kt.begin(SFEM_KT_SPMV, ctx->stream);
-  const bool wide = L->maxrow > 64;
-  const int G = wide ? 16 : 8;
-  const unsigned g = (unsigned)((nb * G + 255) / 256);
-  if (sizeof(T) == 8) {
-    if (wide)
-      hipLaunchKernelGGL((k_spmv_blk<double, T, 16>), dim3(g), dim3(256), 0, ctx->stream, nb, L->d_desc,
-                         ctx->d_colidx, ctx->d_vals, x, y);
-    else
-      hipLaunchKernelGGL((k_spmv_blk<double, T, 8>), dim3(g), dim3(256), 0, ctx->stream, nb, L->d_desc,
-                         ctx->d_colidx, ctx->d_vals, x, y);
-  } else {
-    if (wide)
-      hipLaunchKernelGGL((k_spmv_blk<float, T, 16>), dim3(g), dim3(256), 0, ctx->stream, nb, L->d_desc,
-                         ctx->d_colidx, ctx->d_vals32, x, y);
-    else
-      hipLaunchKernelGGL((k_spmv_blk<float, T, 8>), dim3(g), dim3(256), 0, ctx->stream, nb, L->d_desc,
-                         ctx->d_colidx, ctx->d_vals32, x, y);
-  }
+  const int v = L->tune_spmv[sizeof(T) == 8 ? 0 : 1];
+  if (sizeof(T) == 8) spmv_launch<double, T>(ctx->stream, v, nb, L->d_desc, ctx->d_colidx, ctx->d_vals, x, y);
+  else spmv_launch<float, T>(ctx->stream, v, nb, L->d_desc, ctx->d_colidx, ctx->d_vals32, x, y);
   ctx->kt.end(SFEM_KT_SPMV, ctx->stream);
+}
+
+// Autotuner: time every compiled (G, U) variant of the SpMV and of the ILU(0) apply on
+// the context's own matrix (a few repetitions each, HIP events on the context stream)
+// and keep the fastest.  Runs once per precision, on the first solve.
+template <typename T>
+static int autotune(sfem_ctx* ctx, T* x, T* y) {
+  LinAlg* L = ctx->la;
+  const int slot = sizeof(T) == 8 ? 0 : 1;
+  hipStream_t st = ctx->stream;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return set_error(SFEM_ERR_HIP, "autotune events");
+  const bool on = ctx->kt.on;
+  ctx->kt.on = false;
+  hipLaunchKernelGGL((k_fill<T>), dim3(grid_for(L->n)), dim3(kTPB), 0, st, L->n, x, (T)1);
+  auto time_it = [&](auto&& fn) {
+    fn();  // warm
+    (void)hipEventRecord(e0, st);
+    for (int k = 0; k < 3; ++k) fn();
+    (void)hipEventRecord(e1, st);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    return ms;
+  };
+  float best = 1e30f;
+  for (int v = 0; v < 6; ++v) {
+    L->tune_spmv[slot] = v;
+    const float ms = time_it([&] { spmv<T>(ctx, x, y); });
+    if (ms < best) { best = ms; L->tune_best_spmv[slot] = v; }
+  }
+  L->tune_spmv[slot] = L->tune_best_spmv[slot];
+  if (L->pc_type == SFEM_PC_ILU0) {
+    best = 1e30f;
+    for (int v = 0; v < 6; ++v) {
+      L->tune_ilu[slot] = v;
+      const float ms = time_it([&] { (void)pc_apply<T>(ctx, x, y); });
+      if (ms < best) { best = ms; L->tune_best_ilu[slot] = v; }
+    }
+    L->tune_ilu[slot] = L->tune_best_ilu[slot];
+  }
+  ctx->kt.on = on;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  L->tuned[slot] = true;
+  return hipGetLastError() == hipSuccess ? SFEM_OK : set_error(SFEM_ERR_HIP, "autotune");
+}
+
+int la_tuning(sfem_ctx* ctx, int* out) {
+  LinAlg* L = ctx->la;
+  for (int k = 0; k < 2; ++k) {
+    const bool t = L && L->tuned[k];
+    out[4 * k + 0] = t ? kSpmvG[L->tune_spmv[k]] : 0;
+    out[4 * k + 1] = t ? kSpmvU[L->tune_spmv[k]] : 0;
+    out[4 * k + 2] = t && L->pc_type == SFEM_PC_ILU0 ? kVarG[L->tune_ilu[k]] : 0;
+    out[4 * k + 3] = t && L->pc_type == SFEM_PC_ILU0 ? kVarU[L->tune_ilu[k]] : 0;
+  }
+  return SFEM_OK;
 }
 
 int la_spmv64(sfem_ctx* ctx, const double* x, double* y) {
@@ -1388,9 +1484,10 @@ int la_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int* iters, double* relre
   int rc;
   // the tangent is constant within a time step: rebuild the preconditioner only when it changed
   const int reorder = o->pc_type == SFEM_PC_ILU0 ? o->reorder : 0;
+  const bool bf16f = fp32 && o->pc_type == SFEM_PC_ILU0 && o->factor_bf16 != 0;
   if (L->pc_K_version != ctx->la_K_version || L->pc_type != o->pc_type || L->pc_fp32 != fp32 ||
-      L->pc_reorder != reorder) {
-    if ((rc = la_setup_pc(ctx, o->pc_type, fp32, reorder))) return rc;
+      L->pc_reorder != reorder || L->pc_bf16 != bf16f) {
+    if ((rc = la_setup_pc(ctx, o->pc_type, fp32, reorder, bf16f))) return rc;
     L->pc_reorder = reorder;
     L->pc_K_version = ctx->la_K_version;
   }
@@ -1410,6 +1507,7 @@ int la_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int* iters, double* relre
     double** vs[] = {&L->r, &L->rh, &L->p, &L->v, &L->s, &L->t, &L->ph, &L->sh};
     for (double** q : vs)
       if ((rc = ensure_vec(q, n))) return rc;
+    if (!L->tuned[0] && (rc = autotune<double>(ctx, L->t, L->v))) return rc;
     return bicgstab<double>(ctx, ctx->d_rhs, ctx->d_du, o->rtol, o->max_it, iters, relres, L->r, L->rh,
                             L->p, L->v, L->s, L->t, L->ph, L->sh);
   }
@@ -1417,6 +1515,7 @@ int la_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int* iters, double* relre
   float** vs[] = {&L->r32, &L->rh32, &L->p32, &L->v32, &L->s32, &L->t32, &L->ph32, &L->sh32, &L->x32};
   for (float** q : vs)
     if ((rc = ensure_vec(q, n))) return rc;
+  if (!L->tuned[1] && (rc = autotune<float>(ctx, L->t32, L->v32))) return rc;
   double bn;
   if ((rc = la_norm2(ctx, ctx->d_rhs, &bn))) return rc;
   const int g = grid_for(n);

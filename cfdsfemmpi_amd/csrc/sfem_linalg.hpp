@@ -19,6 +19,7 @@ int set_hip_error(hipError_t e, const char* what);
 // on and its level sets.  With the identity ordering the pattern pointers alias the
 // context's CSR.
 struct BlkDesc;
+struct bf16;
 
 struct IluPlan {
   int reorder = -1;
@@ -38,12 +39,17 @@ struct LinAlg {
   int64_t n = 0;             // free DOFs
   int64_t nblk = 0;          // nodal blocks (nodes with >= 1 free DOF)
   int maxrow = 0;
-  int spmv_sub = 4;          // lanes per CSR row
+  // autotuned kernel variants, [0] FP64 and [1] FP32
+  int tune_spmv[2] = {3, 3}, tune_ilu[2] = {1, 1};
+  int tune_best_spmv[2] = {3, 3}, tune_best_ilu[2] = {1, 1};
+  bool tuned[2] = {false, false};
   int pc_type = SFEM_PC_NONE;
   bool pc_fp32 = false;
   int64_t pc_K_version = -1;  // tangent version the preconditioner was built from
   std::vector<int32_t> h_blkStart, h_diagpos;
   int pc_reorder = 0;
+  bool pc_bf16 = false;
+  bf16* lu16 = nullptr;
   IluPlan* ilu = nullptr;
   int32_t *d_blkStart = nullptr, *d_diagpos = nullptr;
   BlkDesc* d_desc = nullptr;   // per node block, original order
@@ -72,9 +78,10 @@ struct LinAlg {
 
 int la_create(sfem_ctx* ctx);
 void la_destroy(sfem_ctx* ctx);
-int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder);
+int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f);
 int la_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int* iters, double* relres);
 int la_prepare(sfem_ctx* ctx, const sfem_solver_opts* o);
+int la_tuning(sfem_ctx* ctx, int* out);
 int la_norm2(sfem_ctx* ctx, const double* a, double* out);
 int la_spmv64(sfem_ctx* ctx, const double* x, double* y);
 int la_spmv32(sfem_ctx* ctx, const float* x, float* y);

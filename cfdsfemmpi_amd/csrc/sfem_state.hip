@@ -33,6 +33,13 @@ __global__ __launch_bounds__(kTPB) void k_time_update(int64_t n, const double* _
   }
 }
 
+__global__ __launch_bounds__(kTPB) void k_extrapolate(int64_t n, const double* __restrict__ prev,
+                                                      const double* __restrict__ prev2,
+                                                      double* __restrict__ soln) {
+  for (int64_t i = (int64_t)blockIdx.x * kTPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kTPB)
+    soln[i] = 2.0 * prev[i] - prev2[i];
+}
+
 __global__ __launch_bounds__(kTPB) void k_iter_step(int64_t n, TimeCoeffs c, const double* __restrict__ soln,
                                                     const double* __restrict__ prev,
                                                     const double* __restrict__ prev2,
@@ -91,6 +98,11 @@ hipError_t launch_time_update(hipStream_t s, int64_t n, const double* soln, doub
                               double* prev3, double* prev4, const double* dot, double* dotPrev) {
   hipLaunchKernelGGL(k_time_update, dim3(grid1(n)), dim3(kTPB), 0, s, n, soln, prev, prev2, prev3, prev4,
                      dot, dotPrev);
+  return hipGetLastError();
+}
+
+hipError_t launch_extrapolate(hipStream_t s, int64_t n, const double* prev, const double* prev2, double* soln) {
+  hipLaunchKernelGGL(k_extrapolate, dim3(grid1(n)), dim3(kTPB), 0, s, n, prev, prev2, soln);
   return hipGetLastError();
 }
 

@@ -13,6 +13,7 @@ struct TimeCoeffs {
 
 hipError_t launch_time_update(hipStream_t s, int64_t n, const double* soln, double* prev, double* prev2,
                               double* prev3, double* prev4, const double* dot, double* dotPrev);
+hipError_t launch_extrapolate(hipStream_t s, int64_t n, const double* prev, const double* prev2, double* soln);
 hipError_t launch_iter_step(hipStream_t s, int64_t n, const TimeCoeffs& c, const double* soln,
                             const double* prev, const double* prev2, const double* prev3,
                             const double* prev4, double* dot, const double* dotPrev, double* cur,

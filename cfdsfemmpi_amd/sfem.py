@@ -26,6 +26,7 @@ ERRORS = {-1: "SFEM_ERR_ARG", -2: "SFEM_ERR_STATE", -3: "SFEM_ERR_NEGJAC", -4: "
           -5: "SFEM_ERR_DIVERGED", -6: "SFEM_ERR_MAXIT", -7: "SFEM_ERR_NOMEM", -8: "SFEM_ERR_NOTCONV",
           -9: "SFEM_ERR_UNSUPPORTED"}
 PREC_FP64, PREC_MIXED = 0, 1
+PRED_PREVIOUS, PRED_EXTRAPOLATE = 0, 1
 KT_ASSEMBLY, KT_SPMV, KT_PC = 0, 1, 2
 KSP = {"bcgs": 0, "gmres": 1}
 PC = {"none": 0, "jacobi": 1, "bjacobi4": 2, "ilu": 3}
@@ -39,7 +40,7 @@ _vp = C.c_void_p
 
 class SolverOpts(C.Structure):
     _fields_ = [("ksp_type", C.c_int), ("pc_type", C.c_int), ("rtol", C.c_double),
-                ("max_it", C.c_int), ("restart", C.c_int), ("reorder", C.c_int)]
+                ("max_it", C.c_int), ("restart", C.c_int), ("reorder", C.c_int), ("factor_bf16", C.c_int)]
 
 
 class StepStats(C.Structure):
@@ -58,6 +59,7 @@ ABI = [
     ("sfem_get_pattern", C.c_int, [_vp, _i64p, _i32p, _i32p]),
     ("sfem_set_params", C.c_int, [_vp, _f64p, C.c_int, C.c_double, C.c_double]),
     ("sfem_set_precision", C.c_int, [_vp, C.c_int]),
+    ("sfem_set_predictor", C.c_int, [_vp, C.c_int]),
     ("sfem_time_update", C.c_int, [_vp, C.c_double]),
     ("sfem_assemble", C.c_int, [_vp, C.c_int, C.c_int, _f64p]),
     ("sfem_element_KF", C.c_int, [_vp, C.c_int64, _f64p, _f64p]),
@@ -74,6 +76,7 @@ ABI = [
     ("sfem_reset_state", C.c_int, [_vp]),
     ("sfem_enable_kernel_timing", C.c_int, [_vp, C.c_int]),
     ("sfem_kernel_timing", C.c_int, [_vp, C.c_int, _i64p, _f64p]),
+    ("sfem_tuning", C.c_int, [_vp, C.POINTER(C.c_int)]),
     ("sfem_info", C.c_int, [_vp, _i64p, _i64p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("sfem_last_timings", C.c_int, [_vp, _f64p, _f64p]),
 ]
@@ -179,6 +182,9 @@ class Context:
     def set_precision(self, prec: int):
         _check(self.lib.sfem_set_precision(self.h, int(prec)), "sfem_set_precision")
 
+    def set_predictor(self, kind: int):
+        _check(self.lib.sfem_set_predictor(self.h, int(kind)), "sfem_set_predictor")
+
     # -- hot path ----------------------------------------------------------
     def time_update(self, timeFact: float):
         _check(self.lib.sfem_time_update(self.h, float(timeFact)), "sfem_time_update")
@@ -256,6 +262,12 @@ class Context:
         _check(self.lib.sfem_kernel_timing(self.h, int(kind), C.byref(n), C.byref(ms)), "sfem_kernel_timing")
         return n.value, ms.value
 
+    def tuning(self):
+        out = (C.c_int * 8)()
+        _check(self.lib.sfem_tuning(self.h, out), "sfem_tuning")
+        v = list(out)
+        return {"fp64": {"spmv_GU": v[0:2], "ilu_GU": v[2:4]}, "fp32": {"spmv_GU": v[4:6], "ilu_GU": v[6:8]}}
+
     def info(self):
         nf, nz = C.c_int64(0), C.c_int64(0)
         nc, nd = C.c_int(0), C.c_int(0)
@@ -268,12 +280,13 @@ class Context:
         return a.value, s.value
 
 
-def solver_opts(ksp="bcgs", pc="ilu", rtol=1e-8, max_it=1000, restart=30, reorder=0) -> SolverOpts:
+def solver_opts(ksp="bcgs", pc="ilu", rtol=1e-8, max_it=1000, restart=30, reorder=0,
+                factor_bf16=0) -> SolverOpts:
     if pc == "lu":
         # the reference's test decks ask for MUMPS LU; a tight RCM-ordered ILU(0)-BiCGStab
         # solve reproduces the direct answer to well below the decks' tolerances
         ksp, pc, rtol, max_it, reorder = "bcgs", "ilu", 1e-13, max(max_it, 5000), 1
-    return SolverOpts(KSP[ksp], PC[pc], float(rtol), int(max_it), int(restart), int(reorder))
+    return SolverOpts(KSP[ksp], PC[pc], float(rtol), int(max_it), int(restart), int(reorder), int(factor_bf16))
 
 
 class StabFEM:

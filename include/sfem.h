@@ -55,6 +55,8 @@ typedef struct sfem_solver_opts {
   int reorder;       /* ILU(0) node ordering: 0 the matrix's own order, 1 reverse Cuthill-McKee
                         (PETSc's -pc_factor_mat_ordering_type rcm), 2 greedy multicolour
                         (one fully parallel level per colour) */
+  int factor_bf16;   /* mixed precision only: keep the ILU(0) factors in bfloat16 (computed in
+                        FP32, rounded to nearest even) to halve the preconditioner's bytes */
 } sfem_solver_opts;
 
 typedef struct sfem_step_stats {
@@ -107,6 +109,13 @@ int sfem_set_params(sfem_ctx* ctx, const double* elemData, int tis, double rhoIn
 
 /* SFEM_PREC_FP64 (default) or SFEM_PREC_MIXED. */
 int sfem_set_precision(sfem_ctx* ctx, int precision);
+
+/* Newton initial iterate of each step: SFEM_PRED_PREVIOUS (the previous step's
+ * solution, as the reference) or SFEM_PRED_EXTRAPOLATE (2 solnPrev - solnPrev2, the
+ * solnExtrap that SolutionData::timeUpdate computes and the reference leaves unused,
+ * SolutionData.cpp:75).  It changes the Newton/Krylov iteration counts, not the answer. */
+enum { SFEM_PRED_PREVIOUS = 0, SFEM_PRED_EXTRAPOLATE = 1 };
+int sfem_set_predictor(sfem_ctx* ctx, int predictor);
 
 /* Start a time step (StabFEM2.cpp:632-648): td[] from SetTimeParametersFluid,
  * SolutionData::timeUpdate (SolutionData.cpp:64-78), and
@@ -173,6 +182,11 @@ int sfem_reset_state(sfem_ctx* ctx);
 enum { SFEM_KT_ASSEMBLY = 0, SFEM_KT_SPMV = 1, SFEM_KT_PC = 2, SFEM_KT_NKINDS = 3 };
 int sfem_enable_kernel_timing(sfem_ctx* ctx, int enable);
 int sfem_kernel_timing(sfem_ctx* ctx, int kind, int64_t* launches, double* total_ms);
+
+/* Kernel shapes chosen by the solver's autotuner (lanes per node G and columns in
+ * flight per lane U), as out[8] = {SpMV G, U, ILU-apply G, U} for FP64 then FP32;
+ * 0 where not tuned yet.  No counterpart in the reference. */
+int sfem_tuning(sfem_ctx* ctx, int* out);
 
 /* Sizes and counts of the context. */
 int sfem_info(sfem_ctx* ctx, int64_t* nfree, int64_t* nnz, int* ncolors, int* ndof);

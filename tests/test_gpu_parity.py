@@ -266,6 +266,23 @@ def test_rcm_ilu_unstructured_tria():
     ctx.close()
 
 
+@pytest.mark.parametrize("bf16", [0, 1])
+def test_mixed_bf16_factors_and_predictor(bf16):
+    """BF16-stored ILU factors and the extrapolated Newton start leave the converged
+    answer within 1e-6 of the plain FP64 path."""
+    f64, mesh, ctl = _cavity3d(6, 4, newton_tol=1e-10)
+    s64 = f64.solveFullyImplicit()
+    fmx, _, _ = _cavity3d(6, 4, prec=sfem.PREC_MIXED,
+                          opts=sfem.solver_opts("bcgs", "ilu", 1e-4, 2000, reorder=2, factor_bf16=bf16),
+                          newton_tol=1e-10)
+    fmx.ctx.set_predictor(sfem.PRED_EXTRAPOLATE)
+    smx = fmx.solveFullyImplicit()
+    u64, umx = s64.reshape(-1, 4)[:, :3], smx.reshape(-1, 4)[:, :3]
+    eu = np.linalg.norm(umx - u64) / np.linalg.norm(u64)
+    print(f"[mixed bf16={bf16} extrapolated] rel L2 u {eu:.2e}; history {fmx.history}", flush=True)
+    assert eu <= 1e-6
+
+
 def test_negative_jacobian_reported():
     mesh = meshgen.ldc2d_quad(3)
     mesh.conn[4] = mesh.conn[4][[0, 3, 2, 1]]  # clockwise element
