@@ -139,6 +139,19 @@ def cpu_baseline(n_target, fp64_res):
             "cpu": platform.processor() or platform.machine(), "host_threads": os.cpu_count()}
 
 
+def copy_probe(mib):
+    """A plain copy kernel of known size: its FETCH_SIZE / WRITE_SIZE readings give the
+    factor between those counters and bytes moved."""
+    import torch
+    nbytes = mib << 20
+    x = torch.ones(nbytes, dtype=torch.uint8, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    log(f"copy probe: 5 copies of {nbytes} bytes ({mib} MiB) done")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -154,7 +167,12 @@ def main():
     ap.add_argument("--bf16-factors", type=int, default=0, help="mixed path: ILU(0) factors stored in bfloat16")
     ap.add_argument("--predictor", choices=["previous", "extrapolate"], default="previous",
                     help="Newton initial iterate of each step (both paths)")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run 5 device-to-device copies of this many MiB (HBM counter calibration)")
     a = ap.parse_args()
+    if a.copy_probe_mib:
+        copy_probe(a.copy_probe_mib)
+        return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the multi-GPU layout is described "
               f"in DESIGN.md and not built", file=sys.stderr)

@@ -736,6 +736,12 @@ int sfem_tuning(sfem_ctx* c, int* out) {
   return la_tuning(c, out);
 }
 
+int sfem_tuning_levels(sfem_ctx* c, int fp32, int* out, int cap, int* nlevels) {
+  if (!c || !nlevels || (cap > 0 && !out) || cap < 0) return set_error(SFEM_ERR_ARG, "bad argument");
+  if (!c->la) { *nlevels = 0; return SFEM_OK; }
+  return la_tuning_levels(c, fp32, out, cap, nlevels);
+}
+
 int sfem_info(sfem_ctx* c, int64_t* nfree, int64_t* nnz, int* ncolors, int* ndof) {
   if (!c) return set_error(SFEM_ERR_ARG, "null context");
   if (nfree) *nfree = c->nfree;

@@ -237,6 +237,34 @@ __global__ __launch_bounds__(kTPB) void k_gather(int64_t n, const int32_t* __res
     dst[i] = src[map[i]];
 }
 
+// dst0[i] = dst1[i] = src[map[i]]  (the permuted tangent and the factor's initial values)
+template <typename T>
+__global__ __launch_bounds__(kTPB) void k_gather2(int64_t n, const int32_t* __restrict__ map,
+                                                  const T* __restrict__ src, T* __restrict__ dst0,
+                                                  T* __restrict__ dst1) {
+  for (int64_t i = (int64_t)blockIdx.x * kTPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kTPB) {
+    const T v = src[map[i]];
+    dst0[i] = v;
+    dst1[i] = v;
+  }
+}
+
+// b[i] = scale * a[perm[i]]  (into the solver ordering)
+template <typename TA, typename TB>
+__global__ __launch_bounds__(kTPB) void k_permute_in(int64_t n, const int32_t* __restrict__ perm,
+                                                     const TA* __restrict__ a, TB* __restrict__ b, double scale) {
+  for (int64_t i = (int64_t)blockIdx.x * kTPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kTPB)
+    b[i] = (TB)((double)a[perm[i]] * scale);
+}
+
+// b[perm[i]] = scale * a[i]  (back to the DOF ordering)
+template <typename TA, typename TB>
+__global__ __launch_bounds__(kTPB) void k_permute_out(int64_t n, const int32_t* __restrict__ perm,
+                                                      const TA* __restrict__ a, TB* __restrict__ b, double scale) {
+  for (int64_t i = (int64_t)blockIdx.x * kTPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kTPB)
+    b[perm[i]] = (TB)((double)a[i] * scale);
+}
+
 // y = a*x + b*y with host scalars (GMRES)
 template <typename T>
 __global__ __launch_bounds__(kTPB) void k_axpby(int64_t n, double a, const T* __restrict__ x, double b,
@@ -470,6 +498,54 @@ __device__ inline void group_dot(const int32_t* __restrict__ col, const VT* __re
   }
 }
 
+// Logical workgroup index.  Workgroups are dispatched round-robin over the 8 XCDs;
+// with xcd != 0 each XCD walks one contiguous range of logical workgroups instead, so
+// that neighbouring node blocks, which gather overlapping x entries, share an L2.
+__device__ inline int64_t logical_wg(int xcd) {
+  const uint32_t b = blockIdx.x;
+  if (!xcd) return b;
+  const uint32_t q = gridDim.x / 8, r = gridDim.x % 8, x = b % 8, k = b / 8;
+  return (int64_t)(x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
+}
+
+// As group_dot, with every column index and value of up to K columns per lane loaded
+// before any x entry is gathered, so a lane has all its loads in flight at once (two
+// dependent memory round trips per node block instead of two per U columns); columns
+// beyond K*G per lane fall back to group_dot.
+template <int G, int K, typename VT, typename XT, typename AT>
+__device__ inline void group_dot_pf(const int32_t* __restrict__ col, const VT* __restrict__ V,
+                                    const XT* __restrict__ x, int32_t rl, int m, int32_t q0, int32_t q1,
+                                    int lane, AT (&acc)[4]) {
+  int32_t c[K];
+  AT v[4][K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int32_t q = q0 + lane + k * G;
+    const bool ok = q < q1;
+    c[k] = ok ? col[q] : -1;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) v[a][k] = (ok && a < m) ? widen<AT>(V[(int64_t)a * rl + q]) : (AT)0;
+  }
+  AT xv[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) xv[k] = c[k] >= 0 ? (AT)x[c[k]] : (AT)0;
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+#pragma unroll
+    for (int a = 0; a < 4; ++a) acc[a] += v[a][k] * xv[k];
+  if (q0 + K * G < q1) group_dot<G, 2>(col, V, x, rl, m, q0 + K * G, q1, lane, acc);
+}
+
+// U > 0: group_dot with U columns in flight per lane; U == 0: group_dot_pf covering
+// 128 columns per node block before the fallback.
+template <int G, int U, typename VT, typename XT, typename AT>
+__device__ inline void group_dot_v(const int32_t* __restrict__ col, const VT* __restrict__ V,
+                                   const XT* __restrict__ x, int32_t rl, int m, int32_t q0, int32_t q1,
+                                   int lane, AT (&acc)[4]) {
+  if constexpr (U == 0) group_dot_pf<G, 128 / G>(col, V, x, rl, m, q0, q1, lane, acc);
+  else group_dot<G, U>(col, V, x, rl, m, q0, q1, lane, acc);
+}
+
 // Node-grouped CSR SpMV.  The free rows of one node (a block of m <= 4 rows) share one
 // column list, so G lanes walk that list once, load each column index and x entry once,
 // and accumulate the m rows together; values of row a sit at base + a*rl + q.
@@ -477,58 +553,78 @@ template <typename VT, typename XT, int G, int U>
 __global__ __launch_bounds__(256) void k_spmv_blk(int64_t nb, const BlkDesc* __restrict__ desc,
                                                   const int32_t* __restrict__ colidx,
                                                   const VT* __restrict__ vals, const XT* __restrict__ x,
-                                                  XT* __restrict__ y) {
+                                                  XT* __restrict__ y, int xcd) {
   using AT = typename std::conditional<sizeof(VT) == 8, double, float>::type;
-  const int64_t gt = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t gt = logical_wg(xcd) * 256 + threadIdx.x;
   const int64_t b = gt / G;
   const int lane = (int)(gt % G);
   AT acc[4] = {0, 0, 0, 0};
   BlkDesc d = {0, 0, 0, 0};
   if (b < nb) {
     d = desc[b];
-    group_dot<G, U>(colidx + d.base, vals + d.base, x, d.rl, d.dqm >> 16, 0, d.rl, lane, acc);
+    group_dot_v<G, U>(colidx + d.base, vals + d.base, x, d.rl, d.dqm >> 16, 0, d.rl, lane, acc);
   }
 #pragma unroll
   for (int a = 0; a < 4; ++a) acc[a] = group_sum<G>(acc[a]);
   if (b < nb && lane < (d.dqm >> 16)) y[d.r0 + lane] = (XT)pick4(acc, lane);
 }
 
-// Forward solve L y = r over one level, G >= 16 lanes per block: the external lower part
-// as a grouped sparse dot product, then the unit-lower m x m diagonal block, whose
-// entries and right-hand sides are loaded by separate lanes and shared by shuffles.
-// Level blocks are either the contiguous range [first, first + count) or list[first + g].
 template <typename LT> struct AccOf { using type = float; };
 template <> struct AccOf<double> { using type = double; };
 
+// The m x m diagonal block of a node block: entry e = 4a + b (row a, column b) is loaded
+// by lane e % G as its (e / G)-th value (load_diag, issued before the long sparse dot
+// product) and then broadcast to the whole group (bcast_diag); absent entries read 0.
+template <int G> struct DiagPerLane { static constexpr int n = G >= 16 ? 1 : 16 / G; };
+
+template <int G, typename AT, typename LT>
+__device__ inline void load_diag(const LT* __restrict__ lu, const BlkDesc& d, int lane,
+                                 AT (&dv)[DiagPerLane<G>::n]) {
+  const int m = d.dqm >> 16, dq0 = d.dqm & 0xffff;
+#pragma unroll
+  for (int t = 0; t < DiagPerLane<G>::n; ++t) {
+    const int e = lane + t * G;
+    dv[t] = (e < 16 && (e >> 2) < m && (e & 3) < m)
+                ? widen<AT>(lu[d.base + (int64_t)(e >> 2) * d.rl + dq0 + (e & 3)])
+                : (AT)0;
+  }
+}
+
+template <int G, typename AT>
+__device__ inline void bcast_diag(const AT (&dv)[DiagPerLane<G>::n], AT (&D)[16]) {
+#pragma unroll
+  for (int e = 0; e < 16; ++e) D[e] = __shfl(dv[e / G], e % G, G);
+}
+
+// Forward solve L y = r over one level with G >= 4 lanes per node block: the external
+// lower part as a grouped sparse dot product, then the unit-lower m x m diagonal block.
+// Level blocks are either the contiguous range [first, first + count) or list[first + g].
 template <typename T, typename LT, int G, int U>
 __global__ __launch_bounds__(256) void k_ilu_lsolve_blk(const int32_t* __restrict__ list, int first, int count,
                                                         const BlkDesc* __restrict__ desc,
                                                         const int32_t* __restrict__ colidx,
                                                         const LT* __restrict__ lu,
-                                                        const int32_t* __restrict__ perm,
-                                                        const T* __restrict__ r, T* __restrict__ y) {
+                                                        const T* __restrict__ r, T* __restrict__ y, int xcd) {
   using AT = typename AccOf<LT>::type;
-  const int64_t gt = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t gt = logical_wg(xcd) * 256 + threadIdx.x;
   const int64_t g = gt / G;
   const int lane = (int)(gt % G);
   AT acc[4] = {0, 0, 0, 0};
   BlkDesc d = {0, 0, 0, 0};
-  AT dv = 0, rv = 0;
+  AT rv = 0;
+  AT dv[DiagPerLane<G>::n] = {};
   const bool act = g < count;
   if (act) {
     d = desc[list ? list[first + g] : first + (int32_t)g];
     const int m = d.dqm >> 16, dq0 = d.dqm & 0xffff;
-    // diagonal block entry (lane / 4, lane % 4) and right-hand side of row `lane`
-    if (lane < 16 && (lane >> 2) < m && (lane & 3) < m)
-      dv = widen<AT>(lu[d.base + (int64_t)(lane >> 2) * d.rl + dq0 + (lane & 3)]);
-    if (lane < m) rv = (AT)r[perm ? perm[d.r0 + lane] : d.r0 + lane];
-    group_dot<G, U>(colidx + d.base, lu + d.base, y, d.rl, m, 0, dq0, lane, acc);
+    if (lane < m) rv = (AT)r[d.r0 + lane];
+    load_diag<G>(lu, d, lane, dv);
+    group_dot_v<G, U>(colidx + d.base, lu + d.base, y, d.rl, m, 0, dq0, lane, acc);
   }
+  AT D[16], R[4];
+  bcast_diag<G>(dv, D);
 #pragma unroll
   for (int a = 0; a < 4; ++a) acc[a] = group_sum<G>(acc[a]);
-  AT D[16], R[4];
-#pragma unroll
-  for (int t = 0; t < 16; ++t) D[t] = __shfl(dv, t, G);
 #pragma unroll
   for (int a = 0; a < 4; ++a) R[a] = __shfl(rv, a, G);
   const int m = d.dqm >> 16;
@@ -543,36 +639,32 @@ __global__ __launch_bounds__(256) void k_ilu_lsolve_blk(const int32_t* __restric
   if (act && lane < m) y[d.r0 + lane] = (T)pick4(v, lane);
 }
 
-// Backward solve U z = y over one level (levels in reverse order), in place on y
-// (permuted order); the result also goes to zout through perm when perm is non-null.
+// Backward solve U z = y over one level (levels in reverse order), in place on y.
 template <typename T, typename LT, int G, int U>
 __global__ __launch_bounds__(256) void k_ilu_usolve_blk(const int32_t* __restrict__ list, int first, int count,
                                                         const BlkDesc* __restrict__ desc,
                                                         const int32_t* __restrict__ colidx,
-                                                        const LT* __restrict__ lu,
-                                                        const int32_t* __restrict__ perm,
-                                                        T* __restrict__ y, T* __restrict__ zout) {
+                                                        const LT* __restrict__ lu, T* __restrict__ y, int xcd) {
   using AT = typename AccOf<LT>::type;
-  const int64_t gt = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t gt = logical_wg(xcd) * 256 + threadIdx.x;
   const int64_t g = gt / G;
   const int lane = (int)(gt % G);
   AT acc[4] = {0, 0, 0, 0};
   BlkDesc d = {0, 0, 0, 0};
-  AT dv = 0, rv = 0;
+  AT rv = 0;
+  AT dv[DiagPerLane<G>::n] = {};
   const bool act = g < count;
   if (act) {
     d = desc[list ? list[first + g] : first + (int32_t)g];
     const int m = d.dqm >> 16, dq0 = d.dqm & 0xffff;
-    if (lane < 16 && (lane >> 2) < m && (lane & 3) < m)
-      dv = widen<AT>(lu[d.base + (int64_t)(lane >> 2) * d.rl + dq0 + (lane & 3)]);
     if (lane < m) rv = (AT)y[d.r0 + lane];
-    group_dot<G, U>(colidx + d.base, lu + d.base, y, d.rl, m, dq0 + m, d.rl, lane, acc);
+    load_diag<G>(lu, d, lane, dv);
+    group_dot_v<G, U>(colidx + d.base, lu + d.base, y, d.rl, m, dq0 + m, d.rl, lane, acc);
   }
+  AT D[16], R[4];
+  bcast_diag<G>(dv, D);
 #pragma unroll
   for (int a = 0; a < 4; ++a) acc[a] = group_sum<G>(acc[a]);
-  AT D[16], R[4];
-#pragma unroll
-  for (int t = 0; t < 16; ++t) D[t] = __shfl(dv, t, G);
 #pragma unroll
   for (int a = 0; a < 4; ++a) R[a] = __shfl(rv, a, G);
   const int m = d.dqm >> 16;
@@ -587,11 +679,7 @@ __global__ __launch_bounds__(256) void k_ilu_usolve_blk(const int32_t* __restric
       v[a] = s / D[a * 4 + a];
     }
   }
-  if (act && lane < m) {
-    const AT z = pick4(v, lane);
-    y[d.r0 + lane] = (T)z;
-    if (perm) zout[perm[d.r0 + lane]] = (T)z;
-  }
+  if (act && lane < m) y[d.r0 + lane] = (T)pick4(v, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -620,7 +708,7 @@ static hipError_t alloc(T** p, size_t n) {
 LinAlg::~LinAlg() {
   void* ptrs[] = {d_blkStart, d_diagpos, d_part, d_scal, r, rh, p, v, s, t, ph, sh, x, w, z, pr, pz,
                   r32, rh32, p32, v32, s32, t32, ph32, sh32, x32, du32, w32, z32, pr32, pz32,
-                  dinv, dinv32, binv, binv32, lu, lu32, gm_V, gm_V32, d_desc, lu16};
+                  dinv, dinv32, binv, binv32, lu, lu32, gm_V, gm_V32, d_desc, lu16, kp, kp32};
   delete ilu;
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
@@ -801,23 +889,45 @@ static std::vector<int32_t> rcm_order(const std::vector<int64_t>& adjPtr, const 
 // ids sorted by (colour, id).  Blocks of one colour are pairwise non-adjacent, so the
 // ILU(0) factorisation and both triangular solves advance one colour per level.  On a
 // structured hex grid this is the 2x2x2 parity colouring (8 colours).
-static std::vector<int32_t> multicolor_order(const std::vector<int64_t>& adjPtr, const std::vector<int32_t>& adj,
-                                             std::vector<int32_t>* colorPtr) {
+// Greedy colouring visiting the blocks in increasing (reverse = false) or decreasing
+// order; returns the number of colours.
+static int greedy_color(const std::vector<int64_t>& adjPtr, const std::vector<int32_t>& adj, bool reverse,
+                        std::vector<int32_t>& color) {
   const int64_t nb = (int64_t)adjPtr.size() - 1;
-  std::vector<int32_t> color(nb, -1), mark;
+  color.assign(nb, -1);
+  std::vector<int64_t> mark;
   int nc = 0;
-  for (int64_t k = 0; k < nb; ++k) {
+  for (int64_t i = 0; i < nb; ++i) {
+    const int64_t k = reverse ? nb - 1 - i : i;
     for (int64_t q = adjPtr[k]; q < adjPtr[k + 1]; ++q) {
       const int32_t c = color[adj[q]];
       if (c >= 0) {
         if ((int)mark.size() <= c) mark.resize(c + 1, -1);
-        mark[c] = (int32_t)k;
+        mark[c] = k;
       }
     }
     int c = 0;
-    while (c < (int)mark.size() && mark[c] == (int32_t)k) ++c;
+    while (c < (int)mark.size() && mark[c] == k) ++c;
     color[k] = c;
     nc = std::max(nc, c + 1);
+  }
+  return nc;
+}
+
+// Multicolour block ordering: colour classes in order, blocks of a class in their
+// original order.  Greedy colouring is tried in both visiting orders and the one with
+// fewer colours kept: on a structured 27-point node graph with a removed corner node
+// (a pinned pressure) the visit that starts at the removed corner needs 10 colours,
+// the other the optimal 8.
+static std::vector<int32_t> multicolor_order(const std::vector<int64_t>& adjPtr, const std::vector<int32_t>& adj,
+                                             std::vector<int32_t>* colorPtr) {
+  const int64_t nb = (int64_t)adjPtr.size() - 1;
+  std::vector<int32_t> color, color_r;
+  int nc = greedy_color(adjPtr, adj, false, color);
+  const int nc_r = greedy_color(adjPtr, adj, true, color_r);
+  if (nc_r < nc) {
+    nc = nc_r;
+    color.swap(color_r);
   }
   std::vector<int32_t> ptr(nc + 1, 0), order(nb);
   for (int64_t k = 0; k < nb; ++k) ptr[color[k] + 1]++;
@@ -858,6 +968,7 @@ IluPlan::~IluPlan() {
 static int build_ilu_plan(sfem_ctx* ctx, LinAlg* L, int reorder) {
   delete L->ilu;
   L->ilu = new IluPlan;
+  ++L->plan_id;
   IluPlan& P = *L->ilu;
   P.reorder = reorder;
   P.identity = (reorder == 0);
@@ -1027,18 +1138,26 @@ int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f) 
     const IluPlan& P = *L->ilu;
     const int64_t nnz = ctx->nnz;
     const unsigned gz = (unsigned)std::min<int64_t>((nnz + kTPB - 1) / kTPB, 1 << 20);
+    // reordered plan: one gather gives the permuted tangent (for the products of the
+    // Krylov solve) and the factor's starting values
     if (fp32) {
       int rc = ensure_vec(&L->lu32, nnz); if (rc) return rc;
-      if (P.identity)
+      if (P.identity) {
         LA_CHECK(hipMemcpyAsync(L->lu32, ctx->d_vals32, nnz * sizeof(float), hipMemcpyDeviceToDevice, st));
-      else
-        hipLaunchKernelGGL((k_gather<float>), dim3(gz), dim3(kTPB), 0, st, nnz, P.d_pmap, ctx->d_vals32, L->lu32);
+      } else {
+        if ((rc = ensure_vec(&L->kp32, nnz))) return rc;
+        hipLaunchKernelGGL((k_gather2<float>), dim3(gz), dim3(kTPB), 0, st, nnz, P.d_pmap, ctx->d_vals32,
+                           L->kp32, L->lu32);
+      }
     } else {
       int rc = ensure_vec(&L->lu, nnz); if (rc) return rc;
-      if (P.identity)
+      if (P.identity) {
         LA_CHECK(hipMemcpyAsync(L->lu, ctx->d_vals, nnz * sizeof(double), hipMemcpyDeviceToDevice, st));
-      else
-        hipLaunchKernelGGL((k_gather<double>), dim3(gz), dim3(kTPB), 0, st, nnz, P.d_pmap, ctx->d_vals, L->lu);
+      } else {
+        if ((rc = ensure_vec(&L->kp, nnz))) return rc;
+        hipLaunchKernelGGL((k_gather2<double>), dim3(gz), dim3(kTPB), 0, st, nnz, P.d_pmap, ctx->d_vals,
+                           L->kp, L->lu);
+      }
     }
     const int nl = (int)P.levLPtr.size() - 1;
     for (int lv = 0; lv < nl; ++lv) {
@@ -1073,47 +1192,83 @@ static int pc_apply(sfem_ctx* ctx, const T* r, T* z) {
   return rc;
 }
 
-template <typename T, typename LT, int G, int U>
-static void ilu_solve_g(sfem_ctx* ctx, const IluPlan& P, const LT* lu, const int32_t* perm, const T* r, T* y,
-                        T* zout) {
+// Triangular-solve kernel shapes: code = vi | (xcd << 5), with G = kTriG[vi] lanes per
+// node block and U = kTriU[vi] columns in flight per lane (0: all of a lane's columns,
+// up to 128 / G, loaded at once).
+static const int kTriG[15] = {4, 4, 4, 8, 8, 8, 8, 16, 16, 16, 16, 32, 32, 32, 32};
+static const int kTriU[15] = {1, 2, 4, 1, 2, 4, 0, 1, 2, 4, 0, 1, 2, 4, 0};
+constexpr int kTriCodes = 15;
+constexpr int kTriDefault = 8;  // G = 16, U = 2, no XCD remap
+
+template <typename T, typename LT, bool UPPER, int G, int U>
+static void tri_launch_g(hipStream_t st, int xcd, const int32_t* list, int first, int cnt, const BlkDesc* desc,
+                         const int32_t* col, const LT* lu, const T* r, T* y) {
+  const unsigned gr = (unsigned)(((int64_t)cnt * G + 255) / 256);
+  if constexpr (UPPER)
+    hipLaunchKernelGGL((k_ilu_usolve_blk<T, LT, G, U>), dim3(gr), dim3(256), 0, st, list, first, cnt, desc, col,
+                       lu, y, xcd);
+  else
+    hipLaunchKernelGGL((k_ilu_lsolve_blk<T, LT, G, U>), dim3(gr), dim3(256), 0, st, list, first, cnt, desc, col,
+                       lu, r, y, xcd);
+}
+
+template <typename T, typename LT, bool UPPER>
+static void tri_launch(hipStream_t st, int code, const int32_t* list, int first, int cnt, const BlkDesc* desc,
+                       const int32_t* col, const LT* lu, const T* r, T* y) {
+  const int xcd = code >> 5;
+#define SFEM_TRI(i, G, U) \
+  case i: tri_launch_g<T, LT, UPPER, G, U>(st, xcd, list, first, cnt, desc, col, lu, r, y); break;
+  switch (code & 31) {
+    SFEM_TRI(0, 4, 1) SFEM_TRI(1, 4, 2) SFEM_TRI(2, 4, 4)
+    SFEM_TRI(3, 8, 1) SFEM_TRI(4, 8, 2) SFEM_TRI(5, 8, 4) SFEM_TRI(6, 8, 0)
+    SFEM_TRI(7, 16, 1) SFEM_TRI(8, 16, 2) SFEM_TRI(9, 16, 4) SFEM_TRI(10, 16, 0)
+    SFEM_TRI(11, 32, 1) SFEM_TRI(12, 32, 2) SFEM_TRI(13, 32, 4)
+    default: tri_launch_g<T, LT, UPPER, 32, 0>(st, xcd, list, first, cnt, desc, col, lu, r, y); break;
+  }
+#undef SFEM_TRI
+}
+
+// Apply the ILU(0) preconditioner: z = U^{-1} L^{-1} r, the forward solve into z and the
+// backward solve in place.  r and z are in the plan's ordering (the Krylov solve runs
+// there when the plan is reordered).  Each level launch uses its autotuned shape, or
+// `force` for all levels when force >= 0; with ev non-null an event is recorded after
+// every level (ev[0] before the first), forward levels then backward levels.
+template <typename T, typename LT>
+static void ilu_solve(sfem_ctx* ctx, const IluPlan& P, const LT* lu, const T* r, T* z, int force = -1,
+                      hipEvent_t* ev = nullptr) {
+  LinAlg* L = ctx->la;
   hipStream_t st = ctx->stream;
-  const BlkDesc* desc = P.identity ? ctx->la->d_desc : P.d_desc;
+  const int slot = sizeof(T) == 8 ? 0 : 1;
+  const BlkDesc* desc = P.identity ? L->d_desc : P.d_desc;
+  const std::vector<int>& tl = L->tune_L[slot];
+  const std::vector<int>& tu = L->tune_U[slot];
   const int nl = (int)P.levLPtr.size() - 1;
+  const int nu = (int)P.levUPtr.size() - 1;
+  if (ev) (void)hipEventRecord(ev[0], st);
   for (int lv = 0; lv < nl; ++lv) {
     const int32_t* list; int first, cnt;
     level_range(P, false, lv, &list, &first, &cnt);
-    if (!cnt) continue;
-    const unsigned gr = (unsigned)(((int64_t)cnt * G + 255) / 256);
-    hipLaunchKernelGGL((k_ilu_lsolve_blk<T, LT, G, U>), dim3(gr), dim3(256), 0, st, list, first, cnt, desc,
-                       P.colidx, lu, perm, r, y);
+    const int code = force >= 0 ? force : lv < (int)tl.size() ? tl[lv] : kTriDefault;
+    if (cnt) tri_launch<T, LT, false>(st, code, list, first, cnt, desc, P.colidx, lu, r, z);
+    if (ev) (void)hipEventRecord(ev[1 + lv], st);
   }
-  const int nu = (int)P.levUPtr.size() - 1;
   for (int lv = 0; lv < nu; ++lv) {
     const int32_t* list; int first, cnt;
     level_range(P, true, lv, &list, &first, &cnt);
-    if (!cnt) continue;
-    const unsigned gr = (unsigned)(((int64_t)cnt * G + 255) / 256);
-    hipLaunchKernelGGL((k_ilu_usolve_blk<T, LT, G, U>), dim3(gr), dim3(256), 0, st, list, first, cnt, desc,
-                       P.colidx, lu, perm, y, zout);
+    const int code = force >= 0 ? force : lv < (int)tu.size() ? tu[lv] : kTriDefault;
+    if (cnt) tri_launch<T, LT, true>(st, code, list, first, cnt, desc, P.colidx, lu, z, z);
+    if (ev) (void)hipEventRecord(ev[1 + nl + lv], st);
   }
 }
 
-// Apply the ILU(0) preconditioner: z = U^{-1} L^{-1} r.  With a permuted plan the
-// forward solve reads r through the permutation and the backward solve writes z back
-// through it; y is the permuted work vector.  With the identity plan y is z itself.
-template <typename T, typename LT>
-static void ilu_solve(sfem_ctx* ctx, const IluPlan& P, const LT* lu, const T* r, T* y, T* z) {
-  const int32_t* perm = P.identity ? nullptr : P.d_perm;
-  T* yy = P.identity ? z : y;
-  const int v = ctx->la->tune_ilu[sizeof(T) == 8 ? 0 : 1];
-  switch (v) {
-    case 0: ilu_solve_g<T, LT, 16, 1>(ctx, P, lu, perm, r, yy, z); break;
-    case 1: ilu_solve_g<T, LT, 16, 2>(ctx, P, lu, perm, r, yy, z); break;
-    case 2: ilu_solve_g<T, LT, 16, 4>(ctx, P, lu, perm, r, yy, z); break;
-    case 3: ilu_solve_g<T, LT, 32, 1>(ctx, P, lu, perm, r, yy, z); break;
-    case 4: ilu_solve_g<T, LT, 32, 2>(ctx, P, lu, perm, r, yy, z); break;
-    default: ilu_solve_g<T, LT, 32, 4>(ctx, P, lu, perm, r, yy, z); break;
-  }
+// ILU(0) apply with the factor type of the current preconditioner.
+template <typename T>
+static void ilu_apply(sfem_ctx* ctx, const T* r, T* z, int force = -1, hipEvent_t* ev = nullptr) {
+  LinAlg* L = ctx->la;
+  const IluPlan& P = *L->ilu;
+  if (L->pc_bf16) ilu_solve<T, bf16>(ctx, P, L->lu16, r, z, force, ev);
+  else if (L->pc_fp32) ilu_solve<T, float>(ctx, P, L->lu32, r, z, force, ev);
+  else ilu_solve<T, double>(ctx, P, L->lu, r, z, force, ev);
 }
 
 template <typename T>
@@ -1134,24 +1289,9 @@ static int pc_apply_impl(sfem_ctx* ctx, const T* r, T* z) {
       else hipLaunchKernelGGL((k_bjac_apply<T, double>), dim3(gb), dim3(kTPB), 0, st, L->nblk, L->d_blkStart, L->binv, r, z);
       break;
     }
-    case SFEM_PC_ILU0: {
-      const IluPlan& P = *L->ilu;
-      T* y = nullptr;
-      if (!P.identity) {
-        int rc;
-        if (sizeof(T) == 8) {
-          if ((rc = ensure_vec(&L->pz, n))) return rc;
-          y = (T*)L->pz;
-        } else {
-          if ((rc = ensure_vec(&L->pz32, n))) return rc;
-          y = (T*)L->pz32;
-        }
-      }
-      if (L->pc_bf16) ilu_solve<T, bf16>(ctx, P, L->lu16, r, y, z);
-      else if (L->pc_fp32) ilu_solve<T, float>(ctx, P, L->lu32, r, y, z);
-      else ilu_solve<T, double>(ctx, P, L->lu, r, y, z);
+    case SFEM_PC_ILU0:
+      ilu_apply<T>(ctx, r, z);
       break;
-    }
     default:
       hipLaunchKernelGGL((k_copy<T>), dim3(g), dim3(kTPB), 0, st, n, r, z);
   }
@@ -1159,80 +1299,116 @@ static int pc_apply_impl(sfem_ctx* ctx, const T* r, T* z) {
 }
 
 // (lanes per node, columns in flight per lane) of the tunable kernels
-static const int kVarG[6] = {16, 16, 16, 32, 32, 32};
-static const int kVarU[6] = {1, 2, 4, 1, 2, 4};
-static const int kSpmvG[6] = {8, 8, 16, 16, 32, 32};
-static const int kSpmvU[6] = {2, 4, 2, 4, 2, 4};
+// SpMV kernel shapes: code = vi | (xcd << 5), G = kSpmvG[vi], U = kSpmvU[vi] (0: all
+// of a lane's columns, up to 128 / G, loaded at once).
+static const int kSpmvG[9] = {8, 8, 8, 16, 16, 16, 32, 32, 32};
+static const int kSpmvU[9] = {2, 4, 0, 2, 4, 0, 2, 4, 0};
+constexpr int kSpmvCodes = 9;
 
 template <typename VT, typename T>
-static void spmv_launch(hipStream_t st, int v, int64_t nb, const BlkDesc* desc, const int32_t* colidx, const VT* vals,
-                        const T* x, T* y) {
-  const int G = kSpmvG[v];
-  const unsigned g = (unsigned)((nb * G + 255) / 256);
-  switch (v) {
-    case 0: hipLaunchKernelGGL((k_spmv_blk<VT, T, 8, 2>), dim3(g), dim3(256), 0, st, nb, desc, colidx, vals, x, y); break;
-    case 1: hipLaunchKernelGGL((k_spmv_blk<VT, T, 8, 4>), dim3(g), dim3(256), 0, st, nb, desc, colidx, vals, x, y); break;
-    case 2: hipLaunchKernelGGL((k_spmv_blk<VT, T, 16, 2>), dim3(g), dim3(256), 0, st, nb, desc, colidx, vals, x, y); break;
-    case 3: hipLaunchKernelGGL((k_spmv_blk<VT, T, 16, 4>), dim3(g), dim3(256), 0, st, nb, desc, colidx, vals, x, y); break;
-    case 4: hipLaunchKernelGGL((k_spmv_blk<VT, T, 32, 2>), dim3(g), dim3(256), 0, st, nb, desc, colidx, vals, x, y); break;
-    default: hipLaunchKernelGGL((k_spmv_blk<VT, T, 32, 4>), dim3(g), dim3(256), 0, st, nb, desc, colidx, vals, x, y); break;
+static void spmv_launch(hipStream_t st, int code, int64_t nb, const BlkDesc* desc, const int32_t* colidx,
+                        const VT* vals, const T* x, T* y) {
+  const int vi = code & 31, xcd = code >> 5;
+  const unsigned g = (unsigned)((nb * kSpmvG[vi] + 255) / 256);
+#define SFEM_SPMV(i, G, U)                                                                                   \
+  case i:                                                                                                    \
+    hipLaunchKernelGGL((k_spmv_blk<VT, T, G, U>), dim3(g), dim3(256), 0, st, nb, desc, colidx, vals, x, y, xcd); \
+    break;
+  switch (vi) {
+    SFEM_SPMV(0, 8, 2) SFEM_SPMV(1, 8, 4) SFEM_SPMV(2, 8, 0)
+    SFEM_SPMV(3, 16, 2) SFEM_SPMV(4, 16, 4) SFEM_SPMV(5, 16, 0)
+    SFEM_SPMV(6, 32, 2) SFEM_SPMV(7, 32, 4)
+    default: hipLaunchKernelGGL((k_spmv_blk<VT, T, 32, 0>), dim3(g), dim3(256), 0, st, nb, desc, colidx, vals, x, y, xcd); break;
   }
+#undef SFEM_SPMV
 }
 
 template <typename T>
-static void spmv(sfem_ctx* ctx, const T* x, T* y) {
+static void spmv_in(sfem_ctx* ctx, const T* x, T* y, bool perm) {
   LinAlg* L = ctx->la;
   const int64_t nb = L->nblk;
   if (nb == 0) return;
   ctx->kt.begin(SFEM_KT_SPMV, ctx->stream);
   const int v = L->tune_spmv[sizeof(T) == 8 ? 0 : 1];
-  if (sizeof(T) == 8) spmv_launch<double, T>(ctx->stream, v, nb, L->d_desc, ctx->d_colidx, ctx->d_vals, x, y);
-  else spmv_launch<float, T>(ctx->stream, v, nb, L->d_desc, ctx->d_colidx, ctx->d_vals32, x, y);
+  const BlkDesc* desc = perm ? L->ilu->d_desc : L->d_desc;
+  const int32_t* col = perm ? L->ilu->colidx : ctx->d_colidx;
+  if (sizeof(T) == 8) spmv_launch<double, T>(ctx->stream, v, nb, desc, col, perm ? L->kp : ctx->d_vals, x, y);
+  else spmv_launch<float, T>(ctx->stream, v, nb, desc, col, perm ? L->kp32 : ctx->d_vals32, x, y);
   ctx->kt.end(SFEM_KT_SPMV, ctx->stream);
 }
 
-// Autotuner: time every compiled (G, U) variant of the SpMV and of the ILU(0) apply on
-// the context's own matrix (a few repetitions each, HIP events on the context stream)
-// and keep the fastest.  Runs once per precision, on the first solve.
+template <typename T>
+static void spmv(sfem_ctx* ctx, const T* x, T* y) {
+  spmv_in<T>(ctx, x, y, ctx->la->perm_space);
+}
+
+// Autotuner, run once per precision on the first solve with the context's own matrix
+// (HIP events on the context stream).  The SpMV keeps the fastest of its shapes; the
+// ILU(0) apply is timed level by level for every shape and each level keeps its own
+// fastest, since light levels (few lower or upper neighbours) favour few lanes per node
+// block and heavy ones many.
 template <typename T>
 static int autotune(sfem_ctx* ctx, T* x, T* y) {
   LinAlg* L = ctx->la;
   const int slot = sizeof(T) == 8 ? 0 : 1;
   hipStream_t st = ctx->stream;
-  hipEvent_t e0, e1;
-  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return set_error(SFEM_ERR_HIP, "autotune events");
   const bool on = ctx->kt.on;
   ctx->kt.on = false;
-  hipLaunchKernelGGL((k_fill<T>), dim3(grid_for(L->n)), dim3(kTPB), 0, st, L->n, x, (T)1);
-  auto time_it = [&](auto&& fn) {
-    fn();  // warm
-    (void)hipEventRecord(e0, st);
-    for (int k = 0; k < 3; ++k) fn();
-    (void)hipEventRecord(e1, st);
-    (void)hipEventSynchronize(e1);
-    float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    return ms;
+  std::vector<hipEvent_t> ev(2);
+  auto cleanup = [&] {
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
+    ctx->kt.on = on;
   };
+  for (hipEvent_t& e : ev) e = nullptr;
+  for (hipEvent_t& e : ev)
+    if (hipEventCreate(&e) != hipSuccess) { e = nullptr; cleanup(); return set_error(SFEM_ERR_HIP, "autotune events"); }
+  hipLaunchKernelGGL((k_fill<T>), dim3(grid_for(L->n)), dim3(kTPB), 0, st, L->n, x, (T)1);
   float best = 1e30f;
-  for (int v = 0; v < 6; ++v) {
-    L->tune_spmv[slot] = v;
-    const float ms = time_it([&] { spmv<T>(ctx, x, y); });
-    if (ms < best) { best = ms; L->tune_best_spmv[slot] = v; }
-  }
+  for (int xcd = 0; xcd < 2; ++xcd)
+    for (int vi = 0; vi < kSpmvCodes; ++vi) {
+      L->tune_spmv[slot] = vi | (xcd << 5);
+      spmv<T>(ctx, x, y);  // warm
+      (void)hipEventRecord(ev[0], st);
+      for (int k = 0; k < 3; ++k) spmv<T>(ctx, x, y);
+      (void)hipEventRecord(ev[1], st);
+      (void)hipEventSynchronize(ev[1]);
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
+      if (ms < best) { best = ms; L->tune_best_spmv[slot] = vi | (xcd << 5); }
+    }
   L->tune_spmv[slot] = L->tune_best_spmv[slot];
   if (L->pc_type == SFEM_PC_ILU0) {
-    best = 1e30f;
-    for (int v = 0; v < 6; ++v) {
-      L->tune_ilu[slot] = v;
-      const float ms = time_it([&] { (void)pc_apply<T>(ctx, x, y); });
-      if (ms < best) { best = ms; L->tune_best_ilu[slot] = v; }
-    }
-    L->tune_ilu[slot] = L->tune_best_ilu[slot];
+    const IluPlan& P = *L->ilu;
+    const int nl = (int)P.levLPtr.size() - 1, nu = (int)P.levUPtr.size() - 1;
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    ev.assign(nl + nu + 1, nullptr);
+    for (hipEvent_t& e : ev)
+      if (hipEventCreate(&e) != hipSuccess) { e = nullptr; cleanup(); return set_error(SFEM_ERR_HIP, "autotune events"); }
+    std::vector<float> bt(nl + nu, 1e30f), acc(nl + nu);
+    std::vector<int> bc(nl + nu, kTriDefault);
+    for (int xcd = 0; xcd < 2; ++xcd)
+      for (int vi = 0; vi < kTriCodes; ++vi) {
+        const int code = vi | (xcd << 5);
+        ilu_apply<T>(ctx, x, y, code);  // warm
+        std::fill(acc.begin(), acc.end(), 0.f);
+        for (int k = 0; k < 3; ++k) {
+          ilu_apply<T>(ctx, x, y, code, ev.data());
+          if (hipEventSynchronize(ev.back()) != hipSuccess) { cleanup(); return set_error(SFEM_ERR_HIP, "autotune"); }
+          for (int l = 0; l < nl + nu; ++l) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, ev[l], ev[l + 1]);
+            acc[l] += ms;
+          }
+        }
+        for (int l = 0; l < nl + nu; ++l)
+          if (acc[l] < bt[l]) { bt[l] = acc[l]; bc[l] = code; }
+      }
+    L->tune_L[slot].assign(bc.begin(), bc.begin() + nl);
+    L->tune_U[slot].assign(bc.begin() + nl, bc.end());
+    L->tune_plan[slot] = L->plan_id;
   }
-  ctx->kt.on = on;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  cleanup();
   L->tuned[slot] = true;
   return hipGetLastError() == hipSuccess ? SFEM_OK : set_error(SFEM_ERR_HIP, "autotune");
 }
@@ -1241,20 +1417,35 @@ int la_tuning(sfem_ctx* ctx, int* out) {
   LinAlg* L = ctx->la;
   for (int k = 0; k < 2; ++k) {
     const bool t = L && L->tuned[k];
-    out[4 * k + 0] = t ? kSpmvG[L->tune_spmv[k]] : 0;
-    out[4 * k + 1] = t ? kSpmvU[L->tune_spmv[k]] : 0;
-    out[4 * k + 2] = t && L->pc_type == SFEM_PC_ILU0 ? kVarG[L->tune_ilu[k]] : 0;
-    out[4 * k + 3] = t && L->pc_type == SFEM_PC_ILU0 ? kVarU[L->tune_ilu[k]] : 0;
+    const int c = t ? L->tune_spmv[k] : 0;
+    out[4 * k + 0] = t ? kSpmvG[c & 31] : 0;
+    out[4 * k + 1] = t ? kSpmvU[c & 31] : 0;
+    out[4 * k + 2] = t ? c >> 5 : 0;
+    out[4 * k + 3] = t ? (int)(L->tune_L[k].size() + L->tune_U[k].size()) : 0;
+  }
+  return SFEM_OK;
+}
+
+int la_tuning_levels(sfem_ctx* ctx, int fp32, int* out, int cap, int* nlevels) {
+  LinAlg* L = ctx->la;
+  const int k = fp32 ? 1 : 0;
+  std::vector<int> all(L->tune_L[k]);
+  all.insert(all.end(), L->tune_U[k].begin(), L->tune_U[k].end());
+  *nlevels = (int)all.size();
+  for (int i = 0; i < (int)all.size() && i < cap; ++i) {
+    out[3 * i + 0] = kTriG[all[i] & 31];
+    out[3 * i + 1] = kTriU[all[i] & 31];
+    out[3 * i + 2] = all[i] >> 5;
   }
   return SFEM_OK;
 }
 
 int la_spmv64(sfem_ctx* ctx, const double* x, double* y) {
-  spmv<double>(ctx, x, y);
+  spmv_in<double>(ctx, x, y, false);
   return hipGetLastError() == hipSuccess ? SFEM_OK : set_error(SFEM_ERR_HIP, "spmv launch");
 }
 int la_spmv32(sfem_ctx* ctx, const float* x, float* y) {
-  spmv<float>(ctx, x, y);
+  spmv_in<float>(ctx, x, y, false);
   return hipGetLastError() == hipSuccess ? SFEM_OK : set_error(SFEM_ERR_HIP, "spmv launch");
 }
 
@@ -1464,6 +1655,13 @@ static int gmres(sfem_ctx* ctx, const T* b, T* x, int m, double rtol, int maxit,
   return SFEM_OK;
 }
 
+// Shapes are tuned on the first solve of a precision and again when an ILU(0) plan
+// appears that they were not tuned for.
+static bool needs_tuning(const LinAlg* L, int slot) {
+  if (!L->tuned[slot]) return true;
+  return L->pc_type == SFEM_PC_ILU0 && L->tune_plan[slot] != L->plan_id;
+}
+
 // Symbolic set-up of the preconditioner for the given options (ILU(0): ordering,
 // permuted pattern and level sets), so that the first solve does not pay for it.
 int la_prepare(sfem_ctx* ctx, const sfem_solver_opts* o) {
@@ -1491,8 +1689,21 @@ int la_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int* iters, double* relre
     L->pc_reorder = reorder;
     L->pc_K_version = ctx->la_K_version;
   }
+  // a reordered ILU(0) plan moves the whole Krylov solve into its ordering
+  L->perm_space = L->pc_type == SFEM_PC_ILU0 && L->ilu && !L->ilu->identity;
+  const int32_t* perm = L->perm_space ? L->ilu->d_perm : nullptr;
+  const int gn = grid_for(n);
   const int m = std::max(1, o->restart);
   if (!fp32) {
+    const double* b = ctx->d_rhs;
+    double* x = ctx->d_du;
+    if (perm) {
+      if ((rc = ensure_vec(&L->pr, n)) || (rc = ensure_vec(&L->pz, n))) return rc;
+      hipLaunchKernelGGL((k_permute_in<double, double>), dim3(gn), dim3(kTPB), 0, ctx->stream, n, perm,
+                         ctx->d_rhs, L->pr, 1.0);
+      b = L->pr;
+      x = L->pz;
+    }
     if (o->ksp_type == SFEM_KSP_GMRES) {
       if ((rc = ensure_vec(&L->w, n)) || (rc = ensure_vec(&L->z, n))) return rc;
       if (!L->gm_V || L->gm_m < m) {
@@ -1501,21 +1712,27 @@ int la_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int* iters, double* relre
         if ((rc = ensure_vec(&L->gm_V, (int64_t)(m + 1) * n))) return rc;
         L->gm_m = m;
       }
-      return gmres<double>(ctx, ctx->d_rhs, ctx->d_du, m, o->rtol, o->max_it, iters, relres, L->gm_V,
-                           L->w, L->z);
+      rc = gmres<double>(ctx, b, x, m, o->rtol, o->max_it, iters, relres, L->gm_V, L->w, L->z);
+    } else {
+      double** vs[] = {&L->r, &L->rh, &L->p, &L->v, &L->s, &L->t, &L->ph, &L->sh};
+      for (double** q : vs)
+        if ((rc = ensure_vec(q, n))) return rc;
+      if (needs_tuning(L, 0) && (rc = autotune<double>(ctx, L->t, L->v))) return rc;
+      rc = bicgstab<double>(ctx, b, x, o->rtol, o->max_it, iters, relres, L->r, L->rh, L->p, L->v, L->s,
+                            L->t, L->ph, L->sh);
     }
-    double** vs[] = {&L->r, &L->rh, &L->p, &L->v, &L->s, &L->t, &L->ph, &L->sh};
-    for (double** q : vs)
-      if ((rc = ensure_vec(q, n))) return rc;
-    if (!L->tuned[0] && (rc = autotune<double>(ctx, L->t, L->v))) return rc;
-    return bicgstab<double>(ctx, ctx->d_rhs, ctx->d_du, o->rtol, o->max_it, iters, relres, L->r, L->rh,
-                            L->p, L->v, L->s, L->t, L->ph, L->sh);
+    if (perm) {
+      hipLaunchKernelGGL((k_permute_out<double, double>), dim3(gn), dim3(kTPB), 0, ctx->stream, n, perm, x,
+                         ctx->d_du, 1.0);
+      if (hipGetLastError() != hipSuccess) return set_error(SFEM_ERR_HIP, "permute launch");
+    }
+    return rc;
   }
   // mixed precision
   float** vs[] = {&L->r32, &L->rh32, &L->p32, &L->v32, &L->s32, &L->t32, &L->ph32, &L->sh32, &L->x32};
   for (float** q : vs)
     if ((rc = ensure_vec(q, n))) return rc;
-  if (!L->tuned[1] && (rc = autotune<float>(ctx, L->t32, L->v32))) return rc;
+  if (needs_tuning(L, 1) && (rc = autotune<float>(ctx, L->t32, L->v32))) return rc;
   double bn;
   if ((rc = la_norm2(ctx, ctx->d_rhs, &bn))) return rc;
   const int g = grid_for(n);
@@ -1526,7 +1743,12 @@ int la_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int* iters, double* relre
   }
   // b32 = rhs / ||rhs|| into the FP32 "s" slot's sibling (rh32 is overwritten by the solver, use x32 as b)
   float* b32 = L->x32;
-  hipLaunchKernelGGL((k_convert<double, float>), dim3(g), dim3(kTPB), 0, ctx->stream, n, ctx->d_rhs, b32, 1.0 / bn);
+  if (perm)
+    hipLaunchKernelGGL((k_permute_in<double, float>), dim3(g), dim3(kTPB), 0, ctx->stream, n, perm, ctx->d_rhs,
+                       b32, 1.0 / bn);
+  else
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(g), dim3(kTPB), 0, ctx->stream, n, ctx->d_rhs, b32,
+                       1.0 / bn);
   if (!L->du32) { if ((rc = ensure_vec(&L->du32, n))) return rc; }
   if (o->ksp_type == SFEM_KSP_GMRES) {
     if ((rc = ensure_vec(&L->w32, n)) || (rc = ensure_vec(&L->z32, n))) return rc;
@@ -1541,7 +1763,11 @@ int la_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int* iters, double* relre
     rc = bicgstab<float>(ctx, b32, L->du32, o->rtol, o->max_it, iters, relres, L->r32, L->rh32, L->p32,
                          L->v32, L->s32, L->t32, L->ph32, L->sh32);
   }
-  hipLaunchKernelGGL((k_convert<float, double>), dim3(g), dim3(kTPB), 0, ctx->stream, n, L->du32, ctx->d_du, bn);
+  if (perm)
+    hipLaunchKernelGGL((k_permute_out<float, double>), dim3(g), dim3(kTPB), 0, ctx->stream, n, perm, L->du32,
+                       ctx->d_du, bn);
+  else
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(g), dim3(kTPB), 0, ctx->stream, n, L->du32, ctx->d_du, bn);
   if (hipGetLastError() != hipSuccess) return set_error(SFEM_ERR_HIP, "convert launch");
   return rc;
 }

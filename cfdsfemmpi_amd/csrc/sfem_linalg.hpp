@@ -40,9 +40,10 @@ struct LinAlg {
   int64_t nblk = 0;          // nodal blocks (nodes with >= 1 free DOF)
   int maxrow = 0;
   // autotuned kernel variants, [0] FP64 and [1] FP32
-  int tune_spmv[2] = {3, 3}, tune_ilu[2] = {1, 1};
-  int tune_best_spmv[2] = {3, 3}, tune_best_ilu[2] = {1, 1};
+  int tune_spmv[2] = {3, 3}, tune_best_spmv[2] = {3, 3};
+  std::vector<int> tune_L[2], tune_U[2];   // per level of the forward / backward solve
   bool tuned[2] = {false, false};
+  int plan_id = 0, tune_plan[2] = {-1, -1};  // ILU plan the per-level shapes were tuned for
   int pc_type = SFEM_PC_NONE;
   bool pc_fp32 = false;
   int64_t pc_K_version = -1;  // tangent version the preconditioner was built from
@@ -73,6 +74,11 @@ struct LinAlg {
   float* binv32 = nullptr;
   double* lu = nullptr;
   float* lu32 = nullptr;
+  // With a reordered ILU(0) plan the Krylov solve runs in the plan's ordering: the
+  // tangent is kept permuted as well (kp / kp32) and only rhs and solution are permuted.
+  bool perm_space = false;
+  double* kp = nullptr;
+  float* kp32 = nullptr;
   ~LinAlg();
 };
 
@@ -82,6 +88,7 @@ int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f);
 int la_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int* iters, double* relres);
 int la_prepare(sfem_ctx* ctx, const sfem_solver_opts* o);
 int la_tuning(sfem_ctx* ctx, int* out);
+int la_tuning_levels(sfem_ctx* ctx, int fp32, int* out, int cap, int* nlevels);
 int la_norm2(sfem_ctx* ctx, const double* a, double* out);
 int la_spmv64(sfem_ctx* ctx, const double* x, double* y);
 int la_spmv32(sfem_ctx* ctx, const float* x, float* y);

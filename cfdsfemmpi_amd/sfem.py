@@ -77,6 +77,7 @@ ABI = [
     ("sfem_enable_kernel_timing", C.c_int, [_vp, C.c_int]),
     ("sfem_kernel_timing", C.c_int, [_vp, C.c_int, _i64p, _f64p]),
     ("sfem_tuning", C.c_int, [_vp, C.POINTER(C.c_int)]),
+    ("sfem_tuning_levels", C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]),
     ("sfem_info", C.c_int, [_vp, _i64p, _i64p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("sfem_last_timings", C.c_int, [_vp, _f64p, _f64p]),
 ]
@@ -263,10 +264,23 @@ class Context:
         return n.value, ms.value
 
     def tuning(self):
+        """Autotuned kernel shapes per precision: SpMV (G, U, XCD order) and the
+        per-level ILU(0) apply shapes, forward levels then backward levels."""
         out = (C.c_int * 8)()
         _check(self.lib.sfem_tuning(self.h, out), "sfem_tuning")
         v = list(out)
-        return {"fp64": {"spmv_GU": v[0:2], "ilu_GU": v[2:4]}, "fp32": {"spmv_GU": v[4:6], "ilu_GU": v[6:8]}}
+        res = {}
+        for k, name in ((0, "fp64"), (1, "fp32")):
+            w = v[4 * k:4 * k + 4]
+            levels = []
+            if w[3]:
+                buf = (C.c_int * (3 * w[3]))()
+                n = C.c_int(0)
+                _check(self.lib.sfem_tuning_levels(self.h, k, buf, w[3], C.byref(n)), "sfem_tuning_levels")
+                b = list(buf)
+                levels = [tuple(b[3 * i:3 * i + 3]) for i in range(n.value)]
+            res[name] = {"spmv_GUX": w[0:3], "ilu_levels_GUX": levels}
+        return res
 
     def info(self):
         nf, nz = C.c_int64(0), C.c_int64(0)

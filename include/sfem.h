@@ -183,10 +183,17 @@ enum { SFEM_KT_ASSEMBLY = 0, SFEM_KT_SPMV = 1, SFEM_KT_PC = 2, SFEM_KT_NKINDS = 
 int sfem_enable_kernel_timing(sfem_ctx* ctx, int enable);
 int sfem_kernel_timing(sfem_ctx* ctx, int kind, int64_t* launches, double* total_ms);
 
-/* Kernel shapes chosen by the solver's autotuner (lanes per node G and columns in
- * flight per lane U), as out[8] = {SpMV G, U, ILU-apply G, U} for FP64 then FP32;
- * 0 where not tuned yet.  No counterpart in the reference. */
+/* Kernel shapes chosen by the solver's autotuner, as out[8] = {SpMV lanes per node G,
+ * columns in flight per lane U, XCD-contiguous workgroup order (0/1), number of tuned
+ * ILU(0) levels} for FP64 then FP32; 0 where not tuned yet.  No counterpart in the
+ * reference. */
 int sfem_tuning(sfem_ctx* ctx, int* out);
+
+/* Per-level shapes of the ILU(0) apply for one precision (fp32 = 0 FP64, 1 FP32):
+ * out[3*l .. 3*l+2] = {G, U, XCD order} for the forward levels, then the backward
+ * levels; *nlevels receives the count, at most cap entries are written.  No
+ * counterpart in the reference. */
+int sfem_tuning_levels(sfem_ctx* ctx, int fp32, int* out, int cap, int* nlevels);
 
 /* Sizes and counts of the context. */
 int sfem_info(sfem_ctx* ctx, int64_t* nfree, int64_t* nnz, int* ncolors, int* ndof);
