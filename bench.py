@@ -167,6 +167,8 @@ def main():
     ap.add_argument("--bf16-factors", type=int, default=0, help="mixed path: ILU(0) factors stored in bfloat16")
     ap.add_argument("--predictor", choices=["previous", "extrapolate"], default="previous",
                     help="Newton initial iterate of each step (both paths)")
+    ap.add_argument("--ksp", choices=["bcgs", "gmres"], default="bcgs", help="Krylov method of both paths")
+    ap.add_argument("--restart", type=int, default=30, help="GMRES restart length")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run 5 device-to-device copies of this many MiB (HBM counter calibration)")
     a = ap.parse_args()
@@ -182,8 +184,9 @@ def main():
     n = a.n or (115 if a.workload == "c3" else 32)
     mesh, ctl = make_case(n)
     log(f"workload {n}^3 hexes: {mesh.nElem} elements, {mesh.nNode} nodes")
-    o64 = sfem.solver_opts("bcgs", "ilu", 1e-8, 5000, reorder=a.reorder)
-    omx = sfem.solver_opts("bcgs", "ilu", a.inner_rtol, 5000, reorder=a.reorder, factor_bf16=a.bf16_factors)
+    o64 = sfem.solver_opts(a.ksp, "ilu", 1e-8, 5000, a.restart, reorder=a.reorder)
+    omx = sfem.solver_opts(a.ksp, "ilu", a.inner_rtol, 5000, a.restart, reorder=a.reorder,
+                           factor_bf16=a.bf16_factors)
     pred = sfem.PRED_EXTRAPOLATE if a.predictor == "extrapolate" else sfem.PRED_PREVIOUS
     r64 = rmx = None
     if a.only != "mixed":
@@ -252,7 +255,7 @@ def main():
         "data": "synthetic (generated mesh, zero initial state)",
         "config": {"workload": f"3D lid-driven cavity, {n}^3 graded Hexa8, Re=1000, gen-alpha rho_inf=0, "
                                f"dt=0.1, {a.steps} steps", "n_elements": mesh.nElem, "nfree": nf, "nnz": nnz,
-                   "solver": "BiCGStab + ILU(0)", "ilu_ordering": ["natural", "rcm", "multicolour"][a.reorder],
+                   "solver": ("BiCGStab" if a.ksp == "bcgs" else f"GMRES({a.restart})") + " + ILU(0)", "ilu_ordering": ["natural", "rcm", "multicolour"][a.reorder],
                    "newton_tol": a.newton_tol,
                    "inner_rtol_mixed": a.inner_rtol, "ilu_factors_mixed": "bf16" if a.bf16_factors else "fp32",
                    "newton_initial_iterate": a.predictor, "parallelism": "single GPU"},

@@ -736,6 +736,17 @@ int sfem_tuning(sfem_ctx* c, int* out) {
   return la_tuning(c, out);
 }
 
+int sfem_get_ilu(sfem_ctx* c, int64_t* nnz, int32_t* perm, int32_t* rowptr, int32_t* colidx, double* vals) {
+  if (!c) return set_error(SFEM_ERR_ARG, "null context");
+  return la_get_ilu(c, nnz, perm, rowptr, colidx, vals);
+}
+
+int sfem_solver_stats(sfem_ctx* c, int64_t* fp32_fallbacks) {
+  if (!c || !fp32_fallbacks) return set_error(SFEM_ERR_ARG, "null argument");
+  *fp32_fallbacks = c->la ? c->la->fallbacks : 0;
+  return SFEM_OK;
+}
+
 int sfem_tuning_levels(sfem_ctx* c, int fp32, int* out, int cap, int* nlevels) {
   if (!c || !nlevels || (cap > 0 && !out) || cap < 0) return set_error(SFEM_ERR_ARG, "bad argument");
   if (!c->la) { *nlevels = 0; return SFEM_OK; }

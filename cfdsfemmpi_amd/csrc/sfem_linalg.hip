@@ -27,6 +27,7 @@ namespace sfem {
 
 constexpr int kRedBlocks = 1024;  // partials per reduction
 constexpr int kTPB = 256;
+constexpr int kGmresMaxH = 2 * (256 + 8) + 8;  // device / pinned Hessenberg scratch (restart <= 256)
 
 // bfloat16 storage for the mixed path's ILU(0) factors: the top 16 bits of an IEEE
 // float, rounded to nearest even.  Loads widen it back to float exactly.
@@ -98,6 +99,33 @@ __global__ __launch_bounds__(kTPB) void k_dot2(int64_t n, const T* __restrict__ 
     v[1] += (double)a2[i] * (double)b2[i];
   }
   block_sum_store<2>(v, part);
+}
+
+// Partial dot products of w with up to 8 vectors V_j (stored n apart): part[b*8 + j];
+// absent vectors give 0.
+template <typename T>
+__global__ __launch_bounds__(kTPB) void k_mdot8(int64_t n, int nv, const T* __restrict__ V,
+                                                const T* __restrict__ w, double* part) {
+  double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int64_t i = (int64_t)blockIdx.x * kTPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kTPB) {
+    const double wi = (double)w[i];
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (j < nv) v[j] += (double)V[(int64_t)j * n + i] * wi;
+  }
+  block_sum_store<8>(v, part);
+}
+
+// w = w - sum_{j<nv} h[j] V_j  (sign = -1), or w = w + sum (sign = +1); h on the device.
+template <typename T>
+__global__ __launch_bounds__(kTPB) void k_mupdate(int64_t n, int nv, const T* __restrict__ V,
+                                                  const double* __restrict__ h, double sign,
+                                                  T* __restrict__ w) {
+  for (int64_t i = (int64_t)blockIdx.x * kTPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kTPB) {
+    double acc = 0.0;
+    for (int j = 0; j < nv; ++j) acc += h[j] * (double)V[(int64_t)j * n + i];
+    w[i] = (T)((double)w[i] + sign * acc);
+  }
 }
 
 // Finalisation ops: combine summed partials into Krylov scalars.
@@ -466,6 +494,161 @@ struct __align__(16) BlkDesc {
   int32_t r0, base, rl, dqm;
 };
 
+// Ordering point for the lanes of one wavefront exchanging data through LDS: a
+// workgroup of exactly one wavefront needs no s_barrier (its LDS operations complete in
+// issue order), only fences that keep the compiler from moving LDS accesses across this
+// point; unlike __syncthreads() it does not wait for outstanding global loads.
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ILU(0) factorisation of the node blocks of one level, one 64-lane workgroup (one
+// wavefront) per block, organised by lower node blocks: the columns of row block i fall
+// into groups, one per neighbouring node block, and every row of a lower block c shares
+// c's column list.  So per lower block the positions (in i's list) of c's upper columns
+// are searched once, c's m_c upper rows are staged in LDS, and the m_c eliminations of
+// IKJ order run from LDS; the global loads of the next lower block are issued before the
+// eliminations of the current one and land in registers meanwhile.  The result equals
+// the row-by-row IKJ factorisation (same updates in the same order).  Dynamic LDS:
+// factor_nb_lds(cap) bytes for rows of at most cap (a multiple of 64, <= kIluMaxRow)
+// entries.
+template <typename LT>
+__host__ __device__ constexpr size_t factor_nb_lds(int cap) {
+  return (size_t)cap * (4 * sizeof(double) + 4 * sizeof(LT) + 3 * sizeof(int32_t) + sizeof(BlkDesc));
+}
+
+template <typename LT>
+__global__ __launch_bounds__(64) void k_ilu_factor_nb(const int32_t* __restrict__ list, int first, int count,
+                                                      const BlkDesc* __restrict__ desc,
+                                                      const int32_t* __restrict__ colidx,
+                                                      const int32_t* __restrict__ rowBlk, LT* __restrict__ lu,
+                                                      double* __restrict__ scal, int cap) {
+  extern __shared__ double smem[];
+  double* rows = smem;                              // [4][cap], this block's rows
+  LT* urow = (LT*)(rows + 4 * cap);                 // [4][cap], external upper part of c's rows
+  int32_t* cols = (int32_t*)(urow + 4 * cap);       // [cap], this block's column list
+  int32_t* pos = cols + cap;                        // [cap], positions of c's upper columns
+  int32_t* gq = pos + cap;                          // [cap], first column of each lower group
+  BlkDesc* gdesc = (BlkDesc*)(gq + cap);            // [cap], descriptor of each lower group
+  __shared__ double ub[16];                         // c's diagonal block
+  __shared__ double lval[4];
+  __shared__ int ngroups;
+  constexpr int S = kIluMaxRow / 64;                // prefetch slots per lane
+  const int w = blockIdx.x;
+  if (w >= count) return;
+  const int lane = threadIdx.x;
+  const BlkDesc d = desc[list ? list[first + w] : first + w];
+  const int m = d.dqm >> 16, dq0 = d.dqm & 0xffff, rl = d.rl;
+  for (int q = lane; q < rl; q += 64) cols[q] = colidx[d.base + q];
+  for (int a = 0; a < m; ++a)
+    for (int q = lane; q < rl; q += 64) rows[a * cap + q] = (double)lu[d.base + (int64_t)a * rl + q];
+  // lower node groups: block of every lower column, group starts compacted by ballot
+  for (int q = lane; q < dq0; q += 64) pos[q] = rowBlk[colidx[d.base + q]];
+  if (lane == 0) ngroups = 0;
+  wave_sync();
+  for (int q0 = 0; q0 < dq0; q0 += 64) {
+    const int q = q0 + lane;
+    const bool start = q < dq0 && (q == 0 || pos[q] != pos[q - 1]);
+    const uint64_t mask = __ballot(start);
+    const int base = ngroups;
+    if (start) {
+      const int k = base + __popcll(mask & ((1ull << lane) - 1ull));
+      gq[k] = q;
+      gdesc[k] = desc[pos[q]];
+    }
+    wave_sync();
+    if (lane == 0) ngroups = base + __popcll(mask);
+    wave_sync();
+  }
+  const int ng = ngroups;
+  // registers of the prefetched lower block: column indices, upper values, diagonal block
+  int32_t pc[S];
+  LT pu[S][4];
+  LT pd = (LT)0;
+  auto issue = [&](int g) {
+    const BlkDesc dc = gdesc[g];
+    const int mc = dc.dqm >> 16, dq0c = dc.dqm & 0xffff;
+    const int ext0 = dq0c + mc, next = dc.rl - ext0;
+#pragma unroll
+    for (int sl = 0; sl < S; ++sl) {
+      const int t = lane + 64 * sl;
+      if (t < next) {
+        pc[sl] = colidx[dc.base + ext0 + t];
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          if (b < mc) pu[sl][b] = lu[dc.base + (int64_t)b * dc.rl + ext0 + t];
+      }
+    }
+    const int b = lane >> 2, bp = lane & 3;
+    if (lane < 16 && b < mc && bp < mc) pd = lu[dc.base + (int64_t)b * dc.rl + dq0c + bp];
+  };
+  if (ng > 0) issue(0);
+  for (int g = 0; g < ng; ++g) {
+    const int q = gq[g];
+    const BlkDesc dc = gdesc[g];
+    const int mc = dc.dqm >> 16, dq0c = dc.dqm & 0xffff;
+    const int next = dc.rl - (dq0c + mc);
+#pragma unroll
+    for (int sl = 0; sl < S; ++sl) {
+      const int t = lane + 64 * sl;
+      if (t < next) {
+        pos[t] = lds_find(cols, rl, pc[sl]);
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          if (b < mc) urow[b * cap + t] = pu[sl][b];
+      }
+    }
+    if (lane < 16) ub[lane] = ((lane >> 2) < mc && (lane & 3) < mc) ? (double)pd : 0.0;
+    wave_sync();
+    if (g + 1 < ng) issue(g + 1);
+    for (int b = 0; b < mc; ++b) {
+      if (lane < m) {
+        const double dd = ub[b * 4 + b];
+        if (dd == 0.0) scal[S_MISC1] = 2.0;
+        const double l = rows[lane * cap + q + b] / (dd == 0.0 ? 1.0 : dd);
+        rows[lane * cap + q + b] = l;
+        lval[lane] = l;
+      }
+      wave_sync();
+      const int nin = mc - 1 - b;  // remaining columns of block c in this row block
+      for (int t = lane; t < nin + next; t += 64) {
+        int p;
+        double u;
+        if (t < nin) {
+          p = q + b + 1 + t;
+          u = ub[b * 4 + b + 1 + t];
+        } else {
+          p = pos[t - nin];
+          u = (double)urow[b * cap + t - nin];
+        }
+        if (p >= 0)
+          for (int a = 0; a < m; ++a) rows[a * cap + p] -= lval[a] * u;
+      }
+      wave_sync();
+    }
+  }
+  // the block's own diagonal block
+  for (int a = 1; a < m; ++a) {
+    for (int ap = 0; ap < a; ++ap) {
+      if (lane == 0) {
+        const double dd = rows[ap * cap + dq0 + ap];
+        if (dd == 0.0) scal[S_MISC1] = 2.0;
+        const double l = rows[a * cap + dq0 + ap] / (dd == 0.0 ? 1.0 : dd);
+        rows[a * cap + dq0 + ap] = l;
+        lval[0] = l;
+      }
+      wave_sync();
+      const double l = lval[0];
+      for (int qq = dq0 + ap + 1 + lane; qq < rl; qq += 64) rows[a * cap + qq] -= l * rows[ap * cap + qq];
+      wave_sync();
+    }
+  }
+  for (int a = 0; a < m; ++a)
+    for (int qq = lane; qq < rl; qq += 64) lu[d.base + (int64_t)a * rl + qq] = (LT)rows[a * cap + qq];
+}
+
 // acc[a] += sum_{q in [q0, q1)} V[a*rl + q] * x[col[q]] for a < m, lanes of a group of G
 // stepping by G; U columns per lane are loaded before any is used, so a lane keeps
 // U index loads, U gathers and U*m value loads in flight.
@@ -708,11 +891,13 @@ static hipError_t alloc(T** p, size_t n) {
 LinAlg::~LinAlg() {
   void* ptrs[] = {d_blkStart, d_diagpos, d_part, d_scal, r, rh, p, v, s, t, ph, sh, x, w, z, pr, pz,
                   r32, rh32, p32, v32, s32, t32, ph32, sh32, x32, du32, w32, z32, pr32, pz32,
-                  dinv, dinv32, binv, binv32, lu, lu32, gm_V, gm_V32, d_desc, lu16, kp, kp32};
+                  dinv, dinv32, binv, binv32, lu, lu32, gm_V, gm_V32, d_desc, lu16, kp, kp32, d_rowBlk,
+                  d_h, d_mpart};
   delete ilu;
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (h_scal) (void)hipHostFree(h_scal);
+  if (h_h) (void)hipHostFree(h_h);
 }
 
 static std::vector<BlkDesc> make_desc(const std::vector<int32_t>& bs, const std::vector<int64_t>& rowptr,
@@ -751,6 +936,26 @@ int la_create(sfem_ctx* ctx) {
   LA_CHECK(hipMemcpy(L->d_blkStart, bs.data(), bs.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   LA_CHECK(alloc(&L->d_diagpos, n));
   LA_CHECK(hipMemcpy(L->d_diagpos, dp.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+  {
+    std::vector<int32_t> rowBlk(n);
+    for (int64_t k = 0; k < L->nblk; ++k)
+      for (int32_t r = bs[k]; r < bs[k + 1]; ++r) rowBlk[r] = (int32_t)k;
+    // full-block check on the first row of every block (the rows of a block share it)
+    bool full = true;
+    for (int64_t k = 0; k < L->nblk && full; ++k) {
+      const int64_t b = ctx->rowptr[bs[k]], e = ctx->rowptr[bs[k] + 1];
+      for (int64_t q = b; q < e && full;) {
+        const int32_t c = rowBlk[ctx->colidx[q]];
+        const int32_t len = bs[c + 1] - bs[c];
+        full = ctx->colidx[q] == bs[c] && q + len <= e;
+        for (int32_t t = 0; t < len && full; ++t) full = ctx->colidx[q + t] == bs[c] + t;
+        q += len;
+      }
+    }
+    L->full_blocks = full;
+    LA_CHECK(alloc(&L->d_rowBlk, rowBlk.size()));
+    if (n) LA_CHECK(hipMemcpy(L->d_rowBlk, rowBlk.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+  }
   {
     std::vector<BlkDesc> dd = make_desc(bs, ctx->rowptr, dp);
     LA_CHECK(alloc(&L->d_desc, dd.size()));
@@ -960,7 +1165,7 @@ static std::vector<BlkDesc> make_desc(const std::vector<int32_t>& bs, const std:
 }
 
 IluPlan::~IluPlan() {
-  void* ptrs[] = {d_levL, d_levU, d_rowptr, d_colidx, d_diagpos, d_blkStart, d_pmap, d_perm, d_desc};
+  void* ptrs[] = {d_levL, d_levU, d_rowptr, d_colidx, d_diagpos, d_blkStart, d_pmap, d_perm, d_desc, d_rowBlk};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
 }
@@ -981,6 +1186,7 @@ static int build_ilu_plan(sfem_ctx* ctx, LinAlg* L, int reorder) {
     P.colidx = ctx->d_colidx;
     P.diagpos = L->d_diagpos;
     P.blkStart = L->d_blkStart;
+    P.rowBlk = L->d_rowBlk;
   } else {
     // block graph from the first row of each block
     std::vector<int32_t> rowBlk(L->n);
@@ -1052,10 +1258,15 @@ static int build_ilu_plan(sfem_ctx* ctx, LinAlg* L, int reorder) {
       if (e != hipSuccess) return set_hip_error(e, "ILU descriptors");
     }
     std::vector<int32_t> rp32(rp.begin(), rp.end());
+    std::vector<int32_t> rb(L->n);
+    for (int64_t p = 0; p < nb; ++p)
+      for (int32_t r = nbs[p]; r < nbs[p + 1]; ++r) rb[r] = (int32_t)p;
     int rc;
     if ((rc = upload(&P.d_rowptr, rp32)) || (rc = upload(&P.d_colidx, ci)) || (rc = upload(&P.d_diagpos, dp)) ||
-        (rc = upload(&P.d_blkStart, nbs)) || (rc = upload(&P.d_pmap, pmap)) || (rc = upload(&P.d_perm, perm)))
+        (rc = upload(&P.d_blkStart, nbs)) || (rc = upload(&P.d_pmap, pmap)) || (rc = upload(&P.d_perm, perm)) ||
+        (rc = upload(&P.d_rowBlk, rb)))
       return rc;
+    P.rowBlk = P.d_rowBlk;
     P.rowptr = P.d_rowptr;
     P.colidx = P.d_colidx;
     P.diagpos = P.d_diagpos;
@@ -1164,7 +1375,16 @@ int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f) 
       const int32_t* list; int first, cnt;
       level_range(P, false, lv, &list, &first, &cnt);
       if (!cnt) continue;
-      if (fp32)
+      const BlkDesc* desc = P.identity ? L->d_desc : P.d_desc;
+      if (L->full_blocks) {
+        const int cap = std::max(64, (L->maxrow + 63) / 64 * 64);
+        if (fp32)
+          hipLaunchKernelGGL((k_ilu_factor_nb<float>), dim3(cnt), dim3(64), factor_nb_lds<float>(cap), st, list,
+                             first, cnt, desc, P.colidx, P.rowBlk, L->lu32, L->d_scal, cap);
+        else
+          hipLaunchKernelGGL((k_ilu_factor_nb<double>), dim3(cnt), dim3(64), factor_nb_lds<double>(cap), st,
+                             list, first, cnt, desc, P.colidx, P.rowBlk, L->lu, L->d_scal, cap);
+      } else if (fp32)
         hipLaunchKernelGGL((k_ilu_factor_blk<float>), dim3(cnt), dim3(64), 0, st, list, first, cnt,
                            P.blkStart, P.rowptr, P.colidx, P.diagpos, L->lu32, L->d_scal);
       else
@@ -1413,6 +1633,43 @@ static int autotune(sfem_ctx* ctx, T* x, T* y) {
   return hipGetLastError() == hipSuccess ? SFEM_OK : set_error(SFEM_ERR_HIP, "autotune");
 }
 
+// The current ILU(0) factor in its plan's ordering: row permutation (plan row -> DOF
+// row), pattern and values (FP32 / BF16 factors widened to double).
+int la_get_ilu(sfem_ctx* ctx, int64_t* nnz_out, int32_t* perm, int32_t* rowptr, int32_t* colidx, double* vals) {
+  LinAlg* L = ctx->la;
+  if (!L || L->pc_type != SFEM_PC_ILU0 || !L->ilu || L->pc_K_version < 0)
+    return set_error(SFEM_ERR_STATE, "no ILU(0) factor has been built");
+  const IluPlan& P = *L->ilu;
+  const int64_t n = L->n, nnz = ctx->nnz;
+  if (nnz_out) *nnz_out = nnz;
+  LA_CHECK(hipStreamSynchronize(ctx->stream));
+  if (perm) {
+    if (P.identity) for (int64_t i = 0; i < n; ++i) perm[i] = (int32_t)i;
+    else LA_CHECK(hipMemcpy(perm, P.d_perm, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  }
+  if (rowptr) LA_CHECK(hipMemcpy(rowptr, P.rowptr, (n + 1) * sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (colidx) LA_CHECK(hipMemcpy(colidx, P.colidx, nnz * sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (vals) {
+    if (L->pc_bf16) {
+      std::vector<uint16_t> h(nnz);
+      LA_CHECK(hipMemcpy(h.data(), L->lu16, nnz * sizeof(uint16_t), hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < nnz; ++i) {
+        const uint32_t u = (uint32_t)h[i] << 16;
+        float f;
+        std::memcpy(&f, &u, sizeof(f));
+        vals[i] = f;
+      }
+    } else if (L->pc_fp32) {
+      std::vector<float> h(nnz);
+      LA_CHECK(hipMemcpy(h.data(), L->lu32, nnz * sizeof(float), hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < nnz; ++i) vals[i] = h[i];
+    } else {
+      LA_CHECK(hipMemcpy(vals, L->lu, nnz * sizeof(double), hipMemcpyDeviceToHost));
+    }
+  }
+  return SFEM_OK;
+}
+
 int la_tuning(sfem_ctx* ctx, int* out) {
   LinAlg* L = ctx->la;
   for (int k = 0; k < 2; ++k) {
@@ -1464,9 +1721,14 @@ int la_norm2(sfem_ctx* ctx, const double* a, double* out) {
   return SFEM_OK;
 }
 
+// Internal status of bicgstab with a stall window: no 10% improvement of the best
+// residual for `stall` iterations.
+constexpr int kStagnated = -100;
+constexpr int kBicgStall = 100;  // stall window of the FP32 inner BiCGStab
+
 template <typename T>
 static int bicgstab(sfem_ctx* ctx, const T* b, T* x, double rtol, int maxit, int* iters,
-                    double* relres, T* r, T* rh, T* p, T* v, T* s, T* t, T* ph, T* sh) {
+                    double* relres, T* r, T* rh, T* p, T* v, T* s, T* t, T* ph, T* sh, int stall = 0) {
   LinAlg* L = ctx->la;
   hipStream_t st = ctx->stream;
   const int64_t n = L->n;
@@ -1502,6 +1764,8 @@ static int bicgstab(sfem_ctx* ctx, const T* b, T* x, double rtol, int maxit, int
   double rnorm = bnorm;
   bool first = true;
   int restarts = 0;
+  double best = bnorm;
+  int best_it = 0;
   for (int it = 1; it <= maxit; ++it) {
     hipLaunchKernelGGL((k_dot<T>), dim3(g), dim3(kTPB), 0, st, n, rh, r, L->d_part);
     hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(kTPB), 0, st, L->d_part, g, sc, 0, (int)FIN_RHO1);
@@ -1532,6 +1796,13 @@ static int bicgstab(sfem_ctx* ctx, const T* b, T* x, double rtol, int maxit, int
       *relres = rnorm / bnorm;
       return SFEM_OK;
     }
+    if (rnorm < 0.9 * best) {
+      best = rnorm;
+      best_it = it;
+    } else if (stall > 0 && it - best_it >= stall) {
+      *relres = rnorm / bnorm;
+      return kStagnated;
+    }
     if (hs[S_FLAG] == 3.0) {
       // near breakdown: restart with the current residual as shadow residual
       if (++restarts > 100) {
@@ -1550,8 +1821,12 @@ static int bicgstab(sfem_ctx* ctx, const T* b, T* x, double rtol, int maxit, int
   return set_error(SFEM_ERR_MAXIT, "BiCGStab: maximum iterations reached");
 }
 
-// Restarted GMRES(m), right preconditioned, modified Gram-Schmidt; the small
-// Hessenberg least-squares problem is solved on the host with Givens rotations.
+// Restarted GMRES(m), right preconditioned, with classical Gram-Schmidt applied twice
+// (CGS2: the projections onto all earlier basis vectors are one batched reduction and
+// one batched update per pass, so the orthogonalisation is 2 passes over the basis
+// instead of k dependent dot products), deterministic reductions, and one host
+// synchronisation per iteration to fetch the new Hessenberg column.  The small
+// least-squares problem is solved on the host with Givens rotations.
 template <typename T>
 static int gmres(sfem_ctx* ctx, const T* b, T* x, int m, double rtol, int maxit, int* iters,
                  double* relres, T* V, T* w, T* z) {
@@ -1559,34 +1834,53 @@ static int gmres(sfem_ctx* ctx, const T* b, T* x, int m, double rtol, int maxit,
   hipStream_t st = ctx->stream;
   const int64_t n = L->n;
   const int g = grid_for(n);
-  double* sc = L->d_scal;
-  double* hs = L->h_scal;
-  auto dotv = [&](const T* a, const T* c, double* out) -> int {
-    hipLaunchKernelGGL((k_dot<T>), dim3(g), dim3(kTPB), 0, st, n, a, c, L->d_part);
-    hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(kTPB), 0, st, L->d_part, g, sc, (int)S_MISC0, (int)FIN_STORE);
+  if (!L->d_h) {
+    LA_CHECK(alloc(&L->d_h, (size_t)kGmresMaxH));
+    LA_CHECK(alloc(&L->d_mpart, (size_t)kRedBlocks * 8));
+    LA_CHECK(hipHostMalloc((void**)&L->h_h, kGmresMaxH * sizeof(double), hipHostMallocDefault));
+  }
+  if (2 * (m + 8) + 1 > kGmresMaxH) return set_error(SFEM_ERR_ARG, "GMRES: restart length too large");
+  double* dh = L->d_h;             // [0, m+8): first pass, [m+8, 2m+16): second pass, [2m+16]: ||w||^2
+  double* dh2 = L->d_h + (m + 8);
+  double* dnn = L->d_h + 2 * (m + 8);
+  double* hh = L->h_h;
+  // project w against V_0..V_{k1-1} into out[0..k1) (out has room for k1 rounded up to 8)
+  auto project = [&](int k1, double* out) {
+    for (int j0 = 0; j0 < k1; j0 += 8) {
+      const int nv = std::min(8, k1 - j0);
+      hipLaunchKernelGGL((k_mdot8<T>), dim3(g), dim3(kTPB), 0, st, n, nv, V + (size_t)j0 * n, w, L->d_mpart);
+      hipLaunchKernelGGL((k_finalize<8>), dim3(1), dim3(kTPB), 0, st, L->d_mpart, g, out, j0, (int)FIN_STORE);
+    }
+    hipLaunchKernelGGL((k_mupdate<T>), dim3(g), dim3(kTPB), 0, st, n, k1, V, out, -1.0, w);
+  };
+  auto norm2 = [&](const T* a, double* out) {
+    hipLaunchKernelGGL((k_dot<T>), dim3(g), dim3(kTPB), 0, st, n, a, a, L->d_part);
+    hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(kTPB), 0, st, L->d_part, g, out, 0, (int)FIN_STORE);
+  };
+  auto fetch = [&](const double* src, double* dst, int cnt) -> int {
     LA_CHECK(hipGetLastError());
-    LA_CHECK(hipMemcpyAsync(hs + S_MISC0, sc + S_MISC0, sizeof(double), hipMemcpyDeviceToHost, st));
-    LA_CHECK(hipStreamSynchronize(st));
-    *out = hs[S_MISC0];
+    LA_CHECK(hipMemcpyAsync(dst, src, cnt * sizeof(double), hipMemcpyDeviceToHost, st));
     return SFEM_OK;
   };
   hipLaunchKernelGGL((k_fill<T>), dim3(g), dim3(kTPB), 0, st, n, x, (T)0);
-  double bb;
-  int rc = dotv(b, b, &bb); if (rc) return rc;
-  const double bnorm = std::sqrt(bb);
+  norm2(b, dnn);
+  int rc = fetch(dnn, hh, 1); if (rc) return rc;
+  LA_CHECK(hipStreamSynchronize(st));
+  const double bnorm = std::sqrt(hh[0]);
   *iters = 0;
-  if (!(bnorm > 0.0)) { *relres = 0.0; return SFEM_OK; }
+  if (!(bnorm > 0.0)) { *relres = 0.0; return std::isfinite(bnorm) ? SFEM_OK : set_error(SFEM_ERR_DIVERGED, "GMRES: non-finite rhs"); }
   std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), gv(m + 1), y(m);
   int it = 0;
   double rnorm = bnorm;
-  T* r = z;  // reuse z as residual storage at restart
+  T* r = z;  // z holds the residual at a restart
   while (it < maxit) {
     spmv<T>(ctx, x, w);
     hipLaunchKernelGGL((k_copy<T>), dim3(g), dim3(kTPB), 0, st, n, b, r);
     hipLaunchKernelGGL((k_axpby<T>), dim3(g), dim3(kTPB), 0, st, n, -1.0, w, 1.0, r);
-    double beta2;
-    rc = dotv(r, r, &beta2); if (rc) return rc;
-    const double beta = std::sqrt(beta2);
+    norm2(r, dnn);
+    if ((rc = fetch(dnn, hh, 1))) return rc;
+    LA_CHECK(hipStreamSynchronize(st));
+    const double beta = std::sqrt(hh[0]);
     rnorm = beta;
     if (!std::isfinite(beta)) return set_error(SFEM_ERR_DIVERGED, "GMRES: non-finite residual");
     if (beta <= rtol * bnorm) break;
@@ -1597,18 +1891,17 @@ static int gmres(sfem_ctx* ctx, const T* b, T* x, int m, double rtol, int maxit,
     for (; k < m && it < maxit; ++k) {
       ++it;
       T* vk = V + (size_t)k * n;
-      rc = pc_apply<T>(ctx, vk, z); if (rc) return rc;
+      if ((rc = pc_apply<T>(ctx, vk, z))) return rc;
       spmv<T>(ctx, z, w);
-      for (int j = 0; j <= k; ++j) {
-        double h;
-        rc = dotv(w, V + (size_t)j * n, &h); if (rc) return rc;
-        H[(size_t)j * m + k] = h;
-        hipLaunchKernelGGL((k_axpby<T>), dim3(g), dim3(kTPB), 0, st, n, -h, V + (size_t)j * n, 1.0, w);
-      }
-      double hn2;
-      rc = dotv(w, w, &hn2); if (rc) return rc;
-      const double hn = std::sqrt(hn2);
+      project(k + 1, dh);
+      project(k + 1, dh2);
+      norm2(w, dnn);
+      if ((rc = fetch(dh, hh, 2 * (m + 8) + 1))) return rc;
+      LA_CHECK(hipStreamSynchronize(st));
+      for (int j = 0; j <= k; ++j) H[(size_t)j * m + k] = hh[j] + hh[m + 8 + j];
+      const double hn = std::sqrt(std::max(hh[2 * (m + 8)], 0.0));
       H[(size_t)(k + 1) * m + k] = hn;
+      if (!std::isfinite(hn)) return set_error(SFEM_ERR_DIVERGED, "GMRES: non-finite Krylov vector");
       if (hn != 0.0)
         hipLaunchKernelGGL((k_convert<T, T>), dim3(g), dim3(kTPB), 0, st, n, w, V + (size_t)(k + 1) * n, 1.0 / hn);
       for (int j = 0; j < k; ++j) {
@@ -1628,25 +1921,31 @@ static int gmres(sfem_ctx* ctx, const T* b, T* x, int m, double rtol, int maxit,
       if (rnorm <= rtol * bnorm || hn == 0.0) { ++k; break; }
     }
     for (int j = k - 1; j >= 0; --j) {
-      double s = gv[j];
-      for (int q = j + 1; q < k; ++q) s -= H[(size_t)j * m + q] * y[q];
-      y[j] = s / H[(size_t)j * m + j];
+      double sum = gv[j];
+      for (int q = j + 1; q < k; ++q) sum -= H[(size_t)j * m + q] * y[q];
+      y[j] = sum / H[(size_t)j * m + j];
     }
+    // x += M^{-1} (V y)
+    std::copy(y.begin(), y.begin() + k, hh);
+    LA_CHECK(hipMemcpyAsync(dh, hh, k * sizeof(double), hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL((k_fill<T>), dim3(g), dim3(kTPB), 0, st, n, w, (T)0);
-    for (int j = 0; j < k; ++j)
-      hipLaunchKernelGGL((k_axpby<T>), dim3(g), dim3(kTPB), 0, st, n, y[j], V + (size_t)j * n, 1.0, w);
-    rc = pc_apply<T>(ctx, w, z); if (rc) return rc;
+    hipLaunchKernelGGL((k_mupdate<T>), dim3(g), dim3(kTPB), 0, st, n, k, V, dh, 1.0, w);
+    if ((rc = pc_apply<T>(ctx, w, z))) return rc;
     hipLaunchKernelGGL((k_axpby<T>), dim3(g), dim3(kTPB), 0, st, n, 1.0, z, 1.0, x);
     LA_CHECK(hipGetLastError());
-    if (rnorm <= rtol * bnorm) break;
+    // the host copy of y must not be overwritten before the transfer has read it
+    LA_CHECK(hipStreamSynchronize(st));
+    // convergence is decided on the true residual at the top of the loop: in FP32 the
+    // Givens estimate can sit below the tolerance while the true residual does not
   }
   // true residual
   spmv<T>(ctx, x, w);
   hipLaunchKernelGGL((k_copy<T>), dim3(g), dim3(kTPB), 0, st, n, b, z);
   hipLaunchKernelGGL((k_axpby<T>), dim3(g), dim3(kTPB), 0, st, n, -1.0, w, 1.0, z);
-  double rr;
-  rc = dotv(z, z, &rr); if (rc) return rc;
-  rnorm = std::sqrt(rr);
+  norm2(z, dnn);
+  if ((rc = fetch(dnn, hh, 1))) return rc;
+  LA_CHECK(hipStreamSynchronize(st));
+  rnorm = std::sqrt(hh[0]);
   *iters = it;
   *relres = rnorm / bnorm;
   if (!std::isfinite(rnorm)) return set_error(SFEM_ERR_DIVERGED, "GMRES: non-finite residual");
@@ -1760,8 +2059,26 @@ int la_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int* iters, double* relre
     }
     rc = gmres<float>(ctx, b32, L->du32, m, o->rtol, o->max_it, iters, relres, L->gm_V32, L->w32, L->z32);
   } else {
+    // FP32 BiCGStab can stall where FP64 BiCGStab converges (a multicolour-ordered ILU(0)
+    // on small, strongly graded meshes makes the short recurrences sensitive to FP32
+    // rounding): after a stall window the solve restarts from zero as FP32 GMRES.
     rc = bicgstab<float>(ctx, b32, L->du32, o->rtol, o->max_it, iters, relres, L->r32, L->rh32, L->p32,
-                         L->v32, L->s32, L->t32, L->ph32, L->sh32);
+                         L->v32, L->s32, L->t32, L->ph32, L->sh32, kBicgStall);
+    if (rc == kStagnated) {
+      const int used = *iters;
+      const int mg = std::max(1, std::min(o->restart > 0 ? o->restart : 30, 64));
+      if ((rc = ensure_vec(&L->w32, n)) || (rc = ensure_vec(&L->z32, n))) return rc;
+      if (!L->gm_V32 || L->gm_m32 < mg) {
+        if (L->gm_V32) (void)hipFree(L->gm_V32);
+        L->gm_V32 = nullptr;
+        if ((rc = ensure_vec(&L->gm_V32, (int64_t)(mg + 1) * n))) return rc;
+        L->gm_m32 = mg;
+      }
+      ++L->fallbacks;
+      rc = gmres<float>(ctx, b32, L->du32, mg, o->rtol, std::max(1, o->max_it - used), iters, relres, L->gm_V32,
+                        L->w32, L->z32);
+      *iters += used;
+    }
   }
   if (perm)
     hipLaunchKernelGGL((k_permute_out<float, double>), dim3(g), dim3(kTPB), 0, ctx->stream, n, perm, L->du32,

@@ -32,6 +32,8 @@ struct IluPlan {
   int32_t *d_rowptr = nullptr, *d_colidx = nullptr, *d_diagpos = nullptr, *d_blkStart = nullptr;
   int32_t* d_pmap = nullptr;   // permuted nonzero -> original nonzero
   int32_t* d_perm = nullptr;   // permuted row -> original row
+  int32_t* d_rowBlk = nullptr; // row -> node block, in the plan's ordering
+  const int32_t* rowBlk = nullptr;
   ~IluPlan();
 };
 
@@ -43,6 +45,7 @@ struct LinAlg {
   int tune_spmv[2] = {3, 3}, tune_best_spmv[2] = {3, 3};
   std::vector<int> tune_L[2], tune_U[2];   // per level of the forward / backward solve
   bool tuned[2] = {false, false};
+  int64_t fallbacks = 0;     // FP32 BiCGStab solves finished by the GMRES fallback
   int plan_id = 0, tune_plan[2] = {-1, -1};  // ILU plan the per-level shapes were tuned for
   int pc_type = SFEM_PC_NONE;
   bool pc_fp32 = false;
@@ -54,9 +57,16 @@ struct LinAlg {
   IluPlan* ilu = nullptr;
   int32_t *d_blkStart = nullptr, *d_diagpos = nullptr;
   BlkDesc* d_desc = nullptr;   // per node block, original order
+  int32_t* d_rowBlk = nullptr; // row -> node block, original order
+  // every lower neighbour's rows appear as one contiguous group of its full size in each
+  // row (true of element patterns); enables the node-grouped factorisation
+  bool full_blocks = false;
   double* d_part = nullptr;  // reduction partials
   double* d_scal = nullptr;  // Krylov scalars
   double* h_scal = nullptr;  // pinned mirror
+  double* d_h = nullptr;     // GMRES Hessenberg column scratch
+  double* h_h = nullptr;     // pinned mirror
+  double* d_mpart = nullptr; // batched reduction partials (8 per block)
   // FP64 Krylov vectors
   double *r = nullptr, *rh = nullptr, *p = nullptr, *v = nullptr, *s = nullptr, *t = nullptr,
          *ph = nullptr, *sh = nullptr, *x = nullptr, *w = nullptr, *z = nullptr, *gm_V = nullptr,
@@ -88,6 +98,7 @@ int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f);
 int la_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int* iters, double* relres);
 int la_prepare(sfem_ctx* ctx, const sfem_solver_opts* o);
 int la_tuning(sfem_ctx* ctx, int* out);
+int la_get_ilu(sfem_ctx* ctx, int64_t* nnz_out, int32_t* perm, int32_t* rowptr, int32_t* colidx, double* vals);
 int la_tuning_levels(sfem_ctx* ctx, int fp32, int* out, int cap, int* nlevels);
 int la_norm2(sfem_ctx* ctx, const double* a, double* out);
 int la_spmv64(sfem_ctx* ctx, const double* x, double* y);

@@ -76,6 +76,8 @@ ABI = [
     ("sfem_reset_state", C.c_int, [_vp]),
     ("sfem_enable_kernel_timing", C.c_int, [_vp, C.c_int]),
     ("sfem_kernel_timing", C.c_int, [_vp, C.c_int, _i64p, _f64p]),
+    ("sfem_get_ilu", C.c_int, [_vp, _i64p, _i32p, _i32p, _i32p, _f64p]),
+    ("sfem_solver_stats", C.c_int, [_vp, _i64p]),
     ("sfem_tuning", C.c_int, [_vp, C.POINTER(C.c_int)]),
     ("sfem_tuning_levels", C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]),
     ("sfem_info", C.c_int, [_vp, _i64p, _i64p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
@@ -262,6 +264,25 @@ class Context:
         n, ms = C.c_int64(0), C.c_double(0.0)
         _check(self.lib.sfem_kernel_timing(self.h, int(kind), C.byref(n), C.byref(ms)), "sfem_kernel_timing")
         return n.value, ms.value
+
+    def fp32_fallbacks(self) -> int:
+        """Mixed-precision solves finished by the FP32 GMRES fallback."""
+        v = C.c_int64(0)
+        _check(self.lib.sfem_solver_stats(self.h, C.byref(v)), "sfem_solver_stats")
+        return v.value
+
+    def get_ilu(self):
+        """(perm, rowptr, colidx, vals) of the current ILU(0) factor in its ordering."""
+        nnz = C.c_int64(0)
+        _check(self.lib.sfem_get_ilu(self.h, C.byref(nnz), None, None, None, None), "sfem_get_ilu")
+        n = self.nfree
+        perm = np.zeros(max(n, 1), np.int32)
+        rowptr = np.zeros(n + 1, np.int32)
+        colidx = np.zeros(max(nnz.value, 1), np.int32)
+        vals = np.zeros(max(nnz.value, 1))
+        _check(self.lib.sfem_get_ilu(self.h, C.byref(nnz), _p(perm, C.c_int32), _p(rowptr, C.c_int32),
+                                     _p(colidx, C.c_int32), _p(vals, C.c_double)), "sfem_get_ilu")
+        return perm[:n], rowptr, colidx[:nnz.value], vals[:nnz.value]
 
     def tuning(self):
         """Autotuned kernel shapes per precision: SpMV (G, U, XCD order) and the

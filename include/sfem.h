@@ -183,6 +183,19 @@ enum { SFEM_KT_ASSEMBLY = 0, SFEM_KT_SPMV = 1, SFEM_KT_PC = 2, SFEM_KT_NKINDS = 
 int sfem_enable_kernel_timing(sfem_ctx* ctx, int enable);
 int sfem_kernel_timing(sfem_ctx* ctx, int kind, int64_t* launches, double* total_ms);
 
+/* The ILU(0) factor of the last solve (pc = ILU0), in the factor's node ordering:
+ * perm[i] = DOF row of factor row i (n entries), rowptr (n+1) and colidx (nnz) of the
+ * reordered pattern, and vals (nnz): L strictly below the diagonal (unit diagonal
+ * implied), U on and above it, FP32/BF16 factors widened.  Any output may be NULL;
+ * *nnz receives the count.  The reference's factor lives inside PETSc's PCILU
+ * (SolverPetsc.cpp:234-277) and is not exposed; this exists for parity checks. */
+int sfem_get_ilu(sfem_ctx* ctx, int64_t* nnz, int32_t* perm, int32_t* rowptr, int32_t* colidx, double* vals);
+
+/* Number of mixed-precision solves in which the FP32 BiCGStab stalled (no 10% gain of
+ * its best residual over 100 iterations) and the solve was finished by FP32 GMRES from
+ * a zero start.  No counterpart in the reference (PETSc reports a diverged KSP). */
+int sfem_solver_stats(sfem_ctx* ctx, int64_t* fp32_fallbacks);
+
 /* Kernel shapes chosen by the solver's autotuner, as out[8] = {SpMV lanes per node G,
  * columns in flight per lane U, XCD-contiguous workgroup order (0/1), number of tuned
  * ILU(0) levels} for FP64 then FP32; 0 where not tuned yet.  No counterpart in the

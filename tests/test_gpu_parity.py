@@ -224,6 +224,58 @@ def test_krylov_against_direct(ksp, pc, reorder):
     ctx.close()
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("reorder", [0, 1, 2])
+def test_ilu_factor_vs_oracle(reorder, prec):
+    """The device ILU(0) factor against the oracle's IKJ ILU(0) of the same reordered matrix."""
+    import scipy.sparse as sp
+    mesh = meshgen.ldc3d_hex(5, graded=True)
+    ctx = make_ctx(mesh, tis=2, dt=0.1, elemData=np.array([1.0, 1e-2, 0, 0, 0, 0, 0, 0, 1, 1, 0]))
+    if prec == "mixed":
+        ctx.set_precision(sfem.PREC_MIXED)
+    ctx.time_update(1.0)
+    ctx.assemble(0)
+    vals, _ = ctx.get_system()
+    rp, ci, _ = ctx.get_pattern()
+    ctx.solve(sfem.solver_opts("bcgs", "ilu", 1e-6, 5000, 30, reorder))
+    perm, prp, pci, lu = ctx.get_ilu()
+    print(f"[case] ilu factor reorder={reorder} {prec} n={ctx.nfree} nnz={len(lu)}", flush=True)
+    A = sp.csr_matrix((vals, ci, rp), shape=(ctx.nfree, ctx.nfree))
+    Ap = A[perm][:, perm].tocsr()
+    Ap.sort_indices()
+    assert np.array_equal(Ap.indptr, prp) and np.array_equal(Ap.indices, pci)
+    ref = oracle.OraclePC("ilu", Ap.indptr.astype(np.int64), Ap.indices.astype(np.int32), Ap.data).lu(Ap.nnz)
+    worst = 0.0
+    for r in range(ctx.nfree):
+        a, b = prp[r], prp[r + 1]
+        worst = max(worst, np.max(np.abs(lu[a:b] - ref[a:b])) / np.max(np.abs(ref[a:b])))
+    print(f"[ilu] worst row-relative factor difference {worst:.2e}", flush=True)
+    assert worst <= (1e-11 if prec == "fp64" else 1e-4)
+    ctx.close()
+
+
+def test_mixed_bicgstab_stall_falls_back_to_gmres():
+    """The first step of a graded 6^3 cavity: FP64 BiCGStab + multicolour ILU(0) converges
+    in a few dozen iterations while FP32 BiCGStab stalls; the mixed solve then finishes
+    as FP32 GMRES and reaches the tolerance."""
+    import scipy.sparse as sp
+    mesh = meshgen.ldc3d_hex(6, graded=True)
+    ed = np.array([1.0, 1e-3, 0, 0, 0, 0, 0, 0, 1, 1, 0])
+    ctx = make_ctx(mesh, tis=2, dt=0.1, elemData=ed)
+    ctx.set_precision(sfem.PREC_MIXED)
+    ctx.time_update(0.1)
+    ctx.assemble(0)
+    vals, rhs = ctx.get_system()
+    rp, ci, _ = ctx.get_pattern()
+    its, rr = ctx.solve(sfem.solver_opts("bcgs", "ilu", 1e-4, 3000, 30, 2))
+    du = ctx.get_increment()
+    A = sp.csr_matrix((vals, ci, rp), shape=(ctx.nfree, ctx.nfree))
+    res = np.linalg.norm(rhs - A @ du) / np.linalg.norm(rhs)
+    print(f"[stall] its={its} relres={rr:.2e} true={res:.2e} fallbacks={ctx.fp32_fallbacks()}", flush=True)
+    assert res <= 2e-4
+    ctx.close()
+
+
 def test_mixed_precision_close_to_fp64():
     """FP32 tangent + FP32 Krylov inside an FP64 residual loop stays within 1e-6 of FP64."""
     f64, mesh, ctl = _cavity3d(6, 3, newton_tol=1e-10)
