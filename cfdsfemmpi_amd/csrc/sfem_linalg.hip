@@ -6,8 +6,9 @@
 // test ||b - A x||_2 <= rtol ||b||_2 is on the true residual, with a zero
 // initial guess (VecZeroEntries at :249).  Preconditioners: point Jacobi,
 // nodal block Jacobi (the free DOFs of one node form a block of size <= 4),
-// and ILU(0) on the CSR pattern, factorised and applied by level scheduling
-// over nodes (all free rows of a node are handled by one wavefront).
+// and ILU(0), factorised and applied level by level (one level per colour with
+// the multicolour node ordering) in a padded 4 x 4 node-block form of the
+// matrix; with ILU(0) the whole Krylov solve runs in that block form.
 //
 // Reductions are two-stage (per-block partials, then one block summing the
 // partials in index order), so every dot product is deterministic.  Krylov
@@ -257,42 +258,6 @@ __global__ __launch_bounds__(kTPB) void k_bicg_xr(int64_t n, T* __restrict__ x, 
   block_sum_store<1>(acc, part);
 }
 
-// dst[i] = src[map[i]]
-template <typename T>
-__global__ __launch_bounds__(kTPB) void k_gather(int64_t n, const int32_t* __restrict__ map,
-                                                 const T* __restrict__ src, T* __restrict__ dst) {
-  for (int64_t i = (int64_t)blockIdx.x * kTPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kTPB)
-    dst[i] = src[map[i]];
-}
-
-// dst0[i] = dst1[i] = src[map[i]]  (the permuted tangent and the factor's initial values)
-template <typename T>
-__global__ __launch_bounds__(kTPB) void k_gather2(int64_t n, const int32_t* __restrict__ map,
-                                                  const T* __restrict__ src, T* __restrict__ dst0,
-                                                  T* __restrict__ dst1) {
-  for (int64_t i = (int64_t)blockIdx.x * kTPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kTPB) {
-    const T v = src[map[i]];
-    dst0[i] = v;
-    dst1[i] = v;
-  }
-}
-
-// b[i] = scale * a[perm[i]]  (into the solver ordering)
-template <typename TA, typename TB>
-__global__ __launch_bounds__(kTPB) void k_permute_in(int64_t n, const int32_t* __restrict__ perm,
-                                                     const TA* __restrict__ a, TB* __restrict__ b, double scale) {
-  for (int64_t i = (int64_t)blockIdx.x * kTPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kTPB)
-    b[i] = (TB)((double)a[perm[i]] * scale);
-}
-
-// b[perm[i]] = scale * a[i]  (back to the DOF ordering)
-template <typename TA, typename TB>
-__global__ __launch_bounds__(kTPB) void k_permute_out(int64_t n, const int32_t* __restrict__ perm,
-                                                      const TA* __restrict__ a, TB* __restrict__ b, double scale) {
-  for (int64_t i = (int64_t)blockIdx.x * kTPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kTPB)
-    b[perm[i]] = (TB)((double)a[i] * scale);
-}
-
 // y = a*x + b*y with host scalars (GMRES)
 template <typename T>
 __global__ __launch_bounds__(kTPB) void k_axpby(int64_t n, double a, const T* __restrict__ x, double b,
@@ -395,12 +360,8 @@ __global__ __launch_bounds__(kTPB) void k_jac_apply(int64_t n, const PT* __restr
 }
 
 // ---------------------------------------------------------------------------
-// ILU(0), level-scheduled over nodal blocks; one 64-lane workgroup per block.
-// All rows of a block share one column list (the pattern is the union of full
-// node blocks), so the block's rows are staged in LDS against that list.
+// shared helpers of the node-grouped kernels
 // ---------------------------------------------------------------------------
-constexpr int kIluMaxRow = 256;  // longest row handled in LDS (checked on the host)
-
 __device__ inline int lds_find(const int32_t* cols, int len, int32_t c) {
   int lo = 0, hi = len;
   while (lo < hi) {
@@ -422,72 +383,6 @@ __device__ inline AT pick4(const AT (&v)[4], int i) {
   return i == 0 ? v[0] : i == 1 ? v[1] : i == 2 ? v[2] : v[3];
 }
 
-// ILU(0) factorisation of the blocks of one level (one 64-lane workgroup per block).
-// IKJ order per row: external lower columns (earlier blocks, already final) ascending,
-// then the block's own lower columns.  The m rows of the block share the column list, so
-// the position of every upper entry of an external row is looked up once for all rows.
-template <typename LT>
-__global__ __launch_bounds__(64) void k_ilu_factor_blk(const int32_t* __restrict__ list, int first, int count,
-                                                       const int32_t* __restrict__ blkStart,
-                                                       const int32_t* __restrict__ rowptr,
-                                                       const int32_t* __restrict__ colidx,
-                                                       const int32_t* __restrict__ diagpos,
-                                                       LT* __restrict__ lu, double* scal) {
-  __shared__ int32_t cols[kIluMaxRow];
-  __shared__ double rows[4][kIluMaxRow];
-  __shared__ double lval[4];
-  const int w = blockIdx.x;
-  if (w >= count) return;
-  const int lane = threadIdx.x;
-  const int32_t blk = list ? list[first + w] : first + w;
-  const int32_t r0 = blkStart[blk];
-  const int m = blkStart[blk + 1] - r0;
-  const int32_t base = rowptr[r0];
-  const int rl = rowptr[r0 + 1] - base;
-  const int dq0 = diagpos[r0] - base;
-  for (int q = lane; q < rl; q += 64) cols[q] = colidx[base + q];
-  for (int a = 0; a < m; ++a)
-    for (int q = lane; q < rl; q += 64) rows[a][q] = (double)lu[base + a * rl + q];
-  __syncthreads();
-  for (int q = 0; q < dq0; ++q) {
-    const int32_t c = cols[q];
-    if (lane < m) {
-      const double d = (double)lu[diagpos[c]];
-      if (d == 0.0) scal[S_MISC1] = 2.0;
-      const double l = rows[lane][q] / (d == 0.0 ? 1.0 : d);
-      rows[lane][q] = l;
-      lval[lane] = l;
-    }
-    __syncthreads();
-    const int32_t cb = diagpos[c] + 1, ce = rowptr[c + 1];
-    for (int32_t j = cb + lane; j < ce; j += 64) {
-      const int pos = lds_find(cols, rl, colidx[j]);
-      if (pos >= 0) {
-        const double u = (double)lu[j];
-        for (int a = 0; a < m; ++a) rows[a][pos] -= lval[a] * u;
-      }
-    }
-    __syncthreads();
-  }
-  for (int a = 1; a < m; ++a) {
-    for (int ap = 0; ap < a; ++ap) {
-      if (lane == 0) {
-        const double d = rows[ap][dq0 + ap];
-        if (d == 0.0) scal[S_MISC1] = 2.0;
-        const double l = rows[a][dq0 + ap] / (d == 0.0 ? 1.0 : d);
-        rows[a][dq0 + ap] = l;
-        lval[0] = l;
-      }
-      __syncthreads();
-      const double l = lval[0];
-      for (int q = dq0 + ap + 1 + lane; q < rl; q += 64) rows[a][q] -= l * rows[ap][q];
-      __syncthreads();
-    }
-  }
-  for (int a = 0; a < m; ++a)
-    for (int q = lane; q < rl; q += 64) lu[base + a * rl + q] = (LT)rows[a][q];
-}
-
 // Node block descriptor: first row, first value of that row, row length, and
 // (diagonal-block column offset) | (row count << 16).
 struct __align__(16) BlkDesc {
@@ -502,151 +397,6 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// ILU(0) factorisation of the node blocks of one level, one 64-lane workgroup (one
-// wavefront) per block, organised by lower node blocks: the columns of row block i fall
-// into groups, one per neighbouring node block, and every row of a lower block c shares
-// c's column list.  So per lower block the positions (in i's list) of c's upper columns
-// are searched once, c's m_c upper rows are staged in LDS, and the m_c eliminations of
-// IKJ order run from LDS; the global loads of the next lower block are issued before the
-// eliminations of the current one and land in registers meanwhile.  The result equals
-// the row-by-row IKJ factorisation (same updates in the same order).  Dynamic LDS:
-// factor_nb_lds(cap) bytes for rows of at most cap (a multiple of 64, <= kIluMaxRow)
-// entries.
-template <typename LT>
-__host__ __device__ constexpr size_t factor_nb_lds(int cap) {
-  return (size_t)cap * (4 * sizeof(double) + 4 * sizeof(LT) + 3 * sizeof(int32_t) + sizeof(BlkDesc));
-}
-
-template <typename LT>
-__global__ __launch_bounds__(64) void k_ilu_factor_nb(const int32_t* __restrict__ list, int first, int count,
-                                                      const BlkDesc* __restrict__ desc,
-                                                      const int32_t* __restrict__ colidx,
-                                                      const int32_t* __restrict__ rowBlk, LT* __restrict__ lu,
-                                                      double* __restrict__ scal, int cap) {
-  extern __shared__ double smem[];
-  double* rows = smem;                              // [4][cap], this block's rows
-  LT* urow = (LT*)(rows + 4 * cap);                 // [4][cap], external upper part of c's rows
-  int32_t* cols = (int32_t*)(urow + 4 * cap);       // [cap], this block's column list
-  int32_t* pos = cols + cap;                        // [cap], positions of c's upper columns
-  int32_t* gq = pos + cap;                          // [cap], first column of each lower group
-  BlkDesc* gdesc = (BlkDesc*)(gq + cap);            // [cap], descriptor of each lower group
-  __shared__ double ub[16];                         // c's diagonal block
-  __shared__ double lval[4];
-  __shared__ int ngroups;
-  constexpr int S = kIluMaxRow / 64;                // prefetch slots per lane
-  const int w = blockIdx.x;
-  if (w >= count) return;
-  const int lane = threadIdx.x;
-  const BlkDesc d = desc[list ? list[first + w] : first + w];
-  const int m = d.dqm >> 16, dq0 = d.dqm & 0xffff, rl = d.rl;
-  for (int q = lane; q < rl; q += 64) cols[q] = colidx[d.base + q];
-  for (int a = 0; a < m; ++a)
-    for (int q = lane; q < rl; q += 64) rows[a * cap + q] = (double)lu[d.base + (int64_t)a * rl + q];
-  // lower node groups: block of every lower column, group starts compacted by ballot
-  for (int q = lane; q < dq0; q += 64) pos[q] = rowBlk[colidx[d.base + q]];
-  if (lane == 0) ngroups = 0;
-  wave_sync();
-  for (int q0 = 0; q0 < dq0; q0 += 64) {
-    const int q = q0 + lane;
-    const bool start = q < dq0 && (q == 0 || pos[q] != pos[q - 1]);
-    const uint64_t mask = __ballot(start);
-    const int base = ngroups;
-    if (start) {
-      const int k = base + __popcll(mask & ((1ull << lane) - 1ull));
-      gq[k] = q;
-      gdesc[k] = desc[pos[q]];
-    }
-    wave_sync();
-    if (lane == 0) ngroups = base + __popcll(mask);
-    wave_sync();
-  }
-  const int ng = ngroups;
-  // registers of the prefetched lower block: column indices, upper values, diagonal block
-  int32_t pc[S];
-  LT pu[S][4];
-  LT pd = (LT)0;
-  auto issue = [&](int g) {
-    const BlkDesc dc = gdesc[g];
-    const int mc = dc.dqm >> 16, dq0c = dc.dqm & 0xffff;
-    const int ext0 = dq0c + mc, next = dc.rl - ext0;
-#pragma unroll
-    for (int sl = 0; sl < S; ++sl) {
-      const int t = lane + 64 * sl;
-      if (t < next) {
-        pc[sl] = colidx[dc.base + ext0 + t];
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-          if (b < mc) pu[sl][b] = lu[dc.base + (int64_t)b * dc.rl + ext0 + t];
-      }
-    }
-    const int b = lane >> 2, bp = lane & 3;
-    if (lane < 16 && b < mc && bp < mc) pd = lu[dc.base + (int64_t)b * dc.rl + dq0c + bp];
-  };
-  if (ng > 0) issue(0);
-  for (int g = 0; g < ng; ++g) {
-    const int q = gq[g];
-    const BlkDesc dc = gdesc[g];
-    const int mc = dc.dqm >> 16, dq0c = dc.dqm & 0xffff;
-    const int next = dc.rl - (dq0c + mc);
-#pragma unroll
-    for (int sl = 0; sl < S; ++sl) {
-      const int t = lane + 64 * sl;
-      if (t < next) {
-        pos[t] = lds_find(cols, rl, pc[sl]);
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-          if (b < mc) urow[b * cap + t] = pu[sl][b];
-      }
-    }
-    if (lane < 16) ub[lane] = ((lane >> 2) < mc && (lane & 3) < mc) ? (double)pd : 0.0;
-    wave_sync();
-    if (g + 1 < ng) issue(g + 1);
-    for (int b = 0; b < mc; ++b) {
-      if (lane < m) {
-        const double dd = ub[b * 4 + b];
-        if (dd == 0.0) scal[S_MISC1] = 2.0;
-        const double l = rows[lane * cap + q + b] / (dd == 0.0 ? 1.0 : dd);
-        rows[lane * cap + q + b] = l;
-        lval[lane] = l;
-      }
-      wave_sync();
-      const int nin = mc - 1 - b;  // remaining columns of block c in this row block
-      for (int t = lane; t < nin + next; t += 64) {
-        int p;
-        double u;
-        if (t < nin) {
-          p = q + b + 1 + t;
-          u = ub[b * 4 + b + 1 + t];
-        } else {
-          p = pos[t - nin];
-          u = (double)urow[b * cap + t - nin];
-        }
-        if (p >= 0)
-          for (int a = 0; a < m; ++a) rows[a * cap + p] -= lval[a] * u;
-      }
-      wave_sync();
-    }
-  }
-  // the block's own diagonal block
-  for (int a = 1; a < m; ++a) {
-    for (int ap = 0; ap < a; ++ap) {
-      if (lane == 0) {
-        const double dd = rows[ap * cap + dq0 + ap];
-        if (dd == 0.0) scal[S_MISC1] = 2.0;
-        const double l = rows[a * cap + dq0 + ap] / (dd == 0.0 ? 1.0 : dd);
-        rows[a * cap + dq0 + ap] = l;
-        lval[0] = l;
-      }
-      wave_sync();
-      const double l = lval[0];
-      for (int qq = dq0 + ap + 1 + lane; qq < rl; qq += 64) rows[a * cap + qq] -= l * rows[ap * cap + qq];
-      wave_sync();
-    }
-  }
-  for (int a = 0; a < m; ++a)
-    for (int qq = lane; qq < rl; qq += 64) lu[d.base + (int64_t)a * rl + qq] = (LT)rows[a * cap + qq];
 }
 
 // acc[a] += sum_{q in [q0, q1)} V[a*rl + q] * x[col[q]] for a < m, lanes of a group of G
@@ -752,57 +502,139 @@ __global__ __launch_bounds__(256) void k_spmv_blk(int64_t nb, const BlkDesc* __r
   if (b < nb && lane < (d.dqm >> 16)) y[d.r0 + lane] = (XT)pick4(acc, lane);
 }
 
-template <typename LT> struct AccOf { using type = float; };
-template <> struct AccOf<double> { using type = double; };
+// ---------------------------------------------------------------------------
+// ILU(0) in node-block (BSR4) form.  Every node block with a free DOF is a block row
+// and a block column of width 4: its m <= 4 free DOFs occupy slots 0..m-1, slots
+// m..3 are padding.  A block row p stores, for each neighbouring block (bcol, sorted
+// in the plan's order), a 4 x 4 block whose row a (a < m) lies at
+// vbase + a*4*nb + 4*t: so lane t of a group reads row a of its block with one
+// 16-byte load (8 bytes in BF16) and the neighbour's 4 x entries with one more.
+// Padding columns hold 0 and padding slots of every vector the kernels write are 0.
+// ILU(0) on this block pattern equals point ILU(0) on the CSR pattern when the CSR
+// pattern is a union of full node blocks (element patterns are): the padded
+// columns stay 0 through the elimination.
+// ---------------------------------------------------------------------------
+constexpr int kBsrMaxNb = 64;  // neighbour blocks per block row handled (checked on the host)
 
-// The m x m diagonal block of a node block: entry e = 4a + b (row a, column b) is loaded
-// by lane e % G as its (e / G)-th value (load_diag, issued before the long sparse dot
-// product) and then broadcast to the whole group (bcast_diag); absent entries read 0.
-template <int G> struct DiagPerLane { static constexpr int n = G >= 16 ? 1 : 16 / G; };
+// Block row descriptor: first value, first neighbour entry, neighbour count, and
+// (position of the diagonal block) | (free rows m << 16).
+struct __align__(16) BDesc {
+  int32_t vbase, bptr, nb, dm;
+};
 
-template <int G, typename AT, typename LT>
-__device__ inline void load_diag(const LT* __restrict__ lu, const BlkDesc& d, int lane,
-                                 AT (&dv)[DiagPerLane<G>::n]) {
-  const int m = d.dqm >> 16, dq0 = d.dqm & 0xffff;
+template <typename AT> struct V4 { AT v[4]; };
+
+template <typename AT>
+__device__ inline V4<AT> ld4(const double* p) {
+  const double4 f = *(const double4*)p;
+  return {{(AT)f.x, (AT)f.y, (AT)f.z, (AT)f.w}};
+}
+template <typename AT>
+__device__ inline V4<AT> ld4(const float* p) {
+  const float4 f = *(const float4*)p;
+  return {{(AT)f.x, (AT)f.y, (AT)f.z, (AT)f.w}};
+}
+template <typename AT>
+__device__ inline V4<AT> ld4(const bf16* p) {
+  const uint2 u = *(const uint2*)p;
+  return {{(AT)__uint_as_float(u.x << 16), (AT)__uint_as_float(u.x & 0xffff0000u),
+           (AT)__uint_as_float(u.y << 16), (AT)__uint_as_float(u.y & 0xffff0000u)}};
+}
+
+// acc[a] += sum over blocks t in [t0, t1) of V(a, t) . x(bcol[t]) for a < m.  Lanes of a
+// group of G step over t; the first K blocks of every lane are loaded (index, x and m
+// value rows) before any is used, the rest one at a time.
+template <int G, int K, typename VT, typename XT, typename AT>
+__device__ inline void bsr_dot(const int32_t* __restrict__ bc, const VT* __restrict__ V, int32_t rs,
+                               const XT* __restrict__ x, int m, int t0, int t1, int lane, AT (&acc)[4]) {
+  int32_t c[K];
+  V4<AT> v[4][K];
 #pragma unroll
-  for (int t = 0; t < DiagPerLane<G>::n; ++t) {
-    const int e = lane + t * G;
-    dv[t] = (e < 16 && (e >> 2) < m && (e & 3) < m)
-                ? widen<AT>(lu[d.base + (int64_t)(e >> 2) * d.rl + dq0 + (e & 3)])
-                : (AT)0;
+  for (int s = 0; s < K; ++s) {
+    const int t = t0 + lane + s * G;
+    const bool ok = t < t1;
+    c[s] = ok ? bc[t] : -1;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+      v[a][s] = (ok && a < m) ? ld4<AT>(V + (int64_t)a * rs + 4 * t) : V4<AT>{{0, 0, 0, 0}};
+  }
+  V4<AT> xv[K];
+#pragma unroll
+  for (int s = 0; s < K; ++s) xv[s] = c[s] >= 0 ? ld4<AT>(x + 4 * (int64_t)c[s]) : V4<AT>{{0, 0, 0, 0}};
+#pragma unroll
+  for (int s = 0; s < K; ++s)
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+      acc[a] += v[a][s].v[0] * xv[s].v[0] + v[a][s].v[1] * xv[s].v[1] + v[a][s].v[2] * xv[s].v[2] +
+                v[a][s].v[3] * xv[s].v[3];
+  for (int t = t0 + lane + K * G; t < t1; t += G) {
+    const V4<AT> xx = ld4<AT>(x + 4 * (int64_t)bc[t]);
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+      if (a < m) {
+        const V4<AT> w = ld4<AT>(V + (int64_t)a * rs + 4 * t);
+        acc[a] += w.v[0] * xx.v[0] + w.v[1] * xx.v[1] + w.v[2] * xx.v[2] + w.v[3] * xx.v[3];
+      }
   }
 }
 
-template <int G, typename AT>
-__device__ inline void bcast_diag(const AT (&dv)[DiagPerLane<G>::n], AT (&D)[16]) {
+// y = K x in block form; G lanes per block row, padding slots of y written as 0.
+template <typename VT, typename XT, int G, int K>
+__global__ __launch_bounds__(256) void k_bsr_spmv(int64_t nblk, const BDesc* __restrict__ desc,
+                                                  const int32_t* __restrict__ bcol, const VT* __restrict__ V,
+                                                  const XT* __restrict__ x, XT* __restrict__ y, int xcd) {
+  using AT = typename std::conditional<sizeof(VT) == 8, double, float>::type;
+  const int64_t gt = logical_wg(xcd) * 256 + threadIdx.x;
+  const int64_t p = gt / G;
+  const int lane = (int)(gt % G);
+  AT acc[4] = {0, 0, 0, 0};
+  BDesc d = {0, 0, 0, 0};
+  if (p < nblk) {
+    d = desc[p];
+    bsr_dot<G, K>(bcol + d.bptr, V + d.vbase, 4 * d.nb, x, d.dm >> 16, 0, d.nb, lane, acc);
+  }
 #pragma unroll
-  for (int e = 0; e < 16; ++e) D[e] = __shfl(dv[e / G], e % G, G);
+  for (int a = 0; a < 4; ++a) acc[a] = group_sum<G>(acc[a]);
+  if (p < nblk && lane < 4) y[4 * p + lane] = lane < (d.dm >> 16) ? (XT)pick4(acc, lane) : (XT)0;
 }
 
-// Forward solve L y = r over one level with G >= 4 lanes per node block: the external
-// lower part as a grouped sparse dot product, then the unit-lower m x m diagonal block.
-// Level blocks are either the contiguous range [first, first + count) or list[first + g].
-template <typename T, typename LT, int G, int U>
-__global__ __launch_bounds__(256) void k_ilu_lsolve_blk(const int32_t* __restrict__ list, int first, int count,
-                                                        const BlkDesc* __restrict__ desc,
-                                                        const int32_t* __restrict__ colidx,
-                                                        const LT* __restrict__ lu,
-                                                        const T* __restrict__ r, T* __restrict__ y, int xcd) {
+template <typename LT> struct AccOf { using type = float; };
+template <> struct AccOf<double> { using type = double; };
+
+// The diagonal block of block row d, row `lane` loaded by lane < m (before the long
+// sparse product) and broadcast to the group.
+template <int G, typename AT>
+__device__ inline void bcast_diag(const V4<AT>& dv, AT (&D)[16]) {
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) D[a * 4 + b] = __shfl(dv.v[b], a, G);
+}
+
+// Forward solve L y = r over the block rows of one level (unit-lower diagonal blocks).
+// Level rows are the contiguous range [first, first + count) or list[first + g].
+template <typename T, typename LT, int G, int K>
+__global__ __launch_bounds__(256) void k_bsr_lsolve(const int32_t* __restrict__ list, int first, int count,
+                                                    const BDesc* __restrict__ desc,
+                                                    const int32_t* __restrict__ bcol, const LT* __restrict__ lu,
+                                                    const T* __restrict__ r, T* __restrict__ y, int xcd) {
   using AT = typename AccOf<LT>::type;
   const int64_t gt = logical_wg(xcd) * 256 + threadIdx.x;
   const int64_t g = gt / G;
   const int lane = (int)(gt % G);
   AT acc[4] = {0, 0, 0, 0};
-  BlkDesc d = {0, 0, 0, 0};
+  BDesc d = {0, 0, 0, 0};
+  int32_t p = 0;
   AT rv = 0;
-  AT dv[DiagPerLane<G>::n] = {};
+  V4<AT> dv = {{0, 0, 0, 0}};
   const bool act = g < count;
   if (act) {
-    d = desc[list ? list[first + g] : first + (int32_t)g];
-    const int m = d.dqm >> 16, dq0 = d.dqm & 0xffff;
-    if (lane < m) rv = (AT)r[d.r0 + lane];
-    load_diag<G>(lu, d, lane, dv);
-    group_dot_v<G, U>(colidx + d.base, lu + d.base, y, d.rl, m, 0, dq0, lane, acc);
+    p = list ? list[first + g] : first + (int32_t)g;
+    d = desc[p];
+    const int m = d.dm >> 16, td = d.dm & 0xffff;
+    if (lane < 4) rv = (AT)r[4 * (int64_t)p + lane];
+    if (lane < m) dv = ld4<AT>(lu + d.vbase + (int64_t)lane * 4 * d.nb + 4 * td);
+    bsr_dot<G, K>(bcol + d.bptr, lu + d.vbase, 4 * d.nb, y, m, 0, td, lane, acc);
   }
   AT D[16], R[4];
   bcast_diag<G>(dv, D);
@@ -810,39 +642,41 @@ __global__ __launch_bounds__(256) void k_ilu_lsolve_blk(const int32_t* __restric
   for (int a = 0; a < 4; ++a) acc[a] = group_sum<G>(acc[a]);
 #pragma unroll
   for (int a = 0; a < 4; ++a) R[a] = __shfl(rv, a, G);
-  const int m = d.dqm >> 16;
+  const int m = d.dm >> 16;
   AT v[4];
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
     AT s = R[a] - acc[a];
 #pragma unroll
-    for (int ap = 0; ap < a; ++ap) s -= D[a * 4 + ap] * v[ap];
-    v[a] = s;
+    for (int b = 0; b < a; ++b) s -= D[a * 4 + b] * v[b];
+    v[a] = a < m ? s : (AT)0;
   }
-  if (act && lane < m) y[d.r0 + lane] = (T)pick4(v, lane);
+  if (act && lane < 4) y[4 * (int64_t)p + lane] = (T)pick4(v, lane);
 }
 
 // Backward solve U z = y over one level (levels in reverse order), in place on y.
-template <typename T, typename LT, int G, int U>
-__global__ __launch_bounds__(256) void k_ilu_usolve_blk(const int32_t* __restrict__ list, int first, int count,
-                                                        const BlkDesc* __restrict__ desc,
-                                                        const int32_t* __restrict__ colidx,
-                                                        const LT* __restrict__ lu, T* __restrict__ y, int xcd) {
+template <typename T, typename LT, int G, int K>
+__global__ __launch_bounds__(256) void k_bsr_usolve(const int32_t* __restrict__ list, int first, int count,
+                                                    const BDesc* __restrict__ desc,
+                                                    const int32_t* __restrict__ bcol, const LT* __restrict__ lu,
+                                                    T* __restrict__ y, int xcd) {
   using AT = typename AccOf<LT>::type;
   const int64_t gt = logical_wg(xcd) * 256 + threadIdx.x;
   const int64_t g = gt / G;
   const int lane = (int)(gt % G);
   AT acc[4] = {0, 0, 0, 0};
-  BlkDesc d = {0, 0, 0, 0};
+  BDesc d = {0, 0, 0, 0};
+  int32_t p = 0;
   AT rv = 0;
-  AT dv[DiagPerLane<G>::n] = {};
+  V4<AT> dv = {{0, 0, 0, 0}};
   const bool act = g < count;
   if (act) {
-    d = desc[list ? list[first + g] : first + (int32_t)g];
-    const int m = d.dqm >> 16, dq0 = d.dqm & 0xffff;
-    if (lane < m) rv = (AT)y[d.r0 + lane];
-    load_diag<G>(lu, d, lane, dv);
-    group_dot_v<G, U>(colidx + d.base, lu + d.base, y, d.rl, m, dq0 + m, d.rl, lane, acc);
+    p = list ? list[first + g] : first + (int32_t)g;
+    d = desc[p];
+    const int m = d.dm >> 16, td = d.dm & 0xffff;
+    if (lane < 4) rv = (AT)y[4 * (int64_t)p + lane];
+    if (lane < m) dv = ld4<AT>(lu + d.vbase + (int64_t)lane * 4 * d.nb + 4 * td);
+    bsr_dot<G, K>(bcol + d.bptr, lu + d.vbase, 4 * d.nb, y, m, td + 1, d.nb, lane, acc);
   }
   AT D[16], R[4];
   bcast_diag<G>(dv, D);
@@ -850,19 +684,183 @@ __global__ __launch_bounds__(256) void k_ilu_usolve_blk(const int32_t* __restric
   for (int a = 0; a < 4; ++a) acc[a] = group_sum<G>(acc[a]);
 #pragma unroll
   for (int a = 0; a < 4; ++a) R[a] = __shfl(rv, a, G);
-  const int m = d.dqm >> 16;
+  const int m = d.dm >> 16;
   AT v[4] = {0, 0, 0, 0};
 #pragma unroll
   for (int a = 3; a >= 0; --a) {
     if (a < m) {
       AT s = R[a] - acc[a];
 #pragma unroll
-      for (int ap = a + 1; ap < 4; ++ap)
-        if (ap < m) s -= D[a * 4 + ap] * v[ap];
+      for (int b = a + 1; b < 4; ++b) s -= D[a * 4 + b] * v[b];
       v[a] = s / D[a * 4 + a];
     }
   }
-  if (act && lane < m) y[d.r0 + lane] = (T)pick4(v, lane);
+  if (act && lane < 4) y[4 * (int64_t)p + lane] = (T)pick4(v, lane);
+}
+
+// Block values from the CSR values: block (p, t) is the neighbour's free columns in the
+// free rows of p, found in the DOF-ordered CSR at rowptr[r0orig[p] + a] + bcolofs.
+// One 64-lane wavefront per block row; dst1 (optional) receives a second copy.
+template <typename VT>
+__global__ __launch_bounds__(64) void k_bsr_gather(int64_t nblk, const BDesc* __restrict__ desc,
+                                                   const int32_t* __restrict__ bcol,
+                                                   const int32_t* __restrict__ bcolofs,
+                                                   const int32_t* __restrict__ r0orig,
+                                                   const int32_t* __restrict__ rowptr, const VT* __restrict__ src,
+                                                   VT* __restrict__ dst0, VT* __restrict__ dst1) {
+  const int64_t p = blockIdx.x;
+  if (p >= nblk) return;
+  const BDesc d = desc[p];
+  const int m = d.dm >> 16;
+  for (int idx = threadIdx.x; idx < m * d.nb; idx += 64) {
+    const int a = idx / d.nb, t = idx - a * d.nb;
+    const int mj = desc[bcol[d.bptr + t]].dm >> 16;
+    const int64_t s0 = (int64_t)rowptr[r0orig[p] + a] + bcolofs[d.bptr + t];
+    const int64_t o = (int64_t)d.vbase + (int64_t)a * 4 * d.nb + 4 * t;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const VT v = b < mj ? src[s0 + b] : (VT)0;
+      dst0[o + b] = v;
+      if (dst1) dst1[o + b] = v;
+    }
+  }
+}
+
+// ILU(0) factorisation of the block rows of one level, one 64-lane workgroup (one
+// wavefront) per block row, IKJ order by lower neighbour blocks: for neighbour block c
+// (ascending), the positions of c's upper blocks in this row's block list are searched
+// once, c's m_c upper rows are staged in LDS, and c's m_c pivots are eliminated from
+// LDS.  The global loads of the next lower block are issued before the eliminations of
+// the current one and land in registers meanwhile.  Dynamic LDS: bsr_factor_lds(cap)
+// bytes for rows of at most cap <= kBsrMaxNb neighbour blocks.
+template <typename LT>
+__host__ __device__ constexpr size_t bsr_factor_lds(int cap) {
+  return (size_t)cap * (16 * sizeof(double) + 16 * sizeof(LT) + 2 * sizeof(int32_t));
+}
+
+template <typename LT>
+__global__ __launch_bounds__(64) void k_bsr_factor(const int32_t* __restrict__ list, int first, int count,
+                                                   const BDesc* __restrict__ desc,
+                                                   const int32_t* __restrict__ bcol, LT* __restrict__ lu,
+                                                   double* __restrict__ scal, int cap) {
+  extern __shared__ double smem[];
+  double* rows = smem;                          // [4][4*cap], this block row
+  LT* urow = (LT*)(rows + 16 * cap);            // [4][4*cap], upper blocks of the lower block c
+  int32_t* cols = (int32_t*)(urow + 16 * cap);  // [cap], this row's neighbour blocks
+  int32_t* pos = cols + cap;                    // [cap], positions of c's upper blocks here
+  __shared__ double ub[16];                     // c's diagonal block
+  __shared__ double lval[4];
+  const int w = blockIdx.x;
+  if (w >= count) return;
+  const int lane = threadIdx.x;
+  const int32_t p = list ? list[first + w] : first + w;
+  const BDesc d = desc[p];
+  const int m = d.dm >> 16, td = d.dm & 0xffff, nb = d.nb, rs = 4 * nb;
+  for (int t = lane; t < nb; t += 64) cols[t] = bcol[d.bptr + t];
+  for (int a = 0; a < m; ++a)
+    for (int q = lane; q < rs; q += 64) rows[a * 4 * cap + q] = (double)lu[d.vbase + (int64_t)a * rs + q];
+  wave_sync();
+  // registers of the prefetched lower block: upper block index, its 4 x m_c values, and
+  // the diagonal block (lanes < 16)
+  int32_t pc = -1;
+  LT pu[4][4];
+  LT pd = (LT)0;
+  auto issue = [&](int t) {
+    const BDesc dc = desc[cols[t]];
+    const int mc = dc.dm >> 16, tdc = dc.dm & 0xffff, rsc = 4 * dc.nb;
+    const int nup = dc.nb - tdc - 1;
+    if (lane < nup) {
+      pc = bcol[dc.bptr + tdc + 1 + lane];
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int bp = 0; bp < 4; ++bp)
+          pu[b][bp] = b < mc ? lu[dc.vbase + (int64_t)b * rsc + 4 * (tdc + 1 + lane) + bp] : (LT)0;
+    }
+    if (lane < 16) pd = (lane >> 2) < mc ? lu[dc.vbase + (int64_t)(lane >> 2) * rsc + 4 * tdc + (lane & 3)] : (LT)0;
+  };
+  if (td > 0) issue(0);
+  for (int t = 0; t < td; ++t) {
+    const BDesc dc = desc[cols[t]];
+    const int mc = dc.dm >> 16, tdc = dc.dm & 0xffff;
+    const int nup = dc.nb - tdc - 1;
+    if (lane < nup) {
+      pos[lane] = lds_find(cols, nb, pc);
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int bp = 0; bp < 4; ++bp) urow[b * 4 * cap + 4 * lane + bp] = pu[b][bp];
+    }
+    if (lane < 16) ub[lane] = (double)pd;
+    wave_sync();
+    if (t + 1 < td) issue(t + 1);
+    for (int b = 0; b < mc; ++b) {
+      if (lane < m) {
+        const double dd = ub[b * 4 + b];
+        if (dd == 0.0) scal[S_MISC1] = 2.0;
+        const double l = rows[lane * 4 * cap + 4 * t + b] / (dd == 0.0 ? 1.0 : dd);
+        rows[lane * 4 * cap + 4 * t + b] = l;
+        lval[lane] = l;
+      }
+      wave_sync();
+      const int nin = 3 - b;  // remaining columns of block c (padding columns hold 0)
+      for (int i = lane; i < nin + 4 * nup; i += 64) {
+        int q;
+        double u;
+        if (i < nin) {
+          q = 4 * t + b + 1 + i;
+          u = ub[b * 4 + b + 1 + i];
+        } else {
+          const int k = (i - nin) >> 2, bp = (i - nin) & 3;
+          const int ps = pos[k];
+          q = ps >= 0 ? 4 * ps + bp : -1;
+          u = (double)urow[b * 4 * cap + 4 * k + bp];
+        }
+        if (q >= 0)
+          for (int a = 0; a < m; ++a) rows[a * 4 * cap + q] -= lval[a] * u;
+      }
+      wave_sync();
+    }
+  }
+  // the row's own diagonal block
+  const int q0 = 4 * td;
+  for (int a = 1; a < m; ++a) {
+    for (int ap = 0; ap < a; ++ap) {
+      if (lane == 0) {
+        const double dd = rows[ap * 4 * cap + q0 + ap];
+        if (dd == 0.0) scal[S_MISC1] = 2.0;
+        const double l = rows[a * 4 * cap + q0 + ap] / (dd == 0.0 ? 1.0 : dd);
+        rows[a * 4 * cap + q0 + ap] = l;
+        lval[0] = l;
+      }
+      wave_sync();
+      const double l = lval[0];
+      for (int q = q0 + ap + 1 + lane; q < rs; q += 64) rows[a * 4 * cap + q] -= l * rows[ap * 4 * cap + q];
+      wave_sync();
+    }
+  }
+  for (int a = 0; a < m; ++a)
+    for (int q = lane; q < rs; q += 64) lu[d.vbase + (int64_t)a * rs + q] = (LT)rows[a * 4 * cap + q];
+}
+
+// b[i] = scale * a[padmap[i]] into the padded block layout (0 where padmap < 0)
+template <typename TA, typename TB>
+__global__ __launch_bounds__(kTPB) void k_pad_in(int64_t n, const int32_t* __restrict__ padmap,
+                                                 const TA* __restrict__ a, TB* __restrict__ b, double scale) {
+  for (int64_t i = (int64_t)blockIdx.x * kTPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kTPB) {
+    const int32_t k = padmap[i];
+    b[i] = k >= 0 ? (TB)((double)a[k] * scale) : (TB)0;
+  }
+}
+
+// b[padmap[i]] = scale * a[i] back to the DOF layout
+template <typename TA, typename TB>
+__global__ __launch_bounds__(kTPB) void k_pad_out(int64_t n, const int32_t* __restrict__ padmap,
+                                                  const TA* __restrict__ a, TB* __restrict__ b, double scale) {
+  for (int64_t i = (int64_t)blockIdx.x * kTPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kTPB) {
+    const int32_t k = padmap[i];
+    if (k >= 0) b[k] = (TB)((double)a[i] * scale);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -891,8 +889,7 @@ static hipError_t alloc(T** p, size_t n) {
 LinAlg::~LinAlg() {
   void* ptrs[] = {d_blkStart, d_diagpos, d_part, d_scal, r, rh, p, v, s, t, ph, sh, x, w, z, pr, pz,
                   r32, rh32, p32, v32, s32, t32, ph32, sh32, x32, du32, w32, z32, pr32, pz32,
-                  dinv, dinv32, binv, binv32, lu, lu32, gm_V, gm_V32, d_desc, lu16, kp, kp32, d_rowBlk,
-                  d_h, d_mpart};
+                  dinv, dinv32, binv, binv32, lu, lu32, gm_V, gm_V32, d_desc, lu16, kp, kp32, d_h, d_mpart};
   delete ilu;
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
@@ -931,6 +928,8 @@ int la_create(sfem_ctx* ctx) {
     dp[r] = (f != &ctx->colidx[0] + e && *f == r) ? (int32_t)(f - &ctx->colidx[0]) : -1;
   }
   L->maxrow = maxrow;
+  L->nk = n;
+  L->nvec = std::max<int64_t>(n, 4 * L->nblk);  // Krylov vectors fit both layouts
   L->h_diagpos = dp;
   LA_CHECK(alloc(&L->d_blkStart, bs.size()));
   LA_CHECK(hipMemcpy(L->d_blkStart, bs.data(), bs.size() * sizeof(int32_t), hipMemcpyHostToDevice));
@@ -953,8 +952,6 @@ int la_create(sfem_ctx* ctx) {
       }
     }
     L->full_blocks = full;
-    LA_CHECK(alloc(&L->d_rowBlk, rowBlk.size()));
-    if (n) LA_CHECK(hipMemcpy(L->d_rowBlk, rowBlk.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
   }
   {
     std::vector<BlkDesc> dd = make_desc(bs, ctx->rowptr, dp);
@@ -982,35 +979,6 @@ static void bucket_levels(const std::vector<int32_t>& lv, std::vector<int32_t>& 
   list.resize(lv.size());
   std::vector<int32_t> pos(ptr.begin(), ptr.end() - 1);
   for (int64_t k = 0; k < (int64_t)lv.size(); ++k) list[pos[lv[k]]++] = (int32_t)k;
-}
-
-// Level of a block = 1 + max level of the blocks it depends on (lower blocks for the
-// forward solve and the factorisation, upper blocks for the backward solve).
-static void build_levels(const std::vector<int64_t>& rowptr, const std::vector<int32_t>& colidx,
-                         const std::vector<int32_t>& bs, IluPlan& P) {
-  const int64_t nb = (int64_t)bs.size() - 1;
-  std::vector<int32_t> rowBlk(bs.back());
-  for (int64_t k = 0; k < nb; ++k)
-    for (int32_t r = bs[k]; r < bs[k + 1]; ++r) rowBlk[r] = (int32_t)k;
-  std::vector<int32_t> lvL(nb, 0), lvU(nb, 0);
-  for (int64_t k = 0; k < nb; ++k) {
-    int lv = 0;
-    for (int64_t q = rowptr[bs[k]]; q < rowptr[bs[k] + 1]; ++q) {
-      const int32_t bk = rowBlk[colidx[q]];
-      if (bk < k) lv = std::max(lv, lvL[bk] + 1);
-    }
-    lvL[k] = lv;
-  }
-  for (int64_t k = nb - 1; k >= 0; --k) {
-    int lv = 0;
-    for (int64_t q = rowptr[bs[k]]; q < rowptr[bs[k] + 1]; ++q) {
-      const int32_t bk = rowBlk[colidx[q]];
-      if (bk > k) lv = std::max(lv, lvU[bk] + 1);
-    }
-    lvU[k] = lv;
-  }
-  bucket_levels(lvL, P.levLPtr, P.h_levL);
-  bucket_levels(lvU, P.levUPtr, P.h_levU);
 }
 
 // Reverse Cuthill-McKee over the block (node) graph, starting each component from a
@@ -1165,115 +1133,159 @@ static std::vector<BlkDesc> make_desc(const std::vector<int32_t>& bs, const std:
 }
 
 IluPlan::~IluPlan() {
-  void* ptrs[] = {d_levL, d_levU, d_rowptr, d_colidx, d_diagpos, d_blkStart, d_pmap, d_perm, d_desc, d_rowBlk};
+  void* ptrs[] = {d_levL, d_levU, d_bdesc, d_bcol, d_bcolofs, d_r0orig, d_padmap};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
 }
 
+// Block plan of ILU(0) for one node ordering: block rows in the new order, each with
+// its neighbour blocks sorted by new position, the offsets of those neighbours' first
+// columns in the DOF-ordered CSR row (for the value gather), block levels (one per
+// colour with the multicolour ordering), and the padded-slot -> DOF map of vectors.
 static int build_ilu_plan(sfem_ctx* ctx, LinAlg* L, int reorder) {
+  if (!L->full_blocks)
+    return set_error(SFEM_ERR_UNSUPPORTED, "ILU(0): the pattern is not a union of full node blocks");
   delete L->ilu;
-  L->ilu = new IluPlan;
+  L->ilu = nullptr;
+  // factor-sized buffers belong to the previous plan
+  void** vals[] = {(void**)&L->kp, (void**)&L->kp32, (void**)&L->lu, (void**)&L->lu32, (void**)&L->lu16};
+  for (void** q : vals)
+    if (*q) { (void)hipFree(*q); *q = nullptr; }
+  IluPlan* Pp = new IluPlan;
+  L->ilu = Pp;
   ++L->plan_id;
-  IluPlan& P = *L->ilu;
+  IluPlan& P = *Pp;
   P.reorder = reorder;
-  P.identity = (reorder == 0);
   const std::vector<int32_t>& bs = L->h_blkStart;
   const int64_t nb = L->nblk;
-  if (P.identity) {
-    build_levels(ctx->rowptr, ctx->colidx, bs, P);
-    P.d_desc = nullptr;  // the context's descriptors are used
-    P.rowptr = ctx->d_rowptr32;
-    P.colidx = ctx->d_colidx;
-    P.diagpos = L->d_diagpos;
-    P.blkStart = L->d_blkStart;
-    P.rowBlk = L->d_rowBlk;
+  std::vector<int32_t> rowBlk(L->n);
+  for (int64_t k = 0; k < nb; ++k)
+    for (int32_t r = bs[k]; r < bs[k + 1]; ++r) rowBlk[r] = (int32_t)k;
+  // neighbour blocks of every block (self included) with their column offsets
+  std::vector<int64_t> nbPtr(nb + 1, 0);
+  for (int64_t k = 0; k < nb; ++k) {
+    int32_t cnt = 0, prev = -1;
+    for (int64_t q = ctx->rowptr[bs[k]]; q < ctx->rowptr[bs[k] + 1]; ++q) {
+      const int32_t j = rowBlk[ctx->colidx[q]];
+      if (j != prev) ++cnt;
+      prev = j;
+    }
+    nbPtr[k + 1] = nbPtr[k] + cnt;
+  }
+  const int64_t npairs = nbPtr[nb];
+  std::vector<int32_t> nbB(npairs), nbOfs(npairs);
+  parallel_for(nb, [&](int64_t lo, int64_t hi) {
+    for (int64_t k = lo; k < hi; ++k) {
+      int64_t o = nbPtr[k];
+      int32_t prev = -1;
+      const int64_t r0 = ctx->rowptr[bs[k]];
+      for (int64_t q = r0; q < ctx->rowptr[bs[k] + 1]; ++q) {
+        const int32_t j = rowBlk[ctx->colidx[q]];
+        if (j != prev) { nbB[o] = j; nbOfs[o] = (int32_t)(q - r0); ++o; }
+        prev = j;
+      }
+    }
+  });
+  // node ordering on the graph without self loops
+  std::vector<int32_t> order(nb), colorPtr;
+  if (reorder == 0) {
+    for (int64_t k = 0; k < nb; ++k) order[k] = (int32_t)k;
   } else {
-    // block graph from the first row of each block
-    std::vector<int32_t> rowBlk(L->n);
-    for (int64_t k = 0; k < nb; ++k)
-      for (int32_t r = bs[k]; r < bs[k + 1]; ++r) rowBlk[r] = (int32_t)k;
     std::vector<int64_t> adjPtr(nb + 1, 0);
     std::vector<int32_t> adj;
+    adj.reserve(npairs - nb);
     for (int64_t k = 0; k < nb; ++k) {
-      int32_t prev = -1;
-      for (int64_t q = ctx->rowptr[bs[k]]; q < ctx->rowptr[bs[k] + 1]; ++q) {
-        const int32_t b = rowBlk[ctx->colidx[q]];
-        if (b != prev && b != k) adj.push_back(b);
-        prev = b;
-      }
+      for (int64_t q = nbPtr[k]; q < nbPtr[k + 1]; ++q)
+        if (nbB[q] != k) adj.push_back(nbB[q]);
       adjPtr[k + 1] = (int64_t)adj.size();
     }
-    std::vector<int32_t> colorPtr;
-    const std::vector<int32_t> order = reorder == 2 ? multicolor_order(adjPtr, adj, &colorPtr) : rcm_order(adjPtr, adj);
-    std::vector<int32_t> nbs(nb + 1, 0), perm(L->n), iperm(L->n);
-    for (int64_t p = 0; p < nb; ++p) {
+    order = reorder == 2 ? multicolor_order(adjPtr, adj, &colorPtr) : rcm_order(adjPtr, adj);
+  }
+  std::vector<int32_t> iord(nb);
+  for (int64_t p = 0; p < nb; ++p) iord[order[p]] = (int32_t)p;
+  // block rows in the new order
+  std::vector<BDesc> bd(nb);
+  std::vector<int64_t> bptr(nb + 1, 0);
+  for (int64_t p = 0; p < nb; ++p) {
+    const int32_t k = order[p];
+    bptr[p + 1] = bptr[p] + (nbPtr[k + 1] - nbPtr[k]);
+  }
+  std::vector<int32_t> bcol(npairs), bcolofs(npairs), tdiag(nb);
+  int nbmax = 0;
+  for (int64_t k = 0; k < nb; ++k) nbmax = std::max(nbmax, (int)(nbPtr[k + 1] - nbPtr[k]));
+  if (nbmax > kBsrMaxNb) {
+    delete L->ilu;
+    L->ilu = nullptr;
+    return set_error(SFEM_ERR_UNSUPPORTED, "ILU(0): a node couples to more than 64 node blocks");
+  }
+  parallel_for(nb, [&](int64_t lo, int64_t hi) {
+    std::vector<std::pair<int32_t, int32_t>> tmp;
+    for (int64_t p = lo; p < hi; ++p) {
       const int32_t k = order[p];
-      nbs[p + 1] = nbs[p] + (bs[k + 1] - bs[k]);
-      for (int32_t a = 0; a < bs[k + 1] - bs[k]; ++a) perm[nbs[p] + a] = bs[k] + a;
-    }
-    for (int64_t i = 0; i < L->n; ++i) iperm[perm[i]] = (int32_t)i;
-    // permuted rows: every row of a block shares its column list, so sort once per block
-    std::vector<int64_t> rp(L->n + 1, 0);
-    for (int64_t p = 0; p < nb; ++p) {
-      const int32_t k = order[p];
-      const int64_t len = ctx->rowptr[bs[k] + 1] - ctx->rowptr[bs[k]];
-      for (int32_t r = nbs[p]; r < nbs[p + 1]; ++r) rp[r + 1] = rp[r] + len;
-    }
-    std::vector<int32_t> ci(ctx->nnz), pmap(ctx->nnz), dp(L->n, -1);
-    parallel_for(nb, [&](int64_t lo, int64_t hi) {
-      std::vector<std::pair<int32_t, int32_t>> tmp;
-      for (int64_t p = lo; p < hi; ++p) {
-        const int32_t k = order[p];
-        const int64_t b0 = ctx->rowptr[bs[k]];
-        const int64_t len = ctx->rowptr[bs[k] + 1] - b0;
-        tmp.resize(len);
-        for (int64_t q = 0; q < len; ++q) tmp[q] = {iperm[ctx->colidx[b0 + q]], (int32_t)q};
-        std::sort(tmp.begin(), tmp.end());
-        for (int32_t a = 0; a < bs[k + 1] - bs[k]; ++a) {
-          const int32_t rn = nbs[p] + a;
-          const int64_t ob = ctx->rowptr[bs[k] + a];
-          for (int64_t q = 0; q < len; ++q) {
-            ci[rp[rn] + q] = tmp[q].first;
-            pmap[rp[rn] + q] = (int32_t)(ob + tmp[q].second);
-            if (tmp[q].first == rn) dp[rn] = (int32_t)(rp[rn] + q);
-          }
-        }
+      tmp.clear();
+      for (int64_t q = nbPtr[k]; q < nbPtr[k + 1]; ++q) tmp.push_back({iord[nbB[q]], nbOfs[q]});
+      std::sort(tmp.begin(), tmp.end());
+      for (size_t i = 0; i < tmp.size(); ++i) {
+        bcol[bptr[p] + i] = tmp[i].first;
+        bcolofs[bptr[p] + i] = tmp[i].second;
+        if (tmp[i].first == p) tdiag[p] = (int32_t)i;
       }
-    });
-    if (reorder == 2) {
-      // one level per colour: blocks of a colour are independent and contiguous in the
-      // new order; the backward solve walks the colours in reverse
-      P.contiguous = true;
-      P.levLPtr = colorPtr;
-      P.levUPtr = colorPtr;
-      P.h_levL.clear();
-      P.h_levU.clear();
-    } else {
-      build_levels(rp, ci, nbs, P);
     }
-    {
-      std::vector<BlkDesc> dd = make_desc(nbs, rp, dp);
-      hipError_t e = alloc(&P.d_desc, dd.size());
-      if (e == hipSuccess) e = hipMemcpy(P.d_desc, dd.data(), dd.size() * sizeof(BlkDesc), hipMemcpyHostToDevice);
-      if (e != hipSuccess) return set_hip_error(e, "ILU descriptors");
+  });
+  int64_t vb = 0;
+  std::vector<int32_t> r0orig(nb), padmap(4 * nb, -1);
+  for (int64_t p = 0; p < nb; ++p) {
+    const int32_t k = order[p];
+    const int32_t m = bs[k + 1] - bs[k];
+    const int32_t cnt = (int32_t)(bptr[p + 1] - bptr[p]);
+    bd[p] = BDesc{(int32_t)vb, (int32_t)bptr[p], cnt, tdiag[p] | (m << 16)};
+    vb += (int64_t)m * 4 * cnt;
+    if (vb >= ((int64_t)1 << 31)) {
+      delete L->ilu;
+      L->ilu = nullptr;
+      return set_error(SFEM_ERR_UNSUPPORTED, "ILU(0): block values exceed 2^31");
     }
-    std::vector<int32_t> rp32(rp.begin(), rp.end());
-    std::vector<int32_t> rb(L->n);
-    for (int64_t p = 0; p < nb; ++p)
-      for (int32_t r = nbs[p]; r < nbs[p + 1]; ++r) rb[r] = (int32_t)p;
-    int rc;
-    if ((rc = upload(&P.d_rowptr, rp32)) || (rc = upload(&P.d_colidx, ci)) || (rc = upload(&P.d_diagpos, dp)) ||
-        (rc = upload(&P.d_blkStart, nbs)) || (rc = upload(&P.d_pmap, pmap)) || (rc = upload(&P.d_perm, perm)) ||
-        (rc = upload(&P.d_rowBlk, rb)))
-      return rc;
-    P.rowBlk = P.d_rowBlk;
-    P.rowptr = P.d_rowptr;
-    P.colidx = P.d_colidx;
-    P.diagpos = P.d_diagpos;
-    P.blkStart = P.d_blkStart;
+    r0orig[p] = bs[k];
+    for (int32_t a = 0; a < m; ++a) padmap[4 * p + a] = bs[k] + a;
+  }
+  P.nblk = nb;
+  P.npad = 4 * nb;
+  P.nvals = vb;
+  P.npairs = npairs;
+  P.nbmax = nbmax;
+  P.h_order = order;
+  if (reorder == 2) {
+    // one level per colour: blocks of a colour are independent and contiguous in the
+    // new order; the backward solve walks the colours in reverse
+    P.contiguous = true;
+    P.levLPtr = colorPtr;
+    P.levUPtr = colorPtr;
+  } else {
+    // level of a block row = 1 + max level of the rows it depends on (lower
+    // neighbours for the forward solve and the factorisation, upper ones backward)
+    std::vector<int32_t> lvL(nb, 0), lvU(nb, 0);
+    for (int64_t p = 0; p < nb; ++p) {
+      int lv = 0;
+      for (int64_t q = bptr[p]; q < bptr[p] + tdiag[p]; ++q) lv = std::max(lv, lvL[bcol[q]] + 1);
+      lvL[p] = lv;
+    }
+    for (int64_t p = nb - 1; p >= 0; --p) {
+      int lv = 0;
+      for (int64_t q = bptr[p] + tdiag[p] + 1; q < bptr[p + 1]; ++q) lv = std::max(lv, lvU[bcol[q]] + 1);
+      lvU[p] = lv;
+    }
+    bucket_levels(lvL, P.levLPtr, P.h_levL);
+    bucket_levels(lvU, P.levUPtr, P.h_levU);
   }
   int rc;
-  if ((rc = upload(&P.d_levL, P.h_levL)) || (rc = upload(&P.d_levU, P.h_levU))) return rc;
+  {
+    hipError_t e = alloc(&P.d_bdesc, bd.size());
+    if (e == hipSuccess) e = hipMemcpy(P.d_bdesc, bd.data(), bd.size() * sizeof(BDesc), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return set_hip_error(e, "ILU block descriptors");
+  }
+  if ((rc = upload(&P.d_bcol, bcol)) || (rc = upload(&P.d_bcolofs, bcolofs)) || (rc = upload(&P.d_r0orig, r0orig)) ||
+      (rc = upload(&P.d_padmap, padmap)) || (rc = upload(&P.d_levL, P.h_levL)) || (rc = upload(&P.d_levU, P.h_levU)))
+    return rc;
   return SFEM_OK;
 }
 
@@ -1339,7 +1351,6 @@ int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f) 
     }
     LA_CHECK(hipGetLastError());
   } else if (pc_type == SFEM_PC_ILU0) {
-    if (L->maxrow > kIluMaxRow) return set_error(SFEM_ERR_UNSUPPORTED, "ILU(0): row longer than 256 entries");
     for (int64_t r = 0; r < n; ++r)
       if (L->h_diagpos[r] < 0) return set_error(SFEM_ERR_ARG, "ILU(0): missing diagonal entry");
     if (!L->ilu || L->ilu->reorder != reorder) {
@@ -1347,54 +1358,39 @@ int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f) 
       if (rc) return rc;
     }
     const IluPlan& P = *L->ilu;
-    const int64_t nnz = ctx->nnz;
-    const unsigned gz = (unsigned)std::min<int64_t>((nnz + kTPB - 1) / kTPB, 1 << 20);
-    // reordered plan: one gather gives the permuted tangent (for the products of the
-    // Krylov solve) and the factor's starting values
+    const int64_t nv = P.nvals;
+    // one gather gives the block tangent (for the products of the Krylov solve) and the
+    // factor's starting values
+    const unsigned gb = (unsigned)std::max<int64_t>(P.nblk, 1);
     if (fp32) {
-      int rc = ensure_vec(&L->lu32, nnz); if (rc) return rc;
-      if (P.identity) {
-        LA_CHECK(hipMemcpyAsync(L->lu32, ctx->d_vals32, nnz * sizeof(float), hipMemcpyDeviceToDevice, st));
-      } else {
-        if ((rc = ensure_vec(&L->kp32, nnz))) return rc;
-        hipLaunchKernelGGL((k_gather2<float>), dim3(gz), dim3(kTPB), 0, st, nnz, P.d_pmap, ctx->d_vals32,
-                           L->kp32, L->lu32);
-      }
+      int rc;
+      if ((rc = ensure_vec(&L->kp32, nv)) || (rc = ensure_vec(&L->lu32, nv))) return rc;
+      hipLaunchKernelGGL((k_bsr_gather<float>), dim3(gb), dim3(64), 0, st, P.nblk, P.d_bdesc, P.d_bcol,
+                         P.d_bcolofs, P.d_r0orig, ctx->d_rowptr32, ctx->d_vals32, L->kp32, L->lu32);
     } else {
-      int rc = ensure_vec(&L->lu, nnz); if (rc) return rc;
-      if (P.identity) {
-        LA_CHECK(hipMemcpyAsync(L->lu, ctx->d_vals, nnz * sizeof(double), hipMemcpyDeviceToDevice, st));
-      } else {
-        if ((rc = ensure_vec(&L->kp, nnz))) return rc;
-        hipLaunchKernelGGL((k_gather2<double>), dim3(gz), dim3(kTPB), 0, st, nnz, P.d_pmap, ctx->d_vals,
-                           L->kp, L->lu);
-      }
+      int rc;
+      if ((rc = ensure_vec(&L->kp, nv)) || (rc = ensure_vec(&L->lu, nv))) return rc;
+      hipLaunchKernelGGL((k_bsr_gather<double>), dim3(gb), dim3(64), 0, st, P.nblk, P.d_bdesc, P.d_bcol,
+                         P.d_bcolofs, P.d_r0orig, ctx->d_rowptr32, ctx->d_vals, L->kp, L->lu);
     }
+    const int cap = (P.nbmax + 15) / 16 * 16;
     const int nl = (int)P.levLPtr.size() - 1;
     for (int lv = 0; lv < nl; ++lv) {
       const int32_t* list; int first, cnt;
       level_range(P, false, lv, &list, &first, &cnt);
       if (!cnt) continue;
-      const BlkDesc* desc = P.identity ? L->d_desc : P.d_desc;
-      if (L->full_blocks) {
-        const int cap = std::max(64, (L->maxrow + 63) / 64 * 64);
-        if (fp32)
-          hipLaunchKernelGGL((k_ilu_factor_nb<float>), dim3(cnt), dim3(64), factor_nb_lds<float>(cap), st, list,
-                             first, cnt, desc, P.colidx, P.rowBlk, L->lu32, L->d_scal, cap);
-        else
-          hipLaunchKernelGGL((k_ilu_factor_nb<double>), dim3(cnt), dim3(64), factor_nb_lds<double>(cap), st,
-                             list, first, cnt, desc, P.colidx, P.rowBlk, L->lu, L->d_scal, cap);
-      } else if (fp32)
-        hipLaunchKernelGGL((k_ilu_factor_blk<float>), dim3(cnt), dim3(64), 0, st, list, first, cnt,
-                           P.blkStart, P.rowptr, P.colidx, P.diagpos, L->lu32, L->d_scal);
+      if (fp32)
+        hipLaunchKernelGGL((k_bsr_factor<float>), dim3(cnt), dim3(64), bsr_factor_lds<float>(cap), st, list, first,
+                           cnt, P.d_bdesc, P.d_bcol, L->lu32, L->d_scal, cap);
       else
-        hipLaunchKernelGGL((k_ilu_factor_blk<double>), dim3(cnt), dim3(64), 0, st, list, first, cnt,
-                           P.blkStart, P.rowptr, P.colidx, P.diagpos, L->lu, L->d_scal);
+        hipLaunchKernelGGL((k_bsr_factor<double>), dim3(cnt), dim3(64), bsr_factor_lds<double>(cap), st, list,
+                           first, cnt, P.d_bdesc, P.d_bcol, L->lu, L->d_scal, cap);
     }
     L->pc_bf16 = fp32 && bf16f;
     if (L->pc_bf16) {
-      int rc = ensure_vec(&L->lu16, nnz); if (rc) return rc;
-      hipLaunchKernelGGL(k_to_bf16, dim3(gz), dim3(256), 0, st, nnz, L->lu32, L->lu16);
+      int rc = ensure_vec(&L->lu16, nv); if (rc) return rc;
+      const unsigned gz = (unsigned)std::min<int64_t>((nv + 255) / 256, 1 << 20);
+      hipLaunchKernelGGL(k_to_bf16, dim3(gz), dim3(256), 0, st, nv, L->lu32, L->lu16);
     }
     LA_CHECK(hipGetLastError());
   }
@@ -1412,54 +1408,51 @@ static int pc_apply(sfem_ctx* ctx, const T* r, T* z) {
   return rc;
 }
 
-// Triangular-solve kernel shapes: code = vi | (xcd << 5), with G = kTriG[vi] lanes per
-// node block and U = kTriU[vi] columns in flight per lane (0: all of a lane's columns,
-// up to 128 / G, loaded at once).
-static const int kTriG[15] = {4, 4, 4, 8, 8, 8, 8, 16, 16, 16, 16, 32, 32, 32, 32};
-static const int kTriU[15] = {1, 2, 4, 1, 2, 4, 0, 1, 2, 4, 0, 1, 2, 4, 0};
-constexpr int kTriCodes = 15;
-constexpr int kTriDefault = 8;  // G = 16, U = 2, no XCD remap
+// Block kernel shapes (SpMV and triangular solves): code = vi | (xcd << 5), with
+// G = kBlkG[vi] lanes per block row and K = kBlkK[vi] neighbour blocks per lane whose
+// loads are issued together.
+static const int kBlkG[8] = {4, 4, 8, 8, 16, 16, 32, 64};
+static const int kBlkK[8] = {1, 2, 1, 2, 1, 2, 1, 1};
+constexpr int kBlkCodes = 8;
+constexpr int kBlkDefault = 6;  // G = 32, K = 1, no XCD remap
 
-template <typename T, typename LT, bool UPPER, int G, int U>
-static void tri_launch_g(hipStream_t st, int xcd, const int32_t* list, int first, int cnt, const BlkDesc* desc,
-                         const int32_t* col, const LT* lu, const T* r, T* y) {
+template <typename T, typename LT, bool UPPER, int G, int K>
+static void tri_launch_g(hipStream_t st, int xcd, const int32_t* list, int first, int cnt, const IluPlan& P,
+                         const LT* lu, const T* r, T* y) {
   const unsigned gr = (unsigned)(((int64_t)cnt * G + 255) / 256);
   if constexpr (UPPER)
-    hipLaunchKernelGGL((k_ilu_usolve_blk<T, LT, G, U>), dim3(gr), dim3(256), 0, st, list, first, cnt, desc, col,
-                       lu, y, xcd);
+    hipLaunchKernelGGL((k_bsr_usolve<T, LT, G, K>), dim3(gr), dim3(256), 0, st, list, first, cnt, P.d_bdesc,
+                       P.d_bcol, lu, y, xcd);
   else
-    hipLaunchKernelGGL((k_ilu_lsolve_blk<T, LT, G, U>), dim3(gr), dim3(256), 0, st, list, first, cnt, desc, col,
-                       lu, r, y, xcd);
+    hipLaunchKernelGGL((k_bsr_lsolve<T, LT, G, K>), dim3(gr), dim3(256), 0, st, list, first, cnt, P.d_bdesc,
+                       P.d_bcol, lu, r, y, xcd);
 }
 
 template <typename T, typename LT, bool UPPER>
-static void tri_launch(hipStream_t st, int code, const int32_t* list, int first, int cnt, const BlkDesc* desc,
-                       const int32_t* col, const LT* lu, const T* r, T* y) {
+static void tri_launch(hipStream_t st, int code, const int32_t* list, int first, int cnt, const IluPlan& P,
+                       const LT* lu, const T* r, T* y) {
   const int xcd = code >> 5;
-#define SFEM_TRI(i, G, U) \
-  case i: tri_launch_g<T, LT, UPPER, G, U>(st, xcd, list, first, cnt, desc, col, lu, r, y); break;
+#define SFEM_TRI(i, G, K) \
+  case i: tri_launch_g<T, LT, UPPER, G, K>(st, xcd, list, first, cnt, P, lu, r, y); break;
   switch (code & 31) {
-    SFEM_TRI(0, 4, 1) SFEM_TRI(1, 4, 2) SFEM_TRI(2, 4, 4)
-    SFEM_TRI(3, 8, 1) SFEM_TRI(4, 8, 2) SFEM_TRI(5, 8, 4) SFEM_TRI(6, 8, 0)
-    SFEM_TRI(7, 16, 1) SFEM_TRI(8, 16, 2) SFEM_TRI(9, 16, 4) SFEM_TRI(10, 16, 0)
-    SFEM_TRI(11, 32, 1) SFEM_TRI(12, 32, 2) SFEM_TRI(13, 32, 4)
-    default: tri_launch_g<T, LT, UPPER, 32, 0>(st, xcd, list, first, cnt, desc, col, lu, r, y); break;
+    SFEM_TRI(0, 4, 1) SFEM_TRI(1, 4, 2) SFEM_TRI(2, 8, 1) SFEM_TRI(3, 8, 2)
+    SFEM_TRI(4, 16, 1) SFEM_TRI(5, 16, 2) SFEM_TRI(6, 32, 1)
+    default: tri_launch_g<T, LT, UPPER, 64, 1>(st, xcd, list, first, cnt, P, lu, r, y); break;
   }
 #undef SFEM_TRI
 }
 
-// Apply the ILU(0) preconditioner: z = U^{-1} L^{-1} r, the forward solve into z and the
-// backward solve in place.  r and z are in the plan's ordering (the Krylov solve runs
-// there when the plan is reordered).  Each level launch uses its autotuned shape, or
-// `force` for all levels when force >= 0; with ev non-null an event is recorded after
-// every level (ev[0] before the first), forward levels then backward levels.
+// Apply the ILU(0) preconditioner: z = U^{-1} L^{-1} r in the plan's padded block
+// layout, the forward solve into z and the backward solve in place.  Each level launch
+// uses its autotuned shape, or `force` for all levels when force >= 0; with ev non-null
+// an event is recorded after every level (ev[0] before the first), forward levels then
+// backward levels.
 template <typename T, typename LT>
 static void ilu_solve(sfem_ctx* ctx, const IluPlan& P, const LT* lu, const T* r, T* z, int force = -1,
                       hipEvent_t* ev = nullptr) {
   LinAlg* L = ctx->la;
   hipStream_t st = ctx->stream;
   const int slot = sizeof(T) == 8 ? 0 : 1;
-  const BlkDesc* desc = P.identity ? L->d_desc : P.d_desc;
   const std::vector<int>& tl = L->tune_L[slot];
   const std::vector<int>& tu = L->tune_U[slot];
   const int nl = (int)P.levLPtr.size() - 1;
@@ -1468,15 +1461,15 @@ static void ilu_solve(sfem_ctx* ctx, const IluPlan& P, const LT* lu, const T* r,
   for (int lv = 0; lv < nl; ++lv) {
     const int32_t* list; int first, cnt;
     level_range(P, false, lv, &list, &first, &cnt);
-    const int code = force >= 0 ? force : lv < (int)tl.size() ? tl[lv] : kTriDefault;
-    if (cnt) tri_launch<T, LT, false>(st, code, list, first, cnt, desc, P.colidx, lu, r, z);
+    const int code = force >= 0 ? force : lv < (int)tl.size() ? tl[lv] : kBlkDefault;
+    if (cnt) tri_launch<T, LT, false>(st, code, list, first, cnt, P, lu, r, z);
     if (ev) (void)hipEventRecord(ev[1 + lv], st);
   }
   for (int lv = 0; lv < nu; ++lv) {
     const int32_t* list; int first, cnt;
     level_range(P, true, lv, &list, &first, &cnt);
-    const int code = force >= 0 ? force : lv < (int)tu.size() ? tu[lv] : kTriDefault;
-    if (cnt) tri_launch<T, LT, true>(st, code, list, first, cnt, desc, P.colidx, lu, z, z);
+    const int code = force >= 0 ? force : lv < (int)tu.size() ? tu[lv] : kBlkDefault;
+    if (cnt) tri_launch<T, LT, true>(st, code, list, first, cnt, P, lu, z, z);
     if (ev) (void)hipEventRecord(ev[1 + nl + lv], st);
   }
 }
@@ -1543,51 +1536,80 @@ static void spmv_launch(hipStream_t st, int code, int64_t nb, const BlkDesc* des
 #undef SFEM_SPMV
 }
 
+template <typename VT, typename T>
+static void bsr_spmv_launch(hipStream_t st, int code, const IluPlan& P, const VT* vals, const T* x, T* y) {
+  const int vi = code & 31, xcd = code >> 5;
+  const unsigned g = (unsigned)((P.nblk * kBlkG[vi] + 255) / 256);
+#define SFEM_BSPMV(i, G, K)                                                                                  \
+  case i:                                                                                                    \
+    hipLaunchKernelGGL((k_bsr_spmv<VT, T, G, K>), dim3(g), dim3(256), 0, st, P.nblk, P.d_bdesc, P.d_bcol, vals, \
+                       x, y, xcd);                                                                           \
+    break;
+  switch (vi) {
+    SFEM_BSPMV(0, 4, 1) SFEM_BSPMV(1, 4, 2) SFEM_BSPMV(2, 8, 1) SFEM_BSPMV(3, 8, 2)
+    SFEM_BSPMV(4, 16, 1) SFEM_BSPMV(5, 16, 2) SFEM_BSPMV(6, 32, 1)
+    default:
+      hipLaunchKernelGGL((k_bsr_spmv<VT, T, 64, 1>), dim3(g), dim3(256), 0, st, P.nblk, P.d_bdesc, P.d_bcol, vals,
+                         x, y, xcd);
+      break;
+  }
+#undef SFEM_BSPMV
+}
+
+// y = K x in the DOF layout (block = false, CSR) or in the ILU(0) plan's padded block
+// layout (block = true).
 template <typename T>
-static void spmv_in(sfem_ctx* ctx, const T* x, T* y, bool perm) {
+static void spmv_in(sfem_ctx* ctx, const T* x, T* y, bool block) {
   LinAlg* L = ctx->la;
-  const int64_t nb = L->nblk;
-  if (nb == 0) return;
+  const int slot = sizeof(T) == 8 ? 0 : 1;
+  if (L->nblk == 0) return;
   ctx->kt.begin(SFEM_KT_SPMV, ctx->stream);
-  const int v = L->tune_spmv[sizeof(T) == 8 ? 0 : 1];
-  const BlkDesc* desc = perm ? L->ilu->d_desc : L->d_desc;
-  const int32_t* col = perm ? L->ilu->colidx : ctx->d_colidx;
-  if (sizeof(T) == 8) spmv_launch<double, T>(ctx->stream, v, nb, desc, col, perm ? L->kp : ctx->d_vals, x, y);
-  else spmv_launch<float, T>(ctx->stream, v, nb, desc, col, perm ? L->kp32 : ctx->d_vals32, x, y);
+  if (block) {
+    const int code = L->tune_bspmv[slot];
+    if (sizeof(T) == 8) bsr_spmv_launch<double, T>(ctx->stream, code, *L->ilu, L->kp, x, y);
+    else bsr_spmv_launch<float, T>(ctx->stream, code, *L->ilu, L->kp32, x, y);
+  } else {
+    const int v = L->tune_spmv[slot];
+    if (sizeof(T) == 8) spmv_launch<double, T>(ctx->stream, v, L->nblk, L->d_desc, ctx->d_colidx, ctx->d_vals, x, y);
+    else spmv_launch<float, T>(ctx->stream, v, L->nblk, L->d_desc, ctx->d_colidx, ctx->d_vals32, x, y);
+  }
   ctx->kt.end(SFEM_KT_SPMV, ctx->stream);
 }
 
 template <typename T>
 static void spmv(sfem_ctx* ctx, const T* x, T* y) {
-  spmv_in<T>(ctx, x, y, ctx->la->perm_space);
+  spmv_in<T>(ctx, x, y, ctx->la->block_space);
 }
 
-// Autotuner, run once per precision on the first solve with the context's own matrix
-// (HIP events on the context stream).  The SpMV keeps the fastest of its shapes; the
-// ILU(0) apply is timed level by level for every shape and each level keeps its own
-// fastest, since light levels (few lower or upper neighbours) favour few lanes per node
-// block and heavy ones many.
+// Autotuner, run once per precision and layout on the first solve with the context's own
+// matrix (HIP events on the context stream).  The SpMV keeps the fastest of its shapes;
+// the ILU(0) apply is timed level by level for every shape and each level keeps its own
+// fastest, since light levels (few lower or upper neighbours) favour few lanes per block
+// row and heavy ones many.
 template <typename T>
 static int autotune(sfem_ctx* ctx, T* x, T* y) {
   LinAlg* L = ctx->la;
   const int slot = sizeof(T) == 8 ? 0 : 1;
   hipStream_t st = ctx->stream;
   const bool on = ctx->kt.on;
+  const bool block = L->block_space;
   ctx->kt.on = false;
-  std::vector<hipEvent_t> ev(2);
+  std::vector<hipEvent_t> ev(2, nullptr);
   auto cleanup = [&] {
     for (hipEvent_t e : ev)
       if (e) (void)hipEventDestroy(e);
     ctx->kt.on = on;
   };
-  for (hipEvent_t& e : ev) e = nullptr;
   for (hipEvent_t& e : ev)
     if (hipEventCreate(&e) != hipSuccess) { e = nullptr; cleanup(); return set_error(SFEM_ERR_HIP, "autotune events"); }
-  hipLaunchKernelGGL((k_fill<T>), dim3(grid_for(L->n)), dim3(kTPB), 0, st, L->n, x, (T)1);
+  hipLaunchKernelGGL((k_fill<T>), dim3(grid_for(L->nk)), dim3(kTPB), 0, st, L->nk, x, (T)1);
   float best = 1e30f;
+  const int ncodes = block ? kBlkCodes : kSpmvCodes;
+  int* cur = block ? &L->tune_bspmv[slot] : &L->tune_spmv[slot];
+  int bestc = *cur;
   for (int xcd = 0; xcd < 2; ++xcd)
-    for (int vi = 0; vi < kSpmvCodes; ++vi) {
-      L->tune_spmv[slot] = vi | (xcd << 5);
+    for (int vi = 0; vi < ncodes; ++vi) {
+      *cur = vi | (xcd << 5);
       spmv<T>(ctx, x, y);  // warm
       (void)hipEventRecord(ev[0], st);
       for (int k = 0; k < 3; ++k) spmv<T>(ctx, x, y);
@@ -1595,10 +1617,10 @@ static int autotune(sfem_ctx* ctx, T* x, T* y) {
       (void)hipEventSynchronize(ev[1]);
       float ms = 0.f;
       (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
-      if (ms < best) { best = ms; L->tune_best_spmv[slot] = vi | (xcd << 5); }
+      if (ms < best) { best = ms; bestc = *cur; }
     }
-  L->tune_spmv[slot] = L->tune_best_spmv[slot];
-  if (L->pc_type == SFEM_PC_ILU0) {
+  *cur = bestc;
+  if (block) {
     const IluPlan& P = *L->ilu;
     const int nl = (int)P.levLPtr.size() - 1, nu = (int)P.levUPtr.size() - 1;
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
@@ -1606,9 +1628,9 @@ static int autotune(sfem_ctx* ctx, T* x, T* y) {
     for (hipEvent_t& e : ev)
       if (hipEventCreate(&e) != hipSuccess) { e = nullptr; cleanup(); return set_error(SFEM_ERR_HIP, "autotune events"); }
     std::vector<float> bt(nl + nu, 1e30f), acc(nl + nu);
-    std::vector<int> bc(nl + nu, kTriDefault);
+    std::vector<int> bc(nl + nu, kBlkDefault);
     for (int xcd = 0; xcd < 2; ++xcd)
-      for (int vi = 0; vi < kTriCodes; ++vi) {
+      for (int vi = 0; vi < kBlkCodes; ++vi) {
         const int code = vi | (xcd << 5);
         ilu_apply<T>(ctx, x, y, code);  // warm
         std::fill(acc.begin(), acc.end(), 0.f);
@@ -1627,58 +1649,81 @@ static int autotune(sfem_ctx* ctx, T* x, T* y) {
     L->tune_L[slot].assign(bc.begin(), bc.begin() + nl);
     L->tune_U[slot].assign(bc.begin() + nl, bc.end());
     L->tune_plan[slot] = L->plan_id;
+  } else {
+    L->tuned_csr[slot] = true;
   }
   cleanup();
-  L->tuned[slot] = true;
   return hipGetLastError() == hipSuccess ? SFEM_OK : set_error(SFEM_ERR_HIP, "autotune");
 }
 
-// The current ILU(0) factor in its plan's ordering: row permutation (plan row -> DOF
-// row), pattern and values (FP32 / BF16 factors widened to double).
+// The current ILU(0) factor on the reordered point pattern: plan row (block p, slot a)
+// is point row pbs[p] + a, pbs the running count of free rows in plan order.
 int la_get_ilu(sfem_ctx* ctx, int64_t* nnz_out, int32_t* perm, int32_t* rowptr, int32_t* colidx, double* vals) {
   LinAlg* L = ctx->la;
   if (!L || L->pc_type != SFEM_PC_ILU0 || !L->ilu || L->pc_K_version < 0)
     return set_error(SFEM_ERR_STATE, "no ILU(0) factor has been built");
   const IluPlan& P = *L->ilu;
-  const int64_t n = L->n, nnz = ctx->nnz;
-  if (nnz_out) *nnz_out = nnz;
+  const int64_t nb = P.nblk, nv = P.nvals;
   LA_CHECK(hipStreamSynchronize(ctx->stream));
-  if (perm) {
-    if (P.identity) for (int64_t i = 0; i < n; ++i) perm[i] = (int32_t)i;
-    else LA_CHECK(hipMemcpy(perm, P.d_perm, n * sizeof(int32_t), hipMemcpyDeviceToHost));
-  }
-  if (rowptr) LA_CHECK(hipMemcpy(rowptr, P.rowptr, (n + 1) * sizeof(int32_t), hipMemcpyDeviceToHost));
-  if (colidx) LA_CHECK(hipMemcpy(colidx, P.colidx, nnz * sizeof(int32_t), hipMemcpyDeviceToHost));
+  std::vector<BDesc> bd(nb);
+  std::vector<int32_t> bcol(P.npairs), r0(nb);
+  LA_CHECK(hipMemcpy(bd.data(), P.d_bdesc, nb * sizeof(BDesc), hipMemcpyDeviceToHost));
+  LA_CHECK(hipMemcpy(bcol.data(), P.d_bcol, P.npairs * sizeof(int32_t), hipMemcpyDeviceToHost));
+  LA_CHECK(hipMemcpy(r0.data(), P.d_r0orig, nb * sizeof(int32_t), hipMemcpyDeviceToHost));
+  std::vector<double> lu(vals ? nv : 0);
   if (vals) {
     if (L->pc_bf16) {
-      std::vector<uint16_t> h(nnz);
-      LA_CHECK(hipMemcpy(h.data(), L->lu16, nnz * sizeof(uint16_t), hipMemcpyDeviceToHost));
-      for (int64_t i = 0; i < nnz; ++i) {
+      std::vector<uint16_t> h(nv);
+      LA_CHECK(hipMemcpy(h.data(), L->lu16, nv * sizeof(uint16_t), hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < nv; ++i) {
         const uint32_t u = (uint32_t)h[i] << 16;
         float f;
         std::memcpy(&f, &u, sizeof(f));
-        vals[i] = f;
+        lu[i] = f;
       }
     } else if (L->pc_fp32) {
-      std::vector<float> h(nnz);
-      LA_CHECK(hipMemcpy(h.data(), L->lu32, nnz * sizeof(float), hipMemcpyDeviceToHost));
-      for (int64_t i = 0; i < nnz; ++i) vals[i] = h[i];
+      std::vector<float> h(nv);
+      LA_CHECK(hipMemcpy(h.data(), L->lu32, nv * sizeof(float), hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < nv; ++i) lu[i] = h[i];
     } else {
-      LA_CHECK(hipMemcpy(vals, L->lu, nnz * sizeof(double), hipMemcpyDeviceToHost));
+      LA_CHECK(hipMemcpy(lu.data(), L->lu, nv * sizeof(double), hipMemcpyDeviceToHost));
     }
   }
+  std::vector<int32_t> pbs(nb + 1, 0);
+  for (int64_t p = 0; p < nb; ++p) pbs[p + 1] = pbs[p] + (bd[p].dm >> 16);
+  int64_t q = 0;
+  if (rowptr) rowptr[0] = 0;
+  for (int64_t p = 0; p < nb; ++p) {
+    const BDesc& d = bd[p];
+    const int m = d.dm >> 16;
+    for (int a = 0; a < m; ++a) {
+      const int32_t row = pbs[p] + a;
+      if (perm) perm[row] = r0[p] + a;
+      for (int t = 0; t < d.nb; ++t) {
+        const int32_t j = bcol[d.bptr + t];
+        const int mj = bd[j].dm >> 16;
+        for (int b = 0; b < mj; ++b, ++q) {
+          if (colidx) colidx[q] = pbs[j] + b;
+          if (vals) vals[q] = lu[d.vbase + (int64_t)a * 4 * d.nb + 4 * t + b];
+        }
+      }
+      if (rowptr) rowptr[row + 1] = (int32_t)q;
+    }
+  }
+  if (nnz_out) *nnz_out = q;
   return SFEM_OK;
 }
 
 int la_tuning(sfem_ctx* ctx, int* out) {
   LinAlg* L = ctx->la;
   for (int k = 0; k < 2; ++k) {
-    const bool t = L && L->tuned[k];
-    const int c = t ? L->tune_spmv[k] : 0;
-    out[4 * k + 0] = t ? kSpmvG[c & 31] : 0;
-    out[4 * k + 1] = t ? kSpmvU[c & 31] : 0;
-    out[4 * k + 2] = t ? c >> 5 : 0;
-    out[4 * k + 3] = t ? (int)(L->tune_L[k].size() + L->tune_U[k].size()) : 0;
+    const bool blk = L && L->block_space && L->tune_plan[k] == L->plan_id;
+    const bool csr = L && !L->block_space && L->tuned_csr[k];
+    const int c = blk ? L->tune_bspmv[k] : csr ? L->tune_spmv[k] : 0;
+    out[4 * k + 0] = blk ? kBlkG[c & 31] : csr ? kSpmvG[c & 31] : 0;
+    out[4 * k + 1] = blk ? kBlkK[c & 31] : csr ? kSpmvU[c & 31] : 0;
+    out[4 * k + 2] = (blk || csr) ? c >> 5 : 0;
+    out[4 * k + 3] = blk ? (int)(L->tune_L[k].size() + L->tune_U[k].size()) : 0;
   }
   return SFEM_OK;
 }
@@ -1690,8 +1735,8 @@ int la_tuning_levels(sfem_ctx* ctx, int fp32, int* out, int cap, int* nlevels) {
   all.insert(all.end(), L->tune_U[k].begin(), L->tune_U[k].end());
   *nlevels = (int)all.size();
   for (int i = 0; i < (int)all.size() && i < cap; ++i) {
-    out[3 * i + 0] = kTriG[all[i] & 31];
-    out[3 * i + 1] = kTriU[all[i] & 31];
+    out[3 * i + 0] = kBlkG[all[i] & 31];
+    out[3 * i + 1] = kBlkK[all[i] & 31];
     out[3 * i + 2] = all[i] >> 5;
   }
   return SFEM_OK;
@@ -1731,7 +1776,7 @@ static int bicgstab(sfem_ctx* ctx, const T* b, T* x, double rtol, int maxit, int
                     double* relres, T* r, T* rh, T* p, T* v, T* s, T* t, T* ph, T* sh, int stall = 0) {
   LinAlg* L = ctx->la;
   hipStream_t st = ctx->stream;
-  const int64_t n = L->n;
+  const int64_t n = L->nk;
   const int g = grid_for(n);
   double* sc = L->d_scal;
   double* hs = L->h_scal;
@@ -1832,7 +1877,7 @@ static int gmres(sfem_ctx* ctx, const T* b, T* x, int m, double rtol, int maxit,
                  double* relres, T* V, T* w, T* z) {
   LinAlg* L = ctx->la;
   hipStream_t st = ctx->stream;
-  const int64_t n = L->n;
+  const int64_t n = L->nk;
   const int g = grid_for(n);
   if (!L->d_h) {
     LA_CHECK(alloc(&L->d_h, (size_t)kGmresMaxH));
@@ -1954,107 +1999,103 @@ static int gmres(sfem_ctx* ctx, const T* b, T* x, int m, double rtol, int maxit,
   return SFEM_OK;
 }
 
-// Shapes are tuned on the first solve of a precision and again when an ILU(0) plan
-// appears that they were not tuned for.
+// Shapes are tuned on the first solve of a precision and layout, and again when an
+// ILU(0) plan appears that they were not tuned for.
 static bool needs_tuning(const LinAlg* L, int slot) {
-  if (!L->tuned[slot]) return true;
-  return L->pc_type == SFEM_PC_ILU0 && L->tune_plan[slot] != L->plan_id;
+  return L->block_space ? L->tune_plan[slot] != L->plan_id : !L->tuned_csr[slot];
 }
 
 // Symbolic set-up of the preconditioner for the given options (ILU(0): ordering,
-// permuted pattern and level sets), so that the first solve does not pay for it.
+// block pattern and level sets), so that the first solve does not pay for it.
 int la_prepare(sfem_ctx* ctx, const sfem_solver_opts* o) {
   LinAlg* L = ctx->la;
   if (o->pc_type != SFEM_PC_ILU0) return SFEM_OK;
-  if (L->maxrow > kIluMaxRow) return set_error(SFEM_ERR_UNSUPPORTED, "ILU(0): row longer than 256 entries");
   if (L->ilu && L->ilu->reorder == o->reorder) return SFEM_OK;
   return build_ilu_plan(ctx, L, o->reorder);
 }
 
-// Solve A du = rhs.  FP64 mode: everything in double.  Mixed mode: FP32
-// matrix and Krylov vectors; the FP64 rhs is scaled by 1/||rhs|| into FP32 and
-// the FP32 solution scaled back into the FP64 increment.
+// Solve A du = rhs.  FP64 mode: everything in double.  Mixed mode: FP32 matrix and
+// Krylov vectors; the FP64 rhs is scaled by 1/||rhs|| into FP32 and the FP32 solution
+// scaled back into the FP64 increment.  With ILU(0) the Krylov solve runs in the plan's
+// padded block layout (rhs in, solution out through the plan's slot map).
 int la_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int* iters, double* relres) {
   LinAlg* L = ctx->la;
   const int64_t n = L->n;
   const bool fp32 = ctx->precision == SFEM_PREC_MIXED;
+  hipStream_t st = ctx->stream;
   int rc;
   // the tangent is constant within a time step: rebuild the preconditioner only when it changed
   const int reorder = o->pc_type == SFEM_PC_ILU0 ? o->reorder : 0;
   const bool bf16f = fp32 && o->pc_type == SFEM_PC_ILU0 && o->factor_bf16 != 0;
   if (L->pc_K_version != ctx->la_K_version || L->pc_type != o->pc_type || L->pc_fp32 != fp32 ||
       L->pc_reorder != reorder || L->pc_bf16 != bf16f) {
+    L->pc_K_version = -1;
     if ((rc = la_setup_pc(ctx, o->pc_type, fp32, reorder, bf16f))) return rc;
     L->pc_reorder = reorder;
     L->pc_K_version = ctx->la_K_version;
   }
-  // a reordered ILU(0) plan moves the whole Krylov solve into its ordering
-  L->perm_space = L->pc_type == SFEM_PC_ILU0 && L->ilu && !L->ilu->identity;
-  const int32_t* perm = L->perm_space ? L->ilu->d_perm : nullptr;
-  const int gn = grid_for(n);
+  L->block_space = L->pc_type == SFEM_PC_ILU0;
+  const int32_t* padmap = L->block_space ? L->ilu->d_padmap : nullptr;
+  const int64_t nk = L->block_space ? L->ilu->npad : n;
+  L->nk = nk;
+  const int gk = grid_for(nk), gn = grid_for(n);
   const int m = std::max(1, o->restart);
   if (!fp32) {
     const double* b = ctx->d_rhs;
     double* x = ctx->d_du;
-    if (perm) {
-      if ((rc = ensure_vec(&L->pr, n)) || (rc = ensure_vec(&L->pz, n))) return rc;
-      hipLaunchKernelGGL((k_permute_in<double, double>), dim3(gn), dim3(kTPB), 0, ctx->stream, n, perm,
-                         ctx->d_rhs, L->pr, 1.0);
+    if (padmap) {
+      if ((rc = ensure_vec(&L->pr, L->nvec)) || (rc = ensure_vec(&L->pz, L->nvec))) return rc;
+      hipLaunchKernelGGL((k_pad_in<double, double>), dim3(gk), dim3(kTPB), 0, st, nk, padmap, ctx->d_rhs, L->pr, 1.0);
       b = L->pr;
       x = L->pz;
     }
     if (o->ksp_type == SFEM_KSP_GMRES) {
-      if ((rc = ensure_vec(&L->w, n)) || (rc = ensure_vec(&L->z, n))) return rc;
+      if ((rc = ensure_vec(&L->w, L->nvec)) || (rc = ensure_vec(&L->z, L->nvec))) return rc;
       if (!L->gm_V || L->gm_m < m) {
         if (L->gm_V) (void)hipFree(L->gm_V);
         L->gm_V = nullptr;
-        if ((rc = ensure_vec(&L->gm_V, (int64_t)(m + 1) * n))) return rc;
+        if ((rc = ensure_vec(&L->gm_V, (int64_t)(m + 1) * L->nvec))) return rc;
         L->gm_m = m;
       }
       rc = gmres<double>(ctx, b, x, m, o->rtol, o->max_it, iters, relres, L->gm_V, L->w, L->z);
     } else {
       double** vs[] = {&L->r, &L->rh, &L->p, &L->v, &L->s, &L->t, &L->ph, &L->sh};
       for (double** q : vs)
-        if ((rc = ensure_vec(q, n))) return rc;
+        if ((rc = ensure_vec(q, L->nvec))) return rc;
       if (needs_tuning(L, 0) && (rc = autotune<double>(ctx, L->t, L->v))) return rc;
       rc = bicgstab<double>(ctx, b, x, o->rtol, o->max_it, iters, relres, L->r, L->rh, L->p, L->v, L->s,
                             L->t, L->ph, L->sh);
     }
-    if (perm) {
-      hipLaunchKernelGGL((k_permute_out<double, double>), dim3(gn), dim3(kTPB), 0, ctx->stream, n, perm, x,
-                         ctx->d_du, 1.0);
+    if (padmap) {
+      hipLaunchKernelGGL((k_pad_out<double, double>), dim3(gk), dim3(kTPB), 0, st, nk, padmap, x, ctx->d_du, 1.0);
       if (hipGetLastError() != hipSuccess) return set_error(SFEM_ERR_HIP, "permute launch");
     }
     return rc;
   }
   // mixed precision
-  float** vs[] = {&L->r32, &L->rh32, &L->p32, &L->v32, &L->s32, &L->t32, &L->ph32, &L->sh32, &L->x32};
+  float** vs[] = {&L->r32, &L->rh32, &L->p32, &L->v32, &L->s32, &L->t32, &L->ph32, &L->sh32, &L->x32, &L->du32};
   for (float** q : vs)
-    if ((rc = ensure_vec(q, n))) return rc;
+    if ((rc = ensure_vec(q, L->nvec))) return rc;
   if (needs_tuning(L, 1) && (rc = autotune<float>(ctx, L->t32, L->v32))) return rc;
   double bn;
   if ((rc = la_norm2(ctx, ctx->d_rhs, &bn))) return rc;
-  const int g = grid_for(n);
   if (!(bn > 0.0)) {
-    hipLaunchKernelGGL((k_fill<double>), dim3(g), dim3(kTPB), 0, ctx->stream, n, ctx->d_du, 0.0);
+    hipLaunchKernelGGL((k_fill<double>), dim3(gn), dim3(kTPB), 0, st, n, ctx->d_du, 0.0);
     *iters = 0; *relres = 0.0;
     return SFEM_OK;
   }
-  // b32 = rhs / ||rhs|| into the FP32 "s" slot's sibling (rh32 is overwritten by the solver, use x32 as b)
+  // b32 = rhs / ||rhs|| (x32 holds it; the solver overwrites its other vectors)
   float* b32 = L->x32;
-  if (perm)
-    hipLaunchKernelGGL((k_permute_in<double, float>), dim3(g), dim3(kTPB), 0, ctx->stream, n, perm, ctx->d_rhs,
-                       b32, 1.0 / bn);
+  if (padmap)
+    hipLaunchKernelGGL((k_pad_in<double, float>), dim3(gk), dim3(kTPB), 0, st, nk, padmap, ctx->d_rhs, b32, 1.0 / bn);
   else
-    hipLaunchKernelGGL((k_convert<double, float>), dim3(g), dim3(kTPB), 0, ctx->stream, n, ctx->d_rhs, b32,
-                       1.0 / bn);
-  if (!L->du32) { if ((rc = ensure_vec(&L->du32, n))) return rc; }
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(gn), dim3(kTPB), 0, st, n, ctx->d_rhs, b32, 1.0 / bn);
   if (o->ksp_type == SFEM_KSP_GMRES) {
-    if ((rc = ensure_vec(&L->w32, n)) || (rc = ensure_vec(&L->z32, n))) return rc;
+    if ((rc = ensure_vec(&L->w32, L->nvec)) || (rc = ensure_vec(&L->z32, L->nvec))) return rc;
     if (!L->gm_V32 || L->gm_m32 < m) {
       if (L->gm_V32) (void)hipFree(L->gm_V32);
       L->gm_V32 = nullptr;
-      if ((rc = ensure_vec(&L->gm_V32, (int64_t)(m + 1) * n))) return rc;
+      if ((rc = ensure_vec(&L->gm_V32, (int64_t)(m + 1) * L->nvec))) return rc;
       L->gm_m32 = m;
     }
     rc = gmres<float>(ctx, b32, L->du32, m, o->rtol, o->max_it, iters, relres, L->gm_V32, L->w32, L->z32);
@@ -2067,11 +2108,11 @@ int la_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int* iters, double* relre
     if (rc == kStagnated) {
       const int used = *iters;
       const int mg = std::max(1, std::min(o->restart > 0 ? o->restart : 30, 64));
-      if ((rc = ensure_vec(&L->w32, n)) || (rc = ensure_vec(&L->z32, n))) return rc;
+      if ((rc = ensure_vec(&L->w32, L->nvec)) || (rc = ensure_vec(&L->z32, L->nvec))) return rc;
       if (!L->gm_V32 || L->gm_m32 < mg) {
         if (L->gm_V32) (void)hipFree(L->gm_V32);
         L->gm_V32 = nullptr;
-        if ((rc = ensure_vec(&L->gm_V32, (int64_t)(mg + 1) * n))) return rc;
+        if ((rc = ensure_vec(&L->gm_V32, (int64_t)(mg + 1) * L->nvec))) return rc;
         L->gm_m32 = mg;
       }
       ++L->fallbacks;
@@ -2080,11 +2121,10 @@ int la_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int* iters, double* relre
       *iters += used;
     }
   }
-  if (perm)
-    hipLaunchKernelGGL((k_permute_out<float, double>), dim3(g), dim3(kTPB), 0, ctx->stream, n, perm, L->du32,
-                       ctx->d_du, bn);
+  if (padmap)
+    hipLaunchKernelGGL((k_pad_out<float, double>), dim3(gk), dim3(kTPB), 0, st, nk, padmap, L->du32, ctx->d_du, bn);
   else
-    hipLaunchKernelGGL((k_convert<float, double>), dim3(g), dim3(kTPB), 0, ctx->stream, n, L->du32, ctx->d_du, bn);
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(gn), dim3(kTPB), 0, st, n, L->du32, ctx->d_du, bn);
   if (hipGetLastError() != hipSuccess) return set_error(SFEM_ERR_HIP, "convert launch");
   return rc;
 }

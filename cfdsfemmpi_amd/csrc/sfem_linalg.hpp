@@ -15,25 +15,26 @@ namespace sfem {
 int set_error(int code, const char* msg);
 int set_hip_error(hipError_t e, const char* what);
 
-// ILU(0) plan for one node ordering: the (possibly permuted) pattern the factor lives
-// on and its level sets.  With the identity ordering the pattern pointers alias the
-// context's CSR.
+// ILU(0) plan for one node ordering, in node-block form: block rows in the new order
+// with their neighbour blocks, the gather offsets into the DOF-ordered CSR, level sets,
+// and the map from the padded block layout of vectors (4 slots per block) to DOFs.
 struct BlkDesc;
+struct BDesc;
 struct bf16;
 
 struct IluPlan {
   int reorder = -1;
-  bool identity = true;
   bool contiguous = false;   // level l is the block range [levPtr[l], levPtr[l+1])
-  BlkDesc* d_desc = nullptr;
+  int64_t nblk = 0, npad = 0, nvals = 0, npairs = 0;
+  int nbmax = 0;
   std::vector<int32_t> levLPtr, levUPtr, h_levL, h_levU;
   int32_t *d_levL = nullptr, *d_levU = nullptr;
-  const int32_t *rowptr = nullptr, *colidx = nullptr, *diagpos = nullptr, *blkStart = nullptr;
-  int32_t *d_rowptr = nullptr, *d_colidx = nullptr, *d_diagpos = nullptr, *d_blkStart = nullptr;
-  int32_t* d_pmap = nullptr;   // permuted nonzero -> original nonzero
-  int32_t* d_perm = nullptr;   // permuted row -> original row
-  int32_t* d_rowBlk = nullptr; // row -> node block, in the plan's ordering
-  const int32_t* rowBlk = nullptr;
+  BDesc* d_bdesc = nullptr;
+  int32_t* d_bcol = nullptr;     // neighbour blocks (new positions), per block row
+  int32_t* d_bcolofs = nullptr;  // their first column's offset in the DOF-ordered CSR row
+  int32_t* d_r0orig = nullptr;   // first DOF row of each block row
+  int32_t* d_padmap = nullptr;   // padded slot -> DOF row, -1 for padding
+  std::vector<int32_t> h_order;  // new position -> original block
   ~IluPlan();
 };
 
@@ -41,10 +42,14 @@ struct LinAlg {
   int64_t n = 0;             // free DOFs
   int64_t nblk = 0;          // nodal blocks (nodes with >= 1 free DOF)
   int maxrow = 0;
-  // autotuned kernel variants, [0] FP64 and [1] FP32
-  int tune_spmv[2] = {3, 3}, tune_best_spmv[2] = {3, 3};
-  std::vector<int> tune_L[2], tune_U[2];   // per level of the forward / backward solve
-  bool tuned[2] = {false, false};
+  int64_t nk = 0;             // length of the Krylov vectors of the current solve
+  int64_t nvec = 0;           // allocated length of every Krylov vector
+  bool block_space = false;   // the current solve runs in the ILU(0) plan's block layout
+  // autotuned kernel variants, [0] FP64 and [1] FP32: CSR SpMV, block SpMV, and the
+  // block triangular solves per level of the forward / backward sweep
+  int tune_spmv[2] = {3, 3}, tune_bspmv[2] = {6, 6};
+  std::vector<int> tune_L[2], tune_U[2];
+  bool tuned_csr[2] = {false, false};
   int64_t fallbacks = 0;     // FP32 BiCGStab solves finished by the GMRES fallback
   int plan_id = 0, tune_plan[2] = {-1, -1};  // ILU plan the per-level shapes were tuned for
   int pc_type = SFEM_PC_NONE;
@@ -57,7 +62,6 @@ struct LinAlg {
   IluPlan* ilu = nullptr;
   int32_t *d_blkStart = nullptr, *d_diagpos = nullptr;
   BlkDesc* d_desc = nullptr;   // per node block, original order
-  int32_t* d_rowBlk = nullptr; // row -> node block, original order
   // every lower neighbour's rows appear as one contiguous group of its full size in each
   // row (true of element patterns); enables the node-grouped factorisation
   bool full_blocks = false;
@@ -84,9 +88,7 @@ struct LinAlg {
   float* binv32 = nullptr;
   double* lu = nullptr;
   float* lu32 = nullptr;
-  // With a reordered ILU(0) plan the Krylov solve runs in the plan's ordering: the
-  // tangent is kept permuted as well (kp / kp32) and only rhs and solution are permuted.
-  bool perm_space = false;
+  // the tangent in the ILU(0) plan's block form, for the products of the Krylov solve
   double* kp = nullptr;
   float* kp32 = nullptr;
   ~LinAlg();

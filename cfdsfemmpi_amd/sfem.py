@@ -334,6 +334,7 @@ class StabFEM:
         self.ctx: Optional[Context] = None
         self.opts = solver_opts()
         self.newton_tol = 1.0e-8   # hard-coded absolute tolerance (StabFEM2.cpp:749)
+        self.forces = []           # (timeNow, Fx, Fy) per step, the forces file's lines
         self.max_newton = 10       # hard-coded iteration cap (StabFEM2.cpp:654)
         self.precision = PREC_FP64
         self.history = []          # (tstep, iter, rhs_norm, krylov_its)
@@ -390,10 +391,24 @@ class StabFEM:
                 its, _ = ctx.solve(self.opts)
                 self.history.append((tstep, it, nrm, its))
                 ctx.update_solution()
+            # StabFEM2.cpp:808-815: the drag/lift line of the forces file, summed from the
+            # reaction vector at the output nodes; the reference zeroes that vector every
+            # iteration (StabFEM2.cpp:660) and never adds to it, so both sums are 0
+            ndof = self.mesh.ndim + 1
+            reacVec = np.zeros(self.mesh.nNode * ndof)
+            out = np.asarray(self.mesh.out_nodes, np.int64)
+            self.forces.append((timeNow, float(reacVec[out * ndof].sum()), float(reacVec[out * ndof + 1].sum())))
             timeNow = timeNow + c.dt
             if timeNow > c.timeFinal:
                 break
         return ctx.get_solution()
+
+    # the "<mesh>-forces.dat" file opened in StabFEM::readInputData (StabFEM.cpp:249-259)
+    # and written once per step (StabFEM2.cpp:815): fixed notation, 14 decimals
+    def writeForces(self, fname: str):
+        with open(fname, "w") as f:
+            for t, fx, fy in self.forces:
+                f.write(f"{t:.14f}\t{fx:.14f}\t{fy:.14f}\n")
 
     def getSolution(self) -> np.ndarray:
         return self.ctx.get_solution()

@@ -118,3 +118,14 @@ def test_generators_geometry():
     area = 0.5 * ((xx[:, 1, 0] - xx[:, 0, 0]) * (xx[:, 2, 1] - xx[:, 0, 1]) -
                   (xx[:, 2, 0] - xx[:, 0, 0]) * (xx[:, 1, 1] - xx[:, 0, 1]))
     assert np.all(area > 0) and np.isclose(area.sum(), 1.0)
+
+
+def test_forces_file_format(tmp_path):
+    """The forces file: one 'timeNow Fx Fy' line per step in fixed notation with 14
+    decimals (StabFEM.cpp:257-259, StabFEM2.cpp:815)."""
+    fem = sfem.StabFEM()
+    fem.forces = [(0.1, 0.0, 0.0), (0.2, 0.0, 0.0)]
+    fn = tmp_path / "m-forces.dat"
+    fem.writeForces(str(fn))
+    assert fn.read_text() == "0.10000000000000\t0.00000000000000\t0.00000000000000\n" \
+                             "0.20000000000000\t0.00000000000000\t0.00000000000000\n"
