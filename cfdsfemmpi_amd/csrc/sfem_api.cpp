@@ -741,6 +741,11 @@ int sfem_get_ilu(sfem_ctx* c, int64_t* nnz, int32_t* perm, int32_t* rowptr, int3
   return la_get_ilu(c, nnz, perm, rowptr, colidx, vals);
 }
 
+int sfem_block_info(sfem_ctx* c, int64_t* nblk, int64_t* npairs, int64_t* nvals) {
+  if (!c) return set_error(SFEM_ERR_ARG, "null context");
+  return la_block_info(c, nblk, npairs, nvals);
+}
+
 int sfem_solver_stats(sfem_ctx* c, int64_t* fp32_fallbacks) {
   if (!c || !fp32_fallbacks) return set_error(SFEM_ERR_ARG, "null argument");
   *fp32_fallbacks = c->la ? c->la->fallbacks : 0;

@@ -578,24 +578,30 @@ __device__ inline void bsr_dot(const int32_t* __restrict__ bc, const VT* __restr
   }
 }
 
-// y = K x in block form; G lanes per block row, padding slots of y written as 0.
+// y = K x in block form; G lanes per block row, padding slots of y written as 0.  With
+// rows non-null the g-th group takes block row rows[g]: visiting the rows in the
+// original (spatial) node order keeps the x entries a stretch of rows gathers in L2,
+// where the multicolour order would sweep all of x once per colour.
 template <typename VT, typename XT, int G, int K>
 __global__ __launch_bounds__(256) void k_bsr_spmv(int64_t nblk, const BDesc* __restrict__ desc,
                                                   const int32_t* __restrict__ bcol, const VT* __restrict__ V,
-                                                  const XT* __restrict__ x, XT* __restrict__ y, int xcd) {
+                                                  const XT* __restrict__ x, XT* __restrict__ y, int xcd,
+                                                  const int32_t* __restrict__ rows) {
   using AT = typename std::conditional<sizeof(VT) == 8, double, float>::type;
   const int64_t gt = logical_wg(xcd) * 256 + threadIdx.x;
-  const int64_t p = gt / G;
+  const int64_t g = gt / G;
   const int lane = (int)(gt % G);
   AT acc[4] = {0, 0, 0, 0};
   BDesc d = {0, 0, 0, 0};
-  if (p < nblk) {
+  int64_t p = g;
+  if (g < nblk) {
+    if (rows) p = rows[g];
     d = desc[p];
     bsr_dot<G, K>(bcol + d.bptr, V + d.vbase, 4 * d.nb, x, d.dm >> 16, 0, d.nb, lane, acc);
   }
 #pragma unroll
   for (int a = 0; a < 4; ++a) acc[a] = group_sum<G>(acc[a]);
-  if (p < nblk && lane < 4) y[4 * p + lane] = lane < (d.dm >> 16) ? (XT)pick4(acc, lane) : (XT)0;
+  if (g < nblk && lane < 4) y[4 * p + lane] = lane < (d.dm >> 16) ? (XT)pick4(acc, lane) : (XT)0;
 }
 
 template <typename LT> struct AccOf { using type = float; };
@@ -1133,7 +1139,7 @@ static std::vector<BlkDesc> make_desc(const std::vector<int32_t>& bs, const std:
 }
 
 IluPlan::~IluPlan() {
-  void* ptrs[] = {d_levL, d_levU, d_bdesc, d_bcol, d_bcolofs, d_r0orig, d_padmap};
+  void* ptrs[] = {d_levL, d_levU, d_bdesc, d_bcol, d_bcolofs, d_r0orig, d_padmap, d_natrows};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
 }
@@ -1286,6 +1292,7 @@ static int build_ilu_plan(sfem_ctx* ctx, LinAlg* L, int reorder) {
   if ((rc = upload(&P.d_bcol, bcol)) || (rc = upload(&P.d_bcolofs, bcolofs)) || (rc = upload(&P.d_r0orig, r0orig)) ||
       (rc = upload(&P.d_padmap, padmap)) || (rc = upload(&P.d_levL, P.h_levL)) || (rc = upload(&P.d_levU, P.h_levU)))
     return rc;
+  if (reorder != 0 && (rc = upload(&P.d_natrows, iord))) return rc;  // block rows in original order
   return SFEM_OK;
 }
 
@@ -1536,21 +1543,23 @@ static void spmv_launch(hipStream_t st, int code, int64_t nb, const BlkDesc* des
 #undef SFEM_SPMV
 }
 
+// Block SpMV shapes: code = vi | (xcd << 5) | (spatial row order << 6).
 template <typename VT, typename T>
 static void bsr_spmv_launch(hipStream_t st, int code, const IluPlan& P, const VT* vals, const T* x, T* y) {
-  const int vi = code & 31, xcd = code >> 5;
+  const int vi = code & 31, xcd = (code >> 5) & 1;
+  const int32_t* rows = (code >> 6) & 1 ? P.d_natrows : nullptr;
   const unsigned g = (unsigned)((P.nblk * kBlkG[vi] + 255) / 256);
 #define SFEM_BSPMV(i, G, K)                                                                                  \
   case i:                                                                                                    \
     hipLaunchKernelGGL((k_bsr_spmv<VT, T, G, K>), dim3(g), dim3(256), 0, st, P.nblk, P.d_bdesc, P.d_bcol, vals, \
-                       x, y, xcd);                                                                           \
+                       x, y, xcd, rows);                                                                     \
     break;
   switch (vi) {
     SFEM_BSPMV(0, 4, 1) SFEM_BSPMV(1, 4, 2) SFEM_BSPMV(2, 8, 1) SFEM_BSPMV(3, 8, 2)
     SFEM_BSPMV(4, 16, 1) SFEM_BSPMV(5, 16, 2) SFEM_BSPMV(6, 32, 1)
     default:
       hipLaunchKernelGGL((k_bsr_spmv<VT, T, 64, 1>), dim3(g), dim3(256), 0, st, P.nblk, P.d_bdesc, P.d_bcol, vals,
-                         x, y, xcd);
+                         x, y, xcd, rows);
       break;
   }
 #undef SFEM_BSPMV
@@ -1607,9 +1616,11 @@ static int autotune(sfem_ctx* ctx, T* x, T* y) {
   const int ncodes = block ? kBlkCodes : kSpmvCodes;
   int* cur = block ? &L->tune_bspmv[slot] : &L->tune_spmv[slot];
   int bestc = *cur;
+  const int norders = block && L->ilu->d_natrows ? 2 : 1;
+  for (int ord = 0; ord < norders; ++ord)
   for (int xcd = 0; xcd < 2; ++xcd)
     for (int vi = 0; vi < ncodes; ++vi) {
-      *cur = vi | (xcd << 5);
+      *cur = vi | (xcd << 5) | (ord << 6);
       spmv<T>(ctx, x, y);  // warm
       (void)hipEventRecord(ev[0], st);
       for (int k = 0; k < 3; ++k) spmv<T>(ctx, x, y);
@@ -1714,6 +1725,15 @@ int la_get_ilu(sfem_ctx* ctx, int64_t* nnz_out, int32_t* perm, int32_t* rowptr, 
   return SFEM_OK;
 }
 
+int la_block_info(sfem_ctx* ctx, int64_t* nblk, int64_t* npairs, int64_t* nvals) {
+  const LinAlg* L = ctx->la;
+  const IluPlan* P = L ? L->ilu : nullptr;
+  if (nblk) *nblk = P ? P->nblk : 0;
+  if (npairs) *npairs = P ? P->npairs : 0;
+  if (nvals) *nvals = P ? P->nvals : 0;
+  return SFEM_OK;
+}
+
 int la_tuning(sfem_ctx* ctx, int* out) {
   LinAlg* L = ctx->la;
   for (int k = 0; k < 2; ++k) {
@@ -1722,7 +1742,7 @@ int la_tuning(sfem_ctx* ctx, int* out) {
     const int c = blk ? L->tune_bspmv[k] : csr ? L->tune_spmv[k] : 0;
     out[4 * k + 0] = blk ? kBlkG[c & 31] : csr ? kSpmvG[c & 31] : 0;
     out[4 * k + 1] = blk ? kBlkK[c & 31] : csr ? kSpmvU[c & 31] : 0;
-    out[4 * k + 2] = (blk || csr) ? c >> 5 : 0;
+    out[4 * k + 2] = (blk || csr) ? c >> 5 : 0;  // bit 0: XCD order, bit 1: spatial row order
     out[4 * k + 3] = blk ? (int)(L->tune_L[k].size() + L->tune_U[k].size()) : 0;
   }
   return SFEM_OK;

@@ -34,6 +34,7 @@ struct IluPlan {
   int32_t* d_bcolofs = nullptr;  // their first column's offset in the DOF-ordered CSR row
   int32_t* d_r0orig = nullptr;   // first DOF row of each block row
   int32_t* d_padmap = nullptr;   // padded slot -> DOF row, -1 for padding
+  int32_t* d_natrows = nullptr;  // block rows in the original node order (reordered plans)
   std::vector<int32_t> h_order;  // new position -> original block
   ~IluPlan();
 };
@@ -100,6 +101,7 @@ int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f);
 int la_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int* iters, double* relres);
 int la_prepare(sfem_ctx* ctx, const sfem_solver_opts* o);
 int la_tuning(sfem_ctx* ctx, int* out);
+int la_block_info(sfem_ctx* ctx, int64_t* nblk, int64_t* npairs, int64_t* nvals);
 int la_get_ilu(sfem_ctx* ctx, int64_t* nnz_out, int32_t* perm, int32_t* rowptr, int32_t* colidx, double* vals);
 int la_tuning_levels(sfem_ctx* ctx, int fp32, int* out, int cap, int* nlevels);
 int la_norm2(sfem_ctx* ctx, const double* a, double* out);

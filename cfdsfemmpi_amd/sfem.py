@@ -78,6 +78,7 @@ ABI = [
     ("sfem_kernel_timing", C.c_int, [_vp, C.c_int, _i64p, _f64p]),
     ("sfem_get_ilu", C.c_int, [_vp, _i64p, _i32p, _i32p, _i32p, _f64p]),
     ("sfem_solver_stats", C.c_int, [_vp, _i64p]),
+    ("sfem_block_info", C.c_int, [_vp, _i64p, _i64p, _i64p]),
     ("sfem_tuning", C.c_int, [_vp, C.POINTER(C.c_int)]),
     ("sfem_tuning_levels", C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]),
     ("sfem_info", C.c_int, [_vp, _i64p, _i64p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
@@ -264,6 +265,12 @@ class Context:
         n, ms = C.c_int64(0), C.c_double(0.0)
         _check(self.lib.sfem_kernel_timing(self.h, int(kind), C.byref(n), C.byref(ms)), "sfem_kernel_timing")
         return n.value, ms.value
+
+    def block_info(self):
+        """Sizes of the ILU(0) node-block form: block rows, neighbour blocks, values."""
+        a, b, c = C.c_int64(0), C.c_int64(0), C.c_int64(0)
+        _check(self.lib.sfem_block_info(self.h, C.byref(a), C.byref(b), C.byref(c)), "sfem_block_info")
+        return dict(nblk=a.value, npairs=b.value, nvals=c.value)
 
     def fp32_fallbacks(self) -> int:
         """Mixed-precision solves finished by the FP32 GMRES fallback."""

@@ -191,15 +191,21 @@ int sfem_kernel_timing(sfem_ctx* ctx, int kind, int64_t* launches, double* total
  * (SolverPetsc.cpp:234-277) and is not exposed; this exists for parity checks. */
 int sfem_get_ilu(sfem_ctx* ctx, int64_t* nnz, int32_t* perm, int32_t* rowptr, int32_t* colidx, double* vals);
 
+/* Sizes of the ILU(0) node-block form (0 before an ILU(0) solver is set up): block
+ * rows, stored neighbour blocks, and stored values (padded 4-wide blocks, m rows each).
+ * No counterpart in the reference. */
+int sfem_block_info(sfem_ctx* ctx, int64_t* nblk, int64_t* npairs, int64_t* nvals);
+
 /* Number of mixed-precision solves in which the FP32 BiCGStab stalled (no 10% gain of
  * its best residual over 100 iterations) and the solve was finished by FP32 GMRES from
  * a zero start.  No counterpart in the reference (PETSc reports a diverged KSP). */
 int sfem_solver_stats(sfem_ctx* ctx, int64_t* fp32_fallbacks);
 
-/* Kernel shapes chosen by the solver's autotuner, as out[8] = {SpMV lanes per node G,
- * columns in flight per lane U, XCD-contiguous workgroup order (0/1), number of tuned
- * ILU(0) levels} for FP64 then FP32; 0 where not tuned yet.  No counterpart in the
- * reference. */
+/* Kernel shapes chosen by the solver's autotuner, as out[8] = {SpMV lanes per node
+ * (block row) G, columns (neighbour blocks) in flight per lane, order flags (bit 0:
+ * XCD-contiguous workgroups, bit 1: block rows visited in the original node order),
+ * number of tuned ILU(0) levels} for FP64 then FP32; 0 where not tuned yet.  No
+ * counterpart in the reference. */
 int sfem_tuning(sfem_ctx* ctx, int* out);
 
 /* Per-level shapes of the ILU(0) apply for one precision (fp32 = 0 FP64, 1 FP32):
