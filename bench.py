@@ -56,7 +56,7 @@ def make_case(n):
     return mesh, ctl
 
 
-def run_path(mesh, ctl, prec, opts, newton_tol, steps, warmup, label, predictor=0):
+def run_path(mesh, ctl, prec, opts, newton_tol, steps, warmup, label, predictor=0, pc_lag=1):
     from cfdsfemmpi_amd import sfem
     t0 = time.time()
     fem = sfem.StabFEM(device=0)
@@ -67,6 +67,7 @@ def run_path(mesh, ctl, prec, opts, newton_tol, steps, warmup, label, predictor=
     fem.setSolver(opts)
     ctx = fem.ctx
     ctx.set_predictor(predictor)
+    ctx.set_pc_lag(pc_lag)
     info = ctx.info()
     log(f"{label}: setup {time.time() - t0:.1f} s, nfree={info['nfree']} nnz={info['nnz']} colours={info['ncolors']}")
 
@@ -178,6 +179,8 @@ def main():
     ap.add_argument("--bf16-factors", type=int, default=0, help="mixed path: ILU(0) factors stored in bfloat16")
     ap.add_argument("--predictor", choices=["previous", "extrapolate"], default="previous",
                     help="Newton initial iterate of each step (both paths)")
+    ap.add_argument("--pc-lag-mixed", type=int, default=1,
+                    help="mixed path: rebuild the ILU(0) factor on every n-th new tangent only")
     ap.add_argument("--ksp", choices=["bcgs", "gmres"], default="bcgs", help="Krylov method of both paths")
     ap.add_argument("--restart", type=int, default=30, help="GMRES restart length")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -204,7 +207,8 @@ def main():
         r64 = run_path(mesh, ctl, sfem.PREC_FP64, o64, a.newton_tol, a.steps, a.warmup, "fp64", pred)
         log(f"fp64: {r64['steps_per_s']:.3f} steps/s")
     if a.only != "fp64":
-        rmx = run_path(mesh, ctl, sfem.PREC_MIXED, omx, a.newton_tol_mixed, a.steps, a.warmup, "mixed", pred)
+        rmx = run_path(mesh, ctl, sfem.PREC_MIXED, omx, a.newton_tol_mixed, a.steps, a.warmup, "mixed", pred,
+                       a.pc_lag_mixed)
         log(f"mixed: {rmx['steps_per_s']:.3f} steps/s")
     main_res = rmx or r64
     err_u = err_p = None
@@ -278,7 +282,7 @@ def main():
                    "solver": ("BiCGStab" if a.ksp == "bcgs" else f"GMRES({a.restart})") + " + ILU(0)", "ilu_ordering": ["natural", "rcm", "multicolour"][a.reorder],
                    "newton_tol": a.newton_tol, "newton_tol_mixed": a.newton_tol_mixed,
                    "inner_rtol_mixed": a.inner_rtol, "ilu_factors_mixed": "bf16" if a.bf16_factors else "fp32",
-                   "newton_initial_iterate": a.predictor, "parallelism": "single GPU"},
+                   "newton_initial_iterate": a.predictor, "pc_lag_mixed": a.pc_lag_mixed, "parallelism": "single GPU"},
         "fp64_steps_per_s": r64["steps_per_s"] if r64 else None,
         "mixed_steps_per_s": rmx["steps_per_s"] if rmx else None,
         "speedup_mixed_vs_fp64": (rmx["steps_per_s"] / r64["steps_per_s"]) if (r64 and rmx) else None,

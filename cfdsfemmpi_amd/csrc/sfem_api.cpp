@@ -488,6 +488,15 @@ int sfem_set_predictor(sfem_ctx* c, int predictor) {
   return SFEM_OK;
 }
 
+int sfem_set_pc_lag(sfem_ctx* c, int lag) {
+  if (!c) return set_error(SFEM_ERR_ARG, "null context");
+  if (lag < 1) return set_error(SFEM_ERR_ARG, "pc lag must be >= 1");
+  if (!c->la) return set_error(SFEM_ERR_STATE, "build the pattern first");
+  c->la->pc_lag = lag;
+  c->la->lag_count = 0;
+  return SFEM_OK;
+}
+
 int sfem_time_update(sfem_ctx* c, double timeFact) {
   if (!c) return set_error(SFEM_ERR_ARG, "null context");
   if (!(c->status & ST_PARAMS)) return set_error(SFEM_ERR_STATE, "parameters not set");

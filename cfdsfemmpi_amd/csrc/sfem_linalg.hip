@@ -130,7 +130,7 @@ __global__ __launch_bounds__(kTPB) void k_mupdate(int64_t n, int nv, const T* __
 }
 
 // Finalisation ops: combine summed partials into Krylov scalars.
-enum { FIN_STORE = 0, FIN_RHO1, FIN_ALPHA, FIN_SS, FIN_OMEGA, FIN_RR };
+enum { FIN_STORE = 0, FIN_RHO1, FIN_ALPHA, FIN_SS, FIN_OMEGA, FIN_RR, FIN_RR_RHO };
 
 template <int ND>
 __global__ __launch_bounds__(kTPB) void k_finalize(const double* __restrict__ part, int nb,
@@ -191,6 +191,19 @@ __global__ __launch_bounds__(kTPB) void k_finalize(const double* __restrict__ pa
       scal[S_RR] = r[0];
       scal[S_RHO0] = scal[S_RHO1];
       break;
+    case FIN_RR_RHO: {  // r[0] = (r, r); r[1] = (rh, r) = the next iteration's rho1, and its beta
+      scal[S_RR] = r[0];
+      const double rho0 = scal[S_RHO1], om = scal[S_OMEGA], rho1 = ND > 1 ? r[1] : 0.0;
+      scal[S_RHO0] = rho0;
+      scal[S_RHO1] = rho1;
+      if (!(fabs(rho1) > 1e-12 * sqrt(scal[S_RHRH] * r[0])) || rho0 == 0.0 || om == 0.0) {
+        if (scal[S_FLAG] != 1.0) scal[S_FLAG] = 3.0;
+        scal[S_BETA] = 0.0;
+      } else {
+        scal[S_BETA] = (rho1 / rho0) * (scal[S_ALPHA] / om);
+      }
+      break;
+    }
   }
 }
 
@@ -241,21 +254,22 @@ __global__ __launch_bounds__(kTPB) void k_bicg_s(int64_t n, const T* __restrict_
   block_sum_store<1>(acc, part);
 }
 
-// x += alpha ph + omega sh ; r = s - omega t ; partial (r, r)
+// x += alpha ph + omega sh ; r = s - omega t ; partials (r, r) and (rh, r)
 template <typename T>
 __global__ __launch_bounds__(kTPB) void k_bicg_xr(int64_t n, T* __restrict__ x, const T* __restrict__ ph,
                                                   const T* __restrict__ sh, T* __restrict__ r,
                                                   const T* __restrict__ s, const T* __restrict__ t,
-                                                  const double* scal, double* part) {
+                                                  const T* __restrict__ rh, const double* scal, double* part) {
   const T alpha = (T)scal[S_ALPHA], om = (T)scal[S_OMEGA];
-  double acc[1] = {0.0};
+  double acc[2] = {0.0, 0.0};
   for (int64_t i = (int64_t)blockIdx.x * kTPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kTPB) {
     x[i] += alpha * ph[i] + om * sh[i];
     const T ri = s[i] - om * t[i];
     r[i] = ri;
     acc[0] += (double)ri * (double)ri;
+    acc[1] += (double)rh[i] * (double)ri;
   }
-  block_sum_store<1>(acc, part);
+  block_sum_store<2>(acc, part);
 }
 
 // y = a*x + b*y with host scalars (GMRES)
@@ -619,11 +633,14 @@ __device__ inline void bcast_diag(const V4<AT>& dv, AT (&D)[16]) {
 
 // Forward solve L y = r over the block rows of one level (unit-lower diagonal blocks).
 // Level rows are the contiguous range [first, first + count) or list[first + g].
+// With fuse_u the level's rows have no upper neighbour blocks (the last colour of a
+// multicolour plan), so their backward solve is the diagonal block alone and is done here.
 template <typename T, typename LT, int G, int K>
 __global__ __launch_bounds__(256) void k_bsr_lsolve(const int32_t* __restrict__ list, int first, int count,
                                                     const BDesc* __restrict__ desc,
                                                     const int32_t* __restrict__ bcol, const LT* __restrict__ lu,
-                                                    const T* __restrict__ r, T* __restrict__ y, int xcd) {
+                                                    const T* __restrict__ r, T* __restrict__ y, int xcd,
+                                                    int fuse_u) {
   using AT = typename AccOf<LT>::type;
   const int64_t gt = logical_wg(xcd) * 256 + threadIdx.x;
   const int64_t g = gt / G;
@@ -656,6 +673,17 @@ __global__ __launch_bounds__(256) void k_bsr_lsolve(const int32_t* __restrict__ 
 #pragma unroll
     for (int b = 0; b < a; ++b) s -= D[a * 4 + b] * v[b];
     v[a] = a < m ? s : (AT)0;
+  }
+  if (fuse_u) {
+#pragma unroll
+    for (int a = 3; a >= 0; --a) {
+      if (a < m) {
+        AT s = v[a];
+#pragma unroll
+        for (int b = a + 1; b < 4; ++b) s -= D[a * 4 + b] * v[b];
+        v[a] = s / D[a * 4 + a];
+      }
+    }
   }
   if (act && lane < 4) y[4 * (int64_t)p + lane] = (T)pick4(v, lane);
 }
@@ -1266,6 +1294,12 @@ static int build_ilu_plan(sfem_ctx* ctx, LinAlg* L, int reorder) {
     P.contiguous = true;
     P.levLPtr = colorPtr;
     P.levUPtr = colorPtr;
+    // rows of the last colour have no upper neighbours (no two neighbours share a
+    // colour): their backward solve is the diagonal block and joins the forward sweep
+    const int nc = (int)colorPtr.size() - 1;
+    bool fl = nc > 0;
+    for (int64_t p = nc > 0 ? colorPtr[nc - 1] : 0; fl && p < nb; ++p) fl = tdiag[p] + 1 == (int32_t)(bptr[p + 1] - bptr[p]);
+    P.fuse_last = fl;
   } else {
     // level of a block row = 1 + max level of the rows it depends on (lower
     // neighbours for the forward solve and the factorisation, upper ones backward)
@@ -1322,7 +1356,9 @@ static void level_range(const IluPlan& P, bool upper, int lv, const int32_t** li
 }
 
 // Build the preconditioner from the current matrix (FP64 vals or FP32 vals32).
-int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f) {
+// refactor = false (ILU(0) only): refresh the node-block tangent the Krylov products use
+// and keep the existing factor (a lagged preconditioner).
+int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f, bool refactor) {
   LinAlg* L = ctx->la;
   hipStream_t st = ctx->stream;
   const int64_t n = L->n;
@@ -1373,12 +1409,17 @@ int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f) 
       int rc;
       if ((rc = ensure_vec(&L->kp32, nv)) || (rc = ensure_vec(&L->lu32, nv))) return rc;
       hipLaunchKernelGGL((k_bsr_gather<float>), dim3(gb), dim3(64), 0, st, P.nblk, P.d_bdesc, P.d_bcol,
-                         P.d_bcolofs, P.d_r0orig, ctx->d_rowptr32, ctx->d_vals32, L->kp32, L->lu32);
+                         P.d_bcolofs, P.d_r0orig, ctx->d_rowptr32, ctx->d_vals32, L->kp32,
+                         refactor ? L->lu32 : nullptr);
     } else {
       int rc;
       if ((rc = ensure_vec(&L->kp, nv)) || (rc = ensure_vec(&L->lu, nv))) return rc;
       hipLaunchKernelGGL((k_bsr_gather<double>), dim3(gb), dim3(64), 0, st, P.nblk, P.d_bdesc, P.d_bcol,
-                         P.d_bcolofs, P.d_r0orig, ctx->d_rowptr32, ctx->d_vals, L->kp, L->lu);
+                         P.d_bcolofs, P.d_r0orig, ctx->d_rowptr32, ctx->d_vals, L->kp, refactor ? L->lu : nullptr);
+    }
+    if (!refactor) {
+      LA_CHECK(hipGetLastError());
+      return SFEM_OK;
     }
     const int cap = (P.nbmax + 15) / 16 * 16;
     const int nl = (int)P.levLPtr.size() - 1;
@@ -1425,26 +1466,26 @@ constexpr int kBlkDefault = 6;  // G = 32, K = 1, no XCD remap
 
 template <typename T, typename LT, bool UPPER, int G, int K>
 static void tri_launch_g(hipStream_t st, int xcd, const int32_t* list, int first, int cnt, const IluPlan& P,
-                         const LT* lu, const T* r, T* y) {
+                         const LT* lu, const T* r, T* y, int fuse_u) {
   const unsigned gr = (unsigned)(((int64_t)cnt * G + 255) / 256);
   if constexpr (UPPER)
     hipLaunchKernelGGL((k_bsr_usolve<T, LT, G, K>), dim3(gr), dim3(256), 0, st, list, first, cnt, P.d_bdesc,
                        P.d_bcol, lu, y, xcd);
   else
     hipLaunchKernelGGL((k_bsr_lsolve<T, LT, G, K>), dim3(gr), dim3(256), 0, st, list, first, cnt, P.d_bdesc,
-                       P.d_bcol, lu, r, y, xcd);
+                       P.d_bcol, lu, r, y, xcd, fuse_u);
 }
 
 template <typename T, typename LT, bool UPPER>
 static void tri_launch(hipStream_t st, int code, const int32_t* list, int first, int cnt, const IluPlan& P,
-                       const LT* lu, const T* r, T* y) {
+                       const LT* lu, const T* r, T* y, int fuse_u = 0) {
   const int xcd = code >> 5;
 #define SFEM_TRI(i, G, K) \
-  case i: tri_launch_g<T, LT, UPPER, G, K>(st, xcd, list, first, cnt, P, lu, r, y); break;
+  case i: tri_launch_g<T, LT, UPPER, G, K>(st, xcd, list, first, cnt, P, lu, r, y, fuse_u); break;
   switch (code & 31) {
     SFEM_TRI(0, 4, 1) SFEM_TRI(1, 4, 2) SFEM_TRI(2, 8, 1) SFEM_TRI(3, 8, 2)
     SFEM_TRI(4, 16, 1) SFEM_TRI(5, 16, 2) SFEM_TRI(6, 32, 1)
-    default: tri_launch_g<T, LT, UPPER, 64, 1>(st, xcd, list, first, cnt, P, lu, r, y); break;
+    default: tri_launch_g<T, LT, UPPER, 64, 1>(st, xcd, list, first, cnt, P, lu, r, y, fuse_u); break;
   }
 #undef SFEM_TRI
 }
@@ -1465,14 +1506,18 @@ static void ilu_solve(sfem_ctx* ctx, const IluPlan& P, const LT* lu, const T* r,
   const int nl = (int)P.levLPtr.size() - 1;
   const int nu = (int)P.levUPtr.size() - 1;
   if (ev) (void)hipEventRecord(ev[0], st);
+  // the last forward level does the first backward level too when that is the same set
+  // of rows without upper neighbours
+  const int fuse = P.fuse_last ? 1 : 0;
   for (int lv = 0; lv < nl; ++lv) {
     const int32_t* list; int first, cnt;
     level_range(P, false, lv, &list, &first, &cnt);
     const int code = force >= 0 ? force : lv < (int)tl.size() ? tl[lv] : kBlkDefault;
-    if (cnt) tri_launch<T, LT, false>(st, code, list, first, cnt, P, lu, r, z);
+    if (cnt) tri_launch<T, LT, false>(st, code, list, first, cnt, P, lu, r, z, lv == nl - 1 ? fuse : 0);
     if (ev) (void)hipEventRecord(ev[1 + lv], st);
   }
-  for (int lv = 0; lv < nu; ++lv) {
+  if (ev && fuse) (void)hipEventRecord(ev[1 + nl], st);
+  for (int lv = fuse; lv < nu; ++lv) {
     const int32_t* list; int first, cnt;
     level_range(P, true, lv, &list, &first, &cnt);
     const int code = force >= 0 ? force : lv < (int)tu.size() ? tu[lv] : kBlkDefault;
@@ -1823,6 +1868,7 @@ static int bicgstab(sfem_ctx* ctx, const T* b, T* x, double rtol, int maxit, int
   init[S_BB] = hs[S_BB];
   init[S_RR] = hs[S_BB];
   init[S_RHRH] = hs[S_BB];
+  init[S_RHO1] = hs[S_BB];  // (rh, r) with rh = r = b
   init[S_TOL2] = (rtol * bnorm) * (rtol * bnorm);
   std::memcpy(hs, init, sizeof(init));
   LA_CHECK(hipMemcpyAsync(sc, hs, S_NSLOTS * sizeof(double), hipMemcpyHostToDevice, st));
@@ -1832,8 +1878,7 @@ static int bicgstab(sfem_ctx* ctx, const T* b, T* x, double rtol, int maxit, int
   double best = bnorm;
   int best_it = 0;
   for (int it = 1; it <= maxit; ++it) {
-    hipLaunchKernelGGL((k_dot<T>), dim3(g), dim3(kTPB), 0, st, n, rh, r, L->d_part);
-    hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(kTPB), 0, st, L->d_part, g, sc, 0, (int)FIN_RHO1);
+    // rho1 = (rh, r) and beta were formed with the previous iteration's residual update
     hipLaunchKernelGGL((k_bicg_p<T>), dim3(g), dim3(kTPB), 0, st, n, r, p, v, sc, first ? 1 : 0);
     first = false;
     int rc = pc_apply<T>(ctx, p, ph); if (rc) return rc;
@@ -1846,8 +1891,8 @@ static int bicgstab(sfem_ctx* ctx, const T* b, T* x, double rtol, int maxit, int
     spmv<T>(ctx, sh, t);
     hipLaunchKernelGGL((k_dot2<T>), dim3(g), dim3(kTPB), 0, st, n, t, t, t, s, L->d_part);
     hipLaunchKernelGGL((k_finalize<2>), dim3(1), dim3(kTPB), 0, st, L->d_part, g, sc, 0, (int)FIN_OMEGA);
-    hipLaunchKernelGGL((k_bicg_xr<T>), dim3(g), dim3(kTPB), 0, st, n, x, ph, sh, r, s, t, sc, L->d_part);
-    hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(kTPB), 0, st, L->d_part, g, sc, 0, (int)FIN_RR);
+    hipLaunchKernelGGL((k_bicg_xr<T>), dim3(g), dim3(kTPB), 0, st, n, x, ph, sh, r, s, t, rh, sc, L->d_part);
+    hipLaunchKernelGGL((k_finalize<2>), dim3(1), dim3(kTPB), 0, st, L->d_part, g, sc, 0, (int)FIN_RR_RHO);
     LA_CHECK(hipGetLastError());
     LA_CHECK(hipMemcpyAsync(hs, sc, S_NSLOTS * sizeof(double), hipMemcpyDeviceToHost, st));
     LA_CHECK(hipStreamSynchronize(st));
@@ -1878,6 +1923,7 @@ static int bicgstab(sfem_ctx* ctx, const T* b, T* x, double rtol, int maxit, int
       hs[S_FLAG] = 0.0;
       hs[S_RHO0] = 1.0; hs[S_ALPHA] = 1.0; hs[S_OMEGA] = 1.0;
       hs[S_RHRH] = hs[S_RR];
+      hs[S_RHO1] = hs[S_RR];  // (rh, r) with rh = r
       LA_CHECK(hipMemcpyAsync(sc, hs, S_NSLOTS * sizeof(double), hipMemcpyHostToDevice, st));
       first = true;
     }
@@ -2047,12 +2093,18 @@ int la_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int* iters, double* relre
   // the tangent is constant within a time step: rebuild the preconditioner only when it changed
   const int reorder = o->pc_type == SFEM_PC_ILU0 ? o->reorder : 0;
   const bool bf16f = fp32 && o->pc_type == SFEM_PC_ILU0 && o->factor_bf16 != 0;
-  if (L->pc_K_version != ctx->la_K_version || L->pc_type != o->pc_type || L->pc_fp32 != fp32 ||
-      L->pc_reorder != reorder || L->pc_bf16 != bf16f) {
-    L->pc_K_version = -1;
+  const bool same_pc = L->pc_K_version >= 0 && L->pc_type == o->pc_type && L->pc_fp32 == fp32 &&
+                       L->pc_reorder == reorder && L->pc_bf16 == bf16f;
+  if (!same_pc || (L->kb_version != ctx->la_K_version && ++L->lag_count >= L->pc_lag)) {
+    L->pc_K_version = L->kb_version = -1;
     if ((rc = la_setup_pc(ctx, o->pc_type, fp32, reorder, bf16f))) return rc;
     L->pc_reorder = reorder;
-    L->pc_K_version = ctx->la_K_version;
+    L->pc_K_version = L->kb_version = ctx->la_K_version;
+    L->lag_count = 0;
+  } else if (L->kb_version != ctx->la_K_version) {
+    // lagged preconditioner: the products use the new tangent, the factor stays
+    if (o->pc_type == SFEM_PC_ILU0 && (rc = la_setup_pc(ctx, o->pc_type, fp32, reorder, bf16f, false))) return rc;
+    L->kb_version = ctx->la_K_version;
   }
   L->block_space = L->pc_type == SFEM_PC_ILU0;
   const int32_t* padmap = L->block_space ? L->ilu->d_padmap : nullptr;

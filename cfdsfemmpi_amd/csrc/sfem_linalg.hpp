@@ -25,6 +25,7 @@ struct bf16;
 struct IluPlan {
   int reorder = -1;
   bool contiguous = false;   // level l is the block range [levPtr[l], levPtr[l+1])
+  bool fuse_last = false;    // the last forward level also does the first backward level
   int64_t nblk = 0, npad = 0, nvals = 0, npairs = 0;
   int nbmax = 0;
   std::vector<int32_t> levLPtr, levUPtr, h_levL, h_levU;
@@ -56,6 +57,9 @@ struct LinAlg {
   int pc_type = SFEM_PC_NONE;
   bool pc_fp32 = false;
   int64_t pc_K_version = -1;  // tangent version the preconditioner was built from
+  int64_t kb_version = -1;    // tangent version of the node-block copy the products use
+  int pc_lag = 1;             // rebuild the preconditioner every pc_lag-th new tangent
+  int lag_count = 0;
   std::vector<int32_t> h_blkStart, h_diagpos;
   int pc_reorder = 0;
   bool pc_bf16 = false;
@@ -97,7 +101,7 @@ struct LinAlg {
 
 int la_create(sfem_ctx* ctx);
 void la_destroy(sfem_ctx* ctx);
-int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f);
+int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f, bool refactor = true);
 int la_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int* iters, double* relres);
 int la_prepare(sfem_ctx* ctx, const sfem_solver_opts* o);
 int la_tuning(sfem_ctx* ctx, int* out);

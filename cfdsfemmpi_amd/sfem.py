@@ -60,6 +60,7 @@ ABI = [
     ("sfem_set_params", C.c_int, [_vp, _f64p, C.c_int, C.c_double, C.c_double]),
     ("sfem_set_precision", C.c_int, [_vp, C.c_int]),
     ("sfem_set_predictor", C.c_int, [_vp, C.c_int]),
+    ("sfem_set_pc_lag", C.c_int, [_vp, C.c_int]),
     ("sfem_time_update", C.c_int, [_vp, C.c_double]),
     ("sfem_assemble", C.c_int, [_vp, C.c_int, C.c_int, _f64p]),
     ("sfem_element_KF", C.c_int, [_vp, C.c_int64, _f64p, _f64p]),
@@ -277,6 +278,10 @@ class Context:
         v = C.c_int64(0)
         _check(self.lib.sfem_solver_stats(self.h, C.byref(v)), "sfem_solver_stats")
         return v.value
+
+    def set_pc_lag(self, lag: int):
+        """Rebuild the preconditioner on every lag-th new tangent only (1 = always)."""
+        _check(self.lib.sfem_set_pc_lag(self.h, int(lag)), "sfem_set_pc_lag")
 
     def get_ilu(self):
         """(perm, rowptr, colidx, vals) of the current ILU(0) factor in its ordering."""

@@ -117,6 +117,13 @@ int sfem_set_precision(sfem_ctx* ctx, int precision);
 enum { SFEM_PRED_PREVIOUS = 0, SFEM_PRED_EXTRAPOLATE = 1 };
 int sfem_set_predictor(sfem_ctx* ctx, int predictor);
 
+/* Rebuild the preconditioner only on every lag-th new tangent (lag >= 1; 1, the default,
+ * rebuilds it whenever the tangent changes, as PCSetUp does after each assembly in
+ * SolverPetsc::factoriseAndSolve, SolverPetsc.cpp:284-298).  With lag > 1 the Krylov
+ * products still use the current tangent; only the ILU(0) factor (or Jacobi diagonal) is
+ * reused.  No counterpart in the reference. */
+int sfem_set_pc_lag(sfem_ctx* ctx, int lag);
+
 /* Start a time step (StabFEM2.cpp:632-648): td[] from SetTimeParametersFluid,
  * SolutionData::timeUpdate (SolutionData.cpp:64-78), and
  * assignBoundaryConditions with the given ramp factor (StabFEM.cpp:391-410). */
