@@ -347,6 +347,12 @@ class StabFEM:
         self.opts = solver_opts()
         self.newton_tol = 1.0e-8   # hard-coded absolute tolerance (StabFEM2.cpp:749)
         self.forces = []           # (timeNow, Fx, Fy) per step, the forces file's lines
+        # postProcess after every step (StabFEM2.cpp:802-806) writes <infilename>-<n>.vtu;
+        # off unless asked for, the files go through a background writer thread
+        self.postprocess = False
+        self.infilename = "sfem"
+        self.fileCount = 0
+        self._vtu = None
         self.max_newton = 10       # hard-coded iteration cap (StabFEM2.cpp:654)
         self.precision = PREC_FP64
         self.history = []          # (tstep, iter, rhs_norm, krylov_its)
@@ -354,6 +360,11 @@ class StabFEM:
     # StabFEM::readInputData (StabFEM.cpp:80-264)
     def readInputData(self, fname: str):
         self.mesh = _deck.read_mesh(fname)
+        base = os.path.basename(fname)
+        for suffix in (".gz", ".dat"):
+            if base.endswith(suffix):
+                base = base[:-len(suffix)]
+        self.infilename = os.path.join(os.path.dirname(fname), base)
 
     # StabFEM::readControlParameters (StabFEM.cpp:268-322)
     def readControlParameters(self, fname: str):
@@ -410,10 +421,24 @@ class StabFEM:
             reacVec = np.zeros(self.mesh.nNode * ndof)
             out = np.asarray(self.mesh.out_nodes, np.int64)
             self.forces.append((timeNow, float(reacVec[out * ndof].sum()), float(reacVec[out * ndof + 1].sum())))
+            if self.postprocess:
+                self.postProcess()
             timeNow = timeNow + c.dt
             if timeNow > c.timeFinal:
                 break
+        if self._vtu is not None:
+            self._vtu.close()
         return ctx.get_solution()
+
+    # StabFEM::postProcess (StabFEM3.cpp:6-160): velocity and pressure of the current
+    # solution on the mesh as <infilename>-<fileCount %06d>.vtu, fileCount from 0
+    def postProcess(self):
+        from .vtu import VTUWriter
+        m = self.mesh
+        if self._vtu is None:
+            self._vtu = VTUWriter(m.coords, m.conn, m.ndim, asynchronous=True)
+        self._vtu.submit(f"{self.infilename}-{self.fileCount:06d}.vtu", self.ctx.get_solution())
+        self.fileCount += 1
 
     # the "<mesh>-forces.dat" file opened in StabFEM::readInputData (StabFEM.cpp:249-259)
     # and written once per step (StabFEM2.cpp:815): fixed notation, 14 decimals

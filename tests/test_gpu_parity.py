@@ -344,3 +344,24 @@ def test_negative_jacobian_reported():
         ctx.prepare_elements()
     assert ei.value.code == -3 and "element 4" in str(ei.value)
     ctx.close()
+
+
+def test_postprocess_vtu_files(tmp_path, golden):
+    """postProcess after every step writes <infilename>-<%06d>.vtu with the solution."""
+    from cfdsfemmpi_amd import vtu
+    d = os.path.join(golden, "ldc2dquad10")
+    fem = sfem.StabFEM()
+    fem.readInputData(os.path.join(d, "LDC2Dquad10.dat"))
+    fem.readControlParameters(os.path.join(d, "control-parameters-ldc-Re100.dat"))
+    fem.ctl.stepsMax = 2
+    fem.prepareInputData()
+    fem.setSolver(sfem.solver_opts("bcgs", "ilu", 1e-12, 2000, reorder=1))
+    fem.infilename = str(tmp_path / "LDC2Dquad10")
+    fem.postprocess = True
+    s = fem.solveFullyImplicit().reshape(-1, 3)
+    files = sorted(p.name for p in tmp_path.iterdir())
+    print(f"[vtu] {files}", flush=True)
+    assert files == ["LDC2Dquad10-000000.vtu", "LDC2Dquad10-000001.vtu"]
+    a = vtu.read_vtu_arrays(str(tmp_path / files[-1]))
+    assert np.allclose(a["PointData/pressure"], s[:, 2], rtol=1e-6, atol=1e-6)
+    assert np.allclose(a["PointData/velocity"][:, :2], s[:, :2], rtol=1e-6, atol=1e-6)

@@ -129,3 +129,28 @@ def test_forces_file_format(tmp_path):
     fem.writeForces(str(fn))
     assert fn.read_text() == "0.10000000000000\t0.00000000000000\t0.00000000000000\n" \
                              "0.20000000000000\t0.00000000000000\t0.00000000000000\n"
+
+
+def test_vtu_writer_round_trip(tmp_path):
+    """VTU content of StabFEM::postProcess (StabFEM3.cpp:6-160): points with z = 0 in 2D,
+    one cell per element with its VTK type, pressure and velocity point arrays."""
+    from cfdsfemmpi_amd import vtu
+    for mesh, ctype in ((meshgen.ldc2d_quad(3), 9), (meshgen.ldc2d_tria(3), 5), (meshgen.ldc3d_hex(2), 12),
+                        (meshgen.ldc3d_tet(2), 10)):
+        nd = mesh.ndim + 1
+        rng = np.random.default_rng(3)
+        soln = rng.uniform(-1, 1, mesh.nNode * nd)
+        w = vtu.VTUWriter(mesh.coords, mesh.conn, mesh.ndim, asynchronous=True)
+        fn = tmp_path / f"m{mesh.ndim}{mesh.npElem}-000000.vtu"
+        w.submit(str(fn), soln)
+        w.close()
+        a = vtu.read_vtu_arrays(str(fn))
+        s = soln.reshape(-1, nd)
+        assert np.allclose(a["Points/points"][:, :mesh.ndim], mesh.coords, rtol=1e-6, atol=1e-7)
+        assert np.all(a["Points/points"][:, mesh.ndim:] == 0)
+        assert np.array_equal(a["Cells/connectivity"], mesh.conn.reshape(-1))
+        assert np.array_equal(a["Cells/offsets"], np.arange(1, mesh.nElem + 1) * mesh.npElem)
+        assert np.all(a["Cells/types"] == ctype)
+        assert np.allclose(a["PointData/pressure"], s[:, mesh.ndim], rtol=1e-6, atol=1e-7)
+        assert np.allclose(a["PointData/velocity"][:, :mesh.ndim], s[:, :mesh.ndim], rtol=1e-6, atol=1e-7)
+        assert np.all(a["PointData/procId"] == 0) and np.all(a["CellData/procId"] == 0)
