@@ -296,6 +296,30 @@ __device__ inline void accumulate_block(const TeamLds<ET>& L, int i, int j, cons
   }
 }
 
+// Read-modify-write of one CSR row segment of a node-pair block: a full block (all
+// NDOF columns of node j free) goes through one vector access of NDOF consecutive
+// values (the segment is only 8- or 4-byte aligned), a partial one value by value.
+template <int NDOF, typename T>
+__device__ inline void add_row_segment(T* __restrict__ row, const T (&k)[NDOF], int mj) {
+  constexpr int FULL = (1 << NDOF) - 1;
+  if (mj == FULL) {
+    typedef T vec_t __attribute__((ext_vector_type(NDOF == 3 ? 4 : NDOF), aligned(sizeof(T))));
+    if constexpr (NDOF == 4) {
+      vec_t r = *reinterpret_cast<const vec_t*>(row);
+      r.x += k[0]; r.y += k[1]; r.z += k[2]; r.w += k[3];
+      *reinterpret_cast<vec_t*>(row) = r;
+      return;
+    }
+  }
+  int rb = 0;
+#pragma unroll
+  for (int bq = 0; bq < NDOF; ++bq) {
+    if (!((mj >> bq) & 1)) continue;
+    row[rb] += k[bq];
+    ++rb;
+  }
+}
+
 template <int ET, int MODE>
 __global__ __launch_bounds__(256) void k_assemble(AsmView v, ElemParams p,
                                                   const int32_t* __restrict__ elems, int64_t n,
@@ -464,13 +488,7 @@ __global__ __launch_bounds__(256) void k_assemble(AsmView v, ElemParams p,
             int ra = 0;
             for (int a = 0; a < NDOF; ++a) {
               if (!((mi >> a) & 1)) continue;
-              double* row = v.vals + (int64_t)rs + (int64_t)ra * rl + co;
-              int rb = 0;
-              for (int bq = 0; bq < NDOF; ++bq) {
-                if (!((mj >> bq) & 1)) continue;
-                row[rb] += K[a][bq];
-                ++rb;
-              }
+              add_row_segment<NDOF, double>(v.vals + (int64_t)rs + (int64_t)ra * rl + co, K[a], mj);
               ++ra;
             }
           }
@@ -488,13 +506,7 @@ __global__ __launch_bounds__(256) void k_assemble(AsmView v, ElemParams p,
           int ra = 0;
           for (int a = 0; a < NDOF; ++a) {
             if (!((mi >> a) & 1)) continue;
-            float* row = v.vals32 + (int64_t)rs + (int64_t)ra * rl + co;
-            int rb = 0;
-            for (int bq = 0; bq < NDOF; ++bq) {
-              if (!((mj >> bq) & 1)) continue;
-              row[rb] += K[a][bq];
-              ++rb;
-            }
+            add_row_segment<NDOF, float>(v.vals32 + (int64_t)rs + (int64_t)ra * rl + co, K[a], mj);
             ++ra;
           }
         }
