@@ -117,6 +117,7 @@ static AsmView asm_view(sfem_ctx* c) {
   v.rhs = c->d_rhs;
   v.probeK = c->d_probeK;
   v.probeF = c->d_probeF;
+  v.elem_mu = c->d_elem_mu;
   return v;
 }
 
@@ -207,6 +208,8 @@ void sfem_destroy(sfem_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->la) la_destroy(c);
+  if (c->batch) batch_destroy(c);
+  dfree(c->d_elem_mu);
   dfree(c->d_coords); dfree(c->d_conn); dfree(c->d_charlen); dfree(c->d_vol); dfree(c->d_bad);
   dfree(c->d_nodeInfo); dfree(c->d_colofs); dfree(c->d_elemByColor); dfree(c->d_rowptr32);
   dfree(c->d_colidx); dfree(c->d_assy); dfree(c->d_dbc_ind); dfree(c->d_dbc_val); dfree(c->d_dbc_mult);

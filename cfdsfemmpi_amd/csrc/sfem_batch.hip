@@ -1,0 +1,559 @@
+// sfem_batch.hip -- many independent meshes solved together in one context.
+//
+// The meshes are one disjoint union (nodes and elements mesh by mesh), so assembly,
+// the tangent, its node-block form and the ILU(0) factor are block diagonal and every
+// existing kernel handles all meshes in one launch.  What is per mesh: the viscosity
+// (a per-element array read by the element kernel), the Newton test (each mesh stops
+// at ||F_m|| < tol, StabFEM2.cpp:740-749, and is frozen from then on), and the Krylov
+// solve: BiCGStab with per-mesh scalars (alpha, omega, rho, ...), per-mesh convergence
+// ||r_m|| <= rtol ||b_m|| and per-mesh breakdown restarts, so that each mesh follows
+// the iteration it would follow alone.  Reductions are per mesh and deterministic
+// (fixed chunk order).  One BiCGStab iteration of all meshes is captured once per solve
+// as a HIP graph and replayed; the host reads one counter of finished meshes per
+// iteration.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "sfem_internal.hpp"
+#include "sfem_linalg.hpp"
+#include "sfem_state.hpp"
+
+namespace sfem {
+
+namespace {
+constexpr int kTB = 256;          // threads per block of the per-chunk kernels
+constexpr int kChunk = 4096;      // Krylov slots per chunk (at most)
+// per-mesh Krylov scalars
+enum {
+  B_RHO0 = 0, B_RHO1, B_ALPHA, B_OMEGA, B_BETA, B_RR, B_BB, B_TOL2, B_RHRH, B_SS,
+  B_FIRST, B_DONE, B_HALF, B_RESTART, B_ITS, B_BAD, B_N = 16
+};
+}  // namespace
+
+struct Batch {
+  int nmesh = 0;
+  std::vector<int64_t> dof_ptr;   // free-DOF range of each mesh (DOF order)
+  int64_t* d_dof_ptr = nullptr;
+  int plan_id = -1;               // ILU plan the chunk lists were built for
+  int nchunk = 0;
+  int4* d_chunk = nullptr;        // {slot begin, slot end, mesh, 0}
+  int32_t* d_mptr = nullptr;      // chunks of mesh m: d_mlist[d_mptr[m] .. d_mptr[m+1])
+  int32_t* d_mlist = nullptr;
+  double* d_sc = nullptr;         // [nmesh][B_N]
+  double* d_part = nullptr;       // [nchunk][2]
+  double* d_norm = nullptr;       // per-mesh ||F_m||
+  double* h_norm = nullptr;       // pinned
+  int32_t* d_active = nullptr;    // per-mesh Newton activity
+  int32_t* d_ndone = nullptr;     // finished meshes of the current solve
+  int32_t* h_ndone = nullptr;     // pinned
+  double* h_sc = nullptr;         // pinned copy of d_sc
+  void* vec[10] = {};             // r, rh, p, v, s, t, ph, sh, x, b
+  size_t vec_bytes = 0;
+  bool use_graph = true;
+  ~Batch() {
+    void* ptrs[] = {d_dof_ptr, d_chunk, d_mptr, d_mlist, d_sc, d_part, d_norm, d_active, d_ndone};
+    for (void* q : ptrs)
+      if (q) (void)hipFree(q);
+    for (void* q : vec)
+      if (q) (void)hipFree(q);
+    if (h_norm) (void)hipHostFree(h_norm);
+    if (h_ndone) (void)hipHostFree(h_ndone);
+    if (h_sc) (void)hipHostFree(h_sc);
+  }
+};
+
+void batch_destroy(sfem_ctx* ctx) {
+  delete ctx->batch;
+  ctx->batch = nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+__device__ inline double block_sum(double v, double* sh) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += sh[k];
+  __syncthreads();
+  return s;  // valid in thread 0
+}
+
+// ||a||^2 over each mesh's DOF range, one block per mesh
+__global__ __launch_bounds__(kTB) void k_seg_norm2(const int64_t* __restrict__ ptr, const double* __restrict__ a,
+                                                   double* __restrict__ out) {
+  __shared__ double sh[kTB / 64];
+  const int m = blockIdx.x;
+  double s = 0.0;
+  for (int64_t i = ptr[m] + threadIdx.x; i < ptr[m + 1]; i += kTB) s += a[i] * a[i];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) out[m] = sqrt(s);
+}
+
+// chunk-parallel kernels: block c handles slots [chunk.x, chunk.y) of mesh chunk.z
+template <typename T>
+__global__ __launch_bounds__(kTB) void k_b_pad_in(const int4* __restrict__ ch, const int32_t* __restrict__ padmap,
+                                                  const double* __restrict__ rhs, const double* __restrict__ norm,
+                                                  const int32_t* __restrict__ active, T* __restrict__ b) {
+  const int4 c = ch[blockIdx.x];
+  const double nm = norm[c.z];
+  const double sc = (active[c.z] && nm > 0.0) ? 1.0 / nm : 0.0;
+  for (int i = c.x + threadIdx.x; i < c.y; i += kTB) {
+    const int32_t k = padmap[i];
+    b[i] = k >= 0 ? (T)(rhs[k] * sc) : (T)0;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kTB) void k_b_pad_out(const int4* __restrict__ ch, const int32_t* __restrict__ padmap,
+                                                   const T* __restrict__ x, const double* __restrict__ norm,
+                                                   const int32_t* __restrict__ active, double* __restrict__ du) {
+  const int4 c = ch[blockIdx.x];
+  const double sc = active[c.z] ? norm[c.z] : 0.0;
+  for (int i = c.x + threadIdx.x; i < c.y; i += kTB) {
+    const int32_t k = padmap[i];
+    if (k >= 0) du[k] = (double)x[i] * sc;
+  }
+}
+
+// x = 0, r = rh = b, p = v = 0; partial (b, b)
+template <typename T>
+__global__ __launch_bounds__(kTB) void k_b_init(const int4* __restrict__ ch, const T* __restrict__ b, T* __restrict__ x,
+                                                T* __restrict__ r, T* __restrict__ rh, T* __restrict__ p,
+                                                T* __restrict__ v, double* __restrict__ part) {
+  __shared__ double sh[kTB / 64];
+  const int4 c = ch[blockIdx.x];
+  double s = 0.0;
+  for (int i = c.x + threadIdx.x; i < c.y; i += kTB) {
+    const T bi = b[i];
+    x[i] = (T)0; r[i] = bi; rh[i] = bi; p[i] = (T)0; v[i] = (T)0;
+    s += (double)bi * (double)bi;
+  }
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = s; part[2 * blockIdx.x + 1] = 0.0; }
+}
+
+// p = r + beta (p - omega v), or p = r on a (re)start
+template <typename T>
+__global__ __launch_bounds__(kTB) void k_b_p(const int4* __restrict__ ch, const double* __restrict__ sc,
+                                             const T* __restrict__ r, T* __restrict__ p, const T* __restrict__ v) {
+  const int4 c = ch[blockIdx.x];
+  const double* S = sc + (int64_t)c.z * B_N;
+  const bool first = S[B_FIRST] != 0.0;
+  const T beta = first ? (T)0 : (T)S[B_BETA], om = first ? (T)0 : (T)S[B_OMEGA];
+  for (int i = c.x + threadIdx.x; i < c.y; i += kTB) p[i] = first ? r[i] : r[i] + beta * (p[i] - om * v[i]);
+}
+
+// partials (a1, b1), (a2, b2)
+template <typename T>
+__global__ __launch_bounds__(kTB) void k_b_dot2(const int4* __restrict__ ch, const T* __restrict__ a1,
+                                                const T* __restrict__ b1, const T* __restrict__ a2,
+                                                const T* __restrict__ b2, double* __restrict__ part) {
+  __shared__ double sh[kTB / 64];
+  const int4 c = ch[blockIdx.x];
+  double s1 = 0.0, s2 = 0.0;
+  for (int i = c.x + threadIdx.x; i < c.y; i += kTB) {
+    s1 += (double)a1[i] * (double)b1[i];
+    s2 += (double)a2[i] * (double)b2[i];
+  }
+  s1 = block_sum(s1, sh);
+  s2 = block_sum(s2, sh);
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = s1; part[2 * blockIdx.x + 1] = s2; }
+}
+
+// s = r - alpha v ; partial (s, s)
+template <typename T>
+__global__ __launch_bounds__(kTB) void k_b_s(const int4* __restrict__ ch, const double* __restrict__ sc,
+                                             const T* __restrict__ r, const T* __restrict__ v, T* __restrict__ s,
+                                             double* __restrict__ part) {
+  __shared__ double sh[kTB / 64];
+  const int4 c = ch[blockIdx.x];
+  const T alpha = (T)sc[(int64_t)c.z * B_N + B_ALPHA];
+  double a = 0.0;
+  for (int i = c.x + threadIdx.x; i < c.y; i += kTB) {
+    const T si = r[i] - alpha * v[i];
+    s[i] = si;
+    a += (double)si * (double)si;
+  }
+  a = block_sum(a, sh);
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = a; part[2 * blockIdx.x + 1] = 0.0; }
+}
+
+// x += alpha ph + omega sh ; r = s - omega t ; partials (r, r), (rh, r)
+template <typename T>
+__global__ __launch_bounds__(kTB) void k_b_xr(const int4* __restrict__ ch, const double* __restrict__ sc,
+                                              T* __restrict__ x, const T* __restrict__ ph, const T* __restrict__ shv,
+                                              T* __restrict__ r, const T* __restrict__ s, const T* __restrict__ t,
+                                              const T* __restrict__ rh, double* __restrict__ part) {
+  __shared__ double sh[kTB / 64];
+  const int4 c = ch[blockIdx.x];
+  const double* S = sc + (int64_t)c.z * B_N;
+  const T alpha = (T)S[B_ALPHA], om = (T)S[B_OMEGA];
+  double a = 0.0, b = 0.0;
+  for (int i = c.x + threadIdx.x; i < c.y; i += kTB) {
+    x[i] += alpha * ph[i] + om * shv[i];
+    const T ri = s[i] - om * t[i];
+    r[i] = ri;
+    a += (double)ri * (double)ri;
+    b += (double)rh[i] * (double)ri;
+  }
+  a = block_sum(a, sh);
+  b = block_sum(b, sh);
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = a; part[2 * blockIdx.x + 1] = b; }
+}
+
+// restart of a broken-down mesh: rh = r
+template <typename T>
+__global__ __launch_bounds__(kTB) void k_b_restart(const int4* __restrict__ ch, const double* __restrict__ sc,
+                                                   const T* __restrict__ r, T* __restrict__ rh) {
+  const int4 c = ch[blockIdx.x];
+  if (sc[(int64_t)c.z * B_N + B_RESTART] == 0.0) return;
+  for (int i = c.x + threadIdx.x; i < c.y; i += kTB) rh[i] = r[i];
+}
+
+enum { F_INIT = 0, F_ALPHA, F_SS, F_OMEGA, F_RR_RHO };
+
+// Per-mesh scalar updates from the chunk partials (summed in chunk order); one thread
+// per mesh, one block.  F_RR_RHO also counts the finished meshes.
+__global__ __launch_bounds__(kTB) void k_b_final(int nmesh, int op, const int32_t* __restrict__ mptr,
+                                                 const int32_t* __restrict__ mlist, const double* __restrict__ part,
+                                                 double* __restrict__ sc, double rtol, int32_t* __restrict__ ndone) {
+  __shared__ int cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  int mine = 0;
+  for (int m = threadIdx.x; m < nmesh; m += blockDim.x) {
+    double a = 0.0, b = 0.0;
+    for (int k = mptr[m]; k < mptr[m + 1]; ++k) {
+      a += part[2 * mlist[k]];
+      b += part[2 * mlist[k] + 1];
+    }
+    double* S = sc + (int64_t)m * B_N;
+    switch (op) {
+      case F_INIT:
+        for (int q = 0; q < B_N; ++q) S[q] = 0.0;
+        S[B_BB] = a; S[B_RR] = a; S[B_RHRH] = a; S[B_RHO1] = a;
+        S[B_RHO0] = 1.0; S[B_ALPHA] = 1.0; S[B_OMEGA] = 1.0; S[B_FIRST] = 1.0;
+        S[B_TOL2] = rtol * rtol * a;
+        S[B_DONE] = (a > 0.0) ? 0.0 : 1.0;
+        break;
+      case F_ALPHA:  // a = (rh, v), b = (v, v)
+        S[B_RESTART] = 0.0;
+        if (S[B_DONE] != 0.0 || !(fabs(a) > 1e-12 * sqrt(S[B_RHRH] * b))) {
+          S[B_ALPHA] = 0.0;
+          if (S[B_DONE] == 0.0) S[B_RESTART] = 1.0;  // near breakdown: restart after this step
+        } else {
+          S[B_ALPHA] = S[B_RHO1] / a;
+        }
+        break;
+      case F_SS:  // a = (s, s)
+        S[B_SS] = a;
+        S[B_HALF] = (S[B_DONE] == 0.0 && a <= S[B_TOL2]) ? 1.0 : 0.0;
+        break;
+      case F_OMEGA:  // a = (t, t), b = (t, s)
+        S[B_OMEGA] = (S[B_DONE] != 0.0 || S[B_HALF] != 0.0 || a == 0.0) ? 0.0 : b / a;
+        break;
+      case F_RR_RHO: {  // a = (r, r), b = (rh, r)
+        if (S[B_DONE] == 0.0) {
+          S[B_ITS] += 1.0;
+          S[B_RR] = a;
+          if (!isfinite(a)) {
+            S[B_BAD] = 1.0;
+            S[B_DONE] = 1.0;
+          } else if (S[B_HALF] != 0.0 || a <= S[B_TOL2]) {
+            S[B_DONE] = 1.0;
+          } else {
+            const double rho0 = S[B_RHO1], om = S[B_OMEGA];
+            if (S[B_RESTART] != 0.0 || !(fabs(b) > 1e-12 * sqrt(S[B_RHRH] * a)) || rho0 == 0.0 || om == 0.0) {
+              // restart with the current residual as shadow residual (rh = r follows)
+              S[B_RESTART] = 1.0;
+              S[B_RHO0] = 1.0; S[B_ALPHA] = 1.0; S[B_OMEGA] = 1.0;
+              S[B_RHRH] = a; S[B_RHO1] = a; S[B_FIRST] = 1.0;
+            } else {
+              S[B_RHO0] = rho0;
+              S[B_RHO1] = b;
+              S[B_BETA] = (b / rho0) * (S[B_ALPHA] / om);
+              S[B_FIRST] = 0.0;
+            }
+          }
+        }
+        if (S[B_DONE] != 0.0) { S[B_ALPHA] = 0.0; S[B_OMEGA] = 0.0; S[B_RESTART] = 0.0; }
+        mine += S[B_DONE] != 0.0 ? 1 : 0;
+        break;
+      }
+    }
+  }
+  if (op == F_RR_RHO) {
+    atomicAdd(&cnt, mine);
+    __syncthreads();
+    if (threadIdx.x == 0) *ndone = cnt;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+#define BA_CHECK(x)                                   \
+  do {                                                \
+    hipError_t _e = (x);                              \
+    if (_e != hipSuccess) return set_hip_error(_e, #x); \
+  } while (0)
+#define HIP_OK(x) BA_CHECK(x)
+
+template <typename T>
+static hipError_t balloc(T** p, size_t n) {
+  *p = nullptr;
+  return hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T));
+}
+
+// Chunks of the Krylov layout: consecutive node blocks of one mesh, at most kChunk slots.
+static int build_chunks(sfem_ctx* ctx, Batch& B) {
+  LinAlg* L = ctx->la;
+  const IluPlan& P = *L->ilu;
+  std::vector<int32_t> r0(P.nblk);
+  BA_CHECK(hipMemcpy(r0.data(), P.d_r0orig, P.nblk * sizeof(int32_t), hipMemcpyDeviceToHost));
+  auto mesh_of = [&](int32_t dof) {
+    return (int)(std::upper_bound(B.dof_ptr.begin(), B.dof_ptr.end(), (int64_t)dof) - B.dof_ptr.begin()) - 1;
+  };
+  std::vector<int4> ch;
+  int64_t p = 0;
+  while (p < P.nblk) {
+    const int m = mesh_of(r0[p]);
+    int64_t q = p + 1;
+    while (q < P.nblk && q - p < kChunk / 4 && mesh_of(r0[q]) == m) ++q;
+    ch.push_back(int4{(int)(4 * p), (int)(4 * q), m, 0});
+    p = q;
+  }
+  std::vector<int32_t> mptr(B.nmesh + 1, 0), mlist(ch.size());
+  for (const int4& c : ch) mptr[c.z + 1]++;
+  for (int m = 0; m < B.nmesh; ++m) mptr[m + 1] += mptr[m];
+  std::vector<int32_t> pos(mptr.begin(), mptr.end() - 1);
+  for (size_t k = 0; k < ch.size(); ++k) mlist[pos[ch[k].z]++] = (int32_t)k;
+  for (void* q : {(void*)B.d_chunk, (void*)B.d_mptr, (void*)B.d_mlist, (void*)B.d_part})
+    if (q) (void)hipFree(q);
+  B.nchunk = (int)ch.size();
+  BA_CHECK(balloc(&B.d_chunk, ch.size()));
+  BA_CHECK(balloc(&B.d_mptr, mptr.size()));
+  BA_CHECK(balloc(&B.d_mlist, mlist.size()));
+  BA_CHECK(balloc(&B.d_part, 2 * ch.size()));
+  BA_CHECK(hipMemcpy(B.d_chunk, ch.data(), ch.size() * sizeof(int4), hipMemcpyHostToDevice));
+  BA_CHECK(hipMemcpy(B.d_mptr, mptr.data(), mptr.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  BA_CHECK(hipMemcpy(B.d_mlist, mlist.data(), mlist.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  B.plan_id = L->plan_id;
+  return SFEM_OK;
+}
+
+// One batched BiCGStab iteration (all launches on the context stream, no host work).
+template <typename T>
+static int bb_iteration(sfem_ctx* ctx, Batch& B, double rtol) {
+  hipStream_t st = ctx->stream;
+  T** V = (T**)B.vec;
+  T *r = V[0], *rh = V[1], *p = V[2], *v = V[3], *s = V[4], *t = V[5], *ph = V[6], *sh = V[7], *x = V[8];
+  const dim3 g(B.nchunk), tb(kTB);
+  auto pc = [&](const T* a, T* z) { return sizeof(T) == 8 ? la_apply_pc64(ctx, (const double*)a, (double*)z)
+                                                          : la_apply_pc32(ctx, (const float*)a, (float*)z); };
+  auto op = [&](const T* a, T* y) { return sizeof(T) == 8 ? la_apply_op64(ctx, (const double*)a, (double*)y)
+                                                          : la_apply_op32(ctx, (const float*)a, (float*)y); };
+  auto fin = [&](int o) {
+    hipLaunchKernelGGL(k_b_final, dim3(1), dim3(kTB), 0, st, B.nmesh, o, B.d_mptr, B.d_mlist, B.d_part, B.d_sc,
+                       rtol, B.d_ndone);
+  };
+  int rc;
+  hipLaunchKernelGGL((k_b_p<T>), g, tb, 0, st, B.d_chunk, B.d_sc, r, p, v);
+  if ((rc = pc(p, ph)) || (rc = op(ph, v))) return rc;
+  hipLaunchKernelGGL((k_b_dot2<T>), g, tb, 0, st, B.d_chunk, rh, v, v, v, B.d_part);
+  fin(F_ALPHA);
+  hipLaunchKernelGGL((k_b_s<T>), g, tb, 0, st, B.d_chunk, B.d_sc, r, v, s, B.d_part);
+  fin(F_SS);
+  if ((rc = pc(s, sh)) || (rc = op(sh, t))) return rc;
+  hipLaunchKernelGGL((k_b_dot2<T>), g, tb, 0, st, B.d_chunk, t, t, t, s, B.d_part);
+  fin(F_OMEGA);
+  hipLaunchKernelGGL((k_b_xr<T>), g, tb, 0, st, B.d_chunk, B.d_sc, x, ph, sh, r, s, t, rh, B.d_part);
+  fin(F_RR_RHO);
+  hipLaunchKernelGGL((k_b_restart<T>), g, tb, 0, st, B.d_chunk, B.d_sc, r, rh);
+  BA_CHECK(hipGetLastError());
+  return SFEM_OK;
+}
+
+// Batched solve of A du = rhs with per-mesh right-preconditioned BiCGStab; meshes with
+// active[m] = 0 get du_m = 0.  its[m]: iterations of mesh m.
+template <typename T>
+static int batch_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int32_t* its, int* bad_mesh) {
+  Batch& B = *ctx->batch;
+  LinAlg* L = ctx->la;
+  hipStream_t st = ctx->stream;
+  int rc;
+  if ((rc = la_prepare_solve(ctx, o))) return rc;
+  if (!L->block_space) return set_error(SFEM_ERR_UNSUPPORTED, "batched solve: ILU(0) preconditioner required");
+  if (B.plan_id != L->plan_id && (rc = build_chunks(ctx, B))) return rc;
+  const size_t vb = (size_t)L->nvec * sizeof(T);
+  if (B.vec_bytes != vb) {
+    for (void*& q : B.vec)
+      if (q) { (void)hipFree(q); q = nullptr; }
+    for (void*& q : B.vec) BA_CHECK(hipMalloc(&q, vb));
+    B.vec_bytes = vb;
+  }
+  T** V = (T**)B.vec;
+  const dim3 g(B.nchunk), tb(kTB);
+  const IluPlan& P = *L->ilu;
+  hipLaunchKernelGGL((k_b_pad_in<T>), g, tb, 0, st, B.d_chunk, P.d_padmap, ctx->d_rhs, B.d_norm, B.d_active, V[9]);
+  hipLaunchKernelGGL((k_b_init<T>), g, tb, 0, st, B.d_chunk, V[9], V[8], V[0], V[1], V[2], V[3], B.d_part);
+  hipLaunchKernelGGL(k_b_final, dim3(1), dim3(kTB), 0, st, B.nmesh, (int)F_INIT, B.d_mptr, B.d_mlist, B.d_part,
+                     B.d_sc, o->rtol, B.d_ndone);
+  BA_CHECK(hipGetLastError());
+  // one iteration as a graph, replayed
+  hipGraphExec_t exec = nullptr;
+  const bool kt_on = ctx->kt.on;
+  if (B.use_graph) {
+    ctx->kt.on = false;
+    hipGraph_t graph = nullptr;
+    BA_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    rc = bb_iteration<T>(ctx, B, o->rtol);
+    hipError_t ec = hipStreamEndCapture(st, &graph);
+    ctx->kt.on = kt_on;
+    if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+    if (ec != hipSuccess) return set_hip_error(ec, "batched iteration capture");
+    ec = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ec != hipSuccess) return set_hip_error(ec, "batched iteration instantiate");
+  }
+  int it = 0;
+  for (it = 1; it <= o->max_it; ++it) {
+    if (exec) {
+      hipError_t e = hipGraphLaunch(exec, st);
+      if (e != hipSuccess) { (void)hipGraphExecDestroy(exec); return set_hip_error(e, "batched iteration launch"); }
+    } else if ((rc = bb_iteration<T>(ctx, B, o->rtol))) {
+      return rc;
+    }
+    hipError_t e = hipMemcpyAsync(B.h_ndone, B.d_ndone, sizeof(int32_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) { if (exec) (void)hipGraphExecDestroy(exec); return set_hip_error(e, "batched iteration"); }
+    if (*B.h_ndone >= B.nmesh) break;
+  }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  hipLaunchKernelGGL((k_b_pad_out<T>), g, tb, 0, st, B.d_chunk, P.d_padmap, V[8], B.d_norm, B.d_active, ctx->d_du);
+  BA_CHECK(hipGetLastError());
+  BA_CHECK(hipMemcpyAsync(B.h_sc, B.d_sc, (size_t)B.nmesh * B_N * sizeof(double), hipMemcpyDeviceToHost, st));
+  BA_CHECK(hipStreamSynchronize(st));
+  *bad_mesh = -1;
+  for (int m = 0; m < B.nmesh; ++m) {
+    const double* S = B.h_sc + (size_t)m * B_N;
+    its[m] = (int32_t)S[B_ITS];
+    if (S[B_BAD] != 0.0 && *bad_mesh < 0) *bad_mesh = m;
+    if (S[B_DONE] == 0.0 && *bad_mesh < 0) *bad_mesh = m;
+  }
+  return SFEM_OK;
+}
+
+}  // namespace sfem
+
+using namespace sfem;
+
+extern "C" {
+
+int sfem_set_batch(sfem_ctx* c, int32_t nmesh, const int32_t* elem_mesh, const double* mu) {
+  if (!c || nmesh <= 0 || !elem_mesh || !mu) return set_error(SFEM_ERR_ARG, "bad batch arguments");
+  if (!c->la) return set_error(SFEM_ERR_STATE, "build the pattern before sfem_set_batch");
+  HIP_OK(hipSetDevice(c->device));
+  const int npe = c->dims.npe, ndof = c->dims.ndof;
+  // mesh of every node; nodes (and so free DOFs) must come mesh by mesh
+  std::vector<int32_t> nodeMesh(c->nNode, -1);
+  for (int64_t e = 0; e < c->nElem; ++e) {
+    const int32_t m = elem_mesh[e];
+    if (m < 0 || m >= nmesh) return set_error(SFEM_ERR_ARG, "element mesh id out of range");
+    for (int a = 0; a < npe; ++a) {
+      int32_t& nm = nodeMesh[c->conn[e * npe + a]];
+      if (nm >= 0 && nm != m) return set_error(SFEM_ERR_ARG, "a node is shared by two meshes");
+      nm = m;
+    }
+  }
+  std::vector<int64_t> ptr(nmesh + 1, 0);
+  int32_t last = 0;
+  for (int64_t i = 0; i < c->nNode; ++i) {
+    const int32_t m = nodeMesh[i];
+    if (m < 0) continue;
+    if (m < last) return set_error(SFEM_ERR_ARG, "nodes must be numbered mesh by mesh");
+    last = m;
+    for (int d = 0; d < ndof; ++d)
+      if (c->nodeDof[i * ndof + d] >= 0) ptr[m + 1]++;
+  }
+  for (int m = 0; m < nmesh; ++m) ptr[m + 1] += ptr[m];
+  std::vector<double> emu(c->nElem);
+  for (int64_t e = 0; e < c->nElem; ++e) emu[e] = mu[elem_mesh[e]];
+  if (c->batch) batch_destroy(c);
+  Batch* B = new Batch;
+  c->batch = B;
+  B->nmesh = nmesh;
+  B->dof_ptr = ptr;
+  if (!c->d_elem_mu && hipMalloc((void**)&c->d_elem_mu, c->nElem * sizeof(double)) != hipSuccess)
+    return set_error(SFEM_ERR_HIP, "hipMalloc elem mu");
+  HIP_OK(hipMemcpy(c->d_elem_mu, emu.data(), c->nElem * sizeof(double), hipMemcpyHostToDevice));
+  if (balloc(&B->d_dof_ptr, ptr.size()) != hipSuccess || balloc(&B->d_sc, (size_t)nmesh * B_N) != hipSuccess ||
+      balloc(&B->d_norm, nmesh) != hipSuccess || balloc(&B->d_active, nmesh) != hipSuccess ||
+      balloc(&B->d_ndone, 1) != hipSuccess)
+    return set_error(SFEM_ERR_HIP, "hipMalloc batch state");
+  HIP_OK(hipMemcpy(B->d_dof_ptr, ptr.data(), ptr.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  HIP_OK(hipHostMalloc((void**)&B->h_norm, nmesh * sizeof(double), hipHostMallocDefault));
+  HIP_OK(hipHostMalloc((void**)&B->h_ndone, sizeof(int32_t), hipHostMallocDefault));
+  HIP_OK(hipHostMalloc((void**)&B->h_sc, (size_t)nmesh * B_N * sizeof(double), hipHostMallocDefault));
+  return SFEM_OK;
+}
+
+int sfem_batch_use_graph(sfem_ctx* c, int enable) {
+  if (!c || !c->batch) return set_error(SFEM_ERR_STATE, "no batch set");
+  c->batch->use_graph = enable != 0;
+  return SFEM_OK;
+}
+
+int sfem_step_batch(sfem_ctx* c, double timeFact, const sfem_solver_opts* opts, double newton_tol, int max_newton,
+                    int32_t* newton_iters, int32_t* krylov_iters, double* rhs_norm) {
+  if (!c || !opts || !c->batch) return set_error(SFEM_ERR_ARG, "no batch set or null argument");
+  if (opts->pc_type != SFEM_PC_ILU0 || opts->ksp_type != SFEM_KSP_BCGS)
+    return set_error(SFEM_ERR_UNSUPPORTED, "batched steps use BiCGStab + ILU(0)");
+  Batch& B = *c->batch;
+  const int nm = B.nmesh;
+  HIP_OK(hipSetDevice(c->device));
+  int rc;
+  if ((rc = sfem_time_update(c, timeFact))) return rc;
+  std::vector<int32_t> active(nm, 1), its(nm, 0);
+  for (int m = 0; m < nm; ++m) { newton_iters[m] = max_newton; krylov_iters[m] = 0; rhs_norm[m] = 0.0; }
+  for (int iter = 0; iter < max_newton; ++iter) {
+    double gnorm;
+    if ((rc = sfem_assemble(c, iter, iter == 0, &gnorm))) return rc;
+    hipLaunchKernelGGL(k_seg_norm2, dim3(nm), dim3(kTB), 0, c->stream, B.d_dof_ptr, c->d_rhs, B.d_norm);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(B.h_norm, B.d_norm, nm * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    int nact = 0;
+    for (int m = 0; m < nm; ++m) {
+      if (!active[m]) continue;
+      rhs_norm[m] = B.h_norm[m];
+      if (B.h_norm[m] < newton_tol) {  // StabFEM2.cpp:749, per mesh
+        active[m] = 0;
+        newton_iters[m] = iter + 1;
+      } else {
+        ++nact;
+      }
+    }
+    if (nact == 0) break;
+    HIP_OK(hipMemcpyAsync(B.d_active, active.data(), nm * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    int bad = -1;
+    rc = c->precision == SFEM_PREC_MIXED ? batch_solve<float>(c, opts, its.data(), &bad)
+                                         : batch_solve<double>(c, opts, its.data(), &bad);
+    if (rc) return rc;
+    for (int m = 0; m < nm; ++m)
+      if (active[m]) krylov_iters[m] += its[m];
+    if (bad >= 0) return set_error(SFEM_ERR_MAXIT, "batched BiCGStab: a mesh did not converge");
+    if ((rc = sfem_update_solution(c))) return rc;
+  }
+  return SFEM_OK;
+}
+
+}  // extern "C"

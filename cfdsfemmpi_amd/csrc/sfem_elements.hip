@@ -337,6 +337,7 @@ __global__ __launch_bounds__(256) void k_assemble(AsmView v, ElemParams p,
   const bool active = probe ? (slot == 0) : (slot < n);
   const int64_t e = active ? (probe ? probe_e : (int64_t)elems[slot]) : 0;
   const bool doLift = (lift != 0) && !probe;
+  if (v.elem_mu && active) p.mu = v.elem_mu[e];  // batched meshes carry their own viscosity
 
   // P1: nodal data of the element
   if (active && l < NPE) {

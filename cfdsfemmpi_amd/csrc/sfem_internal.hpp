@@ -70,6 +70,7 @@ struct AsmView {
   double* rhs;               // nfree
   double* probeK;            // element probe outputs (NULL in normal runs)
   double* probeF;
+  const double* elem_mu;     // per-element viscosity (batched meshes) or NULL: ElemParams::mu
 };
 
 // Launchers implemented in sfem_elements.hip
@@ -83,6 +84,7 @@ hipError_t launch_assemble(int et, hipStream_t s, const AsmView& v, const ElemPa
 
 // Launchers implemented in sfem_linalg.hip (see there for semantics)
 struct LinAlg;
+struct Batch;
 
 // Event-pair timing of launch groups by kernel class (SFEM_KT_*).
 struct KernelTimer {
@@ -194,10 +196,12 @@ struct sfem_ctx {
   double *d_soln = nullptr, *d_prev = nullptr, *d_prev2 = nullptr, *d_prev3 = nullptr,
          *d_prev4 = nullptr, *d_dot = nullptr, *d_dotPrev = nullptr, *d_cur = nullptr,
          *d_dotCur = nullptr, *d_applied = nullptr;
+  double* d_elem_mu = nullptr;     // per-element viscosity of a batch of meshes, or NULL
   double* d_probeK = nullptr;
   double* d_probeF = nullptr;
 
   sfem::LinAlg* la = nullptr;   // Krylov workspace and preconditioners
+  sfem::Batch* batch = nullptr; // many independent meshes in one context (sfem_set_batch)
   int64_t la_K_version = 0;      // bumped whenever the tangent is re-assembled
 
   sfem::KernelTimer kt;

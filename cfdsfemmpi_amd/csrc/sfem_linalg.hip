@@ -2084,13 +2084,12 @@ int la_prepare(sfem_ctx* ctx, const sfem_solver_opts* o) {
 // Krylov vectors; the FP64 rhs is scaled by 1/||rhs|| into FP32 and the FP32 solution
 // scaled back into the FP64 increment.  With ILU(0) the Krylov solve runs in the plan's
 // padded block layout (rhs in, solution out through the plan's slot map).
-int la_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int* iters, double* relres) {
+// Preconditioner (rebuilt when the tangent or the options changed; the tangent is
+// constant within a time step), Krylov layout, and autotuned kernel shapes for opts.
+int la_prepare_solve(sfem_ctx* ctx, const sfem_solver_opts* o) {
   LinAlg* L = ctx->la;
-  const int64_t n = L->n;
   const bool fp32 = ctx->precision == SFEM_PREC_MIXED;
-  hipStream_t st = ctx->stream;
   int rc;
-  // the tangent is constant within a time step: rebuild the preconditioner only when it changed
   const int reorder = o->pc_type == SFEM_PC_ILU0 ? o->reorder : 0;
   const bool bf16f = fp32 && o->pc_type == SFEM_PC_ILU0 && o->factor_bf16 != 0;
   const bool same_pc = L->pc_K_version >= 0 && L->pc_type == o->pc_type && L->pc_fp32 == fp32 &&
@@ -2107,9 +2106,37 @@ int la_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int* iters, double* relre
     L->kb_version = ctx->la_K_version;
   }
   L->block_space = L->pc_type == SFEM_PC_ILU0;
+  L->nk = L->block_space ? L->ilu->npad : L->n;
+  if (fp32) {
+    if ((rc = ensure_vec(&L->t32, L->nvec)) || (rc = ensure_vec(&L->v32, L->nvec))) return rc;
+    if (needs_tuning(L, 1) && (rc = autotune<float>(ctx, L->t32, L->v32))) return rc;
+  } else {
+    if ((rc = ensure_vec(&L->t, L->nvec)) || (rc = ensure_vec(&L->v, L->nvec))) return rc;
+    if (needs_tuning(L, 0) && (rc = autotune<double>(ctx, L->t, L->v))) return rc;
+  }
+  return SFEM_OK;
+}
+
+int la_apply_pc32(sfem_ctx* ctx, const float* r, float* z) { return pc_apply<float>(ctx, r, z); }
+int la_apply_pc64(sfem_ctx* ctx, const double* r, double* z) { return pc_apply<double>(ctx, r, z); }
+int la_apply_op32(sfem_ctx* ctx, const float* x, float* y) {
+  spmv<float>(ctx, x, y);
+  return hipGetLastError() == hipSuccess ? SFEM_OK : set_error(SFEM_ERR_HIP, "spmv launch");
+}
+int la_apply_op64(sfem_ctx* ctx, const double* x, double* y) {
+  spmv<double>(ctx, x, y);
+  return hipGetLastError() == hipSuccess ? SFEM_OK : set_error(SFEM_ERR_HIP, "spmv launch");
+}
+
+int la_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int* iters, double* relres) {
+  LinAlg* L = ctx->la;
+  const int64_t n = L->n;
+  const bool fp32 = ctx->precision == SFEM_PREC_MIXED;
+  hipStream_t st = ctx->stream;
+  int rc;
+  if ((rc = la_prepare_solve(ctx, o))) return rc;
   const int32_t* padmap = L->block_space ? L->ilu->d_padmap : nullptr;
-  const int64_t nk = L->block_space ? L->ilu->npad : n;
-  L->nk = nk;
+  const int64_t nk = L->nk;
   const int gk = grid_for(nk), gn = grid_for(n);
   const int m = std::max(1, o->restart);
   if (!fp32) {
@@ -2134,7 +2161,6 @@ int la_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int* iters, double* relre
       double** vs[] = {&L->r, &L->rh, &L->p, &L->v, &L->s, &L->t, &L->ph, &L->sh};
       for (double** q : vs)
         if ((rc = ensure_vec(q, L->nvec))) return rc;
-      if (needs_tuning(L, 0) && (rc = autotune<double>(ctx, L->t, L->v))) return rc;
       rc = bicgstab<double>(ctx, b, x, o->rtol, o->max_it, iters, relres, L->r, L->rh, L->p, L->v, L->s,
                             L->t, L->ph, L->sh);
     }
@@ -2148,7 +2174,6 @@ int la_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int* iters, double* relre
   float** vs[] = {&L->r32, &L->rh32, &L->p32, &L->v32, &L->s32, &L->t32, &L->ph32, &L->sh32, &L->x32, &L->du32};
   for (float** q : vs)
     if ((rc = ensure_vec(q, L->nvec))) return rc;
-  if (needs_tuning(L, 1) && (rc = autotune<float>(ctx, L->t32, L->v32))) return rc;
   double bn;
   if ((rc = la_norm2(ctx, ctx->d_rhs, &bn))) return rc;
   if (!(bn > 0.0)) {

@@ -99,6 +99,15 @@ struct LinAlg {
   ~LinAlg();
 };
 
+// Products and preconditioner application inside a solve (sfem_batch.hip drives its own
+// Krylov iteration through these): the current layout and preconditioner of the context.
+int la_apply_pc32(sfem_ctx* ctx, const float* r, float* z);
+int la_apply_pc64(sfem_ctx* ctx, const double* r, double* z);
+int la_apply_op32(sfem_ctx* ctx, const float* x, float* y);
+int la_apply_op64(sfem_ctx* ctx, const double* x, double* y);
+// Bring the preconditioner and the Krylov layout up to date for opts (as la_solve does).
+int la_prepare_solve(sfem_ctx* ctx, const sfem_solver_opts* o);
+
 int la_create(sfem_ctx* ctx);
 void la_destroy(sfem_ctx* ctx);
 int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f, bool refactor = true);
@@ -111,5 +120,8 @@ int la_tuning_levels(sfem_ctx* ctx, int fp32, int* out, int cap, int* nlevels);
 int la_norm2(sfem_ctx* ctx, const double* a, double* out);
 int la_spmv64(sfem_ctx* ctx, const double* x, double* y);
 int la_spmv32(sfem_ctx* ctx, const float* x, float* y);
+
+// Batched meshes (sfem_batch.hip)
+void batch_destroy(sfem_ctx* ctx);
 
 }  // namespace sfem

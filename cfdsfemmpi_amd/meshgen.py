@@ -118,3 +118,36 @@ def ldc3d_tet(n: int, graded: bool = False, p_pin=(0.0, 0.0, 0.0), lid_u: float 
     conn[flip] = conn[flip][:, [0, 2, 1, 3]]
     return Mesh(3, 4, h.coords, conn.astype(np.int32), h.dbc_node, h.dbc_dof, h.dbc_value,
                 h.out_nodes)
+
+
+def c4_batch(count: int = 256, seed: int = 0, n_range=(16, 160), jitter: float = 0.2, re_range=(50.0, 2000.0),
+             lid_u: float = 1.0):
+    """Batch of ``count`` independent unit-square cavities: mesh k has n_k x n_k quads
+    with n_k drawn uniformly from ``n_range`` (inclusive), each quad split into two
+    Tria3, interior nodes jittered by +-``jitter``*h, and Re_k drawn log-uniformly from
+    ``re_range`` (mu_k = 1/Re_k with rho = 1).  All draws come from one generator seeded
+    with ``seed``.
+
+    Returns ``(mesh, elem_mesh, mu, ns, meshes)``: the disjoint union (nodes, elements
+    and boundary conditions mesh by mesh), the mesh of every element, the viscosities,
+    the sizes and the individual meshes.
+    """
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(n_range[0], n_range[1] + 1, size=count)
+    res = np.exp(rng.uniform(np.log(re_range[0]), np.log(re_range[1]), size=count))
+    seeds = rng.integers(0, 2**31 - 1, size=count)
+    meshes = [ldc2d_tria(int(n), jitter=jitter, seed=int(s), lid_u=lid_u) for n, s in zip(ns, seeds)]
+    coords, conn, dn, dd, dv, em = [], [], [], [], [], []
+    off = 0
+    for k, m in enumerate(meshes):
+        coords.append(m.coords)
+        conn.append(m.conn + off)
+        dn.append(m.dbc_node + off)
+        dd.append(m.dbc_dof)
+        dv.append(m.dbc_value)
+        em.append(np.full(m.nElem, k, np.int32))
+        off += m.nNode
+    mesh = Mesh(2, 3, np.concatenate(coords), np.concatenate(conn).astype(np.int32),
+                np.concatenate(dn).astype(np.int32), np.concatenate(dd).astype(np.int32),
+                np.concatenate(dv), np.zeros(0, np.int32))
+    return mesh, np.concatenate(em), 1.0 / res, ns, meshes

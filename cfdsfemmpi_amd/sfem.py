@@ -79,6 +79,9 @@ ABI = [
     ("sfem_kernel_timing", C.c_int, [_vp, C.c_int, _i64p, _f64p]),
     ("sfem_get_ilu", C.c_int, [_vp, _i64p, _i32p, _i32p, _i32p, _f64p]),
     ("sfem_solver_stats", C.c_int, [_vp, _i64p]),
+    ("sfem_set_batch", C.c_int, [_vp, C.c_int32, _i32p, _f64p]),
+    ("sfem_step_batch", C.c_int, [_vp, C.c_double, C.POINTER(SolverOpts), C.c_double, C.c_int, _i32p, _i32p, _f64p]),
+    ("sfem_batch_use_graph", C.c_int, [_vp, C.c_int]),
     ("sfem_block_info", C.c_int, [_vp, _i64p, _i64p, _i64p]),
     ("sfem_tuning", C.c_int, [_vp, C.POINTER(C.c_int)]),
     ("sfem_tuning_levels", C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]),
@@ -272,6 +275,25 @@ class Context:
         a, b, c = C.c_int64(0), C.c_int64(0), C.c_int64(0)
         _check(self.lib.sfem_block_info(self.h, C.byref(a), C.byref(b), C.byref(c)), "sfem_block_info")
         return dict(nblk=a.value, npairs=b.value, nvals=c.value)
+
+    def set_batch(self, elem_mesh, mu):
+        """Declare the mesh as a disjoint union of len(mu) meshes (elem_mesh: mesh of each element)."""
+        em = np.ascontiguousarray(elem_mesh, np.int32)
+        m = np.ascontiguousarray(mu, np.float64)
+        _check(self.lib.sfem_set_batch(self.h, len(m), _p(em, C.c_int32), _p(m, C.c_double)), "sfem_set_batch")
+        self.nmesh = len(m)
+
+    def batch_use_graph(self, enable: bool):
+        _check(self.lib.sfem_batch_use_graph(self.h, 1 if enable else 0), "sfem_batch_use_graph")
+
+    def step_batch(self, timeFact: float, opts: SolverOpts, newton_tol: float = 1e-8, max_newton: int = 10):
+        """One time step of every mesh: (newton_iters, krylov_iters, rhs_norm) per mesh."""
+        ni = np.zeros(self.nmesh, np.int32)
+        ki = np.zeros(self.nmesh, np.int32)
+        rn = np.zeros(self.nmesh)
+        _check(self.lib.sfem_step_batch(self.h, float(timeFact), C.byref(opts), float(newton_tol), int(max_newton),
+                                        _p(ni, C.c_int32), _p(ki, C.c_int32), _p(rn, C.c_double)), "sfem_step_batch")
+        return ni, ki, rn
 
     def fp32_fallbacks(self) -> int:
         """Mixed-precision solves finished by the FP32 GMRES fallback."""

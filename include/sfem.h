@@ -198,6 +198,25 @@ int sfem_kernel_timing(sfem_ctx* ctx, int kind, int64_t* launches, double* total
  * (SolverPetsc.cpp:234-277) and is not exposed; this exists for parity checks. */
 int sfem_get_ilu(sfem_ctx* ctx, int64_t* nnz, int32_t* perm, int32_t* rowptr, int32_t* colidx, double* vals);
 
+/* Batched meshes.  The context's mesh is a disjoint union of nmesh meshes, nodes and
+ * elements numbered mesh by mesh; elem_mesh[e] is the mesh of element e and mu[m] the
+ * viscosity of mesh m (elemData[1] of its control file).  Call after
+ * sfem_build_pattern.  No counterpart in the reference, which runs one mesh per job. */
+int sfem_set_batch(sfem_ctx* ctx, int32_t nmesh, const int32_t* elem_mesh, const double* mu);
+
+/* One time step of every mesh of the batch: the reference's Newton loop
+ * (StabFEM2.cpp:654-799) per mesh, a mesh stopping when ||F_m|| < newton_tol and staying
+ * frozen while the others iterate; each solve is BiCGStab + ILU(0) with per-mesh scalars
+ * and per-mesh convergence (||r_m|| <= rtol ||b_m||), one iteration of all meshes being
+ * one replayed HIP graph.  Per mesh: newton_iters (assemblies), krylov_iters (summed
+ * over the step's solves) and the last ||F_m||. */
+int sfem_step_batch(sfem_ctx* ctx, double timeFact, const sfem_solver_opts* opts, double newton_tol,
+                    int max_newton, int32_t* newton_iters, int32_t* krylov_iters, double* rhs_norm);
+
+/* Replay the batched BiCGStab iteration as a HIP graph (1, default) or launch it kernel
+ * by kernel (0). */
+int sfem_batch_use_graph(sfem_ctx* ctx, int enable);
+
 /* Sizes of the ILU(0) node-block form (0 before an ILU(0) solver is set up): block
  * rows, stored neighbour blocks, and stored values (padded 4-wide blocks, m rows each).
  * No counterpart in the reference. */
