@@ -100,7 +100,7 @@ def run_path(mesh, ctl, prec, opts, newton_tol, steps, warmup, label, predictor=
     log(f"{label}: {steps} steps in {t2 - t1:.3f} s; device ms by class: " +
         ", ".join(f"{k} {v[1]:.1f} ({v[0]} launches)" for k, v in kt.items()))
     res = dict(label=label, seconds=t2 - t1, steps_per_s=steps / (t2 - t1), hist=hist, kt=kt, info=info,
-               soln=s, tune=tune, block=ctx.block_info(), fallbacks=ctx.fp32_fallbacks())
+               soln=s, tune=tune, block=ctx.block_info(), stats=ctx.solver_stats())
     ctx.close()
     return res
 
@@ -147,6 +147,205 @@ def cpu_baseline(n_target, fp64_res):
             "cpu": platform.processor() or platform.machine(), "host_threads": os.cpu_count()}
 
 
+C4_METRIC = "time-steps/s of a 256-cavity Tria3 batch (every mesh advances one step), mixed batch vs FP64 batch"
+C4_DT = 0.05
+TRIA3_FLOPS_KF = 1010    # reference arithmetic per Tria3 element, Ke + Fe (hand count)
+
+
+def c4_case(count, seed):
+    from cfdsfemmpi_amd import meshgen
+    return meshgen.c4_batch(count, seed=seed)
+
+
+def c4_ctx(mesh, em, mu, prec, opts):
+    from cfdsfemmpi_amd import sfem
+    ctx = sfem.Context(mesh, device=0)
+    ctx.set_dirichlet(mesh.dbc_node, mesh.dbc_dof, mesh.dbc_value)
+    ctx.prepare_elements()
+    ctx.build_pattern()
+    ctx.set_params(np.array([1.0, float(np.mean(mu)), 0, 0, 0, 0, 0, 0, 1, 1, 0]), 2, 0.0, C4_DT)
+    ctx.set_precision(prec)
+    if em is not None:
+        ctx.set_batch(em, mu)
+    ctx.setup_solver(opts)
+    return ctx
+
+
+def run_batch(case, prec, opts, newton_tol, steps, warmup, label, graph=True):
+    """Every mesh of the batch advances ``steps`` time steps together (sfem_step_batch)."""
+    from cfdsfemmpi_amd import sfem
+    mesh, em, mu, ns, meshes = case
+    t0 = time.time()
+    ctx = c4_ctx(mesh, em, mu, prec, opts)
+    ctx.batch_use_graph(graph)
+    info = ctx.info()
+    log(f"{label}: setup {time.time() - t0:.1f} s, nfree={info['nfree']} nnz={info['nnz']} colours={info['ncolors']}")
+
+    def advance(k, record):
+        t = C4_DT
+        out = []
+        for s in range(k):
+            ni, ki, rn = ctx.step_batch(min(t, 1.0), opts, newton_tol, 10)
+            out.append((ni.copy(), ki.copy(), rn.copy()))
+            if record:
+                log(f"{label}: step {s + 1}/{k}: newton max {ni.max()} krylov max {ki.max()} "
+                    f"(mean {ki.mean():.1f}) max |F| {rn.max():.3e}")
+            t += C4_DT
+        return out
+
+    if warmup:
+        advance(warmup, False)
+        ctx.reset_state()
+    ctx.enable_kernel_timing(True)
+    ctx.get_solution()
+    t1 = time.perf_counter()
+    hist = advance(steps, True)
+    s = ctx.get_solution()
+    t2 = time.perf_counter()
+    kt = {name: ctx.kernel_timing(k) for name, k in
+          (("assembly", sfem.KT_ASSEMBLY), ("spmv", sfem.KT_SPMV), ("pc", sfem.KT_PC))}
+    log(f"{label}: {steps} steps in {t2 - t1:.3f} s")
+    # one more step issued eagerly, outside the timed region, for per-class device times
+    # (graph replays record no events)
+    kt_eager = None
+    if graph:
+        ctx.batch_use_graph(False)
+        ctx.enable_kernel_timing(True)  # clears the counters
+        ctx.step_batch(min(C4_DT * (steps + 1), 1.0), opts, newton_tol, 10)
+        kt_eager = {name: ctx.kernel_timing(k) for name, k in
+                    (("assembly", sfem.KT_ASSEMBLY), ("spmv", sfem.KT_SPMV), ("pc", sfem.KT_PC))}
+    res = dict(label=label, seconds=t2 - t1, steps_per_s=steps / (t2 - t1), hist=hist, kt=kt, kt_eager=kt_eager,
+               info=info, soln=s, block=ctx.block_info(), tune=ctx.tuning(), stats=ctx.solver_stats())
+    ctx.close()
+    return res
+
+
+def run_sequential_sample(case, prec, opts, newton_tol, steps, stride):
+    """Meshes 0, stride, 2*stride, ... stepped one after another through the single-mesh
+    path (one context each, setup untimed): the rate a caller without batching gets."""
+    from cfdsfemmpi_amd import sfem
+    mesh, em, mu, ns, meshes = case
+    idx = list(range(0, len(meshes), stride))
+    ctxs = [c4_ctx(meshes[k], None, mu[k:k + 1], prec, opts) for k in idx]
+    for c in ctxs:
+        c.get_solution()
+    t1 = time.perf_counter()
+    for c in ctxs:
+        t = C4_DT
+        for s in range(steps):
+            c.step(min(t, 1.0), opts, newton_tol, 10)
+            t += C4_DT
+        c.get_solution()
+    t2 = time.perf_counter()
+    for c in ctxs:
+        c.close()
+    sec_all = (t2 - t1) * len(meshes) / len(idx)
+    return {"meshes_sampled": idx, "seconds_sample": t2 - t1, "steps_per_s_estimate": steps / sec_all}
+
+
+def cpu_baseline_c4(case, stride):
+    """Oracle (single-threaded CPU restatement) on one FP64 time step of meshes 0, stride,
+    ... (BiCGStab + ILU(0) at rtol 1e-8, Newton 1e-8), scaled to the whole batch by count."""
+    import oracle
+    from cfdsfemmpi_amd import deck
+    mesh, em, mu, ns, meshes = case
+    idx = list(range(0, len(meshes), stride))
+    so = deck.SolverOptions("bcgs", "ilu", 1e-8, 5000)
+    t = 0.0
+    for k in idx:
+        ctl = deck.Control(np.array([1.0, mu[k], 0, 0, 0, 0, 0, 0, 1.0, 1.0, 0.0]), 2, 0.0, C4_DT, 1e9, 1, 1, 1e-8)
+        f = oracle.OracleFEM(meshes[k], ctl, so)
+        f.prepare()
+        t0 = time.perf_counter()
+        f.solve_fully_implicit(1e-8, 10, steps=1)
+        t += time.perf_counter() - t0
+    per_step = t * len(meshes) / len(idx)
+    return {"value": 1.0 / per_step, "unit": "time-steps/s", "cores": 1, "kind": "port",
+            "sample": (f"oracle (C++ CPU restatement, 1 thread), first time step of {len(idx)} of the "
+                       f"{len(meshes)} meshes (k % {stride} == 0, n = {[int(ns[k]) for k in idx]}) in "
+                       f"{t:.2f} s, scaled to the batch by mesh count"),
+            "cpu": platform.processor() or platform.machine(), "host_threads": os.cpu_count()}
+
+
+def main_c4(a):
+    from cfdsfemmpi_amd import sfem
+    steps = a.steps if a.steps is not None else 20
+    case = c4_case(a.c4_count, a.c4_seed)
+    mesh, em, mu, ns, meshes = case
+    log(f"workload: {len(ns)} Tria3 cavities, n in [{ns.min()}, {ns.max()}], {mesh.nElem} elements, "
+        f"{mesh.nNode} nodes, Re in [{1 / mu.max():.0f}, {1 / mu.min():.0f}]")
+    o64 = sfem.solver_opts("bcgs", "ilu", 1e-8, 5000, reorder=a.reorder)
+    omx = sfem.solver_opts("bcgs", "ilu", a.inner_rtol, 5000, reorder=a.reorder, factor_bf16=a.bf16_factors)
+    r64 = run_batch(case, sfem.PREC_FP64, o64, a.newton_tol, steps, a.warmup, "fp64-batch")
+    log(f"fp64-batch: {r64['steps_per_s']:.3f} steps/s")
+    rmx = run_batch(case, sfem.PREC_MIXED, omx, a.newton_tol_mixed, steps, a.warmup, "mixed-batch")
+    log(f"mixed-batch: {rmx['steps_per_s']:.3f} steps/s")
+    s64 = r64["soln"].reshape(-1, 3)
+    smx = rmx["soln"].reshape(-1, 3)
+    err_u, err_p, off = [], [], 0
+    for m in meshes:
+        a64, amx = s64[off:off + m.nNode], smx[off:off + m.nNode]
+        err_u.append(float(np.linalg.norm(amx[:, :2] - a64[:, :2]) / np.linalg.norm(a64[:, :2])))
+        err_p.append(float(np.linalg.norm(amx[:, 2] - a64[:, 2]) / max(np.linalg.norm(a64[:, 2]), 1e-300)))
+        off += m.nNode
+    log(f"per-mesh rel L2 mixed vs fp64: u max {max(err_u):.3e}, p max {max(err_p):.3e}")
+    seq = None
+    if a.c4_seq_stride:
+        seq = run_sequential_sample(case, sfem.PREC_MIXED, omx, a.newton_tol_mixed, steps, a.c4_seq_stride)
+        log(f"one mesh at a time (mixed), {len(seq['meshes_sampled'])} meshes in {seq['seconds_sample']:.2f} s: "
+            f"~{seq['steps_per_s_estimate']:.3f} batch-steps/s")
+    kt = rmx["kt_eager"] or rmx["kt"]
+    bi, info = rmx["block"], rmx["info"]
+    nnz, nf = info["nnz"], info["nfree"]
+    cnt, ms = kt["spmv"]
+    sbytes = nnz * 4 + 4 * bi["npairs"] + 16 * bi["nblk"] + 2 * 4 * nf
+    roof = None
+    if cnt:
+        avg = ms / cnt
+        roof = {"bound": "hbm", "kernel": "k_bsr_spmv<float> (node-block SpMV over the whole batch)",
+                "achieved": sbytes / (avg * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "avg_launch_ms": avg, "launches": cnt, "algorithmic_bytes_per_launch": sbytes,
+                "traffic": None, "timed_on": "one eager step after the timed region"}
+        roof["frac"] = roof["achieved"] / roof["peak"]
+    cpu = None if a.no_cpu else cpu_baseline_c4(case, a.c4_cpu_stride)
+    out = {
+        "metric": C4_METRIC,
+        "value": rmx["steps_per_s"],
+        "unit": "time-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * rmx["seconds"] / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 tangent+Krylov" + (" (bf16 ILU factors)" if a.bf16_factors else "") + " / fp64 residual",
+        "data": f"synthetic (generated meshes, seed {a.c4_seed}, zero initial state)",
+        "config": {"workload": f"{len(ns)} jittered unit-square Tria3 cavities, n in [16, 160], Re log-uniform "
+                               f"in [50, 2000], dt={C4_DT}, {steps} steps, one batched solve per step",
+                   "n_elements": mesh.nElem, "n_nodes": mesh.nNode, "nfree": nf, "nnz": nnz,
+                   "seed": a.c4_seed, "solver": "BiCGStab + ILU(0), per-mesh scalars and convergence",
+                   "newton_tol": a.newton_tol, "newton_tol_mixed": a.newton_tol_mixed,
+                   "inner_rtol_mixed": a.inner_rtol, "graph": "one BiCGStab iteration replayed as a HIP graph",
+                   "parallelism": "single GPU"},
+        "fp64_batch_steps_per_s": r64["steps_per_s"],
+        "speedup_mixed_vs_fp64": rmx["steps_per_s"] / r64["steps_per_s"],
+        "error_gate_1e-6_passed": max(err_u) <= 1e-6,
+        "rel_l2_u_mixed_vs_fp64_max_over_meshes": max(err_u),
+        "rel_l2_p_mixed_vs_fp64_max_over_meshes": max(err_p),
+        "fp32_mesh_solves_kept_best_iterate": rmx["stats"][1],
+        "one_mesh_at_a_time_mixed": seq,
+        "speedup_batch_vs_one_at_a_time": (rmx["steps_per_s"] / seq["steps_per_s_estimate"]) if seq else None,
+        "krylov_iters_max": {r["label"]: [int(h[1].max()) for h in r["hist"]] for r in (r64, rmx)},
+        "newton_iters_max": {r["label"]: [int(h[0].max()) for h in r["hist"]] for r in (r64, rmx)},
+        "kernel_ms_eager_step": {r["label"]: {k: {"launches": v[0], "total_ms": v[1]} for k, v in
+                                              (r["kt_eager"] or r["kt"]).items()} for r in (r64, rmx)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def copy_probe(mib):
     """A plain copy kernel of known size: its FETCH_SIZE / WRITE_SIZE readings give the
     factor between those counters and bytes moved."""
@@ -163,9 +362,15 @@ def copy_probe(mib):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (10; 20 for c4)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c3", choices=["c3", "c2"])
+    ap.add_argument("--workload", default="c3", choices=["c3", "c2", "c4"],
+                    help="c3: 115^3 hex cavity (the headline); c2: 32^3; c4: batch of 256 Tria3 cavities")
+    ap.add_argument("--c4-count", type=int, default=256)
+    ap.add_argument("--c4-seed", type=int, default=0)
+    ap.add_argument("--c4-seq-stride", type=int, default=16,
+                    help="c4: time every n-th mesh through the single-mesh path (0: skip)")
+    ap.add_argument("--c4-cpu-stride", type=int, default=32, help="c4: oracle sample of every n-th mesh")
     ap.add_argument("--n", type=int, default=0, help="override the mesh size n (n^3 hexes)")
     ap.add_argument("--inner-rtol", type=float, default=1.4e-4,
                     help="mixed path: FP32 inner solve tolerance; two solves take ||F|| from ~5.7e-3 to "
@@ -196,6 +401,11 @@ def main():
         sys.exit(2)
     from cfdsfemmpi_amd import sfem
     sfem.load_library()
+    if a.workload == "c4":
+        main_c4(a)
+        return
+    if a.steps is None:
+        a.steps = 10
     n = a.n or (115 if a.workload == "c3" else 32)
     mesh, ctl = make_case(n)
     log(f"workload {n}^3 hexes: {mesh.nElem} elements, {mesh.nNode} nodes")
@@ -296,7 +506,8 @@ def main():
         "kernel_ms": {r["label"]: {k: {"launches": v[0], "total_ms": v[1]} for k, v in r["kt"].items()}
                       for r in (r64, rmx) if r},
         "autotuned": {r["label"]: r["tune"] for r in (r64, rmx) if r},
-        "fp32_gmres_fallbacks": rmx["fallbacks"] if rmx else None,
+        "fp32_gmres_fallbacks": rmx["stats"][0] if rmx else None,
+        "fp32_partial_solves": rmx["stats"][1] if rmx else None,
         "block_form": main_res["block"],
         "roofline": roof,
         "roofline_ilu_apply": roof_pc,

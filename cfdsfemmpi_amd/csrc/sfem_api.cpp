@@ -758,9 +758,10 @@ int sfem_block_info(sfem_ctx* c, int64_t* nblk, int64_t* npairs, int64_t* nvals)
   return la_block_info(c, nblk, npairs, nvals);
 }
 
-int sfem_solver_stats(sfem_ctx* c, int64_t* fp32_fallbacks) {
+int sfem_solver_stats(sfem_ctx* c, int64_t* fp32_fallbacks, int64_t* fp32_partial) {
   if (!c || !fp32_fallbacks) return set_error(SFEM_ERR_ARG, "null argument");
   *fp32_fallbacks = c->la ? c->la->fallbacks : 0;
+  if (fp32_partial) *fp32_partial = (c->la ? c->la->partial : 0) + batch_stalled(c);
   return SFEM_OK;
 }
 

@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -30,8 +31,10 @@ constexpr int kChunk = 4096;      // Krylov slots per chunk (at most)
 // per-mesh Krylov scalars
 enum {
   B_RHO0 = 0, B_RHO1, B_ALPHA, B_OMEGA, B_BETA, B_RR, B_BB, B_TOL2, B_RHRH, B_SS,
-  B_FIRST, B_DONE, B_HALF, B_RESTART, B_ITS, B_BAD, B_N = 16
+  B_FIRST, B_DONE, B_HALF, B_RESTART, B_ITS, B_BAD,
+  B_BEST, B_BESTIT, B_SAVED, B_SAVE, B_RESTORE, B_STALL, B_N = 24
 };
+constexpr int kStallWindow = 100;  // FP32: no 10% residual gain in this many iterations
 }  // namespace
 
 struct Batch {
@@ -51,7 +54,8 @@ struct Batch {
   int32_t* d_ndone = nullptr;     // finished meshes of the current solve
   int32_t* h_ndone = nullptr;     // pinned
   double* h_sc = nullptr;         // pinned copy of d_sc
-  void* vec[10] = {};             // r, rh, p, v, s, t, ph, sh, x, b
+  void* vec[11] = {};             // r, rh, p, v, s, t, ph, sh, x, b, best x
+  int64_t stalled = 0;            // FP32 mesh solves ended on a stall (best iterate kept)
   size_t vec_bytes = 0;
   bool use_graph = true;
   ~Batch() {
@@ -70,6 +74,8 @@ void batch_destroy(sfem_ctx* ctx) {
   delete ctx->batch;
   ctx->batch = nullptr;
 }
+
+int64_t batch_stalled(const sfem_ctx* ctx) { return ctx->batch ? ctx->batch->stalled : 0; }
 
 // ---------------------------------------------------------------------------
 // kernels
@@ -127,25 +133,29 @@ __global__ __launch_bounds__(kTB) void k_b_pad_out(const int4* __restrict__ ch, 
 template <typename T>
 __global__ __launch_bounds__(kTB) void k_b_init(const int4* __restrict__ ch, const T* __restrict__ b, T* __restrict__ x,
                                                 T* __restrict__ r, T* __restrict__ rh, T* __restrict__ p,
-                                                T* __restrict__ v, double* __restrict__ part) {
+                                                T* __restrict__ v, T* __restrict__ xb, double* __restrict__ part) {
   __shared__ double sh[kTB / 64];
   const int4 c = ch[blockIdx.x];
   double s = 0.0;
   for (int i = c.x + threadIdx.x; i < c.y; i += kTB) {
     const T bi = b[i];
-    x[i] = (T)0; r[i] = bi; rh[i] = bi; p[i] = (T)0; v[i] = (T)0;
+    x[i] = (T)0; r[i] = bi; rh[i] = bi; p[i] = (T)0; v[i] = (T)0; xb[i] = (T)0;
     s += (double)bi * (double)bi;
   }
   s = block_sum(s, sh);
   if (threadIdx.x == 0) { part[2 * blockIdx.x] = s; part[2 * blockIdx.x + 1] = 0.0; }
 }
 
-// p = r + beta (p - omega v), or p = r on a (re)start
+// p = r + beta (p - omega v), or p = r on a (re)start.  The chunks of a finished mesh
+// return at once here and in k_b_s / k_b_xr: its x and r stay as converged, and its p
+// does not keep growing under a stale beta while other meshes iterate (it would
+// overflow, and x += 0 * inf is NaN).
 template <typename T>
 __global__ __launch_bounds__(kTB) void k_b_p(const int4* __restrict__ ch, const double* __restrict__ sc,
                                              const T* __restrict__ r, T* __restrict__ p, const T* __restrict__ v) {
   const int4 c = ch[blockIdx.x];
   const double* S = sc + (int64_t)c.z * B_N;
+  if (S[B_DONE] != 0.0) return;  // finished meshes keep their vectors
   const bool first = S[B_FIRST] != 0.0;
   const T beta = first ? (T)0 : (T)S[B_BETA], om = first ? (T)0 : (T)S[B_OMEGA];
   for (int i = c.x + threadIdx.x; i < c.y; i += kTB) p[i] = first ? r[i] : r[i] + beta * (p[i] - om * v[i]);
@@ -175,6 +185,10 @@ __global__ __launch_bounds__(kTB) void k_b_s(const int4* __restrict__ ch, const 
                                              double* __restrict__ part) {
   __shared__ double sh[kTB / 64];
   const int4 c = ch[blockIdx.x];
+  if (sc[(int64_t)c.z * B_N + B_DONE] != 0.0) {
+    if (threadIdx.x == 0) { part[2 * blockIdx.x] = 0.0; part[2 * blockIdx.x + 1] = 0.0; }
+    return;
+  }
   const T alpha = (T)sc[(int64_t)c.z * B_N + B_ALPHA];
   double a = 0.0;
   for (int i = c.x + threadIdx.x; i < c.y; i += kTB) {
@@ -195,6 +209,10 @@ __global__ __launch_bounds__(kTB) void k_b_xr(const int4* __restrict__ ch, const
   __shared__ double sh[kTB / 64];
   const int4 c = ch[blockIdx.x];
   const double* S = sc + (int64_t)c.z * B_N;
+  if (S[B_DONE] != 0.0) {
+    if (threadIdx.x == 0) { part[2 * blockIdx.x] = 0.0; part[2 * blockIdx.x + 1] = 0.0; }
+    return;
+  }
   const T alpha = (T)S[B_ALPHA], om = (T)S[B_OMEGA];
   double a = 0.0, b = 0.0;
   for (int i = c.x + threadIdx.x; i < c.y; i += kTB) {
@@ -209,22 +227,30 @@ __global__ __launch_bounds__(kTB) void k_b_xr(const int4* __restrict__ ch, const
   if (threadIdx.x == 0) { part[2 * blockIdx.x] = a; part[2 * blockIdx.x + 1] = b; }
 }
 
-// restart of a broken-down mesh: rh = r
+// end of an iteration, per mesh as flagged: restart of a broken-down mesh (rh = r),
+// saving a new best iterate (xb = x) or, for a stalled FP32 mesh, taking it back (x = xb)
 template <typename T>
-__global__ __launch_bounds__(kTB) void k_b_restart(const int4* __restrict__ ch, const double* __restrict__ sc,
-                                                   const T* __restrict__ r, T* __restrict__ rh) {
+__global__ __launch_bounds__(kTB) void k_b_post(const int4* __restrict__ ch, const double* __restrict__ sc,
+                                                const T* __restrict__ r, T* __restrict__ rh, T* __restrict__ x,
+                                                T* __restrict__ xb) {
   const int4 c = ch[blockIdx.x];
-  if (sc[(int64_t)c.z * B_N + B_RESTART] == 0.0) return;
-  for (int i = c.x + threadIdx.x; i < c.y; i += kTB) rh[i] = r[i];
+  const double* S = sc + (int64_t)c.z * B_N;
+  if (S[B_RESTART] != 0.0)
+    for (int i = c.x + threadIdx.x; i < c.y; i += kTB) rh[i] = r[i];
+  if (S[B_SAVE] != 0.0)
+    for (int i = c.x + threadIdx.x; i < c.y; i += kTB) xb[i] = x[i];
+  if (S[B_RESTORE] != 0.0)
+    for (int i = c.x + threadIdx.x; i < c.y; i += kTB) x[i] = xb[i];
 }
 
-enum { F_INIT = 0, F_ALPHA, F_SS, F_OMEGA, F_RR_RHO };
+enum { F_INIT = 0, F_ALPHA, F_SS, F_OMEGA, F_RR_RHO, F_GIVEUP };
 
 // Per-mesh scalar updates from the chunk partials (summed in chunk order); one thread
 // per mesh, one block.  F_RR_RHO also counts the finished meshes.
 __global__ __launch_bounds__(kTB) void k_b_final(int nmesh, int op, const int32_t* __restrict__ mptr,
                                                  const int32_t* __restrict__ mlist, const double* __restrict__ part,
-                                                 double* __restrict__ sc, double rtol, int32_t* __restrict__ ndone) {
+                                                 double* __restrict__ sc, double rtol, int stall,
+                                                 int32_t* __restrict__ ndone) {
   __shared__ int cnt;
   if (threadIdx.x == 0) cnt = 0;
   __syncthreads();
@@ -243,6 +269,7 @@ __global__ __launch_bounds__(kTB) void k_b_final(int nmesh, int op, const int32_
         S[B_RHO0] = 1.0; S[B_ALPHA] = 1.0; S[B_OMEGA] = 1.0; S[B_FIRST] = 1.0;
         S[B_TOL2] = rtol * rtol * a;
         S[B_DONE] = (a > 0.0) ? 0.0 : 1.0;
+        S[B_BEST] = a; S[B_SAVED] = a;
         break;
       case F_ALPHA:  // a = (rh, v), b = (v, v)
         S[B_RESTART] = 0.0;
@@ -257,19 +284,35 @@ __global__ __launch_bounds__(kTB) void k_b_final(int nmesh, int op, const int32_
         S[B_SS] = a;
         S[B_HALF] = (S[B_DONE] == 0.0 && a <= S[B_TOL2]) ? 1.0 : 0.0;
         break;
+      case F_GIVEUP:  // iteration limit: unfinished meshes take their best iterate back
+        S[B_SAVE] = 0.0;
+        S[B_RESTART] = 0.0;
+        S[B_RESTORE] = S[B_DONE] == 0.0 ? 1.0 : 0.0;
+        if (S[B_DONE] == 0.0) { S[B_STALL] = 1.0; S[B_DONE] = 1.0; }
+        break;
       case F_OMEGA:  // a = (t, t), b = (t, s)
         S[B_OMEGA] = (S[B_DONE] != 0.0 || S[B_HALF] != 0.0 || a == 0.0) ? 0.0 : b / a;
         break;
       case F_RR_RHO: {  // a = (r, r), b = (rh, r)
+        S[B_SAVE] = 0.0;
+        S[B_RESTORE] = 0.0;
         if (S[B_DONE] == 0.0) {
           S[B_ITS] += 1.0;
           S[B_RR] = a;
-          if (!isfinite(a)) {
+          if (a < 0.81 * S[B_BEST]) { S[B_BEST] = a; S[B_BESTIT] = S[B_ITS]; }
+          if (!isfinite(a) && !stall) {
             S[B_BAD] = 1.0;
             S[B_DONE] = 1.0;
           } else if (S[B_HALF] != 0.0 || a <= S[B_TOL2]) {
             S[B_DONE] = 1.0;
+          } else if (!isfinite(a) || (stall && S[B_ITS] - S[B_BESTIT] >= stall)) {
+            // stalled or diverged FP32 solve: keep the best iterate saved so far (the
+            // outer FP64 Newton iteration corrects from there)
+            S[B_STALL] = 1.0;
+            S[B_RESTORE] = 1.0;
+            S[B_DONE] = 1.0;
           } else {
+            if (stall && a < 0.25 * S[B_SAVED]) { S[B_SAVED] = a; S[B_SAVE] = 1.0; }
             const double rho0 = S[B_RHO1], om = S[B_OMEGA];
             if (S[B_RESTART] != 0.0 || !(fabs(b) > 1e-12 * sqrt(S[B_RHRH] * a)) || rho0 == 0.0 || om == 0.0) {
               // restart with the current residual as shadow residual (rh = r follows)
@@ -352,10 +395,11 @@ static int build_chunks(sfem_ctx* ctx, Batch& B) {
 
 // One batched BiCGStab iteration (all launches on the context stream, no host work).
 template <typename T>
-static int bb_iteration(sfem_ctx* ctx, Batch& B, double rtol) {
+static int bb_iteration(sfem_ctx* ctx, Batch& B, double rtol, int stall) {
   hipStream_t st = ctx->stream;
   T** V = (T**)B.vec;
   T *r = V[0], *rh = V[1], *p = V[2], *v = V[3], *s = V[4], *t = V[5], *ph = V[6], *sh = V[7], *x = V[8];
+  T* xb = V[10];
   const dim3 g(B.nchunk), tb(kTB);
   auto pc = [&](const T* a, T* z) { return sizeof(T) == 8 ? la_apply_pc64(ctx, (const double*)a, (double*)z)
                                                           : la_apply_pc32(ctx, (const float*)a, (float*)z); };
@@ -363,7 +407,7 @@ static int bb_iteration(sfem_ctx* ctx, Batch& B, double rtol) {
                                                           : la_apply_op32(ctx, (const float*)a, (float*)y); };
   auto fin = [&](int o) {
     hipLaunchKernelGGL(k_b_final, dim3(1), dim3(kTB), 0, st, B.nmesh, o, B.d_mptr, B.d_mlist, B.d_part, B.d_sc,
-                       rtol, B.d_ndone);
+                       rtol, stall, B.d_ndone);
   };
   int rc;
   hipLaunchKernelGGL((k_b_p<T>), g, tb, 0, st, B.d_chunk, B.d_sc, r, p, v);
@@ -377,7 +421,7 @@ static int bb_iteration(sfem_ctx* ctx, Batch& B, double rtol) {
   fin(F_OMEGA);
   hipLaunchKernelGGL((k_b_xr<T>), g, tb, 0, st, B.d_chunk, B.d_sc, x, ph, sh, r, s, t, rh, B.d_part);
   fin(F_RR_RHO);
-  hipLaunchKernelGGL((k_b_restart<T>), g, tb, 0, st, B.d_chunk, B.d_sc, r, rh);
+  hipLaunchKernelGGL((k_b_post<T>), g, tb, 0, st, B.d_chunk, B.d_sc, r, rh, x, xb);
   BA_CHECK(hipGetLastError());
   return SFEM_OK;
 }
@@ -404,9 +448,12 @@ static int batch_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int32_t* its, i
   const dim3 g(B.nchunk), tb(kTB);
   const IluPlan& P = *L->ilu;
   hipLaunchKernelGGL((k_b_pad_in<T>), g, tb, 0, st, B.d_chunk, P.d_padmap, ctx->d_rhs, B.d_norm, B.d_active, V[9]);
-  hipLaunchKernelGGL((k_b_init<T>), g, tb, 0, st, B.d_chunk, V[9], V[8], V[0], V[1], V[2], V[3], B.d_part);
+  // FP32 solves end a mesh's iteration on a stall and keep its best iterate; FP64
+  // solves iterate to the tolerance or report the mesh
+  const int stall = sizeof(T) == 4 ? kStallWindow : 0;
+  hipLaunchKernelGGL((k_b_init<T>), g, tb, 0, st, B.d_chunk, V[9], V[8], V[0], V[1], V[2], V[3], V[10], B.d_part);
   hipLaunchKernelGGL(k_b_final, dim3(1), dim3(kTB), 0, st, B.nmesh, (int)F_INIT, B.d_mptr, B.d_mlist, B.d_part,
-                     B.d_sc, o->rtol, B.d_ndone);
+                     B.d_sc, o->rtol, stall, B.d_ndone);
   BA_CHECK(hipGetLastError());
   // one iteration as a graph, replayed
   hipGraphExec_t exec = nullptr;
@@ -415,7 +462,7 @@ static int batch_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int32_t* its, i
     ctx->kt.on = false;
     hipGraph_t graph = nullptr;
     BA_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    rc = bb_iteration<T>(ctx, B, o->rtol);
+    rc = bb_iteration<T>(ctx, B, o->rtol, stall);
     hipError_t ec = hipStreamEndCapture(st, &graph);
     ctx->kt.on = kt_on;
     if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
@@ -429,7 +476,7 @@ static int batch_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int32_t* its, i
     if (exec) {
       hipError_t e = hipGraphLaunch(exec, st);
       if (e != hipSuccess) { (void)hipGraphExecDestroy(exec); return set_hip_error(e, "batched iteration launch"); }
-    } else if ((rc = bb_iteration<T>(ctx, B, o->rtol))) {
+    } else if ((rc = bb_iteration<T>(ctx, B, o->rtol, stall))) {
       return rc;
     }
     hipError_t e = hipMemcpyAsync(B.h_ndone, B.d_ndone, sizeof(int32_t), hipMemcpyDeviceToHost, st);
@@ -438,6 +485,12 @@ static int batch_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int32_t* its, i
     if (*B.h_ndone >= B.nmesh) break;
   }
   if (exec) (void)hipGraphExecDestroy(exec);
+  if (stall && *B.h_ndone < B.nmesh) {
+    // FP32 meshes still iterating at the iteration limit also keep their best iterate
+    hipLaunchKernelGGL(k_b_final, dim3(1), dim3(kTB), 0, st, B.nmesh, (int)F_GIVEUP, B.d_mptr, B.d_mlist, B.d_part,
+                       B.d_sc, o->rtol, stall, B.d_ndone);
+    hipLaunchKernelGGL((k_b_post<T>), g, tb, 0, st, B.d_chunk, B.d_sc, V[0], V[1], V[8], V[10]);
+  }
   hipLaunchKernelGGL((k_b_pad_out<T>), g, tb, 0, st, B.d_chunk, P.d_padmap, V[8], B.d_norm, B.d_active, ctx->d_du);
   BA_CHECK(hipGetLastError());
   BA_CHECK(hipMemcpyAsync(B.h_sc, B.d_sc, (size_t)B.nmesh * B_N * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -446,6 +499,7 @@ static int batch_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int32_t* its, i
   for (int m = 0; m < B.nmesh; ++m) {
     const double* S = B.h_sc + (size_t)m * B_N;
     its[m] = (int32_t)S[B_ITS];
+    if (S[B_STALL] != 0.0) ++B.stalled;
     if (S[B_BAD] != 0.0 && *bad_mesh < 0) *bad_mesh = m;
     if (S[B_DONE] == 0.0 && *bad_mesh < 0) *bad_mesh = m;
   }
@@ -550,7 +604,14 @@ int sfem_step_batch(sfem_ctx* c, double timeFact, const sfem_solver_opts* opts, 
     if (rc) return rc;
     for (int m = 0; m < nm; ++m)
       if (active[m]) krylov_iters[m] += its[m];
-    if (bad >= 0) return set_error(SFEM_ERR_MAXIT, "batched BiCGStab: a mesh did not converge");
+    if (bad >= 0) {
+      const double* S = B.h_sc + (size_t)bad * B_N;
+      char msg[200];
+      snprintf(msg, sizeof msg, "batched BiCGStab: mesh %d %s after %d iterations (|r|^2 = %.3e, target %.3e)", bad,
+               S[B_BAD] != 0.0 ? "broke down (non-finite residual)" : "did not converge", its[bad], S[B_RR],
+               S[B_TOL2]);
+      return set_error(SFEM_ERR_MAXIT, msg);
+    }
     if ((rc = sfem_update_solution(c))) return rc;
   }
   return SFEM_OK;

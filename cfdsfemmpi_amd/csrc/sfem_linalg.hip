@@ -2218,6 +2218,13 @@ int la_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int* iters, double* relre
       *iters += used;
     }
   }
+  // Mixed precision is defect correction: an FP32 solve that reached max_it with a
+  // reduced, finite residual still moves the FP64 Newton iterate toward the solution
+  // (whose own test, ||F|| < tol in FP64, decides convergence), so its iterate is used.
+  if (rc == SFEM_ERR_MAXIT && std::isfinite(*relres) && *relres < 1.0) {
+    ++L->partial;
+    rc = SFEM_OK;
+  }
   if (padmap)
     hipLaunchKernelGGL((k_pad_out<float, double>), dim3(gk), dim3(kTPB), 0, st, nk, padmap, L->du32, ctx->d_du, bn);
   else

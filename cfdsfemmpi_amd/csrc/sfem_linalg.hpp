@@ -53,6 +53,7 @@ struct LinAlg {
   std::vector<int> tune_L[2], tune_U[2];
   bool tuned_csr[2] = {false, false};
   int64_t fallbacks = 0;     // FP32 BiCGStab solves finished by the GMRES fallback
+  int64_t partial = 0;       // FP32 solves ended above rtol, best iterate handed to Newton
   int plan_id = 0, tune_plan[2] = {-1, -1};  // ILU plan the per-level shapes were tuned for
   int pc_type = SFEM_PC_NONE;
   bool pc_fp32 = false;
@@ -123,5 +124,7 @@ int la_spmv32(sfem_ctx* ctx, const float* x, float* y);
 
 // Batched meshes (sfem_batch.hip)
 void batch_destroy(sfem_ctx* ctx);
+// FP32 mesh solves of batched steps that stalled and kept their best iterate
+int64_t batch_stalled(const sfem_ctx* ctx);
 
 }  // namespace sfem

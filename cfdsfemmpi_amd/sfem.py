@@ -78,7 +78,7 @@ ABI = [
     ("sfem_enable_kernel_timing", C.c_int, [_vp, C.c_int]),
     ("sfem_kernel_timing", C.c_int, [_vp, C.c_int, _i64p, _f64p]),
     ("sfem_get_ilu", C.c_int, [_vp, _i64p, _i32p, _i32p, _i32p, _f64p]),
-    ("sfem_solver_stats", C.c_int, [_vp, _i64p]),
+    ("sfem_solver_stats", C.c_int, [_vp, _i64p, _i64p]),
     ("sfem_set_batch", C.c_int, [_vp, C.c_int32, _i32p, _f64p]),
     ("sfem_step_batch", C.c_int, [_vp, C.c_double, C.POINTER(SolverOpts), C.c_double, C.c_int, _i32p, _i32p, _f64p]),
     ("sfem_batch_use_graph", C.c_int, [_vp, C.c_int]),
@@ -297,9 +297,13 @@ class Context:
 
     def fp32_fallbacks(self) -> int:
         """Mixed-precision solves finished by the FP32 GMRES fallback."""
-        v = C.c_int64(0)
-        _check(self.lib.sfem_solver_stats(self.h, C.byref(v)), "sfem_solver_stats")
-        return v.value
+        return self.solver_stats()[0]
+
+    def solver_stats(self):
+        """(FP32 GMRES fallbacks, FP32 solves that handed a partial iterate to Newton)."""
+        v, w = C.c_int64(0), C.c_int64(0)
+        _check(self.lib.sfem_solver_stats(self.h, C.byref(v), C.byref(w)), "sfem_solver_stats")
+        return v.value, w.value
 
     def set_pc_lag(self, lag: int):
         """Rebuild the preconditioner on every lag-th new tangent only (1 = always)."""

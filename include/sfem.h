@@ -209,7 +209,10 @@ int sfem_set_batch(sfem_ctx* ctx, int32_t nmesh, const int32_t* elem_mesh, const
  * frozen while the others iterate; each solve is BiCGStab + ILU(0) with per-mesh scalars
  * and per-mesh convergence (||r_m|| <= rtol ||b_m||), one iteration of all meshes being
  * one replayed HIP graph.  Per mesh: newton_iters (assemblies), krylov_iters (summed
- * over the step's solves) and the last ||F_m||. */
+ * over the step's solves) and the last ||F_m||.  In mixed precision a mesh whose FP32
+ * solve stalls (no 10% residual gain in 100 iterations), diverges or reaches max_it
+ * hands its best iterate to the next FP64 Newton iteration (counted by
+ * sfem_solver_stats); in FP64 such a mesh fails the step with SFEM_ERR_MAXIT. */
 int sfem_step_batch(sfem_ctx* ctx, double timeFact, const sfem_solver_opts* opts, double newton_tol,
                     int max_newton, int32_t* newton_iters, int32_t* krylov_iters, double* rhs_norm);
 
@@ -222,10 +225,14 @@ int sfem_batch_use_graph(sfem_ctx* ctx, int enable);
  * No counterpart in the reference. */
 int sfem_block_info(sfem_ctx* ctx, int64_t* nblk, int64_t* npairs, int64_t* nvals);
 
-/* Number of mixed-precision solves in which the FP32 BiCGStab stalled (no 10% gain of
- * its best residual over 100 iterations) and the solve was finished by FP32 GMRES from
- * a zero start.  No counterpart in the reference (PETSc reports a diverged KSP). */
-int sfem_solver_stats(sfem_ctx* ctx, int64_t* fp32_fallbacks);
+/* Mixed-precision solver events since the context was created.  fp32_fallbacks: solves
+ * in which the FP32 BiCGStab stalled (no 10% gain of its best residual over 100
+ * iterations) and which FP32 GMRES finished from a zero start.  fp32_partial (may be
+ * NULL): FP32 solves that ended above rtol (max_it reached with a reduced, finite
+ * residual; in batched steps also a stalled or diverged mesh solve, which keeps its best
+ * iterate) and handed their iterate to the FP64 Newton iteration, whose own test decides
+ * convergence.  No counterpart in the reference (PETSc reports a diverged KSP). */
+int sfem_solver_stats(sfem_ctx* ctx, int64_t* fp32_fallbacks, int64_t* fp32_partial);
 
 /* Kernel shapes chosen by the solver's autotuner, as out[8] = {SpMV lanes per node
  * (block row) G, columns (neighbour blocks) in flight per lane, order flags (bit 0:
