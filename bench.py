@@ -147,7 +147,7 @@ def cpu_baseline(n_target, fp64_res):
             "cpu": platform.processor() or platform.machine(), "host_threads": os.cpu_count()}
 
 
-C4_METRIC = "time-steps/s of a 256-cavity Tria3 batch (every mesh advances one step), mixed batch vs FP64 batch"
+C4_METRIC = "time-steps/s of a 256-cavity Tria3 batch (every mesh advances one step), batched vs one mesh at a time"
 C4_DT = 0.05
 TRIA3_FLOPS_KF = 1010    # reference arithmetic per Tria3 element, Ke + Fe (hand count)
 
@@ -196,25 +196,21 @@ def run_batch(case, prec, opts, newton_tol, steps, warmup, label, graph=True):
     if warmup:
         advance(warmup, False)
         ctx.reset_state()
-    ctx.enable_kernel_timing(True)
     ctx.get_solution()
     t1 = time.perf_counter()
     hist = advance(steps, True)
     s = ctx.get_solution()
     t2 = time.perf_counter()
+    log(f"{label}: {steps} steps in {t2 - t1:.3f} s")
+    # one more step, outside the timed region, issued launch by launch over every mesh
+    # (no graph, no narrowing) for per-class device times of full-size launches
+    ctx.batch_use_graph(False)
+    ctx.batch_narrowing(False)
+    ctx.enable_kernel_timing(True)
+    ctx.step_batch(min(C4_DT * (steps + 1), 1.0), opts, newton_tol, 10)
     kt = {name: ctx.kernel_timing(k) for name, k in
           (("assembly", sfem.KT_ASSEMBLY), ("spmv", sfem.KT_SPMV), ("pc", sfem.KT_PC))}
-    log(f"{label}: {steps} steps in {t2 - t1:.3f} s")
-    # one more step issued eagerly, outside the timed region, for per-class device times
-    # (graph replays record no events)
-    kt_eager = None
-    if graph:
-        ctx.batch_use_graph(False)
-        ctx.enable_kernel_timing(True)  # clears the counters
-        ctx.step_batch(min(C4_DT * (steps + 1), 1.0), opts, newton_tol, 10)
-        kt_eager = {name: ctx.kernel_timing(k) for name, k in
-                    (("assembly", sfem.KT_ASSEMBLY), ("spmv", sfem.KT_SPMV), ("pc", sfem.KT_PC))}
-    res = dict(label=label, seconds=t2 - t1, steps_per_s=steps / (t2 - t1), hist=hist, kt=kt, kt_eager=kt_eager,
+    res = dict(label=label, seconds=t2 - t1, steps_per_s=steps / (t2 - t1), hist=hist, kt=kt,
                info=info, soln=s, block=ctx.block_info(), tune=ctx.tuning(), stats=ctx.solver_stats())
     ctx.close()
     return res
@@ -278,68 +274,96 @@ def main_c4(a):
     omx = sfem.solver_opts("bcgs", "ilu", a.inner_rtol, 5000, reorder=a.reorder, factor_bf16=a.bf16_factors)
     r64 = run_batch(case, sfem.PREC_FP64, o64, a.newton_tol, steps, a.warmup, "fp64-batch")
     log(f"fp64-batch: {r64['steps_per_s']:.3f} steps/s")
-    rmx = run_batch(case, sfem.PREC_MIXED, omx, a.newton_tol_mixed, steps, a.warmup, "mixed-batch")
-    log(f"mixed-batch: {rmx['steps_per_s']:.3f} steps/s")
-    s64 = r64["soln"].reshape(-1, 3)
-    smx = rmx["soln"].reshape(-1, 3)
-    err_u, err_p, off = [], [], 0
-    for m in meshes:
-        a64, amx = s64[off:off + m.nNode], smx[off:off + m.nNode]
-        err_u.append(float(np.linalg.norm(amx[:, :2] - a64[:, :2]) / np.linalg.norm(a64[:, :2])))
-        err_p.append(float(np.linalg.norm(amx[:, 2] - a64[:, 2]) / max(np.linalg.norm(a64[:, 2]), 1e-300)))
-        off += m.nNode
-    log(f"per-mesh rel L2 mixed vs fp64: u max {max(err_u):.3e}, p max {max(err_p):.3e}")
+    node_mesh = np.concatenate([np.full(m.nNode, k) for k, m in enumerate(meshes)])
+
+    def mesh_errors(s):
+        a64, b = r64["soln"].reshape(-1, 3), s.reshape(-1, 3)
+        eu = np.sqrt(np.bincount(node_mesh, ((b[:, :2] - a64[:, :2]) ** 2).sum(1)) /
+                     np.bincount(node_mesh, (a64[:, :2] ** 2).sum(1)))
+        ep = np.sqrt(np.bincount(node_mesh, (b[:, 2] - a64[:, 2]) ** 2) /
+                     np.maximum(np.bincount(node_mesh, a64[:, 2] ** 2), 1e-300))
+        return float(eu.max()), float(ep.max())
+
+    r32 = None
+    if a.c4_store32:
+        o32 = sfem.solver_opts("bcgs", "ilu", a.c4_store32_rtol, 5000, reorder=a.reorder, factor_bf16=a.bf16_factors)
+        r32 = run_batch(case, sfem.PREC_STORE32, o32, a.c4_store32_newton, steps, a.warmup, "store32-batch")
+        r32["err"] = mesh_errors(r32["soln"])
+        log(f"store32-batch: {r32['steps_per_s']:.3f} steps/s, per-mesh rel L2 vs fp64: u max {r32['err'][0]:.3e} "
+            f"p max {r32['err'][1]:.3e}")
+    rmx = err_u = err_p = None
+    if a.c4_mixed:
+        rmx = run_batch(case, sfem.PREC_MIXED, omx, a.newton_tol_mixed, steps, a.warmup, "mixed-batch")
+        log(f"mixed-batch: {rmx['steps_per_s']:.3f} steps/s")
+        err_u, err_p = mesh_errors(rmx["soln"])
+        log(f"per-mesh rel L2 mixed vs fp64: u max {err_u:.3e}, p max {err_p:.3e}")
     seq = None
     if a.c4_seq_stride:
-        seq = run_sequential_sample(case, sfem.PREC_MIXED, omx, a.newton_tol_mixed, steps, a.c4_seq_stride)
-        log(f"one mesh at a time (mixed), {len(seq['meshes_sampled'])} meshes in {seq['seconds_sample']:.2f} s: "
+        seq = run_sequential_sample(case, sfem.PREC_FP64, o64, a.newton_tol, steps, a.c4_seq_stride)
+        log(f"one mesh at a time (fp64), {len(seq['meshes_sampled'])} meshes in {seq['seconds_sample']:.2f} s: "
             f"~{seq['steps_per_s_estimate']:.3f} batch-steps/s")
-    kt = rmx["kt_eager"] or rmx["kt"]
-    bi, info = rmx["block"], rmx["info"]
+    kt = r64["kt"]
+    bi, info = r64["block"], r64["info"]
     nnz, nf = info["nnz"], info["nfree"]
     cnt, ms = kt["spmv"]
-    sbytes = nnz * 4 + 4 * bi["npairs"] + 16 * bi["nblk"] + 2 * 4 * nf
+    sbytes = nnz * 8 + 4 * bi["npairs"] + 16 * bi["nblk"] + 2 * 8 * nf
     roof = None
     if cnt:
         avg = ms / cnt
-        roof = {"bound": "hbm", "kernel": "k_bsr_spmv<float> (node-block SpMV over the whole batch)",
+        roof = {"bound": "hbm", "kernel": "k_bsr_spmv<double> (node-block SpMV over the whole batch)",
                 "achieved": sbytes / (avg * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "avg_launch_ms": avg, "launches": cnt, "algorithmic_bytes_per_launch": sbytes,
-                "traffic": None, "timed_on": "one eager step after the timed region"}
+                "traffic": None, "timed_on": "one step after the timed region, every launch over every mesh"}
         roof["frac"] = roof["achieved"] / roof["peak"]
     cpu = None if a.no_cpu else cpu_baseline_c4(case, a.c4_cpu_stride)
+    # the reported rate: the FP32-stored run when it meets the velocity gate, else FP64
+    best = r32 if (r32 is not None and r32["err"][0] <= 1e-6 and r32["steps_per_s"] > r64["steps_per_s"]) else r64
     out = {
         "metric": C4_METRIC,
-        "value": rmx["steps_per_s"],
+        "value": best["steps_per_s"],
         "unit": "time-steps/s",
         "n_gpus": 1,
         "steps": steps,
         "warmup": a.warmup,
-        "ms_per_step": 1e3 * rmx["seconds"] / steps,
+        "ms_per_step": 1e3 * best["seconds"] / steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 tangent+Krylov" + (" (bf16 ILU factors)" if a.bf16_factors else "") + " / fp64 residual",
+        "dtype": "f64" if best is r64 else "f64 Krylov / f32-stored tangent+ILU" + (" (bf16 factors)" if a.bf16_factors else ""),
         "data": f"synthetic (generated meshes, seed {a.c4_seed}, zero initial state)",
         "config": {"workload": f"{len(ns)} jittered unit-square Tria3 cavities, n in [16, 160], Re log-uniform "
                                f"in [50, 2000], dt={C4_DT}, {steps} steps, one batched solve per step",
                    "n_elements": mesh.nElem, "n_nodes": mesh.nNode, "nfree": nf, "nnz": nnz,
-                   "seed": a.c4_seed, "solver": "BiCGStab + ILU(0), per-mesh scalars and convergence",
-                   "newton_tol": a.newton_tol, "newton_tol_mixed": a.newton_tol_mixed,
-                   "inner_rtol_mixed": a.inner_rtol, "graph": "one BiCGStab iteration replayed as a HIP graph",
-                   "parallelism": "single GPU"},
+                   "seed": a.c4_seed, "reported_run": best["label"],
+                   "solver": "BiCGStab + ILU(0), per-mesh scalars and convergence",
+                   "fp64_run": {"rtol": 1e-8, "newton_tol": a.newton_tol},
+                   "graph": "one BiCGStab iteration replayed as a HIP graph",
+                   "narrowing": "iteration restricted to unfinished meshes", "parallelism": "single GPU"},
         "fp64_batch_steps_per_s": r64["steps_per_s"],
-        "speedup_mixed_vs_fp64": rmx["steps_per_s"] / r64["steps_per_s"],
-        "error_gate_1e-6_passed": max(err_u) <= 1e-6,
-        "rel_l2_u_mixed_vs_fp64_max_over_meshes": max(err_u),
-        "rel_l2_p_mixed_vs_fp64_max_over_meshes": max(err_p),
-        "fp32_mesh_solves_kept_best_iterate": rmx["stats"][1],
-        "one_mesh_at_a_time_mixed": seq,
-        "speedup_batch_vs_one_at_a_time": (rmx["steps_per_s"] / seq["steps_per_s_estimate"]) if seq else None,
-        "krylov_iters_max": {r["label"]: [int(h[1].max()) for h in r["hist"]] for r in (r64, rmx)},
-        "newton_iters_max": {r["label"]: [int(h[0].max()) for h in r["hist"]] for r in (r64, rmx)},
-        "kernel_ms_eager_step": {r["label"]: {k: {"launches": v[0], "total_ms": v[1]} for k, v in
-                                              (r["kt_eager"] or r["kt"]).items()} for r in (r64, rmx)},
+        "one_mesh_at_a_time_fp64": seq,
+        "speedup_fp64_batch_vs_one_at_a_time": (r64["steps_per_s"] / seq["steps_per_s_estimate"]) if seq else None,
+        "store32_batch": None if r32 is None else {
+            "steps_per_s": r32["steps_per_s"], "inner_rtol": a.c4_store32_rtol, "newton_tol": a.c4_store32_newton,
+            "ilu_factors": "bf16" if a.bf16_factors else "fp32",
+            "speedup_vs_fp64_batch": r32["steps_per_s"] / r64["steps_per_s"],
+            "error_gate_1e-6_passed": r32["err"][0] <= 1e-6,
+            "rel_l2_u_vs_fp64_max_over_meshes": r32["err"][0], "rel_l2_p_vs_fp64_max_over_meshes": r32["err"][1],
+            "newton_iters_max": [int(h[0].max()) for h in r32["hist"]],
+            "krylov_iters_max": [int(h[1].max()) for h in r32["hist"]]},
+        "mixed_batch": None if rmx is None else {
+            "steps_per_s": rmx["steps_per_s"], "inner_rtol": a.inner_rtol, "newton_tol": a.newton_tol_mixed,
+            "ilu_factors": "bf16" if a.bf16_factors else "fp32",
+            "speedup_vs_fp64_batch": rmx["steps_per_s"] / r64["steps_per_s"],
+            "error_gate_1e-6_passed": err_u <= 1e-6,
+            "rel_l2_u_vs_fp64_max_over_meshes": err_u, "rel_l2_p_vs_fp64_max_over_meshes": err_p,
+            "fp32_mesh_solves_kept_best_iterate": rmx["stats"][1],
+            "newton_iters_max": [int(h[0].max()) for h in rmx["hist"]],
+            "krylov_iters_max": [int(h[1].max()) for h in rmx["hist"]]},
+        "krylov_iters_max": [int(h[1].max()) for h in r64["hist"]],
+        "krylov_iters_mean": [round(float(h[1].mean()), 1) for h in r64["hist"]],
+        "newton_iters_max": [int(h[0].max()) for h in r64["hist"]],
+        "kernel_ms_full_step": {r["label"]: {k: {"launches": v[0], "total_ms": v[1]} for k, v in r["kt"].items()}
+                                for r in (r64, r32, rmx) if r},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
@@ -371,6 +395,12 @@ def main():
     ap.add_argument("--c4-seq-stride", type=int, default=16,
                     help="c4: time every n-th mesh through the single-mesh path (0: skip)")
     ap.add_argument("--c4-cpu-stride", type=int, default=32, help="c4: oracle sample of every n-th mesh")
+    ap.add_argument("--c4-mixed", type=int, default=0,
+                    help="c4: also run the batch with FP32 Krylov (stalls on these systems; slow)")
+    ap.add_argument("--c4-store32", type=int, default=1,
+                    help="c4: also run the batch with FP64 Krylov over the FP32-stored tangent and ILU(0)")
+    ap.add_argument("--c4-store32-rtol", type=float, default=1e-4, help="c4: inner rtol of the FP32-stored run")
+    ap.add_argument("--c4-store32-newton", type=float, default=1e-8, help="c4: Newton tol of the FP32-stored run")
     ap.add_argument("--n", type=int, default=0, help="override the mesh size n (n^3 hexes)")
     ap.add_argument("--inner-rtol", type=float, default=1.4e-4,
                     help="mixed path: FP32 inner solve tolerance; two solves take ||F|| from ~5.7e-3 to "

@@ -472,7 +472,8 @@ int sfem_set_params(sfem_ctx* c, const double* elemData, int tis, double rhoInf,
 
 int sfem_set_precision(sfem_ctx* c, int precision) {
   if (!c) return set_error(SFEM_ERR_ARG, "null context");
-  if (precision != SFEM_PREC_FP64 && precision != SFEM_PREC_MIXED) return set_error(SFEM_ERR_ARG, "bad precision");
+  if (precision != SFEM_PREC_FP64 && precision != SFEM_PREC_MIXED && precision != SFEM_PREC_STORE32)
+    return set_error(SFEM_ERR_ARG, "bad precision");
   HIP_OK(hipSetDevice(c->device));
   if (precision == SFEM_PREC_MIXED && !c->d_vals32 && (c->status & ST_PATTERN)) {
     int rc = dalloc(&c->d_vals32, c->nnz);

@@ -43,7 +43,14 @@ struct Batch {
   int64_t* d_dof_ptr = nullptr;
   int plan_id = -1;               // ILU plan the chunk lists were built for
   int nchunk = 0;
-  int4* d_chunk = nullptr;        // {slot begin, slot end, mesh, 0}
+  int4* d_chunk = nullptr;        // {slot begin, slot end, mesh, chunk id}
+  int4* d_cchunk = nullptr;       // chunks of the meshes still iterating (capacity nchunk)
+  std::vector<int4> h_chunk;
+  std::vector<int32_t> blk_mesh;  // mesh of every block row (plan position)
+  std::vector<int64_t> mesh_nblk; // block rows of every mesh
+  std::vector<uint8_t> keep;      // scratch: block rows of the next row subset
+  RowSubset sub[2];               // row subsets of a shrinking solve, used alternately
+  int64_t subsets_built = 0;
   int32_t* d_mptr = nullptr;      // chunks of mesh m: d_mlist[d_mptr[m] .. d_mptr[m+1])
   int32_t* d_mlist = nullptr;
   double* d_sc = nullptr;         // [nmesh][B_N]
@@ -58,8 +65,9 @@ struct Batch {
   int64_t stalled = 0;            // FP32 mesh solves ended on a stall (best iterate kept)
   size_t vec_bytes = 0;
   bool use_graph = true;
+  bool narrowing = true;
   ~Batch() {
-    void* ptrs[] = {d_dof_ptr, d_chunk, d_mptr, d_mlist, d_sc, d_part, d_norm, d_active, d_ndone};
+    void* ptrs[] = {d_dof_ptr, d_chunk, d_cchunk, d_mptr, d_mlist, d_sc, d_part, d_norm, d_active, d_ndone};
     for (void* q : ptrs)
       if (q) (void)hipFree(q);
     for (void* q : vec)
@@ -103,7 +111,8 @@ __global__ __launch_bounds__(kTB) void k_seg_norm2(const int64_t* __restrict__ p
   if (threadIdx.x == 0) out[m] = sqrt(s);
 }
 
-// chunk-parallel kernels: block c handles slots [chunk.x, chunk.y) of mesh chunk.z
+// chunk-parallel kernels: block b handles slots [c.x, c.y) of mesh c.z, c = ch[b], and
+// writes its reduction partials to part[2 c.w], c.w being the chunk's id in the full list
 template <typename T>
 __global__ __launch_bounds__(kTB) void k_b_pad_in(const int4* __restrict__ ch, const int32_t* __restrict__ padmap,
                                                   const double* __restrict__ rhs, const double* __restrict__ norm,
@@ -143,7 +152,7 @@ __global__ __launch_bounds__(kTB) void k_b_init(const int4* __restrict__ ch, con
     s += (double)bi * (double)bi;
   }
   s = block_sum(s, sh);
-  if (threadIdx.x == 0) { part[2 * blockIdx.x] = s; part[2 * blockIdx.x + 1] = 0.0; }
+  if (threadIdx.x == 0) { part[2 * c.w] = s; part[2 * c.w + 1] = 0.0; }
 }
 
 // p = r + beta (p - omega v), or p = r on a (re)start.  The chunks of a finished mesh
@@ -175,7 +184,7 @@ __global__ __launch_bounds__(kTB) void k_b_dot2(const int4* __restrict__ ch, con
   }
   s1 = block_sum(s1, sh);
   s2 = block_sum(s2, sh);
-  if (threadIdx.x == 0) { part[2 * blockIdx.x] = s1; part[2 * blockIdx.x + 1] = s2; }
+  if (threadIdx.x == 0) { part[2 * c.w] = s1; part[2 * c.w + 1] = s2; }
 }
 
 // s = r - alpha v ; partial (s, s)
@@ -186,7 +195,7 @@ __global__ __launch_bounds__(kTB) void k_b_s(const int4* __restrict__ ch, const 
   __shared__ double sh[kTB / 64];
   const int4 c = ch[blockIdx.x];
   if (sc[(int64_t)c.z * B_N + B_DONE] != 0.0) {
-    if (threadIdx.x == 0) { part[2 * blockIdx.x] = 0.0; part[2 * blockIdx.x + 1] = 0.0; }
+    if (threadIdx.x == 0) { part[2 * c.w] = 0.0; part[2 * c.w + 1] = 0.0; }
     return;
   }
   const T alpha = (T)sc[(int64_t)c.z * B_N + B_ALPHA];
@@ -197,7 +206,7 @@ __global__ __launch_bounds__(kTB) void k_b_s(const int4* __restrict__ ch, const 
     a += (double)si * (double)si;
   }
   a = block_sum(a, sh);
-  if (threadIdx.x == 0) { part[2 * blockIdx.x] = a; part[2 * blockIdx.x + 1] = 0.0; }
+  if (threadIdx.x == 0) { part[2 * c.w] = a; part[2 * c.w + 1] = 0.0; }
 }
 
 // x += alpha ph + omega sh ; r = s - omega t ; partials (r, r), (rh, r)
@@ -210,7 +219,7 @@ __global__ __launch_bounds__(kTB) void k_b_xr(const int4* __restrict__ ch, const
   const int4 c = ch[blockIdx.x];
   const double* S = sc + (int64_t)c.z * B_N;
   if (S[B_DONE] != 0.0) {
-    if (threadIdx.x == 0) { part[2 * blockIdx.x] = 0.0; part[2 * blockIdx.x + 1] = 0.0; }
+    if (threadIdx.x == 0) { part[2 * c.w] = 0.0; part[2 * c.w + 1] = 0.0; }
     return;
   }
   const T alpha = (T)S[B_ALPHA], om = (T)S[B_OMEGA];
@@ -224,7 +233,7 @@ __global__ __launch_bounds__(kTB) void k_b_xr(const int4* __restrict__ ch, const
   }
   a = block_sum(a, sh);
   b = block_sum(b, sh);
-  if (threadIdx.x == 0) { part[2 * blockIdx.x] = a; part[2 * blockIdx.x + 1] = b; }
+  if (threadIdx.x == 0) { part[2 * c.w] = a; part[2 * c.w + 1] = b; }
 }
 
 // end of an iteration, per mesh as flagged: restart of a broken-down mesh (rh = r),
@@ -366,12 +375,18 @@ static int build_chunks(sfem_ctx* ctx, Batch& B) {
     return (int)(std::upper_bound(B.dof_ptr.begin(), B.dof_ptr.end(), (int64_t)dof) - B.dof_ptr.begin()) - 1;
   };
   std::vector<int4> ch;
+  B.blk_mesh.resize(P.nblk);
+  B.mesh_nblk.assign(B.nmesh, 0);
+  for (int64_t p = 0; p < P.nblk; ++p) {
+    B.blk_mesh[p] = mesh_of(r0[p]);
+    B.mesh_nblk[B.blk_mesh[p]]++;
+  }
   int64_t p = 0;
   while (p < P.nblk) {
-    const int m = mesh_of(r0[p]);
+    const int m = B.blk_mesh[p];
     int64_t q = p + 1;
-    while (q < P.nblk && q - p < kChunk / 4 && mesh_of(r0[q]) == m) ++q;
-    ch.push_back(int4{(int)(4 * p), (int)(4 * q), m, 0});
+    while (q < P.nblk && q - p < kChunk / 4 && B.blk_mesh[q] == m) ++q;
+    ch.push_back(int4{(int)(4 * p), (int)(4 * q), m, (int)ch.size()});
     p = q;
   }
   std::vector<int32_t> mptr(B.nmesh + 1, 0), mlist(ch.size());
@@ -379,49 +394,53 @@ static int build_chunks(sfem_ctx* ctx, Batch& B) {
   for (int m = 0; m < B.nmesh; ++m) mptr[m + 1] += mptr[m];
   std::vector<int32_t> pos(mptr.begin(), mptr.end() - 1);
   for (size_t k = 0; k < ch.size(); ++k) mlist[pos[ch[k].z]++] = (int32_t)k;
-  for (void* q : {(void*)B.d_chunk, (void*)B.d_mptr, (void*)B.d_mlist, (void*)B.d_part})
+  for (void* q : {(void*)B.d_chunk, (void*)B.d_cchunk, (void*)B.d_mptr, (void*)B.d_mlist, (void*)B.d_part})
     if (q) (void)hipFree(q);
   B.nchunk = (int)ch.size();
+  B.keep.assign(P.nblk, 0);
   BA_CHECK(balloc(&B.d_chunk, ch.size()));
+  BA_CHECK(balloc(&B.d_cchunk, ch.size()));
   BA_CHECK(balloc(&B.d_mptr, mptr.size()));
   BA_CHECK(balloc(&B.d_mlist, mlist.size()));
   BA_CHECK(balloc(&B.d_part, 2 * ch.size()));
   BA_CHECK(hipMemcpy(B.d_chunk, ch.data(), ch.size() * sizeof(int4), hipMemcpyHostToDevice));
   BA_CHECK(hipMemcpy(B.d_mptr, mptr.data(), mptr.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   BA_CHECK(hipMemcpy(B.d_mlist, mlist.data(), mlist.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  B.h_chunk.swap(ch);
   B.plan_id = L->plan_id;
   return SFEM_OK;
 }
 
 // One batched BiCGStab iteration (all launches on the context stream, no host work).
 template <typename T>
-static int bb_iteration(sfem_ctx* ctx, Batch& B, double rtol, int stall) {
+static int bb_iteration(sfem_ctx* ctx, Batch& B, double rtol, int stall, const int4* ch, int nch,
+                        const RowSubset* sub) {
   hipStream_t st = ctx->stream;
   T** V = (T**)B.vec;
   T *r = V[0], *rh = V[1], *p = V[2], *v = V[3], *s = V[4], *t = V[5], *ph = V[6], *sh = V[7], *x = V[8];
   T* xb = V[10];
-  const dim3 g(B.nchunk), tb(kTB);
-  auto pc = [&](const T* a, T* z) { return sizeof(T) == 8 ? la_apply_pc64(ctx, (const double*)a, (double*)z)
-                                                          : la_apply_pc32(ctx, (const float*)a, (float*)z); };
-  auto op = [&](const T* a, T* y) { return sizeof(T) == 8 ? la_apply_op64(ctx, (const double*)a, (double*)y)
-                                                          : la_apply_op32(ctx, (const float*)a, (float*)y); };
+  const dim3 g(nch), tb(kTB);
+  auto pc = [&](const T* a, T* z) { return sizeof(T) == 8 ? la_apply_pc64(ctx, (const double*)a, (double*)z, sub)
+                                                          : la_apply_pc32(ctx, (const float*)a, (float*)z, sub); };
+  auto op = [&](const T* a, T* y) { return sizeof(T) == 8 ? la_apply_op64(ctx, (const double*)a, (double*)y, sub)
+                                                          : la_apply_op32(ctx, (const float*)a, (float*)y, sub); };
   auto fin = [&](int o) {
     hipLaunchKernelGGL(k_b_final, dim3(1), dim3(kTB), 0, st, B.nmesh, o, B.d_mptr, B.d_mlist, B.d_part, B.d_sc,
                        rtol, stall, B.d_ndone);
   };
   int rc;
-  hipLaunchKernelGGL((k_b_p<T>), g, tb, 0, st, B.d_chunk, B.d_sc, r, p, v);
+  hipLaunchKernelGGL((k_b_p<T>), g, tb, 0, st, ch, B.d_sc, r, p, v);
   if ((rc = pc(p, ph)) || (rc = op(ph, v))) return rc;
-  hipLaunchKernelGGL((k_b_dot2<T>), g, tb, 0, st, B.d_chunk, rh, v, v, v, B.d_part);
+  hipLaunchKernelGGL((k_b_dot2<T>), g, tb, 0, st, ch, rh, v, v, v, B.d_part);
   fin(F_ALPHA);
-  hipLaunchKernelGGL((k_b_s<T>), g, tb, 0, st, B.d_chunk, B.d_sc, r, v, s, B.d_part);
+  hipLaunchKernelGGL((k_b_s<T>), g, tb, 0, st, ch, B.d_sc, r, v, s, B.d_part);
   fin(F_SS);
   if ((rc = pc(s, sh)) || (rc = op(sh, t))) return rc;
-  hipLaunchKernelGGL((k_b_dot2<T>), g, tb, 0, st, B.d_chunk, t, t, t, s, B.d_part);
+  hipLaunchKernelGGL((k_b_dot2<T>), g, tb, 0, st, ch, t, t, t, s, B.d_part);
   fin(F_OMEGA);
-  hipLaunchKernelGGL((k_b_xr<T>), g, tb, 0, st, B.d_chunk, B.d_sc, x, ph, sh, r, s, t, rh, B.d_part);
+  hipLaunchKernelGGL((k_b_xr<T>), g, tb, 0, st, ch, B.d_sc, x, ph, sh, r, s, t, rh, B.d_part);
   fin(F_RR_RHO);
-  hipLaunchKernelGGL((k_b_post<T>), g, tb, 0, st, B.d_chunk, B.d_sc, r, rh, x, xb);
+  hipLaunchKernelGGL((k_b_post<T>), g, tb, 0, st, ch, B.d_sc, r, rh, x, xb);
   BA_CHECK(hipGetLastError());
   return SFEM_OK;
 }
@@ -455,34 +474,79 @@ static int batch_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int32_t* its, i
   hipLaunchKernelGGL(k_b_final, dim3(1), dim3(kTB), 0, st, B.nmesh, (int)F_INIT, B.d_mptr, B.d_mlist, B.d_part,
                      B.d_sc, o->rtol, stall, B.d_ndone);
   BA_CHECK(hipGetLastError());
-  // one iteration as a graph, replayed
+  // One iteration as a graph, replayed.  While meshes finish, the iteration is narrowed
+  // to the chunks and block rows of those still iterating (re-captured each time the
+  // remaining block rows fall below half of those the iteration covers).
+  const int4* ch = B.d_chunk;
+  int nch = B.nchunk;
+  const RowSubset* sub = nullptr;
+  int64_t cover = P.nblk;
   hipGraphExec_t exec = nullptr;
   const bool kt_on = ctx->kt.on;
-  if (B.use_graph) {
+  auto capture = [&]() -> int {
+    if (exec) { (void)hipGraphExecDestroy(exec); exec = nullptr; }
+    if (!B.use_graph) return SFEM_OK;
     ctx->kt.on = false;
     hipGraph_t graph = nullptr;
     BA_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    rc = bb_iteration<T>(ctx, B, o->rtol, stall);
+    const int rc1 = bb_iteration<T>(ctx, B, o->rtol, stall, ch, nch, sub);
     hipError_t ec = hipStreamEndCapture(st, &graph);
     ctx->kt.on = kt_on;
-    if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+    if (rc1) { if (graph) (void)hipGraphDestroy(graph); return rc1; }
     if (ec != hipSuccess) return set_hip_error(ec, "batched iteration capture");
     ec = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
-    if (ec != hipSuccess) return set_hip_error(ec, "batched iteration instantiate");
-  }
-  int it = 0;
+    if (ec != hipSuccess) { exec = nullptr; return set_hip_error(ec, "batched iteration instantiate"); }
+    return SFEM_OK;
+  };
+  // narrow to the unfinished meshes (per-mesh flags in h_sc)
+  auto narrow = [&]() -> int {
+    std::vector<uint8_t> live(B.nmesh);
+    int64_t left = 0;
+    for (int m = 0; m < B.nmesh; ++m) {
+      live[m] = B.h_sc[(size_t)m * B_N + B_DONE] == 0.0;
+      if (live[m]) left += B.mesh_nblk[m];
+    }
+    if (!B.narrowing || 2 * left >= cover) return SFEM_OK;
+    RowSubset* nx = sub == &B.sub[0] ? &B.sub[1] : &B.sub[0];
+    if (sub) {
+      for (int32_t q : sub->h_rows) B.keep[q] = live[B.blk_mesh[q]];
+    } else {
+      for (int64_t q = 0; q < P.nblk; ++q) B.keep[q] = live[B.blk_mesh[q]];
+    }
+    int rc1 = la_subset_build(ctx, B.keep.data(), nx, sub);
+    if (rc1) return rc1;
+    std::vector<int4> cc;
+    for (const int4& c : B.h_chunk)
+      if (live[c.z]) cc.push_back(c);
+    if (!cc.empty()) BA_CHECK(hipMemcpy(B.d_cchunk, cc.data(), cc.size() * sizeof(int4), hipMemcpyHostToDevice));
+    ch = B.d_cchunk;
+    nch = (int)cc.size();
+    sub = nx;
+    cover = left;
+    ++B.subsets_built;
+    return capture();
+  };
+  if ((rc = capture())) return rc;
+  int it = 0, seen = -1;
   for (it = 1; it <= o->max_it; ++it) {
     if (exec) {
       hipError_t e = hipGraphLaunch(exec, st);
       if (e != hipSuccess) { (void)hipGraphExecDestroy(exec); return set_hip_error(e, "batched iteration launch"); }
-    } else if ((rc = bb_iteration<T>(ctx, B, o->rtol, stall))) {
+    } else if ((rc = bb_iteration<T>(ctx, B, o->rtol, stall, ch, nch, sub))) {
       return rc;
     }
     hipError_t e = hipMemcpyAsync(B.h_ndone, B.d_ndone, sizeof(int32_t), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) { if (exec) (void)hipGraphExecDestroy(exec); return set_hip_error(e, "batched iteration"); }
     if (*B.h_ndone >= B.nmesh) break;
+    if (*B.h_ndone != seen && nch > 0) {
+      seen = *B.h_ndone;
+      e = hipMemcpyAsync(B.h_sc, B.d_sc, (size_t)B.nmesh * B_N * sizeof(double), hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess) e = hipStreamSynchronize(st);
+      if (e != hipSuccess) { if (exec) (void)hipGraphExecDestroy(exec); return set_hip_error(e, "batched flags"); }
+      if ((rc = narrow())) { if (exec) (void)hipGraphExecDestroy(exec); return rc; }
+    }
   }
   if (exec) (void)hipGraphExecDestroy(exec);
   if (stall && *B.h_ndone < B.nmesh) {
@@ -563,6 +627,12 @@ int sfem_set_batch(sfem_ctx* c, int32_t nmesh, const int32_t* elem_mesh, const d
 int sfem_batch_use_graph(sfem_ctx* c, int enable) {
   if (!c || !c->batch) return set_error(SFEM_ERR_STATE, "no batch set");
   c->batch->use_graph = enable != 0;
+  return SFEM_OK;
+}
+
+int sfem_batch_narrowing(sfem_ctx* c, int enable) {
+  if (!c || !c->batch) return set_error(SFEM_ERR_STATE, "no batch set");
+  c->batch->narrowing = enable != 0;
   return SFEM_OK;
 }
 

@@ -601,7 +601,7 @@ __global__ __launch_bounds__(256) void k_bsr_spmv(int64_t nblk, const BDesc* __r
                                                   const int32_t* __restrict__ bcol, const VT* __restrict__ V,
                                                   const XT* __restrict__ x, XT* __restrict__ y, int xcd,
                                                   const int32_t* __restrict__ rows) {
-  using AT = typename std::conditional<sizeof(VT) == 8, double, float>::type;
+  using AT = typename std::conditional<(sizeof(VT) == 8 || sizeof(XT) == 8), double, float>::type;
   const int64_t gt = logical_wg(xcd) * 256 + threadIdx.x;
   const int64_t g = gt / G;
   const int lane = (int)(gt % G);
@@ -618,8 +618,10 @@ __global__ __launch_bounds__(256) void k_bsr_spmv(int64_t nblk, const BDesc* __r
   if (g < nblk && lane < 4) y[4 * p + lane] = lane < (d.dm >> 16) ? (XT)pick4(acc, lane) : (XT)0;
 }
 
-template <typename LT> struct AccOf { using type = float; };
-template <> struct AccOf<double> { using type = double; };
+// Accumulation type of the triangular solves: FP64 when the vectors or the factor are
+// FP64 (an FP32- or BF16-stored factor applied to FP64 vectors accumulates in FP64).
+template <typename T, typename LT>
+using TriAcc = typename std::conditional<(sizeof(T) == 8 || sizeof(LT) == 8), double, float>::type;
 
 // The diagonal block of block row d, row `lane` loaded by lane < m (before the long
 // sparse product) and broadcast to the group.
@@ -641,7 +643,7 @@ __global__ __launch_bounds__(256) void k_bsr_lsolve(const int32_t* __restrict__ 
                                                     const int32_t* __restrict__ bcol, const LT* __restrict__ lu,
                                                     const T* __restrict__ r, T* __restrict__ y, int xcd,
                                                     int fuse_u) {
-  using AT = typename AccOf<LT>::type;
+  using AT = TriAcc<T, LT>;
   const int64_t gt = logical_wg(xcd) * 256 + threadIdx.x;
   const int64_t g = gt / G;
   const int lane = (int)(gt % G);
@@ -694,7 +696,7 @@ __global__ __launch_bounds__(256) void k_bsr_usolve(const int32_t* __restrict__ 
                                                     const BDesc* __restrict__ desc,
                                                     const int32_t* __restrict__ bcol, const LT* __restrict__ lu,
                                                     T* __restrict__ y, int xcd) {
-  using AT = typename AccOf<LT>::type;
+  using AT = TriAcc<T, LT>;
   const int64_t gt = logical_wg(xcd) * 256 + threadIdx.x;
   const int64_t g = gt / G;
   const int lane = (int)(gt % G);
@@ -1358,12 +1360,14 @@ static void level_range(const IluPlan& P, bool upper, int lv, const int32_t** li
 // Build the preconditioner from the current matrix (FP64 vals or FP32 vals32).
 // refactor = false (ILU(0) only): refresh the node-block tangent the Krylov products use
 // and keep the existing factor (a lagged preconditioner).
-int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f, bool refactor) {
+int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f, bool refactor, bool store32) {
   LinAlg* L = ctx->la;
   hipStream_t st = ctx->stream;
   const int64_t n = L->n;
+  if (store32 && (fp32 || pc_type != SFEM_PC_ILU0)) return set_error(SFEM_ERR_ARG, "FP32 storage needs FP64 ILU(0)");
   L->pc_type = pc_type;
   L->pc_fp32 = fp32;
+  L->pc_store32 = false;
   const double* vals = fp32 ? nullptr : ctx->d_vals;
   const float* vals32 = fp32 ? ctx->d_vals32 : nullptr;
   LA_CHECK(hipMemsetAsync(L->d_scal + S_MISC1, 0, sizeof(double), st));
@@ -1417,7 +1421,19 @@ int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f, 
       hipLaunchKernelGGL((k_bsr_gather<double>), dim3(gb), dim3(64), 0, st, P.nblk, P.d_bdesc, P.d_bcol,
                          P.d_bcolofs, P.d_r0orig, ctx->d_rowptr32, ctx->d_vals, L->kp, refactor ? L->lu : nullptr);
     }
+    // FP32 storage: the block tangent (and below, the FP64-computed factor) rounded to FP32
+    auto to32 = [&](const double* a, float** b) -> int {
+      int rc1 = ensure_vec(b, nv);
+      if (rc1) return rc1;
+      hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(nv)), dim3(kTPB), 0, st, nv, a, *b, 1.0);
+      return SFEM_OK;
+    };
+    if (store32) {
+      int rc = to32(L->kp, &L->kp32);
+      if (rc) return rc;
+    }
     if (!refactor) {
+      L->pc_store32 = store32;
       LA_CHECK(hipGetLastError());
       return SFEM_OK;
     }
@@ -1434,7 +1450,13 @@ int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f, 
         hipLaunchKernelGGL((k_bsr_factor<double>), dim3(cnt), dim3(64), bsr_factor_lds<double>(cap), st, list,
                            first, cnt, P.d_bdesc, P.d_bcol, L->lu, L->d_scal, cap);
     }
-    L->pc_bf16 = fp32 && bf16f;
+    if (store32) {
+      int rc = to32(L->lu, &L->lu32);
+      if (rc) return rc;
+      L->pc_fp32 = true;  // the factor the sweeps read is the FP32 copy
+      L->pc_store32 = true;
+    }
+    L->pc_bf16 = (fp32 || store32) && bf16f;
     if (L->pc_bf16) {
       int rc = ensure_vec(&L->lu16, nv); if (rc) return rc;
       const unsigned gz = (unsigned)std::min<int64_t>((nv + 255) / 256, 1 << 20);
@@ -1446,12 +1468,12 @@ int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f, 
 }
 
 template <typename T>
-static int pc_apply_impl(sfem_ctx* ctx, const T* r, T* z);
+static int pc_apply_impl(sfem_ctx* ctx, const T* r, T* z, const RowSubset* sub);
 
 template <typename T>
-static int pc_apply(sfem_ctx* ctx, const T* r, T* z) {
+static int pc_apply(sfem_ctx* ctx, const T* r, T* z, const RowSubset* sub = nullptr) {
   ctx->kt.begin(SFEM_KT_PC, ctx->stream);
-  const int rc = pc_apply_impl<T>(ctx, r, z);
+  const int rc = pc_apply_impl<T>(ctx, r, z, sub);
   ctx->kt.end(SFEM_KT_PC, ctx->stream);
   return rc;
 }
@@ -1497,7 +1519,7 @@ static void tri_launch(hipStream_t st, int code, const int32_t* list, int first,
 // backward levels.
 template <typename T, typename LT>
 static void ilu_solve(sfem_ctx* ctx, const IluPlan& P, const LT* lu, const T* r, T* z, int force = -1,
-                      hipEvent_t* ev = nullptr) {
+                      hipEvent_t* ev = nullptr, const RowSubset* sub = nullptr) {
   LinAlg* L = ctx->la;
   hipStream_t st = ctx->stream;
   const int slot = sizeof(T) == 8 ? 0 : 1;
@@ -1511,7 +1533,8 @@ static void ilu_solve(sfem_ctx* ctx, const IluPlan& P, const LT* lu, const T* r,
   const int fuse = P.fuse_last ? 1 : 0;
   for (int lv = 0; lv < nl; ++lv) {
     const int32_t* list; int first, cnt;
-    level_range(P, false, lv, &list, &first, &cnt);
+    if (sub) { list = sub->d_lev; first = sub->lofs[lv]; cnt = sub->lofs[lv + 1] - first; }
+    else level_range(P, false, lv, &list, &first, &cnt);
     const int code = force >= 0 ? force : lv < (int)tl.size() ? tl[lv] : kBlkDefault;
     if (cnt) tri_launch<T, LT, false>(st, code, list, first, cnt, P, lu, r, z, lv == nl - 1 ? fuse : 0);
     if (ev) (void)hipEventRecord(ev[1 + lv], st);
@@ -1519,7 +1542,8 @@ static void ilu_solve(sfem_ctx* ctx, const IluPlan& P, const LT* lu, const T* r,
   if (ev && fuse) (void)hipEventRecord(ev[1 + nl], st);
   for (int lv = fuse; lv < nu; ++lv) {
     const int32_t* list; int first, cnt;
-    level_range(P, true, lv, &list, &first, &cnt);
+    if (sub) { list = sub->d_lev; first = sub->uofs[lv]; cnt = sub->uofs[lv + 1] - first; }
+    else level_range(P, true, lv, &list, &first, &cnt);
     const int code = force >= 0 ? force : lv < (int)tu.size() ? tu[lv] : kBlkDefault;
     if (cnt) tri_launch<T, LT, true>(st, code, list, first, cnt, P, lu, z, z);
     if (ev) (void)hipEventRecord(ev[1 + nl + lv], st);
@@ -1528,16 +1552,17 @@ static void ilu_solve(sfem_ctx* ctx, const IluPlan& P, const LT* lu, const T* r,
 
 // ILU(0) apply with the factor type of the current preconditioner.
 template <typename T>
-static void ilu_apply(sfem_ctx* ctx, const T* r, T* z, int force = -1, hipEvent_t* ev = nullptr) {
+static void ilu_apply(sfem_ctx* ctx, const T* r, T* z, int force = -1, hipEvent_t* ev = nullptr,
+                      const RowSubset* sub = nullptr) {
   LinAlg* L = ctx->la;
   const IluPlan& P = *L->ilu;
-  if (L->pc_bf16) ilu_solve<T, bf16>(ctx, P, L->lu16, r, z, force, ev);
-  else if (L->pc_fp32) ilu_solve<T, float>(ctx, P, L->lu32, r, z, force, ev);
-  else ilu_solve<T, double>(ctx, P, L->lu, r, z, force, ev);
+  if (L->pc_bf16) ilu_solve<T, bf16>(ctx, P, L->lu16, r, z, force, ev, sub);
+  else if (L->pc_fp32) ilu_solve<T, float>(ctx, P, L->lu32, r, z, force, ev, sub);
+  else ilu_solve<T, double>(ctx, P, L->lu, r, z, force, ev, sub);
 }
 
 template <typename T>
-static int pc_apply_impl(sfem_ctx* ctx, const T* r, T* z) {
+static int pc_apply_impl(sfem_ctx* ctx, const T* r, T* z, const RowSubset* sub) {
   LinAlg* L = ctx->la;
   hipStream_t st = ctx->stream;
   const int64_t n = L->n;
@@ -1555,7 +1580,7 @@ static int pc_apply_impl(sfem_ctx* ctx, const T* r, T* z) {
       break;
     }
     case SFEM_PC_ILU0:
-      ilu_apply<T>(ctx, r, z);
+      ilu_apply<T>(ctx, r, z, -1, nullptr, sub);
       break;
     default:
       hipLaunchKernelGGL((k_copy<T>), dim3(g), dim3(kTPB), 0, st, n, r, z);
@@ -1590,20 +1615,23 @@ static void spmv_launch(hipStream_t st, int code, int64_t nb, const BlkDesc* des
 
 // Block SpMV shapes: code = vi | (xcd << 5) | (spatial row order << 6).
 template <typename VT, typename T>
-static void bsr_spmv_launch(hipStream_t st, int code, const IluPlan& P, const VT* vals, const T* x, T* y) {
+static void bsr_spmv_launch(hipStream_t st, int code, const IluPlan& P, const VT* vals, const T* x, T* y,
+                            const RowSubset* sub = nullptr) {
   const int vi = code & 31, xcd = (code >> 5) & 1;
-  const int32_t* rows = (code >> 6) & 1 ? P.d_natrows : nullptr;
-  const unsigned g = (unsigned)((P.nblk * kBlkG[vi] + 255) / 256);
+  const int32_t* rows = sub ? sub->d_rows : (code >> 6) & 1 ? P.d_natrows : nullptr;
+  const int64_t nr = sub ? sub->nrows : P.nblk;
+  if (nr == 0) return;
+  const unsigned g = (unsigned)((nr * kBlkG[vi] + 255) / 256);
 #define SFEM_BSPMV(i, G, K)                                                                                  \
   case i:                                                                                                    \
-    hipLaunchKernelGGL((k_bsr_spmv<VT, T, G, K>), dim3(g), dim3(256), 0, st, P.nblk, P.d_bdesc, P.d_bcol, vals, \
+    hipLaunchKernelGGL((k_bsr_spmv<VT, T, G, K>), dim3(g), dim3(256), 0, st, nr, P.d_bdesc, P.d_bcol, vals,     \
                        x, y, xcd, rows);                                                                     \
     break;
   switch (vi) {
     SFEM_BSPMV(0, 4, 1) SFEM_BSPMV(1, 4, 2) SFEM_BSPMV(2, 8, 1) SFEM_BSPMV(3, 8, 2)
     SFEM_BSPMV(4, 16, 1) SFEM_BSPMV(5, 16, 2) SFEM_BSPMV(6, 32, 1)
     default:
-      hipLaunchKernelGGL((k_bsr_spmv<VT, T, 64, 1>), dim3(g), dim3(256), 0, st, P.nblk, P.d_bdesc, P.d_bcol, vals,
+      hipLaunchKernelGGL((k_bsr_spmv<VT, T, 64, 1>), dim3(g), dim3(256), 0, st, nr, P.d_bdesc, P.d_bcol, vals,
                          x, y, xcd, rows);
       break;
   }
@@ -1613,15 +1641,15 @@ static void bsr_spmv_launch(hipStream_t st, int code, const IluPlan& P, const VT
 // y = K x in the DOF layout (block = false, CSR) or in the ILU(0) plan's padded block
 // layout (block = true).
 template <typename T>
-static void spmv_in(sfem_ctx* ctx, const T* x, T* y, bool block) {
+static void spmv_in(sfem_ctx* ctx, const T* x, T* y, bool block, const RowSubset* sub = nullptr) {
   LinAlg* L = ctx->la;
   const int slot = sizeof(T) == 8 ? 0 : 1;
   if (L->nblk == 0) return;
   ctx->kt.begin(SFEM_KT_SPMV, ctx->stream);
   if (block) {
     const int code = L->tune_bspmv[slot];
-    if (sizeof(T) == 8) bsr_spmv_launch<double, T>(ctx->stream, code, *L->ilu, L->kp, x, y);
-    else bsr_spmv_launch<float, T>(ctx->stream, code, *L->ilu, L->kp32, x, y);
+    if (sizeof(T) == 8 && !L->pc_store32) bsr_spmv_launch<double, T>(ctx->stream, code, *L->ilu, L->kp, x, y, sub);
+    else bsr_spmv_launch<float, T>(ctx->stream, code, *L->ilu, L->kp32, x, y, sub);
   } else {
     const int v = L->tune_spmv[slot];
     if (sizeof(T) == 8) spmv_launch<double, T>(ctx->stream, v, L->nblk, L->d_desc, ctx->d_colidx, ctx->d_vals, x, y);
@@ -2089,20 +2117,24 @@ int la_prepare(sfem_ctx* ctx, const sfem_solver_opts* o) {
 int la_prepare_solve(sfem_ctx* ctx, const sfem_solver_opts* o) {
   LinAlg* L = ctx->la;
   const bool fp32 = ctx->precision == SFEM_PREC_MIXED;
+  // FP64 Krylov over an FP32-stored tangent and factor (ILU(0) only; other
+  // preconditioners run the plain FP64 path)
+  const bool store32 = ctx->precision == SFEM_PREC_STORE32 && o->pc_type == SFEM_PC_ILU0;
   int rc;
   const int reorder = o->pc_type == SFEM_PC_ILU0 ? o->reorder : 0;
-  const bool bf16f = fp32 && o->pc_type == SFEM_PC_ILU0 && o->factor_bf16 != 0;
-  const bool same_pc = L->pc_K_version >= 0 && L->pc_type == o->pc_type && L->pc_fp32 == fp32 &&
-                       L->pc_reorder == reorder && L->pc_bf16 == bf16f;
+  const bool bf16f = (fp32 || store32) && o->pc_type == SFEM_PC_ILU0 && o->factor_bf16 != 0;
+  const bool same_pc = L->pc_K_version >= 0 && L->pc_type == o->pc_type && L->pc_fp32 == (fp32 || store32) &&
+                       L->pc_store32 == store32 && L->pc_reorder == reorder && L->pc_bf16 == bf16f;
   if (!same_pc || (L->kb_version != ctx->la_K_version && ++L->lag_count >= L->pc_lag)) {
     L->pc_K_version = L->kb_version = -1;
-    if ((rc = la_setup_pc(ctx, o->pc_type, fp32, reorder, bf16f))) return rc;
+    if ((rc = la_setup_pc(ctx, o->pc_type, fp32, reorder, bf16f, true, store32))) return rc;
     L->pc_reorder = reorder;
     L->pc_K_version = L->kb_version = ctx->la_K_version;
     L->lag_count = 0;
   } else if (L->kb_version != ctx->la_K_version) {
     // lagged preconditioner: the products use the new tangent, the factor stays
-    if (o->pc_type == SFEM_PC_ILU0 && (rc = la_setup_pc(ctx, o->pc_type, fp32, reorder, bf16f, false))) return rc;
+    if (o->pc_type == SFEM_PC_ILU0 && (rc = la_setup_pc(ctx, o->pc_type, fp32, reorder, bf16f, false, store32)))
+      return rc;
     L->kb_version = ctx->la_K_version;
   }
   L->block_space = L->pc_type == SFEM_PC_ILU0;
@@ -2117,15 +2149,80 @@ int la_prepare_solve(sfem_ctx* ctx, const sfem_solver_opts* o) {
   return SFEM_OK;
 }
 
-int la_apply_pc32(sfem_ctx* ctx, const float* r, float* z) { return pc_apply<float>(ctx, r, z); }
-int la_apply_pc64(sfem_ctx* ctx, const double* r, double* z) { return pc_apply<double>(ctx, r, z); }
-int la_apply_op32(sfem_ctx* ctx, const float* x, float* y) {
-  spmv<float>(ctx, x, y);
+int la_apply_pc32(sfem_ctx* ctx, const float* r, float* z, const RowSubset* sub) {
+  if (sub && ctx->la->pc_type != SFEM_PC_ILU0) return set_error(SFEM_ERR_UNSUPPORTED, "row subsets need ILU(0)");
+  return pc_apply<float>(ctx, r, z, sub);
+}
+int la_apply_pc64(sfem_ctx* ctx, const double* r, double* z, const RowSubset* sub) {
+  if (sub && ctx->la->pc_type != SFEM_PC_ILU0) return set_error(SFEM_ERR_UNSUPPORTED, "row subsets need ILU(0)");
+  return pc_apply<double>(ctx, r, z, sub);
+}
+int la_apply_op32(sfem_ctx* ctx, const float* x, float* y, const RowSubset* sub) {
+  if (sub && !ctx->la->block_space) return set_error(SFEM_ERR_UNSUPPORTED, "row subsets need the block layout");
+  spmv_in<float>(ctx, x, y, ctx->la->block_space, sub);
   return hipGetLastError() == hipSuccess ? SFEM_OK : set_error(SFEM_ERR_HIP, "spmv launch");
 }
-int la_apply_op64(sfem_ctx* ctx, const double* x, double* y) {
-  spmv<double>(ctx, x, y);
+int la_apply_op64(sfem_ctx* ctx, const double* x, double* y, const RowSubset* sub) {
+  if (sub && !ctx->la->block_space) return set_error(SFEM_ERR_UNSUPPORTED, "row subsets need the block layout");
+  spmv_in<double>(ctx, x, y, ctx->la->block_space, sub);
   return hipGetLastError() == hipSuccess ? SFEM_OK : set_error(SFEM_ERR_HIP, "spmv launch");
+}
+
+RowSubset::~RowSubset() {
+  if (d_rows) (void)hipFree(d_rows);
+  if (d_lev) (void)hipFree(d_lev);
+}
+
+int la_subset_build(sfem_ctx* ctx, const uint8_t* keep, RowSubset* S, const RowSubset* from) {
+  LinAlg* L = ctx->la;
+  if (!L->ilu) return set_error(SFEM_ERR_STATE, "row subset without an ILU(0) plan");
+  const IluPlan& P = *L->ilu;
+  if (from && (from == S || from->plan_id != L->plan_id)) return set_error(SFEM_ERR_ARG, "bad source subset");
+  std::vector<int32_t> rows, lev, lofs, uofs;
+  if (from) {
+    for (int32_t p : from->h_rows)
+      if (keep[p]) rows.push_back(p);
+    auto add = [&](const std::vector<int32_t>& fo, std::vector<int32_t>& ofs) {
+      ofs.assign(1, (int32_t)lev.size());
+      for (size_t lv = 0; lv + 1 < fo.size(); ++lv) {
+        for (int32_t i = fo[lv]; i < fo[lv + 1]; ++i)
+          if (keep[from->h_lev[i]]) lev.push_back(from->h_lev[i]);
+        ofs.push_back((int32_t)lev.size());
+      }
+    };
+    add(from->lofs, lofs);
+    add(from->uofs, uofs);
+  } else {
+    for (int64_t p = 0; p < P.nblk; ++p)
+      if (keep[p]) rows.push_back((int32_t)p);
+    auto add = [&](bool upper, std::vector<int32_t>& ofs) {
+      const int nlev = (int)(upper ? P.levUPtr.size() : P.levLPtr.size()) - 1;
+      const std::vector<int32_t>& hl = upper ? P.h_levU : P.h_levL;
+      ofs.assign(1, (int32_t)lev.size());
+      for (int lv = 0; lv < nlev; ++lv) {
+        const int32_t* dl; int first, cnt;
+        level_range(P, upper, lv, &dl, &first, &cnt);
+        for (int i = 0; i < cnt; ++i) {
+          const int32_t p = dl ? hl[first + i] : first + i;
+          if (keep[p]) lev.push_back(p);
+        }
+        ofs.push_back((int32_t)lev.size());
+      }
+    };
+    add(false, lofs);
+    add(true, uofs);
+  }
+  for (int32_t** q : {&S->d_rows, &S->d_lev})
+    if (*q) { (void)hipFree(*q); *q = nullptr; }
+  S->nrows = (int64_t)rows.size();
+  int rc;
+  if ((rc = upload(&S->d_rows, rows)) || (rc = upload(&S->d_lev, lev))) return rc;
+  S->h_rows.swap(rows);
+  S->h_lev.swap(lev);
+  S->lofs.swap(lofs);
+  S->uofs.swap(uofs);
+  S->plan_id = L->plan_id;
+  return SFEM_OK;
 }
 
 int la_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int* iters, double* relres) {
@@ -2163,6 +2260,11 @@ int la_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int* iters, double* relre
         if ((rc = ensure_vec(q, L->nvec))) return rc;
       rc = bicgstab<double>(ctx, b, x, o->rtol, o->max_it, iters, relres, L->r, L->rh, L->p, L->v, L->s,
                             L->t, L->ph, L->sh);
+    }
+    // FP32-stored operator: the solve is one step of a defect correction, as in mixed mode
+    if (L->pc_store32 && rc == SFEM_ERR_MAXIT && std::isfinite(*relres) && *relres < 1.0) {
+      ++L->partial;
+      rc = SFEM_OK;
     }
     if (padmap) {
       hipLaunchKernelGGL((k_pad_out<double, double>), dim3(gk), dim3(kTPB), 0, st, nk, padmap, x, ctx->d_du, 1.0);

@@ -64,6 +64,7 @@ struct LinAlg {
   std::vector<int32_t> h_blkStart, h_diagpos;
   int pc_reorder = 0;
   bool pc_bf16 = false;
+  bool pc_store32 = false;    // FP64-built ILU(0) whose factor and block tangent are kept in FP32
   bf16* lu16 = nullptr;
   IluPlan* ilu = nullptr;
   int32_t *d_blkStart = nullptr, *d_diagpos = nullptr;
@@ -100,18 +101,38 @@ struct LinAlg {
   ~LinAlg();
 };
 
+// A subset of the ILU(0) plan's block rows (plan positions) that products and
+// preconditioner applications can be restricted to: the rows of the subset (SpMV) and
+// the subset's rows of every forward then backward level (triangular solves), as device
+// lists with host offsets.  Rows outside the subset are neither read as outputs nor
+// written; the subset must be closed under coupling (a union of whole meshes).
+struct RowSubset {
+  int plan_id = -1;
+  int64_t nrows = 0;
+  int32_t* d_rows = nullptr;
+  int32_t* d_lev = nullptr;
+  std::vector<int32_t> lofs, uofs;  // forward level l: d_lev[lofs[l] .. lofs[l+1]), backward: uofs
+  std::vector<int32_t> h_rows, h_lev;
+  ~RowSubset();
+};
+// S = the plan positions p with keep[p] != 0, taken from `from` (a subset of the same
+// plan, so that shrinking costs the size of the current subset) or from all block rows
+int la_subset_build(sfem_ctx* ctx, const uint8_t* keep, RowSubset* S, const RowSubset* from = nullptr);
+
 // Products and preconditioner application inside a solve (sfem_batch.hip drives its own
-// Krylov iteration through these): the current layout and preconditioner of the context.
-int la_apply_pc32(sfem_ctx* ctx, const float* r, float* z);
-int la_apply_pc64(sfem_ctx* ctx, const double* r, double* z);
-int la_apply_op32(sfem_ctx* ctx, const float* x, float* y);
-int la_apply_op64(sfem_ctx* ctx, const double* x, double* y);
+// Krylov iteration through these): the current layout and preconditioner of the context,
+// over all block rows or over a row subset.
+int la_apply_pc32(sfem_ctx* ctx, const float* r, float* z, const RowSubset* sub = nullptr);
+int la_apply_pc64(sfem_ctx* ctx, const double* r, double* z, const RowSubset* sub = nullptr);
+int la_apply_op32(sfem_ctx* ctx, const float* x, float* y, const RowSubset* sub = nullptr);
+int la_apply_op64(sfem_ctx* ctx, const double* x, double* y, const RowSubset* sub = nullptr);
 // Bring the preconditioner and the Krylov layout up to date for opts (as la_solve does).
 int la_prepare_solve(sfem_ctx* ctx, const sfem_solver_opts* o);
 
 int la_create(sfem_ctx* ctx);
 void la_destroy(sfem_ctx* ctx);
-int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f, bool refactor = true);
+int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f, bool refactor = true,
+                bool store32 = false);
 int la_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int* iters, double* relres);
 int la_prepare(sfem_ctx* ctx, const sfem_solver_opts* o);
 int la_tuning(sfem_ctx* ctx, int* out);

@@ -25,7 +25,7 @@ SFEM_OK = 0
 ERRORS = {-1: "SFEM_ERR_ARG", -2: "SFEM_ERR_STATE", -3: "SFEM_ERR_NEGJAC", -4: "SFEM_ERR_HIP",
           -5: "SFEM_ERR_DIVERGED", -6: "SFEM_ERR_MAXIT", -7: "SFEM_ERR_NOMEM", -8: "SFEM_ERR_NOTCONV",
           -9: "SFEM_ERR_UNSUPPORTED"}
-PREC_FP64, PREC_MIXED = 0, 1
+PREC_FP64, PREC_MIXED, PREC_STORE32 = 0, 1, 2
 PRED_PREVIOUS, PRED_EXTRAPOLATE = 0, 1
 KT_ASSEMBLY, KT_SPMV, KT_PC = 0, 1, 2
 KSP = {"bcgs": 0, "gmres": 1}
@@ -82,6 +82,7 @@ ABI = [
     ("sfem_set_batch", C.c_int, [_vp, C.c_int32, _i32p, _f64p]),
     ("sfem_step_batch", C.c_int, [_vp, C.c_double, C.POINTER(SolverOpts), C.c_double, C.c_int, _i32p, _i32p, _f64p]),
     ("sfem_batch_use_graph", C.c_int, [_vp, C.c_int]),
+    ("sfem_batch_narrowing", C.c_int, [_vp, C.c_int]),
     ("sfem_block_info", C.c_int, [_vp, _i64p, _i64p, _i64p]),
     ("sfem_tuning", C.c_int, [_vp, C.POINTER(C.c_int)]),
     ("sfem_tuning_levels", C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]),
@@ -285,6 +286,9 @@ class Context:
 
     def batch_use_graph(self, enable: bool):
         _check(self.lib.sfem_batch_use_graph(self.h, 1 if enable else 0), "sfem_batch_use_graph")
+
+    def batch_narrowing(self, enable: bool):
+        _check(self.lib.sfem_batch_narrowing(self.h, 1 if enable else 0), "sfem_batch_narrowing")
 
     def step_batch(self, timeFact: float, opts: SolverOpts, newton_tol: float = 1e-8, max_newton: int = 10):
         """One time step of every mesh: (newton_iters, krylov_iters, rhs_norm) per mesh."""

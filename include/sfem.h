@@ -39,8 +39,14 @@ enum {
   SFEM_ERR_UNSUPPORTED = -9 /* element type / option not available */
 };
 
-/* Precision of the tangent and Krylov solve (residual and update are always FP64). */
-enum { SFEM_PREC_FP64 = 0, SFEM_PREC_MIXED = 1 };
+/* Precision of the tangent and Krylov solve (residual and update are always FP64).
+ * SFEM_PREC_MIXED: FP32 element tangents, FP32 CSR, FP32 Krylov vectors and ILU(0).
+ * SFEM_PREC_STORE32: FP64 tangent and FP64 ILU(0) factorisation, the node-block tangent
+ * and the factor then kept in FP32 (factor_bf16: BF16) for the Krylov products, which
+ * run on FP64 vectors with FP64 accumulation -- half the matrix and factor bytes per
+ * iteration without FP32 rounding of the Krylov vectors, for systems too ill-conditioned
+ * for FP32 Krylov (ILU(0) only; with other preconditioners it is the FP64 path). */
+enum { SFEM_PREC_FP64 = 0, SFEM_PREC_MIXED = 1, SFEM_PREC_STORE32 = 2 };
 
 /* Krylov methods and preconditioners (vocabulary of the PETSc options files). */
 enum { SFEM_KSP_BCGS = 0, SFEM_KSP_GMRES = 1 };
@@ -55,8 +61,9 @@ typedef struct sfem_solver_opts {
   int reorder;       /* ILU(0) node ordering: 0 the matrix's own order, 1 reverse Cuthill-McKee
                         (PETSc's -pc_factor_mat_ordering_type rcm), 2 greedy multicolour
                         (one fully parallel level per colour) */
-  int factor_bf16;   /* mixed precision only: keep the ILU(0) factors in bfloat16 (computed in
-                        FP32, rounded to nearest even) to halve the preconditioner's bytes */
+  int factor_bf16;   /* SFEM_PREC_MIXED / SFEM_PREC_STORE32 only: keep the ILU(0) factors in
+                        bfloat16 (computed in FP32 / FP64, rounded to nearest even) to halve
+                        the preconditioner's bytes */
 } sfem_solver_opts;
 
 typedef struct sfem_step_stats {
@@ -107,7 +114,7 @@ int sfem_get_pattern(sfem_ctx* ctx, int64_t* rowptr, int32_t* colidx, int32_t* a
  * (UtilitiesGeneral.cpp:5-209). */
 int sfem_set_params(sfem_ctx* ctx, const double* elemData, int tis, double rhoInf, double dt);
 
-/* SFEM_PREC_FP64 (default) or SFEM_PREC_MIXED. */
+/* SFEM_PREC_FP64 (default), SFEM_PREC_MIXED or SFEM_PREC_STORE32. */
 int sfem_set_precision(sfem_ctx* ctx, int precision);
 
 /* Newton initial iterate of each step: SFEM_PRED_PREVIOUS (the previous step's
@@ -219,6 +226,13 @@ int sfem_step_batch(sfem_ctx* ctx, double timeFact, const sfem_solver_opts* opts
 /* Replay the batched BiCGStab iteration as a HIP graph (1, default) or launch it kernel
  * by kernel (0). */
 int sfem_batch_use_graph(sfem_ctx* ctx, int enable);
+
+/* Narrow the batched iteration to the meshes still iterating (1, default): each time the
+ * block rows of the unfinished meshes fall below half of those the iteration covers, its
+ * products, ILU(0) sweeps and vector updates are restricted to them (identical results,
+ * less work when mesh sizes or iteration counts differ).  0: every iteration covers
+ * every mesh. */
+int sfem_batch_narrowing(sfem_ctx* ctx, int enable);
 
 /* Sizes of the ILU(0) node-block form (0 before an ILU(0) solver is set up): block
  * rows, stored neighbour blocks, and stored values (padded 4-wide blocks, m rows each).

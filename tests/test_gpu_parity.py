@@ -335,6 +335,39 @@ def test_mixed_bf16_factors_and_predictor(bf16):
     assert eu <= 1e-6
 
 
+@pytest.mark.parametrize("bf16", [0, 1])
+def test_store32_close_to_fp64(bf16):
+    """FP64 Krylov over the FP32-stored block tangent and ILU(0) factor (FP64-computed),
+    as a defect correction inside the FP64 Newton loop, stays within 1e-6 of FP64 on the
+    graded 3D cavity and on a jittered Tria3 cavity at Re 1000."""
+    f64, mesh, ctl = _cavity3d(6, 4, newton_tol=1e-10)
+    s64 = f64.solveFullyImplicit()
+    fs, _, _ = _cavity3d(6, 4, prec=sfem.PREC_STORE32,
+                         opts=sfem.solver_opts("bcgs", "ilu", 1e-4, 2000, reorder=2, factor_bf16=bf16),
+                         newton_tol=1e-10)
+    ss = fs.solveFullyImplicit()
+    eu = np.linalg.norm(ss.reshape(-1, 4)[:, :3] - s64.reshape(-1, 4)[:, :3]) / np.linalg.norm(s64.reshape(-1, 4)[:, :3])
+    print(f"[store32 bf16={bf16}] hex rel L2 u {eu:.2e}; history {fs.history}", flush=True)
+    assert eu <= 1e-6
+    mesh = meshgen.ldc2d_tria(40, jitter=0.2, seed=3)
+    ed = np.array([1.0, 1e-3, 0, 0, 0, 0, 0, 0, 1, 1, 0])
+    sol = {}
+    for prec, rtol in ((sfem.PREC_FP64, 1e-10), (sfem.PREC_STORE32, 1e-4)):
+        ctx = make_ctx(mesh, tis=2, dt=0.05, elemData=ed)
+        ctx.set_precision(prec)
+        o = sfem.solver_opts("bcgs", "ilu", rtol, 3000, reorder=2, factor_bf16=bf16 if prec else 0)
+        t = 0.05
+        for _ in range(3):
+            st = ctx.step(min(t, 1.0), o, 1e-10, 10)
+            assert st.rhs_norm < 1e-10, (prec, st.newton_iters, st.rhs_norm)
+            t += 0.05
+        sol[prec] = ctx.get_solution().reshape(-1, 3)
+        ctx.close()
+    eu = np.linalg.norm(sol[2][:, :2] - sol[0][:, :2]) / np.linalg.norm(sol[0][:, :2])
+    print(f"[store32 bf16={bf16}] tria rel L2 u {eu:.2e}", flush=True)
+    assert eu <= 1e-6
+
+
 def test_negative_jacobian_reported():
     mesh = meshgen.ldc2d_quad(3)
     mesh.conn[4] = mesh.conn[4][[0, 3, 2, 1]]  # clockwise element
@@ -376,7 +409,7 @@ def _batch_ctx(mesh, em, mu, prec, opts, dt):
     return ctx
 
 
-@pytest.mark.parametrize("prec", [sfem.PREC_FP64, sfem.PREC_MIXED])
+@pytest.mark.parametrize("prec", [sfem.PREC_FP64, sfem.PREC_MIXED, sfem.PREC_STORE32])
 @pytest.mark.parametrize("count,seed,n_range", [(4, 5, (4, 9)), (6, 3, (4, 64))])
 def test_batch_matches_single_meshes(prec, count, seed, n_range):
     """A batch of jittered Tria3 cavities with their own viscosities, stepped together
@@ -440,3 +473,31 @@ def test_batch_graph_replay_matches_eager():
     ctx.close()
     print(f"[graph] max |graph - eager| {np.max(np.abs(res[0] - res[1])):.3e}", flush=True)
     assert np.array_equal(res[0], res[1]) and np.array_equal(res[0], res[2])
+
+
+@pytest.mark.parametrize("prec", [sfem.PREC_FP64, sfem.PREC_MIXED])
+def test_batch_narrowing_matches_full(prec):
+    """Restricting the batched iteration to the unfinished meshes (row subsets of the
+    products and ILU(0) sweeps, compacted chunk lists) gives bit-identical iterates."""
+    mesh, em, mu, ns, meshes = meshgen.c4_batch(6, seed=3, n_range=(4, 64))
+    dt = 0.05
+    rtol = 1e-10 if prec == sfem.PREC_FP64 else 1e-4
+    opts = sfem.solver_opts("bcgs", "ilu", rtol, 3000, reorder=2)
+    ctx = _batch_ctx(mesh, em, mu, prec, opts, dt)
+    res, kry = [], []
+    for narrow in (True, False):
+        ctx.reset_state()
+        ctx.batch_narrowing(narrow)
+        t = dt
+        ks = []
+        for s in range(2):
+            ni, ki, rn = ctx.step_batch(min(t, 1.0), opts, 1e-10, 10)
+            ks.append(ki.tolist())
+            t += dt
+        res.append(ctx.get_solution())
+        kry.append(ks)
+    ctx.close()
+    print(f"[narrow] prec={prec} krylov {kry[0]} max |narrowed - full| {np.max(np.abs(res[0] - res[1])):.3e}",
+          flush=True)
+    assert kry[0] == kry[1]
+    assert np.array_equal(res[0], res[1])
