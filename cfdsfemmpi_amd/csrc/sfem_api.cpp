@@ -112,6 +112,7 @@ static AsmView asm_view(sfem_ctx* c) {
   v.solnApplied = c->d_applied;
   v.nodeInfo = c->d_nodeInfo;
   v.colofs = c->d_colofs;
+  v.sconn = c->d_sconn;
   v.vals = c->d_vals;
   v.vals32 = c->d_vals32;
   v.rhs = c->d_rhs;
@@ -211,7 +212,7 @@ void sfem_destroy(sfem_ctx* c) {
   if (c->batch) batch_destroy(c);
   dfree(c->d_elem_mu);
   dfree(c->d_coords); dfree(c->d_conn); dfree(c->d_charlen); dfree(c->d_vol); dfree(c->d_bad);
-  dfree(c->d_nodeInfo); dfree(c->d_colofs); dfree(c->d_elemByColor); dfree(c->d_rowptr32);
+  dfree(c->d_nodeInfo); dfree(c->d_colofs); dfree(c->d_sconn); dfree(c->d_elemByColor); dfree(c->d_rowptr32);
   dfree(c->d_colidx); dfree(c->d_assy); dfree(c->d_dbc_ind); dfree(c->d_dbc_val); dfree(c->d_dbc_mult);
   dfree(c->d_vals); dfree(c->d_vals32); dfree(c->d_rhs); dfree(c->d_du);
   dfree(c->d_soln); dfree(c->d_prev); dfree(c->d_prev2); dfree(c->d_prev3); dfree(c->d_prev4);
@@ -409,6 +410,18 @@ int sfem_build_pattern(sfem_ctx* c, int64_t* nfree_out, int64_t* nnz_out) {
     std::vector<int32_t> pos(c->colorPtr.begin(), c->colorPtr.end() - 1);
     for (int64_t e = 0; e < nElem; ++e) byColor[pos[color[e]]++] = (int32_t)e;
   }
+  // the element loop walks colour classes slot by slot: the element -> nonzero map and
+  // the connectivity are stored in that order, so each slot's entries are contiguous
+  std::vector<int32_t> scolofs((size_t)nElem * npe * npe), sconn((size_t)nElem * npe);
+  parallel_for(nElem, [&](int64_t lo, int64_t hi) {
+    for (int64_t sl = lo; sl < hi; ++sl) {
+      const int64_t e = byColor[sl];
+      std::copy(colofs.begin() + e * npe * npe, colofs.begin() + (e + 1) * npe * npe,
+                scolofs.begin() + sl * npe * npe);
+      std::copy(c->conn.begin() + e * npe, c->conn.begin() + (e + 1) * npe, sconn.begin() + sl * npe);
+    }
+  });
+  std::vector<int32_t>().swap(colofs);
   // node info {rowStart, rowlen, mask, firstRow}
   std::vector<int32_t> info(nNode * 4);
   for (int64_t i = 0; i < nNode; ++i) {
@@ -420,17 +433,19 @@ int sfem_build_pattern(sfem_ctx* c, int64_t* nfree_out, int64_t* nnz_out) {
   std::vector<int32_t> rp32(nf + 1);
   for (int64_t r = 0; r <= nf; ++r) rp32[r] = (int32_t)c->rowptr[r];
   // upload
-  dfree(c->d_nodeInfo); dfree(c->d_colofs); dfree(c->d_elemByColor); dfree(c->d_rowptr32);
+  dfree(c->d_nodeInfo); dfree(c->d_colofs); dfree(c->d_sconn); dfree(c->d_elemByColor); dfree(c->d_rowptr32);
   dfree(c->d_colidx); dfree(c->d_assy); dfree(c->d_vals); dfree(c->d_vals32); dfree(c->d_rhs); dfree(c->d_du);
   if (c->la) la_destroy(c);
   int rc;
-  if ((rc = dalloc(&c->d_nodeInfo, info.size())) || (rc = dalloc(&c->d_colofs, colofs.size())) ||
+  if ((rc = dalloc(&c->d_nodeInfo, info.size())) || (rc = dalloc(&c->d_colofs, scolofs.size())) ||
+      (rc = dalloc(&c->d_sconn, sconn.size())) ||
       (rc = dalloc(&c->d_elemByColor, byColor.size())) || (rc = dalloc(&c->d_rowptr32, rp32.size())) ||
       (rc = dalloc(&c->d_colidx, c->colidx.size())) || (rc = dalloc(&c->d_assy, c->assyForSoln.size())) ||
       (rc = dalloc(&c->d_vals, c->nnz)) || (rc = dalloc(&c->d_rhs, nf)) || (rc = dalloc(&c->d_du, nf)))
     return rc;
   HIP_OK(hipMemcpy(c->d_nodeInfo, info.data(), info.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(c->d_colofs, colofs.data(), colofs.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(c->d_colofs, scolofs.data(), scolofs.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(c->d_sconn, sconn.data(), sconn.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(c->d_elemByColor, byColor.data(), byColor.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(c->d_rowptr32, rp32.data(), rp32.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(c->d_colidx, c->colidx.data(), c->colidx.size() * sizeof(int32_t), hipMemcpyHostToDevice));
@@ -517,10 +532,14 @@ int sfem_time_update(sfem_ctx* c, double timeFact) {
 }
 
 static int run_element_loop(sfem_ctx* c, int mode, int lift) {
-  const AsmView v = asm_view(c);
+  const AsmView v0 = asm_view(c);
   const ElemParams p = elem_params(c);
+  const int npe = c->dims.npe;
   for (int k = 0; k < c->ncolors; ++k) {
     const int64_t n = c->colorPtr[k + 1] - c->colorPtr[k];
+    AsmView v = v0;  // slot-ordered arrays start at the colour's first slot
+    v.colofs = v0.colofs + (int64_t)c->colorPtr[k] * npe * npe;
+    v.sconn = v0.sconn + (int64_t)c->colorPtr[k] * npe;
     c->kt.begin(SFEM_KT_ASSEMBLY, c->stream);
     HIP_OK(launch_assemble(c->et, c->stream, v, p, c->d_elemByColor + c->colorPtr[k], n, mode, lift, -1));
     c->kt.end(SFEM_KT_ASSEMBLY, c->stream);

@@ -28,6 +28,9 @@
 // deterministic (summation order = colour order).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <type_traits>
+
 #include "sfem_internal.hpp"
 
 namespace sfem {
@@ -251,9 +254,136 @@ struct TeamLds {
   int32_t info[C::NPE][4];           // rowStart, rowlen, mask, firstRow
 };
 
+// Per-wavefront staging of the pipelined Hexa8 loop: the residual and lift partial sums
+// are reduced across lanes in registers, so only the shared per-element data is here.
+struct HexLds {
+  using C = Traits<HEXA8>;
+  static constexpr int NQ = TeamLds<HEXA8>::NQ;
+  double X[8][3];
+  double S[3][8][4];
+  double app[8][4];
+  double phi[8][8][4];
+  double W[8];
+  double Q[8][NQ];
+  double U[8][10];
+  double P[8][8][6];
+  int32_t info[8][4];
+};
+
+// P2: shape gradients of node k at Gauss point gp, lane l = gp * NPE + k (l < NGP * NPE).
+template <int ET, class LDS>
+__device__ inline void phase_geometry(LDS& L, int l) {
+  using C = Traits<ET>;
+  const int gp = l / C::NPE, k = l % C::NPE;
+  double Nk, dNk[3], dvol;
+  gp_geometry<ET>(L.X, gp, k, Nk, dNk, dvol);
+  L.phi[gp][k][0] = Nk;
+  L.phi[gp][k][1] = dNk[0];
+  L.phi[gp][k][2] = dNk[1];
+  L.phi[gp][k][3] = dNk[2];
+  if (k == 0) L.W[gp] = dvol;
+}
+
+// P3: interpolated state at each Gauss point, summed over nodes in node order.
+template <int ET, class LDS>
+__device__ inline void phase_interp(LDS& L, int l) {
+  using C = Traits<ET>;
+  constexpr int NQ = TeamLds<ET>::NQ;
+  const int gp = l / C::NPE, k = l % C::NPE;
+  for (int t = k; t < NQ; t += C::NPE) {
+    int src, comp, der;
+    qdesc<C::NDIM>(t, src, comp, der);
+    double s = 0.0;
+    for (int nn = 0; nn < C::NPE; ++nn) s += L.S[src][nn][comp] * L.phi[gp][nn][der];
+    L.Q[gp][t] = s;
+  }
+}
+
+// P4: per-Gauss-point derived data (stabilisation parameters of
+// evaluateStabParams_algo3, stabilisationRoutines.h:128-143, with G = 4/h^2 I), the
+// residual contribution Fk of node k at gp, and the per-(node, gp) factors of the
+// tangent, for lane l = gp * NPE + k.
+template <int ET, class LDS>
+__device__ inline void phase_gauss(LDS& L, int l, double h, const ElemParams& p, double (&Fk)[4]) {
+  using C = Traits<ET>;
+  constexpr int D = C::NDIM;
+  const int gp = l / C::NPE, k = l % C::NPE;
+  const double* Q = L.Q[gp];
+  const double* vp = Q;
+  const double* gpv = Q + D;                  // gradPrev row-major
+  const double* vel = Q + D + D * D;
+  const double* gr = Q + 2 * D + D * D;       // grad row-major
+  const double pres = Q[2 * D + 2 * D * D];
+  const double* dp = Q + 2 * D + 2 * D * D + 1;
+  const double* vd = Q + 3 * D + 2 * D * D + 1;
+  const double rho = p.rho, mu = p.mu, af = p.af;
+
+  double res2[3], rStab[3];
+  for (int a = 0; a < D; ++a) {
+    double s1 = gpv[a * D] * vel[0], s2 = gr[a * D] * vp[0], s3 = gpv[a * D] * vp[0];
+    for (int c = 1; c < D; ++c) {
+      s1 += gpv[a * D + c] * vel[c];
+      s2 += gr[a * D + c] * vp[c];
+      s3 += gpv[a * D + c] * vp[c];
+    }
+    res2[a] = rho * (vd[a] + (s1 + s2 - s3));
+    rStab[a] = res2[a] + dp[a];
+  }
+  const double fact = 4.0 / h / h;
+  double uGu = 0.0;
+  for (int a = 0; a < D; ++a) uGu += vp[a] * (fact * vp[a]);
+  const double G2 = D * (fact * fact);
+  double tau0 = 1.0 / sqrt(uGu + C::CI * (mu * mu / rho / rho) * G2);
+  tau0 = tau0 / rho;
+  const double tauS = tau0 * p.stab[0];
+  const double tauP = tau0 * p.stab[1];
+  double trace = gr[0];
+  for (int a = 1; a < D; ++a) trace += gr[a * D + a];
+
+  const double w = L.W[gp];
+  const double Nk = L.phi[gp][k][0];
+  double b[3];
+  for (int c = 0; c < D; ++c) b[c] = L.phi[gp][k][1 + c] * w;
+  const double b4 = Nk * w;
+  double Da = vp[0] * b[0];
+  for (int c = 1; c < D; ++c) Da += vp[c] * b[c];
+  Da = rho * Da * tauS;
+  // residual of node k at gp (Hexa8Node.cpp:1049-1060, Quad4Node.cpp:936-950)
+  for (int a = 0; a < D; ++a) {
+    double s = b4 * res2[a];
+    for (int c = 0; c < D; ++c) s += b[c] * (mu * gr[a * D + c] - (a == c ? pres : 0.0));
+    double f = -s;
+    if constexpr (C::SUPG) f -= Da * rStab[a];
+    Fk[a] = f;
+  }
+  double br = b[0] * rStab[0];
+  for (int a = 1; a < D; ++a) br += b[a] * rStab[a];
+  Fk[D] = C::SGN * (-(b4 * trace) - tauP * br);
+  for (int a = D + 1; a < 4; ++a) Fk[a] = 0.0;
+  // tangent factors
+  const double stp = C::SGN * tauP;
+  for (int c = 0; c < D; ++c) {
+    double g = 0.0;
+    for (int a = 0; a < D; ++a) g += b[a] * (af * rho * gpv[a * D + c]);
+    L.P[gp][k][c] = stp * g;
+  }
+  double ud = vp[0] * L.phi[gp][k][1];
+  for (int c = 1; c < D; ++c) ud += vp[c] * L.phi[gp][k][1 + c];
+  const double uk = af * rho * ud;
+  L.P[gp][k][3] = uk;
+  L.P[gp][k][4] = uk + rho * p.acceFact * Nk;
+  L.P[gp][k][5] = Da;
+  if (k == 0) {
+    for (int a = 0; a < D; ++a)
+      for (int c = 0; c < D; ++c) L.U[gp][3 * a + c] = af * rho * gpv[a * D + c];
+    L.U[gp][9] = stp;
+  }
+}
+
 // Ke accumulation of lane (i, j) in scalar type T (double, or float for the mixed path).
-template <int ET, typename T>
-__device__ inline void accumulate_block(const TeamLds<ET>& L, int i, int j, const ElemParams& p,
+// GPU Gauss points are unrolled together.
+template <int ET, typename T, int GPU = Traits<ET>::NGP, class LDS>
+__device__ inline void accumulate_block(const LDS& L, int i, int j, const ElemParams& p,
                                         T (&K)[Traits<ET>::NDOF][Traits<ET>::NDOF]) {
   using C = Traits<ET>;
   constexpr int D = C::NDIM;
@@ -261,7 +391,7 @@ __device__ inline void accumulate_block(const TeamLds<ET>& L, int i, int j, cons
   const T saf = (T)(C::SGN * p.af);
   for (int a = 0; a < C::NDOF; ++a)
     for (int b = 0; b < C::NDOF; ++b) K[a][b] = (T)0;
-#pragma unroll
+#pragma unroll GPU
   for (int gp = 0; gp < C::NGP; ++gp) {
     const T w = (T)L.W[gp];
     const T Ni = (T)L.phi[gp][i][0], Nj = (T)L.phi[gp][j][0];
@@ -320,6 +450,9 @@ __device__ inline void add_row_segment(T* __restrict__ row, const T (&k)[NDOF], 
   }
 }
 
+// Generic element loop (all element types; the Hexa8 colour loops use the pipelined
+// kernel below and this one serves the per-element probe): TEAMS elements per 256-lane
+// workgroup, one per team.
 template <int ET, int MODE>
 __global__ __launch_bounds__(256) void k_assemble(AsmView v, ElemParams p,
                                                   const int32_t* __restrict__ elems, int64_t n,
@@ -327,7 +460,6 @@ __global__ __launch_bounds__(256) void k_assemble(AsmView v, ElemParams p,
   using C = Traits<ET>;
   constexpr int TEAMS = 256 / C::TEAM;
   constexpr int D = C::NDIM, NPE = C::NPE, NGP = C::NGP, NDOF = C::NDOF;
-  constexpr int NQ = TeamLds<ET>::NQ;
   __shared__ TeamLds<ET> lds[TEAMS];
   const int team = threadIdx.x / C::TEAM;
   const int l = threadIdx.x % C::TEAM;
@@ -354,109 +486,15 @@ __global__ __launch_bounds__(256) void k_assemble(AsmView v, ElemParams p,
     L.info[l][0] = inf.x; L.info[l][1] = inf.y; L.info[l][2] = inf.z; L.info[l][3] = inf.w;
   }
   __syncthreads();
-
-  // P2: shape gradients of node k at Gauss point gp
-  if (active && l < NGP * NPE) {
-    const int gp = l / NPE, k = l % NPE;
-    double Nk, dNk[3], dvol;
-    gp_geometry<ET>(L.X, gp, k, Nk, dNk, dvol);
-    L.phi[gp][k][0] = Nk;
-    L.phi[gp][k][1] = dNk[0];
-    L.phi[gp][k][2] = dNk[1];
-    L.phi[gp][k][3] = dNk[2];
-    if (k == 0) L.W[gp] = dvol;
-  }
+  if (active && l < NGP * NPE) phase_geometry<ET>(L, l);
   __syncthreads();
-
-  // P3: interpolated state at each Gauss point, summed over nodes in node order
-  if (active && l < NGP * NPE) {
-    const int gp = l / NPE, k = l % NPE;
-    for (int t = k; t < NQ; t += NPE) {
-      int src, comp, der;
-      qdesc<D>(t, src, comp, der);
-      double s = 0.0;
-      for (int nn = 0; nn < NPE; ++nn) s += L.S[src][nn][comp] * L.phi[gp][nn][der];
-      L.Q[gp][t] = s;
-    }
-  }
+  if (active && l < NGP * NPE) phase_interp<ET>(L, l);
   __syncthreads();
-
-  // P4: per-Gauss-point derived data, residual contributions of node k, and
-  // the per-(node, gp) factors of the tangent
   if (active && l < NGP * NPE) {
+    double Fk[4];
+    phase_gauss<ET>(L, l, v.charlen[e], p, Fk);
     const int gp = l / NPE, k = l % NPE;
-    const double* Q = L.Q[gp];
-    const double* vp = Q;
-    const double* gpv = Q + D;                  // gradPrev row-major
-    const double* vel = Q + D + D * D;
-    const double* gr = Q + 2 * D + D * D;       // grad row-major
-    const double pres = Q[2 * D + 2 * D * D];
-    const double* dp = Q + 2 * D + 2 * D * D + 1;
-    const double* vd = Q + 3 * D + 2 * D * D + 1;
-    const double rho = p.rho, mu = p.mu, af = p.af;
-
-    double res2[3], rStab[3];
-    for (int a = 0; a < D; ++a) {
-      double s1 = gpv[a * D] * vel[0], s2 = gr[a * D] * vp[0], s3 = gpv[a * D] * vp[0];
-      for (int c = 1; c < D; ++c) {
-        s1 += gpv[a * D + c] * vel[c];
-        s2 += gr[a * D + c] * vp[c];
-        s3 += gpv[a * D + c] * vp[c];
-      }
-      res2[a] = rho * (vd[a] + (s1 + s2 - s3));
-      rStab[a] = res2[a] + dp[a];
-    }
-    // evaluateStabParams_algo3 (stabilisationRoutines.h:128-143), G = 4/h^2 I_ndim
-    const double h = v.charlen[e];
-    const double fact = 4.0 / h / h;
-    double uGu = 0.0;
-    for (int a = 0; a < D; ++a) uGu += vp[a] * (fact * vp[a]);
-    const double G2 = D * (fact * fact), trG = D * fact;
-    double tau0 = 1.0 / sqrt(uGu + C::CI * (mu * mu / rho / rho) * G2);
-    (void)trG;
-    tau0 = tau0 / rho;
-    const double tauS = tau0 * p.stab[0];
-    const double tauP = tau0 * p.stab[1];
-    double trace = gr[0];
-    for (int a = 1; a < D; ++a) trace += gr[a * D + a];
-
-    const double w = L.W[gp];
-    const double Nk = L.phi[gp][k][0];
-    double b[3];
-    for (int c = 0; c < D; ++c) b[c] = L.phi[gp][k][1 + c] * w;
-    const double b4 = Nk * w;
-    double Da = vp[0] * b[0];
-    for (int c = 1; c < D; ++c) Da += vp[c] * b[c];
-    Da = rho * Da * tauS;
-    // residual of node k at gp (Hexa8Node.cpp:1049-1060, Quad4Node.cpp:936-950)
-    for (int a = 0; a < D; ++a) {
-      double s = b4 * res2[a];
-      for (int c = 0; c < D; ++c) s += b[c] * (mu * gr[a * D + c] - (a == c ? pres : 0.0));
-      double f = -s;
-      if constexpr (C::SUPG) f -= Da * rStab[a];
-      L.Fk[gp][k][a] = f;
-    }
-    double br = b[0] * rStab[0];
-    for (int a = 1; a < D; ++a) br += b[a] * rStab[a];
-    L.Fk[gp][k][D] = C::SGN * (-(b4 * trace) - tauP * br);
-    // tangent factors
-    const double stp = C::SGN * tauP;
-    for (int c = 0; c < D; ++c) {
-      double g = 0.0;
-      for (int a = 0; a < D; ++a) g += b[a] * (af * rho * gpv[a * D + c]);
-      L.P[gp][k][c] = stp * g;
-    }
-    double ud = vp[0] * L.phi[gp][k][1];
-    for (int c = 1; c < D; ++c) ud += vp[c] * L.phi[gp][k][1 + c];
-    const double uk = af * rho * ud;
-    L.P[gp][k][3] = uk;
-    L.P[gp][k][4] = uk + rho * p.acceFact * Nk;
-    L.P[gp][k][5] = Da;
-    if (k == 0) {
-      for (int a = 0; a < D; ++a)
-        for (int c = 0; c < D; ++c) L.U[gp][3 * a + c] = af * rho * gpv[a * D + c];
-      L.U[gp][9] = stp;
-    }
+    for (int a = 0; a < NDOF; ++a) L.Fk[gp][k][a] = Fk[a];
   }
   __syncthreads();
 
@@ -484,7 +522,7 @@ __global__ __launch_bounds__(256) void k_assemble(AsmView v, ElemParams p,
             }
           }
           if (mi != 0 && mj != 0) {
-            const int32_t co = v.colofs[(e * NPE + i) * NPE + j];
+            const int32_t co = v.colofs[(slot * NPE + i) * NPE + j];
             const int32_t rs = L.info[i][0], rl = L.info[i][1];
             int ra = 0;
             for (int a = 0; a < NDOF; ++a) {
@@ -502,7 +540,7 @@ __global__ __launch_bounds__(256) void k_assemble(AsmView v, ElemParams p,
             for (int bq = 0; bq < NDOF; ++bq)
               v.probeK[(i * NDOF + a) * (NPE * NDOF) + j * NDOF + bq] = (double)K[a][bq];
         } else if (mi != 0 && mj != 0) {
-          const int32_t co = v.colofs[(e * NPE + i) * NPE + j];
+          const int32_t co = v.colofs[(slot * NPE + i) * NPE + j];
           const int32_t rs = L.info[i][0], rl = L.info[i][1];
           int ra = 0;
           for (int a = 0; a < NDOF; ++a) {
@@ -546,6 +584,221 @@ __global__ __launch_bounds__(256) void k_assemble(AsmView v, ElemParams p,
   }
 }
 
+// Orders the LDS traffic of one wavefront (the whole workgroup of the pipelined kernel):
+// the compiler may not move LDS accesses across it, and it waits for the issued ones.
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Hexa8 element loop of one colour class, pipelined: one wavefront per workgroup walks
+// the slots blockIdx.x, blockIdx.x + gridDim.x, ... of the class (the grid is at most
+// what the GPU holds at once).  Lane l is node pair (i, j) = (l / 8, l % 8) of the
+// tangent, Gauss point / node (gp, k) = (l / 8, l % 8) of the geometry and residual
+// phases, and loads quantity q = l / 8 of node l % 8 of the next element:
+//   q = 0 coordinates, 1 solnPrev, 2 solnCur, 3 solnDotCur, 4 solnApplied (lift only),
+//   5 node info {rowStart, rowlen, free mask, first row}.
+// While one element is computed, the next element's connectivity, element -> nonzero
+// offsets and length scale are in flight (issued before the geometry phase), then its
+// nodal data (issued before the tangent phase); the CSR and RHS values an element adds
+// to are loaded before its tangent is accumulated (no other element of the class owns
+// them).  The residual sums over Gauss points and the lift sums over neighbour nodes are
+// butterfly reductions across lanes.
+// Register budget: two wavefronts per SIMD (the tangent phase holds the 4 x 4 block,
+// the CSR values it adds to and two Gauss points' worth of LDS operands).
+constexpr int kHexOcc = 2;
+constexpr int kHexGpUnroll = 2;  // Gauss points of the tangent loop unrolled together
+
+template <int MODE>
+__global__ __launch_bounds__(64, kHexOcc) void k_assemble_hex8(AsmView v, ElemParams p,
+                                                      const int32_t* __restrict__ elems, int64_t n,
+                                                      int lift) {
+  using T = typename std::conditional<MODE == 2, float, double>::type;
+  constexpr int NPE = 8, NDOF = 4;
+  typedef T vec_t __attribute__((ext_vector_type(4), aligned(sizeof(T))));
+  __shared__ HexLds L;
+  const int l0 = threadIdx.x;
+  const bool doLift = (MODE == 0) && (lift != 0);
+  int nn = l0 & 7, q = l0 >> 3;
+  int64_t s = blockIdx.x;
+  if (s >= n) return;
+
+  // staging registers of the element to come
+  int32_t co = v.colofs[s * 64 + l0];
+  int32_t eid = elems[s];
+  double h = v.charlen[eid];
+  double mu = v.elem_mu ? v.elem_mu[eid] : p.mu;
+  double rv[4] = {0.0, 0.0, 0.0, 0.0};
+  int4 ri = {0, 0, 0, 0};
+  int cnt = 0;
+  auto issue_node = [&](int32_t node) {
+    const double* src = nullptr;
+    cnt = 0;
+    if (q == 0) { src = v.coords + (int64_t)node * 3; cnt = 3; }
+    else if (q <= 3) { src = (q == 1 ? v.solnPrev : q == 2 ? v.solnCur : v.solnDotCur) + (int64_t)node * 4; cnt = 4; }
+    else if (q == 4 && doLift) { src = v.solnApplied + (int64_t)node * 4; cnt = 4; }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) rv[c] = c < cnt ? src[c] : 0.0;
+    if (q == 5) ri = reinterpret_cast<const int4*>(v.nodeInfo)[node];
+  };
+  issue_node(v.sconn[s * NPE + nn]);
+
+  for (;;) {
+    // The lane index is re-derived in every iteration (through an empty asm the compiler
+    // cannot see through), so that per-lane LDS addresses of the phases are recomputed
+    // instead of being hoisted out of the loop and held in registers across it.
+    int l = l0;
+    asm volatile("" : "+v"(l));
+    nn = l & 7;
+    q = l >> 3;
+    const int i = l >> 3, j = l & 7;
+    // stage the element's nodal data
+    if (q < 5) {
+      double* dst = q == 0 ? &L.X[nn][0] : q <= 3 ? &L.S[q - 1][nn][0] : &L.app[nn][0];
+      const int w = q == 0 ? 3 : 4;  // app is zero without lift
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        if (c < w) dst[c] = rv[c];
+    } else if (q == 5) {
+      L.info[nn][0] = ri.x; L.info[nn][1] = ri.y; L.info[nn][2] = ri.z; L.info[nn][3] = ri.w;
+    }
+    const int32_t co_cur = co;
+    ElemParams pe = p;
+    pe.mu = mu;
+    const double h_cur = h;
+    wave_sync();
+    // slot-ordered data of the next element
+    const int64_t sn = s + gridDim.x;
+    const bool more = sn < n;
+    int32_t nid_n = 0;
+    if (more) {
+      nid_n = v.sconn[sn * NPE + nn];
+      co = v.colofs[sn * 64 + l0];
+      eid = elems[sn];
+    }
+    phase_geometry<HEXA8>(L, l);
+    wave_sync();
+    phase_interp<HEXA8>(L, l);
+    wave_sync();
+    double Fk[4];
+    phase_gauss<HEXA8>(L, l, h_cur, pe, Fk);
+    // nodal data and length scale of the next element
+    if (more) {
+      issue_node(nid_n);
+      h = v.charlen[eid];
+      if (v.elem_mu) mu = v.elem_mu[eid];
+    }
+    wave_sync();
+
+    const int mi = L.info[i][2], mj = L.info[j][2];
+    // the RHS entries of node i (lanes j == 0) and, below, the CSR segments of block (i, j)
+    const bool rhs_lane = (j == 0) && (mi != 0);
+    const int32_t r0 = L.info[i][3];
+    double rold[4] = {0.0, 0.0, 0.0, 0.0};
+    if (rhs_lane) {
+      int ra = 0;
+#pragma unroll
+      for (int a = 0; a < NDOF; ++a)
+        if ((mi >> a) & 1) rold[a] = v.rhs[r0 + ra++];
+    }
+    double lsum[4] = {0.0, 0.0, 0.0, 0.0};
+    if constexpr (MODE != 1) {
+      T* vals = MODE == 2 ? reinterpret_cast<T*>(v.vals32) : reinterpret_cast<T*>(v.vals);
+      const bool blk = mi != 0 && mj != 0;
+      const int32_t rs = L.info[i][0], rl = L.info[i][1];
+      // CSR offsets of the free rows of node i in block (i, j), and the values there,
+      // loaded before the tangent is accumulated
+      int64_t roff[NDOF];
+      T old[NDOF][NDOF];
+#pragma unroll
+      for (int a = 0; a < NDOF; ++a) {
+        roff[a] = -1;
+#pragma unroll
+        for (int b = 0; b < NDOF; ++b) old[a][b] = (T)0;
+      }
+      if (blk) {
+        int ra = 0;
+#pragma unroll
+        for (int a = 0; a < NDOF; ++a) {
+          if (!((mi >> a) & 1)) continue;
+          roff[a] = (int64_t)rs + (int64_t)ra * rl + co_cur;
+          ++ra;
+          const T* row = vals + roff[a];
+          if (mj == 15) {
+            const vec_t r = *reinterpret_cast<const vec_t*>(row);
+            old[a][0] = r.x; old[a][1] = r.y; old[a][2] = r.z; old[a][3] = r.w;
+          } else {
+            // free column b of node j sits at the number of free columns before it
+#pragma unroll
+            for (int b = 0; b < NDOF; ++b)
+              if ((mj >> b) & 1) old[a][b] = row[__popc(mj & ((1 << b) - 1))];
+          }
+        }
+      }
+      T K[NDOF][NDOF];
+      accumulate_block<HEXA8, T, kHexGpUnroll>(L, i, j, pe, K);
+      if (doLift) {
+#pragma unroll
+        for (int a = 0; a < NDOF; ++a) {
+          double sl = 0.0;
+#pragma unroll
+          for (int b = 0; b < NDOF; ++b) {
+            const double dlt = L.app[j][b];
+            if (!((mj >> b) & 1) && fabs(dlt) > 1.0e-10) sl += (double)K[a][b] * dlt;
+          }
+          lsum[a] = sl;
+        }
+      }
+      if (blk) {
+#pragma unroll
+        for (int a = 0; a < NDOF; ++a) {
+          if (roff[a] < 0) continue;
+          T* row = vals + roff[a];
+          if (mj == 15) {
+            vec_t r;
+            r.x = old[a][0] + K[a][0]; r.y = old[a][1] + K[a][1];
+            r.z = old[a][2] + K[a][2]; r.w = old[a][3] + K[a][3];
+            *reinterpret_cast<vec_t*>(row) = r;
+          } else {
+#pragma unroll
+            for (int b = 0; b < NDOF; ++b)
+              if ((mj >> b) & 1) row[__popc(mj & ((1 << b) - 1))] = old[a][b] + K[a][b];
+          }
+        }
+      }
+    }
+    // residual: sum over Gauss points (lane bits 3-5), then node i's sum to lane 8i
+#pragma unroll
+    for (int a = 0; a < NDOF; ++a) {
+      double f = Fk[a];
+      f += __shfl_xor(f, 8);
+      f += __shfl_xor(f, 16);
+      f += __shfl_xor(f, 32);
+      Fk[a] = __shfl(f, i);
+    }
+    if (doLift) {
+#pragma unroll
+      for (int a = 0; a < NDOF; ++a) {
+        double t = lsum[a];
+        t += __shfl_xor(t, 1);
+        t += __shfl_xor(t, 2);
+        t += __shfl_xor(t, 4);
+        Fk[a] -= t;
+      }
+    }
+    if (rhs_lane) {
+      int ra = 0;
+#pragma unroll
+      for (int a = 0; a < NDOF; ++a)
+        if ((mi >> a) & 1) { v.rhs[r0 + ra] = rold[a] + Fk[a]; ++ra; }
+    }
+    wave_sync();
+    if (!more) break;
+    s = sn;
+  }
+}
+
 template <int ET>
 static hipError_t prepare_t(hipStream_t s, int64_t nElem, const double* coords, const int32_t* conn,
                             double* charlen, double* vol, int64_t* bad) {
@@ -585,8 +838,34 @@ static hipError_t assemble_t(hipStream_t s, const AsmView& v, const ElemParams& 
   return hipGetLastError();
 }
 
+// Workgroups of the pipelined Hexa8 kernel the current device holds at once (each walks
+// its share of a colour class).
+template <int MODE>
+static hipError_t hex8_launch(hipStream_t s, const AsmView& v, const ElemParams& p, const int32_t* elems,
+                              int64_t n, int lift) {
+  static int64_t resident[16];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) dev = 0;
+  if (resident[dev] == 0) {
+    int cus = 0, per = 0;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_assemble_hex8<MODE>, 64, 0) != hipSuccess || per < 1)
+      per = 4 * kHexOcc;
+    resident[dev] = (int64_t)(cus > 0 ? cus : 256) * per;
+  }
+  const unsigned grid = (unsigned)std::min<int64_t>(n, resident[dev]);
+  hipLaunchKernelGGL((k_assemble_hex8<MODE>), dim3(grid), dim3(64), 0, s, v, p, elems, n, lift);
+  return hipGetLastError();
+}
+
 hipError_t launch_assemble(int et, hipStream_t s, const AsmView& v, const ElemParams& p,
                            const int32_t* elems, int64_t n, int mode, int lift, int64_t probe_e) {
+  if (et == HEXA8 && probe_e < 0) {
+    if (n == 0) return hipSuccess;
+    if (mode == 0) return hex8_launch<0>(s, v, p, elems, n, lift);
+    if (mode == 1) return hex8_launch<1>(s, v, p, elems, n, lift);
+    return hex8_launch<2>(s, v, p, elems, n, lift);
+  }
   switch (et) {
     case TRIA3: return assemble_t<TRIA3>(s, v, p, elems, n, mode, lift, probe_e);
     case QUAD4: return assemble_t<QUAD4>(s, v, p, elems, n, mode, lift, probe_e);

@@ -64,7 +64,8 @@ struct AsmView {
   const double* solnDotCur;
   const double* solnApplied;
   const int32_t* nodeInfo;   // per node: {rowStart, rowlen, free-DOF mask, first free row}
-  const int32_t* colofs;     // nElem*npe*npe, -1 where node j has no free dof
+  const int32_t* colofs;     // [slot][npe][npe] in colour-slot order, -1 where node j has no free dof
+  const int32_t* sconn;      // [slot][npe]: connectivity in colour-slot order
   double* vals;              // CSR values (FP64) or NULL
   float* vals32;             // CSR values (FP32) or NULL
   double* rhs;               // nfree
@@ -180,7 +181,8 @@ struct sfem_ctx {
   double* d_vol = nullptr;
   int64_t* d_bad = nullptr;
   int32_t* d_nodeInfo = nullptr;
-  int32_t* d_colofs = nullptr;
+  int32_t* d_colofs = nullptr;      // element -> nonzero map in colour-slot order
+  int32_t* d_sconn = nullptr;       // connectivity in colour-slot order
   int32_t* d_elemByColor = nullptr;
   int32_t* d_rowptr32 = nullptr;   // int32 copy for kernels (nnz < 2^31 is checked)
   int32_t* d_colidx = nullptr;

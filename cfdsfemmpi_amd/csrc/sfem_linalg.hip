@@ -767,12 +767,16 @@ __global__ __launch_bounds__(64) void k_bsr_gather(int64_t nblk, const BDesc* __
 }
 
 // ILU(0) factorisation of the block rows of one level, one 64-lane workgroup (one
-// wavefront) per block row, IKJ order by lower neighbour blocks: for neighbour block c
-// (ascending), the positions of c's upper blocks in this row's block list are searched
-// once, c's m_c upper rows are staged in LDS, and c's m_c pivots are eliminated from
-// LDS.  The global loads of the next lower block are issued before the eliminations of
-// the current one and land in registers meanwhile.  Dynamic LDS: bsr_factor_lds(cap)
-// bytes for rows of at most cap <= kBsrMaxNb neighbour blocks.
+// wavefront) per block row, IKJ order by lower neighbour blocks, a 4 x 4 block at a time:
+// for lower block c (ascending), c's upper blocks and diagonal block are staged in LDS
+// (their global loads were issued before the previous block's work and sat in
+// registers meanwhile), the positions of c's upper blocks in this row are searched once,
+// L_pc = R_pc U_cc^{-1} is one triangular solve per row (lanes < m), and every upper
+// block of c present in this row is updated, R_pj -= L_pc U_cj, one (block, row,
+// column) entry per lane.  Point ILU(0) on a pattern of full node blocks is exactly
+// this: the subtractions run in the point algorithm's pivot order, so the factor is the
+// same to the last bit.  Dynamic LDS: bsr_factor_lds(cap) bytes for rows of at most
+// cap <= kBsrMaxNb neighbour blocks.
 template <typename LT>
 __host__ __device__ constexpr size_t bsr_factor_lds(int cap) {
   return (size_t)cap * (16 * sizeof(double) + 16 * sizeof(LT) + 2 * sizeof(int32_t));
@@ -790,6 +794,7 @@ __global__ __launch_bounds__(64) void k_bsr_factor(const int32_t* __restrict__ l
   int32_t* pos = cols + cap;                    // [cap], positions of c's upper blocks here
   __shared__ double ub[16];                     // c's diagonal block
   __shared__ double lval[4];
+  __shared__ double lblk[16];                   // L_pc, row-major
   const int w = blockIdx.x;
   if (w >= count) return;
   const int lane = threadIdx.x;
@@ -834,33 +839,36 @@ __global__ __launch_bounds__(64) void k_bsr_factor(const int32_t* __restrict__ l
     if (lane < 16) ub[lane] = (double)pd;
     wave_sync();
     if (t + 1 < td) issue(t + 1);
-    for (int b = 0; b < mc; ++b) {
-      if (lane < m) {
-        const double dd = ub[b * 4 + b];
-        if (dd == 0.0) scal[S_MISC1] = 2.0;
-        const double l = rows[lane * 4 * cap + 4 * t + b] / (dd == 0.0 ? 1.0 : dd);
-        rows[lane * 4 * cap + 4 * t + b] = l;
-        lval[lane] = l;
-      }
-      wave_sync();
-      const int nin = 3 - b;  // remaining columns of block c (padding columns hold 0)
-      for (int i = lane; i < nin + 4 * nup; i += 64) {
-        int q;
-        double u;
-        if (i < nin) {
-          q = 4 * t + b + 1 + i;
-          u = ub[b * 4 + b + 1 + i];
-        } else {
-          const int k = (i - nin) >> 2, bp = (i - nin) & 3;
-          const int ps = pos[k];
-          q = ps >= 0 ? 4 * ps + bp : -1;
-          u = (double)urow[b * 4 * cap + 4 * k + bp];
+    // L_pc: row a (lane a < m) by forward substitution against U_cc
+    if (lane < m) {
+      double* r = rows + lane * 4 * cap + 4 * t;
+      double l[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        if (b < mc) {
+          double x = r[b];
+#pragma unroll
+          for (int bq = 0; bq < b; ++bq) x -= l[bq] * ub[bq * 4 + b];
+          const double dd = ub[b * 4 + b];
+          if (dd == 0.0) scal[S_MISC1] = 2.0;
+          l[b] = x / (dd == 0.0 ? 1.0 : dd);
+          r[b] = l[b];
         }
-        if (q >= 0)
-          for (int a = 0; a < m; ++a) rows[a * 4 * cap + q] -= lval[a] * u;
       }
-      wave_sync();
+#pragma unroll
+      for (int b = 0; b < 4; ++b) lblk[lane * 4 + b] = l[b];
     }
+    wave_sync();
+    // R_pj -= L_pc U_cj over the upper blocks of c that this row holds
+    for (int it = lane; it < 16 * nup; it += 64) {
+      const int k = it >> 4, a = (it >> 2) & 3, bp = it & 3;
+      const int ps = pos[k];
+      if (ps < 0 || a >= m) continue;
+      double x = rows[a * 4 * cap + 4 * ps + bp];
+      for (int b = 0; b < mc; ++b) x -= lblk[a * 4 + b] * (double)urow[b * 4 * cap + 4 * k + bp];
+      rows[a * 4 * cap + 4 * ps + bp] = x;
+    }
+    wave_sync();
   }
   // the row's own diagonal block
   const int q0 = 4 * td;
