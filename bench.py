@@ -412,7 +412,9 @@ def main():
                     help="mixed path: absolute Newton tolerance on the FP64 residual")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None)
-    ap.add_argument("--reorder", type=int, default=2, help="ILU(0) node ordering: 0 natural, 1 RCM, 2 multicolour")
+    ap.add_argument("--reorder", type=int, default=None,
+                    help="ILU(0) node ordering: 0 natural, 1 RCM, 2 multicolour, 3 tiled multicolour "
+                         "(default: 3 for the hex cavities, 2 for the c4 batch)")
     ap.add_argument("--bf16-factors", type=int, default=0,
                     help="mixed path: ILU(0) factors stored in bfloat16 (8%% cheaper applies, ~10%% more iterations)")
     ap.add_argument("--predictor", choices=["previous", "extrapolate"], default="previous",
@@ -433,6 +435,8 @@ def main():
         sys.exit(2)
     from cfdsfemmpi_amd import sfem
     sfem.load_library()
+    if a.reorder is None:
+        a.reorder = 2 if a.workload == "c4" else 3
     if a.workload == "c4":
         main_c4(a)
         return
@@ -522,7 +526,7 @@ def main():
         "data": "synthetic (generated mesh, zero initial state)",
         "config": {"workload": f"3D lid-driven cavity, {n}^3 graded Hexa8, Re=1000, gen-alpha rho_inf=0, "
                                f"dt=0.1, {a.steps} steps", "n_elements": mesh.nElem, "nfree": nf, "nnz": nnz,
-                   "solver": ("BiCGStab" if a.ksp == "bcgs" else f"GMRES({a.restart})") + " + ILU(0)", "ilu_ordering": ["natural", "rcm", "multicolour"][a.reorder],
+                   "solver": ("BiCGStab" if a.ksp == "bcgs" else f"GMRES({a.restart})") + " + ILU(0)", "ilu_ordering": ["natural", "rcm", "multicolour", "tiled multicolour"][a.reorder],
                    "newton_tol": a.newton_tol, "newton_tol_mixed": a.newton_tol_mixed,
                    "inner_rtol_mixed": a.inner_rtol, "ilu_factors_mixed": "bf16" if a.bf16_factors else "fp32",
                    "newton_initial_iterate": a.predictor, "pc_lag_mixed": a.pc_lag_mixed, "parallelism": "single GPU"},

@@ -773,6 +773,12 @@ int sfem_get_ilu(sfem_ctx* c, int64_t* nnz, int32_t* perm, int32_t* rowptr, int3
   return la_get_ilu(c, nnz, perm, rowptr, colidx, vals);
 }
 
+int sfem_pc_apply(sfem_ctx* c, const double* r, double* z) {
+  if (!c || !r || !z) return set_error(SFEM_ERR_ARG, "null argument");
+  HIP_OK(hipSetDevice(c->device));
+  return la_pc_apply_host(c, r, z);
+}
+
 int sfem_block_info(sfem_ctx* c, int64_t* nblk, int64_t* npairs, int64_t* nvals) {
   if (!c) return set_error(SFEM_ERR_ARG, "null context");
   return la_block_info(c, nblk, npairs, nvals);

@@ -37,6 +37,14 @@ struct IluPlan {
   int32_t* d_padmap = nullptr;   // padded slot -> DOF row, -1 for padding
   int32_t* d_natrows = nullptr;  // block rows in the original node order (reordered plans)
   std::vector<int32_t> h_order;  // new position -> original block
+  // tiled multicolour plan (reorder 3): the block rows of a tile (the node blocks of one
+  // element, at most 8) are consecutive, tiles of a colour share no neighbour, and the
+  // sweeps run one launch per colour with one wavefront per tile
+  bool tiled = false;
+  bool tile_fuse_last = false;         // the last colour's tiles have no upper blocks outside themselves
+  std::vector<int32_t> h_tileStart;    // tile t = block rows [h_tileStart[t], h_tileStart[t+1])
+  std::vector<int32_t> tileColPtr;     // colour c = tiles [tileColPtr[c], tileColPtr[c+1])
+  int32_t* d_tileStart = nullptr;
   ~IluPlan();
 };
 
@@ -138,6 +146,7 @@ int la_prepare(sfem_ctx* ctx, const sfem_solver_opts* o);
 int la_tuning(sfem_ctx* ctx, int* out);
 int la_block_info(sfem_ctx* ctx, int64_t* nblk, int64_t* npairs, int64_t* nvals);
 int la_get_ilu(sfem_ctx* ctx, int64_t* nnz_out, int32_t* perm, int32_t* rowptr, int32_t* colidx, double* vals);
+int la_pc_apply_host(sfem_ctx* ctx, const double* r, double* z);
 int la_tuning_levels(sfem_ctx* ctx, int fp32, int* out, int cap, int* nlevels);
 int la_norm2(sfem_ctx* ctx, const double* a, double* out);
 int la_spmv64(sfem_ctx* ctx, const double* x, double* y);

@@ -78,6 +78,7 @@ ABI = [
     ("sfem_enable_kernel_timing", C.c_int, [_vp, C.c_int]),
     ("sfem_kernel_timing", C.c_int, [_vp, C.c_int, _i64p, _f64p]),
     ("sfem_get_ilu", C.c_int, [_vp, _i64p, _i32p, _i32p, _i32p, _f64p]),
+    ("sfem_pc_apply", C.c_int, [_vp, _f64p, _f64p]),
     ("sfem_solver_stats", C.c_int, [_vp, _i64p, _i64p]),
     ("sfem_set_batch", C.c_int, [_vp, C.c_int32, _i32p, _f64p]),
     ("sfem_step_batch", C.c_int, [_vp, C.c_double, C.POINTER(SolverOpts), C.c_double, C.c_int, _i32p, _i32p, _f64p]),
@@ -325,6 +326,13 @@ class Context:
         _check(self.lib.sfem_get_ilu(self.h, C.byref(nnz), _p(perm, C.c_int32), _p(rowptr, C.c_int32),
                                      _p(colidx, C.c_int32), _p(vals, C.c_double)), "sfem_get_ilu")
         return perm[:n], rowptr, colidx[:nnz.value], vals[:nnz.value]
+
+    def pc_apply(self, r):
+        """z = M^{-1} r with the preconditioner of the last solve (DOF order)."""
+        r = np.ascontiguousarray(r, np.float64)
+        z = np.zeros(max(self.nfree, 1))
+        _check(self.lib.sfem_pc_apply(self.h, _p(r, C.c_double), _p(z, C.c_double)), "sfem_pc_apply")
+        return z[:self.nfree]
 
     def tuning(self):
         """Autotuned kernel shapes per precision: SpMV (G, U, XCD order) and the

@@ -60,7 +60,9 @@ typedef struct sfem_solver_opts {
   int restart;       /* -ksp_gmres_restart */
   int reorder;       /* ILU(0) node ordering: 0 the matrix's own order, 1 reverse Cuthill-McKee
                         (PETSc's -pc_factor_mat_ordering_type rcm), 2 greedy multicolour
-                        (one fully parallel level per colour) */
+                        (one fully parallel level per colour), 3 tiled multicolour (the node
+                        blocks of one element form a tile, tiles are coloured; one launch per
+                        colour and sweep, sequential inside a tile) */
   int factor_bf16;   /* SFEM_PREC_MIXED / SFEM_PREC_STORE32 only: keep the ILU(0) factors in
                         bfloat16 (computed in FP32 / FP64, rounded to nearest even) to halve
                         the preconditioner's bytes */
@@ -204,6 +206,12 @@ int sfem_kernel_timing(sfem_ctx* ctx, int kind, int64_t* launches, double* total
  * *nnz receives the count.  The reference's factor lives inside PETSc's PCILU
  * (SolverPetsc.cpp:234-277) and is not exposed; this exists for parity checks. */
 int sfem_get_ilu(sfem_ctx* ctx, int64_t* nnz, int32_t* perm, int32_t* rowptr, int32_t* colidx, double* vals);
+
+/* z = M^{-1} r with the preconditioner of the last solve (r, z: nfree entries in DOF
+ * order; FP32/BF16 factors are applied to FP64 vectors).  The reference applies its
+ * preconditioner only inside KSPSolve (PCApply, SolverPetsc.cpp:234-277); this exists
+ * for parity checks of the triangular solves. */
+int sfem_pc_apply(sfem_ctx* ctx, const double* r, double* z);
 
 /* Batched meshes.  The context's mesh is a disjoint union of nmesh meshes, nodes and
  * elements numbered mesh by mesh; elem_mesh[e] is the mesh of element e and mu[m] the

@@ -200,7 +200,8 @@ def test_cavity3d_vs_oracle():
 
 
 @pytest.mark.parametrize("ksp,pc,reorder", [("bcgs", "ilu", 0), ("bcgs", "ilu", 1), ("bcgs", "ilu", 2),
-                                            ("gmres", "ilu", 2), ("bcgs", "bjacobi4", 0),
+                                            ("gmres", "ilu", 2), ("bcgs", "ilu", 3), ("gmres", "ilu", 3),
+                                            ("bcgs", "bjacobi4", 0),
                                             ("bcgs", "jacobi", 0), ("gmres", "ilu", 0), ("gmres", "ilu", 1),
                                             ("gmres", "bjacobi4", 0)])
 def test_krylov_against_direct(ksp, pc, reorder):
@@ -225,7 +226,7 @@ def test_krylov_against_direct(ksp, pc, reorder):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-@pytest.mark.parametrize("reorder", [0, 1, 2])
+@pytest.mark.parametrize("reorder", [0, 1, 2, 3])
 def test_ilu_factor_vs_oracle(reorder, prec):
     """The device ILU(0) factor against the oracle's IKJ ILU(0) of the same reordered matrix."""
     import scipy.sparse as sp
@@ -251,6 +252,45 @@ def test_ilu_factor_vs_oracle(reorder, prec):
         worst = max(worst, np.max(np.abs(lu[a:b] - ref[a:b])) / np.max(np.abs(ref[a:b])))
     print(f"[ilu] worst row-relative factor difference {worst:.2e}", flush=True)
     assert worst <= (1e-11 if prec == "fp64" else 1e-4)
+    ctx.close()
+
+
+@pytest.mark.parametrize("mesh_kind,prec,bf16", [("hex", "fp64", 0), ("hex", "mixed", 0), ("hex", "mixed", 1),
+                                                ("quad", "fp64", 0), ("tria", "fp64", 0), ("tet", "fp64", 0)])
+@pytest.mark.parametrize("reorder", [0, 2, 3])
+def test_ilu_apply_vs_factor(reorder, mesh_kind, prec, bf16):
+    """z = M^{-1} r from the device sweeps (level kernels, or the tiled kernels with
+    reorder 3) against forward and backward substitution with the exported factor."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    if mesh_kind == "hex":
+        mesh = meshgen.ldc3d_hex(6, graded=True)
+    elif mesh_kind == "quad":
+        mesh = meshgen.ldc2d_quad(9)
+    elif mesh_kind == "tria":
+        mesh = meshgen.ldc2d_tria(9, jitter=0.2, seed=4)
+    else:
+        mesh = meshgen.ldc3d_tet(4)
+    ctx = make_ctx(mesh, tis=2, dt=0.1, elemData=np.array([1.0, 1e-2, 0, 0, 0, 0, 0, 0, 1, 1, 0]))
+    if prec == "mixed":
+        ctx.set_precision(sfem.PREC_MIXED)
+    ctx.time_update(1.0)
+    ctx.assemble(0)
+    ctx.solve(sfem.solver_opts("bcgs", "ilu", 1e-6, 5000, 30, reorder, factor_bf16=bf16))
+    perm, prp, pci, lu = ctx.get_ilu()
+    n = ctx.nfree
+    print(f"[case] ilu apply {mesh_kind} reorder={reorder} {prec} bf16={bf16} n={n}", flush=True)
+    F = sp.csr_matrix((lu, pci, prp), shape=(n, n))
+    Lm = (sp.tril(F, -1) + sp.identity(n)).tocsr()
+    Um = sp.triu(F).tocsr()
+    r = np.random.default_rng(7).standard_normal(n)
+    zp = spla.spsolve_triangular(Um, spla.spsolve_triangular(Lm, r[perm], lower=True), lower=False)
+    z_ref = np.empty(n)
+    z_ref[perm] = zp
+    z = ctx.pc_apply(r)
+    err = np.linalg.norm(z - z_ref) / np.linalg.norm(z_ref)
+    print(f"[ilu-apply] rel diff {err:.2e}", flush=True)
+    assert err <= (1e-12 if prec == "fp64" else 1e-4)
     ctx.close()
 
 
