@@ -738,15 +738,15 @@ __global__ __launch_bounds__(256) void k_bsr_usolve(const int32_t* __restrict__ 
   if (act && lane < 4) y[4 * (int64_t)p + lane] = (T)pick4(v, lane);
 }
 
-// Triangular solves of a tiled plan, one launch per colour, one wavefront per tile and
-// G = 8 lanes per block row of the tile (row g of tile = lanes 8g..8g+7).  First every
-// row sums its blocks outside the tile against the vector entries earlier launches
-// wrote (lower blocks forward, upper blocks backward); then the rows of the tile are
-// finished one after another (forward in tile order, backward in reverse), each
-// subtracting its blocks against the rows of the tile already finished (kept in LDS)
-// and solving its diagonal block.  The intra-tile blocks of a row are the ones next to
-// its diagonal block (the tile's rows are consecutive in the plan order).
-constexpr int kTileG = 8;
+// Triangular solves of a tiled plan, one launch per colour and one workgroup of 8 G
+// lanes per tile, G lanes per block row of the tile (row g = lanes G g .. G g + G - 1).
+// First every row sums its blocks outside the tile against the vector entries earlier
+// launches wrote (lower blocks forward, upper blocks backward), and loads its blocks
+// inside the tile into registers; then the rows of the tile are finished one after
+// another (forward in tile order, backward in reverse), each subtracting its blocks
+// against the rows of the tile already finished (kept in LDS) and solving its diagonal
+// block.  The intra-tile blocks of a row are the ones next to its diagonal block (the
+// tile's rows are consecutive in the plan order and all coupled).
 constexpr int kTileMax = 8;
 
 template <typename AT>
@@ -754,25 +754,29 @@ __device__ inline AT dot4(const V4<AT>& a, const AT* b) {
   return a.v[0] * b[0] + a.v[1] * b[1] + a.v[2] * b[2] + a.v[3] * b[3];
 }
 
+// Orders the tile's LDS exchanges: a wavefront-scope barrier when the tile is one
+// wavefront, the workgroup barrier otherwise.
+template <int G>
+__device__ inline void tile_sync() {
+  if constexpr (G * kTileMax <= 64) wave_sync();
+  else __syncthreads();
+}
+
 // Forward sweep of one colour: y = L^{-1} r on its tiles.  With fuse_u the tiles have no
 // upper blocks outside themselves (the last colour) and their backward sweep follows.
-template <typename T, typename LT>
-__global__ __launch_bounds__(256) void k_bsr_tile_lsolve(const int32_t* __restrict__ tstart, int tfirst,
-                                                         int tcount, const BDesc* __restrict__ desc,
-                                                         const int32_t* __restrict__ bcol,
-                                                         const LT* __restrict__ lu, const T* __restrict__ r,
-                                                         T* __restrict__ y, int fuse_u) {
+template <typename T, typename LT, int G>
+__global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_lsolve(const int32_t* __restrict__ tstart, int tfirst,
+                                                                  int tcount, const BDesc* __restrict__ desc,
+                                                                  const int32_t* __restrict__ bcol,
+                                                                  const LT* __restrict__ lu, const T* __restrict__ r,
+                                                                  T* __restrict__ y, int fuse_u) {
   using AT = TriAcc<T, LT>;
-  constexpr int G = kTileG;
-  __shared__ AT ys[4][kTileMax][4];
-  const int wv = threadIdx.x >> 6, l64 = threadIdx.x & 63;
-  const int g = l64 / G, lane = l64 % G;
-  const int tix = blockIdx.x * 4 + wv;
-  int ts = 0, tsz = 0;
-  if (tix < tcount) {
-    ts = tstart[tfirst + tix];
-    tsz = tstart[tfirst + tix + 1] - ts;
-  }
+  __shared__ AT Y[kTileMax][4];
+  const int g = threadIdx.x / G, lane = threadIdx.x % G;
+  const int tix = blockIdx.x;
+  if (tix >= tcount) return;
+  const int ts = tstart[tfirst + tix];
+  const int tsz = tstart[tfirst + tix + 1] - ts;
   const bool act = g < tsz;
   const int32_t p = ts + g;
   BDesc d = {0, 0, 0, 0};
@@ -802,7 +806,6 @@ __global__ __launch_bounds__(256) void k_bsr_tile_lsolve(const int32_t* __restri
   for (int a = 0; a < 4; ++a) acc[a] = group_sum<G>(acc[a]);
 #pragma unroll
   for (int a = 0; a < 4; ++a) R[a] = __shfl(rv, a, G);
-  AT (*Y)[4] = ys[wv];
   // the loops over tile rows are unrolled so that lin[] / uin[] stay in registers
 #pragma unroll
   for (int h = 0; h < kTileMax; ++h) {
@@ -826,7 +829,7 @@ __global__ __launch_bounds__(256) void k_bsr_tile_lsolve(const int32_t* __restri
       }
       if (lane < 4) Y[h][lane] = pick4(v, lane);
     }
-    wave_sync();
+    tile_sync<G>();
   }
   if (fuse_u) {
     // backward within the tile: rows in reverse, upper blocks next to the diagonal
@@ -864,29 +867,25 @@ __global__ __launch_bounds__(256) void k_bsr_tile_lsolve(const int32_t* __restri
         }
         if (lane < 4) Y[h][lane] = pick4(v, lane);
       }
-      wave_sync();
+      tile_sync<G>();
     }
   }
   if (act && lane < 4) y[4 * (int64_t)p + lane] = (T)Y[g][lane];
 }
 
 // Backward sweep of one colour, in place: z = U^{-1} y on its tiles.
-template <typename T, typename LT>
-__global__ __launch_bounds__(256) void k_bsr_tile_usolve(const int32_t* __restrict__ tstart, int tfirst,
-                                                         int tcount, const BDesc* __restrict__ desc,
-                                                         const int32_t* __restrict__ bcol,
-                                                         const LT* __restrict__ lu, T* __restrict__ y) {
+template <typename T, typename LT, int G>
+__global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_usolve(const int32_t* __restrict__ tstart, int tfirst,
+                                                                  int tcount, const BDesc* __restrict__ desc,
+                                                                  const int32_t* __restrict__ bcol,
+                                                                  const LT* __restrict__ lu, T* __restrict__ y) {
   using AT = TriAcc<T, LT>;
-  constexpr int G = kTileG;
-  __shared__ AT zs[4][kTileMax][4];
-  const int wv = threadIdx.x >> 6, l64 = threadIdx.x & 63;
-  const int g = l64 / G, lane = l64 % G;
-  const int tix = blockIdx.x * 4 + wv;
-  int ts = 0, tsz = 0;
-  if (tix < tcount) {
-    ts = tstart[tfirst + tix];
-    tsz = tstart[tfirst + tix + 1] - ts;
-  }
+  __shared__ AT Z[kTileMax][4];
+  const int g = threadIdx.x / G, lane = threadIdx.x % G;
+  const int tix = blockIdx.x;
+  if (tix >= tcount) return;
+  const int ts = tstart[tfirst + tix];
+  const int tsz = tstart[tfirst + tix + 1] - ts;
   const bool act = g < tsz;
   const int32_t p = ts + g;
   BDesc d = {0, 0, 0, 0};
@@ -917,8 +916,7 @@ __global__ __launch_bounds__(256) void k_bsr_tile_usolve(const int32_t* __restri
   for (int a = 0; a < 4; ++a) acc[a] = group_sum<G>(acc[a]);
 #pragma unroll
   for (int a = 0; a < 4; ++a) R[a] = __shfl(rv, a, G);
-  AT (*Z)[4] = zs[wv];
-  // the loops over tile rows are unrolled so that uin[] stays in registers
+  // the loop over tile rows is unrolled so that uin[] stays in registers
 #pragma unroll
   for (int h = kTileMax - 1; h >= 0; --h) {
     if (h >= tsz) continue;
@@ -944,9 +942,35 @@ __global__ __launch_bounds__(256) void k_bsr_tile_usolve(const int32_t* __restri
       }
       if (lane < 4) Z[h][lane] = pick4(v, lane);
     }
-    wave_sync();
+    tile_sync<G>();
   }
   if (act && lane < 4) y[4 * (int64_t)p + lane] = (T)Z[g][lane];
+}
+
+// Tile sweep shapes: code c -> G = kTileGs[c] lanes per block row.
+static const int kTileGs[3] = {8, 16, 32};
+constexpr int kTileCodes = 3;
+constexpr int kTileDefault = 1;
+
+template <typename T, typename LT, bool UPPER, int G>
+static void tile_launch_g(hipStream_t st, const IluPlan& P, int t0, int cnt, const LT* lu, const T* r, T* y,
+                          int fuse_u) {
+  if constexpr (UPPER)
+    hipLaunchKernelGGL((k_bsr_tile_usolve<T, LT, G>), dim3((unsigned)cnt), dim3(G * kTileMax), 0, st,
+                       P.d_tileStart, t0, cnt, P.d_bdesc, P.d_bcol, lu, y);
+  else
+    hipLaunchKernelGGL((k_bsr_tile_lsolve<T, LT, G>), dim3((unsigned)cnt), dim3(G * kTileMax), 0, st,
+                       P.d_tileStart, t0, cnt, P.d_bdesc, P.d_bcol, lu, r, y, fuse_u);
+}
+
+template <typename T, typename LT, bool UPPER>
+static void tile_launch(hipStream_t st, int code, const IluPlan& P, int t0, int cnt, const LT* lu, const T* r, T* y,
+                        int fuse_u = 0) {
+  switch (code) {
+    case 0: tile_launch_g<T, LT, UPPER, 8>(st, P, t0, cnt, lu, r, y, fuse_u); break;
+    case 2: tile_launch_g<T, LT, UPPER, 32>(st, P, t0, cnt, lu, r, y, fuse_u); break;
+    default: tile_launch_g<T, LT, UPPER, 16>(st, P, t0, cnt, lu, r, y, fuse_u); break;
+  }
 }
 
 // Block values from the CSR values: block (p, t) is the neighbour's free columns in the
@@ -1870,22 +1894,23 @@ static void ilu_solve(sfem_ctx* ctx, const IluPlan& P, const LT* lu, const T* r,
   const int nu = (int)P.levUPtr.size() - 1;
   if (ev) (void)hipEventRecord(ev[0], st);
   if (P.tiled && !sub) {
-    // one launch per colour and sweep, one wavefront per tile
+    // one launch per colour and sweep, one workgroup per tile; with ev an event after
+    // every launch (forward colours, then backward colours)
     const int nc = (int)P.tileColPtr.size() - 1;
     const int tf = P.tile_fuse_last ? 1 : 0;
     for (int c = 0; c < nc; ++c) {
       const int t0 = P.tileColPtr[c], cnt = P.tileColPtr[c + 1] - t0;
-      if (cnt)
-        hipLaunchKernelGGL((k_bsr_tile_lsolve<T, LT>), dim3((unsigned)((cnt + 3) / 4)), dim3(256), 0, st,
-                           P.d_tileStart, t0, cnt, P.d_bdesc, P.d_bcol, lu, r, z, c == nc - 1 ? tf : 0);
+      const int code = force >= 0 ? force : c < (int)tl.size() ? tl[c] : kTileDefault;
+      if (cnt) tile_launch<T, LT, false>(st, code, P, t0, cnt, lu, r, z, c == nc - 1 ? tf : 0);
+      if (ev) (void)hipEventRecord(ev[1 + c], st);
     }
-    for (int c = nc - 1 - tf; c >= 0; --c) {
+    for (int k = 0; k < nc - tf; ++k) {
+      const int c = nc - 1 - tf - k;
       const int t0 = P.tileColPtr[c], cnt = P.tileColPtr[c + 1] - t0;
-      if (cnt)
-        hipLaunchKernelGGL((k_bsr_tile_usolve<T, LT>), dim3((unsigned)((cnt + 3) / 4)), dim3(256), 0, st,
-                           P.d_tileStart, t0, cnt, P.d_bdesc, P.d_bcol, lu, z);
+      const int code = force >= 0 ? force : k < (int)tu.size() ? tu[k] : kTileDefault;
+      if (cnt) tile_launch<T, LT, true>(st, code, P, t0, cnt, lu, z, z);
+      if (ev) (void)hipEventRecord(ev[1 + nc + k], st);
     }
-    if (ev) (void)hipEventRecord(ev[1], st);
     return;
   }
   // the last forward level does the first backward level too when that is the same set
@@ -2064,22 +2089,22 @@ static int autotune(sfem_ctx* ctx, T* x, T* y) {
       if (ms < best) { best = ms; bestc = *cur; }
     }
   *cur = bestc;
-  if (block && L->ilu->tiled) {
-    L->tune_L[slot].clear();  // the tiled sweeps have a single shape
-    L->tune_U[slot].clear();
-    L->tune_plan[slot] = L->plan_id;
-  } else if (block) {
+  if (block) {
     const IluPlan& P = *L->ilu;
-    const int nl = (int)P.levLPtr.size() - 1, nu = (int)P.levUPtr.size() - 1;
+    // launches per sweep: levels, or colours for a tiled plan (whose last colour may do
+    // its backward sweep within the forward launch)
+    const int ntc = P.tiled ? (int)P.tileColPtr.size() - 1 : 0;
+    const int nl = P.tiled ? ntc : (int)P.levLPtr.size() - 1;
+    const int nu = P.tiled ? ntc - (P.tile_fuse_last ? 1 : 0) : (int)P.levUPtr.size() - 1;
+    const int ncode = P.tiled ? kTileCodes : 2 * kBlkCodes;
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     ev.assign(nl + nu + 1, nullptr);
     for (hipEvent_t& e : ev)
       if (hipEventCreate(&e) != hipSuccess) { e = nullptr; cleanup(); return set_error(SFEM_ERR_HIP, "autotune events"); }
     std::vector<float> bt(nl + nu, 1e30f), acc(nl + nu);
-    std::vector<int> bc(nl + nu, kBlkDefault);
-    for (int xcd = 0; xcd < 2; ++xcd)
-      for (int vi = 0; vi < kBlkCodes; ++vi) {
-        const int code = vi | (xcd << 5);
+    std::vector<int> bc(nl + nu, P.tiled ? kTileDefault : kBlkDefault);
+    for (int ci = 0; ci < ncode; ++ci) {
+        const int code = P.tiled ? ci : (ci % kBlkCodes) | ((ci / kBlkCodes) << 5);
         ilu_apply<T>(ctx, x, y, code);  // warm
         std::fill(acc.begin(), acc.end(), 0.f);
         for (int k = 0; k < 3; ++k) {
@@ -2191,10 +2216,13 @@ int la_tuning_levels(sfem_ctx* ctx, int fp32, int* out, int cap, int* nlevels) {
   std::vector<int> all(L->tune_L[k]);
   all.insert(all.end(), L->tune_U[k].begin(), L->tune_U[k].end());
   *nlevels = (int)all.size();
+  const bool tiled = L->ilu && L->ilu->tiled;
   for (int i = 0; i < (int)all.size() && i < cap; ++i) {
-    out[3 * i + 0] = kBlkG[all[i] & 31];
-    out[3 * i + 1] = kBlkK[all[i] & 31];
-    out[3 * i + 2] = all[i] >> 5;
+    // (lanes per block row, blocks per lane loaded together, XCD order); a tiled plan's
+    // launches have G lanes per row and one block in flight per lane
+    out[3 * i + 0] = tiled ? kTileGs[all[i]] : kBlkG[all[i] & 31];
+    out[3 * i + 1] = tiled ? 1 : kBlkK[all[i] & 31];
+    out[3 * i + 2] = tiled ? 0 : all[i] >> 5;
   }
   return SFEM_OK;
 }
