@@ -740,19 +740,21 @@ __global__ __launch_bounds__(256) void k_bsr_usolve(const int32_t* __restrict__ 
 
 // Triangular solves of a tiled plan, one launch per colour and one workgroup of 8 G
 // lanes per tile, G lanes per block row of the tile (row g = lanes G g .. G g + G - 1).
-// First every row sums its blocks outside the tile against the vector entries earlier
-// launches wrote (lower blocks forward, upper blocks backward), and loads its blocks
-// inside the tile into registers; then the rows of the tile are finished one after
-// another (forward in tile order, backward in reverse), each subtracting its blocks
-// against the rows of the tile already finished (kept in LDS) and solving its diagonal
-// block.  The intra-tile blocks of a row are the ones next to its diagonal block (the
-// tile's rows are consecutive in the plan order and all coupled).
+// Every row first sums its blocks outside the tile against the vector entries earlier
+// launches wrote (lower blocks forward, upper blocks backward).  The tile's own 32 x 32
+// triangle (the 4 slots of its 8 rows) is then applied as a product with its inverse,
+// computed once per factorisation (k_tile_invert), so the rows of a tile need no
+// sequential steps: slot i = 4 g + a (lane a < 4 of row g) takes the dot product of row
+// i of the inverse with the tile's right-hand sides, staged in LDS.
 constexpr int kTileMax = 8;
+constexpr int kTileN = 4 * kTileMax;                 // slots of a tile
+constexpr int kTileTri = kTileN * (kTileN + 1) / 2;  // packed triangle
+constexpr int kTileInv = 2 * kTileTri;               // L^{-1} then U^{-1}
 
-template <typename AT>
-__device__ inline AT dot4(const V4<AT>& a, const AT* b) {
-  return a.v[0] * b[0] + a.v[1] * b[1] + a.v[2] * b[2] + a.v[3] * b[3];
-}
+// Packed offsets: row i of the lower triangle holds columns 0..i, row i of the upper
+// triangle columns i..kTileN-1.
+__host__ __device__ constexpr int tri_lo(int i) { return i * (i + 1) / 2; }
+__host__ __device__ constexpr int tri_up(int i) { return kTileTri + i * kTileN - i * (i - 1) / 2; }
 
 // Orders the tile's LDS exchanges: a wavefront-scope barrier when the tile is one
 // wavefront, the workgroup barrier otherwise.
@@ -762,189 +764,176 @@ __device__ inline void tile_sync() {
   else __syncthreads();
 }
 
+// Inverses of the triangular factors of each tile's own 32 x 32 block (padding slots and
+// missing rows are identity): column j of L^{-1} (unit lower) and of U^{-1} by lane j, in
+// FP64, from the factor values the sweeps read.  One 64-lane workgroup per tile.
+template <typename LT, typename IT>
+__global__ __launch_bounds__(64) void k_tile_invert(const int32_t* __restrict__ tstart, int ntiles,
+                                                    const BDesc* __restrict__ desc, const LT* __restrict__ lu,
+                                                    IT* __restrict__ tinv, double* __restrict__ scal) {
+  __shared__ double A[kTileN][kTileN + 1];
+  __shared__ double X[kTileN][kTileN + 1];
+  const int t = blockIdx.x;
+  if (t >= ntiles) return;
+  const int lane = threadIdx.x;
+  const int ts = tstart[t], tsz = tstart[t + 1] - ts;
+  for (int q = lane; q < kTileN * kTileN; q += 64) {
+    const int i = q / kTileN, j = q % kTileN;
+    A[i][j] = i == j ? 1.0 : 0.0;
+  }
+  wave_sync();
+  // row a of block (g, h) of the tile: lane loops over (g, a, h)
+  for (int q = lane; q < tsz * 4 * tsz; q += 64) {
+    const int g = q / (4 * tsz), a = (q / tsz) % 4, h = q % tsz;
+    const BDesc d = desc[ts + g];
+    const int m = d.dm >> 16, td = d.dm & 0xffff;
+    if (a >= m) continue;
+    const V4<double> v = ld4<double>(lu + d.vbase + (int64_t)a * 4 * d.nb + 4 * (td - g + h));
+#pragma unroll
+    for (int b = 0; b < 4; ++b) A[4 * g + a][4 * h + b] = v.v[b];
+  }
+  wave_sync();
+  if (lane < kTileN) {
+    const int j = lane;
+    // L^{-1} e_j: unit lower triangle of A
+    for (int i = 0; i < kTileN; ++i) X[i][j] = 0.0;
+    X[j][j] = 1.0;
+    for (int i = j + 1; i < kTileN; ++i) {
+      double x = 0.0;
+      for (int k = j; k < i; ++k) x -= A[i][k] * X[k][j];
+      X[i][j] = x;
+    }
+  }
+  wave_sync();
+  IT* out = tinv + (int64_t)t * kTileInv;
+  for (int q = lane; q < kTileTri; q += 64) {
+    int i = 0;
+    while (tri_lo(i + 1) <= q) ++i;
+    out[q] = (IT)X[i][q - tri_lo(i)];
+  }
+  wave_sync();
+  if (lane < kTileN) {
+    const int j = lane;
+    // U^{-1} e_j: upper triangle of A with its diagonal
+    for (int i = 0; i < kTileN; ++i) X[i][j] = 0.0;
+    const double dj = A[j][j];
+    if (dj == 0.0) scal[S_MISC1] = 2.0;
+    X[j][j] = 1.0 / (dj == 0.0 ? 1.0 : dj);
+    for (int i = j - 1; i >= 0; --i) {
+      double x = 0.0;
+      for (int k = i + 1; k <= j; ++k) x -= A[i][k] * X[k][j];
+      const double di = A[i][i];
+      if (di == 0.0) scal[S_MISC1] = 2.0;
+      X[i][j] = x / (di == 0.0 ? 1.0 : di);
+    }
+  }
+  wave_sync();
+  for (int q = lane; q < kTileTri; q += 64) {
+    int i = 0;
+    while (i + 1 < kTileN && tri_up(i + 1) - kTileTri <= q) ++i;
+    out[kTileTri + q] = (IT)X[i][i + (q - (tri_up(i) - kTileTri))];
+  }
+}
+
+// y_i = sum over the packed row of a triangle (row `row`, columns [c0, c1)) times v.
+template <typename IT, typename AT>
+__device__ inline AT tile_row_dot(const IT* __restrict__ row, int c0, int c1, const AT* v) {
+  AT s = 0;
+#pragma unroll 8
+  for (int j = c0; j < c1; ++j) s += (AT)row[j - c0] * v[j];
+  return s;
+}
+
 // Forward sweep of one colour: y = L^{-1} r on its tiles.  With fuse_u the tiles have no
 // upper blocks outside themselves (the last colour) and their backward sweep follows.
-template <typename T, typename LT, int G>
+template <typename T, typename LT, typename IT, int G>
 __global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_lsolve(const int32_t* __restrict__ tstart, int tfirst,
                                                                   int tcount, const BDesc* __restrict__ desc,
                                                                   const int32_t* __restrict__ bcol,
-                                                                  const LT* __restrict__ lu, const T* __restrict__ r,
-                                                                  T* __restrict__ y, int fuse_u) {
+                                                                  const LT* __restrict__ lu,
+                                                                  const IT* __restrict__ tinv,
+                                                                  const T* __restrict__ r, T* __restrict__ y,
+                                                                  int fuse_u) {
   using AT = TriAcc<T, LT>;
-  __shared__ AT Y[kTileMax][4];
+  __shared__ AT Bv[kTileN];
   const int g = threadIdx.x / G, lane = threadIdx.x % G;
   const int tix = blockIdx.x;
   if (tix >= tcount) return;
-  const int ts = tstart[tfirst + tix];
-  const int tsz = tstart[tfirst + tix + 1] - ts;
+  const int tg = tfirst + tix;
+  const int ts = tstart[tg];
+  const int tsz = tstart[tg + 1] - ts;
   const bool act = g < tsz;
   const int32_t p = ts + g;
-  BDesc d = {0, 0, 0, 0};
+  const IT* inv = tinv + (int64_t)tg * kTileInv;
+  const int i = 4 * g + lane;
+  // row i of L^{-1}, loaded before the sums over the blocks outside the tile
+  IT lrow[kTileN];
+#pragma unroll
+  for (int j = 0; j < kTileN; ++j) lrow[j] = (act && lane < 4 && j <= i) ? inv[tri_lo(i) + j] : (IT)0;
   AT acc[4] = {0, 0, 0, 0};
   AT rv = 0;
-  V4<AT> dv = {{0, 0, 0, 0}};
-  V4<AT> lin[kTileMax - 1];  // row `lane` of the intra-tile lower blocks (tile rows 0..g-1)
-#pragma unroll
-  for (int h = 0; h < kTileMax - 1; ++h) lin[h] = V4<AT>{{0, 0, 0, 0}};
-  int m = 0;
   if (act) {
-    d = desc[p];
-    m = d.dm >> 16;
-    const int td = d.dm & 0xffff, rs = 4 * d.nb;
+    const BDesc d = desc[p];
+    const int m = d.dm >> 16, td = d.dm & 0xffff;
     if (lane < 4) rv = (AT)r[4 * (int64_t)p + lane];
-    if (lane < m) {
-      dv = ld4<AT>(lu + d.vbase + (int64_t)lane * rs + 4 * td);
-#pragma unroll
-      for (int h = 0; h < kTileMax - 1; ++h)
-        if (h < g) lin[h] = ld4<AT>(lu + d.vbase + (int64_t)lane * rs + 4 * (td - g + h));
-    }
-    bsr_dot<G, 1>(bcol + d.bptr, lu + d.vbase, rs, y, m, 0, td - g, lane, acc);
+    bsr_dot<G, 1>(bcol + d.bptr, lu + d.vbase, 4 * d.nb, y, m, 0, td - g, lane, acc);
   }
-  AT D[16], R[4];
-  bcast_diag<G>(dv, D);
 #pragma unroll
   for (int a = 0; a < 4; ++a) acc[a] = group_sum<G>(acc[a]);
+  if (lane < 4) Bv[4 * g + lane] = act ? rv - pick4(acc, lane) : (AT)0;
+  tile_sync<G>();
+  AT yi = 0;
 #pragma unroll
-  for (int a = 0; a < 4; ++a) R[a] = __shfl(rv, a, G);
-  // the loops over tile rows are unrolled so that lin[] / uin[] stay in registers
-#pragma unroll
-  for (int h = 0; h < kTileMax; ++h) {
-    if (h >= tsz) break;
-    if (g == h) {
-      AT sv = 0;
-      if (lane < 4) {
-        sv = pick4(R, lane) - pick4(acc, lane);
-#pragma unroll
-        for (int hp = 0; hp < h; ++hp) sv -= dot4(lin[hp], Y[hp]);
-      }
-      AT S[4], v[4];
-#pragma unroll
-      for (int a = 0; a < 4; ++a) S[a] = __shfl(sv, a, G);
-#pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        AT x = S[a];
-#pragma unroll
-        for (int b = 0; b < a; ++b) x -= D[a * 4 + b] * v[b];
-        v[a] = a < m ? x : (AT)0;
-      }
-      if (lane < 4) Y[h][lane] = pick4(v, lane);
-    }
-    tile_sync<G>();
-  }
+  for (int j = 0; j < kTileN; ++j) yi += (AT)lrow[j] * Bv[j];
   if (fuse_u) {
-    // backward within the tile: rows in reverse, upper blocks next to the diagonal
-    V4<AT> uin[kTileMax - 1];  // row `lane` of the intra-tile upper blocks (tile rows g+1..)
-#pragma unroll
-    for (int h = 0; h < kTileMax - 1; ++h) uin[h] = V4<AT>{{0, 0, 0, 0}};
-    if (act && lane < m) {
-      const int td = d.dm & 0xffff, rs = 4 * d.nb;
-#pragma unroll
-      for (int h = 0; h < kTileMax - 1; ++h)
-        if (g + 1 + h < tsz) uin[h] = ld4<AT>(lu + d.vbase + (int64_t)lane * rs + 4 * (td + 1 + h));
-    }
-#pragma unroll
-    for (int h = kTileMax - 1; h >= 0; --h) {
-      if (h >= tsz) continue;
-      if (g == h) {
-        AT sv = 0;
-        if (lane < 4) {
-          sv = Y[h][lane];
-#pragma unroll
-          for (int hp = h + 1; hp < kTileMax; ++hp)
-            if (hp < tsz) sv -= dot4(uin[hp - h - 1], Y[hp]);
-        }
-        AT S[4], v[4] = {0, 0, 0, 0};
-#pragma unroll
-        for (int a = 0; a < 4; ++a) S[a] = __shfl(sv, a, G);
-#pragma unroll
-        for (int a = 3; a >= 0; --a) {
-          if (a < m) {
-            AT x = S[a];
-#pragma unroll
-            for (int b = a + 1; b < 4; ++b) x -= D[a * 4 + b] * v[b];
-            v[a] = x / D[a * 4 + a];
-          }
-        }
-        if (lane < 4) Y[h][lane] = pick4(v, lane);
-      }
-      tile_sync<G>();
-    }
+    tile_sync<G>();
+    if (lane < 4) Bv[i] = yi;
+    tile_sync<G>();
+    if (act && lane < 4) yi = tile_row_dot(inv + tri_up(i), i, 4 * tsz, (const AT*)Bv);
   }
-  if (act && lane < 4) y[4 * (int64_t)p + lane] = (T)Y[g][lane];
+  if (act && lane < 4) y[4 * (int64_t)p + lane] = (T)yi;
 }
 
 // Backward sweep of one colour, in place: z = U^{-1} y on its tiles.
-template <typename T, typename LT, int G>
+template <typename T, typename LT, typename IT, int G>
 __global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_usolve(const int32_t* __restrict__ tstart, int tfirst,
                                                                   int tcount, const BDesc* __restrict__ desc,
                                                                   const int32_t* __restrict__ bcol,
-                                                                  const LT* __restrict__ lu, T* __restrict__ y) {
+                                                                  const LT* __restrict__ lu,
+                                                                  const IT* __restrict__ tinv, T* __restrict__ y) {
   using AT = TriAcc<T, LT>;
-  __shared__ AT Z[kTileMax][4];
+  __shared__ AT Bv[kTileN];
   const int g = threadIdx.x / G, lane = threadIdx.x % G;
   const int tix = blockIdx.x;
   if (tix >= tcount) return;
-  const int ts = tstart[tfirst + tix];
-  const int tsz = tstart[tfirst + tix + 1] - ts;
+  const int tg = tfirst + tix;
+  const int ts = tstart[tg];
+  const int tsz = tstart[tg + 1] - ts;
   const bool act = g < tsz;
   const int32_t p = ts + g;
-  BDesc d = {0, 0, 0, 0};
+  const IT* inv = tinv + (int64_t)tg * kTileInv;
+  const int i = 4 * g + lane;
+  // row i of U^{-1}, loaded before the sums over the blocks outside the tile
+  IT urow[kTileN];
+#pragma unroll
+  for (int j = 0; j < kTileN; ++j) urow[j] = (act && lane < 4 && j >= i) ? inv[tri_up(i) + j - i] : (IT)0;
   AT acc[4] = {0, 0, 0, 0};
   AT rv = 0;
-  V4<AT> dv = {{0, 0, 0, 0}};
-  V4<AT> uin[kTileMax - 1];  // row `lane` of the intra-tile upper blocks (tile rows g+1..)
-#pragma unroll
-  for (int h = 0; h < kTileMax - 1; ++h) uin[h] = V4<AT>{{0, 0, 0, 0}};
-  int m = 0;
   if (act) {
-    d = desc[p];
-    m = d.dm >> 16;
-    const int td = d.dm & 0xffff, rs = 4 * d.nb;
+    const BDesc d = desc[p];
+    const int m = d.dm >> 16, td = d.dm & 0xffff;
     const int nin = tsz - 1 - g;  // intra-tile upper blocks
     if (lane < 4) rv = (AT)y[4 * (int64_t)p + lane];
-    if (lane < m) {
-      dv = ld4<AT>(lu + d.vbase + (int64_t)lane * rs + 4 * td);
-#pragma unroll
-      for (int h = 0; h < kTileMax - 1; ++h)
-        if (h < nin) uin[h] = ld4<AT>(lu + d.vbase + (int64_t)lane * rs + 4 * (td + 1 + h));
-    }
-    bsr_dot<G, 1>(bcol + d.bptr, lu + d.vbase, rs, y, m, td + 1 + nin, d.nb, lane, acc);
+    bsr_dot<G, 1>(bcol + d.bptr, lu + d.vbase, 4 * d.nb, y, m, td + 1 + nin, d.nb, lane, acc);
   }
-  AT D[16], R[4];
-  bcast_diag<G>(dv, D);
 #pragma unroll
   for (int a = 0; a < 4; ++a) acc[a] = group_sum<G>(acc[a]);
+  if (lane < 4) Bv[4 * g + lane] = act ? rv - pick4(acc, lane) : (AT)0;
+  tile_sync<G>();
+  AT zi = 0;
 #pragma unroll
-  for (int a = 0; a < 4; ++a) R[a] = __shfl(rv, a, G);
-  // the loop over tile rows is unrolled so that uin[] stays in registers
-#pragma unroll
-  for (int h = kTileMax - 1; h >= 0; --h) {
-    if (h >= tsz) continue;
-    if (g == h) {
-      AT sv = 0;
-      if (lane < 4) {
-        sv = pick4(R, lane) - pick4(acc, lane);
-#pragma unroll
-        for (int hp = h + 1; hp < kTileMax; ++hp)
-          if (hp < tsz) sv -= dot4(uin[hp - h - 1], Z[hp]);
-      }
-      AT S[4], v[4] = {0, 0, 0, 0};
-#pragma unroll
-      for (int a = 0; a < 4; ++a) S[a] = __shfl(sv, a, G);
-#pragma unroll
-      for (int a = 3; a >= 0; --a) {
-        if (a < m) {
-          AT x = S[a];
-#pragma unroll
-          for (int b = a + 1; b < 4; ++b) x -= D[a * 4 + b] * v[b];
-          v[a] = x / D[a * 4 + a];
-        }
-      }
-      if (lane < 4) Z[h][lane] = pick4(v, lane);
-    }
-    tile_sync<G>();
-  }
-  if (act && lane < 4) y[4 * (int64_t)p + lane] = (T)Z[g][lane];
+  for (int j = 0; j < kTileN; ++j) zi += (AT)urow[j] * Bv[j];
+  if (act && lane < 4) y[4 * (int64_t)p + lane] = (T)zi;
 }
 
 // Tile sweep shapes: code c -> G = kTileGs[c] lanes per block row.
@@ -953,23 +942,26 @@ constexpr int kTileCodes = 3;
 constexpr int kTileDefault = 1;
 
 template <typename T, typename LT, bool UPPER, int G>
-static void tile_launch_g(hipStream_t st, const IluPlan& P, int t0, int cnt, const LT* lu, const T* r, T* y,
-                          int fuse_u) {
+static void tile_launch_g(hipStream_t st, const IluPlan& P, int t0, int cnt, const LT* lu, const void* tinv,
+                          const T* r, T* y, int fuse_u) {
+  // the tile inverses are FP64 beside an FP64 factor, FP32 otherwise
+  using IT = typename std::conditional<std::is_same<LT, double>::value, double, float>::type;
+  const IT* ti = (const IT*)tinv;
   if constexpr (UPPER)
-    hipLaunchKernelGGL((k_bsr_tile_usolve<T, LT, G>), dim3((unsigned)cnt), dim3(G * kTileMax), 0, st,
-                       P.d_tileStart, t0, cnt, P.d_bdesc, P.d_bcol, lu, y);
+    hipLaunchKernelGGL((k_bsr_tile_usolve<T, LT, IT, G>), dim3((unsigned)cnt), dim3(G * kTileMax), 0, st,
+                       P.d_tileStart, t0, cnt, P.d_bdesc, P.d_bcol, lu, ti, y);
   else
-    hipLaunchKernelGGL((k_bsr_tile_lsolve<T, LT, G>), dim3((unsigned)cnt), dim3(G * kTileMax), 0, st,
-                       P.d_tileStart, t0, cnt, P.d_bdesc, P.d_bcol, lu, r, y, fuse_u);
+    hipLaunchKernelGGL((k_bsr_tile_lsolve<T, LT, IT, G>), dim3((unsigned)cnt), dim3(G * kTileMax), 0, st,
+                       P.d_tileStart, t0, cnt, P.d_bdesc, P.d_bcol, lu, ti, r, y, fuse_u);
 }
 
 template <typename T, typename LT, bool UPPER>
-static void tile_launch(hipStream_t st, int code, const IluPlan& P, int t0, int cnt, const LT* lu, const T* r, T* y,
-                        int fuse_u = 0) {
+static void tile_launch(hipStream_t st, int code, const IluPlan& P, int t0, int cnt, const LT* lu, const void* tinv,
+                        const T* r, T* y, int fuse_u = 0) {
   switch (code) {
-    case 0: tile_launch_g<T, LT, UPPER, 8>(st, P, t0, cnt, lu, r, y, fuse_u); break;
-    case 2: tile_launch_g<T, LT, UPPER, 32>(st, P, t0, cnt, lu, r, y, fuse_u); break;
-    default: tile_launch_g<T, LT, UPPER, 16>(st, P, t0, cnt, lu, r, y, fuse_u); break;
+    case 0: tile_launch_g<T, LT, UPPER, 8>(st, P, t0, cnt, lu, tinv, r, y, fuse_u); break;
+    case 2: tile_launch_g<T, LT, UPPER, 32>(st, P, t0, cnt, lu, tinv, r, y, fuse_u); break;
+    default: tile_launch_g<T, LT, UPPER, 16>(st, P, t0, cnt, lu, tinv, r, y, fuse_u); break;
   }
 }
 
@@ -1172,7 +1164,8 @@ static hipError_t alloc(T** p, size_t n) {
 LinAlg::~LinAlg() {
   void* ptrs[] = {d_blkStart, d_diagpos, d_part, d_scal, r, rh, p, v, s, t, ph, sh, x, w, z, pr, pz,
                   r32, rh32, p32, v32, s32, t32, ph32, sh32, x32, du32, w32, z32, pr32, pz32,
-                  dinv, dinv32, binv, binv32, lu, lu32, gm_V, gm_V32, d_desc, lu16, kp, kp32, d_h, d_mpart};
+                  dinv, dinv32, binv, binv32, lu, lu32, gm_V, gm_V32, d_desc, lu16, kp, kp32, d_h, d_mpart,
+                  tinv, tinv32};
   delete ilu;
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
@@ -1517,7 +1510,8 @@ static int build_ilu_plan(sfem_ctx* ctx, LinAlg* L, int reorder) {
   delete L->ilu;
   L->ilu = nullptr;
   // factor-sized buffers belong to the previous plan
-  void** vals[] = {(void**)&L->kp, (void**)&L->kp32, (void**)&L->lu, (void**)&L->lu32, (void**)&L->lu16};
+  void** vals[] = {(void**)&L->kp, (void**)&L->kp32, (void**)&L->lu, (void**)&L->lu32, (void**)&L->lu16,
+                   (void**)&L->tinv, (void**)&L->tinv32};
   for (void** q : vals)
     if (*q) { (void)hipFree(*q); *q = nullptr; }
   IluPlan* Pp = new IluPlan;
@@ -1827,6 +1821,25 @@ int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f, 
       const unsigned gz = (unsigned)std::min<int64_t>((nv + 255) / 256, 1 << 20);
       hipLaunchKernelGGL(k_to_bf16, dim3(gz), dim3(256), 0, st, nv, L->lu32, L->lu16);
     }
+    if (P.tiled) {
+      // inverses of every tile's triangles, from the factor the sweeps read
+      const int ntl = (int)P.h_tileStart.size() - 1;
+      const int64_t ni = (int64_t)ntl * kTileInv;
+      int rc;
+      if (L->pc_bf16) {
+        if ((rc = ensure_vec(&L->tinv32, ni))) return rc;
+        hipLaunchKernelGGL((k_tile_invert<bf16, float>), dim3(ntl), dim3(64), 0, st, P.d_tileStart, ntl, P.d_bdesc,
+                           L->lu16, L->tinv32, L->d_scal);
+      } else if (L->pc_fp32) {
+        if ((rc = ensure_vec(&L->tinv32, ni))) return rc;
+        hipLaunchKernelGGL((k_tile_invert<float, float>), dim3(ntl), dim3(64), 0, st, P.d_tileStart, ntl, P.d_bdesc,
+                           L->lu32, L->tinv32, L->d_scal);
+      } else {
+        if ((rc = ensure_vec(&L->tinv, ni))) return rc;
+        hipLaunchKernelGGL((k_tile_invert<double, double>), dim3(ntl), dim3(64), 0, st, P.d_tileStart, ntl,
+                           P.d_bdesc, L->lu, L->tinv, L->d_scal);
+      }
+    }
     LA_CHECK(hipGetLastError());
   }
   return SFEM_OK;
@@ -1898,17 +1911,18 @@ static void ilu_solve(sfem_ctx* ctx, const IluPlan& P, const LT* lu, const T* r,
     // every launch (forward colours, then backward colours)
     const int nc = (int)P.tileColPtr.size() - 1;
     const int tf = P.tile_fuse_last ? 1 : 0;
+    const void* ti = std::is_same<LT, double>::value ? (const void*)L->tinv : (const void*)L->tinv32;
     for (int c = 0; c < nc; ++c) {
       const int t0 = P.tileColPtr[c], cnt = P.tileColPtr[c + 1] - t0;
       const int code = force >= 0 ? force : c < (int)tl.size() ? tl[c] : kTileDefault;
-      if (cnt) tile_launch<T, LT, false>(st, code, P, t0, cnt, lu, r, z, c == nc - 1 ? tf : 0);
+      if (cnt) tile_launch<T, LT, false>(st, code, P, t0, cnt, lu, ti, r, z, c == nc - 1 ? tf : 0);
       if (ev) (void)hipEventRecord(ev[1 + c], st);
     }
     for (int k = 0; k < nc - tf; ++k) {
       const int c = nc - 1 - tf - k;
       const int t0 = P.tileColPtr[c], cnt = P.tileColPtr[c + 1] - t0;
       const int code = force >= 0 ? force : k < (int)tu.size() ? tu[k] : kTileDefault;
-      if (cnt) tile_launch<T, LT, true>(st, code, P, t0, cnt, lu, z, z);
+      if (cnt) tile_launch<T, LT, true>(st, code, P, t0, cnt, lu, ti, z, z);
       if (ev) (void)hipEventRecord(ev[1 + nc + k], st);
     }
     return;

@@ -103,6 +103,11 @@ struct LinAlg {
   float* binv32 = nullptr;
   double* lu = nullptr;
   float* lu32 = nullptr;
+  // tiled plans: per tile the inverses of the tile's unit-lower and upper triangular
+  // factors (kTileInv values: two packed 32 x 32 triangles), FP64 for an FP64 factor and
+  // FP32 for an FP32 / BF16 one
+  double* tinv = nullptr;
+  float* tinv32 = nullptr;
   // the tangent in the ILU(0) plan's block form, for the products of the Krylov solve
   double* kp = nullptr;
   float* kp32 = nullptr;
