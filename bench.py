@@ -100,7 +100,7 @@ def run_path(mesh, ctl, prec, opts, newton_tol, steps, warmup, label, predictor=
     log(f"{label}: {steps} steps in {t2 - t1:.3f} s; device ms by class: " +
         ", ".join(f"{k} {v[1]:.1f} ({v[0]} launches)" for k, v in kt.items()))
     res = dict(label=label, seconds=t2 - t1, steps_per_s=steps / (t2 - t1), hist=hist, kt=kt, info=info,
-               soln=s, tune=tune, block=ctx.block_info(), stats=ctx.solver_stats())
+               soln=s, tune=tune, block=ctx.block_info(), tiles=ctx.tile_info(), stats=ctx.solver_stats())
     ctx.close()
     return res
 
@@ -404,8 +404,8 @@ def main():
     ap.add_argument("--n", type=int, default=0, help="override the mesh size n (n^3 hexes)")
     ap.add_argument("--inner-rtol", type=float, default=2e-4,
                     help="mixed path: FP32 inner solve tolerance; two solves per step take ||F|| from ~5.7e-3 "
-                         "below the mixed Newton tolerance; velocity stays within ~5e-7 of FP64 after 3 steps "
-                         "and ~2e-7 after 10 (3e-4: 8%% faster, but 1.1e-6 after 3 steps)")
+                         "below the mixed Newton tolerance; velocity stays within ~7e-7 of FP64 after 1 step "
+                         "and ~1.4e-7 after 10 (3e-4: 4%% faster, 1.9e-7 after 10 steps)")
     ap.add_argument("--newton-tol", type=float, default=1e-8,
                     help="FP64 path: absolute Newton tolerance on ||F|| (the reference's 1e-8)")
     ap.add_argument("--newton-tol-mixed", type=float, default=1e-9,
@@ -416,7 +416,8 @@ def main():
                     help="ILU(0) node ordering: 0 natural, 1 RCM, 2 multicolour, 3 tiled multicolour "
                          "(default: 3 for the hex cavities, 2 for the c4 batch)")
     ap.add_argument("--bf16-factors", type=int, default=0,
-                    help="mixed path: ILU(0) factors stored in bfloat16 (8%% cheaper applies, ~10%% more iterations)")
+                    help="mixed path: ILU(0) factors stored in bfloat16 (tile inverses in FP32): 5%% more steps/s "
+                         "with the tiled ordering, but velocity 1.6e-6 from FP64 after a single step")
     ap.add_argument("--predictor", choices=["previous", "extrapolate"], default="previous",
                     help="Newton initial iterate of each step (both paths)")
     ap.add_argument("--pc-lag-mixed", type=int, default=1,
@@ -494,10 +495,20 @@ def main():
     roof["traffic_source"] = TRAFFIC_SOURCE if roof["traffic"] else None
     # ILU(0) apply: both sweeps together read every factor value, block index and
     # descriptor once; r read, y written, y read and written, plus the gathered y once
-    # per sweep (5 vector passes)
+    # per sweep (5 vector passes).  With the tiled ordering the values inside the tiles are
+    # replaced by the tiles' packed inverse triangles (2 x 528 values per tile, FP64
+    # beside an FP64 factor, FP32 otherwise).
     lu_s = 2 if (fp32 and a.bf16_factors) else sv
-    roof_pc = hbm_roof("pc", "ILU(0) apply (2 x colours launches)",
-                       nnz * lu_s + 4 * bi["npairs"] + 32 * bi["nblk"] + 5 * sv * nf)
+    ti = main_res["tiles"]
+    pc_bytes = nnz * lu_s + 4 * bi["npairs"] + 32 * bi["nblk"] + 5 * sv * nf
+    pc_name = "ILU(0) apply (2 x colours launches)"
+    if ti["ntiles"]:
+        inv_s = 8 if not fp32 else 4
+        pc_bytes += ti["ntiles"] * 2 * 528 * inv_s - ti["tile_vals"] * lu_s
+        pc_name = (f"ILU(0) apply, tiled: k_bsr_tile_lsolve/usolve, {2 * ti['ncolors'] - 1} launches, "
+                   f"{ti['ntiles']} tiles")
+    roof_pc = hbm_roof("pc", pc_name, pc_bytes)
+    roof_pc["tiles"] = ti
     n_k = len(main_res["hist"])
     n_f = sum(h[0] for h in main_res["hist"]) - n_k
     cnt, ms = kt["assembly"]

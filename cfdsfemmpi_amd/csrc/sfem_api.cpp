@@ -773,6 +773,11 @@ int sfem_get_ilu(sfem_ctx* c, int64_t* nnz, int32_t* perm, int32_t* rowptr, int3
   return la_get_ilu(c, nnz, perm, rowptr, colidx, vals);
 }
 
+int sfem_tile_info(sfem_ctx* c, int64_t* ntiles, int64_t* ncolors, int64_t* tile_vals) {
+  if (!c) return set_error(SFEM_ERR_ARG, "null context");
+  return la_tile_info(c, ntiles, ncolors, tile_vals);
+}
+
 int sfem_pc_apply(sfem_ctx* c, const double* r, double* z) {
   if (!c || !r || !z) return set_error(SFEM_ERR_ARG, "null argument");
   HIP_OK(hipSetDevice(c->device));

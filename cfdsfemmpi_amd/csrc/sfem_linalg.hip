@@ -846,7 +846,7 @@ __device__ inline AT tile_row_dot(const IT* __restrict__ row, int c0, int c1, co
 
 // Forward sweep of one colour: y = L^{-1} r on its tiles.  With fuse_u the tiles have no
 // upper blocks outside themselves (the last colour) and their backward sweep follows.
-template <typename T, typename LT, typename IT, int G>
+template <typename T, typename LT, typename IT, int G, int K>
 __global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_lsolve(const int32_t* __restrict__ tstart, int tfirst,
                                                                   int tcount, const BDesc* __restrict__ desc,
                                                                   const int32_t* __restrict__ bcol,
@@ -876,7 +876,7 @@ __global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_lsolve(const int32_t*
     const BDesc d = desc[p];
     const int m = d.dm >> 16, td = d.dm & 0xffff;
     if (lane < 4) rv = (AT)r[4 * (int64_t)p + lane];
-    bsr_dot<G, 1>(bcol + d.bptr, lu + d.vbase, 4 * d.nb, y, m, 0, td - g, lane, acc);
+    bsr_dot<G, K>(bcol + d.bptr, lu + d.vbase, 4 * d.nb, y, m, 0, td - g, lane, acc);
   }
 #pragma unroll
   for (int a = 0; a < 4; ++a) acc[a] = group_sum<G>(acc[a]);
@@ -895,7 +895,7 @@ __global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_lsolve(const int32_t*
 }
 
 // Backward sweep of one colour, in place: z = U^{-1} y on its tiles.
-template <typename T, typename LT, typename IT, int G>
+template <typename T, typename LT, typename IT, int G, int K>
 __global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_usolve(const int32_t* __restrict__ tstart, int tfirst,
                                                                   int tcount, const BDesc* __restrict__ desc,
                                                                   const int32_t* __restrict__ bcol,
@@ -924,7 +924,7 @@ __global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_usolve(const int32_t*
     const int m = d.dm >> 16, td = d.dm & 0xffff;
     const int nin = tsz - 1 - g;  // intra-tile upper blocks
     if (lane < 4) rv = (AT)y[4 * (int64_t)p + lane];
-    bsr_dot<G, 1>(bcol + d.bptr, lu + d.vbase, 4 * d.nb, y, m, td + 1 + nin, d.nb, lane, acc);
+    bsr_dot<G, K>(bcol + d.bptr, lu + d.vbase, 4 * d.nb, y, m, td + 1 + nin, d.nb, lane, acc);
   }
 #pragma unroll
   for (int a = 0; a < 4; ++a) acc[a] = group_sum<G>(acc[a]);
@@ -936,22 +936,24 @@ __global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_usolve(const int32_t*
   if (act && lane < 4) y[4 * (int64_t)p + lane] = (T)zi;
 }
 
-// Tile sweep shapes: code c -> G = kTileGs[c] lanes per block row.
-static const int kTileGs[3] = {8, 16, 32};
-constexpr int kTileCodes = 3;
+// Tile sweep shapes: code c -> G = kTileGs[c] lanes per block row, K = kTileKs[c] blocks
+// per lane loaded together.
+static const int kTileGs[4] = {8, 16, 8, 16};
+static const int kTileKs[4] = {1, 1, 2, 2};
+constexpr int kTileCodes = 4;
 constexpr int kTileDefault = 1;
 
-template <typename T, typename LT, bool UPPER, int G>
+template <typename T, typename LT, bool UPPER, int G, int K>
 static void tile_launch_g(hipStream_t st, const IluPlan& P, int t0, int cnt, const LT* lu, const void* tinv,
                           const T* r, T* y, int fuse_u) {
   // the tile inverses are FP64 beside an FP64 factor, FP32 otherwise
   using IT = typename std::conditional<std::is_same<LT, double>::value, double, float>::type;
   const IT* ti = (const IT*)tinv;
   if constexpr (UPPER)
-    hipLaunchKernelGGL((k_bsr_tile_usolve<T, LT, IT, G>), dim3((unsigned)cnt), dim3(G * kTileMax), 0, st,
+    hipLaunchKernelGGL((k_bsr_tile_usolve<T, LT, IT, G, K>), dim3((unsigned)cnt), dim3(G * kTileMax), 0, st,
                        P.d_tileStart, t0, cnt, P.d_bdesc, P.d_bcol, lu, ti, y);
   else
-    hipLaunchKernelGGL((k_bsr_tile_lsolve<T, LT, IT, G>), dim3((unsigned)cnt), dim3(G * kTileMax), 0, st,
+    hipLaunchKernelGGL((k_bsr_tile_lsolve<T, LT, IT, G, K>), dim3((unsigned)cnt), dim3(G * kTileMax), 0, st,
                        P.d_tileStart, t0, cnt, P.d_bdesc, P.d_bcol, lu, ti, r, y, fuse_u);
 }
 
@@ -959,9 +961,10 @@ template <typename T, typename LT, bool UPPER>
 static void tile_launch(hipStream_t st, int code, const IluPlan& P, int t0, int cnt, const LT* lu, const void* tinv,
                         const T* r, T* y, int fuse_u = 0) {
   switch (code) {
-    case 0: tile_launch_g<T, LT, UPPER, 8>(st, P, t0, cnt, lu, tinv, r, y, fuse_u); break;
-    case 2: tile_launch_g<T, LT, UPPER, 32>(st, P, t0, cnt, lu, tinv, r, y, fuse_u); break;
-    default: tile_launch_g<T, LT, UPPER, 16>(st, P, t0, cnt, lu, tinv, r, y, fuse_u); break;
+    case 0: tile_launch_g<T, LT, UPPER, 8, 1>(st, P, t0, cnt, lu, tinv, r, y, fuse_u); break;
+    case 2: tile_launch_g<T, LT, UPPER, 8, 2>(st, P, t0, cnt, lu, tinv, r, y, fuse_u); break;
+    case 3: tile_launch_g<T, LT, UPPER, 16, 2>(st, P, t0, cnt, lu, tinv, r, y, fuse_u); break;
+    default: tile_launch_g<T, LT, UPPER, 16, 1>(st, P, t0, cnt, lu, tinv, r, y, fuse_u); break;
   }
 }
 
@@ -1677,6 +1680,11 @@ static int build_ilu_plan(sfem_ctx* ctx, LinAlg* L, int reorder) {
         return set_error(SFEM_ERR_UNSUPPORTED, "tiled ILU(0) plan: a tile whose rows are not all coupled");
       }
     }
+    P.tile_vals = 0;
+    for (int64_t t = 0; t < ntl; ++t) {
+      const int32_t tsz = P.h_tileStart[t + 1] - P.h_tileStart[t];
+      for (int32_t p = P.h_tileStart[t]; p < P.h_tileStart[t + 1]; ++p) P.tile_vals += (int64_t)(bd[p].dm >> 16) * 4 * tsz;
+    }
     // the last colour's tiles: no upper blocks outside the tile, so their backward sweep
     // joins the forward one
     const int nc = (int)P.tileColPtr.size() - 1;
@@ -2210,6 +2218,16 @@ int la_block_info(sfem_ctx* ctx, int64_t* nblk, int64_t* npairs, int64_t* nvals)
   return SFEM_OK;
 }
 
+int la_tile_info(sfem_ctx* ctx, int64_t* ntiles, int64_t* ncolors, int64_t* tile_vals) {
+  const LinAlg* L = ctx->la;
+  const IluPlan* P = L ? L->ilu : nullptr;
+  const bool t = P && P->tiled;
+  if (ntiles) *ntiles = t ? (int64_t)P->h_tileStart.size() - 1 : 0;
+  if (ncolors) *ncolors = t ? (int64_t)P->tileColPtr.size() - 1 : 0;
+  if (tile_vals) *tile_vals = t ? P->tile_vals : 0;
+  return SFEM_OK;
+}
+
 int la_tuning(sfem_ctx* ctx, int* out) {
   LinAlg* L = ctx->la;
   for (int k = 0; k < 2; ++k) {
@@ -2232,10 +2250,9 @@ int la_tuning_levels(sfem_ctx* ctx, int fp32, int* out, int cap, int* nlevels) {
   *nlevels = (int)all.size();
   const bool tiled = L->ilu && L->ilu->tiled;
   for (int i = 0; i < (int)all.size() && i < cap; ++i) {
-    // (lanes per block row, blocks per lane loaded together, XCD order); a tiled plan's
-    // launches have G lanes per row and one block in flight per lane
+    // (lanes per block row, blocks per lane loaded together, XCD order)
     out[3 * i + 0] = tiled ? kTileGs[all[i]] : kBlkG[all[i] & 31];
-    out[3 * i + 1] = tiled ? 1 : kBlkK[all[i] & 31];
+    out[3 * i + 1] = tiled ? kTileKs[all[i]] : kBlkK[all[i] & 31];
     out[3 * i + 2] = tiled ? 0 : all[i] >> 5;
   }
   return SFEM_OK;

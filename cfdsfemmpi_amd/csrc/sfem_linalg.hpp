@@ -44,6 +44,7 @@ struct IluPlan {
   bool tile_fuse_last = false;         // the last colour's tiles have no upper blocks outside themselves
   std::vector<int32_t> h_tileStart;    // tile t = block rows [h_tileStart[t], h_tileStart[t+1])
   std::vector<int32_t> tileColPtr;     // colour c = tiles [tileColPtr[c], tileColPtr[c+1])
+  int64_t tile_vals = 0;               // factor values inside the tiles (applied through the inverses)
   int32_t* d_tileStart = nullptr;
   ~IluPlan();
 };
@@ -151,6 +152,7 @@ int la_prepare(sfem_ctx* ctx, const sfem_solver_opts* o);
 int la_tuning(sfem_ctx* ctx, int* out);
 int la_block_info(sfem_ctx* ctx, int64_t* nblk, int64_t* npairs, int64_t* nvals);
 int la_get_ilu(sfem_ctx* ctx, int64_t* nnz_out, int32_t* perm, int32_t* rowptr, int32_t* colidx, double* vals);
+int la_tile_info(sfem_ctx* ctx, int64_t* ntiles, int64_t* ncolors, int64_t* tile_vals);
 int la_pc_apply_host(sfem_ctx* ctx, const double* r, double* z);
 int la_tuning_levels(sfem_ctx* ctx, int fp32, int* out, int cap, int* nlevels);
 int la_norm2(sfem_ctx* ctx, const double* a, double* out);

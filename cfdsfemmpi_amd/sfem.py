@@ -85,6 +85,7 @@ ABI = [
     ("sfem_batch_use_graph", C.c_int, [_vp, C.c_int]),
     ("sfem_batch_narrowing", C.c_int, [_vp, C.c_int]),
     ("sfem_block_info", C.c_int, [_vp, _i64p, _i64p, _i64p]),
+    ("sfem_tile_info", C.c_int, [_vp, _i64p, _i64p, _i64p]),
     ("sfem_tuning", C.c_int, [_vp, C.POINTER(C.c_int)]),
     ("sfem_tuning_levels", C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]),
     ("sfem_info", C.c_int, [_vp, _i64p, _i64p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
@@ -277,6 +278,12 @@ class Context:
         a, b, c = C.c_int64(0), C.c_int64(0), C.c_int64(0)
         _check(self.lib.sfem_block_info(self.h, C.byref(a), C.byref(b), C.byref(c)), "sfem_block_info")
         return dict(nblk=a.value, npairs=b.value, nvals=c.value)
+
+    def tile_info(self):
+        """Tiles of a tiled ILU(0) plan: count, colours, factor values inside tiles."""
+        a, b, c = C.c_int64(0), C.c_int64(0), C.c_int64(0)
+        _check(self.lib.sfem_tile_info(self.h, C.byref(a), C.byref(b), C.byref(c)), "sfem_tile_info")
+        return dict(ntiles=a.value, ncolors=b.value, tile_vals=c.value)
 
     def set_batch(self, elem_mesh, mu):
         """Declare the mesh as a disjoint union of len(mu) meshes (elem_mesh: mesh of each element)."""

@@ -247,6 +247,12 @@ int sfem_batch_narrowing(sfem_ctx* ctx, int enable);
  * No counterpart in the reference. */
 int sfem_block_info(sfem_ctx* ctx, int64_t* nblk, int64_t* npairs, int64_t* nvals);
 
+/* Tiles of a tiled multicolour ILU(0) plan (reorder = 3; zeros otherwise): number of
+ * tiles, tile colours, and the factor values inside tiles, which the sweeps apply through
+ * the tiles' inverse triangles instead.  No counterpart in the reference (PETSc's PCILU
+ * keeps its ordering internal); for bandwidth accounting. */
+int sfem_tile_info(sfem_ctx* ctx, int64_t* ntiles, int64_t* ncolors, int64_t* tile_vals);
+
 /* Mixed-precision solver events since the context was created.  fp32_fallbacks: solves
  * in which the FP32 BiCGStab stalled (no 10% gain of its best residual over 100
  * iterations) and which FP32 GMRES finished from a zero start.  fp32_partial (may be
