@@ -189,9 +189,13 @@ def _cavity3d(n, steps, prec=sfem.PREC_FP64, opts=None, newton_tol=1e-10):
     return fem, mesh, ctl
 
 
-def test_cavity3d_vs_oracle():
-    """Graded 6^3 Hexa8 cavity, generalised-alpha, 3 steps: fields within 1e-8 of the oracle."""
-    fem, mesh, ctl = _cavity3d(6, 3)
+@pytest.mark.parametrize("reorder", [0, 3])
+def test_cavity3d_vs_oracle(reorder):
+    """Graded 6^3 / 7^3 Hexa8 cavity, generalised-alpha, 3 steps, BiCGStab + ILU(0) in natural
+    or tiled multicolour order (7^3: 8 nodes per side, every node in a full 2 x 2 x 2 tile):
+    fields within 1e-8 of the oracle's direct solves."""
+    n = 7 if reorder == 3 else 6
+    fem, mesh, ctl = _cavity3d(n, 3, opts=sfem.solver_opts("bcgs", "ilu", 1e-12, 2000, reorder=reorder))
     s = fem.solveFullyImplicit()
     o = oracle.OracleFEM(mesh, ctl)
     o.prepare()
@@ -516,6 +520,38 @@ def test_batch_graph_replay_matches_eager():
     ctx.close()
     print(f"[graph] max |graph - eager| {np.max(np.abs(res[0] - res[1])):.3e}", flush=True)
     assert np.array_equal(res[0], res[1]) and np.array_equal(res[0], res[2])
+
+
+def test_batch_tiled_ilu_matches_single_meshes():
+    """A batch stepped with the tiled multicolour ILU(0) (whose narrowed iterations run the
+    factorisation's level lists) against each mesh stepped alone, FP64."""
+    mesh, em, mu, ns, meshes = meshgen.c4_batch(4, seed=11, n_range=(6, 20))
+    dt, steps, ntol = 0.05, 2, 1e-10
+    opts = sfem.solver_opts("bcgs", "ilu", 1e-10, 3000, reorder=3)
+    ctx = _batch_ctx(mesh, em, mu, sfem.PREC_FP64, opts, dt)
+    t = dt
+    for s in range(steps):
+        ni, ki, rn = ctx.step_batch(min(t, 1.0), opts, ntol, 10)
+        print(f"[batch-tiled] step {s}: newton {ni.tolist()} krylov {ki.tolist()}", flush=True)
+        assert np.all(rn < ntol)
+        t += dt
+    sb = ctx.get_solution().reshape(-1, 3)
+    ctx.close()
+    off = 0
+    for k, m in enumerate(meshes):
+        ed = np.array([1.0, mu[k], 0, 0, 0, 0, 0, 0, 1, 1, 0])
+        c1 = make_ctx(m, tis=2, dt=dt, elemData=ed)
+        t = dt
+        for s in range(steps):
+            st = c1.step(min(t, 1.0), opts, ntol, 10)
+            assert st.rhs_norm < ntol
+            t += dt
+        s1 = c1.get_solution().reshape(-1, 3)
+        c1.close()
+        d = np.linalg.norm(sb[off:off + m.nNode] - s1) / np.linalg.norm(s1)
+        print(f"[batch-tiled] mesh {k}: rel diff vs alone {d:.2e}", flush=True)
+        assert d <= 1e-7
+        off += m.nNode
 
 
 @pytest.mark.parametrize("prec", [sfem.PREC_FP64, sfem.PREC_MIXED])
