@@ -151,3 +151,125 @@ def c4_batch(count: int = 256, seed: int = 0, n_range=(16, 160), jitter: float =
                 np.concatenate(dn).astype(np.int32), np.concatenate(dd).astype(np.int32),
                 np.concatenate(dv), np.zeros(0, np.int32))
     return mesh, np.concatenate(em), 1.0 / res, ns, meshes
+
+
+def _stretch(h0: float, length: float, n: int) -> np.ndarray:
+    """n cells over ``length`` growing geometrically from a first cell of ``h0``
+    (uniform when n * h0 >= length): offsets 0..length."""
+    if n * h0 >= length:
+        return np.linspace(0.0, length, n + 1)
+    lo, hi = 1.0, 2.0
+    while h0 * (hi ** n - 1.0) / (hi - 1.0) < length:
+        hi *= 2.0
+    for _ in range(200):
+        q = 0.5 * (lo + hi)
+        if h0 * (q ** n - 1.0) / (q - 1.0) < length:
+            lo = q
+        else:
+            hi = q
+    q = 0.5 * (lo + hi)
+    x = np.concatenate([[0.0], np.cumsum(h0 * q ** np.arange(n))])
+    return x * (length / x[-1])
+
+
+def cylinder_channel(nr: int = 64, nth: int = 256, ratio: float = 1.05, radius: float = 0.5,
+                     core: float = 2.5, box=(-10.0, 30.0, -10.0, 10.0), n_left: int = 24,
+                     n_right: int = 150, n_bottom: int = 24, n_top: int = 24, u_in: float = 1.0,
+                     perturb: float = 0.0, seed: int = 0) -> Mesh:
+    """Flow past a cylinder of radius ``radius`` at the origin in the channel ``box``,
+    Quad4.  Near field: an O-grid of ``nr`` radial x ``nth`` circumferential cells
+    between the circle and the square [-core, core]^2, radial spacing growing
+    geometrically by ``ratio``; each ring point lies on the ray of its square-boundary
+    point.  Far field: eight structured blocks around the square, uniform along the
+    square's sides (``nth`` / 4 cells each) and stretched geometrically away from it.
+
+    Boundary conditions (all values ramped by the reference's min(t, 1)):
+    inflow x = box[0]: u = u_in (1 + perturb xi(y)), v = 0, xi a seeded smooth function
+    with max |xi| = 1; top and bottom walls: slip, v = 0; cylinder: u = v = 0; outflow
+    x = box[1]: traction free (no entry).  No pressure is pinned.
+    """
+    assert nth % 4 == 0
+    m = nth // 4
+    x0, x1, y0, y1 = box
+    h0 = 2.0 * core / m
+    xl = -core - _stretch(h0, -core - x0, n_left)[::-1]
+    xr = core + _stretch(h0, x1 - core, n_right)
+    yb = -core - _stretch(h0, -core - y0, n_bottom)[::-1]
+    yt = core + _stretch(h0, y1 - core, n_top)
+    xm = np.linspace(-core, core, m + 1)
+    X = np.concatenate([xl[:-1], xm, xr[1:]])
+    Y = np.concatenate([yb[:-1], xm, yt[1:]])
+    NX, NY = len(X) - 1, len(Y) - 1
+    i0, j0 = n_left, n_bottom  # square = grid [i0, i0+m] x [j0, j0+m]
+    gi, gj = np.meshgrid(np.arange(NX + 1), np.arange(NY + 1), indexing="xy")
+    gi, gj = gi.ravel(), gj.ravel()
+    keep = ~((gi > i0) & (gi < i0 + m) & (gj > j0) & (gj < j0 + m))
+    gid = -np.ones((NY + 1) * (NX + 1), np.int64)
+    gid[keep] = np.arange(int(keep.sum()))
+    grid = lambda i, j: int(gid[j * (NX + 1) + i])
+    coords = [np.stack([X[gi[keep]], Y[gj[keep]]], axis=1)]
+    nb = int(keep.sum())
+    # far-field cells
+    ci, cj = np.meshgrid(np.arange(NX), np.arange(NY), indexing="xy")
+    ci, cj = ci.ravel(), cj.ravel()
+    outside = ~((ci >= i0) & (ci < i0 + m) & (cj >= j0) & (cj < j0 + m))
+    ci, cj = ci[outside], cj[outside]
+    n00 = gid[cj * (NX + 1) + ci]
+    conn = [np.stack([n00, gid[cj * (NX + 1) + ci + 1], gid[(cj + 1) * (NX + 1) + ci + 1],
+                      gid[(cj + 1) * (NX + 1) + ci]], axis=1)]
+    # square boundary, counter-clockwise from the corner (core, -core)
+    sq = []
+    for k in range(m):
+        sq.append(grid(i0 + m, j0 + k))
+    for k in range(m):
+        sq.append(grid(i0 + m - k, j0 + m))
+    for k in range(m):
+        sq.append(grid(i0, j0 + m - k))
+    for k in range(m):
+        sq.append(grid(i0 + k, j0))
+    sq = np.array(sq, np.int64)
+    S = coords[0][sq]
+    th = np.arctan2(S[:, 1], S[:, 0])
+    C = radius * np.stack([np.cos(th), np.sin(th)], axis=1)
+    s = (ratio ** np.arange(nr + 1) - 1.0) / (ratio ** nr - 1.0)
+    ring = np.empty((nr + 1, nth), np.int64)
+    ring[nr] = sq
+    for k in range(nr):
+        ring[k] = nb + k * nth + np.arange(nth)
+        coords.append((1.0 - s[k]) * C + s[k] * S)
+    jj = np.arange(nth)
+    for k in range(nr):
+        conn.append(np.stack([ring[k, jj], ring[k + 1, jj], ring[k + 1, (jj + 1) % nth],
+                              ring[k, (jj + 1) % nth]], axis=1))
+    coords = np.concatenate(coords)
+    conn = np.concatenate(conn).astype(np.int32)
+    # boundary conditions
+    xy = coords
+    tol = 1e-9
+    inflow = np.abs(xy[:, 0] - x0) < tol
+    walls = (np.abs(xy[:, 1] - y0) < tol) | (np.abs(xy[:, 1] - y1) < tol)
+    cyl = np.zeros(len(xy), bool)
+    cyl[ring[0]] = True
+    if perturb:
+        rng = np.random.default_rng(seed)
+        modes = np.arange(1, 6)
+        amp = rng.standard_normal(len(modes)) / modes
+        ph = rng.uniform(0.0, 2.0 * np.pi, len(modes))
+        yy = np.linspace(y0, y1, 2001)
+        f = lambda y: (amp[None, :] * np.sin(np.pi * modes[None, :] * (y[:, None] - y0) / (y1 - y0)
+                                             + ph[None, :])).sum(axis=1)
+        scale = 1.0 / np.max(np.abs(f(yy)))
+        xi = lambda y: scale * f(y)
+    else:
+        xi = lambda y: np.zeros_like(y)
+    dn, dd, dv = [], [], []
+    uin = u_in * (1.0 + perturb * xi(xy[:, 1]))
+    for node in range(len(xy)):
+        if inflow[node]:
+            dn += [node, node]; dd += [0, 1]; dv += [float(uin[node]), 0.0]
+        elif cyl[node]:
+            dn += [node, node]; dd += [0, 1]; dv += [0.0, 0.0]
+        elif walls[node]:
+            dn.append(node); dd.append(1); dv.append(0.0)
+    return Mesh(2, 4, coords, conn, np.array(dn, np.int32), np.array(dd, np.int32), np.array(dv, np.float64),
+                np.zeros(0, np.int32))

@@ -120,6 +120,43 @@ def test_generators_geometry():
     assert np.all(area > 0) and np.isclose(area.sum(), 1.0)
 
 
+def test_cylinder_channel_mesh():
+    """The cylinder-in-channel Quad4 mesh: counts of the default 64 x 256 O-grid, every
+    quad convex and counter-clockwise, the area equal to the channel minus the polygonal
+    cylinder, each node used, and the boundary conditions on the right node sets."""
+    m = meshgen.cylinder_channel()
+    assert m.nElem == 64 * 256 + (24 + 64 + 150) * (24 + 64 + 24) - 64 * 64
+    assert 38000 <= m.nElem <= 42000
+    assert np.array_equal(np.unique(m.conn), np.arange(m.nNode))
+    xx = m.coords[m.conn]
+    cross = [(xx[:, (k + 1) % 4, 0] - xx[:, k, 0]) * (xx[:, (k + 2) % 4, 1] - xx[:, (k + 1) % 4, 1])
+             - (xx[:, (k + 1) % 4, 1] - xx[:, k, 1]) * (xx[:, (k + 2) % 4, 0] - xx[:, (k + 1) % 4, 0])
+             for k in range(4)]
+    assert np.all(np.array(cross) > 0)
+    area = 0.5 * np.sum(xx[:, :, 0] * np.roll(xx[:, :, 1], -1, axis=1) - np.roll(xx[:, :, 0], -1, axis=1) * xx[:, :, 1])
+    c = m.coords
+    r = np.hypot(c[:, 0], c[:, 1])
+    cyl = np.isclose(r, 0.5)
+    ring = c[cyl][np.argsort(np.arctan2(c[cyl, 1], c[cyl, 0]))]  # the inscribed polygon of the cylinder
+    polygon = 0.5 * np.sum(ring[:, 0] * np.roll(ring[:, 1], -1) - np.roll(ring[:, 0], -1) * ring[:, 1])
+    assert 0.785 < polygon < np.pi / 4
+    assert np.isclose(area, 40.0 * 20.0 - polygon, rtol=1e-12)
+    inflow = np.isclose(c[:, 0], -10.0)
+    walls = (np.isclose(c[:, 1], -10.0) | np.isclose(c[:, 1], 10.0)) & ~inflow
+    assert cyl.sum() == 256 and inflow.sum() == 24 + 64 + 24 + 1
+    fixed = {(int(n), int(d)): v for n, d, v in zip(m.dbc_node, m.dbc_dof, m.dbc_value)}
+    assert len(fixed) == m.nDBC == 2 * (cyl.sum() + inflow.sum()) + walls.sum()
+    assert all(fixed[(n, 0)] == 1.0 and fixed[(n, 1)] == 0.0 for n in np.where(inflow)[0])
+    assert all(fixed[(n, 0)] == 0.0 and fixed[(n, 1)] == 0.0 for n in np.where(cyl)[0])
+    assert all((n, 0) not in fixed and fixed[(n, 1)] == 0.0 for n in np.where(walls)[0])
+    assert not any(d == 2 for _, d in fixed)
+    # the seeded inlet perturbation: relative amplitude exactly 1% at its peak, same seed same profile
+    p1 = meshgen.cylinder_channel(perturb=0.01, seed=3)
+    u_in = np.array([v for n, d, v in zip(p1.dbc_node, p1.dbc_dof, p1.dbc_value) if d == 0 and inflow[n]])
+    assert 0.99 - 1e-12 <= u_in.min() and u_in.max() <= 1.01 + 1e-12 and np.ptp(u_in) > 0.005
+    assert np.array_equal(p1.dbc_value, meshgen.cylinder_channel(perturb=0.01, seed=3).dbc_value)
+
+
 def test_forces_file_format(tmp_path):
     """The forces file: one 'timeNow Fx Fy' line per step in fixed notation with 14
     decimals (StabFEM.cpp:257-259, StabFEM2.cpp:815)."""
