@@ -370,6 +370,134 @@ def main_c4(a):
     print(json.dumps(out), flush=True)
 
 
+C5_METRIC = "C5: Newton/Krylov iterations and time-steps/s, learned vs extrapolated initial iterate"
+
+
+def c5_context(mesh, mu, opts):
+    from cfdsfemmpi_amd import deck, sfem
+    ctl = deck.Control(np.array([1.0, mu, 0, 0, 0, 0, 0, 0, 1.0, 1.0, 0.0]), 2, 0.0, 0.1, 1e9, 100000, 1, 1e-8)
+    fem = sfem.StabFEM(device=0)
+    fem.setMesh(mesh)
+    fem.setControl(ctl)
+    fem.prepareInputData()
+    fem.setSolver(opts)
+    fem.ctx.set_predictor(sfem.PRED_EXTRAPOLATE)
+    return fem
+
+
+def c5_run(ctx, opts, steps, newton_tol, record=None, record_from=0, label="", judged=False):
+    """steps time steps from the zero state (t = dt, 2 dt, ..., inflow ramped by min(t, 1));
+    per-step Newton iterations, Krylov iterations and iteration-0 residual norms.  With
+    a learner, the converged states of steps record_from, record_from + 1, ... are
+    recorded as one trajectory."""
+    newton, krylov, r0, cand = [], [], [], []
+    t, dt = 0.1, 0.1
+    t0 = time.perf_counter()
+    for k in range(steps):
+        st = ctx.step(min(t, 1.0), opts, newton_tol, 10, allow_notconv=False)
+        newton.append(st.newton_iters)
+        krylov.append(st.krylov_iters)
+        r0.append(st.rhs_norm0)
+        if judged:
+            cand.append([float(v) for v in ctx.predictor_stats()[2]])
+        if record is not None and k + 1 >= record_from:
+            record.record(ctx, new_trajectory=(k + 1 == record_from))
+        t += dt
+    sec = time.perf_counter() - t0
+    log(f"{label}: {steps} steps in {sec:.2f} s ({steps / sec:.2f} steps/s), Newton {sum(newton)}, "
+        f"Krylov {sum(krylov)}")
+    return {"seconds": sec, "steps_per_s": steps / sec, "newton": newton, "krylov": krylov, "r0": r0,
+            "candidates": cand}
+
+
+def main_c5(a):
+    """Learned initial iterate on the cylinder-in-channel Quad4 mesh: trajectories at
+    several Reynolds numbers with seeded 1% inlet perturbations train the per-node MLP,
+    then a run at Re = 100 with an unseen perturbation is timed from the extrapolated
+    start and from the learned start (judged per step by the iteration-0 residual)."""
+    from cfdsfemmpi_amd import meshgen, sfem
+    steps = a.steps if a.steps is not None else 60
+    res = [float(r) for r in a.c5_train_re.split(",")]
+    opts = sfem.solver_opts("bcgs", "ilu", 1e-12, 20000, reorder=a.reorder, atol=0.5 * a.newton_tol)
+    meshes = [meshgen.cylinder_channel(perturb=0.01, seed=1000 + k) for k in range(len(res))]
+    m0 = meshes[0]
+    log(f"workload: cylinder in channel, {m0.nElem} Quad4, {m0.nNode} nodes; training Re {res}, "
+        f"{a.c5_train_steps} steps each; evaluation Re 100, {steps} steps")
+    learner = None
+    train_rates = []
+    t_gen = time.perf_counter()
+    for k, (mesh, re) in enumerate(zip(meshes, res)):
+        fem = c5_context(mesh, 1.0 / re, opts)
+        if learner is None:
+            learner = sfem.Learner(fem.ctx, len(res) * a.c5_train_steps, seed=a.c5_seed)
+        r = c5_run(fem.ctx, opts, a.c5_train_steps, a.newton_tol, record=learner, record_from=a.c5_record_from,
+                   label=f"train Re {re:g}")
+        train_rates.append(r["steps_per_s"])
+        fem.ctx.close()
+    t_gen = time.perf_counter() - t_gen
+    t_fit = time.perf_counter()
+    loss = learner.train(a.c5_train_iters, a.c5_batch, a.c5_lr)
+    t_fit = time.perf_counter() - t_fit
+    log(f"training: {a.c5_train_iters} Adam steps x {a.c5_batch} samples in {t_fit:.2f} s, loss "
+        f"{loss[:50].mean():.4e} -> {loss[-50:].mean():.4e}")
+    evalmesh = meshgen.cylinder_channel(perturb=0.01, seed=99)
+    runs = {}
+    for kind in ("extrapolated", "learned"):
+        fem = c5_context(evalmesh, 0.01, opts)
+        if kind == "learned":
+            fem.ctx.set_learner(learner)
+            fem.ctx.set_predictor(sfem.PRED_LEARNED)
+        r = c5_run(fem.ctx, opts, steps, a.newton_tol, label=f"eval Re 100 {kind}", judged=(kind == "learned"))
+        r["soln"] = fem.ctx.get_solution()
+        if kind == "learned":
+            nl, nf, _ = fem.ctx.predictor_stats()
+            r["learned_starts"], r["fallback_starts"] = nl, nf
+        fem.ctx.close()
+        runs[kind] = r
+    ex, le = runs["extrapolated"], runs["learned"]
+    diff = float(np.max(np.abs(le["soln"] - ex["soln"])) / np.max(np.abs(ex["soln"])))
+    skip = min(a.c5_record_from, steps // 2)  # after the inflow ramp and its transient
+    out = {
+        "metric": C5_METRIC,
+        "value": le["steps_per_s"],
+        "unit": "time-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 0,
+        "ms_per_step": 1e3 * le["seconds"] / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 solve; MLP bf16 MFMA inference, f32 training",
+        "data": "synthetic (generated mesh, seeded inlet perturbations, zero initial state)",
+        "config": {"workload": f"2D cylinder D=1 in channel [-10,30]x[-10,10], O-grid 64x256 ratio 1.05 + "
+                               f"structured blocks, {m0.nElem} Quad4, inflow u=1 (+1% seeded perturbation), "
+                               f"slip walls, traction-free outflow, Re=100, dt=0.1, {steps} steps from rest",
+                   "training": {"trajectories": len(res), "re": res, "steps_each": a.c5_train_steps,
+                                "recorded_from_step": a.c5_record_from, "windows": learner.info()[1],
+                                "adam_steps": a.c5_train_iters, "batch": a.c5_batch, "lr": a.c5_lr,
+                                "seconds_generating": t_gen, "seconds_fitting": t_fit,
+                                "loss_first50": float(loss[:50].mean()), "loss_last50": float(loss[-50:].mean())},
+                   "solver": f"FP64 BiCGStab + ILU(0) (ordering {a.reorder}), stop at ||r|| <= max(1e-12 ||b||, "
+                             f"{0.5 * a.newton_tol:g}); Newton tol {a.newton_tol:g}",
+                   "parallelism": "single GPU"},
+        "extrapolated": {"steps_per_s": ex["steps_per_s"], "newton_total": sum(ex["newton"]),
+                         "krylov_total": sum(ex["krylov"]), "krylov_after_ramp": sum(ex["krylov"][skip:]),
+                         "krylov_per_step": ex["krylov"], "r0_per_step": ex["r0"]},
+        "learned": {"steps_per_s": le["steps_per_s"], "newton_total": sum(le["newton"]),
+                    "krylov_total": sum(le["krylov"]), "krylov_after_ramp": sum(le["krylov"][skip:]),
+                    "krylov_per_step": le["krylov"], "r0_per_step": le["r0"],
+                    "learned_starts": le["learned_starts"], "fallback_starts": le["fallback_starts"],
+                    "candidate_norms_per_step": le["candidates"]},
+        "krylov_ratio_learned_vs_extrapolated": sum(le["krylov"]) / max(1, sum(ex["krylov"])),
+        "steps_per_s_ratio_learned_vs_extrapolated": le["steps_per_s"] / ex["steps_per_s"],
+        "max_rel_diff_final_state_learned_vs_extrapolated": diff,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def copy_probe(mib):
     """A plain copy kernel of known size: its FETCH_SIZE / WRITE_SIZE readings give the
     factor between those counters and bytes moved."""
@@ -388,8 +516,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (10; 20 for c4)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c3", choices=["c3", "c2", "c4"],
+    ap.add_argument("--workload", default="c3", choices=["c3", "c2", "c4", "c5"],
                     help="c3: 115^3 hex cavity (the headline); c2: 32^3; c4: batch of 256 Tria3 cavities")
+    ap.add_argument("--c5-train-re", default="60,100,140,200", help="c5: Reynolds numbers of the training runs")
+    ap.add_argument("--c5-train-steps", type=int, default=80, help="c5: steps of each training run")
+    ap.add_argument("--c5-record-from", type=int, default=13,
+                    help="c5: first recorded step of a training run (the inflow ramp ends at step 10)")
+    ap.add_argument("--c5-train-iters", type=int, default=2000, help="c5: Adam steps")
+    ap.add_argument("--c5-batch", type=int, default=8192, help="c5: samples per Adam step")
+    ap.add_argument("--c5-lr", type=float, default=1e-3)
+    ap.add_argument("--c5-seed", type=int, default=0)
     ap.add_argument("--c4-count", type=int, default=256)
     ap.add_argument("--c4-seed", type=int, default=0)
     ap.add_argument("--c4-seq-stride", type=int, default=16,
@@ -440,6 +576,9 @@ def main():
         a.reorder = 2 if a.workload == "c4" else 3
     if a.workload == "c4":
         main_c4(a)
+        return
+    if a.workload == "c5":
+        main_c5(a)
         return
     if a.steps is None:
         a.steps = 10

@@ -217,7 +217,7 @@ void sfem_destroy(sfem_ctx* c) {
   dfree(c->d_vals); dfree(c->d_vals32); dfree(c->d_rhs); dfree(c->d_du);
   dfree(c->d_soln); dfree(c->d_prev); dfree(c->d_prev2); dfree(c->d_prev3); dfree(c->d_prev4);
   dfree(c->d_dot); dfree(c->d_dotPrev); dfree(c->d_cur); dfree(c->d_dotCur); dfree(c->d_applied);
-  dfree(c->d_probeK); dfree(c->d_probeF);
+  dfree(c->d_probeK); dfree(c->d_probeF); dfree(c->d_cand);
   if (c->h_pinned) (void)hipHostFree(c->h_pinned);
   hipEvent_t evs[] = {c->ev0, c->ev1, c->ev2, c->ev3};
   for (hipEvent_t e : evs)
@@ -502,8 +502,18 @@ int sfem_set_precision(sfem_ctx* c, int precision) {
 
 int sfem_set_predictor(sfem_ctx* c, int predictor) {
   if (!c) return set_error(SFEM_ERR_ARG, "null context");
-  if (predictor != SFEM_PRED_PREVIOUS && predictor != SFEM_PRED_EXTRAPOLATE) return set_error(SFEM_ERR_ARG, "bad predictor");
+  if (predictor != SFEM_PRED_PREVIOUS && predictor != SFEM_PRED_EXTRAPOLATE && predictor != SFEM_PRED_LEARNED)
+    return set_error(SFEM_ERR_ARG, "bad predictor");
+  if (predictor == SFEM_PRED_LEARNED && !c->learner) return set_error(SFEM_ERR_STATE, "attach a learner first");
   c->predictor = predictor;
+  return SFEM_OK;
+}
+
+int sfem_predictor_stats(sfem_ctx* c, int64_t* learned, int64_t* fallback, double* last_norms) {
+  if (!c) return set_error(SFEM_ERR_ARG, "null context");
+  if (learned) *learned = c->n_learned;
+  if (fallback) *fallback = c->n_fallback;
+  if (last_norms) { last_norms[0] = c->pred_norms[0]; last_norms[1] = c->pred_norms[1]; }
   return SFEM_OK;
 }
 
@@ -516,6 +526,46 @@ int sfem_set_pc_lag(sfem_ctx* c, int lag) {
   return SFEM_OK;
 }
 
+static int run_element_loop(sfem_ctx* c, int mode, int lift);
+
+// ||F_free|| at the state x with the prescribed values of this step in place (x is
+// overwritten at the prescribed DOFs).  The residual is affine in the state (the
+// tangent and the stabilisation use only previous steps), so this is the
+// iteration-0 norm of a Newton loop started from x.
+static int residual_norm_at(sfem_ctx* c, double* x, double timeFact, double* nrm) {
+  const int64_t n = nstate(c);
+  HIP_OK(launch_set_bc(c->stream, (int64_t)c->dbc_ind.size(), c->d_dbc_ind, c->d_dbc_val, timeFact, x));
+  HIP_OK(launch_iter_step(c->stream, n, time_coeffs(c), x, c->d_prev, c->d_prev2, c->d_prev3, c->d_prev4,
+                          c->d_dot, c->d_dotPrev, c->d_cur, c->d_dotCur));
+  HIP_OK(launch_zero_f64(c->stream, c->nfree, c->d_rhs));
+  int rc = run_element_loop(c, 1, 0);
+  if (rc) return rc;
+  rc = la_norm2(c, c->d_rhs, nrm);
+  c->kt.resolve();  // la_norm2 drained the stream
+  return rc;
+}
+
+// d_soln holds the extrapolated start; replace it by the learned start when that has
+// the smaller residual norm.
+static int judge_learned_start(sfem_ctx* c, double timeFact) {
+  if (!c->learner || !c->d_cand) return set_error(SFEM_ERR_STATE, "no learner attached");
+  const int64_t n = nstate(c);
+  int rc;
+  double ne = 0.0, nl = 0.0;
+  if ((rc = learned_candidate(c, c->d_prev, c->d_prev2, c->d_prev3, c->d_cand))) return rc;
+  if ((rc = residual_norm_at(c, c->d_cand, timeFact, &nl))) return rc;
+  if ((rc = residual_norm_at(c, c->d_soln, timeFact, &ne))) return rc;
+  c->pred_norms[0] = ne;
+  c->pred_norms[1] = nl;
+  if (std::isfinite(nl) && nl < ne) {
+    HIP_OK(hipMemcpyAsync(c->d_soln, c->d_cand, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    ++c->n_learned;
+  } else {
+    ++c->n_fallback;
+  }
+  return SFEM_OK;
+}
+
 int sfem_time_update(sfem_ctx* c, double timeFact) {
   if (!c) return set_error(SFEM_ERR_ARG, "null context");
   if (!(c->status & ST_PARAMS)) return set_error(SFEM_ERR_STATE, "parameters not set");
@@ -524,8 +574,12 @@ int sfem_time_update(sfem_ctx* c, double timeFact) {
   const int64_t n = nstate(c);
   HIP_OK(launch_time_update(c->stream, n, c->d_soln, c->d_prev, c->d_prev2, c->d_prev3, c->d_prev4, c->d_dot,
                             c->d_dotPrev));
-  if (c->predictor == SFEM_PRED_EXTRAPOLATE)
+  if (c->predictor == SFEM_PRED_EXTRAPOLATE || c->predictor == SFEM_PRED_LEARNED)
     HIP_OK(launch_extrapolate(c->stream, n, c->d_prev, c->d_prev2, c->d_soln));
+  if (c->predictor == SFEM_PRED_LEARNED) {
+    int rc = judge_learned_start(c, timeFact);
+    if (rc) return rc;
+  }
   HIP_OK(launch_assign_bc(c->stream, (int64_t)c->dbc_ind.size(), c->d_dbc_ind, c->d_dbc_val, timeFact, c->d_soln,
                           c->d_applied));
   return SFEM_OK;
@@ -627,7 +681,7 @@ int sfem_get_system(sfem_ctx* c, double* vals, double* rhs) {
 int sfem_solve(sfem_ctx* c, const sfem_solver_opts* o, int* iters, double* relres) {
   if (!c || !o) return set_error(SFEM_ERR_ARG, "null argument");
   if (!(c->status & ST_ASSEMBLED)) return set_error(SFEM_ERR_STATE, "assemble the system first");  // SolverPetsc.cpp:288-292
-  if (o->rtol <= 0.0 || o->max_it <= 0) return set_error(SFEM_ERR_ARG, "bad solver options");
+  if (o->rtol <= 0.0 || o->max_it <= 0 || !(o->atol >= 0.0)) return set_error(SFEM_ERR_ARG, "bad solver options");
   if (o->ksp_type != SFEM_KSP_BCGS && o->ksp_type != SFEM_KSP_GMRES) return set_error(SFEM_ERR_ARG, "bad ksp_type");
   if (o->pc_type < SFEM_PC_NONE || o->pc_type > SFEM_PC_ILU0) return set_error(SFEM_ERR_ARG, "bad pc_type");
   HIP_OK(hipSetDevice(c->device));

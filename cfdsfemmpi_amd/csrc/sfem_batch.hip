@@ -641,6 +641,7 @@ int sfem_step_batch(sfem_ctx* c, double timeFact, const sfem_solver_opts* opts, 
   if (!c || !opts || !c->batch) return set_error(SFEM_ERR_ARG, "no batch set or null argument");
   if (opts->pc_type != SFEM_PC_ILU0 || opts->ksp_type != SFEM_KSP_BCGS)
     return set_error(SFEM_ERR_UNSUPPORTED, "batched steps use BiCGStab + ILU(0)");
+  if (opts->atol != 0.0) return set_error(SFEM_ERR_UNSUPPORTED, "batched steps use the relative test only (atol = 0)");
   Batch& B = *c->batch;
   const int nm = B.nmesh;
   HIP_OK(hipSetDevice(c->device));

@@ -210,4 +210,14 @@ struct sfem_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   double last_asm_ms = 0.0, last_solve_ms = 0.0;
   double* h_pinned = nullptr;   // small pinned host buffer for scalar readback
+
+  // learned initial iterate (sfem_learned.hip): candidate buffer and the judge's record
+  sfem_learner* learner = nullptr;
+  double* d_cand = nullptr;          // nNode*ndof
+  int64_t n_learned = 0, n_fallback = 0;
+  double pred_norms[2] = {0.0, 0.0};  // ||F|| at the extrapolated and the learned start
 };
+
+namespace sfem {
+int learned_candidate(sfem_ctx* c, const double* s0, const double* s1, const double* s2, double* out);
+}

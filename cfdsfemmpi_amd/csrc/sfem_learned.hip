@@ -1,0 +1,751 @@
+// sfem_learned.hip -- learned Newton initial iterate for 2D meshes (u, v, p per node).
+//
+// A per-node MLP 12 -> 64 -> 64 -> 3 (SiLU) predicts the correction of the next
+// step's increment over the linearly extrapolated one.  Inputs per node i, each
+// component c scaled by sigma_c, the RMS over all nodes of the last increment:
+//   d1 = s_n - s_{n-1}, d2 = s_{n-1} - s_{n-2}         (6 values)
+//   mean over the 1-ring node neighbours j of d1_j, d2_j, minus d1, d2 (6 values)
+// The candidate is s_n + d1 + sigma * MLP(x), so an MLP with zero output gives the
+// extrapolated start 2 s_n - s_{n-1} exactly.
+//
+// Training (k_mlp_grad + k_adam): FP32 on the vector ALUs, one sample per lane,
+// per-workgroup partial gradients reduced by the Adam kernel.  Inference
+// (k_mlp_infer): BF16 weights and activations on MFMA (v_mfma_f32_16x16x16_bf16)
+// with FP32 accumulation, one wavefront per 16 nodes.
+//
+// No counterpart in the reference: its Newton loop starts from the previous
+// solution (StabFEM2.cpp:593-832, solnExtrap unused).  The step's Newton residual
+// decides convergence whatever the start, so the predictor changes iteration
+// counts, not the converged answer.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "sfem_internal.hpp"
+#include "sfem_linalg.hpp"
+
+namespace sfem {
+
+constexpr int kIn = 12, kHid = 64, kOut = 3;
+constexpr int P_W1 = 0, P_B1 = P_W1 + kHid * kIn, P_W2 = P_B1 + kHid, P_B2 = P_W2 + kHid * kHid,
+              P_W3 = P_B2 + kHid, P_B3 = P_W3 + kOut * kHid, kNP = P_B3 + kOut;
+static_assert(kNP == SFEM_LEARNER_NPARAM, "parameter count");
+constexpr int kNPpad = 5192;          // row pitch of the partial-gradient buffer
+constexpr int kGradTB = 64;           // samples per training workgroup, one per lane
+constexpr int kLd = kHid + 1;         // LDS row pitch (conflict-free row and column access)
+constexpr int kPackV4 = (4 + 16 + 4) * 64;  // packed BF16 operand tiles, 4 values each
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// splitmix64 finaliser (Steele, Lea, Flood 2014): per-sample hash for minibatches.
+__host__ __device__ inline uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__device__ inline float sigm(float z) { return 1.0f / (1.0f + __expf(-z)); }
+
+// f32 -> bf16, round to nearest even (finite inputs)
+__device__ inline short to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (short)(u >> 16);
+}
+
+// Features of node i from three consecutive states (S = float snapshots or double
+// solution vectors, 3 components per node); d1 returned in double for the epilogue.
+template <typename S>
+__device__ inline void node_features(int64_t i, const S* __restrict__ s0, const S* __restrict__ s1,
+                                     const S* __restrict__ s2, const int32_t* __restrict__ adjptr,
+                                     const int32_t* __restrict__ adj, const float isig[3], float x[kIn],
+                                     double d1[3]) {
+  double d2[3], m1[3] = {0.0, 0.0, 0.0}, m2[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double a = (double)s0[i * 3 + c], b = (double)s1[i * 3 + c], e = (double)s2[i * 3 + c];
+    d1[c] = a - b;
+    d2[c] = b - e;
+  }
+  const int32_t lo = adjptr[i], hi = adjptr[i + 1];
+  for (int32_t k = lo; k < hi; ++k) {
+    const int64_t j = adj[k];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double a = (double)s0[j * 3 + c], b = (double)s1[j * 3 + c], e = (double)s2[j * 3 + c];
+      m1[c] += a - b;
+      m2[c] += b - e;
+    }
+  }
+  const double inv = hi > lo ? 1.0 / (double)(hi - lo) : 0.0;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    x[c] = (float)(d1[c] * isig[c]);
+    x[3 + c] = (float)(d2[c] * isig[c]);
+    x[6 + c] = hi > lo ? (float)((m1[c] * inv - d1[c]) * isig[c]) : 0.0f;
+    x[9 + c] = hi > lo ? (float)((m2[c] * inv - d2[c]) * isig[c]) : 0.0f;
+  }
+}
+
+// acc[c] += sum over nodes of (a - b)^2 for component c (3 components per node).
+template <typename S>
+__global__ __launch_bounds__(256) void k_sq_increment(int64_t nNode, const S* __restrict__ a,
+                                                      const S* __restrict__ b, double* __restrict__ acc) {
+  __shared__ double red[3][256];
+  double s[3] = {0.0, 0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < nNode; i += (int64_t)gridDim.x * 256) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double d = (double)a[i * 3 + c] - (double)b[i * 3 + c];
+      s[c] += d * d;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) red[c][threadIdx.x] = s[c];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int c = 0; c < 3; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) atomicAdd(&acc[threadIdx.x], red[threadIdx.x][0]);
+}
+
+__global__ void k_zero3(double* acc) {
+  if (threadIdx.x < 3) acc[threadIdx.x] = 0.0;
+}
+
+// rms[c] = sqrt(acc[c] / nNode) (FP32 copy for the training store)
+__global__ void k_store_rms(const double* __restrict__ acc, int64_t nNode, float* __restrict__ rms) {
+  if (threadIdx.x < 3) rms[threadIdx.x] = (float)sqrt(acc[threadIdx.x] / (double)nNode);
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void k_snapshot(int64_t n, const S* __restrict__ src, float* __restrict__ dst) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    dst[i] = (float)src[i];
+}
+
+// ---------------------------------------------------------------------------------
+// Training: loss = sum over samples and free components of (y - t)^2 / (3 batch),
+// t = (d0 - d1) / sigma with d0 = s_{n+1} - s_n.  Sample k of iteration `iter` is
+// (window, node) from explicit lists, or from mix64(seed ^ mix64(iter * 2^32 + k)):
+// window = windows[(h >> 32) % nwin], node = (h & 0xffffffff) % nNode.
+// Each workgroup writes its partial gradient (kNP values) and partial loss.
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(kGradTB) void k_mlp_grad(
+    const float* __restrict__ w, int64_t nNode, int64_t nstate, const float* __restrict__ snap,
+    const float* __restrict__ rms, const int32_t* __restrict__ windows, int64_t nwin,
+    const int32_t* __restrict__ adjptr, const int32_t* __restrict__ adj, const uint8_t* __restrict__ mask,
+    uint64_t seed, int64_t iter, int batch, const int32_t* __restrict__ ex_win,
+    const int32_t* __restrict__ ex_node, float inv_count, float* __restrict__ part,
+    float* __restrict__ part_loss) {
+  __shared__ float A[kGradTB][kLd];   // Z1, then H1 = silu(Z1)
+  __shared__ float B[kGradTB][kLd];   // Z2, then H2, then G1 = dL/dZ1
+  __shared__ float Cg[kGradTB][kLd];  // G2 = dL/dZ2
+  __shared__ float X[kGradTB][kIn + 1];
+  __shared__ float GY[kGradTB][4];
+  const int s = threadIdx.x;
+  const int64_t k = (int64_t)blockIdx.x * kGradTB + s;
+  float x[kIn], t[3], mk[3];
+#pragma unroll
+  for (int q = 0; q < kIn; ++q) x[q] = 0.0f;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) { t[c] = 0.0f; mk[c] = 0.0f; }
+  if (k < batch) {
+    int64_t win, node;
+    if (ex_win) {
+      win = ex_win[k];
+      node = ex_node[k];
+    } else {
+      const uint64_t h = mix64(seed ^ mix64((uint64_t)iter * 0x100000000ull + (uint64_t)k));
+      win = windows[(h >> 32) % (uint64_t)nwin];
+      node = (int64_t)((h & 0xffffffffull) % (uint64_t)nNode);
+    }
+    const float* s0 = snap + win * nstate;
+    float isig[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float r = rms[win * 3 + c];
+      isig[c] = r > 0.0f ? 1.0f / r : 0.0f;
+    }
+    double d1[3];
+    node_features<float>(node, s0, s0 - nstate, s0 - 2 * nstate, adjptr, adj, isig, x, d1);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double d0 = (double)s0[nstate + node * 3 + c] - (double)s0[node * 3 + c];
+      t[c] = (float)((d0 - d1[c]) * isig[c]);
+      mk[c] = mask[node * 3 + c] ? 1.0f : 0.0f;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kIn; ++q) X[s][q] = x[q];
+  // forward, layer 1 (h1 kept in registers)
+  float h[kHid];
+#pragma unroll
+  for (int o = 0; o < kHid; ++o) {
+    float z = w[P_B1 + o];
+#pragma unroll
+    for (int q = 0; q < kIn; ++q) z += w[P_W1 + o * kIn + q] * x[q];
+    A[s][o] = z;
+    h[o] = z * sigm(z);
+  }
+  // layer 2
+  for (int o = 0; o < kHid; ++o) {
+    float z = w[P_B2 + o];
+#pragma unroll
+    for (int i = 0; i < kHid; ++i) z += w[P_W2 + o * kHid + i] * h[i];
+    B[s][o] = z;
+  }
+  // layer 3, loss and dL/dy
+  float y[3] = {w[P_B3], w[P_B3 + 1], w[P_B3 + 2]};
+#pragma unroll
+  for (int i = 0; i < kHid; ++i) {
+    const float z = B[s][i];
+    const float hz = z * sigm(z);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) y[c] += w[P_W3 + c * kHid + i] * hz;
+  }
+  float gy[3], loss = 0.0f;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float e = (y[c] - t[c]) * mk[c];
+    loss += e * e;
+    gy[c] = 2.0f * e * inv_count;
+    GY[s][c] = gy[c];
+  }
+  GY[s][3] = loss;
+  // G2 = (W3^T gy) * silu'(Z2); B becomes H2
+#pragma unroll
+  for (int i = 0; i < kHid; ++i) {
+    const float z = B[s][i];
+    const float sg = sigm(z);
+    const float g = w[P_W3 + i] * gy[0] + w[P_W3 + kHid + i] * gy[1] + w[P_W3 + 2 * kHid + i] * gy[2];
+    Cg[s][i] = g * sg * (1.0f + z * (1.0f - sg));
+    B[s][i] = z * sg;
+  }
+  __syncthreads();
+  float* pg = part + (int64_t)blockIdx.x * kNPpad;
+  // dW3, db3 (lane = hidden unit)
+  {
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+    for (int r = 0; r < kGradTB; ++r) {
+      const float hv = B[r][s];
+      a0 += GY[r][0] * hv;
+      a1 += GY[r][1] * hv;
+      a2 += GY[r][2] * hv;
+    }
+    pg[P_W3 + s] = a0;
+    pg[P_W3 + kHid + s] = a1;
+    pg[P_W3 + 2 * kHid + s] = a2;
+    if (s < 3) {
+      float b = 0.f;
+      for (int r = 0; r < kGradTB; ++r) b += GY[r][s];
+      pg[P_B3 + s] = b;
+    }
+    if (s == 0) {
+      float l = 0.f;
+      for (int r = 0; r < kGradTB; ++r) l += GY[r][3];
+      part_loss[blockIdx.x] = l;
+    }
+  }
+  __syncthreads();
+  // G1 = (W2^T G2) * silu'(Z1) into B; A becomes H1
+  {
+    float acc[kHid];
+#pragma unroll
+    for (int i = 0; i < kHid; ++i) acc[i] = 0.f;
+    for (int o = 0; o < kHid; ++o) {
+      const float g = Cg[s][o];
+#pragma unroll
+      for (int i = 0; i < kHid; ++i) acc[i] += w[P_W2 + o * kHid + i] * g;
+    }
+#pragma unroll
+    for (int i = 0; i < kHid; ++i) {
+      const float z = A[s][i];
+      const float sg = sigm(z);
+      B[s][i] = acc[i] * sg * (1.0f + z * (1.0f - sg));
+      A[s][i] = z * sg;
+    }
+  }
+  __syncthreads();
+  // dW2[o][lane] = sum_r G2[r][o] H1[r][lane]; db2, dW1, db1 (lane = unit)
+  for (int o = 0; o < kHid; ++o) {
+    float a = 0.f;
+    for (int r = 0; r < kGradTB; ++r) a += Cg[r][o] * A[r][s];
+    pg[P_W2 + o * kHid + s] = a;
+  }
+  {
+    float b2 = 0.f, b1 = 0.f, a1[kIn];
+#pragma unroll
+    for (int q = 0; q < kIn; ++q) a1[q] = 0.f;
+    for (int r = 0; r < kGradTB; ++r) {
+      const float g1 = B[r][s];
+      b2 += Cg[r][s];
+      b1 += g1;
+#pragma unroll
+      for (int q = 0; q < kIn; ++q) a1[q] += g1 * X[r][q];
+    }
+    pg[P_B2 + s] = b2;
+    pg[P_B1 + s] = b1;
+#pragma unroll
+    for (int q = 0; q < kIn; ++q) pg[P_W1 + s * kIn + q] = a1[q];
+  }
+}
+
+// Sum of the partial gradients; with lr > 0 one Adam step (Kingma & Ba 2015, bias
+// corrected) on the FP32 master weights.  grad_out (optional) receives the summed
+// gradient, loss_out[0] the summed loss.
+__global__ __launch_bounds__(256) void k_adam(int nblk, const float* __restrict__ part,
+                                              const float* __restrict__ part_loss, float* __restrict__ w,
+                                              float* __restrict__ m, float* __restrict__ v, float lr, float b1,
+                                              float b2, float eps, float c1, float c2,
+                                              float* __restrict__ grad_out, float* __restrict__ loss_out) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p < kNP) {
+    float g = 0.f;
+    for (int b = 0; b < nblk; ++b) g += part[(int64_t)b * kNPpad + p];
+    if (grad_out) grad_out[p] = g;
+    if (lr > 0.f) {
+      const float mm = b1 * m[p] + (1.f - b1) * g;
+      const float vv = b2 * v[p] + (1.f - b2) * g * g;
+      m[p] = mm;
+      v[p] = vv;
+      w[p] -= lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0 && loss_out) {
+    float l = 0.f;
+    for (int b = 0; b < nblk; ++b) l += part_loss[b];
+    loss_out[0] = l;
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// Inference on MFMA.  v_mfma_f32_16x16x16_bf16 operand layout (lane l, j = 0..3):
+//   A[m][k]: m = l % 16, k = 4 (l / 16) + j;  B[k][n]: n = l % 16, k = 4 (l / 16) + j;
+//   D[m][n]: n = l % 16, m = 4 (l / 16) + j.
+// The layers are computed transposed (H^T = W X^T): nodes are the n index, so the
+// D tile of one layer is, unchanged, the B operand of the next layer's K block.
+// Packed tiles (4 bf16 per lane): W1 [mo][lane], W2 [mo][kt][lane], W3 [kt][lane].
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_pack_weights(const float* __restrict__ w, short* __restrict__ pk) {
+  const int e = blockIdx.x * 256 + threadIdx.x;  // one bf16 value
+  if (e >= kPackV4 * 4) return;
+  const int j = e & 3, l = (e >> 2) & 63, tile = e >> 8;
+  const int m = l & 15, k = 4 * (l >> 4) + j;
+  float val = 0.f;
+  if (tile < 4) {  // W1: mo = tile
+    const int row = tile * 16 + m;
+    val = k < kIn ? w[P_W1 + row * kIn + k] : 0.f;
+  } else if (tile < 20) {  // W2: (mo, kt)
+    const int mo = (tile - 4) >> 2, kt = (tile - 4) & 3;
+    val = w[P_W2 + (mo * 16 + m) * kHid + kt * 16 + k];
+  } else {  // W3: kt
+    const int kt = tile - 20;
+    val = m < kOut ? w[P_W3 + m * kHid + kt * 16 + k] : 0.f;
+  }
+  pk[e] = to_bf16(val);
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void k_mlp_infer(int64_t nNode, const S* __restrict__ s0,
+                                                   const S* __restrict__ s1, const S* __restrict__ s2,
+                                                   const int32_t* __restrict__ adjptr,
+                                                   const int32_t* __restrict__ adj, const double* __restrict__ sqacc,
+                                                   const float* __restrict__ w, const v4s* __restrict__ pk,
+                                                   double* __restrict__ out) {
+  const int l = threadIdx.x & 63, g = l >> 4;
+  const int64_t node = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16 + (l & 15);
+  const bool valid = node < nNode;
+  float sig[3], isig[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    sig[c] = (float)sqrt(sqacc[c] / (double)nNode);
+    isig[c] = sig[c] > 0.f ? 1.f / sig[c] : 0.f;
+  }
+  float x[kIn];
+  double d1[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int q = 0; q < kIn; ++q) x[q] = 0.f;
+  if (valid) node_features<S>(node, s0, s1, s2, adjptr, adj, isig, x, d1);
+  v4s xb;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const float v = g == 0 ? x[j] : g == 1 ? x[4 + j] : g == 2 ? x[8 + j] : 0.f;
+    xb[j] = to_bf16(v);
+  }
+  const v4f zero = {0.f, 0.f, 0.f, 0.f};
+  v4s hb[4];
+#pragma unroll
+  for (int mo = 0; mo < 4; ++mo) {
+    const v4f acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(pk[mo * 64 + l], xb, zero, 0, 0, 0);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const float z = acc[j] + w[P_B1 + mo * 16 + 4 * g + j];
+      hb[mo][j] = to_bf16(z * sigm(z));
+    }
+  }
+  v4s h2[4];
+#pragma unroll
+  for (int mo = 0; mo < 4; ++mo) {
+    v4f acc = zero;
+#pragma unroll
+    for (int kt = 0; kt < 4; ++kt)
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(pk[256 + (mo * 4 + kt) * 64 + l], hb[kt], acc, 0, 0, 0);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const float z = acc[j] + w[P_B2 + mo * 16 + 4 * g + j];
+      h2[mo][j] = to_bf16(z * sigm(z));
+    }
+  }
+  v4f acc = zero;
+#pragma unroll
+  for (int kt = 0; kt < 4; ++kt)
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(pk[1280 + kt * 64 + l], h2[kt], acc, 0, 0, 0);
+  if (valid && g == 0) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double y = (double)(acc[c] + w[P_B3 + c]);
+      out[node * 3 + c] = (double)s0[node * 3 + c] + d1[c] + (double)sig[c] * y;
+    }
+  }
+}
+
+}  // namespace sfem
+
+using namespace sfem;
+
+struct sfem_learner {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int64_t nNode = 0, nstate = 0;
+  int32_t *d_adjptr = nullptr, *d_adj = nullptr;
+  uint8_t* d_mask = nullptr;
+  int64_t cap = 0, count = 0;
+  float* d_snap = nullptr;  // cap x nstate
+  float* d_rms = nullptr;   // cap x 3
+  std::vector<int32_t> traj;
+  int32_t cur_traj = -1;
+  std::vector<int32_t> windows;
+  int32_t* d_windows = nullptr;
+  int64_t dev_windows = -1;
+  float *d_w = nullptr, *d_m = nullptr, *d_v = nullptr;
+  int64_t adam_t = 0;
+  short* d_pack = nullptr;
+  bool pack_stale = true;
+  float *d_part = nullptr, *d_ploss = nullptr, *d_grad = nullptr, *d_loss = nullptr;
+  int part_blocks = 0;
+  double* d_acc = nullptr;
+  int32_t *d_exw = nullptr, *d_exn = nullptr;
+  int ex_cap = 0;
+  uint64_t seed = 0;
+};
+
+namespace {
+
+#define LHIP(x)                                                                      \
+  do {                                                                               \
+    hipError_t e_ = (x);                                                             \
+    if (e_ != hipSuccess) return set_error(SFEM_ERR_HIP, (std::string(#x) + ": " + hipGetErrorString(e_)).c_str()); \
+  } while (0)
+
+template <typename T>
+int lalloc(T** p, int64_t n) {
+  if (hipMalloc((void**)p, (size_t)std::max<int64_t>(n, 1) * sizeof(T)) != hipSuccess)
+    return set_error(SFEM_ERR_NOMEM, "learner: device allocation failed");
+  return SFEM_OK;
+}
+
+int ensure_part(sfem_learner* L, int nblk) {
+  if (L->part_blocks >= nblk) return SFEM_OK;
+  if (L->d_part) (void)hipFree(L->d_part);
+  if (L->d_ploss) (void)hipFree(L->d_ploss);
+  L->d_part = nullptr;
+  L->d_ploss = nullptr;
+  int rc;
+  if ((rc = lalloc(&L->d_part, (int64_t)nblk * kNPpad)) || (rc = lalloc(&L->d_ploss, nblk))) return rc;
+  L->part_blocks = nblk;
+  return SFEM_OK;
+}
+
+int repack(sfem_learner* L, hipStream_t st) {
+  if (!L->pack_stale) return SFEM_OK;
+  hipLaunchKernelGGL(k_pack_weights, dim3((kPackV4 * 4 + 255) / 256), dim3(256), 0, st, L->d_w, L->d_pack);
+  LHIP(hipGetLastError());
+  L->pack_stale = false;
+  return SFEM_OK;
+}
+
+// one gradient launch + reduction (lr > 0: Adam step)
+int grad_step(sfem_learner* L, int batch, const int32_t* exw, const int32_t* exn, float lr, float* loss_dev) {
+  const int nblk = (batch + kGradTB - 1) / kGradTB;
+  int rc;
+  if ((rc = ensure_part(L, nblk))) return rc;
+  const float inv_count = 1.0f / (3.0f * (float)batch);
+  hipLaunchKernelGGL(k_mlp_grad, dim3(nblk), dim3(kGradTB), 0, L->stream, L->d_w, L->nNode, L->nstate,
+                     L->d_snap, L->d_rms, L->d_windows, (int64_t)L->windows.size(), L->d_adjptr, L->d_adj,
+                     L->d_mask, L->seed, L->adam_t, batch, exw, exn, inv_count, L->d_part, L->d_ploss);
+  LHIP(hipGetLastError());
+  float c1 = 1.f, c2 = 1.f;
+  const float b1 = 0.9f, b2 = 0.999f;
+  if (lr > 0.f) {
+    ++L->adam_t;
+    c1 = 1.f - (float)std::pow((double)b1, (double)L->adam_t);
+    c2 = 1.f - (float)std::pow((double)b2, (double)L->adam_t);
+    L->pack_stale = true;
+  }
+  hipLaunchKernelGGL(k_adam, dim3((kNP + 255) / 256), dim3(256), 0, L->stream, nblk, L->d_part, L->d_ploss,
+                     L->d_w, L->d_m, L->d_v, lr, b1, b2, 1e-8f, c1, c2, lr > 0.f ? nullptr : L->d_grad,
+                     loss_dev);
+  LHIP(hipGetLastError());
+  return SFEM_OK;
+}
+
+}  // namespace
+
+namespace sfem {
+
+// Learned candidate from three consecutive states (s0 newest) of the context into
+// `out` (nNode*3 doubles, device), on the context's stream.
+int learned_candidate(sfem_ctx* c, const double* s0, const double* s1, const double* s2, double* out) {
+  sfem_learner* L = c->learner;
+  if (!L) return set_error(SFEM_ERR_STATE, "no learner set");
+  int rc;
+  if ((rc = repack(L, c->stream))) return rc;
+  hipLaunchKernelGGL(k_zero3, dim3(1), dim3(64), 0, c->stream, L->d_acc);
+  const int64_t nn = L->nNode;
+  const unsigned gr = (unsigned)std::min<int64_t>((nn + 255) / 256, 1024);
+  hipLaunchKernelGGL((k_sq_increment<double>), dim3(gr), dim3(256), 0, c->stream, nn, s0, s1, L->d_acc);
+  hipLaunchKernelGGL((k_mlp_infer<double>), dim3((unsigned)((nn + 63) / 64)), dim3(256), 0, c->stream, nn, s0, s1,
+                     s2, L->d_adjptr, L->d_adj, L->d_acc, L->d_w, (const v4s*)L->d_pack, out);
+  LHIP(hipGetLastError());
+  return SFEM_OK;
+}
+
+}  // namespace sfem
+
+extern "C" {
+
+sfem_learner* sfem_learner_create(sfem_ctx* c, int64_t capacity, uint64_t seed) {
+  if (!c || capacity < 4) { set_error(SFEM_ERR_ARG, "learner: null context or capacity < 4"); return nullptr; }
+  if (c->dims.ndof != 3) { set_error(SFEM_ERR_UNSUPPORTED, "learner: 2D meshes (u, v, p per node) only"); return nullptr; }
+  if (!(c->status & 2)) { set_error(SFEM_ERR_STATE, "learner: build the pattern first"); return nullptr; }
+  if (hipSetDevice(c->device) != hipSuccess) { set_error(SFEM_ERR_HIP, "learner: hipSetDevice"); return nullptr; }
+  sfem_learner* L = new sfem_learner();
+  L->device = c->device;
+  L->nNode = c->nNode;
+  L->nstate = c->nNode * 3;
+  L->cap = capacity;
+  L->seed = seed;
+  // node graph of the mesh (1-ring, no self)
+  const int npe = c->dims.npe;
+  std::vector<std::vector<int32_t>> nb(c->nNode);
+  for (int64_t e = 0; e < c->nElem; ++e)
+    for (int a = 0; a < npe; ++a)
+      for (int b = 0; b < npe; ++b)
+        if (a != b) nb[c->conn[e * npe + a]].push_back(c->conn[e * npe + b]);
+  std::vector<int32_t> ptr(c->nNode + 1, 0), adj;
+  for (int64_t i = 0; i < c->nNode; ++i) {
+    auto& v = nb[i];
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    adj.insert(adj.end(), v.begin(), v.end());
+    ptr[i + 1] = (int32_t)adj.size();
+  }
+  std::vector<uint8_t> mask(L->nstate);
+  for (int64_t i = 0; i < L->nstate; ++i) mask[i] = c->nodeDof[i] >= 0 ? 1 : 0;
+  // initial weights: uniform(-a, a), a = sqrt(6 / (fan_in + fan_out)), last layer zero
+  std::vector<float> w(kNP, 0.f);
+  uint64_t st = seed;
+  auto uni = [&](float a) {
+    st = mix64(st);
+    return (float)(((double)(st >> 11) * (1.0 / 9007199254740992.0)) * 2.0 - 1.0) * a;
+  };
+  const float a1 = std::sqrt(6.f / (kIn + kHid)), a2 = std::sqrt(6.f / (kHid + kHid));
+  for (int i = 0; i < kHid * kIn; ++i) w[P_W1 + i] = uni(a1);
+  for (int i = 0; i < kHid * kHid; ++i) w[P_W2 + i] = uni(a2);
+  int rc = SFEM_OK;
+  auto up = [&](auto** p, const auto& v) {
+    if (rc) return;
+    if ((rc = lalloc(p, (int64_t)v.size()))) return;
+    if (hipMemcpy(*p, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice) != hipSuccess)
+      rc = set_error(SFEM_ERR_HIP, "learner: upload");
+  };
+  up(&L->d_adjptr, ptr);
+  up(&L->d_adj, adj);
+  up(&L->d_mask, mask);
+  up(&L->d_w, w);
+  std::vector<float> zeros(kNP, 0.f);
+  up(&L->d_m, zeros);
+  up(&L->d_v, zeros);
+  if (!rc) rc = lalloc(&L->d_snap, capacity * L->nstate);
+  if (!rc) rc = lalloc(&L->d_rms, capacity * 3);
+  if (!rc) rc = lalloc(&L->d_pack, (int64_t)kPackV4 * 4);
+  if (!rc) rc = lalloc(&L->d_grad, kNP);
+  if (!rc) rc = lalloc(&L->d_loss, 1);
+  if (!rc) rc = lalloc(&L->d_acc, 3);
+  if (!rc && hipStreamCreate(&L->stream) != hipSuccess) rc = set_error(SFEM_ERR_HIP, "learner: stream");
+  if (rc) {
+    sfem_learner_destroy(L);
+    return nullptr;
+  }
+  return L;
+}
+
+void sfem_learner_destroy(sfem_learner* L) {
+  if (!L) return;
+  (void)hipSetDevice(L->device);
+  if (L->stream) (void)hipStreamSynchronize(L->stream);
+  void* ps[] = {L->d_adjptr, L->d_adj, L->d_mask, L->d_snap, L->d_rms, L->d_windows, L->d_w, L->d_m, L->d_v,
+                L->d_pack, L->d_part, L->d_ploss, L->d_grad, L->d_loss, L->d_acc, L->d_exw, L->d_exn};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  if (L->stream) (void)hipStreamDestroy(L->stream);
+  delete L;
+}
+
+int sfem_learner_record(sfem_learner* L, sfem_ctx* c, int new_trajectory) {
+  if (!L || !c) return set_error(SFEM_ERR_ARG, "null argument");
+  if (c->nNode != L->nNode || c->dims.ndof != 3) return set_error(SFEM_ERR_ARG, "learner: mesh does not match");
+  if (L->count >= L->cap) return set_error(SFEM_ERR_ARG, "learner: snapshot store is full");
+  LHIP(hipSetDevice(L->device));
+  LHIP(hipStreamSynchronize(c->stream));
+  if (new_trajectory || L->cur_traj < 0) ++L->cur_traj;
+  const int64_t t = L->count;
+  float* dst = L->d_snap + t * L->nstate;
+  const unsigned gr = (unsigned)std::min<int64_t>((L->nstate + 255) / 256, 4096);
+  hipLaunchKernelGGL((k_snapshot<double>), dim3(gr), dim3(256), 0, L->stream, L->nstate, c->d_soln, dst);
+  hipLaunchKernelGGL(k_zero3, dim3(1), dim3(64), 0, L->stream, L->d_acc);
+  const bool cont = t > 0 && L->traj[t - 1] == L->cur_traj;
+  if (cont) {
+    const unsigned gn = (unsigned)std::min<int64_t>((L->nNode + 255) / 256, 1024);
+    hipLaunchKernelGGL((k_sq_increment<float>), dim3(gn), dim3(256), 0, L->stream, L->nNode, dst,
+                       dst - L->nstate, L->d_acc);
+  }
+  hipLaunchKernelGGL(k_store_rms, dim3(1), dim3(64), 0, L->stream, L->d_acc, L->nNode, L->d_rms + t * 3);
+  LHIP(hipGetLastError());
+  LHIP(hipStreamSynchronize(L->stream));
+  L->traj.push_back(L->cur_traj);
+  L->count = t + 1;
+  // window at t-1: snapshots t-3 .. t of one trajectory
+  if (t >= 3 && L->traj[t - 3] == L->cur_traj) L->windows.push_back((int32_t)(t - 1));
+  return SFEM_OK;
+}
+
+int sfem_learner_info(sfem_learner* L, int64_t* snapshots, int64_t* windows, int64_t* adam_steps) {
+  if (!L) return set_error(SFEM_ERR_ARG, "null learner");
+  if (snapshots) *snapshots = L->count;
+  if (windows) *windows = (int64_t)L->windows.size();
+  if (adam_steps) *adam_steps = L->adam_t;
+  return SFEM_OK;
+}
+
+static int upload_windows(sfem_learner* L) {
+  if (L->dev_windows == (int64_t)L->windows.size()) return SFEM_OK;
+  if (L->d_windows) (void)hipFree(L->d_windows);
+  L->d_windows = nullptr;
+  int rc = lalloc(&L->d_windows, (int64_t)L->windows.size());
+  if (rc) return rc;
+  LHIP(hipMemcpy(L->d_windows, L->windows.data(), L->windows.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  L->dev_windows = (int64_t)L->windows.size();
+  return SFEM_OK;
+}
+
+int sfem_learner_train(sfem_learner* L, int iters, int batch, double lr, double* loss) {
+  if (!L || iters < 0 || batch < 1 || !(lr > 0.0)) return set_error(SFEM_ERR_ARG, "learner: bad training arguments");
+  if (L->windows.empty()) return set_error(SFEM_ERR_STATE, "learner: no training windows recorded");
+  LHIP(hipSetDevice(L->device));
+  int rc;
+  if ((rc = upload_windows(L))) return rc;
+  float* dl = nullptr;
+  if ((rc = lalloc(&dl, std::max(iters, 1)))) return rc;
+  for (int it = 0; it < iters && !rc; ++it) rc = grad_step(L, batch, nullptr, nullptr, (float)lr, dl + it);
+  std::vector<float> h(std::max(iters, 1));
+  hipError_t e = hipStreamSynchronize(L->stream);
+  if (!rc && e == hipSuccess && loss) e = hipMemcpy(h.data(), dl, iters * sizeof(float), hipMemcpyDeviceToHost);
+  (void)hipFree(dl);
+  if (rc) return rc;
+  if (e != hipSuccess) return set_error(SFEM_ERR_HIP, "learner: training");
+  if (loss)
+    for (int it = 0; it < iters; ++it) loss[it] = (double)h[it] / (3.0 * batch);
+  return SFEM_OK;
+}
+
+int sfem_learner_gradient(sfem_learner* L, int n, const int32_t* window, const int32_t* node, float* grad,
+                          double* loss) {
+  if (!L || n < 1 || !window || !node || !grad) return set_error(SFEM_ERR_ARG, "null argument");
+  for (int k = 0; k < n; ++k) {
+    if (window[k] < 2 || window[k] + 1 >= L->count || node[k] < 0 || node[k] >= L->nNode)
+      return set_error(SFEM_ERR_ARG, "learner: sample out of range");
+  }
+  LHIP(hipSetDevice(L->device));
+  int rc;
+  if (L->ex_cap < n) {
+    if (L->d_exw) (void)hipFree(L->d_exw);
+    if (L->d_exn) (void)hipFree(L->d_exn);
+    L->d_exw = L->d_exn = nullptr;
+    L->ex_cap = 0;
+    if ((rc = lalloc(&L->d_exw, n)) || (rc = lalloc(&L->d_exn, n))) return rc;
+    L->ex_cap = n;
+  }
+  LHIP(hipMemcpy(L->d_exw, window, n * sizeof(int32_t), hipMemcpyHostToDevice));
+  LHIP(hipMemcpy(L->d_exn, node, n * sizeof(int32_t), hipMemcpyHostToDevice));
+  if ((rc = grad_step(L, n, L->d_exw, L->d_exn, 0.f, L->d_loss))) return rc;
+  LHIP(hipStreamSynchronize(L->stream));
+  LHIP(hipMemcpy(grad, L->d_grad, kNP * sizeof(float), hipMemcpyDeviceToHost));
+  float l = 0.f;
+  LHIP(hipMemcpy(&l, L->d_loss, sizeof(float), hipMemcpyDeviceToHost));
+  if (loss) *loss = (double)l / (3.0 * n);
+  return SFEM_OK;
+}
+
+int sfem_learner_weights(sfem_learner* L, float* get, const float* set) {
+  if (!L || (!get && !set)) return set_error(SFEM_ERR_ARG, "null argument");
+  LHIP(hipSetDevice(L->device));
+  LHIP(hipStreamSynchronize(L->stream));
+  if (get) LHIP(hipMemcpy(get, L->d_w, kNP * sizeof(float), hipMemcpyDeviceToHost));
+  if (set) {
+    LHIP(hipMemcpy(L->d_w, set, kNP * sizeof(float), hipMemcpyHostToDevice));
+    LHIP(hipMemset(L->d_m, 0, kNP * sizeof(float)));
+    LHIP(hipMemset(L->d_v, 0, kNP * sizeof(float)));
+    L->adam_t = 0;
+    L->pack_stale = true;
+  }
+  return SFEM_OK;
+}
+
+int sfem_set_learner(sfem_ctx* c, sfem_learner* L) {
+  if (!c) return set_error(SFEM_ERR_ARG, "null context");
+  if (L && (L->nNode != c->nNode || c->dims.ndof != 3 || L->device != c->device))
+    return set_error(SFEM_ERR_ARG, "learner: mesh or device does not match");
+  if (L)
+    for (int32_t m : c->dbc_mult)
+      if (m != 1) return set_error(SFEM_ERR_UNSUPPORTED, "learner: a Dirichlet DOF is listed more than once");
+  if (L && !c->d_cand) {
+    if (hipSetDevice(c->device) != hipSuccess) return set_error(SFEM_ERR_HIP, "hipSetDevice");
+    int rc = lalloc(&c->d_cand, c->nNode * 3);
+    if (rc) return rc;
+  }
+  if (L) LHIP(hipStreamSynchronize(L->stream));  // training done before the weights are read
+  c->learner = L;
+  if (!L && c->predictor == SFEM_PRED_LEARNED) c->predictor = SFEM_PRED_EXTRAPOLATE;
+  return SFEM_OK;
+}
+
+int sfem_learned_candidate(sfem_ctx* c, double* out) {
+  if (!c || !out) return set_error(SFEM_ERR_ARG, "null argument");
+  if (!c->learner) return set_error(SFEM_ERR_STATE, "no learner set");
+  LHIP(hipSetDevice(c->device));
+  int rc;
+  if ((rc = learned_candidate(c, c->d_soln, c->d_prev, c->d_prev2, c->d_cand))) return rc;
+  LHIP(hipStreamSynchronize(c->stream));
+  LHIP(hipMemcpy(out, c->d_cand, c->nNode * 3 * sizeof(double), hipMemcpyDeviceToHost));
+  return SFEM_OK;
+}
+
+}  // extern "C"

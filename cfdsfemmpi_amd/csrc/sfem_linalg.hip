@@ -2720,8 +2720,24 @@ int la_subset_build(sfem_ctx* ctx, const uint8_t* keep, RowSubset* S, const RowS
   return SFEM_OK;
 }
 
-int la_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int* iters, double* relres) {
+int la_solve(sfem_ctx* ctx, const sfem_solver_opts* o_in, int* iters, double* relres) {
   LinAlg* L = ctx->la;
+  // PETSc's default test stops at ||r|| <= max(rtol ||b||, atol): with a zero initial
+  // guess that is the relative test at rtol' = max(rtol, atol / ||b||)
+  sfem_solver_opts oa = *o_in;
+  const sfem_solver_opts* o = &oa;
+  if (o_in->atol > 0.0) {
+    double b0;
+    int rc0 = la_norm2(ctx, ctx->d_rhs, &b0);
+    if (rc0) return rc0;
+    if (b0 <= o_in->atol) {
+      hipLaunchKernelGGL((k_fill<double>), dim3(grid_for(L->n)), dim3(kTPB), 0, ctx->stream, L->n, ctx->d_du, 0.0);
+      *iters = 0;
+      *relres = 0.0;
+      return hipGetLastError() == hipSuccess ? SFEM_OK : set_error(SFEM_ERR_HIP, "fill launch");
+    }
+    oa.rtol = std::max(o_in->rtol, o_in->atol / b0);
+  }
   const int64_t n = L->n;
   const bool fp32 = ctx->precision == SFEM_PREC_MIXED;
   hipStream_t st = ctx->stream;

@@ -68,6 +68,14 @@ __global__ __launch_bounds__(kTPB) void k_assign_bc(int64_t n, const int32_t* __
   applied[i] = val[k] * timeFact - soln[i];
 }
 
+__global__ __launch_bounds__(kTPB) void k_set_bc(int64_t n, const int32_t* __restrict__ ind,
+                                                 const double* __restrict__ val, double timeFact,
+                                                 double* __restrict__ x) {
+  const int64_t k = (int64_t)blockIdx.x * kTPB + threadIdx.x;
+  if (k >= n) return;
+  x[ind[k]] = val[k] * timeFact;
+}
+
 __global__ __launch_bounds__(kTPB) void k_add_bc(int64_t n, const int32_t* __restrict__ ind,
                                                  const int32_t* __restrict__ mult,
                                                  const double* __restrict__ applied,
@@ -120,6 +128,13 @@ hipError_t launch_assign_bc(hipStream_t s, int64_t n, const int32_t* ind, const 
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_assign_bc, dim3((unsigned)((n + kTPB - 1) / kTPB)), dim3(kTPB), 0, s, n, ind, val,
                      timeFact, soln, applied);
+  return hipGetLastError();
+}
+
+hipError_t launch_set_bc(hipStream_t s, int64_t n, const int32_t* ind, const double* val, double timeFact,
+                         double* x) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_set_bc, dim3((unsigned)((n + kTPB - 1) / kTPB)), dim3(kTPB), 0, s, n, ind, val, timeFact, x);
   return hipGetLastError();
 }
 

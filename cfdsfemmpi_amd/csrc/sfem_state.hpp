@@ -20,6 +20,9 @@ hipError_t launch_iter_step(hipStream_t s, int64_t n, const TimeCoeffs& c, const
                             double* dotCur);
 hipError_t launch_assign_bc(hipStream_t s, int64_t n, const int32_t* ind, const double* val,
                             double timeFact, const double* soln, double* applied);
+// x[ind[k]] = val[k] * timeFact (the prescribed values of a step written into a state)
+hipError_t launch_set_bc(hipStream_t s, int64_t n, const int32_t* ind, const double* val, double timeFact,
+                         double* x);
 hipError_t launch_add_bc(hipStream_t s, int64_t n, const int32_t* ind, const int32_t* mult,
                          const double* applied, double* soln);
 hipError_t launch_update_solution(hipStream_t s, int64_t n, const int32_t* assy, const double* du,

@@ -26,7 +26,8 @@ ERRORS = {-1: "SFEM_ERR_ARG", -2: "SFEM_ERR_STATE", -3: "SFEM_ERR_NEGJAC", -4: "
           -5: "SFEM_ERR_DIVERGED", -6: "SFEM_ERR_MAXIT", -7: "SFEM_ERR_NOMEM", -8: "SFEM_ERR_NOTCONV",
           -9: "SFEM_ERR_UNSUPPORTED"}
 PREC_FP64, PREC_MIXED, PREC_STORE32 = 0, 1, 2
-PRED_PREVIOUS, PRED_EXTRAPOLATE = 0, 1
+PRED_PREVIOUS, PRED_EXTRAPOLATE, PRED_LEARNED = 0, 1, 2
+LEARNER_NPARAM = 5187
 KT_ASSEMBLY, KT_SPMV, KT_PC = 0, 1, 2
 KSP = {"bcgs": 0, "gmres": 1}
 PC = {"none": 0, "jacobi": 1, "bjacobi4": 2, "ilu": 3}
@@ -40,7 +41,8 @@ _vp = C.c_void_p
 
 class SolverOpts(C.Structure):
     _fields_ = [("ksp_type", C.c_int), ("pc_type", C.c_int), ("rtol", C.c_double),
-                ("max_it", C.c_int), ("restart", C.c_int), ("reorder", C.c_int), ("factor_bf16", C.c_int)]
+                ("max_it", C.c_int), ("restart", C.c_int), ("reorder", C.c_int), ("factor_bf16", C.c_int),
+                ("atol", C.c_double)]
 
 
 class StepStats(C.Structure):
@@ -90,6 +92,16 @@ ABI = [
     ("sfem_tuning_levels", C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]),
     ("sfem_info", C.c_int, [_vp, _i64p, _i64p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("sfem_last_timings", C.c_int, [_vp, _f64p, _f64p]),
+    ("sfem_learner_create", _vp, [_vp, C.c_int64, C.c_uint64]),
+    ("sfem_learner_destroy", None, [_vp]),
+    ("sfem_learner_record", C.c_int, [_vp, _vp, C.c_int]),
+    ("sfem_learner_info", C.c_int, [_vp, _i64p, _i64p, _i64p]),
+    ("sfem_learner_train", C.c_int, [_vp, C.c_int, C.c_int, C.c_double, _f64p]),
+    ("sfem_learner_gradient", C.c_int, [_vp, C.c_int, _i32p, _i32p, C.POINTER(C.c_float), _f64p]),
+    ("sfem_learner_weights", C.c_int, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    ("sfem_set_learner", C.c_int, [_vp, _vp]),
+    ("sfem_learned_candidate", C.c_int, [_vp, _f64p]),
+    ("sfem_predictor_stats", C.c_int, [_vp, _i64p, _i64p, _f64p]),
 ]
 
 _LIB = None
@@ -195,6 +207,24 @@ class Context:
 
     def set_predictor(self, kind: int):
         _check(self.lib.sfem_set_predictor(self.h, int(kind)), "sfem_set_predictor")
+
+    def set_learner(self, learner: Optional["Learner"]):
+        _check(self.lib.sfem_set_learner(self.h, learner.h if learner is not None else None), "sfem_set_learner")
+        self._learner = learner  # keeps the learner alive while attached
+
+    def learned_candidate(self) -> np.ndarray:
+        out = np.zeros(self.mesh.nNode * self.ndof)
+        _check(self.lib.sfem_learned_candidate(self.h, _p(out, C.c_double)), "sfem_learned_candidate")
+        return out
+
+    def predictor_stats(self):
+        """(steps started from the learned start, from the extrapolated start,
+        [||F|| extrapolated, ||F|| learned] of the last step)."""
+        a, b = C.c_int64(0), C.c_int64(0)
+        nrm = np.zeros(2)
+        _check(self.lib.sfem_predictor_stats(self.h, C.byref(a), C.byref(b), _p(nrm, C.c_double)),
+               "sfem_predictor_stats")
+        return a.value, b.value, nrm
 
     # -- hot path ----------------------------------------------------------
     def time_update(self, timeFact: float):
@@ -372,13 +402,70 @@ class Context:
         return a.value, s.value
 
 
+class Learner:
+    """Owner of one ``sfem_learner``: the training store and the MLP of the learned
+    initial iterate for the mesh of ``ctx`` (2D meshes)."""
+
+    def __init__(self, ctx: "Context", capacity: int, seed: int = 0):
+        self.lib = load_library()
+        h = self.lib.sfem_learner_create(ctx.h, int(capacity), int(seed) & (2 ** 64 - 1))
+        if not h:
+            raise RuntimeError("sfem_learner_create failed: " + self.lib.sfem_last_error().decode())
+        self.h = C.c_void_p(h)
+
+    def record(self, ctx: "Context", new_trajectory: bool = False):
+        _check(self.lib.sfem_learner_record(self.h, ctx.h, 1 if new_trajectory else 0), "sfem_learner_record")
+
+    def info(self):
+        a, b, c = C.c_int64(0), C.c_int64(0), C.c_int64(0)
+        _check(self.lib.sfem_learner_info(self.h, C.byref(a), C.byref(b), C.byref(c)), "sfem_learner_info")
+        return a.value, b.value, c.value
+
+    def train(self, iters: int, batch: int, lr: float) -> np.ndarray:
+        loss = np.zeros(max(int(iters), 1))
+        _check(self.lib.sfem_learner_train(self.h, int(iters), int(batch), float(lr), _p(loss, C.c_double)),
+               "sfem_learner_train")
+        return loss[:int(iters)]
+
+    def gradient(self, window: np.ndarray, node: np.ndarray):
+        w = np.ascontiguousarray(window, np.int32)
+        n = np.ascontiguousarray(node, np.int32)
+        g = np.zeros(LEARNER_NPARAM, np.float32)
+        loss = C.c_double(0.0)
+        _check(self.lib.sfem_learner_gradient(self.h, len(w), _p(w, C.c_int32), _p(n, C.c_int32),
+                                              _p(g, C.c_float), C.byref(loss)), "sfem_learner_gradient")
+        return g, loss.value
+
+    def get_weights(self) -> np.ndarray:
+        w = np.zeros(LEARNER_NPARAM, np.float32)
+        _check(self.lib.sfem_learner_weights(self.h, _p(w, C.c_float), None), "sfem_learner_weights")
+        return w
+
+    def set_weights(self, w: np.ndarray):
+        w = np.ascontiguousarray(w, np.float32)
+        assert w.size == LEARNER_NPARAM
+        _check(self.lib.sfem_learner_weights(self.h, None, _p(w, C.c_float)), "sfem_learner_weights")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.sfem_learner_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def solver_opts(ksp="bcgs", pc="ilu", rtol=1e-8, max_it=1000, restart=30, reorder=0,
-                factor_bf16=0) -> SolverOpts:
+                factor_bf16=0, atol=0.0) -> SolverOpts:
     if pc == "lu":
         # the reference's test decks ask for MUMPS LU; a tight RCM-ordered ILU(0)-BiCGStab
         # solve reproduces the direct answer to well below the decks' tolerances
         ksp, pc, rtol, max_it, reorder = "bcgs", "ilu", 1e-13, max(max_it, 5000), 1
-    return SolverOpts(KSP[ksp], PC[pc], float(rtol), int(max_it), int(restart), int(reorder), int(factor_bf16))
+    return SolverOpts(KSP[ksp], PC[pc], float(rtol), int(max_it), int(restart), int(reorder), int(factor_bf16),
+                      float(atol))
 
 
 class StabFEM:

@@ -1,0 +1,277 @@
+"""Learned initial iterate on the GPU: MFMA inference against a float64 restatement
+of the same BF16 network, training gradients against torch autograd (float64, CPU),
+the Adam step, and the residual judge (the start never changes the converged answer).
+
+The MLP has no counterpart in the reference (its Newton loop starts from the
+previous solution, StabFEM2.cpp:593-832); the checks here pin the kernels to the
+mathematics of the network as documented in include/sfem.h."""
+import numpy as np
+import pytest
+import torch
+
+from cfdsfemmpi_amd import deck, meshgen, sfem
+
+pytestmark = pytest.mark.gpu
+
+NP = sfem.LEARNER_NPARAM
+SL = {"W1": (0, (64, 12)), "b1": (768, (64,)), "W2": (832, (64, 64)), "b2": (4928, (64,)),
+      "W3": (4992, (3, 64)), "b3": (5184, (3,))}
+
+
+def _tiny_cylinder(perturb=0.01, seed=1):
+    return meshgen.cylinder_channel(nr=8, nth=32, n_left=4, n_right=12, n_bottom=4, n_top=4,
+                                    perturb=perturb, seed=seed)
+
+
+def _ctx(mesh, mu=0.01, reorder=1):
+    ctl = deck.Control(np.array([1.0, mu, 0, 0, 0, 0, 0, 0, 1.0, 1.0, 0.0]), 2, 0.0, 0.1, 1e9, 100, 1, 1e-8)
+    fem = sfem.StabFEM(device=0)
+    fem.setMesh(mesh)
+    fem.setControl(ctl)
+    fem.prepareInputData()
+    opts = sfem.solver_opts("bcgs", "ilu", 1e-10, 20000, reorder=reorder)
+    fem.setSolver(opts)
+    return fem, fem.ctx, opts
+
+
+def _split(w):
+    return {k: np.asarray(w[o:o + int(np.prod(s))], np.float64).reshape(s) for k, (o, s) in SL.items()}
+
+
+def _bf16(x):
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def _neighbours(mesh):
+    nb = [set() for _ in range(mesh.nNode)]
+    for e in mesh.conn:
+        for a in e:
+            nb[a].update(int(b) for b in e if b != a)
+    return [np.array(sorted(s), np.int64) for s in nb]
+
+
+def _features(S0, S1, S2, nbrs, isig):
+    """The 12 inputs per node (float64 differences, scaled, rounded to float32)."""
+    s0, s1, s2 = (np.asarray(S, np.float64).reshape(-1, 3) for S in (S0, S1, S2))
+    d1, d2 = s0 - s1, s1 - s2
+    m1 = np.array([d1[j].mean(axis=0) for j in nbrs])
+    m2 = np.array([d2[j].mean(axis=0) for j in nbrs])
+    x = np.concatenate([d1 * isig, d2 * isig, (m1 - d1) * isig, (m2 - d2) * isig], axis=1)
+    return x.astype(np.float32).astype(np.float64), d1
+
+
+def _silu(z):
+    return z / (1.0 + np.exp(-z))
+
+
+def _random_weights(seed, scale=0.4):
+    rng = np.random.default_rng(seed)
+    return (scale * rng.standard_normal(NP)).astype(np.float32)
+
+
+def _set_history(ctx, S0, S1, S2):
+    """soln = S0, solnPrev = S1, solnPrev2 = S2 through the time-update shift."""
+    ctx.reset_state()
+    ctx.set_solution(S2)
+    ctx.time_update(0.0)
+    ctx.set_solution(S1)
+    ctx.time_update(0.0)
+    ctx.set_solution(S0)
+
+
+def _smooth_states(mesh, seed):
+    rng = np.random.default_rng(seed)
+    xy = mesh.coords
+    out = []
+    base = np.stack([np.sin(0.3 * xy[:, 0]) * np.cos(0.5 * xy[:, 1]), np.cos(0.2 * xy[:, 0] + xy[:, 1]),
+                     0.1 * xy[:, 0]], axis=1)
+    vel = rng.standard_normal((1, 3)) * 0.05 + 0.02 * np.stack([np.cos(xy[:, 1]), np.sin(xy[:, 0]), xy[:, 1] / 10], 1)
+    for k in range(3):
+        out.append((base - k * vel - 0.3 * k * k * vel ** 2).ravel())
+    return out  # S0, S1, S2
+
+
+def test_zero_network_gives_the_extrapolated_start():
+    mesh = _tiny_cylinder()
+    fem, ctx, _ = _ctx(mesh)
+    L = sfem.Learner(ctx, 8, seed=3)
+    L.set_weights(np.zeros(NP, np.float32))
+    ctx.set_learner(L)
+    S0, S1, S2 = _smooth_states(mesh, 0)
+    _set_history(ctx, S0, S1, S2)
+    cand = ctx.learned_candidate()
+    assert np.allclose(cand, 2 * S0 - S1, rtol=0, atol=1e-14 * np.max(np.abs(S0)))
+
+
+def test_mfma_inference_matches_bf16_network():
+    """The MFMA forward pass equals the network with BF16-rounded weights, inputs and
+    hidden activations (FP32 biases and accumulation) to accumulation-order accuracy;
+    a transposed or misplaced operand tile would show O(1) differences."""
+    mesh = _tiny_cylinder()
+    fem, ctx, _ = _ctx(mesh)
+    L = sfem.Learner(ctx, 8, seed=5)
+    w = _random_weights(11)
+    L.set_weights(w)
+    ctx.set_learner(L)
+    S0, S1, S2 = _smooth_states(mesh, 1)
+    _set_history(ctx, S0, S1, S2)
+    cand = ctx.learned_candidate().reshape(-1, 3)
+    d1 = (S0 - S1).reshape(-1, 3)
+    sig = np.sqrt(np.mean(d1 ** 2, axis=0)).astype(np.float32).astype(np.float64)
+    isig = (np.float32(1.0) / sig.astype(np.float32)).astype(np.float64)
+    x, d1 = _features(S0, S1, S2, _neighbours(mesh), isig)
+    P = _split(w)
+    h1 = _bf16(_silu(_bf16(x) @ _bf16(P["W1"]).T + P["b1"]))
+    h2 = _bf16(_silu(h1 @ _bf16(P["W2"]).T + P["b2"]))
+    y = h2 @ _bf16(P["W3"]).T + P["b3"]
+    y_gpu = (cand - S0.reshape(-1, 3) - d1) / sig
+    err = np.max(np.abs(y_gpu - y))
+    assert np.max(np.abs(y)) > 0.5
+    assert err <= 2e-2 * np.max(np.abs(y)), err
+    # every output row checked, not only a few: the rows of one wavefront share nothing
+    assert np.mean(np.abs(y_gpu - y)) <= 2e-3 * np.max(np.abs(y))
+
+
+def _record_random_trajectory(ctx, L, mesh, nsnap, seed):
+    rng = np.random.default_rng(seed)
+    xy = mesh.coords
+    snaps = []
+    for t in range(nsnap):
+        s = np.stack([np.sin(0.2 * xy[:, 0] + 0.3 * t), np.cos(0.3 * xy[:, 1] - 0.2 * t), 0.05 * t * xy[:, 0]], 1)
+        s = s + 0.01 * rng.standard_normal(s.shape)
+        ctx.set_solution(s.ravel())
+        L.record(ctx, new_trajectory=(t == 0))
+        snaps.append(s.ravel().astype(np.float32))
+    return snaps
+
+
+def _torch_loss_grad(w, snaps, windows, nodes, nbrs, mask):
+    """Mean squared error of the FP32 training network, float64 autograd on the CPU."""
+    P = {k: torch.tensor(v, dtype=torch.float64, requires_grad=True) for k, v in _split(w).items()}
+    X, T, M = [], [], []
+    for t, i in zip(windows, nodes):
+        s0, s1, s2, s3 = (snaps[t + d].astype(np.float64).reshape(-1, 3) for d in (0, -1, -2, 1))
+        dd = s0 - s1
+        rms = np.sqrt(np.mean(dd ** 2, axis=0)).astype(np.float32)
+        isig = (np.float32(1.0) / rms).astype(np.float64)
+        d1 = dd[i]
+        d2 = s1[i] - s2[i]
+        m1 = dd[nbrs[i]].mean(axis=0)
+        m2 = (s1 - s2)[nbrs[i]].mean(axis=0)
+        x = np.concatenate([d1 * isig, d2 * isig, (m1 - d1) * isig, (m2 - d2) * isig]).astype(np.float32)
+        X.append(x)
+        T.append(((s3[i] - s0[i] - d1) * isig).astype(np.float32))
+        M.append(mask[3 * i:3 * i + 3])
+    X = torch.tensor(np.array(X, np.float64))
+    T = torch.tensor(np.array(T, np.float64))
+    M = torch.tensor(np.array(M, np.float64))
+    h1 = torch.nn.functional.silu(X @ P["W1"].T + P["b1"])
+    h2 = torch.nn.functional.silu(h1 @ P["W2"].T + P["b2"])
+    y = h2 @ P["W3"].T + P["b3"]
+    loss = (((y - T) * M) ** 2).sum() / (3 * len(windows))
+    loss.backward()
+    g = np.zeros(NP)
+    for k, (o, s) in SL.items():
+        g[o:o + int(np.prod(s))] = P[k].grad.numpy().ravel()
+    return float(loss), g
+
+
+def _free_mask(mesh):
+    m = np.ones(mesh.nNode * 3)
+    m[np.asarray(mesh.dbc_node, np.int64) * 3 + np.asarray(mesh.dbc_dof, np.int64)] = 0.0
+    return m
+
+
+def test_training_gradient_matches_autograd():
+    mesh = _tiny_cylinder()
+    fem, ctx, _ = _ctx(mesh)
+    L = sfem.Learner(ctx, 16, seed=7)
+    snaps = _record_random_trajectory(ctx, L, mesh, 7, seed=2)
+    assert L.info()[:2] == (7, 4)
+    w = _random_weights(13, 0.3)
+    L.set_weights(w)
+    rng = np.random.default_rng(4)
+    n = 150  # three workgroups, the last one partly filled
+    windows = rng.integers(2, 6, size=n)
+    nodes = rng.integers(0, mesh.nNode, size=n)
+    nodes[:5] = np.asarray(mesh.dbc_node[:5])  # prescribed components are masked out
+    g, loss = L.gradient(windows, nodes)
+    lref, gref = _torch_loss_grad(w, snaps, windows, nodes, _neighbours(mesh), _free_mask(mesh))
+    assert loss == pytest.approx(lref, rel=1e-4)
+    assert np.max(np.abs(g - gref)) <= 1e-4 * np.max(np.abs(gref))
+    for k, (o, s) in SL.items():  # each block's gradient separately, so no block can hide
+        blk = slice(o, o + int(np.prod(s)))
+        assert np.linalg.norm(g[blk] - gref[blk]) <= 1e-4 * np.linalg.norm(gref[blk]) + 1e-12, k
+
+
+def _mix64(z):
+    M = (1 << 64) - 1
+    z = (z + 0x9E3779B97F4A7C15) & M
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+    return z ^ (z >> 31)
+
+
+def test_adam_step_on_the_hashed_minibatch():
+    """One training step samples its minibatch with the documented splitmix64 rule and
+    moves each weight by lr * g / (|g| + eps) (the first bias-corrected Adam step)."""
+    mesh = _tiny_cylinder()
+    fem, ctx, _ = _ctx(mesh)
+    seed = 12345
+    L = sfem.Learner(ctx, 16, seed=seed)
+    _record_random_trajectory(ctx, L, mesh, 6, seed=3)
+    w0 = _random_weights(17, 0.3)
+    L.set_weights(w0)
+    batch, lr = 200, 1e-3
+    wins = np.array([2, 3, 4])  # windows of a 6-state trajectory
+    sel = [_mix64(seed ^ _mix64(0 * (1 << 32) + k)) for k in range(batch)]
+    windows = np.array([wins[(h >> 32) % len(wins)] for h in sel])
+    nodes = np.array([(h & 0xFFFFFFFF) % mesh.nNode for h in sel])
+    g, lref = L.gradient(windows, nodes)
+    loss = L.train(1, batch, lr)
+    w1 = L.get_weights()
+    assert loss[0] == pytest.approx(lref, rel=1e-5)
+    step = w0.astype(np.float64) - w1
+    expect = lr * g / (np.abs(g) + 1e-8)
+    assert np.max(np.abs(step - expect)) <= 2e-6, np.max(np.abs(step - expect))
+    assert L.info()[2] == 1
+
+
+def test_learned_start_never_changes_the_answer():
+    """Train briefly on a short trajectory, then rerun it with the learned predictor:
+    each step starts from the smaller-residual candidate, and every converged state
+    agrees with the extrapolated run's to the Newton tolerance's effect."""
+    mesh = _tiny_cylinder()
+    nsteps, tol = 14, 1e-10
+    fem, ctx, opts = _ctx(mesh)
+    ctx.set_predictor(sfem.PRED_EXTRAPOLATE)
+    L = sfem.Learner(ctx, nsteps + 1, seed=21)
+    L.record(ctx, new_trajectory=True)
+    ref, its_ref = [], []
+    t = 0.1
+    for s in range(nsteps):
+        st = ctx.step(min(t, 1.0), opts, tol, 10, allow_notconv=False)
+        its_ref.append(st.krylov_iters)
+        L.record(ctx)
+        ref.append(ctx.get_solution())
+        t += 0.1
+    loss = L.train(300, 1024, 2e-3)
+    assert np.all(np.isfinite(loss)) and loss[-20:].mean() < loss[:20].mean()
+    fem2, ctx2, _ = _ctx(mesh)
+    ctx2.set_learner(L)
+    ctx2.set_predictor(sfem.PRED_LEARNED)
+    t = 0.1
+    picked = []
+    for s in range(nsteps):
+        st = ctx2.step(min(t, 1.0), opts, tol, 10, allow_notconv=False)
+        nl, nf, nrm = ctx2.predictor_stats()
+        picked.append(nrm)
+        assert st.rhs_norm0 == pytest.approx(min(nrm), rel=1e-6)
+        sol = ctx2.get_solution()
+        scale = np.max(np.abs(ref[s]))
+        assert np.max(np.abs(sol - ref[s])) <= 1e-5 * scale, (s, np.max(np.abs(sol - ref[s])) / scale)
+        t += 0.1
+    nl, nf, _ = ctx2.predictor_stats()
+    assert nl + nf == nsteps
