@@ -416,7 +416,7 @@ def main_c5(a):
     then a run at Re = 100 with an unseen perturbation is timed from the extrapolated
     start and from the learned start (judged per step by the iteration-0 residual)."""
     from cfdsfemmpi_amd import meshgen, sfem
-    steps = a.steps if a.steps is not None else 60
+    steps = a.steps if a.steps is not None else 100
     res = [float(r) for r in a.c5_train_re.split(",")]
     opts = sfem.solver_opts("bcgs", "ilu", 1e-12, 20000, reorder=a.reorder, atol=0.5 * a.newton_tol)
     meshes = [meshgen.cylinder_channel(perturb=0.01, seed=1000 + k) for k in range(len(res))]
@@ -436,9 +436,11 @@ def main_c5(a):
         fem.ctx.close()
     t_gen = time.perf_counter() - t_gen
     t_fit = time.perf_counter()
-    loss = learner.train(a.c5_train_iters, a.c5_batch, a.c5_lr)
+    # four equal phases, the learning rate halved after each (Adam's moments carry over)
+    chunk = max(1, a.c5_train_iters // 4)
+    loss = np.concatenate([learner.train(chunk, a.c5_batch, a.c5_lr * 0.5 ** k) for k in range(4)])
     t_fit = time.perf_counter() - t_fit
-    log(f"training: {a.c5_train_iters} Adam steps x {a.c5_batch} samples in {t_fit:.2f} s, loss "
+    log(f"training: {len(loss)} Adam steps x {a.c5_batch} samples in {t_fit:.2f} s, loss "
         f"{loss[:50].mean():.4e} -> {loss[-50:].mean():.4e}")
     evalmesh = meshgen.cylinder_channel(perturb=0.01, seed=99)
     runs = {}
@@ -475,7 +477,8 @@ def main_c5(a):
                                f"slip walls, traction-free outflow, Re=100, dt=0.1, {steps} steps from rest",
                    "training": {"trajectories": len(res), "re": res, "steps_each": a.c5_train_steps,
                                 "recorded_from_step": a.c5_record_from, "windows": learner.info()[1],
-                                "adam_steps": a.c5_train_iters, "batch": a.c5_batch, "lr": a.c5_lr,
+                                "adam_steps": len(loss), "batch": a.c5_batch,
+                                "lr": [a.c5_lr * 0.5 ** k for k in range(4)],
                                 "seconds_generating": t_gen, "seconds_fitting": t_fit,
                                 "loss_first50": float(loss[:50].mean()), "loss_last50": float(loss[-50:].mean())},
                    "solver": f"FP64 BiCGStab + ILU(0) (ordering {a.reorder}), stop at ||r|| <= max(1e-12 ||b||, "
@@ -518,11 +521,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c3", choices=["c3", "c2", "c4", "c5"],
                     help="c3: 115^3 hex cavity (the headline); c2: 32^3; c4: batch of 256 Tria3 cavities")
-    ap.add_argument("--c5-train-re", default="60,100,140,200", help="c5: Reynolds numbers of the training runs")
+    ap.add_argument("--c5-train-re", default="60,80,110,140,170,200",
+                    help="c5: Reynolds numbers of the training runs")
     ap.add_argument("--c5-train-steps", type=int, default=80, help="c5: steps of each training run")
     ap.add_argument("--c5-record-from", type=int, default=13,
                     help="c5: first recorded step of a training run (the inflow ramp ends at step 10)")
-    ap.add_argument("--c5-train-iters", type=int, default=2000, help="c5: Adam steps")
+    ap.add_argument("--c5-train-iters", type=int, default=8000, help="c5: Adam steps (four phases, lr halved)")
     ap.add_argument("--c5-batch", type=int, default=8192, help="c5: samples per Adam step")
     ap.add_argument("--c5-lr", type=float, default=1e-3)
     ap.add_argument("--c5-seed", type=int, default=0)
