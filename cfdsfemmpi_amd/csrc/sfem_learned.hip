@@ -440,7 +440,8 @@ struct sfem_learner {
   bool pack_stale = true;
   float *d_part = nullptr, *d_ploss = nullptr, *d_grad = nullptr, *d_loss = nullptr;
   int part_blocks = 0;
-  double* d_acc = nullptr;
+  double* d_acc = nullptr;      // recording (learner stream)
+  double* d_acc_inf = nullptr;  // inference (the attached context's stream)
   int32_t *d_exw = nullptr, *d_exn = nullptr;
   int ex_cap = 0;
   uint64_t seed = 0;
@@ -517,12 +518,12 @@ int learned_candidate(sfem_ctx* c, const double* s0, const double* s1, const dou
   if (!L) return set_error(SFEM_ERR_STATE, "no learner set");
   int rc;
   if ((rc = repack(L, c->stream))) return rc;
-  hipLaunchKernelGGL(k_zero3, dim3(1), dim3(64), 0, c->stream, L->d_acc);
+  hipLaunchKernelGGL(k_zero3, dim3(1), dim3(64), 0, c->stream, L->d_acc_inf);
   const int64_t nn = L->nNode;
   const unsigned gr = (unsigned)std::min<int64_t>((nn + 255) / 256, 1024);
-  hipLaunchKernelGGL((k_sq_increment<double>), dim3(gr), dim3(256), 0, c->stream, nn, s0, s1, L->d_acc);
+  hipLaunchKernelGGL((k_sq_increment<double>), dim3(gr), dim3(256), 0, c->stream, nn, s0, s1, L->d_acc_inf);
   hipLaunchKernelGGL((k_mlp_infer<double>), dim3((unsigned)((nn + 63) / 64)), dim3(256), 0, c->stream, nn, s0, s1,
-                     s2, L->d_adjptr, L->d_adj, L->d_acc, L->d_w, (const v4s*)L->d_pack, out);
+                     s2, L->d_adjptr, L->d_adj, L->d_acc_inf, L->d_w, (const v4s*)L->d_pack, out);
   LHIP(hipGetLastError());
   return SFEM_OK;
 }
@@ -589,6 +590,7 @@ sfem_learner* sfem_learner_create(sfem_ctx* c, int64_t capacity, uint64_t seed) 
   if (!rc) rc = lalloc(&L->d_grad, kNP);
   if (!rc) rc = lalloc(&L->d_loss, 1);
   if (!rc) rc = lalloc(&L->d_acc, 3);
+  if (!rc) rc = lalloc(&L->d_acc_inf, 3);
   if (!rc && hipStreamCreate(&L->stream) != hipSuccess) rc = set_error(SFEM_ERR_HIP, "learner: stream");
   if (rc) {
     sfem_learner_destroy(L);
@@ -602,7 +604,8 @@ void sfem_learner_destroy(sfem_learner* L) {
   (void)hipSetDevice(L->device);
   if (L->stream) (void)hipStreamSynchronize(L->stream);
   void* ps[] = {L->d_adjptr, L->d_adj, L->d_mask, L->d_snap, L->d_rms, L->d_windows, L->d_w, L->d_m, L->d_v,
-                L->d_pack, L->d_part, L->d_ploss, L->d_grad, L->d_loss, L->d_acc, L->d_exw, L->d_exn};
+                L->d_pack, L->d_part, L->d_ploss, L->d_grad, L->d_loss, L->d_acc, L->d_acc_inf, L->d_exw,
+                L->d_exn};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   if (L->stream) (void)hipStreamDestroy(L->stream);
@@ -723,6 +726,7 @@ int sfem_set_learner(sfem_ctx* c, sfem_learner* L) {
   if (!c) return set_error(SFEM_ERR_ARG, "null context");
   if (L && (L->nNode != c->nNode || c->dims.ndof != 3 || L->device != c->device))
     return set_error(SFEM_ERR_ARG, "learner: mesh or device does not match");
+  if (L && !c->la) return set_error(SFEM_ERR_STATE, "learner: build the pattern first");
   if (L)
     for (int32_t m : c->dbc_mult)
       if (m != 1) return set_error(SFEM_ERR_UNSUPPORTED, "learner: a Dirichlet DOF is listed more than once");

@@ -162,7 +162,9 @@ int sfem_learner_gradient(sfem_learner* learner, int n, const int32_t* window, c
                           float* grad, double* loss);
 /* Read (get) and/or replace (set) the FP32 master weights; set also resets Adam. */
 int sfem_learner_weights(sfem_learner* learner, float* get, const float* set);
-/* Attach a learner (NULL detaches; the learner must outlive its use by ctx). */
+/* Attach a learner (NULL detaches; the learner must outlive its use by ctx).  A learner
+ * is not thread-safe: record, train and the steps of contexts it is attached to are
+ * called from one host thread (each call returns with its device work finished). */
 int sfem_set_learner(sfem_ctx* ctx, sfem_learner* learner);
 /* The learned start computed from ctx's current solution and its two predecessors
  * (nNode*3 doubles, host). */
