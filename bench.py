@@ -399,7 +399,7 @@ def c5_run(ctx, opts, steps, newton_tol, record=None, record_from=0, label="", j
         krylov.append(st.krylov_iters)
         r0.append(st.rhs_norm0)
         if judged:
-            cand.append([float(v) for v in ctx.predictor_stats()[2]])
+            cand.append([float(v) for v in ctx.predictor_stats()[1]])
         if record is not None and k + 1 >= record_from:
             record.record(ctx, new_trajectory=(k + 1 == record_from))
         t += dt
@@ -444,19 +444,21 @@ def main_c5(a):
         f"{loss[:50].mean():.4e} -> {loss[-50:].mean():.4e}")
     evalmesh = meshgen.cylinder_channel(perturb=0.01, seed=99)
     runs = {}
-    for kind in ("extrapolated", "learned"):
+    for kind in ("extrapolated", "quadratic", "learned"):
         fem = c5_context(evalmesh, 0.01, opts)
+        if kind == "quadratic":
+            fem.ctx.set_predictor(sfem.PRED_QUADRATIC)
         if kind == "learned":
             fem.ctx.set_learner(learner)
             fem.ctx.set_predictor(sfem.PRED_LEARNED)
         r = c5_run(fem.ctx, opts, steps, a.newton_tol, label=f"eval Re 100 {kind}", judged=(kind == "learned"))
         r["soln"] = fem.ctx.get_solution()
         if kind == "learned":
-            nl, nf, _ = fem.ctx.predictor_stats()
-            r["learned_starts"], r["fallback_starts"] = nl, nf
+            r["starts"] = [int(v) for v in fem.ctx.predictor_stats()[0]]
+            log(f"starts taken (linear, quadratic, learned): {r['starts']}")
         fem.ctx.close()
         runs[kind] = r
-    ex, le = runs["extrapolated"], runs["learned"]
+    ex, qu, le = runs["extrapolated"], runs["quadratic"], runs["learned"]
     diff = float(np.max(np.abs(le["soln"] - ex["soln"])) / np.max(np.abs(ex["soln"])))
     skip = min(a.c5_record_from, steps // 2)  # after the inflow ramp and its transient
     out = {
@@ -487,12 +489,16 @@ def main_c5(a):
         "extrapolated": {"steps_per_s": ex["steps_per_s"], "newton_total": sum(ex["newton"]),
                          "krylov_total": sum(ex["krylov"]), "krylov_after_ramp": sum(ex["krylov"][skip:]),
                          "krylov_per_step": ex["krylov"], "r0_per_step": ex["r0"]},
+        "quadratic": {"steps_per_s": qu["steps_per_s"], "newton_total": sum(qu["newton"]),
+                      "krylov_total": sum(qu["krylov"]), "krylov_after_ramp": sum(qu["krylov"][skip:]),
+                      "krylov_per_step": qu["krylov"], "r0_per_step": qu["r0"]},
         "learned": {"steps_per_s": le["steps_per_s"], "newton_total": sum(le["newton"]),
                     "krylov_total": sum(le["krylov"]), "krylov_after_ramp": sum(le["krylov"][skip:]),
                     "krylov_per_step": le["krylov"], "r0_per_step": le["r0"],
-                    "learned_starts": le["learned_starts"], "fallback_starts": le["fallback_starts"],
-                    "candidate_norms_per_step": le["candidates"]},
+                    "starts_linear_quadratic_learned": le["starts"],
+                    "candidate_norms_per_step_linear_quadratic_learned": le["candidates"]},
         "krylov_ratio_learned_vs_extrapolated": sum(le["krylov"]) / max(1, sum(ex["krylov"])),
+        "krylov_ratio_learned_vs_quadratic": sum(le["krylov"]) / max(1, sum(qu["krylov"])),
         "steps_per_s_ratio_learned_vs_extrapolated": le["steps_per_s"] / ex["steps_per_s"],
         "max_rel_diff_final_state_learned_vs_extrapolated": diff,
         "roofline": None,

@@ -502,18 +502,20 @@ int sfem_set_precision(sfem_ctx* c, int precision) {
 
 int sfem_set_predictor(sfem_ctx* c, int predictor) {
   if (!c) return set_error(SFEM_ERR_ARG, "null context");
-  if (predictor != SFEM_PRED_PREVIOUS && predictor != SFEM_PRED_EXTRAPOLATE && predictor != SFEM_PRED_LEARNED)
+  if (predictor != SFEM_PRED_PREVIOUS && predictor != SFEM_PRED_EXTRAPOLATE && predictor != SFEM_PRED_LEARNED &&
+      predictor != SFEM_PRED_QUADRATIC)
     return set_error(SFEM_ERR_ARG, "bad predictor");
   if (predictor == SFEM_PRED_LEARNED && !c->learner) return set_error(SFEM_ERR_STATE, "attach a learner first");
   c->predictor = predictor;
   return SFEM_OK;
 }
 
-int sfem_predictor_stats(sfem_ctx* c, int64_t* learned, int64_t* fallback, double* last_norms) {
+int sfem_predictor_stats(sfem_ctx* c, int64_t* counts, double* last_norms) {
   if (!c) return set_error(SFEM_ERR_ARG, "null context");
-  if (learned) *learned = c->n_learned;
-  if (fallback) *fallback = c->n_fallback;
-  if (last_norms) { last_norms[0] = c->pred_norms[0]; last_norms[1] = c->pred_norms[1]; }
+  for (int k = 0; k < 3; ++k) {
+    if (counts) counts[k] = c->pred_count[k];
+    if (last_norms) last_norms[k] = c->pred_norms[k];
+  }
   return SFEM_OK;
 }
 
@@ -545,24 +547,23 @@ static int residual_norm_at(sfem_ctx* c, double* x, double timeFact, double* nrm
   return rc;
 }
 
-// d_soln holds the extrapolated start; replace it by the learned start when that has
-// the smaller residual norm.
+// d_soln holds the linearly extrapolated start; replace it by the quadratically
+// extrapolated or the learned start, whichever has the smallest residual norm.
 static int judge_learned_start(sfem_ctx* c, double timeFact) {
   if (!c->learner || !c->d_cand) return set_error(SFEM_ERR_STATE, "no learner attached");
   const int64_t n = nstate(c);
+  double* cand[3] = {c->d_soln, c->d_cand + n, c->d_cand};  // linear, quadratic, learned
   int rc;
-  double ne = 0.0, nl = 0.0;
-  if ((rc = learned_candidate(c, c->d_prev, c->d_prev2, c->d_prev3, c->d_cand))) return rc;
-  if ((rc = residual_norm_at(c, c->d_cand, timeFact, &nl))) return rc;
-  if ((rc = residual_norm_at(c, c->d_soln, timeFact, &ne))) return rc;
-  c->pred_norms[0] = ne;
-  c->pred_norms[1] = nl;
-  if (std::isfinite(nl) && nl < ne) {
-    HIP_OK(hipMemcpyAsync(c->d_soln, c->d_cand, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    ++c->n_learned;
-  } else {
-    ++c->n_fallback;
+  HIP_OK(launch_extrapolate2(c->stream, n, c->d_prev, c->d_prev2, c->d_prev3, cand[1]));
+  if ((rc = learned_candidate(c, c->d_prev, c->d_prev2, c->d_prev3, cand[2]))) return rc;
+  int best = 0;
+  for (int k = 0; k < 3; ++k) {
+    if ((rc = residual_norm_at(c, cand[k], timeFact, &c->pred_norms[k]))) return rc;
+    if (k > 0 && std::isfinite(c->pred_norms[k]) && c->pred_norms[k] < c->pred_norms[best]) best = k;
   }
+  if (best > 0)
+    HIP_OK(hipMemcpyAsync(c->d_soln, cand[best], n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  ++c->pred_count[best];
   return SFEM_OK;
 }
 
@@ -576,6 +577,8 @@ int sfem_time_update(sfem_ctx* c, double timeFact) {
                             c->d_dotPrev));
   if (c->predictor == SFEM_PRED_EXTRAPOLATE || c->predictor == SFEM_PRED_LEARNED)
     HIP_OK(launch_extrapolate(c->stream, n, c->d_prev, c->d_prev2, c->d_soln));
+  if (c->predictor == SFEM_PRED_QUADRATIC)
+    HIP_OK(launch_extrapolate2(c->stream, n, c->d_prev, c->d_prev2, c->d_prev3, c->d_soln));
   if (c->predictor == SFEM_PRED_LEARNED) {
     int rc = judge_learned_start(c, timeFact);
     if (rc) return rc;

@@ -213,9 +213,9 @@ struct sfem_ctx {
 
   // learned initial iterate (sfem_learned.hip): candidate buffer and the judge's record
   sfem_learner* learner = nullptr;
-  double* d_cand = nullptr;          // nNode*ndof
-  int64_t n_learned = 0, n_fallback = 0;
-  double pred_norms[2] = {0.0, 0.0};  // ||F|| at the extrapolated and the learned start
+  double* d_cand = nullptr;               // 2 x nNode*ndof: learned, quadratic candidates
+  int64_t pred_count[3] = {0, 0, 0};       // steps started from the linear / quadratic / learned start
+  double pred_norms[3] = {0.0, 0.0, 0.0};  // their ||F|| at the last step
 };
 
 namespace sfem {

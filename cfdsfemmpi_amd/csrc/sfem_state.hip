@@ -40,6 +40,14 @@ __global__ __launch_bounds__(kTPB) void k_extrapolate(int64_t n, const double* _
     soln[i] = 2.0 * prev[i] - prev2[i];
 }
 
+__global__ __launch_bounds__(kTPB) void k_extrapolate2(int64_t n, const double* __restrict__ prev,
+                                                       const double* __restrict__ prev2,
+                                                       const double* __restrict__ prev3,
+                                                       double* __restrict__ soln) {
+  for (int64_t i = (int64_t)blockIdx.x * kTPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kTPB)
+    soln[i] = 3.0 * prev[i] - 3.0 * prev2[i] + prev3[i];
+}
+
 __global__ __launch_bounds__(kTPB) void k_iter_step(int64_t n, TimeCoeffs c, const double* __restrict__ soln,
                                                     const double* __restrict__ prev,
                                                     const double* __restrict__ prev2,
@@ -111,6 +119,12 @@ hipError_t launch_time_update(hipStream_t s, int64_t n, const double* soln, doub
 
 hipError_t launch_extrapolate(hipStream_t s, int64_t n, const double* prev, const double* prev2, double* soln) {
   hipLaunchKernelGGL(k_extrapolate, dim3(grid1(n)), dim3(kTPB), 0, s, n, prev, prev2, soln);
+  return hipGetLastError();
+}
+
+hipError_t launch_extrapolate2(hipStream_t s, int64_t n, const double* prev, const double* prev2,
+                              const double* prev3, double* soln) {
+  hipLaunchKernelGGL(k_extrapolate2, dim3(grid1(n)), dim3(kTPB), 0, s, n, prev, prev2, prev3, soln);
   return hipGetLastError();
 }
 

@@ -20,6 +20,9 @@ hipError_t launch_iter_step(hipStream_t s, int64_t n, const TimeCoeffs& c, const
                             double* dotCur);
 hipError_t launch_assign_bc(hipStream_t s, int64_t n, const int32_t* ind, const double* val,
                             double timeFact, const double* soln, double* applied);
+// soln = 3 prev - 3 prev2 + prev3 (quadratic extrapolation of the last three states)
+hipError_t launch_extrapolate2(hipStream_t s, int64_t n, const double* prev, const double* prev2,
+                              const double* prev3, double* soln);
 // x[ind[k]] = val[k] * timeFact (the prescribed values of a step written into a state)
 hipError_t launch_set_bc(hipStream_t s, int64_t n, const int32_t* ind, const double* val, double timeFact,
                          double* x);

@@ -26,8 +26,8 @@ ERRORS = {-1: "SFEM_ERR_ARG", -2: "SFEM_ERR_STATE", -3: "SFEM_ERR_NEGJAC", -4: "
           -5: "SFEM_ERR_DIVERGED", -6: "SFEM_ERR_MAXIT", -7: "SFEM_ERR_NOMEM", -8: "SFEM_ERR_NOTCONV",
           -9: "SFEM_ERR_UNSUPPORTED"}
 PREC_FP64, PREC_MIXED, PREC_STORE32 = 0, 1, 2
-PRED_PREVIOUS, PRED_EXTRAPOLATE, PRED_LEARNED = 0, 1, 2
-LEARNER_NPARAM = 5187
+PRED_PREVIOUS, PRED_EXTRAPOLATE, PRED_LEARNED, PRED_QUADRATIC = 0, 1, 2, 3
+LEARNER_NPARAM = 5190
 KT_ASSEMBLY, KT_SPMV, KT_PC = 0, 1, 2
 KSP = {"bcgs": 0, "gmres": 1}
 PC = {"none": 0, "jacobi": 1, "bjacobi4": 2, "ilu": 3}
@@ -101,7 +101,7 @@ ABI = [
     ("sfem_learner_weights", C.c_int, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     ("sfem_set_learner", C.c_int, [_vp, _vp]),
     ("sfem_learned_candidate", C.c_int, [_vp, _f64p]),
-    ("sfem_predictor_stats", C.c_int, [_vp, _i64p, _i64p, _f64p]),
+    ("sfem_predictor_stats", C.c_int, [_vp, _i64p, _f64p]),
 ]
 
 _LIB = None
@@ -218,13 +218,13 @@ class Context:
         return out
 
     def predictor_stats(self):
-        """(steps started from the learned start, from the extrapolated start,
-        [||F|| extrapolated, ||F|| learned] of the last step)."""
-        a, b = C.c_int64(0), C.c_int64(0)
-        nrm = np.zeros(2)
-        _check(self.lib.sfem_predictor_stats(self.h, C.byref(a), C.byref(b), _p(nrm, C.c_double)),
+        """(steps started from the [linear, quadratic, learned] start, their ||F|| at the
+        last step) under PRED_LEARNED."""
+        cnt = np.zeros(3, np.int64)
+        nrm = np.zeros(3)
+        _check(self.lib.sfem_predictor_stats(self.h, _p(cnt, C.c_int64), _p(nrm, C.c_double)),
                "sfem_predictor_stats")
-        return a.value, b.value, nrm
+        return cnt, nrm
 
     # -- hot path ----------------------------------------------------------
     def time_update(self, timeFact: float):

@@ -123,23 +123,31 @@ int sfem_set_params(sfem_ctx* ctx, const double* elemData, int tis, double rhoIn
 int sfem_set_precision(sfem_ctx* ctx, int precision);
 
 /* Newton initial iterate of each step: SFEM_PRED_PREVIOUS (the previous step's
- * solution, as the reference) or SFEM_PRED_EXTRAPOLATE (2 solnPrev - solnPrev2, the
+ * solution, as the reference), SFEM_PRED_EXTRAPOLATE (2 solnPrev - solnPrev2, the
  * solnExtrap that SolutionData::timeUpdate computes and the reference leaves unused,
- * SolutionData.cpp:75).  It changes the Newton/Krylov iteration counts, not the answer. */
-enum { SFEM_PRED_PREVIOUS = 0, SFEM_PRED_EXTRAPOLATE = 1, SFEM_PRED_LEARNED = 2 };
+ * SolutionData.cpp:75), SFEM_PRED_QUADRATIC (3 solnPrev - 3 solnPrev2 + solnPrev3; no
+ * counterpart in the reference) or SFEM_PRED_LEARNED (below).  It changes the
+ * Newton/Krylov iteration counts, not the answer. */
+enum { SFEM_PRED_PREVIOUS = 0, SFEM_PRED_EXTRAPOLATE = 1, SFEM_PRED_LEARNED = 2, SFEM_PRED_QUADRATIC = 3 };
 int sfem_set_predictor(sfem_ctx* ctx, int predictor);
 
 /* ---- Learned initial iterate (2D meshes: u, v, p per node).  No counterpart in the
  * reference, whose Newton loop starts from the previous solution (StabFEM2.cpp:593-832).
  * A per-node MLP 12 -> 64 -> 64 -> 3 (SiLU) over the last three states of a node and the
  * mean of its 1-ring node neighbours predicts the next increment's correction over the
- * extrapolated one: start = s_n + d1 + sigma * MLP(x), d1 = s_n - s_{n-1}, sigma the
- * per-component RMS of d1 over all nodes.  With SFEM_PRED_LEARNED each step evaluates the
- * residual norm at the extrapolated and at the learned start (one residual-only element
- * pass each; the residual is affine in the state, so these are the iteration-0 norms)
- * and starts Newton from the smaller: the predictor changes iteration counts, and the
- * Newton test decides convergence as before. */
-#define SFEM_LEARNER_NPARAM 5187 /* W1 64x12, b1 64, W2 64x64, b2 64, W3 3x64, b3 3 (row-major) */
+ * quadratic extrapolation: start = s_n + 2 d1 - d2 + sigma * tau * lambda * MLP(x / lambda),
+ * d1 = s_n - s_{n-1}, d2 = s_{n-1} - s_{n-2}, x the 12 sigma-scaled inputs, sigma the
+ * per-component RMS of d1 over all nodes, lambda the node's activity scale (RMS of its six
+ * sigma-scaled own increments + 1e-3), tau a fixed output scale set before the first Adam
+ * step to the RMS of (d0 - 2 d1 + d2) / (sigma lambda) over 65,536 hashed samples
+ * (minibatch tag 2^32 - 1), so that the outputs are O(1).  With SFEM_PRED_LEARNED each
+ * step evaluates the residual norm at the linearly extrapolated, the quadratically
+ * extrapolated and the learned start (one residual-only element pass each; the residual
+ * is affine in the state, so these are the iteration-0 norms) and starts Newton from the
+ * smallest: the predictor changes iteration counts, and the Newton test decides
+ * convergence as before. */
+#define SFEM_LEARNER_NPARAM 5190 /* W1 64x12, b1 64, W2 64x64, b2 64, W3 3x64, b3 3 (row-major),
+                                     tau 3: the fixed output scale (not trained) */
 typedef struct sfem_learner sfem_learner;
 /* A learner for the mesh of ctx (pattern built; the node graph and the free-DOF mask are
  * taken from it) with room for `capacity` recorded states.  NULL on failure. */
@@ -151,7 +159,7 @@ int sfem_learner_record(sfem_learner* learner, sfem_ctx* ctx, int new_trajectory
 int sfem_learner_info(sfem_learner* learner, int64_t* snapshots, int64_t* windows, int64_t* adam_steps);
 /* iters Adam steps (lr, betas 0.9 / 0.999, eps 1e-8) on minibatches of `batch` (window,
  * node) samples; loss[iters] (may be NULL) receives each minibatch's loss, the mean over
- * samples and components of (y - t)^2 with t = (d0 - d1) / sigma, d0 the recorded next
+ * samples and components of (y - t)^2 with t = (d0 - 2 d1 + d2) / (sigma lambda tau), d0 the recorded next
  * increment (prescribed components count as zero).  Sample k of Adam step a is drawn by
  * h = splitmix64(seed ^ splitmix64(a * 2^32 + k)): window (h >> 32) mod nwindows, node
  * (h mod 2^32) mod nNode. */
@@ -160,7 +168,8 @@ int sfem_learner_train(sfem_learner* learner, int iters, int batch, double lr, d
  * window[k] is the index of the newest of the three input states in the store. */
 int sfem_learner_gradient(sfem_learner* learner, int n, const int32_t* window, const int32_t* node,
                           float* grad, double* loss);
-/* Read (get) and/or replace (set) the FP32 master weights; set also resets Adam. */
+/* Read (get) and/or replace (set) the FP32 master weights and tau; set also resets Adam
+ * and fixes tau. */
 int sfem_learner_weights(sfem_learner* learner, float* get, const float* set);
 /* Attach a learner (NULL detaches; the learner must outlive its use by ctx).  A learner
  * is not thread-safe: record, train and the steps of contexts it is attached to are
@@ -169,9 +178,9 @@ int sfem_set_learner(sfem_ctx* ctx, sfem_learner* learner);
 /* The learned start computed from ctx's current solution and its two predecessors
  * (nNode*3 doubles, host). */
 int sfem_learned_candidate(sfem_ctx* ctx, double* out);
-/* Steps that started from the learned / the extrapolated start under SFEM_PRED_LEARNED,
- * and last_norms[2] = {||F|| extrapolated, ||F|| learned} of the last step. */
-int sfem_predictor_stats(sfem_ctx* ctx, int64_t* learned, int64_t* fallback, double* last_norms);
+/* Under SFEM_PRED_LEARNED: counts[3] = steps started from the linear, the quadratic and
+ * the learned start, and last_norms[3] = their ||F|| at the last step (either may be NULL). */
+int sfem_predictor_stats(sfem_ctx* ctx, int64_t* counts, double* last_norms);
 
 /* Rebuild the preconditioner only on every lag-th new tangent (lag >= 1; 1, the default,
  * rebuilds it whenever the tangent changes, as PCSetUp does after each assembly in

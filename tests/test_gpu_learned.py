@@ -15,7 +15,8 @@ pytestmark = pytest.mark.gpu
 
 NP = sfem.LEARNER_NPARAM
 SL = {"W1": (0, (64, 12)), "b1": (768, (64,)), "W2": (832, (64, 64)), "b2": (4928, (64,)),
-      "W3": (4992, (3, 64)), "b3": (5184, (3,))}
+      "W3": (4992, (3, 64)), "b3": (5184, (3,)), "tau": (5187, (3,))}
+TAU = np.array([0.5, 0.7, 1.3], np.float32)
 
 
 def _tiny_cylinder(perturb=0.01, seed=1):
@@ -52,14 +53,23 @@ def _neighbours(mesh):
     return [np.array(sorted(s), np.int64) for s in nb]
 
 
+def _activity(x32):
+    """Divide float32 features by the node activity scale lambda (returned, float32)."""
+    lam = np.sqrt((x32[:, :6].astype(np.float32) ** 2).sum(axis=1, dtype=np.float32) / np.float32(6)) \
+        + np.float32(1e-3)
+    return (x32 * (np.float32(1) / lam)[:, None]).astype(np.float32), lam.astype(np.float32)
+
+
 def _features(S0, S1, S2, nbrs, isig):
-    """The 12 inputs per node (float64 differences, scaled, rounded to float32)."""
+    """The 12 inputs per node (float64 differences, scaled, rounded to float32, divided by
+    the node's activity scale lambda), d1, d2 and lambda."""
     s0, s1, s2 = (np.asarray(S, np.float64).reshape(-1, 3) for S in (S0, S1, S2))
     d1, d2 = s0 - s1, s1 - s2
     m1 = np.array([d1[j].mean(axis=0) for j in nbrs])
     m2 = np.array([d2[j].mean(axis=0) for j in nbrs])
     x = np.concatenate([d1 * isig, d2 * isig, (m1 - d1) * isig, (m2 - d2) * isig], axis=1)
-    return x.astype(np.float32).astype(np.float64), d1
+    x, lam = _activity(x.astype(np.float32))
+    return x.astype(np.float64), d1, d2, lam.astype(np.float64)
 
 
 def _silu(z):
@@ -68,7 +78,9 @@ def _silu(z):
 
 def _random_weights(seed, scale=0.4):
     rng = np.random.default_rng(seed)
-    return (scale * rng.standard_normal(NP)).astype(np.float32)
+    w = (scale * rng.standard_normal(NP)).astype(np.float32)
+    w[SL["tau"][0]:] = TAU
+    return w
 
 
 def _set_history(ctx, S0, S1, S2):
@@ -93,7 +105,18 @@ def _smooth_states(mesh, seed):
     return out  # S0, S1, S2
 
 
-def test_zero_network_gives_the_extrapolated_start():
+def test_quadratic_predictor_start():
+    """SFEM_PRED_QUADRATIC starts each step from 3 s_n - 3 s_(n-1) + s_(n-2)."""
+    mesh = _tiny_cylinder()
+    fem, ctx, _ = _ctx(mesh)
+    S0, S1, S2 = _smooth_states(mesh, 4)
+    _set_history(ctx, S0, S1, S2)
+    ctx.set_predictor(sfem.PRED_QUADRATIC)
+    ctx.time_update(0.5)
+    assert np.allclose(ctx.get_solution(), 3 * S0 - 3 * S1 + S2, rtol=0, atol=1e-14 * np.max(np.abs(S0)))
+
+
+def test_zero_network_gives_the_quadratic_extrapolation():
     mesh = _tiny_cylinder()
     fem, ctx, _ = _ctx(mesh)
     L = sfem.Learner(ctx, 8, seed=3)
@@ -102,7 +125,7 @@ def test_zero_network_gives_the_extrapolated_start():
     S0, S1, S2 = _smooth_states(mesh, 0)
     _set_history(ctx, S0, S1, S2)
     cand = ctx.learned_candidate()
-    assert np.allclose(cand, 2 * S0 - S1, rtol=0, atol=1e-14 * np.max(np.abs(S0)))
+    assert np.allclose(cand, 3 * S0 - 3 * S1 + S2, rtol=0, atol=1e-14 * np.max(np.abs(S0)))
 
 
 def test_mfma_inference_matches_bf16_network():
@@ -121,12 +144,12 @@ def test_mfma_inference_matches_bf16_network():
     d1 = (S0 - S1).reshape(-1, 3)
     sig = np.sqrt(np.mean(d1 ** 2, axis=0)).astype(np.float32).astype(np.float64)
     isig = (np.float32(1.0) / sig.astype(np.float32)).astype(np.float64)
-    x, d1 = _features(S0, S1, S2, _neighbours(mesh), isig)
+    x, d1, d2, lam = _features(S0, S1, S2, _neighbours(mesh), isig)
     P = _split(w)
     h1 = _bf16(_silu(_bf16(x) @ _bf16(P["W1"]).T + P["b1"]))
     h2 = _bf16(_silu(h1 @ _bf16(P["W2"]).T + P["b2"]))
     y = h2 @ _bf16(P["W3"]).T + P["b3"]
-    y_gpu = (cand - S0.reshape(-1, 3) - d1) / sig
+    y_gpu = (cand - S0.reshape(-1, 3) - 2 * d1 + d2) / (sig * P["tau"] * lam[:, None])
     err = np.max(np.abs(y_gpu - y))
     assert np.max(np.abs(y)) > 0.5
     assert err <= 2e-2 * np.max(np.abs(y)), err
@@ -161,8 +184,9 @@ def _torch_loss_grad(w, snaps, windows, nodes, nbrs, mask):
         m1 = dd[nbrs[i]].mean(axis=0)
         m2 = (s1 - s2)[nbrs[i]].mean(axis=0)
         x = np.concatenate([d1 * isig, d2 * isig, (m1 - d1) * isig, (m2 - d2) * isig]).astype(np.float32)
-        X.append(x)
-        T.append(((s3[i] - s0[i] - d1) * isig).astype(np.float32))
+        x, lam = _activity(x[None, :])
+        X.append(x[0])
+        T.append(((s3[i] - s0[i] - 2 * d1 + d2) * isig).astype(np.float32) / lam[0] / TAU)
         M.append(mask[3 * i:3 * i + 3])
     X = torch.tensor(np.array(X, np.float64))
     T = torch.tensor(np.array(T, np.float64))
@@ -174,8 +198,9 @@ def _torch_loss_grad(w, snaps, windows, nodes, nbrs, mask):
     loss.backward()
     g = np.zeros(NP)
     for k, (o, s) in SL.items():
-        g[o:o + int(np.prod(s))] = P[k].grad.numpy().ravel()
-    return float(loss), g
+        if k != "tau":  # fixed, not trained: its gradient slot is zero
+            g[o:o + int(np.prod(s))] = P[k].grad.numpy().ravel()
+    return float(loss.detach()), g
 
 
 def _free_mask(mesh):
@@ -201,7 +226,10 @@ def test_training_gradient_matches_autograd():
     lref, gref = _torch_loss_grad(w, snaps, windows, nodes, _neighbours(mesh), _free_mask(mesh))
     assert loss == pytest.approx(lref, rel=1e-4)
     assert np.max(np.abs(g - gref)) <= 1e-4 * np.max(np.abs(gref))
+    assert np.all(g[SL["tau"][0]:] == 0)
     for k, (o, s) in SL.items():  # each block's gradient separately, so no block can hide
+        if k == "tau":
+            continue
         blk = slice(o, o + int(np.prod(s)))
         assert np.linalg.norm(g[blk] - gref[blk]) <= 1e-4 * np.linalg.norm(gref[blk]) + 1e-12, k
 
@@ -236,6 +264,7 @@ def test_adam_step_on_the_hashed_minibatch():
     step = w0.astype(np.float64) - w1
     expect = lr * g / (np.abs(g) + 1e-8)
     assert np.max(np.abs(step - expect)) <= 2e-6, np.max(np.abs(step - expect))
+    assert np.array_equal(w1[SL["tau"][0]:], TAU)  # set_weights fixed the output scale
     assert L.info()[2] == 1
 
 
@@ -259,6 +288,8 @@ def test_learned_start_never_changes_the_answer():
         t += 0.1
     loss = L.train(300, 1024, 2e-3)
     assert np.all(np.isfinite(loss)) and loss[-20:].mean() < loss[:20].mean()
+    tau = L.get_weights()[SL["tau"][0]:]
+    assert np.all(np.isfinite(tau)) and np.all(tau > 0)
     fem2, ctx2, _ = _ctx(mesh)
     ctx2.set_learner(L)
     ctx2.set_predictor(sfem.PRED_LEARNED)
@@ -266,12 +297,45 @@ def test_learned_start_never_changes_the_answer():
     picked = []
     for s in range(nsteps):
         st = ctx2.step(min(t, 1.0), opts, tol, 10, allow_notconv=False)
-        nl, nf, nrm = ctx2.predictor_stats()
+        cnt, nrm = ctx2.predictor_stats()
         picked.append(nrm)
         assert st.rhs_norm0 == pytest.approx(min(nrm), rel=1e-6)
         sol = ctx2.get_solution()
         scale = np.max(np.abs(ref[s]))
         assert np.max(np.abs(sol - ref[s])) <= 1e-5 * scale, (s, np.max(np.abs(sol - ref[s])) / scale)
         t += 0.1
-    nl, nf, _ = ctx2.predictor_stats()
-    assert nl + nf == nsteps
+    cnt, _ = ctx2.predictor_stats()
+    assert cnt.sum() == nsteps
+
+
+def test_output_scale_from_hashed_samples():
+    """Before the first Adam step tau_c is set to the RMS of the scaled targets over the
+    65,536 samples of minibatch tag 2^32 - 1 (free components only)."""
+    mesh = _tiny_cylinder()
+    fem, ctx, _ = _ctx(mesh)
+    seed = 99
+    L = sfem.Learner(ctx, 16, seed=seed)
+    snaps = _record_random_trajectory(ctx, L, mesh, 6, seed=5)
+    L.train(1, 64, 1e-3)
+    tau = L.get_weights()[SL["tau"][0]:].astype(np.float64)
+    wins = [2, 3, 4]
+    tag = 0xFFFFFFFF
+    mask = _free_mask(mesh).reshape(-1, 3)
+    sq, cnt = np.zeros(3), np.zeros(3)
+    per_win = {}
+    nbrs = _neighbours(mesh)
+    for t in wins:
+        s0, s1, s2, s3 = (snaps[t + d].astype(np.float64).reshape(-1, 3) for d in (0, -1, -2, 1))
+        dd = s0 - s1
+        isig = (np.float32(1.0) / np.sqrt(np.mean(dd ** 2, axis=0)).astype(np.float32)).astype(np.float64)
+        _, _, _, lam = _features(s0, s1, s2, nbrs, isig)
+        tt = ((s3 - s0 - 2 * dd + (s1 - s2)) * isig).astype(np.float32)
+        per_win[t] = (tt / lam[:, None].astype(np.float32)).astype(np.float64)
+    for k in range(65536):
+        h = _mix64(seed ^ _mix64(tag * (1 << 32) + k))
+        t, i = wins[(h >> 32) % len(wins)], (h & 0xFFFFFFFF) % mesh.nNode
+        sq += mask[i] * per_win[t][i] ** 2
+        cnt += mask[i]
+    ref = np.sqrt(sq / cnt)
+    assert np.all(ref > 0)
+    assert np.allclose(tau, ref, rtol=1e-5), (tau, ref)
