@@ -35,6 +35,11 @@
 namespace sfem {
 
 constexpr int kIn = 12, kHid = 64, kOut = 3;
+// Components the network corrects: u and v.  The pressure keeps the quadratic
+// extrapolation: its corrections, a non-local (elliptic) quantity predicted from a
+// 1-ring stencil, gave starts with a smaller residual norm but slowly converging
+// pressure errors (more Krylov iterations than the plain quadratic start).
+constexpr int kCorrected = 2;
 constexpr int P_W1 = 0, P_B1 = P_W1 + kHid * kIn, P_W2 = P_B1 + kHid, P_B2 = P_W2 + kHid * kHid,
               P_W3 = P_B2 + kHid, P_B3 = P_W3 + kOut * kHid, P_TAU = P_B3 + kOut, kNP = P_TAU + kOut;
 static_assert(kNP == SFEM_LEARNER_NPARAM, "parameter count");
@@ -180,7 +185,7 @@ __device__ inline void load_sample(int64_t k, int64_t nNode, int64_t nstate, con
   for (int c = 0; c < 3; ++c) {
     const double d0 = (double)s0[nstate + node * 3 + c] - (double)s0[node * 3 + c];
     t[c] = (float)((d0 - 2.0 * d1[c] + d2[c]) * isig[c]) / lam;
-    mk[c] = mask[node * 3 + c] ? 1.0f : 0.0f;
+    mk[c] = (c < kCorrected && mask[node * 3 + c]) ? 1.0f : 0.0f;
   }
 }
 
@@ -482,7 +487,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer(int64_t nNode, const S* __res
   if (valid && g == 0) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const double y = (double)(acc[c] + w[P_B3 + c]) * (double)w[P_TAU + c] * (double)lam;
+      const double y = c < kCorrected ? (double)(acc[c] + w[P_B3 + c]) * (double)w[P_TAU + c] * (double)lam : 0.0;
       out[node * 3 + c] = (double)s0[node * 3 + c] + 2.0 * d1[c] - d2[c] + (double)sig[c] * y;
     }
   }

@@ -135,7 +135,8 @@ int sfem_set_predictor(sfem_ctx* ctx, int predictor);
  * reference, whose Newton loop starts from the previous solution (StabFEM2.cpp:593-832).
  * A per-node MLP 12 -> 64 -> 64 -> 3 (SiLU) over the last three states of a node and the
  * mean of its 1-ring node neighbours predicts the next increment's correction over the
- * quadratic extrapolation: start = s_n + 2 d1 - d2 + sigma * tau * lambda * MLP(x / lambda),
+ * quadratic extrapolation for u and v (the pressure keeps the quadratic extrapolation;
+ * the third output is not used and not trained): start = s_n + 2 d1 - d2 + sigma * tau * lambda * MLP(x / lambda),
  * d1 = s_n - s_{n-1}, d2 = s_{n-1} - s_{n-2}, x the 12 sigma-scaled inputs, sigma the
  * per-component RMS of d1 over all nodes, lambda the node's activity scale (RMS of its six
  * sigma-scaled own increments + 1e-3), tau a fixed output scale set before the first Adam
@@ -160,7 +161,7 @@ int sfem_learner_info(sfem_learner* learner, int64_t* snapshots, int64_t* window
 /* iters Adam steps (lr, betas 0.9 / 0.999, eps 1e-8) on minibatches of `batch` (window,
  * node) samples; loss[iters] (may be NULL) receives each minibatch's loss, the mean over
  * samples and components of (y - t)^2 with t = (d0 - 2 d1 + d2) / (sigma lambda tau), d0 the recorded next
- * increment (prescribed components count as zero).  Sample k of Adam step a is drawn by
+ * increment (prescribed components and the pressure count as zero).  Sample k of Adam step a is drawn by
  * h = splitmix64(seed ^ splitmix64(a * 2^32 + k)): window (h >> 32) mod nwindows, node
  * (h mod 2^32) mod nNode. */
 int sfem_learner_train(sfem_learner* learner, int iters, int batch, double lr, double* loss);

@@ -149,7 +149,10 @@ def test_mfma_inference_matches_bf16_network():
     h1 = _bf16(_silu(_bf16(x) @ _bf16(P["W1"]).T + P["b1"]))
     h2 = _bf16(_silu(h1 @ _bf16(P["W2"]).T + P["b2"]))
     y = h2 @ _bf16(P["W3"]).T + P["b3"]
-    y_gpu = (cand - S0.reshape(-1, 3) - 2 * d1 + d2) / (sig * P["tau"] * lam[:, None])
+    quad = S0.reshape(-1, 3) + 2 * d1 - d2
+    # the pressure keeps the quadratic extrapolation
+    assert np.allclose(cand[:, 2], quad[:, 2], rtol=0, atol=1e-14 * np.max(np.abs(S0)))
+    y, y_gpu = y[:, :2], ((cand - quad) / (sig * P["tau"] * lam[:, None]))[:, :2]
     err = np.max(np.abs(y_gpu - y))
     assert np.max(np.abs(y)) > 0.5
     assert err <= 2e-2 * np.max(np.abs(y)), err
@@ -187,7 +190,7 @@ def _torch_loss_grad(w, snaps, windows, nodes, nbrs, mask):
         x, lam = _activity(x[None, :])
         X.append(x[0])
         T.append(((s3[i] - s0[i] - 2 * d1 + d2) * isig).astype(np.float32) / lam[0] / TAU)
-        M.append(mask[3 * i:3 * i + 3])
+        M.append(mask[3 * i:3 * i + 3] * np.array([1.0, 1.0, 0.0]))  # pressure not trained
     X = torch.tensor(np.array(X, np.float64))
     T = torch.tensor(np.array(T, np.float64))
     M = torch.tensor(np.array(M, np.float64))
@@ -310,7 +313,7 @@ def test_learned_start_never_changes_the_answer():
 
 def test_output_scale_from_hashed_samples():
     """Before the first Adam step tau_c is set to the RMS of the scaled targets over the
-    65,536 samples of minibatch tag 2^32 - 1 (free components only)."""
+    65,536 samples of minibatch tag 2^32 - 1 (free velocity components only)."""
     mesh = _tiny_cylinder()
     fem, ctx, _ = _ctx(mesh)
     seed = 99
@@ -336,6 +339,7 @@ def test_output_scale_from_hashed_samples():
         t, i = wins[(h >> 32) % len(wins)], (h & 0xFFFFFFFF) % mesh.nNode
         sq += mask[i] * per_win[t][i] ** 2
         cnt += mask[i]
-    ref = np.sqrt(sq / cnt)
+    ref = np.sqrt(sq[:2] / cnt[:2])
     assert np.all(ref > 0)
-    assert np.allclose(tau, ref, rtol=1e-5), (tau, ref)
+    assert np.allclose(tau[:2], ref, rtol=1e-5), (tau, ref)
+    assert tau[2] == 1.0  # the pressure is not corrected: no samples, unit scale
