@@ -835,6 +835,34 @@ __global__ __launch_bounds__(64) void k_tile_invert(const int32_t* __restrict__ 
   }
 }
 
+// One packed triangle of a tile inverse (kTileTri values) moved global -> registers ->
+// LDS in 16-byte pieces by all NT threads of the workgroup: coalesced vector loads,
+// issued before the sums over the blocks outside the tile and stored to LDS after them.
+template <typename IT, int NT>
+struct TriStage {
+  static constexpr int PV = 16 / (int)sizeof(IT);
+  static constexpr int NV = kTileTri / PV;
+  static constexpr int R = (NV + NT - 1) / NT;
+  static_assert(kTileTri % PV == 0, "packed triangle is a whole number of 16-byte pieces");
+  uint4 v[R];
+  __device__ inline void load(const IT* __restrict__ src, int tid) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int q = tid + r * NT;
+      // every piece but the last is in range for every thread ((R - 1) NT <= NV)
+      const int qc = (r + 1 < R || q < NV) ? q : NV - 1;
+      v[r] = *reinterpret_cast<const uint4*>(src + qc * PV);
+    }
+  }
+  __device__ inline void store(IT* dst, int tid) const {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int q = tid + r * NT;
+      if (q < NV) *reinterpret_cast<uint4*>(dst + q * PV) = v[r];
+    }
+  }
+};
+
 // y_i = sum over the packed row of a triangle (row `row`, columns [c0, c1)) times v.
 template <typename IT, typename AT>
 __device__ inline AT tile_row_dot(const IT* __restrict__ row, int c0, int c1, const AT* v) {
@@ -856,6 +884,7 @@ __global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_lsolve(const int32_t*
                                                                   int fuse_u) {
   using AT = TriAcc<T, LT>;
   __shared__ AT Bv[kTileN];
+  __shared__ __align__(16) IT Tri[kTileTri];
   const int g = threadIdx.x / G, lane = threadIdx.x % G;
   const int tix = blockIdx.x;
   if (tix >= tcount) return;
@@ -866,10 +895,9 @@ __global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_lsolve(const int32_t*
   const int32_t p = ts + g;
   const IT* inv = tinv + (int64_t)tg * kTileInv;
   const int i = 4 * g + lane;
-  // row i of L^{-1}, loaded before the sums over the blocks outside the tile
-  IT lrow[kTileN];
-#pragma unroll
-  for (int j = 0; j < kTileN; ++j) lrow[j] = (act && lane < 4 && j <= i) ? inv[tri_lo(i) + j] : (IT)0;
+  // L^{-1} of the tile, loaded before the sums over the blocks outside the tile
+  TriStage<IT, G * kTileMax> lst;
+  lst.load(inv, threadIdx.x);
   AT acc[4] = {0, 0, 0, 0};
   AT rv = 0;
   if (act) {
@@ -880,11 +908,16 @@ __global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_lsolve(const int32_t*
   }
 #pragma unroll
   for (int a = 0; a < 4; ++a) acc[a] = group_sum<G>(acc[a]);
+  lst.store(Tri, threadIdx.x);
   if (lane < 4) Bv[4 * g + lane] = act ? rv - pick4(acc, lane) : (AT)0;
   tile_sync<G>();
   AT yi = 0;
+  if (lane < 4) {
+    const IT* row = Tri + tri_lo(i);
 #pragma unroll
-  for (int j = 0; j < kTileN; ++j) yi += (AT)lrow[j] * Bv[j];
+    for (int j = 0; j < kTileN; ++j)
+      if (j <= i) yi += (AT)row[j] * Bv[j];
+  }
   if (fuse_u) {
     tile_sync<G>();
     if (lane < 4) Bv[i] = yi;
@@ -903,6 +936,7 @@ __global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_usolve(const int32_t*
                                                                   const IT* __restrict__ tinv, T* __restrict__ y) {
   using AT = TriAcc<T, LT>;
   __shared__ AT Bv[kTileN];
+  __shared__ __align__(16) IT Tri[kTileTri];
   const int g = threadIdx.x / G, lane = threadIdx.x % G;
   const int tix = blockIdx.x;
   if (tix >= tcount) return;
@@ -913,10 +947,9 @@ __global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_usolve(const int32_t*
   const int32_t p = ts + g;
   const IT* inv = tinv + (int64_t)tg * kTileInv;
   const int i = 4 * g + lane;
-  // row i of U^{-1}, loaded before the sums over the blocks outside the tile
-  IT urow[kTileN];
-#pragma unroll
-  for (int j = 0; j < kTileN; ++j) urow[j] = (act && lane < 4 && j >= i) ? inv[tri_up(i) + j - i] : (IT)0;
+  // U^{-1} of the tile, loaded before the sums over the blocks outside the tile
+  TriStage<IT, G * kTileMax> ust;
+  ust.load(inv + kTileTri, threadIdx.x);
   AT acc[4] = {0, 0, 0, 0};
   AT rv = 0;
   if (act) {
@@ -928,11 +961,16 @@ __global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_usolve(const int32_t*
   }
 #pragma unroll
   for (int a = 0; a < 4; ++a) acc[a] = group_sum<G>(acc[a]);
+  ust.store(Tri, threadIdx.x);
   if (lane < 4) Bv[4 * g + lane] = act ? rv - pick4(acc, lane) : (AT)0;
   tile_sync<G>();
   AT zi = 0;
+  if (lane < 4) {
+    const IT* row = Tri + (tri_up(i) - kTileTri) - i;  // row[j] = U^{-1}(i, j), j >= i
 #pragma unroll
-  for (int j = 0; j < kTileN; ++j) zi += (AT)urow[j] * Bv[j];
+    for (int j = 0; j < kTileN; ++j)
+      if (j >= i) zi += (AT)row[j] * Bv[j];
+  }
   if (act && lane < 4) y[4 * (int64_t)p + lane] = (T)zi;
 }
 
