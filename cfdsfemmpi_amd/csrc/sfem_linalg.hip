@@ -863,6 +863,26 @@ struct TriStage {
   }
 };
 
+// Row ii = 4 g + (lane & 3) of a tile's triangle times the tile's right-hand sides: the
+// G lanes of group g split the 32 columns, G / 4 lanes per row (column j goes to lane
+// j mod (G / 4)), and are summed by shuffles; every lane of the row gets the sum.
+// `row` points at the packed row with row[j] the entry of column j; only columns with
+// lo <= j <= hi are present.
+template <int G, typename IT, typename AT>
+__device__ inline AT tile_tri_row(const IT* row, const AT* v, int lane, int lo, int hi) {
+  constexpr int S = G / 4;
+  const int sub = lane >> 2;
+  AT part = 0;
+#pragma unroll
+  for (int jj = 0; jj < kTileN / S; ++jj) {
+    const int j = jj * S + sub;
+    if (j >= lo && j <= hi) part += (AT)row[j] * v[j];
+  }
+#pragma unroll
+  for (int o = G / 2; o >= 4; o >>= 1) part += __shfl_xor(part, o, G);
+  return part;
+}
+
 // y_i = sum over the packed row of a triangle (row `row`, columns [c0, c1)) times v.
 template <typename IT, typename AT>
 __device__ inline AT tile_row_dot(const IT* __restrict__ row, int c0, int c1, const AT* v) {
@@ -911,13 +931,8 @@ __global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_lsolve(const int32_t*
   lst.store(Tri, threadIdx.x);
   if (lane < 4) Bv[4 * g + lane] = act ? rv - pick4(acc, lane) : (AT)0;
   tile_sync<G>();
-  AT yi = 0;
-  if (lane < 4) {
-    const IT* row = Tri + tri_lo(i);
-#pragma unroll
-    for (int j = 0; j < kTileN; ++j)
-      if (j <= i) yi += (AT)row[j] * Bv[j];
-  }
+  const int ii = 4 * g + (lane & 3);  // the row this lane helps with (ii == i for lane < 4)
+  AT yi = tile_tri_row<G>(Tri + tri_lo(ii), (const AT*)Bv, lane, 0, ii);
   if (fuse_u) {
     tile_sync<G>();
     if (lane < 4) Bv[i] = yi;
@@ -964,13 +979,9 @@ __global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_usolve(const int32_t*
   ust.store(Tri, threadIdx.x);
   if (lane < 4) Bv[4 * g + lane] = act ? rv - pick4(acc, lane) : (AT)0;
   tile_sync<G>();
-  AT zi = 0;
-  if (lane < 4) {
-    const IT* row = Tri + (tri_up(i) - kTileTri) - i;  // row[j] = U^{-1}(i, j), j >= i
-#pragma unroll
-    for (int j = 0; j < kTileN; ++j)
-      if (j >= i) zi += (AT)row[j] * Bv[j];
-  }
+  const int ii = 4 * g + (lane & 3);  // the row this lane helps with (ii == i for lane < 4)
+  // row[j] = U^{-1}(ii, j) for j >= ii
+  AT zi = tile_tri_row<G>(Tri + (tri_up(ii) - kTileTri) - ii, (const AT*)Bv, lane, ii, kTileN - 1);
   if (act && lane < 4) y[4 * (int64_t)p + lane] = (T)zi;
 }
 
