@@ -883,15 +883,6 @@ __device__ inline AT tile_tri_row(const IT* row, const AT* v, int lane, int lo, 
   return part;
 }
 
-// y_i = sum over the packed row of a triangle (row `row`, columns [c0, c1)) times v.
-template <typename IT, typename AT>
-__device__ inline AT tile_row_dot(const IT* __restrict__ row, int c0, int c1, const AT* v) {
-  AT s = 0;
-#pragma unroll 8
-  for (int j = c0; j < c1; ++j) s += (AT)row[j - c0] * v[j];
-  return s;
-}
-
 // Forward sweep of one colour: y = L^{-1} r on its tiles.  With fuse_u the tiles have no
 // upper blocks outside themselves (the last colour) and their backward sweep follows.
 template <typename T, typename LT, typename IT, int G, int K>
@@ -934,10 +925,14 @@ __global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_lsolve(const int32_t*
   const int ii = 4 * g + (lane & 3);  // the row this lane helps with (ii == i for lane < 4)
   AT yi = tile_tri_row<G>(Tri + tri_lo(ii), (const AT*)Bv, lane, 0, ii);
   if (fuse_u) {
-    tile_sync<G>();
+    // the tile's backward sweep: U^{-1} staged like L^{-1} (padding rows and columns of
+    // the inverse are identity, so the padded right-hand sides, 0, add nothing)
+    tile_sync<G>();  // every lane is done with L^{-1} and Bv
+    for (int q = threadIdx.x; q < TriStage<IT, G * kTileMax>::NV; q += G * kTileMax)
+      reinterpret_cast<uint4*>(Tri)[q] = reinterpret_cast<const uint4*>(inv + kTileTri)[q];
     if (lane < 4) Bv[i] = yi;
     tile_sync<G>();
-    if (act && lane < 4) yi = tile_row_dot(inv + tri_up(i), i, 4 * tsz, (const AT*)Bv);
+    yi = tile_tri_row<G>(Tri + (tri_up(ii) - kTileTri) - ii, (const AT*)Bv, lane, ii, kTileN - 1);
   }
   if (act && lane < 4) y[4 * (int64_t)p + lane] = (T)yi;
 }
