@@ -229,6 +229,37 @@ def test_krylov_against_direct(ksp, pc, reorder):
     ctx.close()
 
 
+@pytest.mark.parametrize("ksp,reorder", [("bcgs", 3), ("gmres", 2)])
+def test_krylov_absolute_tolerance(ksp, reorder):
+    """-ksp_atol: the solve stops once ||b - A x|| <= max(rtol ||b||, atol), in fewer
+    iterations than the relative test alone asks for, and a right-hand side already below
+    atol gives a zero increment without iterating."""
+    import scipy.sparse as sp
+    mesh = meshgen.ldc3d_hex(5, graded=True)
+    ctx = make_ctx(mesh, tis=2, dt=0.1, elemData=np.array([1.0, 1e-2, 0, 0, 0, 0, 0, 0, 1, 1, 0]))
+    ctx.time_update(1.0)
+    ctx.assemble(0)
+    vals, rhs = ctx.get_system()
+    rp, ci, _ = ctx.get_pattern()
+    A = sp.csr_matrix((vals, ci, rp), shape=(ctx.nfree, ctx.nfree))
+    bn = np.linalg.norm(rhs)
+    its_rel, _ = ctx.solve(sfem.solver_opts(ksp, "ilu", 1e-12, 5000, 40, reorder))
+    atol = 1e-6 * bn
+    its_abs, rr = ctx.solve(sfem.solver_opts(ksp, "ilu", 1e-12, 5000, 40, reorder, atol=atol))
+    du = ctx.get_increment()
+    true_res = np.linalg.norm(rhs - A @ du)
+    print(f"[atol] {ksp} reorder={reorder}: its rel-only {its_rel}, with atol {its_abs}, "
+          f"true {true_res / bn:.2e}", flush=True)
+    assert 0 < its_abs < its_rel
+    assert true_res <= 1.01 * atol
+    its0, rr0 = ctx.solve(sfem.solver_opts(ksp, "ilu", 1e-12, 5000, 40, reorder, atol=2.0 * bn))
+    assert its0 == 0 and rr0 == 0.0
+    assert np.all(ctx.get_increment() == 0.0)
+    with pytest.raises(sfem.SfemError):
+        ctx.solve(sfem.solver_opts(ksp, "ilu", 1e-12, 5000, 40, reorder, atol=-1.0))
+    ctx.close()
+
+
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 @pytest.mark.parametrize("reorder", [0, 1, 2, 3])
 def test_ilu_factor_vs_oracle(reorder, prec):
