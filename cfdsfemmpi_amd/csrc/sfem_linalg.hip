@@ -559,10 +559,11 @@ __device__ inline V4<AT> ld4(const bf16* p) {
            (AT)__uint_as_float(u.y << 16), (AT)__uint_as_float(u.y & 0xffff0000u)}};
 }
 
-// acc[a] += sum over blocks t in [t0, t1) of V(a, t) . x(bcol[t]) for a < m.  Lanes of a
-// group of G step over t; the first K blocks of every lane are loaded (index, x and m
+// acc[a] += sum over blocks t in [t0, t1) of V(a, t) . x(bcol[t]) for a < m, row a of
+// block t at V + a rs + BS t (BS = 4: a block row's rows interleaved across its blocks;
+// BS = 16, rs = 4: block by block).  Lanes of a group of G step over t; the first K blocks of every lane are loaded (index, x and m
 // value rows) before any is used, the rest one at a time.
-template <int G, int K, typename VT, typename XT, typename AT>
+template <int G, int K, int BS = 4, typename VT, typename XT, typename AT>
 __device__ inline void bsr_dot(const int32_t* __restrict__ bc, const VT* __restrict__ V, int32_t rs,
                                const XT* __restrict__ x, int m, int t0, int t1, int lane, AT (&acc)[4]) {
   int32_t c[K];
@@ -574,7 +575,7 @@ __device__ inline void bsr_dot(const int32_t* __restrict__ bc, const VT* __restr
     c[s] = ok ? bc[t] : -1;
 #pragma unroll
     for (int a = 0; a < 4; ++a)
-      v[a][s] = (ok && a < m) ? ld4<AT>(V + (int64_t)a * rs + 4 * t) : V4<AT>{{0, 0, 0, 0}};
+      v[a][s] = (ok && a < m) ? ld4<AT>(V + (int64_t)a * rs + BS * t) : V4<AT>{{0, 0, 0, 0}};
   }
   V4<AT> xv[K];
 #pragma unroll
@@ -590,7 +591,7 @@ __device__ inline void bsr_dot(const int32_t* __restrict__ bc, const VT* __restr
 #pragma unroll
     for (int a = 0; a < 4; ++a)
       if (a < m) {
-        const V4<AT> w = ld4<AT>(V + (int64_t)a * rs + 4 * t);
+        const V4<AT> w = ld4<AT>(V + (int64_t)a * rs + BS * t);
         acc[a] += w.v[0] * xx.v[0] + w.v[1] * xx.v[1] + w.v[2] * xx.v[2] + w.v[3] * xx.v[3];
       }
   }
@@ -889,7 +890,8 @@ template <typename T, typename LT, typename IT, int G, int K>
 __global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_lsolve(const int32_t* __restrict__ tstart, int tfirst,
                                                                   int tcount, const BDesc* __restrict__ desc,
                                                                   const int32_t* __restrict__ bcol,
-                                                                  const LT* __restrict__ lu,
+                                                                  const LT* __restrict__ sw,
+                                                                  const int4* __restrict__ swoff,
                                                                   const IT* __restrict__ tinv,
                                                                   const T* __restrict__ r, T* __restrict__ y,
                                                                   int fuse_u) {
@@ -914,8 +916,9 @@ __global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_lsolve(const int32_t*
   if (act) {
     const BDesc d = desc[p];
     const int m = d.dm >> 16, td = d.dm & 0xffff;
+    const int4 o = swoff[p];  // o.z == td - g lower blocks outside the tile
     if (lane < 4) rv = (AT)r[4 * (int64_t)p + lane];
-    bsr_dot<G, K>(bcol + d.bptr, lu + d.vbase, 4 * d.nb, y, m, 0, td - g, lane, acc);
+    bsr_dot<G, K, 16>(bcol + d.bptr, sw + 16 * (int64_t)o.x, 4, y, m, 0, o.z, lane, acc);
   }
 #pragma unroll
   for (int a = 0; a < 4; ++a) acc[a] = group_sum<G>(acc[a]);
@@ -942,7 +945,8 @@ template <typename T, typename LT, typename IT, int G, int K>
 __global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_usolve(const int32_t* __restrict__ tstart, int tfirst,
                                                                   int tcount, const BDesc* __restrict__ desc,
                                                                   const int32_t* __restrict__ bcol,
-                                                                  const LT* __restrict__ lu,
+                                                                  const LT* __restrict__ sw,
+                                                                  const int4* __restrict__ swoff,
                                                                   const IT* __restrict__ tinv, T* __restrict__ y) {
   using AT = TriAcc<T, LT>;
   __shared__ AT Bv[kTileN];
@@ -965,9 +969,9 @@ __global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_usolve(const int32_t*
   if (act) {
     const BDesc d = desc[p];
     const int m = d.dm >> 16, td = d.dm & 0xffff;
-    const int nin = tsz - 1 - g;  // intra-tile upper blocks
+    const int4 o = swoff[p];  // o.w == td + 1 + (intra-tile upper blocks)
     if (lane < 4) rv = (AT)y[4 * (int64_t)p + lane];
-    bsr_dot<G, K>(bcol + d.bptr, lu + d.vbase, 4 * d.nb, y, m, td + 1 + nin, d.nb, lane, acc);
+    bsr_dot<G, K, 16>(bcol + d.bptr + o.w, sw + 16 * (int64_t)o.y, 4, y, m, 0, d.nb - o.w, lane, acc);
   }
 #pragma unroll
   for (int a = 0; a < 4; ++a) acc[a] = group_sum<G>(acc[a]);
@@ -980,6 +984,27 @@ __global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_usolve(const int32_t*
   if (act && lane < 4) y[4 * (int64_t)p + lane] = (T)zi;
 }
 
+// The tile sweeps' block-by-block copy of a factor's blocks outside the tiles (swoff):
+// one 64-lane workgroup per block row; rows a >= m of a block are written as zero.
+template <typename LT>
+__global__ __launch_bounds__(64) void k_tile_sweep_copy(int64_t nblk, const BDesc* __restrict__ desc,
+                                                        const int4* __restrict__ swoff, const LT* __restrict__ lu,
+                                                        LT* __restrict__ sw) {
+  const int64_t p = blockIdx.x;
+  if (p >= nblk) return;
+  const BDesc d = desc[p];
+  const int4 o = swoff[p];
+  const int m = d.dm >> 16;
+  const int total = (o.z + d.nb - o.w) * 16;
+  for (int q = threadIdx.x; q < total; q += 64) {
+    const int j = q >> 4, a = (q >> 2) & 3, b = q & 3;
+    const bool lower = j < o.z;
+    const int t = lower ? j : o.w + (j - o.z);
+    const int64_t dst = ((int64_t)(lower ? o.x + j : o.y + (j - o.z))) * 16 + (q & 15);
+    sw[dst] = a < m ? lu[(int64_t)d.vbase + (int64_t)a * 4 * d.nb + 4 * t + b] : LT{};
+  }
+}
+
 // Tile sweep shapes: code c -> G = kTileGs[c] lanes per block row, K = kTileKs[c] blocks
 // per lane loaded together.
 static const int kTileGs[4] = {8, 16, 8, 16};
@@ -988,27 +1013,27 @@ constexpr int kTileCodes = 4;
 constexpr int kTileDefault = 1;
 
 template <typename T, typename LT, bool UPPER, int G, int K>
-static void tile_launch_g(hipStream_t st, const IluPlan& P, int t0, int cnt, const LT* lu, const void* tinv,
+static void tile_launch_g(hipStream_t st, const IluPlan& P, int t0, int cnt, const LT* sw, const void* tinv,
                           const T* r, T* y, int fuse_u) {
   // the tile inverses are FP64 beside an FP64 factor, FP32 otherwise
   using IT = typename std::conditional<std::is_same<LT, double>::value, double, float>::type;
   const IT* ti = (const IT*)tinv;
   if constexpr (UPPER)
     hipLaunchKernelGGL((k_bsr_tile_usolve<T, LT, IT, G, K>), dim3((unsigned)cnt), dim3(G * kTileMax), 0, st,
-                       P.d_tileStart, t0, cnt, P.d_bdesc, P.d_bcol, lu, ti, y);
+                       P.d_tileStart, t0, cnt, P.d_bdesc, P.d_bcol, sw, P.d_swoff, ti, y);
   else
     hipLaunchKernelGGL((k_bsr_tile_lsolve<T, LT, IT, G, K>), dim3((unsigned)cnt), dim3(G * kTileMax), 0, st,
-                       P.d_tileStart, t0, cnt, P.d_bdesc, P.d_bcol, lu, ti, r, y, fuse_u);
+                       P.d_tileStart, t0, cnt, P.d_bdesc, P.d_bcol, sw, P.d_swoff, ti, r, y, fuse_u);
 }
 
 template <typename T, typename LT, bool UPPER>
-static void tile_launch(hipStream_t st, int code, const IluPlan& P, int t0, int cnt, const LT* lu, const void* tinv,
+static void tile_launch(hipStream_t st, int code, const IluPlan& P, int t0, int cnt, const LT* sw, const void* tinv,
                         const T* r, T* y, int fuse_u = 0) {
   switch (code) {
-    case 0: tile_launch_g<T, LT, UPPER, 8, 1>(st, P, t0, cnt, lu, tinv, r, y, fuse_u); break;
-    case 2: tile_launch_g<T, LT, UPPER, 8, 2>(st, P, t0, cnt, lu, tinv, r, y, fuse_u); break;
-    case 3: tile_launch_g<T, LT, UPPER, 16, 2>(st, P, t0, cnt, lu, tinv, r, y, fuse_u); break;
-    default: tile_launch_g<T, LT, UPPER, 16, 1>(st, P, t0, cnt, lu, tinv, r, y, fuse_u); break;
+    case 0: tile_launch_g<T, LT, UPPER, 8, 1>(st, P, t0, cnt, sw, tinv, r, y, fuse_u); break;
+    case 2: tile_launch_g<T, LT, UPPER, 8, 2>(st, P, t0, cnt, sw, tinv, r, y, fuse_u); break;
+    case 3: tile_launch_g<T, LT, UPPER, 16, 2>(st, P, t0, cnt, sw, tinv, r, y, fuse_u); break;
+    default: tile_launch_g<T, LT, UPPER, 16, 1>(st, P, t0, cnt, sw, tinv, r, y, fuse_u); break;
   }
 }
 
@@ -1212,7 +1237,7 @@ LinAlg::~LinAlg() {
   void* ptrs[] = {d_blkStart, d_diagpos, d_part, d_scal, r, rh, p, v, s, t, ph, sh, x, w, z, pr, pz,
                   r32, rh32, p32, v32, s32, t32, ph32, sh32, x32, du32, w32, z32, pr32, pz32,
                   dinv, dinv32, binv, binv32, lu, lu32, gm_V, gm_V32, d_desc, lu16, kp, kp32, d_h, d_mpart,
-                  tinv, tinv32};
+                  tinv, tinv32, sw, sw32, sw16};
   delete ilu;
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
@@ -1456,7 +1481,8 @@ static std::vector<BlkDesc> make_desc(const std::vector<int32_t>& bs, const std:
 }
 
 IluPlan::~IluPlan() {
-  void* ptrs[] = {d_levL, d_levU, d_bdesc, d_bcol, d_bcolofs, d_r0orig, d_padmap, d_natrows, d_tileStart};
+  void* ptrs[] = {d_levL, d_levU, d_bdesc, d_bcol, d_bcolofs, d_r0orig, d_padmap, d_natrows, d_tileStart,
+                  d_swoff};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
 }
@@ -1558,7 +1584,8 @@ static int build_ilu_plan(sfem_ctx* ctx, LinAlg* L, int reorder) {
   L->ilu = nullptr;
   // factor-sized buffers belong to the previous plan
   void** vals[] = {(void**)&L->kp, (void**)&L->kp32, (void**)&L->lu, (void**)&L->lu32, (void**)&L->lu16,
-                   (void**)&L->tinv, (void**)&L->tinv32};
+                   (void**)&L->tinv, (void**)&L->tinv32, (void**)&L->sw, (void**)&L->sw32,
+                   (void**)&L->sw16};
   for (void** q : vals)
     if (*q) { (void)hipFree(*q); *q = nullptr; }
   IluPlan* Pp = new IluPlan;
@@ -1739,6 +1766,27 @@ static int build_ilu_plan(sfem_ctx* ctx, LinAlg* L, int reorder) {
     }
     P.tile_fuse_last = fl;
     if ((rc = upload(&P.d_tileStart, P.h_tileStart))) return rc;
+    // offsets of the sweeps' block-by-block copy of the blocks outside the tiles
+    std::vector<int4> swoff(nb);
+    int64_t nlo = 0, nup = 0;
+    for (int64_t t = 0; t < ntl; ++t) {
+      const int32_t ts = P.h_tileStart[t], te = P.h_tileStart[t + 1];
+      for (int32_t q = ts; q < te; ++q) {
+        const int32_t cnt = (int32_t)(bptr[q + 1] - bptr[q]);
+        const int32_t lo = tdiag[q] - (q - ts), u0 = tdiag[q] + 1 + (te - 1 - q);
+        swoff[q] = make_int4((int32_t)nlo, (int32_t)nup, lo, u0);
+        nlo += lo;
+        nup += cnt - u0;
+      }
+    }
+    for (auto& o : swoff) o.y += (int32_t)nlo;
+    if (nlo + nup >= ((int64_t)1 << 31) / 16) {
+      delete L->ilu;
+      L->ilu = nullptr;
+      return set_error(SFEM_ERR_UNSUPPORTED, "tiled ILU(0) plan: too many blocks outside the tiles");
+    }
+    P.sw_blocks = nlo + nup;
+    if ((rc = upload(&P.d_swoff, swoff))) return rc;
   }
   return SFEM_OK;
 }
@@ -1891,6 +1939,22 @@ int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f, 
         hipLaunchKernelGGL((k_tile_invert<double, double>), dim3(ntl), dim3(64), 0, st, P.d_tileStart, ntl,
                            P.d_bdesc, L->lu, L->tinv, L->d_scal);
       }
+      // the sweeps' block-by-block copy of the blocks outside the tiles, from the same factor
+      const int64_t nsw = 16 * std::max<int64_t>(P.sw_blocks, 1);
+      const unsigned nbk = (unsigned)P.nblk;
+      if (L->pc_bf16) {
+        if ((rc = ensure_vec(&L->sw16, nsw))) return rc;
+        hipLaunchKernelGGL((k_tile_sweep_copy<bf16>), dim3(nbk), dim3(64), 0, st, P.nblk, P.d_bdesc, P.d_swoff,
+                           L->lu16, L->sw16);
+      } else if (L->pc_fp32) {
+        if ((rc = ensure_vec(&L->sw32, nsw))) return rc;
+        hipLaunchKernelGGL((k_tile_sweep_copy<float>), dim3(nbk), dim3(64), 0, st, P.nblk, P.d_bdesc, P.d_swoff,
+                           L->lu32, L->sw32);
+      } else {
+        if ((rc = ensure_vec(&L->sw, nsw))) return rc;
+        hipLaunchKernelGGL((k_tile_sweep_copy<double>), dim3(nbk), dim3(64), 0, st, P.nblk, P.d_bdesc, P.d_swoff,
+                           L->lu, L->sw);
+      }
     }
     LA_CHECK(hipGetLastError());
   }
@@ -1964,17 +2028,21 @@ static void ilu_solve(sfem_ctx* ctx, const IluPlan& P, const LT* lu, const T* r,
     const int nc = (int)P.tileColPtr.size() - 1;
     const int tf = P.tile_fuse_last ? 1 : 0;
     const void* ti = std::is_same<LT, double>::value ? (const void*)L->tinv : (const void*)L->tinv32;
+    // the sweeps read the block-by-block copy of the factor's blocks outside the tiles
+    const LT* sw = std::is_same<LT, double>::value  ? (const LT*)(const void*)L->sw
+                   : std::is_same<LT, float>::value ? (const LT*)(const void*)L->sw32
+                                                    : (const LT*)(const void*)L->sw16;
     for (int c = 0; c < nc; ++c) {
       const int t0 = P.tileColPtr[c], cnt = P.tileColPtr[c + 1] - t0;
       const int code = force >= 0 ? force : c < (int)tl.size() ? tl[c] : kTileDefault;
-      if (cnt) tile_launch<T, LT, false>(st, code, P, t0, cnt, lu, ti, r, z, c == nc - 1 ? tf : 0);
+      if (cnt) tile_launch<T, LT, false>(st, code, P, t0, cnt, sw, ti, r, z, c == nc - 1 ? tf : 0);
       if (ev) (void)hipEventRecord(ev[1 + c], st);
     }
     for (int k = 0; k < nc - tf; ++k) {
       const int c = nc - 1 - tf - k;
       const int t0 = P.tileColPtr[c], cnt = P.tileColPtr[c + 1] - t0;
       const int code = force >= 0 ? force : k < (int)tu.size() ? tu[k] : kTileDefault;
-      if (cnt) tile_launch<T, LT, true>(st, code, P, t0, cnt, lu, ti, z, z);
+      if (cnt) tile_launch<T, LT, true>(st, code, P, t0, cnt, sw, ti, z, z);
       if (ev) (void)hipEventRecord(ev[1 + nc + k], st);
     }
     return;

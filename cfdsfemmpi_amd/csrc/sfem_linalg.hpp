@@ -46,6 +46,12 @@ struct IluPlan {
   std::vector<int32_t> tileColPtr;     // colour c = tiles [tileColPtr[c], tileColPtr[c+1])
   int64_t tile_vals = 0;               // factor values inside the tiles (applied through the inverses)
   int32_t* d_tileStart = nullptr;
+  // the tile sweeps' copy of the factor blocks outside the tiles, block by block (16
+  // values, rows a >= m zero): per block row {first lower block, first upper block,
+  // lower count, first upper block's position in the row}; lower parts of all rows
+  // in plan order, then the upper parts
+  int4* d_swoff = nullptr;
+  int64_t sw_blocks = 0;
   ~IluPlan();
 };
 
@@ -109,6 +115,9 @@ struct LinAlg {
   // FP32 for an FP32 / BF16 one
   double* tinv = nullptr;
   float* tinv32 = nullptr;
+  double* sw = nullptr;    // tile sweeps' off-tile factor blocks (d_swoff), FP64 / FP32 / BF16
+  float* sw32 = nullptr;
+  bf16* sw16 = nullptr;
   // the tangent in the ILU(0) plan's block form, for the products of the Krylov solve
   double* kp = nullptr;
   float* kp32 = nullptr;
