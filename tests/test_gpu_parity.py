@@ -531,7 +531,10 @@ def test_batch_matches_single_meshes(prec, count, seed, n_range):
         c1.close()
         d = np.linalg.norm(sb[off:off + m.nNode] - s1) / np.linalg.norm(s1)
         print(f"[batch] mesh {k}: rel diff vs FP64 alone {d:.2e}", flush=True)
-        assert d <= 1e-7
+        # both runs meet ||F|| < 1e-10; with FP32 inner solves the converged iterate also
+        # depends on where those solves stop (their rounding follows the autotuned shapes),
+        # which on the high-Re meshes moves it by up to a few 1e-7
+        assert d <= (1e-7 if prec == sfem.PREC_FP64 else 5e-7)
         off += m.nNode
 
 
