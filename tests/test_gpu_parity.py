@@ -291,6 +291,7 @@ def test_ilu_factor_vs_oracle(reorder, prec):
 
 
 @pytest.mark.parametrize("mesh_kind,prec,bf16", [("hex", "fp64", 0), ("hex", "mixed", 0), ("hex", "mixed", 1),
+                                                ("hex", "store32", 0),
                                                 ("quad", "fp64", 0), ("tria", "fp64", 0), ("tet", "fp64", 0)])
 @pytest.mark.parametrize("reorder", [0, 2, 3])
 def test_ilu_apply_vs_factor(reorder, mesh_kind, prec, bf16):
@@ -309,6 +310,8 @@ def test_ilu_apply_vs_factor(reorder, mesh_kind, prec, bf16):
     ctx = make_ctx(mesh, tis=2, dt=0.1, elemData=np.array([1.0, 1e-2, 0, 0, 0, 0, 0, 0, 1, 1, 0]))
     if prec == "mixed":
         ctx.set_precision(sfem.PREC_MIXED)
+    elif prec == "store32":  # FP64 vectors through the FP32-stored factor
+        ctx.set_precision(sfem.PREC_STORE32)
     ctx.time_update(1.0)
     ctx.assemble(0)
     ctx.solve(sfem.solver_opts("bcgs", "ilu", 1e-6, 5000, 30, reorder, factor_bf16=bf16))
