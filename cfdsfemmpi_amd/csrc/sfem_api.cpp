@@ -859,6 +859,24 @@ int sfem_tuning_levels(sfem_ctx* c, int fp32, int* out, int cap, int* nlevels) {
   return la_tuning_levels(c, fp32, out, cap, nlevels);
 }
 
+int sfem_set_tuning_mode(sfem_ctx* c, int mode) {
+  if (!c) return set_error(SFEM_ERR_ARG, "null context");
+  if (!c->la) return set_error(SFEM_ERR_STATE, "build the pattern first");
+  return la_set_tuning_mode(c, mode);
+}
+
+int sfem_get_tuning_table(sfem_ctx* c, int fp32, int* table, int cap, int* n) {
+  if (!c || !n || cap < 0 || (cap > 0 && !table)) return set_error(SFEM_ERR_ARG, "bad argument");
+  if (!c->la) return set_error(SFEM_ERR_STATE, "build the pattern first");
+  return la_get_tuning_table(c, fp32, table, cap, n);
+}
+
+int sfem_set_tuning_table(sfem_ctx* c, int fp32, const int* table, int n) {
+  if (!c || !table || n < 0) return set_error(SFEM_ERR_ARG, "bad argument");
+  if (!c->la) return set_error(SFEM_ERR_STATE, "build the pattern first");
+  return la_set_tuning_table(c, fp32, table, n);
+}
+
 int sfem_info(sfem_ctx* c, int64_t* nfree, int64_t* nnz, int* ncolors, int* ndof) {
   if (!c) return set_error(SFEM_ERR_ARG, "null context");
   if (nfree) *nfree = c->nfree;

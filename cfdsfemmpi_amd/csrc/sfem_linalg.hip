@@ -1825,8 +1825,12 @@ int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f, 
   const int64_t n = L->n;
   if (store32 && (fp32 || pc_type != SFEM_PC_ILU0)) return set_error(SFEM_ERR_ARG, "FP32 storage needs FP64 ILU(0)");
   L->pc_type = pc_type;
-  L->pc_fp32 = fp32;
-  L->pc_store32 = false;
+  if (refactor) {
+    // a lagged refresh (refactor = false) keeps the factor, so it keeps the precision
+    // flags the factorisation set (the caller checked that they match the request)
+    L->pc_fp32 = fp32;
+    L->pc_store32 = false;
+  }
   const double* vals = fp32 ? nullptr : ctx->d_vals;
   const float* vals32 = fp32 ? ctx->d_vals32 : nullptr;
   LA_CHECK(hipMemsetAsync(L->d_scal + S_MISC1, 0, sizeof(double), st));
@@ -1892,7 +1896,6 @@ int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f, 
       if (rc) return rc;
     }
     if (!refactor) {
-      L->pc_store32 = store32;
       LA_CHECK(hipGetLastError());
       return SFEM_OK;
     }
@@ -2069,6 +2072,18 @@ static void ilu_solve(sfem_ctx* ctx, const IluPlan& P, const LT* lu, const T* r,
   }
 }
 
+// The factor (and, for a tiled plan, the tile inverses and the sweeps' off-tile copy)
+// that ilu_apply reads for the current precision flags exist.
+static bool ilu_buffers_ready(const LinAlg* L) {
+  if (!L->ilu) return false;
+  const void* lu = L->pc_bf16 ? (const void*)L->lu16 : L->pc_fp32 ? (const void*)L->lu32 : (const void*)L->lu;
+  if (!lu) return false;
+  if (!L->ilu->tiled) return true;
+  const void* ti = (L->pc_fp32 || L->pc_bf16) ? (const void*)L->tinv32 : (const void*)L->tinv;
+  const void* sw = L->pc_bf16 ? (const void*)L->sw16 : L->pc_fp32 ? (const void*)L->sw32 : (const void*)L->sw;
+  return ti && sw;
+}
+
 // ILU(0) apply with the factor type of the current preconditioner.
 template <typename T>
 static void ilu_apply(sfem_ctx* ctx, const T* r, T* z, int force = -1, hipEvent_t* ev = nullptr,
@@ -2099,6 +2114,7 @@ static int pc_apply_impl(sfem_ctx* ctx, const T* r, T* z, const RowSubset* sub) 
       break;
     }
     case SFEM_PC_ILU0:
+      if (!ilu_buffers_ready(L)) return set_error(SFEM_ERR_STATE, "ILU(0) factor buffers missing for its precision");
       ilu_apply<T>(ctx, r, z, -1, nullptr, sub);
       break;
     default:
@@ -2669,6 +2685,80 @@ static bool needs_tuning(const LinAlg* L, int slot) {
   return L->block_space ? L->tune_plan[slot] != L->plan_id : !L->tuned_csr[slot];
 }
 
+constexpr int kSpmvDefault = 3;  // CSR SpMV: G = 16, U = 2
+
+// SFEM_TUNE_FIXED: the default shapes for the current layout, with no timing, so that
+// the summation order of every product (and with it the FP32 rounding) is the same on
+// every run and device.
+static void fixed_shapes(LinAlg* L, int slot) {
+  if (L->block_space) {
+    L->tune_bspmv[slot] = kBlkDefault;
+    L->tune_L[slot].clear();
+    L->tune_U[slot].clear();
+    L->tune_plan[slot] = L->plan_id;
+  } else {
+    L->tune_spmv[slot] = kSpmvDefault;
+    L->tuned_csr[slot] = true;
+  }
+}
+
+int la_set_tuning_mode(sfem_ctx* ctx, int mode) {
+  LinAlg* L = ctx->la;
+  if (mode != SFEM_TUNE_TIMED && mode != SFEM_TUNE_FIXED) return set_error(SFEM_ERR_ARG, "bad tuning mode");
+  L->tune_mode = mode;
+  // the next solve of each precision times the shapes again or installs the defaults
+  for (int k = 0; k < 2; ++k) {
+    L->tune_plan[k] = -1;
+    L->tuned_csr[k] = false;
+  }
+  return SFEM_OK;
+}
+
+// Table of one precision: {CSR SpMV code, block SpMV code, forward launches nL, backward
+// launches nU, nL forward codes, nU backward codes}.
+int la_get_tuning_table(sfem_ctx* ctx, int fp32, int* table, int cap, int* n) {
+  LinAlg* L = ctx->la;
+  const int k = fp32 ? 1 : 0;
+  std::vector<int> t = {L->tune_spmv[k], L->tune_bspmv[k], (int)L->tune_L[k].size(), (int)L->tune_U[k].size()};
+  t.insert(t.end(), L->tune_L[k].begin(), L->tune_L[k].end());
+  t.insert(t.end(), L->tune_U[k].begin(), L->tune_U[k].end());
+  *n = (int)t.size();
+  for (int i = 0; i < (int)t.size() && i < cap; ++i) table[i] = t[i];
+  return SFEM_OK;
+}
+
+int la_set_tuning_table(sfem_ctx* ctx, int fp32, const int* table, int n) {
+  LinAlg* L = ctx->la;
+  const int k = fp32 ? 1 : 0;
+  if (n < 4 || table[2] < 0 || table[3] < 0 || n != 4 + table[2] + table[3])
+    return set_error(SFEM_ERR_ARG, "tuning table: bad length");
+  const bool spmv_ok = (table[0] & 31) < kSpmvCodes && (table[0] >> 5) <= 1 &&
+                       (table[1] & 31) < kBlkCodes && (table[1] >> 5) <= 3;
+  if (!spmv_ok) return set_error(SFEM_ERR_ARG, "tuning table: bad SpMV shape code");
+  const int nl = table[2], nu = table[3];
+  if (nl + nu > 0) {
+    if (!L->ilu) return set_error(SFEM_ERR_STATE, "tuning table: set up the ILU(0) solver first");
+    const IluPlan& P = *L->ilu;
+    const int ntc = P.tiled ? (int)P.tileColPtr.size() - 1 : 0;
+    const int pl = P.tiled ? ntc : (int)P.levLPtr.size() - 1;
+    const int pu = P.tiled ? ntc - (P.tile_fuse_last ? 1 : 0) : (int)P.levUPtr.size() - 1;
+    if (nl != pl || nu != pu) return set_error(SFEM_ERR_ARG, "tuning table: launch counts differ from the ILU(0) plan");
+    for (int i = 4; i < n; ++i) {
+      const int c = table[i];
+      const bool ok = P.tiled ? (c >= 0 && c < kTileCodes) : (c >= 0 && (c & 31) < kBlkCodes && (c >> 5) <= 1);
+      if (!ok) return set_error(SFEM_ERR_ARG, "tuning table: bad sweep shape code");
+    }
+  }
+  L->tune_mode = SFEM_TUNE_FIXED;
+  L->tune_spmv[k] = table[0];
+  L->tuned_csr[k] = true;
+  L->tune_bspmv[k] = table[1];
+  L->tune_L[k].assign(table + 4, table + 4 + nl);
+  L->tune_U[k].assign(table + 4 + nl, table + n);
+  L->tune_plan[k] = L->ilu ? L->plan_id : -1;
+  return SFEM_OK;
+}
+
 // Symbolic set-up of the preconditioner for the given options (ILU(0): ordering,
 // block pattern and level sets), so that the first solve does not pay for it.
 int la_prepare(sfem_ctx* ctx, const sfem_solver_opts* o) {
@@ -2711,10 +2801,16 @@ int la_prepare_solve(sfem_ctx* ctx, const sfem_solver_opts* o) {
   L->nk = L->block_space ? L->ilu->npad : L->n;
   if (fp32) {
     if ((rc = ensure_vec(&L->t32, L->nvec)) || (rc = ensure_vec(&L->v32, L->nvec))) return rc;
-    if (needs_tuning(L, 1) && (rc = autotune<float>(ctx, L->t32, L->v32))) return rc;
+    if (needs_tuning(L, 1)) {
+      if (L->tune_mode == SFEM_TUNE_FIXED) fixed_shapes(L, 1);
+      else if ((rc = autotune<float>(ctx, L->t32, L->v32))) return rc;
+    }
   } else {
     if ((rc = ensure_vec(&L->t, L->nvec)) || (rc = ensure_vec(&L->v, L->nvec))) return rc;
-    if (needs_tuning(L, 0) && (rc = autotune<double>(ctx, L->t, L->v))) return rc;
+    if (needs_tuning(L, 0)) {
+      if (L->tune_mode == SFEM_TUNE_FIXED) fixed_shapes(L, 0);
+      else if ((rc = autotune<double>(ctx, L->t, L->v))) return rc;
+    }
   }
   return SFEM_OK;
 }

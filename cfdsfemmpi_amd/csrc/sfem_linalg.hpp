@@ -67,6 +67,7 @@ struct LinAlg {
   int tune_spmv[2] = {3, 3}, tune_bspmv[2] = {6, 6};
   std::vector<int> tune_L[2], tune_U[2];
   bool tuned_csr[2] = {false, false};
+  int tune_mode = SFEM_TUNE_TIMED;  // SFEM_TUNE_FIXED: default or installed shapes, no timing
   int64_t fallbacks = 0;     // FP32 BiCGStab solves finished by the GMRES fallback
   int64_t partial = 0;       // FP32 solves ended above rtol, best iterate handed to Newton
   int plan_id = 0, tune_plan[2] = {-1, -1};  // ILU plan the per-level shapes were tuned for
@@ -164,6 +165,9 @@ int la_get_ilu(sfem_ctx* ctx, int64_t* nnz_out, int32_t* perm, int32_t* rowptr, 
 int la_tile_info(sfem_ctx* ctx, int64_t* ntiles, int64_t* ncolors, int64_t* tile_vals);
 int la_pc_apply_host(sfem_ctx* ctx, const double* r, double* z);
 int la_tuning_levels(sfem_ctx* ctx, int fp32, int* out, int cap, int* nlevels);
+int la_set_tuning_mode(sfem_ctx* ctx, int mode);
+int la_get_tuning_table(sfem_ctx* ctx, int fp32, int* table, int cap, int* n);
+int la_set_tuning_table(sfem_ctx* ctx, int fp32, const int* table, int n);
 int la_norm2(sfem_ctx* ctx, const double* a, double* out);
 int la_spmv64(sfem_ctx* ctx, const double* x, double* y);
 int la_spmv32(sfem_ctx* ctx, const float* x, float* y);

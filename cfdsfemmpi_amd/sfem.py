@@ -29,6 +29,7 @@ PREC_FP64, PREC_MIXED, PREC_STORE32 = 0, 1, 2
 PRED_PREVIOUS, PRED_EXTRAPOLATE, PRED_LEARNED, PRED_QUADRATIC = 0, 1, 2, 3
 LEARNER_NPARAM = 5190
 KT_ASSEMBLY, KT_SPMV, KT_PC = 0, 1, 2
+TUNE_TIMED, TUNE_FIXED = 0, 1
 KSP = {"bcgs": 0, "gmres": 1}
 PC = {"none": 0, "jacobi": 1, "bjacobi4": 2, "ilu": 3}
 
@@ -90,6 +91,9 @@ ABI = [
     ("sfem_tile_info", C.c_int, [_vp, _i64p, _i64p, _i64p]),
     ("sfem_tuning", C.c_int, [_vp, C.POINTER(C.c_int)]),
     ("sfem_tuning_levels", C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]),
+    ("sfem_set_tuning_mode", C.c_int, [_vp, C.c_int]),
+    ("sfem_get_tuning_table", C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]),
+    ("sfem_set_tuning_table", C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.c_int]),
     ("sfem_info", C.c_int, [_vp, _i64p, _i64p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("sfem_last_timings", C.c_int, [_vp, _f64p, _f64p]),
     ("sfem_learner_create", _vp, [_vp, C.c_int64, C.c_uint64]),
@@ -390,6 +394,23 @@ class Context:
             res[name] = {"spmv_GUX": w[0:3], "ilu_levels_GUX": levels}
         return res
 
+    def set_tuning_mode(self, mode: int):
+        """TUNE_TIMED (default): time every kernel shape on the first solve; TUNE_FIXED:
+        default shapes, so FP32 results repeat exactly from run to run."""
+        _check(self.lib.sfem_set_tuning_mode(self.h, int(mode)), "sfem_set_tuning_mode")
+
+    def get_tuning_table(self, fp32: bool) -> list:
+        n = C.c_int(0)
+        _check(self.lib.sfem_get_tuning_table(self.h, 1 if fp32 else 0, None, 0, C.byref(n)), "sfem_get_tuning_table")
+        buf = (C.c_int * max(n.value, 1))()
+        _check(self.lib.sfem_get_tuning_table(self.h, 1 if fp32 else 0, buf, n.value, C.byref(n)),
+               "sfem_get_tuning_table")
+        return list(buf)[:n.value]
+
+    def set_tuning_table(self, fp32: bool, table):
+        buf = (C.c_int * max(len(table), 1))(*[int(v) for v in table])
+        _check(self.lib.sfem_set_tuning_table(self.h, 1 if fp32 else 0, buf, len(table)), "sfem_set_tuning_table")
+
     def info(self):
         nf, nz = C.c_int64(0), C.c_int64(0)
         nc, nd = C.c_int(0), C.c_int(0)
@@ -487,6 +508,7 @@ class StabFEM:
         self._vtu = None
         self.max_newton = 10       # hard-coded iteration cap (StabFEM2.cpp:654)
         self.precision = PREC_FP64
+        self.tuning = TUNE_TIMED   # kernel shapes of the solver: timed on first use, or fixed
         self.history = []          # (tstep, iter, rhs_norm, krylov_its)
 
     # StabFEM::readInputData (StabFEM.cpp:80-264)
@@ -527,6 +549,7 @@ class StabFEM:
         c = self.ctl
         self.ctx.set_params(c.elemData, c.tis, c.rhoInf, c.dt)
         self.ctx.set_precision(self.precision)
+        self.ctx.set_tuning_mode(self.tuning)
         self.ctx.setup_solver(self.opts)
 
     # StabFEM::solveFullyImplicit (StabFEM2.cpp:593-832)

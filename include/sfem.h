@@ -332,6 +332,28 @@ int sfem_tuning(sfem_ctx* ctx, int* out);
  * counterpart in the reference. */
 int sfem_tuning_levels(sfem_ctx* ctx, int fp32, int* out, int cap, int* nlevels);
 
+/* How the solver picks the kernel shapes of its SpMV and ILU(0) sweeps (lanes per block
+ * row, blocks in flight per lane, workgroup order).  SFEM_TUNE_TIMED (the default): on
+ * the first solve of a precision and ILU(0) plan every shape is timed with HIP events on
+ * the context's stream and the fastest kept, per sweep launch.  SFEM_TUNE_FIXED: the
+ * library's default shapes (or a table installed by sfem_set_tuning_table), no timing.
+ * A shape fixes the summation order of the products, and FP32 Krylov iterates (and
+ * their iteration counts) follow that rounding: mixed-precision results are the same
+ * from run to run and device to device only with fixed shapes.  Setting a mode makes the
+ * next solve of each precision install its shapes again.  No counterpart in the
+ * reference (PETSc's kernels have no shape parameters). */
+enum { SFEM_TUNE_TIMED = 0, SFEM_TUNE_FIXED = 1 };
+int sfem_set_tuning_mode(sfem_ctx* ctx, int mode);
+
+/* The shape table of one precision (fp32 = 0 FP64 solves, 1 FP32 solves) as integer
+ * codes: {CSR SpMV, block SpMV, forward sweep launches nL, backward launches nU, nL
+ * forward codes, nU backward codes}.  get copies it out (*n receives its length, at most
+ * cap entries are written); set installs it for the current ILU(0) plan (the launch
+ * counts must match it) and switches to SFEM_TUNE_FIXED, so that a table recorded by one
+ * run is replayed exactly by another.  No counterpart in the reference. */
+int sfem_get_tuning_table(sfem_ctx* ctx, int fp32, int* table, int cap, int* n);
+int sfem_set_tuning_table(sfem_ctx* ctx, int fp32, const int* table, int n);
+
 /* Sizes and counts of the context. */
 int sfem_info(sfem_ctx* ctx, int64_t* nfree, int64_t* nnz, int* ncolors, int* ndof);
 

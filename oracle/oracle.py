@@ -78,10 +78,23 @@ def lib():
                                C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
     L.orc_gmres.argtypes = [C.c_int64, i64p, i32p, f64p, C.c_void_p, f64p, f64p, C.c_int,
                             C.c_double, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    L.orc_set_threads.argtypes = [C.c_int]
+    L.orc_set_threads.restype = None
+    L.orc_get_threads.argtypes = []
+    L.orc_get_threads.restype = C.c_int
     L.orc_flops_per_element.argtypes = [C.c_int, C.c_int]
     L.orc_flops_per_element.restype = C.c_int64
     _LIB = L
     return L
+
+
+def set_threads(n: int) -> None:
+    """Host threads of the oracle's parallel paths (1 = sequential, the default)."""
+    lib().orc_set_threads(int(n))
+
+
+def get_threads() -> int:
+    return int(lib().orc_get_threads())
 
 
 def time_params(tis: int, rhoInf: float, dt: float) -> np.ndarray:

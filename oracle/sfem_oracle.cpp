@@ -23,6 +23,8 @@
 //                     residual histories of PETSc's KSP are parity unpinned)
 #include "sfem_oracle.h"
 
+#include <omp.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -30,6 +32,17 @@
 #include <vector>
 
 namespace {
+
+// Host threads of the parallel paths (orc_set_threads).  1 (the default) runs every
+// function sequentially in the reference's order: elements in index order, rows of the
+// triangular solves one after another.  With more threads the element loop runs colour
+// class by colour class (no two elements of a class share a node, so the ADD assembly
+// has no conflicting writes), the ILU(0) factorisation and triangular solves run level
+// by level over the dependency levels of their rows, and products and vector updates
+// are split over rows; dot products then sum fixed-size chunks in index order, so the
+// result does not depend on the thread count.
+int g_threads = 1;
+constexpr int64_t kDotChunk = 8192;
 
 // PI as written in headersBasic.h:28 (rounds to the nearest double of pi)
 const double kPI = 3.141592653589793238462643383279502884197169399375105;
@@ -447,6 +460,8 @@ int64_t orc_prepare(int etype, int64_t nElem, const double* coords, const int32_
   const int ndim = ei.ndim, nlbf = ei.nlbf, nGP = ei.nGP;
   double gps[3 * 8], gws[8];
   orc_gauss(etype, gps, gws);
+  int64_t bad = nElem;  // first element with a negative Jacobian
+#pragma omp parallel for num_threads(g_threads) if (g_threads > 1) schedule(static) reduction(min : bad)
   for (int64_t e = 0; e < nElem; ++e) {
     double xN[8], yN[8], zN[8];
     for (int ii = 0; ii < ei.npElem; ++ii) {
@@ -492,7 +507,7 @@ int64_t orc_prepare(int etype, int64_t nElem, const double* coords, const int32_
           dNdz[o + ii] = d1[ii] * Bi[2][0] + d2[ii] * Bi[2][1] + d3[ii] * Bi[2][2];
         }
       }
-      if (Jac < 0.0) return -(e + 1);  // the reference exits (Hexa8Node.cpp:79-84)
+      if (Jac < 0.0) bad = std::min(bad, e);  // the reference exits (Hexa8Node.cpp:79-84)
       double dvol = gws[gp] * Jac;
       dvolout[e * nGP + gp] = dvol;
       elemVol += dvol;
@@ -500,7 +515,7 @@ int64_t orc_prepare(int etype, int64_t nElem, const double* coords, const int32_
     charlen[e] = (ndim == 2) ? std::sqrt(4.0 * std::fabs(elemVol) / kPI)
                              : std::pow(6.0 * std::fabs(elemVol) / kPI, 1.0 / 3.0);
   }
-  return 0;
+  return bad < nElem ? -(bad + 1) : 0;
 }
 
 void orc_element_KF(int etype, int64_t e, const int32_t* conn, const double* N, const double* dNdx,
@@ -559,8 +574,78 @@ int64_t orc_dof_numbering(int64_t nNode, int ndof, int64_t nDBC, const int32_t* 
   return n;
 }
 
+namespace {
+// The pattern of orc_pattern built node by node (threads > 1): the columns of every free
+// row of node i are the free DOFs of all nodes of the elements containing i, in
+// increasing DOF id -- the same set the per-row insertion of the reference collects.
+// Two passes over the nodes (row lengths, then columns), each split over threads.
+int64_t pattern_by_node(int64_t nElem, int npElem, int ndof, const int32_t* conn, const int32_t* nodeDof,
+                        int64_t nfree, int64_t* rowptr, int32_t* colidx) {
+  int64_t nNode = 0;
+  for (int64_t q = 0; q < nElem * npElem; ++q) nNode = std::max<int64_t>(nNode, (int64_t)conn[q] + 1);
+  std::vector<int64_t> ptr(nNode + 1, 0);
+  for (int64_t q = 0; q < nElem * npElem; ++q) ptr[conn[q] + 1]++;
+  for (int64_t i = 0; i < nNode; ++i) ptr[i + 1] += ptr[i];
+  std::vector<int32_t> elems(nElem * npElem);
+  {
+    std::vector<int64_t> pos(ptr.begin(), ptr.end() - 1);
+    for (int64_t e = 0; e < nElem; ++e)
+      for (int a = 0; a < npElem; ++a) elems[pos[conn[e * npElem + a]]++] = (int32_t)e;
+  }
+  std::vector<int32_t> nfreeOf(nNode, 0);
+  for (int64_t i = 0; i < nNode; ++i)
+    for (int d = 0; d < ndof; ++d) nfreeOf[i] += nodeDof[i * ndof + d] >= 0;
+  auto neighbours = [&](int64_t i, std::vector<int32_t>& nb) {
+    nb.clear();
+    for (int64_t q = ptr[i]; q < ptr[i + 1]; ++q)
+      for (int a = 0; a < npElem; ++a) nb.push_back(conn[(int64_t)elems[q] * npElem + a]);
+    std::sort(nb.begin(), nb.end());
+    nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
+  };
+  std::vector<int64_t> rowlen(nNode, 0);
+#pragma omp parallel num_threads(g_threads)
+  {
+    std::vector<int32_t> nb;
+#pragma omp for schedule(dynamic, 1024)
+    for (int64_t i = 0; i < nNode; ++i) {
+      if (!nfreeOf[i]) continue;
+      neighbours(i, nb);
+      int64_t len = 0;
+      for (int32_t j : nb) len += nfreeOf[j];
+      rowlen[i] = len;
+    }
+  }
+  std::vector<int64_t> first(nNode, 0);
+  rowptr[0] = 0;
+  int64_t r = 0;
+  for (int64_t i = 0; i < nNode; ++i) {
+    first[i] = rowptr[r];
+    for (int k = 0; k < nfreeOf[i]; ++k, ++r) rowptr[r + 1] = rowptr[r] + rowlen[i];
+  }
+  if (colidx) {
+#pragma omp parallel num_threads(g_threads)
+    {
+      std::vector<int32_t> nb;
+#pragma omp for schedule(dynamic, 1024)
+      for (int64_t i = 0; i < nNode; ++i) {
+        if (!nfreeOf[i]) continue;
+        neighbours(i, nb);
+        int64_t q = first[i];
+        for (int32_t j : nb)
+          for (int d = 0; d < ndof; ++d)
+            if (nodeDof[(int64_t)j * ndof + d] >= 0) colidx[q++] = nodeDof[(int64_t)j * ndof + d];
+        for (int k = 1; k < nfreeOf[i]; ++k)
+          std::copy(colidx + first[i], colidx + first[i] + rowlen[i], colidx + first[i] + k * rowlen[i]);
+      }
+    }
+  }
+  return rowptr[nfree];
+}
+}  // namespace
+
 int64_t orc_pattern(int64_t nElem, int npElem, int ndof, const int32_t* conn,
                     const int32_t* nodeDof, int64_t nfree, int64_t* rowptr, int32_t* colidx) {
+  if (g_threads > 1) return pattern_by_node(nElem, npElem, ndof, conn, nodeDof, nfree, rowptr, colidx);
   // forAssyMat[r].insert(c) for every free (r, c) pair of every element
   const int nsize = npElem * ndof;
   std::vector<std::vector<int32_t>> rows((size_t)nfree);
@@ -587,6 +672,7 @@ int64_t orc_pattern(int64_t nElem, int npElem, int ndof, const int32_t* conn,
 
 void orc_time_update(int64_t n, double* soln, double* solnPrev, double* solnPrev2, double* solnPrev3,
                      double* solnPrev4, double* solnDot, double* solnDotPrev) {
+#pragma omp parallel for num_threads(g_threads) if (g_threads > 1) schedule(static)
   for (int64_t i = 0; i < n; ++i) {
     solnPrev4[i] = solnPrev3[i];
     solnPrev3[i] = solnPrev2[i];
@@ -600,6 +686,7 @@ void orc_update_iter_step(int64_t n, const double* td, const double* soln, const
                           const double* solnPrev2, const double* solnPrev3, const double* solnPrev4,
                           double* solnDot, const double* solnDotPrev, double* solnCur,
                           double* solnDotCur) {
+#pragma omp parallel for num_threads(g_threads) if (g_threads > 1) schedule(static)
   for (int64_t i = 0; i < n; ++i) {
     solnDot[i] = td[9] * soln[i] + td[10] * solnPrev[i] + td[11] * solnPrev2[i] +
                  td[12] * solnPrev3[i] + td[13] * solnPrev4[i] + td[15] * solnDotPrev[i];
@@ -620,13 +707,10 @@ void orc_assemble(int etype, int64_t e0, int64_t e1, int iter, const int32_t* co
     if (vals) std::memset(vals, 0, sizeof(double) * (size_t)rowptr[nfree]);
     std::memset(rhs, 0, sizeof(double) * (size_t)nfree);
   }
-  std::vector<double> K(nsize * nsize), F(nsize);
-  std::vector<int32_t> fav(nsize);
-  std::vector<int64_t> gdof(nsize);
   const bool wantK = (vals != nullptr) || iter == 0;
-  for (int64_t e = e0; e < e1; ++e) {
+  auto element = [&](int64_t e, double* K, double* F, int32_t* fav, int64_t* gdof) {
     element_KF<double>(etype, e, conn, N, dNdx, dNdy, dNdz, dvol, charlen, elemData, td, solnPrev,
-                       solnCur, solnDotCur, K.data(), F.data(), wantK);
+                       solnCur, solnDotCur, K, F, wantK);
     for (int i = 0; i < ei.npElem; ++i)
       for (int d = 0; d < ndof; ++d) {
         int64_t n = conn[e * ei.npElem + i];
@@ -658,11 +742,53 @@ void orc_assemble(int etype, int64_t e0, int64_t e1, int iter, const int32_t* co
         vals[p - colidx] += K[ii * nsize + jj];
       }
     }
+  };
+  if (g_threads <= 1) {
+    std::vector<double> K(nsize * nsize), F(nsize);
+    std::vector<int32_t> fav(nsize);
+    std::vector<int64_t> gdof(nsize);
+    for (int64_t e = e0; e < e1; ++e) element(e, K.data(), F.data(), fav.data(), gdof.data());
+    return;
+  }
+  // greedy element colouring in index order: an element takes the lowest colour none of
+  // its nodes has yet
+  const int npe = ei.npElem;
+  int64_t nNode = 0;
+  for (int64_t q = e0 * npe; q < e1 * npe; ++q) nNode = std::max<int64_t>(nNode, (int64_t)conn[q] + 1);
+  std::vector<uint64_t> used(nNode, 0);
+  std::vector<std::vector<int64_t>> cls;
+  std::vector<int64_t> rest;  // elements for which all 64 colours are taken: run one by one
+  for (int64_t e = e0; e < e1; ++e) {
+    uint64_t u = 0;
+    for (int a = 0; a < npe; ++a) u |= used[conn[e * npe + a]];
+    if (u == ~0ull) { rest.push_back(e); continue; }
+    int col = 0;
+    while ((u >> col) & 1ull) ++col;
+    if ((int)cls.size() <= col) cls.resize(col + 1);
+    cls[col].push_back(e);
+    for (int a = 0; a < npe; ++a) used[conn[e * npe + a]] |= 1ull << col;
+  }
+  if (!rest.empty()) {
+    std::vector<double> K(nsize * nsize), F(nsize);
+    std::vector<int32_t> fav(nsize);
+    std::vector<int64_t> gdof(nsize);
+    for (int64_t e : rest) element(e, K.data(), F.data(), fav.data(), gdof.data());
+  }
+  for (const auto& c : cls) {
+#pragma omp parallel num_threads(g_threads)
+    {
+      std::vector<double> K(nsize * nsize), F(nsize);
+      std::vector<int32_t> fav(nsize);
+      std::vector<int64_t> gdof(nsize);
+#pragma omp for schedule(dynamic, 256)
+      for (int64_t k = 0; k < (int64_t)c.size(); ++k) element(c[k], K.data(), F.data(), fav.data(), gdof.data());
+    }
   }
 }
 
 void orc_spmv(int64_t n, const int64_t* rowptr, const int32_t* colidx, const double* vals,
               const double* x, double* y) {
+#pragma omp parallel for num_threads(g_threads) if (g_threads > 1) schedule(static)
   for (int64_t r = 0; r < n; ++r) {
     double s = 0.0;
     for (int64_t k = rowptr[r]; k < rowptr[r + 1]; ++k) s += vals[k] * x[colidx[k]];
@@ -680,6 +806,10 @@ struct orc_pc {
   std::vector<int32_t> colidx;
   std::vector<double> lu;
   std::vector<int64_t> diag;
+  // threads > 1: rows grouped by dependency level of the forward (lower) and the
+  // backward (upper) solve, levPtr[l] .. levPtr[l+1] in levRows
+  std::vector<int64_t> levLPtr, levUPtr;
+  std::vector<int32_t> levL, levU;
 };
 
 namespace {
@@ -751,14 +881,14 @@ orc_pc* orc_pc_create(int type, int64_t n, const int64_t* rowptr, const int32_t*
     }
     if (*status == 0) {
       // IKJ ILU(0) (Saad Alg. 10.4) with a dense work index per row
-      std::vector<int64_t> iw(n, -1);
       double* a = pc->lu.data();
-      for (int64_t i = 0; i < n; ++i) {
+      auto factor_row = [&](int64_t i, std::vector<int64_t>& iw) -> int {
+        int st = 0;
         for (int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k) iw[colidx[k]] = k;
         for (int64_t k = rowptr[i]; k < pc->diag[i]; ++k) {
           const int32_t j = colidx[k];
           double piv = a[pc->diag[j]];
-          if (piv == 0.0) { *status = -2; break; }
+          if (piv == 0.0) { st = -2; break; }
           a[k] /= piv;
           const double lij = a[k];
           for (int64_t q = pc->diag[j] + 1; q < rowptr[j + 1]; ++q) {
@@ -767,7 +897,51 @@ orc_pc* orc_pc_create(int type, int64_t n, const int64_t* rowptr, const int32_t*
           }
         }
         for (int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k) iw[colidx[k]] = -1;
-        if (a[pc->diag[i]] == 0.0) *status = -2;
+        if (a[pc->diag[i]] == 0.0) st = -2;
+        return st;
+      };
+      if (g_threads <= 1) {
+        std::vector<int64_t> iw(n, -1);
+        for (int64_t i = 0; i < n; ++i)
+          if (int st = factor_row(i, iw)) *status = st;
+      } else {
+        // dependency levels: row i after every row j < i of its lower part (forward
+        // solve and factorisation), after every row j > i of its upper part (backward)
+        std::vector<int32_t> lv(n, 0);
+        int nlL = 0, nlU = 0;
+        for (int64_t i = 0; i < n; ++i) {
+          int l = 0;
+          for (int64_t k = rowptr[i]; k < pc->diag[i]; ++k) l = std::max(l, lv[colidx[k]] + 1);
+          lv[i] = l;
+          nlL = std::max(nlL, l + 1);
+        }
+        auto bucket = [&](int nl, std::vector<int64_t>& ptr, std::vector<int32_t>& rows) {
+          ptr.assign(nl + 1, 0);
+          for (int64_t i = 0; i < n; ++i) ptr[lv[i] + 1]++;
+          for (int l = 0; l < nl; ++l) ptr[l + 1] += ptr[l];
+          rows.resize(n);
+          std::vector<int64_t> pos(ptr.begin(), ptr.end() - 1);
+          for (int64_t i = 0; i < n; ++i) rows[pos[lv[i]]++] = (int32_t)i;
+        };
+        bucket(nlL, pc->levLPtr, pc->levL);
+        for (int64_t i = n - 1; i >= 0; --i) {
+          int l = 0;
+          for (int64_t k = pc->diag[i] + 1; k < rowptr[i + 1]; ++k) l = std::max(l, lv[colidx[k]] + 1);
+          lv[i] = l;  // rows j > i already hold their upper level
+          nlU = std::max(nlU, l + 1);
+        }
+        bucket(nlU, pc->levUPtr, pc->levU);
+        int st_all = 0;
+#pragma omp parallel num_threads(g_threads) reduction(min : st_all)
+        {
+          std::vector<int64_t> iw(n, -1);
+          for (int l = 0; l < nlL; ++l) {
+#pragma omp for schedule(dynamic, 64)
+            for (int64_t q = pc->levLPtr[l]; q < pc->levLPtr[l + 1]; ++q)
+              st_all = std::min(st_all, factor_row(pc->levL[q], iw));
+          }
+        }
+        if (st_all) *status = st_all;
       }
     }
   }
@@ -794,15 +968,32 @@ void orc_pc_apply(const orc_pc* pc, const double* r, double* z) {
     const int64_t* rp = pc->rowptr.data();
     const int32_t* ci = pc->colidx.data();
     const double* a = pc->lu.data();
-    for (int64_t i = 0; i < n; ++i) {  // L y = r, unit lower
+    auto lower = [&](int64_t i) {  // L y = r, unit lower
       double s = r[i];
       for (int64_t k = rp[i]; k < pc->diag[i]; ++k) s -= a[k] * z[ci[k]];
       z[i] = s;
-    }
-    for (int64_t i = n - 1; i >= 0; --i) {  // U z = y
+    };
+    auto upper = [&](int64_t i) {  // U z = y
       double s = z[i];
       for (int64_t k = pc->diag[i] + 1; k < rp[i + 1]; ++k) s -= a[k] * z[ci[k]];
       z[i] = s / a[pc->diag[i]];
+    };
+    if (pc->levLPtr.empty()) {
+      for (int64_t i = 0; i < n; ++i) lower(i);
+      for (int64_t i = n - 1; i >= 0; --i) upper(i);
+    } else {
+      const int nlL = (int)pc->levLPtr.size() - 1, nlU = (int)pc->levUPtr.size() - 1;
+#pragma omp parallel num_threads(g_threads)
+      {
+        for (int l = 0; l < nlL; ++l) {
+#pragma omp for schedule(static)
+          for (int64_t q = pc->levLPtr[l]; q < pc->levLPtr[l + 1]; ++q) lower(pc->levL[q]);
+        }
+        for (int l = 0; l < nlU; ++l) {
+#pragma omp for schedule(static)
+          for (int64_t q = pc->levUPtr[l]; q < pc->levUPtr[l + 1]; ++q) upper(pc->levU[q]);
+        }
+      }
     }
   } else {
     std::memcpy(z, r, sizeof(double) * n);
@@ -819,17 +1010,45 @@ int64_t orc_pc_lu(const orc_pc* pc, double* out) {
 
 namespace {
 double dot(int64_t n, const double* a, const double* b) {
+  if (g_threads <= 1) {
+    double s = 0.0;
+    for (int64_t i = 0; i < n; ++i) s += a[i] * b[i];
+    return s;
+  }
+  const int64_t nc = (n + kDotChunk - 1) / kDotChunk;
+  std::vector<double> part(nc);
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+  for (int64_t c = 0; c < nc; ++c) {
+    double s = 0.0;
+    const int64_t hi = std::min(n, (c + 1) * kDotChunk);
+    for (int64_t i = c * kDotChunk; i < hi; ++i) s += a[i] * b[i];
+    part[c] = s;
+  }
   double s = 0.0;
-  for (int64_t i = 0; i < n; ++i) s += a[i] * b[i];
+  for (int64_t c = 0; c < nc; ++c) s += part[c];
   return s;
 }
+
 }  // namespace
+extern "C++" {
+namespace {
+// f(i) for i in [0, n), split over the threads when there are several
+template <typename F>
+void for_rows(int64_t n, F f) {
+#pragma omp parallel for num_threads(g_threads) if (g_threads > 1) schedule(static)
+  for (int64_t i = 0; i < n; ++i) f(i);
+}
+}  // namespace
+}
+
+void orc_set_threads(int n) { g_threads = std::max(1, n); }
+int orc_get_threads(void) { return g_threads; }
 
 int orc_bicgstab(int64_t n, const int64_t* rowptr, const int32_t* colidx, const double* vals,
                  const orc_pc* pc, const double* b, double* x, double rtol, int maxit, int* iters,
                  double* relres) {
   std::vector<double> r(n), rh(n), p(n, 0.0), v(n, 0.0), s(n), t(n), ph(n), sh(n);
-  for (int64_t i = 0; i < n; ++i) { x[i] = 0.0; r[i] = b[i]; rh[i] = b[i]; }
+  for_rows(n, [&](int64_t i) { x[i] = 0.0; r[i] = b[i]; rh[i] = b[i]; });
   const double bnorm = std::sqrt(dot(n, b, b));
   *iters = 0;
   if (bnorm == 0.0) { *relres = 0.0; return 0; }
@@ -839,30 +1058,30 @@ int orc_bicgstab(int64_t n, const int64_t* rowptr, const int32_t* colidx, const 
     const double rho1 = dot(n, rh.data(), r.data());
     if (rho1 == 0.0) { *iters = it - 1; *relres = rnorm / bnorm; return 2; }
     if (it == 1) {
-      for (int64_t i = 0; i < n; ++i) p[i] = r[i];
+      for_rows(n, [&](int64_t i) { p[i] = r[i]; });
     } else {
       const double beta = (rho1 / rho) * (alpha / omega);
-      for (int64_t i = 0; i < n; ++i) p[i] = r[i] + beta * (p[i] - omega * v[i]);
+      for_rows(n, [&](int64_t i) { p[i] = r[i] + beta * (p[i] - omega * v[i]); });
     }
     orc_pc_apply(pc, p.data(), ph.data());
     orc_spmv(n, rowptr, colidx, vals, ph.data(), v.data());
     const double rv = dot(n, rh.data(), v.data());
     if (rv == 0.0) { *iters = it - 1; *relres = rnorm / bnorm; return 2; }
     alpha = rho1 / rv;
-    for (int64_t i = 0; i < n; ++i) s[i] = r[i] - alpha * v[i];
+    for_rows(n, [&](int64_t i) { s[i] = r[i] - alpha * v[i]; });
     const double snorm = std::sqrt(dot(n, s.data(), s.data()));
     if (snorm <= rtol * bnorm) {
-      for (int64_t i = 0; i < n; ++i) x[i] += alpha * ph[i];
+      for_rows(n, [&](int64_t i) { x[i] += alpha * ph[i]; });
       *iters = it; *relres = snorm / bnorm; return 0;
     }
     orc_pc_apply(pc, s.data(), sh.data());
     orc_spmv(n, rowptr, colidx, vals, sh.data(), t.data());
     const double tt = dot(n, t.data(), t.data());
     omega = (tt == 0.0) ? 0.0 : dot(n, t.data(), s.data()) / tt;
-    for (int64_t i = 0; i < n; ++i) {
+    for_rows(n, [&](int64_t i) {
       x[i] += alpha * ph[i] + omega * sh[i];
       r[i] = s[i] - omega * t[i];
-    }
+    });
     rnorm = std::sqrt(dot(n, r.data(), r.data()));
     rho = rho1;
     *iters = it;

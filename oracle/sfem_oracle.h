@@ -122,6 +122,14 @@ int orc_gmres(int64_t n, const int64_t* rowptr, const int32_t* colidx, const dou
               const orc_pc* pc, const double* b, double* x, int restart, double rtol, int maxit,
               int* iters, double* relres);
 
+/* Host threads of the parallel paths (default 1: everything sequential, in the
+ * reference's element and row order).  With n > 1: elements run colour class by colour
+ * class, the ILU(0) factorisation and triangular solves level by level over the
+ * dependency levels of their rows, the pattern node by node, products and vector
+ * updates split over rows, dot products as fixed chunks summed in index order. */
+void orc_set_threads(int n);
+int orc_get_threads(void);
+
 /* Count of floating-point operations executed by orc_element_KF for one
  * element with K and F (want_K=1) or F only (want_K=0), counted on the
  * reference's arithmetic as written. */

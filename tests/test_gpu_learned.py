@@ -24,13 +24,13 @@ def _tiny_cylinder(perturb=0.01, seed=1):
                                     perturb=perturb, seed=seed)
 
 
-def _ctx(mesh, mu=0.01, reorder=1):
+def _ctx(mesh, mu=0.01, reorder=1, rtol=1e-10):
     ctl = deck.Control(np.array([1.0, mu, 0, 0, 0, 0, 0, 0, 1.0, 1.0, 0.0]), 2, 0.0, 0.1, 1e9, 100, 1, 1e-8)
     fem = sfem.StabFEM(device=0)
     fem.setMesh(mesh)
     fem.setControl(ctl)
     fem.prepareInputData()
-    opts = sfem.solver_opts("bcgs", "ilu", 1e-10, 20000, reorder=reorder)
+    opts = sfem.solver_opts("bcgs", "ilu", rtol, 20000, reorder=reorder)
     fem.setSolver(opts)
     return fem, fem.ctx, opts
 
@@ -274,9 +274,11 @@ def test_adam_step_on_the_hashed_minibatch():
 def test_learned_start_never_changes_the_answer():
     """Train briefly on a short trajectory, then rerun it with the learned predictor:
     each step starts from the smaller-residual candidate, and every converged state
-    agrees with the extrapolated run's to the Newton tolerance's effect."""
+    agrees with the extrapolated run's within 1e-8 of the state's largest entry.  Both
+    runs iterate Newton to ||F|| < 1e-12 (Krylov rtol 1e-10 per solve), so the difference
+    left is that of two solutions of the same discrete system at that residual level."""
     mesh = _tiny_cylinder()
-    nsteps, tol = 14, 1e-10
+    nsteps, tol = 14, 1e-12
     fem, ctx, opts = _ctx(mesh)
     ctx.set_predictor(sfem.PRED_EXTRAPOLATE)
     L = sfem.Learner(ctx, nsteps + 1, seed=21)
@@ -305,7 +307,7 @@ def test_learned_start_never_changes_the_answer():
         assert st.rhs_norm0 == pytest.approx(min(nrm), rel=1e-6)
         sol = ctx2.get_solution()
         scale = np.max(np.abs(ref[s]))
-        assert np.max(np.abs(sol - ref[s])) <= 1e-5 * scale, (s, np.max(np.abs(sol - ref[s])) / scale)
+        assert np.max(np.abs(sol - ref[s])) <= 1e-8 * scale, (s, np.max(np.abs(sol - ref[s])) / scale)
         t += 0.1
     cnt, _ = ctx2.predictor_stats()
     assert cnt.sum() == nsteps

@@ -176,7 +176,70 @@ def test_ldc2dquad_mesh2_reference(golden):
     assert err < 2e-8
 
 
-def _cavity3d(n, steps, prec=sfem.PREC_FP64, opts=None, newton_tol=1e-10):
+def test_c1_config_vs_oracle():
+    """The 2D configuration as stated (10 x 10 Quad4, Re 100, SUPG/PSPG, p pinned at
+    (0, 0), generalised-alpha rho_inf = 0, dt 0.1, 50 steps, Newton 1e-10, ILU-GMRES):
+    every step's field within 1e-8 of the oracle's direct-solve run (StabFEM2.cpp:593-832)."""
+    mesh = meshgen.ldc2d_quad(10, p_pin=(0.0, 0.0))
+    ctl = deck.Control(np.array([1.0, 0.01, 0, 0, 0, 0, 0, 0, 1.0, 1.0, 0.0]), 2, 0.0, 0.1, 1e9, 50, 1, 1e-6)
+    fem = sfem.StabFEM()
+    fem.setMesh(mesh)
+    fem.setControl(ctl)
+    fem.newton_tol = 1e-10
+    fem.prepareInputData()
+    fem.setSolver(sfem.solver_opts("gmres", "ilu", 1e-13, 5000, 30, reorder=1))
+    assert (fem.ctx.nfree, fem.ctx.nnz) == (282, 6366)
+    o = oracle.OracleFEM(mesh, ctl)
+    o.prepare()
+    worst = 0.0
+    timeNow = ctl.dt
+    for step in range(50):
+        s = _advance_one(fem, timeNow)
+        so = _oracle_one(o, timeNow)
+        worst = max(worst, rel_err(s, so))
+        timeNow += ctl.dt
+    print(f"[c1] 50 steps: worst max-norm relative difference {worst:.2e}; final |u| max "
+          f"{np.max(np.abs(s.reshape(-1, 3)[:, :2])):.3f}", flush=True)
+    assert worst <= 1e-8
+
+
+def _advance_one(fem, timeNow):
+    """One more time step of a StabFEM run (the body of solveFullyImplicit's loop)."""
+    ctx = fem.ctx
+    ctx.time_update(min(timeNow, 1.0))
+    for it in range(fem.max_newton):
+        nrm = ctx.assemble(it, want_K=(it == 0))
+        if nrm < fem.newton_tol:
+            break
+        ctx.solve(fem.opts)
+        ctx.update_solution()
+    return ctx.get_solution()
+
+
+def _oracle_one(o, timeNow, newton_tol=1e-10):
+    """One more time step of an OracleFEM run (solve_fully_implicit's loop body)."""
+    c, m = o.ctl, o.mesh
+    o.td = oracle.time_params(c.tis, c.rhoInf, c.dt)
+    o.time_update()
+    o.assign_boundary_conditions(min(timeNow, 1.0))
+    for it in range(10):
+        o.update_iter_step()
+        o.assemble(it)
+        if it == 0:
+            for k in range(m.nDBC):
+                jj = int(m.dbc_node[k]) * o.ndof + int(m.dbc_dof[k])
+                o.soln[jj] += o.solnApplied[jj]
+        if np.linalg.norm(o.rhs) < newton_tol:
+            break
+        du, _, _ = o.linear_solve()
+        o.soln[o.assyForSoln] += du
+    return o.soln.copy()
+
+
+def _cavity3d(n, steps, prec=sfem.PREC_FP64, opts=None, newton_tol=1e-10, tuning=sfem.TUNE_FIXED):
+    """Graded Hexa8 cavity at Re 1000 (tis 2, dt 0.1).  The kernel shapes are fixed by
+    default: the shapes set the FP32 summation order, so a mixed-precision run repeats
+    exactly only with fixed shapes."""
     mesh = meshgen.ldc3d_hex(n, graded=True)
     ctl = deck.Control(np.array([1.0, 1e-3, 0, 0, 0, 0, 0, 0, 1, 1, 0]), 2, 0.0, 0.1, 1e9, steps, 1, 1e-6)
     fem = sfem.StabFEM()
@@ -184,9 +247,62 @@ def _cavity3d(n, steps, prec=sfem.PREC_FP64, opts=None, newton_tol=1e-10):
     fem.setControl(ctl)
     fem.precision = prec
     fem.newton_tol = newton_tol
+    fem.tuning = tuning
     fem.prepareInputData()
     fem.setSolver(opts or sfem.solver_opts("bcgs", "ilu", 1e-12, 2000))
     return fem, mesh, ctl
+
+
+def test_c2_config_vs_oracle():
+    """The 3D configuration's mesh and physics (32^3 cosine-graded Hexa8, 35,937 nodes,
+    Re 1000, lid u = 1 ramped by min(t, 1), p pinned at (0, 0, 0), generalised-alpha
+    rho_inf = 0, dt 0.1) over its 20 steps, BiCGStab + tiled ILU(0): the field after 10
+    and after 20 steps within 1e-8 of the oracle (its own BiCGStab + natural-order
+    ILU(0), on the host's cores).  Both sides solve each Newton system to rtol 1e-11 and
+    iterate Newton to ||F|| < 1e-10 (StabFEM2.cpp:593-832), so that the comparison is
+    not of two solver stopping points: at the configuration's own rtol of 1e-8 two
+    correct runs differ by up to about 1e-6 through their Krylov rounding alone."""
+    n, steps = 32, 20
+    fem, mesh, ctl = _cavity3d(n, steps, opts=sfem.solver_opts("bcgs", "ilu", 1e-11, 5000, reorder=3))
+    assert (fem.ctx.nfree, fem.ctx.nnz) == (125309, 12520933)
+    threads0 = oracle.get_threads()
+    oracle.set_threads(min(16, os.cpu_count() or 1))
+    try:
+        o = oracle.OracleFEM(mesh, ctl, deck.SolverOptions("bcgs", "ilu", 1e-11, 5000))
+        o.prepare()
+        errs = []
+        for half in range(2):
+            s = fem.solveFullyImplicit(steps=10) if half == 0 else _advance_steps(fem, 10, 10)
+            so = o.solve_fully_implicit(newton_tol=1e-10, steps=10) if half == 0 else _oracle_steps(o, 10, 10)
+            errs.append(rel_err(s, so))
+            print(f"[c2] after {10 * (half + 1)} steps: max-norm relative difference {errs[-1]:.2e}", flush=True)
+    finally:
+        oracle.set_threads(threads0)
+    assert max(errs) <= 1e-8
+
+
+def _advance_steps(fem, first, count):
+    """Steps first+1 .. first+count of a StabFEM run (timeNow accumulated by += dt, as
+    StabFEM2.cpp:819 does, so the ramp sees the same rounding)."""
+    timeNow = fem.ctl.dt
+    for _ in range(first):
+        timeNow += fem.ctl.dt
+    s = None
+    for _ in range(count):
+        s = _advance_one(fem, timeNow)
+        timeNow += fem.ctl.dt
+    return s
+
+
+def _oracle_steps(o, first, count, newton_tol=1e-10):
+    timeNow = o.ctl.dt
+    for _ in range(first):
+        timeNow += o.ctl.dt
+    s = None
+    for _ in range(count):
+        s = _oracle_one(o, timeNow, newton_tol)
+        timeNow += o.ctl.dt
+    return s
 
 
 @pytest.mark.parametrize("reorder", [0, 3])
@@ -341,6 +457,7 @@ def test_mixed_bicgstab_stall_falls_back_to_gmres():
     ed = np.array([1.0, 1e-3, 0, 0, 0, 0, 0, 0, 1, 1, 0])
     ctx = make_ctx(mesh, tis=2, dt=0.1, elemData=ed)
     ctx.set_precision(sfem.PREC_MIXED)
+    ctx.set_tuning_mode(sfem.TUNE_FIXED)
     ctx.time_update(0.1)
     ctx.assemble(0)
     vals, rhs = ctx.get_system()
@@ -361,11 +478,16 @@ def test_mixed_bicgstab_stall_falls_back_to_gmres():
 
 
 def test_mixed_precision_close_to_fp64():
-    """FP32 tangent + FP32 Krylov inside an FP64 residual loop stays within 1e-6 of FP64."""
-    f64, mesh, ctl = _cavity3d(6, 3, newton_tol=1e-10)
+    """FP32 tangent + FP32 Krylov inside an FP64 residual loop converges to the FP64
+    answer.  Both runs are driven to the same residual level, ||F|| < 1e-12 (the
+    reference's absolute Newton test, StabFEM2.cpp:740-749, at a tighter tolerance), so
+    that the comparison measures the mixed path and not where each run stopped: the
+    difference is A^-1 (r_mixed - r_fp64).  Velocity within 1e-8; pressure, the weakly
+    observed field of this PSPG system, within 1e-6."""
+    f64, mesh, ctl = _cavity3d(6, 3, newton_tol=1e-12)
     s64 = f64.solveFullyImplicit()
     fmx, _, _ = _cavity3d(6, 3, prec=sfem.PREC_MIXED, opts=sfem.solver_opts("bcgs", "ilu", 1e-4, 2000, reorder=2),
-                          newton_tol=1e-10)
+                          newton_tol=1e-12)
     smx = fmx.solveFullyImplicit()
     nd = 4
     u64 = s64.reshape(-1, nd)[:, :3]
@@ -375,7 +497,131 @@ def test_mixed_precision_close_to_fp64():
     eu = np.linalg.norm(umx - u64) / np.linalg.norm(u64)
     ep = np.linalg.norm(pmx - p64) / np.linalg.norm(p64)
     print(f"[mixed] rel L2 u {eu:.2e} p {ep:.2e}; history {fmx.history}", flush=True)
-    assert eu <= 1e-6 and ep <= 1e-6
+    assert all(h[2] < 1e-12 for h in fmx.history if h[3] == 0)
+    assert eu <= 1e-8 and ep <= 1e-6
+
+
+def test_mixed_precision_repeats_exactly_with_fixed_shapes():
+    """With fixed kernel shapes the FP32 summation order is fixed, so two mixed runs of
+    the same case give bit-identical answers and iteration counts."""
+    out = []
+    for _ in range(2):
+        fmx, _, _ = _cavity3d(6, 2, prec=sfem.PREC_MIXED,
+                              opts=sfem.solver_opts("bcgs", "ilu", 1e-4, 2000, reorder=3), newton_tol=1e-10)
+        out.append((fmx.solveFullyImplicit(), list(fmx.history)))
+        fmx.ctx.close()
+    print(f"[repeat] histories {out[0][1]} / {out[1][1]}", flush=True)
+    assert out[0][1] == out[1][1]
+    assert np.array_equal(out[0][0], out[1][0])
+
+
+@pytest.mark.parametrize("name", list(ETYPES))
+def test_element_KF_fp32_tangent_parity(name):
+    """The FP32 tangent of the mixed path (Ke in FP32 arithmetic from FP64 inputs, Fe in
+    FP64) against the oracle's FP64 element (LagrangeElem*Node::calcStiffnessAndResidual):
+    Ke within 2e-6 of the largest entry, Fe within 1e-12."""
+    mesh = small_mesh(name)
+    print(f"[case] element_KF fp32 tangent {name} nElem={mesh.nElem}", flush=True)
+    et = ETYPES[name]
+    ctx = make_ctx(mesh, tis=2)
+    ctx.set_precision(sfem.PREC_MIXED)
+    n = mesh.nNode * (mesh.ndim + 1)
+    rng = np.random.default_rng(12)
+    prev, cur, dcur = (rng.uniform(-1, 1, n) for _ in range(3))
+    ctx.set_state(solnPrev=prev, solnCur=cur, solnDotCur=dcur)
+    prep = oracle.prepare(et, mesh.coords, mesh.conn)
+    td = oracle.time_params(2, 0.0, 0.1)
+    worst = 0.0
+    for e in range(mesh.nElem):
+        Kg, Fg = ctx.element_KF(e)
+        Ko, Fo = oracle.element_KF(et, e, mesh.conn, prep, ELEM_DATA, td, prev, cur, dcur)
+        assert np.array_equal(Kg, Kg.astype(np.float32).astype(np.float64))  # FP32 values
+        worst = max(worst, rel_err(Kg, Ko))
+        assert rel_err(Fg, Fo) <= 1e-12, (name, e, rel_err(Fg, Fo))
+    print(f"[fp32-tangent] {name}: worst Ke max-norm relative difference {worst:.2e}", flush=True)
+    assert worst <= 2e-6
+    ctx.close()
+
+
+@pytest.mark.parametrize("name", list(ETYPES))
+def test_assembled_fp32_system_parity(name):
+    """Iteration-0 assembly of the mixed path (the production element loop with the FP32
+    tangent: the pipelined Hexa8 kernel for hexahedra) against the oracle's FP64 assembly
+    with the BC lift: CSR values within 2e-6 of the largest, and the FP64 residual -- which
+    the mixed path evaluates after adding the prescribed values, so that no lift is needed
+    -- within 1e-11 of the oracle's lifted residual (the residual is affine in the state,
+    F(s + d) = F(s) - K d)."""
+    mesh = small_mesh(name)
+    print(f"[case] fp32 assembly {name}", flush=True)
+    tis, dt, tf = 2, 0.1, 0.3
+    n = mesh.nNode * (mesh.ndim + 1)
+    rng = np.random.default_rng(6)
+    S0, P0, D0 = (rng.uniform(-1, 1, n) for _ in range(3))
+    f = _oracle_fem(mesh, tis, dt)
+    f.soln[:] = S0; f.solnPrev[:] = P0; f.solnDot[:] = D0
+    f.td = oracle.time_params(tis, 0.0, dt)
+    f.time_update()
+    f.assign_boundary_conditions(tf)
+    f.update_iter_step()
+    f.assemble(0)
+    for k in range(mesh.nDBC):
+        j = int(mesh.dbc_node[k]) * (mesh.ndim + 1) + int(mesh.dbc_dof[k])
+        f.soln[j] += f.solnApplied[j]
+    ctx = make_ctx(mesh, tis=tis, dt=dt)
+    ctx.set_precision(sfem.PREC_MIXED)
+    ctx.set_state(soln=S0, solnPrev=P0, solnDot=D0)
+    ctx.time_update(tf)
+    nrm = ctx.assemble(0)
+    vals, rhs = ctx.get_system()
+    ev, er = rel_err(vals, f.vals), rel_err(rhs, f.rhs)
+    print(f"[fp32-assembly] {name}: values {ev:.2e} rhs {er:.2e}", flush=True)
+    assert np.array_equal(vals, vals.astype(np.float32).astype(np.float64))
+    assert ev <= 2e-6
+    assert er <= 1e-11
+    assert abs(nrm - np.linalg.norm(f.rhs)) <= 1e-11 * np.linalg.norm(f.rhs)
+    assert rel_err(ctx.get_solution(), f.soln) <= 1e-15
+    ctx.close()
+
+
+@pytest.mark.parametrize("reorder", [0, 3])
+def test_store32_lagged_preconditioner(reorder):
+    """FP32-stored ILU(0) with a lagged preconditioner (rebuilt on every second tangent):
+    the solve on the second tangent keeps the first tangent's FP32-stored factor (exported
+    values are FP32 and unchanged), its sweeps apply that factor (pc_apply against forward
+    and backward substitution with the export), and it still converges as a defect
+    correction."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    mesh = meshgen.ldc3d_hex(6, graded=True)
+    ed = np.array([1.0, 1e-2, 0, 0, 0, 0, 0, 0, 1, 1, 0])
+    ctx = make_ctx(mesh, tis=2, dt=0.1, elemData=ed)
+    ctx.set_precision(sfem.PREC_STORE32)
+    ctx.set_pc_lag(2)
+    o = sfem.solver_opts("bcgs", "ilu", 1e-4, 3000, reorder=reorder)
+    st = ctx.step(0.1, o, 1e-10, 10)
+    assert st.rhs_norm < 1e-10
+    perm0, prp0, pci0, lu0 = ctx.get_ilu()
+    st = ctx.step(0.2, o, 1e-10, 10)  # second tangent: lagged factor
+    print(f"[store32-lag] reorder={reorder}: newton {st.newton_iters} krylov {st.krylov_iters} "
+          f"|F| {st.rhs_norm:.2e}", flush=True)
+    assert st.rhs_norm < 1e-10
+    perm, prp, pci, lu = ctx.get_ilu()
+    assert np.array_equal(lu, lu.astype(np.float32).astype(np.float64))
+    assert np.array_equal(lu, lu0) and np.array_equal(perm, perm0)
+    n = ctx.nfree
+    F = sp.csr_matrix((lu, pci, prp), shape=(n, n))
+    Lm = (sp.tril(F, -1) + sp.identity(n)).tocsr()
+    Um = sp.triu(F).tocsr()
+    r = np.random.default_rng(8).standard_normal(n)
+    zp = spla.spsolve_triangular(Um, spla.spsolve_triangular(Lm, r[perm], lower=True), lower=False)
+    z_ref = np.empty(n)
+    z_ref[perm] = zp
+    err = np.linalg.norm(ctx.pc_apply(r) - z_ref) / np.linalg.norm(z_ref)
+    print(f"[store32-lag] pc_apply vs exported factor {err:.2e}", flush=True)
+    # level sweeps substitute with the FP32 values in FP64 arithmetic; tiled sweeps apply
+    # the tiles' inverse triangles, computed in FP64 and stored in FP32 beside FP32 factors
+    assert err <= (1e-12 if reorder == 0 else 1e-4)
+    ctx.close()
 
 
 def test_rcm_ilu_unstructured_tria():
@@ -439,6 +685,7 @@ def test_store32_close_to_fp64(bf16):
     for prec, rtol in ((sfem.PREC_FP64, 1e-10), (sfem.PREC_STORE32, 1e-4)):
         ctx = make_ctx(mesh, tis=2, dt=0.05, elemData=ed)
         ctx.set_precision(prec)
+        ctx.set_tuning_mode(sfem.TUNE_FIXED)
         o = sfem.solver_opts("bcgs", "ilu", rtol, 3000, reorder=2, factor_bf16=bf16 if prec else 0)
         t = 0.05
         for _ in range(3):
@@ -488,6 +735,7 @@ def _batch_ctx(mesh, em, mu, prec, opts, dt):
     ed = np.array([1.0, 0.01, 0, 0, 0, 0, 0, 0, 1, 1, 0])
     ctx = make_ctx(mesh, tis=2, dt=dt, elemData=ed)
     ctx.set_precision(prec)
+    ctx.set_tuning_mode(sfem.TUNE_FIXED)  # FP32 rounding order independent of timing
     ctx.set_batch(em, mu)
     ctx.setup_solver(opts)
     return ctx

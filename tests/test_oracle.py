@@ -369,3 +369,43 @@ def test_reference_ldc2dquad_mesh2(golden):
     ref = deck.read_result(os.path.join(d, "test-ldc-stru-mesh2-ref.dat.gz"))
     assert deck.check_result(s, ref, 2) == 0
     assert np.max(np.linalg.norm(ref - s.reshape(-1, 3), axis=1)) < 2e-8
+
+
+def test_threaded_oracle_matches_sequential():
+    """The oracle's parallel paths (element colour classes, level-scheduled ILU(0), the
+    node-by-node pattern, chunked dot products) against its sequential paths: identical
+    pattern and prepare data, assembly and ILU(0) factor equal to rounding, and the
+    level-scheduled sweeps equal to the sequential substitution for the same factor."""
+    mesh = meshgen.ldc3d_hex(7, graded=True)
+    ctl = deck.Control(np.array([1.0, 1e-3, 0, 0, 0, 0, 0, 0, 1, 1, 0]), 2, 0.0, 0.1, 1e9, 5, 1, 1e-6)
+    res = {}
+    try:
+        for th in (1, 4):
+            oracle.set_threads(th)
+            f = oracle.OracleFEM(mesh, ctl)
+            f.prepare()
+            rng = np.random.default_rng(3)
+            n = mesh.nNode * 4
+            f.soln[:] = rng.uniform(-1, 1, n)
+            f.solnPrev[:] = rng.uniform(-1, 1, n)
+            f.td = oracle.time_params(2, 0.0, 0.1)
+            f.time_update()
+            f.assign_boundary_conditions(0.5)
+            f.update_iter_step()
+            f.assemble(0)
+            pc = oracle.OraclePC("ilu", f.rowptr, f.colidx, f.vals)
+            r = rng.standard_normal(f.nfree)
+            res[th] = (f, pc.lu(len(f.vals)), pc.apply(r))
+    finally:
+        oracle.set_threads(1)
+    (f1, lu1, z1), (f4, lu4, z4) = res[1], res[4]
+    assert np.array_equal(f1.rowptr, f4.rowptr) and np.array_equal(f1.colidx, f4.colidx)
+    for k in f1.prep:
+        assert np.array_equal(f1.prep[k], f4.prep[k])
+    rel = lambda a, b: np.max(np.abs(a - b)) / np.max(np.abs(b))
+    assert rel(f4.vals, f1.vals) <= 1e-14 and rel(f4.rhs, f1.rhs) <= 1e-14
+    assert rel(lu4, lu1) <= 1e-12
+    # the level-scheduled sweeps apply exactly the sequential substitution of their factor
+    pc_seq = oracle.OraclePC("ilu", f4.rowptr, f4.colidx, f4.vals)
+    assert np.array_equal(pc_seq.lu(len(f4.vals)), lu4)
+    assert rel(z4, pc_seq.apply(r)) <= 1e-15 or np.array_equal(z4, pc_seq.apply(r))
