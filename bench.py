@@ -56,18 +56,21 @@ def make_case(n):
     return mesh, ctl
 
 
-def run_path(mesh, ctl, prec, opts, newton_tol, steps, warmup, label, predictor=0, pc_lag=1):
+def run_path(mesh, ctl, prec, opts, newton_tol, steps, warmup, label, predictor=0, pc_lag=1, inner=(0.0, 0.0),
+             tuning=0):
     from cfdsfemmpi_amd import sfem
     t0 = time.time()
     fem = sfem.StabFEM(device=0)
     fem.setMesh(mesh)
     fem.setControl(ctl)
     fem.precision = prec
+    fem.tuning = tuning
     fem.prepareInputData()
     fem.setSolver(opts)
     ctx = fem.ctx
     ctx.set_predictor(predictor)
     ctx.set_pc_lag(pc_lag)
+    ctx.set_inner_tolerance(*inner)
     info = ctx.info()
     log(f"{label}: setup {time.time() - t0:.1f} s, nfree={info['nfree']} nnz={info['nnz']} colours={info['ncolors']}")
 
@@ -566,6 +569,12 @@ def main():
                          "with the tiled ordering, but velocity 1.6e-6 from FP64 after a single step")
     ap.add_argument("--predictor", choices=["previous", "extrapolate"], default="previous",
                     help="Newton initial iterate of each step (both paths)")
+    ap.add_argument("--inner-theta", type=float, default=0.0,
+                    help="mixed path: inexact Newton, inner rtol = max(rtol-min, min(inner-rtol, "
+                         "theta * newton_tol / ||F||)) per Newton iteration (0: fixed inner rtol)")
+    ap.add_argument("--inner-rtol-min", type=float, default=0.0, help="mixed path: floor of the inexact-Newton rtol")
+    ap.add_argument("--tuning", choices=["timed", "fixed"], default="timed",
+                    help="kernel shapes: timed autotuner (default) or fixed default shapes (reproducible FP32)")
     ap.add_argument("--pc-lag-mixed", type=int, default=1,
                     help="mixed path: rebuild the ILU(0) factor on every n-th new tangent only")
     ap.add_argument("--ksp", choices=["bcgs", "gmres"], default="bcgs", help="Krylov method of both paths")
@@ -599,13 +608,15 @@ def main():
     omx = sfem.solver_opts(a.ksp, "ilu", a.inner_rtol, 5000, a.restart, reorder=a.reorder,
                            factor_bf16=a.bf16_factors)
     pred = sfem.PRED_EXTRAPOLATE if a.predictor == "extrapolate" else sfem.PRED_PREVIOUS
+    tune_mode = sfem.TUNE_FIXED if a.tuning == "fixed" else sfem.TUNE_TIMED
     r64 = rmx = None
     if a.only != "mixed":
-        r64 = run_path(mesh, ctl, sfem.PREC_FP64, o64, a.newton_tol, a.steps, a.warmup, "fp64", pred)
+        r64 = run_path(mesh, ctl, sfem.PREC_FP64, o64, a.newton_tol, a.steps, a.warmup, "fp64", pred,
+                       tuning=tune_mode)
         log(f"fp64: {r64['steps_per_s']:.3f} steps/s")
     if a.only != "fp64":
         rmx = run_path(mesh, ctl, sfem.PREC_MIXED, omx, a.newton_tol_mixed, a.steps, a.warmup, "mixed", pred,
-                       a.pc_lag_mixed)
+                       a.pc_lag_mixed, (a.inner_theta, a.inner_rtol_min), tune_mode)
         log(f"mixed: {rmx['steps_per_s']:.3f} steps/s")
     main_res = rmx or r64
     err_u = err_p = None
@@ -689,7 +700,9 @@ def main():
                    "solver": ("BiCGStab" if a.ksp == "bcgs" else f"GMRES({a.restart})") + " + ILU(0)", "ilu_ordering": ["natural", "rcm", "multicolour", "tiled multicolour"][a.reorder],
                    "newton_tol": a.newton_tol, "newton_tol_mixed": a.newton_tol_mixed,
                    "inner_rtol_mixed": a.inner_rtol, "ilu_factors_mixed": "bf16" if a.bf16_factors else "fp32",
-                   "newton_initial_iterate": a.predictor, "pc_lag_mixed": a.pc_lag_mixed, "parallelism": "single GPU"},
+                   "newton_initial_iterate": a.predictor, "pc_lag_mixed": a.pc_lag_mixed,
+                   "inner_theta_mixed": a.inner_theta, "inner_rtol_min_mixed": a.inner_rtol_min,
+                   "kernel_shapes": a.tuning, "parallelism": "single GPU"},
         "fp64_steps_per_s": r64["steps_per_s"] if r64 else None,
         "mixed_steps_per_s": rmx["steps_per_s"] if rmx else None,
         "speedup_mixed_vs_fp64": (rmx["steps_per_s"] / r64["steps_per_s"]) if (r64 and rmx) else None,

@@ -519,6 +519,15 @@ int sfem_predictor_stats(sfem_ctx* c, int64_t* counts, double* last_norms) {
   return SFEM_OK;
 }
 
+int sfem_set_inner_tolerance(sfem_ctx* c, double theta, double rtol_min) {
+  if (!c) return set_error(SFEM_ERR_ARG, "null context");
+  if (!(theta >= 0.0) || !(rtol_min >= 0.0) || !std::isfinite(theta) || !std::isfinite(rtol_min))
+    return set_error(SFEM_ERR_ARG, "theta and rtol_min must be finite and >= 0");
+  c->inner_theta = theta;
+  c->inner_rtol_min = rtol_min;
+  return SFEM_OK;
+}
+
 int sfem_set_pc_lag(sfem_ctx* c, int lag) {
   if (!c) return set_error(SFEM_ERR_ARG, "null context");
   if (lag < 1) return set_error(SFEM_ERR_ARG, "pc lag must be >= 1");
@@ -738,7 +747,10 @@ int sfem_step(sfem_ctx* c, double timeFact, const sfem_solver_opts* o, double ne
     if (nrm < newton_tol) { conv = true; break; }
     int its = 0;
     double rr = 0.0;
-    rc = sfem_solve(c, o, &its, &rr);
+    sfem_solver_opts oi = *o;
+    if (c->inner_theta > 0.0 && c->precision != SFEM_PREC_FP64 && nrm > 0.0)
+      oi.rtol = std::max(c->inner_rtol_min, std::min(o->rtol, c->inner_theta * newton_tol / nrm));
+    rc = sfem_solve(c, &oi, &its, &rr);
     st.krylov_iters += its;
     st.t_solver_ms += c->last_solve_ms;
     if (rc) { if (stats) *stats = st; return rc; }

@@ -64,6 +64,7 @@ ABI = [
     ("sfem_set_precision", C.c_int, [_vp, C.c_int]),
     ("sfem_set_predictor", C.c_int, [_vp, C.c_int]),
     ("sfem_set_pc_lag", C.c_int, [_vp, C.c_int]),
+    ("sfem_set_inner_tolerance", C.c_int, [_vp, C.c_double, C.c_double]),
     ("sfem_time_update", C.c_int, [_vp, C.c_double]),
     ("sfem_assemble", C.c_int, [_vp, C.c_int, C.c_int, _f64p]),
     ("sfem_element_KF", C.c_int, [_vp, C.c_int64, _f64p, _f64p]),
@@ -350,6 +351,12 @@ class Context:
         v, w = C.c_int64(0), C.c_int64(0)
         _check(self.lib.sfem_solver_stats(self.h, C.byref(v), C.byref(w)), "sfem_solver_stats")
         return v.value, w.value
+
+    def set_inner_tolerance(self, theta: float, rtol_min: float = 0.0):
+        """Mixed paths: each Newton iteration's inner rtol becomes
+        max(rtol_min, min(opts.rtol, theta * newton_tol / ||F||)) (theta 0: fixed rtol)."""
+        _check(self.lib.sfem_set_inner_tolerance(self.h, float(theta), float(rtol_min)),
+               "sfem_set_inner_tolerance")
 
     def set_pc_lag(self, lag: int):
         """Rebuild the preconditioner on every lag-th new tangent only (1 = always)."""

@@ -275,10 +275,12 @@ def test_learned_start_never_changes_the_answer():
     """Train briefly on a short trajectory, then rerun it with the learned predictor:
     each step starts from the smaller-residual candidate, and every converged state
     agrees with the extrapolated run's within 1e-8 of the state's largest entry.  Both
-    runs iterate Newton to ||F|| < 1e-12 (Krylov rtol 1e-10 per solve), so the difference
-    left is that of two solutions of the same discrete system at that residual level."""
+    runs iterate Newton to ||F|| < 5e-12 (Krylov rtol 1e-10 per solve; the oracle's
+    direct-solve Newton iteration levels off at 1e-13 to 3e-13 on this mesh), so the
+    difference left is that of two solutions of the same discrete system at that
+    residual level."""
     mesh = _tiny_cylinder()
-    nsteps, tol = 14, 1e-12
+    nsteps, tol = 14, 5e-12
     fem, ctx, opts = _ctx(mesh)
     ctx.set_predictor(sfem.PRED_EXTRAPOLATE)
     L = sfem.Learner(ctx, nsteps + 1, seed=21)
@@ -307,6 +309,9 @@ def test_learned_start_never_changes_the_answer():
         assert st.rhs_norm0 == pytest.approx(min(nrm), rel=1e-6)
         sol = ctx2.get_solution()
         scale = np.max(np.abs(ref[s]))
+        print(f"[learned] step {s}: start {['linear', 'quadratic', 'learned'][int(np.argmin(nrm))]}, "
+              f"newton {st.newton_iters}, |F| {st.rhs_norm:.2e}, max |diff| / max |ref| "
+              f"{np.max(np.abs(sol - ref[s])) / scale:.2e}", flush=True)
         assert np.max(np.abs(sol - ref[s])) <= 1e-8 * scale, (s, np.max(np.abs(sol - ref[s])) / scale)
         t += 0.1
     cnt, _ = ctx2.predictor_stats()
