@@ -172,6 +172,14 @@ sfem_ctx* sfem_create(int ndim, int npElem, int64_t nNode, int64_t nElem, const 
   } else {
     (void)hipGetDevice(&c->device);
   }
+  // the allocation-free entry point's scratch array: this device's address of it, looked
+  // up once per device with the device current
+  if (la_scratch_init(c->device) != SFEM_OK) {
+    std::string msg = g_err;
+    delete c;
+    set_error(SFEM_ERR_HIP, msg.c_str());
+    return nullptr;
+  }
   c->et = et;
   c->dims = elem_dims(et);
   c->nNode = nNode;
@@ -851,6 +859,12 @@ int sfem_pc_apply(sfem_ctx* c, const double* r, double* z) {
   if (!c || !r || !z) return set_error(SFEM_ERR_ARG, "null argument");
   HIP_OK(hipSetDevice(c->device));
   return la_pc_apply_host(c, r, z);
+}
+
+int sfem_pc_apply_noalloc(sfem_ctx* c, const double* r, double* z) {
+  if (!c || !r || !z) return set_error(SFEM_ERR_ARG, "null argument");
+  HIP_OK(hipSetDevice(c->device));
+  return la_pc_apply_noalloc(c, r, z);
 }
 
 int sfem_block_info(sfem_ctx* c, int64_t* nblk, int64_t* npairs, int64_t* nvals) {

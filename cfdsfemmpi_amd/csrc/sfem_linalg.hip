@@ -21,8 +21,10 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <type_traits>
+#include <mutex>
 #include <vector>
 
 #include "sfem_internal.hpp"
@@ -2858,6 +2860,80 @@ int la_pc_apply_host(sfem_ctx* ctx, const double* r, double* z) {
     rc = set_error(SFEM_ERR_HIP, "pc apply");
   cleanup();
   return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free preconditioner application (sfem_pc_apply_noalloc).  The staging
+// vectors that la_pc_apply_host allocates on every call live instead in one file-scope
+// device array, `scratch`, of kNoAllocDoubles doubles (64 MiB), which every GPU the
+// library runs on holds until the process ends.  Host code never passes the array's
+// name to a kernel or a copy (on the host it is only a stand-in symbol): each device's
+// address of its own copy is looked up once, with that device current, when the first
+// context on it is created (la_scratch_init), and that address is used instead.
+// ---------------------------------------------------------------------------
+constexpr int64_t kNoAllocDoubles = int64_t(8) << 20;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+constexpr int kMaxScratchDevices = 64;
+static double* g_scratch_ptr[kMaxScratchDevices] = {nullptr};
+static std::mutex g_scratch_mu;
+
+int la_scratch_init(int device) {
+  if (device < 0 || device >= kMaxScratchDevices) return set_error(SFEM_ERR_UNSUPPORTED, "device index too large");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return SFEM_OK;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) return set_hip_error(e, "hipGetSymbolAddress(scratch)");
+  g_scratch_ptr[device] = (double*)ptr;
+  return SFEM_OK;
+}
+
+int64_t la_noalloc_capacity() { return kNoAllocDoubles; }
+
+// z = M^{-1} r as la_pc_apply_host, with dr, dz (n each) and, for ILU(0), pr, pz (npad
+// each) carved from this device's scratch array.  Every slot a kernel reads is written
+// earlier in the same call (k_pad_in writes all npad slots, the sweeps every slot of each
+// block row), so nothing is cleared.  Calls that overlap on one GPU share the array and
+// race; this call returns after its work on the context's stream has finished.
+int la_pc_apply_noalloc(sfem_ctx* ctx, const double* r, double* z) {
+  LinAlg* L = ctx->la;
+  if (!L || L->pc_K_version < 0) return set_error(SFEM_ERR_STATE, "no preconditioner has been built");
+  double* base = (ctx->device >= 0 && ctx->device < kMaxScratchDevices) ? g_scratch_ptr[ctx->device] : nullptr;
+  if (!base) return set_error(SFEM_ERR_STATE, "allocation-free scratch not set up on this device");
+  hipStream_t st = ctx->stream;
+  const int64_t n = L->n;
+  const bool blk = L->pc_type == SFEM_PC_ILU0 && L->ilu;
+  const int64_t nk = blk ? L->ilu->npad : 0;
+  const int64_t need = 2 * n + 2 * nk;
+  if (need > kNoAllocDoubles) {
+    char msg[160];
+    std::snprintf(msg, sizeof msg, "allocation-free apply needs %lld doubles of scratch, the maximum is %lld",
+                  (long long)need, (long long)kNoAllocDoubles);
+    return set_error(SFEM_ERR_ARG, msg);
+  }
+  double* dr = base;
+  double* dz = base + n;
+  double* pr = base + 2 * n;
+  double* pz = pr + nk;
+  LA_CHECK(hipMemcpyAsync(dr, r, n * sizeof(double), hipMemcpyHostToDevice, st));
+  int rc;
+  if (blk) {
+    const int32_t* padmap = L->ilu->d_padmap;
+    hipLaunchKernelGGL((k_pad_in<double, double>), dim3(grid_for(nk)), dim3(kTPB), 0, st, nk, padmap, dr, pr, 1.0);
+    rc = pc_apply<double>(ctx, pr, pz);
+    if (!rc)
+      hipLaunchKernelGGL((k_pad_out<double, double>), dim3(grid_for(nk)), dim3(kTPB), 0, st, nk, padmap, pz, dz, 1.0);
+  } else {
+    rc = pc_apply<double>(ctx, dr, dz);
+  }
+  if (rc) {
+    (void)hipStreamSynchronize(st);
+    return rc;
+  }
+  LA_CHECK(hipGetLastError());
+  LA_CHECK(hipMemcpyAsync(z, dz, n * sizeof(double), hipMemcpyDeviceToHost, st));
+  LA_CHECK(hipStreamSynchronize(st));
+  return SFEM_OK;
 }
 
 int la_apply_op32(sfem_ctx* ctx, const float* x, float* y, const RowSubset* sub) {

@@ -164,6 +164,9 @@ int la_block_info(sfem_ctx* ctx, int64_t* nblk, int64_t* npairs, int64_t* nvals)
 int la_get_ilu(sfem_ctx* ctx, int64_t* nnz_out, int32_t* perm, int32_t* rowptr, int32_t* colidx, double* vals);
 int la_tile_info(sfem_ctx* ctx, int64_t* ntiles, int64_t* ncolors, int64_t* tile_vals);
 int la_pc_apply_host(sfem_ctx* ctx, const double* r, double* z);
+int la_scratch_init(int device);
+int64_t la_noalloc_capacity();
+int la_pc_apply_noalloc(sfem_ctx* ctx, const double* r, double* z);
 int la_tuning_levels(sfem_ctx* ctx, int fp32, int* out, int cap, int* nlevels);
 int la_set_tuning_mode(sfem_ctx* ctx, int mode);
 int la_get_tuning_table(sfem_ctx* ctx, int fp32, int* table, int cap, int* n);

@@ -83,6 +83,7 @@ ABI = [
     ("sfem_kernel_timing", C.c_int, [_vp, C.c_int, _i64p, _f64p]),
     ("sfem_get_ilu", C.c_int, [_vp, _i64p, _i32p, _i32p, _i32p, _f64p]),
     ("sfem_pc_apply", C.c_int, [_vp, _f64p, _f64p]),
+    ("sfem_pc_apply_noalloc", C.c_int, [_vp, _f64p, _f64p]),
     ("sfem_solver_stats", C.c_int, [_vp, _i64p, _i64p]),
     ("sfem_set_batch", C.c_int, [_vp, C.c_int32, _i32p, _f64p]),
     ("sfem_step_batch", C.c_int, [_vp, C.c_double, C.POINTER(SolverOpts), C.c_double, C.c_int, _i32p, _i32p, _f64p]),

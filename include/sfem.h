@@ -280,6 +280,16 @@ int sfem_get_ilu(sfem_ctx* ctx, int64_t* nnz, int32_t* perm, int32_t* rowptr, in
  * for parity checks of the triangular solves. */
 int sfem_pc_apply(sfem_ctx* ctx, const double* r, double* z);
 
+/* sfem_pc_apply without allocating: no counterpart in the reference.  For callers that
+ * cannot allocate device memory (tight serving loops, code that must not call hipMalloc):
+ * the same z = M^{-1} r as sfem_pc_apply, whose staging vectors (r and z on the device,
+ * and for ILU(0) their padded block-layout copies) come from a fixed 64 MiB device array
+ * that each GPU holds for the life of the process, instead of hipMalloc / hipFree per
+ * call.  Limit: 2 nfree + 2 npad <= 8,388,608 doubles (npad = 4 x node blocks for
+ * ILU(0), else 0), about 2M free DOFs; larger systems get SFEM_ERR_ARG.  Calls that
+ * overlap on one GPU share the array and race; each call returns after its device work. */
+int sfem_pc_apply_noalloc(sfem_ctx* ctx, const double* r, double* z);
+
 /* Batched meshes.  The context's mesh is a disjoint union of nmesh meshes, nodes and
  * elements numbered mesh by mesh; elem_mesh[e] is the mesh of element e and mu[m] the
  * viscosity of mesh m (elemData[1] of its control file).  Call after
