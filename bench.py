@@ -10,9 +10,10 @@ min(t, 1), p pinned at (0, 0, 0), generalised-alpha rho_inf = 0, dt = 0.1.
 Two runs of K time steps from the zero state, each after W warm-up steps (the
 state is reset after warm-up):
   * FP64: FP64 tangent and CSR, BiCGStab + ILU(0) at rtol 1e-8;
-  * mixed: FP32 element tangents and FP32 CSR, FP32 BiCGStab + FP32 ILU(0) inside
-    an FP64 residual / Newton update loop (defect correction), Newton tolerance
-    equal to the FP64 run's.
+  * mixed: FP32 element tangents and FP32 CSR, FP32 BiCGStab + ILU(0) (factor and
+    tile inverses stored in BF16) inside an FP64 residual / Newton update loop
+    (defect correction) to Newton 1e-9; each FP32 solve is asked for
+    max(1e-5, 0.5 * 1e-9 / ||F||) (inexact Newton).
 ``value`` is the mixed run's time steps per second; the JSON line also carries
 the FP64 rate, the speed-up, the relative L2 error of u and p (mixed vs FP64),
 per-kernel-class device times measured with HIP events on the library's stream,
@@ -56,7 +57,7 @@ def make_case(n):
     return mesh, ctl
 
 
-def run_path(mesh, ctl, prec, opts, newton_tol, steps, warmup, label, predictor=0, pc_lag=1, inner=(0.0, 0.0),
+def run_path(mesh, ctl, prec, opts, newton_tol, steps, warmup, label, predictor=0, pc_lag=1, inner=(0.0, 0.5),
              tuning=0):
     from cfdsfemmpi_amd import sfem
     t0 = time.time()
@@ -108,46 +109,72 @@ def run_path(mesh, ctl, prec, opts, newton_tol, steps, warmup, label, predictor=
     return res
 
 
-def cpu_baseline(n_target, fp64_res):
-    """Oracle (single-threaded CPU restatement) on one time step of the 32^3 cavity,
-    scaled to the target mesh by element count (assembly) and by nonzeros x the
-    target FP64 run's Krylov iterations (solve)."""
+def cpu_threads():
+    """Host threads for the CPU baseline: the OMP_NUM_THREADS the host grants (16 on the
+    one-GPU boxes), else every core."""
+    try:
+        return max(1, int(os.environ.get("OMP_NUM_THREADS", "") or (os.cpu_count() or 1)))
+    except ValueError:
+        return max(1, os.cpu_count() or 1)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def cpu_baseline(n_target, c2=True):
+    """The oracle (C++ CPU restatement of the reference path: same element arithmetic, DOF
+    numbering and CSR, MatSetValues-style binary-search ADD assembly, BiCGStab + ILU(0)
+    at rtol 1e-8, Newton 1e-8) on all granted host threads: element colour classes in
+    parallel, level-scheduled ILU(0), row-parallel products.  Measured, not scaled: the
+    first time step of the target cavity (set-up -- pattern and prepareElemData -- not
+    timed, as the reference's time-loop timer excludes it), and the 32^3 cavity's 20 steps
+    in full.  The tangent is assembled once per step, as the HIP path does."""
     import oracle
     from cfdsfemmpi_amd import deck
-    ns = 32
-    mesh, ctl = make_case(ns)
-    so = deck.SolverOptions("bcgs", "ilu", 1e-8, 2000)
+    th = cpu_threads()
+    oracle.set_threads(th)
+    so = deck.SolverOptions("bcgs", "ilu", 1e-8, 5000)
+    mesh, ctl = make_case(n_target)
+    t0 = time.perf_counter()
     f = oracle.OracleFEM(mesh, ctl, so)
     f.prepare()
-    f.td = oracle.time_params(2, 0.0, 0.1)
-    f.time_update()
-    f.assign_boundary_conditions(0.1)
-    f.update_iter_step()
+    t_setup = time.perf_counter() - t0
+    tm = {}
     t0 = time.perf_counter()
-    f.assemble(0)
-    t_asm = time.perf_counter() - t0
-    for k in range(mesh.nDBC):
-        j = int(mesh.dbc_node[k]) * 4 + int(mesh.dbc_dof[k])
-        f.soln[j] += f.solnApplied[j]
-    pc = oracle.OraclePC("ilu", f.rowptr, f.colidx, f.vals)
-    t0 = time.perf_counter()
-    x, rc, its, rr = oracle.krylov("bcgs", pc, f.rowptr, f.colidx, f.vals, f.rhs, 1e-8, 2000)
-    t_krylov = time.perf_counter() - t0
-    nnz_s = len(f.colidx)
-    per_elem = t_asm / mesh.nElem
-    per_iter_nnz = t_krylov / max(its, 1) / nnz_s
-    ne_t = n_target ** 3
-    nnz_t = fp64_res["info"]["nnz"]
-    steps = len(fp64_res["hist"])
-    n_asm = sum(h[0] for h in fp64_res["hist"])          # assemblies (Newton iterations)
-    n_kry = sum(h[1] for h in fp64_res["hist"])
-    t_step = (n_asm * ne_t * per_elem + n_kry * nnz_t * per_iter_nnz) / steps
-    return {"value": 1.0 / t_step, "unit": "time-steps/s", "cores": 1, "kind": "port",
-            "sample": (f"oracle (C++ CPU restatement, 1 thread) on one step of the 32^3 graded cavity: "
-                       f"assembly {t_asm:.2f} s for {mesh.nElem} hexes, BiCGStab+ILU(0) {its} its in "
-                       f"{t_krylov:.2f} s on {nnz_s} nnz; scaled to {n_target}^3 by elements x "
-                       f"{n_asm} assemblies and nnz x {n_kry} Krylov its of the FP64 GPU run"),
-            "cpu": platform.processor() or platform.machine(), "host_threads": os.cpu_count()}
+    f.solve_fully_implicit(1e-8, 10, steps=1, reuse_K=True, timers=tm)
+    t_step = time.perf_counter() - t0
+    log(f"cpu baseline: {n_target}^3 step 1 in {t_step:.1f} s (assembly {tm['assembly']:.1f} s, solver "
+        f"{tm['solver']:.1f} s, {tm.get('krylov_its', 0)} BiCGStab its) on {th} threads; set-up {t_setup:.1f} s")
+    del f
+    c2_res = None
+    if c2:
+        mesh2, ctl2 = make_case(32)
+        f2 = oracle.OracleFEM(mesh2, ctl2, so)
+        f2.prepare()
+        tm2 = {}
+        t0 = time.perf_counter()
+        f2.solve_fully_implicit(1e-8, 10, steps=20, reuse_K=True, timers=tm2)
+        t_c2 = time.perf_counter() - t0
+        c2_res = {"workload": "32^3 graded Hexa8 cavity, 20 steps", "seconds": t_c2, "steps_per_s": 20 / t_c2,
+                  "assembly_s": tm2["assembly"], "solver_s": tm2["solver"], "krylov_its": tm2.get("krylov_its")}
+        log(f"cpu baseline: 32^3, 20 steps in {t_c2:.1f} s ({20 / t_c2:.3f} steps/s)")
+        del f2
+    oracle.set_threads(1)
+    return {"value": 1.0 / t_step, "unit": "time-steps/s", "cores": th, "kind": "port",
+            "sample": (f"oracle (C++ CPU restatement of the reference path) on {th} threads, measured: the first "
+                       f"time step of the {n_target}^3 cavity (assembly {tm['assembly']:.2f} s, BiCGStab + "
+                       f"natural-order ILU(0) {tm.get('krylov_its', 0)} its {tm['solver']:.2f} s); set-up "
+                       f"{t_setup:.1f} s not timed"),
+            "phases_s": {"assembly": tm["assembly"], "solver": tm["solver"]},
+            "c2_full_run": c2_res, "cpu": cpu_model(), "host_threads": os.cpu_count()}
 
 
 C4_METRIC = "time-steps/s of a 256-cavity Tria3 batch (every mesh advances one step), batched vs one mesh at a time"
@@ -551,28 +578,27 @@ def main():
     ap.add_argument("--c4-store32-rtol", type=float, default=1e-4, help="c4: inner rtol of the FP32-stored run")
     ap.add_argument("--c4-store32-newton", type=float, default=1e-8, help="c4: Newton tol of the FP32-stored run")
     ap.add_argument("--n", type=int, default=0, help="override the mesh size n (n^3 hexes)")
-    ap.add_argument("--inner-rtol", type=float, default=2e-4,
-                    help="mixed path: FP32 inner solve tolerance; two solves per step take ||F|| from ~5.7e-3 "
-                         "below the mixed Newton tolerance; velocity stays within ~7e-7 of FP64 after 1 step "
-                         "and ~1.4e-7 after 10 (3e-4: 4%% faster, 1.9e-7 after 10 steps)")
+    ap.add_argument("--inner-rtol", type=float, default=1e-5,
+                    help="mixed path: the deepest reduction asked of one FP32 inner solve (with --inner-theta, "
+                         "a solve is asked for no more than the Newton test still needs)")
     ap.add_argument("--newton-tol", type=float, default=1e-8,
                     help="FP64 path: absolute Newton tolerance on ||F|| (the reference's 1e-8)")
     ap.add_argument("--newton-tol-mixed", type=float, default=1e-9,
                     help="mixed path: absolute Newton tolerance on the FP64 residual")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu-c2", action="store_true", help="skip the CPU baseline's 20-step 32^3 run")
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None)
     ap.add_argument("--reorder", type=int, default=None,
                     help="ILU(0) node ordering: 0 natural, 1 RCM, 2 multicolour, 3 tiled multicolour "
                          "(default: 3 for the hex cavities, 2 for the c4 batch)")
-    ap.add_argument("--bf16-factors", type=int, default=0,
-                    help="mixed path: ILU(0) factors stored in bfloat16 (tile inverses in FP32): 5%% more steps/s "
-                         "with the tiled ordering, but velocity 1.6e-6 from FP64 after a single step")
+    ap.add_argument("--bf16-factors", type=int, default=1,
+                    help="mixed path: ILU(0) factors and tile inverses stored in bfloat16 (0: FP32)")
     ap.add_argument("--predictor", choices=["previous", "extrapolate"], default="previous",
                     help="Newton initial iterate of each step (both paths)")
-    ap.add_argument("--inner-theta", type=float, default=0.0,
-                    help="mixed path: inexact Newton, inner rtol = max(rtol-min, min(inner-rtol, "
-                         "theta * newton_tol / ||F||)) per Newton iteration (0: fixed inner rtol)")
-    ap.add_argument("--inner-rtol-min", type=float, default=0.0, help="mixed path: floor of the inexact-Newton rtol")
+    ap.add_argument("--inner-theta", type=float, default=0.5,
+                    help="mixed path: inexact Newton, inner rtol = min(inner-rtol-max, max(inner-rtol, "
+                         "theta * newton_tol / ||F||)) per Newton iteration (0: inner-rtol throughout)")
+    ap.add_argument("--inner-rtol-max", type=float, default=0.5, help="mixed path: cap of the inexact-Newton rtol")
     ap.add_argument("--tuning", choices=["timed", "fixed"], default="timed",
                     help="kernel shapes: timed autotuner (default) or fixed default shapes (reproducible FP32)")
     ap.add_argument("--pc-lag-mixed", type=int, default=1,
@@ -616,7 +642,7 @@ def main():
         log(f"fp64: {r64['steps_per_s']:.3f} steps/s")
     if a.only != "fp64":
         rmx = run_path(mesh, ctl, sfem.PREC_MIXED, omx, a.newton_tol_mixed, a.steps, a.warmup, "mixed", pred,
-                       a.pc_lag_mixed, (a.inner_theta, a.inner_rtol_min), tune_mode)
+                       a.pc_lag_mixed, (a.inner_theta, a.inner_rtol_max), tune_mode)
         log(f"mixed: {rmx['steps_per_s']:.3f} steps/s")
     main_res = rmx or r64
     err_u = err_p = None
@@ -656,14 +682,14 @@ def main():
     # ILU(0) apply: both sweeps together read every factor value, block index and
     # descriptor once; r read, y written, y read and written, plus the gathered y once
     # per sweep (5 vector passes).  With the tiled ordering the values inside the tiles are
-    # replaced by the tiles' packed inverse triangles (2 x 528 values per tile, FP64
-    # beside an FP64 factor, FP32 otherwise).
+    # replaced by the tiles' packed inverse triangles (2 x 528 values per tile, stored in
+    # the factor's type).
     lu_s = 2 if (fp32 and a.bf16_factors) else sv
     ti = main_res["tiles"]
     pc_bytes = nnz * lu_s + 4 * bi["npairs"] + 32 * bi["nblk"] + 5 * sv * nf
     pc_name = "ILU(0) apply (2 x colours launches)"
     if ti["ntiles"]:
-        inv_s = 8 if not fp32 else 4
+        inv_s = lu_s  # the tile inverses are stored like the factor
         pc_bytes += ti["ntiles"] * 2 * 528 * inv_s - ti["tile_vals"] * lu_s
         pc_name = (f"ILU(0) apply, tiled: k_bsr_tile_lsolve/usolve, {2 * ti['ncolors'] - 1} launches, "
                    f"{ti['ntiles']} tiles")
@@ -680,7 +706,7 @@ def main():
     cpu = None
     if not a.no_cpu and r64 is not None:
         log("cpu baseline (oracle) ...")
-        cpu = cpu_baseline(n, r64)
+        cpu = cpu_baseline(n, c2=not a.no_cpu_c2 and n != 32)
     out = {
         "metric": METRIC,
         "value": main_res["steps_per_s"],
@@ -701,7 +727,7 @@ def main():
                    "newton_tol": a.newton_tol, "newton_tol_mixed": a.newton_tol_mixed,
                    "inner_rtol_mixed": a.inner_rtol, "ilu_factors_mixed": "bf16" if a.bf16_factors else "fp32",
                    "newton_initial_iterate": a.predictor, "pc_lag_mixed": a.pc_lag_mixed,
-                   "inner_theta_mixed": a.inner_theta, "inner_rtol_min_mixed": a.inner_rtol_min,
+                   "inner_theta_mixed": a.inner_theta, "inner_rtol_max_mixed": a.inner_rtol_max,
                    "kernel_shapes": a.tuning, "parallelism": "single GPU"},
         "fp64_steps_per_s": r64["steps_per_s"] if r64 else None,
         "mixed_steps_per_s": rmx["steps_per_s"] if rmx else None,

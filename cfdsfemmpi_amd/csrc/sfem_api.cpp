@@ -527,12 +527,12 @@ int sfem_predictor_stats(sfem_ctx* c, int64_t* counts, double* last_norms) {
   return SFEM_OK;
 }
 
-int sfem_set_inner_tolerance(sfem_ctx* c, double theta, double rtol_min) {
+int sfem_set_inner_tolerance(sfem_ctx* c, double theta, double rtol_max) {
   if (!c) return set_error(SFEM_ERR_ARG, "null context");
-  if (!(theta >= 0.0) || !(rtol_min >= 0.0) || !std::isfinite(theta) || !std::isfinite(rtol_min))
-    return set_error(SFEM_ERR_ARG, "theta and rtol_min must be finite and >= 0");
+  if (!(theta >= 0.0) || !std::isfinite(theta)) return set_error(SFEM_ERR_ARG, "theta must be finite and >= 0");
+  if (!(rtol_max > 0.0 && rtol_max < 1.0)) return set_error(SFEM_ERR_ARG, "rtol_max must lie in (0, 1)");
   c->inner_theta = theta;
-  c->inner_rtol_min = rtol_min;
+  c->inner_rtol_max = rtol_max;
   return SFEM_OK;
 }
 
@@ -757,7 +757,7 @@ int sfem_step(sfem_ctx* c, double timeFact, const sfem_solver_opts* o, double ne
     double rr = 0.0;
     sfem_solver_opts oi = *o;
     if (c->inner_theta > 0.0 && c->precision != SFEM_PREC_FP64 && nrm > 0.0)
-      oi.rtol = std::max(c->inner_rtol_min, std::min(o->rtol, c->inner_theta * newton_tol / nrm));
+      oi.rtol = std::min(c->inner_rtol_max, std::max(o->rtol, c->inner_theta * newton_tol / nrm));
     rc = sfem_solve(c, &oi, &its, &rr);
     st.krylov_iters += its;
     st.t_solver_ms += c->last_solve_ms;

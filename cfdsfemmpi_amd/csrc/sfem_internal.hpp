@@ -148,7 +148,7 @@ struct sfem_ctx {
   int precision = SFEM_PREC_FP64;
   int predictor = SFEM_PRED_PREVIOUS;
   double inner_theta = 0.0;      // sfem_set_inner_tolerance: 0 = fixed inner rtol
-  double inner_rtol_min = 0.0;
+  double inner_rtol_max = 0.5;
   int status = 0;  // bit flags: 1 prepared, 2 pattern, 4 params, 8 assembled
 
   // host mesh

@@ -49,11 +49,19 @@ __device__ inline AT widen(float v) { return (AT)v; }
 template <typename AT>
 __device__ inline AT widen(bf16 v) { return (AT)__uint_as_float(((uint32_t)v.bits) << 16); }
 
+__device__ inline uint16_t bf16_bits(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+// Stored value of an FP64 result in the storage type of a factor or tile inverse.
+template <typename ST>
+__device__ inline ST store_as(double v) { return (ST)v; }
+template <>
+__device__ inline bf16 store_as<bf16>(double v) { return bf16{bf16_bits((float)v)}; }
+
 __global__ __launch_bounds__(256) void k_to_bf16(int64_t n, const float* __restrict__ a, bf16* __restrict__ b) {
-  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
-    const uint32_t u = __float_as_uint(a[i]);
-    b[i].bits = (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
-  }
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    b[i].bits = bf16_bits(a[i]);
 }
 
 // scalar slots
@@ -812,7 +820,7 @@ __global__ __launch_bounds__(64) void k_tile_invert(const int32_t* __restrict__ 
   for (int q = lane; q < kTileTri; q += 64) {
     int i = 0;
     while (tri_lo(i + 1) <= q) ++i;
-    out[q] = (IT)X[i][q - tri_lo(i)];
+    out[q] = store_as<IT>(X[i][q - tri_lo(i)]);
   }
   wave_sync();
   if (lane < kTileN) {
@@ -834,7 +842,7 @@ __global__ __launch_bounds__(64) void k_tile_invert(const int32_t* __restrict__ 
   for (int q = lane; q < kTileTri; q += 64) {
     int i = 0;
     while (i + 1 < kTileN && tri_up(i + 1) - kTileTri <= q) ++i;
-    out[kTileTri + q] = (IT)X[i][i + (q - (tri_up(i) - kTileTri))];
+    out[kTileTri + q] = store_as<IT>(X[i][i + (q - (tri_up(i) - kTileTri))]);
   }
 }
 
@@ -879,7 +887,7 @@ __device__ inline AT tile_tri_row(const IT* row, const AT* v, int lane, int lo, 
 #pragma unroll
   for (int jj = 0; jj < kTileN / S; ++jj) {
     const int j = jj * S + sub;
-    if (j >= lo && j <= hi) part += (AT)row[j] * v[j];
+    if (j >= lo && j <= hi) part += widen<AT>(row[j]) * v[j];
   }
 #pragma unroll
   for (int o = G / 2; o >= 4; o >>= 1) part += __shfl_xor(part, o, G);
@@ -888,12 +896,15 @@ __device__ inline AT tile_tri_row(const IT* row, const AT* v, int lane, int lo, 
 
 // Forward sweep of one colour: y = L^{-1} r on its tiles.  With fuse_u the tiles have no
 // upper blocks outside themselves (the last colour) and their backward sweep follows.
+// Row g of tile tg in a sweep: rec = trow[tg * kTileMax + g] = {block row p (-1 past the
+// tile's rows), first of its blocks outside the tile in bcol, their count | m << 16, their
+// first block in the sweeps' block-by-block copy}, so a workgroup reads its rows' data with
+// one load indexed by the tile.
 template <typename T, typename LT, typename IT, int G, int K>
-__global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_lsolve(const int32_t* __restrict__ tstart, int tfirst,
-                                                                  int tcount, const BDesc* __restrict__ desc,
+__global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_lsolve(int tfirst, int tcount,
+                                                                  const int4* __restrict__ trow,
                                                                   const int32_t* __restrict__ bcol,
                                                                   const LT* __restrict__ sw,
-                                                                  const int4* __restrict__ swoff,
                                                                   const IT* __restrict__ tinv,
                                                                   const T* __restrict__ r, T* __restrict__ y,
                                                                   int fuse_u) {
@@ -904,10 +915,9 @@ __global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_lsolve(const int32_t*
   const int tix = blockIdx.x;
   if (tix >= tcount) return;
   const int tg = tfirst + tix;
-  const int ts = tstart[tg];
-  const int tsz = tstart[tg + 1] - ts;
-  const bool act = g < tsz;
-  const int32_t p = ts + g;
+  const int4 rec = trow[(int64_t)tg * kTileMax + g];
+  const bool act = rec.x >= 0;
+  const int32_t p = rec.x;
   const IT* inv = tinv + (int64_t)tg * kTileInv;
   const int i = 4 * g + lane;
   // L^{-1} of the tile, loaded before the sums over the blocks outside the tile
@@ -916,11 +926,8 @@ __global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_lsolve(const int32_t*
   AT acc[4] = {0, 0, 0, 0};
   AT rv = 0;
   if (act) {
-    const BDesc d = desc[p];
-    const int m = d.dm >> 16, td = d.dm & 0xffff;
-    const int4 o = swoff[p];  // o.z == td - g lower blocks outside the tile
     if (lane < 4) rv = (AT)r[4 * (int64_t)p + lane];
-    bsr_dot<G, K, 16>(bcol + d.bptr, sw + 16 * (int64_t)o.x, 4, y, m, 0, o.z, lane, acc);
+    bsr_dot<G, K, 16>(bcol + rec.y, sw + 16 * (int64_t)rec.w, 4, y, rec.z >> 16, 0, rec.z & 0xffff, lane, acc);
   }
 #pragma unroll
   for (int a = 0; a < 4; ++a) acc[a] = group_sum<G>(acc[a]);
@@ -944,11 +951,10 @@ __global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_lsolve(const int32_t*
 
 // Backward sweep of one colour, in place: z = U^{-1} y on its tiles.
 template <typename T, typename LT, typename IT, int G, int K>
-__global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_usolve(const int32_t* __restrict__ tstart, int tfirst,
-                                                                  int tcount, const BDesc* __restrict__ desc,
+__global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_usolve(int tfirst, int tcount,
+                                                                  const int4* __restrict__ trow,
                                                                   const int32_t* __restrict__ bcol,
                                                                   const LT* __restrict__ sw,
-                                                                  const int4* __restrict__ swoff,
                                                                   const IT* __restrict__ tinv, T* __restrict__ y) {
   using AT = TriAcc<T, LT>;
   __shared__ AT Bv[kTileN];
@@ -957,10 +963,9 @@ __global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_usolve(const int32_t*
   const int tix = blockIdx.x;
   if (tix >= tcount) return;
   const int tg = tfirst + tix;
-  const int ts = tstart[tg];
-  const int tsz = tstart[tg + 1] - ts;
-  const bool act = g < tsz;
-  const int32_t p = ts + g;
+  const int4 rec = trow[(int64_t)tg * kTileMax + g];
+  const bool act = rec.x >= 0;
+  const int32_t p = rec.x;
   const IT* inv = tinv + (int64_t)tg * kTileInv;
   const int i = 4 * g + lane;
   // U^{-1} of the tile, loaded before the sums over the blocks outside the tile
@@ -969,11 +974,8 @@ __global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_usolve(const int32_t*
   AT acc[4] = {0, 0, 0, 0};
   AT rv = 0;
   if (act) {
-    const BDesc d = desc[p];
-    const int m = d.dm >> 16, td = d.dm & 0xffff;
-    const int4 o = swoff[p];  // o.w == td + 1 + (intra-tile upper blocks)
     if (lane < 4) rv = (AT)y[4 * (int64_t)p + lane];
-    bsr_dot<G, K, 16>(bcol + d.bptr + o.w, sw + 16 * (int64_t)o.y, 4, y, m, 0, d.nb - o.w, lane, acc);
+    bsr_dot<G, K, 16>(bcol + rec.y, sw + 16 * (int64_t)rec.w, 4, y, rec.z >> 16, 0, rec.z & 0xffff, lane, acc);
   }
 #pragma unroll
   for (int a = 0; a < 4; ++a) acc[a] = group_sum<G>(acc[a]);
@@ -1017,15 +1019,15 @@ constexpr int kTileDefault = 1;
 template <typename T, typename LT, bool UPPER, int G, int K>
 static void tile_launch_g(hipStream_t st, const IluPlan& P, int t0, int cnt, const LT* sw, const void* tinv,
                           const T* r, T* y, int fuse_u) {
-  // the tile inverses are FP64 beside an FP64 factor, FP32 otherwise
-  using IT = typename std::conditional<std::is_same<LT, double>::value, double, float>::type;
+  // the tile inverses are stored like the factor: FP64, FP32 or BF16
+  using IT = LT;
   const IT* ti = (const IT*)tinv;
   if constexpr (UPPER)
     hipLaunchKernelGGL((k_bsr_tile_usolve<T, LT, IT, G, K>), dim3((unsigned)cnt), dim3(G * kTileMax), 0, st,
-                       P.d_tileStart, t0, cnt, P.d_bdesc, P.d_bcol, sw, P.d_swoff, ti, y);
+                       t0, cnt, P.d_trowU, P.d_bcol, sw, ti, y);
   else
     hipLaunchKernelGGL((k_bsr_tile_lsolve<T, LT, IT, G, K>), dim3((unsigned)cnt), dim3(G * kTileMax), 0, st,
-                       P.d_tileStart, t0, cnt, P.d_bdesc, P.d_bcol, sw, P.d_swoff, ti, r, y, fuse_u);
+                       t0, cnt, P.d_trowL, P.d_bcol, sw, ti, r, y, fuse_u);
 }
 
 template <typename T, typename LT, bool UPPER>
@@ -1239,7 +1241,7 @@ LinAlg::~LinAlg() {
   void* ptrs[] = {d_blkStart, d_diagpos, d_part, d_scal, r, rh, p, v, s, t, ph, sh, x, w, z, pr, pz,
                   r32, rh32, p32, v32, s32, t32, ph32, sh32, x32, du32, w32, z32, pr32, pz32,
                   dinv, dinv32, binv, binv32, lu, lu32, gm_V, gm_V32, d_desc, lu16, kp, kp32, d_h, d_mpart,
-                  tinv, tinv32, sw, sw32, sw16};
+                  tinv, tinv32, tinv16, sw, sw32, sw16};
   delete ilu;
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
@@ -1484,7 +1486,7 @@ static std::vector<BlkDesc> make_desc(const std::vector<int32_t>& bs, const std:
 
 IluPlan::~IluPlan() {
   void* ptrs[] = {d_levL, d_levU, d_bdesc, d_bcol, d_bcolofs, d_r0orig, d_padmap, d_natrows, d_tileStart,
-                  d_swoff};
+                  d_swoff, d_trowL, d_trowU};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
 }
@@ -1586,7 +1588,7 @@ static int build_ilu_plan(sfem_ctx* ctx, LinAlg* L, int reorder) {
   L->ilu = nullptr;
   // factor-sized buffers belong to the previous plan
   void** vals[] = {(void**)&L->kp, (void**)&L->kp32, (void**)&L->lu, (void**)&L->lu32, (void**)&L->lu16,
-                   (void**)&L->tinv, (void**)&L->tinv32, (void**)&L->sw, (void**)&L->sw32,
+                   (void**)&L->tinv, (void**)&L->tinv32, (void**)&L->tinv16, (void**)&L->sw, (void**)&L->sw32,
                    (void**)&L->sw16};
   for (void** q : vals)
     if (*q) { (void)hipFree(*q); *q = nullptr; }
@@ -1789,6 +1791,21 @@ static int build_ilu_plan(sfem_ctx* ctx, LinAlg* L, int reorder) {
     }
     P.sw_blocks = nlo + nup;
     if ((rc = upload(&P.d_swoff, swoff))) return rc;
+    // the sweeps' per (tile, row) records: {p, first bcol entry, count | m << 16, first
+    // block of the copy} for the blocks outside the tile, lower (forward) and upper
+    // (backward); rows past a tile's size hold p = -1
+    std::vector<int4> trL((size_t)ntl * kTileMax, make_int4(-1, 0, 0, 0)), trU = trL;
+    for (int64_t t = 0; t < ntl; ++t) {
+      const int32_t ts = P.h_tileStart[t], te = P.h_tileStart[t + 1];
+      for (int32_t q = ts; q < te; ++q) {
+        const int4 o = swoff[q];
+        const int32_t cnt = (int32_t)(bptr[q + 1] - bptr[q]), m = bd[q].dm >> 16;
+        const size_t k = (size_t)t * kTileMax + (q - ts);
+        trL[k] = make_int4(q, (int32_t)bptr[q], o.z | (m << 16), o.x);
+        trU[k] = make_int4(q, (int32_t)(bptr[q] + o.w), (cnt - o.w) | (m << 16), o.y);
+      }
+    }
+    if ((rc = upload(&P.d_trowL, trL)) || (rc = upload(&P.d_trowU, trU))) return rc;
   }
   return SFEM_OK;
 }
@@ -1932,9 +1949,9 @@ int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f, 
       const int64_t ni = (int64_t)ntl * kTileInv;
       int rc;
       if (L->pc_bf16) {
-        if ((rc = ensure_vec(&L->tinv32, ni))) return rc;
-        hipLaunchKernelGGL((k_tile_invert<bf16, float>), dim3(ntl), dim3(64), 0, st, P.d_tileStart, ntl, P.d_bdesc,
-                           L->lu16, L->tinv32, L->d_scal);
+        if ((rc = ensure_vec(&L->tinv16, ni))) return rc;
+        hipLaunchKernelGGL((k_tile_invert<bf16, bf16>), dim3(ntl), dim3(64), 0, st, P.d_tileStart, ntl, P.d_bdesc,
+                           L->lu16, L->tinv16, L->d_scal);
       } else if (L->pc_fp32) {
         if ((rc = ensure_vec(&L->tinv32, ni))) return rc;
         hipLaunchKernelGGL((k_tile_invert<float, float>), dim3(ntl), dim3(64), 0, st, P.d_tileStart, ntl, P.d_bdesc,
@@ -2032,7 +2049,9 @@ static void ilu_solve(sfem_ctx* ctx, const IluPlan& P, const LT* lu, const T* r,
     // every launch (forward colours, then backward colours)
     const int nc = (int)P.tileColPtr.size() - 1;
     const int tf = P.tile_fuse_last ? 1 : 0;
-    const void* ti = std::is_same<LT, double>::value ? (const void*)L->tinv : (const void*)L->tinv32;
+    const void* ti = std::is_same<LT, double>::value  ? (const void*)L->tinv
+                     : std::is_same<LT, float>::value ? (const void*)L->tinv32
+                                                      : (const void*)L->tinv16;
     // the sweeps read the block-by-block copy of the factor's blocks outside the tiles
     const LT* sw = std::is_same<LT, double>::value  ? (const LT*)(const void*)L->sw
                    : std::is_same<LT, float>::value ? (const LT*)(const void*)L->sw32
@@ -2081,7 +2100,7 @@ static bool ilu_buffers_ready(const LinAlg* L) {
   const void* lu = L->pc_bf16 ? (const void*)L->lu16 : L->pc_fp32 ? (const void*)L->lu32 : (const void*)L->lu;
   if (!lu) return false;
   if (!L->ilu->tiled) return true;
-  const void* ti = (L->pc_fp32 || L->pc_bf16) ? (const void*)L->tinv32 : (const void*)L->tinv;
+  const void* ti = L->pc_bf16 ? (const void*)L->tinv16 : L->pc_fp32 ? (const void*)L->tinv32 : (const void*)L->tinv;
   const void* sw = L->pc_bf16 ? (const void*)L->sw16 : L->pc_fp32 ? (const void*)L->sw32 : (const void*)L->sw;
   return ti && sw;
 }

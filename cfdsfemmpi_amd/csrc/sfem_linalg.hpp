@@ -52,6 +52,11 @@ struct IluPlan {
   // in plan order, then the upper parts
   int4* d_swoff = nullptr;
   int64_t sw_blocks = 0;
+  // per (tile, row) records of the sweeps, kTileMax per tile: {block row (-1 past the
+  // tile), first of its blocks outside the tile in d_bcol, their count | m << 16, their
+  // first block in the copy}, forward (lower) and backward (upper)
+  int4* d_trowL = nullptr;
+  int4* d_trowU = nullptr;
   ~IluPlan();
 };
 
@@ -112,10 +117,11 @@ struct LinAlg {
   double* lu = nullptr;
   float* lu32 = nullptr;
   // tiled plans: per tile the inverses of the tile's unit-lower and upper triangular
-  // factors (kTileInv values: two packed 32 x 32 triangles), FP64 for an FP64 factor and
-  // FP32 for an FP32 / BF16 one
+  // factors (kTileInv values: two packed 32 x 32 triangles), computed in FP64 and stored
+  // like the factor (FP64, FP32 or BF16)
   double* tinv = nullptr;
   float* tinv32 = nullptr;
+  bf16* tinv16 = nullptr;
   double* sw = nullptr;    // tile sweeps' off-tile factor blocks (d_swoff), FP64 / FP32 / BF16
   float* sw32 = nullptr;
   bf16* sw16 = nullptr;

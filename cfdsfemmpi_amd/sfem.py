@@ -353,10 +353,10 @@ class Context:
         _check(self.lib.sfem_solver_stats(self.h, C.byref(v), C.byref(w)), "sfem_solver_stats")
         return v.value, w.value
 
-    def set_inner_tolerance(self, theta: float, rtol_min: float = 0.0):
+    def set_inner_tolerance(self, theta: float, rtol_max: float = 0.5):
         """Mixed paths: each Newton iteration's inner rtol becomes
-        max(rtol_min, min(opts.rtol, theta * newton_tol / ||F||)) (theta 0: fixed rtol)."""
-        _check(self.lib.sfem_set_inner_tolerance(self.h, float(theta), float(rtol_min)),
+        min(rtol_max, max(opts.rtol, theta * newton_tol / ||F||)) (theta 0: opts.rtol)."""
+        _check(self.lib.sfem_set_inner_tolerance(self.h, float(theta), float(rtol_max)),
                "sfem_set_inner_tolerance")
 
     def set_pc_lag(self, lag: int):

@@ -192,13 +192,15 @@ int sfem_set_pc_lag(sfem_ctx* ctx, int lag);
 
 /* Inexact Newton for the mixed-precision paths (SFEM_PREC_MIXED, SFEM_PREC_STORE32): in
  * sfem_step each Newton iteration's inner solve stops at
- *   rtol_k = max(rtol_min, min(opts->rtol, theta * newton_tol / ||F_k||)),
- * so that no solve is asked for more reduction than the Newton test still needs (theta
- * < 1 leaves room for the FP32 solve's true residual lagging its recurrence residual; a
- * solve that stops short is followed by another Newton iteration, as before).  theta = 0
- * (the default) solves every iteration to opts->rtol.  The FP64 path ignores it.  No
+ *   rtol_k = min(rtol_max, max(opts->rtol, theta * newton_tol / ||F_k||)),
+ * so that no solve is asked for more reduction than the Newton test still needs, and
+ * none for more than opts->rtol, the deepest reduction an FP32 solve is trusted with
+ * (theta < 1 leaves room for the FP32 solve's true residual lagging its recurrence
+ * residual; a solve that stops short is followed by another Newton iteration, as
+ * before).  0 < rtol_max < 1 keeps every solve reducing its residual.  theta = 0 (the
+ * default) solves every iteration to opts->rtol.  The FP64 path ignores it.  No
  * counterpart in the reference, whose KSP tolerance is fixed by the options file. */
-int sfem_set_inner_tolerance(sfem_ctx* ctx, double theta, double rtol_min);
+int sfem_set_inner_tolerance(sfem_ctx* ctx, double theta, double rtol_max);
 
 /* Start a time step (StabFEM2.cpp:632-648): td[] from SetTimeParametersFluid,
  * SolutionData::timeUpdate (SolutionData.cpp:64-78), and

@@ -290,12 +290,23 @@ class OracleFEM:
             raise RuntimeError(f"Krylov solver did not converge (rc={rc}, its={its}, relres={rr})")
         return du, rc, its
 
-    def solve_fully_implicit(self, newton_tol=1.0e-8, max_iter=10, steps=None):
-        """StabFEM::solveFullyImplicit (StabFEM2.cpp:593-832) on one rank."""
+    def solve_fully_implicit(self, newton_tol=1.0e-8, max_iter=10, steps=None, reuse_K=False, timers=None):
+        """StabFEM::solveFullyImplicit (StabFEM2.cpp:593-832) on one rank.
+
+        reuse_K: assemble the tangent at iteration 0 only and the residual alone after
+        it.  The residual is affine in the state (the tangent depends on the previous
+        step only), so the reference's re-assembled tangent is the same matrix; the HIP
+        path does the same.  timers: a dict that receives the wall time of the element
+        loops ("assembly") and of the linear solves ("solver"), the split of the
+        reference's MPI_Wtime phase timers (StabFEM.cpp:711-727)."""
+        import time as _time
         c = self.ctl
         m = self.mesh
         stepsMax = c.stepsMax if steps is None else steps
         timeNow = c.dt
+        if timers is not None:
+            timers.setdefault("assembly", 0.0)
+            timers.setdefault("solver", 0.0)
         for tstep in range(stepsMax):
             self.td = time_params(c.tis, c.rhoInf, c.dt)
             self.time_update()
@@ -303,7 +314,10 @@ class OracleFEM:
             self.assign_boundary_conditions(timeFact)
             for it in range(max_iter):
                 self.update_iter_step()
-                self.assemble(it)
+                t0 = _time.perf_counter()
+                self.assemble(it, want_K=(it == 0 or not reuse_K))
+                if timers is not None:
+                    timers["assembly"] += _time.perf_counter() - t0
                 if it == 0:
                     for k in range(m.nDBC):
                         jj = int(m.dbc_node[k]) * self.ndof + int(m.dbc_dof[k])
@@ -312,7 +326,11 @@ class OracleFEM:
                 self.log.append((tstep, it, norm_rhs))
                 if norm_rhs < newton_tol:
                     break
+                t0 = _time.perf_counter()
                 du, rc, its = self.linear_solve()
+                if timers is not None:
+                    timers["solver"] += _time.perf_counter() - t0
+                    timers["krylov_its"] = timers.get("krylov_its", 0) + its
                 self.soln[self.assyForSoln] += du
             timeNow = timeNow + c.dt
             if timeNow > c.timeFinal:

@@ -453,7 +453,12 @@ def test_ilu_apply_vs_factor(reorder, mesh_kind, prec, bf16):
     z = ctx.pc_apply(r)
     err = np.linalg.norm(z - z_ref) / np.linalg.norm(z_ref)
     print(f"[ilu-apply] rel diff {err:.2e}", flush=True)
-    assert err <= (1e-12 if prec == "fp64" else 1e-4)
+    # FP32/BF16 factors are exported widened, so the level sweeps match their substitution
+    # to rounding; a tiled plan applies each tile's own triangle through its inverse,
+    # computed in FP64 and stored like the factor: FP32 (1e-4), or BF16, whose 2^-9
+    # relative rounding of every inverse entry moves the result by up to ~1e-3
+    bar = 1e-12 if prec == "fp64" else (5e-3 if (bf16 and reorder == 3) else 1e-4)
+    assert err <= bar
     ctx.close()
 
 
