@@ -569,6 +569,74 @@ __device__ inline V4<AT> ld4(const bf16* p) {
            (AT)__uint_as_float(u.y << 16), (AT)__uint_as_float(u.y & 0xffff0000u)}};
 }
 
+// A whole 4 x 4 block stored block by block (16 consecutive values, rows a >= m zero):
+// the widest loads the storage type allows (two 16-byte loads in BF16, four in FP32).
+template <typename AT>
+struct Blk16 {
+  V4<AT> r[4];
+};
+template <typename AT>
+__device__ inline Blk16<AT> ld_blk16(const double* p) {
+  Blk16<AT> b;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) b.r[a] = ld4<AT>(p + 4 * a);
+  return b;
+}
+template <typename AT>
+__device__ inline Blk16<AT> ld_blk16(const float* p) {
+  Blk16<AT> b;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) b.r[a] = ld4<AT>(p + 4 * a);
+  return b;
+}
+template <typename AT>
+__device__ inline Blk16<AT> ld_blk16(const bf16* p) {
+  const uint4 u0 = reinterpret_cast<const uint4*>(p)[0], u1 = reinterpret_cast<const uint4*>(p)[1];
+  const uint32_t w[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
+  Blk16<AT> b;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+    b.r[a] = {{(AT)__uint_as_float(w[2 * a] << 16), (AT)__uint_as_float(w[2 * a] & 0xffff0000u),
+               (AT)__uint_as_float(w[2 * a + 1] << 16), (AT)__uint_as_float(w[2 * a + 1] & 0xffff0000u)}};
+  return b;
+}
+
+// acc[a] += sum over the blocks of the copy, V[16 t ..), block t against x(bc[t]), for
+// t in [t0, t1): every row of a block loaded with the block's wide loads (rows a >= m
+// of the copy are zero, so they add nothing).  Lanes of a group of G step over t; the
+// first K blocks of every lane are loaded before any is used.
+template <int G, int K, typename VT, typename XT, typename AT>
+__device__ inline void blk16_dot(const int32_t* __restrict__ bc, const VT* __restrict__ V,
+                                 const XT* __restrict__ x, int t0, int t1, int lane, AT (&acc)[4]) {
+  int32_t c[K];
+  Blk16<AT> v[K];
+#pragma unroll
+  for (int s = 0; s < K; ++s) {
+    const int t = t0 + lane + s * G;
+    const bool ok = t < t1;
+    c[s] = ok ? bc[t] : -1;
+    if (ok) v[s] = ld_blk16<AT>(V + 16 * (int64_t)t);
+  }
+  V4<AT> xv[K];
+#pragma unroll
+  for (int s = 0; s < K; ++s) xv[s] = c[s] >= 0 ? ld4<AT>(x + 4 * (int64_t)c[s]) : V4<AT>{{0, 0, 0, 0}};
+#pragma unroll
+  for (int s = 0; s < K; ++s)
+    if (c[s] >= 0) {
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+        acc[a] += v[s].r[a].v[0] * xv[s].v[0] + v[s].r[a].v[1] * xv[s].v[1] + v[s].r[a].v[2] * xv[s].v[2] +
+                  v[s].r[a].v[3] * xv[s].v[3];
+    }
+  for (int t = t0 + lane + K * G; t < t1; t += G) {
+    const V4<AT> xx = ld4<AT>(x + 4 * (int64_t)bc[t]);
+    const Blk16<AT> w = ld_blk16<AT>(V + 16 * (int64_t)t);
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+      acc[a] += w.r[a].v[0] * xx.v[0] + w.r[a].v[1] * xx.v[1] + w.r[a].v[2] * xx.v[2] + w.r[a].v[3] * xx.v[3];
+  }
+}
+
 // acc[a] += sum over blocks t in [t0, t1) of V(a, t) . x(bcol[t]) for a < m, row a of
 // block t at V + a rs + BS t (BS = 4: a block row's rows interleaved across its blocks;
 // BS = 16, rs = 4: block by block).  Lanes of a group of G step over t; the first K blocks of every lane are loaded (index, x and m
@@ -771,8 +839,8 @@ __host__ __device__ constexpr int tri_up(int i) { return kTileTri + i * kTileN -
 // wavefront, the workgroup barrier otherwise.
 template <int G>
 __device__ inline void tile_sync() {
-  if constexpr (G * kTileMax <= 64) wave_sync();
-  else __syncthreads();
+  if constexpr (G * kTileMax <= 64) wave_sync();  // the tile is one wavefront (or part of one)
+  else __syncthreads();                            // one tile per workgroup
 }
 
 // Inverses of the triangular factors of each tile's own 32 x 32 block (padding slots and
@@ -900,20 +968,32 @@ __device__ inline AT tile_tri_row(const IT* row, const AT* v, int lane, int lo, 
 // tile's rows), first of its blocks outside the tile in bcol, their count | m << 16, their
 // first block in the sweeps' block-by-block copy}, so a workgroup reads its rows' data with
 // one load indexed by the tile.
+// Tiles per workgroup: a tile takes G * kTileMax lanes.  Tiles of at most one wavefront
+// are packed 256 lanes to a workgroup (G = 4: eight tiles, two per wavefront; G = 8:
+// four, one per wavefront), since a CU holds a limited number of workgroups and
+// one-wavefront workgroups would leave its wavefront slots half empty.  A tile never
+// spans two wavefronts then, so its exchanges need only wavefront-scope ordering and a
+// tile past the launch's last one leaves without stalling the others.
+template <int G>
+__host__ __device__ constexpr int tile_tpw() {
+  return G * kTileMax <= 64 ? 256 / (G * kTileMax) : 1;
+}
+
 template <typename T, typename LT, typename IT, int G, int K>
-__global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_lsolve(int tfirst, int tcount,
-                                                                  const int4* __restrict__ trow,
-                                                                  const int32_t* __restrict__ bcol,
-                                                                  const LT* __restrict__ sw,
-                                                                  const IT* __restrict__ tinv,
-                                                                  const T* __restrict__ r, T* __restrict__ y,
-                                                                  int fuse_u) {
+__global__ __launch_bounds__(G * kTileMax * tile_tpw<G>()) void k_bsr_tile_lsolve(
+    int tfirst, int tcount, const int4* __restrict__ trow, const int32_t* __restrict__ bcol,
+    const LT* __restrict__ sw, const IT* __restrict__ tinv, const T* __restrict__ r, T* __restrict__ y,
+    int fuse_u) {
   using AT = TriAcc<T, LT>;
-  __shared__ AT Bv[kTileN];
-  __shared__ __align__(16) IT Tri[kTileTri];
-  const int g = threadIdx.x / G, lane = threadIdx.x % G;
-  const int tix = blockIdx.x;
+  constexpr int TPW = tile_tpw<G>(), NT = G * kTileMax;
+  __shared__ AT BvA[TPW][kTileN];
+  __shared__ __align__(16) IT TriA[TPW][kTileTri];
+  const int sub = threadIdx.x / NT, tid = threadIdx.x % NT;
+  const int g = tid / G, lane = tid % G;
+  const int tix = blockIdx.x * TPW + sub;
   if (tix >= tcount) return;
+  AT* Bv = BvA[sub];
+  IT* Tri = TriA[sub];
   const int tg = tfirst + tix;
   const int4 rec = trow[(int64_t)tg * kTileMax + g];
   const bool act = rec.x >= 0;
@@ -921,17 +1001,17 @@ __global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_lsolve(int tfirst, in
   const IT* inv = tinv + (int64_t)tg * kTileInv;
   const int i = 4 * g + lane;
   // L^{-1} of the tile, loaded before the sums over the blocks outside the tile
-  TriStage<IT, G * kTileMax> lst;
-  lst.load(inv, threadIdx.x);
+  TriStage<IT, NT> lst;
+  lst.load(inv, tid);
   AT acc[4] = {0, 0, 0, 0};
   AT rv = 0;
   if (act) {
     if (lane < 4) rv = (AT)r[4 * (int64_t)p + lane];
-    bsr_dot<G, K, 16>(bcol + rec.y, sw + 16 * (int64_t)rec.w, 4, y, rec.z >> 16, 0, rec.z & 0xffff, lane, acc);
+    blk16_dot<G, K>(bcol + rec.y, sw + 16 * (int64_t)rec.w, y, 0, rec.z & 0xffff, lane, acc);
   }
 #pragma unroll
   for (int a = 0; a < 4; ++a) acc[a] = group_sum<G>(acc[a]);
-  lst.store(Tri, threadIdx.x);
+  lst.store(Tri, tid);
   if (lane < 4) Bv[4 * g + lane] = act ? rv - pick4(acc, lane) : (AT)0;
   tile_sync<G>();
   const int ii = 4 * g + (lane & 3);  // the row this lane helps with (ii == i for lane < 4)
@@ -940,7 +1020,7 @@ __global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_lsolve(int tfirst, in
     // the tile's backward sweep: U^{-1} staged like L^{-1} (padding rows and columns of
     // the inverse are identity, so the padded right-hand sides, 0, add nothing)
     tile_sync<G>();  // every lane is done with L^{-1} and Bv
-    for (int q = threadIdx.x; q < TriStage<IT, G * kTileMax>::NV; q += G * kTileMax)
+    for (int q = tid; q < TriStage<IT, NT>::NV; q += NT)
       reinterpret_cast<uint4*>(Tri)[q] = reinterpret_cast<const uint4*>(inv + kTileTri)[q];
     if (lane < 4) Bv[i] = yi;
     tile_sync<G>();
@@ -951,17 +1031,19 @@ __global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_lsolve(int tfirst, in
 
 // Backward sweep of one colour, in place: z = U^{-1} y on its tiles.
 template <typename T, typename LT, typename IT, int G, int K>
-__global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_usolve(int tfirst, int tcount,
-                                                                  const int4* __restrict__ trow,
-                                                                  const int32_t* __restrict__ bcol,
-                                                                  const LT* __restrict__ sw,
-                                                                  const IT* __restrict__ tinv, T* __restrict__ y) {
+__global__ __launch_bounds__(G * kTileMax * tile_tpw<G>()) void k_bsr_tile_usolve(
+    int tfirst, int tcount, const int4* __restrict__ trow, const int32_t* __restrict__ bcol,
+    const LT* __restrict__ sw, const IT* __restrict__ tinv, T* __restrict__ y) {
   using AT = TriAcc<T, LT>;
-  __shared__ AT Bv[kTileN];
-  __shared__ __align__(16) IT Tri[kTileTri];
-  const int g = threadIdx.x / G, lane = threadIdx.x % G;
-  const int tix = blockIdx.x;
+  constexpr int TPW = tile_tpw<G>(), NT = G * kTileMax;
+  __shared__ AT BvA[TPW][kTileN];
+  __shared__ __align__(16) IT TriA[TPW][kTileTri];
+  const int sub = threadIdx.x / NT, tid = threadIdx.x % NT;
+  const int g = tid / G, lane = tid % G;
+  const int tix = blockIdx.x * TPW + sub;
   if (tix >= tcount) return;
+  AT* Bv = BvA[sub];
+  IT* Tri = TriA[sub];
   const int tg = tfirst + tix;
   const int4 rec = trow[(int64_t)tg * kTileMax + g];
   const bool act = rec.x >= 0;
@@ -969,17 +1051,17 @@ __global__ __launch_bounds__(G * kTileMax) void k_bsr_tile_usolve(int tfirst, in
   const IT* inv = tinv + (int64_t)tg * kTileInv;
   const int i = 4 * g + lane;
   // U^{-1} of the tile, loaded before the sums over the blocks outside the tile
-  TriStage<IT, G * kTileMax> ust;
-  ust.load(inv + kTileTri, threadIdx.x);
+  TriStage<IT, NT> ust;
+  ust.load(inv + kTileTri, tid);
   AT acc[4] = {0, 0, 0, 0};
   AT rv = 0;
   if (act) {
     if (lane < 4) rv = (AT)y[4 * (int64_t)p + lane];
-    bsr_dot<G, K, 16>(bcol + rec.y, sw + 16 * (int64_t)rec.w, 4, y, rec.z >> 16, 0, rec.z & 0xffff, lane, acc);
+    blk16_dot<G, K>(bcol + rec.y, sw + 16 * (int64_t)rec.w, y, 0, rec.z & 0xffff, lane, acc);
   }
 #pragma unroll
   for (int a = 0; a < 4; ++a) acc[a] = group_sum<G>(acc[a]);
-  ust.store(Tri, threadIdx.x);
+  ust.store(Tri, tid);
   if (lane < 4) Bv[4 * g + lane] = act ? rv - pick4(acc, lane) : (AT)0;
   tile_sync<G>();
   const int ii = 4 * g + (lane & 3);  // the row this lane helps with (ii == i for lane < 4)
@@ -1011,9 +1093,9 @@ __global__ __launch_bounds__(64) void k_tile_sweep_copy(int64_t nblk, const BDes
 
 // Tile sweep shapes: code c -> G = kTileGs[c] lanes per block row, K = kTileKs[c] blocks
 // per lane loaded together.
-static const int kTileGs[4] = {8, 16, 8, 16};
-static const int kTileKs[4] = {1, 1, 2, 2};
-constexpr int kTileCodes = 4;
+static const int kTileGs[6] = {8, 16, 8, 16, 4, 4};
+static const int kTileKs[6] = {1, 1, 2, 2, 2, 3};
+constexpr int kTileCodes = 6;
 constexpr int kTileDefault = 1;
 
 template <typename T, typename LT, bool UPPER, int G, int K>
@@ -1022,11 +1104,13 @@ static void tile_launch_g(hipStream_t st, const IluPlan& P, int t0, int cnt, con
   // the tile inverses are stored like the factor: FP64, FP32 or BF16
   using IT = LT;
   const IT* ti = (const IT*)tinv;
+  constexpr int tpw = tile_tpw<G>();
+  const unsigned gr = (unsigned)((cnt + tpw - 1) / tpw);
   if constexpr (UPPER)
-    hipLaunchKernelGGL((k_bsr_tile_usolve<T, LT, IT, G, K>), dim3((unsigned)cnt), dim3(G * kTileMax), 0, st,
+    hipLaunchKernelGGL((k_bsr_tile_usolve<T, LT, IT, G, K>), dim3(gr), dim3(G * kTileMax * tpw), 0, st,
                        t0, cnt, P.d_trowU, P.d_bcol, sw, ti, y);
   else
-    hipLaunchKernelGGL((k_bsr_tile_lsolve<T, LT, IT, G, K>), dim3((unsigned)cnt), dim3(G * kTileMax), 0, st,
+    hipLaunchKernelGGL((k_bsr_tile_lsolve<T, LT, IT, G, K>), dim3(gr), dim3(G * kTileMax * tpw), 0, st,
                        t0, cnt, P.d_trowL, P.d_bcol, sw, ti, r, y, fuse_u);
 }
 
@@ -1037,6 +1121,8 @@ static void tile_launch(hipStream_t st, int code, const IluPlan& P, int t0, int 
     case 0: tile_launch_g<T, LT, UPPER, 8, 1>(st, P, t0, cnt, sw, tinv, r, y, fuse_u); break;
     case 2: tile_launch_g<T, LT, UPPER, 8, 2>(st, P, t0, cnt, sw, tinv, r, y, fuse_u); break;
     case 3: tile_launch_g<T, LT, UPPER, 16, 2>(st, P, t0, cnt, sw, tinv, r, y, fuse_u); break;
+    case 4: tile_launch_g<T, LT, UPPER, 4, 2>(st, P, t0, cnt, sw, tinv, r, y, fuse_u); break;
+    case 5: tile_launch_g<T, LT, UPPER, 4, 3>(st, P, t0, cnt, sw, tinv, r, y, fuse_u); break;
     default: tile_launch_g<T, LT, UPPER, 16, 1>(st, P, t0, cnt, sw, tinv, r, y, fuse_u); break;
   }
 }
