@@ -1128,29 +1128,37 @@ static void tile_launch(hipStream_t st, int code, const IluPlan& P, int t0, int 
 }
 
 // Block values from the CSR values: block (p, t) is the neighbour's free columns in the
-// free rows of p, found in the DOF-ordered CSR at rowptr[r0orig[p] + a] + bcolofs.
-// One 64-lane wavefront per block row; dst1 (optional) receives a second copy.
+// free rows of p, found in the DOF-ordered CSR at rowptr[r0orig[p] + a] + (bcolofs &
+// 0xffffff); bcolofs >> 24 is the neighbour's free-column count, the columns the
+// padded 4-wide block row takes from the CSR (the rest are 0).  64 lanes per block row,
+// four block rows per workgroup; every lane stores one 4-wide block row with one vector
+// store.
 template <typename VT>
-__global__ __launch_bounds__(64) void k_bsr_gather(int64_t nblk, const BDesc* __restrict__ desc,
-                                                   const int32_t* __restrict__ bcol,
-                                                   const int32_t* __restrict__ bcolofs,
-                                                   const int32_t* __restrict__ r0orig,
-                                                   const int32_t* __restrict__ rowptr, const VT* __restrict__ src,
-                                                   VT* __restrict__ dst0, VT* __restrict__ dst1) {
-  const int64_t p = blockIdx.x;
+__global__ __launch_bounds__(256) void k_bsr_gather(int64_t nblk, const BDesc* __restrict__ desc,
+                                                    const int32_t* __restrict__ bcolofs,
+                                                    const int32_t* __restrict__ r0orig,
+                                                    const int32_t* __restrict__ rowptr, const VT* __restrict__ src,
+                                                    VT* __restrict__ dst) {
+  const int64_t p = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
   if (p >= nblk) return;
+  const int lane = threadIdx.x % 64;
   const BDesc d = desc[p];
   const int m = d.dm >> 16;
-  for (int idx = threadIdx.x; idx < m * d.nb; idx += 64) {
+  const int32_t r0 = r0orig[p];
+  for (int idx = lane; idx < m * d.nb; idx += 64) {
     const int a = idx / d.nb, t = idx - a * d.nb;
-    const int mj = desc[bcol[d.bptr + t]].dm >> 16;
-    const int64_t s0 = (int64_t)rowptr[r0orig[p] + a] + bcolofs[d.bptr + t];
-    const int64_t o = (int64_t)d.vbase + (int64_t)a * 4 * d.nb + 4 * t;
+    const int32_t oc = bcolofs[d.bptr + t];
+    const int mj = oc >> 24;
+    const int64_t s0 = (int64_t)rowptr[r0 + a] + (oc & 0xffffff);
+    VT v[4];
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const VT v = b < mj ? src[s0 + b] : (VT)0;
-      dst0[o + b] = v;
-      if (dst1) dst1[o + b] = v;
+    for (int b = 0; b < 4; ++b) v[b] = b < mj ? src[s0 + b] : (VT)0;
+    VT* o = dst + (int64_t)d.vbase + (int64_t)a * 4 * d.nb + 4 * t;
+    if constexpr (sizeof(VT) == 4) {
+      *reinterpret_cast<float4*>(o) = make_float4(v[0], v[1], v[2], v[3]);
+    } else {
+      reinterpret_cast<double2*>(o)[0] = make_double2(v[0], v[1]);
+      reinterpret_cast<double2*>(o)[1] = make_double2(v[2], v[3]);
     }
   }
 }
@@ -1164,8 +1172,10 @@ __global__ __launch_bounds__(64) void k_bsr_gather(int64_t nblk, const BDesc* __
 // block of c present in this row is updated, R_pj -= L_pc U_cj, one (block, row,
 // column) entry per lane.  Point ILU(0) on a pattern of full node blocks is exactly
 // this: the subtractions run in the point algorithm's pivot order, so the factor is the
-// same to the last bit.  Dynamic LDS: bsr_factor_lds(cap) bytes for rows of at most
-// cap <= kBsrMaxNb neighbour blocks.
+// same to the last bit.  A row starts from its values in a0 (the block tangent) and its
+// factor is written to lu, so no separate copy of the tangent seeds the factor.
+// Dynamic LDS: bsr_factor_lds(cap) bytes for rows of at most cap <= kBsrMaxNb
+// neighbour blocks.
 template <typename LT>
 __host__ __device__ constexpr size_t bsr_factor_lds(int cap) {
   return (size_t)cap * (16 * sizeof(double) + 16 * sizeof(LT) + 2 * sizeof(int32_t));
@@ -1175,7 +1185,7 @@ template <typename LT>
 __global__ __launch_bounds__(64) void k_bsr_factor(const int32_t* __restrict__ list, int first, int count,
                                                    const BDesc* __restrict__ desc,
                                                    const int32_t* __restrict__ bcol, LT* __restrict__ lu,
-                                                   double* __restrict__ scal, int cap) {
+                                                   const LT* __restrict__ a0, double* __restrict__ scal, int cap) {
   extern __shared__ double smem[];
   double* rows = smem;                          // [4][4*cap], this block row
   LT* urow = (LT*)(rows + 16 * cap);            // [4][4*cap], upper blocks of the lower block c
@@ -1192,7 +1202,7 @@ __global__ __launch_bounds__(64) void k_bsr_factor(const int32_t* __restrict__ l
   const int m = d.dm >> 16, td = d.dm & 0xffff, nb = d.nb, rs = 4 * nb;
   for (int t = lane; t < nb; t += 64) cols[t] = bcol[d.bptr + t];
   for (int a = 0; a < m; ++a)
-    for (int q = lane; q < rs; q += 64) rows[a * 4 * cap + q] = (double)lu[d.vbase + (int64_t)a * rs + q];
+    for (int q = lane; q < rs; q += 64) rows[a * 4 * cap + q] = (double)a0[d.vbase + (int64_t)a * rs + q];
   wave_sync();
   // registers of the prefetched lower block: upper block index, its 4 x m_c values, and
   // the diagonal block (lanes < 16)
@@ -1817,6 +1827,9 @@ static int build_ilu_plan(sfem_ctx* ctx, LinAlg* L, int reorder) {
     if (e == hipSuccess) e = hipMemcpy(P.d_bdesc, bd.data(), bd.size() * sizeof(BDesc), hipMemcpyHostToDevice);
     if (e != hipSuccess) return set_hip_error(e, "ILU block descriptors");
   }
+  // the gather's column offset of each neighbour block with the neighbour's free-column
+  // count in the top byte (row offsets stay below 4 * kBsrMaxNb)
+  for (int64_t q = 0; q < npairs; ++q) bcolofs[q] |= (bd[bcol[q]].dm >> 16) << 24;
   if ((rc = upload(&P.d_bcol, bcol)) || (rc = upload(&P.d_bcolofs, bcolofs)) || (rc = upload(&P.d_r0orig, r0orig)) ||
       (rc = upload(&P.d_padmap, padmap)) || (rc = upload(&P.d_levL, P.h_levL)) || (rc = upload(&P.d_levU, P.h_levU)))
     return rc;
@@ -1974,20 +1987,19 @@ int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f, 
     }
     const IluPlan& P = *L->ilu;
     const int64_t nv = P.nvals;
-    // one gather gives the block tangent (for the products of the Krylov solve) and the
-    // factor's starting values
-    const unsigned gb = (unsigned)std::max<int64_t>(P.nblk, 1);
+    // the gather gives the block tangent of the Krylov products; the factorisation reads
+    // its starting values from it
+    const unsigned gb = (unsigned)std::max<int64_t>((P.nblk + 3) / 4, 1);
     if (fp32) {
       int rc;
       if ((rc = ensure_vec(&L->kp32, nv)) || (rc = ensure_vec(&L->lu32, nv))) return rc;
-      hipLaunchKernelGGL((k_bsr_gather<float>), dim3(gb), dim3(64), 0, st, P.nblk, P.d_bdesc, P.d_bcol,
-                         P.d_bcolofs, P.d_r0orig, ctx->d_rowptr32, ctx->d_vals32, L->kp32,
-                         refactor ? L->lu32 : nullptr);
+      hipLaunchKernelGGL((k_bsr_gather<float>), dim3(gb), dim3(256), 0, st, P.nblk, P.d_bdesc, P.d_bcolofs,
+                         P.d_r0orig, ctx->d_rowptr32, ctx->d_vals32, L->kp32);
     } else {
       int rc;
       if ((rc = ensure_vec(&L->kp, nv)) || (rc = ensure_vec(&L->lu, nv))) return rc;
-      hipLaunchKernelGGL((k_bsr_gather<double>), dim3(gb), dim3(64), 0, st, P.nblk, P.d_bdesc, P.d_bcol,
-                         P.d_bcolofs, P.d_r0orig, ctx->d_rowptr32, ctx->d_vals, L->kp, refactor ? L->lu : nullptr);
+      hipLaunchKernelGGL((k_bsr_gather<double>), dim3(gb), dim3(256), 0, st, P.nblk, P.d_bdesc, P.d_bcolofs,
+                         P.d_r0orig, ctx->d_rowptr32, ctx->d_vals, L->kp);
     }
     // FP32 storage: the block tangent (and below, the FP64-computed factor) rounded to FP32
     auto to32 = [&](const double* a, float** b) -> int {
@@ -2012,10 +2024,10 @@ int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f, 
       if (!cnt) continue;
       if (fp32)
         hipLaunchKernelGGL((k_bsr_factor<float>), dim3(cnt), dim3(64), bsr_factor_lds<float>(cap), st, list, first,
-                           cnt, P.d_bdesc, P.d_bcol, L->lu32, L->d_scal, cap);
+                           cnt, P.d_bdesc, P.d_bcol, L->lu32, L->kp32, L->d_scal, cap);
       else
         hipLaunchKernelGGL((k_bsr_factor<double>), dim3(cnt), dim3(64), bsr_factor_lds<double>(cap), st, list,
-                           first, cnt, P.d_bdesc, P.d_bcol, L->lu, L->d_scal, cap);
+                           first, cnt, P.d_bdesc, P.d_bcol, L->lu, L->kp, L->d_scal, cap);
     }
     if (store32) {
       int rc = to32(L->lu, &L->lu32);
