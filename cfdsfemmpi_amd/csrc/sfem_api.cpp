@@ -702,8 +702,11 @@ int sfem_solve(sfem_ctx* c, const sfem_solver_opts* o, int* iters, double* relre
   if (!c || !o) return set_error(SFEM_ERR_ARG, "null argument");
   if (!(c->status & ST_ASSEMBLED)) return set_error(SFEM_ERR_STATE, "assemble the system first");  // SolverPetsc.cpp:288-292
   if (o->rtol <= 0.0 || o->max_it <= 0 || !(o->atol >= 0.0)) return set_error(SFEM_ERR_ARG, "bad solver options");
-  if (o->ksp_type != SFEM_KSP_BCGS && o->ksp_type != SFEM_KSP_GMRES) return set_error(SFEM_ERR_ARG, "bad ksp_type");
-  if (o->pc_type < SFEM_PC_NONE || o->pc_type > SFEM_PC_ILU0) return set_error(SFEM_ERR_ARG, "bad pc_type");
+  if (o->ksp_type != SFEM_KSP_BCGS && o->ksp_type != SFEM_KSP_GMRES && o->ksp_type != SFEM_KSP_PREONLY)
+    return set_error(SFEM_ERR_ARG, "bad ksp_type");
+  if (o->pc_type < SFEM_PC_NONE || o->pc_type > SFEM_PC_LU) return set_error(SFEM_ERR_ARG, "bad pc_type");
+  if ((o->ksp_type == SFEM_KSP_PREONLY) != (o->pc_type == SFEM_PC_LU))
+    return set_error(SFEM_ERR_UNSUPPORTED, "preonly runs with the LU direct solver, and LU only as preonly");
   HIP_OK(hipSetDevice(c->device));
   int it = 0;
   double rr = 0.0;
@@ -718,6 +721,12 @@ int sfem_solve(sfem_ctx* c, const sfem_solver_opts* o, int* iters, double* relre
   if (iters) *iters = it;
   if (relres) *relres = rr;
   return rc;
+}
+
+int sfem_direct_info(sfem_ctx* c, int* kl, int* ku) {
+  if (!c || !kl || !ku) return set_error(SFEM_ERR_ARG, "null argument");
+  if (!c->la) { *kl = *ku = 0; return SFEM_OK; }
+  return la_direct_info(c, kl, ku);
 }
 
 int sfem_setup_solver(sfem_ctx* c, const sfem_solver_opts* o) {

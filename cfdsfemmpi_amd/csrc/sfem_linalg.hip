@@ -1334,6 +1334,7 @@ static hipError_t alloc(T** p, size_t n) {
 }
 
 LinAlg::~LinAlg() {
+  la_direct_destroy(this);
   void* ptrs[] = {d_blkStart, d_diagpos, d_part, d_scal, r, rh, p, v, s, t, ph, sh, x, w, z, pr, pz,
                   r32, rh32, p32, v32, s32, t32, ph32, sh32, x32, du32, w32, z32, pr32, pz32,
                   dinv, dinv32, binv, binv32, lu, lu32, gm_V, gm_V32, d_desc, lu16, kp, kp32, d_h, d_mpart,
@@ -2529,6 +2530,16 @@ int la_norm2(sfem_ctx* ctx, const double* a, double* out) {
   return SFEM_OK;
 }
 
+int la_residual_norm(sfem_ctx* ctx, const double* b, const double* x, double* out) {
+  LinAlg* L = ctx->la;
+  int rc;
+  if ((rc = ensure_vec(&L->w, L->nvec))) return rc;
+  spmv_in<double>(ctx, x, L->w, false);
+  hipLaunchKernelGGL((k_axpby<double>), dim3(grid_for(L->n)), dim3(kTPB), 0, ctx->stream, L->n, 1.0, b, -1.0, L->w);
+  LA_CHECK(hipGetLastError());
+  return la_norm2(ctx, L->w, out);
+}
+
 // Internal status of bicgstab with a stall window: no 10% improvement of the best
 // residual for `stall` iterations.
 constexpr int kStagnated = -100;
@@ -2882,6 +2893,7 @@ int la_set_tuning_table(sfem_ctx* ctx, int fp32, const int* table, int n) {
 // block pattern and level sets), so that the first solve does not pay for it.
 int la_prepare(sfem_ctx* ctx, const sfem_solver_opts* o) {
   LinAlg* L = ctx->la;
+  if (o->pc_type == SFEM_PC_LU) return la_direct_setup(ctx);
   if (o->pc_type != SFEM_PC_ILU0) return SFEM_OK;
   if (L->ilu && L->ilu->reorder == o->reorder) return SFEM_OK;
   return build_ilu_plan(ctx, L, o->reorder);
@@ -3123,6 +3135,7 @@ int la_subset_build(sfem_ctx* ctx, const uint8_t* keep, RowSubset* S, const RowS
 
 int la_solve(sfem_ctx* ctx, const sfem_solver_opts* o_in, int* iters, double* relres) {
   LinAlg* L = ctx->la;
+  if (o_in->pc_type == SFEM_PC_LU) return la_direct_solve(ctx, iters, relres);
   // PETSc's default test stops at ||r|| <= max(rtol ||b||, atol): with a zero initial
   // guess that is the relative test at rtol' = max(rtol, atol / ||b||)
   sfem_solver_opts oa = *o_in;

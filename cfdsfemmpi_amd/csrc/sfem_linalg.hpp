@@ -60,6 +60,8 @@ struct IluPlan {
   ~IluPlan();
 };
 
+struct DirectLU;  // banded LU of the direct solver (sfem_direct.hip)
+
 struct LinAlg {
   int64_t n = 0;             // free DOFs
   int64_t nblk = 0;          // nodal blocks (nodes with >= 1 free DOF)
@@ -128,6 +130,7 @@ struct LinAlg {
   // the tangent in the ILU(0) plan's block form, for the products of the Krylov solve
   double* kp = nullptr;
   float* kp32 = nullptr;
+  DirectLU* direct = nullptr;  // direct solver (pc_type LU), set up on first use
   ~LinAlg();
 };
 
@@ -178,6 +181,13 @@ int la_set_tuning_mode(sfem_ctx* ctx, int mode);
 int la_get_tuning_table(sfem_ctx* ctx, int fp32, int* table, int cap, int* n);
 int la_set_tuning_table(sfem_ctx* ctx, int fp32, const int* table, int n);
 int la_norm2(sfem_ctx* ctx, const double* a, double* out);
+// ||b - K x|| with the assembled FP64 CSR tangent
+int la_residual_norm(sfem_ctx* ctx, const double* b, const double* x, double* out);
+// direct solver (sfem_direct.hip)
+int la_direct_setup(sfem_ctx* ctx);
+int la_direct_solve(sfem_ctx* ctx, int* iters, double* relres);
+int la_direct_info(sfem_ctx* ctx, int* kl, int* ku);
+void la_direct_destroy(LinAlg* L);
 int la_spmv64(sfem_ctx* ctx, const double* x, double* y);
 int la_spmv32(sfem_ctx* ctx, const float* x, float* y);
 

@@ -22,6 +22,8 @@ import io
 import os
 from typing import Optional
 
+import warnings
+
 import numpy as np
 
 
@@ -179,8 +181,10 @@ def read_solver_options(path: str) -> SolverOptions:
     """Map a PETSc options file onto :class:`SolverOptions`.
 
     ``-pc_type bjacobi`` with ``-sub_pc_type ilu`` (input/petsc_options_bcgs.dat)
-    becomes ILU(0) on the single block a one-GPU run has; ``-pc_type lu``
-    (the test decks) becomes a direct solve.
+    becomes ILU(0) on the single block a one-GPU run has; fill levels other than 0
+    (``-sub_pc_factor_levels`` / ``-pc_factor_levels``) are not built, and a warning says
+    that ILU(0) runs instead.  ``-pc_type lu`` (the test decks) is kept as "lu"; the
+    driver maps it onto the device direct solver.
     """
     opts = {}
     with _open_text(path) as f:
@@ -204,6 +208,10 @@ def read_solver_options(path: str) -> SolverOptions:
         sub = opts.get("sub_pc_type", "ilu")
         pc = {"ilu": "ilu", "jacobi": "jacobi", "none": "none"}.get(sub, sub)
     so.pc_type = pc
+    for key in ("sub_pc_factor_levels", "pc_factor_levels"):
+        if key in opts and opts[key] not in ("", "0"):
+            warnings.warn(f"{path}: -{key} {opts[key]} asks for ILU({opts[key]}); ILU(0) runs instead",
+                          stacklevel=2)
     if "ksp_rtol" in opts:
         so.rtol = float(opts["ksp_rtol"])
     if "ksp_max_it" in opts:

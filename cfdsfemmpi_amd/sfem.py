@@ -30,8 +30,8 @@ PRED_PREVIOUS, PRED_EXTRAPOLATE, PRED_LEARNED, PRED_QUADRATIC = 0, 1, 2, 3
 LEARNER_NPARAM = 5190
 KT_ASSEMBLY, KT_SPMV, KT_PC = 0, 1, 2
 TUNE_TIMED, TUNE_FIXED = 0, 1
-KSP = {"bcgs": 0, "gmres": 1}
-PC = {"none": 0, "jacobi": 1, "bjacobi4": 2, "ilu": 3}
+KSP = {"bcgs": 0, "gmres": 1, "preonly": 2}
+PC = {"none": 0, "jacobi": 1, "bjacobi4": 2, "ilu": 3, "lu": 4}
 
 # (function name, restype, argtypes) of every entry point declared in include/sfem.h
 _i64p = C.POINTER(C.c_int64)
@@ -96,6 +96,7 @@ ABI = [
     ("sfem_set_tuning_mode", C.c_int, [_vp, C.c_int]),
     ("sfem_get_tuning_table", C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]),
     ("sfem_set_tuning_table", C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.c_int]),
+    ("sfem_direct_info", C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("sfem_info", C.c_int, [_vp, _i64p, _i64p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("sfem_last_timings", C.c_int, [_vp, _f64p, _f64p]),
     ("sfem_learner_create", _vp, [_vp, C.c_int64, C.c_uint64]),
@@ -419,6 +420,12 @@ class Context:
         buf = (C.c_int * max(len(table), 1))(*[int(v) for v in table])
         _check(self.lib.sfem_set_tuning_table(self.h, 1 if fp32 else 0, buf, len(table)), "sfem_set_tuning_table")
 
+    def direct_info(self):
+        """(kl, ku) of the direct solver's reordered tangent (0, 0 before an LU solve)."""
+        kl, ku = C.c_int(0), C.c_int(0)
+        _check(self.lib.sfem_direct_info(self.h, C.byref(kl), C.byref(ku)), "sfem_direct_info")
+        return kl.value, ku.value
+
     def info(self):
         nf, nz = C.c_int64(0), C.c_int64(0)
         nc, nd = C.c_int(0), C.c_int(0)
@@ -490,9 +497,9 @@ class Learner:
 def solver_opts(ksp="bcgs", pc="ilu", rtol=1e-8, max_it=1000, restart=30, reorder=0,
                 factor_bf16=0, atol=0.0) -> SolverOpts:
     if pc == "lu":
-        # the reference's test decks ask for MUMPS LU; a tight RCM-ordered ILU(0)-BiCGStab
-        # solve reproduces the direct answer to well below the decks' tolerances
-        ksp, pc, rtol, max_it, reorder = "bcgs", "ilu", 1e-13, max(max_it, 5000), 1
+        # the reference's test decks (-ksp_type preonly -pc_type lu, MUMPS): the device
+        # banded LU direct solve, whatever -ksp_type the options file names
+        ksp = "preonly"
     return SolverOpts(KSP[ksp], PC[pc], float(rtol), int(max_it), int(restart), int(reorder), int(factor_bf16),
                       float(atol))
 

@@ -48,9 +48,14 @@ enum {
  * for FP32 Krylov (ILU(0) only; with other preconditioners it is the FP64 path). */
 enum { SFEM_PREC_FP64 = 0, SFEM_PREC_MIXED = 1, SFEM_PREC_STORE32 = 2 };
 
-/* Krylov methods and preconditioners (vocabulary of the PETSc options files). */
-enum { SFEM_KSP_BCGS = 0, SFEM_KSP_GMRES = 1 };
-enum { SFEM_PC_NONE = 0, SFEM_PC_JACOBI = 1, SFEM_PC_BJACOBI4 = 2, SFEM_PC_ILU0 = 3 };
+/* Krylov methods and preconditioners (vocabulary of the PETSc options files).
+ * SFEM_KSP_PREONLY with SFEM_PC_LU is the direct solve of the reference's test decks
+ * (-ksp_type preonly -pc_type lu, testdata/ldc2dquad10/petsc_options.dat:1-3, MUMPS in
+ * the reference): a banded LU with partial pivoting on the device, in the reverse
+ * Cuthill-McKee order of the free-DOF graph, factorised once per new tangent (FP64
+ * only; sized for small decks: the band takes nfree x (2 kl + ku + 1) doubles). */
+enum { SFEM_KSP_BCGS = 0, SFEM_KSP_GMRES = 1, SFEM_KSP_PREONLY = 2 };
+enum { SFEM_PC_NONE = 0, SFEM_PC_JACOBI = 1, SFEM_PC_BJACOBI4 = 2, SFEM_PC_ILU0 = 3, SFEM_PC_LU = 4 };
 
 typedef struct sfem_solver_opts {
   int ksp_type;      /* SFEM_KSP_* (-ksp_type) */
@@ -375,6 +380,10 @@ int sfem_set_tuning_mode(sfem_ctx* ctx, int mode);
  * run is replayed exactly by another.  No counterpart in the reference. */
 int sfem_get_tuning_table(sfem_ctx* ctx, int fp32, int* table, int cap, int* n);
 int sfem_set_tuning_table(sfem_ctx* ctx, int fp32, const int* table, int n);
+
+/* Bandwidths of the direct solver's reordered tangent (kl sub- and ku super-diagonals;
+ * zeros before an SFEM_PC_LU solve was set up).  No counterpart in the reference. */
+int sfem_direct_info(sfem_ctx* ctx, int* kl, int* ku);
 
 /* Sizes and counts of the context. */
 int sfem_info(sfem_ctx* ctx, int64_t* nfree, int64_t* nnz, int* ncolors, int* ndof);

@@ -176,10 +176,12 @@ def test_ldc2dquad_mesh2_reference(golden):
     assert err < 2e-8
 
 
-def test_c1_config_vs_oracle():
+@pytest.mark.parametrize("solver", ["gmres-ilu", "lu"])
+def test_c1_config_vs_oracle(solver):
     """The 2D configuration as stated (10 x 10 Quad4, Re 100, SUPG/PSPG, p pinned at
-    (0, 0), generalised-alpha rho_inf = 0, dt 0.1, 50 steps, Newton 1e-10, ILU-GMRES):
-    every step's field within 1e-8 of the oracle's direct-solve run (StabFEM2.cpp:593-832)."""
+    (0, 0), generalised-alpha rho_inf = 0, dt 0.1, 50 steps, Newton 1e-10, ILU-GMRES or
+    the direct solve): every step's field within 1e-8 of the oracle's direct-solve run
+    (StabFEM2.cpp:593-832)."""
     mesh = meshgen.ldc2d_quad(10, p_pin=(0.0, 0.0))
     ctl = deck.Control(np.array([1.0, 0.01, 0, 0, 0, 0, 0, 0, 1.0, 1.0, 0.0]), 2, 0.0, 0.1, 1e9, 50, 1, 1e-6)
     fem = sfem.StabFEM()
@@ -187,7 +189,8 @@ def test_c1_config_vs_oracle():
     fem.setControl(ctl)
     fem.newton_tol = 1e-10
     fem.prepareInputData()
-    fem.setSolver(sfem.solver_opts("gmres", "ilu", 1e-13, 5000, 30, reorder=1))
+    fem.setSolver(sfem.solver_opts("gmres", "ilu", 1e-13, 5000, 30, reorder=1) if solver == "gmres-ilu"
+                  else sfem.solver_opts("preonly", "lu"))
     assert (fem.ctx.nfree, fem.ctx.nnz) == (282, 6366)
     o = oracle.OracleFEM(mesh, ctl)
     o.prepare()
@@ -351,6 +354,43 @@ def test_krylov_against_direct(ksp, pc, reorder):
     true_res = np.linalg.norm(rhs - A @ du) / np.linalg.norm(rhs)
     assert true_res <= 1.01e-10
     assert np.linalg.norm(du - x_ref) <= 1e-6 * np.linalg.norm(x_ref)
+    ctx.close()
+
+
+@pytest.mark.parametrize("mesh_kind", ["quad", "tria", "hex", "tet"])
+def test_direct_lu_against_scipy(mesh_kind):
+    """-ksp_type preonly -pc_type lu: the device banded LU (partial pivoting, RCM order)
+    against scipy's sparse direct solve of the same assembled system, to 1e-10, with a
+    true residual below 1e-12; the second solve with the same tangent reuses the factor."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    mesh = {"quad": lambda: meshgen.ldc2d_quad(12, graded=True),
+            "tria": lambda: meshgen.ldc2d_tria(14, jitter=0.2, seed=5),
+            "hex": lambda: meshgen.ldc3d_hex(5, graded=True),
+            "tet": lambda: meshgen.ldc3d_tet(3)}[mesh_kind]()
+    ctx = make_ctx(mesh, tis=2, dt=0.1, elemData=np.array([1.0, 1e-3, 0, 0, 0, 0, 0, 0, 1, 1, 0]))
+    ctx.time_update(1.0)
+    ctx.assemble(0)
+    vals, rhs = ctx.get_system()
+    rp, ci, _ = ctx.get_pattern()
+    A = sp.csr_matrix((vals, ci, rp), shape=(ctx.nfree, ctx.nfree))
+    o = sfem.solver_opts("preonly", "lu")
+    its, rr = ctx.solve(o)
+    du = ctx.get_increment()
+    kl, ku = ctx.direct_info()
+    x_ref = spla.spsolve(A.tocsc(), rhs)
+    err = np.linalg.norm(du - x_ref) / np.linalg.norm(x_ref)
+    print(f"[lu] {mesh_kind}: n={ctx.nfree} kl={kl} ku={ku} relres={rr:.1e} vs scipy {err:.1e}", flush=True)
+    assert its == 1 and rr <= 1e-12 and err <= 1e-10
+    assert 0 < kl < ctx.nfree and 0 < ku < ctx.nfree
+    ctx.update_solution()
+    ctx.assemble(1, want_K=False)  # same tangent: the factor is reused
+    _, rhs1 = ctx.get_system()
+    its, rr = ctx.solve(o)
+    x1 = spla.spsolve(A.tocsc(), rhs1)
+    assert np.linalg.norm(ctx.get_increment() - x1) <= 1e-10 * max(np.linalg.norm(x1), 1e-300) + 1e-300
+    with pytest.raises(sfem.SfemError):  # LU runs as preonly only
+        ctx.solve(sfem.SolverOpts(sfem.KSP["bcgs"], sfem.PC["lu"], 1e-8, 100, 30, 0, 0, 0.0))
     ctx.close()
 
 
