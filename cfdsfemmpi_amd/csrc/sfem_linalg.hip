@@ -872,16 +872,24 @@ __global__ __launch_bounds__(64) void k_tile_invert(const int32_t* __restrict__ 
     for (int b = 0; b < 4; ++b) A[4 * g + a][4 * h + b] = v.v[b];
   }
   wave_sync();
+  // Column j of each inverse is lane j's, held in registers while it is formed; the
+  // rows of A are read by every lane at once (one LDS broadcast per entry).  The sums
+  // run over all earlier (later) rows, whose entries of the column are 0 before the
+  // column's own row, so each entry takes the same subtractions, in the same order, as
+  // substitution from the column's diagonal on.
+  const int j = lane;
+  double xc[kTileN];
   if (lane < kTileN) {
-    const int j = lane;
     // L^{-1} e_j: unit lower triangle of A
-    for (int i = 0; i < kTileN; ++i) X[i][j] = 0.0;
-    X[j][j] = 1.0;
-    for (int i = j + 1; i < kTileN; ++i) {
+#pragma unroll
+    for (int i = 0; i < kTileN; ++i) {
       double x = 0.0;
-      for (int k = j; k < i; ++k) x -= A[i][k] * X[k][j];
-      X[i][j] = x;
+#pragma unroll
+      for (int k = 0; k < i; ++k) x -= A[i][k] * xc[k];
+      xc[i] = i < j ? 0.0 : (i == j ? 1.0 : x);
     }
+#pragma unroll
+    for (int i = 0; i < kTileN; ++i) X[i][j] = xc[i];
   }
   wave_sync();
   IT* out = tinv + (int64_t)t * kTileInv;
@@ -892,19 +900,19 @@ __global__ __launch_bounds__(64) void k_tile_invert(const int32_t* __restrict__ 
   }
   wave_sync();
   if (lane < kTileN) {
-    const int j = lane;
     // U^{-1} e_j: upper triangle of A with its diagonal
-    for (int i = 0; i < kTileN; ++i) X[i][j] = 0.0;
-    const double dj = A[j][j];
-    if (dj == 0.0) scal[S_MISC1] = 2.0;
-    X[j][j] = 1.0 / (dj == 0.0 ? 1.0 : dj);
-    for (int i = j - 1; i >= 0; --i) {
+#pragma unroll
+    for (int i = kTileN - 1; i >= 0; --i) {
       double x = 0.0;
-      for (int k = i + 1; k <= j; ++k) x -= A[i][k] * X[k][j];
+#pragma unroll
+      for (int k = i + 1; k < kTileN; ++k) x -= A[i][k] * xc[k];
       const double di = A[i][i];
-      if (di == 0.0) scal[S_MISC1] = 2.0;
-      X[i][j] = x / (di == 0.0 ? 1.0 : di);
+      if (i <= j && di == 0.0) scal[S_MISC1] = 2.0;
+      const double q = i == j ? 1.0 / (di == 0.0 ? 1.0 : di) : x / (di == 0.0 ? 1.0 : di);
+      xc[i] = i > j ? 0.0 : q;
     }
+#pragma unroll
+    for (int i = 0; i < kTileN; ++i) X[i][j] = xc[i];
   }
   wave_sync();
   for (int q = lane; q < kTileTri; q += 64) {
@@ -1163,54 +1171,67 @@ __global__ __launch_bounds__(256) void k_bsr_gather(int64_t nblk, const BDesc* _
   }
 }
 
-// ILU(0) factorisation of the block rows of one level, one 64-lane workgroup (one
-// wavefront) per block row, IKJ order by lower neighbour blocks, a 4 x 4 block at a time:
-// for lower block c (ascending), c's upper blocks and diagonal block are staged in LDS
-// (their global loads were issued before the previous block's work and sat in
-// registers meanwhile), the positions of c's upper blocks in this row are searched once,
-// L_pc = R_pc U_cc^{-1} is one triangular solve per row (lanes < m), and every upper
-// block of c present in this row is updated, R_pj -= L_pc U_cj, one (block, row,
-// column) entry per lane.  Point ILU(0) on a pattern of full node blocks is exactly
-// this: the subtractions run in the point algorithm's pivot order, so the factor is the
-// same to the last bit.  A row starts from its values in a0 (the block tangent) and its
-// factor is written to lu, so no separate copy of the tangent seeds the factor.
-// Dynamic LDS: bsr_factor_lds(cap) bytes for rows of at most cap <= kBsrMaxNb
-// neighbour blocks.
+// LDS of one row being factorised: its values in FP64 (rows[a][4 cap]), a pivot row's
+// upper blocks (urow, storage type), the row's neighbour blocks and the positions of the
+// pivot's upper blocks in it, and small per-pivot scratch.
 template <typename LT>
-__host__ __device__ constexpr size_t bsr_factor_lds(int cap) {
-  return (size_t)cap * (16 * sizeof(double) + 16 * sizeof(LT) + 2 * sizeof(int32_t));
-}
+struct FacRow {
+  double* rows;
+  LT* urow;
+  int32_t* cols;
+  int32_t* pos;
+  double* ub;    // [16] pivot's diagonal block
+  double* lblk;  // [16] L_pc, row-major
+  double* lval;  // [4]
+};
 
 template <typename LT>
-__global__ __launch_bounds__(64) void k_bsr_factor(const int32_t* __restrict__ list, int first, int count,
-                                                   const BDesc* __restrict__ desc,
-                                                   const int32_t* __restrict__ bcol, LT* __restrict__ lu,
-                                                   const LT* __restrict__ a0, double* __restrict__ scal, int cap) {
-  extern __shared__ double smem[];
-  double* rows = smem;                          // [4][4*cap], this block row
-  LT* urow = (LT*)(rows + 16 * cap);            // [4][4*cap], upper blocks of the lower block c
-  int32_t* cols = (int32_t*)(urow + 16 * cap);  // [cap], this row's neighbour blocks
-  int32_t* pos = cols + cap;                    // [cap], positions of c's upper blocks here
-  __shared__ double ub[16];                     // c's diagonal block
-  __shared__ double lval[4];
-  __shared__ double lblk[16];                   // L_pc, row-major
-  const int w = blockIdx.x;
-  if (w >= count) return;
-  const int lane = threadIdx.x;
-  const int32_t p = list ? list[first + w] : first + w;
-  const BDesc d = desc[p];
-  const int m = d.dm >> 16, td = d.dm & 0xffff, nb = d.nb, rs = 4 * nb;
-  for (int t = lane; t < nb; t += 64) cols[t] = bcol[d.bptr + t];
+__device__ inline FacRow<LT> fac_row_at(double* base, int cap) {
+  FacRow<LT> f;
+  f.rows = base;
+  f.urow = (LT*)(f.rows + 16 * cap);
+  f.cols = (int32_t*)(f.urow + 16 * cap);
+  f.pos = f.cols + cap;
+  f.ub = (double*)(((uintptr_t)(f.pos + cap) + 7) & ~(uintptr_t)7);
+  f.lblk = f.ub + 16;
+  f.lval = f.lblk + 16;
+  return f;
+}
+
+// bytes of one FacRow for rows of at most cap neighbour blocks (8-byte aligned)
+template <typename LT>
+__host__ __device__ constexpr size_t fac_row_bytes(int cap) {
+  return ((size_t)cap * (16 * sizeof(double) + 16 * sizeof(LT) + 2 * sizeof(int32_t)) + 7) / 8 * 8 +
+         36 * sizeof(double);
+}
+
+// Load block row d of a0 (the block tangent) and its neighbour list into f (one wavefront).
+template <typename LT>
+__device__ inline void fac_load_row(const FacRow<LT>& f, const BDesc& d, const int32_t* __restrict__ bcol,
+                                    const LT* __restrict__ a0, int cap, int lane) {
+  const int m = d.dm >> 16, nb = d.nb, rs = 4 * nb;
+  for (int t = lane; t < nb; t += 64) f.cols[t] = bcol[d.bptr + t];
   for (int a = 0; a < m; ++a)
-    for (int q = lane; q < rs; q += 64) rows[a * 4 * cap + q] = (double)a0[d.vbase + (int64_t)a * rs + q];
-  wave_sync();
-  // registers of the prefetched lower block: upper block index, its 4 x m_c values, and
-  // the diagonal block (lanes < 16)
+    for (int q = lane; q < rs; q += 64) f.rows[a * 4 * cap + q] = (double)a0[d.vbase + (int64_t)a * rs + q];
+}
+
+// IKJ steps of a row (one wavefront) for its lower blocks t in [t0, t1), whose rows are
+// final in lu: for lower block c, c's upper blocks and diagonal block are staged in LDS
+// (their global loads issued before the previous block's work and held in registers
+// meanwhile), the positions of c's upper blocks in this row are searched once, L_pc =
+// R_pc U_cc^{-1} is one triangular solve per row (lanes < m), and every upper block of
+// c present in this row is updated, R_pj -= L_pc U_cj, one (block, row, column) entry
+// per lane.
+template <typename LT>
+__device__ inline void fac_pivots_global(const FacRow<LT>& f, const BDesc& d, int t0, int t1,
+                                         const BDesc* __restrict__ desc, const int32_t* __restrict__ bcol,
+                                         const LT* __restrict__ lu, double* __restrict__ scal, int cap, int lane) {
+  const int m = d.dm >> 16, nb = d.nb;
   int32_t pc = -1;
   LT pu[4][4];
   LT pd = (LT)0;
   auto issue = [&](int t) {
-    const BDesc dc = desc[cols[t]];
+    const BDesc dc = desc[f.cols[t]];
     const int mc = dc.dm >> 16, tdc = dc.dm & 0xffff, rsc = 4 * dc.nb;
     const int nup = dc.nb - tdc - 1;
     if (lane < nup) {
@@ -1223,71 +1244,109 @@ __global__ __launch_bounds__(64) void k_bsr_factor(const int32_t* __restrict__ l
     }
     if (lane < 16) pd = (lane >> 2) < mc ? lu[dc.vbase + (int64_t)(lane >> 2) * rsc + 4 * tdc + (lane & 3)] : (LT)0;
   };
-  if (td > 0) issue(0);
-  for (int t = 0; t < td; ++t) {
-    const BDesc dc = desc[cols[t]];
+  if (t0 < t1) issue(t0);
+  for (int t = t0; t < t1; ++t) {
+    const BDesc dc = desc[f.cols[t]];
     const int mc = dc.dm >> 16, tdc = dc.dm & 0xffff;
     const int nup = dc.nb - tdc - 1;
     if (lane < nup) {
-      pos[lane] = lds_find(cols, nb, pc);
+      f.pos[lane] = lds_find(f.cols, nb, pc);
 #pragma unroll
       for (int b = 0; b < 4; ++b)
 #pragma unroll
-        for (int bp = 0; bp < 4; ++bp) urow[b * 4 * cap + 4 * lane + bp] = pu[b][bp];
+        for (int bp = 0; bp < 4; ++bp) f.urow[b * 4 * cap + 4 * lane + bp] = pu[b][bp];
     }
-    if (lane < 16) ub[lane] = (double)pd;
+    if (lane < 16) f.ub[lane] = (double)pd;
     wave_sync();
-    if (t + 1 < td) issue(t + 1);
+    if (t + 1 < t1) issue(t + 1);
     // L_pc: row a (lane a < m) by forward substitution against U_cc
     if (lane < m) {
-      double* r = rows + lane * 4 * cap + 4 * t;
+      double* r = f.rows + lane * 4 * cap + 4 * t;
       double l[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
         if (b < mc) {
           double x = r[b];
 #pragma unroll
-          for (int bq = 0; bq < b; ++bq) x -= l[bq] * ub[bq * 4 + b];
-          const double dd = ub[b * 4 + b];
+          for (int bq = 0; bq < b; ++bq) x -= l[bq] * f.ub[bq * 4 + b];
+          const double dd = f.ub[b * 4 + b];
           if (dd == 0.0) scal[S_MISC1] = 2.0;
           l[b] = x / (dd == 0.0 ? 1.0 : dd);
           r[b] = l[b];
         }
       }
 #pragma unroll
-      for (int b = 0; b < 4; ++b) lblk[lane * 4 + b] = l[b];
+      for (int b = 0; b < 4; ++b) f.lblk[lane * 4 + b] = l[b];
     }
     wave_sync();
     // R_pj -= L_pc U_cj over the upper blocks of c that this row holds
     for (int it = lane; it < 16 * nup; it += 64) {
       const int k = it >> 4, a = (it >> 2) & 3, bp = it & 3;
-      const int ps = pos[k];
+      const int ps = f.pos[k];
       if (ps < 0 || a >= m) continue;
-      double x = rows[a * 4 * cap + 4 * ps + bp];
-      for (int b = 0; b < mc; ++b) x -= lblk[a * 4 + b] * (double)urow[b * 4 * cap + 4 * k + bp];
-      rows[a * 4 * cap + 4 * ps + bp] = x;
+      double x = f.rows[a * 4 * cap + 4 * ps + bp];
+      for (int b = 0; b < mc; ++b) x -= f.lblk[a * 4 + b] * (double)f.urow[b * 4 * cap + 4 * k + bp];
+      f.rows[a * 4 * cap + 4 * ps + bp] = x;
     }
     wave_sync();
   }
-  // the row's own diagonal block
+}
+
+// The row's own diagonal block (point IKJ inside it), after all its lower blocks.
+template <typename LT>
+__device__ inline void fac_diag(const FacRow<LT>& f, const BDesc& d, double* __restrict__ scal, int cap, int lane) {
+  const int m = d.dm >> 16, td = d.dm & 0xffff, rs = 4 * d.nb;
   const int q0 = 4 * td;
   for (int a = 1; a < m; ++a) {
     for (int ap = 0; ap < a; ++ap) {
       if (lane == 0) {
-        const double dd = rows[ap * 4 * cap + q0 + ap];
+        const double dd = f.rows[ap * 4 * cap + q0 + ap];
         if (dd == 0.0) scal[S_MISC1] = 2.0;
-        const double l = rows[a * 4 * cap + q0 + ap] / (dd == 0.0 ? 1.0 : dd);
-        rows[a * 4 * cap + q0 + ap] = l;
-        lval[0] = l;
+        const double l = f.rows[a * 4 * cap + q0 + ap] / (dd == 0.0 ? 1.0 : dd);
+        f.rows[a * 4 * cap + q0 + ap] = l;
+        f.lval[0] = l;
       }
       wave_sync();
-      const double l = lval[0];
-      for (int q = q0 + ap + 1 + lane; q < rs; q += 64) rows[a * 4 * cap + q] -= l * rows[ap * 4 * cap + q];
+      const double l = f.lval[0];
+      for (int q = q0 + ap + 1 + lane; q < rs; q += 64) f.rows[a * 4 * cap + q] -= l * f.rows[ap * 4 * cap + q];
       wave_sync();
     }
   }
+}
+
+// Store the factored row in the factor's storage type.
+template <typename LT>
+__device__ inline void fac_store_row(const FacRow<LT>& f, const BDesc& d, LT* __restrict__ lu, int cap, int lane) {
+  const int m = d.dm >> 16, rs = 4 * d.nb;
   for (int a = 0; a < m; ++a)
-    for (int q = lane; q < rs; q += 64) lu[d.vbase + (int64_t)a * rs + q] = (LT)rows[a * 4 * cap + q];
+    for (int q = lane; q < rs; q += 64) lu[d.vbase + (int64_t)a * rs + q] = (LT)f.rows[a * 4 * cap + q];
+}
+
+// ILU(0) factorisation of the block rows of one level, one 64-lane workgroup (one
+// wavefront) per block row, IKJ order by lower neighbour blocks, a 4 x 4 block at a time
+// (fac_pivots_global), then the row's own diagonal block (fac_diag).  Point ILU(0) on a
+// pattern of full node blocks is exactly this: the subtractions run in the point
+// algorithm's pivot order, so the factor is the same to the last bit.  A row starts from
+// its values in a0 (the block tangent) and its factor is written to lu, so no separate
+// copy of the tangent seeds the factor.  Dynamic LDS: fac_row_bytes(cap) for rows of at
+// most cap <= kBsrMaxNb neighbour blocks.
+template <typename LT>
+__global__ __launch_bounds__(64) void k_bsr_factor(const int32_t* __restrict__ list, int first, int count,
+                                                   const BDesc* __restrict__ desc,
+                                                   const int32_t* __restrict__ bcol, LT* __restrict__ lu,
+                                                   const LT* __restrict__ a0, double* __restrict__ scal, int cap) {
+  extern __shared__ double smem[];
+  const int w = blockIdx.x;
+  if (w >= count) return;
+  const int lane = threadIdx.x;
+  const int32_t p = list ? list[first + w] : first + w;
+  const BDesc d = desc[p];
+  const FacRow<LT> f = fac_row_at<LT>(smem, cap);
+  fac_load_row<LT>(f, d, bcol, a0, cap, lane);
+  wave_sync();
+  fac_pivots_global<LT>(f, d, 0, d.dm & 0xffff, desc, bcol, lu, scal, cap, lane);
+  fac_diag<LT>(f, d, scal, cap, lane);
+  fac_store_row<LT>(f, d, lu, cap, lane);
 }
 
 // b[i] = scale * a[padmap[i]] into the padded block layout (0 where padmap < 0)
@@ -2024,10 +2083,10 @@ int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f, 
       level_range(P, false, lv, &list, &first, &cnt);
       if (!cnt) continue;
       if (fp32)
-        hipLaunchKernelGGL((k_bsr_factor<float>), dim3(cnt), dim3(64), bsr_factor_lds<float>(cap), st, list, first,
+        hipLaunchKernelGGL((k_bsr_factor<float>), dim3(cnt), dim3(64), fac_row_bytes<float>(cap), st, list, first,
                            cnt, P.d_bdesc, P.d_bcol, L->lu32, L->kp32, L->d_scal, cap);
       else
-        hipLaunchKernelGGL((k_bsr_factor<double>), dim3(cnt), dim3(64), bsr_factor_lds<double>(cap), st, list,
+        hipLaunchKernelGGL((k_bsr_factor<double>), dim3(cnt), dim3(64), fac_row_bytes<double>(cap), st, list,
                            first, cnt, P.d_bdesc, P.d_bcol, L->lu, L->kp, L->d_scal, cap);
     }
     if (store32) {
