@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <type_traits>
 #include <mutex>
@@ -562,6 +563,14 @@ __device__ inline V4<AT> ld4(const float* p) {
   const float4 f = *(const float4*)p;
   return {{(AT)f.x, (AT)f.y, (AT)f.z, (AT)f.w}};
 }
+__device__ inline void st4(double* p, double a, double b, double c, double d) {
+  reinterpret_cast<double2*>(p)[0] = make_double2(a, b);
+  reinterpret_cast<double2*>(p)[1] = make_double2(c, d);
+}
+__device__ inline void st4(float* p, float a, float b, float c, float d) {
+  *reinterpret_cast<float4*>(p) = make_float4(a, b, c, d);
+}
+
 template <typename AT>
 __device__ inline V4<AT> ld4(const bf16* p) {
   const uint2 u = *(const uint2*)p;
@@ -1210,9 +1219,25 @@ template <typename LT>
 __device__ inline void fac_load_row(const FacRow<LT>& f, const BDesc& d, const int32_t* __restrict__ bcol,
                                     const LT* __restrict__ a0, int cap, int lane) {
   const int m = d.dm >> 16, nb = d.nb, rs = 4 * nb;
-  for (int t = lane; t < nb; t += 64) f.cols[t] = bcol[d.bptr + t];
-  for (int a = 0; a < m; ++a)
-    for (int q = lane; q < rs; q += 64) f.rows[a * 4 * cap + q] = (double)a0[d.vbase + (int64_t)a * rs + q];
+  constexpr int KC = 4 * kBsrMaxNb / 64;
+  // every load issued before the first LDS store
+  const int32_t c = lane < nb ? bcol[d.bptr + lane] : 0;
+  double v[4][KC];
+#pragma unroll
+  for (int k = 0; k < KC; ++k)
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int q = lane + 64 * k;
+      v[a][k] = (a < m && q < rs) ? (double)a0[d.vbase + (int64_t)a * rs + q] : 0.0;
+    }
+  if (lane < nb) f.cols[lane] = c;
+#pragma unroll
+  for (int k = 0; k < KC; ++k)
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int q = lane + 64 * k;
+      if (a < m && q < rs) f.rows[a * 4 * cap + q] = v[a][k];
+    }
 }
 
 // IKJ steps of a row (one wavefront) for its lower blocks t in [t0, t1), whose rows are
@@ -1349,6 +1374,241 @@ __global__ __launch_bounds__(64) void k_bsr_factor(const int32_t* __restrict__ l
   fac_store_row<LT>(f, d, lu, cap, lane);
 }
 
+// Pivot-pair schedule of the factorisation (symbolic, once per plan).  For block row p
+// a pair is (lower block t of p, upper block q of pivot row c = bcol[t]) whose column is
+// also a block of row p, at position ps: the pairs are exactly the block updates
+// R_p,ps -= L_pt U_cq of the IKJ factorisation.  Record: t | q << 8 | ps << 16, in the
+// order (t, q) ascending.  One wavefront per row; WRITE = false counts the pairs of
+// each row into cnt, WRITE = true writes them from pofs[p] on.
+constexpr int kFacPairsWaves = 4;
+template <bool WRITE>
+__global__ __launch_bounds__(64 * kFacPairsWaves) void k_fac_pairs(int64_t nblk, const BDesc* __restrict__ desc,
+                                                                   const int32_t* __restrict__ bcol,
+                                                                   const int64_t* __restrict__ pofs,
+                                                                   int32_t* __restrict__ cnt,
+                                                                   uint32_t* __restrict__ prec) {
+  __shared__ int32_t s_cols[kFacPairsWaves][kBsrMaxNb];
+  __shared__ int32_t s_cofs[kFacPairsWaves][kBsrMaxNb + 1];
+  __shared__ int32_t s_cb[kFacPairsWaves][kBsrMaxNb];
+  __shared__ int32_t s_cq[kFacPairsWaves][kBsrMaxNb];
+  const int wv = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int64_t p = (int64_t)blockIdx.x * kFacPairsWaves + wv;
+  if (p >= nblk) return;
+  int32_t* cols = s_cols[wv];
+  int32_t* cofs = s_cofs[wv];
+  const BDesc d = desc[p];
+  const int nb = d.nb, td = d.dm & 0xffff;
+  int32_t c = -1;
+  if (lane < nb) {
+    c = bcol[d.bptr + lane];
+    cols[lane] = c;
+  }
+  int nup = 0;
+  if (lane < td) {
+    const BDesc dc = desc[c];
+    const int tdc = dc.dm & 0xffff;
+    nup = dc.nb - tdc - 1;
+    s_cb[wv][lane] = dc.bptr;
+    s_cq[wv][lane] = tdc + 1;
+  }
+  // inclusive scan of the upper-block counts of the pivots
+  int inc = nup;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int v = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += v;
+  }
+  if (lane == 0) cofs[0] = 0;
+  if (lane < td) cofs[lane + 1] = inc;
+  wave_sync();
+  const int ncand = td > 0 ? cofs[td] : 0;
+  int base = 0;
+  for (int i0 = 0; i0 < ncand; i0 += 64) {
+    const int i = i0 + lane;
+    int t = 0, q = 0, ps = -1;
+    if (i < ncand) {
+      int lo = 0, hi = td;
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (cofs[mid] <= i) lo = mid; else hi = mid;
+      }
+      t = lo;
+      q = s_cq[wv][t] + (i - cofs[t]);
+      ps = lds_find(cols, nb, bcol[s_cb[wv][t] + q]);
+    }
+    const uint64_t mask = __ballot(ps >= 0);
+    if (WRITE && ps >= 0) {
+      const int r = __popcll(mask & ((1ull << lane) - 1));
+      prec[pofs[p] + base + r] = (uint32_t)t | ((uint32_t)q << 8) | ((uint32_t)ps << 16);
+    }
+    base += __popcll(mask);
+  }
+  if (!WRITE && lane == 0) cnt[p] = base;
+}
+
+// v of lane (lane & ~3) | B, in every lane of each group of four (DPP quad permute)
+template <int B>
+__device__ inline double quad_bcast(double v) {
+  constexpr int ctrl = B | (B << 2) | (B << 4) | (B << 6);
+  const long long i = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)i, ctrl, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(i >> 32), ctrl, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+constexpr int kFacRowsPerWave = 4;  // block rows per wavefront of the scheduled factorisation
+
+// LDS of one row of the scheduled factorisation: the row in FP64 ([4][4 nbc]), per pivot
+// {first neighbour entry of the pivot row, its diagonal position | free rows << 16} and
+// {first, end} of its pairs, and the row's pair records.
+__host__ __device__ constexpr size_t fac_group_bytes(int nbc, int maxpairs) {
+  return ((size_t)nbc * (16 * sizeof(double) + 4 * sizeof(int32_t)) + (size_t)maxpairs * sizeof(uint32_t) + 15) /
+         16 * 16;
+}
+
+// ILU(0) factorisation of the block rows of one level from the pivot-pair schedule: 16
+// lanes per row, lane 4 a + b' holding entry (a, b') of the 4 x 4 block being worked on,
+// four rows per wavefront.  Pivot t: L_pt = R_pt U_cc^{-1} by substitution across the
+// four lanes of row a (each multiplier handed on by a DPP quad broadcast), then
+// R_p,ps -= L_pt U_cq for the pairs of t, U_cc and U_cq read from the block copy fu
+// (one load of column b' per lane); then the diagonal block by point IKJ, and the row
+// stored to lu and its diagonal and upper blocks to fu.  Every entry takes the
+// subtractions of the search-based kernel (k_bsr_factor) and the point IKJ algorithm in
+// the same order, so the factor is the same to the last bit.  Dynamic LDS:
+// kFacRowsPerWave * fac_group_bytes(nbc, maxpairs), nbc >= every row's neighbour count,
+// maxpairs >= every row's pair count.
+template <typename LT>
+__global__ __launch_bounds__(64) void k_bsr_factor_grp(const int32_t* __restrict__ list, int first, int count,
+                                                       const BDesc* __restrict__ desc,
+                                                       const int32_t* __restrict__ bcol,
+                                                       const int64_t* __restrict__ pofs,
+                                                       const uint32_t* __restrict__ prec, LT* __restrict__ lu,
+                                                       LT* __restrict__ fu, const LT* __restrict__ a0,
+                                                       double* __restrict__ scal, int nbc, int maxpairs) {
+  extern __shared__ double smem[];
+  const int lane = threadIdx.x, g = lane >> 4, j = lane & 15;
+  const int a = j >> 2, bp = j & 3;
+  const int w = blockIdx.x * kFacRowsPerWave + g;
+  if (w >= count) return;  // no lane exchanges data outside its own row
+  const int32_t p = list ? list[first + w] : first + w;
+  const BDesc d = desc[p];
+  const int m = d.dm >> 16, td = d.dm & 0xffff, nb = d.nb, rs = 4 * nb, RS = 4 * nbc;
+  double* rows = (double*)((char*)smem + g * fac_group_bytes(nbc, maxpairs));
+  int2* pv = (int2*)(rows + 4 * RS);
+  int2* pse = pv + nbc;
+  uint32_t* recs = (uint32_t*)(pse + nbc);
+  const int64_t r0 = pofs[p];
+  const int np = (int)(pofs[p + 1] - r0);
+  for (int i = j; i < np; i += 16) recs[i] = prec[r0 + i];
+  for (int t = j; t < td; t += 16) {
+    const BDesc dc = desc[bcol[d.bptr + t]];
+    pv[t] = make_int2(dc.bptr, dc.dm);
+    pse[t] = make_int2(0, 0);
+  }
+  for (int ar = 0; ar < m; ++ar)
+    for (int q0 = 0; q0 < rs; q0 += 128) {
+      double v[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int q = q0 + j + 16 * k;
+        v[k] = q < rs ? (double)a0[d.vbase + (int64_t)ar * rs + q] : 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int q = q0 + j + 16 * k;
+        if (q < rs) rows[ar * RS + q] = v[k];
+      }
+    }
+  wave_sync();
+  // pair range of each pivot (records sorted by pivot)
+  for (int i = j; i < np; i += 16) {
+    const int t = recs[i] & 0xff;
+    if (i == 0 || (int)(recs[i - 1] & 0xff) != t) pse[t].x = i;
+    if (i == np - 1 || (int)(recs[i + 1] & 0xff) != t) pse[t].y = i + 1;
+  }
+  wave_sync();
+  const bool act = a < m;
+  for (int t = 0; t < td; ++t) {
+    const int2 pvt = pv[t];
+    const int mc = pvt.y >> 16;
+    const int2 pr = pse[t];
+    const LT* ucc = fu + 16 * ((int64_t)pvt.x + (pvt.y & 0xffff));
+    // column bp of U_cc and its diagonal
+    const V4<LT> uc = ld4<LT>(ucc + 4 * bp);
+    double dg[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) dg[b] = (double)ucc[5 * b];
+    double x = rows[a * RS + 4 * t + bp];
+    double lv[4] = {0.0, 0.0, 0.0, 0.0};
+    double mine = 0.0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      if (b < mc) {
+        const double dd = dg[b];
+        if (dd == 0.0) scal[S_MISC1] = 2.0;
+        const double l = x / (dd == 0.0 ? 1.0 : dd);
+        if (bp == b) mine = l;
+        switch (b) {
+          case 0: lv[0] = quad_bcast<0>(l); break;
+          case 1: lv[1] = quad_bcast<1>(l); break;
+          case 2: lv[2] = quad_bcast<2>(l); break;
+          default: lv[3] = quad_bcast<3>(l);
+        }
+        if (bp > b) x -= lv[b] * (double)uc.v[b];
+      }
+    }
+    if (act && bp < mc) rows[a * RS + 4 * t + bp] = mine;
+    // R_p,ps -= L_pt U_cq, eight pairs at a time with their loads in flight together
+    for (int k0 = pr.x; k0 < pr.y; k0 += 8) {
+      V4<LT> u[8];
+      int pos[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        pos[k] = -1;
+        u[k] = V4<LT>{{(LT)0, (LT)0, (LT)0, (LT)0}};
+        if (k0 + k < pr.y) {
+          const uint32_t rc = recs[k0 + k];
+          pos[k] = (rc >> 16) & 0xff;
+          u[k] = ld4<LT>(fu + 16 * ((int64_t)pvt.x + ((rc >> 8) & 0xff)) + 4 * bp);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        if (act && pos[k] >= 0) {
+          double* e = rows + a * RS + 4 * pos[k] + bp;
+          double y = *e;
+#pragma unroll
+          for (int b = 0; b < 4; ++b)
+            if (b < mc) y -= lv[b] * (double)u[k].v[b];
+          *e = y;
+        }
+      }
+    }
+    wave_sync();
+  }
+  // the diagonal block, point IKJ
+  const int q0 = 4 * td;
+  for (int ar = 1; ar < m; ++ar) {
+    for (int ap = 0; ap < ar; ++ap) {
+      const int col = q0 + ap;
+      const double dd = rows[ap * RS + col];
+      const double num = rows[ar * RS + col];
+      if (dd == 0.0) scal[S_MISC1] = 2.0;
+      const double l = num / (dd == 0.0 ? 1.0 : dd);
+      wave_sync();
+      if (j == 0) rows[ar * RS + col] = l;
+      for (int q = col + 1 + j; q < rs; q += 16) rows[ar * RS + q] -= l * rows[ap * RS + q];
+      wave_sync();
+    }
+  }
+  for (int ar = 0; ar < m; ++ar)
+    for (int q = j; q < rs; q += 16) lu[d.vbase + (int64_t)ar * rs + q] = (LT)rows[ar * RS + q];
+  // diagonal and upper blocks into the block copy: lane j = 4 b' + b holds U[b][b']
+  const int cb = j >> 2, rb = j & 3;
+  for (int t = td; t < nb; ++t)
+    fu[16 * ((int64_t)d.bptr + t) + j] = rb < m ? (LT)rows[rb * RS + 4 * t + cb] : (LT)0;
+}
+
 // b[i] = scale * a[padmap[i]] into the padded block layout (0 where padmap < 0)
 template <typename TA, typename TB>
 __global__ __launch_bounds__(kTPB) void k_pad_in(int64_t n, const int32_t* __restrict__ padmap,
@@ -1397,7 +1657,7 @@ LinAlg::~LinAlg() {
   void* ptrs[] = {d_blkStart, d_diagpos, d_part, d_scal, r, rh, p, v, s, t, ph, sh, x, w, z, pr, pz,
                   r32, rh32, p32, v32, s32, t32, ph32, sh32, x32, du32, w32, z32, pr32, pz32,
                   dinv, dinv32, binv, binv32, lu, lu32, gm_V, gm_V32, d_desc, lu16, kp, kp32, d_h, d_mpart,
-                  tinv, tinv32, tinv16, sw, sw32, sw16};
+                  tinv, tinv32, tinv16, sw, sw32, sw16, fu, fu32};
   delete ilu;
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
@@ -1642,7 +1902,7 @@ static std::vector<BlkDesc> make_desc(const std::vector<int32_t>& bs, const std:
 
 IluPlan::~IluPlan() {
   void* ptrs[] = {d_levL, d_levU, d_bdesc, d_bcol, d_bcolofs, d_r0orig, d_padmap, d_natrows, d_tileStart,
-                  d_swoff, d_trowL, d_trowU};
+                  d_swoff, d_trowL, d_trowU, d_fpofs, d_fprec};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
 }
@@ -1737,6 +1997,49 @@ static std::vector<int32_t> tiled_order(sfem_ctx* ctx, const std::vector<int32_t
 // its neighbour blocks sorted by new position, the offsets of those neighbours' first
 // columns in the DOF-ordered CSR row (for the value gather), block levels (one per
 // colour with the multicolour ordering), and the padded-slot -> DOF map of vectors.
+// Blocks of level lv of the forward (upper=false) or backward (upper=true) sweep: a list
+// slice, or with a contiguous plan the block range of colour lv (reversed for backward).
+static void level_range(const IluPlan& P, bool upper, int lv, const int32_t** list, int* first, int* count);
+
+
+// The factorisation's pivot-pair schedule (k_fac_pairs) for the plan's block rows.
+static int build_fac_pairs(sfem_ctx* ctx, IluPlan& P) {
+  hipStream_t st = ctx->stream;
+  const int64_t nb = P.nblk;
+  if (nb == 0) return SFEM_OK;
+  int32_t* d_cnt = nullptr;
+  hipError_t e = alloc(&d_cnt, (size_t)nb);
+  if (e != hipSuccess) return set_hip_error(e, "factorisation schedule");
+  const unsigned g = (unsigned)((nb + kFacPairsWaves - 1) / kFacPairsWaves);
+  hipLaunchKernelGGL((k_fac_pairs<false>), dim3(g), dim3(64 * kFacPairsWaves), 0, st, nb, P.d_bdesc, P.d_bcol,
+                     nullptr, d_cnt, nullptr);
+  std::vector<int32_t> cnt(nb);
+  e = hipMemcpyAsync(cnt.data(), d_cnt, (size_t)nb * sizeof(int32_t), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipFree(d_cnt);
+  if (e != hipSuccess) return set_hip_error(e, "factorisation schedule");
+  std::vector<int64_t> ofs(nb + 1, 0);
+  for (int64_t p = 0; p < nb; ++p) ofs[p + 1] = ofs[p] + cnt[p];
+  P.fac_pairs = ofs[nb];
+  int rc = upload(&P.d_fpofs, ofs);
+  if (rc) return rc;
+  e = alloc(&P.d_fprec, (size_t)std::max<int64_t>(P.fac_pairs, 1));
+  if (e != hipSuccess) return set_hip_error(e, "factorisation schedule");
+  hipLaunchKernelGGL((k_fac_pairs<true>), dim3(g), dim3(64 * kFacPairsWaves), 0, st, nb, P.d_bdesc, P.d_bcol,
+                     P.d_fpofs, nullptr, P.d_fprec);
+  e = hipGetLastError();
+  if (e != hipSuccess) return set_hip_error(e, "factorisation schedule");
+  int mx = 0;
+  for (int64_t p = 0; p < nb; ++p) mx = std::max(mx, cnt[p]);
+  P.fac_maxpairs = std::max(mx, 1);
+  // SFEM_FACTOR_KERNEL=search: every level on the search-based kernel (a diagnostic
+  // switch; both kernels give the same factor)
+  const char* fk = getenv("SFEM_FACTOR_KERNEL");
+  P.fac_sched = !(fk && strcmp(fk, "search") == 0) &&
+                kFacRowsPerWave * fac_group_bytes(P.nbmax, P.fac_maxpairs) <= 65536;
+  return SFEM_OK;
+}
+
 static int build_ilu_plan(sfem_ctx* ctx, LinAlg* L, int reorder) {
   if (!L->full_blocks)
     return set_error(SFEM_ERR_UNSUPPORTED, "ILU(0): the pattern is not a union of full node blocks");
@@ -1745,7 +2048,7 @@ static int build_ilu_plan(sfem_ctx* ctx, LinAlg* L, int reorder) {
   // factor-sized buffers belong to the previous plan
   void** vals[] = {(void**)&L->kp, (void**)&L->kp32, (void**)&L->lu, (void**)&L->lu32, (void**)&L->lu16,
                    (void**)&L->tinv, (void**)&L->tinv32, (void**)&L->tinv16, (void**)&L->sw, (void**)&L->sw32,
-                   (void**)&L->sw16};
+                   (void**)&L->sw16, (void**)&L->fu, (void**)&L->fu32};
   for (void** q : vals)
     if (*q) { (void)hipFree(*q); *q = nullptr; }
   IluPlan* Pp = new IluPlan;
@@ -1894,6 +2197,7 @@ static int build_ilu_plan(sfem_ctx* ctx, LinAlg* L, int reorder) {
       (rc = upload(&P.d_padmap, padmap)) || (rc = upload(&P.d_levL, P.h_levL)) || (rc = upload(&P.d_levU, P.h_levU)))
     return rc;
   if (reorder != 0 && (rc = upload(&P.d_natrows, iord))) return rc;  // block rows in original order
+  if ((rc = build_fac_pairs(ctx, P))) return rc;
   if (P.tiled) {
     // what the tile sweeps assume: at most kTileMax rows per tile, every row coupled to
     // every other row of its tile, so that its intra-tile blocks are the ones next to its
@@ -1975,6 +2279,19 @@ static int ensure_vec(T** p, int64_t n) {
   hipError_t e = alloc(p, (size_t)n);
   if (e != hipSuccess) return set_hip_error(e, "hipMalloc krylov vector");
   return SFEM_OK;
+}
+
+// One factorisation level: the scheduled kernel, or the search-based one.
+template <typename LT>
+static void fac_launch(hipStream_t st, const int32_t* list, int first, int cnt, const IluPlan& P, LT* lu, LT* fu,
+                       const LT* a0, double* scal, int cap) {
+  if (P.fac_sched)
+    hipLaunchKernelGGL((k_bsr_factor_grp<LT>), dim3((cnt + kFacRowsPerWave - 1) / kFacRowsPerWave), dim3(64),
+                       kFacRowsPerWave * fac_group_bytes(P.nbmax, P.fac_maxpairs), st, list, first, cnt, P.d_bdesc,
+                       P.d_bcol, P.d_fpofs, P.d_fprec, lu, fu, a0, scal, P.nbmax, P.fac_maxpairs);
+  else
+    hipLaunchKernelGGL((k_bsr_factor<LT>), dim3(cnt), dim3(64), fac_row_bytes<LT>(cap), st, list, first, cnt,
+                       P.d_bdesc, P.d_bcol, lu, a0, scal, cap);
 }
 
 // Blocks of level lv of the forward (upper=false) or backward (upper=true) sweep: a list
@@ -2078,16 +2395,16 @@ int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f, 
     }
     const int cap = (P.nbmax + 15) / 16 * 16;
     const int nl = (int)P.levLPtr.size() - 1;
+    if (P.fac_sched) {
+      int rc = fp32 ? ensure_vec(&L->fu32, P.npairs * 16) : ensure_vec(&L->fu, P.npairs * 16);
+      if (rc) return rc;
+    }
     for (int lv = 0; lv < nl; ++lv) {
       const int32_t* list; int first, cnt;
       level_range(P, false, lv, &list, &first, &cnt);
       if (!cnt) continue;
-      if (fp32)
-        hipLaunchKernelGGL((k_bsr_factor<float>), dim3(cnt), dim3(64), fac_row_bytes<float>(cap), st, list, first,
-                           cnt, P.d_bdesc, P.d_bcol, L->lu32, L->kp32, L->d_scal, cap);
-      else
-        hipLaunchKernelGGL((k_bsr_factor<double>), dim3(cnt), dim3(64), fac_row_bytes<double>(cap), st, list,
-                           first, cnt, P.d_bdesc, P.d_bcol, L->lu, L->kp, L->d_scal, cap);
+      if (fp32) fac_launch<float>(st, list, first, cnt, P, L->lu32, L->fu32, L->kp32, L->d_scal, cap);
+      else fac_launch<double>(st, list, first, cnt, P, L->lu, L->fu, L->kp, L->d_scal, cap);
     }
     if (store32) {
       int rc = to32(L->lu, &L->lu32);

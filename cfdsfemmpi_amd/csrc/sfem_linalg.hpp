@@ -57,6 +57,13 @@ struct IluPlan {
   // first block in the copy}, forward (lower) and backward (upper)
   int4* d_trowL = nullptr;
   int4* d_trowU = nullptr;
+  // the factorisation's pivot-pair schedule: per block row the updates R_p,ps -= L_pt U_cq
+  // (t | q << 8 | ps << 16, from d_fpofs[p]); fac_sched false runs the search-based kernel
+  int64_t* d_fpofs = nullptr;
+  uint32_t* d_fprec = nullptr;
+  int64_t fac_pairs = 0;
+  int fac_maxpairs = 1;
+  bool fac_sched = false;
   ~IluPlan();
 };
 
@@ -124,6 +131,10 @@ struct LinAlg {
   double* tinv = nullptr;
   float* tinv32 = nullptr;
   bf16* tinv16 = nullptr;
+  // the factorisation's copy of every row's diagonal and upper blocks, 16 values per
+  // block (transposed: column b' then row b), at the block's neighbour entry (bcol index)
+  double* fu = nullptr;
+  float* fu32 = nullptr;
   double* sw = nullptr;    // tile sweeps' off-tile factor blocks (d_swoff), FP64 / FP32 / BF16
   float* sw32 = nullptr;
   bf16* sw16 = nullptr;

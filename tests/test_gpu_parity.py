@@ -455,6 +455,38 @@ def test_ilu_factor_vs_oracle(reorder, prec):
     ctx.close()
 
 
+@pytest.mark.parametrize("mesh_kind,prec,reorder", [("hex", "fp64", 0), ("hex", "fp64", 3), ("hex", "mixed", 3),
+                                                   ("hex", "fp64", 1), ("tet", "fp64", 0), ("quad", "fp64", 2),
+                                                   ("tria", "mixed", 0)])
+def test_factor_schedule_matches_search_kernel(mesh_kind, prec, reorder, monkeypatch):
+    """The factorisation from the pivot-pair schedule against the search-based kernel
+    (SFEM_FACTOR_KERNEL=search): the same subtractions in the same order, so the two
+    factors are equal bit for bit."""
+    if mesh_kind == "hex":
+        mesh = meshgen.ldc3d_hex(7, graded=True)
+    elif mesh_kind == "quad":
+        mesh = meshgen.ldc2d_quad(9)
+    elif mesh_kind == "tria":
+        mesh = meshgen.ldc2d_tria(9, jitter=0.2, seed=4)
+    else:
+        mesh = meshgen.ldc3d_tet(4)
+    out = []
+    for kernel in ("schedule", "search"):
+        if kernel == "search":
+            monkeypatch.setenv("SFEM_FACTOR_KERNEL", "search")
+        ctx = make_ctx(mesh, tis=2, dt=0.1, elemData=np.array([1.0, 1e-2, 0, 0, 0, 0, 0, 0, 1, 1, 0]))
+        if prec == "mixed":
+            ctx.set_precision(sfem.PREC_MIXED)
+        ctx.time_update(1.0)
+        ctx.assemble(0)
+        ctx.solve(sfem.solver_opts("bcgs", "ilu", 1e-6, 5000, 30, reorder))
+        out.append(ctx.get_ilu())
+        ctx.close()
+    print(f"[case] factor schedule {mesh_kind} {prec} reorder={reorder} nnz={len(out[0][3])}", flush=True)
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("mesh_kind,prec,bf16", [("hex", "fp64", 0), ("hex", "mixed", 0), ("hex", "mixed", 1),
                                                 ("hex", "store32", 0),
                                                 ("quad", "fp64", 0), ("tria", "fp64", 0), ("tet", "fp64", 0)])
