@@ -601,8 +601,9 @@ def main():
     ap.add_argument("--inner-rtol-max", type=float, default=0.5, help="mixed path: cap of the inexact-Newton rtol")
     ap.add_argument("--tuning", choices=["timed", "fixed"], default="timed",
                     help="kernel shapes: timed autotuner (default) or fixed default shapes (reproducible FP32)")
-    ap.add_argument("--pc-lag-mixed", type=int, default=1,
-                    help="mixed path: rebuild the ILU(0) factor on every n-th new tangent only")
+    ap.add_argument("--pc-lag", type=int, default=4,
+                    help="both paths: rebuild the ILU(0) factor on every n-th new tangent only (1: every tangent, "
+                         "as the PETSc KSP of the reference does)")
     ap.add_argument("--ksp", choices=["bcgs", "gmres"], default="bcgs", help="Krylov method of both paths")
     ap.add_argument("--restart", type=int, default=30, help="GMRES restart length")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -637,12 +638,12 @@ def main():
     tune_mode = sfem.TUNE_FIXED if a.tuning == "fixed" else sfem.TUNE_TIMED
     r64 = rmx = None
     if a.only != "mixed":
-        r64 = run_path(mesh, ctl, sfem.PREC_FP64, o64, a.newton_tol, a.steps, a.warmup, "fp64", pred,
+        r64 = run_path(mesh, ctl, sfem.PREC_FP64, o64, a.newton_tol, a.steps, a.warmup, "fp64", pred, a.pc_lag,
                        tuning=tune_mode)
         log(f"fp64: {r64['steps_per_s']:.3f} steps/s")
     if a.only != "fp64":
         rmx = run_path(mesh, ctl, sfem.PREC_MIXED, omx, a.newton_tol_mixed, a.steps, a.warmup, "mixed", pred,
-                       a.pc_lag_mixed, (a.inner_theta, a.inner_rtol_max), tune_mode)
+                       a.pc_lag, (a.inner_theta, a.inner_rtol_max), tune_mode)
         log(f"mixed: {rmx['steps_per_s']:.3f} steps/s")
     main_res = rmx or r64
     err_u = err_p = None
@@ -726,7 +727,7 @@ def main():
                    "solver": ("BiCGStab" if a.ksp == "bcgs" else f"GMRES({a.restart})") + " + ILU(0)", "ilu_ordering": ["natural", "rcm", "multicolour", "tiled multicolour"][a.reorder],
                    "newton_tol": a.newton_tol, "newton_tol_mixed": a.newton_tol_mixed,
                    "inner_rtol_mixed": a.inner_rtol, "ilu_factors_mixed": "bf16" if a.bf16_factors else "fp32",
-                   "newton_initial_iterate": a.predictor, "pc_lag_mixed": a.pc_lag_mixed,
+                   "newton_initial_iterate": a.predictor, "pc_lag": a.pc_lag,
                    "inner_theta_mixed": a.inner_theta, "inner_rtol_max_mixed": a.inner_rtol_max,
                    "kernel_shapes": a.tuning, "parallelism": "single GPU"},
         "fp64_steps_per_s": r64["steps_per_s"] if r64 else None,
