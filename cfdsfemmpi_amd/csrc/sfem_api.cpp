@@ -115,6 +115,9 @@ static AsmView asm_view(sfem_ctx* c) {
   v.sconn = c->d_sconn;
   v.vals = c->d_vals;
   v.vals32 = c->d_vals32;
+  v.blkofs = c->d_blkofs;
+  v.kb = c->d_kb;
+  v.kb32 = c->d_kb32;
   v.rhs = c->d_rhs;
   v.probeK = c->d_probeK;
   v.probeF = c->d_probeF;
@@ -223,6 +226,7 @@ void sfem_destroy(sfem_ctx* c) {
   dfree(c->d_nodeInfo); dfree(c->d_colofs); dfree(c->d_sconn); dfree(c->d_elemByColor); dfree(c->d_rowptr32);
   dfree(c->d_colidx); dfree(c->d_assy); dfree(c->d_dbc_ind); dfree(c->d_dbc_val); dfree(c->d_dbc_mult);
   dfree(c->d_vals); dfree(c->d_vals32); dfree(c->d_rhs); dfree(c->d_du);
+  dfree(c->d_blkofs); dfree(c->d_natofs); dfree(c->d_blkPtr); dfree(c->d_kb); dfree(c->d_kb32);
   dfree(c->d_soln); dfree(c->d_prev); dfree(c->d_prev2); dfree(c->d_prev3); dfree(c->d_prev4);
   dfree(c->d_dot); dfree(c->d_dotPrev); dfree(c->d_cur); dfree(c->d_dotCur); dfree(c->d_applied);
   dfree(c->d_probeK); dfree(c->d_probeF); dfree(c->d_cand);
@@ -395,6 +399,42 @@ int sfem_build_pattern(sfem_ctx* c, int64_t* nfree_out, int64_t* nnz_out) {
         }
       }
   });
+  // node-pair blocks: node block i (nodes with free dofs, node order) owns the pairs
+  // blkPtr[i] .. blkPtr[i + 1], one per neighbour node in increasing node order
+  std::vector<int64_t> nodePair(nNode + 1, 0);
+  for (int64_t i = 0; i < nNode; ++i) nodePair[i + 1] = nodePair[i] + (cnt[i] ? (int64_t)nbr[i].size() : 0);
+  c->nbpairs = nodePair[nNode];
+  if (c->nbpairs >= ((int64_t)1 << 30) / 16)
+    return set_error(SFEM_ERR_UNSUPPORTED, "node-pair blocks exceed the element map's range");
+  c->blkPtr.clear();
+  for (int64_t i = 0; i < nNode; ++i)
+    if (cnt[i]) c->blkPtr.push_back(nodePair[i]);
+  c->blkPtr.push_back(c->nbpairs);
+  std::vector<int32_t> natofs(c->nbpairs);
+  for (int64_t i = 0; i < nNode; ++i) {
+    if (!cnt[i]) continue;
+    int32_t off = 0;
+    for (size_t t = 0; t < nbr[i].size(); ++t) {
+      const int32_t j = nbr[i][t];
+      natofs[nodePair[i] + t] = off | (cnt[j] << 24);
+      off += cnt[j];
+    }
+  }
+  std::vector<int32_t> blkofs(nElem * npe * npe, -1);
+  parallel_for(nElem, [&](int64_t lo, int64_t hi) {
+    for (int64_t e = lo; e < hi; ++e)
+      for (int a = 0; a < npe; ++a) {
+        const int32_t i = c->conn[e * npe + a];
+        if (!cnt[i]) continue;
+        const auto& nb = nbr[i];
+        for (int b = 0; b < npe; ++b) {
+          const int32_t j = c->conn[e * npe + b];
+          if (!cnt[j]) continue;
+          const int64_t t = std::lower_bound(nb.begin(), nb.end(), j) - nb.begin();
+          blkofs[(e * npe + a) * npe + b] = (int32_t)(nodePair[i] + t);
+        }
+      }
+  });
   // greedy element colouring: no two elements of a colour share a node
   std::vector<uint64_t> used(nNode, 0);
   std::vector<int32_t> color(nElem);
@@ -421,15 +461,34 @@ int sfem_build_pattern(sfem_ctx* c, int64_t* nfree_out, int64_t* nnz_out) {
   // the element loop walks colour classes slot by slot: the element -> nonzero map and
   // the connectivity are stored in that order, so each slot's entries are contiguous
   std::vector<int32_t> scolofs((size_t)nElem * npe * npe), sconn((size_t)nElem * npe);
+  std::vector<int32_t> sblkofs((size_t)nElem * npe * npe);
   parallel_for(nElem, [&](int64_t lo, int64_t hi) {
     for (int64_t sl = lo; sl < hi; ++sl) {
       const int64_t e = byColor[sl];
       std::copy(colofs.begin() + e * npe * npe, colofs.begin() + (e + 1) * npe * npe,
                 scolofs.begin() + sl * npe * npe);
+      std::copy(blkofs.begin() + e * npe * npe, blkofs.begin() + (e + 1) * npe * npe,
+                sblkofs.begin() + sl * npe * npe);
       std::copy(c->conn.begin() + e * npe, c->conn.begin() + (e + 1) * npe, sconn.begin() + sl * npe);
     }
   });
   std::vector<int32_t>().swap(colofs);
+  std::vector<int32_t>().swap(blkofs);
+  // the first element (in colour-slot order, the order the colour launches run in) to
+  // reach a block stores it (bit 30) instead of adding to it, so the block form needs no
+  // clearing; every block is some element's node pair
+  {
+    std::vector<uint8_t> seen(c->nbpairs, 0);
+    for (size_t k = 0; k < sblkofs.size(); ++k) {
+      const int32_t q = sblkofs[k];
+      if (q >= 0 && !seen[q]) {
+        seen[q] = 1;
+        sblkofs[k] = q | (1 << 30);
+      }
+    }
+    for (int64_t q = 0; q < c->nbpairs; ++q)
+      if (!seen[q]) return set_error(SFEM_ERR_STATE, "a node-pair block no element reaches");
+  }
   // node info {rowStart, rowlen, mask, firstRow}
   std::vector<int32_t> info(nNode * 4);
   for (int64_t i = 0; i < nNode; ++i) {
@@ -443,7 +502,11 @@ int sfem_build_pattern(sfem_ctx* c, int64_t* nfree_out, int64_t* nnz_out) {
   // upload
   dfree(c->d_nodeInfo); dfree(c->d_colofs); dfree(c->d_sconn); dfree(c->d_elemByColor); dfree(c->d_rowptr32);
   dfree(c->d_colidx); dfree(c->d_assy); dfree(c->d_vals); dfree(c->d_vals32); dfree(c->d_rhs); dfree(c->d_du);
+  dfree(c->d_blkofs); dfree(c->d_natofs); dfree(c->d_blkPtr); dfree(c->d_kb); dfree(c->d_kb32);
   if (c->la) la_destroy(c);
+  // the Hexa8 loop assembles the node-pair block form
+  c->kb_on = c->et == HEXA8;
+  c->csr_version = -1;
   int rc;
   if ((rc = dalloc(&c->d_nodeInfo, info.size())) || (rc = dalloc(&c->d_colofs, scolofs.size())) ||
       (rc = dalloc(&c->d_sconn, sconn.size())) ||
@@ -453,6 +516,15 @@ int sfem_build_pattern(sfem_ctx* c, int64_t* nfree_out, int64_t* nnz_out) {
     return rc;
   HIP_OK(hipMemcpy(c->d_nodeInfo, info.data(), info.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(c->d_colofs, scolofs.data(), scolofs.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  if (c->kb_on) {
+    if ((rc = dalloc(&c->d_blkofs, sblkofs.size())) || (rc = dalloc(&c->d_natofs, natofs.size())) ||
+        (rc = dalloc(&c->d_blkPtr, c->blkPtr.size())) || (rc = dalloc(&c->d_kb, c->nbpairs * 16)))
+      return rc;
+    HIP_OK(hipMemcpy(c->d_blkofs, sblkofs.data(), sblkofs.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(c->d_natofs, natofs.data(), natofs.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(c->d_blkPtr, c->blkPtr.data(), c->blkPtr.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    HIP_OK(hipMemset(c->d_kb, 0, c->nbpairs * 16 * sizeof(double)));
+  }
   HIP_OK(hipMemcpy(c->d_sconn, sconn.data(), sconn.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(c->d_elemByColor, byColor.data(), byColor.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(c->d_rowptr32, rp32.data(), rp32.size() * sizeof(int32_t), hipMemcpyHostToDevice));
@@ -613,6 +685,7 @@ static int run_element_loop(sfem_ctx* c, int mode, int lift) {
     const int64_t n = c->colorPtr[k + 1] - c->colorPtr[k];
     AsmView v = v0;  // slot-ordered arrays start at the colour's first slot
     v.colofs = v0.colofs + (int64_t)c->colorPtr[k] * npe * npe;
+    if (v0.blkofs) v.blkofs = v0.blkofs + (int64_t)c->colorPtr[k] * npe * npe;
     v.sconn = v0.sconn + (int64_t)c->colorPtr[k] * npe;
     c->kt.begin(SFEM_KT_ASSEMBLY, c->stream);
     HIP_OK(launch_assemble(c->et, c->stream, v, p, c->d_elemByColor + c->colorPtr[k], n, mode, lift, -1));
@@ -633,6 +706,10 @@ int sfem_assemble(sfem_ctx* c, int iter, int want_K, double* rhs_norm) {
     int rc = dalloc(&c->d_vals32, c->nnz);
     if (rc) return rc;
   }
+  if (mixed && c->kb_on && !c->d_kb32) {
+    int rc = dalloc(&c->d_kb32, c->nbpairs * 16);
+    if (rc) return rc;
+  }
   const int64_t nd = (int64_t)c->dbc_ind.size();
   if (mixed && iter == 0)  // prescribed values first; the FP64 residual then needs no lift
     HIP_OK(launch_add_bc(c->stream, nd, c->d_dbc_ind, c->d_dbc_mult, c->d_applied, c->d_soln));
@@ -642,8 +719,13 @@ int sfem_assemble(sfem_ctx* c, int iter, int want_K, double* rhs_norm) {
   HIP_OK(launch_zero_f64(c->stream, c->nfree, c->d_rhs));
   int mode;
   if (want_K) {
-    if (mixed) HIP_OK(launch_zero_f32(c->stream, c->nnz, c->d_vals32));
-    else HIP_OK(launch_zero_f64(c->stream, c->nnz, c->d_vals));
+    if (c->kb_on) {
+      // the block form is written whole by its first elements (no clearing)
+    } else if (mixed) {
+      HIP_OK(launch_zero_f32(c->stream, c->nnz, c->d_vals32));
+    } else {
+      HIP_OK(launch_zero_f64(c->stream, c->nnz, c->d_vals));
+    }
     mode = mixed ? 2 : 0;
   } else {
     mode = 1;
@@ -659,7 +741,10 @@ int sfem_assemble(sfem_ctx* c, int iter, int want_K, double* rhs_norm) {
   HIP_OK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_asm_ms = ms;
   c->kt.resolve();  // la_norm2 drained the stream
-  if (want_K) c->la_K_version++;
+  if (want_K) {
+    c->la_K_version++;
+    c->kb_fp32 = mixed;
+  }
   if (rhs_norm) *rhs_norm = nrm;
   c->status |= ST_ASSEMBLED;
   return SFEM_OK;
@@ -684,6 +769,10 @@ int sfem_get_system(sfem_ctx* c, double* vals, double* rhs) {
   if (!c) return set_error(SFEM_ERR_ARG, "null context");
   if (!(c->status & ST_ASSEMBLED)) return set_error(SFEM_ERR_STATE, "nothing assembled");
   HIP_OK(hipSetDevice(c->device));
+  if (vals) {
+    int rc = la_ensure_csr(c);
+    if (rc) return rc;
+  }
   HIP_OK(hipStreamSynchronize(c->stream));
   if (vals) {
     if (c->precision == SFEM_PREC_MIXED) {

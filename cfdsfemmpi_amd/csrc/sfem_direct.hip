@@ -348,6 +348,8 @@ int la_direct_info(sfem_ctx* ctx, int* kl, int* ku) {
 static int direct_factor(sfem_ctx* ctx) {
   DirectLU* D = ctx->la->direct;
   hipStream_t st = ctx->stream;
+  int rc0 = la_ensure_csr(ctx);  // the band is filled from the CSR values
+  if (rc0) return rc0;
   const int64_t n = D->n;
   const int64_t nab = n * D->ldab;
   hipLaunchKernelGGL(k_lu_fill0, dim3((unsigned)std::min<int64_t>((nab + 255) / 256, 1 << 20)), dim3(256), 0, st, nab,

@@ -625,7 +625,7 @@ __global__ __launch_bounds__(64, kHexOcc) void k_assemble_hex8(AsmView v, ElemPa
   if (s >= n) return;
 
   // staging registers of the element to come
-  int32_t co = v.colofs[s * 64 + l0];
+  int32_t co = MODE != 1 ? v.blkofs[s * 64 + l0] : -1;
   int32_t eid = elems[s];
   double h = v.charlen[eid];
   double mu = v.elem_mu ? v.elem_mu[eid] : p.mu;
@@ -674,7 +674,7 @@ __global__ __launch_bounds__(64, kHexOcc) void k_assemble_hex8(AsmView v, ElemPa
     int32_t nid_n = 0;
     if (more) {
       nid_n = v.sconn[sn * NPE + nn];
-      co = v.colofs[sn * 64 + l0];
+      if (MODE != 1) co = v.blkofs[sn * 64 + l0];
       eid = elems[sn];
     }
     phase_geometry<HEXA8>(L, l);
@@ -704,38 +704,17 @@ __global__ __launch_bounds__(64, kHexOcc) void k_assemble_hex8(AsmView v, ElemPa
     }
     double lsum[4] = {0.0, 0.0, 0.0, 0.0};
     if constexpr (MODE != 1) {
-      T* vals = MODE == 2 ? reinterpret_cast<T*>(v.vals32) : reinterpret_cast<T*>(v.vals);
+      // block (i, j) of the node-pair block form: 16 values, one aligned run, loaded
+      // before the tangent is accumulated (no other element of the class adds to it)
+      typedef T blk_t __attribute__((ext_vector_type(4)));
+      T* kbv = MODE == 2 ? reinterpret_cast<T*>(v.kb32) : reinterpret_cast<T*>(v.kb);
+      // (bit 30 of the map: this element is the block's first, and stores it whole)
       const bool blk = mi != 0 && mj != 0;
-      const int32_t rs = L.info[i][0], rl = L.info[i][1];
-      // CSR offsets of the free rows of node i in block (i, j), and the values there,
-      // loaded before the tangent is accumulated
-      int64_t roff[NDOF];
-      T old[NDOF][NDOF];
+      const bool first = (co_cur >> 30) & 1;
+      blk_t* bq = reinterpret_cast<blk_t*>(kbv + (int64_t)(blk ? (co_cur & ((1 << 30) - 1)) : 0) * 16);
+      blk_t old[NDOF];
 #pragma unroll
-      for (int a = 0; a < NDOF; ++a) {
-        roff[a] = -1;
-#pragma unroll
-        for (int b = 0; b < NDOF; ++b) old[a][b] = (T)0;
-      }
-      if (blk) {
-        int ra = 0;
-#pragma unroll
-        for (int a = 0; a < NDOF; ++a) {
-          if (!((mi >> a) & 1)) continue;
-          roff[a] = (int64_t)rs + (int64_t)ra * rl + co_cur;
-          ++ra;
-          const T* row = vals + roff[a];
-          if (mj == 15) {
-            const vec_t r = *reinterpret_cast<const vec_t*>(row);
-            old[a][0] = r.x; old[a][1] = r.y; old[a][2] = r.z; old[a][3] = r.w;
-          } else {
-            // free column b of node j sits at the number of free columns before it
-#pragma unroll
-            for (int b = 0; b < NDOF; ++b)
-              if ((mj >> b) & 1) old[a][b] = row[__popc(mj & ((1 << b) - 1))];
-          }
-        }
-      }
+      for (int a = 0; a < NDOF; ++a) old[a] = (blk && !first) ? bq[a] : blk_t{(T)0, (T)0, (T)0, (T)0};
       T K[NDOF][NDOF];
       accumulate_block<HEXA8, T, kHexGpUnroll>(L, i, j, pe, K);
       if (doLift) {
@@ -751,20 +730,40 @@ __global__ __launch_bounds__(64, kHexOcc) void k_assemble_hex8(AsmView v, ElemPa
         }
       }
       if (blk) {
+        if (mi == 15 && mj == 15) {
 #pragma unroll
-        for (int a = 0; a < NDOF; ++a) {
-          if (roff[a] < 0) continue;
-          T* row = vals + roff[a];
-          if (mj == 15) {
-            vec_t r;
-            r.x = old[a][0] + K[a][0]; r.y = old[a][1] + K[a][1];
-            r.z = old[a][2] + K[a][2]; r.w = old[a][3] + K[a][3];
-            *reinterpret_cast<vec_t*>(row) = r;
-          } else {
-#pragma unroll
-            for (int b = 0; b < NDOF; ++b)
-              if ((mj >> b) & 1) row[__popc(mj & ((1 << b) - 1))] = old[a][b] + K[a][b];
+          for (int a = 0; a < NDOF; ++a) {
+            blk_t r = old[a];
+            r.x += K[a][0]; r.y += K[a][1]; r.z += K[a][2]; r.w += K[a][3];
+            bq[a] = r;
           }
+        } else {
+          // free rows and columns packed to the front of the block
+          T nv[NDOF][NDOF];
+#pragma unroll
+          for (int a = 0; a < NDOF; ++a) {
+            nv[a][0] = old[a].x; nv[a][1] = old[a].y; nv[a][2] = old[a].z; nv[a][3] = old[a].w;
+          }
+          int ra = 0;
+#pragma unroll
+          for (int a = 0; a < NDOF; ++a) {
+            if (!((mi >> a) & 1)) continue;
+            int rb = 0;
+#pragma unroll
+            for (int b = 0; b < NDOF; ++b) {
+              if (!((mj >> b) & 1)) continue;
+              // rows and columns are only ever packed towards the front: ra <= a, rb <= b
+#pragma unroll
+              for (int x = 0; x < NDOF; ++x)
+#pragma unroll
+                for (int y = 0; y < NDOF; ++y)
+                  if (x == ra && y == rb) nv[x][y] += K[a][b];
+              ++rb;
+            }
+            ++ra;
+          }
+#pragma unroll
+          for (int a = 0; a < NDOF; ++a) bq[a] = blk_t{nv[a][0], nv[a][1], nv[a][2], nv[a][3]};
         }
       }
     }

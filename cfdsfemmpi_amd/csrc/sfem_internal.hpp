@@ -68,6 +68,14 @@ struct AsmView {
   const int32_t* sconn;      // [slot][npe]: connectivity in colour-slot order
   double* vals;              // CSR values (FP64) or NULL
   float* vals32;             // CSR values (FP32) or NULL
+  // node-pair block form of the tangent (the Hexa8 loop adds into it instead of the CSR):
+  // block q = 16 values, free rows then free columns packed (row a at 4 a), at the
+  // block's index in the node-block pattern; blkofs [slot][npe][npe] gives q per element
+  // node pair (-1 where a node has no free dof), bit 30 set for the block's first
+  // element in colour-slot order, which stores the block instead of adding to it
+  const int32_t* blkofs;
+  double* kb;
+  float* kb32;
   double* rhs;               // nfree
   double* probeK;            // element probe outputs (NULL in normal runs)
   double* probeF;
@@ -194,6 +202,19 @@ struct sfem_ctx {
   int32_t* d_dbc_mult = nullptr;
   double* d_vals = nullptr;
   float* d_vals32 = nullptr;
+  // node-pair block form of the tangent (element types whose loop assembles it: kb_on);
+  // the CSR values are then derived from it on demand (csr_version = la_K_version of the
+  // last conversion)
+  bool kb_on = false;
+  int64_t nbpairs = 0;                  // node-pair blocks of the pattern
+  std::vector<int64_t> blkPtr;          // node block -> its first pair (nodes with free dofs, node order)
+  int32_t* d_blkofs = nullptr;          // element node pair -> block, colour-slot order
+  int32_t* d_natofs = nullptr;          // per block: CSR column offset in its rows | free columns << 24
+  int64_t* d_blkPtr = nullptr;
+  double* d_kb = nullptr;
+  float* d_kb32 = nullptr;
+  int64_t csr_version = -1;
+  bool kb_fp32 = false;                 // the last tangent went to d_kb32 (mixed precision)
   double* d_rhs = nullptr;
   double* d_du = nullptr;
   // SolutionData vectors (nNode*ndof each)

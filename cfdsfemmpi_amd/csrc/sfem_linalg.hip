@@ -1180,6 +1180,52 @@ __global__ __launch_bounds__(256) void k_bsr_gather(int64_t nblk, const BDesc* _
   }
 }
 
+// Block values from the assembled node-pair block form: block (p, t) is block kbsrc of
+// kb (16 values, free rows and columns packed to the front, the rest 0).  64 lanes per
+// block row, four block rows per workgroup; one 4-wide load and store per lane.
+template <typename VT>
+__global__ __launch_bounds__(256) void k_bsr_gather_kb(int64_t nblk, const BDesc* __restrict__ desc,
+                                                       const int32_t* __restrict__ kbsrc,
+                                                       const VT* __restrict__ kb, VT* __restrict__ dst) {
+  const int64_t p = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
+  if (p >= nblk) return;
+  const int lane = threadIdx.x % 64;
+  const BDesc d = desc[p];
+  const int m = d.dm >> 16;
+  for (int idx = lane; idx < m * d.nb; idx += 64) {
+    const int a = idx / d.nb, t = idx - a * d.nb;
+    const V4<VT> v = ld4<VT>(kb + 16 * (int64_t)kbsrc[d.bptr + t] + 4 * a);
+    st4(dst + (int64_t)d.vbase + (int64_t)a * 4 * d.nb + 4 * t, v.v[0], v.v[1], v.v[2], v.v[3]);
+  }
+}
+
+// CSR values from the node-pair block form: node block k's rows r0 + a, its pair q's
+// free columns at (natofs[q] & 0xffffff) in each row (natofs[q] >> 24 of them).  One
+// wavefront per node block, four per workgroup.
+template <typename VT>
+__global__ __launch_bounds__(256) void k_kb_to_csr(int64_t nblk, const BlkDesc* __restrict__ desc,
+                                                   const int64_t* __restrict__ blkPtr,
+                                                   const int32_t* __restrict__ natofs, const VT* __restrict__ kb,
+                                                   VT* __restrict__ vals) {
+  const int64_t k = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
+  if (k >= nblk) return;
+  const int lane = threadIdx.x % 64;
+  const BlkDesc d = desc[k];
+  const int m = d.dqm >> 16;
+  const int64_t q0 = blkPtr[k];
+  const int nbk = (int)(blkPtr[k + 1] - q0);
+  for (int idx = lane; idx < m * nbk; idx += 64) {
+    const int a = idx / nbk, t = idx - a * nbk;
+    const int32_t o = natofs[q0 + t];
+    const int mj = o >> 24;
+    const V4<VT> v = ld4<VT>(kb + 16 * (q0 + t) + 4 * a);
+    VT* row = vals + (int64_t)d.base + (int64_t)a * d.rl + (o & 0xffffff);
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      if (b < mj) row[b] = v.v[b];
+  }
+}
+
 // LDS of one row being factorised: its values in FP64 (rows[a][4 cap]), a pivot row's
 // upper blocks (urow, storage type), the row's neighbour blocks and the positions of the
 // pivot's upper blocks in it, and small per-pivot scratch.
@@ -1902,7 +1948,7 @@ static std::vector<BlkDesc> make_desc(const std::vector<int32_t>& bs, const std:
 
 IluPlan::~IluPlan() {
   void* ptrs[] = {d_levL, d_levU, d_bdesc, d_bcol, d_bcolofs, d_r0orig, d_padmap, d_natrows, d_tileStart,
-                  d_swoff, d_trowL, d_trowU, d_fpofs, d_fprec};
+                  d_swoff, d_trowL, d_trowU, d_fpofs, d_fprec, d_kbsrc};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
 }
@@ -2119,16 +2165,25 @@ static int build_ilu_plan(sfem_ctx* ctx, LinAlg* L, int reorder) {
     L->ilu = nullptr;
     return set_error(SFEM_ERR_UNSUPPORTED, "ILU(0): a node couples to more than 64 node blocks");
   }
+  // the assembled block form numbers a block row's pairs like nbPtr (node order)
+  const bool kb = ctx->kb_on;
+  if (kb && (ctx->blkPtr.size() != (size_t)nb + 1 || ctx->blkPtr[nb] != npairs)) {
+    delete L->ilu;
+    L->ilu = nullptr;
+    return set_error(SFEM_ERR_STATE, "ILU(0): node-pair blocks do not match the pattern");
+  }
+  std::vector<int32_t> kbsrc(kb ? npairs : 0);
   parallel_for(nb, [&](int64_t lo, int64_t hi) {
-    std::vector<std::pair<int32_t, int32_t>> tmp;
+    std::vector<std::pair<int32_t, int64_t>> tmp;
     for (int64_t p = lo; p < hi; ++p) {
       const int32_t k = order[p];
       tmp.clear();
-      for (int64_t q = nbPtr[k]; q < nbPtr[k + 1]; ++q) tmp.push_back({iord[nbB[q]], nbOfs[q]});
+      for (int64_t q = nbPtr[k]; q < nbPtr[k + 1]; ++q) tmp.push_back({iord[nbB[q]], q});
       std::sort(tmp.begin(), tmp.end());
       for (size_t i = 0; i < tmp.size(); ++i) {
         bcol[bptr[p] + i] = tmp[i].first;
-        bcolofs[bptr[p] + i] = tmp[i].second;
+        bcolofs[bptr[p] + i] = nbOfs[tmp[i].second];
+        if (kb) kbsrc[bptr[p] + i] = (int32_t)tmp[i].second;
         if (tmp[i].first == p) tdiag[p] = (int32_t)i;
       }
     }
@@ -2197,6 +2252,7 @@ static int build_ilu_plan(sfem_ctx* ctx, LinAlg* L, int reorder) {
       (rc = upload(&P.d_padmap, padmap)) || (rc = upload(&P.d_levL, P.h_levL)) || (rc = upload(&P.d_levU, P.h_levU)))
     return rc;
   if (reorder != 0 && (rc = upload(&P.d_natrows, iord))) return rc;  // block rows in original order
+  if (kb && (rc = upload(&P.d_kbsrc, kbsrc))) return rc;
   if ((rc = build_fac_pairs(ctx, P))) return rc;
   if (P.tiled) {
     // what the tile sweeps assume: at most kTileMax rows per tile, every row coupled to
@@ -2326,6 +2382,10 @@ int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f, 
     L->pc_fp32 = fp32;
     L->pc_store32 = false;
   }
+  if (pc_type != SFEM_PC_ILU0) {
+    int rc = la_ensure_csr(ctx);
+    if (rc) return rc;
+  }
   const double* vals = fp32 ? nullptr : ctx->d_vals;
   const float* vals32 = fp32 ? ctx->d_vals32 : nullptr;
   LA_CHECK(hipMemsetAsync(L->d_scal + S_MISC1, 0, sizeof(double), st));
@@ -2367,16 +2427,25 @@ int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f, 
     // the gather gives the block tangent of the Krylov products; the factorisation reads
     // its starting values from it
     const unsigned gb = (unsigned)std::max<int64_t>((P.nblk + 3) / 4, 1);
+    if (ctx->kb_on && ctx->kb_fp32 != fp32) return set_error(SFEM_ERR_STATE, "the tangent was assembled in the other precision");
     if (fp32) {
       int rc;
       if ((rc = ensure_vec(&L->kp32, nv)) || (rc = ensure_vec(&L->lu32, nv))) return rc;
-      hipLaunchKernelGGL((k_bsr_gather<float>), dim3(gb), dim3(256), 0, st, P.nblk, P.d_bdesc, P.d_bcolofs,
-                         P.d_r0orig, ctx->d_rowptr32, ctx->d_vals32, L->kp32);
+      if (ctx->kb_on)
+        hipLaunchKernelGGL((k_bsr_gather_kb<float>), dim3(gb), dim3(256), 0, st, P.nblk, P.d_bdesc, P.d_kbsrc,
+                           ctx->d_kb32, L->kp32);
+      else
+        hipLaunchKernelGGL((k_bsr_gather<float>), dim3(gb), dim3(256), 0, st, P.nblk, P.d_bdesc, P.d_bcolofs,
+                           P.d_r0orig, ctx->d_rowptr32, ctx->d_vals32, L->kp32);
     } else {
       int rc;
       if ((rc = ensure_vec(&L->kp, nv)) || (rc = ensure_vec(&L->lu, nv))) return rc;
-      hipLaunchKernelGGL((k_bsr_gather<double>), dim3(gb), dim3(256), 0, st, P.nblk, P.d_bdesc, P.d_bcolofs,
-                         P.d_r0orig, ctx->d_rowptr32, ctx->d_vals, L->kp);
+      if (ctx->kb_on)
+        hipLaunchKernelGGL((k_bsr_gather_kb<double>), dim3(gb), dim3(256), 0, st, P.nblk, P.d_bdesc, P.d_kbsrc,
+                           ctx->d_kb, L->kp);
+      else
+        hipLaunchKernelGGL((k_bsr_gather<double>), dim3(gb), dim3(256), 0, st, P.nblk, P.d_bdesc, P.d_bcolofs,
+                           P.d_r0orig, ctx->d_rowptr32, ctx->d_vals, L->kp);
     }
     // FP32 storage: the block tangent (and below, the FP64-computed factor) rounded to FP32
     auto to32 = [&](const double* a, float** b) -> int {
@@ -2882,11 +2951,32 @@ int la_tuning_levels(sfem_ctx* ctx, int fp32, int* out, int cap, int* nlevels) {
   return SFEM_OK;
 }
 
+int la_ensure_csr(sfem_ctx* ctx) {
+  if (!ctx->kb_on || ctx->csr_version == ctx->la_K_version) return SFEM_OK;
+  LinAlg* L = ctx->la;
+  const unsigned g = (unsigned)std::max<int64_t>((L->nblk + 3) / 4, 1);
+  if (ctx->kb_fp32) {
+    if (!ctx->d_vals32 || !ctx->d_kb32) return set_error(SFEM_ERR_STATE, "no FP32 tangent assembled");
+    hipLaunchKernelGGL((k_kb_to_csr<float>), dim3(g), dim3(256), 0, ctx->stream, L->nblk, L->d_desc, ctx->d_blkPtr,
+                       ctx->d_natofs, ctx->d_kb32, ctx->d_vals32);
+  } else {
+    hipLaunchKernelGGL((k_kb_to_csr<double>), dim3(g), dim3(256), 0, ctx->stream, L->nblk, L->d_desc, ctx->d_blkPtr,
+                       ctx->d_natofs, ctx->d_kb, ctx->d_vals);
+  }
+  LA_CHECK(hipGetLastError());
+  ctx->csr_version = ctx->la_K_version;
+  return SFEM_OK;
+}
+
 int la_spmv64(sfem_ctx* ctx, const double* x, double* y) {
+  int rc = la_ensure_csr(ctx);
+  if (rc) return rc;
   spmv_in<double>(ctx, x, y, false);
   return hipGetLastError() == hipSuccess ? SFEM_OK : set_error(SFEM_ERR_HIP, "spmv launch");
 }
 int la_spmv32(sfem_ctx* ctx, const float* x, float* y) {
+  int rc = la_ensure_csr(ctx);
+  if (rc) return rc;
   spmv_in<float>(ctx, x, y, false);
   return hipGetLastError() == hipSuccess ? SFEM_OK : set_error(SFEM_ERR_HIP, "spmv launch");
 }
@@ -2909,7 +2999,7 @@ int la_norm2(sfem_ctx* ctx, const double* a, double* out) {
 int la_residual_norm(sfem_ctx* ctx, const double* b, const double* x, double* out) {
   LinAlg* L = ctx->la;
   int rc;
-  if ((rc = ensure_vec(&L->w, L->nvec))) return rc;
+  if ((rc = la_ensure_csr(ctx)) || (rc = ensure_vec(&L->w, L->nvec))) return rc;
   spmv_in<double>(ctx, x, L->w, false);
   hipLaunchKernelGGL((k_axpby<double>), dim3(grid_for(L->n)), dim3(kTPB), 0, ctx->stream, L->n, 1.0, b, -1.0, L->w);
   LA_CHECK(hipGetLastError());
@@ -3290,6 +3380,7 @@ int la_prepare_solve(sfem_ctx* ctx, const sfem_solver_opts* o) {
   int rc;
   const int reorder = o->pc_type == SFEM_PC_ILU0 ? o->reorder : 0;
   const bool bf16f = (fp32 || store32) && o->pc_type == SFEM_PC_ILU0 && o->factor_bf16 != 0;
+  if (o->pc_type != SFEM_PC_ILU0 && (rc = la_ensure_csr(ctx))) return rc;  // CSR products
   const bool same_pc = L->pc_K_version >= 0 && L->pc_type == o->pc_type && L->pc_fp32 == (fp32 || store32) &&
                        L->pc_store32 == store32 && L->pc_reorder == reorder && L->pc_bf16 == bf16f;
   if (!same_pc || (L->kb_version != ctx->la_K_version && ++L->lag_count >= L->pc_lag)) {

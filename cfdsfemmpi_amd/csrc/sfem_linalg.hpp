@@ -33,6 +33,7 @@ struct IluPlan {
   BDesc* d_bdesc = nullptr;
   int32_t* d_bcol = nullptr;     // neighbour blocks (new positions), per block row
   int32_t* d_bcolofs = nullptr;  // their first column's offset in the DOF-ordered CSR row
+  int32_t* d_kbsrc = nullptr;    // their node-pair block in the assembled block form (ctx->kb_on)
   int32_t* d_r0orig = nullptr;   // first DOF row of each block row
   int32_t* d_padmap = nullptr;   // padded slot -> DOF row, -1 for padding
   int32_t* d_natrows = nullptr;  // block rows in the original node order (reordered plans)
@@ -199,6 +200,9 @@ int la_direct_setup(sfem_ctx* ctx);
 int la_direct_solve(sfem_ctx* ctx, int* iters, double* relres);
 int la_direct_info(sfem_ctx* ctx, int* kl, int* ku);
 void la_direct_destroy(LinAlg* L);
+// CSR values from the assembled node-pair block form when the tangent changed since
+// the last conversion (element loops that assemble the block form, ctx->kb_on)
+int la_ensure_csr(sfem_ctx* ctx);
 int la_spmv64(sfem_ctx* ctx, const double* x, double* y);
 int la_spmv32(sfem_ctx* ctx, const float* x, float* y);
 
