@@ -303,7 +303,7 @@ __device__ inline void phase_interp(LDS& L, int l) {
 // evaluateStabParams_algo3, stabilisationRoutines.h:128-143, with G = 4/h^2 I), the
 // residual contribution Fk of node k at gp, and the per-(node, gp) factors of the
 // tangent, for lane l = gp * NPE + k.
-template <int ET, class LDS>
+template <int ET, bool WANT_K = true, class LDS>
 __device__ inline void phase_gauss(LDS& L, int l, double h, const ElemParams& p, double (&Fk)[4]) {
   using C = Traits<ET>;
   constexpr int D = C::NDIM;
@@ -360,6 +360,7 @@ __device__ inline void phase_gauss(LDS& L, int l, double h, const ElemParams& p,
   for (int a = 1; a < D; ++a) br += b[a] * rStab[a];
   Fk[D] = C::SGN * (-(b4 * trace) - tauP * br);
   for (int a = D + 1; a < 4; ++a) Fk[a] = 0.0;
+  if constexpr (!WANT_K) return;
   // tangent factors
   const double stp = C::SGN * tauP;
   for (int c = 0; c < D; ++c) {
@@ -605,13 +606,20 @@ __device__ inline void wave_sync() {
 // to are loaded before its tangent is accumulated (no other element of the class owns
 // them).  The residual sums over Gauss points and the lift sums over neighbour nodes are
 // butterfly reductions across lanes.
-// Register budget: two wavefronts per SIMD (the tangent phase holds the 4 x 4 block,
-// the CSR values it adds to and two Gauss points' worth of LDS operands).
-constexpr int kHexOcc = 2;
-constexpr int kHexGpUnroll = 2;  // Gauss points of the tangent loop unrolled together
+// Register budget per mode: FP64 tangent (0), two wavefronts per SIMD, the tangent
+// phase holding the 4 x 4 block, the block values it adds to (loaded before the
+// accumulation) and two Gauss points' worth of LDS operands; FP32 tangent (2), three
+// wavefronts per SIMD, one Gauss point at a time and the block values loaded after the
+// accumulation; residual only (1), four wavefronts per SIMD.
+template <int MODE>
+struct HexCfg {
+  static constexpr int occ = MODE == 2 ? 3 : MODE == 1 ? 4 : 2;
+  static constexpr int unroll = MODE == 2 ? 1 : 2;  // Gauss points of the tangent loop unrolled together
+  static constexpr bool late_old = MODE == 2;
+};
 
 template <int MODE>
-__global__ __launch_bounds__(64, kHexOcc) void k_assemble_hex8(AsmView v, ElemParams p,
+__global__ __launch_bounds__(64, HexCfg<MODE>::occ) void k_assemble_hex8(AsmView v, ElemParams p,
                                                       const int32_t* __restrict__ elems, int64_t n,
                                                       int lift) {
   using T = typename std::conditional<MODE == 2, float, double>::type;
@@ -682,7 +690,7 @@ __global__ __launch_bounds__(64, kHexOcc) void k_assemble_hex8(AsmView v, ElemPa
     phase_interp<HEXA8>(L, l);
     wave_sync();
     double Fk[4];
-    phase_gauss<HEXA8>(L, l, h_cur, pe, Fk);
+    phase_gauss<HEXA8, MODE != 1>(L, l, h_cur, pe, Fk);
     // nodal data and length scale of the next element
     if (more) {
       issue_node(nid_n);
@@ -713,10 +721,16 @@ __global__ __launch_bounds__(64, kHexOcc) void k_assemble_hex8(AsmView v, ElemPa
       const bool first = (co_cur >> 30) & 1;
       blk_t* bq = reinterpret_cast<blk_t*>(kbv + (int64_t)(blk ? (co_cur & ((1 << 30) - 1)) : 0) * 16);
       blk_t old[NDOF];
+      if constexpr (!HexCfg<MODE>::late_old) {
 #pragma unroll
-      for (int a = 0; a < NDOF; ++a) old[a] = (blk && !first) ? bq[a] : blk_t{(T)0, (T)0, (T)0, (T)0};
+        for (int a = 0; a < NDOF; ++a) old[a] = (blk && !first) ? bq[a] : blk_t{(T)0, (T)0, (T)0, (T)0};
+      }
       T K[NDOF][NDOF];
-      accumulate_block<HEXA8, T, kHexGpUnroll>(L, i, j, pe, K);
+      accumulate_block<HEXA8, T, HexCfg<MODE>::unroll>(L, i, j, pe, K);
+      if constexpr (HexCfg<MODE>::late_old) {
+#pragma unroll
+        for (int a = 0; a < NDOF; ++a) old[a] = (blk && !first) ? bq[a] : blk_t{(T)0, (T)0, (T)0, (T)0};
+      }
       if (doLift) {
 #pragma unroll
         for (int a = 0; a < NDOF; ++a) {
@@ -849,7 +863,7 @@ static hipError_t hex8_launch(hipStream_t s, const AsmView& v, const ElemParams&
     int cus = 0, per = 0;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_assemble_hex8<MODE>, 64, 0) != hipSuccess || per < 1)
-      per = 4 * kHexOcc;
+      per = 4 * HexCfg<MODE>::occ;
     resident[dev] = (int64_t)(cus > 0 ? cus : 256) * per;
   }
   const unsigned grid = (unsigned)std::min<int64_t>(n, resident[dev]);
