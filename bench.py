@@ -103,8 +103,20 @@ def run_path(mesh, ctl, prec, opts, newton_tol, steps, warmup, label, predictor=
           (("assembly", sfem.KT_ASSEMBLY), ("spmv", sfem.KT_SPMV), ("pc", sfem.KT_PC))}
     log(f"{label}: {steps} steps in {t2 - t1:.3f} s; device ms by class: " +
         ", ".join(f"{k} {v[1]:.1f} ({v[0]} launches)" for k, v in kt.items()))
+    # one tangent + residual pass and one residual-only pass timed on their own (best of
+    # three), after the timed steps (the state is not advanced by them)
+    def pass_ms(want_k):
+        best = None
+        for _ in range(3):
+            c0, m0 = ctx.kernel_timing(sfem.KT_ASSEMBLY)
+            ctx.assemble(1, want_K=want_k)
+            c1, m1 = ctx.kernel_timing(sfem.KT_ASSEMBLY)
+            best = (m1 - m0) if best is None else min(best, m1 - m0)
+        return best
+    asm_pass = {"K_F_ms": pass_ms(True), "F_ms": pass_ms(False)}
     res = dict(label=label, seconds=t2 - t1, steps_per_s=steps / (t2 - t1), hist=hist, kt=kt, info=info,
-               soln=s, tune=tune, block=ctx.block_info(), tiles=ctx.tile_info(), stats=ctx.solver_stats())
+               soln=s, tune=tune, block=ctx.block_info(), tiles=ctx.tile_info(), stats=ctx.solver_stats(),
+               asm_pass=asm_pass)
     ctx.close()
     return res
 
@@ -700,10 +712,24 @@ def main():
     n_f = sum(h[0] for h in main_res["hist"]) - n_k
     cnt, ms = kt["assembly"]
     flops = mesh.nElem * (n_k * HEXA8_FLOPS_KF + n_f * HEXA8_FLOPS_F)
-    roof_asm = {"bound": "vector", "kernel": "k_assemble (all colour launches of the run)",
-                "achieved": flops / (ms * 1e-3) / 1e12, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "passes_K_F": n_k, "passes_F": n_f}
+    # one tangent + residual pass: the tangent arithmetic (Ke) runs in the tangent's type
+    # (FP32 on the mixed path, against the FP32 peak), the residual (Fe) in FP64; the peak
+    # quoted is the rate at which that mix would run with both units at their peaks
+    ap_ = main_res["asm_pass"]
+    fl_k, fl_f = mesh.nElem * (HEXA8_FLOPS_KF - HEXA8_FLOPS_F), mesh.nElem * HEXA8_FLOPS_F
+    t_bound = fl_k / ((FP32_PEAK_TFLOPS if fp32 else FP64_PEAK_TFLOPS) * 1e12) + fl_f / (FP64_PEAK_TFLOPS * 1e12)
+    roof_asm = {"bound": "vector", "kernel": "k_assemble_hex8<%d> (one tangent + residual pass, %d colour launches)"
+                % (2 if fp32 else 0, info["ncolors"]),
+                "achieved": (fl_k + fl_f) / (ap_["K_F_ms"] * 1e-3) / 1e12,
+                "peak": (fl_k + fl_f) / t_bound / 1e12, "unit": "TFLOP/s",
+                "tangent_type": "fp32" if fp32 else "fp64", "pass_ms": ap_["K_F_ms"],
+                "flops_per_element": HEXA8_FLOPS_KF,
+                "residual_only_pass": {"kernel": "k_assemble_hex8<1>", "pass_ms": ap_["F_ms"],
+                                       "achieved": fl_f / (ap_["F_ms"] * 1e-3) / 1e12, "peak": FP64_PEAK_TFLOPS,
+                                       "flops_per_element": HEXA8_FLOPS_F},
+                "run": {"passes_K_F": n_k, "passes_F": n_f, "achieved": flops / (ms * 1e-3) / 1e12}}
     roof_asm["frac"] = roof_asm["achieved"] / roof_asm["peak"]
+    roof_asm["residual_only_pass"]["frac"] = roof_asm["residual_only_pass"]["achieved"] / FP64_PEAK_TFLOPS
     cpu = None
     if not a.no_cpu and r64 is not None:
         log("cpu baseline (oracle) ...")
