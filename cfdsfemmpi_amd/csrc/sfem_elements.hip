@@ -610,10 +610,10 @@ __device__ inline void wave_sync() {
 // phase holding the 4 x 4 block, the block values it adds to (loaded before the
 // accumulation) and two Gauss points' worth of LDS operands; FP32 tangent (2), three
 // wavefronts per SIMD, one Gauss point at a time and the block values loaded after the
-// accumulation; residual only (1), four wavefronts per SIMD.
+// accumulation; residual only (1), three wavefronts per SIMD without the tangent factors.
 template <int MODE>
 struct HexCfg {
-  static constexpr int occ = MODE == 2 ? 3 : MODE == 1 ? 4 : 2;
+  static constexpr int occ = MODE == 0 ? 2 : 3;
   static constexpr int unroll = MODE == 2 ? 1 : 2;  // Gauss points of the tangent loop unrolled together
   static constexpr bool late_old = MODE == 2;
 };
