@@ -506,6 +506,7 @@ int sfem_build_pattern(sfem_ctx* c, int64_t* nfree_out, int64_t* nnz_out) {
   if (c->la) la_destroy(c);
   // the Hexa8 loop assembles the node-pair block form
   c->kb_on = c->et == HEXA8;
+  for (int m = 0; m < 3; ++m) c->asm_variant[m] = -1;
   c->csr_version = -1;
   int rc;
   if ((rc = dalloc(&c->d_nodeInfo, info.size())) || (rc = dalloc(&c->d_colofs, scolofs.size())) ||
@@ -617,7 +618,7 @@ int sfem_set_pc_lag(sfem_ctx* c, int lag) {
   return SFEM_OK;
 }
 
-static int run_element_loop(sfem_ctx* c, int mode, int lift);
+static int run_element_loop(sfem_ctx* c, int mode, int lift, int variant = -1, bool timed = true);
 
 // ||F_free|| at the state x with the prescribed values of this step in place (x is
 // overwritten at the prescribed DOFs).  The residual is affine in the state (the
@@ -677,22 +678,75 @@ int sfem_time_update(sfem_ctx* c, double timeFact) {
   return SFEM_OK;
 }
 
-static int run_element_loop(sfem_ctx* c, int mode, int lift) {
+static int run_element_loop(sfem_ctx* c, int mode, int lift, int variant, bool timed) {
   const AsmView v0 = asm_view(c);
   const ElemParams p = elem_params(c);
   const int npe = c->dims.npe;
+  if (variant < 0) variant = std::max(0, c->asm_variant[mode]);
   for (int k = 0; k < c->ncolors; ++k) {
     const int64_t n = c->colorPtr[k + 1] - c->colorPtr[k];
     AsmView v = v0;  // slot-ordered arrays start at the colour's first slot
     v.colofs = v0.colofs + (int64_t)c->colorPtr[k] * npe * npe;
     if (v0.blkofs) v.blkofs = v0.blkofs + (int64_t)c->colorPtr[k] * npe * npe;
     v.sconn = v0.sconn + (int64_t)c->colorPtr[k] * npe;
-    c->kt.begin(SFEM_KT_ASSEMBLY, c->stream);
-    HIP_OK(launch_assemble(c->et, c->stream, v, p, c->d_elemByColor + c->colorPtr[k], n, mode, lift, -1));
-    c->kt.end(SFEM_KT_ASSEMBLY, c->stream);
+    if (timed) c->kt.begin(SFEM_KT_ASSEMBLY, c->stream);
+    HIP_OK(launch_assemble(c->et, c->stream, v, p, c->d_elemByColor + c->colorPtr[k], n, mode, lift, -1, variant));
+    if (timed) c->kt.end(SFEM_KT_ASSEMBLY, c->stream);
   }
   return SFEM_OK;
 }
+
+// Element-loop variant of an assembly mode: the fastest of the variants timed over the
+// whole colour loop (each run starts from cleared outputs; the caller clears and
+// assembles again), or variant 0 under fixed tuning and for Hexa8.
+static int choose_element_variant(sfem_ctx* c, int mode, int lift) {
+  if (c->asm_variant[mode] >= 0) return SFEM_OK;
+  if (c->et == HEXA8 || !c->la || c->la->tune_mode == SFEM_TUNE_FIXED) {
+    c->asm_variant[mode] = 0;
+    return SFEM_OK;
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIP_OK(hipEventCreate(&e0));
+  HIP_OK(hipEventCreate(&e1));
+  int best = 0;
+  float best_ms = 0.f;
+  for (int var = 0; var < kAsmVariants; ++var) {
+    HIP_OK(launch_zero_f64(c->stream, c->nfree, c->d_rhs));
+    if (mode == 0) HIP_OK(launch_zero_f64(c->stream, c->nnz, c->d_vals));
+    if (mode == 2) HIP_OK(launch_zero_f32(c->stream, c->nnz, c->d_vals32));
+    HIP_OK(hipEventRecord(e0, c->stream));
+    int rc = run_element_loop(c, mode, lift, var, false);
+    if (rc) return rc;
+    HIP_OK(hipEventRecord(e1, c->stream));
+    HIP_OK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+    if (var == 0 || ms < best_ms) { best = var; best_ms = ms; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c->asm_variant[mode] = best;
+  return SFEM_OK;
+}
+
+// The residual-only assembly of a Newton iteration after the first, as kernel launches
+// only (no events, no host synchronisation), so that it can be captured in a graph:
+// iteration state, cleared RHS, the element loop.  asm_prepare_residual chooses the
+// element-loop variant first (it times variants and synchronises).
+}  // extern "C"
+
+namespace sfem {
+int asm_prepare_residual(sfem_ctx* c) { return choose_element_variant(c, 1, 0); }
+
+int asm_residual_kernels(sfem_ctx* c) {
+  HIP_OK(launch_iter_step(c->stream, nstate(c), time_coeffs(c), c->d_soln, c->d_prev, c->d_prev2, c->d_prev3,
+                          c->d_prev4, c->d_dot, c->d_dotPrev, c->d_cur, c->d_dotCur));
+  HIP_OK(launch_zero_f64(c->stream, c->nfree, c->d_rhs));
+  return run_element_loop(c, 1, 0, -1, false);
+}
+}  // namespace sfem
+
+extern "C" {
 
 int sfem_assemble(sfem_ctx* c, int iter, int want_K, double* rhs_norm) {
   if (!c) return set_error(SFEM_ERR_ARG, "null context");
@@ -715,6 +769,11 @@ int sfem_assemble(sfem_ctx* c, int iter, int want_K, double* rhs_norm) {
     HIP_OK(launch_add_bc(c->stream, nd, c->d_dbc_ind, c->d_dbc_mult, c->d_applied, c->d_soln));
   HIP_OK(launch_iter_step(c->stream, n, time_coeffs(c), c->d_soln, c->d_prev, c->d_prev2, c->d_prev3, c->d_prev4,
                           c->d_dot, c->d_dotPrev, c->d_cur, c->d_dotCur));
+  const int mode0 = want_K ? (mixed ? 2 : 0) : 1;
+  if (c->asm_variant[mode0] < 0) {
+    int rc0 = choose_element_variant(c, mode0, (!mixed && iter == 0) ? 1 : 0);
+    if (rc0) return rc0;
+  }
   HIP_OK(hipEventRecord(c->ev0, c->stream));
   HIP_OK(launch_zero_f64(c->stream, c->nfree, c->d_rhs));
   int mode;
@@ -935,6 +994,12 @@ int sfem_kernel_timing(sfem_ctx* c, int kind, int64_t* launches, double* total_m
   c->kt.resolve();
   if (launches) *launches = c->kt.count[kind];
   if (total_ms) *total_ms = c->kt.ms[kind];
+  return SFEM_OK;
+}
+
+int sfem_element_tuning(sfem_ctx* c, int* out) {
+  if (!c || !out) return set_error(SFEM_ERR_ARG, "null argument");
+  for (int m = 0; m < 3; ++m) out[m] = c->asm_variant[m];
   return SFEM_OK;
 }
 

@@ -8,9 +8,11 @@
 // solve: BiCGStab with per-mesh scalars (alpha, omega, rho, ...), per-mesh convergence
 // ||r_m|| <= rtol ||b_m|| and per-mesh breakdown restarts, so that each mesh follows
 // the iteration it would follow alone.  Reductions are per mesh and deterministic
-// (fixed chunk order).  One BiCGStab iteration of all meshes is captured once per solve
-// as a HIP graph and replayed; the host reads one counter of finished meshes per
-// iteration.
+// (fixed chunk order).  A time step runs its Newton iterations as HIP graphs
+// (step_batch_graph): the residual assembly, per-mesh norms and Newton test, the start
+// and end of each solve and the solution update are captured once per step, the BiCGStab
+// iterations of all meshes replay in graphs of a few iterations, and the host reads one
+// status word per replay.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -66,8 +68,17 @@ struct Batch {
   size_t vec_bytes = 0;
   bool use_graph = true;
   bool narrowing = true;
+  // graph-captured Newton step: per-mesh Newton iterations, Krylov iterations, residual
+  // norms and the step status (k_b_step_reset), on the device
+  int32_t* d_newton = nullptr;
+  int32_t* d_kry = nullptr;
+  double* d_rhsn = nullptr;
+  int32_t* d_status = nullptr;
+  int32_t* h_status = nullptr;    // pinned
+  int its_per_graph = 4;          // BiCGStab iterations per replayed graph in the graph-captured step
   ~Batch() {
-    void* ptrs[] = {d_dof_ptr, d_chunk, d_cchunk, d_mptr, d_mlist, d_sc, d_part, d_norm, d_active, d_ndone};
+    void* ptrs[] = {d_dof_ptr, d_chunk, d_cchunk, d_mptr, d_mlist, d_sc, d_part, d_norm, d_active, d_ndone,
+                    d_newton, d_kry, d_rhsn, d_status};
     for (void* q : ptrs)
       if (q) (void)hipFree(q);
     for (void* q : vec)
@@ -75,6 +86,7 @@ struct Batch {
     if (h_norm) (void)hipHostFree(h_norm);
     if (h_ndone) (void)hipHostFree(h_ndone);
     if (h_sc) (void)hipHostFree(h_sc);
+    if (h_status) (void)hipHostFree(h_status);
   }
 };
 
@@ -342,10 +354,77 @@ __global__ __launch_bounds__(kTB) void k_b_final(int nmesh, int op, const int32_
       }
     }
   }
-  if (op == F_RR_RHO) {
+  if (op == F_RR_RHO || op == F_INIT) {
+    if (op == F_INIT)
+      for (int m = threadIdx.x; m < nmesh; m += blockDim.x) mine += sc[(int64_t)m * B_N + B_DONE] != 0.0 ? 1 : 0;
     atomicAdd(&cnt, mine);
     __syncthreads();
     if (threadIdx.x == 0) *ndone = cnt;
+  }
+}
+
+// Per-step device state of the graph-captured Newton step: status = {finished meshes of
+// the current solve (written by k_b_final), meshes still iterating Newton, first failed
+// mesh (INT_MAX: none), Newton iterations done}.
+enum { ST_NDONE = 0, ST_NACTIVE, ST_ERR, ST_ITER, ST_NSTALL, ST_N = 8 };
+
+// start of a step: every mesh active, counters cleared
+__global__ __launch_bounds__(kTB) void k_b_step_reset(int nmesh, int max_newton, int32_t* __restrict__ active,
+                                                      int32_t* __restrict__ newton, int32_t* __restrict__ kry,
+                                                      double* __restrict__ rhsn, int32_t* __restrict__ status) {
+  for (int m = threadIdx.x; m < nmesh; m += blockDim.x) {
+    active[m] = 1;
+    newton[m] = max_newton;
+    kry[m] = 0;
+    rhsn[m] = 0.0;
+  }
+  if (threadIdx.x == 0) {
+    status[ST_NDONE] = 0;
+    status[ST_NACTIVE] = nmesh;
+    status[ST_ERR] = 0x7fffffff;
+    status[ST_ITER] = 0;
+    status[ST_NSTALL] = 0;
+  }
+}
+
+// Newton test of every active mesh on the device (||F_m|| < tol ends its iterations,
+// StabFEM2.cpp:740-749): its residual norm and iteration count are recorded, and the
+// meshes still iterating are counted.
+__global__ __launch_bounds__(kTB) void k_b_activity(int nmesh, const double* __restrict__ norm, double tol,
+                                                    int32_t* __restrict__ active, int32_t* __restrict__ newton,
+                                                    double* __restrict__ rhsn, int32_t* __restrict__ status) {
+  __shared__ int cnt;
+  const int it = status[ST_ITER];
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  int mine = 0;
+  for (int m = threadIdx.x; m < nmesh; m += blockDim.x) {
+    if (!active[m]) continue;
+    rhsn[m] = norm[m];
+    if (norm[m] < tol) {
+      active[m] = 0;
+      newton[m] = it + 1;
+    } else {
+      ++mine;
+    }
+  }
+  atomicAdd(&cnt, mine);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    status[ST_NACTIVE] = cnt;
+    status[ST_ITER] = it + 1;
+  }
+}
+
+// end of a batched solve: Krylov iterations per mesh, stalled FP32 mesh solves, and the
+// first mesh that broke down or did not converge
+__global__ __launch_bounds__(kTB) void k_b_tail(int nmesh, const double* __restrict__ sc, int32_t* __restrict__ kry,
+                                                int32_t* __restrict__ status) {
+  for (int m = threadIdx.x; m < nmesh; m += blockDim.x) {
+    const double* S = sc + (int64_t)m * B_N;
+    kry[m] += (int32_t)S[B_ITS];
+    if (S[B_STALL] != 0.0) atomicAdd(&status[ST_NSTALL], 1);
+    if (S[B_BAD] != 0.0 || S[B_DONE] == 0.0) atomicMin(&status[ST_ERR], m);
   }
 }
 
@@ -446,12 +525,17 @@ static int bb_iteration(sfem_ctx* ctx, Batch& B, double rtol, int stall, const i
 }
 
 // Batched solve of A du = rhs with per-mesh right-preconditioned BiCGStab; meshes with
-// active[m] = 0 get du_m = 0.  its[m]: iterations of mesh m.
+// active[m] = 0 get du_m = 0.  In pieces: solve_prepare (preconditioner and Krylov
+// layout; host work), solve_begin and solve_end (kernel launches only, capturable), and
+// solve_iterate (BiCGStab iterations of all meshes, replayed as graphs of `per_graph`
+// iterations until every mesh is done; the host reads the count of finished meshes
+// after each replay).  A finished mesh's vectors and scalars are left alone by every
+// kernel of an iteration, so replaying more iterations than a mesh needs changes
+// nothing for it.
 template <typename T>
-static int batch_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int32_t* its, int* bad_mesh) {
+static int solve_prepare(sfem_ctx* ctx, const sfem_solver_opts* o) {
   Batch& B = *ctx->batch;
   LinAlg* L = ctx->la;
-  hipStream_t st = ctx->stream;
   int rc;
   if ((rc = la_prepare_solve(ctx, o))) return rc;
   if (!L->block_space) return set_error(SFEM_ERR_UNSUPPORTED, "batched solve: ILU(0) preconditioner required");
@@ -463,33 +547,75 @@ static int batch_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int32_t* its, i
     for (void*& q : B.vec) BA_CHECK(hipMalloc(&q, vb));
     B.vec_bytes = vb;
   }
+  return SFEM_OK;
+}
+
+template <typename T>
+static int solve_begin(sfem_ctx* ctx, const sfem_solver_opts* o) {
+  Batch& B = *ctx->batch;
+  hipStream_t st = ctx->stream;
   T** V = (T**)B.vec;
   const dim3 g(B.nchunk), tb(kTB);
-  const IluPlan& P = *L->ilu;
-  hipLaunchKernelGGL((k_b_pad_in<T>), g, tb, 0, st, B.d_chunk, P.d_padmap, ctx->d_rhs, B.d_norm, B.d_active, V[9]);
-  // FP32 solves end a mesh's iteration on a stall and keep its best iterate; FP64
-  // solves iterate to the tolerance or report the mesh
+  const IluPlan& P = *ctx->la->ilu;
   const int stall = sizeof(T) == 4 ? kStallWindow : 0;
+  hipLaunchKernelGGL((k_b_pad_in<T>), g, tb, 0, st, B.d_chunk, P.d_padmap, ctx->d_rhs, B.d_norm, B.d_active, V[9]);
   hipLaunchKernelGGL((k_b_init<T>), g, tb, 0, st, B.d_chunk, V[9], V[8], V[0], V[1], V[2], V[3], V[10], B.d_part);
   hipLaunchKernelGGL(k_b_final, dim3(1), dim3(kTB), 0, st, B.nmesh, (int)F_INIT, B.d_mptr, B.d_mlist, B.d_part,
                      B.d_sc, o->rtol, stall, B.d_ndone);
   BA_CHECK(hipGetLastError());
-  // One iteration as a graph, replayed.  While meshes finish, the iteration is narrowed
-  // to the chunks and block rows of those still iterating (re-captured each time the
-  // remaining block rows fall below half of those the iteration covers).
+  return SFEM_OK;
+}
+
+// giveup: FP32 meshes still iterating take their best iterate back
+template <typename T>
+static int solve_end(sfem_ctx* ctx, const sfem_solver_opts* o, bool giveup) {
+  Batch& B = *ctx->batch;
+  hipStream_t st = ctx->stream;
+  T** V = (T**)B.vec;
+  const dim3 g(B.nchunk), tb(kTB);
+  const IluPlan& P = *ctx->la->ilu;
+  const int stall = sizeof(T) == 4 ? kStallWindow : 0;
+  if (stall && giveup) {
+    hipLaunchKernelGGL(k_b_final, dim3(1), dim3(kTB), 0, st, B.nmesh, (int)F_GIVEUP, B.d_mptr, B.d_mlist, B.d_part,
+                       B.d_sc, o->rtol, stall, B.d_ndone);
+    hipLaunchKernelGGL((k_b_post<T>), g, tb, 0, st, B.d_chunk, B.d_sc, V[0], V[1], V[8], V[10]);
+  }
+  hipLaunchKernelGGL((k_b_pad_out<T>), g, tb, 0, st, B.d_chunk, P.d_padmap, V[8], B.d_norm, B.d_active, ctx->d_du);
+  BA_CHECK(hipGetLastError());
+  return SFEM_OK;
+}
+
+// Iterations until every mesh is done or max_it; returns with *B.h_ndone current.
+template <typename T>
+static int solve_iterate(sfem_ctx* ctx, const sfem_solver_opts* o, int per_graph) {
+  Batch& B = *ctx->batch;
+  LinAlg* L = ctx->la;
+  hipStream_t st = ctx->stream;
+  const IluPlan& P = *L->ilu;
+  const int stall = sizeof(T) == 4 ? kStallWindow : 0;
+  // While meshes finish, the iteration is narrowed to the chunks and block rows of those
+  // still iterating (re-captured each time the remaining block rows fall below half of
+  // those the iteration covers).
   const int4* ch = B.d_chunk;
   int nch = B.nchunk;
   const RowSubset* sub = nullptr;
   int64_t cover = P.nblk;
   hipGraphExec_t exec = nullptr;
   const bool kt_on = ctx->kt.on;
+  auto iterations = [&](int k) -> int {
+    for (int q = 0; q < k; ++q) {
+      int rc1 = bb_iteration<T>(ctx, B, o->rtol, stall, ch, nch, sub);
+      if (rc1) return rc1;
+    }
+    return SFEM_OK;
+  };
   auto capture = [&]() -> int {
     if (exec) { (void)hipGraphExecDestroy(exec); exec = nullptr; }
     if (!B.use_graph) return SFEM_OK;
     ctx->kt.on = false;
     hipGraph_t graph = nullptr;
     BA_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    const int rc1 = bb_iteration<T>(ctx, B, o->rtol, stall, ch, nch, sub);
+    const int rc1 = iterations(per_graph);
     hipError_t ec = hipStreamEndCapture(st, &graph);
     ctx->kt.on = kt_on;
     if (rc1) { if (graph) (void)hipGraphDestroy(graph); return rc1; }
@@ -527,16 +653,21 @@ static int batch_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int32_t* its, i
     ++B.subsets_built;
     return capture();
   };
+  int rc;
+  hipError_t e = hipMemcpyAsync(B.h_ndone, B.d_ndone, sizeof(int32_t), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return set_hip_error(e, "batched solve start");
+  if (*B.h_ndone >= B.nmesh) return SFEM_OK;
   if ((rc = capture())) return rc;
-  int it = 0, seen = -1;
-  for (it = 1; it <= o->max_it; ++it) {
+  int seen = -1;
+  for (int it = 0; it < o->max_it; it += per_graph) {
     if (exec) {
-      hipError_t e = hipGraphLaunch(exec, st);
+      e = hipGraphLaunch(exec, st);
       if (e != hipSuccess) { (void)hipGraphExecDestroy(exec); return set_hip_error(e, "batched iteration launch"); }
-    } else if ((rc = bb_iteration<T>(ctx, B, o->rtol, stall, ch, nch, sub))) {
+    } else if ((rc = iterations(per_graph))) {
       return rc;
     }
-    hipError_t e = hipMemcpyAsync(B.h_ndone, B.d_ndone, sizeof(int32_t), hipMemcpyDeviceToHost, st);
+    e = hipMemcpyAsync(B.h_ndone, B.d_ndone, sizeof(int32_t), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) { if (exec) (void)hipGraphExecDestroy(exec); return set_hip_error(e, "batched iteration"); }
     if (*B.h_ndone >= B.nmesh) break;
@@ -549,23 +680,119 @@ static int batch_solve(sfem_ctx* ctx, const sfem_solver_opts* o, int32_t* its, i
     }
   }
   if (exec) (void)hipGraphExecDestroy(exec);
-  if (stall && *B.h_ndone < B.nmesh) {
-    // FP32 meshes still iterating at the iteration limit also keep their best iterate
-    hipLaunchKernelGGL(k_b_final, dim3(1), dim3(kTB), 0, st, B.nmesh, (int)F_GIVEUP, B.d_mptr, B.d_mlist, B.d_part,
-                       B.d_sc, o->rtol, stall, B.d_ndone);
-    hipLaunchKernelGGL((k_b_post<T>), g, tb, 0, st, B.d_chunk, B.d_sc, V[0], V[1], V[8], V[10]);
-  }
-  hipLaunchKernelGGL((k_b_pad_out<T>), g, tb, 0, st, B.d_chunk, P.d_padmap, V[8], B.d_norm, B.d_active, ctx->d_du);
+  return SFEM_OK;
+}
+
+static int batch_error(Batch& B, int bad, const int32_t* its) {
+  const double* S = B.h_sc + (size_t)bad * B_N;
+  char msg[200];
+  snprintf(msg, sizeof msg, "batched BiCGStab: mesh %d %s after %d iterations (|r|^2 = %.3e, target %.3e)", bad,
+           S[B_BAD] != 0.0 ? "broke down (non-finite residual)" : "did not converge", its ? its[bad] : (int)S[B_ITS],
+           S[B_RR], S[B_TOL2]);
+  return set_error(SFEM_ERR_MAXIT, msg);
+}
+
+// Capture `body` (kernel launches only) on the context stream into an executable graph.
+template <class F>
+static int capture_graph(sfem_ctx* ctx, F body, hipGraphExec_t* exec) {
+  hipStream_t st = ctx->stream;
+  const bool kt_on = ctx->kt.on;
+  ctx->kt.on = false;
+  hipGraph_t graph = nullptr;
+  BA_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  const int rc1 = body();
+  hipError_t ec = hipStreamEndCapture(st, &graph);
+  ctx->kt.on = kt_on;
+  if (rc1) { if (graph) (void)hipGraphDestroy(graph); return rc1; }
+  if (ec != hipSuccess) return set_hip_error(ec, "Newton step capture");
+  ec = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ec != hipSuccess) { *exec = nullptr; return set_hip_error(ec, "Newton step instantiate"); }
+  return SFEM_OK;
+}
+
+// One time step of every mesh with the Newton iteration as graphs (StabFEM2.cpp:654-799
+// per mesh): iteration 0 assembles the tangent and builds the preconditioner eagerly;
+// every later iteration replays a captured head (residual assembly, per-mesh norms, the
+// Newton test on the device, start of the solve), the BiCGStab iterations replay in
+// graphs of B.its_per_graph iterations until every mesh is done, and a captured tail
+// ends the solve (best iterates, increment, iteration counts, failure flags) and updates
+// the solution.  The host reads one status word per head and per replay.
+template <typename T>
+static int step_batch_graph(sfem_ctx* c, const sfem_solver_opts* o, double tol, int max_newton,
+                            int32_t* newton_iters, int32_t* krylov_iters, double* rhs_norm) {
+  Batch& B = *c->batch;
+  hipStream_t st = c->stream;
+  const int nm = B.nmesh;
+  const int stall = sizeof(T) == 4 ? kStallWindow : 0;
+  hipLaunchKernelGGL(k_b_step_reset, dim3(1), dim3(kTB), 0, st, nm, max_newton, B.d_active, B.d_newton, B.d_kry,
+                     B.d_rhsn, B.d_status);
   BA_CHECK(hipGetLastError());
-  BA_CHECK(hipMemcpyAsync(B.h_sc, B.d_sc, (size_t)B.nmesh * B_N * sizeof(double), hipMemcpyDeviceToHost, st));
-  BA_CHECK(hipStreamSynchronize(st));
-  *bad_mesh = -1;
-  for (int m = 0; m < B.nmesh; ++m) {
-    const double* S = B.h_sc + (size_t)m * B_N;
-    its[m] = (int32_t)S[B_ITS];
-    if (S[B_STALL] != 0.0) ++B.stalled;
-    if (S[B_BAD] != 0.0 && *bad_mesh < 0) *bad_mesh = m;
-    if (S[B_DONE] == 0.0 && *bad_mesh < 0) *bad_mesh = m;
+  auto head = [&]() -> int {
+    hipLaunchKernelGGL(k_seg_norm2, dim3(nm), dim3(kTB), 0, st, B.d_dof_ptr, c->d_rhs, B.d_norm);
+    hipLaunchKernelGGL(k_b_activity, dim3(1), dim3(kTB), 0, st, nm, B.d_norm, tol, B.d_active, B.d_newton, B.d_rhsn,
+                       B.d_status);
+    BA_CHECK(hipGetLastError());
+    return solve_begin<T>(c, o);
+  };
+  auto tail = [&]() -> int {
+    int rc1 = solve_end<T>(c, o, stall != 0);
+    if (rc1) return rc1;
+    hipLaunchKernelGGL(k_b_tail, dim3(1), dim3(kTB), 0, st, nm, B.d_sc, B.d_kry, B.d_status);
+    BA_CHECK(hipGetLastError());
+    BA_CHECK(launch_update_solution(st, c->nfree, c->d_assy, c->d_du, c->d_soln));
+    return SFEM_OK;
+  };
+  auto read_status = [&]() -> int {
+    BA_CHECK(hipMemcpyAsync(B.h_status, B.d_status, ST_N * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    BA_CHECK(hipStreamSynchronize(st));
+    return SFEM_OK;
+  };
+  hipGraphExec_t head_exec = nullptr, tail_exec = nullptr;
+  auto cleanup = [&]() {
+    if (head_exec) (void)hipGraphExecDestroy(head_exec);
+    if (tail_exec) (void)hipGraphExecDestroy(tail_exec);
+  };
+  int rc = SFEM_OK;
+  for (int iter = 0; iter < max_newton && rc == SFEM_OK; ++iter) {
+    if (iter == 0) {
+      double gnorm;
+      if ((rc = sfem_assemble(c, 0, 1, &gnorm)) || (rc = solve_prepare<T>(c, o)) || (rc = asm_prepare_residual(c)) ||
+          (rc = head()))
+        break;
+    } else if (B.use_graph) {
+      if (!head_exec &&
+          (rc = capture_graph(c, [&]() -> int { int r = asm_residual_kernels(c); return r ? r : head(); }, &head_exec)))
+        break;
+      hipError_t e = hipGraphLaunch(head_exec, st);
+      if (e != hipSuccess) { rc = set_hip_error(e, "Newton step head"); break; }
+    } else if ((rc = asm_residual_kernels(c)) || (rc = head())) {
+      break;
+    }
+    if ((rc = read_status())) break;
+    if (B.h_status[ST_ERR] != 0x7fffffff) break;  // a mesh failed in the previous solve
+    if (B.h_status[ST_NACTIVE] == 0) break;
+    if ((rc = solve_iterate<T>(c, o, B.use_graph ? B.its_per_graph : 1))) break;
+    if (B.use_graph) {
+      if (!tail_exec && (rc = capture_graph(c, tail, &tail_exec))) break;
+      hipError_t e = hipGraphLaunch(tail_exec, st);
+      if (e != hipSuccess) { rc = set_hip_error(e, "Newton step tail"); break; }
+    } else if ((rc = tail())) {
+      break;
+    }
+  }
+  cleanup();
+  if (rc) return rc;
+  std::vector<double> rn(nm);
+  BA_CHECK(hipMemcpyAsync(newton_iters, B.d_newton, nm * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  BA_CHECK(hipMemcpyAsync(krylov_iters, B.d_kry, nm * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  BA_CHECK(hipMemcpyAsync(rhs_norm, B.d_rhsn, nm * sizeof(double), hipMemcpyDeviceToHost, st));
+  if ((rc = read_status())) return rc;
+  B.stalled += B.h_status[ST_NSTALL];
+  const int bad = B.h_status[ST_ERR];
+  if (bad != 0x7fffffff) {
+    BA_CHECK(hipMemcpy(B.h_sc, B.d_sc, (size_t)nm * B_N * sizeof(double), hipMemcpyDeviceToHost));
+    return batch_error(B, bad, nullptr);
   }
   return SFEM_OK;
 }
@@ -615,12 +842,15 @@ int sfem_set_batch(sfem_ctx* c, int32_t nmesh, const int32_t* elem_mesh, const d
   HIP_OK(hipMemcpy(c->d_elem_mu, emu.data(), c->nElem * sizeof(double), hipMemcpyHostToDevice));
   if (balloc(&B->d_dof_ptr, ptr.size()) != hipSuccess || balloc(&B->d_sc, (size_t)nmesh * B_N) != hipSuccess ||
       balloc(&B->d_norm, nmesh) != hipSuccess || balloc(&B->d_active, nmesh) != hipSuccess ||
-      balloc(&B->d_ndone, 1) != hipSuccess)
+      balloc(&B->d_ndone, 1) != hipSuccess || balloc(&B->d_newton, nmesh) != hipSuccess ||
+      balloc(&B->d_kry, nmesh) != hipSuccess || balloc(&B->d_rhsn, nmesh) != hipSuccess ||
+      balloc(&B->d_status, ST_N) != hipSuccess)
     return set_error(SFEM_ERR_HIP, "hipMalloc batch state");
   HIP_OK(hipMemcpy(B->d_dof_ptr, ptr.data(), ptr.size() * sizeof(int64_t), hipMemcpyHostToDevice));
   HIP_OK(hipHostMalloc((void**)&B->h_norm, nmesh * sizeof(double), hipHostMallocDefault));
   HIP_OK(hipHostMalloc((void**)&B->h_ndone, sizeof(int32_t), hipHostMallocDefault));
   HIP_OK(hipHostMalloc((void**)&B->h_sc, (size_t)nmesh * B_N * sizeof(double), hipHostMallocDefault));
+  HIP_OK(hipHostMalloc((void**)&B->h_status, ST_N * sizeof(int32_t), hipHostMallocDefault));
   return SFEM_OK;
 }
 
@@ -642,50 +872,12 @@ int sfem_step_batch(sfem_ctx* c, double timeFact, const sfem_solver_opts* opts, 
   if (opts->pc_type != SFEM_PC_ILU0 || opts->ksp_type != SFEM_KSP_BCGS)
     return set_error(SFEM_ERR_UNSUPPORTED, "batched steps use BiCGStab + ILU(0)");
   if (opts->atol != 0.0) return set_error(SFEM_ERR_UNSUPPORTED, "batched steps use the relative test only (atol = 0)");
-  Batch& B = *c->batch;
-  const int nm = B.nmesh;
   HIP_OK(hipSetDevice(c->device));
   int rc;
   if ((rc = sfem_time_update(c, timeFact))) return rc;
-  std::vector<int32_t> active(nm, 1), its(nm, 0);
-  for (int m = 0; m < nm; ++m) { newton_iters[m] = max_newton; krylov_iters[m] = 0; rhs_norm[m] = 0.0; }
-  for (int iter = 0; iter < max_newton; ++iter) {
-    double gnorm;
-    if ((rc = sfem_assemble(c, iter, iter == 0, &gnorm))) return rc;
-    hipLaunchKernelGGL(k_seg_norm2, dim3(nm), dim3(kTB), 0, c->stream, B.d_dof_ptr, c->d_rhs, B.d_norm);
-    HIP_OK(hipGetLastError());
-    HIP_OK(hipMemcpyAsync(B.h_norm, B.d_norm, nm * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIP_OK(hipStreamSynchronize(c->stream));
-    int nact = 0;
-    for (int m = 0; m < nm; ++m) {
-      if (!active[m]) continue;
-      rhs_norm[m] = B.h_norm[m];
-      if (B.h_norm[m] < newton_tol) {  // StabFEM2.cpp:749, per mesh
-        active[m] = 0;
-        newton_iters[m] = iter + 1;
-      } else {
-        ++nact;
-      }
-    }
-    if (nact == 0) break;
-    HIP_OK(hipMemcpyAsync(B.d_active, active.data(), nm * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
-    int bad = -1;
-    rc = c->precision == SFEM_PREC_MIXED ? batch_solve<float>(c, opts, its.data(), &bad)
-                                         : batch_solve<double>(c, opts, its.data(), &bad);
-    if (rc) return rc;
-    for (int m = 0; m < nm; ++m)
-      if (active[m]) krylov_iters[m] += its[m];
-    if (bad >= 0) {
-      const double* S = B.h_sc + (size_t)bad * B_N;
-      char msg[200];
-      snprintf(msg, sizeof msg, "batched BiCGStab: mesh %d %s after %d iterations (|r|^2 = %.3e, target %.3e)", bad,
-               S[B_BAD] != 0.0 ? "broke down (non-finite residual)" : "did not converge", its[bad], S[B_RR],
-               S[B_TOL2]);
-      return set_error(SFEM_ERR_MAXIT, msg);
-    }
-    if ((rc = sfem_update_solution(c))) return rc;
-  }
-  return SFEM_OK;
+  return c->precision == SFEM_PREC_MIXED
+             ? step_batch_graph<float>(c, opts, newton_tol, max_newton, newton_iters, krylov_iters, rhs_norm)
+             : step_batch_graph<double>(c, opts, newton_tol, max_newton, newton_iters, krylov_iters, rhs_norm);
 }
 
 }  // extern "C"

@@ -454,18 +454,26 @@ __device__ inline void add_row_segment(T* __restrict__ row, const T (&k)[NDOF], 
 // Generic element loop (all element types; the Hexa8 colour loops use the pipelined
 // kernel below and this one serves the per-element probe): TEAMS elements per 256-lane
 // workgroup, one per team.
-template <int ET, int MODE>
-__global__ __launch_bounds__(256) void k_assemble(AsmView v, ElemParams p,
+// WGS lanes per workgroup (WGS / TEAM elements side by side); PERSIST: the grid is what
+// the GPU holds at once and each workgroup walks the slots blockIdx.x, blockIdx.x +
+// gridDim.x, ... (in groups of WGS / TEAM), instead of one group per workgroup.  Every
+// variant does the same arithmetic per element, and no two elements of a colour class
+// add to the same entry, so all variants give the same result.
+template <int ET, int MODE, int WGS = 256, bool PERSIST = false>
+__global__ __launch_bounds__(WGS) void k_assemble(AsmView v, ElemParams p0,
                                                   const int32_t* __restrict__ elems, int64_t n,
                                                   int lift, int64_t probe_e) {
   using C = Traits<ET>;
-  constexpr int TEAMS = 256 / C::TEAM;
+  constexpr int TEAMS = WGS / C::TEAM;
   constexpr int D = C::NDIM, NPE = C::NPE, NGP = C::NGP, NDOF = C::NDOF;
   __shared__ TeamLds<ET> lds[TEAMS];
   const int team = threadIdx.x / C::TEAM;
   const int l = threadIdx.x % C::TEAM;
   TeamLds<ET>& L = lds[team];
-  const int64_t slot = (int64_t)blockIdx.x * TEAMS + team;
+  const int64_t limit = probe_e >= 0 ? 1 : n;
+  for (int64_t grp = blockIdx.x; grp * TEAMS < limit; grp += gridDim.x) {
+  ElemParams p = p0;
+  const int64_t slot = grp * TEAMS + team;
   const bool probe = probe_e >= 0;
   const bool active = probe ? (slot == 0) : (slot < n);
   const int64_t e = active ? (probe ? probe_e : (int64_t)elems[slot]) : 0;
@@ -582,6 +590,9 @@ __global__ __launch_bounds__(256) void k_assemble(AsmView v, ElemParams p,
         }
       }
     }
+  }
+  if constexpr (!PERSIST) break;  // the grid covers every group
+  __syncthreads();                // the team's LDS is reused by the next group
   }
 }
 
@@ -832,23 +843,53 @@ hipError_t launch_prepare(int et, hipStream_t s, int64_t nElem, const double* co
   }
 }
 
-template <int ET>
-static hipError_t assemble_t(hipStream_t s, const AsmView& v, const ElemParams& p,
-                             const int32_t* elems, int64_t n, int mode, int lift, int64_t probe_e) {
-  constexpr int TEAMS = 256 / Traits<ET>::TEAM;
+// One launch of variant (WGS, PERSIST) of the generic element loop.
+template <int ET, int MODE, int WGS, bool PERSIST>
+static hipError_t assemble_v(hipStream_t s, const AsmView& v, const ElemParams& p, const int32_t* elems, int64_t n,
+                             int lift, int64_t probe_e) {
+  constexpr int TEAMS = WGS / Traits<ET>::TEAM;
   const int64_t cnt = probe_e >= 0 ? 1 : n;
   if (cnt == 0) return hipSuccess;
-  const int64_t grid = (cnt + TEAMS - 1) / TEAMS;
-  if (mode == 0)
-    hipLaunchKernelGGL((k_assemble<ET, 0>), dim3((unsigned)grid), dim3(256), 0, s, v, p, elems, n,
-                       lift, probe_e);
-  else if (mode == 1)
-    hipLaunchKernelGGL((k_assemble<ET, 1>), dim3((unsigned)grid), dim3(256), 0, s, v, p, elems, n,
-                       lift, probe_e);
-  else
-    hipLaunchKernelGGL((k_assemble<ET, 2>), dim3((unsigned)grid), dim3(256), 0, s, v, p, elems, n,
-                       lift, probe_e);
+  int64_t grid = (cnt + TEAMS - 1) / TEAMS;
+  if (PERSIST) {
+    static int64_t resident[16];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) dev = 0;
+    if (resident[dev] == 0) {
+      int cus = 0, per = 0;
+      (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_assemble<ET, MODE, WGS, PERSIST>, WGS, 0) !=
+              hipSuccess || per < 1)
+        per = 1;
+      resident[dev] = (int64_t)(cus > 0 ? cus : 256) * per;
+    }
+    grid = std::min(grid, resident[dev]);
+  }
+  hipLaunchKernelGGL((k_assemble<ET, MODE, WGS, PERSIST>), dim3((unsigned)grid), dim3(WGS), 0, s, v, p, elems, n,
+                     lift, probe_e);
   return hipGetLastError();
+}
+
+template <int ET, int MODE>
+static hipError_t assemble_m(hipStream_t s, const AsmView& v, const ElemParams& p, const int32_t* elems, int64_t n,
+                             int lift, int64_t probe_e, int variant) {
+  switch (variant) {
+    case 1: return assemble_v<ET, MODE, 128, false>(s, v, p, elems, n, lift, probe_e);
+    case 2: return assemble_v<ET, MODE, 64, false>(s, v, p, elems, n, lift, probe_e);
+    case 3: return assemble_v<ET, MODE, 256, true>(s, v, p, elems, n, lift, probe_e);
+    case 4: return assemble_v<ET, MODE, 128, true>(s, v, p, elems, n, lift, probe_e);
+    case 5: return assemble_v<ET, MODE, 64, true>(s, v, p, elems, n, lift, probe_e);
+    default: return assemble_v<ET, MODE, 256, false>(s, v, p, elems, n, lift, probe_e);
+  }
+}
+
+template <int ET>
+static hipError_t assemble_t(hipStream_t s, const AsmView& v, const ElemParams& p,
+                             const int32_t* elems, int64_t n, int mode, int lift, int64_t probe_e, int variant) {
+  if (probe_e >= 0) variant = 0;
+  if (mode == 0) return assemble_m<ET, 0>(s, v, p, elems, n, lift, probe_e, variant);
+  if (mode == 1) return assemble_m<ET, 1>(s, v, p, elems, n, lift, probe_e, variant);
+  return assemble_m<ET, 2>(s, v, p, elems, n, lift, probe_e, variant);
 }
 
 // Workgroups of the pipelined Hexa8 kernel the current device holds at once (each walks
@@ -872,7 +913,7 @@ static hipError_t hex8_launch(hipStream_t s, const AsmView& v, const ElemParams&
 }
 
 hipError_t launch_assemble(int et, hipStream_t s, const AsmView& v, const ElemParams& p,
-                           const int32_t* elems, int64_t n, int mode, int lift, int64_t probe_e) {
+                           const int32_t* elems, int64_t n, int mode, int lift, int64_t probe_e, int variant) {
   if (et == HEXA8 && probe_e < 0) {
     if (n == 0) return hipSuccess;
     if (mode == 0) return hex8_launch<0>(s, v, p, elems, n, lift);
@@ -880,10 +921,10 @@ hipError_t launch_assemble(int et, hipStream_t s, const AsmView& v, const ElemPa
     return hex8_launch<2>(s, v, p, elems, n, lift);
   }
   switch (et) {
-    case TRIA3: return assemble_t<TRIA3>(s, v, p, elems, n, mode, lift, probe_e);
-    case QUAD4: return assemble_t<QUAD4>(s, v, p, elems, n, mode, lift, probe_e);
-    case TETRA4: return assemble_t<TETRA4>(s, v, p, elems, n, mode, lift, probe_e);
-    default: return assemble_t<HEXA8>(s, v, p, elems, n, mode, lift, probe_e);
+    case TRIA3: return assemble_t<TRIA3>(s, v, p, elems, n, mode, lift, probe_e, variant);
+    case QUAD4: return assemble_t<QUAD4>(s, v, p, elems, n, mode, lift, probe_e, variant);
+    case TETRA4: return assemble_t<TETRA4>(s, v, p, elems, n, mode, lift, probe_e, variant);
+    default: return assemble_t<HEXA8>(s, v, p, elems, n, mode, lift, probe_e, 0);
   }
 }
 

@@ -88,8 +88,16 @@ hipError_t launch_prepare(int et, hipStream_t s, int64_t nElem, const double* co
 // mode: 0 = K64 + F64, 1 = F64 only, 2 = K32 + F64.  lift: apply the iteration-0
 // Dirichlet lift.  elems: element ids (a colour class) of length n, or NULL with
 // probe_e >= 0 for the single-element probe.
+// variant (element types other than Hexa8): 0 = 256 lanes per workgroup, 1 = 128, 2 = 64,
+// one group of elements per workgroup; 3-5 = the same sizes with workgroups that walk
+// their colour class (a grid of what the GPU holds at once)
+constexpr int kAsmVariants = 6;
+// residual-only assembly of Newton iterations after the first as kernel launches only
+// (capturable); asm_prepare_residual must have run first (sfem_api.cpp)
+int asm_prepare_residual(sfem_ctx* c);
+int asm_residual_kernels(sfem_ctx* c);
 hipError_t launch_assemble(int et, hipStream_t s, const AsmView& v, const ElemParams& p,
-                           const int32_t* elems, int64_t n, int mode, int lift, int64_t probe_e);
+                           const int32_t* elems, int64_t n, int mode, int lift, int64_t probe_e, int variant = 0);
 
 // Launchers implemented in sfem_linalg.hip (see there for semantics)
 struct LinAlg;
@@ -215,6 +223,7 @@ struct sfem_ctx {
   float* d_kb32 = nullptr;
   int64_t csr_version = -1;
   bool kb_fp32 = false;                 // the last tangent went to d_kb32 (mixed precision)
+  int asm_variant[3] = {-1, -1, -1};    // element-loop variant per assembly mode (-1: not chosen)
   double* d_rhs = nullptr;
   double* d_du = nullptr;
   // SolutionData vectors (nNode*ndof each)

@@ -93,6 +93,7 @@ ABI = [
     ("sfem_tile_info", C.c_int, [_vp, _i64p, _i64p, _i64p]),
     ("sfem_tuning", C.c_int, [_vp, C.POINTER(C.c_int)]),
     ("sfem_tuning_levels", C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]),
+    ("sfem_element_tuning", C.c_int, [_vp, C.POINTER(C.c_int)]),
     ("sfem_set_tuning_mode", C.c_int, [_vp, C.c_int]),
     ("sfem_get_tuning_table", C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]),
     ("sfem_set_tuning_table", C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.c_int]),
@@ -383,6 +384,14 @@ class Context:
         z = np.zeros(max(self.nfree, 1))
         _check(self.lib.sfem_pc_apply(self.h, _p(r, C.c_double), _p(z, C.c_double)), "sfem_pc_apply")
         return z[:self.nfree]
+
+    def element_tuning(self):
+        """Element-loop variant per assembly mode {"K64": .., "F": .., "K32": ..} (-1: not
+        chosen yet): 0-2 = 256/128/64 lanes per workgroup, 3-5 = the same with workgroups
+        walking their colour class."""
+        out = (C.c_int * 3)()
+        _check(self.lib.sfem_element_tuning(self.h, out), "sfem_element_tuning")
+        return dict(zip(("K64", "F", "K32"), list(out)))
 
     def tuning(self):
         """Autotuned kernel shapes per precision: SpMV (G, U, XCD order) and the

@@ -359,6 +359,17 @@ int sfem_tuning(sfem_ctx* ctx, int* out);
  * counterpart in the reference. */
 int sfem_tuning_levels(sfem_ctx* ctx, int fp32, int* out, int cap, int* nlevels);
 
+/* The element loop's variant per assembly mode (out[0] tangent + residual FP64, out[1]
+ * residual only, out[2] FP32 tangent + FP64 residual), -1 where not chosen yet.  For
+ * Tria3, Quad4 and Tetra4 the first assembly of a mode under SFEM_TUNE_TIMED times six
+ * variants over the whole colour loop and keeps the fastest: 256, 128 or 64 lanes per
+ * workgroup (16, 8 or 4 elements side by side), each with one group of elements per
+ * workgroup (0-2) or with workgroups that walk their colour class (3-5).  Under
+ * SFEM_TUNE_FIXED, variant 0.  All variants give the same result (no two elements of a
+ * colour class add to the same entry).  The Hexa8 loop has one form (0).  No
+ * counterpart in the reference (PETSc assembly is serial per rank). */
+int sfem_element_tuning(sfem_ctx* ctx, int* out);
+
 /* How the solver picks the kernel shapes of its SpMV and ILU(0) sweeps (lanes per block
  * row, blocks in flight per lane, workgroup order).  SFEM_TUNE_TIMED (the default): on
  * the first solve of a precision and ILU(0) plan every shape is timed with HIP events on

@@ -455,6 +455,40 @@ def test_ilu_factor_vs_oracle(reorder, prec):
     ctx.close()
 
 
+@pytest.mark.parametrize("mesh_kind", ["tria", "quad", "tet"])
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_element_loop_variants_agree(mesh_kind, prec):
+    """The element-loop variant the timed autotuner picks (lanes per workgroup, walking
+    workgroups) assembles the same system, bit for bit, as the fixed variant 0: the
+    variants differ only in how elements are laid on workgroups."""
+    if mesh_kind == "quad":
+        mesh = meshgen.ldc2d_quad(40)
+    elif mesh_kind == "tria":
+        mesh = meshgen.ldc2d_tria(40, jitter=0.2, seed=3)
+    else:
+        mesh = meshgen.ldc3d_tet(8)
+    out = []
+    for mode in (sfem.TUNE_FIXED, sfem.TUNE_TIMED):
+        ctx = make_ctx(mesh, tis=2, dt=0.1)
+        ctx.set_tuning_mode(mode)
+        if prec == "mixed":
+            ctx.set_precision(sfem.PREC_MIXED)
+        ctx.time_update(1.0)
+        ctx.assemble(0)
+        ctx.assemble(1, want_K=False)
+        vals, rhs = ctx.get_system()
+        ctx.assemble(1, want_K=True)
+        vals2, _ = ctx.get_system()
+        tun = ctx.element_tuning()
+        out.append((vals, rhs, vals2, tun))
+        ctx.close()
+    print(f"[case] element variants {mesh_kind} {prec}: fixed {out[0][3]} timed {out[1][3]}", flush=True)
+    assert all(v == 0 for v in out[0][3].values() if v >= 0)
+    assert all(0 <= v < 6 for v in out[1][3].values() if v >= 0)
+    for a, b in zip(out[0][:3], out[1][:3]):
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("mesh_kind,prec,reorder", [("hex", "fp64", 0), ("hex", "fp64", 3), ("hex", "mixed", 3),
                                                    ("hex", "fp64", 1), ("tet", "fp64", 0), ("quad", "fp64", 2),
                                                    ("tria", "mixed", 0)])
