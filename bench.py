@@ -253,7 +253,8 @@ def run_batch(case, prec, opts, newton_tol, steps, warmup, label, graph=True):
     kt = {name: ctx.kernel_timing(k) for name, k in
           (("assembly", sfem.KT_ASSEMBLY), ("spmv", sfem.KT_SPMV), ("pc", sfem.KT_PC))}
     res = dict(label=label, seconds=t2 - t1, steps_per_s=steps / (t2 - t1), hist=hist, kt=kt,
-               info=info, soln=s, block=ctx.block_info(), tune=ctx.tuning(), stats=ctx.solver_stats())
+               info=info, soln=s, block=ctx.block_info(), tune=ctx.tuning(), stats=ctx.solver_stats(),
+               element_variant=ctx.element_tuning())
     ctx.close()
     return res
 
@@ -404,6 +405,7 @@ def main_c4(a):
         "krylov_iters_max": [int(h[1].max()) for h in r64["hist"]],
         "krylov_iters_mean": [round(float(h[1].mean()), 1) for h in r64["hist"]],
         "newton_iters_max": [int(h[0].max()) for h in r64["hist"]],
+        "element_loop_variant": {r["label"]: r["element_variant"] for r in (r64, r32, rmx) if r},
         "kernel_ms_full_step": {r["label"]: {k: {"launches": v[0], "total_ms": v[1]} for k, v in r["kt"].items()}
                                 for r in (r64, r32, rmx) if r},
         "roofline": roof,
