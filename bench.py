@@ -592,7 +592,7 @@ def main():
     ap.add_argument("--c4-store32-rtol", type=float, default=1e-4, help="c4: inner rtol of the FP32-stored run")
     ap.add_argument("--c4-store32-newton", type=float, default=1e-8, help="c4: Newton tol of the FP32-stored run")
     ap.add_argument("--n", type=int, default=0, help="override the mesh size n (n^3 hexes)")
-    ap.add_argument("--inner-rtol", type=float, default=1e-5,
+    ap.add_argument("--inner-rtol", type=float, default=4e-5,
                     help="mixed path: the deepest reduction asked of one FP32 inner solve (with --inner-theta, "
                          "a solve is asked for no more than the Newton test still needs)")
     ap.add_argument("--newton-tol", type=float, default=1e-8,
@@ -609,7 +609,7 @@ def main():
                     help="mixed path: ILU(0) factors and tile inverses stored in bfloat16 (0: FP32)")
     ap.add_argument("--predictor", choices=["previous", "extrapolate"], default="previous",
                     help="Newton initial iterate of each step (both paths)")
-    ap.add_argument("--inner-theta", type=float, default=0.5,
+    ap.add_argument("--inner-theta", type=float, default=0.8,
                     help="mixed path: inexact Newton, inner rtol = min(inner-rtol-max, max(inner-rtol, "
                          "theta * newton_tol / ||F||)) per Newton iteration (0: inner-rtol throughout)")
     ap.add_argument("--inner-rtol-max", type=float, default=0.5, help="mixed path: cap of the inexact-Newton rtol")
