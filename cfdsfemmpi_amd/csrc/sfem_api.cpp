@@ -657,8 +657,10 @@ static int judge_learned_start(sfem_ctx* c, double timeFact) {
   return SFEM_OK;
 }
 
-int sfem_time_update(sfem_ctx* c, double timeFact) {
+int sfem_time_update(sfem_ctx* c, double timeNow) {
   if (!c) return set_error(SFEM_ERR_ARG, "null context");
+  // the lid ramp of StabFEM2.cpp:636-644 (a ramp factor <= 1 passes through unchanged)
+  const double timeFact = timeNow <= 1.0 ? timeNow : 1.0;
   if (!(c->status & ST_PARAMS)) return set_error(SFEM_ERR_STATE, "parameters not set");
   HIP_OK(hipSetDevice(c->device));
   time_params(c->tis, c->rhoInf, c->dt, c->td);  // setTimeParam each step (StabFEM2.cpp:632)
@@ -892,12 +894,12 @@ int sfem_update_solution(sfem_ctx* c) {
   return SFEM_OK;
 }
 
-int sfem_step(sfem_ctx* c, double timeFact, const sfem_solver_opts* o, double newton_tol, int max_iter,
+int sfem_step(sfem_ctx* c, double timeNow, const sfem_solver_opts* o, double newton_tol, int max_iter,
               sfem_step_stats* stats) {
   if (!c || !o) return set_error(SFEM_ERR_ARG, "null argument");
   sfem_step_stats st;
   std::memset(&st, 0, sizeof st);
-  int rc = sfem_time_update(c, timeFact);
+  int rc = sfem_time_update(c, timeNow);
   if (rc) return rc;
   bool conv = false;
   for (int iter = 0; iter < max_iter; ++iter) {

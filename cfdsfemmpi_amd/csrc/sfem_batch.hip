@@ -866,7 +866,7 @@ int sfem_batch_narrowing(sfem_ctx* c, int enable) {
   return SFEM_OK;
 }
 
-int sfem_step_batch(sfem_ctx* c, double timeFact, const sfem_solver_opts* opts, double newton_tol, int max_newton,
+int sfem_step_batch(sfem_ctx* c, double timeNow, const sfem_solver_opts* opts, double newton_tol, int max_newton,
                     int32_t* newton_iters, int32_t* krylov_iters, double* rhs_norm) {
   if (!c || !opts || !c->batch) return set_error(SFEM_ERR_ARG, "no batch set or null argument");
   if (opts->pc_type != SFEM_PC_ILU0 || opts->ksp_type != SFEM_KSP_BCGS)
@@ -874,7 +874,7 @@ int sfem_step_batch(sfem_ctx* c, double timeFact, const sfem_solver_opts* opts, 
   if (opts->atol != 0.0) return set_error(SFEM_ERR_UNSUPPORTED, "batched steps use the relative test only (atol = 0)");
   HIP_OK(hipSetDevice(c->device));
   int rc;
-  if ((rc = sfem_time_update(c, timeFact))) return rc;
+  if ((rc = sfem_time_update(c, timeNow))) return rc;
   return c->precision == SFEM_PREC_MIXED
              ? step_batch_graph<float>(c, opts, newton_tol, max_newton, newton_iters, krylov_iters, rhs_norm)
              : step_batch_graph<double>(c, opts, newton_tol, max_newton, newton_iters, krylov_iters, rhs_norm);

@@ -235,8 +235,10 @@ class Context:
         return cnt, nrm
 
     # -- hot path ----------------------------------------------------------
-    def time_update(self, timeFact: float):
-        _check(self.lib.sfem_time_update(self.h, float(timeFact)), "sfem_time_update")
+    def time_update(self, timeNow: float):
+        """Start a time step at the accumulated time timeNow (the library applies the lid
+        ramp min(timeNow, 1))."""
+        _check(self.lib.sfem_time_update(self.h, float(timeNow)), "sfem_time_update")
 
     def assemble(self, it: int, want_K: bool = True) -> float:
         nrm = C.c_double(0.0)
@@ -270,10 +272,10 @@ class Context:
     def update_solution(self):
         _check(self.lib.sfem_update_solution(self.h), "sfem_update_solution")
 
-    def step(self, timeFact: float, opts: SolverOpts, newton_tol: float = 1e-8, max_iter: int = 10,
+    def step(self, timeNow: float, opts: SolverOpts, newton_tol: float = 1e-8, max_iter: int = 10,
              allow_notconv: bool = True) -> StepStats:
         st = StepStats()
-        rc = self.lib.sfem_step(self.h, float(timeFact), C.byref(opts), float(newton_tol), int(max_iter),
+        rc = self.lib.sfem_step(self.h, float(timeNow), C.byref(opts), float(newton_tol), int(max_iter),
                                 C.byref(st))
         if rc == -8 and allow_notconv:  # the reference moves on after 10 iterations
             return st
@@ -336,12 +338,12 @@ class Context:
     def batch_narrowing(self, enable: bool):
         _check(self.lib.sfem_batch_narrowing(self.h, 1 if enable else 0), "sfem_batch_narrowing")
 
-    def step_batch(self, timeFact: float, opts: SolverOpts, newton_tol: float = 1e-8, max_newton: int = 10):
+    def step_batch(self, timeNow: float, opts: SolverOpts, newton_tol: float = 1e-8, max_newton: int = 10):
         """One time step of every mesh: (newton_iters, krylov_iters, rhs_norm) per mesh."""
         ni = np.zeros(self.nmesh, np.int32)
         ki = np.zeros(self.nmesh, np.int32)
         rn = np.zeros(self.nmesh)
-        _check(self.lib.sfem_step_batch(self.h, float(timeFact), C.byref(opts), float(newton_tol), int(max_newton),
+        _check(self.lib.sfem_step_batch(self.h, float(timeNow), C.byref(opts), float(newton_tol), int(max_newton),
                                         _p(ni, C.c_int32), _p(ki, C.c_int32), _p(rn, C.c_double)), "sfem_step_batch")
         return ni, ki, rn
 
@@ -583,8 +585,7 @@ class StabFEM:
         stepsMax = c.stepsMax if steps is None else steps
         timeNow = c.dt
         for tstep in range(stepsMax):
-            timeFact = timeNow if timeNow <= 1.0 else 1.0
-            ctx.time_update(timeFact)
+            ctx.time_update(timeNow)  # the lid ramp timeFact = min(timeNow, 1) is applied inside
             for it in range(self.max_newton):
                 nrm = ctx.assemble(it, want_K=(it == 0))
                 if nrm < self.newton_tol:

@@ -209,8 +209,10 @@ int sfem_set_inner_tolerance(sfem_ctx* ctx, double theta, double rtol_max);
 
 /* Start a time step (StabFEM2.cpp:632-648): td[] from SetTimeParametersFluid,
  * SolutionData::timeUpdate (SolutionData.cpp:64-78), and
- * assignBoundaryConditions with the given ramp factor (StabFEM.cpp:391-410). */
-int sfem_time_update(sfem_ctx* ctx, double timeFact);
+ * assignBoundaryConditions (StabFEM.cpp:391-410) with the lid ramp
+ * timeFact = timeNow <= 1 ? timeNow : 1 (StabFEM2.cpp:636-644); timeNow is the
+ * caller's accumulated time (timeNow += dt, as the reference accumulates it). */
+int sfem_time_update(sfem_ctx* ctx, double timeNow);
 
 /* One element loop of the Newton iteration `iter` (StabFEM2.cpp:656-740):
  * updateIterStep, zeroMtx, calcStiffnessAndResidual for every element, the
@@ -247,7 +249,7 @@ int sfem_update_solution(sfem_ctx* ctx);
  * StabFEM2.cpp:654-799 (at most max_iter iterations, stop when
  * ||rhs_free||_2 < newton_tol before solving).  Returns SFEM_ERR_NOTCONV if
  * the cap is hit (stats are still filled). */
-int sfem_step(sfem_ctx* ctx, double timeFact, const sfem_solver_opts* opts, double newton_tol,
+int sfem_step(sfem_ctx* ctx, double timeNow, const sfem_solver_opts* opts, double newton_tol,
               int max_iter, sfem_step_stats* stats);
 
 /* Solution vectors (node-major, nNode*ndof): soln, and the solver increment du[nfree]. */
@@ -312,7 +314,7 @@ int sfem_set_batch(sfem_ctx* ctx, int32_t nmesh, const int32_t* elem_mesh, const
  * solve stalls (no 10% residual gain in 100 iterations), diverges or reaches max_it
  * hands its best iterate to the next FP64 Newton iteration (counted by
  * sfem_solver_stats); in FP64 such a mesh fails the step with SFEM_ERR_MAXIT. */
-int sfem_step_batch(sfem_ctx* ctx, double timeFact, const sfem_solver_opts* opts, double newton_tol,
+int sfem_step_batch(sfem_ctx* ctx, double timeNow, const sfem_solver_opts* opts, double newton_tol,
                     int max_newton, int32_t* newton_iters, int32_t* krylov_iters, double* rhs_norm);
 
 /* Replay the batched BiCGStab iteration as a HIP graph (1, default) or launch it kernel

@@ -140,6 +140,33 @@ def test_assembled_system_parity(name):
     ctx.close()
 
 
+@pytest.mark.parametrize("timeNow", [0.3, 1.0, 2.5])
+def test_time_update_applies_lid_ramp(timeNow):
+    """sfem_time_update takes the accumulated time and scales the Dirichlet values by
+    timeFact = min(timeNow, 1) (StabFEM2.cpp:636-644): past t = 1 the lid stays at full
+    speed."""
+    mesh = small_mesh("tria3")
+    print(f"[case] lid ramp timeNow={timeNow}", flush=True)
+    tis, dt = 2, 0.1
+    n = mesh.nNode * (mesh.ndim + 1)
+    rng = np.random.default_rng(7)
+    S0, P0, D0 = (rng.uniform(-1, 1, n) for _ in range(3))
+    f = _oracle_fem(mesh, tis, dt)
+    f.soln[:] = S0; f.solnPrev[:] = P0; f.solnDot[:] = D0
+    f.td = oracle.time_params(tis, 0.0, dt)
+    f.time_update()
+    f.assign_boundary_conditions(min(timeNow, 1.0))
+    f.update_iter_step()
+    f.assemble(0)
+    ctx = make_ctx(mesh, tis=tis, dt=dt)
+    ctx.set_state(soln=S0, solnPrev=P0, solnDot=D0)
+    ctx.time_update(timeNow)
+    ctx.assemble(0)
+    _, rhs = ctx.get_system()
+    assert rel_err(rhs, f.rhs) <= 1e-12
+    ctx.close()
+
+
 def test_ldc2dquad10_reference(golden):
     """src/test-ldc-stru-nelem10.cpp: run the deck, check at 1e-4 per node (StabFEM.cpp:522-539)."""
     d = os.path.join(golden, "ldc2dquad10")
