@@ -519,7 +519,7 @@ int sfem_build_pattern(sfem_ctx* c, int64_t* nfree_out, int64_t* nnz_out) {
   HIP_OK(hipMemcpy(c->d_colofs, scolofs.data(), scolofs.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   if (c->kb_on) {
     if ((rc = dalloc(&c->d_blkofs, sblkofs.size())) || (rc = dalloc(&c->d_natofs, natofs.size())) ||
-        (rc = dalloc(&c->d_blkPtr, c->blkPtr.size())) || (rc = dalloc(&c->d_kb, c->nbpairs * 16)))
+        (rc = dalloc(&c->d_blkPtr, c->blkPtr.size())) || (rc = dalloc(&c->d_kb, std::max<int64_t>(c->nbpairs, 1) * 16)))
       return rc;
     HIP_OK(hipMemcpy(c->d_blkofs, sblkofs.data(), sblkofs.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(c->d_natofs, natofs.data(), natofs.size() * sizeof(int32_t), hipMemcpyHostToDevice));
@@ -763,7 +763,7 @@ int sfem_assemble(sfem_ctx* c, int iter, int want_K, double* rhs_norm) {
     if (rc) return rc;
   }
   if (mixed && c->kb_on && !c->d_kb32) {
-    int rc = dalloc(&c->d_kb32, c->nbpairs * 16);
+    int rc = dalloc(&c->d_kb32, std::max<int64_t>(c->nbpairs, 1) * 16);
     if (rc) return rc;
   }
   const int64_t nd = (int64_t)c->dbc_ind.size();

@@ -29,6 +29,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
 #include <type_traits>
 
 #include "sfem_internal.hpp"
@@ -823,6 +825,514 @@ __global__ __launch_bounds__(64, HexCfg<MODE>::occ) void k_assemble_hex8(AsmView
   }
 }
 
+// ---------------------------------------------------------------------------
+// Batched Hexa8 element loop: four elements of a colour class per wavefront iteration.
+//
+// Phase G, lane (e, g, h) = (l / 16, (l / 2) % 8, l % 2): Gauss point g of element e.
+// Both lanes of a pair form the Jacobian at g; the shape gradients of the eight nodes
+// go to LDS; both lanes interpolate the state (31 sums over the nodes, in node order)
+// and form the stabilisation parameter; lane h then forms the residual of nodes h, h + 2,
+// h + 4, h + 6 at g.  The residual is summed over Gauss points through LDS by lane
+// (e, k, c) = (l / 16, (l / 2) % 8, l % 2), which ends with components c and c + 2 of
+// node k.  For the tangent, the per-(Gauss point) record and, for the FP32 tangent, the
+// per-(Gauss point, node) records follow.
+// Phase K, lane (i, j) = (l / 8, l % 8): block (i, j) of each of the four elements in
+// turn, summed over the Gauss points from the records, added to the node-pair block form
+// (the element whose map entry carries bit 30 stores the block instead).
+// The next batch's connectivity, block offsets and element ids are loaded while phase G
+// runs, its nodal data while the residual is summed and phase K runs.
+// ---------------------------------------------------------------------------
+constexpr int kHexEB = 4;  // elements per batch
+
+template <int MODE>
+struct HexBCfg {
+  using T = typename std::conditional<MODE == 2, float, double>::type;  // tangent type
+  static constexpr bool KM = MODE != 1;
+  // per (element, Gauss point): w, s tauP, A (3 x 3, A = af rho gradPrev), mu af, and
+  // for the FP64 tangent af rho uPrev
+  static constexpr int CW = MODE == 2 ? 12 : 16;
+  static constexpr int occ = 2;
+};
+
+template <int MODE>
+struct HexBLds {
+  using Cfg = HexBCfg<MODE>;
+  using T = typename Cfg::T;
+  union {
+    struct {
+      double X[kHexEB][8][4];     // coordinates
+      double S[3][kHexEB][8][4];  // solnPrev, solnCur, solnDotCur
+    } nd;
+    double R[kHexEB][8][8][4];    // residual (element, Gauss point, node), components 0, 2, 1, 3
+    struct {
+      // FP32 tangent, per (element, Gauss point, node): dN/dx, dN/dy, dN/dz, N,
+      // g(0..2) = s tauP sum_a w dN_a A_a., u = af rho uPrev.dN
+      float rec[MODE == 2 ? kHexEB : 1][8][8][8];
+      T cg[Cfg::KM ? kHexEB : 1][8][Cfg::CW];
+      double lift[MODE == 0 ? kHexEB : 1][8][4];  // lift sums of node i of each element
+    } k;
+  } a;
+  // per (element, Gauss point, node): dN/dx, dN/dy, dN/dz, N (the FP64 tangent's record)
+  double gd[kHexEB][8][8][4];
+  double app[MODE == 0 ? kHexEB : 1][8][4];  // prescribed increments (iteration-0 lift)
+  int32_t info[kHexEB][8][2];                 // free-DOF mask, first free row
+};
+
+// Adds lane (i, j)'s block K to block form entry co (free rows and columns packed to the
+// front); old = its previous values (zero when this element stores the block).
+template <typename T>
+__device__ inline void hex_block_store(T* __restrict__ kbv, int32_t co, int mi, int mj, const T (&K)[4][4],
+                                       const T (&old)[4][4]) {
+  typedef T blk_t __attribute__((ext_vector_type(4)));
+  blk_t* bq = reinterpret_cast<blk_t*>(kbv + (int64_t)(co & ((1 << 30) - 1)) * 16);
+  if (mi == 15 && mj == 15) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+      bq[a] = blk_t{old[a][0] + K[a][0], old[a][1] + K[a][1], old[a][2] + K[a][2], old[a][3] + K[a][3]};
+    return;
+  }
+  T nv[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) nv[a][b] = old[a][b];
+  int ra = 0;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    if (!((mi >> a) & 1)) continue;
+    int rb = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      if (!((mj >> b) & 1)) continue;
+      // rows and columns are only ever packed towards the front: ra <= a, rb <= b
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+          if (x == ra && y == rb) nv[x][y] += K[a][b];
+      ++rb;
+    }
+    ++ra;
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a) bq[a] = blk_t{nv[a][0], nv[a][1], nv[a][2], nv[a][3]};
+}
+
+// Per-Gauss-point results of phase G kept in registers for the records.
+struct HexGp {
+  double w, tauP, A[3][3], vp[3];
+};
+
+// Phase G of lane (e, g, h), up to the residual written to L.a.R.
+template <int MODE>
+__device__ inline void hexb_gauss(HexBLds<MODE>& L, int e, int g, int h, double hlen, double mu,
+                                  const ElemParams& p, HexGp& out) {
+  constexpr double SGN = Traits<HEXA8>::SGN, CI = Traits<HEXA8>::CI;
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  double xi[3];
+  xi[0] = (g & 1) ? kG2 : -kG2;
+  xi[1] = (g & 2) ? kG2 : -kG2;
+  xi[2] = (g & 4) ? kG2 : -kG2;
+  double w;
+  {
+    double N[8], d1[8], d2v[8], d3[8];
+    shape<HEXA8>(xi, N, d1, d2v, d3);
+    // Jacobian B(r, c) = sum_n x_c(n) dN_n/dxi_r and its inverse (computeBasisFunctions3D)
+    double B[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+      const double* x = L.a.nd.X[e][n];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        B[0][c] += x[c] * d1[n];
+        B[1][c] += x[c] * d2v[n];
+        B[2][c] += x[c] * d3[n];
+      }
+    }
+    const double c00 = B[1][1] * B[2][2] - B[1][2] * B[2][1];
+    const double c01 = B[1][2] * B[2][0] - B[1][0] * B[2][2];
+    const double c02 = B[1][0] * B[2][1] - B[1][1] * B[2][0];
+    const double Jac = B[0][0] * c00 + B[0][1] * c01 + B[0][2] * c02;
+    const double id = 1.0 / Jac;
+    const double i00 = c00 * id, i01 = (B[0][2] * B[2][1] - B[0][1] * B[2][2]) * id,
+                 i02 = (B[0][1] * B[1][2] - B[0][2] * B[1][1]) * id;
+    const double i10 = c01 * id, i11 = (B[0][0] * B[2][2] - B[0][2] * B[2][0]) * id,
+                 i12 = (B[0][2] * B[1][0] - B[0][0] * B[1][2]) * id;
+    const double i20 = c02 * id, i21 = (B[0][1] * B[2][0] - B[0][0] * B[2][1]) * id,
+                 i22 = (B[0][0] * B[1][1] - B[0][1] * B[1][0]) * id;
+    w = 1.0 * Jac;  // weight 1 (UtilitiesQuadrature.cpp:285-297)
+    // shape gradients of nodes 4h .. 4h + 3 to LDS (the pair's other lane writes the rest)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if ((k >> 2) != h) continue;
+      const double g0 = d1[k] * i00 + d2v[k] * i01 + d3[k] * i02;
+      const double g1 = d1[k] * i10 + d2v[k] * i11 + d3[k] * i12;
+      const double g2 = d1[k] * i20 + d2v[k] * i21 + d3[k] * i22;
+      d2* dst = reinterpret_cast<d2*>(L.gd[e][g][k]);
+      dst[0] = d2{g0, g1};
+      dst[1] = d2{g2, N[k]};
+    }
+  }
+  wave_sync();
+  // interpolated state (Hexa8Node.cpp:828-889), summed in node order
+  double vp[3] = {0, 0, 0}, gpv[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  double vel[3] = {0, 0, 0}, gr[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  double pres = 0.0, dp[3] = {0, 0, 0}, vd[3] = {0, 0, 0};
+#pragma unroll
+  for (int n = 0; n < 8; ++n) {
+    const double* sp = L.a.nd.S[0][e][n];
+    const double* sc = L.a.nd.S[1][e][n];
+    const double* sd = L.a.nd.S[2][e][n];
+    const double* gn = L.gd[e][g][n];
+    const double Nn = gn[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      vp[a] += sp[a] * Nn;
+      vel[a] += sc[a] * Nn;
+      vd[a] += sd[a] * Nn;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        gpv[a][c] += sp[a] * gn[c];
+        gr[a][c] += sc[a] * gn[c];
+      }
+    }
+    pres += sc[3] * Nn;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) dp[c] += sc[3] * gn[c];
+    __builtin_amdgcn_sched_barrier(0);  // one node's LDS operands in flight at a time
+  }
+  // stabilisation (evaluateStabParams_algo3, stabilisationRoutines.h:128-143, G = 4/h^2 I)
+  const double rho = p.rho;
+  double res2[3], rStab[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    double s1 = gpv[a][0] * vel[0], s2 = gr[a][0] * vp[0], s3 = gpv[a][0] * vp[0];
+#pragma unroll
+    for (int c = 1; c < 3; ++c) {
+      s1 += gpv[a][c] * vel[c];
+      s2 += gr[a][c] * vp[c];
+      s3 += gpv[a][c] * vp[c];
+    }
+    res2[a] = rho * (vd[a] + (s1 + s2 - s3));
+    rStab[a] = res2[a] + dp[a];
+  }
+  const double fact = 4.0 / hlen / hlen;
+  double uGu = 0.0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) uGu += vp[a] * (fact * vp[a]);
+  const double G2 = 3 * (fact * fact);
+  double tau0 = 1.0 / sqrt(uGu + CI * (mu * mu / rho / rho) * G2);
+  tau0 = tau0 / rho;
+  const double tauP = tau0 * p.stab[1];
+  // residual (Hexa8Node.cpp:1049-1060): component a of node k is -w (N_k X_a + dN_k . Y_a)
+  // with X_a = res2_a, Y_a = mu grad_a - p e_a for a < 3, X_3 = s trace, Y_3 = s tauP rStab
+  double Y[4][3], X[4];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    X[a] = res2[a];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) Y[a][c] = mu * gr[a][c] - (a == c ? pres : 0.0);
+  }
+  X[3] = SGN * (gr[0][0] + gr[1][1] + gr[2][2]);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) Y[3][c] = SGN * tauP * rStab[c];
+  out.w = w;
+  out.tauP = tauP;
+  const double afr = p.af * rho;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    out.vp[a] = vp[a];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out.A[a][c] = afr * gpv[a][c];
+  }
+  wave_sync();  // the interpolation's reads of the nodal data are done (R overlays it)
+#pragma unroll
+  for (int kk = 0; kk < 4; ++kk) {
+    const int k = 2 * kk + h;
+    const d2* src = reinterpret_cast<const d2*>(L.gd[e][g][k]);
+    const d2 ga = src[0], gb = src[1];
+    double F[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) F[a] = -w * (gb.y * X[a] + ga.x * Y[a][0] + ga.y * Y[a][1] + gb.x * Y[a][2]);
+    d2* dst = reinterpret_cast<d2*>(L.a.R[e][g][k]);
+    dst[0] = d2{F[0], F[2]};
+    dst[1] = d2{F[1], F[3]};
+  }
+}
+
+// Phase G records of lane (e, g, h) for the tangent (after the residual sums).
+template <int MODE>
+__device__ inline void hexb_records(HexBLds<MODE>& L, int e, int g, int h, double mu, const ElemParams& p,
+                                    const HexGp& q) {
+  using T = typename HexBCfg<MODE>::T;
+  constexpr double SGN = Traits<HEXA8>::SGN;
+  const double stp = SGN * q.tauP, afr = p.af * p.rho;
+  if constexpr (MODE == 2) {
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    typedef float f4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      const int k = 2 * kk + h;
+      const d2* src = reinterpret_cast<const d2*>(L.gd[e][g][k]);
+      const d2 ga = src[0], gb = src[1];
+      const double dk[3] = {ga.x, ga.y, gb.x};
+      double gk[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        gk[c] = stp * ((dk[0] * q.w) * q.A[0][c] + (dk[1] * q.w) * q.A[1][c] + (dk[2] * q.w) * q.A[2][c]);
+      const double ud = q.vp[0] * dk[0] + q.vp[1] * dk[1] + q.vp[2] * dk[2];
+      f4* r = reinterpret_cast<f4*>(L.a.k.rec[e][g][k]);
+      r[0] = f4{(float)dk[0], (float)dk[1], (float)dk[2], (float)gb.y};
+      r[1] = f4{(float)gk[0], (float)gk[1], (float)gk[2], (float)(afr * ud)};
+    }
+  }
+  if (h == 0) {
+    T* c = L.a.k.cg[e][g];
+    c[0] = (T)q.w;
+    c[1] = (T)stp;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) c[2 + 3 * a + b] = (T)q.A[a][b];
+    c[11] = (T)(mu * p.af);
+    if constexpr (MODE == 0) {
+#pragma unroll
+      for (int b = 0; b < 3; ++b) c[12 + b] = afr * q.vp[b];
+    }
+  }
+}
+
+// Phase K: block (i, j) of element e summed over the Gauss points from the records.
+template <int MODE>
+__device__ inline void hexb_block(const HexBLds<MODE>& L, int e, int i, int j, const ElemParams& p,
+                                  typename HexBCfg<MODE>::T (&K)[4][4]) {
+  using T = typename HexBCfg<MODE>::T;
+  constexpr double SGN = Traits<HEXA8>::SGN;
+  const T af = (T)p.af, cst = (T)(p.rho * p.acceFact), saf = (T)(SGN * p.af);
+  const T muaf = L.a.k.cg[e][0][11];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) K[a][b] = (T)0;
+#pragma unroll
+  for (int g = 0; g < 8; ++g) {
+    const T* q = L.a.k.cg[e][g];
+    const T w = q[0], stp = q[1];
+    T dNi[3], dNj[3], Ni, Nj, gi[3], uj;
+    if constexpr (MODE == 2) {
+      const float* ri = L.a.k.rec[e][g][i];
+      const float* rj = L.a.k.rec[e][g][j];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) { dNi[c] = ri[c]; dNj[c] = rj[c]; gi[c] = ri[4 + c]; }
+      Ni = ri[3]; Nj = rj[3];
+      uj = rj[7];
+    } else {
+      const double* ri = L.gd[e][g][i];
+      const double* rj = L.gd[e][g][j];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) { dNi[c] = ri[c]; dNj[c] = rj[c]; }
+      Ni = ri[3]; Nj = rj[3];
+    }
+    T wdNi[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) wdNi[c] = w * dNi[c];
+    if constexpr (MODE != 2) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) gi[c] = stp * (wdNi[0] * q[2 + c] + wdNi[1] * q[5 + c] + wdNi[2] * q[8 + c]);
+      uj = q[12] * dNj[0] + q[13] * dNj[1] + q[14] * dNj[2];
+    }
+    const T wNi = w * Ni;
+    const T s1 = wNi * Nj;
+    const T dd = wdNi[0] * dNj[0] + wdNi[1] * dNj[1] + wdNi[2] * dNj[2];
+    const T dj = uj + cst * Nj;
+    const T s2 = cst * s1 + muaf * dd + wNi * uj;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+#pragma unroll
+      for (int b = 0; b < 3; ++b) K[a][b] += s1 * q[2 + 3 * a + b];
+      K[a][a] += s2;
+      K[a][3] -= af * wdNi[a] * Nj;
+    }
+#pragma unroll
+    for (int b = 0; b < 3; ++b) K[3][b] += saf * wNi * dNj[b] + Nj * gi[b] + stp * wdNi[b] * dj;
+    K[3][3] += stp * af * dd;
+    __builtin_amdgcn_sched_barrier(0);  // one Gauss point's LDS operands in flight at a time
+  }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(64, HexBCfg<MODE>::occ) void k_hex8_batch(AsmView v, ElemParams p,
+                                                                const int32_t* __restrict__ elems, int64_t n,
+                                                                int lift) {
+  using Cfg = HexBCfg<MODE>;
+  using T = typename Cfg::T;
+  __shared__ __align__(16) HexBLds<MODE> L;
+  const int l0 = threadIdx.x;
+  const bool doLift = (MODE == 0) && (lift != 0);
+  const int64_t nb = (n + kHexEB - 1) / kHexEB;
+  int64_t b = blockIdx.x;
+  if (b >= nb) return;
+  // lane (e, k, c) of staging and of the residual sums; e is also phase G's element
+  const int se = l0 >> 4, sk = (l0 >> 1) & 7, sc = l0 & 1;
+
+  // Staging registers of the batch to come.  Every lane issues the same loads, with no
+  // branch around them (slots past the class end read its last element, whose results
+  // are not stored), so the compiler can wait for each group of loads by count.
+  int32_t node = 0, eid = 0;
+  int32_t co[kHexEB];
+  auto issue_ids = [&](int64_t bb) {
+    const int64_t sl = min<int64_t>(bb * kHexEB + se, n - 1);
+    node = v.sconn[sl * 8 + sk];
+    eid = elems[sl];
+#pragma unroll
+    for (int q = 0; q < kHexEB; ++q) {
+      const int64_t s2 = min<int64_t>(bb * kHexEB + q, n - 1);
+      co[q] = Cfg::KM ? v.blkofs[s2 * 64 + l0] : 0;
+    }
+  };
+  typedef double d4 __attribute__((ext_vector_type(4)));
+  d4 rA, rB;
+  double rX[3];
+  int2 ri;
+  double hlen, mu;
+  auto issue_nodal = [&]() {
+    const int64_t o = (int64_t)node * 4;
+    // lane c = 0: solnPrev and (lift) solnApplied; c = 1: solnCur and solnDotCur
+    rA = *reinterpret_cast<const d4*>((sc ? v.solnCur : v.solnPrev) + o);
+    rB = *reinterpret_cast<const d4*>((sc == 0 && doLift ? v.solnApplied : v.solnDotCur) + o);
+    const double* x = v.coords + (int64_t)node * 3;
+    rX[0] = x[0]; rX[1] = x[1]; rX[2] = x[2];
+    ri = reinterpret_cast<const int2*>(v.nodeInfo)[2 * (int64_t)node + 1];  // free mask, first row
+    hlen = v.charlen[eid];
+    mu = v.elem_mu ? v.elem_mu[eid] : p.mu;
+  };
+  issue_ids(b);
+  issue_nodal();
+
+  for (;;) {
+    const int64_t slot0 = b * kHexEB;
+    const int nv = (int)min<int64_t>(kHexEB, n - slot0);  // elements of this batch
+    // stage the batch's nodal data
+    if (sc == 0) {
+      L.a.nd.X[se][sk][0] = rX[0]; L.a.nd.X[se][sk][1] = rX[1]; L.a.nd.X[se][sk][2] = rX[2];
+      *reinterpret_cast<d4*>(L.a.nd.S[0][se][sk]) = rA;
+      if (doLift) *reinterpret_cast<d4*>(L.app[se][sk]) = rB;
+    } else {
+      *reinterpret_cast<d4*>(L.a.nd.S[1][se][sk]) = rA;
+      *reinterpret_cast<d4*>(L.a.nd.S[2][se][sk]) = rB;
+      L.info[se][sk][0] = ri.x;
+      L.info[se][sk][1] = ri.y;
+    }
+    const double hlen_cur = hlen, mu_cur = mu;
+    int32_t co_cur[kHexEB];
+#pragma unroll
+    for (int q = 0; q < kHexEB; ++q) co_cur[q] = co[q];
+    wave_sync();
+    // connectivity, element ids and block offsets of the next batch
+    const int64_t bn = b + gridDim.x;
+    const bool more = bn < nb;
+    issue_ids(more ? bn : b);
+    // phase G.  The lane's Gauss point is re-derived through an empty asm in every
+    // iteration, so that its shape values are recomputed here instead of being hoisted
+    // out of the loop and held in registers across it.
+    int lg = l0;
+    asm volatile("" : "+v"(lg));
+    HexGp gq;
+    hexb_gauss<MODE>(L, se, (lg >> 1) & 7, sc, hlen_cur, mu_cur, p, gq);
+    wave_sync();
+    // residual summed over Gauss points: lane (e, k, c) ends with components c, c + 2 of node k
+    double fsum[2];
+    {
+      typedef double d2 __attribute__((ext_vector_type(2)));
+      d2 s = *reinterpret_cast<const d2*>(&L.a.R[se][0][sk][2 * sc]);
+#pragma unroll
+      for (int g = 1; g < 8; ++g) s += *reinterpret_cast<const d2*>(&L.a.R[se][g][sk][2 * sc]);
+      fsum[0] = s.x;
+      fsum[1] = s.y;
+    }
+    // RHS entries of node k this lane adds to (loaded before the tangent phase)
+    const int mk = L.info[se][sk][0], rk = L.info[se][sk][1];
+    const bool own = se < nv;
+    int row[2];
+    double rold[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int a = sc + 2 * r;
+      row[r] = (own && ((mk >> a) & 1)) ? rk + __popc(mk & ((1 << a) - 1)) : -1;
+      rold[r] = v.rhs[row[r] >= 0 ? row[r] : 0];  // row 0 exists whenever this kernel runs
+    }
+    issue_nodal();  // (the last batch reloads its own data, unused)
+    if constexpr (Cfg::KM) {
+      wave_sync();  // the residual sums have read L.a.R (the records overlay it)
+      hexb_records<MODE>(L, se, sk, sc, mu_cur, p, gq);
+      wave_sync();
+      T* kbv = MODE == 2 ? reinterpret_cast<T*>(v.kb32) : reinterpret_cast<T*>(v.kb);
+      for (int e = 0; e < nv; ++e) {
+        // (re-derived per element, so that the per-lane LDS addresses are not hoisted)
+        int lk = l0;
+        asm volatile("" : "+v"(lk));
+        const int i = lk >> 3, j = lk & 7;
+        const int32_t coe = e == 0 ? co_cur[0] : e == 1 ? co_cur[1] : e == 2 ? co_cur[2] : co_cur[3];
+        const int mi = L.info[e][i][0], mj = L.info[e][j][0];
+        const bool blk = mi != 0 && mj != 0;
+        const bool first = (coe >> 30) & 1;
+        typedef T blk_t __attribute__((ext_vector_type(4)));
+        const blk_t* bq = reinterpret_cast<const blk_t*>(kbv + (int64_t)(blk ? (coe & ((1 << 30) - 1)) : 0) * 16);
+        const bool keep = blk && !first;
+        T old[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          const blk_t o = bq[a];  // block 0 where this lane adds nothing (always present)
+          old[a][0] = keep ? o.x : (T)0; old[a][1] = keep ? o.y : (T)0;
+          old[a][2] = keep ? o.z : (T)0; old[a][3] = keep ? o.w : (T)0;
+        }
+        T K[4][4];
+        hexb_block<MODE>(L, e, i, j, p, K);
+        // the block is formed here, before the branches that use it (sunk into the store
+        // branch, it left every Gauss point's LDS operands live at once)
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int bb = 0; bb < 4; ++bb) asm volatile("" : "+v"(K[a][bb]));
+        if (doLift) {
+          double ls[4];
+#pragma unroll
+          for (int a = 0; a < 4; ++a) {
+            double sl = 0.0;
+#pragma unroll
+            for (int bb = 0; bb < 4; ++bb) {
+              const double dlt = L.app[e][j][bb];
+              if (!((mj >> bb) & 1) && fabs(dlt) > 1.0e-10) sl += (double)K[a][bb] * dlt;
+            }
+            sl += __shfl_xor(sl, 1);
+            sl += __shfl_xor(sl, 2);
+            sl += __shfl_xor(sl, 4);
+            ls[a] = sl;
+          }
+          if (j == 0) {
+#pragma unroll
+            for (int a = 0; a < 4; ++a) L.a.k.lift[e][i][a] = ls[a];
+          }
+        }
+        if (blk) hex_block_store<T>(kbv, coe, mi, mj, K, old);
+      }
+      if (doLift) {
+        wave_sync();
+        if (own) {
+          fsum[0] -= L.a.k.lift[se][sk][sc];
+          fsum[1] -= L.a.k.lift[se][sk][sc + 2];
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+      if (row[r] >= 0) v.rhs[row[r]] = rold[r] + fsum[r];
+    wave_sync();
+    if (!more) break;
+    b = bn;
+  }
+}
+
 template <int ET>
 static hipError_t prepare_t(hipStream_t s, int64_t nElem, const double* coords, const int32_t* conn,
                             double* charlen, double* vol, int64_t* bad) {
@@ -912,13 +1422,43 @@ static hipError_t hex8_launch(hipStream_t s, const AsmView& v, const ElemParams&
   return hipGetLastError();
 }
 
+// Workgroups of the batched Hexa8 kernel the current device holds at once.
+template <int MODE>
+static hipError_t hex8_batch_launch(hipStream_t s, const AsmView& v, const ElemParams& p, const int32_t* elems,
+                                    int64_t n, int lift) {
+  static int64_t resident[16];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) dev = 0;
+  if (resident[dev] == 0) {
+    int cus = 0, per = 0;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_hex8_batch<MODE>, 64, 0) != hipSuccess || per < 1)
+      per = 4 * HexBCfg<MODE>::occ;
+    resident[dev] = (int64_t)(cus > 0 ? cus : 256) * per;
+  }
+  const int64_t nb = (n + kHexEB - 1) / kHexEB;
+  const unsigned grid = (unsigned)std::min<int64_t>(nb, resident[dev]);
+  hipLaunchKernelGGL((k_hex8_batch<MODE>), dim3(grid), dim3(64), 0, s, v, p, elems, n, lift);
+  return hipGetLastError();
+}
+
+static bool hex8_use_pipelined() {
+  const char* e = getenv("SFEM_HEX_KERNEL");
+  return e && strcmp(e, "pipelined") == 0;
+}
+
 hipError_t launch_assemble(int et, hipStream_t s, const AsmView& v, const ElemParams& p,
                            const int32_t* elems, int64_t n, int mode, int lift, int64_t probe_e, int variant) {
   if (et == HEXA8 && probe_e < 0) {
     if (n == 0) return hipSuccess;
-    if (mode == 0) return hex8_launch<0>(s, v, p, elems, n, lift);
-    if (mode == 1) return hex8_launch<1>(s, v, p, elems, n, lift);
-    return hex8_launch<2>(s, v, p, elems, n, lift);
+    if (hex8_use_pipelined()) {
+      if (mode == 0) return hex8_launch<0>(s, v, p, elems, n, lift);
+      if (mode == 1) return hex8_launch<1>(s, v, p, elems, n, lift);
+      return hex8_launch<2>(s, v, p, elems, n, lift);
+    }
+    if (mode == 0) return hex8_batch_launch<0>(s, v, p, elems, n, lift);
+    if (mode == 1) return hex8_batch_launch<1>(s, v, p, elems, n, lift);
+    return hex8_batch_launch<2>(s, v, p, elems, n, lift);
   }
   switch (et) {
     case TRIA3: return assemble_t<TRIA3>(s, v, p, elems, n, mode, lift, probe_e, variant);
