@@ -1,12 +1,13 @@
 // sfem_elements.hip -- element geometry and the stabilised Navier-Stokes element
 // kernels (residual Fe and tangent Ke) with scatter-add into the device CSR.
 //
-// One "team" of lanes per element: a team of TEAM lanes (64 for Hexa8 = one
-// wavefront, 16 for the 2D / Tetra4 elements = a quarter wavefront) owns one
-// element.  Lane (i, j) of the team owns the ndof x ndof block of node pair
-// (i, j) of Ke and keeps it in registers across all Gauss points; the
-// per-Gauss-point data (shape gradients, interpolated state, stabilisation
+// Tria3 / Quad4 / Tetra4 (and the per-element probe): one "team" of TEAM lanes per
+// element (16, a quarter wavefront; 64 for the Hexa8 probe).  Lane (i, j) of the team owns the ndof x ndof
+// block of node pair (i, j) of Ke and keeps it in registers across all Gauss points;
+// the per-Gauss-point data (shape gradients, interpolated state, stabilisation
 // parameters) is computed cooperatively by lane (gp, k) and staged in LDS.
+// Hexa8 colour loops: the batched kernel k_hex8_batch (four elements per wavefront
+// iteration, Gauss-point lanes for the residual, node-pair lanes for the tangent).
 //
 // The arithmetic restates calcStiffnessAndResidual of the reference
 // (src/base/LagrangeElem3DNavierStokesHexa8Node.cpp:764-1074,
@@ -29,8 +30,6 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <cstdlib>
-#include <cstring>
 #include <type_traits>
 
 #include "sfem_internal.hpp"
@@ -256,22 +255,6 @@ struct TeamLds {
   int32_t info[C::NPE][4];           // rowStart, rowlen, mask, firstRow
 };
 
-// Per-wavefront staging of the pipelined Hexa8 loop: the residual and lift partial sums
-// are reduced across lanes in registers, so only the shared per-element data is here.
-struct HexLds {
-  using C = Traits<HEXA8>;
-  static constexpr int NQ = TeamLds<HEXA8>::NQ;
-  double X[8][3];
-  double S[3][8][4];
-  double app[8][4];
-  double phi[8][8][4];
-  double W[8];
-  double Q[8][NQ];
-  double U[8][10];
-  double P[8][8][6];
-  int32_t info[8][4];
-};
-
 // P2: shape gradients of node k at Gauss point gp, lane l = gp * NPE + k (l < NGP * NPE).
 template <int ET, class LDS>
 __device__ inline void phase_geometry(LDS& L, int l) {
@@ -453,7 +436,7 @@ __device__ inline void add_row_segment(T* __restrict__ row, const T (&k)[NDOF], 
   }
 }
 
-// Generic element loop (all element types; the Hexa8 colour loops use the pipelined
+// Generic element loop (all element types; the Hexa8 colour loops use the batched
 // kernel below and this one serves the per-element probe): TEAMS elements per 256-lane
 // workgroup, one per team.
 // WGS lanes per workgroup (WGS / TEAM elements side by side); PERSIST: the grid is what
@@ -598,231 +581,12 @@ __global__ __launch_bounds__(WGS) void k_assemble(AsmView v, ElemParams p0,
   }
 }
 
-// Orders the LDS traffic of one wavefront (the whole workgroup of the pipelined kernel):
+// Orders the LDS traffic of one wavefront (the whole workgroup of the batched kernel):
 // the compiler may not move LDS accesses across it, and it waits for the issued ones.
 __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Hexa8 element loop of one colour class, pipelined: one wavefront per workgroup walks
-// the slots blockIdx.x, blockIdx.x + gridDim.x, ... of the class (the grid is at most
-// what the GPU holds at once).  Lane l is node pair (i, j) = (l / 8, l % 8) of the
-// tangent, Gauss point / node (gp, k) = (l / 8, l % 8) of the geometry and residual
-// phases, and loads quantity q = l / 8 of node l % 8 of the next element:
-//   q = 0 coordinates, 1 solnPrev, 2 solnCur, 3 solnDotCur, 4 solnApplied (lift only),
-//   5 node info {rowStart, rowlen, free mask, first row}.
-// While one element is computed, the next element's connectivity, element -> nonzero
-// offsets and length scale are in flight (issued before the geometry phase), then its
-// nodal data (issued before the tangent phase); the CSR and RHS values an element adds
-// to are loaded before its tangent is accumulated (no other element of the class owns
-// them).  The residual sums over Gauss points and the lift sums over neighbour nodes are
-// butterfly reductions across lanes.
-// Register budget per mode: FP64 tangent (0), two wavefronts per SIMD, the tangent
-// phase holding the 4 x 4 block, the block values it adds to (loaded before the
-// accumulation) and two Gauss points' worth of LDS operands; FP32 tangent (2), three
-// wavefronts per SIMD, one Gauss point at a time and the block values loaded after the
-// accumulation; residual only (1), three wavefronts per SIMD without the tangent factors.
-template <int MODE>
-struct HexCfg {
-  static constexpr int occ = MODE == 0 ? 2 : 3;
-  static constexpr int unroll = MODE == 2 ? 1 : 2;  // Gauss points of the tangent loop unrolled together
-  static constexpr bool late_old = MODE == 2;
-};
-
-template <int MODE>
-__global__ __launch_bounds__(64, HexCfg<MODE>::occ) void k_assemble_hex8(AsmView v, ElemParams p,
-                                                      const int32_t* __restrict__ elems, int64_t n,
-                                                      int lift) {
-  using T = typename std::conditional<MODE == 2, float, double>::type;
-  constexpr int NPE = 8, NDOF = 4;
-  typedef T vec_t __attribute__((ext_vector_type(4), aligned(sizeof(T))));
-  __shared__ HexLds L;
-  const int l0 = threadIdx.x;
-  const bool doLift = (MODE == 0) && (lift != 0);
-  int nn = l0 & 7, q = l0 >> 3;
-  int64_t s = blockIdx.x;
-  if (s >= n) return;
-
-  // staging registers of the element to come
-  int32_t co = MODE != 1 ? v.blkofs[s * 64 + l0] : -1;
-  int32_t eid = elems[s];
-  double h = v.charlen[eid];
-  double mu = v.elem_mu ? v.elem_mu[eid] : p.mu;
-  double rv[4] = {0.0, 0.0, 0.0, 0.0};
-  int4 ri = {0, 0, 0, 0};
-  int cnt = 0;
-  auto issue_node = [&](int32_t node) {
-    const double* src = nullptr;
-    cnt = 0;
-    if (q == 0) { src = v.coords + (int64_t)node * 3; cnt = 3; }
-    else if (q <= 3) { src = (q == 1 ? v.solnPrev : q == 2 ? v.solnCur : v.solnDotCur) + (int64_t)node * 4; cnt = 4; }
-    else if (q == 4 && doLift) { src = v.solnApplied + (int64_t)node * 4; cnt = 4; }
-#pragma unroll
-    for (int c = 0; c < 4; ++c) rv[c] = c < cnt ? src[c] : 0.0;
-    if (q == 5) ri = reinterpret_cast<const int4*>(v.nodeInfo)[node];
-  };
-  issue_node(v.sconn[s * NPE + nn]);
-
-  for (;;) {
-    // The lane index is re-derived in every iteration (through an empty asm the compiler
-    // cannot see through), so that per-lane LDS addresses of the phases are recomputed
-    // instead of being hoisted out of the loop and held in registers across it.
-    int l = l0;
-    asm volatile("" : "+v"(l));
-    nn = l & 7;
-    q = l >> 3;
-    const int i = l >> 3, j = l & 7;
-    // stage the element's nodal data
-    if (q < 5) {
-      double* dst = q == 0 ? &L.X[nn][0] : q <= 3 ? &L.S[q - 1][nn][0] : &L.app[nn][0];
-      const int w = q == 0 ? 3 : 4;  // app is zero without lift
-#pragma unroll
-      for (int c = 0; c < 4; ++c)
-        if (c < w) dst[c] = rv[c];
-    } else if (q == 5) {
-      L.info[nn][0] = ri.x; L.info[nn][1] = ri.y; L.info[nn][2] = ri.z; L.info[nn][3] = ri.w;
-    }
-    const int32_t co_cur = co;
-    ElemParams pe = p;
-    pe.mu = mu;
-    const double h_cur = h;
-    wave_sync();
-    // slot-ordered data of the next element
-    const int64_t sn = s + gridDim.x;
-    const bool more = sn < n;
-    int32_t nid_n = 0;
-    if (more) {
-      nid_n = v.sconn[sn * NPE + nn];
-      if (MODE != 1) co = v.blkofs[sn * 64 + l0];
-      eid = elems[sn];
-    }
-    phase_geometry<HEXA8>(L, l);
-    wave_sync();
-    phase_interp<HEXA8>(L, l);
-    wave_sync();
-    double Fk[4];
-    phase_gauss<HEXA8, MODE != 1>(L, l, h_cur, pe, Fk);
-    // nodal data and length scale of the next element
-    if (more) {
-      issue_node(nid_n);
-      h = v.charlen[eid];
-      if (v.elem_mu) mu = v.elem_mu[eid];
-    }
-    wave_sync();
-
-    const int mi = L.info[i][2], mj = L.info[j][2];
-    // the RHS entries of node i (lanes j == 0) and, below, the CSR segments of block (i, j)
-    const bool rhs_lane = (j == 0) && (mi != 0);
-    const int32_t r0 = L.info[i][3];
-    double rold[4] = {0.0, 0.0, 0.0, 0.0};
-    if (rhs_lane) {
-      int ra = 0;
-#pragma unroll
-      for (int a = 0; a < NDOF; ++a)
-        if ((mi >> a) & 1) rold[a] = v.rhs[r0 + ra++];
-    }
-    double lsum[4] = {0.0, 0.0, 0.0, 0.0};
-    if constexpr (MODE != 1) {
-      // block (i, j) of the node-pair block form: 16 values, one aligned run, loaded
-      // before the tangent is accumulated (no other element of the class adds to it)
-      typedef T blk_t __attribute__((ext_vector_type(4)));
-      T* kbv = MODE == 2 ? reinterpret_cast<T*>(v.kb32) : reinterpret_cast<T*>(v.kb);
-      // (bit 30 of the map: this element is the block's first, and stores it whole)
-      const bool blk = mi != 0 && mj != 0;
-      const bool first = (co_cur >> 30) & 1;
-      blk_t* bq = reinterpret_cast<blk_t*>(kbv + (int64_t)(blk ? (co_cur & ((1 << 30) - 1)) : 0) * 16);
-      blk_t old[NDOF];
-      if constexpr (!HexCfg<MODE>::late_old) {
-#pragma unroll
-        for (int a = 0; a < NDOF; ++a) old[a] = (blk && !first) ? bq[a] : blk_t{(T)0, (T)0, (T)0, (T)0};
-      }
-      T K[NDOF][NDOF];
-      accumulate_block<HEXA8, T, HexCfg<MODE>::unroll>(L, i, j, pe, K);
-      if constexpr (HexCfg<MODE>::late_old) {
-#pragma unroll
-        for (int a = 0; a < NDOF; ++a) old[a] = (blk && !first) ? bq[a] : blk_t{(T)0, (T)0, (T)0, (T)0};
-      }
-      if (doLift) {
-#pragma unroll
-        for (int a = 0; a < NDOF; ++a) {
-          double sl = 0.0;
-#pragma unroll
-          for (int b = 0; b < NDOF; ++b) {
-            const double dlt = L.app[j][b];
-            if (!((mj >> b) & 1) && fabs(dlt) > 1.0e-10) sl += (double)K[a][b] * dlt;
-          }
-          lsum[a] = sl;
-        }
-      }
-      if (blk) {
-        if (mi == 15 && mj == 15) {
-#pragma unroll
-          for (int a = 0; a < NDOF; ++a) {
-            blk_t r = old[a];
-            r.x += K[a][0]; r.y += K[a][1]; r.z += K[a][2]; r.w += K[a][3];
-            bq[a] = r;
-          }
-        } else {
-          // free rows and columns packed to the front of the block
-          T nv[NDOF][NDOF];
-#pragma unroll
-          for (int a = 0; a < NDOF; ++a) {
-            nv[a][0] = old[a].x; nv[a][1] = old[a].y; nv[a][2] = old[a].z; nv[a][3] = old[a].w;
-          }
-          int ra = 0;
-#pragma unroll
-          for (int a = 0; a < NDOF; ++a) {
-            if (!((mi >> a) & 1)) continue;
-            int rb = 0;
-#pragma unroll
-            for (int b = 0; b < NDOF; ++b) {
-              if (!((mj >> b) & 1)) continue;
-              // rows and columns are only ever packed towards the front: ra <= a, rb <= b
-#pragma unroll
-              for (int x = 0; x < NDOF; ++x)
-#pragma unroll
-                for (int y = 0; y < NDOF; ++y)
-                  if (x == ra && y == rb) nv[x][y] += K[a][b];
-              ++rb;
-            }
-            ++ra;
-          }
-#pragma unroll
-          for (int a = 0; a < NDOF; ++a) bq[a] = blk_t{nv[a][0], nv[a][1], nv[a][2], nv[a][3]};
-        }
-      }
-    }
-    // residual: sum over Gauss points (lane bits 3-5), then node i's sum to lane 8i
-#pragma unroll
-    for (int a = 0; a < NDOF; ++a) {
-      double f = Fk[a];
-      f += __shfl_xor(f, 8);
-      f += __shfl_xor(f, 16);
-      f += __shfl_xor(f, 32);
-      Fk[a] = __shfl(f, i);
-    }
-    if (doLift) {
-#pragma unroll
-      for (int a = 0; a < NDOF; ++a) {
-        double t = lsum[a];
-        t += __shfl_xor(t, 1);
-        t += __shfl_xor(t, 2);
-        t += __shfl_xor(t, 4);
-        Fk[a] -= t;
-      }
-    }
-    if (rhs_lane) {
-      int ra = 0;
-#pragma unroll
-      for (int a = 0; a < NDOF; ++a)
-        if ((mi >> a) & 1) { v.rhs[r0 + ra] = rold[a] + Fk[a]; ++ra; }
-    }
-    wave_sync();
-    if (!more) break;
-    s = sn;
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -848,11 +612,19 @@ template <int MODE>
 struct HexBCfg {
   using T = typename std::conditional<MODE == 2, float, double>::type;  // tangent type
   static constexpr bool KM = MODE != 1;
-  // per (element, Gauss point): w, s tauP, A (3 x 3, A = af rho gradPrev), mu af, and
-  // for the FP64 tangent af rho uPrev
-  static constexpr int CW = MODE == 2 ? 12 : 16;
+  // Per (element, Gauss point) coefficients of the tangent, with the weight w folded in
+  // (A = af rho gradPrev, s = -1, the Hexa8 sign):
+  //   0-8 w A, 9 w rho td8, 10 w mu af, 11 w af, 12 w s af, 13 w s tauP, 14 w s tauP af,
+  //   15 w; the FP64 tangent adds 16-18 af rho uPrev and 19 s tauP.
+  // Rows are padded so that consecutive Gauss points start in different LDS banks.
+  static constexpr int CW = MODE == 2 ? 20 : 22;
   static constexpr int occ = 2;
 };
+
+// Per-(element, Gauss point) rows of 8 nodes x 4 values, padded by 16 bytes so that the
+// eight Gauss points of an element start in different LDS banks.
+constexpr int kHexRow = 34;   // doubles: dN/dx, dN/dy, dN/dz, N of each node
+constexpr int kHexRow32 = 68; // floats: the FP32 tangent's 8 values of each node
 
 template <int MODE>
 struct HexBLds {
@@ -863,20 +635,33 @@ struct HexBLds {
       double X[kHexEB][8][4];     // coordinates
       double S[3][kHexEB][8][4];  // solnPrev, solnCur, solnDotCur
     } nd;
-    double R[kHexEB][8][8][4];    // residual (element, Gauss point, node), components 0, 2, 1, 3
+    // residual (element, Gauss point, node), components 0, 2, 1, 3 (the residual-only
+    // loop keeps it over the shape gradients instead)
+    double R[MODE == 1 ? 1 : kHexEB][8][kHexRow];
     struct {
-      // FP32 tangent, per (element, Gauss point, node): dN/dx, dN/dy, dN/dz, N,
-      // g(0..2) = s tauP sum_a w dN_a A_a., u = af rho uPrev.dN
-      float rec[MODE == 2 ? kHexEB : 1][8][8][8];
       T cg[Cfg::KM ? kHexEB : 1][8][Cfg::CW];
       double lift[MODE == 0 ? kHexEB : 1][8][4];  // lift sums of node i of each element
     } k;
   } a;
-  // per (element, Gauss point, node): dN/dx, dN/dy, dN/dz, N (the FP64 tangent's record)
-  double gd[kHexEB][8][8][4];
+  union {
+    // per (element, Gauss point, node): dN/dx, dN/dy, dN/dz, N (the FP64 tangent's record)
+    double gd[kHexEB][8][kHexRow];
+    // FP32 tangent, per (element, Gauss point, node): dN/dx, dN/dy, dN/dz, N,
+    // g(0..2) = s tauP sum_a w dN_a A_a., u = af rho uPrev.dN; each node's 32 bytes are
+    // written over its 32 bytes of gd once they are read
+    float rec[kHexEB][8][kHexRow32];
+  } b;
   double app[MODE == 0 ? kHexEB : 1][8][4];  // prescribed increments (iteration-0 lift)
   int32_t info[kHexEB][8][2];                 // free-DOF mask, first free row
 };
+
+// The residual rows: in the residual-only loop they overlay the shape gradients (each
+// lane reads a node's gradients before writing that node's residual at the same place).
+template <int MODE>
+__device__ inline double* hexb_res_row(HexBLds<MODE>& L, int e, int g) {
+  if constexpr (MODE == 1) return L.b.gd[e][g];
+  else return L.a.R[e][g];
+}
 
 // Adds lane (i, j)'s block K to block form entry co (free rows and columns packed to the
 // front); old = its previous values (zero when this element stores the block).
@@ -923,7 +708,7 @@ struct HexGp {
   double w, tauP, A[3][3], vp[3];
 };
 
-// Phase G of lane (e, g, h), up to the residual written to L.a.R.
+// Phase G of lane (e, g, h), up to the residual written to the residual rows.
 template <int MODE>
 __device__ inline void hexb_gauss(HexBLds<MODE>& L, int e, int g, int h, double hlen, double mu,
                                   const ElemParams& p, HexGp& out) {
@@ -968,7 +753,7 @@ __device__ inline void hexb_gauss(HexBLds<MODE>& L, int e, int g, int h, double 
       const double g0 = d1[k] * i00 + d2v[k] * i01 + d3[k] * i02;
       const double g1 = d1[k] * i10 + d2v[k] * i11 + d3[k] * i12;
       const double g2 = d1[k] * i20 + d2v[k] * i21 + d3[k] * i22;
-      d2* dst = reinterpret_cast<d2*>(L.gd[e][g][k]);
+      d2* dst = reinterpret_cast<d2*>(&L.b.gd[e][g][4 * k]);
       dst[0] = d2{g0, g1};
       dst[1] = d2{g2, N[k]};
     }
@@ -983,7 +768,7 @@ __device__ inline void hexb_gauss(HexBLds<MODE>& L, int e, int g, int h, double 
     const double* sp = L.a.nd.S[0][e][n];
     const double* sc = L.a.nd.S[1][e][n];
     const double* sd = L.a.nd.S[2][e][n];
-    const double* gn = L.gd[e][g][n];
+    const double* gn = &L.b.gd[e][g][4 * n];
     const double Nn = gn[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -1049,12 +834,12 @@ __device__ inline void hexb_gauss(HexBLds<MODE>& L, int e, int g, int h, double 
 #pragma unroll
   for (int kk = 0; kk < 4; ++kk) {
     const int k = 2 * kk + h;
-    const d2* src = reinterpret_cast<const d2*>(L.gd[e][g][k]);
+    const d2* src = reinterpret_cast<const d2*>(&L.b.gd[e][g][4 * k]);
     const d2 ga = src[0], gb = src[1];
     double F[4];
 #pragma unroll
     for (int a = 0; a < 4; ++a) F[a] = -w * (gb.y * X[a] + ga.x * Y[a][0] + ga.y * Y[a][1] + gb.x * Y[a][2]);
-    d2* dst = reinterpret_cast<d2*>(L.a.R[e][g][k]);
+    d2* dst = reinterpret_cast<d2*>(hexb_res_row<MODE>(L, e, g) + 4 * k);
     dst[0] = d2{F[0], F[2]};
     dst[1] = d2{F[1], F[3]};
   }
@@ -1073,7 +858,7 @@ __device__ inline void hexb_records(HexBLds<MODE>& L, int e, int g, int h, doubl
 #pragma unroll
     for (int kk = 0; kk < 4; ++kk) {
       const int k = 2 * kk + h;
-      const d2* src = reinterpret_cast<const d2*>(L.gd[e][g][k]);
+      const d2* src = reinterpret_cast<const d2*>(&L.b.gd[e][g][4 * k]);
       const d2 ga = src[0], gb = src[1];
       const double dk[3] = {ga.x, ga.y, gb.x};
       double gk[3];
@@ -1081,81 +866,85 @@ __device__ inline void hexb_records(HexBLds<MODE>& L, int e, int g, int h, doubl
       for (int c = 0; c < 3; ++c)
         gk[c] = stp * ((dk[0] * q.w) * q.A[0][c] + (dk[1] * q.w) * q.A[1][c] + (dk[2] * q.w) * q.A[2][c]);
       const double ud = q.vp[0] * dk[0] + q.vp[1] * dk[1] + q.vp[2] * dk[2];
-      f4* r = reinterpret_cast<f4*>(L.a.k.rec[e][g][k]);
+      f4* r = reinterpret_cast<f4*>(&L.b.rec[e][g][8 * k]);
       r[0] = f4{(float)dk[0], (float)dk[1], (float)dk[2], (float)gb.y};
       r[1] = f4{(float)gk[0], (float)gk[1], (float)gk[2], (float)(afr * ud)};
     }
   }
   if (h == 0) {
     T* c = L.a.k.cg[e][g];
-    c[0] = (T)q.w;
-    c[1] = (T)stp;
+    const double w = q.w;
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
-      for (int b = 0; b < 3; ++b) c[2 + 3 * a + b] = (T)q.A[a][b];
-    c[11] = (T)(mu * p.af);
+      for (int b = 0; b < 3; ++b) c[3 * a + b] = (T)(w * q.A[a][b]);
+    c[9] = (T)(w * (p.rho * p.acceFact));
+    c[10] = (T)(w * (mu * p.af));
+    c[11] = (T)(w * p.af);
+    c[12] = (T)(w * (SGN * p.af));
+    c[13] = (T)(w * stp);
+    c[14] = (T)(w * stp * p.af);
+    c[15] = (T)w;
     if constexpr (MODE == 0) {
 #pragma unroll
-      for (int b = 0; b < 3; ++b) c[12 + b] = afr * q.vp[b];
+      for (int b = 0; b < 3; ++b) c[16 + b] = afr * q.vp[b];
+      c[19] = stp;
     }
   }
 }
 
 // Phase K: block (i, j) of element e summed over the Gauss points from the records.
+// With p = Ni Nj, d = dNi.dNj and the weight folded into the coefficients c:
+//   K_uu(a,b) += p wA_ab + delta_ab [w rho td8 p + w mu af d + w Ni u_j]
+//   K_up(a)   -= (w af Nj) dNi_a
+//   K_pu(b)   += (w s af Ni) dNj_b + Nj g_i(b) + (w s tauP d_j) dNi_b
+//   K_pp      += w s tauP af d
+// with d_j = u_j + rho td8 Nj (accumulate_block's terms, regrouped).
 template <int MODE>
 __device__ inline void hexb_block(const HexBLds<MODE>& L, int e, int i, int j, const ElemParams& p,
                                   typename HexBCfg<MODE>::T (&K)[4][4]) {
   using T = typename HexBCfg<MODE>::T;
-  constexpr double SGN = Traits<HEXA8>::SGN;
-  const T af = (T)p.af, cst = (T)(p.rho * p.acceFact), saf = (T)(SGN * p.af);
-  const T muaf = L.a.k.cg[e][0][11];
+  const T cst = (T)(p.rho * p.acceFact);
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) K[a][b] = (T)0;
 #pragma unroll
   for (int g = 0; g < 8; ++g) {
-    const T* q = L.a.k.cg[e][g];
-    const T w = q[0], stp = q[1];
+    const T* c = L.a.k.cg[e][g];
     T dNi[3], dNj[3], Ni, Nj, gi[3], uj;
     if constexpr (MODE == 2) {
-      const float* ri = L.a.k.rec[e][g][i];
-      const float* rj = L.a.k.rec[e][g][j];
+      const float* ri = &L.b.rec[e][g][8 * i];
+      const float* rj = &L.b.rec[e][g][8 * j];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) { dNi[c] = ri[c]; dNj[c] = rj[c]; gi[c] = ri[4 + c]; }
+      for (int x = 0; x < 3; ++x) { dNi[x] = ri[x]; dNj[x] = rj[x]; gi[x] = ri[4 + x]; }
       Ni = ri[3]; Nj = rj[3];
       uj = rj[7];
     } else {
-      const double* ri = L.gd[e][g][i];
-      const double* rj = L.gd[e][g][j];
+      const double* ri = &L.b.gd[e][g][4 * i];
+      const double* rj = &L.b.gd[e][g][4 * j];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) { dNi[c] = ri[c]; dNj[c] = rj[c]; }
+      for (int x = 0; x < 3; ++x) { dNi[x] = ri[x]; dNj[x] = rj[x]; }
       Ni = ri[3]; Nj = rj[3];
-    }
-    T wdNi[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) wdNi[c] = w * dNi[c];
-    if constexpr (MODE != 2) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) gi[c] = stp * (wdNi[0] * q[2 + c] + wdNi[1] * q[5 + c] + wdNi[2] * q[8 + c]);
-      uj = q[12] * dNj[0] + q[13] * dNj[1] + q[14] * dNj[2];
+      for (int b = 0; b < 3; ++b) gi[b] = c[19] * (dNi[0] * c[b] + dNi[1] * c[3 + b] + dNi[2] * c[6 + b]);
+      uj = c[16] * dNj[0] + c[17] * dNj[1] + c[18] * dNj[2];
     }
-    const T wNi = w * Ni;
-    const T s1 = wNi * Nj;
-    const T dd = wdNi[0] * dNj[0] + wdNi[1] * dNj[1] + wdNi[2] * dNj[2];
+    const T pp = Ni * Nj;
+    const T dd = dNi[0] * dNj[0] + dNi[1] * dNj[1] + dNi[2] * dNj[2];
     const T dj = uj + cst * Nj;
-    const T s2 = cst * s1 + muaf * dd + wNi * uj;
+    const T s2 = c[9] * pp + c[10] * dd + (c[15] * Ni) * uj;
+    const T c1 = c[11] * Nj, c2 = c[12] * Ni, c3 = c[13] * dj;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
 #pragma unroll
-      for (int b = 0; b < 3; ++b) K[a][b] += s1 * q[2 + 3 * a + b];
+      for (int b = 0; b < 3; ++b) K[a][b] += pp * c[3 * a + b];
       K[a][a] += s2;
-      K[a][3] -= af * wdNi[a] * Nj;
+      K[a][3] -= c1 * dNi[a];
     }
 #pragma unroll
-    for (int b = 0; b < 3; ++b) K[3][b] += saf * wNi * dNj[b] + Nj * gi[b] + stp * wdNi[b] * dj;
-    K[3][3] += stp * af * dd;
+    for (int b = 0; b < 3; ++b) K[3][b] += c2 * dNj[b] + Nj * gi[b] + c3 * dNi[b];
+    K[3][3] += c[14] * dd;
     __builtin_amdgcn_sched_barrier(0);  // one Gauss point's LDS operands in flight at a time
   }
 }
@@ -1244,9 +1033,9 @@ __global__ __launch_bounds__(64, HexBCfg<MODE>::occ) void k_hex8_batch(AsmView v
     double fsum[2];
     {
       typedef double d2 __attribute__((ext_vector_type(2)));
-      d2 s = *reinterpret_cast<const d2*>(&L.a.R[se][0][sk][2 * sc]);
+      d2 s = *reinterpret_cast<const d2*>(hexb_res_row<MODE>(L, se, 0) + 4 * sk + 2 * sc);
 #pragma unroll
-      for (int g = 1; g < 8; ++g) s += *reinterpret_cast<const d2*>(&L.a.R[se][g][sk][2 * sc]);
+      for (int g = 1; g < 8; ++g) s += *reinterpret_cast<const d2*>(hexb_res_row<MODE>(L, se, g) + 4 * sk + 2 * sc);
       fsum[0] = s.x;
       fsum[1] = s.y;
     }
@@ -1263,11 +1052,40 @@ __global__ __launch_bounds__(64, HexBCfg<MODE>::occ) void k_hex8_batch(AsmView v
     }
     issue_nodal();  // (the last batch reloads its own data, unused)
     if constexpr (Cfg::KM) {
-      wave_sync();  // the residual sums have read L.a.R (the records overlay it)
+      T* kbv = MODE == 2 ? reinterpret_cast<T*>(v.kb32) : reinterpret_cast<T*>(v.kb);
+      typedef T blk_t __attribute__((ext_vector_type(4)));
+      // Block values an element adds to, loaded one element ahead (the first element's
+      // while the records are formed).  Lanes that add nothing, and lanes whose element
+      // stores the block (bit 30), read block 0 instead (always present, and one cached
+      // line for all of them); the last element's reload past the batch end is not used.
+      auto blk_load = [&](int e, blk_t (&o)[4]) {
+        int lk = l0;
+        asm volatile("" : "+v"(lk));
+        const int32_t coe = e == 0 ? co_cur[0] : e == 1 ? co_cur[1] : e == 2 ? co_cur[2] : co_cur[3];
+        const bool keep = L.info[e][lk >> 3][0] != 0 && L.info[e][lk & 7][0] != 0 && !((coe >> 30) & 1);
+        const blk_t* bq = reinterpret_cast<const blk_t*>(kbv + (int64_t)(keep ? (coe & ((1 << 30) - 1)) : 0) * 16);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) o[a] = bq[a];
+      };
+      // (the FP64 loop has no registers to spare for the look-ahead: it loads each
+      // element's block values as it starts on the element)
+      // The element loop is unrolled, the two look-ahead buffers used in turn (a copy
+      // from one to the other at the loop's end would wait for the load just issued).
+      constexpr bool ahead = MODE == 2;
+      blk_t ob[2][4];
+      if constexpr (ahead) blk_load(0, ob[0]);
+      wave_sync();  // the residual sums have read the residual rows (the coefficients overlay them)
       hexb_records<MODE>(L, se, sk, sc, mu_cur, p, gq);
       wave_sync();
-      T* kbv = MODE == 2 ? reinterpret_cast<T*>(v.kb32) : reinterpret_cast<T*>(v.kb);
-      for (int e = 0; e < nv; ++e) {
+#pragma unroll
+      for (int e = 0; e < kHexEB; ++e) {
+        if (e >= nv) break;
+        blk_t (&ocur)[4] = ob[e & 1];
+        if constexpr (ahead) {
+          if (e + 1 < kHexEB) blk_load(e + 1 < nv ? e + 1 : e, ob[(e + 1) & 1]);
+        } else {
+          blk_load(e, ocur);
+        }
         // (re-derived per element, so that the per-lane LDS addresses are not hoisted)
         int lk = l0;
         asm volatile("" : "+v"(lk));
@@ -1275,17 +1093,7 @@ __global__ __launch_bounds__(64, HexBCfg<MODE>::occ) void k_hex8_batch(AsmView v
         const int32_t coe = e == 0 ? co_cur[0] : e == 1 ? co_cur[1] : e == 2 ? co_cur[2] : co_cur[3];
         const int mi = L.info[e][i][0], mj = L.info[e][j][0];
         const bool blk = mi != 0 && mj != 0;
-        const bool first = (coe >> 30) & 1;
-        typedef T blk_t __attribute__((ext_vector_type(4)));
-        const blk_t* bq = reinterpret_cast<const blk_t*>(kbv + (int64_t)(blk ? (coe & ((1 << 30) - 1)) : 0) * 16);
-        const bool keep = blk && !first;
-        T old[4][4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-          const blk_t o = bq[a];  // block 0 where this lane adds nothing (always present)
-          old[a][0] = keep ? o.x : (T)0; old[a][1] = keep ? o.y : (T)0;
-          old[a][2] = keep ? o.z : (T)0; old[a][3] = keep ? o.w : (T)0;
-        }
+        const bool keep = blk && !((coe >> 30) & 1);  // bit 30: this element stores the block
         T K[4][4];
         hexb_block<MODE>(L, e, i, j, p, K);
         // the block is formed here, before the branches that use it (sunk into the store
@@ -1294,6 +1102,12 @@ __global__ __launch_bounds__(64, HexBCfg<MODE>::occ) void k_hex8_batch(AsmView v
         for (int a = 0; a < 4; ++a)
 #pragma unroll
           for (int bb = 0; bb < 4; ++bb) asm volatile("" : "+v"(K[a][bb]));
+        T old[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          old[a][0] = keep ? ocur[a].x : (T)0; old[a][1] = keep ? ocur[a].y : (T)0;
+          old[a][2] = keep ? ocur[a].z : (T)0; old[a][3] = keep ? ocur[a].w : (T)0;
+        }
         if (doLift) {
           double ls[4];
 #pragma unroll
@@ -1402,26 +1216,6 @@ static hipError_t assemble_t(hipStream_t s, const AsmView& v, const ElemParams& 
   return assemble_m<ET, 2>(s, v, p, elems, n, lift, probe_e, variant);
 }
 
-// Workgroups of the pipelined Hexa8 kernel the current device holds at once (each walks
-// its share of a colour class).
-template <int MODE>
-static hipError_t hex8_launch(hipStream_t s, const AsmView& v, const ElemParams& p, const int32_t* elems,
-                              int64_t n, int lift) {
-  static int64_t resident[16];
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) dev = 0;
-  if (resident[dev] == 0) {
-    int cus = 0, per = 0;
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_assemble_hex8<MODE>, 64, 0) != hipSuccess || per < 1)
-      per = 4 * HexCfg<MODE>::occ;
-    resident[dev] = (int64_t)(cus > 0 ? cus : 256) * per;
-  }
-  const unsigned grid = (unsigned)std::min<int64_t>(n, resident[dev]);
-  hipLaunchKernelGGL((k_assemble_hex8<MODE>), dim3(grid), dim3(64), 0, s, v, p, elems, n, lift);
-  return hipGetLastError();
-}
-
 // Workgroups of the batched Hexa8 kernel the current device holds at once.
 template <int MODE>
 static hipError_t hex8_batch_launch(hipStream_t s, const AsmView& v, const ElemParams& p, const int32_t* elems,
@@ -1442,20 +1236,10 @@ static hipError_t hex8_batch_launch(hipStream_t s, const AsmView& v, const ElemP
   return hipGetLastError();
 }
 
-static bool hex8_use_pipelined() {
-  const char* e = getenv("SFEM_HEX_KERNEL");
-  return e && strcmp(e, "pipelined") == 0;
-}
-
 hipError_t launch_assemble(int et, hipStream_t s, const AsmView& v, const ElemParams& p,
                            const int32_t* elems, int64_t n, int mode, int lift, int64_t probe_e, int variant) {
   if (et == HEXA8 && probe_e < 0) {
     if (n == 0) return hipSuccess;
-    if (hex8_use_pipelined()) {
-      if (mode == 0) return hex8_launch<0>(s, v, p, elems, n, lift);
-      if (mode == 1) return hex8_launch<1>(s, v, p, elems, n, lift);
-      return hex8_launch<2>(s, v, p, elems, n, lift);
-    }
     if (mode == 0) return hex8_batch_launch<0>(s, v, p, elems, n, lift);
     if (mode == 1) return hex8_batch_launch<1>(s, v, p, elems, n, lift);
     return hex8_batch_launch<2>(s, v, p, elems, n, lift);
