@@ -10,6 +10,9 @@
 //
 // Parity pinning (what fixes each function independently of this text):
 //   orc_time_params   closed forms of the generalised-alpha / BDF coefficients
+//   orc_update_iter_step  the generalised-alpha update in its trapezoidal form
+//                     (rho_inf 0, 0.5, 1); BDF1-4 exact on polynomials of their order
+//   orc_time_update   the history shift, entry by entry
 //   orc_gauss         literal tables (incl. the 0.577350269189626 literal)
 //   orc_shape         partition of unity, sum dN = 0, exact linear reproduction,
 //                     and the Hexa8 derivative table checked entry by entry

@@ -41,6 +41,77 @@ def test_time_params_closed_forms():
     assert abs(td[9:14].sum()) < 1e-12
 
 
+def _iter_step(tis, rho_inf, dt, soln, prev, prev2, prev3, prev4, dot_prev):
+    td = oracle.time_params(tis, rho_inf, dt)
+    n = soln.size
+    dot, cur, dot_cur = np.zeros(n), np.zeros(n), np.zeros(n)
+    oracle.lib().orc_update_iter_step(n, td, soln, prev, prev2, prev3, prev4, dot, dot_prev, cur, dot_cur)
+    return dot, cur, dot_cur
+
+
+def test_update_iter_step_generalised_alpha():
+    """SolutionData::updateIterStep (SolutionData.cpp:83-91) for gen-alpha against the
+    Jansen-Whiting-Hulbert update written in its trapezoidal form,
+    u_{n+1} = u_n + dt ((1 - gamma) udot_n + gamma udot_{n+1}),
+    u_{n+af} = u_n + af (u_{n+1} - u_n), udot_{n+am} = udot_n + am (udot_{n+1} - udot_n)."""
+    rng = np.random.default_rng(7)
+    n, dt = 64, 0.1
+    soln, prev, dot_prev = (rng.standard_normal(n) for _ in range(3))
+    junk = [rng.standard_normal(n) for _ in range(3)]  # older history must not enter
+    for rho in (0.0, 0.5, 1.0):
+        am, af = 0.5 * (3 - rho) / (1 + rho), 1 / (1 + rho)
+        gam = 0.5 + am - af
+        dot, cur, dot_cur = _iter_step(2, rho, dt, soln, prev, *junk, dot_prev)
+        dot_ref = ((soln - prev) / dt - (1 - gam) * dot_prev) / gam
+        assert np.allclose(dot, dot_ref, rtol=1e-13, atol=1e-13)
+        assert np.allclose(prev + dt * ((1 - gam) * dot_prev + gam * dot), soln, rtol=1e-13, atol=1e-13)
+        assert np.allclose(cur, prev + af * (soln - prev), rtol=1e-14, atol=1e-14)
+        assert np.allclose(dot_cur, dot_prev + am * (dot_ref - dot_prev), rtol=1e-13, atol=1e-13)
+    # rho_inf = 0 (the configs' setting): af = 1, am = 3/2, gamma = 1
+    dot, cur, dot_cur = _iter_step(2, 0.0, dt, soln, prev, *junk, dot_prev)
+    assert np.allclose(dot, (soln - prev) / dt, rtol=1e-14, atol=1e-14)
+    assert np.array_equal(cur, soln)
+    assert np.allclose(dot_cur, 1.5 * dot - 0.5 * dot_prev, rtol=1e-14, atol=1e-14)
+
+
+@pytest.mark.parametrize("tis,order", [(3, 1), (5, 2), (6, 3), (7, 4)])
+def test_update_iter_step_bdf_exact_on_polynomials(tis, order):
+    """BDF-k differentiates a polynomial of degree k in time exactly: with the history
+    u(t_{n+1}), u(t_n), ..., the updated velocity is u'(t_{n+1}) for every node; the stage
+    values are the new state (af = am = 1)."""
+    rng = np.random.default_rng(tis)
+    n, dt, t1 = 32, 0.05, 1.3
+    c = rng.standard_normal((order + 1, n))
+    u = lambda t: sum(c[k] * t ** k for k in range(order + 1))
+    du = lambda t: sum(k * c[k] * t ** (k - 1) for k in range(1, order + 1))
+    hist = [u(t1 - m * dt) for m in range(5)]  # soln, prev, prev2, prev3, prev4
+    dot_prev = rng.standard_normal(n)          # BDF ignores the old velocity
+    dot, cur, dot_cur = _iter_step(tis, 0.0, dt, *hist, dot_prev)
+    assert np.allclose(dot, du(t1), rtol=1e-10, atol=1e-10)
+    assert np.array_equal(cur, hist[0]) and np.array_equal(dot_cur, dot)
+    # one degree higher is no longer exact (the history weights are the order-k ones)
+    c2 = np.vstack([c, rng.standard_normal((1, n))])
+    u2 = lambda t: sum(c2[k] * t ** k for k in range(order + 2))
+    du2 = lambda t: sum(k * c2[k] * t ** (k - 1) for k in range(1, order + 2))
+    dot2, _, _ = _iter_step(tis, 0.0, dt, *[u2(t1 - m * dt) for m in range(5)], dot_prev)
+    assert np.max(np.abs(dot2 - du2(t1))) > 1e-6
+
+
+def test_time_update_shifts_history():
+    """SolutionData::timeUpdate (SolutionData.cpp:64-78): prev4 <- prev3 <- prev2 <- prev <- soln,
+    dotPrev <- dot; soln and dot are left as they are."""
+    rng = np.random.default_rng(3)
+    n = 16
+    a = [rng.standard_normal(n) for _ in range(7)]  # soln, prev, prev2, prev3, prev4, dot, dotPrev
+    b = [x.copy() for x in a]
+    oracle.lib().orc_time_update(n, *b)
+    soln, prev, prev2, prev3, prev4, dot, dot_prev = b
+    assert np.array_equal(soln, a[0]) and np.array_equal(dot, a[5])
+    assert np.array_equal(prev, a[0]) and np.array_equal(prev2, a[1])
+    assert np.array_equal(prev3, a[2]) and np.array_equal(prev4, a[3])
+    assert np.array_equal(dot_prev, a[5])
+
+
 def test_gauss_tables():
     gp, gw = oracle.gauss(oracle.HEXA8)
     g = 0.577350269189626
