@@ -7,15 +7,20 @@ x_i = 0.5 (1 - cos(pi i / 115)), rho = 1, mu = 1e-3, lid u = 1 on z = 1 ramped b
 min(t, 1), p pinned at (0, 0, 0), generalised-alpha rho_inf = 0, dt = 0.1.
 ``c2`` is the same at 32^3.
 
-Two runs of K time steps from the zero state, each after W warm-up steps (the
+Three runs of K time steps from the zero state, each after W warm-up steps (the
 state is reset after warm-up):
-  * FP64: FP64 tangent and CSR, BiCGStab + ILU(0) at rtol 1e-8;
+  * FP64 plain: the reference's solver configuration, FP64 tangent and CSR, BiCGStab +
+    ILU(0) at rtol 1e-8, the factor rebuilt for every new tangent;
+  * FP64: the same with the solver options of the mixed run: ILU(0) plus the additive
+    coarse pressure correction on 8^3 boxes, the factor rebuilt on every 6th tangent;
   * mixed: FP32 element tangents and FP32 CSR, FP32 BiCGStab + ILU(0) (factor and
-    tile inverses stored in BF16) inside an FP64 residual / Newton update loop
-    (defect correction) to Newton 1e-9; each FP32 solve is asked for
-    max(1e-5, 0.5 * 1e-9 / ||F||) (inexact Newton).
+    tile inverses stored in BF16) with the coarse correction without its global
+    constant mode, inside an FP64 residual / Newton update loop (defect correction) to
+    Newton 1e-9 whose every iteration first takes the FP64 coarse step
+    x += Z E^-1 Z^T F(x); each FP32 solve is asked for
+    min(0.5, max(4e-5, 0.8 * 1e-9 / ||F||)) (inexact Newton).
 ``value`` is the mixed run's time steps per second; the JSON line also carries
-the FP64 rate, the speed-up, the relative L2 error of u and p (mixed vs FP64),
+both FP64 rates, the speed-ups, the relative L2 error of u and p (mixed vs FP64),
 per-kernel-class device times measured with HIP events on the library's stream,
 the roofline of the dominant kernel and a CPU baseline.
 """
@@ -58,7 +63,7 @@ def make_case(n):
 
 
 def run_path(mesh, ctl, prec, opts, newton_tol, steps, warmup, label, predictor=0, pc_lag=1, inner=(0.0, 0.5),
-             tuning=0):
+             tuning=0, coarse=0):
     from cfdsfemmpi_amd import sfem
     t0 = time.time()
     fem = sfem.StabFEM(device=0)
@@ -66,6 +71,7 @@ def run_path(mesh, ctl, prec, opts, newton_tol, steps, warmup, label, predictor=
     fem.setControl(ctl)
     fem.precision = prec
     fem.tuning = tuning
+    fem.coarse_boxes = coarse
     fem.prepareInputData()
     fem.setSolver(opts)
     ctx = fem.ctx
@@ -116,7 +122,7 @@ def run_path(mesh, ctl, prec, opts, newton_tol, steps, warmup, label, predictor=
     asm_pass = {"K_F_ms": pass_ms(True), "F_ms": pass_ms(False)}
     res = dict(label=label, seconds=t2 - t1, steps_per_s=steps / (t2 - t1), hist=hist, kt=kt, info=info,
                soln=s, tune=tune, block=ctx.block_info(), tiles=ctx.tile_info(), stats=ctx.solver_stats(),
-               asm_pass=asm_pass)
+               asm_pass=asm_pass, coarse=ctx.coarse_info())
     ctx.close()
     return res
 
@@ -615,10 +621,15 @@ def main():
     ap.add_argument("--inner-rtol-max", type=float, default=0.5, help="mixed path: cap of the inexact-Newton rtol")
     ap.add_argument("--tuning", choices=["timed", "fixed"], default="timed",
                     help="kernel shapes: timed autotuner (default) or fixed default shapes (reproducible FP32)")
-    ap.add_argument("--pc-lag", type=int, default=4,
+    ap.add_argument("--pc-lag", type=int, default=6,
                     help="both paths: rebuild the ILU(0) factor on every n-th new tangent only (1: every tangent, "
                          "as the PETSc KSP of the reference does)")
     ap.add_argument("--ksp", choices=["bcgs", "gmres"], default="bcgs", help="Krylov method of both paths")
+    ap.add_argument("--coarse", type=int, default=8,
+                    help="fp64 and mixed paths: additive coarse pressure correction of ILU(0) on n^3 boxes (0: off)")
+    ap.add_argument("--plain", type=int, default=1,
+                    help="also time the FP64 path with the reference's solver configuration (ILU(0) alone, "
+                         "rebuilt for every tangent)")
     ap.add_argument("--restart", type=int, default=30, help="GMRES restart length")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run 5 device-to-device copies of this many MiB (HBM counter calibration)")
@@ -650,14 +661,20 @@ def main():
                            factor_bf16=a.bf16_factors)
     pred = sfem.PRED_EXTRAPOLATE if a.predictor == "extrapolate" else sfem.PRED_PREVIOUS
     tune_mode = sfem.TUNE_FIXED if a.tuning == "fixed" else sfem.TUNE_TIMED
-    r64 = rmx = None
+    r64 = rmx = rpl = None
+    if a.only is None and a.plain:
+        # the reference's solver configuration: BiCGStab + ILU(0) alone, the factor rebuilt
+        # for every new tangent (PCSetUp after each assembly)
+        rpl = run_path(mesh, ctl, sfem.PREC_FP64, o64, a.newton_tol, a.steps, a.warmup, "fp64_plain", pred, 1,
+                       tuning=tune_mode, coarse=0)
+        log(f"fp64_plain: {rpl['steps_per_s']:.3f} steps/s")
     if a.only != "mixed":
         r64 = run_path(mesh, ctl, sfem.PREC_FP64, o64, a.newton_tol, a.steps, a.warmup, "fp64", pred, a.pc_lag,
-                       tuning=tune_mode)
+                       tuning=tune_mode, coarse=a.coarse)
         log(f"fp64: {r64['steps_per_s']:.3f} steps/s")
     if a.only != "fp64":
         rmx = run_path(mesh, ctl, sfem.PREC_MIXED, omx, a.newton_tol_mixed, a.steps, a.warmup, "mixed", pred,
-                       a.pc_lag, (a.inner_theta, a.inner_rtol_max), tune_mode)
+                       a.pc_lag, (a.inner_theta, a.inner_rtol_max), tune_mode, a.coarse)
         log(f"mixed: {rmx['steps_per_s']:.3f} steps/s")
     main_res = rmx or r64
     err_u = err_p = None
@@ -667,6 +684,13 @@ def main():
         err_u = float(np.linalg.norm(smx[:, :3] - s64[:, :3]) / np.linalg.norm(s64[:, :3]))
         err_p = float(np.linalg.norm(smx[:, 3] - s64[:, 3]) / np.linalg.norm(s64[:, 3]))
         log(f"rel L2 error mixed vs fp64: u {err_u:.3e} p {err_p:.3e}")
+    err_u_pl = err_p_pl = None
+    if rpl and rmx:
+        spl = rpl["soln"].reshape(-1, 4)
+        smx = rmx["soln"].reshape(-1, 4)
+        err_u_pl = float(np.linalg.norm(smx[:, :3] - spl[:, :3]) / np.linalg.norm(spl[:, :3]))
+        err_p_pl = float(np.linalg.norm(smx[:, 3] - spl[:, 3]) / np.linalg.norm(spl[:, 3]))
+        log(f"rel L2 error mixed vs plain fp64: u {err_u_pl:.3e} p {err_p_pl:.3e}")
     # Roofline of the dominant single kernel of the reported run.  The SpMV is one launch
     # per product; the ILU(0) apply is a group of 2 x (colours) launches, none of which
     # outweighs the SpMV, and the element loop is one launch per colour class.
@@ -674,7 +698,9 @@ def main():
     info = main_res["info"]
     nnz, nf = info["nnz"], info["nfree"]
     fp32 = main_res is rmx
-    sv = 4 if fp32 else 8
+    sm = 4 if fp32 else 8                      # matrix values
+    sv = 4 if (fp32 and not a.coarse) else 8   # Krylov vectors (FP64 under the coarse correction)
+    vt = "float" if sv == 4 else "double"
 
     def hbm_roof(kind, name, bytes_per):
         cnt, ms = kt[kind]
@@ -689,17 +715,17 @@ def main():
     # descriptor per block row, x once and y once.  The CSR figure nnz*(s+4) + 4(n+1)
     # + 2 s n is reported beside it.
     bi = main_res["block"]
-    spmv_bytes = nnz * sv + 4 * bi["npairs"] + 16 * bi["nblk"] + 2 * sv * nf
-    roof = hbm_roof("spmv", f"k_bsr_spmv<{'float' if fp32 else 'double'}> (node-block SpMV)", spmv_bytes)
-    roof["csr_equivalent_bytes_per_launch"] = nnz * (sv + 4) + 4 * (nf + 1) + 2 * sv * nf
-    roof["traffic"] = MEASURED_TRAFFIC.get(("k_bsr_spmv", "float" if fp32 else "double", n))
+    spmv_bytes = nnz * sm + 4 * bi["npairs"] + 16 * bi["nblk"] + 2 * sv * nf
+    roof = hbm_roof("spmv", f"k_bsr_spmv<{'float' if fp32 else 'double'}, {vt}> (node-block SpMV)", spmv_bytes)
+    roof["csr_equivalent_bytes_per_launch"] = nnz * (sm + 4) + 4 * (nf + 1) + 2 * sv * nf
+    roof["traffic"] = MEASURED_TRAFFIC.get(("k_bsr_spmv", "float" if fp32 else "double", n)) if sv == sm else None
     roof["traffic_source"] = TRAFFIC_SOURCE if roof["traffic"] else None
     # ILU(0) apply: both sweeps together read every factor value, block index and
     # descriptor once; r read, y written, y read and written, plus the gathered y once
     # per sweep (5 vector passes).  With the tiled ordering the values inside the tiles are
     # replaced by the tiles' packed inverse triangles (2 x 528 values per tile, stored in
     # the factor's type).
-    lu_s = 2 if (fp32 and a.bf16_factors) else sv
+    lu_s = 2 if (fp32 and a.bf16_factors) else sm
     ti = main_res["tiles"]
     pc_bytes = nnz * lu_s + 4 * bi["npairs"] + 32 * bi["nblk"] + 5 * sv * nf
     pc_name = "ILU(0) apply (2 x colours launches)"
@@ -708,6 +734,12 @@ def main():
         pc_bytes += ti["ntiles"] * 2 * 528 * inv_s - ti["tile_vals"] * lu_s
         pc_name = (f"ILU(0) apply, tiled: k_bsr_tile_lsolve/usolve, {2 * ti['ncolors'] - 1} launches, "
                    f"{ti['ntiles']} tiles")
+    co = main_res["coarse"]
+    if co["built"]:
+        # coarse correction: r read, z read and written at the pressure slots, their slot
+        # indices read twice
+        pc_bytes += co["pressure_dofs"] * (3 * sv + 8)
+        pc_name += f", + coarse pressure correction ({co['unknowns']} unknowns, 3 launches)"
     roof_pc = hbm_roof("pc", pc_name, pc_bytes)
     roof_pc["tiles"] = ti
     n_k = len(main_res["hist"])
@@ -751,7 +783,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("fp32 tangent+Krylov" + (" (bf16 ILU factors)" if a.bf16_factors else "") + " / fp64 residual")
+        "dtype": (("fp32 tangent and matrix" + (", bf16 ILU factors" if a.bf16_factors else "") +
+                   ", fp64 Krylov vectors (coarse correction) / fp64 residual") if a.coarse else
+                  ("fp32 tangent+Krylov" + (" (bf16 ILU factors)" if a.bf16_factors else "") + " / fp64 residual"))
                  if fp32 else "f64",
         "data": "synthetic (generated mesh, zero initial state)",
         "config": {"workload": f"3D lid-driven cavity, {n}^3 graded Hexa8, Re=1000, gen-alpha rho_inf=0, "
@@ -761,19 +795,27 @@ def main():
                    "inner_rtol_mixed": a.inner_rtol, "ilu_factors_mixed": "bf16" if a.bf16_factors else "fp32",
                    "newton_initial_iterate": a.predictor, "pc_lag": a.pc_lag,
                    "inner_theta_mixed": a.inner_theta, "inner_rtol_max_mixed": a.inner_rtol_max,
-                   "kernel_shapes": a.tuning, "parallelism": "single GPU"},
+                   "kernel_shapes": a.tuning, "coarse_pressure_boxes": a.coarse,
+                   "coarse_unknowns": main_res["coarse"]["unknowns"], "parallelism": "single GPU"},
+        # fp64: the FP64 path with the same solver options as the mixed run (ILU(0), coarse
+        # pressure correction, preconditioner lag); fp64_plain: the reference's solver
+        # configuration (ILU(0) alone, rebuilt for every tangent)
         "fp64_steps_per_s": r64["steps_per_s"] if r64 else None,
+        "fp64_plain_steps_per_s": rpl["steps_per_s"] if rpl else None,
         "mixed_steps_per_s": rmx["steps_per_s"] if rmx else None,
         "speedup_mixed_vs_fp64": (rmx["steps_per_s"] / r64["steps_per_s"]) if (r64 and rmx) else None,
+        "speedup_mixed_vs_fp64_plain": (rmx["steps_per_s"] / rpl["steps_per_s"]) if (rpl and rmx) else None,
         # the gate is on velocity (rel. L2 of u vs the FP64 run); pressure is reported beside it
         "error_gate_1e-6_passed": (err_u <= 1e-6) if err_u is not None else None,
         "rel_l2_u_mixed_vs_fp64": err_u,
         "rel_l2_p_mixed_vs_fp64": err_p,
-        "krylov_iters": {r["label"]: [h[1] for h in r["hist"]] for r in (r64, rmx) if r},
-        "newton_iters": {r["label"]: [h[0] for h in r["hist"]] for r in (r64, rmx) if r},
+        "rel_l2_u_mixed_vs_fp64_plain": err_u_pl,
+        "rel_l2_p_mixed_vs_fp64_plain": err_p_pl,
+        "krylov_iters": {r["label"]: [h[1] for h in r["hist"]] for r in (rpl, r64, rmx) if r},
+        "newton_iters": {r["label"]: [h[0] for h in r["hist"]] for r in (rpl, r64, rmx) if r},
         "kernel_ms": {r["label"]: {k: {"launches": v[0], "total_ms": v[1]} for k, v in r["kt"].items()}
-                      for r in (r64, rmx) if r},
-        "autotuned": {r["label"]: r["tune"] for r in (r64, rmx) if r},
+                      for r in (rpl, r64, rmx) if r},
+        "autotuned": {r["label"]: r["tune"] for r in (rpl, r64, rmx) if r},
         "fp32_gmres_fallbacks": rmx["stats"][0] if rmx else None,
         "fp32_partial_solves": rmx["stats"][1] if rmx else None,
         "block_form": main_res["block"],

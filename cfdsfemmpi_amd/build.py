@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libsfem_hip.so")
 BUILD = os.path.join(HERE, "_build")
 SOURCES = ["sfem_elements.hip", "sfem_linalg.hip", "sfem_direct.hip", "sfem_state.hip", "sfem_batch.hip",
-           "sfem_learned.hip", "sfem_api.cpp"]
+           "sfem_learned.hip", "sfem_coarse.hip", "sfem_api.cpp"]
 ARCH = os.environ.get("SFEM_OFFLOAD_ARCH", "gfx950")
 
 

@@ -618,6 +618,36 @@ int sfem_set_pc_lag(sfem_ctx* c, int lag) {
   return SFEM_OK;
 }
 
+int sfem_set_coarse_correction(sfem_ctx* c, int k) {
+  if (!c) return set_error(SFEM_ERR_ARG, "null context");
+  if (!c->la) return set_error(SFEM_ERR_STATE, "build the pattern first");
+  if (c->batch && k) return set_error(SFEM_ERR_UNSUPPORTED, "coarse correction: single meshes only");
+  if (k < 0) return set_error(SFEM_ERR_ARG, "boxes per direction must be >= 0");
+  int rc = sfem::coarse_set(c, k);
+  if (rc) return rc;
+  c->la->pc_K_version = -1;  // the next solve builds the preconditioner again
+  return SFEM_OK;
+}
+
+int sfem_coarse_info(sfem_ctx* c, int64_t* info) {
+  if (!c || !info) return set_error(SFEM_ERR_ARG, "null argument");
+  if (!c->la) return set_error(SFEM_ERR_STATE, "build the pattern first");
+  return sfem::coarse_info(c, info);
+}
+
+int sfem_coarse_inverse(sfem_ctx* c, int which, double* out, int cap) {
+  if (!c || !out || (which != 0 && which != 1)) return set_error(SFEM_ERR_ARG, "bad argument");
+  if (!c->la) return set_error(SFEM_ERR_STATE, "build the pattern first");
+  return sfem::coarse_get_inverse(c, which, out, cap);
+}
+
+int sfem_coarse_step(sfem_ctx* c, const sfem_solver_opts* o) {
+  if (!c || !o) return set_error(SFEM_ERR_ARG, "null argument");
+  if (!(c->status & ST_ASSEMBLED)) return set_error(SFEM_ERR_STATE, "assemble the system first");
+  HIP_OK(hipSetDevice(c->device));
+  return sfem::la_coarse_step(c, o);
+}
+
 static int run_element_loop(sfem_ctx* c, int mode, int lift, int variant = -1, bool timed = true);
 
 // ||F_free|| at the state x with the prescribed values of this step in place (x is
@@ -912,6 +942,16 @@ int sfem_step(sfem_ctx* c, double timeNow, const sfem_solver_opts* o, double new
     st.newton_iters = iter + 1;
     if (!std::isfinite(nrm)) { if (stats) *stats = st; return set_error(SFEM_ERR_DIVERGED, "non-finite residual"); }
     if (nrm < newton_tol) { conv = true; break; }
+    if (c->precision == SFEM_PREC_MIXED && c->la->coarse_k > 0 && o->pc_type == SFEM_PC_ILU0) {
+      // the coarse step in FP64 (x += Z E^-1 Z^T F), then the residual again: the FP32
+      // inner solve below leaves the global constant pressure mode to it
+      if ((rc = sfem_coarse_step(c, o)) || (rc = sfem_update_solution(c))) { if (stats) *stats = st; return rc; }
+      if ((rc = sfem_assemble(c, iter + 1, 0, &nrm))) return rc;
+      st.t_assembly_ms += c->last_asm_ms;
+      st.rhs_norm = nrm;
+      if (!std::isfinite(nrm)) { if (stats) *stats = st; return set_error(SFEM_ERR_DIVERGED, "non-finite residual"); }
+      if (nrm < newton_tol) { conv = true; break; }
+    }
     int its = 0;
     double rr = 0.0;
     sfem_solver_opts oi = *o;

@@ -806,6 +806,7 @@ extern "C" {
 int sfem_set_batch(sfem_ctx* c, int32_t nmesh, const int32_t* elem_mesh, const double* mu) {
   if (!c || nmesh <= 0 || !elem_mesh || !mu) return set_error(SFEM_ERR_ARG, "bad batch arguments");
   if (!c->la) return set_error(SFEM_ERR_STATE, "build the pattern before sfem_set_batch");
+  if (c->la->coarse_k) return set_error(SFEM_ERR_UNSUPPORTED, "coarse correction: single meshes only");
   HIP_OK(hipSetDevice(c->device));
   const int npe = c->dims.npe, ndof = c->dims.ndof;
   // mesh of every node; nodes (and so free DOFs) must come mesh by mesh

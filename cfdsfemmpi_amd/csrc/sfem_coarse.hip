@@ -1,0 +1,600 @@
+// sfem_coarse.hip -- additive coarse pressure correction of the ILU(0) preconditioner.
+//
+// No counterpart in the reference: it preconditions every Newton system with PETSc's
+// ILU(0) (SolverPetsc::factoriseAndSolve, src/base/SolverPetsc.cpp:284-298), whose
+// BiCGStab iteration count grows with the mesh because ILU(0) only damps the pressure
+// error locally.  This adds a two-level correction:
+//
+//   z = M_ILU^{-1} r + Z E^{-1} Z^T r,   E = Z^T A Z,
+//
+// with Z the piecewise-constant pressure on k^d boxes of the mesh's bounding box (one
+// column per box holding at least one free pressure DOF; velocity DOFs get no coarse
+// component).  The preconditioned operator changes, the linear system and its
+// tolerance do not.
+//
+//   * Layout: the ILU(0) plan's padded block vectors (4 slots per node block).  The free
+//     pressure slots are listed sorted by (box, slot) and cut into chunks of at most
+//     kChunk slots that never straddle a box; every sum below runs over chunks and lanes
+//     in a fixed order, so E, its inverse and every correction are the same from run to
+//     run.
+//   * E: at each preconditioner rebuild, one wavefront per chunk reads the pressure row
+//     of each of its block rows in the plan's block tangent (the tangent the Krylov
+//     products use) and sums its pressure columns by the box offset (-1, 0, +1 per
+//     direction) of the neighbour's box; one workgroup adds the chunk sums into E and
+//     inverts it by Gauss-Jordan elimination with partial pivoting (largest magnitude,
+//     first row on ties).  Boxes at least one element wide keep every neighbour within
+//     one box in each direction (checked on the host over the elements).
+//   * E's diagonal: the pressure-pressure block of the stabilised tangent is the PSPG
+//     term, integral of tau grad N_i . grad N_j, whose rows sum to zero (the shape
+//     functions sum to one) except where a neighbour's pressure is prescribed.  Summing
+//     a box's own block directly cancels tens of thousands of entries down to its
+//     couplings with the neighbouring boxes and the prescribed pressures, which FP32
+//     entries cannot resolve; so E(c, c) is formed as the row sums of the rows next to a
+//     prescribed pressure minus the box's couplings to the other boxes,
+//     sum_{i in c} rowsum_i - sum_{c' != c} E(c, c'), the same value in exact arithmetic.
+//   * Apply: restriction (chunk sums of r at the pressure slots, before the ILU(0)
+//     sweeps, so z may alias r), the coarse solve (box sums, then y = E^{-1} c), and the
+//     prolongation z[slot] += y[box] after the sweeps.
+// E and y are FP64.  The correction's global constant pressure mode (E's smallest
+// eigenvalue is set by the prescribed pressures alone, orders of magnitude below the
+// others) outgrows the rest of a preconditioned vector by more than FP32 resolves, so a
+// mixed-precision solve with the correction keeps its Krylov vectors in FP64 (over the
+// FP32 tangent and factor).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "sfem_internal.hpp"
+#include "sfem_linalg.hpp"
+
+namespace sfem {
+
+constexpr int kChunk = 512;       // pressure slots per chunk (8 per lane of a wavefront)
+constexpr int kOffs = 27;         // box offsets (-1, 0, +1)^3
+constexpr int kSelf = 13;         // offset (0, 0, 0)
+constexpr int kOffsPad = 29;      // LDS row stride of the per-lane offset sums (odd: no bank conflicts)
+constexpr int kMaxCoarse = 1024;  // coarse unknowns handled
+
+#define CO_CHECK(x)                                     \
+  do {                                                  \
+    hipError_t _e = (x);                                \
+    if (_e != hipSuccess) return set_hip_error(_e, #x); \
+  } while (0)
+
+struct Coarse {
+  int k = 0;
+  int plan_id = -1;
+  int nc = 0;             // coarse unknowns (boxes with a free pressure slot)
+  int64_t npres = 0;      // free pressure slots
+  int nchunk = 0;
+  bool ready = false;     // E^{-1} built for the current factor
+  bool fp32 = false;      // built from the FP32 tangent (mixed precision)
+  int32_t* d_slot = nullptr;      // npres: padded slot 4p + a, sorted by (box, slot)
+  int32_t* d_chunk = nullptr;     // nchunk + 1: first entry of d_slot per chunk
+  int32_t* d_chunkbox = nullptr;  // nchunk: coarse unknown of the chunk
+  int32_t* d_boxchunk = nullptr;  // nc + 1: chunks of each coarse unknown
+  int32_t* d_cinfo = nullptr;     // per block row: coarse unknown << 3 | (next to a prescribed
+                                  // pressure) << 2 | pressure slot, -1 without pressure
+  int32_t* d_cpos = nullptr;      // per coarse unknown: box coordinates bx | by << 10 | bz << 20
+  int32_t* d_boxnb = nullptr;     // nc * 27: coarse unknown of the box at each offset, -1 if none
+  double* d_epart = nullptr;      // nchunk * 27 chunk sums of E's rows by offset
+  double* d_gj = nullptr;         // nc x 2nc Gauss-Jordan workspace [E | I]
+  double* d_einv = nullptr;       // nc x nc: E^{-1}, row-major
+  double* d_einvb = nullptr;      // the correction without the global constant (FP32 Krylov
+                                  // vectors), or NULL
+  double* d_cpart = nullptr;      // nchunk restricted chunk sums
+  double* d_y = nullptr;          // nc coarse solution
+  int32_t* d_flag = nullptr;      // 1: E was singular (correction switched off)
+  ~Coarse() {
+    void* ptrs[] = {d_slot, d_chunk, d_chunkbox, d_boxchunk, d_cinfo, d_cpos, d_boxnb,
+                    d_epart, d_gj, d_einv, d_einvb, d_cpart, d_y, d_flag};
+    for (void* q : ptrs)
+      if (q) (void)hipFree(q);
+  }
+};
+
+void coarse_destroy(Coarse* C) { delete C; }
+
+// ---------------------------------------------------------------------------
+// Device kernels
+// ---------------------------------------------------------------------------
+
+__device__ inline int box_offset(int32_t a, int32_t b) {
+  const int dx = (b & 1023) - (a & 1023);
+  const int dy = ((b >> 10) & 1023) - ((a >> 10) & 1023);
+  const int dz = ((b >> 20) & 1023) - ((a >> 20) & 1023);
+  return (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+}
+
+// Chunk sums of E's rows: for the block rows of a chunk (one wavefront, each lane its
+// rows in order), the pressure row's entries in pressure columns of other boxes, summed
+// per box offset of the column's box in neighbour order, and at the box's own offset the
+// full row sums of rows next to a prescribed pressure; then per offset the 64 lanes in
+// order.
+template <typename VT>
+__global__ void __launch_bounds__(64) k_coarse_rows(const int32_t* __restrict__ slot, const int32_t* __restrict__ chunk,
+                                                     const int4* __restrict__ bdesc, const int32_t* __restrict__ bcol,
+                                                     const VT* __restrict__ kp, const int32_t* __restrict__ cinfo,
+                                                     const int32_t* __restrict__ cpos, double* __restrict__ epart) {
+  __shared__ double acc[64 * kOffsPad];
+  const int lane = threadIdx.x, ch = blockIdx.x;
+  double* my = acc + lane * kOffsPad;
+  for (int o = 0; o < kOffs; ++o) my[o] = 0.0;
+  const int i0 = chunk[ch], i1 = chunk[ch + 1];
+  for (int i = i0 + lane; i < i1; i += 64) {
+    const int s = slot[i];
+    const int p = s >> 2, a = s & 3;
+    const int4 d = bdesc[p];
+    const int vbase = d.x, bptr = d.y, nb = d.z;
+    const int32_t own = cinfo[p];
+    const int32_t pos = cpos[own >> 3];
+    const VT* row = kp + vbase + (int64_t)a * 4 * nb;
+    double rs = 0.0;
+    for (int t = 0; t < nb; ++t) {
+      const int info = cinfo[bcol[bptr + t]];
+      if (info < 0) continue;
+      const double v = (double)row[4 * t + (info & 3)];
+      const int o = box_offset(pos, cpos[info >> 3]);
+      if (o != kSelf && (unsigned)o < (unsigned)kOffs) my[o] += v;
+      rs += v;
+    }
+    if (own & 4) my[kSelf] += rs;
+  }
+  __syncthreads();
+  if (lane < kOffs) {
+    double s = 0.0;
+    for (int l = 0; l < 64; ++l) s += acc[l * kOffsPad + lane];
+    epart[(int64_t)ch * kOffs + lane] = s;
+  }
+}
+
+// [E | I] from the chunk sums, one thread per row of E: each (unknown, offset) cell is one
+// entry of E, summed over the unknown's chunks in order.
+__global__ void __launch_bounds__(64) k_coarse_fill(int nc, const int32_t* __restrict__ boxchunk,
+                                                     const int32_t* __restrict__ boxnb,
+                                                     const double* __restrict__ epart, double* __restrict__ gj) {
+  const int c = blockIdx.x * 64 + threadIdx.x;
+  if (c >= nc) return;
+  const int w = 2 * nc;
+  double* row = gj + (int64_t)c * w;
+  for (int j = 0; j < w; ++j) row[j] = j == nc + c ? 1.0 : 0.0;
+  double own = 0.0;
+  for (int o = 0; o < kOffs; ++o) {
+    const int cc = boxnb[c * kOffs + o];
+    double s = 0.0;
+    for (int q = boxchunk[c]; q < boxchunk[c + 1]; ++q) s += epart[(int64_t)q * kOffs + o];
+    if (o == kSelf) own += s;       // row sums next to prescribed pressures
+    else if (cc >= 0) { row[cc] = s; own -= s; }
+  }
+  row[c] = own;  // E(c, c) = those row sums - the couplings to the other boxes
+}
+
+// Gauss-Jordan step j without row scaling: row i -= (E(i, j) / E(j, j)) row j for every
+// i != j, one workgroup per row, over columns j+1 .. nc+j (row j is zero beyond them;
+// earlier columns are not read again).  Row j itself is left alone, so no workgroup
+// writes what another reads.  E is symmetric and definite (the pressure block of the
+// tangent is a weighted Laplacian), so no pivoting is needed; the finishing kernel checks
+// the pivots.
+__global__ void __launch_bounds__(256) k_gj_step(int nc, int j, double* __restrict__ gj) {
+  const int i = blockIdx.x;
+  if (i == j) return;
+  const int w = 2 * nc;
+  double* ri = gj + (int64_t)i * w;
+  const double* rj = gj + (int64_t)j * w;
+  const double f = ri[j] / rj[j];
+  if (f == 0.0) return;
+  for (int c = j + 1 + threadIdx.x; c <= nc + j; c += 256) ri[c] -= f * rj[c];
+}
+
+// E^{-1} (row-major) = the right half scaled by the diagonal the steps left; with
+// border = 1 also B = E^{-1} - (E^{-1} 1)(1^T E^{-1}) / (1^T E^{-1} 1), the leading block
+// of [E b1; b1^T 0]^{-1}: the correction whose output sums to zero.  A pivot below
+// 1e-15 of the largest (or not finite) zeroes both and sets *flag.  One workgroup; sums
+// in a fixed order.
+__global__ void __launch_bounds__(256) k_coarse_finish(int nc, int border, const double* __restrict__ gj,
+                                                       double* __restrict__ einv, double* __restrict__ einvb,
+                                                       int32_t* __restrict__ flag) {
+  __shared__ double red[256];
+  __shared__ double u[kMaxCoarse], v[kMaxCoarse];
+  __shared__ double s_sum;
+  __shared__ int s_bad;
+  const int tid = threadIdx.x;
+  const int w = 2 * nc;
+  double dmax = 0.0;
+  for (int i = tid; i < nc; i += 256) dmax = fmax(dmax, fabs(gj[(int64_t)i * w + i]));
+  red[tid] = dmax;
+  if (tid == 0) s_bad = 0;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (tid < h) red[tid] = fmax(red[tid], red[tid + h]);
+    __syncthreads();
+  }
+  const double tol = 1e-15 * red[0];
+  for (int i = tid; i < nc; i += 256) {
+    const double d = gj[(int64_t)i * w + i];
+    if (!(fabs(d) > tol) || !isfinite(d)) s_bad = 1;
+  }
+  __syncthreads();
+  for (int e = tid; e < nc * nc; e += 256) {
+    const int i = e / nc, c = e - i * nc;
+    const double x = gj[(int64_t)i * w + nc + c] / gj[(int64_t)i * w + i];
+    einv[e] = x;
+    if (!isfinite(x)) s_bad = 1;
+  }
+  __syncthreads();
+  if (border) {
+    for (int i = tid; i < nc; i += 256) {
+      double a = 0.0, b = 0.0;
+      for (int c = 0; c < nc; ++c) {
+        a += einv[(int64_t)i * nc + c];   // row sums: E^{-1} 1
+        b += einv[(int64_t)c * nc + i];   // column sums: 1^T E^{-1}
+      }
+      u[i] = a;
+      v[i] = b;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double t = 0.0;
+      for (int i = 0; i < nc; ++i) t += u[i];
+      s_sum = t;
+      if (!(fabs(t) > 0.0) || !isfinite(t)) s_bad = 1;
+    }
+    __syncthreads();
+    const double sinv = 1.0 / s_sum;
+    for (int e = tid; e < nc * nc; e += 256) {
+      const int i = e / nc, c = e - i * nc;
+      einvb[e] = einv[e] - u[i] * v[c] * sinv;
+    }
+    __syncthreads();
+  }
+  if (s_bad) {
+    for (int e = tid; e < nc * nc; e += 256) {
+      einv[e] = 0.0;
+      if (border) einvb[e] = 0.0;
+    }
+  }
+  if (tid == 0) *flag = s_bad;
+}
+
+// Restriction: chunk sums of r over the chunk's pressure slots (lanes in order, then a
+// fixed tree over the wavefront).
+template <typename T>
+__global__ void __launch_bounds__(64) k_coarse_restrict(const int32_t* __restrict__ slot,
+                                                         const int32_t* __restrict__ chunk, const T* __restrict__ r,
+                                                         double* __restrict__ cpart) {
+  const int lane = threadIdx.x, ch = blockIdx.x;
+  double s = 0.0;
+  for (int i = chunk[ch] + lane; i < chunk[ch + 1]; i += 64) s += (double)r[slot[i]];
+  for (int h = 32; h > 0; h >>= 1) s += __shfl_xor(s, h);
+  if (lane == 0) cpart[ch] = s;
+}
+
+// Coarse solve: box sums of the chunk sums (each workgroup, one lane per box, chunks in
+// order), then y = E^{-1} c with one wavefront per row (lanes over columns in order, a
+// fixed shuffle tree).
+__global__ void __launch_bounds__(256) k_coarse_solve(int nc, const int32_t* __restrict__ boxchunk,
+                                                      const double* __restrict__ cpart,
+                                                      const double* __restrict__ einv, double* __restrict__ y) {
+  __shared__ double c[kMaxCoarse];
+  const int tid = threadIdx.x;
+  for (int b = tid; b < nc; b += 256) {
+    double s = 0.0;
+    for (int q = boxchunk[b]; q < boxchunk[b + 1]; ++q) s += cpart[q];
+    c[b] = s;
+  }
+  __syncthreads();
+  const int i = blockIdx.x * 4 + (tid >> 6), lane = tid & 63;
+  if (i >= nc) return;
+  const double* row = einv + (int64_t)i * nc;
+  double s = 0.0;
+  for (int j = lane; j < nc; j += 64) s += row[j] * c[j];
+  for (int h = 32; h > 0; h >>= 1) s += __shfl_xor(s, h);
+  if (lane == 0) y[i] = s;
+}
+
+// Prolongation: z[slot] += y[box of the slot's chunk].
+template <typename T>
+__global__ void __launch_bounds__(64) k_coarse_prolong(const int32_t* __restrict__ slot,
+                                                        const int32_t* __restrict__ chunk,
+                                                        const int32_t* __restrict__ chunkbox,
+                                                        const double* __restrict__ y, T* __restrict__ z) {
+  const int lane = threadIdx.x, ch = blockIdx.x;
+  const double v = y[chunkbox[ch]];
+  for (int i = chunk[ch] + lane; i < chunk[ch + 1]; i += 64) {
+    const int s = slot[i];
+    z[s] = (T)((double)z[s] + v);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+template <typename X>
+static int upload(X** d, const std::vector<X>& h) {
+  if (*d) (void)hipFree(*d);
+  *d = nullptr;
+  CO_CHECK(hipMalloc((void**)d, std::max<size_t>(h.size(), 1) * sizeof(X)));
+  if (!h.empty()) CO_CHECK(hipMemcpy(*d, h.data(), h.size() * sizeof(X), hipMemcpyHostToDevice));
+  return SFEM_OK;
+}
+
+// Box coordinates of every node: box b_d = min(k - 1, floor(k (x_d - min_d) / extent_d)).
+static void node_boxes(const sfem_ctx* ctx, int k, std::vector<int32_t>& pos) {
+  const int nd = ctx->dims.ndim;
+  const int64_t nn = ctx->nNode;
+  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  for (int d = 0; d < nd; ++d) {
+    lo[d] = hi[d] = ctx->coords[d];
+    for (int64_t v = 1; v < nn; ++v) {
+      lo[d] = std::min(lo[d], ctx->coords[v * nd + d]);
+      hi[d] = std::max(hi[d], ctx->coords[v * nd + d]);
+    }
+  }
+  pos.assign(nn, 0);
+  for (int64_t v = 0; v < nn; ++v) {
+    int32_t code = 0;
+    for (int d = 0; d < nd; ++d) {
+      const double ext = hi[d] - lo[d];
+      int b = ext > 0 ? (int)std::floor((ctx->coords[v * nd + d] - lo[d]) / ext * k) : 0;
+      b = std::min(std::max(b, 0), k - 1);
+      code |= b << (10 * d);
+    }
+    pos[v] = code;
+  }
+}
+
+// k^dim boxes, and every element's nodes within one box of each other in each direction,
+// so that the neighbours of a block row sit at box offsets in {-1, 0, 1}^dim.
+static int coarse_check(const sfem_ctx* ctx, int k) {
+  const int nd = ctx->dims.ndim, npe = ctx->dims.npe;
+  int64_t nbox = 1;
+  for (int d = 0; d < nd; ++d) nbox *= k;
+  if (k < 1 || nbox > kMaxCoarse) return set_error(SFEM_ERR_ARG, "coarse boxes: k^dim must lie in [1, 1024]");
+  std::vector<int32_t> npos;
+  node_boxes(ctx, k, npos);
+  for (int64_t e = 0; e < ctx->nElem; ++e) {
+    const int32_t* en = &ctx->conn[e * npe];
+    for (int d = 0; d < nd; ++d) {
+      int mn = 1 << 20, mx = -1;
+      for (int a = 0; a < npe; ++a) {
+        const int b = (npos[en[a]] >> (10 * d)) & 1023;
+        mn = std::min(mn, b);
+        mx = std::max(mx, b);
+      }
+      if (mx - mn > 1) return set_error(SFEM_ERR_ARG, "coarse boxes narrower than an element: use fewer boxes");
+    }
+  }
+  return SFEM_OK;
+}
+
+// Host lists for the current ILU(0) plan: pressure slots by box, chunks, the per-block
+// coarse info and the box neighbourhoods.
+static int coarse_plan(sfem_ctx* ctx, LinAlg* L, Coarse* C) {
+  const IluPlan& P = *L->ilu;
+  const int k = C->k;
+  const int nd = ctx->dims.ndim, ndof = ctx->dims.ndof, npe = ctx->dims.npe;
+  std::vector<int32_t> npos;
+  node_boxes(ctx, k, npos);
+  const int64_t nb = P.nblk;
+  const std::vector<int32_t>& bs = L->h_blkStart;
+  // nodes sharing an element with a node whose pressure is prescribed: their pressure
+  // rows do not sum to zero
+  std::vector<uint8_t> nearfix(ctx->nNode, 0);
+  for (int64_t e = 0; e < ctx->nElem; ++e) {
+    const int32_t* en = &ctx->conn[e * npe];
+    bool fix = false;
+    for (int a = 0; a < npe; ++a) fix = fix || ctx->nodeDof[(int64_t)en[a] * ndof + ndof - 1] < 0;
+    if (fix)
+      for (int a = 0; a < npe; ++a) nearfix[en[a]] = 1;
+  }
+  // pressure slot of each block row (plan position), its box, and its node's mark
+  std::vector<int32_t> pslot(nb, -1), pbox(nb, 0);
+  std::vector<uint8_t> pfix(nb, 0);
+  int64_t npres = 0;
+  for (int64_t p = 0; p < nb; ++p) {
+    const int32_t b = P.h_order[p];
+    for (int32_t r = bs[b]; r < bs[b + 1]; ++r) {
+      const int32_t g = ctx->assyForSoln[r];
+      if (g % ndof == ndof - 1) {
+        pslot[p] = r - bs[b];
+        pbox[p] = npos[g / ndof];
+        pfix[p] = nearfix[g / ndof];
+        ++npres;
+      }
+    }
+  }
+  if (npres == 0) return set_error(SFEM_ERR_ARG, "coarse correction: no free pressure DOF");
+  // coarse unknowns: non-empty boxes in box order
+  std::vector<int32_t> boxes;
+  boxes.reserve(npres);
+  for (int64_t p = 0; p < nb; ++p)
+    if (pslot[p] >= 0) boxes.push_back(pbox[p]);
+  auto key = [](int32_t pos) {  // box order: x fastest
+    return (int64_t)((pos >> 20) & 1023) << 20 | (int64_t)((pos >> 10) & 1023) << 10 | (pos & 1023);
+  };
+  std::sort(boxes.begin(), boxes.end(), [&](int32_t a, int32_t b) { return key(a) < key(b); });
+  boxes.erase(std::unique(boxes.begin(), boxes.end()), boxes.end());
+  const int nc = (int)boxes.size();
+  std::vector<int64_t> bkeys(nc);
+  for (int c = 0; c < nc; ++c) bkeys[c] = key(boxes[c]);
+  auto coarse_of = [&](int32_t pos) -> int {
+    const auto it = std::lower_bound(bkeys.begin(), bkeys.end(), key(pos));
+    return it != bkeys.end() && *it == key(pos) ? (int)(it - bkeys.begin()) : -1;
+  };
+  std::vector<int32_t> cinfo(nb, -1);
+  std::vector<std::pair<int32_t, int32_t>> order;  // (coarse unknown, slot)
+  order.reserve(npres);
+  for (int64_t p = 0; p < nb; ++p)
+    if (pslot[p] >= 0) {
+      const int c = coarse_of(pbox[p]);
+      cinfo[p] = c << 3 | pfix[p] << 2 | pslot[p];
+      order.push_back({c, (int32_t)(4 * p + pslot[p])});
+    }
+  std::sort(order.begin(), order.end());
+  std::vector<int32_t> slot(npres), chunk, chunkbox, boxchunk(nc + 1, 0);
+  for (int64_t i = 0; i < npres; ++i) slot[i] = order[i].second;
+  for (int64_t i = 0; i < npres;) {
+    const int c = order[i].first;
+    int64_t e = i;
+    while (e < npres && order[e].first == c) ++e;
+    boxchunk[c] = (int32_t)chunkbox.size();
+    for (int64_t s = i; s < e; s += kChunk) {
+      chunk.push_back((int32_t)s);
+      chunkbox.push_back(c);
+    }
+    i = e;
+  }
+  chunk.push_back((int32_t)npres);
+  boxchunk[nc] = (int32_t)chunkbox.size();
+  std::vector<int32_t> boxnb((size_t)nc * kOffs, -1);
+  const int ky = nd > 1 ? k : 1, kz = nd > 2 ? k : 1;
+  for (int c = 0; c < nc; ++c) {
+    const int32_t pos = boxes[c];
+    const int bx = pos & 1023, by = (pos >> 10) & 1023, bz = (pos >> 20) & 1023;
+    for (int o = 0; o < kOffs; ++o) {
+      const int x = bx + o % 3 - 1, y = by + (o / 3) % 3 - 1, z = bz + o / 9 - 1;
+      if (x < 0 || y < 0 || z < 0 || x >= k || y >= ky || z >= kz) continue;
+      boxnb[(size_t)c * kOffs + o] = coarse_of(x | y << 10 | z << 20);
+    }
+  }
+  int rc;
+  if ((rc = upload(&C->d_slot, slot)) || (rc = upload(&C->d_chunk, chunk)) || (rc = upload(&C->d_chunkbox, chunkbox)) ||
+      (rc = upload(&C->d_boxchunk, boxchunk)) || (rc = upload(&C->d_cinfo, cinfo)) || (rc = upload(&C->d_cpos, boxes)) ||
+      (rc = upload(&C->d_boxnb, boxnb)))
+    return rc;
+  C->nc = nc;
+  C->npres = npres;
+  C->nchunk = (int)chunkbox.size();
+  void* old[] = {C->d_epart, C->d_gj, C->d_einv, C->d_einvb, C->d_cpart, C->d_y, C->d_flag};
+  C->d_einvb = nullptr;
+  for (void* q : old)
+    if (q) (void)hipFree(q);
+  CO_CHECK(hipMalloc((void**)&C->d_epart, (size_t)C->nchunk * kOffs * sizeof(double)));
+  CO_CHECK(hipMalloc((void**)&C->d_gj, (size_t)nc * 2 * nc * sizeof(double)));
+  CO_CHECK(hipMalloc((void**)&C->d_einv, (size_t)nc * nc * sizeof(double)));
+  CO_CHECK(hipMalloc((void**)&C->d_cpart, (size_t)C->nchunk * sizeof(double)));
+  CO_CHECK(hipMalloc((void**)&C->d_y, (size_t)nc * sizeof(double)));
+  CO_CHECK(hipMalloc((void**)&C->d_flag, sizeof(int32_t)));
+  C->plan_id = L->plan_id;
+  return SFEM_OK;
+}
+
+int coarse_set(sfem_ctx* ctx, int k) {
+  LinAlg* L = ctx->la;
+  if (k == 0) {
+    delete L->coarse;
+    L->coarse = nullptr;
+    L->coarse_k = 0;
+    return SFEM_OK;
+  }
+  int rc = coarse_check(ctx, k);
+  if (rc) return rc;
+  if (L->coarse && L->coarse->k != k) {
+    delete L->coarse;
+    L->coarse = nullptr;
+  }
+  L->coarse_k = k;
+  return SFEM_OK;
+}
+
+// E and E^{-1} from the plan's block tangent (FP64 kp or FP32 kp32); called after each
+// ILU(0) factorisation.
+int coarse_build(sfem_ctx* ctx, const double* kp, const float* kp32) {
+  LinAlg* L = ctx->la;
+  if (L->coarse_k <= 0 || ctx->batch) return SFEM_OK;
+  if (!L->coarse) {
+    L->coarse = new Coarse;
+    L->coarse->k = L->coarse_k;
+  }
+  Coarse* C = L->coarse;
+  C->ready = false;
+  if (C->plan_id != L->plan_id) {
+    int rc = coarse_plan(ctx, L, C);
+    if (rc) return rc;
+  }
+  const IluPlan& P = *L->ilu;
+  hipStream_t st = ctx->stream;
+  const int4* bd = reinterpret_cast<const int4*>(P.d_bdesc);
+  if (kp32)
+    hipLaunchKernelGGL((k_coarse_rows<float>), dim3(C->nchunk), dim3(64), 0, st, C->d_slot, C->d_chunk, bd, P.d_bcol,
+                       kp32, C->d_cinfo, C->d_cpos, C->d_epart);
+  else
+    hipLaunchKernelGGL((k_coarse_rows<double>), dim3(C->nchunk), dim3(64), 0, st, C->d_slot, C->d_chunk, bd, P.d_bcol,
+                       kp, C->d_cinfo, C->d_cpos, C->d_epart);
+  const int nc = C->nc;
+  hipLaunchKernelGGL(k_coarse_fill, dim3((nc + 63) / 64), dim3(64), 0, st, nc, C->d_boxchunk, C->d_boxnb, C->d_epart,
+                     C->d_gj);
+  for (int j = 0; j < nc; ++j) hipLaunchKernelGGL(k_gj_step, dim3(nc), dim3(256), 0, st, nc, j, C->d_gj);
+  // FP32 Krylov vectors: their correction leaves the global constant pressure to the FP64
+  // coarse step of the Newton loop
+  const int border = kp32 != nullptr;
+  if (border && !C->d_einvb) CO_CHECK(hipMalloc((void**)&C->d_einvb, (size_t)nc * nc * sizeof(double)));
+  hipLaunchKernelGGL(k_coarse_finish, dim3(1), dim3(256), 0, st, nc, border, C->d_gj, C->d_einv, C->d_einvb,
+                     C->d_flag);
+  C->fp32 = border;
+  CO_CHECK(hipGetLastError());
+  C->ready = true;
+  return SFEM_OK;
+}
+
+bool coarse_active(const LinAlg* L) { return L->coarse && L->coarse->ready && L->coarse->plan_id == L->plan_id; }
+
+template <typename T>
+int coarse_restrict(sfem_ctx* ctx, const T* r) {
+  Coarse* C = ctx->la->coarse;
+  hipLaunchKernelGGL((k_coarse_restrict<T>), dim3(C->nchunk), dim3(64), 0, ctx->stream, C->d_slot, C->d_chunk, r,
+                     C->d_cpart);
+  return SFEM_OK;
+}
+
+template <typename T>
+int coarse_correct(sfem_ctx* ctx, T* z) {
+  Coarse* C = ctx->la->coarse;
+  const int g = (C->nc + 3) / 4;  // four rows per workgroup
+  const double* inv = sizeof(T) == 4 && C->d_einvb && C->fp32 ? C->d_einvb : C->d_einv;
+  hipLaunchKernelGGL(k_coarse_solve, dim3(g), dim3(256), 0, ctx->stream, C->nc, C->d_boxchunk, C->d_cpart, inv,
+                     C->d_y);
+  hipLaunchKernelGGL((k_coarse_prolong<T>), dim3(C->nchunk), dim3(64), 0, ctx->stream, C->d_slot, C->d_chunk,
+                     C->d_chunkbox, C->d_y, z);
+  return SFEM_OK;
+}
+
+template int coarse_restrict<float>(sfem_ctx*, const float*);
+template int coarse_restrict<double>(sfem_ctx*, const double*);
+template int coarse_correct<float>(sfem_ctx*, float*);
+template int coarse_correct<double>(sfem_ctx*, double*);
+
+int coarse_info(sfem_ctx* ctx, int64_t* out) {
+  LinAlg* L = ctx->la;
+  const Coarse* C = L->coarse;
+  out[0] = L->coarse_k;
+  out[1] = C ? C->nc : 0;
+  out[2] = C ? C->npres : 0;
+  out[3] = C ? C->nchunk : 0;
+  int32_t flag = 0;
+  if (C && C->d_flag && C->ready) {
+    CO_CHECK(hipStreamSynchronize(ctx->stream));
+    CO_CHECK(hipMemcpy(&flag, C->d_flag, sizeof(flag), hipMemcpyDeviceToHost));
+  }
+  out[4] = flag;
+  out[5] = C && C->ready ? 1 : 0;
+  return SFEM_OK;
+}
+
+// E^{-1} (which = 0) or the FP32 vectors' inverse without the global constant (which = 1),
+// nc x nc row-major, of the last build.
+int coarse_get_inverse(sfem_ctx* ctx, int which, double* out, int cap) {
+  const Coarse* C = ctx->la->coarse;
+  if (!C || !C->ready) return set_error(SFEM_ERR_STATE, "coarse correction not built");
+  if (which == 1 && !(C->fp32 && C->d_einvb)) return set_error(SFEM_ERR_STATE, "no FP32 coarse inverse");
+  if (cap < C->nc * C->nc) return set_error(SFEM_ERR_ARG, "buffer too small");
+  CO_CHECK(hipStreamSynchronize(ctx->stream));
+  CO_CHECK(hipMemcpy(out, which ? C->d_einvb : C->d_einv, (size_t)C->nc * C->nc * sizeof(double),
+                     hipMemcpyDeviceToHost));
+  return SFEM_OK;
+}
+
+}  // namespace sfem

@@ -1705,6 +1705,7 @@ LinAlg::~LinAlg() {
                   dinv, dinv32, binv, binv32, lu, lu32, gm_V, gm_V32, d_desc, lu16, kp, kp32, d_h, d_mpart,
                   tinv, tinv32, tinv16, sw, sw32, sw16, fu, fu32};
   delete ilu;
+  coarse_destroy(coarse);
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (h_scal) (void)hipHostFree(h_scal);
@@ -2523,6 +2524,11 @@ int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f, 
       }
     }
     LA_CHECK(hipGetLastError());
+    // the coarse pressure correction's E = Z^T A Z from the tangent of this factor
+    if (L->coarse_k > 0) {
+      int rc = coarse_build(ctx, fp32 ? nullptr : L->kp, fp32 ? L->kp32 : nullptr);
+      if (rc) return rc;
+    }
   }
   return SFEM_OK;
 }
@@ -2680,7 +2686,14 @@ static int pc_apply_impl(sfem_ctx* ctx, const T* r, T* z, const RowSubset* sub) 
     }
     case SFEM_PC_ILU0:
       if (!ilu_buffers_ready(L)) return set_error(SFEM_ERR_STATE, "ILU(0) factor buffers missing for its precision");
-      ilu_apply<T>(ctx, r, z, -1, nullptr, sub);
+      if (!sub && coarse_active(L)) {
+        // additive coarse correction: r restricted before the sweeps (z may alias r)
+        coarse_restrict<T>(ctx, r);
+        ilu_apply<T>(ctx, r, z, -1, nullptr, sub);
+        coarse_correct<T>(ctx, z);
+      } else {
+        ilu_apply<T>(ctx, r, z, -1, nullptr, sub);
+      }
       break;
     default:
       hipLaunchKernelGGL((k_copy<T>), dim3(g), dim3(kTPB), 0, st, n, r, z);
@@ -2748,7 +2761,8 @@ static void spmv_in(sfem_ctx* ctx, const T* x, T* y, bool block, const RowSubset
   ctx->kt.begin(SFEM_KT_SPMV, ctx->stream);
   if (block) {
     const int code = L->tune_bspmv[slot];
-    if (sizeof(T) == 8 && !L->pc_store32) bsr_spmv_launch<double, T>(ctx->stream, code, *L->ilu, L->kp, x, y, sub);
+    if (sizeof(T) == 8 && !L->pc_store32 && !L->pc_fp32)
+      bsr_spmv_launch<double, T>(ctx->stream, code, *L->ilu, L->kp, x, y, sub);
     else bsr_spmv_launch<float, T>(ctx->stream, code, *L->ilu, L->kp32, x, y, sub);
   } else {
     const int v = L->tune_spmv[slot];
@@ -3537,6 +3551,29 @@ int la_apply_op32(sfem_ctx* ctx, const float* x, float* y, const RowSubset* sub)
   spmv_in<float>(ctx, x, y, ctx->la->block_space, sub);
   return hipGetLastError() == hipSuccess ? SFEM_OK : set_error(SFEM_ERR_HIP, "spmv launch");
 }
+// The Newton loop's coarse step: increment = Z E^{-1} Z^T rhs in FP64 (DOF layout), with
+// the preconditioner (and E) brought up to date for opts first.
+int la_coarse_step(sfem_ctx* ctx, const sfem_solver_opts* o) {
+  LinAlg* L = ctx->la;
+  if (L->coarse_k <= 0 || o->pc_type != SFEM_PC_ILU0 || ctx->batch)
+    return set_error(SFEM_ERR_STATE, "coarse step: needs the coarse correction of ILU(0) on a single mesh");
+  int rc = la_prepare_solve(ctx, o);
+  if (rc) return rc;
+  if (!coarse_active(L)) return set_error(SFEM_ERR_STATE, "coarse step: correction not built");
+  hipStream_t st = ctx->stream;
+  const int64_t nk = L->nk;
+  const int32_t* padmap = L->ilu->d_padmap;
+  if ((rc = ensure_vec(&L->pr, L->nvec)) || (rc = ensure_vec(&L->pz, L->nvec))) return rc;
+  hipLaunchKernelGGL((k_pad_in<double, double>), dim3(grid_for(nk)), dim3(kTPB), 0, st, nk, padmap, ctx->d_rhs, L->pr,
+                     1.0);
+  hipLaunchKernelGGL((k_fill<double>), dim3(grid_for(nk)), dim3(kTPB), 0, st, nk, L->pz, 0.0);
+  coarse_restrict<double>(ctx, L->pr);
+  coarse_correct<double>(ctx, L->pz);
+  hipLaunchKernelGGL((k_pad_out<double, double>), dim3(grid_for(nk)), dim3(kTPB), 0, st, nk, padmap, L->pz, ctx->d_du,
+                     1.0);
+  return hipGetLastError() == hipSuccess ? SFEM_OK : set_error(SFEM_ERR_HIP, "coarse step launch");
+}
+
 int la_apply_op64(sfem_ctx* ctx, const double* x, double* y, const RowSubset* sub) {
   if (sub && !ctx->la->block_space) return set_error(SFEM_ERR_UNSUPPORTED, "row subsets need the block layout");
   spmv_in<double>(ctx, x, y, ctx->la->block_space, sub);

@@ -69,6 +69,7 @@ struct IluPlan {
 };
 
 struct DirectLU;  // banded LU of the direct solver (sfem_direct.hip)
+struct Coarse;    // coarse pressure correction of ILU(0) (sfem_coarse.hip)
 
 struct LinAlg {
   int64_t n = 0;             // free DOFs
@@ -143,6 +144,8 @@ struct LinAlg {
   double* kp = nullptr;
   float* kp32 = nullptr;
   DirectLU* direct = nullptr;  // direct solver (pc_type LU), set up on first use
+  int coarse_k = 0;            // coarse pressure boxes per direction (0: no correction)
+  Coarse* coarse = nullptr;
   ~LinAlg();
 };
 
@@ -200,6 +203,16 @@ int la_direct_setup(sfem_ctx* ctx);
 int la_direct_solve(sfem_ctx* ctx, int* iters, double* relres);
 int la_direct_info(sfem_ctx* ctx, int* kl, int* ku);
 void la_direct_destroy(LinAlg* L);
+// coarse pressure correction (sfem_coarse.hip)
+int coarse_set(sfem_ctx* ctx, int k);
+int coarse_build(sfem_ctx* ctx, const double* kp, const float* kp32);
+bool coarse_active(const LinAlg* L);
+template <typename T> int coarse_restrict(sfem_ctx* ctx, const T* r);
+template <typename T> int coarse_correct(sfem_ctx* ctx, T* z);
+int coarse_info(sfem_ctx* ctx, int64_t* out);
+int coarse_get_inverse(sfem_ctx* ctx, int which, double* out, int cap);
+int la_coarse_step(sfem_ctx* ctx, const sfem_solver_opts* o);
+void coarse_destroy(Coarse* C);
 // CSR values from the assembled node-pair block form when the tangent changed since
 // the last conversion (element loops that assemble the block form, ctx->kb_on)
 int la_ensure_csr(sfem_ctx* ctx);

@@ -64,6 +64,10 @@ ABI = [
     ("sfem_set_precision", C.c_int, [_vp, C.c_int]),
     ("sfem_set_predictor", C.c_int, [_vp, C.c_int]),
     ("sfem_set_pc_lag", C.c_int, [_vp, C.c_int]),
+    ("sfem_set_coarse_correction", C.c_int, [_vp, C.c_int]),
+    ("sfem_coarse_info", C.c_int, [_vp, _i64p]),
+    ("sfem_coarse_inverse", C.c_int, [_vp, C.c_int, _f64p, C.c_int]),
+    ("sfem_coarse_step", C.c_int, [_vp, C.POINTER(SolverOpts)]),
     ("sfem_set_inner_tolerance", C.c_int, [_vp, C.c_double, C.c_double]),
     ("sfem_time_update", C.c_int, [_vp, C.c_double]),
     ("sfem_assemble", C.c_int, [_vp, C.c_int, C.c_int, _f64p]),
@@ -367,6 +371,31 @@ class Context:
         """Rebuild the preconditioner on every lag-th new tangent only (1 = always)."""
         _check(self.lib.sfem_set_pc_lag(self.h, int(lag)), "sfem_set_pc_lag")
 
+    def set_coarse_correction(self, boxes_per_dim: int):
+        """Add Z E^-1 Z^T r to every ILU(0) application: piecewise-constant pressure on
+        boxes_per_dim boxes per direction, E = Z^T A Z (0 switches it off)."""
+        _check(self.lib.sfem_set_coarse_correction(self.h, int(boxes_per_dim)), "sfem_set_coarse_correction")
+
+    def coarse_info(self) -> dict:
+        """State of the coarse pressure correction."""
+        v = np.zeros(6, np.int64)
+        _check(self.lib.sfem_coarse_info(self.h, _p(v, C.c_int64)), "sfem_coarse_info")
+        return {"k": int(v[0]), "unknowns": int(v[1]), "pressure_dofs": int(v[2]), "chunks": int(v[3]),
+                "singular": bool(v[4]), "built": bool(v[5])}
+
+    def coarse_inverse(self, which: int = 0) -> np.ndarray:
+        """E^-1 (which 0) or the mixed-precision inverse B without the global constant
+        (which 1) of the last preconditioner rebuild (unknowns x unknowns)."""
+        nc = self.coarse_info()["unknowns"]
+        out = np.zeros(max(nc * nc, 1))
+        _check(self.lib.sfem_coarse_inverse(self.h, int(which), _p(out, C.c_double), out.size),
+               "sfem_coarse_inverse")
+        return out[:nc * nc].reshape(nc, nc)
+
+    def coarse_step(self, opts: SolverOpts):
+        """Increment = Z E^-1 Z^T rhs (FP64); apply it with update_solution()."""
+        _check(self.lib.sfem_coarse_step(self.h, C.byref(opts)), "sfem_coarse_step")
+
     def get_ilu(self):
         """(perm, rowptr, colidx, vals) of the current ILU(0) factor in its ordering."""
         nnz = C.c_int64(0)
@@ -535,6 +564,7 @@ class StabFEM:
         self.max_newton = 10       # hard-coded iteration cap (StabFEM2.cpp:654)
         self.precision = PREC_FP64
         self.tuning = TUNE_TIMED   # kernel shapes of the solver: timed on first use, or fixed
+        self.coarse_boxes = 0      # coarse pressure correction of ILU(0): boxes per direction (0: off)
         self.history = []          # (tstep, iter, rhs_norm, krylov_its)
 
     # StabFEM::readInputData (StabFEM.cpp:80-264)
@@ -576,6 +606,8 @@ class StabFEM:
         self.ctx.set_params(c.elemData, c.tis, c.rhoInf, c.dt)
         self.ctx.set_precision(self.precision)
         self.ctx.set_tuning_mode(self.tuning)
+        if self.coarse_boxes:
+            self.ctx.set_coarse_correction(self.coarse_boxes)
         self.ctx.setup_solver(self.opts)
 
     # StabFEM::solveFullyImplicit (StabFEM2.cpp:593-832)
@@ -591,6 +623,14 @@ class StabFEM:
                 if nrm < self.newton_tol:
                     self.history.append((tstep, it, nrm, 0))
                     break
+                if self.coarse_boxes and self.precision == PREC_MIXED:
+                    # FP64 coarse step before the FP32 inner solve (as sfem_step does)
+                    ctx.coarse_step(self.opts)
+                    ctx.update_solution()
+                    nrm = ctx.assemble(it + 1, want_K=False)
+                    if nrm < self.newton_tol:
+                        self.history.append((tstep, it, nrm, 0))
+                        break
                 its, _ = ctx.solve(self.opts)
                 self.history.append((tstep, it, nrm, its))
                 ctx.update_solution()

@@ -195,6 +195,34 @@ int sfem_predictor_stats(sfem_ctx* ctx, int64_t* counts, double* last_norms);
  * reused.  No counterpart in the reference. */
 int sfem_set_pc_lag(sfem_ctx* ctx, int lag);
 
+/* Additive coarse pressure correction of the ILU(0) preconditioner:
+ * z = M_ILU^{-1} r + Z E^{-1} Z^T r, with Z the piecewise-constant pressure on k boxes
+ * per direction of the mesh's bounding box (k^dim <= 1024; every box at least one element
+ * wide) and E = Z^T A Z formed on the device from the tangent at each preconditioner
+ * rebuild.  k = 0 (the default) switches it off.  It changes the preconditioned operator,
+ * not the linear system or its tolerance; it cuts BiCGStab iterations on meshes whose
+ * pressure error ILU(0) damps only locally.
+ * Under SFEM_PREC_MIXED the FP32 Krylov vectors cannot hold the global constant pressure
+ * mode (E's smallest eigenvalue, set by the prescribed pressures alone, lies orders of
+ * magnitude below the others) beside the rest of a preconditioned vector, and the FP32
+ * tangent does not resolve it either: there the correction uses B, the leading block of
+ * [E b1; b1^T 0]^{-1} (its outputs sum to zero), and sfem_step (and the host driver)
+ * adds a coarse step in FP64 before each inner solve -- x += Z E^{-1} Z^T F(x), then the
+ * residual again (sfem_coarse_step).  Single meshes only (SFEM_ERR_UNSUPPORTED on a
+ * batch).  No counterpart in the reference, which preconditions with PETSc's ILU(0)
+ * alone (SolverPetsc::factoriseAndSolve, SolverPetsc.cpp:284-298). */
+int sfem_set_coarse_correction(sfem_ctx* ctx, int boxes_per_dim);
+/* info[6] = {k, coarse unknowns, free pressure DOFs, restriction chunks, 1 if the last E
+ * was singular (correction off until the next rebuild), 1 if built}. */
+int sfem_coarse_info(sfem_ctx* ctx, int64_t* info);
+/* which = 0: E^{-1}; which = 1: B (mixed precision only); of the last rebuild, row-major
+ * (cap >= unknowns^2 doubles). */
+int sfem_coarse_inverse(sfem_ctx* ctx, int which, double* out, int cap);
+/* The Newton loop's coarse step: the increment (sfem_get_increment, applied by
+ * sfem_update_solution) = Z E^{-1} Z^T rhs of the last assembly, in FP64, with the
+ * preconditioner brought up to date for opts first.  Needs the coarse correction. */
+int sfem_coarse_step(sfem_ctx* ctx, const sfem_solver_opts* opts);
+
 /* Inexact Newton for the mixed-precision paths (SFEM_PREC_MIXED, SFEM_PREC_STORE32): in
  * sfem_step each Newton iteration's inner solve stops at
  *   rtol_k = min(rtol_max, max(opts->rtol, theta * newton_tol / ||F_k||)),

@@ -1,0 +1,249 @@
+"""GPU tests of the additive coarse pressure correction of ILU(0), through the C ABI.
+
+The correction adds Z E^-1 Z^T r to every ILU(0) application, with Z the
+piecewise-constant pressure on k boxes per direction of the bounding box and
+E = Z^T A Z.  It has no counterpart in the reference (PETSc ILU(0) alone,
+SolverPetsc.cpp:284-298), so the checks are against what the definition fixes:
+E^-1 against numpy's inverse of Z^T A Z formed from the exported CSR tangent, the
+change of M^-1 r against Z E^-1 Z^T r, and converged Newton solutions against the
+uncorrected solver at the same tolerances (a preconditioner changes iteration
+counts, never the answer).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from cfdsfemmpi_amd import deck, meshgen, sfem
+
+pytestmark = pytest.mark.gpu
+
+ED = np.array([1.0, 1e-2, 0, 0, 0, 0, 0, 0, 1, 1, 0])
+
+
+def _ctx(mesh, prec=sfem.PREC_FP64):
+    ctx = sfem.Context(mesh)
+    ctx.set_dirichlet(mesh.dbc_node, mesh.dbc_dof, mesh.dbc_value)
+    ctx.prepare_elements()
+    ctx.build_pattern()
+    ctx.set_params(ED, 2, 0.0, 0.1)
+    ctx.set_precision(prec)
+    ctx.set_tuning_mode(sfem.TUNE_FIXED)
+    return ctx
+
+
+def _coarse_space(mesh, k):
+    """Z (free DOFs x coarse unknowns) by the documented rule: node box
+    b_d = min(k - 1, floor(k (x_d - min_d) / extent_d)), one column per box holding a free
+    pressure DOF, boxes in x-fastest order; and per free pressure row whether its node
+    shares an element with a node whose pressure is prescribed."""
+    nd = mesh.ndim
+    ndof = nd + 1
+    _, assy, nfree = oracle.dof_numbering(mesh.nNode, ndof, mesh.dbc_node, mesh.dbc_dof)
+    assy = np.asarray(assy, np.int64)
+    x = np.asarray(mesh.coords, np.float64).reshape(-1, nd)
+    lo, hi = x.min(0), x.max(0)
+    b = np.minimum(np.floor((x - lo) / (hi - lo) * k).astype(np.int64), k - 1)
+    lin = b[:, 0] + (k * b[:, 1] if nd > 1 else 0) + (k * k * b[:, 2] if nd > 2 else 0)
+    rows = np.nonzero(assy % ndof == ndof - 1)[0]
+    boxes = lin[assy[rows] // ndof]
+    uniq, col = np.unique(boxes, return_inverse=True)
+    Z = np.zeros((nfree, len(uniq)))
+    Z[rows, col] = 1.0
+    conn = np.asarray(mesh.conn, np.int64).reshape(mesh.nElem, -1)
+    fixed = np.zeros(mesh.nNode, bool)
+    dn, dd = np.asarray(mesh.dbc_node), np.asarray(mesh.dbc_dof)
+    fixed[dn[dd == ndof - 1]] = True
+    near = np.zeros(mesh.nNode, bool)
+    near[np.unique(conn[fixed[conn].any(1)])] = True
+    return Z, rows, col, near[assy[rows] // ndof]
+
+
+def _coarse_matrices(mesh, A, k):
+    """E = Z^T A Z summed directly, and E formed as the library documents it: the
+    couplings between different boxes summed directly, each diagonal entry as the row
+    sums of the rows next to a prescribed pressure minus the box's couplings to the other
+    boxes."""
+    Z, rows, col, near = _coarse_space(mesh, k)
+    E = Z.T @ (A @ Z)
+    App = A[rows][:, rows].tocoo()
+    nc = Z.shape[1]
+    ci, cj = col[App.row], col[App.col]
+    off = ci != cj
+    Eid = np.zeros((nc, nc))
+    np.add.at(Eid, (ci[off], cj[off]), App.data[off])
+    rs = np.asarray(A[rows][:, rows].sum(axis=1)).ravel()
+    diag = np.zeros(nc)
+    np.add.at(diag, col[near], rs[near])
+    Eid[np.arange(nc), np.arange(nc)] = diag - Eid.sum(axis=1)
+    return Z, E, Eid
+
+
+def _system(ctx):
+    import scipy.sparse as sp
+    rp, ci, _ = ctx.get_pattern()
+    vals, rhs = ctx.get_system()
+    n = ctx.nfree
+    return sp.csr_matrix((vals, ci, rp), shape=(n, n)), rhs
+
+
+CASES = [("hex", 6, 2), ("hex", 8, 3), ("quad", 16, 4), ("tria", 12, 3)]
+
+
+def _mesh(kind, n):
+    if kind == "hex":
+        return meshgen.ldc3d_hex(n, graded=True)
+    if kind == "quad":
+        return meshgen.ldc2d_quad(n, graded=True)
+    return meshgen.ldc2d_tria(n, jitter=0.2, seed=5)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("kind,n,k", CASES)
+def test_coarse_inverse_and_apply(kind, n, k, prec):
+    """E^-1 from the device (E assembled from the plan's block tangent, inverted by
+    Gauss-Jordan with partial pivoting) against numpy on the exported CSR tangent (FP64,
+    or the FP32 tangent widened).  FP64: the documented E equals Z^T A Z summed directly
+    within 1e-10 (the pressure rows of the stabilised tangent sum to zero away from
+    prescribed pressures) and E^-1 E = I within 1e-9.  Both: E^-1 within 1e-8 relative of
+    numpy's inverse of the documented E; M_c^-1 r - M^-1 r = Z E^-1 Z^T r within 1e-10 of
+    ||M_c^-1 r||, the same twice in a row bit for bit."""
+    mesh = _mesh(kind, n)
+    print(f"[case] coarse inverse/apply {kind} n={n} k={k} {prec}", flush=True)
+    ctx = _ctx(mesh, sfem.PREC_MIXED if prec == "mixed" else sfem.PREC_FP64)
+    ctx.time_update(1.0)
+    ctx.assemble(0)
+    opts = sfem.solver_opts("bcgs", "ilu", 1e-8 if prec == "fp64" else 1e-4, 2000, reorder=3)
+    ctx.solve(opts)
+    r = np.random.default_rng(3).standard_normal(ctx.nfree)
+    z0 = ctx.pc_apply(r)
+    ctx.set_coarse_correction(k)
+    ctx.solve(opts)
+    info = ctx.coarse_info()
+    A, _ = _system(ctx)
+    Z, E, Eid = _coarse_matrices(mesh, A, k)
+    nc = Z.shape[1]
+    assert info["built"] and not info["singular"]
+    assert info["unknowns"] == nc and info["pressure_dofs"] == int(Z.sum())
+    Einv = ctx.coarse_inverse()
+    Eref = np.linalg.inv(Eid)
+    if prec == "fp64":
+        eid = np.max(np.abs(Eid - E)) / np.max(np.abs(E))
+        e1 = np.max(np.abs(Einv @ Eid - np.eye(nc)))
+        assert eid <= 1e-10 and e1 <= 1e-9, (eid, e1)
+    e2 = np.max(np.abs(Einv - Eref)) / np.max(np.abs(Eref))
+    print(f"[coarse] nc={nc} cond={np.linalg.cond(Eid):.2e} rel={e2:.2e}", flush=True)
+    assert e2 <= 1e-8
+    if prec == "mixed":
+        # the FP32 Krylov vectors' inverse: leading block of [E b1; b1^T 0]^-1
+        bb = np.max(np.abs(np.diag(Eid)))
+        Bord = np.zeros((nc + 1, nc + 1))
+        Bord[:nc, :nc] = Eid
+        Bord[:nc, nc] = bb
+        Bord[nc, :nc] = bb
+        Bref = np.linalg.inv(Bord)[:nc, :nc]
+        B = ctx.coarse_inverse(1)
+        eb = np.max(np.abs(B - Bref)) / np.max(np.abs(Bref))
+        y = B @ np.random.default_rng(5).standard_normal(nc)
+        print(f"[coarse] mean-free inverse rel={eb:.2e}", flush=True)
+        assert eb <= 1e-8 and abs(y.sum()) <= 1e-9 * np.abs(y).sum()
+    else:
+        with pytest.raises(sfem.SfemError):
+            ctx.coarse_inverse(1)
+    z1 = ctx.pc_apply(r)
+    z2 = ctx.pc_apply(r)
+    assert np.array_equal(z1, z2)
+    corr = Z @ (Eref @ (Z.T @ r))
+    d = np.linalg.norm((z1 - z0) - corr) / np.linalg.norm(z1)
+    print(f"[coarse] apply diff {d:.2e}, |corr|/|z| {np.linalg.norm(corr) / np.linalg.norm(z1):.2e}", flush=True)
+    assert np.linalg.norm(corr) > 1e-6 * np.linalg.norm(z1)
+    assert d <= 1e-10
+    ctx.close()
+
+
+@pytest.mark.parametrize("prec", [sfem.PREC_FP64, sfem.PREC_MIXED])
+def test_coarse_step_increment(prec):
+    """sfem_coarse_step: the increment equals Z E^-1 Z^T rhs (numpy, from the exported
+    tangent and residual) within 1e-10 relative."""
+    mesh = meshgen.ldc3d_hex(7, graded=True)
+    ctx = _ctx(mesh, prec)
+    ctx.time_update(1.0)
+    ctx.assemble(0)
+    opts = sfem.solver_opts("bcgs", "ilu", 1e-6, 2000, reorder=3)
+    ctx.set_coarse_correction(2)
+    ctx.coarse_step(opts)
+    du = ctx.get_increment()
+    A, rhs = _system(ctx)
+    Z, _, Eid = _coarse_matrices(mesh, A, 2)
+    ref = Z @ np.linalg.solve(Eid, Z.T @ rhs)
+    d = np.linalg.norm(du - ref) / np.linalg.norm(ref)
+    print(f"[coarse step] prec={prec} rel diff {d:.2e}", flush=True)
+    assert d <= 1e-10
+    ctx.close()
+
+
+def _cavity(n, steps, prec, k, rtol, newton_tol):
+    mesh = meshgen.ldc3d_hex(n, graded=True)
+    ctl = deck.Control(np.array([1.0, 1e-3, 0, 0, 0, 0, 0, 0, 1, 1, 0]), 2, 0.0, 0.1, 1e9, steps, 1, 1e-6)
+    fem = sfem.StabFEM()
+    fem.setMesh(mesh)
+    fem.setControl(ctl)
+    fem.precision = prec
+    fem.newton_tol = newton_tol
+    fem.tuning = sfem.TUNE_FIXED
+    fem.coarse_boxes = k
+    fem.prepareInputData()
+    fem.setSolver(sfem.solver_opts("bcgs", "ilu", rtol, 5000, reorder=3))
+    return fem
+
+
+def _fields(s):
+    s = s.reshape(-1, 4)
+    return s[:, :3], s[:, 3]
+
+
+@pytest.mark.parametrize("prec", [sfem.PREC_FP64, sfem.PREC_MIXED])
+def test_coarse_cavity_same_answer_fewer_iterations(prec):
+    """Graded 12^3 cavity at Re 1000, 4 steps, against the FP64 solver without the
+    correction, both driven to ||F|| < 1e-11: velocity within 1e-8 and pressure within
+    1e-6 (relative L2; pressure is the weakly observed field of this PSPG system), and
+    no more Krylov iterations in total than the same precision without it (the mixed
+    path adds the FP64 coarse step before each FP32 inner solve)."""
+    n, steps = 12, 4
+    ref = _cavity(n, steps, sfem.PREC_FP64, 0, 1e-12, 1e-11)
+    s_ref = ref.solveFullyImplicit()
+    base = _cavity(n, steps, prec, 0, 1e-12 if prec == sfem.PREC_FP64 else 1e-4, 1e-11)
+    base.solveFullyImplicit()
+    co = _cavity(n, steps, prec, 4, 1e-12 if prec == sfem.PREC_FP64 else 1e-4, 1e-11)
+    s_co = co.solveFullyImplicit()
+    its_base = sum(h[3] for h in base.history)
+    its_co = sum(h[3] for h in co.history)
+    u0, p0 = _fields(s_ref)
+    u1, p1 = _fields(s_co)
+    eu = np.linalg.norm(u1 - u0) / np.linalg.norm(u0)
+    ep = np.linalg.norm(p1 - p0) / np.linalg.norm(p0)
+    print(f"[coarse cavity] prec={prec} its {its_base} -> {its_co}; rel L2 u {eu:.2e} p {ep:.2e}", flush=True)
+    assert all(h[2] < 1e-11 for h in co.history if h[3] == 0)
+    assert eu <= 1e-8 and ep <= 1e-6
+    assert its_co <= its_base
+    for f in (ref, base, co):
+        f.ctx.close()
+
+
+def test_coarse_rejections():
+    """Too many boxes, boxes narrower than an element, and batches are refused."""
+    mesh = meshgen.ldc3d_hex(4, graded=True)
+    ctx = _ctx(mesh)
+    with pytest.raises(sfem.SfemError):
+        ctx.set_coarse_correction(11)  # 1331 boxes > 1024
+    with pytest.raises(sfem.SfemError):
+        ctx.set_coarse_correction(8)   # boxes of 1/8 < the 4^3 mesh's elements
+    ctx.set_coarse_correction(2)
+    ctx.set_coarse_correction(0)
+    assert ctx.coarse_info()["k"] == 0
+    ctx.close()
+    m2 = meshgen.ldc2d_tria(6, jitter=0.2, seed=1)
+    ctx = _ctx(m2)
+    ctx.set_batch(np.zeros(m2.nElem, np.int32), np.array([1e-2]))
+    with pytest.raises(sfem.SfemError):
+        ctx.set_coarse_correction(2)
+    ctx.close()
