@@ -698,9 +698,7 @@ def main():
     info = main_res["info"]
     nnz, nf = info["nnz"], info["nfree"]
     fp32 = main_res is rmx
-    sm = 4 if fp32 else 8                      # matrix values
-    sv = 4 if (fp32 and not a.coarse) else 8   # Krylov vectors (FP64 under the coarse correction)
-    vt = "float" if sv == 4 else "double"
+    sm = sv = 4 if fp32 else 8   # matrix values and Krylov vectors
 
     def hbm_roof(kind, name, bytes_per):
         cnt, ms = kt[kind]
@@ -716,9 +714,9 @@ def main():
     # + 2 s n is reported beside it.
     bi = main_res["block"]
     spmv_bytes = nnz * sm + 4 * bi["npairs"] + 16 * bi["nblk"] + 2 * sv * nf
-    roof = hbm_roof("spmv", f"k_bsr_spmv<{'float' if fp32 else 'double'}, {vt}> (node-block SpMV)", spmv_bytes)
+    roof = hbm_roof("spmv", f"k_bsr_spmv<{'float' if fp32 else 'double'}> (node-block SpMV)", spmv_bytes)
     roof["csr_equivalent_bytes_per_launch"] = nnz * (sm + 4) + 4 * (nf + 1) + 2 * sv * nf
-    roof["traffic"] = MEASURED_TRAFFIC.get(("k_bsr_spmv", "float" if fp32 else "double", n)) if sv == sm else None
+    roof["traffic"] = MEASURED_TRAFFIC.get(("k_bsr_spmv", "float" if fp32 else "double", n))
     roof["traffic_source"] = TRAFFIC_SOURCE if roof["traffic"] else None
     # ILU(0) apply: both sweeps together read every factor value, block index and
     # descriptor once; r read, y written, y read and written, plus the gathered y once
@@ -783,14 +781,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": (("fp32 tangent and matrix" + (", bf16 ILU factors" if a.bf16_factors else "") +
-                   ", fp64 Krylov vectors (coarse correction) / fp64 residual") if a.coarse else
-                  ("fp32 tangent+Krylov" + (" (bf16 ILU factors)" if a.bf16_factors else "") + " / fp64 residual"))
-                 if fp32 else "f64",
+        "dtype": ("fp32 tangent+Krylov" + (" (bf16 ILU factors)" if a.bf16_factors else "") +
+                  (", fp64 coarse step" if a.coarse else "") + " / fp64 residual") if fp32 else "f64",
         "data": "synthetic (generated mesh, zero initial state)",
         "config": {"workload": f"3D lid-driven cavity, {n}^3 graded Hexa8, Re=1000, gen-alpha rho_inf=0, "
                                f"dt=0.1, {a.steps} steps", "n_elements": mesh.nElem, "nfree": nf, "nnz": nnz,
-                   "solver": ("BiCGStab" if a.ksp == "bcgs" else f"GMRES({a.restart})") + " + ILU(0)", "ilu_ordering": ["natural", "rcm", "multicolour", "tiled multicolour"][a.reorder],
+                   "solver": ("BiCGStab" if a.ksp == "bcgs" else f"GMRES({a.restart})") + " + ILU(0)" +
+                             (f" + coarse pressure correction ({a.coarse}^3 boxes)" if a.coarse else ""),
+                   "ilu_ordering": ["natural", "rcm", "multicolour", "tiled multicolour"][a.reorder],
                    "newton_tol": a.newton_tol, "newton_tol_mixed": a.newton_tol_mixed,
                    "inner_rtol_mixed": a.inner_rtol, "ilu_factors_mixed": "bf16" if a.bf16_factors else "fp32",
                    "newton_initial_iterate": a.predictor, "pc_lag": a.pc_lag,
