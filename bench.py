@@ -11,8 +11,9 @@ Three runs of K time steps from the zero state, each after W warm-up steps (the
 state is reset after warm-up):
   * FP64 plain: the reference's solver configuration, FP64 tangent and CSR, BiCGStab +
     ILU(0) at rtol 1e-8, the factor rebuilt for every new tangent;
-  * FP64: the same with the solver options of the mixed run: ILU(0) plus the additive
-    coarse pressure correction on 8^3 boxes, the factor rebuilt on every 6th tangent;
+  * FP64: the same with the solver options of the mixed run: ILU(0) with the
+    multiplicative coarse pressure correction on 8^3 boxes, the factor rebuilt on every
+    6th tangent;
   * mixed: FP32 element tangents and FP32 CSR, FP32 BiCGStab + ILU(0) (factor and
     tile inverses stored in BF16) with the coarse correction without its global
     constant mode, inside an FP64 residual / Newton update loop (defect correction) to
@@ -63,7 +64,7 @@ def make_case(n):
 
 
 def run_path(mesh, ctl, prec, opts, newton_tol, steps, warmup, label, predictor=0, pc_lag=1, inner=(0.0, 0.5),
-             tuning=0, coarse=0):
+             tuning=0, coarse=0, coarse_mult=False):
     from cfdsfemmpi_amd import sfem
     t0 = time.time()
     fem = sfem.StabFEM(device=0)
@@ -72,6 +73,7 @@ def run_path(mesh, ctl, prec, opts, newton_tol, steps, warmup, label, predictor=
     fem.precision = prec
     fem.tuning = tuning
     fem.coarse_boxes = coarse
+    fem.coarse_mult = coarse_mult
     fem.prepareInputData()
     fem.setSolver(opts)
     ctx = fem.ctx
@@ -627,6 +629,8 @@ def main():
     ap.add_argument("--ksp", choices=["bcgs", "gmres"], default="bcgs", help="Krylov method of both paths")
     ap.add_argument("--coarse", type=int, default=8,
                     help="fp64 and mixed paths: additive coarse pressure correction of ILU(0) on n^3 boxes (0: off)")
+    ap.add_argument("--coarse-mult", type=int, default=1,
+                    help="the coarse correction's multiplicative form (0: additive)")
     ap.add_argument("--plain", type=int, default=1,
                     help="also time the FP64 path with the reference's solver configuration (ILU(0) alone, "
                          "rebuilt for every tangent)")
@@ -670,11 +674,11 @@ def main():
         log(f"fp64_plain: {rpl['steps_per_s']:.3f} steps/s")
     if a.only != "mixed":
         r64 = run_path(mesh, ctl, sfem.PREC_FP64, o64, a.newton_tol, a.steps, a.warmup, "fp64", pred, a.pc_lag,
-                       tuning=tune_mode, coarse=a.coarse)
+                       tuning=tune_mode, coarse=a.coarse, coarse_mult=bool(a.coarse_mult))
         log(f"fp64: {r64['steps_per_s']:.3f} steps/s")
     if a.only != "fp64":
         rmx = run_path(mesh, ctl, sfem.PREC_MIXED, omx, a.newton_tol_mixed, a.steps, a.warmup, "mixed", pred,
-                       a.pc_lag, (a.inner_theta, a.inner_rtol_max), tune_mode, a.coarse)
+                       a.pc_lag, (a.inner_theta, a.inner_rtol_max), tune_mode, a.coarse, bool(a.coarse_mult))
         log(f"mixed: {rmx['steps_per_s']:.3f} steps/s")
     main_res = rmx or r64
     err_u = err_p = None
@@ -737,7 +741,12 @@ def main():
         # coarse correction: r read, z read and written at the pressure slots, their slot
         # indices read twice
         pc_bytes += co["pressure_dofs"] * (3 * sv + 8)
-        pc_name += f", + coarse pressure correction ({co['unknowns']} unknowns, 3 launches)"
+        if a.coarse_mult:
+            # multiplicative: the product's pressure slots read once more, and A Z y added to
+            # the product (its index and value entries, and the product, read and written)
+            pc_bytes += co["pressure_dofs"] * sv + 2 * sv * nf
+        pc_name += (f", + {'multiplicative' if a.coarse_mult else 'additive'} coarse pressure correction "
+                    f"({co['unknowns']} unknowns, {4 if a.coarse_mult else 3} launches)")
     roof_pc = hbm_roof("pc", pc_name, pc_bytes)
     roof_pc["tiles"] = ti
     n_k = len(main_res["hist"])
@@ -794,6 +803,7 @@ def main():
                    "newton_initial_iterate": a.predictor, "pc_lag": a.pc_lag,
                    "inner_theta_mixed": a.inner_theta, "inner_rtol_max_mixed": a.inner_rtol_max,
                    "kernel_shapes": a.tuning, "coarse_pressure_boxes": a.coarse,
+                   "coarse_form": "multiplicative" if a.coarse_mult else "additive",
                    "coarse_unknowns": main_res["coarse"]["unknowns"], "parallelism": "single GPU"},
         # fp64: the FP64 path with the same solver options as the mixed run (ILU(0), coarse
         # pressure correction, preconditioner lag); fp64_plain: the reference's solver

@@ -87,9 +87,18 @@ struct Coarse {
   double* d_cpart = nullptr;      // nchunk restricted chunk sums
   double* d_y = nullptr;          // nc coarse solution
   int32_t* d_flag = nullptr;      // 1: E was singular (correction switched off)
+  // A Z for the multiplicative form: per block row the coarse unknowns of its neighbours'
+  // pressure DOFs (d_azoff[p] .. d_azoff[p+1] in d_azbox, ascending) and, per entry, the 4
+  // rows' sums of A over that unknown's pressure columns (FP64 or FP32, like the tangent)
+  int64_t naz = 0;
+  int32_t* d_azoff = nullptr;
+  int32_t* d_azbox = nullptr;
+  void* d_az = nullptr;
+  bool az_fp32 = false;
+  void* d_tmp = nullptr;          // A z of a standalone multiplicative application (npad x 8 bytes)
   ~Coarse() {
     void* ptrs[] = {d_slot, d_chunk, d_chunkbox, d_boxchunk, d_cinfo, d_cpos, d_boxnb,
-                    d_epart, d_gj, d_einv, d_einvb, d_cpart, d_y, d_flag};
+                    d_epart, d_gj, d_einv, d_einvb, d_cpart, d_y, d_flag, d_azoff, d_azbox, d_az, d_tmp};
     for (void* q : ptrs)
       if (q) (void)hipFree(q);
   }
@@ -258,17 +267,77 @@ __global__ void __launch_bounds__(256) k_coarse_finish(int nc, int border, const
   if (tid == 0) *flag = s_bad;
 }
 
-// Restriction: chunk sums of r over the chunk's pressure slots (lanes in order, then a
-// fixed tree over the wavefront).
+// Restriction: chunk sums of r (or of r - w) over the chunk's pressure slots (lanes in
+// order, then a fixed tree over the wavefront).
 template <typename T>
 __global__ void __launch_bounds__(64) k_coarse_restrict(const int32_t* __restrict__ slot,
                                                          const int32_t* __restrict__ chunk, const T* __restrict__ r,
-                                                         double* __restrict__ cpart) {
+                                                         const T* __restrict__ w, double* __restrict__ cpart) {
   const int lane = threadIdx.x, ch = blockIdx.x;
   double s = 0.0;
-  for (int i = chunk[ch] + lane; i < chunk[ch + 1]; i += 64) s += (double)r[slot[i]];
+  for (int i = chunk[ch] + lane; i < chunk[ch + 1]; i += 64) {
+    const int q = slot[i];
+    s += w ? (double)r[q] - (double)w[q] : (double)r[q];
+  }
   for (int h = 32; h > 0; h >>= 1) s += __shfl_xor(s, h);
   if (lane == 0) cpart[ch] = s;
+}
+
+// A Z: for block row p and each coarse unknown of its neighbours' pressure DOFs, the 4
+// rows' sums of the block tangent over that unknown's pressure columns, neighbours in
+// order.  One thread per block row (once per preconditioner rebuild).
+template <typename VT>
+__global__ void __launch_bounds__(256) k_coarse_az_build(int64_t nblk, const int4* __restrict__ bdesc,
+                                                          const int32_t* __restrict__ bcol, const VT* __restrict__ kp,
+                                                          const int32_t* __restrict__ cinfo,
+                                                          const int32_t* __restrict__ azoff,
+                                                          const int32_t* __restrict__ azbox, VT* __restrict__ az) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= nblk) return;
+  const int4 d = bdesc[p];
+  const int vbase = d.x, bptr = d.y, nb = d.z, m = d.w >> 16;
+  for (int k = azoff[p]; k < azoff[p + 1]; ++k) {
+    const int c = azbox[k];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    for (int t = 0; t < nb; ++t) {
+      const int info = cinfo[bcol[bptr + t]];
+      if (info < 0 || (info >> 3) != c) continue;
+      const VT* col = kp + vbase + 4 * t + (info & 3);
+      a0 += (double)col[0];
+      if (m > 1) a1 += (double)col[4 * nb];
+      if (m > 2) a2 += (double)col[8 * nb];
+      if (m > 3) a3 += (double)col[12 * nb];
+    }
+    VT* o = az + 4 * (int64_t)k;
+    o[0] = (VT)a0;
+    o[1] = (VT)a1;
+    o[2] = (VT)a2;
+    o[3] = (VT)a3;
+  }
+}
+
+// v += A Z y: one thread per block row (rows a >= m carry 0, so padding slots stay 0).
+template <typename T, typename VT>
+__global__ void __launch_bounds__(256) k_coarse_az_apply(int64_t nblk, const int32_t* __restrict__ azoff,
+                                                          const int32_t* __restrict__ azbox,
+                                                          const VT* __restrict__ az, const double* __restrict__ y,
+                                                          T* __restrict__ v) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= nblk) return;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  for (int k = azoff[p]; k < azoff[p + 1]; ++k) {
+    const double yk = y[azbox[k]];
+    const VT* o = az + 4 * (int64_t)k;
+    a0 += (double)o[0] * yk;
+    a1 += (double)o[1] * yk;
+    a2 += (double)o[2] * yk;
+    a3 += (double)o[3] * yk;
+  }
+  T* vp = v + 4 * p;
+  vp[0] = (T)((double)vp[0] + a0);
+  vp[1] = (T)((double)vp[1] + a1);
+  vp[2] = (T)((double)vp[2] + a2);
+  vp[3] = (T)((double)vp[3] + a3);
 }
 
 // Coarse solve: box sums of the chunk sums (each workgroup, one lane per box, chunks in
@@ -460,7 +529,31 @@ static int coarse_plan(sfem_ctx* ctx, LinAlg* L, Coarse* C) {
       boxnb[(size_t)c * kOffs + o] = coarse_of(x | y << 10 | z << 20);
     }
   }
+  // A Z's sparsity: per block row the coarse unknowns of its neighbours' pressure DOFs
+  std::vector<int4> hbd(nb);
+  std::vector<int32_t> hbcol(P.npairs);
+  CO_CHECK(hipMemcpy(hbd.data(), P.d_bdesc, nb * sizeof(int4), hipMemcpyDeviceToHost));
+  CO_CHECK(hipMemcpy(hbcol.data(), P.d_bcol, P.npairs * sizeof(int32_t), hipMemcpyDeviceToHost));
+  std::vector<int32_t> azoff(nb + 1, 0), azbox;
+  azbox.reserve(nb + nb / 2);
+  for (int64_t p = 0; p < nb; ++p) {
+    int32_t cs[64];
+    int ncs = 0;
+    for (int t = 0; t < hbd[p].z; ++t) {
+      const int32_t info = cinfo[hbcol[hbd[p].y + t]];
+      if (info < 0) continue;
+      const int32_t c = info >> 3;
+      bool seen = false;
+      for (int u = 0; u < ncs; ++u) seen = seen || cs[u] == c;
+      if (!seen && ncs < 64) cs[ncs++] = c;
+    }
+    std::sort(cs, cs + ncs);
+    azbox.insert(azbox.end(), cs, cs + ncs);
+    azoff[p + 1] = (int32_t)azbox.size();
+  }
   int rc;
+  if ((rc = upload(&C->d_azoff, azoff)) || (rc = upload(&C->d_azbox, azbox))) return rc;
+  C->naz = (int64_t)azbox.size();
   if ((rc = upload(&C->d_slot, slot)) || (rc = upload(&C->d_chunk, chunk)) || (rc = upload(&C->d_chunkbox, chunkbox)) ||
       (rc = upload(&C->d_boxchunk, boxchunk)) || (rc = upload(&C->d_cinfo, cinfo)) || (rc = upload(&C->d_cpos, boxes)) ||
       (rc = upload(&C->d_boxnb, boxnb)))
@@ -468,7 +561,7 @@ static int coarse_plan(sfem_ctx* ctx, LinAlg* L, Coarse* C) {
   C->nc = nc;
   C->npres = npres;
   C->nchunk = (int)chunkbox.size();
-  void* old[] = {C->d_epart, C->d_gj, C->d_einv, C->d_einvb, C->d_cpart, C->d_y, C->d_flag};
+  void* old[] = {C->d_epart, C->d_gj, C->d_einv, C->d_einvb, C->d_cpart, C->d_y, C->d_flag, C->d_az, C->d_tmp};
   C->d_einvb = nullptr;
   for (void* q : old)
     if (q) (void)hipFree(q);
@@ -478,12 +571,16 @@ static int coarse_plan(sfem_ctx* ctx, LinAlg* L, Coarse* C) {
   CO_CHECK(hipMalloc((void**)&C->d_cpart, (size_t)C->nchunk * sizeof(double)));
   CO_CHECK(hipMalloc((void**)&C->d_y, (size_t)nc * sizeof(double)));
   CO_CHECK(hipMalloc((void**)&C->d_flag, sizeof(int32_t)));
+  CO_CHECK(hipMalloc(&C->d_az, std::max<size_t>((size_t)C->naz, 1) * 4 * sizeof(double)));
+  CO_CHECK(hipMalloc(&C->d_tmp, (size_t)P.npad * sizeof(double)));
+  CO_CHECK(hipMemset(C->d_tmp, 0, (size_t)P.npad * sizeof(double)));
   C->plan_id = L->plan_id;
   return SFEM_OK;
 }
 
-int coarse_set(sfem_ctx* ctx, int k) {
+int coarse_set(sfem_ctx* ctx, int k, int mult) {
   LinAlg* L = ctx->la;
+  L->coarse_mult = k > 0 && mult != 0;
   if (k == 0) {
     delete L->coarse;
     L->coarse = nullptr;
@@ -524,6 +621,16 @@ int coarse_build(sfem_ctx* ctx, const double* kp, const float* kp32) {
   else
     hipLaunchKernelGGL((k_coarse_rows<double>), dim3(C->nchunk), dim3(64), 0, st, C->d_slot, C->d_chunk, bd, P.d_bcol,
                        kp, C->d_cinfo, C->d_cpos, C->d_epart);
+  if (L->coarse_mult) {
+    const unsigned gz = (unsigned)((P.nblk + 255) / 256);
+    if (kp32)
+      hipLaunchKernelGGL((k_coarse_az_build<float>), dim3(gz), dim3(256), 0, st, P.nblk, bd, P.d_bcol, kp32,
+                         C->d_cinfo, C->d_azoff, C->d_azbox, (float*)C->d_az);
+    else
+      hipLaunchKernelGGL((k_coarse_az_build<double>), dim3(gz), dim3(256), 0, st, P.nblk, bd, P.d_bcol, kp,
+                         C->d_cinfo, C->d_azoff, C->d_azbox, (double*)C->d_az);
+    C->az_fp32 = kp32 != nullptr;
+  }
   const int nc = C->nc;
   hipLaunchKernelGGL(k_coarse_fill, dim3((nc + 63) / 64), dim3(64), 0, st, nc, C->d_boxchunk, C->d_boxnb, C->d_epart,
                      C->d_gj);
@@ -543,12 +650,38 @@ int coarse_build(sfem_ctx* ctx, const double* kp, const float* kp32) {
 bool coarse_active(const LinAlg* L) { return L->coarse && L->coarse->ready && L->coarse->plan_id == L->plan_id; }
 
 template <typename T>
-int coarse_restrict(sfem_ctx* ctx, const T* r) {
+int coarse_restrict(sfem_ctx* ctx, const T* r, const T* w) {
   Coarse* C = ctx->la->coarse;
-  hipLaunchKernelGGL((k_coarse_restrict<T>), dim3(C->nchunk), dim3(64), 0, ctx->stream, C->d_slot, C->d_chunk, r,
+  hipLaunchKernelGGL((k_coarse_restrict<T>), dim3(C->nchunk), dim3(64), 0, ctx->stream, C->d_slot, C->d_chunk, r, w,
                      C->d_cpart);
   return SFEM_OK;
 }
+
+// v += A Z y with the y of the last coarse_correct (the multiplicative form's update of the
+// product that followed the preconditioner).
+template <typename T>
+int coarse_az(sfem_ctx* ctx, T* v) {
+  Coarse* C = ctx->la->coarse;
+  const int64_t nb = ctx->la->ilu->nblk;
+  const unsigned g = (unsigned)((nb + 255) / 256);
+  if (C->az_fp32)
+    hipLaunchKernelGGL((k_coarse_az_apply<T, float>), dim3(g), dim3(256), 0, ctx->stream, nb, C->d_azoff, C->d_azbox,
+                       (const float*)C->d_az, C->d_y, v);
+  else
+    hipLaunchKernelGGL((k_coarse_az_apply<T, double>), dim3(g), dim3(256), 0, ctx->stream, nb, C->d_azoff,
+                       C->d_azbox, (const double*)C->d_az, C->d_y, v);
+  return SFEM_OK;
+}
+
+template <typename T>
+T* coarse_tmp(sfem_ctx* ctx) {
+  return (T*)ctx->la->coarse->d_tmp;
+}
+
+template int coarse_az<float>(sfem_ctx*, float*);
+template int coarse_az<double>(sfem_ctx*, double*);
+template float* coarse_tmp<float>(sfem_ctx*);
+template double* coarse_tmp<double>(sfem_ctx*);
 
 template <typename T>
 int coarse_correct(sfem_ctx* ctx, T* z) {
@@ -562,8 +695,8 @@ int coarse_correct(sfem_ctx* ctx, T* z) {
   return SFEM_OK;
 }
 
-template int coarse_restrict<float>(sfem_ctx*, const float*);
-template int coarse_restrict<double>(sfem_ctx*, const double*);
+template int coarse_restrict<float>(sfem_ctx*, const float*, const float*);
+template int coarse_restrict<double>(sfem_ctx*, const double*, const double*);
 template int coarse_correct<float>(sfem_ctx*, float*);
 template int coarse_correct<double>(sfem_ctx*, double*);
 

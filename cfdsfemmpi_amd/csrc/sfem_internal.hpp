@@ -108,7 +108,7 @@ struct KernelTimer {
   bool on = false;
   std::vector<hipEvent_t> pool;          // free events
   std::vector<hipEvent_t> open;          // begin events awaiting their end, by kind
-  struct Pending { int kind; hipEvent_t a, b; };
+  struct Pending { int kind; hipEvent_t a, b; bool counted; };
   std::vector<Pending> pending;
   int64_t count[SFEM_KT_NKINDS] = {0, 0, 0};
   double ms[SFEM_KT_NKINDS] = {0.0, 0.0, 0.0};
@@ -125,18 +125,23 @@ struct KernelTimer {
     begin_ev[kind] = get();
     (void)hipEventRecord(begin_ev[kind], s);
   }
-  void end(int kind, hipStream_t s) {
+  // counted = false adds the bracket's time to the previous counted one of its kind (a
+  // second part of the same operation)
+  void end(int kind, hipStream_t s, bool counted = true) {
     if (!on || !begin_ev[kind]) return;
     hipEvent_t b = get();
     (void)hipEventRecord(b, s);
-    pending.push_back({kind, begin_ev[kind], b});
+    pending.push_back({kind, begin_ev[kind], b, counted});
     begin_ev[kind] = nullptr;
   }
   // call only when the stream has drained past every recorded event
   void resolve() {
     for (auto& p : pending) {
       float t = 0.f;
-      if (hipEventElapsedTime(&t, p.a, p.b) == hipSuccess) { ms[p.kind] += t; count[p.kind] += 1; }
+      if (hipEventElapsedTime(&t, p.a, p.b) == hipSuccess) {
+        ms[p.kind] += t;
+        if (p.counted) count[p.kind] += 1;
+      }
       pool.push_back(p.a);
       pool.push_back(p.b);
     }

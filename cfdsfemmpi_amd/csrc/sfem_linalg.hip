@@ -2535,6 +2535,8 @@ int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f, 
 
 template <typename T>
 static int pc_apply_impl(sfem_ctx* ctx, const T* r, T* z, const RowSubset* sub);
+template <typename T>
+static void spmv_in(sfem_ctx* ctx, const T* x, T* y, bool block, const RowSubset* sub = nullptr);
 
 template <typename T>
 static int pc_apply(sfem_ctx* ctx, const T* r, T* z, const RowSubset* sub = nullptr) {
@@ -2686,10 +2688,19 @@ static int pc_apply_impl(sfem_ctx* ctx, const T* r, T* z, const RowSubset* sub) 
     }
     case SFEM_PC_ILU0:
       if (!ilu_buffers_ready(L)) return set_error(SFEM_ERR_STATE, "ILU(0) factor buffers missing for its precision");
-      if (!sub && coarse_active(L)) {
+      if (!sub && coarse_active(L) && !L->coarse_mult) {
         // additive coarse correction: r restricted before the sweeps (z may alias r)
         coarse_restrict<T>(ctx, r);
         ilu_apply<T>(ctx, r, z, -1, nullptr, sub);
+        coarse_correct<T>(ctx, z);
+      } else if (!sub && coarse_active(L)) {
+        // multiplicative: z = M^-1 r, then z += Z E^-1 Z^T (r - A z) (the product into the
+        // correction's own buffer; BiCGStab uses the fused form in pc_op instead)
+        if (r == z) return set_error(SFEM_ERR_ARG, "multiplicative coarse correction: r and z must differ");
+        T* az = coarse_tmp<T>(ctx);
+        ilu_apply<T>(ctx, r, z, -1, nullptr, sub);
+        spmv_in<T>(ctx, z, az, true);
+        coarse_restrict<T>(ctx, r, az);
         coarse_correct<T>(ctx, z);
       } else {
         ilu_apply<T>(ctx, r, z, -1, nullptr, sub);
@@ -2754,7 +2765,7 @@ static void bsr_spmv_launch(hipStream_t st, int code, const IluPlan& P, const VT
 // y = K x in the DOF layout (block = false, CSR) or in the ILU(0) plan's padded block
 // layout (block = true).
 template <typename T>
-static void spmv_in(sfem_ctx* ctx, const T* x, T* y, bool block, const RowSubset* sub = nullptr) {
+static void spmv_in(sfem_ctx* ctx, const T* x, T* y, bool block, const RowSubset* sub) {
   LinAlg* L = ctx->la;
   const int slot = sizeof(T) == 8 ? 0 : 1;
   if (L->nblk == 0) return;
@@ -3026,6 +3037,32 @@ constexpr int kStagnated = -100;
 constexpr int kBicgStall = 100;  // stall window of the FP32 inner BiCGStab
 constexpr int kMaxReplacements = 8;  // true-residual replacements per BiCGStab solve
 
+// z = M^-1 r and out = A z.  With the multiplicative coarse correction the product of the
+// ILU(0) output is formed once and reused: z0 = M_ILU^-1 r, out0 = A z0, y = E^-1 Z^T
+// (r - out0), z = z0 + Z y, out = out0 + (A Z) y, with A Z kept per block row.
+template <typename T>
+static int pc_op(sfem_ctx* ctx, const T* r, T* z, T* out) {
+  LinAlg* L = ctx->la;
+  if (!(L->pc_type == SFEM_PC_ILU0 && L->coarse_mult && coarse_active(L))) {
+    int rc = pc_apply<T>(ctx, r, z);
+    if (rc) return rc;
+    spmv<T>(ctx, z, out);
+    return SFEM_OK;
+  }
+  if (!ilu_buffers_ready(L)) return set_error(SFEM_ERR_STATE, "ILU(0) factor buffers missing for its precision");
+  hipStream_t st = ctx->stream;
+  ctx->kt.begin(SFEM_KT_PC, st);
+  ilu_apply<T>(ctx, r, z);
+  ctx->kt.end(SFEM_KT_PC, st);
+  spmv_in<T>(ctx, z, out, true);
+  ctx->kt.begin(SFEM_KT_PC, st);
+  coarse_restrict<T>(ctx, r, out);
+  coarse_correct<T>(ctx, z);
+  coarse_az<T>(ctx, out);
+  ctx->kt.end(SFEM_KT_PC, st, false);
+  return hipGetLastError() == hipSuccess ? SFEM_OK : set_error(SFEM_ERR_HIP, "preconditioner launch");
+}
+
 template <typename T>
 static int bicgstab(sfem_ctx* ctx, const T* b, T* x, double rtol, int maxit, int* iters,
                     double* relres, T* r, T* rh, T* p, T* v, T* s, T* t, T* ph, T* sh, int stall = 0) {
@@ -3072,14 +3109,12 @@ static int bicgstab(sfem_ctx* ctx, const T* b, T* x, double rtol, int maxit, int
     // rho1 = (rh, r) and beta were formed with the previous iteration's residual update
     hipLaunchKernelGGL((k_bicg_p<T>), dim3(g), dim3(kTPB), 0, st, n, r, p, v, sc, first ? 1 : 0);
     first = false;
-    int rc = pc_apply<T>(ctx, p, ph); if (rc) return rc;
-    spmv<T>(ctx, ph, v);
+    int rc = pc_op<T>(ctx, p, ph, v); if (rc) return rc;
     hipLaunchKernelGGL((k_dot2<T>), dim3(g), dim3(kTPB), 0, st, n, rh, v, v, v, L->d_part);
     hipLaunchKernelGGL((k_finalize<2>), dim3(1), dim3(kTPB), 0, st, L->d_part, g, sc, 0, (int)FIN_ALPHA);
     hipLaunchKernelGGL((k_bicg_s<T>), dim3(g), dim3(kTPB), 0, st, n, r, v, s, sc, L->d_part);
     hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(kTPB), 0, st, L->d_part, g, sc, 0, (int)FIN_SS);
-    rc = pc_apply<T>(ctx, s, sh); if (rc) return rc;
-    spmv<T>(ctx, sh, t);
+    rc = pc_op<T>(ctx, s, sh, t); if (rc) return rc;
     hipLaunchKernelGGL((k_dot2<T>), dim3(g), dim3(kTPB), 0, st, n, t, t, t, s, L->d_part);
     hipLaunchKernelGGL((k_finalize<2>), dim3(1), dim3(kTPB), 0, st, L->d_part, g, sc, 0, (int)FIN_OMEGA);
     hipLaunchKernelGGL((k_bicg_xr<T>), dim3(g), dim3(kTPB), 0, st, n, x, ph, sh, r, s, t, rh, sc, L->d_part);

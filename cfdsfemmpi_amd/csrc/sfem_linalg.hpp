@@ -145,6 +145,7 @@ struct LinAlg {
   float* kp32 = nullptr;
   DirectLU* direct = nullptr;  // direct solver (pc_type LU), set up on first use
   int coarse_k = 0;            // coarse pressure boxes per direction (0: no correction)
+  bool coarse_mult = false;    // multiplicative form z + Z E^-1 Z^T (r - A z) instead of additive
   Coarse* coarse = nullptr;
   ~LinAlg();
 };
@@ -204,10 +205,12 @@ int la_direct_solve(sfem_ctx* ctx, int* iters, double* relres);
 int la_direct_info(sfem_ctx* ctx, int* kl, int* ku);
 void la_direct_destroy(LinAlg* L);
 // coarse pressure correction (sfem_coarse.hip)
-int coarse_set(sfem_ctx* ctx, int k);
+int coarse_set(sfem_ctx* ctx, int k, int mult);
 int coarse_build(sfem_ctx* ctx, const double* kp, const float* kp32);
 bool coarse_active(const LinAlg* L);
-template <typename T> int coarse_restrict(sfem_ctx* ctx, const T* r);
+template <typename T> int coarse_restrict(sfem_ctx* ctx, const T* r, const T* w = nullptr);
+template <typename T> int coarse_az(sfem_ctx* ctx, T* v);
+template <typename T> T* coarse_tmp(sfem_ctx* ctx);
 template <typename T> int coarse_correct(sfem_ctx* ctx, T* z);
 int coarse_info(sfem_ctx* ctx, int64_t* out);
 int coarse_get_inverse(sfem_ctx* ctx, int which, double* out, int cap);

@@ -64,7 +64,7 @@ ABI = [
     ("sfem_set_precision", C.c_int, [_vp, C.c_int]),
     ("sfem_set_predictor", C.c_int, [_vp, C.c_int]),
     ("sfem_set_pc_lag", C.c_int, [_vp, C.c_int]),
-    ("sfem_set_coarse_correction", C.c_int, [_vp, C.c_int]),
+    ("sfem_set_coarse_correction", C.c_int, [_vp, C.c_int, C.c_int]),
     ("sfem_coarse_info", C.c_int, [_vp, _i64p]),
     ("sfem_coarse_inverse", C.c_int, [_vp, C.c_int, _f64p, C.c_int]),
     ("sfem_coarse_step", C.c_int, [_vp, C.POINTER(SolverOpts)]),
@@ -371,10 +371,12 @@ class Context:
         """Rebuild the preconditioner on every lag-th new tangent only (1 = always)."""
         _check(self.lib.sfem_set_pc_lag(self.h, int(lag)), "sfem_set_pc_lag")
 
-    def set_coarse_correction(self, boxes_per_dim: int):
-        """Add Z E^-1 Z^T r to every ILU(0) application: piecewise-constant pressure on
-        boxes_per_dim boxes per direction, E = Z^T A Z (0 switches it off)."""
-        _check(self.lib.sfem_set_coarse_correction(self.h, int(boxes_per_dim)), "sfem_set_coarse_correction")
+    def set_coarse_correction(self, boxes_per_dim: int, multiplicative: bool = False):
+        """Coarse pressure correction of every ILU(0) application: piecewise-constant
+        pressure on boxes_per_dim boxes per direction, E = Z^T A Z (0 switches it off);
+        additive z = M^-1 r + Z E^-1 Z^T r, or multiplicative z0 + Z E^-1 Z^T (r - A z0)."""
+        _check(self.lib.sfem_set_coarse_correction(self.h, int(boxes_per_dim), int(bool(multiplicative))),
+               "sfem_set_coarse_correction")
 
     def coarse_info(self) -> dict:
         """State of the coarse pressure correction."""
@@ -565,6 +567,7 @@ class StabFEM:
         self.precision = PREC_FP64
         self.tuning = TUNE_TIMED   # kernel shapes of the solver: timed on first use, or fixed
         self.coarse_boxes = 0      # coarse pressure correction of ILU(0): boxes per direction (0: off)
+        self.coarse_mult = False   # its multiplicative form
         self.history = []          # (tstep, iter, rhs_norm, krylov_its)
 
     # StabFEM::readInputData (StabFEM.cpp:80-264)
@@ -607,7 +610,7 @@ class StabFEM:
         self.ctx.set_precision(self.precision)
         self.ctx.set_tuning_mode(self.tuning)
         if self.coarse_boxes:
-            self.ctx.set_coarse_correction(self.coarse_boxes)
+            self.ctx.set_coarse_correction(self.coarse_boxes, self.coarse_mult)
         self.ctx.setup_solver(self.opts)
 
     # StabFEM::solveFullyImplicit (StabFEM2.cpp:593-832)

@@ -195,8 +195,10 @@ int sfem_predictor_stats(sfem_ctx* ctx, int64_t* counts, double* last_norms);
  * reused.  No counterpart in the reference. */
 int sfem_set_pc_lag(sfem_ctx* ctx, int lag);
 
-/* Additive coarse pressure correction of the ILU(0) preconditioner:
- * z = M_ILU^{-1} r + Z E^{-1} Z^T r, with Z the piecewise-constant pressure on k boxes
+/* Coarse pressure correction of the ILU(0) preconditioner, additive
+ * (z = M_ILU^{-1} r + Z E^{-1} Z^T r) or with multiplicative != 0 multiplicative
+ * (z0 = M_ILU^{-1} r, z = z0 + Z E^{-1} Z^T (r - A z0); inside BiCGStab the product A z0 is
+ * the one the iteration takes anyway, corrected by A Z y), with Z the piecewise-constant pressure on k boxes
  * per direction of the mesh's bounding box (k^dim <= 1024; every box at least one element
  * wide) and E = Z^T A Z formed on the device from the tangent at each preconditioner
  * rebuild.  k = 0 (the default) switches it off.  It changes the preconditioned operator,
@@ -211,7 +213,7 @@ int sfem_set_pc_lag(sfem_ctx* ctx, int lag);
  * residual again (sfem_coarse_step).  Single meshes only (SFEM_ERR_UNSUPPORTED on a
  * batch).  No counterpart in the reference, which preconditions with PETSc's ILU(0)
  * alone (SolverPetsc::factoriseAndSolve, SolverPetsc.cpp:284-298). */
-int sfem_set_coarse_correction(sfem_ctx* ctx, int boxes_per_dim);
+int sfem_set_coarse_correction(sfem_ctx* ctx, int boxes_per_dim, int multiplicative);
 /* info[6] = {k, coarse unknowns, free pressure DOFs, restriction chunks, 1 if the last E
  * was singular (correction off until the next rebuild), 1 if built}. */
 int sfem_coarse_info(sfem_ctx* ctx, int64_t* info);

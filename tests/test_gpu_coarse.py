@@ -181,7 +181,34 @@ def test_coarse_step_increment(prec):
     ctx.close()
 
 
-def _cavity(n, steps, prec, k, rtol, newton_tol):
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_coarse_multiplicative_apply(prec):
+    """Multiplicative form, standalone application (host vectors): with z0 = M_ILU^-1 r
+    from the same factor, M_c^-1 r - z0 = Z E^-1 Z^T (r - A z0) (numpy, exported tangent)
+    within 1e-10 of ||M_c^-1 r||."""
+    mesh = meshgen.ldc3d_hex(8, graded=True)
+    ctx = _ctx(mesh, sfem.PREC_MIXED if prec == "mixed" else sfem.PREC_FP64)
+    ctx.time_update(1.0)
+    ctx.assemble(0)
+    opts = sfem.solver_opts("bcgs", "ilu", 1e-8 if prec == "fp64" else 1e-4, 2000, reorder=3)
+    ctx.solve(opts)
+    r = np.random.default_rng(4).standard_normal(ctx.nfree)
+    z0 = ctx.pc_apply(r)
+    ctx.set_coarse_correction(3, True)
+    ctx.solve(opts)
+    z1 = ctx.pc_apply(r)
+    A, _ = _system(ctx)
+    Z, _, Eid = _coarse_matrices(mesh, A, 3)
+    corr = Z @ np.linalg.solve(Eid, Z.T @ (r - A @ z0))
+    d = np.linalg.norm((z1 - z0) - corr) / np.linalg.norm(z1)
+    print(f"[coarse mult] {prec} apply diff {d:.2e}, |corr|/|z| {np.linalg.norm(corr) / np.linalg.norm(z1):.2e}",
+          flush=True)
+    assert np.linalg.norm(corr) > 1e-6 * np.linalg.norm(z1)
+    assert d <= 1e-10
+    ctx.close()
+
+
+def _cavity(n, steps, prec, k, rtol, newton_tol, mult=False):
     mesh = meshgen.ldc3d_hex(n, graded=True)
     ctl = deck.Control(np.array([1.0, 1e-3, 0, 0, 0, 0, 0, 0, 1, 1, 0]), 2, 0.0, 0.1, 1e9, steps, 1, 1e-6)
     fem = sfem.StabFEM()
@@ -191,6 +218,7 @@ def _cavity(n, steps, prec, k, rtol, newton_tol):
     fem.newton_tol = newton_tol
     fem.tuning = sfem.TUNE_FIXED
     fem.coarse_boxes = k
+    fem.coarse_mult = mult
     fem.prepareInputData()
     fem.setSolver(sfem.solver_opts("bcgs", "ilu", rtol, 5000, reorder=3))
     return fem
@@ -201,8 +229,9 @@ def _fields(s):
     return s[:, :3], s[:, 3]
 
 
+@pytest.mark.parametrize("mult", [False, True])
 @pytest.mark.parametrize("prec", [sfem.PREC_FP64, sfem.PREC_MIXED])
-def test_coarse_cavity_same_answer_fewer_iterations(prec):
+def test_coarse_cavity_same_answer_fewer_iterations(prec, mult):
     """Graded 12^3 cavity at Re 1000, 4 steps, against the FP64 solver without the
     correction, both driven to ||F|| < 1e-11: velocity within 1e-8 and pressure within
     1e-6 (relative L2; pressure is the weakly observed field of this PSPG system), and
@@ -213,7 +242,7 @@ def test_coarse_cavity_same_answer_fewer_iterations(prec):
     s_ref = ref.solveFullyImplicit()
     base = _cavity(n, steps, prec, 0, 1e-12 if prec == sfem.PREC_FP64 else 1e-4, 1e-11)
     base.solveFullyImplicit()
-    co = _cavity(n, steps, prec, 4, 1e-12 if prec == sfem.PREC_FP64 else 1e-4, 1e-11)
+    co = _cavity(n, steps, prec, 4, 1e-12 if prec == sfem.PREC_FP64 else 1e-4, 1e-11, mult)
     s_co = co.solveFullyImplicit()
     its_base = sum(h[3] for h in base.history)
     its_co = sum(h[3] for h in co.history)
@@ -221,7 +250,8 @@ def test_coarse_cavity_same_answer_fewer_iterations(prec):
     u1, p1 = _fields(s_co)
     eu = np.linalg.norm(u1 - u0) / np.linalg.norm(u0)
     ep = np.linalg.norm(p1 - p0) / np.linalg.norm(p0)
-    print(f"[coarse cavity] prec={prec} its {its_base} -> {its_co}; rel L2 u {eu:.2e} p {ep:.2e}", flush=True)
+    print(f"[coarse cavity] prec={prec} mult={mult} its {its_base} -> {its_co}; rel L2 u {eu:.2e} p {ep:.2e}",
+          flush=True)
     assert all(h[2] < 1e-11 for h in co.history if h[3] == 0)
     assert eu <= 1e-8 and ep <= 1e-6
     assert its_co <= its_base
