@@ -621,16 +621,6 @@ int coarse_build(sfem_ctx* ctx, const double* kp, const float* kp32) {
   else
     hipLaunchKernelGGL((k_coarse_rows<double>), dim3(C->nchunk), dim3(64), 0, st, C->d_slot, C->d_chunk, bd, P.d_bcol,
                        kp, C->d_cinfo, C->d_cpos, C->d_epart);
-  if (L->coarse_mult) {
-    const unsigned gz = (unsigned)((P.nblk + 255) / 256);
-    if (kp32)
-      hipLaunchKernelGGL((k_coarse_az_build<float>), dim3(gz), dim3(256), 0, st, P.nblk, bd, P.d_bcol, kp32,
-                         C->d_cinfo, C->d_azoff, C->d_azbox, (float*)C->d_az);
-    else
-      hipLaunchKernelGGL((k_coarse_az_build<double>), dim3(gz), dim3(256), 0, st, P.nblk, bd, P.d_bcol, kp,
-                         C->d_cinfo, C->d_azoff, C->d_azbox, (double*)C->d_az);
-    C->az_fp32 = kp32 != nullptr;
-  }
   const int nc = C->nc;
   hipLaunchKernelGGL(k_coarse_fill, dim3((nc + 63) / 64), dim3(64), 0, st, nc, C->d_boxchunk, C->d_boxnb, C->d_epart,
                      C->d_gj);
@@ -645,6 +635,44 @@ int coarse_build(sfem_ctx* ctx, const double* kp, const float* kp32) {
   CO_CHECK(hipGetLastError());
   C->ready = true;
   return SFEM_OK;
+}
+
+// A Z of the multiplicative form from the block tangent the Krylov products use (kp32 if
+// given, else kp), after every new tangent: the fused form's corrected product must be the
+// current operator's, even when the preconditioner (E) is lagged.
+int coarse_refresh_az(sfem_ctx* ctx, const double* kp, const float* kp32) {
+  LinAlg* L = ctx->la;
+  Coarse* C = L->coarse;
+  if (!L->coarse_mult || !C || C->plan_id != L->plan_id) return SFEM_OK;
+  const IluPlan& P = *L->ilu;
+  const int4* bd = reinterpret_cast<const int4*>(P.d_bdesc);
+  const unsigned gz = (unsigned)((P.nblk + 255) / 256);
+  if (kp32)
+    hipLaunchKernelGGL((k_coarse_az_build<float>), dim3(gz), dim3(256), 0, ctx->stream, P.nblk, bd, P.d_bcol, kp32,
+                       C->d_cinfo, C->d_azoff, C->d_azbox, (float*)C->d_az);
+  else
+    hipLaunchKernelGGL((k_coarse_az_build<double>), dim3(gz), dim3(256), 0, ctx->stream, P.nblk, bd, P.d_bcol, kp,
+                       C->d_cinfo, C->d_azoff, C->d_azbox, (double*)C->d_az);
+  C->az_fp32 = kp32 != nullptr;
+  CO_CHECK(hipGetLastError());
+  return SFEM_OK;
+}
+
+// The arrays a gather that forms A Z itself needs (false when the correction is additive
+// or not yet planned for the current ILU(0) plan).
+bool coarse_az_targets(const LinAlg* L, const int32_t** cinfo, const int32_t** azoff, const int32_t** azbox,
+                       void** az) {
+  const Coarse* C = L->coarse;
+  if (!L->coarse_mult || !C || C->plan_id != L->plan_id) return false;
+  *cinfo = C->d_cinfo;
+  *azoff = C->d_azoff;
+  *azbox = C->d_azbox;
+  *az = C->d_az;
+  return true;
+}
+
+void coarse_set_az_precision(LinAlg* L, bool fp32) {
+  if (L->coarse) L->coarse->az_fp32 = fp32;
 }
 
 bool coarse_active(const LinAlg* L) { return L->coarse && L->coarse->ready && L->coarse->plan_id == L->plan_id; }

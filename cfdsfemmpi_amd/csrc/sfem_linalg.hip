@@ -1199,6 +1199,49 @@ __global__ __launch_bounds__(256) void k_bsr_gather_kb(int64_t nblk, const BDesc
   }
 }
 
+// The same gather, also forming the multiplicative coarse correction's A Z of each block
+// row from the values it moves: the pressure column of every neighbour block, summed per
+// coarse unknown of the neighbour's pressure DOF, neighbours in order (as
+// k_coarse_az_build sums them).  Row a's values go through LDS, one wavefront per row.
+template <typename VT>
+__global__ __launch_bounds__(256) void k_bsr_gather_kb_az(int64_t nblk, const BDesc* __restrict__ desc,
+                                                          const int32_t* __restrict__ kbsrc,
+                                                          const VT* __restrict__ kb, VT* __restrict__ dst,
+                                                          const int32_t* __restrict__ bcol,
+                                                          const int32_t* __restrict__ cinfo,
+                                                          const int32_t* __restrict__ azoff,
+                                                          const int32_t* __restrict__ azbox, VT* __restrict__ az) {
+  __shared__ double sv[4][4][kBsrMaxNb];
+  __shared__ int sb[4][kBsrMaxNb];
+  const int w = threadIdx.x / 64;
+  const int64_t p = (int64_t)blockIdx.x * 4 + w;
+  if (p >= nblk) return;
+  const int lane = threadIdx.x % 64;
+  const BDesc d = desc[p];
+  const int m = d.dm >> 16;
+  for (int idx = lane; idx < m * d.nb; idx += 64) {
+    const int a = idx / d.nb, t = idx - a * d.nb;
+    const V4<VT> v = ld4<VT>(kb + 16 * (int64_t)kbsrc[d.bptr + t] + 4 * a);
+    st4(dst + (int64_t)d.vbase + (int64_t)a * 4 * d.nb + 4 * t, v.v[0], v.v[1], v.v[2], v.v[3]);
+    const int info = cinfo[bcol[d.bptr + t]];
+    const int b = info & 3;
+    const VT pv = b == 0 ? v.v[0] : b == 1 ? v.v[1] : b == 2 ? v.v[2] : v.v[3];
+    sv[w][a][t] = info >= 0 ? (double)pv : 0.0;
+    if (a == 0) sb[w][t] = info >= 0 ? info >> 3 : -1;
+  }
+  wave_sync();
+  const int off = azoff[p], cnt = azoff[p + 1] - off;
+  for (int q = lane; q < 4 * cnt; q += 64) {
+    const int k = q >> 2, a = q & 3;
+    const int c = azbox[off + k];
+    double s = 0.0;
+    if (a < m)
+      for (int t = 0; t < d.nb; ++t)
+        if (sb[w][t] == c) s += sv[w][a][t];
+    az[4 * (int64_t)(off + k) + a] = (VT)s;
+  }
+}
+
 // CSR values from the node-pair block form: node block k's rows r0 + a, its pair q's
 // free columns at (natofs[q] & 0xffffff) in each row (natofs[q] >> 24 of them).  One
 // wavefront per node block, four per workgroup.
@@ -2429,10 +2472,18 @@ int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f, 
     // its starting values from it
     const unsigned gb = (unsigned)std::max<int64_t>((P.nblk + 3) / 4, 1);
     if (ctx->kb_on && ctx->kb_fp32 != fp32) return set_error(SFEM_ERR_STATE, "the tangent was assembled in the other precision");
+    // the multiplicative coarse correction's A Z formed by the gather (plan already known to
+    // the correction, products on the gathered tangent itself)
+    const int32_t *az_ci = nullptr, *az_off = nullptr, *az_box = nullptr;
+    void* az_val = nullptr;
+    const bool fuse_az = ctx->kb_on && !store32 && coarse_az_targets(L, &az_ci, &az_off, &az_box, &az_val);
     if (fp32) {
       int rc;
       if ((rc = ensure_vec(&L->kp32, nv)) || (rc = ensure_vec(&L->lu32, nv))) return rc;
-      if (ctx->kb_on)
+      if (fuse_az)
+        hipLaunchKernelGGL((k_bsr_gather_kb_az<float>), dim3(gb), dim3(256), 0, st, P.nblk, P.d_bdesc, P.d_kbsrc,
+                           ctx->d_kb32, L->kp32, P.d_bcol, az_ci, az_off, az_box, (float*)az_val);
+      else if (ctx->kb_on)
         hipLaunchKernelGGL((k_bsr_gather_kb<float>), dim3(gb), dim3(256), 0, st, P.nblk, P.d_bdesc, P.d_kbsrc,
                            ctx->d_kb32, L->kp32);
       else
@@ -2441,7 +2492,10 @@ int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f, 
     } else {
       int rc;
       if ((rc = ensure_vec(&L->kp, nv)) || (rc = ensure_vec(&L->lu, nv))) return rc;
-      if (ctx->kb_on)
+      if (fuse_az)
+        hipLaunchKernelGGL((k_bsr_gather_kb_az<double>), dim3(gb), dim3(256), 0, st, P.nblk, P.d_bdesc, P.d_kbsrc,
+                           ctx->d_kb, L->kp, P.d_bcol, az_ci, az_off, az_box, (double*)az_val);
+      else if (ctx->kb_on)
         hipLaunchKernelGGL((k_bsr_gather_kb<double>), dim3(gb), dim3(256), 0, st, P.nblk, P.d_bdesc, P.d_kbsrc,
                            ctx->d_kb, L->kp);
       else
@@ -2459,7 +2513,15 @@ int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f, 
       int rc = to32(L->kp, &L->kp32);
       if (rc) return rc;
     }
+    // the tangent the Krylov products use: FP32 when built in FP32 or stored in FP32
+    const bool op32 = fp32 || store32 || (!refactor && L->pc_store32);
+    if (fuse_az) coarse_set_az_precision(L, fp32);
     if (!refactor) {
+      // lagged factor; the multiplicative coarse correction's A Z follows the new tangent
+      if (L->coarse_k > 0 && !fuse_az) {
+        int rc = coarse_refresh_az(ctx, op32 ? nullptr : L->kp, op32 ? L->kp32 : nullptr);
+        if (rc) return rc;
+      }
       LA_CHECK(hipGetLastError());
       return SFEM_OK;
     }
@@ -2527,6 +2589,7 @@ int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f, 
     // the coarse pressure correction's E = Z^T A Z from the tangent of this factor
     if (L->coarse_k > 0) {
       int rc = coarse_build(ctx, fp32 ? nullptr : L->kp, fp32 ? L->kp32 : nullptr);
+      if (!rc && !fuse_az) rc = coarse_refresh_az(ctx, op32 ? nullptr : L->kp, op32 ? L->kp32 : nullptr);
       if (rc) return rc;
     }
   }

@@ -207,6 +207,10 @@ void la_direct_destroy(LinAlg* L);
 // coarse pressure correction (sfem_coarse.hip)
 int coarse_set(sfem_ctx* ctx, int k, int mult);
 int coarse_build(sfem_ctx* ctx, const double* kp, const float* kp32);
+int coarse_refresh_az(sfem_ctx* ctx, const double* kp, const float* kp32);
+bool coarse_az_targets(const LinAlg* L, const int32_t** cinfo, const int32_t** azoff, const int32_t** azbox,
+                       void** az);
+void coarse_set_az_precision(LinAlg* L, bool fp32);
 bool coarse_active(const LinAlg* L);
 template <typename T> int coarse_restrict(sfem_ctx* ctx, const T* r, const T* w = nullptr);
 template <typename T> int coarse_az(sfem_ctx* ctx, T* v);
