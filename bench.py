@@ -207,7 +207,7 @@ def c4_case(count, seed):
     return meshgen.c4_batch(count, seed=seed)
 
 
-def c4_ctx(mesh, em, mu, prec, opts):
+def c4_ctx(mesh, em, mu, prec, opts, coarse=0):
     from cfdsfemmpi_amd import sfem
     ctx = sfem.Context(mesh, device=0)
     ctx.set_dirichlet(mesh.dbc_node, mesh.dbc_dof, mesh.dbc_value)
@@ -217,16 +217,18 @@ def c4_ctx(mesh, em, mu, prec, opts):
     ctx.set_precision(prec)
     if em is not None:
         ctx.set_batch(em, mu)
+    if coarse:
+        ctx.set_coarse_correction(coarse, True)  # per mesh for a batch
     ctx.setup_solver(opts)
     return ctx
 
 
-def run_batch(case, prec, opts, newton_tol, steps, warmup, label, graph=True):
+def run_batch(case, prec, opts, newton_tol, steps, warmup, label, graph=True, coarse=0):
     """Every mesh of the batch advances ``steps`` time steps together (sfem_step_batch)."""
     from cfdsfemmpi_amd import sfem
     mesh, em, mu, ns, meshes = case
     t0 = time.time()
-    ctx = c4_ctx(mesh, em, mu, prec, opts)
+    ctx = c4_ctx(mesh, em, mu, prec, opts, coarse)
     ctx.batch_use_graph(graph)
     info = ctx.info()
     log(f"{label}: setup {time.time() - t0:.1f} s, nfree={info['nfree']} nnz={info['nnz']} colours={info['ncolors']}")
@@ -267,13 +269,13 @@ def run_batch(case, prec, opts, newton_tol, steps, warmup, label, graph=True):
     return res
 
 
-def run_sequential_sample(case, prec, opts, newton_tol, steps, stride):
+def run_sequential_sample(case, prec, opts, newton_tol, steps, stride, coarse=0):
     """Meshes 0, stride, 2*stride, ... stepped one after another through the single-mesh
     path (one context each, setup untimed): the rate a caller without batching gets."""
     from cfdsfemmpi_amd import sfem
     mesh, em, mu, ns, meshes = case
     idx = list(range(0, len(meshes), stride))
-    ctxs = [c4_ctx(meshes[k], None, mu[k:k + 1], prec, opts) for k in idx]
+    ctxs = [c4_ctx(meshes[k], None, mu[k:k + 1], prec, opts, coarse) for k in idx]
     for c in ctxs:
         c.get_solution()
     t1 = time.perf_counter()
@@ -323,7 +325,13 @@ def main_c4(a):
         f"{mesh.nNode} nodes, Re in [{1 / mu.max():.0f}, {1 / mu.min():.0f}]")
     o64 = sfem.solver_opts("bcgs", "ilu", 1e-8, 5000, reorder=a.reorder)
     omx = sfem.solver_opts("bcgs", "ilu", a.inner_rtol, 5000, reorder=a.reorder, factor_bf16=a.bf16_factors)
-    r64 = run_batch(case, sfem.PREC_FP64, o64, a.newton_tol, steps, a.warmup, "fp64-batch")
+    co = a.c4_coarse
+    rpl = None
+    if co and a.plain:
+        # the reference's solver configuration: ILU(0) alone
+        rpl = run_batch(case, sfem.PREC_FP64, o64, a.newton_tol, steps, a.warmup, "fp64-batch-plain")
+        log(f"fp64-batch-plain: {rpl['steps_per_s']:.3f} steps/s")
+    r64 = run_batch(case, sfem.PREC_FP64, o64, a.newton_tol, steps, a.warmup, "fp64-batch", coarse=co)
     log(f"fp64-batch: {r64['steps_per_s']:.3f} steps/s")
     node_mesh = np.concatenate([np.full(m.nNode, k) for k, m in enumerate(meshes)])
 
@@ -338,19 +346,20 @@ def main_c4(a):
     r32 = None
     if a.c4_store32:
         o32 = sfem.solver_opts("bcgs", "ilu", a.c4_store32_rtol, 5000, reorder=a.reorder, factor_bf16=a.bf16_factors)
-        r32 = run_batch(case, sfem.PREC_STORE32, o32, a.c4_store32_newton, steps, a.warmup, "store32-batch")
+        r32 = run_batch(case, sfem.PREC_STORE32, o32, a.c4_store32_newton, steps, a.warmup, "store32-batch",
+                        coarse=co)
         r32["err"] = mesh_errors(r32["soln"])
         log(f"store32-batch: {r32['steps_per_s']:.3f} steps/s, per-mesh rel L2 vs fp64: u max {r32['err'][0]:.3e} "
             f"p max {r32['err'][1]:.3e}")
     rmx = err_u = err_p = None
     if a.c4_mixed:
-        rmx = run_batch(case, sfem.PREC_MIXED, omx, a.newton_tol_mixed, steps, a.warmup, "mixed-batch")
+        rmx = run_batch(case, sfem.PREC_MIXED, omx, a.newton_tol_mixed, steps, a.warmup, "mixed-batch", coarse=co)
         log(f"mixed-batch: {rmx['steps_per_s']:.3f} steps/s")
         err_u, err_p = mesh_errors(rmx["soln"])
         log(f"per-mesh rel L2 mixed vs fp64: u max {err_u:.3e}, p max {err_p:.3e}")
     seq = None
     if a.c4_seq_stride:
-        seq = run_sequential_sample(case, sfem.PREC_FP64, o64, a.newton_tol, steps, a.c4_seq_stride)
+        seq = run_sequential_sample(case, sfem.PREC_FP64, o64, a.newton_tol, steps, a.c4_seq_stride, co)
         log(f"one mesh at a time (fp64), {len(seq['meshes_sampled'])} meshes in {seq['seconds_sample']:.2f} s: "
             f"~{seq['steps_per_s_estimate']:.3f} batch-steps/s")
     kt = r64["kt"]
@@ -386,11 +395,14 @@ def main_c4(a):
                                f"in [50, 2000], dt={C4_DT}, {steps} steps, one batched solve per step",
                    "n_elements": mesh.nElem, "n_nodes": mesh.nNode, "nfree": nf, "nnz": nnz,
                    "seed": a.c4_seed, "reported_run": best["label"],
-                   "solver": "BiCGStab + ILU(0), per-mesh scalars and convergence",
+                   "solver": "BiCGStab + ILU(0), per-mesh scalars and convergence" +
+                             (f", multiplicative coarse pressure correction on {co}^2 boxes per mesh" if co else ""),
                    "fp64_run": {"rtol": 1e-8, "newton_tol": a.newton_tol},
                    "graph": "one BiCGStab iteration replayed as a HIP graph",
                    "narrowing": "iteration restricted to unfinished meshes", "parallelism": "single GPU"},
         "fp64_batch_steps_per_s": r64["steps_per_s"],
+        "fp64_batch_plain_steps_per_s": rpl["steps_per_s"] if rpl else None,
+        "fp64_batch_plain_krylov_iters_max": [int(h[1].max()) for h in rpl["hist"]] if rpl else None,
         "one_mesh_at_a_time_fp64": seq,
         "speedup_fp64_batch_vs_one_at_a_time": (r64["steps_per_s"] / seq["steps_per_s_estimate"]) if seq else None,
         "store32_batch": None if r32 is None else {
@@ -593,6 +605,8 @@ def main():
     ap.add_argument("--c4-seq-stride", type=int, default=16,
                     help="c4: time every n-th mesh through the single-mesh path (0: skip)")
     ap.add_argument("--c4-cpu-stride", type=int, default=32, help="c4: oracle sample of every n-th mesh")
+    ap.add_argument("--c4-coarse", type=int, default=8,
+                    help="c4: multiplicative coarse pressure correction on n^2 boxes per mesh (0: off)")
     ap.add_argument("--c4-mixed", type=int, default=0,
                     help="c4: also run the batch with FP32 Krylov (stalls on these systems; slow)")
     ap.add_argument("--c4-store32", type=int, default=1,

@@ -621,7 +621,6 @@ int sfem_set_pc_lag(sfem_ctx* c, int lag) {
 int sfem_set_coarse_correction(sfem_ctx* c, int k, int mult) {
   if (!c) return set_error(SFEM_ERR_ARG, "null context");
   if (!c->la) return set_error(SFEM_ERR_STATE, "build the pattern first");
-  if (c->batch && k) return set_error(SFEM_ERR_UNSUPPORTED, "coarse correction: single meshes only");
   if (k < 0) return set_error(SFEM_ERR_ARG, "boxes per direction must be >= 0");
   int rc = sfem::coarse_set(c, k, mult);
   if (rc) return rc;

@@ -42,6 +42,7 @@ constexpr int kStallWindow = 100;  // FP32: no 10% residual gain in this many it
 struct Batch {
   int nmesh = 0;
   std::vector<int64_t> dof_ptr;   // free-DOF range of each mesh (DOF order)
+  std::vector<int32_t> node_mesh; // mesh of every node (-1: in no element)
   int64_t* d_dof_ptr = nullptr;
   int plan_id = -1;               // ILU plan the chunk lists were built for
   int nchunk = 0;
@@ -75,10 +76,11 @@ struct Batch {
   double* d_rhsn = nullptr;
   int32_t* d_status = nullptr;
   int32_t* h_status = nullptr;    // pinned
+  int32_t* d_cmask = nullptr;     // per mesh: takes the FP64 coarse step this Newton iteration
   int its_per_graph = 4;          // BiCGStab iterations per replayed graph in the graph-captured step
   ~Batch() {
     void* ptrs[] = {d_dof_ptr, d_chunk, d_cchunk, d_mptr, d_mlist, d_sc, d_part, d_norm, d_active, d_ndone,
-                    d_newton, d_kry, d_rhsn, d_status};
+                    d_newton, d_kry, d_rhsn, d_status, d_cmask};
     for (void* q : ptrs)
       if (q) (void)hipFree(q);
     for (void* q : vec)
@@ -96,6 +98,10 @@ void batch_destroy(sfem_ctx* ctx) {
 }
 
 int64_t batch_stalled(const sfem_ctx* ctx) { return ctx->batch ? ctx->batch->stalled : 0; }
+
+const std::vector<int32_t>* batch_node_mesh(const sfem_ctx* ctx) {
+  return ctx->batch ? &ctx->batch->node_mesh : nullptr;
+}
 
 // ---------------------------------------------------------------------------
 // kernels
@@ -416,6 +422,20 @@ __global__ __launch_bounds__(kTB) void k_b_activity(int nmesh, const double* __r
   }
 }
 
+// meshes that take the FP64 coarse step: still iterating and not yet below the tolerance
+__global__ __launch_bounds__(kTB) void k_b_coarse_mask(int nmesh, const double* __restrict__ norm, double tol,
+                                                       const int32_t* __restrict__ active, int32_t* __restrict__ cmask) {
+  for (int m = threadIdx.x; m < nmesh; m += blockDim.x) cmask[m] = active[m] && !(norm[m] < tol);
+}
+
+// the coarse step's increment kept only for those meshes, one block per mesh
+__global__ __launch_bounds__(kTB) void k_b_mask_du(const int64_t* __restrict__ ptr, const int32_t* __restrict__ cmask,
+                                                   double* __restrict__ du) {
+  const int m = blockIdx.x;
+  if (cmask[m]) return;
+  for (int64_t i = ptr[m] + threadIdx.x; i < ptr[m + 1]; i += kTB) du[i] = 0.0;
+}
+
 // end of a batched solve: Krylov iterations per mesh, stalled FP32 mesh solves, and the
 // first mesh that broke down or did not converge
 __global__ __launch_bounds__(kTB) void k_b_tail(int nmesh, const double* __restrict__ sc, int32_t* __restrict__ kry,
@@ -499,22 +519,23 @@ static int bb_iteration(sfem_ctx* ctx, Batch& B, double rtol, int stall, const i
   T *r = V[0], *rh = V[1], *p = V[2], *v = V[3], *s = V[4], *t = V[5], *ph = V[6], *sh = V[7], *x = V[8];
   T* xb = V[10];
   const dim3 g(nch), tb(kTB);
-  auto pc = [&](const T* a, T* z) { return sizeof(T) == 8 ? la_apply_pc64(ctx, (const double*)a, (double*)z, sub)
-                                                          : la_apply_pc32(ctx, (const float*)a, (float*)z, sub); };
-  auto op = [&](const T* a, T* y) { return sizeof(T) == 8 ? la_apply_op64(ctx, (const double*)a, (double*)y, sub)
-                                                          : la_apply_op32(ctx, (const float*)a, (float*)y, sub); };
+  // z = M^-1 a and y = A z (one fused operation with the multiplicative coarse correction)
+  auto pcop = [&](const T* a, T* z, T* y) {
+    return sizeof(T) == 8 ? la_apply_pc_op64(ctx, (const double*)a, (double*)z, (double*)y, sub)
+                          : la_apply_pc_op32(ctx, (const float*)a, (float*)z, (float*)y, sub);
+  };
   auto fin = [&](int o) {
     hipLaunchKernelGGL(k_b_final, dim3(1), dim3(kTB), 0, st, B.nmesh, o, B.d_mptr, B.d_mlist, B.d_part, B.d_sc,
                        rtol, stall, B.d_ndone);
   };
   int rc;
   hipLaunchKernelGGL((k_b_p<T>), g, tb, 0, st, ch, B.d_sc, r, p, v);
-  if ((rc = pc(p, ph)) || (rc = op(ph, v))) return rc;
+  if ((rc = pcop(p, ph, v))) return rc;
   hipLaunchKernelGGL((k_b_dot2<T>), g, tb, 0, st, ch, rh, v, v, v, B.d_part);
   fin(F_ALPHA);
   hipLaunchKernelGGL((k_b_s<T>), g, tb, 0, st, ch, B.d_sc, r, v, s, B.d_part);
   fin(F_SS);
-  if ((rc = pc(s, sh)) || (rc = op(sh, t))) return rc;
+  if ((rc = pcop(s, sh, t))) return rc;
   hipLaunchKernelGGL((k_b_dot2<T>), g, tb, 0, st, ch, t, t, t, s, B.d_part);
   fin(F_OMEGA);
   hipLaunchKernelGGL((k_b_xr<T>), g, tb, 0, st, ch, B.d_sc, x, ph, sh, r, s, t, rh, B.d_part);
@@ -728,7 +749,25 @@ static int step_batch_graph(sfem_ctx* c, const sfem_solver_opts* o, double tol, 
   hipLaunchKernelGGL(k_b_step_reset, dim3(1), dim3(kTB), 0, st, nm, max_newton, B.d_active, B.d_newton, B.d_kry,
                      B.d_rhsn, B.d_status);
   BA_CHECK(hipGetLastError());
+  // FP32 inner solves with the coarse correction: every Newton iteration first takes the
+  // FP64 coarse step x += Z E^-1 Z^T F on the meshes still iterating, then the residual
+  // again (their inner solves leave each mesh's global constant pressure to it)
+  const bool cstep = sizeof(T) == 4 && c->la->coarse_k > 0;
+  auto coarse = [&]() -> int {
+    hipLaunchKernelGGL(k_seg_norm2, dim3(nm), dim3(kTB), 0, st, B.d_dof_ptr, c->d_rhs, B.d_norm);
+    hipLaunchKernelGGL(k_b_coarse_mask, dim3(1), dim3(kTB), 0, st, nm, B.d_norm, tol, B.d_active, B.d_cmask);
+    int rc1 = la_coarse_step_kernels(c);
+    if (rc1) return rc1;
+    hipLaunchKernelGGL(k_b_mask_du, dim3(nm), dim3(kTB), 0, st, B.d_dof_ptr, B.d_cmask, c->d_du);
+    BA_CHECK(hipGetLastError());
+    BA_CHECK(launch_update_solution(st, c->nfree, c->d_assy, c->d_du, c->d_soln));
+    return asm_residual_kernels(c);
+  };
   auto head = [&]() -> int {
+    if (cstep) {
+      int rc1 = coarse();
+      if (rc1) return rc1;
+    }
     hipLaunchKernelGGL(k_seg_norm2, dim3(nm), dim3(kTB), 0, st, B.d_dof_ptr, c->d_rhs, B.d_norm);
     hipLaunchKernelGGL(k_b_activity, dim3(1), dim3(kTB), 0, st, nm, B.d_norm, tol, B.d_active, B.d_newton, B.d_rhsn,
                        B.d_status);
@@ -758,7 +797,7 @@ static int step_batch_graph(sfem_ctx* c, const sfem_solver_opts* o, double tol, 
     if (iter == 0) {
       double gnorm;
       if ((rc = sfem_assemble(c, 0, 1, &gnorm)) || (rc = solve_prepare<T>(c, o)) || (rc = asm_prepare_residual(c)) ||
-          (rc = head()))
+          (cstep && (rc = la_coarse_step_prepare(c))) || (rc = head()))
         break;
     } else if (B.use_graph) {
       if (!head_exec &&
@@ -806,7 +845,6 @@ extern "C" {
 int sfem_set_batch(sfem_ctx* c, int32_t nmesh, const int32_t* elem_mesh, const double* mu) {
   if (!c || nmesh <= 0 || !elem_mesh || !mu) return set_error(SFEM_ERR_ARG, "bad batch arguments");
   if (!c->la) return set_error(SFEM_ERR_STATE, "build the pattern before sfem_set_batch");
-  if (c->la->coarse_k) return set_error(SFEM_ERR_UNSUPPORTED, "coarse correction: single meshes only");
   HIP_OK(hipSetDevice(c->device));
   const int npe = c->dims.npe, ndof = c->dims.ndof;
   // mesh of every node; nodes (and so free DOFs) must come mesh by mesh
@@ -838,6 +876,7 @@ int sfem_set_batch(sfem_ctx* c, int32_t nmesh, const int32_t* elem_mesh, const d
   c->batch = B;
   B->nmesh = nmesh;
   B->dof_ptr = ptr;
+  B->node_mesh = nodeMesh;
   if (!c->d_elem_mu && hipMalloc((void**)&c->d_elem_mu, c->nElem * sizeof(double)) != hipSuccess)
     return set_error(SFEM_ERR_HIP, "hipMalloc elem mu");
   HIP_OK(hipMemcpy(c->d_elem_mu, emu.data(), c->nElem * sizeof(double), hipMemcpyHostToDevice));
@@ -845,13 +884,19 @@ int sfem_set_batch(sfem_ctx* c, int32_t nmesh, const int32_t* elem_mesh, const d
       balloc(&B->d_norm, nmesh) != hipSuccess || balloc(&B->d_active, nmesh) != hipSuccess ||
       balloc(&B->d_ndone, 1) != hipSuccess || balloc(&B->d_newton, nmesh) != hipSuccess ||
       balloc(&B->d_kry, nmesh) != hipSuccess || balloc(&B->d_rhsn, nmesh) != hipSuccess ||
-      balloc(&B->d_status, ST_N) != hipSuccess)
+      balloc(&B->d_status, ST_N) != hipSuccess || balloc(&B->d_cmask, nmesh) != hipSuccess)
     return set_error(SFEM_ERR_HIP, "hipMalloc batch state");
   HIP_OK(hipMemcpy(B->d_dof_ptr, ptr.data(), ptr.size() * sizeof(int64_t), hipMemcpyHostToDevice));
   HIP_OK(hipHostMalloc((void**)&B->h_norm, nmesh * sizeof(double), hipHostMallocDefault));
   HIP_OK(hipHostMalloc((void**)&B->h_ndone, sizeof(int32_t), hipHostMallocDefault));
   HIP_OK(hipHostMalloc((void**)&B->h_sc, (size_t)nmesh * B_N * sizeof(double), hipHostMallocDefault));
   HIP_OK(hipHostMalloc((void**)&B->h_status, ST_N * sizeof(int32_t), hipHostMallocDefault));
+  // a coarse correction set before the batch is planned again per mesh
+  if (c->la->coarse_k > 0) {
+    int rc = coarse_set(c, c->la->coarse_k, c->la->coarse_mult ? 1 : 0);
+    if (rc) return rc;
+    c->la->pc_K_version = -1;
+  }
   return SFEM_OK;
 }
 

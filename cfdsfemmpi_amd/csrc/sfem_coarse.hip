@@ -1,29 +1,29 @@
-// sfem_coarse.hip -- additive coarse pressure correction of the ILU(0) preconditioner.
+// sfem_coarse.hip -- coarse pressure correction of the ILU(0) preconditioner.
 //
 // No counterpart in the reference: it preconditions every Newton system with PETSc's
 // ILU(0) (SolverPetsc::factoriseAndSolve, src/base/SolverPetsc.cpp:284-298), whose
 // BiCGStab iteration count grows with the mesh because ILU(0) only damps the pressure
-// error locally.  This adds a two-level correction:
+// error locally.  This adds a two-level correction, additive or multiplicative:
 //
-//   z = M_ILU^{-1} r + Z E^{-1} Z^T r,   E = Z^T A Z,
+//   z = M_ILU^{-1} r + Z E^{-1} Z^T r,   or   z0 = M_ILU^{-1} r, z = z0 + Z E^{-1} Z^T (r - A z0),
+//   E = Z^T A Z,
 //
 // with Z the piecewise-constant pressure on k^d boxes of the mesh's bounding box (one
 // column per box holding at least one free pressure DOF; velocity DOFs get no coarse
-// component).  The preconditioned operator changes, the linear system and its
-// tolerance do not.
+// component).  A batch of independent meshes (sfem_set_batch) gets k^d boxes per mesh,
+// on the mesh's own bounding box: E is then block diagonal, one block per mesh.  The
+// preconditioned operator changes, the linear system and its tolerance do not.
 //
 //   * Layout: the ILU(0) plan's padded block vectors (4 slots per node block).  The free
-//     pressure slots are listed sorted by (box, slot) and cut into chunks of at most
+//     pressure slots are listed sorted by (mesh, box, slot) and cut into chunks of at most
 //     kChunk slots that never straddle a box; every sum below runs over chunks and lanes
 //     in a fixed order, so E, its inverse and every correction are the same from run to
 //     run.
 //   * E: at each preconditioner rebuild, one wavefront per chunk reads the pressure row
 //     of each of its block rows in the plan's block tangent (the tangent the Krylov
 //     products use) and sums its pressure columns by the box offset (-1, 0, +1 per
-//     direction) of the neighbour's box; one workgroup adds the chunk sums into E and
-//     inverts it by Gauss-Jordan elimination with partial pivoting (largest magnitude,
-//     first row on ties).  Boxes at least one element wide keep every neighbour within
-//     one box in each direction (checked on the host over the elements).
+//     direction) of the neighbour's box.  Boxes at least one element wide keep every
+//     neighbour within one box in each direction (checked on the host over the elements).
 //   * E's diagonal: the pressure-pressure block of the stabilised tangent is the PSPG
 //     term, integral of tau grad N_i . grad N_j, whose rows sum to zero (the shape
 //     functions sum to one) except where a neighbour's pressure is prescribed.  Summing
@@ -32,14 +32,15 @@
 //     entries cannot resolve; so E(c, c) is formed as the row sums of the rows next to a
 //     prescribed pressure minus the box's couplings to the other boxes,
 //     sum_{i in c} rowsum_i - sum_{c' != c} E(c, c'), the same value in exact arithmetic.
+//   * Inverse: Gauss-Jordan elimination without pivoting (E is symmetric and definite),
+//     one launch per step with one workgroup per row for a single mesh, one workgroup per
+//     mesh for a batch; FP32 Krylov solves use the inverse without the global constant
+//     pressure of each mesh, whose correction their vectors cannot hold beside the rest
+//     (the FP64 Newton loop takes it as a coarse step).
 //   * Apply: restriction (chunk sums of r at the pressure slots, before the ILU(0)
 //     sweeps, so z may alias r), the coarse solve (box sums, then y = E^{-1} c), and the
-//     prolongation z[slot] += y[box] after the sweeps.
-// E and y are FP64.  The correction's global constant pressure mode (E's smallest
-// eigenvalue is set by the prescribed pressures alone, orders of magnitude below the
-// others) outgrows the rest of a preconditioned vector by more than FP32 resolves, so a
-// mixed-precision solve with the correction keeps its Krylov vectors in FP64 (over the
-// FP32 tangent and factor).
+//     prolongation z[slot] += y[box] after the sweeps; the multiplicative form restricts
+//     r - A z0 and corrects the product A z0 with A Z y.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -79,9 +80,22 @@ struct Coarse {
                                   // pressure) << 2 | pressure slot, -1 without pressure
   int32_t* d_cpos = nullptr;      // per coarse unknown: box coordinates bx | by << 10 | bz << 20
   int32_t* d_boxnb = nullptr;     // nc * 27: coarse unknown of the box at each offset, -1 if none
+  // meshes (one for a single mesh): unknowns of mesh g are [goff[g], goff[g+1]); its
+  // blocks of the Gauss-Jordan workspace (ng x 2ng) and of the inverses (ng x ng) start at
+  // gjbase[g] and ebase[g]; the coarse solve's workgroups are {mesh, first row} records
+  int ngroup = 1;
+  std::vector<int32_t> h_goff;
+  std::vector<int64_t> h_ebase;
+  int32_t* d_cgrp = nullptr;
+  int32_t* d_goff = nullptr;
+  int64_t* d_gjbase = nullptr;
+  int64_t* d_ebase = nullptr;
+  int2* d_swg = nullptr;
+  int nswg = 0;
+  int maxng = 0;
   double* d_epart = nullptr;      // nchunk * 27 chunk sums of E's rows by offset
-  double* d_gj = nullptr;         // nc x 2nc Gauss-Jordan workspace [E | I]
-  double* d_einv = nullptr;       // nc x nc: E^{-1}, row-major
+  double* d_gj = nullptr;         // per mesh ng x 2ng Gauss-Jordan workspace [E | I]
+  double* d_einv = nullptr;       // per mesh ng x ng: E^{-1}, row-major
   double* d_einvb = nullptr;      // the correction without the global constant (FP32 Krylov
                                   // vectors), or NULL
   double* d_cpart = nullptr;      // nchunk restricted chunk sums
@@ -98,7 +112,8 @@ struct Coarse {
   void* d_tmp = nullptr;          // A z of a standalone multiplicative application (npad x 8 bytes)
   ~Coarse() {
     void* ptrs[] = {d_slot, d_chunk, d_chunkbox, d_boxchunk, d_cinfo, d_cpos, d_boxnb,
-                    d_epart, d_gj, d_einv, d_einvb, d_cpart, d_y, d_flag, d_azoff, d_azbox, d_az, d_tmp};
+                    d_epart, d_gj, d_einv, d_einvb, d_cpart, d_y, d_flag, d_azoff, d_azbox, d_az, d_tmp,
+                    d_cgrp, d_goff, d_gjbase, d_ebase, d_swg};
     for (void* q : ptrs)
       if (q) (void)hipFree(q);
   }
@@ -159,25 +174,29 @@ __global__ void __launch_bounds__(64) k_coarse_rows(const int32_t* __restrict__ 
   }
 }
 
-// [E | I] from the chunk sums, one thread per row of E: each (unknown, offset) cell is one
-// entry of E, summed over the unknown's chunks in order.
-__global__ void __launch_bounds__(64) k_coarse_fill(int nc, const int32_t* __restrict__ boxchunk,
+// [E | I] from the chunk sums, one thread per row of E, in its mesh's block: each
+// (unknown, offset) cell is one entry of E, summed over the unknown's chunks in order.
+__global__ void __launch_bounds__(64) k_coarse_fill(int nc, const int32_t* __restrict__ cgrp,
+                                                     const int32_t* __restrict__ goff,
+                                                     const int64_t* __restrict__ gjbase,
+                                                     const int32_t* __restrict__ boxchunk,
                                                      const int32_t* __restrict__ boxnb,
                                                      const double* __restrict__ epart, double* __restrict__ gj) {
   const int c = blockIdx.x * 64 + threadIdx.x;
   if (c >= nc) return;
-  const int w = 2 * nc;
-  double* row = gj + (int64_t)c * w;
-  for (int j = 0; j < w; ++j) row[j] = j == nc + c ? 1.0 : 0.0;
+  const int g = cgrp[c], g0 = goff[g], ng = goff[g + 1] - g0, lc = c - g0;
+  const int w = 2 * ng;
+  double* row = gj + gjbase[g] + (int64_t)lc * w;
+  for (int j = 0; j < w; ++j) row[j] = j == ng + lc ? 1.0 : 0.0;
   double own = 0.0;
   for (int o = 0; o < kOffs; ++o) {
     const int cc = boxnb[c * kOffs + o];
     double s = 0.0;
     for (int q = boxchunk[c]; q < boxchunk[c + 1]; ++q) s += epart[(int64_t)q * kOffs + o];
     if (o == kSelf) own += s;       // row sums next to prescribed pressures
-    else if (cc >= 0) { row[cc] = s; own -= s; }
+    else if (cc >= 0) { row[cc - g0] = s; own -= s; }
   }
-  row[c] = own;  // E(c, c) = those row sums - the couplings to the other boxes
+  row[lc] = own;  // E(c, c) = those row sums - the couplings to the other boxes
 }
 
 // Gauss-Jordan step j without row scaling: row i -= (E(i, j) / E(j, j)) row j for every
@@ -197,18 +216,45 @@ __global__ void __launch_bounds__(256) k_gj_step(int nc, int j, double* __restri
   for (int c = j + 1 + threadIdx.x; c <= nc + j; c += 256) ri[c] -= f * rj[c];
 }
 
-// E^{-1} (row-major) = the right half scaled by the diagonal the steps left; with
-// border = 1 also B = E^{-1} - (E^{-1} 1)(1^T E^{-1}) / (1^T E^{-1} 1), the leading block
-// of [E b1; b1^T 0]^{-1}: the correction whose output sums to zero.  A pivot below
-// 1e-15 of the largest (or not finite) zeroes both and sets *flag.  One workgroup; sums
-// in a fixed order.
-__global__ void __launch_bounds__(256) k_coarse_finish(int nc, int border, const double* __restrict__ gj,
-                                                       double* __restrict__ einv, double* __restrict__ einvb,
+// All Gauss-Jordan steps of one mesh's block (the same steps as k_gj_step), one
+// workgroup per mesh: the batches' blocks are small (k^dim unknowns).
+__global__ void __launch_bounds__(256) k_gj_group(const int32_t* __restrict__ goff,
+                                                  const int64_t* __restrict__ gjbase, double* __restrict__ gj) {
+  const int g = blockIdx.x;
+  const int ng = goff[g + 1] - goff[g], w = 2 * ng;
+  double* a = gj + gjbase[g];
+  for (int j = 0; j < ng; ++j) {
+    const double* rj = a + (int64_t)j * w;
+    for (int e = threadIdx.x; e < ng * ng; e += 256) {
+      const int i = e / ng, c = j + 1 + (e - i * ng);
+      if (i == j) continue;
+      double* ri = a + (int64_t)i * w;
+      const double f = ri[j] / rj[j];
+      if (f != 0.0) ri[c] -= f * rj[c];
+    }
+    __syncthreads();
+  }
+}
+
+// Per mesh (one workgroup each): E^{-1} (row-major) = the right half scaled by the
+// diagonal the steps left; with border = 1 also B = E^{-1} - (E^{-1} 1)(1^T E^{-1}) /
+// (1^T E^{-1} 1), the leading block of [E b1; b1^T 0]^{-1}: the correction whose output
+// sums to zero.  A pivot below 1e-15 of the mesh's largest (or not finite) zeroes both
+// for that mesh and sets flag[g].  Sums in a fixed order.
+__global__ void __launch_bounds__(256) k_coarse_finish(int border, const int32_t* __restrict__ goff,
+                                                       const int64_t* __restrict__ gjbase,
+                                                       const int64_t* __restrict__ ebase, const double* __restrict__ gjall,
+                                                       double* __restrict__ einvall, double* __restrict__ einvball,
                                                        int32_t* __restrict__ flag) {
   __shared__ double red[256];
   __shared__ double u[kMaxCoarse], v[kMaxCoarse];
   __shared__ double s_sum;
   __shared__ int s_bad;
+  const int g = blockIdx.x;
+  const int nc = goff[g + 1] - goff[g];
+  const double* gj = gjall + gjbase[g];
+  double* einv = einvall + ebase[g];
+  double* einvb = border ? einvball + ebase[g] : nullptr;
   const int tid = threadIdx.x;
   const int w = 2 * nc;
   double dmax = 0.0;
@@ -264,7 +310,7 @@ __global__ void __launch_bounds__(256) k_coarse_finish(int nc, int border, const
       if (border) einvb[e] = 0.0;
     }
   }
-  if (tid == 0) *flag = s_bad;
+  if (tid == 0) flag[g] = s_bad;
 }
 
 // Restriction: chunk sums of r (or of r - w) over the chunk's pressure slots (lanes in
@@ -340,27 +386,31 @@ __global__ void __launch_bounds__(256) k_coarse_az_apply(int64_t nblk, const int
   vp[3] = (T)((double)vp[3] + a3);
 }
 
-// Coarse solve: box sums of the chunk sums (each workgroup, one lane per box, chunks in
-// order), then y = E^{-1} c with one wavefront per row (lanes over columns in order, a
-// fixed shuffle tree).
-__global__ void __launch_bounds__(256) k_coarse_solve(int nc, const int32_t* __restrict__ boxchunk,
+// Coarse solve, per workgroup record {mesh g, first row r0}: the mesh's box sums of the
+// chunk sums (one lane per box, chunks in order), then y = E_g^{-1} c for rows r0 .. r0+3
+// of the mesh, one wavefront per row (lanes over columns in order, a fixed shuffle tree).
+__global__ void __launch_bounds__(256) k_coarse_solve(const int2* __restrict__ swg, const int32_t* __restrict__ goff,
+                                                      const int64_t* __restrict__ ebase,
+                                                      const int32_t* __restrict__ boxchunk,
                                                       const double* __restrict__ cpart,
                                                       const double* __restrict__ einv, double* __restrict__ y) {
   __shared__ double c[kMaxCoarse];
   const int tid = threadIdx.x;
-  for (int b = tid; b < nc; b += 256) {
+  const int2 rec = swg[blockIdx.x];
+  const int g0 = goff[rec.x], ng = goff[rec.x + 1] - g0;
+  for (int b = tid; b < ng; b += 256) {
     double s = 0.0;
-    for (int q = boxchunk[b]; q < boxchunk[b + 1]; ++q) s += cpart[q];
+    for (int q = boxchunk[g0 + b]; q < boxchunk[g0 + b + 1]; ++q) s += cpart[q];
     c[b] = s;
   }
   __syncthreads();
-  const int i = blockIdx.x * 4 + (tid >> 6), lane = tid & 63;
-  if (i >= nc) return;
-  const double* row = einv + (int64_t)i * nc;
+  const int i = rec.y + (tid >> 6), lane = tid & 63;
+  if (i >= ng) return;
+  const double* row = einv + ebase[rec.x] + (int64_t)i * ng;
   double s = 0.0;
-  for (int j = lane; j < nc; j += 64) s += row[j] * c[j];
+  for (int j = lane; j < ng; j += 64) s += row[j] * c[j];
   for (int h = 32; h > 0; h >>= 1) s += __shfl_xor(s, h);
-  if (lane == 0) y[i] = s;
+  if (lane == 0) y[g0 + i] = s;
 }
 
 // Prolongation: z[slot] += y[box of the slot's chunk].
@@ -390,24 +440,42 @@ static int upload(X** d, const std::vector<X>& h) {
   return SFEM_OK;
 }
 
-// Box coordinates of every node: box b_d = min(k - 1, floor(k (x_d - min_d) / extent_d)).
-static void node_boxes(const sfem_ctx* ctx, int k, std::vector<int32_t>& pos) {
+// Mesh of every node (0 for a single mesh; a batch's meshes, nodes of no element -1).
+static int node_groups(const sfem_ctx* ctx, std::vector<int32_t>& grp) {
+  const std::vector<int32_t>* nm = batch_node_mesh(ctx);
+  int ng = 1;
+  if (nm) {
+    grp = *nm;
+    for (int32_t m : grp) ng = std::max(ng, m + 1);
+  } else {
+    grp.assign(ctx->nNode, 0);
+  }
+  return ng;
+}
+
+// Box coordinates of every node on its mesh's bounding box:
+// b_d = min(k - 1, floor(k (x_d - min_d) / extent_d)).
+static void node_boxes(const sfem_ctx* ctx, int k, const std::vector<int32_t>& grp, int ngroup,
+                       std::vector<int32_t>& pos) {
   const int nd = ctx->dims.ndim;
   const int64_t nn = ctx->nNode;
-  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-  for (int d = 0; d < nd; ++d) {
-    lo[d] = hi[d] = ctx->coords[d];
-    for (int64_t v = 1; v < nn; ++v) {
-      lo[d] = std::min(lo[d], ctx->coords[v * nd + d]);
-      hi[d] = std::max(hi[d], ctx->coords[v * nd + d]);
+  std::vector<double> lo((size_t)ngroup * 3, 1e300), hi((size_t)ngroup * 3, -1e300);
+  for (int64_t v = 0; v < nn; ++v) {
+    const int g = grp[v];
+    if (g < 0) continue;
+    for (int d = 0; d < nd; ++d) {
+      lo[g * 3 + d] = std::min(lo[g * 3 + d], ctx->coords[v * nd + d]);
+      hi[g * 3 + d] = std::max(hi[g * 3 + d], ctx->coords[v * nd + d]);
     }
   }
   pos.assign(nn, 0);
   for (int64_t v = 0; v < nn; ++v) {
+    const int g = grp[v];
+    if (g < 0) continue;
     int32_t code = 0;
     for (int d = 0; d < nd; ++d) {
-      const double ext = hi[d] - lo[d];
-      int b = ext > 0 ? (int)std::floor((ctx->coords[v * nd + d] - lo[d]) / ext * k) : 0;
+      const double ext = hi[g * 3 + d] - lo[g * 3 + d];
+      int b = ext > 0 ? (int)std::floor((ctx->coords[v * nd + d] - lo[g * 3 + d]) / ext * k) : 0;
       b = std::min(std::max(b, 0), k - 1);
       code |= b << (10 * d);
     }
@@ -415,15 +483,16 @@ static void node_boxes(const sfem_ctx* ctx, int k, std::vector<int32_t>& pos) {
   }
 }
 
-// k^dim boxes, and every element's nodes within one box of each other in each direction,
-// so that the neighbours of a block row sit at box offsets in {-1, 0, 1}^dim.
+// k^dim boxes (per mesh), and every element's nodes within one box of each other in each
+// direction, so that the neighbours of a block row sit at box offsets in {-1, 0, 1}^dim.
 static int coarse_check(const sfem_ctx* ctx, int k) {
   const int nd = ctx->dims.ndim, npe = ctx->dims.npe;
   int64_t nbox = 1;
   for (int d = 0; d < nd; ++d) nbox *= k;
   if (k < 1 || nbox > kMaxCoarse) return set_error(SFEM_ERR_ARG, "coarse boxes: k^dim must lie in [1, 1024]");
-  std::vector<int32_t> npos;
-  node_boxes(ctx, k, npos);
+  std::vector<int32_t> grp, npos;
+  const int ngroup = node_groups(ctx, grp);
+  node_boxes(ctx, k, grp, ngroup, npos);
   for (int64_t e = 0; e < ctx->nElem; ++e) {
     const int32_t* en = &ctx->conn[e * npe];
     for (int d = 0; d < nd; ++d) {
@@ -445,8 +514,9 @@ static int coarse_plan(sfem_ctx* ctx, LinAlg* L, Coarse* C) {
   const IluPlan& P = *L->ilu;
   const int k = C->k;
   const int nd = ctx->dims.ndim, ndof = ctx->dims.ndof, npe = ctx->dims.npe;
-  std::vector<int32_t> npos;
-  node_boxes(ctx, k, npos);
+  std::vector<int32_t> grp, npos;
+  const int ngroup = node_groups(ctx, grp);
+  node_boxes(ctx, k, grp, ngroup, npos);
   const int64_t nb = P.nblk;
   const std::vector<int32_t>& bs = L->h_blkStart;
   // nodes sharing an element with a node whose pressure is prescribed: their pressure
@@ -459,8 +529,10 @@ static int coarse_plan(sfem_ctx* ctx, LinAlg* L, Coarse* C) {
     if (fix)
       for (int a = 0; a < npe; ++a) nearfix[en[a]] = 1;
   }
-  // pressure slot of each block row (plan position), its box, and its node's mark
-  std::vector<int32_t> pslot(nb, -1), pbox(nb, 0);
+  // pressure slot of each block row (plan position), its box (mesh << 30 | box
+  // coordinates), and its node's mark
+  std::vector<int32_t> pslot(nb, -1);
+  std::vector<int64_t> pbox(nb, 0);
   std::vector<uint8_t> pfix(nb, 0);
   int64_t npres = 0;
   for (int64_t p = 0; p < nb; ++p) {
@@ -469,29 +541,35 @@ static int coarse_plan(sfem_ctx* ctx, LinAlg* L, Coarse* C) {
       const int32_t g = ctx->assyForSoln[r];
       if (g % ndof == ndof - 1) {
         pslot[p] = r - bs[b];
-        pbox[p] = npos[g / ndof];
+        pbox[p] = (int64_t)std::max(grp[g / ndof], 0) << 30 | npos[g / ndof];
         pfix[p] = nearfix[g / ndof];
         ++npres;
       }
     }
   }
   if (npres == 0) return set_error(SFEM_ERR_ARG, "coarse correction: no free pressure DOF");
-  // coarse unknowns: non-empty boxes in box order
-  std::vector<int32_t> boxes;
-  boxes.reserve(npres);
-  for (int64_t p = 0; p < nb; ++p)
-    if (pslot[p] >= 0) boxes.push_back(pbox[p]);
-  auto key = [](int32_t pos) {  // box order: x fastest
-    return (int64_t)((pos >> 20) & 1023) << 20 | (int64_t)((pos >> 10) & 1023) << 10 | (pos & 1023);
+  // coarse unknowns: non-empty boxes in (mesh, box) order, x fastest
+  auto key = [](int64_t gb) {
+    const int32_t pos = (int32_t)(gb & ((1 << 30) - 1));
+    return (gb >> 30) << 30 | (int64_t)((pos >> 20) & 1023) << 20 | (int64_t)((pos >> 10) & 1023) << 10 | (pos & 1023);
   };
-  std::sort(boxes.begin(), boxes.end(), [&](int32_t a, int32_t b) { return key(a) < key(b); });
-  boxes.erase(std::unique(boxes.begin(), boxes.end()), boxes.end());
-  const int nc = (int)boxes.size();
-  std::vector<int64_t> bkeys(nc);
-  for (int c = 0; c < nc; ++c) bkeys[c] = key(boxes[c]);
-  auto coarse_of = [&](int32_t pos) -> int {
-    const auto it = std::lower_bound(bkeys.begin(), bkeys.end(), key(pos));
-    return it != bkeys.end() && *it == key(pos) ? (int)(it - bkeys.begin()) : -1;
+  std::vector<int64_t> bkeys;
+  bkeys.reserve(npres);
+  for (int64_t p = 0; p < nb; ++p)
+    if (pslot[p] >= 0) bkeys.push_back(key(pbox[p]));
+  std::sort(bkeys.begin(), bkeys.end());
+  bkeys.erase(std::unique(bkeys.begin(), bkeys.end()), bkeys.end());
+  const int nc = (int)bkeys.size();
+  std::vector<int32_t> boxes(nc), cgrp(nc);
+  for (int c = 0; c < nc; ++c) {
+    const int64_t kk = bkeys[c];
+    const int32_t bx = kk & 1023, by = (kk >> 10) & 1023, bz = (kk >> 20) & 1023;
+    boxes[c] = bx | by << 10 | bz << 20;
+    cgrp[c] = (int32_t)(kk >> 30);
+  }
+  auto coarse_of = [&](int64_t gb) -> int {
+    const auto it = std::lower_bound(bkeys.begin(), bkeys.end(), key(gb));
+    return it != bkeys.end() && *it == key(gb) ? (int)(it - bkeys.begin()) : -1;
   };
   std::vector<int32_t> cinfo(nb, -1);
   std::vector<std::pair<int32_t, int32_t>> order;  // (coarse unknown, slot)
@@ -526,8 +604,25 @@ static int coarse_plan(sfem_ctx* ctx, LinAlg* L, Coarse* C) {
     for (int o = 0; o < kOffs; ++o) {
       const int x = bx + o % 3 - 1, y = by + (o / 3) % 3 - 1, z = bz + o / 9 - 1;
       if (x < 0 || y < 0 || z < 0 || x >= k || y >= ky || z >= kz) continue;
-      boxnb[(size_t)c * kOffs + o] = coarse_of(x | y << 10 | z << 20);
+      boxnb[(size_t)c * kOffs + o] = coarse_of((int64_t)cgrp[c] << 30 | (x | y << 10 | z << 20));
     }
+  }
+  // per-mesh blocks: unknown ranges, workspace and inverse offsets, solve records
+  std::vector<int32_t> goff(ngroup + 1, 0);
+  for (int c = 0; c < nc; ++c) goff[cgrp[c] + 1]++;
+  for (int g = 0; g < ngroup; ++g) goff[g + 1] += goff[g];
+  std::vector<int64_t> gjbase(ngroup), ebase(ngroup);
+  std::vector<int2> swg;
+  int64_t gjtot = 0, etot = 0;
+  int maxng = 0;
+  for (int g = 0; g < ngroup; ++g) {
+    const int ng = goff[g + 1] - goff[g];
+    gjbase[g] = gjtot;
+    ebase[g] = etot;
+    gjtot += 2 * (int64_t)ng * ng;
+    etot += (int64_t)ng * ng;
+    maxng = std::max(maxng, ng);
+    for (int r0 = 0; r0 < ng; r0 += 4) swg.push_back(int2{g, r0});
   }
   // A Z's sparsity: per block row the coarse unknowns of its neighbours' pressure DOFs
   std::vector<int4> hbd(nb);
@@ -554,6 +649,14 @@ static int coarse_plan(sfem_ctx* ctx, LinAlg* L, Coarse* C) {
   int rc;
   if ((rc = upload(&C->d_azoff, azoff)) || (rc = upload(&C->d_azbox, azbox))) return rc;
   C->naz = (int64_t)azbox.size();
+  if ((rc = upload(&C->d_cgrp, cgrp)) || (rc = upload(&C->d_goff, goff)) || (rc = upload(&C->d_gjbase, gjbase)) ||
+      (rc = upload(&C->d_ebase, ebase)) || (rc = upload(&C->d_swg, swg)))
+    return rc;
+  C->ngroup = ngroup;
+  C->h_goff = goff;
+  C->h_ebase = ebase;
+  C->nswg = (int)swg.size();
+  C->maxng = maxng;
   if ((rc = upload(&C->d_slot, slot)) || (rc = upload(&C->d_chunk, chunk)) || (rc = upload(&C->d_chunkbox, chunkbox)) ||
       (rc = upload(&C->d_boxchunk, boxchunk)) || (rc = upload(&C->d_cinfo, cinfo)) || (rc = upload(&C->d_cpos, boxes)) ||
       (rc = upload(&C->d_boxnb, boxnb)))
@@ -566,11 +669,11 @@ static int coarse_plan(sfem_ctx* ctx, LinAlg* L, Coarse* C) {
   for (void* q : old)
     if (q) (void)hipFree(q);
   CO_CHECK(hipMalloc((void**)&C->d_epart, (size_t)C->nchunk * kOffs * sizeof(double)));
-  CO_CHECK(hipMalloc((void**)&C->d_gj, (size_t)nc * 2 * nc * sizeof(double)));
-  CO_CHECK(hipMalloc((void**)&C->d_einv, (size_t)nc * nc * sizeof(double)));
+  CO_CHECK(hipMalloc((void**)&C->d_gj, std::max<size_t>((size_t)gjtot, 1) * sizeof(double)));
+  CO_CHECK(hipMalloc((void**)&C->d_einv, std::max<size_t>((size_t)etot, 1) * sizeof(double)));
   CO_CHECK(hipMalloc((void**)&C->d_cpart, (size_t)C->nchunk * sizeof(double)));
   CO_CHECK(hipMalloc((void**)&C->d_y, (size_t)nc * sizeof(double)));
-  CO_CHECK(hipMalloc((void**)&C->d_flag, sizeof(int32_t)));
+  CO_CHECK(hipMalloc((void**)&C->d_flag, ngroup * sizeof(int32_t)));
   CO_CHECK(hipMalloc(&C->d_az, std::max<size_t>((size_t)C->naz, 1) * 4 * sizeof(double)));
   CO_CHECK(hipMalloc(&C->d_tmp, (size_t)P.npad * sizeof(double)));
   CO_CHECK(hipMemset(C->d_tmp, 0, (size_t)P.npad * sizeof(double)));
@@ -589,10 +692,9 @@ int coarse_set(sfem_ctx* ctx, int k, int mult) {
   }
   int rc = coarse_check(ctx, k);
   if (rc) return rc;
-  if (L->coarse && L->coarse->k != k) {
-    delete L->coarse;
-    L->coarse = nullptr;
-  }
+  // planned again at the next preconditioner build (boxes, meshes and form may differ)
+  delete L->coarse;
+  L->coarse = nullptr;
   L->coarse_k = k;
   return SFEM_OK;
 }
@@ -601,7 +703,7 @@ int coarse_set(sfem_ctx* ctx, int k, int mult) {
 // ILU(0) factorisation.
 int coarse_build(sfem_ctx* ctx, const double* kp, const float* kp32) {
   LinAlg* L = ctx->la;
-  if (L->coarse_k <= 0 || ctx->batch) return SFEM_OK;
+  if (L->coarse_k <= 0) return SFEM_OK;
   if (!L->coarse) {
     L->coarse = new Coarse;
     L->coarse->k = L->coarse_k;
@@ -622,15 +724,20 @@ int coarse_build(sfem_ctx* ctx, const double* kp, const float* kp32) {
     hipLaunchKernelGGL((k_coarse_rows<double>), dim3(C->nchunk), dim3(64), 0, st, C->d_slot, C->d_chunk, bd, P.d_bcol,
                        kp, C->d_cinfo, C->d_cpos, C->d_epart);
   const int nc = C->nc;
-  hipLaunchKernelGGL(k_coarse_fill, dim3((nc + 63) / 64), dim3(64), 0, st, nc, C->d_boxchunk, C->d_boxnb, C->d_epart,
-                     C->d_gj);
-  for (int j = 0; j < nc; ++j) hipLaunchKernelGGL(k_gj_step, dim3(nc), dim3(256), 0, st, nc, j, C->d_gj);
+  hipLaunchKernelGGL(k_coarse_fill, dim3((nc + 63) / 64), dim3(64), 0, st, nc, C->d_cgrp, C->d_goff, C->d_gjbase,
+                     C->d_boxchunk, C->d_boxnb, C->d_epart, C->d_gj);
+  if (C->ngroup == 1)  // one large block: one launch per step, one workgroup per row
+    for (int j = 0; j < nc; ++j) hipLaunchKernelGGL(k_gj_step, dim3(nc), dim3(256), 0, st, nc, j, C->d_gj);
+  else                 // a batch's small blocks: one workgroup per mesh
+    hipLaunchKernelGGL(k_gj_group, dim3(C->ngroup), dim3(256), 0, st, C->d_goff, C->d_gjbase, C->d_gj);
   // FP32 Krylov vectors: their correction leaves the global constant pressure to the FP64
   // coarse step of the Newton loop
   const int border = kp32 != nullptr;
-  if (border && !C->d_einvb) CO_CHECK(hipMalloc((void**)&C->d_einvb, (size_t)nc * nc * sizeof(double)));
-  hipLaunchKernelGGL(k_coarse_finish, dim3(1), dim3(256), 0, st, nc, border, C->d_gj, C->d_einv, C->d_einvb,
-                     C->d_flag);
+  const size_t etot = (size_t)(C->h_ebase.back() + (int64_t)(C->h_goff[C->ngroup] - C->h_goff[C->ngroup - 1]) *
+                                                       (C->h_goff[C->ngroup] - C->h_goff[C->ngroup - 1]));
+  if (border && !C->d_einvb) CO_CHECK(hipMalloc((void**)&C->d_einvb, std::max<size_t>(etot, 1) * sizeof(double)));
+  hipLaunchKernelGGL(k_coarse_finish, dim3(C->ngroup), dim3(256), 0, st, border, C->d_goff, C->d_gjbase, C->d_ebase,
+                     C->d_gj, C->d_einv, C->d_einvb, C->d_flag);
   C->fp32 = border;
   CO_CHECK(hipGetLastError());
   C->ready = true;
@@ -714,10 +821,9 @@ template double* coarse_tmp<double>(sfem_ctx*);
 template <typename T>
 int coarse_correct(sfem_ctx* ctx, T* z) {
   Coarse* C = ctx->la->coarse;
-  const int g = (C->nc + 3) / 4;  // four rows per workgroup
   const double* inv = sizeof(T) == 4 && C->d_einvb && C->fp32 ? C->d_einvb : C->d_einv;
-  hipLaunchKernelGGL(k_coarse_solve, dim3(g), dim3(256), 0, ctx->stream, C->nc, C->d_boxchunk, C->d_cpart, inv,
-                     C->d_y);
+  hipLaunchKernelGGL(k_coarse_solve, dim3(C->nswg), dim3(256), 0, ctx->stream, C->d_swg, C->d_goff, C->d_ebase,
+                     C->d_boxchunk, C->d_cpart, inv, C->d_y);
   hipLaunchKernelGGL((k_coarse_prolong<T>), dim3(C->nchunk), dim3(64), 0, ctx->stream, C->d_slot, C->d_chunk,
                      C->d_chunkbox, C->d_y, z);
   return SFEM_OK;
@@ -737,8 +843,10 @@ int coarse_info(sfem_ctx* ctx, int64_t* out) {
   out[3] = C ? C->nchunk : 0;
   int32_t flag = 0;
   if (C && C->d_flag && C->ready) {
+    std::vector<int32_t> f(C->ngroup);
     CO_CHECK(hipStreamSynchronize(ctx->stream));
-    CO_CHECK(hipMemcpy(&flag, C->d_flag, sizeof(flag), hipMemcpyDeviceToHost));
+    CO_CHECK(hipMemcpy(f.data(), C->d_flag, f.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    for (int32_t v : f) flag |= v;
   }
   out[4] = flag;
   out[5] = C && C->ready ? 1 : 0;
@@ -746,15 +854,23 @@ int coarse_info(sfem_ctx* ctx, int64_t* out) {
 }
 
 // E^{-1} (which = 0) or the FP32 vectors' inverse without the global constant (which = 1),
-// nc x nc row-major, of the last build.
+// nc x nc row-major (block diagonal for a batch, one block per mesh), of the last build.
 int coarse_get_inverse(sfem_ctx* ctx, int which, double* out, int cap) {
   const Coarse* C = ctx->la->coarse;
   if (!C || !C->ready) return set_error(SFEM_ERR_STATE, "coarse correction not built");
   if (which == 1 && !(C->fp32 && C->d_einvb)) return set_error(SFEM_ERR_STATE, "no FP32 coarse inverse");
-  if (cap < C->nc * C->nc) return set_error(SFEM_ERR_ARG, "buffer too small");
+  const int64_t nc = C->nc;
+  if (cap < nc * nc) return set_error(SFEM_ERR_ARG, "buffer too small");
+  const int64_t ng1 = C->h_goff[C->ngroup] - C->h_goff[C->ngroup - 1];
+  std::vector<double> blk((size_t)(C->h_ebase.back() + ng1 * ng1));
   CO_CHECK(hipStreamSynchronize(ctx->stream));
-  CO_CHECK(hipMemcpy(out, which ? C->d_einvb : C->d_einv, (size_t)C->nc * C->nc * sizeof(double),
-                     hipMemcpyDeviceToHost));
+  CO_CHECK(hipMemcpy(blk.data(), which ? C->d_einvb : C->d_einv, blk.size() * sizeof(double), hipMemcpyDeviceToHost));
+  std::fill(out, out + nc * nc, 0.0);
+  for (int g = 0; g < C->ngroup; ++g) {
+    const int g0 = C->h_goff[g], ng = C->h_goff[g + 1] - g0;
+    for (int i = 0; i < ng; ++i)
+      for (int j = 0; j < ng; ++j) out[(g0 + i) * nc + g0 + j] = blk[C->h_ebase[g] + (int64_t)i * ng + j];
+  }
   return SFEM_OK;
 }
 

@@ -1786,6 +1786,7 @@ int la_create(sfem_ctx* ctx) {
     dp[r] = (f != &ctx->colidx[0] + e && *f == r) ? (int32_t)(f - &ctx->colidx[0]) : -1;
   }
   L->maxrow = maxrow;
+  L->all_diag = std::find(dp.begin(), dp.end(), -1) == dp.end();
   L->nk = n;
   L->nvec = std::max<int64_t>(n, 4 * L->nblk);  // Krylov vectors fit both layouts
   L->h_diagpos = dp;
@@ -2434,8 +2435,7 @@ int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f, 
   const float* vals32 = fp32 ? ctx->d_vals32 : nullptr;
   LA_CHECK(hipMemsetAsync(L->d_scal + S_MISC1, 0, sizeof(double), st));
   if (pc_type == SFEM_PC_JACOBI) {
-    for (int64_t r = 0; r < n; ++r)
-      if (L->h_diagpos[r] < 0) return set_error(SFEM_ERR_ARG, "Jacobi: missing diagonal entry");
+    if (!L->all_diag) return set_error(SFEM_ERR_ARG, "Jacobi: missing diagonal entry");
     if (fp32) {
       int rc = ensure_vec(&L->dinv32, n); if (rc) return rc;
       hipLaunchKernelGGL((k_jac_setup<float>), dim3((unsigned)((n + kTPB - 1) / kTPB)), dim3(kTPB), 0, st,
@@ -2460,8 +2460,7 @@ int la_setup_pc(sfem_ctx* ctx, int pc_type, bool fp32, int reorder, bool bf16f, 
     }
     LA_CHECK(hipGetLastError());
   } else if (pc_type == SFEM_PC_ILU0) {
-    for (int64_t r = 0; r < n; ++r)
-      if (L->h_diagpos[r] < 0) return set_error(SFEM_ERR_ARG, "ILU(0): missing diagonal entry");
+    if (!L->all_diag) return set_error(SFEM_ERR_ARG, "ILU(0): missing diagonal entry");
     if (!L->ilu || L->ilu->reorder != reorder) {
       int rc = build_ilu_plan(ctx, L, reorder);
       if (rc) return rc;
@@ -2751,18 +2750,18 @@ static int pc_apply_impl(sfem_ctx* ctx, const T* r, T* z, const RowSubset* sub) 
     }
     case SFEM_PC_ILU0:
       if (!ilu_buffers_ready(L)) return set_error(SFEM_ERR_STATE, "ILU(0) factor buffers missing for its precision");
-      if (!sub && coarse_active(L) && !L->coarse_mult) {
+      if (coarse_active(L) && !L->coarse_mult) {
         // additive coarse correction: r restricted before the sweeps (z may alias r)
         coarse_restrict<T>(ctx, r);
         ilu_apply<T>(ctx, r, z, -1, nullptr, sub);
         coarse_correct<T>(ctx, z);
-      } else if (!sub && coarse_active(L)) {
+      } else if (coarse_active(L)) {
         // multiplicative: z = M^-1 r, then z += Z E^-1 Z^T (r - A z) (the product into the
         // correction's own buffer; BiCGStab uses the fused form in pc_op instead)
         if (r == z) return set_error(SFEM_ERR_ARG, "multiplicative coarse correction: r and z must differ");
         T* az = coarse_tmp<T>(ctx);
         ilu_apply<T>(ctx, r, z, -1, nullptr, sub);
-        spmv_in<T>(ctx, z, az, true);
+        spmv_in<T>(ctx, z, az, true, sub);
         coarse_restrict<T>(ctx, r, az);
         coarse_correct<T>(ctx, z);
       } else {
@@ -3104,20 +3103,20 @@ constexpr int kMaxReplacements = 8;  // true-residual replacements per BiCGStab 
 // ILU(0) output is formed once and reused: z0 = M_ILU^-1 r, out0 = A z0, y = E^-1 Z^T
 // (r - out0), z = z0 + Z y, out = out0 + (A Z) y, with A Z kept per block row.
 template <typename T>
-static int pc_op(sfem_ctx* ctx, const T* r, T* z, T* out) {
+static int pc_op(sfem_ctx* ctx, const T* r, T* z, T* out, const RowSubset* sub = nullptr) {
   LinAlg* L = ctx->la;
   if (!(L->pc_type == SFEM_PC_ILU0 && L->coarse_mult && coarse_active(L))) {
-    int rc = pc_apply<T>(ctx, r, z);
+    int rc = pc_apply<T>(ctx, r, z, sub);
     if (rc) return rc;
-    spmv<T>(ctx, z, out);
+    spmv_in<T>(ctx, z, out, L->block_space, sub);
     return SFEM_OK;
   }
   if (!ilu_buffers_ready(L)) return set_error(SFEM_ERR_STATE, "ILU(0) factor buffers missing for its precision");
   hipStream_t st = ctx->stream;
   ctx->kt.begin(SFEM_KT_PC, st);
-  ilu_apply<T>(ctx, r, z);
+  ilu_apply<T>(ctx, r, z, -1, nullptr, sub);
   ctx->kt.end(SFEM_KT_PC, st);
-  spmv_in<T>(ctx, z, out, true);
+  spmv_in<T>(ctx, z, out, true, sub);
   ctx->kt.begin(SFEM_KT_PC, st);
   coarse_restrict<T>(ctx, r, out);
   coarse_correct<T>(ctx, z);
@@ -3649,6 +3648,32 @@ int la_apply_op32(sfem_ctx* ctx, const float* x, float* y, const RowSubset* sub)
   spmv_in<float>(ctx, x, y, ctx->la->block_space, sub);
   return hipGetLastError() == hipSuccess ? SFEM_OK : set_error(SFEM_ERR_HIP, "spmv launch");
 }
+// The coarse step's buffers (FP64 padded vectors), allocated before any capture.
+int la_coarse_step_prepare(sfem_ctx* ctx) {
+  LinAlg* L = ctx->la;
+  int rc;
+  if ((rc = ensure_vec(&L->pr, L->nvec)) || (rc = ensure_vec(&L->pz, L->nvec))) return rc;
+  return SFEM_OK;
+}
+
+// The coarse step's kernels alone (capturable): increment = Z E^{-1} Z^T rhs in FP64
+// (DOF layout), with the correction already built and la_coarse_step_prepare run.
+int la_coarse_step_kernels(sfem_ctx* ctx) {
+  LinAlg* L = ctx->la;
+  if (!coarse_active(L) || !L->pr || !L->pz) return set_error(SFEM_ERR_STATE, "coarse step: correction not built");
+  hipStream_t st = ctx->stream;
+  const int64_t nk = L->nk;
+  const int32_t* padmap = L->ilu->d_padmap;
+  hipLaunchKernelGGL((k_pad_in<double, double>), dim3(grid_for(nk)), dim3(kTPB), 0, st, nk, padmap, ctx->d_rhs, L->pr,
+                     1.0);
+  hipLaunchKernelGGL((k_fill<double>), dim3(grid_for(nk)), dim3(kTPB), 0, st, nk, L->pz, 0.0);
+  coarse_restrict<double>(ctx, L->pr);
+  coarse_correct<double>(ctx, L->pz);
+  hipLaunchKernelGGL((k_pad_out<double, double>), dim3(grid_for(nk)), dim3(kTPB), 0, st, nk, padmap, L->pz, ctx->d_du,
+                     1.0);
+  return hipGetLastError() == hipSuccess ? SFEM_OK : set_error(SFEM_ERR_HIP, "coarse step launch");
+}
+
 // The Newton loop's coarse step: increment = Z E^{-1} Z^T rhs in FP64 (DOF layout), with
 // the preconditioner (and E) brought up to date for opts first.
 int la_coarse_step(sfem_ctx* ctx, const sfem_solver_opts* o) {
@@ -3670,6 +3695,15 @@ int la_coarse_step(sfem_ctx* ctx, const sfem_solver_opts* o) {
   hipLaunchKernelGGL((k_pad_out<double, double>), dim3(grid_for(nk)), dim3(kTPB), 0, st, nk, padmap, L->pz, ctx->d_du,
                      1.0);
   return hipGetLastError() == hipSuccess ? SFEM_OK : set_error(SFEM_ERR_HIP, "coarse step launch");
+}
+
+int la_apply_pc_op32(sfem_ctx* ctx, const float* r, float* z, float* out, const RowSubset* sub) {
+  if (sub && ctx->la->pc_type != SFEM_PC_ILU0) return set_error(SFEM_ERR_UNSUPPORTED, "row subsets need ILU(0)");
+  return pc_op<float>(ctx, r, z, out, sub);
+}
+int la_apply_pc_op64(sfem_ctx* ctx, const double* r, double* z, double* out, const RowSubset* sub) {
+  if (sub && ctx->la->pc_type != SFEM_PC_ILU0) return set_error(SFEM_ERR_UNSUPPORTED, "row subsets need ILU(0)");
+  return pc_op<double>(ctx, r, z, out, sub);
 }
 
 int la_apply_op64(sfem_ctx* ctx, const double* x, double* y, const RowSubset* sub) {

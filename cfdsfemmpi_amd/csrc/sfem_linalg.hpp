@@ -94,6 +94,7 @@ struct LinAlg {
   int pc_lag = 1;             // rebuild the preconditioner every pc_lag-th new tangent
   int lag_count = 0;
   std::vector<int32_t> h_blkStart, h_diagpos;
+  bool all_diag = false;      // every row's pattern holds its diagonal (checked once per pattern)
   int pc_reorder = 0;
   bool pc_bf16 = false;
   bool pc_store32 = false;    // FP64-built ILU(0) whose factor and block tangent are kept in FP32
@@ -175,6 +176,10 @@ int la_apply_pc32(sfem_ctx* ctx, const float* r, float* z, const RowSubset* sub 
 int la_apply_pc64(sfem_ctx* ctx, const double* r, double* z, const RowSubset* sub = nullptr);
 int la_apply_op32(sfem_ctx* ctx, const float* x, float* y, const RowSubset* sub = nullptr);
 int la_apply_op64(sfem_ctx* ctx, const double* x, double* y, const RowSubset* sub = nullptr);
+// z = M^{-1} r and out = A z together (the multiplicative coarse correction reuses the
+// product of the ILU(0) output)
+int la_apply_pc_op32(sfem_ctx* ctx, const float* r, float* z, float* out, const RowSubset* sub = nullptr);
+int la_apply_pc_op64(sfem_ctx* ctx, const double* r, double* z, double* out, const RowSubset* sub = nullptr);
 // Bring the preconditioner and the Krylov layout up to date for opts (as la_solve does).
 int la_prepare_solve(sfem_ctx* ctx, const sfem_solver_opts* o);
 
@@ -219,6 +224,8 @@ template <typename T> int coarse_correct(sfem_ctx* ctx, T* z);
 int coarse_info(sfem_ctx* ctx, int64_t* out);
 int coarse_get_inverse(sfem_ctx* ctx, int which, double* out, int cap);
 int la_coarse_step(sfem_ctx* ctx, const sfem_solver_opts* o);
+int la_coarse_step_prepare(sfem_ctx* ctx);
+int la_coarse_step_kernels(sfem_ctx* ctx);
 void coarse_destroy(Coarse* C);
 // CSR values from the assembled node-pair block form when the tangent changed since
 // the last conversion (element loops that assemble the block form, ctx->kb_on)
@@ -230,5 +237,7 @@ int la_spmv32(sfem_ctx* ctx, const float* x, float* y);
 void batch_destroy(sfem_ctx* ctx);
 // FP32 mesh solves of batched steps that stalled and kept their best iterate
 int64_t batch_stalled(const sfem_ctx* ctx);
+// mesh of every node of a batch (NULL without one)
+const std::vector<int32_t>* batch_node_mesh(const sfem_ctx* ctx);
 
 }  // namespace sfem

@@ -260,7 +260,7 @@ def test_coarse_cavity_same_answer_fewer_iterations(prec, mult):
 
 
 def test_coarse_rejections():
-    """Too many boxes, boxes narrower than an element, and batches are refused."""
+    """Too many boxes and boxes narrower than an element are refused."""
     mesh = meshgen.ldc3d_hex(4, graded=True)
     ctx = _ctx(mesh)
     with pytest.raises(sfem.SfemError):
@@ -271,9 +271,97 @@ def test_coarse_rejections():
     ctx.set_coarse_correction(0)
     assert ctx.coarse_info()["k"] == 0
     ctx.close()
-    m2 = meshgen.ldc2d_tria(6, jitter=0.2, seed=1)
-    ctx = _ctx(m2)
-    ctx.set_batch(np.zeros(m2.nElem, np.int32), np.array([1e-2]))
-    with pytest.raises(sfem.SfemError):
-        ctx.set_coarse_correction(2)
+
+
+def _batch(count, seed, n_range, prec, opts, dt, k, mult):
+    mesh, em, mu, ns, meshes = meshgen.c4_batch(count, seed=seed, n_range=n_range)
+    ed = np.array([1.0, 0.01, 0, 0, 0, 0, 0, 0, 1, 1, 0])
+    ctx = sfem.Context(mesh)
+    ctx.set_dirichlet(mesh.dbc_node, mesh.dbc_dof, mesh.dbc_value)
+    ctx.prepare_elements()
+    ctx.build_pattern()
+    ctx.set_params(ed, 2, 0.0, dt)
+    ctx.set_precision(prec)
+    ctx.set_tuning_mode(sfem.TUNE_FIXED)
+    ctx.set_batch(em, mu)
+    ctx.set_coarse_correction(k, mult)
+    ctx.setup_solver(opts)
+    return ctx, mesh, em, mu, ns, meshes
+
+
+def test_coarse_batch_block_inverse():
+    """A batch gets k^2 boxes per mesh on the mesh's own bounding box: E is block diagonal,
+    one block per mesh, each equal to that mesh's own E (numpy on the exported tangent of
+    the mesh alone) within 1e-8 relative in its inverse."""
+    opts = sfem.solver_opts("bcgs", "ilu", 1e-8, 3000, reorder=2)
+    ctx, mesh, em, mu, ns, meshes = _batch(3, 7, (6, 14), sfem.PREC_FP64, opts, 0.05, 3, True)
+    ctx.step_batch(0.05, opts, 1e-8, 10)
+    Einv = ctx.coarse_inverse()
+    info = ctx.coarse_info()
+    print(f"[coarse batch] sizes {list(ns)} unknowns {info['unknowns']}", flush=True)
+    assert info["built"] and not info["singular"]
     ctx.close()
+    off = 0
+    for k, m in enumerate(meshes):
+        ed = np.array([1.0, mu[k], 0, 0, 0, 0, 0, 0, 1, 1, 0])
+        c1 = sfem.Context(m)
+        c1.set_dirichlet(m.dbc_node, m.dbc_dof, m.dbc_value)
+        c1.prepare_elements()
+        c1.build_pattern()
+        c1.set_params(ed, 2, 0.0, 0.05)
+        c1.time_update(0.05)
+        c1.assemble(0)
+        A, _ = _system(c1)
+        Z, _, Eid = _coarse_matrices(m, A, 3)
+        nc = Z.shape[1]
+        blk = Einv[off:off + nc, off:off + nc]
+        # the batch's block comes from the batch's first tangent (zero state, as here)
+        ref = np.linalg.inv(Eid)
+        e = np.max(np.abs(blk - ref)) / np.max(np.abs(ref))
+        print(f"[coarse batch] mesh {k}: {nc} unknowns, rel diff {e:.2e}", flush=True)
+        assert e <= 1e-8
+        assert np.all(Einv[off:off + nc, :off] == 0) and np.all(Einv[off:off + nc, off + nc:] == 0)
+        off += nc
+        c1.close()
+
+
+@pytest.mark.parametrize("prec", [sfem.PREC_FP64, sfem.PREC_STORE32, sfem.PREC_MIXED])
+def test_coarse_batch_matches_single_meshes(prec):
+    """A batch of jittered Tria3 cavities with the multiplicative coarse correction (4^2
+    boxes per mesh), stepped together, against each mesh stepped alone through the FP64
+    single-mesh path without it, both to ||F|| < 1e-10: within 1e-7 (FP64 Krylov) or 5e-7
+    (FP32), the bars of the batch test without the correction."""
+    dt, steps, ntol = 0.05, 3, 1e-10
+    rtol = 1e-10 if prec == sfem.PREC_FP64 else 1e-4
+    opts = sfem.solver_opts("bcgs", "ilu", rtol, 3000, reorder=2)
+    ctx, mesh, em, mu, ns, meshes = _batch(6, 3, (8, 48), prec, opts, dt, 4, True)
+    t = dt
+    tot = 0
+    for s in range(steps):
+        ni, ki, rn = ctx.step_batch(min(t, 1.0), opts, ntol, 10)
+        print(f"[coarse batch] prec={prec} step {s}: newton {ni.tolist()} krylov {ki.tolist()}", flush=True)
+        assert np.all(rn < ntol)
+        tot += int(np.sum(ki))
+        t += dt
+    sb = ctx.get_solution().reshape(-1, 3)
+    ctx.close()
+    off = 0
+    o64 = sfem.solver_opts("bcgs", "ilu", 1e-10, 3000, reorder=2)
+    for k, m in enumerate(meshes):
+        ed = np.array([1.0, mu[k], 0, 0, 0, 0, 0, 0, 1, 1, 0])
+        c1 = sfem.Context(m)
+        c1.set_dirichlet(m.dbc_node, m.dbc_dof, m.dbc_value)
+        c1.prepare_elements()
+        c1.build_pattern()
+        c1.set_params(ed, 2, 0.0, dt)
+        t = dt
+        for s in range(steps):
+            st = c1.step(min(t, 1.0), o64, ntol, 10)
+            assert st.rhs_norm < ntol
+            t += dt
+        s1 = c1.get_solution().reshape(-1, 3)
+        c1.close()
+        d = np.linalg.norm(sb[off:off + m.nNode] - s1) / np.linalg.norm(s1)
+        print(f"[coarse batch] mesh {k}: rel diff vs FP64 alone {d:.2e}", flush=True)
+        assert d <= (1e-7 if prec == sfem.PREC_FP64 else 5e-7)
+        off += m.nNode
