@@ -210,15 +210,17 @@ int sfem_set_pc_lag(sfem_ctx* ctx, int lag);
  * tangent does not resolve it either: there the correction uses B, the leading block of
  * [E b1; b1^T 0]^{-1} (its outputs sum to zero), and sfem_step (and the host driver)
  * adds a coarse step in FP64 before each inner solve -- x += Z E^{-1} Z^T F(x), then the
- * residual again (sfem_coarse_step).  Single meshes only (SFEM_ERR_UNSUPPORTED on a
- * batch).  No counterpart in the reference, which preconditions with PETSc's ILU(0)
- * alone (SolverPetsc::factoriseAndSolve, SolverPetsc.cpp:284-298). */
+ * residual again (sfem_coarse_step).  On a batch (sfem_set_batch, before or after this
+ * call) every mesh gets its own k boxes per direction on its own bounding box, so E is
+ * block diagonal, and sfem_step_batch takes the coarse step per mesh.  No counterpart in
+ * the reference, which preconditions with PETSc's ILU(0) alone
+ * (SolverPetsc::factoriseAndSolve, SolverPetsc.cpp:284-298). */
 int sfem_set_coarse_correction(sfem_ctx* ctx, int boxes_per_dim, int multiplicative);
 /* info[6] = {k, coarse unknowns, free pressure DOFs, restriction chunks, 1 if the last E
  * was singular (correction off until the next rebuild), 1 if built}. */
 int sfem_coarse_info(sfem_ctx* ctx, int64_t* info);
 /* which = 0: E^{-1}; which = 1: B (mixed precision only); of the last rebuild, row-major
- * (cap >= unknowns^2 doubles). */
+ * (cap >= unknowns^2 doubles; block diagonal on a batch). */
 int sfem_coarse_inverse(sfem_ctx* ctx, int which, double* out, int cap);
 /* The Newton loop's coarse step: the increment (sfem_get_increment, applied by
  * sfem_update_solution) = Z E^{-1} Z^T rhs of the last assembly, in FP64, with the
