@@ -362,12 +362,15 @@ __global__ void __launch_bounds__(256) k_coarse_az_build(int64_t nblk, const int
   }
 }
 
-// v += A Z y: one thread per block row (rows a >= m carry 0, so padding slots stay 0).
+// The multiplicative form's tail in one pass over the block rows: out += A Z y (rows
+// a >= m carry 0, so padding slots stay 0) and z[slot] += y[box] at the row's pressure
+// slot (as k_coarse_prolong; each slot is one row's, so the order does not matter).
 template <typename T, typename VT>
-__global__ void __launch_bounds__(256) k_coarse_az_apply(int64_t nblk, const int32_t* __restrict__ azoff,
-                                                          const int32_t* __restrict__ azbox,
-                                                          const VT* __restrict__ az, const double* __restrict__ y,
-                                                          T* __restrict__ v) {
+__global__ void __launch_bounds__(256) k_coarse_post(int64_t nblk, const int32_t* __restrict__ cinfo,
+                                                      const int32_t* __restrict__ azoff,
+                                                      const int32_t* __restrict__ azbox, const VT* __restrict__ az,
+                                                      const double* __restrict__ y, T* __restrict__ z,
+                                                      T* __restrict__ v) {
   const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (p >= nblk) return;
   double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
@@ -384,6 +387,11 @@ __global__ void __launch_bounds__(256) k_coarse_az_apply(int64_t nblk, const int
   vp[1] = (T)((double)vp[1] + a1);
   vp[2] = (T)((double)vp[2] + a2);
   vp[3] = (T)((double)vp[3] + a3);
+  const int info = cinfo[p];
+  if (info >= 0) {
+    T* zs = z + 4 * p + (info & 3);
+    *zs = (T)((double)*zs + y[info >> 3]);
+  }
 }
 
 // Coarse solve, per workgroup record {mesh g, first row r0}: the mesh's box sums of the
@@ -792,29 +800,33 @@ int coarse_restrict(sfem_ctx* ctx, const T* r, const T* w) {
   return SFEM_OK;
 }
 
-// v += A Z y with the y of the last coarse_correct (the multiplicative form's update of the
-// product that followed the preconditioner).
+// The multiplicative form's coarse solve and tail: y = E^{-1} c (c from the last
+// restriction), then z += Z y and out += A Z y in one pass.
 template <typename T>
-int coarse_az(sfem_ctx* ctx, T* v) {
+int coarse_correct_az(sfem_ctx* ctx, T* z, T* out) {
   Coarse* C = ctx->la->coarse;
+  const double* inv = sizeof(T) == 4 && C->d_einvb && C->fp32 ? C->d_einvb : C->d_einv;
+  hipLaunchKernelGGL(k_coarse_solve, dim3(C->nswg), dim3(256), 0, ctx->stream, C->d_swg, C->d_goff, C->d_ebase,
+                     C->d_boxchunk, C->d_cpart, inv, C->d_y);
   const int64_t nb = ctx->la->ilu->nblk;
   const unsigned g = (unsigned)((nb + 255) / 256);
   if (C->az_fp32)
-    hipLaunchKernelGGL((k_coarse_az_apply<T, float>), dim3(g), dim3(256), 0, ctx->stream, nb, C->d_azoff, C->d_azbox,
-                       (const float*)C->d_az, C->d_y, v);
+    hipLaunchKernelGGL((k_coarse_post<T, float>), dim3(g), dim3(256), 0, ctx->stream, nb, C->d_cinfo, C->d_azoff,
+                       C->d_azbox, (const float*)C->d_az, C->d_y, z, out);
   else
-    hipLaunchKernelGGL((k_coarse_az_apply<T, double>), dim3(g), dim3(256), 0, ctx->stream, nb, C->d_azoff,
-                       C->d_azbox, (const double*)C->d_az, C->d_y, v);
+    hipLaunchKernelGGL((k_coarse_post<T, double>), dim3(g), dim3(256), 0, ctx->stream, nb, C->d_cinfo, C->d_azoff,
+                       C->d_azbox, (const double*)C->d_az, C->d_y, z, out);
   return SFEM_OK;
 }
+
+template int coarse_correct_az<float>(sfem_ctx*, float*, float*);
+template int coarse_correct_az<double>(sfem_ctx*, double*, double*);
 
 template <typename T>
 T* coarse_tmp(sfem_ctx* ctx) {
   return (T*)ctx->la->coarse->d_tmp;
 }
 
-template int coarse_az<float>(sfem_ctx*, float*);
-template int coarse_az<double>(sfem_ctx*, double*);
 template float* coarse_tmp<float>(sfem_ctx*);
 template double* coarse_tmp<double>(sfem_ctx*);
 

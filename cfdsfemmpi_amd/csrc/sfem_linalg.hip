@@ -3119,8 +3119,7 @@ static int pc_op(sfem_ctx* ctx, const T* r, T* z, T* out, const RowSubset* sub =
   spmv_in<T>(ctx, z, out, true, sub);
   ctx->kt.begin(SFEM_KT_PC, st);
   coarse_restrict<T>(ctx, r, out);
-  coarse_correct<T>(ctx, z);
-  coarse_az<T>(ctx, out);
+  coarse_correct_az<T>(ctx, z, out);
   ctx->kt.end(SFEM_KT_PC, st, false);
   return hipGetLastError() == hipSuccess ? SFEM_OK : set_error(SFEM_ERR_HIP, "preconditioner launch");
 }

@@ -218,7 +218,7 @@ bool coarse_az_targets(const LinAlg* L, const int32_t** cinfo, const int32_t** a
 void coarse_set_az_precision(LinAlg* L, bool fp32);
 bool coarse_active(const LinAlg* L);
 template <typename T> int coarse_restrict(sfem_ctx* ctx, const T* r, const T* w = nullptr);
-template <typename T> int coarse_az(sfem_ctx* ctx, T* v);
+template <typename T> int coarse_correct_az(sfem_ctx* ctx, T* z, T* out);
 template <typename T> T* coarse_tmp(sfem_ctx* ctx);
 template <typename T> int coarse_correct(sfem_ctx* ctx, T* z);
 int coarse_info(sfem_ctx* ctx, int64_t* out);
