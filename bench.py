@@ -437,12 +437,14 @@ def main_c4(a):
 C5_METRIC = "C5: Newton/Krylov iterations and time-steps/s, learned vs extrapolated initial iterate"
 
 
-def c5_context(mesh, mu, opts):
+def c5_context(mesh, mu, opts, coarse=0):
     from cfdsfemmpi_amd import deck, sfem
     ctl = deck.Control(np.array([1.0, mu, 0, 0, 0, 0, 0, 0, 1.0, 1.0, 0.0]), 2, 0.0, 0.1, 1e9, 100000, 1, 1e-8)
     fem = sfem.StabFEM(device=0)
     fem.setMesh(mesh)
     fem.setControl(ctl)
+    fem.coarse_boxes = coarse  # no prescribed pressure: the correction is E's pseudo-inverse
+    fem.coarse_mult = True
     fem.prepareInputData()
     fem.setSolver(opts)
     fem.ctx.set_predictor(sfem.PRED_EXTRAPOLATE)
@@ -474,6 +476,51 @@ def c5_run(ctx, opts, steps, newton_tol, record=None, record_from=0, label="", j
             "candidates": cand}
 
 
+def c5_train_batch(meshes, res, a):
+    """Every training trajectory in one batched context (the meshes side by side, one
+    viscosity each, sfem_step_batch per time step; FP64 BiCGStab + ILU(0) at the relative
+    tolerance --c5-train-rtol, Newton to the absolute tolerance per mesh), the batch's
+    converged states recorded from step --c5-record-from on.  The learner's samples are
+    (window, node) pairs, so one store over the union is the trajectories' stores side by
+    side.  Returns a learner trained on nothing yet (the union's store) and the rate."""
+    from cfdsfemmpi_amd import meshgen, sfem
+    mesh, em = meshgen.disjoint_union(meshes)
+    mu = 1.0 / np.asarray(res, np.float64)
+    opts = sfem.solver_opts("bcgs", "ilu", a.c5_train_rtol, 20000, reorder=a.reorder)
+    ctx = sfem.Context(mesh, device=0)
+    ctx.set_dirichlet(mesh.dbc_node, mesh.dbc_dof, mesh.dbc_value)
+    ctx.prepare_elements()
+    ctx.build_pattern()
+    ctx.set_params(np.array([1.0, float(np.mean(mu)), 0, 0, 0, 0, 0, 0, 1.0, 1.0, 0.0]), 2, 0.0, 0.1)
+    ctx.set_batch(em, mu)
+    if a.c5_coarse:
+        ctx.set_coarse_correction(a.c5_coarse, True)
+    ctx.setup_solver(opts)
+    ctx.batch_use_graph(True)
+    nrec = a.c5_train_steps - a.c5_record_from + 1
+    learner = sfem.Learner(ctx, max(nrec, 4), seed=a.c5_seed)
+    log(f"training data: {len(meshes)} meshes in one batch ({mesh.nNode} nodes), store of {nrec} states "
+        f"({nrec * mesh.nNode * 12 / 2**30:.1f} GiB)")
+    t, kry, nwt = 0.1, 0, 0
+    t0 = t1 = time.perf_counter()
+    for k in range(a.c5_train_steps):
+        ni, ki, rn = ctx.step_batch(min(t, 1.0), opts, a.newton_tol, 10)
+        kry += int(ki.max())
+        nwt += int(ni.max())
+        if k + 1 >= a.c5_record_from:
+            learner.record(ctx, new_trajectory=(k + 1 == a.c5_record_from))
+        if (k + 1) % 50 == 0 or k + 1 == a.c5_train_steps:
+            t2 = time.perf_counter()
+            n = (k + 1) % 50 or 50
+            log(f"training data: steps {k + 2 - n}-{k + 1} of {a.c5_train_steps}: {n / (t2 - t1):.2f} batch-steps/s, "
+                f"Newton {nwt / n:.2f} and Krylov {kry / n:.1f} per step (largest of the meshes), max |F| {rn.max():.2e}")
+            t1, kry, nwt = t2, 0, 0
+        t += 0.1
+    rate = a.c5_train_steps / (time.perf_counter() - t0)
+    ctx.close()
+    return learner, rate
+
+
 def main_c5(a):
     """Learned initial iterate on the cylinder-in-channel Quad4 mesh: trajectories at
     several Reynolds numbers with seeded 1% inlet perturbations train the per-node MLP,
@@ -481,7 +528,11 @@ def main_c5(a):
     start and from the learned start (judged per step by the iteration-0 residual)."""
     from cfdsfemmpi_amd import meshgen, sfem
     steps = a.steps if a.steps is not None else 100
-    res = [float(r) for r in a.c5_train_re.split(",")]
+    if ":" in a.c5_train_re:  # lo:hi:count, evenly spaced
+        lo, hi, cnt = a.c5_train_re.split(":")
+        res = [float(v) for v in np.linspace(float(lo), float(hi), int(cnt))]
+    else:
+        res = [float(r) for r in a.c5_train_re.split(",")]
     opts = sfem.solver_opts("bcgs", "ilu", 1e-12, 20000, reorder=a.reorder, atol=0.5 * a.newton_tol)
     meshes = [meshgen.cylinder_channel(perturb=0.01, seed=1000 + k) for k in range(len(res))]
     m0 = meshes[0]
@@ -490,39 +541,65 @@ def main_c5(a):
     learner = None
     train_rates = []
     t_gen = time.perf_counter()
-    for k, (mesh, re) in enumerate(zip(meshes, res)):
-        fem = c5_context(mesh, 1.0 / re, opts)
-        if learner is None:
-            learner = sfem.Learner(fem.ctx, len(res) * a.c5_train_steps, seed=a.c5_seed)
-        r = c5_run(fem.ctx, opts, a.c5_train_steps, a.newton_tol, record=learner, record_from=a.c5_record_from,
-                   label=f"train Re {re:g}")
-        train_rates.append(r["steps_per_s"])
-        fem.ctx.close()
+    if a.c5_train_batch:
+        learner, gen = c5_train_batch(meshes, res, a)
+        train_rates.append(gen)
+    else:
+        for k, (mesh, re) in enumerate(zip(meshes, res)):
+            fem = c5_context(mesh, 1.0 / re, opts, a.c5_coarse)
+            if learner is None:
+                learner = sfem.Learner(fem.ctx, len(res) * a.c5_train_steps, seed=a.c5_seed)
+            r = c5_run(fem.ctx, opts, a.c5_train_steps, a.newton_tol, record=learner, record_from=a.c5_record_from,
+                       label=f"train Re {re:g}")
+            train_rates.append(r["steps_per_s"])
+            fem.ctx.close()
     t_gen = time.perf_counter() - t_gen
-    t_fit = time.perf_counter()
-    # four equal phases, the learning rate halved after each (Adam's moments carry over)
+    # one network per component setting (--c5-components), each from the same initial
+    # weights and trained on the same store; four equal phases, the learning rate halved
+    # after each (Adam's moments carry over)
+    comps = list(dict.fromkeys(int(c) for c in a.c5_components.split(",")))
+    w_init = learner.get_weights()
     chunk = max(1, a.c5_train_iters // 4)
-    loss = np.concatenate([learner.train(chunk, a.c5_batch, a.c5_lr * 0.5 ** k) for k in range(4)])
-    t_fit = time.perf_counter() - t_fit
-    log(f"training: {len(loss)} Adam steps x {a.c5_batch} samples in {t_fit:.2f} s, loss "
-        f"{loss[:50].mean():.4e} -> {loss[-50:].mean():.4e}")
+    weights, losses, t_fit = {}, {}, 0.0
+    for j, nc in enumerate(comps):
+        if j:
+            learner.set_weights(w_init)
+        learner.set_components(nc)  # a change of components takes the output scales again
+        t0 = time.perf_counter()
+        loss = np.concatenate([learner.train(chunk, a.c5_batch, a.c5_lr * 0.5 ** k) for k in range(4)])
+        t_fit += time.perf_counter() - t0
+        log(f"training ({nc} components): {len(loss)} Adam steps x {a.c5_batch} samples in "
+            f"{time.perf_counter() - t0:.2f} s, loss {loss[:50].mean():.4e} -> {loss[-50:].mean():.4e}")
+        weights[nc], losses[nc] = learner.get_weights(), loss
+    # the trained networks move to a learner of the evaluation mesh (weights and output
+    # scales); the training store is released
+    windows = learner.info()[1]
+    learner.close()
     evalmesh = meshgen.cylinder_channel(perturb=0.01, seed=99)
     runs = {}
-    for kind in ("extrapolated", "quadratic", "learned"):
-        fem = c5_context(evalmesh, 0.01, opts)
+    for kind in ["extrapolated", "quadratic"] + [f"learned{nc}" for nc in comps]:
+        fem = c5_context(evalmesh, 0.01, opts, a.c5_coarse)
         if kind == "quadratic":
             fem.ctx.set_predictor(sfem.PRED_QUADRATIC)
-        if kind == "learned":
-            fem.ctx.set_learner(learner)
+        ev = None
+        if kind.startswith("learned"):
+            nc = int(kind[7:])
+            ev = sfem.Learner(fem.ctx, 4, seed=a.c5_seed)
+            ev.set_components(nc)
+            ev.set_weights(weights[nc])
+            fem.ctx.set_learner(ev)
             fem.ctx.set_predictor(sfem.PRED_LEARNED)
-        r = c5_run(fem.ctx, opts, steps, a.newton_tol, label=f"eval Re 100 {kind}", judged=(kind == "learned"))
+        r = c5_run(fem.ctx, opts, steps, a.newton_tol, label=f"eval Re 100 {kind}", judged=ev is not None)
         r["soln"] = fem.ctx.get_solution()
-        if kind == "learned":
+        if ev is not None:
             r["starts"] = [int(v) for v in fem.ctx.predictor_stats()[0]]
             log(f"starts taken (linear, quadratic, learned): {r['starts']}")
         fem.ctx.close()
+        if ev is not None:
+            ev.close()
         runs[kind] = r
-    ex, qu, le = runs["extrapolated"], runs["quadratic"], runs["learned"]
+    ex, qu, le = runs["extrapolated"], runs["quadratic"], runs[f"learned{comps[0]}"]
+    loss = losses[comps[0]]
     diff = float(np.max(np.abs(le["soln"] - ex["soln"])) / np.max(np.abs(ex["soln"])))
     skip = min(a.c5_record_from, steps // 2)  # after the inflow ramp and its transient
     out = {
@@ -542,12 +619,17 @@ def main_c5(a):
                                f"structured blocks, {m0.nElem} Quad4, inflow u=1 (+1% seeded perturbation), "
                                f"slip walls, traction-free outflow, Re=100, dt=0.1, {steps} steps from rest",
                    "training": {"trajectories": len(res), "re": res, "steps_each": a.c5_train_steps,
-                                "recorded_from_step": a.c5_record_from, "windows": learner.info()[1],
+                                "recorded_from_step": a.c5_record_from, "windows": windows,
                                 "adam_steps": len(loss), "batch": a.c5_batch,
                                 "lr": [a.c5_lr * 0.5 ** k for k in range(4)],
                                 "seconds_generating": t_gen, "seconds_fitting": t_fit,
+                                "generation": (f"one batched context, BiCGStab rtol {a.c5_train_rtol:g}, "
+                                               f"{train_rates[0]:.2f} batch-steps/s") if a.c5_train_batch
+                                else "one context per trajectory, the evaluation solver",
                                 "loss_first50": float(loss[:50].mean()), "loss_last50": float(loss[-50:].mean())},
-                   "solver": f"FP64 BiCGStab + ILU(0) (ordering {a.reorder}), stop at ||r|| <= max(1e-12 ||b||, "
+                   "solver": f"FP64 BiCGStab + ILU(0) (ordering {a.reorder})"
+                             + (f" + multiplicative coarse pressure correction ({a.c5_coarse}^2 boxes)" if a.c5_coarse else "")
+                             + f", stop at ||r|| <= max(1e-12 ||b||, "
                              f"{0.5 * a.newton_tol:g}); Newton tol {a.newton_tol:g}",
                    "parallelism": "single GPU"},
         "extrapolated": {"steps_per_s": ex["steps_per_s"], "newton_total": sum(ex["newton"]),
@@ -556,11 +638,20 @@ def main_c5(a):
         "quadratic": {"steps_per_s": qu["steps_per_s"], "newton_total": sum(qu["newton"]),
                       "krylov_total": sum(qu["krylov"]), "krylov_after_ramp": sum(qu["krylov"][skip:]),
                       "krylov_per_step": qu["krylov"], "r0_per_step": qu["r0"]},
-        "learned": {"steps_per_s": le["steps_per_s"], "newton_total": sum(le["newton"]),
+        "learned": {"components": comps[0], "steps_per_s": le["steps_per_s"], "newton_total": sum(le["newton"]),
                     "krylov_total": sum(le["krylov"]), "krylov_after_ramp": sum(le["krylov"][skip:]),
                     "krylov_per_step": le["krylov"], "r0_per_step": le["r0"],
                     "starts_linear_quadratic_learned": le["starts"],
                     "candidate_norms_per_step_linear_quadratic_learned": le["candidates"]},
+        "learned_other_components": {
+            str(nc): {"steps_per_s": runs[f"learned{nc}"]["steps_per_s"],
+                      "newton_total": sum(runs[f"learned{nc}"]["newton"]),
+                      "krylov_total": sum(runs[f"learned{nc}"]["krylov"]),
+                      "krylov_after_ramp": sum(runs[f"learned{nc}"]["krylov"][skip:]),
+                      "krylov_per_step": runs[f"learned{nc}"]["krylov"],
+                      "starts_linear_quadratic_learned": runs[f"learned{nc}"]["starts"],
+                      "loss_first50": float(losses[nc][:50].mean()), "loss_last50": float(losses[nc][-50:].mean())}
+            for nc in comps[1:]},
         "krylov_ratio_learned_vs_extrapolated": sum(le["krylov"]) / max(1, sum(ex["krylov"])),
         "krylov_ratio_learned_vs_quadratic": sum(le["krylov"]) / max(1, sum(qu["krylov"])),
         "steps_per_s_ratio_learned_vs_extrapolated": le["steps_per_s"] / ex["steps_per_s"],
@@ -600,6 +691,14 @@ def main():
     ap.add_argument("--c5-batch", type=int, default=8192, help="c5: samples per Adam step")
     ap.add_argument("--c5-lr", type=float, default=1e-3)
     ap.add_argument("--c5-seed", type=int, default=0)
+    ap.add_argument("--c5-train-batch", type=int, default=1,
+                    help="c5: generate the training trajectories in one batched context (0: one after another)")
+    ap.add_argument("--c5-train-rtol", type=float, default=1e-6,
+                    help="c5: BiCGStab relative tolerance of the batched training runs")
+    ap.add_argument("--c5-components", default="2",
+                    help="c5: components the network corrects, 2 (u, v) or 3 (u, v, p); a list trains one each")
+    ap.add_argument("--c5-coarse", type=int, default=0,
+                    help="c5: boxes per direction of the multiplicative coarse pressure correction (0: off)")
     ap.add_argument("--c4-count", type=int, default=256)
     ap.add_argument("--c4-seed", type=int, default=0)
     ap.add_argument("--c4-seq-stride", type=int, default=16,

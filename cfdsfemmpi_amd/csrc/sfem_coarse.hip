@@ -37,6 +37,12 @@
 //     mesh for a batch; FP32 Krylov solves use the inverse without the global constant
 //     pressure of each mesh, whose correction their vectors cannot hold beside the rest
 //     (the FP64 Newton loop takes it as a coarse step).
+//   * A mesh with no prescribed pressure (a traction-free outflow fixes the pressure level
+//     through the momentum rows instead): every row of its E sums to zero, the constant is
+//     its null vector, and the correction is E's pseudo-inverse, formed as the inverse of
+//     E + (m / n) 1 1^T (m the mean diagonal, n the mesh's unknowns) without the constant,
+//     B = F - (F 1)(1^T F) / (1^T F 1) with F that inverse; the constant pressure is left
+//     to the Krylov iteration.
 //   * Apply: restriction (chunk sums of r at the pressure slots, before the ILU(0)
 //     sweeps, so z may alias r), the coarse solve (box sums, then y = E^{-1} c), and the
 //     prolongation z[slot] += y[box] after the sweeps; the multiplicative form restricts
@@ -101,6 +107,8 @@ struct Coarse {
   double* d_cpart = nullptr;      // nchunk restricted chunk sums
   double* d_y = nullptr;          // nc coarse solution
   int32_t* d_flag = nullptr;      // 1: E was singular (correction switched off)
+  int32_t* d_gfloat = nullptr;    // per mesh, 1: no prescribed pressure (E 1 = 0)
+  bool any_float = false;
   // A Z for the multiplicative form: per block row the coarse unknowns of its neighbours'
   // pressure DOFs (d_azoff[p] .. d_azoff[p+1] in d_azbox, ascending) and, per entry, the 4
   // rows' sums of A over that unknown's pressure columns (FP64 or FP32, like the tangent)
@@ -113,7 +121,7 @@ struct Coarse {
   ~Coarse() {
     void* ptrs[] = {d_slot, d_chunk, d_chunkbox, d_boxchunk, d_cinfo, d_cpos, d_boxnb,
                     d_epart, d_gj, d_einv, d_einvb, d_cpart, d_y, d_flag, d_azoff, d_azbox, d_az, d_tmp,
-                    d_cgrp, d_goff, d_gjbase, d_ebase, d_swg};
+                    d_gfloat, d_cgrp, d_goff, d_gjbase, d_ebase, d_swg};
     for (void* q : ptrs)
       if (q) (void)hipFree(q);
   }
@@ -199,6 +207,30 @@ __global__ void __launch_bounds__(64) k_coarse_fill(int nc, const int32_t* __res
   row[lc] = own;  // E(c, c) = those row sums - the couplings to the other boxes
 }
 
+// Meshes without a prescribed pressure (gfloat[g] = 1), one workgroup each: E += (m / n)
+// 1 1^T, m the mean of E's diagonal summed in row order, so that the elimination sees a
+// definite matrix whose inverse, without the constant, is E's pseudo-inverse.
+__global__ void __launch_bounds__(256) k_coarse_float(const int32_t* __restrict__ goff,
+                                                      const int64_t* __restrict__ gjbase,
+                                                      const int32_t* __restrict__ gfloat, double* __restrict__ gj) {
+  __shared__ double s_shift;
+  const int g = blockIdx.x;
+  if (!gfloat[g]) return;
+  const int ng = goff[g + 1] - goff[g], w = 2 * ng;
+  double* a = gj + gjbase[g];
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int i = 0; i < ng; ++i) t += a[(int64_t)i * w + i];
+    s_shift = t / ((double)ng * (double)ng);
+  }
+  __syncthreads();
+  const double sh = s_shift;
+  for (int e = threadIdx.x; e < ng * ng; e += 256) {
+    const int i = e / ng, j = e - i * ng;
+    a[(int64_t)i * w + j] += sh;
+  }
+}
+
 // Gauss-Jordan step j without row scaling: row i -= (E(i, j) / E(j, j)) row j for every
 // i != j, one workgroup per row, over columns j+1 .. nc+j (row j is zero beyond them;
 // earlier columns are not read again).  Row j itself is left alone, so no workgroup
@@ -239,9 +271,12 @@ __global__ void __launch_bounds__(256) k_gj_group(const int32_t* __restrict__ go
 // Per mesh (one workgroup each): E^{-1} (row-major) = the right half scaled by the
 // diagonal the steps left; with border = 1 also B = E^{-1} - (E^{-1} 1)(1^T E^{-1}) /
 // (1^T E^{-1} 1), the leading block of [E b1; b1^T 0]^{-1}: the correction whose output
-// sums to zero.  A pivot below 1e-15 of the mesh's largest (or not finite) zeroes both
-// for that mesh and sets flag[g].  Sums in a fixed order.
-__global__ void __launch_bounds__(256) k_coarse_finish(int border, const int32_t* __restrict__ goff,
+// sums to zero.  A mesh without a prescribed pressure (gfloat[g]) gets that form in both
+// (its E was shifted by a multiple of 1 1^T: the result is E's pseudo-inverse).  A pivot
+// below 1e-15 of the mesh's largest (or not finite) zeroes both for that mesh and sets
+// flag[g].  Sums in a fixed order.
+__global__ void __launch_bounds__(256) k_coarse_finish(int border, const int32_t* __restrict__ gfloat,
+                                                       const int32_t* __restrict__ goff,
                                                        const int64_t* __restrict__ gjbase,
                                                        const int64_t* __restrict__ ebase, const double* __restrict__ gjall,
                                                        double* __restrict__ einvall, double* __restrict__ einvball,
@@ -255,6 +290,7 @@ __global__ void __launch_bounds__(256) k_coarse_finish(int border, const int32_t
   const double* gj = gjall + gjbase[g];
   double* einv = einvall + ebase[g];
   double* einvb = border ? einvball + ebase[g] : nullptr;
+  const bool fl = gfloat[g] != 0;
   const int tid = threadIdx.x;
   const int w = 2 * nc;
   double dmax = 0.0;
@@ -279,7 +315,7 @@ __global__ void __launch_bounds__(256) k_coarse_finish(int border, const int32_t
     if (!isfinite(x)) s_bad = 1;
   }
   __syncthreads();
-  if (border) {
+  if (border || fl) {
     for (int i = tid; i < nc; i += 256) {
       double a = 0.0, b = 0.0;
       for (int c = 0; c < nc; ++c) {
@@ -300,7 +336,9 @@ __global__ void __launch_bounds__(256) k_coarse_finish(int border, const int32_t
     const double sinv = 1.0 / s_sum;
     for (int e = tid; e < nc * nc; e += 256) {
       const int i = e / nc, c = e - i * nc;
-      einvb[e] = einv[e] - u[i] * v[c] * sinv;
+      const double b = einv[e] - u[i] * v[c] * sinv;
+      if (border) einvb[e] = b;
+      if (fl) einv[e] = b;
     }
     __syncthreads();
   }
@@ -530,6 +568,9 @@ static int coarse_plan(sfem_ctx* ctx, LinAlg* L, Coarse* C) {
   // nodes sharing an element with a node whose pressure is prescribed: their pressure
   // rows do not sum to zero
   std::vector<uint8_t> nearfix(ctx->nNode, 0);
+  std::vector<int32_t> gfloat(ngroup, 1);
+  for (int64_t n = 0; n < ctx->nNode; ++n)
+    if (grp[n] >= 0 && ctx->nodeDof[n * ndof + ndof - 1] < 0) gfloat[grp[n]] = 0;
   for (int64_t e = 0; e < ctx->nElem; ++e) {
     const int32_t* en = &ctx->conn[e * npe];
     bool fix = false;
@@ -658,8 +699,9 @@ static int coarse_plan(sfem_ctx* ctx, LinAlg* L, Coarse* C) {
   if ((rc = upload(&C->d_azoff, azoff)) || (rc = upload(&C->d_azbox, azbox))) return rc;
   C->naz = (int64_t)azbox.size();
   if ((rc = upload(&C->d_cgrp, cgrp)) || (rc = upload(&C->d_goff, goff)) || (rc = upload(&C->d_gjbase, gjbase)) ||
-      (rc = upload(&C->d_ebase, ebase)) || (rc = upload(&C->d_swg, swg)))
+      (rc = upload(&C->d_ebase, ebase)) || (rc = upload(&C->d_swg, swg)) || (rc = upload(&C->d_gfloat, gfloat)))
     return rc;
+  C->any_float = std::find(gfloat.begin(), gfloat.end(), 1) != gfloat.end();
   C->ngroup = ngroup;
   C->h_goff = goff;
   C->h_ebase = ebase;
@@ -734,6 +776,8 @@ int coarse_build(sfem_ctx* ctx, const double* kp, const float* kp32) {
   const int nc = C->nc;
   hipLaunchKernelGGL(k_coarse_fill, dim3((nc + 63) / 64), dim3(64), 0, st, nc, C->d_cgrp, C->d_goff, C->d_gjbase,
                      C->d_boxchunk, C->d_boxnb, C->d_epart, C->d_gj);
+  if (C->any_float)
+    hipLaunchKernelGGL(k_coarse_float, dim3(C->ngroup), dim3(256), 0, st, C->d_goff, C->d_gjbase, C->d_gfloat, C->d_gj);
   if (C->ngroup == 1)  // one large block: one launch per step, one workgroup per row
     for (int j = 0; j < nc; ++j) hipLaunchKernelGGL(k_gj_step, dim3(nc), dim3(256), 0, st, nc, j, C->d_gj);
   else                 // a batch's small blocks: one workgroup per mesh
@@ -744,8 +788,8 @@ int coarse_build(sfem_ctx* ctx, const double* kp, const float* kp32) {
   const size_t etot = (size_t)(C->h_ebase.back() + (int64_t)(C->h_goff[C->ngroup] - C->h_goff[C->ngroup - 1]) *
                                                        (C->h_goff[C->ngroup] - C->h_goff[C->ngroup - 1]));
   if (border && !C->d_einvb) CO_CHECK(hipMalloc((void**)&C->d_einvb, std::max<size_t>(etot, 1) * sizeof(double)));
-  hipLaunchKernelGGL(k_coarse_finish, dim3(C->ngroup), dim3(256), 0, st, border, C->d_goff, C->d_gjbase, C->d_ebase,
-                     C->d_gj, C->d_einv, C->d_einvb, C->d_flag);
+  hipLaunchKernelGGL(k_coarse_finish, dim3(C->ngroup), dim3(256), 0, st, border, C->d_gfloat, C->d_goff, C->d_gjbase,
+                     C->d_ebase, C->d_gj, C->d_einv, C->d_einvb, C->d_flag);
   C->fp32 = border;
   CO_CHECK(hipGetLastError());
   C->ready = true;

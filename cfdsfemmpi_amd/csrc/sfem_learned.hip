@@ -35,11 +35,12 @@
 namespace sfem {
 
 constexpr int kIn = 12, kHid = 64, kOut = 3;
-// Components the network corrects: u and v.  The pressure keeps the quadratic
-// extrapolation: its corrections, a non-local (elliptic) quantity predicted from a
-// 1-ring stencil, gave starts with a smaller residual norm but slowly converging
-// pressure errors (more Krylov iterations than the plain quadratic start).
-constexpr int kCorrected = 2;
+// Components the network corrects (sfem_learner_set_components): by default u and v, the
+// pressure keeping the quadratic extrapolation.  Pressure corrections, a non-local
+// (elliptic) quantity predicted from a 1-ring stencil, gave starts with a smaller residual
+// norm but pressure errors that ILU(0) alone converges slowly; with all three the
+// network predicts the whole (u, v, p) increment.
+constexpr int kCorrectedDefault = 2;
 constexpr int P_W1 = 0, P_B1 = P_W1 + kHid * kIn, P_W2 = P_B1 + kHid, P_B2 = P_W2 + kHid * kHid,
               P_W3 = P_B2 + kHid, P_B3 = P_W3 + kOut * kHid, P_TAU = P_B3 + kOut, kNP = P_TAU + kOut;
 static_assert(kNP == SFEM_LEARNER_NPARAM, "parameter count");
@@ -162,7 +163,8 @@ __device__ inline void load_sample(int64_t k, int64_t nNode, int64_t nstate, con
                                    int64_t nwin, const int32_t* __restrict__ adjptr,
                                    const int32_t* __restrict__ adj, const uint8_t* __restrict__ mask,
                                    uint64_t seed, int64_t iter, const int32_t* __restrict__ ex_win,
-                                   const int32_t* __restrict__ ex_node, float x[kIn], float t[3], float mk[3]) {
+                                   const int32_t* __restrict__ ex_node, int ncorr, float x[kIn], float t[3],
+                                   float mk[3]) {
   int64_t win, node;
   if (ex_win) {
     win = ex_win[k];
@@ -185,7 +187,7 @@ __device__ inline void load_sample(int64_t k, int64_t nNode, int64_t nstate, con
   for (int c = 0; c < 3; ++c) {
     const double d0 = (double)s0[nstate + node * 3 + c] - (double)s0[node * 3 + c];
     t[c] = (float)((d0 - 2.0 * d1[c] + d2[c]) * isig[c]) / lam;
-    mk[c] = (c < kCorrected && mask[node * 3 + c]) ? 1.0f : 0.0f;
+    mk[c] = (c < ncorr && mask[node * 3 + c]) ? 1.0f : 0.0f;
   }
 }
 
@@ -197,14 +199,14 @@ __global__ __launch_bounds__(256) void k_target_sq(int64_t n, int64_t nNode, int
                                                    const int32_t* __restrict__ adjptr,
                                                    const int32_t* __restrict__ adj,
                                                    const uint8_t* __restrict__ mask, uint64_t seed, int64_t iter,
-                                                   double* __restrict__ acc) {
+                                                   int ncorr, double* __restrict__ acc) {
   __shared__ double red[6][256];
   double a[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (k < n) {
     float x[kIn], t[3], mk[3];
-    load_sample(k, nNode, nstate, snap, rms, windows, nwin, adjptr, adj, mask, seed, iter, nullptr, nullptr, x, t,
-                mk);
+    load_sample(k, nNode, nstate, snap, rms, windows, nwin, adjptr, adj, mask, seed, iter, nullptr, nullptr, ncorr, x,
+                t, mk);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       a[c] = (double)mk[c] * (double)t[c] * (double)t[c];
@@ -234,7 +236,7 @@ __global__ __launch_bounds__(kGradTB) void k_mlp_grad(
     const float* __restrict__ rms, const int32_t* __restrict__ windows, int64_t nwin,
     const int32_t* __restrict__ adjptr, const int32_t* __restrict__ adj, const uint8_t* __restrict__ mask,
     uint64_t seed, int64_t iter, int batch, const int32_t* __restrict__ ex_win,
-    const int32_t* __restrict__ ex_node, float inv_count, float* __restrict__ part,
+    const int32_t* __restrict__ ex_node, int ncorr, float inv_count, float* __restrict__ part,
     float* __restrict__ part_loss) {
   __shared__ float A[kGradTB][kLd];   // Z1, then H1 = silu(Z1)
   __shared__ float B[kGradTB][kLd];   // Z2, then H2, then G1 = dL/dZ1
@@ -249,8 +251,8 @@ __global__ __launch_bounds__(kGradTB) void k_mlp_grad(
 #pragma unroll
   for (int c = 0; c < 3; ++c) { t[c] = 0.0f; mk[c] = 0.0f; }
   if (k < batch) {
-    load_sample(k, nNode, nstate, snap, rms, windows, nwin, adjptr, adj, mask, seed, iter, ex_win, ex_node, x, t,
-                mk);
+    load_sample(k, nNode, nstate, snap, rms, windows, nwin, adjptr, adj, mask, seed, iter, ex_win, ex_node, ncorr, x,
+                t, mk);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const float tau = w[P_TAU + c];
@@ -434,7 +436,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer(int64_t nNode, const S* __res
                                                    const int32_t* __restrict__ adjptr,
                                                    const int32_t* __restrict__ adj, const double* __restrict__ sqacc,
                                                    const float* __restrict__ w, const v4s* __restrict__ pk,
-                                                   double* __restrict__ out) {
+                                                   int ncorr, double* __restrict__ out) {
   const int l = threadIdx.x & 63, g = l >> 4;
   const int64_t node = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16 + (l & 15);
   const bool valid = node < nNode;
@@ -487,7 +489,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer(int64_t nNode, const S* __res
   if (valid && g == 0) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const double y = c < kCorrected ? (double)(acc[c] + w[P_B3 + c]) * (double)w[P_TAU + c] * (double)lam : 0.0;
+      const double y = c < ncorr ? (double)(acc[c] + w[P_B3 + c]) * (double)w[P_TAU + c] * (double)lam : 0.0;
       out[node * 3 + c] = (double)s0[node * 3 + c] + 2.0 * d1[c] - d2[c] + (double)sig[c] * y;
     }
   }
@@ -523,6 +525,7 @@ struct sfem_learner {
   int ex_cap = 0;
   uint64_t seed = 0;
   bool tau_set = false;  // output scale fixed (before the first Adam step, or by set_weights)
+  int ncorr = kCorrectedDefault;  // components corrected: 2 (u, v) or 3 (u, v, p)
 };
 
 namespace {
@@ -568,7 +571,7 @@ int grad_step(sfem_learner* L, int batch, const int32_t* exw, const int32_t* exn
   const float inv_count = 1.0f / (3.0f * (float)batch);
   hipLaunchKernelGGL(k_mlp_grad, dim3(nblk), dim3(kGradTB), 0, L->stream, L->d_w, L->nNode, L->nstate,
                      L->d_snap, L->d_rms, L->d_windows, (int64_t)L->windows.size(), L->d_adjptr, L->d_adj,
-                     L->d_mask, L->seed, L->adam_t, batch, exw, exn, inv_count, L->d_part, L->d_ploss);
+                     L->d_mask, L->seed, L->adam_t, batch, exw, exn, L->ncorr, inv_count, L->d_part, L->d_ploss);
   LHIP(hipGetLastError());
   float c1 = 1.f, c2 = 1.f;
   const float b1 = 0.9f, b2 = 0.999f;
@@ -601,7 +604,7 @@ int learned_candidate(sfem_ctx* c, const double* s0, const double* s1, const dou
   const unsigned gr = (unsigned)std::min<int64_t>((nn + 255) / 256, 1024);
   hipLaunchKernelGGL((k_sq_increment<double>), dim3(gr), dim3(256), 0, c->stream, nn, s0, s1, L->d_acc_inf);
   hipLaunchKernelGGL((k_mlp_infer<double>), dim3((unsigned)((nn + 63) / 64)), dim3(256), 0, c->stream, nn, s0, s1,
-                     s2, L->d_adjptr, L->d_adj, L->d_acc_inf, L->d_w, (const v4s*)L->d_pack, out);
+                     s2, L->d_adjptr, L->d_adj, L->d_acc_inf, L->d_w, (const v4s*)L->d_pack, L->ncorr, out);
   LHIP(hipGetLastError());
   return SFEM_OK;
 }
@@ -752,7 +755,7 @@ int sfem_learner_train(sfem_learner* L, int iters, int batch, double lr, double*
     LHIP(hipMemsetAsync(acc, 0, 6 * sizeof(double), L->stream));
     hipLaunchKernelGGL(k_target_sq, dim3((unsigned)(ns / 256)), dim3(256), 0, L->stream, ns, L->nNode, L->nstate,
                        L->d_snap, L->d_rms, L->d_windows, (int64_t)L->windows.size(), L->d_adjptr, L->d_adj,
-                       L->d_mask, L->seed, (int64_t)0xFFFFFFFF, acc);
+                       L->d_mask, L->seed, (int64_t)0xFFFFFFFF, L->ncorr, acc);
     double h[6] = {0, 0, 0, 0, 0, 0};
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(L->stream);
@@ -807,6 +810,17 @@ int sfem_learner_gradient(sfem_learner* L, int n, const int32_t* window, const i
   float l = 0.f;
   LHIP(hipMemcpy(&l, L->d_loss, sizeof(float), hipMemcpyDeviceToHost));
   if (loss) *loss = (double)l / (3.0 * n);
+  return SFEM_OK;
+}
+
+int sfem_learner_set_components(sfem_learner* L, int ncomp) {
+  if (!L || (ncomp != 2 && ncomp != 3)) return set_error(SFEM_ERR_ARG, "learner: 2 (u, v) or 3 (u, v, p) components");
+  LHIP(hipSetDevice(L->device));
+  LHIP(hipStreamSynchronize(L->stream));
+  if (ncomp != L->ncorr) {
+    L->ncorr = ncomp;
+    L->tau_set = false;  // the output scales are taken again before the next Adam step
+  }
   return SFEM_OK;
 }
 

@@ -137,9 +137,18 @@ def c4_batch(count: int = 256, seed: int = 0, n_range=(16, 160), jitter: float =
     res = np.exp(rng.uniform(np.log(re_range[0]), np.log(re_range[1]), size=count))
     seeds = rng.integers(0, 2**31 - 1, size=count)
     meshes = [ldc2d_tria(int(n), jitter=jitter, seed=int(s), lid_u=lid_u) for n, s in zip(ns, seeds)]
+    mesh, em = disjoint_union(meshes)
+    return mesh, em, 1.0 / res, ns, meshes
+
+
+def disjoint_union(meshes):
+    """One mesh holding ``meshes`` side by side (nodes, elements and boundary conditions
+    mesh by mesh, node numbers offset), and the mesh of every element: the input of a
+    batched context (``Context.set_batch``)."""
     coords, conn, dn, dd, dv, em = [], [], [], [], [], []
     off = 0
     for k, m in enumerate(meshes):
+        assert m.ndim == meshes[0].ndim and m.npElem == meshes[0].npElem
         coords.append(m.coords)
         conn.append(m.conn + off)
         dn.append(m.dbc_node + off)
@@ -147,10 +156,11 @@ def c4_batch(count: int = 256, seed: int = 0, n_range=(16, 160), jitter: float =
         dv.append(m.dbc_value)
         em.append(np.full(m.nElem, k, np.int32))
         off += m.nNode
-    mesh = Mesh(2, 3, np.concatenate(coords), np.concatenate(conn).astype(np.int32),
-                np.concatenate(dn).astype(np.int32), np.concatenate(dd).astype(np.int32),
-                np.concatenate(dv), np.zeros(0, np.int32))
-    return mesh, np.concatenate(em), 1.0 / res, ns, meshes
+    m0 = meshes[0]
+    mesh = Mesh(m0.ndim, m0.npElem, np.concatenate(coords),
+                np.concatenate(conn).astype(np.int32), np.concatenate(dn).astype(np.int32),
+                np.concatenate(dd).astype(np.int32), np.concatenate(dv), np.zeros(0, np.int32))
+    return mesh, np.concatenate(em)
 
 
 def _stretch(h0: float, length: float, n: int) -> np.ndarray:

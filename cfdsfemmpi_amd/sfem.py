@@ -111,6 +111,7 @@ ABI = [
     ("sfem_learner_train", C.c_int, [_vp, C.c_int, C.c_int, C.c_double, _f64p]),
     ("sfem_learner_gradient", C.c_int, [_vp, C.c_int, _i32p, _i32p, C.POINTER(C.c_float), _f64p]),
     ("sfem_learner_weights", C.c_int, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    ("sfem_learner_set_components", C.c_int, [_vp, C.c_int]),
     ("sfem_set_learner", C.c_int, [_vp, _vp]),
     ("sfem_learned_candidate", C.c_int, [_vp, _f64p]),
     ("sfem_predictor_stats", C.c_int, [_vp, _i64p, _f64p]),
@@ -523,6 +524,10 @@ class Learner:
         w = np.ascontiguousarray(w, np.float32)
         assert w.size == LEARNER_NPARAM
         _check(self.lib.sfem_learner_weights(self.h, None, _p(w, C.c_float)), "sfem_learner_weights")
+
+    def set_components(self, n: int):
+        """Correct (and train) u, v (n = 2, the default) or u, v, p (n = 3)."""
+        _check(self.lib.sfem_learner_set_components(self.h, int(n)), "sfem_learner_set_components")
 
     def close(self):
         if getattr(self, "h", None):

@@ -141,7 +141,8 @@ int sfem_set_predictor(sfem_ctx* ctx, int predictor);
  * A per-node MLP 12 -> 64 -> 64 -> 3 (SiLU) over the last three states of a node and the
  * mean of its 1-ring node neighbours predicts the next increment's correction over the
  * quadratic extrapolation for u and v (the pressure keeps the quadratic extrapolation;
- * the third output is not used and not trained): start = s_n + 2 d1 - d2 + sigma * tau * lambda * MLP(x / lambda),
+ * the third output is not used and not trained), or for u, v and p
+ * (sfem_learner_set_components): start = s_n + 2 d1 - d2 + sigma * tau * lambda * MLP(x / lambda),
  * d1 = s_n - s_{n-1}, d2 = s_{n-1} - s_{n-2}, x the 12 sigma-scaled inputs, sigma the
  * per-component RMS of d1 over all nodes, lambda the node's activity scale (RMS of its six
  * sigma-scaled own increments + 1e-3), tau a fixed output scale set before the first Adam
@@ -165,8 +166,9 @@ int sfem_learner_record(sfem_learner* learner, sfem_ctx* ctx, int new_trajectory
 int sfem_learner_info(sfem_learner* learner, int64_t* snapshots, int64_t* windows, int64_t* adam_steps);
 /* iters Adam steps (lr, betas 0.9 / 0.999, eps 1e-8) on minibatches of `batch` (window,
  * node) samples; loss[iters] (may be NULL) receives each minibatch's loss, the mean over
- * samples and components of (y - t)^2 with t = (d0 - 2 d1 + d2) / (sigma lambda tau), d0 the recorded next
- * increment (prescribed components and the pressure count as zero).  Sample k of Adam step a is drawn by
+ * samples and components of (y - t)^2 with t = (d0 - 2 d1 + d2) / (sigma lambda tau), d0 the
+ * recorded next increment (prescribed components, and the pressure unless it is corrected,
+ * count as zero).  Sample k of Adam step a is drawn by
  * h = splitmix64(seed ^ splitmix64(a * 2^32 + k)): window (h >> 32) mod nwindows, node
  * (h mod 2^32) mod nNode. */
 int sfem_learner_train(sfem_learner* learner, int iters, int batch, double lr, double* loss);
@@ -177,6 +179,10 @@ int sfem_learner_gradient(sfem_learner* learner, int n, const int32_t* window, c
 /* Read (get) and/or replace (set) the FP32 master weights and tau; set also resets Adam
  * and fixes tau. */
 int sfem_learner_weights(sfem_learner* learner, float* get, const float* set);
+/* Components the network corrects and is trained on: 2 (u, v; the default, the pressure
+ * keeps the quadratic extrapolation) or 3 (u, v, p).  A change takes the output scales
+ * tau again before the next Adam step (sfem_learner_weights fixes them, as before). */
+int sfem_learner_set_components(sfem_learner* learner, int ncomp);
 /* Attach a learner (NULL detaches; the learner must outlive its use by ctx).  A learner
  * is not thread-safe: record, train and the steps of contexts it is attached to are
  * called from one host thread (each call returns with its device work finished). */

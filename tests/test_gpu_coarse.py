@@ -259,6 +259,96 @@ def test_coarse_cavity_same_answer_fewer_iterations(prec, mult):
         f.ctx.close()
 
 
+def _channel():
+    return meshgen.cylinder_channel(nr=8, nth=32, ratio=1.2, n_left=6, n_right=16, n_bottom=6, n_top=6,
+                                    perturb=0.01, seed=3)
+
+
+def _pinv_sym(E):
+    """Pseudo-inverse of the symmetric part of E without its eigenvalue nearest zero
+    (numpy's symmetric eigensolver)."""
+    w, V = np.linalg.eigh(0.5 * (E + E.T))
+    keep = np.argsort(np.abs(w))[1:]
+    return (V[:, keep] / w[keep]) @ V[:, keep].T
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_coarse_without_prescribed_pressure(prec):
+    """Channel past a cylinder with a traction-free outflow and no prescribed pressure
+    (Quad4): every row of the documented E sums to zero, so E is singular with the
+    constant as null vector, and the device's correction is its pseudo-inverse: both
+    inverses the library exposes within 1e-8 relative of numpy's (symmetric
+    eigendecomposition without the null eigenvalue).  FP64: the multiplicative
+    application's change M_c^-1 r - M^-1 r equals Z E^+ Z^T (r - A M^-1 r) within 1e-10
+    of ||M_c^-1 r||."""
+    mesh = _channel()
+    print(f"[case] coarse without prescribed pressure {prec}", flush=True)
+    ctx = _ctx(mesh, sfem.PREC_MIXED if prec == "mixed" else sfem.PREC_FP64)
+    ctx.time_update(1.0)
+    ctx.assemble(0)
+    opts = sfem.solver_opts("bcgs", "ilu", 1e-8 if prec == "fp64" else 0.5, 3000, reorder=3)
+    ctx.solve(opts)
+    r = np.random.default_rng(6).standard_normal(ctx.nfree)
+    z0 = ctx.pc_apply(r)
+    k = 6
+    ctx.set_coarse_correction(k, True)
+    ctx.solve(opts)
+    info = ctx.coarse_info()
+    assert info["built"] and not info["singular"]
+    A, _ = _system(ctx)
+    Z, E, Eid = _coarse_matrices(mesh, A, k)
+    nc = Z.shape[1]
+    assert info["unknowns"] == nc
+    dmax = np.max(np.abs(np.diag(Eid)))
+    assert np.max(np.abs(Eid.sum(axis=1))) <= 1e-12 * dmax
+    ref = _pinv_sym(Eid)
+    for which in ((0, 1) if prec == "mixed" else (0,)):
+        P = ctx.coarse_inverse(which)
+        e = np.max(np.abs(P - ref)) / np.max(np.abs(ref))
+        print(f"[coarse float] {prec} inverse {which}: nc={nc} rel={e:.2e}", flush=True)
+        assert e <= 1e-8
+    if prec == "fp64":
+        z1 = ctx.pc_apply(r)
+        corr = Z @ (ref @ (Z.T @ (r - A @ z0)))
+        d = np.linalg.norm((z1 - z0) - corr) / np.linalg.norm(z1)
+        print(f"[coarse float] apply diff {d:.2e}, |corr|/|z| {np.linalg.norm(corr) / np.linalg.norm(z1):.2e}",
+              flush=True)
+        assert np.linalg.norm(corr) > 1e-6 * np.linalg.norm(z1)
+        assert d <= 1e-10
+    ctx.close()
+
+
+def test_coarse_channel_same_answer_fewer_iterations():
+    """The channel at Re 100, 4 steps from rest (inflow ramped), FP64 BiCGStab + ILU(0)
+    stopped at ||r|| <= max(1e-12 ||b||, 5e-11), Newton to ||F|| < 1e-10, with and without
+    the multiplicative correction on 6^2 boxes: velocity within 1e-8 and pressure within
+    1e-6 (relative L2), and fewer Krylov iterations with it."""
+    mesh = _channel()
+    ctl = deck.Control(np.array([1.0, 0.01, 0, 0, 0, 0, 0, 0, 1, 1, 0]), 2, 0.0, 0.1, 1e9, 4, 1, 1e-6)
+    sols, its = [], []
+    for k in (0, 6):
+        print(f"[case] channel, coarse boxes {k}", flush=True)
+        fem = sfem.StabFEM()
+        fem.setMesh(mesh)
+        fem.setControl(ctl)
+        fem.newton_tol = 1e-10
+        fem.tuning = sfem.TUNE_FIXED
+        fem.coarse_boxes = k
+        fem.coarse_mult = True
+        fem.prepareInputData()
+        fem.setSolver(sfem.solver_opts("bcgs", "ilu", 1e-12, 5000, reorder=3, atol=5e-11))
+        sols.append(fem.solveFullyImplicit().reshape(-1, 3))
+        its.append(sum(h[3] for h in fem.history))
+        assert all(h[2] < 1e-10 for h in fem.history if h[3] == 0)
+        fem.ctx.close()
+    (a, b) = sols
+    eu = np.linalg.norm(b[:, :2] - a[:, :2]) / np.linalg.norm(a[:, :2])
+    ep = np.linalg.norm(b[:, 2] - a[:, 2]) / np.linalg.norm(a[:, 2])
+    print(f"[coarse channel] its {its[0]} -> {its[1]}; rel L2 u {eu:.2e} p {ep:.2e}", flush=True)
+    assert eu <= 1e-8 and ep <= 1e-6
+    assert its[1] < its[0]
+
+
 def test_coarse_rejections():
     """Too many boxes and boxes narrower than an element are refused."""
     mesh = meshgen.ldc3d_hex(4, graded=True)

@@ -128,13 +128,18 @@ def test_zero_network_gives_the_quadratic_extrapolation():
     assert np.allclose(cand, 3 * S0 - 3 * S1 + S2, rtol=0, atol=1e-14 * np.max(np.abs(S0)))
 
 
-def test_mfma_inference_matches_bf16_network():
+@pytest.mark.parametrize("ncomp", [2, 3])
+def test_mfma_inference_matches_bf16_network(ncomp):
     """The MFMA forward pass equals the network with BF16-rounded weights, inputs and
     hidden activations (FP32 biases and accumulation) to accumulation-order accuracy;
-    a transposed or misplaced operand tile would show O(1) differences."""
+    a transposed or misplaced operand tile would show O(1) differences.  With two
+    corrected components the pressure keeps the quadratic extrapolation."""
     mesh = _tiny_cylinder()
     fem, ctx, _ = _ctx(mesh)
     L = sfem.Learner(ctx, 8, seed=5)
+    with pytest.raises(sfem.SfemError):
+        L.set_components(1)
+    L.set_components(ncomp)
     w = _random_weights(11)
     L.set_weights(w)
     ctx.set_learner(L)
@@ -150,9 +155,9 @@ def test_mfma_inference_matches_bf16_network():
     h2 = _bf16(_silu(h1 @ _bf16(P["W2"]).T + P["b2"]))
     y = h2 @ _bf16(P["W3"]).T + P["b3"]
     quad = S0.reshape(-1, 3) + 2 * d1 - d2
-    # the pressure keeps the quadratic extrapolation
-    assert np.allclose(cand[:, 2], quad[:, 2], rtol=0, atol=1e-14 * np.max(np.abs(S0)))
-    y, y_gpu = y[:, :2], ((cand - quad) / (sig * P["tau"] * lam[:, None]))[:, :2]
+    if ncomp == 2:  # the pressure keeps the quadratic extrapolation
+        assert np.allclose(cand[:, 2], quad[:, 2], rtol=0, atol=1e-14 * np.max(np.abs(S0)))
+    y, y_gpu = y[:, :ncomp], ((cand - quad) / (sig * P["tau"] * lam[:, None]))[:, :ncomp]
     err = np.max(np.abs(y_gpu - y))
     assert np.max(np.abs(y)) > 0.5
     assert err <= 2e-2 * np.max(np.abs(y)), err
@@ -173,7 +178,7 @@ def _record_random_trajectory(ctx, L, mesh, nsnap, seed):
     return snaps
 
 
-def _torch_loss_grad(w, snaps, windows, nodes, nbrs, mask):
+def _torch_loss_grad(w, snaps, windows, nodes, nbrs, mask, ncomp=2):
     """Mean squared error of the FP32 training network, float64 autograd on the CPU."""
     P = {k: torch.tensor(v, dtype=torch.float64, requires_grad=True) for k, v in _split(w).items()}
     X, T, M = [], [], []
@@ -190,7 +195,7 @@ def _torch_loss_grad(w, snaps, windows, nodes, nbrs, mask):
         x, lam = _activity(x[None, :])
         X.append(x[0])
         T.append(((s3[i] - s0[i] - 2 * d1 + d2) * isig).astype(np.float32) / lam[0] / TAU)
-        M.append(mask[3 * i:3 * i + 3] * np.array([1.0, 1.0, 0.0]))  # pressure not trained
+        M.append(mask[3 * i:3 * i + 3] * np.array([1.0, 1.0, 1.0 if ncomp == 3 else 0.0]))
     X = torch.tensor(np.array(X, np.float64))
     T = torch.tensor(np.array(T, np.float64))
     M = torch.tensor(np.array(M, np.float64))
@@ -212,10 +217,14 @@ def _free_mask(mesh):
     return m
 
 
-def test_training_gradient_matches_autograd():
+@pytest.mark.parametrize("ncomp", [2, 3])
+def test_training_gradient_matches_autograd(ncomp):
+    """Loss and gradient of an explicit sample list against float64 autograd; the pressure
+    target counts only when the pressure is corrected."""
     mesh = _tiny_cylinder()
     fem, ctx, _ = _ctx(mesh)
     L = sfem.Learner(ctx, 16, seed=7)
+    L.set_components(ncomp)
     snaps = _record_random_trajectory(ctx, L, mesh, 7, seed=2)
     assert L.info()[:2] == (7, 4)
     w = _random_weights(13, 0.3)
@@ -226,7 +235,7 @@ def test_training_gradient_matches_autograd():
     nodes = rng.integers(0, mesh.nNode, size=n)
     nodes[:5] = np.asarray(mesh.dbc_node[:5])  # prescribed components are masked out
     g, loss = L.gradient(windows, nodes)
-    lref, gref = _torch_loss_grad(w, snaps, windows, nodes, _neighbours(mesh), _free_mask(mesh))
+    lref, gref = _torch_loss_grad(w, snaps, windows, nodes, _neighbours(mesh), _free_mask(mesh), ncomp)
     assert loss == pytest.approx(lref, rel=1e-4)
     assert np.max(np.abs(g - gref)) <= 1e-4 * np.max(np.abs(gref))
     assert np.all(g[SL["tau"][0]:] == 0)
@@ -318,13 +327,15 @@ def test_learned_start_never_changes_the_answer():
     assert cnt.sum() == nsteps
 
 
-def test_output_scale_from_hashed_samples():
+@pytest.mark.parametrize("ncomp", [2, 3])
+def test_output_scale_from_hashed_samples(ncomp):
     """Before the first Adam step tau_c is set to the RMS of the scaled targets over the
-    65,536 samples of minibatch tag 2^32 - 1 (free velocity components only)."""
+    65,536 samples of minibatch tag 2^32 - 1 (free components that are corrected)."""
     mesh = _tiny_cylinder()
     fem, ctx, _ = _ctx(mesh)
     seed = 99
     L = sfem.Learner(ctx, 16, seed=seed)
+    L.set_components(ncomp)
     snaps = _record_random_trajectory(ctx, L, mesh, 6, seed=5)
     L.train(1, 64, 1e-3)
     tau = L.get_weights()[SL["tau"][0]:].astype(np.float64)
@@ -346,7 +357,8 @@ def test_output_scale_from_hashed_samples():
         t, i = wins[(h >> 32) % len(wins)], (h & 0xFFFFFFFF) % mesh.nNode
         sq += mask[i] * per_win[t][i] ** 2
         cnt += mask[i]
-    ref = np.sqrt(sq[:2] / cnt[:2])
+    ref = np.sqrt(sq[:ncomp] / cnt[:ncomp])
     assert np.all(ref > 0)
-    assert np.allclose(tau[:2], ref, rtol=1e-5), (tau, ref)
-    assert tau[2] == 1.0  # the pressure is not corrected: no samples, unit scale
+    assert np.allclose(tau[:ncomp], ref, rtol=1e-5), (tau, ref)
+    if ncomp == 2:
+        assert tau[2] == 1.0  # the pressure is not corrected: no samples, unit scale
