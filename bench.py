@@ -527,7 +527,7 @@ def main_c5(a):
     then a run at Re = 100 with an unseen perturbation is timed from the extrapolated
     start and from the learned start (judged per step by the iteration-0 residual)."""
     from cfdsfemmpi_amd import meshgen, sfem
-    steps = a.steps if a.steps is not None else 100
+    steps = a.steps if a.steps is not None else 200
     if ":" in a.c5_train_re:  # lo:hi:count, evenly spaced
         lo, hi, cnt = a.c5_train_re.split(":")
         res = [float(v) for v in np.linspace(float(lo), float(hi), int(cnt))]
@@ -682,9 +682,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c3", choices=["c3", "c2", "c4", "c5"],
                     help="c3: 115^3 hex cavity (the headline); c2: 32^3; c4: batch of 256 Tria3 cavities")
-    ap.add_argument("--c5-train-re", default="60,80,110,140,170,200",
-                    help="c5: Reynolds numbers of the training runs")
-    ap.add_argument("--c5-train-steps", type=int, default=80, help="c5: steps of each training run")
+    ap.add_argument("--c5-train-re", default="60:200:12",
+                    help="c5: Reynolds numbers of the training runs (a list, or lo:hi:count evenly spaced)")
+    ap.add_argument("--c5-train-steps", type=int, default=200, help="c5: steps of each training run")
     ap.add_argument("--c5-record-from", type=int, default=13,
                     help="c5: first recorded step of a training run (the inflow ramp ends at step 10)")
     ap.add_argument("--c5-train-iters", type=int, default=8000, help="c5: Adam steps (four phases, lr halved)")
@@ -695,9 +695,9 @@ def main():
                     help="c5: generate the training trajectories in one batched context (0: one after another)")
     ap.add_argument("--c5-train-rtol", type=float, default=1e-6,
                     help="c5: BiCGStab relative tolerance of the batched training runs")
-    ap.add_argument("--c5-components", default="2",
+    ap.add_argument("--c5-components", default="3,2",
                     help="c5: components the network corrects, 2 (u, v) or 3 (u, v, p); a list trains one each")
-    ap.add_argument("--c5-coarse", type=int, default=0,
+    ap.add_argument("--c5-coarse", type=int, default=16,
                     help="c5: boxes per direction of the multiplicative coarse pressure correction (0: off)")
     ap.add_argument("--c4-count", type=int, default=256)
     ap.add_argument("--c4-seed", type=int, default=0)
