@@ -157,6 +157,26 @@ def test_cylinder_channel_mesh():
     assert np.array_equal(p1.dbc_value, meshgen.cylinder_channel(perturb=0.01, seed=3).dbc_value)
 
 
+def test_disjoint_union_of_meshes():
+    """Meshes side by side: node numbers of mesh k offset by the nodes before it, its
+    elements and boundary conditions carried over in order, one mesh id per element."""
+    ms = [meshgen.ldc2d_tria(n, jitter=0.2, seed=n) for n in (3, 5, 4)]
+    u, em = meshgen.disjoint_union(ms)
+    off = np.cumsum([0] + [m.nNode for m in ms])
+    assert u.nNode == off[-1] and u.nElem == sum(m.nElem for m in ms) and u.npElem == 3
+    e0 = 0
+    for k, m in enumerate(ms):
+        assert np.array_equal(u.coords[off[k]:off[k + 1]], m.coords)
+        assert np.array_equal(u.conn[e0:e0 + m.nElem], m.conn + off[k])
+        assert np.all(em[e0:e0 + m.nElem] == k)
+        e0 += m.nElem
+    assert np.array_equal(u.dbc_node, np.concatenate([m.dbc_node + off[k] for k, m in enumerate(ms)]))
+    assert np.array_equal(u.dbc_value, np.concatenate([m.dbc_value for m in ms]))
+    b, bem, _, ns, bms = meshgen.c4_batch(3, seed=2, n_range=(4, 6))
+    b2, bem2 = meshgen.disjoint_union(bms)
+    assert np.array_equal(b.conn, b2.conn) and np.array_equal(bem, bem2)
+
+
 def test_forces_file_format(tmp_path):
     """The forces file: one 'timeNow Fx Fy' line per step in fixed notation with 14
     decimals (StabFEM.cpp:257-259, StabFEM2.cpp:815)."""
