@@ -329,9 +329,11 @@ def main_c4(a):
     rpl = None
     if co and a.plain:
         # the reference's solver configuration: ILU(0) alone
-        rpl = run_batch(case, sfem.PREC_FP64, o64, a.newton_tol, steps, a.warmup, "fp64-batch-plain")
+        rpl = run_batch(case, sfem.PREC_FP64, o64, a.newton_tol, steps, a.warmup, "fp64-batch-plain",
+                        graph=bool(a.c4_graph))
         log(f"fp64-batch-plain: {rpl['steps_per_s']:.3f} steps/s")
-    r64 = run_batch(case, sfem.PREC_FP64, o64, a.newton_tol, steps, a.warmup, "fp64-batch", coarse=co)
+    r64 = run_batch(case, sfem.PREC_FP64, o64, a.newton_tol, steps, a.warmup, "fp64-batch", coarse=co,
+                    graph=bool(a.c4_graph))
     log(f"fp64-batch: {r64['steps_per_s']:.3f} steps/s")
     node_mesh = np.concatenate([np.full(m.nNode, k) for k, m in enumerate(meshes)])
 
@@ -347,13 +349,14 @@ def main_c4(a):
     if a.c4_store32:
         o32 = sfem.solver_opts("bcgs", "ilu", a.c4_store32_rtol, 5000, reorder=a.reorder, factor_bf16=a.bf16_factors)
         r32 = run_batch(case, sfem.PREC_STORE32, o32, a.c4_store32_newton, steps, a.warmup, "store32-batch",
-                        coarse=co)
+                        coarse=co, graph=bool(a.c4_graph))
         r32["err"] = mesh_errors(r32["soln"])
         log(f"store32-batch: {r32['steps_per_s']:.3f} steps/s, per-mesh rel L2 vs fp64: u max {r32['err'][0]:.3e} "
             f"p max {r32['err'][1]:.3e}")
     rmx = err_u = err_p = None
     if a.c4_mixed:
-        rmx = run_batch(case, sfem.PREC_MIXED, omx, a.newton_tol_mixed, steps, a.warmup, "mixed-batch", coarse=co)
+        rmx = run_batch(case, sfem.PREC_MIXED, omx, a.newton_tol_mixed, steps, a.warmup, "mixed-batch", coarse=co,
+                        graph=bool(a.c4_graph))
         log(f"mixed-batch: {rmx['steps_per_s']:.3f} steps/s")
         err_u, err_p = mesh_errors(rmx["soln"])
         log(f"per-mesh rel L2 mixed vs fp64: u max {err_u:.3e}, p max {err_p:.3e}")
@@ -700,6 +703,8 @@ def main():
     ap.add_argument("--c5-coarse", type=int, default=16,
                     help="c5: boxes per direction of the multiplicative coarse pressure correction (0: off)")
     ap.add_argument("--c4-count", type=int, default=256)
+    ap.add_argument("--c4-graph", type=int, default=1,
+                    help="c4: Newton steps as HIP graphs (0: every kernel launched one by one)")
     ap.add_argument("--c4-seed", type=int, default=0)
     ap.add_argument("--c4-seq-stride", type=int, default=16,
                     help="c4: time every n-th mesh through the single-mesh path (0: skip)")
