@@ -207,7 +207,7 @@ def c4_case(count, seed):
     return meshgen.c4_batch(count, seed=seed)
 
 
-def c4_ctx(mesh, em, mu, prec, opts, coarse=0):
+def c4_ctx(mesh, em, mu, prec, opts, coarse=0, mult=True):
     from cfdsfemmpi_amd import sfem
     ctx = sfem.Context(mesh, device=0)
     ctx.set_dirichlet(mesh.dbc_node, mesh.dbc_dof, mesh.dbc_value)
@@ -218,17 +218,17 @@ def c4_ctx(mesh, em, mu, prec, opts, coarse=0):
     if em is not None:
         ctx.set_batch(em, mu)
     if coarse:
-        ctx.set_coarse_correction(coarse, True)  # per mesh for a batch
+        ctx.set_coarse_correction(coarse, mult)  # per mesh for a batch
     ctx.setup_solver(opts)
     return ctx
 
 
-def run_batch(case, prec, opts, newton_tol, steps, warmup, label, graph=True, coarse=0):
+def run_batch(case, prec, opts, newton_tol, steps, warmup, label, graph=True, coarse=0, mult=True):
     """Every mesh of the batch advances ``steps`` time steps together (sfem_step_batch)."""
     from cfdsfemmpi_amd import sfem
     mesh, em, mu, ns, meshes = case
     t0 = time.time()
-    ctx = c4_ctx(mesh, em, mu, prec, opts, coarse)
+    ctx = c4_ctx(mesh, em, mu, prec, opts, coarse, mult)
     ctx.batch_use_graph(graph)
     info = ctx.info()
     log(f"{label}: setup {time.time() - t0:.1f} s, nfree={info['nfree']} nnz={info['nnz']} colours={info['ncolors']}")
@@ -330,10 +330,10 @@ def main_c4(a):
     if co and a.plain:
         # the reference's solver configuration: ILU(0) alone
         rpl = run_batch(case, sfem.PREC_FP64, o64, a.newton_tol, steps, a.warmup, "fp64-batch-plain",
-                        graph=bool(a.c4_graph))
+                        graph=bool(a.c4_graph), mult=bool(a.c4_coarse_mult))
         log(f"fp64-batch-plain: {rpl['steps_per_s']:.3f} steps/s")
     r64 = run_batch(case, sfem.PREC_FP64, o64, a.newton_tol, steps, a.warmup, "fp64-batch", coarse=co,
-                    graph=bool(a.c4_graph))
+                    graph=bool(a.c4_graph), mult=bool(a.c4_coarse_mult))
     log(f"fp64-batch: {r64['steps_per_s']:.3f} steps/s")
     node_mesh = np.concatenate([np.full(m.nNode, k) for k, m in enumerate(meshes)])
 
@@ -349,14 +349,14 @@ def main_c4(a):
     if a.c4_store32:
         o32 = sfem.solver_opts("bcgs", "ilu", a.c4_store32_rtol, 5000, reorder=a.reorder, factor_bf16=a.bf16_factors)
         r32 = run_batch(case, sfem.PREC_STORE32, o32, a.c4_store32_newton, steps, a.warmup, "store32-batch",
-                        coarse=co, graph=bool(a.c4_graph))
+                        coarse=co, graph=bool(a.c4_graph), mult=bool(a.c4_coarse_mult))
         r32["err"] = mesh_errors(r32["soln"])
         log(f"store32-batch: {r32['steps_per_s']:.3f} steps/s, per-mesh rel L2 vs fp64: u max {r32['err'][0]:.3e} "
             f"p max {r32['err'][1]:.3e}")
     rmx = err_u = err_p = None
     if a.c4_mixed:
         rmx = run_batch(case, sfem.PREC_MIXED, omx, a.newton_tol_mixed, steps, a.warmup, "mixed-batch", coarse=co,
-                        graph=bool(a.c4_graph))
+                        graph=bool(a.c4_graph), mult=bool(a.c4_coarse_mult))
         log(f"mixed-batch: {rmx['steps_per_s']:.3f} steps/s")
         err_u, err_p = mesh_errors(rmx["soln"])
         log(f"per-mesh rel L2 mixed vs fp64: u max {err_u:.3e}, p max {err_p:.3e}")
@@ -703,6 +703,8 @@ def main():
     ap.add_argument("--c5-coarse", type=int, default=16,
                     help="c5: boxes per direction of the multiplicative coarse pressure correction (0: off)")
     ap.add_argument("--c4-count", type=int, default=256)
+    ap.add_argument("--c4-coarse-mult", type=int, default=1,
+                    help="c4: multiplicative (1) or additive (0) form of the coarse correction")
     ap.add_argument("--c4-graph", type=int, default=1,
                     help="c4: Newton steps as HIP graphs (0: every kernel launched one by one)")
     ap.add_argument("--c4-seed", type=int, default=0)
