@@ -318,6 +318,51 @@ def test_coarse_without_prescribed_pressure(prec):
     ctx.close()
 
 
+def test_coarse_batch_of_channels_pseudo_inverse():
+    """A batch of two channels without a prescribed pressure and one pinned cavity: each
+    channel's block of the correction is the pseudo-inverse of its own E (numpy, on the
+    exported tangent of that mesh alone, at the batch's first tangent), the cavity's block
+    the inverse of its E, and nothing couples the blocks."""
+    ch = [meshgen.cylinder_channel(nr=8, nth=32, ratio=1.2, n_left=6, n_right=16, n_bottom=6, n_top=6,
+                                   perturb=0.01, seed=s) for s in (3, 4)]
+    cav = meshgen.ldc2d_quad(12, graded=True)
+    meshes = [ch[0], cav, ch[1]]
+    mesh, em = meshgen.disjoint_union(meshes)
+    mu = np.array([0.01, 0.01, 0.01])
+    k = 4
+    opts = sfem.solver_opts("bcgs", "ilu", 1e-8, 3000, reorder=2)
+    ctx = sfem.Context(mesh)
+    ctx.set_dirichlet(mesh.dbc_node, mesh.dbc_dof, mesh.dbc_value)
+    ctx.prepare_elements()
+    ctx.build_pattern()
+    ctx.set_params(ED, 2, 0.0, 0.1)
+    ctx.set_tuning_mode(sfem.TUNE_FIXED)
+    ctx.set_batch(em, mu)
+    ctx.set_coarse_correction(k, True)
+    ctx.setup_solver(opts)
+    ctx.step_batch(0.1, opts, 1e-8, 10)
+    info = ctx.coarse_info()
+    assert info["built"] and not info["singular"]
+    P = ctx.coarse_inverse()
+    ctx.close()
+    off = 0
+    for j, m in enumerate(meshes):
+        c1 = _ctx(m)
+        c1.time_update(0.1)
+        c1.assemble(0)
+        A, _ = _system(c1)
+        Z, _, Eid = _coarse_matrices(m, A, k)
+        nc = Z.shape[1]
+        ref = _pinv_sym(Eid) if j != 1 else np.linalg.inv(Eid)
+        blk = P[off:off + nc, off:off + nc]
+        e = np.max(np.abs(blk - ref)) / np.max(np.abs(ref))
+        print(f"[coarse batch float] mesh {j}: {nc} unknowns, rel diff {e:.2e}", flush=True)
+        assert e <= 1e-8
+        assert np.all(P[off:off + nc, :off] == 0) and np.all(P[off:off + nc, off + nc:] == 0)
+        off += nc
+        c1.close()
+
+
 def test_coarse_channel_same_answer_fewer_iterations():
     """The channel at Re 100, 4 steps from rest (inflow ramped), FP64 BiCGStab + ILU(0)
     stopped at ||r|| <= max(1e-12 ||b||, 5e-11), Newton to ||F|| < 1e-10, with and without
