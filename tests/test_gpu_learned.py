@@ -280,8 +280,10 @@ def test_adam_step_on_the_hashed_minibatch():
     assert L.info()[2] == 1
 
 
-def test_learned_start_never_changes_the_answer():
-    """Train briefly on a short trajectory, then rerun it with the learned predictor:
+@pytest.mark.parametrize("ncomp", [2, 3])
+def test_learned_start_never_changes_the_answer(ncomp):
+    """Train briefly on a short trajectory, then rerun it with the learned predictor
+    (correcting u, v or u, v, p):
     each step starts from the smaller-residual candidate, and every converged state
     agrees with the extrapolated run's within 1e-8 of the state's largest entry.  Both
     runs iterate Newton to ||F|| < 5e-12 (Krylov rtol 1e-10 per solve; the oracle's
@@ -293,6 +295,7 @@ def test_learned_start_never_changes_the_answer():
     fem, ctx, opts = _ctx(mesh)
     ctx.set_predictor(sfem.PRED_EXTRAPOLATE)
     L = sfem.Learner(ctx, nsteps + 1, seed=21)
+    L.set_components(ncomp)
     L.record(ctx, new_trajectory=True)
     ref, its_ref = [], []
     t = 0.1
