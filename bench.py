@@ -762,8 +762,8 @@ def main():
         copy_probe(a.copy_probe_mib)
         return
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {a.gpus}); the multi-GPU layout is described "
-              f"in DESIGN.md and not built", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {a.gpus}): this build is the single-GPU path; "
+              f"a multi-GPU layout is not built", file=sys.stderr)
         sys.exit(2)
     from cfdsfemmpi_amd import sfem
     sfem.load_library()
