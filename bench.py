@@ -130,8 +130,8 @@ def run_path(mesh, ctl, prec, opts, newton_tol, steps, warmup, label, predictor=
 
 
 def cpu_threads():
-    """Host threads for the CPU baseline: the OMP_NUM_THREADS the host grants (16 on the
-    one-GPU boxes), else every core."""
+    """Host threads for the CPU baseline: the OMP_NUM_THREADS the host grants (a machine's
+    share of cores per GPU), else every core."""
     try:
         return max(1, int(os.environ.get("OMP_NUM_THREADS", "") or (os.cpu_count() or 1)))
     except ValueError:
